@@ -1,0 +1,403 @@
+"""Synthetic boundary meshes, boundary conditions and the BEM DOF model.
+
+This module produces the flat arrays that the reference solver reads from its
+``Network`` objects (``WATERS``): node coordinates, outward-oriented linear
+triangles, a Dirichlet/Neumann tag per face, and the per-(point, face) DOF
+numbering of ``setNodeFaceIndices`` with multiple nodes and CORNER points.
+
+Reference behaviour mirrored here:
+  * point classes Neumann / Dirichlet / CORNER and ``setIsMultipleNode``
+    (20 degree rule): bem/BEM_setBoundaryTypes.hpp:433-440, 560-566
+  * ``isNeumannID_BEM`` / ``isDirichletID_BEM`` / ``pf2ID``:
+    bem/BEM_setBoundaryTypes.hpp:29-54
+  * one DOF per (point, face-key); Morton (Z-order, 21 bits per axis) DOF
+    numbering with a stable tie-break: bem/BEM_solveBVP_GMRES_FMM.hpp:3152-3246
+    (the reference breaks ties by pointer value; here by point then face index)
+
+DOF kinds (``dof_kind``): 0 = Dirichlet (unknown phi_n, known phi),
+1 = Neumann (unknown phi, known phi_n), 2 = CORNER Neumann face DOF (unknown
+phi, constraint row phi = phi_D, known phi_n).
+"""
+from __future__ import annotations
+
+import dataclasses
+import math
+
+import numpy as np
+
+DIRICHLET, NEUMANN, CORNER_NEUMANN = 0, 1, 2
+
+
+@dataclasses.dataclass
+class Mesh:
+    X: np.ndarray            # (n_nodes, 3) float64
+    tri: np.ndarray          # (n_faces, 3) int32, outward (out of the fluid) orientation
+    face_dirichlet: np.ndarray  # (n_faces,) bool
+    face_group: np.ndarray   # (n_faces,) int32 patch id (informational)
+
+    @property
+    def n_nodes(self) -> int:
+        return int(self.X.shape[0])
+
+    @property
+    def n_faces(self) -> int:
+        return int(self.tri.shape[0])
+
+    def face_normals(self) -> np.ndarray:
+        a, b, c = (self.X[self.tri[:, k]] for k in range(3))
+        cr = np.cross(b - a, c - a)
+        return cr / np.linalg.norm(cr, axis=1, keepdims=True)
+
+    def face_areas(self) -> np.ndarray:
+        a, b, c = (self.X[self.tri[:, k]] for k in range(3))
+        return 0.5 * np.linalg.norm(np.cross(b - a, c - a), axis=1)
+
+
+@dataclasses.dataclass
+class DofModel:
+    n_dof: int
+    face_dof: np.ndarray     # (n_faces, 3) int32
+    dof_kind: np.ndarray     # (n_dof,) int32
+    dof_point: np.ndarray    # (n_dof,) int32
+    dof_face: np.ndarray     # (n_dof,) int32, -1 for the point's null-key DOF
+    point_class: np.ndarray  # (n_nodes,) int8: 0 Dirichlet, 1 Neumann, 2 CORNER
+    point_multiple: np.ndarray  # (n_nodes,) bool
+
+
+# ---------------------------------------------------------------------------
+# mesh assembly from structured patches, welded by coordinates
+# ---------------------------------------------------------------------------
+class _Builder:
+    def __init__(self, weld_tol: float = 1e-9):
+        self.tol = weld_tol
+        self.key2id: dict = {}
+        self.pts: list = []
+        self.tris: list = []
+        self.dir: list = []
+        self.grp: list = []
+
+    def _ids(self, P: np.ndarray) -> np.ndarray:
+        flat = P.reshape(-1, 3)
+        keys = np.round(flat / self.tol).astype(np.int64)
+        out = np.empty(flat.shape[0], dtype=np.int64)
+        for r in range(flat.shape[0]):
+            k = (int(keys[r, 0]), int(keys[r, 1]), int(keys[r, 2]))
+            i = self.key2id.get(k)
+            if i is None:
+                i = len(self.pts)
+                self.key2id[k] = i
+                self.pts.append(flat[r])
+            out[r] = i
+        return out.reshape(P.shape[:2])
+
+    def grid(self, P: np.ndarray, outward, dirichlet: bool, group: int, mask=None):
+        """P: (nu+1, nv+1, 3) grid of points; quads split with alternating diagonals.
+
+        outward: callable((k,3) points) -> (k,3) approximate outward normals, used to orient faces.
+        mask: optional (nu, nv) bool array, False = skip that quad.
+        """
+        nu, nv = P.shape[0] - 1, P.shape[1] - 1
+        if mask is None:
+            mask = np.ones((nu, nv), bool)
+        used = np.zeros((nu + 1, nv + 1), bool)
+        used[:-1, :-1] |= mask; used[1:, :-1] |= mask; used[1:, 1:] |= mask; used[:-1, 1:] |= mask
+        ids = np.full((nu + 1, nv + 1), -1, dtype=np.int64)
+        ids[used] = self._ids(P[used][:, None, :])[:, 0]
+        I, J = np.nonzero(mask)
+        a, b, c, d = ids[I, J], ids[I + 1, J], ids[I + 1, J + 1], ids[I, J + 1]
+        even = (I + J) % 2 == 0
+        t1 = np.where(even[:, None], np.stack([a, b, c], 1), np.stack([a, b, d], 1))
+        t2 = np.where(even[:, None], np.stack([a, c, d], 1), np.stack([b, c, d], 1))
+        T = np.concatenate([t1, t2], 0)
+        X = np.asarray(self.pts)
+        x0, x1, x2 = X[T[:, 0]], X[T[:, 1]], X[T[:, 2]]
+        nrm = np.cross(x1 - x0, x2 - x0)
+        ref = outward((x0 + x1 + x2) / 3.0)
+        flip = np.einsum("ij,ij->i", nrm, ref) < 0
+        T[flip] = T[flip][:, [0, 2, 1]]
+        self.tris.append(T)
+        self.dir.append(np.full(T.shape[0], dirichlet))
+        self.grp.append(np.full(T.shape[0], group))
+
+    def mesh(self) -> Mesh:
+        return Mesh(np.asarray(self.pts, dtype=np.float64), np.concatenate(self.tris).astype(np.int32),
+                    np.concatenate(self.dir).astype(bool), np.concatenate(self.grp).astype(np.int32))
+
+
+def _plane(o, u, v, nu, nv):
+    s = np.linspace(0.0, 1.0, nu + 1)[:, None, None]
+    t = np.linspace(0.0, 1.0, nv + 1)[None, :, None]
+    return np.asarray(o)[None, None, :] + s * np.asarray(u)[None, None, :] + t * np.asarray(v)[None, None, :]
+
+
+def box_tank(Lx: float, Ly: float, Lz: float, nx: int, ny: int, nz: int, x0=(0.0, 0.0, 0.0)) -> Mesh:
+    """Closed rectangular tank [x0, x0+L]; the top face z = z0+Lz is the free surface (Dirichlet).
+
+    nx, ny, nz are cell counts; the 4x1x1 m tank with 42x11x11 cells has N = 2,092
+    nodes and F = 4,180 triangles.
+    """
+    b = _Builder()
+    ox, oy, oz = x0
+    def out(n):
+        return lambda p: np.broadcast_to(np.asarray(n, dtype=float), p.shape)
+    b.grid(_plane((ox, oy, oz), (Lx, 0, 0), (0, Ly, 0), nx, ny), out((0, 0, -1)), False, 0)
+    b.grid(_plane((ox, oy, oz + Lz), (Lx, 0, 0), (0, Ly, 0), nx, ny), out((0, 0, 1)), True, 1)
+    b.grid(_plane((ox, oy, oz), (Lx, 0, 0), (0, 0, Lz), nx, nz), out((0, -1, 0)), False, 2)
+    b.grid(_plane((ox, oy + Ly, oz), (Lx, 0, 0), (0, 0, Lz), nx, nz), out((0, 1, 0)), False, 3)
+    b.grid(_plane((ox, oy, oz), (0, Ly, 0), (0, 0, Lz), ny, nz), out((-1, 0, 0)), False, 4)
+    b.grid(_plane((ox + Lx, oy, oz), (0, Ly, 0), (0, 0, Lz), ny, nz), out((1, 0, 0)), False, 5)
+    return b.mesh()
+
+
+def _square_ring(a: float, m: int) -> np.ndarray:
+    """Points on the boundary of the square [-a,a]^2, m segments per side, counter-clockwise."""
+    s = np.linspace(-a, a, m + 1)[:-1]
+    pts = [np.stack([s, np.full_like(s, -a)], 1), np.stack([np.full_like(s, a), s], 1),
+           np.stack([-s, np.full_like(s, a)], 1), np.stack([np.full_like(s, -a), -s], 1)]
+    return np.concatenate(pts, 0)
+
+
+def cylinder_tank(Lx=20.0, Ly=4.0, Lz=2.0, R=0.5, draft=1.0, h=0.036, a=None, cx=0.0, cy=0.0) -> Mesh:
+    """Tank [-Lx/2,Lx/2]x[-Ly/2,Ly/2]x[0,Lz] with a vertical surface-piercing cylinder.
+
+    The cylinder (radius R, axis at (cx, cy)) occupies z in [Lz-draft, Lz] and is
+    excluded from the fluid. The free surface z = Lz (Dirichlet) is an O-grid
+    around the cylinder inside the square [-a,a]^2 and a structured grid outside;
+    all other faces are Neumann.
+    """
+    if a is None:
+        a = max(2.0 * R, h * round(2.0 * R / h))
+    nx = int(round(Lx / h)); ny = int(round(Ly / h)); nz = int(round(Lz / h))
+    hx, hy = Lx / nx, Ly / ny
+    # snap the O-grid square to the free-surface grid
+    ia = int(round(a / hx)); ja = int(round(a / hy))
+    a_x, a_y = ia * hx, ja * hy
+    b = _Builder()
+    x0, y0 = -Lx / 2, -Ly / 2
+    const = lambda n: (lambda p: np.broadcast_to(np.asarray(n, dtype=float), p.shape))
+    b.grid(_plane((x0, y0, 0.0), (Lx, 0, 0), (0, Ly, 0), nx, ny), const((0, 0, -1)), False, 0)
+    b.grid(_plane((x0, y0, 0.0), (Lx, 0, 0), (0, 0, Lz), nx, nz), const((0, -1, 0)), False, 2)
+    b.grid(_plane((x0, y0 + Ly, 0.0), (Lx, 0, 0), (0, 0, Lz), nx, nz), const((0, 1, 0)), False, 3)
+    b.grid(_plane((x0, y0, 0.0), (0, Ly, 0), (0, 0, Lz), ny, nz), const((-1, 0, 0)), False, 4)
+    b.grid(_plane((x0 + Lx, y0, 0.0), (0, Ly, 0), (0, 0, Lz), ny, nz), const((1, 0, 0)), False, 5)
+    # free surface minus the O-grid square
+    P = _plane((x0, y0, Lz), (Lx, 0, 0), (0, Ly, 0), nx, ny)
+    xc = x0 + (np.arange(nx) + 0.5) * hx
+    yc = y0 + (np.arange(ny) + 0.5) * hy
+    inside = (np.abs(xc - cx)[:, None] < a_x) & (np.abs(yc - cy)[None, :] < a_y)
+    b.grid(P, const((0, 0, 1)), True, 1, mask=~inside)
+    # O-grid annulus between the square boundary and the circle (free surface)
+    m_side_x, m_side_y = 2 * ia, 2 * ja
+    ring = []
+    s = np.linspace(-a_x, a_x, m_side_x + 1)[:-1]
+    ring.append(np.stack([s, np.full_like(s, -a_y)], 1))
+    s = np.linspace(-a_y, a_y, m_side_y + 1)[:-1]
+    ring.append(np.stack([np.full_like(s, a_x), s], 1))
+    s = np.linspace(a_x, -a_x, m_side_x + 1)[:-1]
+    ring.append(np.stack([s, np.full_like(s, a_y)], 1))
+    s = np.linspace(a_y, -a_y, m_side_y + 1)[:-1]
+    ring.append(np.stack([np.full_like(s, -a_x), s], 1))
+    sq = np.concatenate(ring, 0)
+    sq = np.concatenate([sq, sq[:1]], 0)  # closed
+    ang = np.arctan2(sq[:, 1], sq[:, 0])
+    circ = np.stack([R * np.cos(ang), R * np.sin(ang)], 1)
+    circ[-1] = circ[0]
+    nr = max(2, int(round((min(a_x, a_y) - R) / h)))
+    t = np.linspace(0.0, 1.0, nr + 1)
+    G = np.zeros((sq.shape[0], nr + 1, 3))
+    G[:, :, 0] = cx + circ[:, None, 0] * (1 - t[None, :]) + sq[:, None, 0] * t[None, :]
+    G[:, :, 1] = cy + circ[:, None, 1] * (1 - t[None, :]) + sq[:, None, 1] * t[None, :]
+    G[:, :, 2] = Lz
+    b.grid(G, const((0, 0, 1)), True, 1)
+    # cylinder side: out of the fluid = towards the axis
+    nzc = max(2, int(round(draft / h)))
+    zc = np.linspace(Lz - draft, Lz, nzc + 1)
+    S = np.zeros((sq.shape[0], nzc + 1, 3))
+    S[:, :, 0] = cx + circ[:, None, 0]
+    S[:, :, 1] = cy + circ[:, None, 1]
+    S[:, :, 2] = zc[None, :]
+    b.grid(S, lambda p: -np.stack([p[:, 0] - cx, p[:, 1] - cy, np.zeros(len(p))], 1), False, 6)
+    # cylinder bottom disk (z = Lz - draft): out of the fluid = +z; inner square + annulus
+    bb = 0.5 * R
+    sqb = sq * np.array([bb / a_x, bb / a_y])[None, :]
+    nrb = max(2, int(round((R - bb) / h)))
+    t = np.linspace(0.0, 1.0, nrb + 1)
+    D = np.zeros((sq.shape[0], nrb + 1, 3))
+    D[:, :, 0] = cx + circ[:, None, 0] * (1 - t[None, :]) + sqb[:, None, 0] * t[None, :]
+    D[:, :, 1] = cy + circ[:, None, 1] * (1 - t[None, :]) + sqb[:, None, 1] * t[None, :]
+    D[:, :, 2] = Lz - draft
+    b.grid(D, const((0, 0, 1)), False, 7)
+    Q = _plane((cx - bb, cy - bb, Lz - draft), (2 * bb, 0, 0), (0, 2 * bb, 0), m_side_x, m_side_y)
+    b.grid(Q, const((0, 0, 1)), False, 7)
+    return b.mesh()
+
+
+# ---------------------------------------------------------------------------
+# DOF model
+# ---------------------------------------------------------------------------
+def _part1by2(v: np.ndarray) -> np.ndarray:
+    v = v.astype(np.uint64) & np.uint64(0x1FFFFF)
+    v = (v | (v << np.uint64(32))) & np.uint64(0x1F00000000FFFF)
+    v = (v | (v << np.uint64(16))) & np.uint64(0x1F0000FF0000FF)
+    v = (v | (v << np.uint64(8))) & np.uint64(0x100F00F00F00F00F)
+    v = (v | (v << np.uint64(4))) & np.uint64(0x10C30C30C30C30C3)
+    v = (v | (v << np.uint64(2))) & np.uint64(0x1249249249249249)
+    return v
+
+
+def morton_keys(X: np.ndarray) -> np.ndarray:
+    """21-bit/axis Morton keys in the point bounding box (GMRES_FMM.hpp:3164-3222)."""
+    lo, hi = X.min(0), X.max(0)
+    maxv = float((1 << 21) - 1)
+    q = np.zeros(X.shape, dtype=np.uint64)
+    for a in range(3):
+        if hi[a] > lo[a]:
+            t = np.clip((X[:, a] - lo[a]) / (hi[a] - lo[a]), 0.0, 1.0)
+            q[:, a] = (t * maxv + 0.5).astype(np.uint64)
+    return _part1by2(q[:, 0]) | (_part1by2(q[:, 1]) << np.uint64(1)) | (_part1by2(q[:, 2]) << np.uint64(2))
+
+
+def build_dofs(mesh: Mesh, multiple_angle_deg: float = 20.0) -> DofModel:
+    n, F = mesh.n_nodes, mesh.n_faces
+    tri = mesh.tri
+    fn = mesh.face_normals()
+    fa = mesh.face_areas()
+    # incidence
+    order = np.argsort(tri.reshape(-1), kind="stable")
+    ptr = np.zeros(n + 1, dtype=np.int64)
+    np.add.at(ptr, tri.reshape(-1) + 1, 1)
+    ptr = np.cumsum(ptr)
+    inc_face = (order // 3).astype(np.int64)
+    nd = mesh.face_dirichlet
+    all_neu = np.ones(n, bool); all_dir = np.ones(n, bool)
+    np.logical_and.at(all_neu, tri.reshape(-1), np.repeat(~nd, 3))
+    np.logical_and.at(all_dir, tri.reshape(-1), np.repeat(nd, 3))
+    pclass = np.where(all_dir, 0, np.where(all_neu, 1, 2)).astype(np.int8)
+    # point normal: area-weighted mean of face normals; multiple node if any face deviates > angle
+    pn = np.zeros((n, 3))
+    np.add.at(pn, tri.reshape(-1), np.repeat(fn * fa[:, None], 3, axis=0))
+    pn /= np.linalg.norm(pn, axis=1, keepdims=True)
+    cosang = np.einsum("ij,ij->i", fn[inc_face], pn[np.repeat(np.arange(n), np.diff(ptr))])
+    dev = cosang < math.cos(math.radians(multiple_angle_deg))
+    multiple = np.zeros(n, bool)
+    np.logical_or.at(multiple, np.repeat(np.arange(n), np.diff(ptr)), dev)
+    multiple |= pclass == 2
+    # DOF keys: (point, face) for Neumann faces of multiple Neumann/CORNER points; else (point, -1)
+    entries = []  # (point, face, kind)
+    for p in range(n):
+        pc = pclass[p]
+        neu_or_corner = pc in (1, 2)
+        has_null = (neu_or_corner and not multiple[p]) or pc in (0, 2)
+        if has_null:
+            entries.append((p, -1, 0 if pc in (0, 2) else 1))
+        if neu_or_corner and multiple[p]:
+            for f in inc_face[ptr[p]:ptr[p + 1]]:
+                if not nd[f]:
+                    entries.append((p, int(f), 2 if pc == 2 else 1))
+    ent = np.asarray(entries, dtype=np.int64)
+    keys = morton_keys(mesh.X)[ent[:, 0]]
+    srt = np.lexsort((ent[:, 1], ent[:, 0], keys))
+    ent = ent[srt]
+    n_dof = ent.shape[0]
+    lookup = {(int(p), int(f)): i for i, (p, f, _) in enumerate(ent)}
+    face_dof = np.empty((F, 3), dtype=np.int32)
+    for f in range(F):
+        for k in range(3):
+            p = int(tri[f, k])
+            key = (p, f) if ((pclass[p] in (1, 2)) and multiple[p] and not nd[f]) else (p, -1)
+            face_dof[f, k] = lookup[key]
+    return DofModel(n_dof, face_dof, ent[:, 2].astype(np.int32), ent[:, 0].astype(np.int32),
+                    ent[:, 1].astype(np.int32), pclass, multiple)
+
+
+def point_normals(mesh: Mesh) -> np.ndarray:
+    fn, fa = mesh.face_normals(), mesh.face_areas()
+    pn = np.zeros((mesh.n_nodes, 3))
+    np.add.at(pn, mesh.tri.reshape(-1), np.repeat(fn * fa[:, None], 3, axis=0))
+    return pn / np.linalg.norm(pn, axis=1, keepdims=True)
+
+
+def dof_normals(mesh: Mesh, dofs: DofModel) -> np.ndarray:
+    """Normal a DOF's phi_n refers to: its face normal for per-face DOFs; for the null-key DOF
+    of a CORNER point, the mean normal of its Dirichlet faces; else the point normal."""
+    pn = point_normals(mesh)
+    fn, fa = mesh.face_normals(), mesh.face_areas()
+    dn = np.zeros((mesh.n_nodes, 3))
+    w = (fn * fa[:, None]) * mesh.face_dirichlet[:, None]
+    np.add.at(dn, mesh.tri.reshape(-1), np.repeat(w, 3, axis=0))
+    nd = np.linalg.norm(dn, axis=1)
+    corner = dofs.point_class == 2
+    pn[corner] = dn[corner] / nd[corner, None]
+    out = pn[dofs.dof_point].copy()
+    per_face = dofs.dof_face >= 0
+    out[per_face] = fn[dofs.dof_face[per_face]]
+    return out
+
+
+# ---------------------------------------------------------------------------
+# boundary conditions of the benchmark workloads
+# ---------------------------------------------------------------------------
+def free_surface_modes(rng: np.random.Generator, n_modes: int = 8, amp: float = 0.01):
+    a = rng.normal(0.0, amp, n_modes)
+    k = rng.integers(1, 5, n_modes)
+    l = rng.integers(1, 5, n_modes)
+    th = rng.uniform(0.0, 2 * math.pi, n_modes)
+    return a, k, l, th
+
+
+def tank_bc(mesh: Mesh, dofs: DofModel, seed: int, Lx: float, Ly: float, body_velocity_sigma: float = 0.0,
+            body_group: tuple = (6, 7)):
+    """phi on Dirichlet (free surface): seeded sum of 8 sine modes; phi_n = 0 on walls and bottom;
+    phi_n = n.U on the body faces (groups 6, 7) with U ~ N(0, sigma) (seeded)."""
+    rng = np.random.default_rng(seed)
+    a, k, l, th = free_surface_modes(rng)
+    X = mesh.X
+    phi_pt = np.zeros(mesh.n_nodes)
+    for i in range(len(a)):
+        phi_pt += a[i] * np.sin(math.pi * (k[i] * X[:, 0] / Lx + l[i] * X[:, 1] / Ly) + th[i])
+    phi = phi_pt[dofs.dof_point].copy()
+    phin = np.zeros(dofs.n_dof)
+    if body_velocity_sigma > 0.0:
+        U = rng.normal(0.0, body_velocity_sigma, 3)
+        on_body = np.zeros(mesh.n_nodes, bool)
+        body_faces = np.isin(mesh.face_group, body_group)
+        on_body[mesh.tri[body_faces].reshape(-1)] = True
+        nrm = dof_normals(mesh, dofs)
+        sel = (dofs.dof_kind != DIRICHLET) & on_body[dofs.dof_point]
+        # per-face DOF: only if its face is a body face
+        pf = dofs.dof_face >= 0
+        sel &= ~pf | np.isin(mesh.face_group[np.maximum(dofs.dof_face, 0)], body_group)
+        phin[sel] = nrm[sel] @ U
+    return phi, phin
+
+
+def perturb_free_surface(mesh: Mesh, seed: int, sigma: float) -> Mesh:
+    """z-offsets ~ N(0, sigma) on free-surface nodes that are interior to the top face."""
+    rng = np.random.default_rng(seed)
+    X = mesh.X.copy()
+    top = np.zeros(mesh.n_nodes, bool)
+    top[mesh.tri[mesh.face_dirichlet].reshape(-1)] = True
+    other = np.zeros(mesh.n_nodes, bool)
+    other[mesh.tri[~mesh.face_dirichlet].reshape(-1)] = True
+    interior = top & ~other
+    X[interior, 2] += rng.normal(0.0, sigma, int(interior.sum()))
+    return Mesh(X, mesh.tri.copy(), mesh.face_dirichlet.copy(), mesh.face_group.copy())
+
+
+def analytic_bc(mesh: Mesh, dofs: DofModel, kind: str = "linear"):
+    """Known harmonic field: returns (phi, phin) BC arrays and the exact (phi, phin) per DOF."""
+    X = mesh.X[dofs.dof_point]
+    nrm = dof_normals(mesh, dofs)
+    if kind == "linear":
+        g = np.array([1.0, -0.5, 0.25])
+        phi_ex = X @ g
+        phin_ex = nrm @ g
+    elif kind == "exp":
+        kk = math.pi / 2.0
+        phi_ex = np.exp(kk * X[:, 2]) * np.cos(kk * X[:, 0])
+        grad = np.stack([-kk * np.exp(kk * X[:, 2]) * np.sin(kk * X[:, 0]), np.zeros(len(X)),
+                         kk * np.exp(kk * X[:, 2]) * np.cos(kk * X[:, 0])], 1)
+        phin_ex = np.einsum("ij,ij->i", grad, nrm)
+    else:
+        raise ValueError(kind)
+    return phi_ex.copy(), phin_ex.copy(), phi_ex, phin_ex

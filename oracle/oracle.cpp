@@ -1,0 +1,1219 @@
+// ORACLE — test infrastructure only. NOT part of the product.
+//
+// FP64 CPU restatement of the reference's FMM-accelerated GMRES BVP path
+// (tomoakihirakawa/BEM_for_Nonlinear_Waves), used by tests/, bench.py's
+// cpu_baseline leg and __graft_entry__.smoke() as the checker. The product
+// (libbemfmm.so) never links, loads or calls this code.
+//
+// Every stage follows the reference file:line cited at its definition:
+//   scaling / orchestration   bem/BEM_solveBVP_GMRES_FMM.hpp:3056-3506
+//   sources (linear faces)    bem/BEM_solveBVP_GMRES_FMM.hpp:1799-1937
+//   octree (Buckets)          lib/include/lib_spatial_partitioning.hpp:223-1307
+//   MAC + lists               lib/include/moments.hpp:948-997
+//   P2M/M2M/M2L/L2L/L2P       lib/include/moments.hpp:165-885, lib_multipole_expansion.hpp:779-1150
+//   near field                lib/include/lib_multipole_expansion.hpp:1180-1302
+//   matvec                    bem/BEM_solveBVP_GMRES_FMM.hpp:34-158
+//   rigid-mode alpha          bem/BEM_solveBVP_GMRES_FMM.hpp:2674-2716
+//   GMRES                     bem/BEM_solveBVP_GMRES_FMM.hpp:2007-2355, lib/include/basic_linear_systems.hpp:3137-3451
+//
+// The reference's translation tables AAA_{M2M,M2L,L2L}_FMM.hpp are absent, so
+// the three operators are evaluated from the Greengard & Rokhlin (1997)
+// closed forms (Theorems 3.3, 3.5, 3.6) with the coefficient A_n^m in long
+// double; the reference's harmonics use libstdc++ std::sph_legendre, which is
+// what this file uses too.
+//
+// Pinning status of each function is listed in the file header of
+// tests/test_oracle.py (known-answer tests, closed forms, invariants).
+
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <cstdint>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <tuple>
+#include <vector>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace {
+
+using cd = std::complex<double>;
+using V3 = std::array<double, 3>;
+
+inline V3 operator-(const V3& a, const V3& b) { return {a[0] - b[0], a[1] - b[1], a[2] - b[2]}; }
+inline V3 operator+(const V3& a, const V3& b) { return {a[0] + b[0], a[1] + b[1], a[2] + b[2]}; }
+inline V3 operator*(double s, const V3& a) { return {s * a[0], s * a[1], s * a[2]}; }
+inline double dot(const V3& a, const V3& b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+inline double norm(const V3& a) { return std::hypot(a[0], a[1], a[2]); }  // Norm: basic_arithmetic_array_operations.hpp:860
+inline V3 cross(const V3& a, const V3& b) { return {a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2], a[0] * b[1] - a[1] * b[0]}; }
+
+// ---------------------------------------------------------------------------
+// Quadrature tables (values as printed in lib/include/basic_constants.hpp:1326-1350
+// and lib/include/dunavant_rules.hpp:18-46; checked against Gauss-Legendre and
+// Dunavant monomial exactness in tests/test_oracle.py).
+// ---------------------------------------------------------------------------
+const double kGW5[25][3] = {
+    {0.046910077030668, 0.046910077030668, 0.0140335872156063}, {0.046910077030668, 0.230765344947158, 0.0283499999999991},
+    {0.046910077030668, 0.5, 0.0336962680968792},               {0.046910077030668, 0.769234655052841, 0.0283499999999991},
+    {0.046910077030668, 0.953089922969332, 0.0140335872156063}, {0.230765344947158, 0.046910077030668, 0.0283499999999991},
+    {0.230765344947158, 0.230765344947158, 0.0572713510559978}, {0.230765344947158, 0.5, 0.0680716331376877},
+    {0.230765344947158, 0.769234655052841, 0.0572713510559978}, {0.230765344947158, 0.953089922969332, 0.0283499999999991},
+    {0.5, 0.046910077030668, 0.0336962680968792},               {0.5, 0.230765344947158, 0.0680716331376877},
+    {0.5, 0.5, 0.0809086419753086},                             {0.5, 0.769234655052841, 0.0680716331376877},
+    {0.5, 0.953089922969332, 0.0336962680968792},               {0.769234655052841, 0.046910077030668, 0.0283499999999991},
+    {0.769234655052841, 0.230765344947158, 0.0572713510559978}, {0.769234655052841, 0.5, 0.0680716331376877},
+    {0.769234655052841, 0.769234655052841, 0.0572713510559978}, {0.769234655052841, 0.953089922969332, 0.0283499999999991},
+    {0.953089922969332, 0.046910077030668, 0.0140335872156063}, {0.953089922969332, 0.230765344947158, 0.0283499999999991},
+    {0.953089922969332, 0.5, 0.0336962680968792},               {0.953089922969332, 0.769234655052841, 0.0283499999999991},
+    {0.953089922969332, 0.953089922969332, 0.0140335872156063}};
+
+const std::vector<std::array<double, 4>>& dunavant(int q) {
+  static const std::vector<std::array<double, 4>> d1 = {{1. / 3., 1. / 3., 1. / 3., 0.5}};
+  static const std::vector<std::array<double, 4>> d3 = {
+      {2. / 3., 1. / 6., 1. / 6., 1. / 6.}, {1. / 6., 2. / 3., 1. / 6., 1. / 6.}, {1. / 6., 1. / 6., 2. / 3., 1. / 6.}};
+  static const std::vector<std::array<double, 4>> d6 = {
+      {0.816847572980459, 0.091576213509771, 0.091576213509771, 0.054975871827661},
+      {0.091576213509771, 0.816847572980459, 0.091576213509771, 0.054975871827661},
+      {0.091576213509771, 0.091576213509771, 0.816847572980459, 0.054975871827661},
+      {0.108103018168070, 0.445948490915965, 0.445948490915965, 0.111690794839006},
+      {0.445948490915965, 0.108103018168070, 0.445948490915965, 0.111690794839006},
+      {0.445948490915965, 0.445948490915965, 0.108103018168070, 0.111690794839006}};
+  static const std::vector<std::array<double, 4>> d7 = {
+      {1. / 3., 1. / 3., 1. / 3., 0.1125},
+      {0.059715871789770, 0.470142064105115, 0.470142064105115, 0.066197076394253},
+      {0.470142064105115, 0.059715871789770, 0.470142064105115, 0.066197076394253},
+      {0.470142064105115, 0.470142064105115, 0.059715871789770, 0.066197076394253},
+      {0.797426985353087, 0.101286507323456, 0.101286507323456, 0.062969590272414},
+      {0.101286507323456, 0.797426985353087, 0.101286507323456, 0.062969590272414},
+      {0.101286507323456, 0.101286507323456, 0.797426985353087, 0.062969590272414}};
+  switch (q) {
+  case 1: return d1;
+  case 3: return d3;
+  case 6: return d6;
+  case 7: return d7;
+  default: throw std::runtime_error("oracle: unsupported p2m_quadrature_points");
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Spherical harmonics, reference convention (lib_multipole_expansion.hpp:578-614, 878-890):
+//   Y_n^m = sqrt(4 pi/(2n+1)) * std::sph_legendre(n,|m|,theta) * exp(i m phi)
+//   R_n^m(x) = rho^n Y_n^m,   I_n^m(x) = Y_n^m / rho^(n+1)
+// Coordinates (lib_multipole_expansion.hpp:788-799): rho = hypot(x,y,z),
+// theta = atan2(hypot(x,y), z), phi = atan2(y, x).
+// Storage: idx(n,m) = n*(n+1)+m for -n<=m<=n.
+// ---------------------------------------------------------------------------
+inline int idx(int n, int m) { return n * (n + 1) + m; }
+
+struct Harm {
+  int P;
+  std::vector<cd> Y;  // Y_n^m for n<=P
+  double rho;
+  void eval(const V3& x, int P_) {
+    P = P_;
+    Y.assign((P + 1) * (P + 1), 0.);
+    rho = norm(x);
+    const double r2d = std::hypot(x[0], x[1]);
+    const double theta = std::atan2(r2d, x[2]);
+    const double phi = std::atan2(x[1], x[0]);
+    const double two_pi_sqrt = std::sqrt(2.0 * M_PI);
+    for (int n = 0; n <= P; ++n) {
+      const double s = two_pi_sqrt / std::sqrt(n + 0.5);
+      for (int m = 0; m <= n; ++m) {
+        const double a = std::sph_legendre(n, m, theta) * s;
+        Y[idx(n, m)] = std::polar(a, m * phi);
+        Y[idx(n, -m)] = std::polar(a, -m * phi);
+      }
+    }
+  }
+};
+
+// Regular solid harmonics R_n^m and their normal derivatives grad(R_n^{-m}).nhat,
+// as used by P2M (lib_multipole_expansion.hpp:996-1027, m -> -m).  The gradient is
+// evaluated with the ladder identities of the unnormalised harmonic
+// NR_n^m = rho^n P_n^m(cos t) e^{i m phi}/(n+m)!:
+//   d/dz NR_n^m = NR_{n-1}^m, (dx + i dy) NR_n^m = -NR_{n-1}^{m+1}, (dx - i dy) NR_n^m = NR_{n-1}^{m-1},
+// and R_n^q = c(n,q) NR_n^q with c(n,q) = (-1)^max(q,0) sqrt((n-|q|)!(n+|q|)!).
+// This is the same derivative the reference writes with the polar-angle chain
+// rule, but stays finite on the z axis.
+double lfact(int k) { return std::lgamma(k + 1.0); }
+double cfac(int n, int q) {
+  const int a = std::abs(q);
+  const double v = std::exp(0.5 * (lfact(n - a) + lfact(n + a)));
+  return (q > 0 && (q & 1)) ? -v : v;
+}
+
+struct SolidR {
+  int P;
+  std::vector<cd> R;   // R_n^m(x), all m
+  std::vector<cd> NR;  // NR_n^m
+  void eval(const V3& x, int P_) {
+    P = P_;
+    Harm h;
+    h.eval(x, P);
+    R.assign((P + 1) * (P + 1), 0.);
+    NR.assign((P + 1) * (P + 1), 0.);
+    for (int n = 0; n <= P; ++n) {
+      const double rn = std::pow(h.rho, n);
+      for (int m = -n; m <= n; ++m) {
+        R[idx(n, m)] = h.Y[idx(n, m)] * rn;
+        NR[idx(n, m)] = R[idx(n, m)] / cfac(n, m);
+      }
+    }
+  }
+  cd nr(int n, int m) const { return (n < 0 || std::abs(m) > n) ? cd(0.) : NR[idx(n, m)]; }
+  // grad(R_n^q) . nhat
+  cd dR(int n, int q, const V3& nh) const {
+    if (n == 0) return 0.;
+    const cd a(nh[0] * 0.5, -nh[1] * 0.5);  // (nx - i ny)/2
+    const cd b(nh[0] * 0.5, nh[1] * 0.5);   // (nx + i ny)/2
+    const cd g = -a * nr(n - 1, q + 1) + b * nr(n - 1, q - 1) + nh[2] * nr(n - 1, q);
+    return cfac(n, q) * g;
+  }
+};
+
+// ---------------------------------------------------------------------------
+// Greengard & Rokhlin (1997) translation coefficients; A_n^m = (-1)^n / sqrt((n-m)!(n+m)!)
+// (lib_multipole_expansion_constants.hpp:14-28).  Index maps follow moments.hpp:204-270.
+// ---------------------------------------------------------------------------
+long double Anm(int n, int m) {
+  const int a = std::abs(m);
+  if (a > n) return 0.L;
+  long double f1 = 1.L, f2 = 1.L;
+  for (int k = 2; k <= n - a; ++k) f1 *= k;
+  for (int k = 2; k <= n + a; ++k) f2 *= k;
+  const long double v = 1.L / std::sqrt(f1 * f2);
+  return (n & 1) ? -v : v;
+}
+cd ipow(int e) {
+  e = ((e % 4) + 4) % 4;
+  static const cd t[4] = {cd(1, 0), cd(0, 1), cd(-1, 0), cd(0, -1)};
+  return t[e];
+}
+// M2M: M_j^k(parent) += sum_{n,m} AAA_M2M[j][k][n][m] * R_n^{-m}(child-parent) * M_{j-n}^{k-m}(child)
+cd aaa_m2m(int j, int k, int n, int m) {
+  if (std::abs(k - m) > j - n || std::abs(m) > n) return 0.;
+  const long double v = Anm(n, m) * Anm(j - n, k - m) / Anm(j, k);
+  return ipow(std::abs(k) - std::abs(m) - std::abs(k - m)) * (double)v;
+}
+// M2L: L_j^k(target) += sum_{n,m} AAA_M2L[j][k][n][m] * I_{j+n}^{m-k}(source-target) * M_n^m(source)
+cd aaa_m2l(int j, int k, int n, int m) {
+  const long double v = Anm(n, m) * Anm(j, k) / (((n & 1) ? -1.L : 1.L) * Anm(j + n, m - k));
+  return ipow(std::abs(k - m) - std::abs(k) - std::abs(m)) * (double)v;
+}
+// L2L: L_j^k(child) += sum_{n>=j,m} AAA_L2L[j][k][n][m] * R_{n-j}^{m-k}(parent-child) * L_n^m(parent)
+cd aaa_l2l(int j, int k, int n, int m) {
+  if (std::abs(m - k) > n - j) return 0.;
+  const long double v = Anm(n - j, m - k) * Anm(j, k) / ((((n + j) & 1) ? -1.L : 1.L) * Anm(n, m));
+  return ipow(std::abs(m) - std::abs(m - k) - std::abs(k)) * (double)v;
+}
+
+struct Tables {
+  int P;
+  std::vector<cd> m2m, m2l, l2l;  // [j][k][n][m] flattened, (P+1)^2 x (P+1)^2
+  int C;
+  void build(int P_) {
+    P = P_;
+    C = (P + 1) * (P + 1);
+    m2m.assign((size_t)C * C, 0.);
+    m2l.assign((size_t)C * C, 0.);
+    l2l.assign((size_t)C * C, 0.);
+    for (int j = 0; j <= P; ++j)
+      for (int k = -j; k <= j; ++k)
+        for (int n = 0; n <= P; ++n)
+          for (int m = -n; m <= n; ++m) {
+            const size_t o = (size_t)idx(j, k) * C + idx(n, m);
+            if (n <= j) m2m[o] = aaa_m2m(j, k, n, m);
+            m2l[o] = aaa_m2l(j, k, n, m);
+            if (n >= j) l2l[o] = aaa_l2l(j, k, n, m);
+          }
+  }
+};
+
+// ---------------------------------------------------------------------------
+// Octree (Buckets), lib/include/lib_spatial_partitioning.hpp:223-1307.
+// Each node is a grid of ceil(width/dL) cells per axis (2 in practice); a child's
+// bounds are its parent's cell and its cell size is parent.dL*(0.5+1e-13)
+// (lib_spatial_partitioning.hpp:451-456, 1087-1094).
+// ---------------------------------------------------------------------------
+struct Bucket {
+  int level = 0, max_level = 0;
+  double bmin[3], bmax[3];
+  V3 X;  // centre = mean of bounds (basic_geometry.hpp:633-700)
+  double dL = 0, inv_dL = 0;
+  int size[3] = {1, 1, 1};
+  Bucket* parent = nullptr;
+  std::vector<Bucket*> cells;  // size[0]*size[1]*size[2], x-major
+  std::vector<std::vector<int>> cell_data;
+  std::vector<int> data;  // data1D_vector: all descendant faces (insertion order)
+  double off = 0.;        // max_internal_offset
+  std::vector<Bucket*> m2l_list, near, l2m;
+  std::vector<cd> M0, M1, L0, L1;
+  int id = -1;
+  bool has_children() const {
+    for (auto* c : cells)
+      if (c) return true;
+    return false;
+  }
+  static int calc_size(double lo, double hi, double dl) {
+    double s = std::ceil((hi - lo) / dl);
+    if (s <= 0.) s = 1.;
+    return (int)s;
+  }
+  void init(const double lo[3], const double hi[3], double dl) {
+    for (int a = 0; a < 3; ++a) {
+      bmin[a] = lo[a];
+      bmax[a] = hi[a];
+      X[a] = (lo[a] + hi[a]) / 2.;
+      size[a] = calc_size(lo[a], hi[a], dl);
+    }
+    dL = dl;
+    inv_dL = 1.0 / dl;
+    cell_data.assign(size[0] * size[1] * size[2], {});
+    cells.assign(size[0] * size[1] * size[2], nullptr);
+  }
+  void index_nc(const V3& x, int ijk[3]) const {
+    for (int a = 0; a < 3; ++a) ijk[a] = (int)std::floor((x[a] - bmin[a]) * inv_dL);
+  }
+  bool inside(const V3& x) const {
+    int ijk[3];
+    index_nc(x, ijk);
+    for (int a = 0; a < 3; ++a)
+      if (ijk[a] < 0 || ijk[a] >= size[a]) return false;
+    return true;
+  }
+  int cell_of(const V3& x) const {
+    int ijk[3];
+    index_nc(x, ijk);
+    for (int a = 0; a < 3; ++a) ijk[a] = std::clamp(ijk[a], 0, size[a] - 1);
+    return (ijk[0] * size[1] + ijk[1]) * size[2] + ijk[2];
+  }
+  void cell_bounds(int c, double lo[3], double hi[3]) const {
+    int k = c % size[2], j = (c / size[2]) % size[1], i = c / (size[2] * size[1]);
+    const int ijk[3] = {i, j, k};
+    for (int a = 0; a < 3; ++a) {
+      lo[a] = dL * ijk[a] + bmin[a];
+      hi[a] = dL * (ijk[a] + 1) + bmin[a];
+    }
+  }
+};
+
+struct FaceRec {
+  int v[3];
+  int dof[3];
+  V3 X[3];
+  V3 centroid;  // (a+b+c)/3
+  V3 pole;      // Dot({1/3,1/3,1/3}, X012): tree position
+  V3 nhat;
+  V3 cr;        // 2*area*normal
+  double J;     // 2*area
+  double maxoff;
+  std::vector<V3> qx;
+  std::vector<double> qw;                  // W = J * w_q
+  std::vector<std::array<double, 3>> qb;   // barycentric
+};
+
+struct Oracle {
+  // inputs
+  int n_nodes = 0, n_faces = 0, n_dof = 0;
+  std::vector<V3> Xn;  // scaled node coordinates
+  std::vector<FaceRec> F;
+  std::vector<int> dof_kind, dof_point;
+  std::vector<std::vector<int>> point_dofs;
+  std::vector<double> phi_known, phin_known;  // scaled knowns
+  double Lscale = 1.0, scale = 1.0, near_region = 0.;
+  // settings
+  int P = 10, leaf_max = 50, max_level = 7, q = 6, m2l_mode = 0;
+  double theta = 0.25;
+  // tree
+  Bucket root;
+  std::vector<std::unique_ptr<Bucket>> pool;
+  std::vector<std::vector<Bucket*>> level_buckets;
+  std::vector<Bucket*> leaves;
+  std::vector<Bucket*> target_box;  // per node
+  Tables T;
+  // near field per target node
+  std::vector<std::vector<int>> near_idx;
+  std::vector<std::vector<double>> near_wG, near_wGn;
+  std::vector<double> alpha;  // per DOF
+  // working dense arrays
+  std::vector<double> phi, phin;
+  // stats
+  long long n_m2l_pairs = 0;
+  long long nnz = 0;
+  double t_setup = 0;
+
+  // ---- tree ----------------------------------------------------------------
+  Bucket* new_bucket() {
+    pool.emplace_back(new Bucket());
+    return pool.back().get();
+  }
+  void add(Bucket* b, int f) {
+    const V3& x = F[f].pole;
+    b->data.push_back(f);
+    b->cell_data[b->cell_of(x)].push_back(f);
+  }
+  // generateTree (lib_spatial_partitioning.hpp:1079-1115) with the grow condition
+  // of BEM_solveBVP_GMRES_FMM.hpp:1199.
+  void grow(Bucket* b) {
+    const bool cond = !b->data.empty() && (int)b->data.size() > leaf_max && b->level < b->max_level;
+    if (!(cond && b->level < b->max_level)) return;
+    for (size_t c = 0; c < b->cells.size(); ++c) {
+      if (b->cell_data[c].empty()) continue;
+      double lo[3], hi[3];
+      b->cell_bounds((int)c, lo, hi);
+      Bucket* ch = new_bucket();
+      ch->init(lo, hi, b->dL * (0.5 + 1e-13));
+      ch->parent = b;
+      ch->level = b->level + 1;
+      ch->max_level = b->max_level;
+      for (int f : b->cell_data[c])
+        if (ch->inside(F[f].pole)) add(ch, f);
+      b->cells[c] = ch;
+      grow(ch);
+    }
+  }
+  void collect(Bucket* b) {
+    // rebuildHierarchyLists (lib_spatial_partitioning.hpp:966-984): preorder traversal
+    for (auto* c : b->cells) {
+      if (!c) continue;
+      if (!c->data.empty()) {
+        level_buckets[c->level].push_back(c);
+        if (!c->has_children()) leaves.push_back(c);
+      }
+      collect(c);
+    }
+  }
+  Bucket* deepest(const V3& x) {
+    // getBucketAtDeepest (lib_spatial_partitioning.hpp:1270-1279)
+    Bucket* cur = &root;
+    while (cur->has_children()) {
+      Bucket* nx = cur->cells[cur->cell_of(x)];
+      if (!nx) return cur;
+      cur = nx;
+    }
+    return cur;
+  }
+  bool is_far(const Bucket* A, const Bucket* B) const {
+    // moments.hpp:948-954
+    const double eff = std::max(A->dL + 2.0 * A->off, B->dL + 2.0 * B->off);
+    return theta > std::sqrt(3.0) * eff / norm(B->X - A->X);
+  }
+  void lists(Bucket* A, Bucket* B) {
+    // buildNearAndM2LLists (moments.hpp:968-979)
+    if (!is_far(A, B)) {
+      if (A != B) A->near.push_back(B);
+      for (auto* Ac : A->cells) {
+        if (!Ac) continue;
+        for (auto* Bc : B->cells) {
+          if (!Bc) continue;
+          if (Ac != Bc && is_far(Ac, Bc))
+            Ac->m2l_list.push_back(Bc);
+          else
+            lists(Ac, Bc);
+        }
+      }
+    }
+  }
+
+  // ---- setup ----------------------------------------------------------------
+  void setup(const double* X, const int* tri, const int* fdof, const int* kind, const int* dpoint, const double* phi_in,
+             const double* phin_in) {
+    // Coordinate scaling: L = bounding-box diagonal (Network.hpp:3215-3232; GMRES_FMM.hpp:3076-3091)
+    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+    for (int i = 0; i < n_nodes; ++i)
+      for (int a = 0; a < 3; ++a) {
+        lo[a] = std::min(lo[a], X[3 * i + a]);
+        hi[a] = std::max(hi[a], X[3 * i + a]);
+      }
+    Lscale = std::sqrt((hi[0] - lo[0]) * (hi[0] - lo[0]) + (hi[1] - lo[1]) * (hi[1] - lo[1]) + (hi[2] - lo[2]) * (hi[2] - lo[2]));
+    Xn.resize(n_nodes);
+    double slo[3] = {1e300, 1e300, 1e300}, shi[3] = {-1e300, -1e300, -1e300};
+    for (int i = 0; i < n_nodes; ++i)
+      for (int a = 0; a < 3; ++a) {
+        Xn[i][a] = X[3 * i + a] / Lscale;
+        slo[a] = std::min(slo[a], Xn[i][a]);
+        shi[a] = std::max(shi[a], Xn[i][a]);
+      }
+    scale = norm(V3{shi[0] - slo[0], shi[1] - slo[1], shi[2] - slo[2]});  // getScale (basic_geometry.hpp:736-739)
+    near_region = scale / 25.;
+    // DOFs, knowns (Neumann phin scaled by L: copyToFMM, GMRES_FMM.hpp:1203-1250)
+    dof_kind.assign(kind, kind + n_dof);
+    dof_point.assign(dpoint, dpoint + n_dof);
+    point_dofs.assign(n_nodes, {});
+    for (int i = 0; i < n_dof; ++i) point_dofs[dof_point[i]].push_back(i);
+    phi_known.assign(phi_in, phi_in + n_dof);
+    phin_known.resize(n_dof);
+    for (int i = 0; i < n_dof; ++i) phin_known[i] = (kind[i] != 0) ? phin_in[i] * Lscale : phin_in[i];
+    // Faces and P2M sources (GMRES_FMM.hpp:1799-1937)
+    const auto& rule = dunavant(q);
+    F.resize(n_faces);
+    for (int f = 0; f < n_faces; ++f) {
+      FaceRec& R = F[f];
+      for (int k = 0; k < 3; ++k) {
+        R.v[k] = tri[3 * f + k];
+        R.dof[k] = fdof[3 * f + k];
+        R.X[k] = Xn[R.v[k]];
+      }
+      const V3 c = cross(R.X[1] - R.X[0], R.X[2] - R.X[0]);
+      R.J = norm(c);
+      R.nhat = (1.0 / R.J) * c;
+      R.cr = R.J * R.nhat;
+      for (int a = 0; a < 3; ++a) {
+        R.centroid[a] = (R.X[0][a] + R.X[1][a] + R.X[2][a]) / 3.;
+        R.pole[a] = (1. / 3.) * R.X[0][a] + (1. / 3.) * R.X[1][a] + (1. / 3.) * R.X[2][a];
+      }
+      R.qx.clear();
+      R.qw.clear();
+      R.qb.clear();
+      R.maxoff = 0.;
+      for (const auto& r : rule) {
+        V3 xq;
+        for (int a = 0; a < 3; ++a) xq[a] = r[0] * R.X[0][a] + r[1] * R.X[1][a] + r[2] * R.X[2][a];
+        R.qx.push_back(xq);
+        R.qw.push_back(R.J * r[3]);
+        R.qb.push_back({r[0], r[1], r[2]});
+        R.maxoff = std::max(R.maxoff, norm(xq - R.pole));
+      }
+    }
+    // Root bucket (initializeBucket, GMRES_FMM.hpp:1178-1201)
+    {
+      V3 Cn;
+      for (int a = 0; a < 3; ++a) Cn[a] = (slo[a] + shi[a]) / 2.;
+      const double dL = 1.05 * scale / 2.;
+      double blo[3], bhi[3];
+      for (int a = 0; a < 3; ++a) {
+        blo[a] = Cn[a] - dL;
+        bhi[a] = Cn[a] + dL;
+      }
+      root.init(blo, bhi, dL);
+      root.level = 0;
+      root.max_level = max_level;
+    }
+    for (int f = 0; f < n_faces; ++f)
+      if (root.inside(F[f].pole)) add(&root, f);
+    grow(&root);
+    level_buckets.assign(max_level + 1, {});
+    leaves.clear();
+    if (!root.data.empty()) {
+      level_buckets[0].push_back(&root);
+      if (!root.has_children()) leaves.push_back(&root);
+    }
+    collect(&root);
+    int id = 0;
+    for (auto& lv : level_buckets)
+      for (auto* b : lv) b->id = id++;
+    // max_internal_offset (moments.hpp:1323-1337)
+    for (auto* b : leaves) {
+      b->off = 0.;
+      for (int f : b->data) b->off = std::max(b->off, F[f].maxoff);
+    }
+    for (int lv = max_level - 1; lv >= 0; --lv)
+      for (auto* b : level_buckets[lv]) {
+        if (!b->has_children()) continue;
+        b->off = 0.;
+        for (auto* c : b->cells)
+          if (c) b->off = std::max(b->off, c->off);
+      }
+    // lists (setBucketsForM2L, moments.hpp:983-997)
+    for (auto* A : root.cells) {
+      if (!A) continue;
+      for (auto* B : root.cells)
+        if (B) lists(A, B);
+    }
+    for (auto& lv : level_buckets)
+      for (auto* A : lv)
+        for (auto* B : A->m2l_list) B->l2m.push_back(A);
+    n_m2l_pairs = 0;
+    for (auto& lv : level_buckets)
+      for (auto* A : lv) n_m2l_pairs += (long long)A->l2m.size();
+    const int C = (P + 1) * (P + 1);
+    for (auto& b : pool) {
+      b->M0.assign(C, 0.);
+      b->M1.assign(C, 0.);
+      b->L0.assign(C, 0.);
+      b->L1.assign(C, 0.);
+    }
+    root.M0.assign(C, 0.);
+    root.M1.assign(C, 0.);
+    root.L0.assign(C, 0.);
+    root.L1.assign(C, 0.);
+    T.build(P);
+    // targets
+    target_box.resize(n_nodes);
+    for (int i = 0; i < n_nodes; ++i) target_box[i] = deepest(Xn[i]);
+    build_near();
+    phi.assign(n_dof, 0.);
+    phin.assign(n_dof, 0.);
+    compute_alpha();
+  }
+
+  // ---- near field (setDirectIntegration, lib_multipole_expansion.hpp:1180-1271) ----
+  void face_direct(int f, int t, std::vector<double>& accG, std::vector<double>& accGn, std::vector<int>& touched,
+                   std::vector<char>& flag) const {
+    const FaceRec& R = F[f];
+    const V3& x = Xn[t];
+    auto addw = [&](int d, double g, double gn) {
+      accG[d] += g;
+      accGn[d] += gn;
+      if (!flag[d]) {
+        flag[d] = 1;
+        touched.push_back(d);
+      }
+    };
+    int slot = -1;
+    for (int k = 0; k < 3; ++k)
+      if (R.v[k] == t) slot = k;
+    double WG[3] = {0, 0, 0}, WGn[3] = {0, 0, 0};
+    if (slot >= 0) {
+      // adjacent: fill_direct_linear (GMRES_FMM.hpp:1824-1860), GW5x5 collapsed at the target vertex
+      int perm[3];  // rotated vertex r -> original corner
+      if (slot == 0) { perm[0] = 0; perm[1] = 1; perm[2] = 2; }
+      else if (slot == 1) { perm[0] = 1; perm[1] = 2; perm[2] = 0; }
+      else { perm[0] = 2; perm[1] = 0; perm[2] = 1; }
+      double G[3] = {0, 0, 0}, Gn[3] = {0, 0, 0};
+      for (const auto& w : kGW5) {
+        const double t0 = w[0], t1 = w[1], ww = w[2];
+        const double N[3] = {t0, t1 * (1. - t0), (t0 - 1.) * (t1 - 1.)};
+        V3 xq;
+        for (int a = 0; a < 3; ++a) xq[a] = N[0] * R.X[perm[0]][a] + N[1] * R.X[perm[1]][a] + N[2] * R.X[perm[2]][a];
+        const V3 Rv = xq - x;
+        const double nr_inv = 1. / norm(Rv);
+        const double tmp = ww * (1. - t0) * nr_inv;
+        const double rc = -dot(nr_inv * Rv, R.cr);
+        for (int r = 0; r < 3; ++r) {
+          G[r] += R.J * (tmp * N[r]);
+          Gn[r] += rc * (tmp * N[r]) * nr_inv;
+        }
+      }
+      for (int r = 0; r < 3; ++r) {
+        WG[perm[r]] = G[r];
+        WGn[perm[r]] = Gn[r];
+      }
+      WGn[0] = 0.0;  // acc.add(idx0, WGN[idx[0]], 0.0): corner-0 Gn dropped
+    } else {
+      // non-adjacent: fill_nonadj_linear (GMRES_FMM.hpp:1863-1891)
+      if (norm(R.centroid - x) < near_region) {
+        for (const auto& w : kGW5) {
+          const double t0 = w[0], t1 = w[1], ww = w[2];
+          const double N[3] = {t0, t1 * (1. - t0), (t0 - 1.) * (t1 - 1.)};
+          V3 xq;
+          for (int a = 0; a < 3; ++a) xq[a] = N[0] * R.X[0][a] + N[1] * R.X[1][a] + N[2] * R.X[2][a];
+          const V3 Rv = xq - x;
+          const double nr_inv = 1. / norm(Rv);
+          const double rc = dot(nr_inv * Rv, R.cr);
+          for (int r = 0; r < 3; ++r) {
+            const double tmp = (ww * (1. - t0) * N[r]) * nr_inv;
+            WG[r] += R.J * tmp;
+            WGn[r] -= rc * (tmp * nr_inv);
+          }
+        }
+      } else {
+        for (size_t s = 0; s < R.qx.size(); ++s) {
+          const V3 Rv = R.qx[s] - x;
+          const double nr_inv = 1. / norm(Rv);
+          const double rc = dot(nr_inv * Rv, R.cr);
+          const double ww = R.qw[s] / R.J;
+          for (int r = 0; r < 3; ++r) {
+            const double tmp = ww * R.qb[s][r] * nr_inv;
+            WG[r] += R.J * tmp;
+            WGn[r] -= rc * (tmp * nr_inv);
+          }
+        }
+      }
+    }
+    for (int r = 0; r < 3; ++r) addw(R.dof[r], WG[r], WGn[r]);
+  }
+
+  std::vector<int> near_faces_of(const Bucket* bd) const {
+    std::vector<int> out;
+    auto setNear = [&](const Bucket* B) { out.insert(out.end(), B->data.begin(), B->data.end()); };
+    setNear(bd);
+    for (auto* b : bd->near) setNear(b);
+    for (const Bucket* b = bd->parent; b; b = b->parent)
+      for (auto* B : b->near)
+        if (!B->has_children()) setNear(B);
+    return out;
+  }
+
+  void build_near() {
+    near_idx.assign(n_nodes, {});
+    near_wG.assign(n_nodes, {});
+    near_wGn.assign(n_nodes, {});
+#pragma omp parallel
+    {
+      std::vector<double> accG(n_dof, 0.), accGn(n_dof, 0.);
+      std::vector<char> flag(n_dof, 0);
+      std::vector<int> touched;
+#pragma omp for schedule(dynamic, 16)
+      for (int t = 0; t < n_nodes; ++t) {
+        touched.clear();
+        for (int f : near_faces_of(target_box[t])) face_direct(f, t, accG, accGn, touched, flag);
+        std::sort(touched.begin(), touched.end());
+        for (int d : touched) {
+          near_idx[t].push_back(d);
+          near_wG[t].push_back(accG[d]);
+          near_wGn[t].push_back(accGn[d]);
+          accG[d] = accGn[d] = 0.;
+          flag[d] = 0;
+        }
+      }
+    }
+    nnz = 0;
+    for (auto& v : near_idx) nnz += (long long)v.size();
+  }
+
+  // ---- FMM passes (updateFMM, moments.hpp:1547-1574) ---------------------------
+  void p2m() {
+    const int C = (P + 1) * (P + 1);
+#pragma omp parallel for schedule(dynamic, 4)
+    for (size_t li = 0; li < leaves.size(); ++li) {
+      Bucket* b = leaves[li];
+      std::fill(b->M0.begin(), b->M0.end(), cd(0.));
+      std::fill(b->M1.begin(), b->M1.end(), cd(0.));
+      SolidR S;
+      for (int f : b->data) {
+        const FaceRec& R = F[f];
+        for (size_t s = 0; s < R.qx.size(); ++s) {
+          const auto& bb = R.qb[s];
+          const double w_phi = R.qw[s] * (bb[0] * phi[R.dof[0]] + bb[1] * phi[R.dof[1]] + bb[2] * phi[R.dof[2]]);
+          const double w_phin = R.qw[s] * (bb[0] * phin[R.dof[0]] + bb[1] * phin[R.dof[1]] + bb[2] * phin[R.dof[2]]);
+          S.eval(R.qx[s] - b->X, P);
+          for (int n = 0; n <= P; ++n)
+            for (int m = 0; m <= n; ++m) {
+              b->M0[idx(n, m)] += w_phin * S.R[idx(n, -m)];
+              b->M1[idx(n, m)] += w_phi * S.dR(n, -m, R.nhat);
+            }
+        }
+      }
+      conj_fill(b->M0);
+      conj_fill(b->M1);
+    }
+    (void)C;
+  }
+  void conj_fill(std::vector<cd>& v) const {
+    for (int n = 0; n <= P; ++n)
+      for (int m = 1; m <= n; ++m) v[idx(n, -m)] = std::conj(v[idx(n, m)]);
+  }
+  void m2m() {
+    // M2M per level bottom-up (moments.hpp:401-436, 912-915); real-field rows k>=0.
+    const int C = (P + 1) * (P + 1);
+    for (int lv = max_level - 1; lv >= 0; --lv) {
+#pragma omp parallel for schedule(dynamic, 4)
+      for (size_t bi = 0; bi < level_buckets[lv].size(); ++bi) {
+        Bucket* b = level_buckets[lv][bi];
+        if (!b->has_children()) continue;
+        std::fill(b->M0.begin(), b->M0.end(), cd(0.));
+        std::fill(b->M1.begin(), b->M1.end(), cd(0.));
+        SolidR S;
+        for (auto* c : b->cells) {
+          if (!c) continue;
+          S.eval(c->X - b->X, P);
+          for (int j = 0; j <= P; ++j)
+            for (int k = 0; k <= j; ++k) {
+              cd s0 = 0., s1 = 0.;
+              for (int n = 0; n <= j; ++n)
+                for (int m = -n; m <= n; ++m) {
+                  if (std::abs(k - m) > j - n) continue;
+                  const cd a = T.m2m[(size_t)idx(j, k) * C + idx(n, m)] * S.R[idx(n, -m)];
+                  s0 += a * c->M0[idx(j - n, k - m)];
+                  s1 += a * c->M1[idx(j - n, k - m)];
+                }
+              b->M0[idx(j, k)] += s0;
+              b->M1[idx(j, k)] += s1;
+            }
+        }
+        conj_fill(b->M0);
+        conj_fill(b->M1);
+      }
+    }
+  }
+  void m2l() {
+    // SimpleM2L (moments.hpp:493-598, 732-764): L_j^k += sum AAA_M2L * I_{j+n}^{m-k}(s-t) * M_n^m(s)
+    const int C = (P + 1) * (P + 1);
+    std::vector<Bucket*> all;
+    for (auto& lv : level_buckets)
+      for (auto* b : lv) all.push_back(b);
+    // per-(exact offset) operator cache for large problems (m2l_mode 1)
+#pragma omp parallel for schedule(dynamic, 1)
+    for (size_t ai = 0; ai < all.size(); ++ai) {
+      Bucket* A = all[ai];
+      std::fill(A->L0.begin(), A->L0.end(), cd(0.));
+      std::fill(A->L1.begin(), A->L1.end(), cd(0.));
+      Harm H;
+      std::vector<cd> Iv((2 * P + 1) * (2 * P + 1));
+      for (auto* B : A->l2m) {
+        const V3 d = B->X - A->X;
+        H.eval(d, 2 * P);
+        for (int l = 0; l <= 2 * P; ++l) {
+          const double rl = 1.0 / (std::pow(H.rho, l) * H.rho);
+          for (int qq = -l; qq <= l; ++qq) Iv[idx(l, qq)] = H.Y[idx(l, qq)] * rl;
+        }
+        for (int j = 0; j <= P; ++j)
+          for (int k = 0; k <= j; ++k) {
+            cd s0 = 0., s1 = 0.;
+            for (int n = 0; n <= P; ++n)
+              for (int m = -n; m <= n; ++m) {
+                const cd a = T.m2l[(size_t)idx(j, k) * C + idx(n, m)] * Iv[idx(j + n, m - k)];
+                s0 += a * B->M0[idx(n, m)];
+                s1 += a * B->M1[idx(n, m)];
+              }
+            A->L0[idx(j, k)] += s0;
+            A->L1[idx(j, k)] += s1;
+          }
+      }
+      conj_fill(A->L0);
+      conj_fill(A->L1);
+    }
+  }
+  void l2l() {
+    // L2L top-down (moments.hpp:862-884, 925-929)
+    const int C = (P + 1) * (P + 1);
+    for (int lv = 0; lv <= max_level; ++lv) {
+#pragma omp parallel for schedule(dynamic, 4)
+      for (size_t bi = 0; bi < level_buckets[lv].size(); ++bi) {
+        Bucket* p = level_buckets[lv][bi];
+        SolidR S;
+        for (auto* c : p->cells) {
+          if (!c) continue;
+          S.eval(p->X - c->X, P);
+          for (int j = 0; j <= P; ++j)
+            for (int k = 0; k <= j; ++k) {
+              cd s0 = 0., s1 = 0.;
+              for (int n = j; n <= P; ++n)
+                for (int m = -n; m <= n; ++m) {
+                  if (std::abs(m - k) > n - j) continue;
+                  const cd a = T.l2l[(size_t)idx(j, k) * C + idx(n, m)] * S.R[idx(n - j, m - k)];
+                  s0 += a * p->L0[idx(n, m)];
+                  s1 += a * p->L1[idx(n, m)];
+                }
+              c->L0[idx(j, k)] += s0;
+              c->L1[idx(j, k)] += s1;
+            }
+          conj_fill(c->L0);
+          conj_fill(c->L1);
+        }
+      }
+    }
+  }
+  void update_fmm() {
+    p2m();
+    m2m();
+    m2l();
+    l2l();
+  }
+  // near + far per target node (compute_Ax_minus_b inner loop, GMRES_FMM.hpp:62-108)
+  void eval_targets(std::vector<double>& gphin, std::vector<double>& gnphi) {
+    gphin.assign(n_nodes, 0.);
+    gnphi.assign(n_nodes, 0.);
+#pragma omp parallel for schedule(static)
+    for (int t = 0; t < n_nodes; ++t) {
+      double n0 = 0., n1 = 0.;
+      for (size_t i = 0; i < near_idx[t].size(); ++i) {
+        n0 += phin[near_idx[t][i]] * near_wG[t][i];
+        n1 += phi[near_idx[t][i]] * near_wGn[t][i];
+      }
+      const Bucket* b = target_box[t];
+      SolidR S;
+      S.eval(Xn[t] - b->X, P);
+      cd f0 = 0., f1 = 0.;
+      for (int n = 0; n <= P; ++n)
+        for (int m = -n; m <= n; ++m) {
+          f0 += S.R[idx(n, m)] * b->L0[idx(n, m)];
+          f1 += S.R[idx(n, m)] * b->L1[idx(n, m)];
+        }
+      gphin[t] = n0 + f0.real();
+      gnphi[t] = n1 + f1.real();
+    }
+  }
+  void set_unknowns(const double* x) {
+    for (int i = 0; i < n_dof; ++i) {
+      if (dof_kind[i] == 0) phin[i] = x[i];
+      else phi[i] = x[i];
+    }
+  }
+  void set_knowns_zero() {
+    for (int i = 0; i < n_dof; ++i) {
+      if (dof_kind[i] == 0) phi[i] = 0.;
+      else phin[i] = 0.;
+    }
+  }
+  void set_knowns() {
+    for (int i = 0; i < n_dof; ++i) {
+      if (dof_kind[i] == 0) phi[i] = phi_known[i];
+      else phin[i] = phin_known[i];
+    }
+  }
+  void compute_alpha() {
+    // rigid mode (GMRES_FMM.hpp:2674-2716): phi=1, phin=0 on all DOFs; alpha = -GnPhi
+    alpha.assign(n_dof, 1.0);
+    std::fill(phi.begin(), phi.end(), 1.0);
+    std::fill(phin.begin(), phin.end(), 0.0);
+    update_fmm();
+    std::vector<double> g0, g1;
+    eval_targets(g0, g1);
+    for (int i = 0; i < n_dof; ++i) {
+      double a = -g1[dof_point[i]];
+      if (std::abs(a) < 1e-15) a = 1.0;
+      alpha[i] = a;
+    }
+  }
+  // compute_Ax_minus_b (GMRES_FMM.hpp:34-158) with the current phi/phin
+  void ax_minus_b(std::vector<double>& Av) {
+    update_fmm();
+    std::vector<double> g0, g1;
+    eval_targets(g0, g1);
+    Av.assign(n_dof, 0.);
+    for (int i = 0; i < n_dof; ++i) {
+      const int t = dof_point[i];
+      if (dof_kind[i] == 2) Av[i] = phi[i];
+      else Av[i] = g0[t] - (g1[t] + alpha[i] * phi[i]);
+    }
+  }
+  void matvec(const double* x, double* y) {
+    set_unknowns(x);
+    set_knowns_zero();
+    std::vector<double> Av;
+    ax_minus_b(Av);
+    std::copy(Av.begin(), Av.end(), y);
+  }
+  void rhs(double* b) {
+    // solveSystemGMRES: setUnknowns(0); b = -compute_Ax_minus_b(); CORNER rows <- phi_D (GMRES_FMM.hpp:2025-2047)
+    std::vector<double> z(n_dof, 0.);
+    set_unknowns(z.data());
+    set_knowns();
+    std::vector<double> Av;
+    ax_minus_b(Av);
+    for (int i = 0; i < n_dof; ++i) b[i] = (dof_kind[i] == 2) ? phi_known[i] : -Av[i];
+  }
+
+  // ---- FMM-consistent dense operator: near weights + direct sum over far P2M sources
+  void dense(double* A) {
+    std::fill(A, A + (size_t)n_dof * n_dof, 0.);
+    std::vector<double> rowG(n_dof), rowGn(n_dof);
+    for (int t = 0; t < n_nodes; ++t) {
+      std::fill(rowG.begin(), rowG.end(), 0.);
+      std::fill(rowGn.begin(), rowGn.end(), 0.);
+      for (size_t i = 0; i < near_idx[t].size(); ++i) {
+        rowG[near_idx[t][i]] += near_wG[t][i];
+        rowGn[near_idx[t][i]] += near_wGn[t][i];
+      }
+      std::vector<char> isnear(n_faces, 0);
+      for (int f : near_faces_of(target_box[t])) isnear[f] = 1;
+      for (int f = 0; f < n_faces; ++f) {
+        if (isnear[f]) continue;
+        const FaceRec& R = F[f];
+        for (size_t s = 0; s < R.qx.size(); ++s) {
+          const V3 Rv = R.qx[s] - Xn[t];
+          const double r = norm(Rv);
+          const double g = R.qw[s] / r;
+          const double gn = -R.qw[s] * dot(Rv, R.nhat) / (r * r * r);
+          for (int k = 0; k < 3; ++k) {
+            rowG[R.dof[k]] += g * R.qb[s][k];
+            rowGn[R.dof[k]] += gn * R.qb[s][k];
+          }
+        }
+      }
+      for (int i : point_dofs[t]) {
+        double* row = A + (size_t)i * n_dof;
+        if (dof_kind[i] == 2) {
+          row[i] = 1.0;
+          continue;
+        }
+        for (int j = 0; j < n_dof; ++j) {
+          // unknown of DOF j: phin (Dirichlet) enters through G, phi (Neumann) through -Gn
+          row[j] = (dof_kind[j] == 0) ? rowG[j] : -rowGn[j];
+        }
+        if (dof_kind[i] == 1) row[i] -= alpha[i];  // alpha*phi_i is an unknown term only for Neumann rows
+      }
+    }
+  }
+
+  // ---- GMRES: run_gmres control flow (GMRES_FMM.hpp:2275-2355) with the gmres
+  // class (basic_linear_systems.hpp:3137-3451): MGS Arnoldi, Givens QR.
+  int solve(const double* x0, double* xout, double tol, int max_iter, int restart, int* iters_out, double* res_out) {
+    std::vector<double> b(n_dof);
+    rhs(b.data());
+    auto Av = [&](const std::vector<double>& v) {
+      std::vector<double> y(n_dof);
+      matvec(v.data(), y.data());
+      return y;
+    };
+    auto nrm = [](const std::vector<double>& v) {
+      double s = 0;
+      for (double a : v) s += a * a;
+      return std::sqrt(s);
+    };
+    std::vector<double> x(x0, x0 + n_dof);
+    int iters = 0;
+    double cur_err = 1e300;
+    const int restart_max = std::max(1, restart);
+    while (iters < max_iter && cur_err > tol) {
+      const int remaining = max_iter - iters;
+      const int k_cycle_max = std::min(restart_max, remaining);
+      // gmres(A, b, x, 1): r0 = b - A x; Arnoldi with one vector
+      std::vector<double> r0 = Av(x);
+      for (int i = 0; i < n_dof; ++i) r0[i] = b[i] - r0[i];
+      const double beta = nrm(r0);
+      std::vector<std::vector<double>> V;
+      std::vector<std::vector<double>> H;  // H[i][j]
+      std::vector<double> cs, sn, g;
+      std::vector<std::vector<double>> Rm;
+      const std::vector<double> xbase = x;
+      auto add_basis = [&](const std::vector<double>& vlast) {
+        std::vector<double> w = Av(vlast);
+        const size_t nb = V.size();
+        std::vector<double> hcol(nb + 1, 0.);
+        for (size_t i = 0; i < nb; ++i) {
+          double h = 0;
+          for (int a = 0; a < n_dof; ++a) h += V[i][a] * w[a];
+          hcol[i] = h;
+          for (int a = 0; a < n_dof; ++a) w[a] -= h * V[i][a];
+        }
+        const double hn = nrm(w);
+        hcol[nb] = hn;
+        for (int a = 0; a < n_dof; ++a) w[a] /= hn;
+        V.push_back(w);
+        return hcol;
+      };
+      auto givens_col = [&](std::vector<double> col) {
+        const size_t c = cs.size();
+        for (size_t i = 0; i < c; ++i) {
+          const double tmp = std::fma(cs[i], col[i], sn[i] * col[i + 1]);
+          col[i + 1] = std::fma(cs[i], col[i + 1], -sn[i] * col[i]);
+          col[i] = tmp;
+        }
+        double cc = 1.0, ss = 0.0;
+        const double r = std::hypot(col[c], col[c + 1]);
+        if (r != 0.0) {
+          cc = col[c] / r;
+          ss = col[c + 1] / r;
+        }
+        const double tmp = std::fma(cc, col[c], ss * col[c + 1]);
+        col[c + 1] = std::fma(cc, col[c + 1], -ss * col[c]);
+        col[c] = tmp;
+        cs.push_back(cc);
+        sn.push_back(ss);
+        g.push_back(0.);
+        const double gt = std::fma(cc, g[c], ss * g[c + 1]);
+        g[c + 1] = std::fma(cc, g[c + 1], -ss * g[c]);
+        g[c] = gt;
+        Rm.push_back(col);
+      };
+      auto update_solution = [&]() {
+        x = xbase;
+        if (beta == 0.) return;
+        const size_t n = cs.size();
+        std::vector<double> y(n, 0.);
+        for (int i = (int)n - 1; i >= 0; --i) {
+          double s = g[i];
+          for (size_t j = i + 1; j < n; ++j) s -= Rm[j][i] * y[j];
+          y[i] = s / Rm[i][i];
+        }
+        for (size_t i = 0; i < n; ++i)
+          for (int a = 0; a < n_dof; ++a) x[a] += y[i] * V[i][a];
+      };
+      std::vector<double> v0(n_dof);
+      for (int a = 0; a < n_dof; ++a) v0[a] = r0[a] / beta;
+      V.push_back(v0);
+      g.assign(1, beta);
+      givens_col(add_basis(v0));
+      iters += 1;
+      double est = std::abs(g.back());
+      int sat = 0;
+      while (true) {
+        givens_col(add_basis(V.back()));
+        ++iters;
+        est = std::abs(g.back());
+        if (est <= tol * 10.) {
+          update_solution();
+          std::vector<double> r = Av(x);
+          for (int a = 0; a < n_dof; ++a) r[a] -= b[a];
+          cur_err = nrm(r);
+          if (cur_err <= tol) break;
+          if (sat++ >= 5) break;
+        }
+        if (iters >= max_iter && (int)cs.size() >= k_cycle_max) {
+          update_solution();
+          if (est <= tol * 10.) {
+            std::vector<double> r = Av(x);
+            for (int a = 0; a < n_dof; ++a) r[a] -= b[a];
+            cur_err = nrm(r);
+          } else
+            cur_err = est;
+          break;
+        }
+      }
+    }
+    // unscale phin on Dirichlet DOFs (GMRES_FMM.hpp:3462-3486)
+    for (int i = 0; i < n_dof; ++i) xout[i] = (dof_kind[i] == 0) ? x[i] / Lscale : x[i];
+    if (iters_out) *iters_out = iters;
+    if (res_out) *res_out = cur_err;
+    return cur_err <= tol ? 0 : 1;
+  }
+};
+
+}  // namespace
+
+extern "C" {
+
+struct oracle_stats {
+  int n_levels;
+  int n_buckets;
+  int n_leaves;
+  long long n_m2l_pairs;
+  long long near_nnz;
+  double L;
+  double scale;
+};
+
+void* oracle_create(int n_nodes, const double* X, int n_faces, const int* tri, const int* face_dof, int n_dof,
+                    const int* dof_kind, const int* dof_point, const double* phi, const double* phin, int p,
+                    double theta, int leaf_max, int max_level, int q) {
+  try {
+    auto* o = new Oracle();
+    o->n_nodes = n_nodes;
+    o->n_faces = n_faces;
+    o->n_dof = n_dof;
+    o->P = p;
+    o->theta = theta;
+    o->leaf_max = leaf_max;
+    o->max_level = max_level;
+    o->q = q;
+    o->setup(X, tri, face_dof, dof_kind, dof_point, phi, phin);
+    return o;
+  } catch (...) {
+    return nullptr;
+  }
+}
+void oracle_destroy(void* h) { delete static_cast<Oracle*>(h); }
+void oracle_matvec(void* h, const double* x, double* y) { static_cast<Oracle*>(h)->matvec(x, y); }
+void oracle_rhs(void* h, double* b) { static_cast<Oracle*>(h)->rhs(b); }
+void oracle_alpha(void* h, double* a) {
+  auto* o = static_cast<Oracle*>(h);
+  std::copy(o->alpha.begin(), o->alpha.end(), a);
+}
+void oracle_dense(void* h, double* A) { static_cast<Oracle*>(h)->dense(A); }
+int oracle_solve(void* h, const double* x0, double* x, double tol, int max_iter, int restart, int* iters, double* res) {
+  return static_cast<Oracle*>(h)->solve(x0, x, tol, max_iter, restart, iters, res);
+}
+void oracle_get_stats(void* h, oracle_stats* s) {
+  auto* o = static_cast<Oracle*>(h);
+  s->n_levels = (int)o->level_buckets.size();
+  int nb = 0;
+  for (auto& lv : o->level_buckets) nb += (int)lv.size();
+  s->n_buckets = nb;
+  s->n_leaves = (int)o->leaves.size();
+  s->n_m2l_pairs = o->n_m2l_pairs;
+  s->near_nnz = o->nnz;
+  s->L = o->Lscale;
+  s->scale = o->scale;
+}
+// near-field row of a target node (sorted DOF indices) — for structural parity
+int oracle_near_row(void* h, int t, int* idx_out, double* wg, double* wgn, int cap) {
+  auto* o = static_cast<Oracle*>(h);
+  const int n = (int)o->near_idx[t].size();
+  for (int i = 0; i < std::min(n, cap); ++i) {
+    idx_out[i] = o->near_idx[t][i];
+    wg[i] = o->near_wG[t][i];
+    wgn[i] = o->near_wGn[t][i];
+  }
+  return n;
+}
+
+// Harmonics probe for known-answer tests: Y_n^m (reference convention) at x, all |m|<=n<=P.
+void oracle_sph_harmonics(const double* x, int P, double* out_re, double* out_im) {
+  Harm h;
+  h.eval({x[0], x[1], x[2]}, P);
+  for (int i = 0; i < (P + 1) * (P + 1); ++i) {
+    out_re[i] = h.Y[i].real();
+    out_im[i] = h.Y[i].imag();
+  }
+}
+// grad(R_n^q).nhat for all (n,q)
+void oracle_dR(const double* x, const double* nh, int P, double* out_re, double* out_im) {
+  SolidR S;
+  S.eval({x[0], x[1], x[2]}, P);
+  for (int n = 0; n <= P; ++n)
+    for (int q = -n; q <= n; ++q) {
+      const cd v = S.dR(n, q, {nh[0], nh[1], nh[2]});
+      out_re[idx(n, q)] = v.real();
+      out_im[idx(n, q)] = v.imag();
+    }
+}
+// Point-source FMM invariant: potential of unit charges q_i at y_i (and dipoles d_i along nh_i),
+// expanded P2M about c1, translated M2M to c2, M2L to c3, L2L to c4, evaluated L2P at x.
+// Returns (single-layer, double-layer) potentials.
+void oracle_translate_chain(int ns, const double* ys, const double* qs, const double* nhs, const double* c1,
+                            const double* c2, const double* c3, const double* c4, const double* x, int P, double* out) {
+  Tables T;
+  T.build(P);
+  const int C = (P + 1) * (P + 1);
+  std::vector<cd> M0(C, 0.), M1(C, 0.);
+  SolidR S;
+  for (int s = 0; s < ns; ++s) {
+    const V3 y = {ys[3 * s], ys[3 * s + 1], ys[3 * s + 2]};
+    const V3 nh = {nhs[3 * s], nhs[3 * s + 1], nhs[3 * s + 2]};
+    S.eval(y - V3{c1[0], c1[1], c1[2]}, P);
+    for (int n = 0; n <= P; ++n)
+      for (int m = -n; m <= n; ++m) {
+        M0[idx(n, m)] += qs[s] * S.R[idx(n, -m)];
+        M1[idx(n, m)] += qs[s] * S.dR(n, -m, nh);
+      }
+  }
+  // M2M c1 -> c2
+  std::vector<cd> N0(C, 0.), N1(C, 0.);
+  S.eval(V3{c1[0] - c2[0], c1[1] - c2[1], c1[2] - c2[2]}, P);
+  for (int j = 0; j <= P; ++j)
+    for (int k = -j; k <= j; ++k)
+      for (int n = 0; n <= j; ++n)
+        for (int m = -n; m <= n; ++m) {
+          if (std::abs(k - m) > j - n) continue;
+          const cd a = T.m2m[(size_t)idx(j, k) * C + idx(n, m)] * S.R[idx(n, -m)];
+          N0[idx(j, k)] += a * M0[idx(j - n, k - m)];
+          N1[idx(j, k)] += a * M1[idx(j - n, k - m)];
+        }
+  // M2L c2 -> c3
+  std::vector<cd> L0(C, 0.), L1(C, 0.);
+  Harm H;
+  H.eval(V3{c2[0] - c3[0], c2[1] - c3[1], c2[2] - c3[2]}, 2 * P);
+  for (int j = 0; j <= P; ++j)
+    for (int k = -j; k <= j; ++k)
+      for (int n = 0; n <= P; ++n)
+        for (int m = -n; m <= n; ++m) {
+          const double rl = 1.0 / (std::pow(H.rho, j + n) * H.rho);
+          const cd a = T.m2l[(size_t)idx(j, k) * C + idx(n, m)] * H.Y[idx(j + n, m - k)] * rl;
+          L0[idx(j, k)] += a * N0[idx(n, m)];
+          L1[idx(j, k)] += a * N1[idx(n, m)];
+        }
+  // L2L c3 -> c4
+  std::vector<cd> K0(C, 0.), K1(C, 0.);
+  S.eval(V3{c3[0] - c4[0], c3[1] - c4[1], c3[2] - c4[2]}, P);
+  for (int j = 0; j <= P; ++j)
+    for (int k = -j; k <= j; ++k)
+      for (int n = j; n <= P; ++n)
+        for (int m = -n; m <= n; ++m) {
+          if (std::abs(m - k) > n - j) continue;
+          const cd a = T.l2l[(size_t)idx(j, k) * C + idx(n, m)] * S.R[idx(n - j, m - k)];
+          K0[idx(j, k)] += a * L0[idx(n, m)];
+          K1[idx(j, k)] += a * L1[idx(n, m)];
+        }
+  S.eval(V3{x[0] - c4[0], x[1] - c4[1], x[2] - c4[2]}, P);
+  cd f0 = 0., f1 = 0.;
+  for (int n = 0; n <= P; ++n)
+    for (int m = -n; m <= n; ++m) {
+      f0 += S.R[idx(n, m)] * K0[idx(n, m)];
+      f1 += S.R[idx(n, m)] * K1[idx(n, m)];
+    }
+  out[0] = f0.real();
+  out[1] = f1.real();
+  out[2] = f0.imag();
+  out[3] = f1.imag();
+}
+
+}  // extern "C"

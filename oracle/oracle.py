@@ -1,0 +1,159 @@
+"""ctypes binding of the CPU oracle (test infrastructure only; never imported by the product).
+
+The oracle is an FP64 CPU restatement of the reference's FMM-GMRES BVP path
+(see oracle.cpp for the per-stage citations). ``build()`` compiles it with the
+Makefile next to this file.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "_build", "liboracle.so")
+_lib = None
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+
+
+class OracleStats(ctypes.Structure):
+    _fields_ = [("n_levels", ctypes.c_int), ("n_buckets", ctypes.c_int), ("n_leaves", ctypes.c_int),
+                ("n_m2l_pairs", ctypes.c_longlong), ("near_nnz", ctypes.c_longlong), ("L", ctypes.c_double),
+                ("scale", ctypes.c_double)]
+
+
+def build(force: bool = False) -> str:
+    if force or not os.path.exists(_LIB_PATH) or os.path.getmtime(_LIB_PATH) < os.path.getmtime(
+            os.path.join(_HERE, "oracle.cpp")):
+        subprocess.run(["make", "-s", "-C", _HERE], check=True)
+    return _LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = ctypes.CDLL(_LIB_PATH)
+        L.oracle_create.restype = ctypes.c_void_p
+        L.oracle_create.argtypes = [ctypes.c_int, _dp, ctypes.c_int, _ip, _ip, ctypes.c_int, _ip, _ip, _dp, _dp,
+                                    ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+        L.oracle_destroy.argtypes = [ctypes.c_void_p]
+        L.oracle_matvec.argtypes = [ctypes.c_void_p, _dp, _dp]
+        L.oracle_rhs.argtypes = [ctypes.c_void_p, _dp]
+        L.oracle_alpha.argtypes = [ctypes.c_void_p, _dp]
+        L.oracle_dense.argtypes = [ctypes.c_void_p, _dp]
+        L.oracle_solve.restype = ctypes.c_int
+        L.oracle_solve.argtypes = [ctypes.c_void_p, _dp, _dp, ctypes.c_double, ctypes.c_int, ctypes.c_int, _ip, _dp]
+        L.oracle_get_stats.argtypes = [ctypes.c_void_p, ctypes.POINTER(OracleStats)]
+        L.oracle_near_row.restype = ctypes.c_int
+        L.oracle_near_row.argtypes = [ctypes.c_void_p, ctypes.c_int, _ip, _dp, _dp, ctypes.c_int]
+        L.oracle_sph_harmonics.argtypes = [_dp, ctypes.c_int, _dp, _dp]
+        L.oracle_dR.argtypes = [_dp, _dp, ctypes.c_int, _dp, _dp]
+        L.oracle_translate_chain.argtypes = [ctypes.c_int, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, ctypes.c_int, _dp]
+        _lib = L
+    return _lib
+
+
+def _d(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _i(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def _p(a, t):
+    return a.ctypes.data_as(t)
+
+
+class Oracle:
+    """CPU FP64 FMM-GMRES problem: same inputs as the product's C ABI."""
+
+    def __init__(self, mesh, dofs, phi, phin, p=10, theta=0.25, leaf_max=50, max_level=7, q=6):
+        L = lib()
+        self._keep = [_d(mesh.X), _i(mesh.tri), _i(dofs.face_dof), _i(dofs.dof_kind), _i(dofs.dof_point), _d(phi),
+                      _d(phin)]
+        X, tri, fd, kind, pt, ph, phn = self._keep
+        self.n_dof = int(dofs.n_dof)
+        self.h = L.oracle_create(mesh.n_nodes, _p(X, _dp), mesh.n_faces, _p(tri, _ip), _p(fd, _ip), self.n_dof,
+                                 _p(kind, _ip), _p(pt, _ip), _p(ph, _dp), _p(phn, _dp), p, theta, leaf_max,
+                                 max_level, q)
+        if not self.h:
+            raise RuntimeError("oracle_create failed")
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().oracle_destroy(self.h)
+            self.h = None
+
+    def matvec(self, x):
+        x = _d(x)
+        y = np.empty(self.n_dof)
+        lib().oracle_matvec(self.h, _p(x, _dp), _p(y, _dp))
+        return y
+
+    def rhs(self):
+        b = np.empty(self.n_dof)
+        lib().oracle_rhs(self.h, _p(b, _dp))
+        return b
+
+    def alpha(self):
+        a = np.empty(self.n_dof)
+        lib().oracle_alpha(self.h, _p(a, _dp))
+        return a
+
+    def dense(self):
+        A = np.empty((self.n_dof, self.n_dof))
+        lib().oracle_dense(self.h, _p(A, _dp))
+        return A
+
+    def solve(self, x0=None, tol=1e-10, max_iter=1000, restart=1000):
+        x0 = np.zeros(self.n_dof) if x0 is None else _d(x0)
+        x = np.empty(self.n_dof)
+        it = ctypes.c_int(0)
+        res = ctypes.c_double(0)
+        rc = lib().oracle_solve(self.h, _p(x0, _dp), _p(x, _dp), tol, max_iter, restart, ctypes.byref(it),
+                                ctypes.byref(res))
+        return x, {"rc": rc, "iters": it.value, "residual": res.value}
+
+    def stats(self):
+        s = OracleStats()
+        lib().oracle_get_stats(self.h, ctypes.byref(s))
+        return {k: getattr(s, k) for k, _ in OracleStats._fields_}
+
+    def near_row(self, t, cap=100000):
+        idx = np.empty(cap, np.int32)
+        wg = np.empty(cap)
+        wgn = np.empty(cap)
+        n = lib().oracle_near_row(self.h, t, _p(idx, _ip), _p(wg, _dp), _p(wgn, _dp), cap)
+        return idx[:n].copy(), wg[:n].copy(), wgn[:n].copy()
+
+
+def sph_harmonics(x, P):
+    x = _d(x)
+    re = np.empty((P + 1) ** 2)
+    im = np.empty((P + 1) ** 2)
+    lib().oracle_sph_harmonics(_p(x, _dp), P, _p(re, _dp), _p(im, _dp))
+    return re + 1j * im
+
+
+def grad_R_dot_n(x, nh, P):
+    x, nh = _d(x), _d(nh)
+    re = np.empty((P + 1) ** 2)
+    im = np.empty((P + 1) ** 2)
+    lib().oracle_dR(_p(x, _dp), _p(nh, _dp), P, _p(re, _dp), _p(im, _dp))
+    return re + 1j * im
+
+
+def translate_chain(ys, qs, nhs, c1, c2, c3, c4, x, P):
+    ys, qs, nhs = _d(ys), _d(qs), _d(nhs)
+    c1, c2, c3, c4, x = (_d(v) for v in (c1, c2, c3, c4, x))
+    out = np.empty(4)
+    lib().oracle_translate_chain(len(qs), _p(ys, _dp), _p(qs, _dp), _p(nhs, _dp), _p(c1, _dp), _p(c2, _dp),
+                                 _p(c3, _dp), _p(c4, _dp), _p(x, _dp), P, _p(out, _dp))
+    return out
