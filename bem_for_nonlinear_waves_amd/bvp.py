@@ -1,0 +1,145 @@
+"""Host-side mirror of the reference's BEM_BVP GMRES/FMM entry point.
+
+``BEM_BVP`` wraps one ``bemfmm_ctx`` (include/bemfmm.h). Method names follow the
+reference (bem/BEM_solveBVP_GMRES_FMM.hpp): ``solveGMRES`` (:3056) performs the
+setup then ``solveSystemGMRES`` (:2007); ``compute_Ax`` corresponds to
+``return_A_dot_v`` without preconditioner (:2124); ``diag_coeffs`` is the
+rigid-mode alpha (:2674-2716). Errors raise ``BEMError`` carrying the C return
+code, the analogue of the reference's ``error_message`` / non-converged flag.
+"""
+from __future__ import annotations
+
+import ctypes
+import time
+
+import numpy as np
+
+from . import _lib
+from .mesh import DofModel, Mesh
+
+
+class BEMError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"bemfmm error {code}: {msg}")
+        self.code = code
+
+
+def _d(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _i(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def _ptr(a, t):
+    return a.ctypes.data_as(t)
+
+
+def is_available() -> bool:
+    return bool(_lib.load().bemfmm_is_available())
+
+
+def device_name() -> str:
+    return _lib.load().bemfmm_device_name().decode()
+
+
+class BEM_BVP:
+    """One BVP on one GPU: geometry + DOF model + boundary values -> FMM-GMRES solution."""
+
+    def __init__(self, mesh: Mesh, dofs: DofModel, p: int = 10, theta: float = 0.25, leaf_max: int = 50,
+                 max_level: int = 7, q: int = 6, precision: int = _lib.PRECISION_FP64,
+                 preconditioner: int = _lib.PRECOND_NONE, device: int = 0):
+        self.lib = _lib.load()
+        o = _lib.Opts()
+        self.lib.bemfmm_default_opts(ctypes.byref(o))
+        o.p, o.theta, o.leaf_max, o.max_level, o.q = p, theta, leaf_max, max_level, q
+        o.precision, o.preconditioner = precision, preconditioner
+        self.ctx = self.lib.bemfmm_ctx_create(device, ctypes.byref(o))
+        if not self.ctx:
+            raise BEMError(_lib.E_DEVICE, "bemfmm_ctx_create failed (no usable GPU?)")
+        self.n_dof = int(dofs.n_dof)
+        self._geom = [_d(mesh.X), _i(mesh.tri), _i(dofs.face_dof), _i(dofs.dof_kind), _i(dofs.dof_point)]
+        X, tri, fd, kd, pt = self._geom
+        self._check(self.lib.bemfmm_set_geometry(self.ctx, mesh.n_nodes, _ptr(X, _lib._dp), mesh.n_faces,
+                                                 _ptr(tri, _lib._ip), _ptr(fd, _lib._ip), self.n_dof,
+                                                 _ptr(kd, _lib._ip), _ptr(pt, _lib._ip)))
+        self.prepared = False
+        self.time_setup = 0.0
+
+    def _check(self, rc: int):
+        if rc != _lib.OK:
+            raise BEMError(rc, self.lib.bemfmm_last_error(self.ctx).decode())
+
+    def close(self):
+        if getattr(self, "ctx", None):
+            self.lib.bemfmm_ctx_destroy(self.ctx)
+            self.ctx = None
+
+    def __del__(self):
+        self.close()
+
+    # -- setup ------------------------------------------------------------------
+    def set_bc(self, phi, phin):
+        self._bc = [_d(phi), _d(phin)]
+        self._check(self.lib.bemfmm_set_bc(self.ctx, _ptr(self._bc[0], _lib._dp), _ptr(self._bc[1], _lib._dp)))
+
+    def prepare(self):
+        t = time.perf_counter()
+        self._check(self.lib.bemfmm_prepare(self.ctx))
+        self.time_setup = time.perf_counter() - t
+        self.prepared = True
+
+    # -- operators ----------------------------------------------------------------
+    def compute_Ax(self, x) -> np.ndarray:
+        x = _d(x)
+        y = np.empty(self.n_dof)
+        self._check(self.lib.bemfmm_matvec(self.ctx, _ptr(x, _lib._dp), _ptr(y, _lib._dp)))
+        return y
+
+    def matvec_device(self, x_ptr: int, y_ptr: int):
+        self._check(self.lib.bemfmm_matvec_device(self.ctx, ctypes.c_void_p(x_ptr), ctypes.c_void_p(y_ptr)))
+
+    def sync(self):
+        self._check(self.lib.bemfmm_sync(self.ctx))
+
+    def rhs(self) -> np.ndarray:
+        b = np.empty(self.n_dof)
+        self._check(self.lib.bemfmm_rhs(self.ctx, _ptr(b, _lib._dp)))
+        return b
+
+    def diag_coeffs(self) -> np.ndarray:
+        a = np.empty(self.n_dof)
+        self._check(self.lib.bemfmm_alpha(self.ctx, _ptr(a, _lib._dp)))
+        return a
+
+    def stats(self) -> dict:
+        s = _lib.Stats()
+        self._check(self.lib.bemfmm_get_stats(self.ctx, ctypes.byref(s)))
+        return {k: getattr(s, k) for k, _ in _lib.Stats._fields_}
+
+    def profile(self):
+        out = np.zeros(7)
+        self._check(self.lib.bemfmm_matvec_profile(self.ctx, _ptr(out, _lib._dp)))
+        return out
+
+    # -- solve --------------------------------------------------------------------
+    def solveSystemGMRES(self, x0=None, tol: float = 1e-10, max_iter: int = 1000, restart: int = 1000,
+                         raise_on_nonconvergence: bool = False):
+        x0 = np.zeros(self.n_dof) if x0 is None else _d(x0)
+        x = np.empty(self.n_dof)
+        r = _lib.Result()
+        rc = self.lib.bemfmm_solve(self.ctx, _ptr(x0, _lib._dp), _ptr(x, _lib._dp), tol, max_iter, restart,
+                                   ctypes.byref(r))
+        info = {k: getattr(r, k) for k, _ in _lib.Result._fields_}
+        info["time_setup"] = self.time_setup
+        if rc == _lib.E_NOT_CONVERGED and not raise_on_nonconvergence:
+            return x, info
+        self._check(rc)
+        return x, info
+
+    def solveGMRES(self, phi, phin, x0=None, tol: float = 1e-10, max_iter: int = 1000, restart: int = 1000):
+        """Setup (scaling, tree, operators, near field, alpha) + GMRES, as BEM_BVP::solveGMRES."""
+        self.set_bc(phi, phin)
+        self.prepare()
+        return self.solveSystemGMRES(x0, tol, max_iter, restart)

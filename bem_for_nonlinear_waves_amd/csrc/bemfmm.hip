@@ -1,0 +1,1016 @@
+// MI355X (gfx950) device side of the FMM-accelerated GMRES BVP engine and its C ABI.
+//
+// Matvec pipeline on one HIP stream, all data resident in HBM (the reference's
+// compute_Ax_minus_b, bem/BEM_solveBVP_GMRES_FMM.hpp:34-158, with updateFMM,
+// lib/include/moments.hpp:1547-1574):
+//   scatter   x + knowns -> phi_n, phi per DOF                (setUnknowns/setKnowns, :1142-1176)
+//   P2M       per-leaf dense GEMV over the leaf's DOF columns (increment_M_reuse, moments.hpp:358-368)
+//   M2M/M2L/L2L  operator-grouped batched dense products       (moments.hpp:419-436, 732-764, 869-884)
+//   near+L2P  per-target-box dense GEMV over near DOF columns and the box's local coefficients
+//             (integrateNearField/integrateFarField, lib_multipole_expansion.hpp:1143-1150, 1273-1302)
+//   assemble  Av_i = GPhin - (GnPhi + alpha_i phi_i); CORNER rows Av_i = phi_i
+// Setup: the host builds the plan (plan.cpp); the near-field quadrature
+// (GW5x5 singular/near, Dunavant far; GMRES_FMM.hpp:1824-1891) runs on the GPU.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/bemfmm.h"
+#include "nearquad.hpp"
+#include "plan.hpp"
+
+namespace {
+
+using bemfmm::Plan;
+
+struct HipError : std::runtime_error {
+  int code;
+  HipError(const std::string& s, int c) : std::runtime_error(s), code(c) {}
+};
+
+#define HIPCHECK(expr)                                                                                        \
+  do {                                                                                                        \
+    hipError_t e_ = (expr);                                                                                   \
+    if (e_ != hipSuccess)                                                                                     \
+      throw HipError(std::string(#expr) + ": " + hipGetErrorString(e_),                                      \
+                     e_ == hipErrorOutOfMemory ? BEMFMM_E_OOM : BEMFMM_E_DEVICE);                             \
+  } while (0)
+
+double now() { return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count(); }
+
+// ---------------------------------------------------------------------------
+// GW5x5 product rule on the unit square, (t0, t1, w) (reference basic_constants.hpp:1326-1350).
+// ---------------------------------------------------------------------------
+__constant__ double c_gw5[25][3] = {
+    {0.046910077030668, 0.046910077030668, 0.0140335872156063}, {0.046910077030668, 0.230765344947158, 0.0283499999999991},
+    {0.046910077030668, 0.5, 0.0336962680968792},               {0.046910077030668, 0.769234655052841, 0.0283499999999991},
+    {0.046910077030668, 0.953089922969332, 0.0140335872156063}, {0.230765344947158, 0.046910077030668, 0.0283499999999991},
+    {0.230765344947158, 0.230765344947158, 0.0572713510559978}, {0.230765344947158, 0.5, 0.0680716331376877},
+    {0.230765344947158, 0.769234655052841, 0.0572713510559978}, {0.230765344947158, 0.953089922969332, 0.0283499999999991},
+    {0.5, 0.046910077030668, 0.0336962680968792},               {0.5, 0.230765344947158, 0.0680716331376877},
+    {0.5, 0.5, 0.0809086419753086},                             {0.5, 0.769234655052841, 0.0680716331376877},
+    {0.5, 0.953089922969332, 0.0336962680968792},               {0.769234655052841, 0.046910077030668, 0.0283499999999991},
+    {0.769234655052841, 0.230765344947158, 0.0572713510559978}, {0.769234655052841, 0.5, 0.0680716331376877},
+    {0.769234655052841, 0.769234655052841, 0.0572713510559978}, {0.769234655052841, 0.953089922969332, 0.0283499999999991},
+    {0.953089922969332, 0.046910077030668, 0.0140335872156063}, {0.953089922969332, 0.230765344947158, 0.0283499999999991},
+    {0.953089922969332, 0.5, 0.0336962680968792},               {0.953089922969332, 0.769234655052841, 0.0283499999999991},
+    {0.953089922969332, 0.953089922969332, 0.0140335872156063}};
+// Dunavant rule used for non-adjacent, non-near faces ({b0,b1,b2,w}); filled per context (q points).
+__constant__ double c_dun[7][4];
+
+// ---------------------------------------------------------------------------
+// kernels
+// ---------------------------------------------------------------------------
+__global__ void k_fill(double* __restrict__ p, int64_t n, double v) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) p[i] = v;
+}
+
+// mode 0: matvec (unknowns = x, knowns = 0); 1: rhs (unknowns = 0, knowns = BC); 2: rigid mode (phi = 1, phi_n = 0)
+__global__ void k_scatter(int mode, int n, const int32_t* __restrict__ kind, const double* __restrict__ x,
+                          const double* __restrict__ phi_k, const double* __restrict__ phin_k, double* __restrict__ phin,
+                          double* __restrict__ phi) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const bool dir = kind[i] == 0;
+  double a = 0., b = 0.;  // a = phi_n, b = phi
+  if (mode == 0) {
+    if (dir) a = x[i];
+    else b = x[i];
+  } else if (mode == 1) {
+    if (dir) b = phi_k[i];
+    else a = phin_k[i];
+  } else {
+    b = 1.0;
+  }
+  phin[i] = a;
+  phi[i] = b;
+}
+
+// Box-blocked dense GEMV (see bemfmm::BlockGemv). One workgroup per block,
+// gathered inputs staged through LDS in chunks, one wavefront per row.
+constexpr int kGemvChunk = 2048;
+constexpr int kGemvMaxRows = 512;
+template <typename T>
+__global__ __launch_bounds__(256) void k_block_gemv(const int64_t* __restrict__ row_ptr, const int64_t* __restrict__ col_ptr,
+                                                    const int64_t* __restrict__ mat_ptr, const int32_t* __restrict__ row_out,
+                                                    const int32_t* __restrict__ col_idx, const T* __restrict__ mat,
+                                                    const double* __restrict__ x0, const double* __restrict__ x1,
+                                                    double* __restrict__ y0, double* __restrict__ y1) {
+  __shared__ double xs0[kGemvChunk], xs1[kGemvChunk];
+  __shared__ double acc[2 * kGemvMaxRows];
+  const int b = blockIdx.x;
+  const int64_t r0 = row_ptr[b];
+  const int nrow = (int)(row_ptr[b + 1] - r0);
+  const int64_t c0 = col_ptr[b];
+  const int ncol = (int)(col_ptr[b + 1] - c0);
+  const T* M = mat + mat_ptr[b];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (int i = threadIdx.x; i < 2 * nrow; i += blockDim.x) acc[i] = 0.;
+  for (int cs = 0; cs < ncol; cs += kGemvChunk) {
+    const int cn = min(kGemvChunk, ncol - cs);
+    __syncthreads();
+    for (int i = threadIdx.x; i < cn; i += blockDim.x) {
+      const int g = col_idx[c0 + cs + i];
+      xs0[i] = x0[g];
+      xs1[i] = x1[g];
+    }
+    __syncthreads();
+    for (int r = wave; r < nrow; r += 4) {
+      const T* A0 = M + (int64_t)r * 2 * ncol + cs;
+      const T* A1 = A0 + ncol;
+      double s0 = 0., s1 = 0.;
+      for (int c = lane; c < cn; c += 64) {
+        s0 += (double)A0[c] * xs0[c];
+        s1 += (double)A1[c] * xs1[c];
+      }
+      for (int off = 32; off > 0; off >>= 1) {
+        s0 += __shfl_down(s0, off);
+        s1 += __shfl_down(s1, off);
+      }
+      if (lane == 0) {
+        acc[2 * r] += s0;
+        acc[2 * r + 1] += s1;
+      }
+    }
+  }
+  __syncthreads();
+  for (int r = threadIdx.x; r < nrow; r += blockDim.x) {
+    const int o = row_out[r0 + r];
+    y0[o] = acc[2 * r];
+    y1[o] = acc[2 * r + 1];
+  }
+}
+
+// Operator-grouped translation: for every pair (dst, src) of a tile (<= 16 pairs, one operator),
+// D[dst] += T_op * S[src] for both layers. T_op is C x C stored [k_in][r_out].
+constexpr int kTP = 16;
+constexpr int kTCols = 2 * kTP;
+constexpr int kTStride = kTCols + 1;
+__global__ __launch_bounds__(256) void k_translate(int C, const double* __restrict__ ops, const int32_t* __restrict__ tile_op,
+                                                   const int64_t* __restrict__ tile_p0, const int64_t* __restrict__ tile_p1,
+                                                   const int32_t* __restrict__ dst, const int32_t* __restrict__ src,
+                                                   const double* __restrict__ S0, const double* __restrict__ S1,
+                                                   double* __restrict__ D0, double* __restrict__ D1) {
+  extern __shared__ double Bs[];  // C * kTStride
+  const int t = blockIdx.x;
+  const int op = tile_op[t];
+  const int64_t p0 = tile_p0[t];
+  const int np = (int)(tile_p1[t] - p0);
+  for (int i = threadIdx.x; i < kTCols * C; i += blockDim.x) {
+    const int j = i / C, k = i - j * C;  // k fastest: coalesced global reads
+    const int pair = j >> 1, layer = j & 1;
+    double v = 0.;
+    if (pair < np) v = (layer ? S1 : S0)[(int64_t)src[p0 + pair] * C + k];
+    Bs[k * kTStride + j] = v;
+  }
+  __syncthreads();
+  const int r = threadIdx.x & 127;
+  const int cg = threadIdx.x >> 7;
+  if (r >= C) return;
+  const double* O = ops + (int64_t)op * C * C;
+  double acc[kTP];
+#pragma unroll
+  for (int j = 0; j < kTP; ++j) acc[j] = 0.;
+  for (int k = 0; k < C; ++k) {
+    const double a = O[(int64_t)k * C + r];
+    const double* bk = Bs + k * kTStride + cg * kTP;
+#pragma unroll
+    for (int j = 0; j < kTP; ++j) acc[j] += a * bk[j];
+  }
+#pragma unroll
+  for (int j = 0; j < kTP; ++j) {
+    const int col = cg * kTP + j;
+    const int pair = col >> 1, layer = col & 1;
+    if (pair < np) atomicAdd(&(layer ? D1 : D0)[(int64_t)dst[p0 + pair] * C + r], acc[j]);
+  }
+}
+
+// Av_i = GPhin - (GnPhi + alpha_i phi_i) (CORNER Neumann rows: Av_i = phi_i). mode 1: b = -Av (CORNER: phi_D).
+__global__ void k_assemble(int mode, int n, const int32_t* __restrict__ kind, const int32_t* __restrict__ point,
+                           const double* __restrict__ alpha, const double* __restrict__ phi, const double* __restrict__ y0,
+                           const double* __restrict__ y1, const double* __restrict__ phi_k, double* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int p = point[i];
+  if (mode == 2) {
+    double a = -y1[p];
+    if (fabs(a) < 1e-15) a = 1.0;
+    out[i] = a;
+    return;
+  }
+  double v = (kind[i] == 2) ? phi[i] : y0[p] - (y1[p] + alpha[i] * phi[i]);
+  if (mode == 1) v = (kind[i] == 2) ? phi_k[i] : -v;
+  out[i] = v;
+}
+
+// Near-field quadrature: one workgroup per near row (target node); threads stride over the
+// faces of the target box's near ranges and scatter (WG, WGn) of the three corners into the row.
+// fill_direct_linear / fill_nonadj_linear (BEM_solveBVP_GMRES_FMM.hpp:1824-1891).
+__global__ __launch_bounds__(256) void k_near_quad(
+    int C, int q, double near_region, const int32_t* __restrict__ row_block, const int64_t* __restrict__ row_ptr,
+    const int64_t* __restrict__ col_ptr, const int64_t* __restrict__ mat_ptr, const int32_t* __restrict__ row_out,
+    const int32_t* __restrict__ col_idx, const int32_t* __restrict__ ncol_dof, const int64_t* __restrict__ range_ptr,
+    const int64_t* __restrict__ ranges, const int64_t* __restrict__ range_prefix, const int32_t* __restrict__ face_perm,
+    const double* __restrict__ Xs, const double* __restrict__ fX, const double* __restrict__ fcent,
+    const double* __restrict__ fcr, const double* __restrict__ fJ, const int32_t* __restrict__ fv,
+    const int32_t* __restrict__ fdof, const double* __restrict__ l2p, double* __restrict__ mat) {
+  const int64_t r = blockIdx.x;
+  const int b = row_block[r];
+  const int ncd = ncol_dof[b];
+  const int64_t c0 = col_ptr[b];
+  const int ncol = (int)(col_ptr[b + 1] - c0);
+  double* A0 = mat + mat_ptr[b] + (r - row_ptr[b]) * 2 * ncol;
+  double* A1 = A0 + ncol;
+  const int t = row_out[r];
+  for (int c = threadIdx.x; c < C; c += blockDim.x) {
+    A0[ncd + c] = l2p[r * C + c];
+    A1[ncd + c] = l2p[r * C + c];
+  }
+  const double x = Xs[3 * t], y = Xs[3 * t + 1], z = Xs[3 * t + 2];
+  const int64_t g0 = range_ptr[b], g1 = range_ptr[b + 1];
+  const int64_t nf = range_prefix[g1 - 1] + (ranges[2 * (g1 - 1) + 1] - ranges[2 * (g1 - 1)]);
+  const int32_t* cols = col_idx + c0;
+  for (int64_t i = threadIdx.x; i < nf; i += blockDim.x) {
+    // locate the range holding local face i
+    int64_t lo = g0, hi = g1 - 1;
+    while (lo < hi) {
+      const int64_t mid = (lo + hi + 1) >> 1;
+      if (range_prefix[mid] <= i) lo = mid;
+      else hi = mid - 1;
+    }
+    const int f = face_perm[ranges[2 * lo] + (i - range_prefix[lo])];
+    int slot = -1;
+    for (int k = 0; k < 3; ++k)
+      if (fv[3 * f + k] == t) slot = k;
+    double WG[3], WGn[3];
+    bemfmm::face_weights(fX + 9 * (int64_t)f, fJ[f], fcr + 3 * (int64_t)f, fcent + 3 * (int64_t)f, slot, x, y, z,
+                         near_region, c_gw5, c_dun, q, WG, WGn);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const int d = fdof[3 * f + k];
+      int lo2 = 0, hi2 = ncd - 1;
+      while (lo2 < hi2) {
+        const int mid = (lo2 + hi2) >> 1;
+        if (cols[mid] < d) lo2 = mid + 1;
+        else hi2 = mid;
+      }
+      atomicAdd(&A0[lo2], WG[k]);
+      atomicAdd(&A1[lo2], WGn[k]);
+    }
+  }
+}
+
+// ---- GMRES vector kernels (FP64) ----
+constexpr int kDotBlocks = 256;
+// partial[i * kDotBlocks + blk] = sum over the block's chunk of V_i . w   (grid: kDotBlocks x k)
+__global__ __launch_bounds__(256) void k_dots(const double* __restrict__ V, int64_t ldv, const double* __restrict__ w, int64_t n,
+                                              double* __restrict__ partial) {
+  __shared__ double red[4];
+  const int i = blockIdx.y;
+  const double* v = V + i * ldv;
+  double s = 0.;
+  for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < n; j += (int64_t)gridDim.x * blockDim.x) s += v[j] * w[j];
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) partial[(int64_t)i * gridDim.x + blockIdx.x] = red[0] + red[1] + red[2] + red[3];
+}
+__global__ void k_reduce_partials(const double* __restrict__ partial, int nblk, int k, double* __restrict__ out) {
+  const int i = blockIdx.x;
+  __shared__ double red[4];
+  double s = 0.;
+  for (int j = threadIdx.x; j < nblk; j += blockDim.x) s += partial[(int64_t)i * nblk + j];
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) out[i] = red[0] + red[1] + red[2] + red[3];
+  (void)k;
+}
+// w[j] += sign * sum_{i<k} V_i[j] h[i]
+__global__ void k_vsum(const double* __restrict__ V, int64_t ldv, int k, const double* __restrict__ h, double sign,
+                       double* __restrict__ w, int64_t n) {
+  for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < n; j += (int64_t)gridDim.x * blockDim.x) {
+    double s = 0.;
+    for (int i = 0; i < k; ++i) s += V[i * ldv + j] * h[i];
+    w[j] += sign * s;
+  }
+}
+__global__ void k_axpby(int64_t n, double a, const double* __restrict__ x, double b, double* __restrict__ y) {
+  for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < n; j += (int64_t)gridDim.x * blockDim.x) y[j] = a * x[j] + b * y[j];
+}
+__global__ void k_sub(int64_t n, const double* __restrict__ a, const double* __restrict__ b, double* __restrict__ out) {
+  for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < n; j += (int64_t)gridDim.x * blockDim.x) out[j] = a[j] - b[j];
+}
+__global__ void k_mul(int64_t n, const double* __restrict__ d, double* __restrict__ y) {
+  for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < n; j += (int64_t)gridDim.x * blockDim.x) y[j] *= d[j];
+}
+
+// ---------------------------------------------------------------------------
+// device buffers
+// ---------------------------------------------------------------------------
+template <typename T>
+struct DBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  DBuf() = default;
+  DBuf(const DBuf&) = delete;
+  DBuf& operator=(const DBuf&) = delete;
+  DBuf(DBuf&& o) noexcept : p(o.p), n(o.n) {
+    o.p = nullptr;
+    o.n = 0;
+  }
+  DBuf& operator=(DBuf&& o) noexcept {
+    if (this != &o) {
+      release();
+      p = o.p;
+      n = o.n;
+      o.p = nullptr;
+      o.n = 0;
+    }
+    return *this;
+  }
+  ~DBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  void alloc(size_t count) {
+    release();
+    if (count == 0) count = 1;
+    HIPCHECK(hipMalloc((void**)&p, count * sizeof(T)));
+    n = count;
+  }
+  void upload(const std::vector<T>& v, hipStream_t s) {
+    alloc(v.size());
+    if (!v.empty()) HIPCHECK(hipMemcpyAsync(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s));
+  }
+  size_t bytes() const { return n * sizeof(T); }
+};
+
+struct DGemv {
+  DBuf<int64_t> row_ptr, col_ptr, mat_ptr;
+  DBuf<int32_t> row_out, col_idx;
+  DBuf<double> mat;
+  int nblk = 0;
+  void upload(const bemfmm::BlockGemv& g, hipStream_t s) {
+    row_ptr.upload(g.row_ptr, s);
+    col_ptr.upload(g.col_ptr, s);
+    mat_ptr.upload(g.mat_ptr, s);
+    row_out.upload(g.row_out, s);
+    col_idx.upload(g.col_idx, s);
+    nblk = g.n_blocks();
+  }
+  size_t bytes() const {
+    return row_ptr.bytes() + col_ptr.bytes() + mat_ptr.bytes() + row_out.bytes() + col_idx.bytes() + mat.bytes();
+  }
+};
+
+struct DTiles {
+  DBuf<int32_t> op, dst, src;
+  DBuf<int64_t> p0, p1;
+  int ntiles = 0;
+  void upload(const bemfmm::PairGroups& g, hipStream_t s) {
+    std::vector<int32_t> top;
+    std::vector<int64_t> tp0, tp1;
+    for (size_t k = 0; k < g.op.size(); ++k)
+      for (int64_t a = g.pair_ptr[k]; a < g.pair_ptr[k + 1]; a += kTP) {
+        top.push_back(g.op[k]);
+        tp0.push_back(a);
+        tp1.push_back(std::min(g.pair_ptr[k + 1], a + (int64_t)kTP));
+      }
+    ntiles = (int)top.size();
+    op.upload(top, s);
+    p0.upload(tp0, s);
+    p1.upload(tp1, s);
+    dst.upload(g.dst, s);
+    src.upload(g.src, s);
+  }
+  size_t bytes() const { return op.bytes() + dst.bytes() + src.bytes() + p0.bytes() + p1.bytes(); }
+};
+
+int grid1(int64_t n, int bs = 256) { return (int)std::max<int64_t>(1, std::min<int64_t>((n + bs - 1) / bs, 65536)); }
+
+}  // namespace
+
+struct bemfmm_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  bemfmm_opts opts{};
+  std::string err;
+  // host inputs
+  int n_nodes = 0, n_faces = 0, n_dof = 0;
+  std::vector<double> X;
+  std::vector<int32_t> tri, face_dof, dof_kind, dof_point;
+  std::vector<double> phi_bc, phin_bc;
+  bool have_geom = false, have_bc = false, prepared = false;
+  // plan and device state
+  std::unique_ptr<Plan> plan;
+  int C = 0, n_boxes = 0;
+  DGemv near, p2m;
+  std::vector<DTiles> m2m, l2l;
+  DTiles m2l;
+  DBuf<double> ops;
+  DBuf<int32_t> kind, point;
+  DBuf<double> alpha, phi_k, phin_k;
+  DBuf<double> buf0, buf1;   // [phi_n | L0], [phi | L1]
+  DBuf<double> M0, M1;
+  DBuf<double> y0, y1;
+  DBuf<double> xin, yout, bvec, tmpv, xsol, xbase, jacobi;
+  DBuf<double> V;            // Krylov basis
+  int64_t v_cap = 0;
+  DBuf<double> partial, hbuf;
+  bemfmm_stats stats{};
+  // profiling
+  bool prof = false;
+  hipEvent_t ev[8] = {};
+  double prof_ms[7] = {};
+  int n_matvec = 0;
+};
+
+namespace {
+
+void set_err(bemfmm_ctx* c, const std::string& s) {
+  if (c) c->err = s;
+}
+
+template <typename F>
+int guarded(bemfmm_ctx* c, F&& f) {
+  try {
+    if (c && c->device >= 0) HIPCHECK(hipSetDevice(c->device));
+    return f();
+  } catch (const HipError& e) {
+    set_err(c, e.what());
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    set_err(c, "host allocation failed");
+    return BEMFMM_E_OOM;
+  } catch (const std::exception& e) {
+    set_err(c, e.what());
+    return BEMFMM_E_INVALID;
+  }
+}
+
+void launch_gemv(bemfmm_ctx* c, DGemv& g, const double* x0, const double* x1, double* y0, double* y1) {
+  if (g.nblk == 0) return;
+  hipLaunchKernelGGL(k_block_gemv<double>, dim3(g.nblk), dim3(256), 0, c->stream, g.row_ptr.p, g.col_ptr.p, g.mat_ptr.p,
+                     g.row_out.p, g.col_idx.p, g.mat.p, x0, x1, y0, y1);
+}
+
+void launch_translate(bemfmm_ctx* c, DTiles& t, const double* S0, const double* S1, double* D0, double* D1) {
+  if (t.ntiles == 0) return;
+  const size_t lds = (size_t)c->C * kTStride * sizeof(double);
+  hipLaunchKernelGGL(k_translate, dim3(t.ntiles), dim3(256), lds, c->stream, c->C, c->ops.p, t.op.p, t.p0.p, t.p1.p,
+                     t.dst.p, t.src.p, S0, S1, D0, D1);
+}
+
+void fill(bemfmm_ctx* c, double* p, int64_t n, double v) {
+  hipLaunchKernelGGL(k_fill, dim3(grid1(n)), dim3(256), 0, c->stream, p, n, v);
+}
+
+// One evaluation of the BIE operator. mode: 0 = A x (x = in), 1 = rhs, 2 = rigid-mode alpha.
+void evaluate(bemfmm_ctx* c, int mode, const double* in, double* out) {
+  Plan& pl = *c->plan;
+  const int n = c->n_dof;
+  const int64_t nbC = (int64_t)c->n_boxes * c->C;
+  double* phin = c->buf0.p;
+  double* phi = c->buf1.p;
+  double* L0 = c->buf0.p + n;
+  double* L1 = c->buf1.p + n;
+  auto rec = [&](int i) {
+    if (c->prof) HIPCHECK(hipEventRecord(c->ev[i], c->stream));
+  };
+  rec(0);
+  hipLaunchKernelGGL(k_scatter, dim3(grid1(n)), dim3(256), 0, c->stream, mode == 0 ? 0 : (mode == 1 ? 1 : 2), n,
+                     c->kind.p, in, c->phi_k.p, c->phin_k.p, phin, phi);
+  fill(c, c->M0.p, nbC, 0.);
+  fill(c, c->M1.p, nbC, 0.);
+  fill(c, L0, nbC, 0.);
+  fill(c, L1, nbC, 0.);
+  rec(1);
+  launch_gemv(c, c->p2m, phin, phi, c->M0.p, c->M1.p);
+  rec(2);
+  for (int l = pl.n_levels - 1; l >= 1; --l) launch_translate(c, c->m2m[l], c->M0.p, c->M1.p, c->M0.p, c->M1.p);
+  rec(3);
+  launch_translate(c, c->m2l, c->M0.p, c->M1.p, L0, L1);
+  rec(4);
+  for (int l = 1; l < pl.n_levels; ++l) launch_translate(c, c->l2l[l], L0, L1, L0, L1);
+  rec(5);
+  launch_gemv(c, c->near, c->buf0.p, c->buf1.p, c->y0.p, c->y1.p);
+  rec(6);
+  hipLaunchKernelGGL(k_assemble, dim3(grid1(n)), dim3(256), 0, c->stream, mode, n, c->kind.p, c->point.p, c->alpha.p, phi,
+                     c->y0.p, c->y1.p, c->phi_k.p, out);
+  rec(7);
+  HIPCHECK(hipGetLastError());
+  c->n_matvec++;
+}
+
+void do_prepare(bemfmm_ctx* c) {
+  if (!c->have_geom) throw std::runtime_error("bemfmm_prepare: geometry not set");
+  if (!c->have_bc) throw std::runtime_error("bemfmm_prepare: boundary values not set");
+  const double t0 = now();
+  bemfmm::Options o;
+  o.p = c->opts.p;
+  o.theta = c->opts.theta;
+  o.leaf_max = c->opts.leaf_max;
+  o.max_level = c->opts.max_level;
+  o.q = c->opts.q;
+  c->plan.reset(new Plan());
+  Plan& pl = *c->plan;
+  bemfmm::build_plan(pl, o, c->n_nodes, c->X.data(), c->n_faces, c->tri.data(), c->face_dof.data(), c->n_dof,
+                     c->dof_kind.data(), c->dof_point.data());
+  c->C = pl.C;
+  c->n_boxes = pl.n_boxes;
+  if (pl.C > kGemvMaxRows) throw std::runtime_error("expansion order too large for the P2M block GEMV");
+  hipStream_t s = c->stream;
+  const double t1 = now();
+  // ---- upload ----
+  c->near.upload(pl.near, s);
+  c->near.mat.alloc(pl.near.mat_size);
+  c->p2m.upload(pl.p2m, s);
+  c->p2m.mat.upload(pl.p2m_mat, s);
+  c->m2m.clear();
+  c->l2l.clear();
+  c->m2m.resize(pl.n_levels);
+  c->l2l.resize(pl.n_levels);
+  for (int l = 1; l < pl.n_levels; ++l) {
+    c->m2m[l].upload(pl.m2m_by_level[l], s);
+    c->l2l[l].upload(pl.l2l_by_level[l], s);
+  }
+  c->m2l.upload(pl.m2l, s);
+  c->ops.upload(pl.ops, s);
+  c->kind.upload(pl.dof_kind, s);
+  c->point.upload(pl.dof_point, s);
+  std::vector<double> phik(c->phi_bc), phink(c->n_dof);
+  for (int i = 0; i < c->n_dof; ++i) phink[i] = (c->dof_kind[i] != 0) ? c->phin_bc[i] * pl.Lscale : c->phin_bc[i];
+  c->phi_k.upload(phik, s);
+  c->phin_k.upload(phink, s);
+  const int64_t nvec = (int64_t)c->n_dof + (int64_t)pl.n_boxes * pl.C;
+  c->buf0.alloc(nvec);
+  c->buf1.alloc(nvec);
+  c->M0.alloc((size_t)pl.n_boxes * pl.C);
+  c->M1.alloc((size_t)pl.n_boxes * pl.C);
+  c->y0.alloc(c->n_nodes);
+  c->y1.alloc(c->n_nodes);
+  c->alpha.alloc(c->n_dof);
+  for (auto* b : {&c->xin, &c->yout, &c->bvec, &c->tmpv, &c->xsol, &c->xbase, &c->jacobi}) b->alloc(c->n_dof);
+  // quadrature inputs
+  DBuf<int32_t> row_block, ncol_dof, face_perm, fv, fdof;
+  DBuf<int64_t> range_ptr, ranges, range_prefix;
+  DBuf<double> Xs, fX, fcent, fcr, fJ, l2p;
+  {
+    std::vector<int32_t> rb(pl.near.row_out.size());
+    for (int b = 0; b < pl.near.n_blocks(); ++b)
+      for (int64_t r = pl.near.row_ptr[b]; r < pl.near.row_ptr[b + 1]; ++r) rb[r] = b;
+    row_block.upload(rb, s);
+  }
+  ncol_dof.upload(pl.near_ncol_dof, s);
+  face_perm.upload(pl.face_perm, s);
+  fv.upload(pl.fv, s);
+  fdof.upload(pl.fdof, s);
+  range_ptr.upload(pl.near_range_ptr, s);
+  ranges.upload(pl.near_ranges, s);
+  range_prefix.upload(pl.near_range_prefix, s);
+  Xs.upload(pl.Xs, s);
+  fX.upload(pl.fX, s);
+  fcent.upload(pl.fcent, s);
+  fcr.upload(pl.fcr, s);
+  fJ.upload(pl.fJ, s);
+  l2p.upload(pl.l2p_coef, s);
+  {
+    double dun[7][4] = {};
+    static const double d1[1][4] = {{1. / 3., 1. / 3., 1. / 3., 0.5}};
+    static const double d3[3][4] = {{2. / 3., 1. / 6., 1. / 6., 1. / 6.}, {1. / 6., 2. / 3., 1. / 6., 1. / 6.}, {1. / 6., 1. / 6., 2. / 3., 1. / 6.}};
+    static const double d6[6][4] = {{0.816847572980459, 0.091576213509771, 0.091576213509771, 0.054975871827661},
+                                    {0.091576213509771, 0.816847572980459, 0.091576213509771, 0.054975871827661},
+                                    {0.091576213509771, 0.091576213509771, 0.816847572980459, 0.054975871827661},
+                                    {0.108103018168070, 0.445948490915965, 0.445948490915965, 0.111690794839006},
+                                    {0.445948490915965, 0.108103018168070, 0.445948490915965, 0.111690794839006},
+                                    {0.445948490915965, 0.445948490915965, 0.108103018168070, 0.111690794839006}};
+    static const double d7[7][4] = {{1. / 3., 1. / 3., 1. / 3., 0.1125},
+                                    {0.059715871789770, 0.470142064105115, 0.470142064105115, 0.066197076394253},
+                                    {0.470142064105115, 0.059715871789770, 0.470142064105115, 0.066197076394253},
+                                    {0.470142064105115, 0.470142064105115, 0.059715871789770, 0.066197076394253},
+                                    {0.797426985353087, 0.101286507323456, 0.101286507323456, 0.062969590272414},
+                                    {0.101286507323456, 0.797426985353087, 0.101286507323456, 0.062969590272414},
+                                    {0.101286507323456, 0.101286507323456, 0.797426985353087, 0.062969590272414}};
+    const double(*src)[4] = o.q == 1 ? d1 : o.q == 3 ? d3 : o.q == 6 ? d6 : d7;
+    for (int g = 0; g < o.q; ++g)
+      for (int a = 0; a < 4; ++a) dun[g][a] = src[g][a];
+    HIPCHECK(hipMemcpyToSymbolAsync(HIP_SYMBOL(c_dun), dun, sizeof(dun), 0, hipMemcpyHostToDevice, s));
+  }
+  HIPCHECK(hipStreamSynchronize(s));
+  const double t2 = now();
+  // ---- near-field quadrature on the GPU ----
+  fill(c, c->near.mat.p, pl.near.mat_size, 0.);
+  const int64_t nrows = (int64_t)pl.near.row_out.size();
+  if (nrows > 0)
+    hipLaunchKernelGGL(k_near_quad, dim3((unsigned)nrows), dim3(256), 0, s, pl.C, o.q, pl.near_region, row_block.p,
+                       c->near.row_ptr.p, c->near.col_ptr.p, c->near.mat_ptr.p, c->near.row_out.p, c->near.col_idx.p,
+                       ncol_dof.p, range_ptr.p, ranges.p, range_prefix.p, face_perm.p, Xs.p, fX.p, fcent.p, fcr.p,
+                       fJ.p, fv.p, fdof.p, l2p.p, c->near.mat.p);
+  HIPCHECK(hipGetLastError());
+  HIPCHECK(hipStreamSynchronize(s));
+  const double t3 = now();
+  // ---- rigid-mode alpha (updateGeometricProperties) ----
+  evaluate(c, 2, nullptr, c->alpha.p);
+  if (c->opts.preconditioner == BEMFMM_PRECOND_JACOBI) {
+    // diagonal of A: alpha-scaled near self weight for Neumann rows, G self weight for Dirichlet rows
+    throw std::runtime_error("Jacobi preconditioner not available in this build");
+  }
+  HIPCHECK(hipStreamSynchronize(s));
+  const double t4 = now();
+  c->prepared = true;
+  bemfmm_stats& st = c->stats;
+  st = bemfmm_stats{};
+  st.n_nodes = c->n_nodes;
+  st.n_faces = c->n_faces;
+  st.n_dof = c->n_dof;
+  st.n_levels = pl.n_levels;
+  st.n_boxes = pl.n_boxes;
+  st.n_leaves = (int)pl.n_leaves;
+  st.n_m2l_pairs = pl.n_m2l_pairs;
+  st.near_nnz = pl.near_nnz;
+  st.n_m2l_ops = (int)pl.m2l.op.size();
+  st.L = pl.Lscale;
+  st.device_bytes = (double)(c->near.bytes() + c->p2m.bytes() + c->m2l.bytes() + c->ops.bytes() + c->buf0.bytes() +
+                             c->buf1.bytes() + c->M0.bytes() + c->M1.bytes());
+  st.t_tree = pl.t_tree;
+  st.t_lists = pl.t_lists;
+  st.t_near_struct = pl.t_near_struct;
+  st.t_p2m = pl.t_p2m;
+  st.t_ops = pl.t_ops;
+  st.t_upload = t2 - t1;
+  st.t_near_quad = t3 - t2;
+  st.t_alpha = t4 - t3;
+  (void)t0;
+}
+
+void ensure_basis(bemfmm_ctx* c, int64_t need) {
+  if (need <= c->v_cap) return;
+  int64_t cap = std::max<int64_t>(need, std::max<int64_t>(64, 2 * c->v_cap));
+  DBuf<double> nv;
+  nv.alloc((size_t)cap * c->n_dof);
+  if (c->v_cap > 0)
+    HIPCHECK(hipMemcpyAsync(nv.p, c->V.p, (size_t)c->v_cap * c->n_dof * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+  std::swap(c->V.p, nv.p);
+  std::swap(c->V.n, nv.n);
+  c->v_cap = cap;
+  c->partial.alloc((size_t)kDotBlocks * (cap + 1));
+  c->hbuf.alloc((size_t)cap + 1);
+}
+
+// h[0..k) = V[0..k) . w  (device), copied to host
+void dots(bemfmm_ctx* c, int k, const double* Vp, const double* w, std::vector<double>& h) {
+  const int64_t n = c->n_dof;
+  hipLaunchKernelGGL(k_dots, dim3(kDotBlocks, k), dim3(256), 0, c->stream, Vp, n, w, n, c->partial.p);
+  hipLaunchKernelGGL(k_reduce_partials, dim3(k), dim3(256), 0, c->stream, c->partial.p, kDotBlocks, k, c->hbuf.p);
+  h.resize(k);
+  HIPCHECK(hipMemcpyAsync(h.data(), c->hbuf.p, k * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHECK(hipStreamSynchronize(c->stream));
+}
+double norm2(bemfmm_ctx* c, const double* w) {
+  std::vector<double> h;
+  dots(c, 1, w, w, h);
+  return std::sqrt(h[0]);
+}
+
+// GMRES (run_gmres control flow, BEM_solveBVP_GMRES_FMM.hpp:2275-2355; Arnoldi + Givens,
+// basic_linear_systems.hpp:3137-3451) with classical Gram-Schmidt applied twice on the device.
+int do_solve(bemfmm_ctx* c, const double* x0, double* xout, double tol, int max_iter, int restart, bemfmm_result* res) {
+  if (!c->prepared) throw std::runtime_error("bemfmm_solve: call bemfmm_prepare first");
+  const double ts = now();
+  const int64_t n = c->n_dof;
+  hipStream_t s = c->stream;
+  const int n_mv0 = c->n_matvec;
+  evaluate(c, 1, nullptr, c->bvec.p);  // b
+  HIPCHECK(hipMemcpyAsync(c->xsol.p, x0, n * sizeof(double), hipMemcpyHostToDevice, s));
+  auto Av = [&](const double* v, double* w) { evaluate(c, 0, v, w); };
+  const int gN = grid1(n);
+  int iters = 0;
+  double cur_err = 1e300;
+  const int restart_max = std::max(1, restart);
+  std::vector<double> h1, h2;
+  while (iters < max_iter && cur_err > tol) {
+    const int remaining = max_iter - iters;
+    const int k_cycle_max = std::min(restart_max, remaining);
+    ensure_basis(c, 8);
+    // r0 = b - A x
+    Av(c->xsol.p, c->tmpv.p);
+    hipLaunchKernelGGL(k_sub, dim3(gN), dim3(256), 0, s, n, c->bvec.p, c->tmpv.p, c->V.p);
+    const double beta = norm2(c, c->V.p);
+    HIPCHECK(hipMemcpyAsync(c->xbase.p, c->xsol.p, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+    std::vector<double> cs, sn, g(1, beta);
+    std::vector<std::vector<double>> Rm;
+    if (beta == 0.) {
+      cur_err = 0.;
+      break;
+    }
+    hipLaunchKernelGGL(k_axpby, dim3(gN), dim3(256), 0, s, n, 1.0 / beta, c->V.p, 0.0, c->V.p);
+    int nb = 1;  // basis vectors in V
+    auto add_basis = [&]() {
+      ensure_basis(c, nb + 1);
+      double* w = c->V.p + (int64_t)nb * n;
+      Av(c->V.p + (int64_t)(nb - 1) * n, w);
+      dots(c, nb, c->V.p, w, h1);
+      hipLaunchKernelGGL(k_vsum, dim3(gN), dim3(256), 0, s, c->V.p, n, nb, c->hbuf.p, -1.0, w, n);
+      dots(c, nb, c->V.p, w, h2);
+      hipLaunchKernelGGL(k_vsum, dim3(gN), dim3(256), 0, s, c->V.p, n, nb, c->hbuf.p, -1.0, w, n);
+      std::vector<double> col(nb + 1);
+      for (int i = 0; i < nb; ++i) col[i] = h1[i] + h2[i];
+      const double hn = norm2(c, w);
+      col[nb] = hn;
+      hipLaunchKernelGGL(k_axpby, dim3(gN), dim3(256), 0, s, n, 1.0 / hn, w, 0.0, w);
+      ++nb;
+      // Givens update of the new column
+      const size_t cc = cs.size();
+      for (size_t i = 0; i < cc; ++i) {
+        const double tmp = std::fma(cs[i], col[i], sn[i] * col[i + 1]);
+        col[i + 1] = std::fma(cs[i], col[i + 1], -sn[i] * col[i]);
+        col[i] = tmp;
+      }
+      double cg = 1.0, sg = 0.0;
+      const double r = std::hypot(col[cc], col[cc + 1]);
+      if (r != 0.0) {
+        cg = col[cc] / r;
+        sg = col[cc + 1] / r;
+      }
+      const double tmp = std::fma(cg, col[cc], sg * col[cc + 1]);
+      col[cc + 1] = std::fma(cg, col[cc + 1], -sg * col[cc]);
+      col[cc] = tmp;
+      cs.push_back(cg);
+      sn.push_back(sg);
+      g.push_back(0.);
+      const double gt = std::fma(cg, g[cc], sg * g[cc + 1]);
+      g[cc + 1] = std::fma(cg, g[cc + 1], -sg * g[cc]);
+      g[cc] = gt;
+      Rm.push_back(col);
+    };
+    auto update_solution = [&]() {
+      const size_t k = cs.size();
+      std::vector<double> y(k, 0.);
+      for (int i = (int)k - 1; i >= 0; --i) {
+        double sum = g[i];
+        for (size_t j = i + 1; j < k; ++j) sum -= Rm[j][i] * y[j];
+        y[i] = sum / Rm[i][i];
+      }
+      HIPCHECK(hipMemcpyAsync(c->xsol.p, c->xbase.p, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+      HIPCHECK(hipMemcpyAsync(c->hbuf.p, y.data(), k * sizeof(double), hipMemcpyHostToDevice, s));
+      hipLaunchKernelGGL(k_vsum, dim3(gN), dim3(256), 0, s, c->V.p, n, (int)k, c->hbuf.p, 1.0, c->xsol.p, n);
+      HIPCHECK(hipStreamSynchronize(s));
+    };
+    auto true_residual = [&]() {
+      Av(c->xsol.p, c->tmpv.p);
+      hipLaunchKernelGGL(k_sub, dim3(gN), dim3(256), 0, s, n, c->tmpv.p, c->bvec.p, c->tmpv.p);
+      return norm2(c, c->tmpv.p);
+    };
+    add_basis();  // gmres(A, b, x, 1)
+    iters += 1;
+    int sat = 0;
+    while (true) {
+      add_basis();
+      ++iters;
+      const double est = std::abs(g.back());
+      if (!std::isfinite(est)) throw HipError("GMRES breakdown: non-finite residual estimate", BEMFMM_E_NONFINITE);
+      if (est <= tol * 10.) {
+        update_solution();
+        cur_err = true_residual();
+        if (cur_err <= tol) break;
+        if (sat++ >= 5) break;
+      }
+      if (iters >= max_iter && (int)cs.size() >= k_cycle_max) {
+        update_solution();
+        cur_err = (est <= tol * 10.) ? true_residual() : est;
+        break;
+      }
+    }
+  }
+  std::vector<double> x(n);
+  HIPCHECK(hipMemcpyAsync(x.data(), c->xsol.p, n * sizeof(double), hipMemcpyDeviceToHost, s));
+  HIPCHECK(hipStreamSynchronize(s));
+  const double L = c->plan->Lscale;
+  for (int64_t i = 0; i < n; ++i) xout[i] = (c->dof_kind[i] == 0) ? x[i] / L : x[i];
+  if (res) {
+    res->iters = iters;
+    res->n_matvec = c->n_matvec - n_mv0;
+    res->converged = cur_err <= tol ? 1 : 0;
+    res->residual = cur_err;
+    res->time_solve = now() - ts;
+    res->refinement_steps = 0;
+  }
+  for (int64_t i = 0; i < n; ++i)
+    if (!std::isfinite(xout[i])) throw HipError("non-finite solution", BEMFMM_E_NONFINITE);
+  return cur_err <= tol ? BEMFMM_OK : BEMFMM_E_NOT_CONVERGED;
+}
+
+}  // namespace
+
+extern "C" {
+
+int bemfmm_is_available(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  return n > 0 ? 1 : 0;
+}
+
+const char* bemfmm_device_name(void) {
+  static char name[256] = {0};
+  hipDeviceProp_t p;
+  if (hipGetDeviceProperties(&p, 0) != hipSuccess) return "unavailable";
+  std::snprintf(name, sizeof(name), "%s (%s)", p.name, p.gcnArchName);
+  return name;
+}
+
+void bemfmm_default_opts(bemfmm_opts* o) {
+  if (!o) return;
+  o->p = 10;
+  o->theta = 0.25;
+  o->leaf_max = 50;
+  o->max_level = 7;
+  o->q = 6;
+  o->precision = BEMFMM_PRECISION_FP64;
+  o->preconditioner = BEMFMM_PRECOND_NONE;
+}
+
+bemfmm_ctx* bemfmm_ctx_create(int device, const bemfmm_opts* opts) {
+  auto* c = new bemfmm_ctx();
+  c->device = device;
+  if (opts) c->opts = *opts;
+  else bemfmm_default_opts(&c->opts);
+  const int rc = guarded(c, [&]() {
+    int n = 0;
+    HIPCHECK(hipGetDeviceCount(&n));
+    if (device < 0 || device >= n) throw HipError("invalid device index", BEMFMM_E_DEVICE);
+    HIPCHECK(hipSetDevice(device));
+    HIPCHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    return BEMFMM_OK;
+  });
+  if (rc != BEMFMM_OK) {
+    std::fprintf(stderr, "bemfmm_ctx_create: %s\n", c->err.c_str());
+    delete c;
+    return nullptr;
+  }
+  return c;
+}
+
+void bemfmm_ctx_destroy(bemfmm_ctx* c) {
+  if (!c) return;
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  for (auto& e : c->ev)
+    if (e) (void)hipEventDestroy(e);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+const char* bemfmm_last_error(const bemfmm_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int bemfmm_set_geometry(bemfmm_ctx* c, int n_nodes, const double* X, int n_faces, const int32_t* tri, const int32_t* face_dof,
+                        int n_dof, const int32_t* dof_kind, const int32_t* dof_point) {
+  if (!c) return BEMFMM_E_INVALID;
+  return guarded(c, [&]() {
+    if (n_nodes <= 0 || n_faces <= 0 || n_dof <= 0 || !X || !tri || !face_dof || !dof_kind || !dof_point)
+      throw std::runtime_error("bemfmm_set_geometry: empty or null input");
+    c->n_nodes = n_nodes;
+    c->n_faces = n_faces;
+    c->n_dof = n_dof;
+    c->X.assign(X, X + 3 * (size_t)n_nodes);
+    c->tri.assign(tri, tri + 3 * (size_t)n_faces);
+    c->face_dof.assign(face_dof, face_dof + 3 * (size_t)n_faces);
+    c->dof_kind.assign(dof_kind, dof_kind + n_dof);
+    c->dof_point.assign(dof_point, dof_point + n_dof);
+    c->have_geom = true;
+    c->have_bc = false;
+    c->prepared = false;
+    return BEMFMM_OK;
+  });
+}
+
+int bemfmm_set_bc(bemfmm_ctx* c, const double* phi, const double* phin) {
+  if (!c) return BEMFMM_E_INVALID;
+  return guarded(c, [&]() {
+    if (!c->have_geom) throw HipError("bemfmm_set_bc: geometry not set", BEMFMM_E_STATE);
+    if (!phi || !phin) throw std::runtime_error("bemfmm_set_bc: null input");
+    for (int i = 0; i < c->n_dof; ++i) {
+      const bool dir = c->dof_kind[i] == 0;
+      if ((dir || c->dof_kind[i] == 2) && !std::isfinite(phi[i])) throw HipError("non-finite phi", BEMFMM_E_NONFINITE);
+      if (!dir && !std::isfinite(phin[i])) throw HipError("non-finite phin", BEMFMM_E_NONFINITE);
+    }
+    c->phi_bc.assign(phi, phi + c->n_dof);
+    c->phin_bc.assign(phin, phin + c->n_dof);
+    c->have_bc = true;
+    if (c->prepared) {
+      // boundary values only change the right-hand side: refresh the knowns in place
+      std::vector<double> phink(c->n_dof);
+      for (int i = 0; i < c->n_dof; ++i)
+        phink[i] = (c->dof_kind[i] != 0) ? c->phin_bc[i] * c->plan->Lscale : c->phin_bc[i];
+      HIPCHECK(hipMemcpyAsync(c->phi_k.p, c->phi_bc.data(), c->n_dof * sizeof(double), hipMemcpyHostToDevice, c->stream));
+      HIPCHECK(hipMemcpyAsync(c->phin_k.p, phink.data(), c->n_dof * sizeof(double), hipMemcpyHostToDevice, c->stream));
+      HIPCHECK(hipStreamSynchronize(c->stream));
+    }
+    return BEMFMM_OK;
+  });
+}
+
+int bemfmm_prepare(bemfmm_ctx* c) {
+  if (!c) return BEMFMM_E_INVALID;
+  return guarded(c, [&]() {
+    if (!c->have_geom || !c->have_bc) throw HipError("bemfmm_prepare: set geometry and boundary values first", BEMFMM_E_STATE);
+    do_prepare(c);
+    return BEMFMM_OK;
+  });
+}
+
+int bemfmm_matvec(bemfmm_ctx* c, const double* x, double* y) {
+  if (!c) return BEMFMM_E_INVALID;
+  return guarded(c, [&]() {
+    if (!c->prepared) throw HipError("bemfmm_matvec: call bemfmm_prepare first", BEMFMM_E_STATE);
+    HIPCHECK(hipMemcpyAsync(c->xin.p, x, c->n_dof * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    evaluate(c, 0, c->xin.p, c->yout.p);
+    HIPCHECK(hipMemcpyAsync(y, c->yout.p, c->n_dof * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(hipStreamSynchronize(c->stream));
+    return BEMFMM_OK;
+  });
+}
+
+int bemfmm_matvec_device(bemfmm_ctx* c, const double* x, double* y) {
+  if (!c) return BEMFMM_E_INVALID;
+  return guarded(c, [&]() {
+    if (!c->prepared) throw HipError("bemfmm_matvec_device: call bemfmm_prepare first", BEMFMM_E_STATE);
+    evaluate(c, 0, x, y);
+    return BEMFMM_OK;
+  });
+}
+
+int bemfmm_sync(bemfmm_ctx* c) {
+  if (!c) return BEMFMM_E_INVALID;
+  return guarded(c, [&]() {
+    HIPCHECK(hipStreamSynchronize(c->stream));
+    return BEMFMM_OK;
+  });
+}
+
+int bemfmm_rhs(bemfmm_ctx* c, double* b) {
+  if (!c) return BEMFMM_E_INVALID;
+  return guarded(c, [&]() {
+    if (!c->prepared) throw HipError("bemfmm_rhs: call bemfmm_prepare first", BEMFMM_E_STATE);
+    evaluate(c, 1, nullptr, c->yout.p);
+    HIPCHECK(hipMemcpyAsync(b, c->yout.p, c->n_dof * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(hipStreamSynchronize(c->stream));
+    return BEMFMM_OK;
+  });
+}
+
+int bemfmm_alpha(bemfmm_ctx* c, double* a) {
+  if (!c) return BEMFMM_E_INVALID;
+  return guarded(c, [&]() {
+    if (!c->prepared) throw HipError("bemfmm_alpha: call bemfmm_prepare first", BEMFMM_E_STATE);
+    HIPCHECK(hipMemcpyAsync(a, c->alpha.p, c->n_dof * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(hipStreamSynchronize(c->stream));
+    return BEMFMM_OK;
+  });
+}
+
+int bemfmm_solve(bemfmm_ctx* c, const double* x0, double* x, double tol, int max_iter, int restart, bemfmm_result* r) {
+  if (!c) return BEMFMM_E_INVALID;
+  return guarded(c, [&]() {
+    if (!x0 || !x || !(tol > 0) || max_iter < 1) throw std::runtime_error("bemfmm_solve: invalid arguments");
+    return do_solve(c, x0, x, tol, max_iter, restart, r);
+  });
+}
+
+int bemfmm_get_stats(const bemfmm_ctx* c, bemfmm_stats* s) {
+  if (!c || !s) return BEMFMM_E_INVALID;
+  if (!c->prepared) return BEMFMM_E_STATE;
+  *s = c->stats;
+  return BEMFMM_OK;
+}
+
+int bemfmm_matvec_profile(bemfmm_ctx* c, double* ms7) {
+  if (!c || !ms7) return BEMFMM_E_INVALID;
+  return guarded(c, [&]() {
+    if (!c->prof) {
+      for (auto& e : c->ev) HIPCHECK(hipEventCreate(&e));
+      c->prof = true;
+      for (int i = 0; i < 7; ++i) ms7[i] = 0.;
+      return BEMFMM_OK;
+    }
+    HIPCHECK(hipEventSynchronize(c->ev[7]));
+    for (int i = 0; i < 7; ++i) {
+      float ms = 0.f;
+      HIPCHECK(hipEventElapsedTime(&ms, c->ev[i], c->ev[i + 1]));
+      ms7[i] = ms;
+    }
+    return BEMFMM_OK;
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,763 @@
+// Host-side plan builder for one FMM-GMRES BVP (see plan.hpp).
+//
+// Follows the reference setup path (bem/BEM_solveBVP_GMRES_FMM.hpp:3056-3450 and
+// lib/include/moments.hpp:1239-1541): coordinate scaling, per-face P2M sources,
+// the Buckets octree and its MAC/interaction lists, per-target near lists and
+// the translation pairs. Arithmetic that decides tree membership and the MAC is
+// kept bit-identical to the reference's (cell index = floor((x-xmin)*inv_dL),
+// child cell size = dL*(0.5+1e-13), centre = mean of bounds, Norm = hypot).
+#include <omp.h>
+
+#include <algorithm>
+#include <array>
+#include <chrono>
+#include <cmath>
+#include <functional>
+#include <map>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <tuple>
+
+#include "plan.hpp"
+
+namespace bemfmm {
+namespace {
+
+using V3 = std::array<double, 3>;
+inline double hyp3(double a, double b, double c) { return std::hypot(a, b, c); }
+
+double now() { return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count(); }
+
+// Dunavant rules for the P2M sources (lib/include/dunavant_rules.hpp:18-46), {b0,b1,b2,w}, sum w = 1/2.
+const std::vector<std::array<double, 4>>& dunavant(int q) {
+  static const std::vector<std::array<double, 4>> d1 = {{1. / 3., 1. / 3., 1. / 3., 0.5}};
+  static const std::vector<std::array<double, 4>> d3 = {
+      {2. / 3., 1. / 6., 1. / 6., 1. / 6.}, {1. / 6., 2. / 3., 1. / 6., 1. / 6.}, {1. / 6., 1. / 6., 2. / 3., 1. / 6.}};
+  static const std::vector<std::array<double, 4>> d6 = {
+      {0.816847572980459, 0.091576213509771, 0.091576213509771, 0.054975871827661},
+      {0.091576213509771, 0.816847572980459, 0.091576213509771, 0.054975871827661},
+      {0.091576213509771, 0.091576213509771, 0.816847572980459, 0.054975871827661},
+      {0.108103018168070, 0.445948490915965, 0.445948490915965, 0.111690794839006},
+      {0.445948490915965, 0.108103018168070, 0.445948490915965, 0.111690794839006},
+      {0.445948490915965, 0.445948490915965, 0.108103018168070, 0.111690794839006}};
+  static const std::vector<std::array<double, 4>> d7 = {
+      {1. / 3., 1. / 3., 1. / 3., 0.1125},
+      {0.059715871789770, 0.470142064105115, 0.470142064105115, 0.066197076394253},
+      {0.470142064105115, 0.059715871789770, 0.470142064105115, 0.066197076394253},
+      {0.470142064105115, 0.470142064105115, 0.059715871789770, 0.066197076394253},
+      {0.797426985353087, 0.101286507323456, 0.101286507323456, 0.062969590272414},
+      {0.101286507323456, 0.797426985353087, 0.101286507323456, 0.062969590272414},
+      {0.101286507323456, 0.101286507323456, 0.797426985353087, 0.062969590272414}};
+  switch (q) {
+  case 1: return d1;
+  case 3: return d3;
+  case 6: return d6;
+  case 7: return d7;
+  default: throw std::runtime_error("p2m_quadrature_points must be one of 1, 3, 6, 7");
+  }
+}
+
+struct TBox {
+  int level = 0, parent = -1;
+  double bmin[3], bmax[3];
+  V3 X;
+  double dL = 0, inv_dL = 0;
+  int size[3] = {1, 1, 1};
+  std::vector<int> cells;
+  std::vector<std::vector<int>> cell_data;
+  std::vector<int> data;
+  double off = 0.;
+  bool has_children = false;
+  int id = -1;  // final level-major id
+  std::vector<int> near, m2l;
+  void init(const double lo[3], const double hi[3], double dl) {
+    for (int a = 0; a < 3; ++a) {
+      bmin[a] = lo[a];
+      bmax[a] = hi[a];
+      X[a] = (lo[a] + hi[a]) / 2.;
+      double s = std::ceil((hi[a] - lo[a]) / dl);
+      if (s <= 0.) s = 1.;
+      size[a] = (int)s;
+    }
+    dL = dl;
+    inv_dL = 1.0 / dl;
+    cells.assign(size[0] * size[1] * size[2], -1);
+    cell_data.assign(size[0] * size[1] * size[2], {});
+  }
+  bool inside(const V3& x) const {
+    for (int a = 0; a < 3; ++a) {
+      const int i = (int)std::floor((x[a] - bmin[a]) * inv_dL);
+      if (i < 0 || i >= size[a]) return false;
+    }
+    return true;
+  }
+  int cell_of(const V3& x) const {
+    int ijk[3];
+    for (int a = 0; a < 3; ++a) ijk[a] = std::clamp((int)std::floor((x[a] - bmin[a]) * inv_dL), 0, size[a] - 1);
+    return (ijk[0] * size[1] + ijk[1]) * size[2] + ijk[2];
+  }
+};
+
+struct Builder {
+  Plan& pl;
+  const Options& opt;
+  std::vector<TBox> B;
+  std::vector<V3> pole;
+  std::vector<double> maxoff;
+  explicit Builder(Plan& p, const Options& o) : pl(p), opt(o) {}
+
+  void add(int b, int f) {
+    B[b].data.push_back(f);
+    B[b].cell_data[B[b].cell_of(pole[f])].push_back(f);
+  }
+  void grow(int b) {
+    // generateTree with the grow condition of BEM_solveBVP_GMRES_FMM.hpp:1199
+    const bool cond = !B[b].data.empty() && (int)B[b].data.size() > opt.leaf_max && B[b].level < opt.max_level;
+    if (!cond) return;
+    const int ncell = (int)B[b].cells.size();
+    for (int c = 0; c < ncell; ++c) {
+      if (B[b].cell_data[c].empty()) continue;
+      const int k = c % B[b].size[2], j = (c / B[b].size[2]) % B[b].size[1], i = c / (B[b].size[2] * B[b].size[1]);
+      const int ijk[3] = {i, j, k};
+      double lo[3], hi[3];
+      for (int a = 0; a < 3; ++a) {
+        lo[a] = B[b].dL * ijk[a] + B[b].bmin[a];
+        hi[a] = B[b].dL * (ijk[a] + 1) + B[b].bmin[a];
+      }
+      TBox ch;
+      ch.init(lo, hi, B[b].dL * (0.5 + 1e-13));
+      ch.parent = b;
+      ch.level = B[b].level + 1;
+      const int cid = (int)B.size();
+      B.push_back(std::move(ch));
+      for (int f : B[b].cell_data[c])
+        if (B[cid].inside(pole[f])) add(cid, f);
+      B[b].cells[c] = cid;
+      B[b].has_children = true;
+      grow(cid);
+    }
+  }
+  bool is_far(int a, int b) const {
+    // isFar (moments.hpp:948-954)
+    const double eff = std::max(B[a].dL + 2.0 * B[a].off, B[b].dL + 2.0 * B[b].off);
+    const double d = hyp3(B[b].X[0] - B[a].X[0], B[b].X[1] - B[a].X[1], B[b].X[2] - B[a].X[2]);
+    return opt.theta > std::sqrt(3.0) * eff / d;
+  }
+  int deepest(const V3& x) const {
+    int cur = 0;
+    while (B[cur].has_children) {
+      const int nx = B[cur].cells[B[cur].cell_of(x)];
+      if (nx < 0) return cur;
+      cur = nx;
+    }
+    return cur;
+  }
+};
+
+// Solid harmonics in double: NR_n^m (m >= 0, all n <= P) by the recurrences
+// NR_m^m = (x+iy)/(2m) NR_{m-1}^{m-1}, NR_n^m = ((2n-1) z NR_{n-1}^m - r^2 NR_{n-2}^m)/((n+m)(n-m)).
+struct Harm {
+  int P;
+  std::vector<double> re, im;  // idx(n,m) = n*(n+1)/2 + m
+  std::vector<double> cnm;     // (-1)^m sqrt((n-m)!(n+m)!)
+  explicit Harm(int P_) : P(P_), re((P_ + 2) * (P_ + 3) / 2), im((P_ + 2) * (P_ + 3) / 2), cnm((P_ + 2) * (P_ + 3) / 2) {
+    for (int n = 0; n <= P + 1; ++n)
+      for (int m = 0; m <= n; ++m) {
+        double f1 = 1, f2 = 1;
+        for (int k = 2; k <= n - m; ++k) f1 *= k;
+        for (int k = 2; k <= n + m; ++k) f2 *= k;
+        const double v = std::sqrt(f1 * f2);
+        cnm[t(n, m)] = (m & 1) ? -v : v;
+      }
+  }
+  static int t(int n, int m) { return n * (n + 1) / 2 + m; }
+  void eval(double x, double y, double z, int L) {
+    const double r2 = x * x + y * y + z * z;
+    re[0] = 1.0;
+    im[0] = 0.0;
+    for (int m = 1; m <= L; ++m) {
+      const double pr = re[t(m - 1, m - 1)], pi = im[t(m - 1, m - 1)];
+      const double s = 1.0 / (2.0 * m);
+      re[t(m, m)] = s * (x * pr - y * pi);
+      im[t(m, m)] = s * (x * pi + y * pr);
+    }
+    for (int m = 0; m <= L; ++m)
+      for (int n = m + 1; n <= L; ++n) {
+        double ar = (2 * n - 1) * z * re[t(n - 1, m)], ai = (2 * n - 1) * z * im[t(n - 1, m)];
+        if (n - 2 >= m) {
+          ar -= r2 * re[t(n - 2, m)];
+          ai -= r2 * im[t(n - 2, m)];
+        }
+        const double s = 1.0 / ((double)(n + m) * (n - m));
+        re[t(n, m)] = s * ar;
+        im[t(n, m)] = s * ai;
+      }
+  }
+  // NR_n^q for any q (NR_n^{-m} = (-1)^m conj(NR_n^m)); zero if |q| > n or n < 0
+  void nr(int n, int q, double& r, double& i) const {
+    if (n < 0 || q > n || -q > n) {
+      r = i = 0;
+      return;
+    }
+    if (q >= 0) {
+      r = re[t(n, q)];
+      i = im[t(n, q)];
+    } else {
+      const int m = -q;
+      const double s = (m & 1) ? -1.0 : 1.0;
+      r = s * re[t(n, m)];
+      i = -s * im[t(n, m)];
+    }
+  }
+};
+
+}  // namespace
+
+void build_plan(Plan& pl, const Options& opt, int n_nodes, const double* X, int n_faces, const int32_t* tri,
+                const int32_t* face_dof, int n_dof, const int32_t* dof_kind, const int32_t* dof_point) {
+  if (opt.p < 1 || opt.p > 20) throw std::runtime_error("expansion order p must be in [1, 20]");
+  if (n_nodes <= 0 || n_faces <= 0 || n_dof <= 0) throw std::runtime_error("empty geometry");
+  pl.opt = opt;
+  pl.P = opt.p;
+  pl.C = (opt.p + 1) * (opt.p + 1);
+  const int P = pl.P, C = pl.C;
+  pl.n_nodes = n_nodes;
+  pl.n_faces = n_faces;
+  pl.n_dof = n_dof;
+  for (int f = 0; f < 3 * n_faces; ++f) {
+    if (tri[f] < 0 || tri[f] >= n_nodes) throw std::runtime_error("triangle node index out of range");
+    if (face_dof[f] < 0 || face_dof[f] >= n_dof) throw std::runtime_error("face DOF index out of range");
+  }
+  for (int i = 0; i < n_dof; ++i) {
+    if (dof_kind[i] < 0 || dof_kind[i] > 2) throw std::runtime_error("dof_kind must be 0 (Dirichlet), 1 (Neumann) or 2 (CORNER Neumann)");
+    if (dof_point[i] < 0 || dof_point[i] >= n_nodes) throw std::runtime_error("dof_point out of range");
+  }
+  pl.dof_kind.assign(dof_kind, dof_kind + n_dof);
+  pl.dof_point.assign(dof_point, dof_point + n_dof);
+  const double t0 = now();
+
+  // --- coordinate scaling (Network::computeCharacteristicLength / applyScaling) ---
+  double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+  for (int i = 0; i < n_nodes; ++i)
+    for (int a = 0; a < 3; ++a) {
+      const double v = X[3 * i + a];
+      if (!std::isfinite(v)) throw std::runtime_error("non-finite node coordinate");
+      lo[a] = std::min(lo[a], v);
+      hi[a] = std::max(hi[a], v);
+    }
+  pl.Lscale = std::sqrt((hi[0] - lo[0]) * (hi[0] - lo[0]) + (hi[1] - lo[1]) * (hi[1] - lo[1]) + (hi[2] - lo[2]) * (hi[2] - lo[2]));
+  if (!(pl.Lscale > 1e-10)) throw std::runtime_error("degenerate geometry (zero bounding box)");
+  pl.Xs.resize(3 * (size_t)n_nodes);
+  double slo[3] = {1e300, 1e300, 1e300}, shi[3] = {-1e300, -1e300, -1e300};
+  for (int i = 0; i < n_nodes; ++i)
+    for (int a = 0; a < 3; ++a) {
+      const double v = X[3 * i + a] / pl.Lscale;
+      pl.Xs[3 * i + a] = v;
+      slo[a] = std::min(slo[a], v);
+      shi[a] = std::max(shi[a], v);
+    }
+  pl.scale = hyp3(shi[0] - slo[0], shi[1] - slo[1], shi[2] - slo[2]);
+  pl.near_region = pl.scale / 25.;
+
+  // --- faces and P2M sources (GMRES_FMM.hpp:1799-1937) ---
+  Builder bd(pl, opt);
+  const auto& rule = dunavant(opt.q);
+  pl.fX.resize(9 * (size_t)n_faces);
+  pl.fcent.resize(3 * (size_t)n_faces);
+  pl.fcr.resize(3 * (size_t)n_faces);
+  pl.fJ.resize(n_faces);
+  pl.fv.assign(tri, tri + 3 * (size_t)n_faces);
+  pl.fdof.assign(face_dof, face_dof + 3 * (size_t)n_faces);
+  bd.pole.resize(n_faces);
+  bd.maxoff.resize(n_faces);
+  for (int f = 0; f < n_faces; ++f) {
+    V3 x[3];
+    for (int k = 0; k < 3; ++k)
+      for (int a = 0; a < 3; ++a) {
+        x[k][a] = pl.Xs[3 * tri[3 * f + k] + a];
+        pl.fX[9 * f + 3 * k + a] = x[k][a];
+      }
+    const V3 e1 = {x[1][0] - x[0][0], x[1][1] - x[0][1], x[1][2] - x[0][2]};
+    const V3 e2 = {x[2][0] - x[0][0], x[2][1] - x[0][1], x[2][2] - x[0][2]};
+    const V3 cr = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2], e1[0] * e2[1] - e1[1] * e2[0]};
+    const double J = hyp3(cr[0], cr[1], cr[2]);
+    if (!(J > 0)) throw std::runtime_error("degenerate triangle (zero area)");
+    pl.fJ[f] = J;
+    for (int a = 0; a < 3; ++a) {
+      pl.fcr[3 * f + a] = J * (cr[a] / J);
+      pl.fcent[3 * f + a] = (x[0][a] + x[1][a] + x[2][a]) / 3.;
+      bd.pole[f][a] = (1. / 3.) * x[0][a] + (1. / 3.) * x[1][a] + (1. / 3.) * x[2][a];
+    }
+    double mo = 0.;
+    for (const auto& r : rule) {
+      V3 xq;
+      for (int a = 0; a < 3; ++a) xq[a] = r[0] * x[0][a] + r[1] * x[1][a] + r[2] * x[2][a];
+      mo = std::max(mo, hyp3(xq[0] - bd.pole[f][0], xq[1] - bd.pole[f][1], xq[2] - bd.pole[f][2]));
+    }
+    bd.maxoff[f] = mo;
+  }
+
+  // --- octree (initializeBucket GMRES_FMM.hpp:1178-1201; generateTree) ---
+  {
+    double blo[3], bhi[3];
+    const double dL = 1.05 * pl.scale / 2.;
+    for (int a = 0; a < 3; ++a) {
+      const double c = (slo[a] + shi[a]) / 2.;
+      blo[a] = c - dL;
+      bhi[a] = c + dL;
+    }
+    TBox root;
+    root.init(blo, bhi, dL);
+    bd.B.push_back(std::move(root));
+    pl.root_side = bhi[0] - blo[0];
+  }
+  for (int f = 0; f < n_faces; ++f)
+    if (bd.B[0].inside(bd.pole[f])) bd.add(0, f);
+  if ((int)bd.B[0].data.size() != n_faces) throw std::runtime_error("face outside the root box");
+  bd.grow(0);
+  auto& Bx = bd.B;
+  // level lists in preorder (rebuildHierarchyLists)
+  std::vector<std::vector<int>> lv(opt.max_level + 1);
+  {
+    // preorder with children in cell order
+    std::function<void(int)> visit = [&](int b) {
+      if (Bx[b].data.empty()) throw std::runtime_error("empty octree box (face on a box boundary)");
+      lv[Bx[b].level].push_back(b);
+      for (int c : Bx[b].cells)
+        if (c >= 0) visit(c);
+    };
+    visit(0);
+  }
+  pl.n_levels = 0;
+  for (int l = 0; l <= opt.max_level; ++l)
+    if (!lv[l].empty()) pl.n_levels = l + 1;
+  pl.level_ptr.assign(pl.n_levels + 1, 0);
+  int nb = 0;
+  for (int l = 0; l < pl.n_levels; ++l) {
+    for (int b : lv[l]) Bx[b].id = nb++;
+    pl.level_ptr[l + 1] = nb;
+  }
+  pl.n_boxes = nb;
+  std::vector<int> id2b(nb);
+  for (int b = 0; b < (int)Bx.size(); ++b) id2b[Bx[b].id] = b;
+  pl.box_level.resize(nb);
+  pl.box_parent.resize(nb);
+  pl.box_leaf.resize(nb);
+  pl.box_center.resize(3 * (size_t)nb);
+  pl.level_side.resize(pl.n_levels);
+  for (int l = 0; l < pl.n_levels; ++l) pl.level_side[l] = pl.root_side / std::ldexp(1.0, l);
+  for (int i = 0; i < nb; ++i) {
+    const TBox& t = Bx[id2b[i]];
+    pl.box_level[i] = t.level;
+    pl.box_parent[i] = t.parent >= 0 ? Bx[t.parent].id : -1;
+    pl.box_leaf[i] = t.has_children ? 0 : 1;
+    for (int a = 0; a < 3; ++a) pl.box_center[3 * i + a] = t.X[a];
+  }
+  // DFS face order; every box's faces are a contiguous range
+  pl.face_perm.clear();
+  pl.face_perm.reserve(n_faces);
+  pl.box_range.assign(2 * (size_t)nb, 0);
+  {
+    std::function<void(int)> emit = [&](int b) {
+      const int64_t s = (int64_t)pl.face_perm.size();
+      if (!Bx[b].has_children) {
+        pl.face_perm.insert(pl.face_perm.end(), Bx[b].data.begin(), Bx[b].data.end());
+      } else {
+        // faces of this box that no child kept (reference: lost by the child's InsideQ)
+        std::vector<int> inc;
+        for (int c : Bx[b].cells)
+          if (c >= 0) inc.insert(inc.end(), Bx[c].data.begin(), Bx[c].data.end());
+        std::sort(inc.begin(), inc.end());
+        for (int f : Bx[b].data)
+          if (!std::binary_search(inc.begin(), inc.end(), f)) pl.face_perm.push_back(f);
+        for (int c : Bx[b].cells)
+          if (c >= 0) emit(c);
+      }
+      pl.box_range[2 * Bx[b].id] = s;
+      pl.box_range[2 * Bx[b].id + 1] = (int64_t)pl.face_perm.size();
+    };
+    emit(0);
+  }
+  pl.n_leaves = 0;
+  for (int i = 0; i < nb; ++i) pl.n_leaves += pl.box_leaf[i];
+  pl.t_tree = now() - t0;
+
+  // --- max_internal_offset (moments.hpp:1323-1337) ---
+  const double t1 = now();
+  for (int i = nb - 1; i >= 0; --i) {
+    TBox& t = Bx[id2b[i]];
+    t.off = 0.;
+    if (!t.has_children) {
+      for (int f : t.data) t.off = std::max(t.off, bd.maxoff[f]);
+    } else {
+      for (int c : t.cells)
+        if (c >= 0) t.off = std::max(t.off, Bx[c].off);
+    }
+  }
+  // --- near / M2L lists (buildNearAndM2LLists, moments.hpp:968-979), level-synchronous ---
+  {
+    std::vector<std::pair<int, int>> pairs;  // near pairs at the current level (A, B) as raw indices
+    for (int a : Bx[0].cells)
+      if (a >= 0)
+        for (int b : Bx[0].cells)
+          if (b >= 0) pairs.push_back({a, b});
+    // the root's children: buildNearAndM2LLists(A,B) is entered only when isNear(A,B)
+    while (!pairs.empty()) {
+      std::vector<std::pair<int, int>> next;
+      const int np = (int)pairs.size();
+      const int nt = omp_get_max_threads();
+      std::vector<std::vector<std::pair<int, int>>> next_t(nt), m2l_t(nt), near_t(nt);
+#pragma omp parallel
+      {
+        const int tid = omp_get_thread_num();
+#pragma omp for schedule(dynamic, 64)
+        for (int i = 0; i < np; ++i) {
+          const int A = pairs[i].first, Bb = pairs[i].second;
+          if (bd.is_far(A, Bb)) continue;  // isNear(A,B) is false
+          if (A != Bb) near_t[tid].push_back({A, Bb});
+          for (int ac : Bx[A].cells) {
+            if (ac < 0) continue;
+            for (int bc : Bx[Bb].cells) {
+              if (bc < 0) continue;
+              if (ac != bc && bd.is_far(ac, bc))
+                m2l_t[tid].push_back({ac, bc});
+              else
+                next_t[tid].push_back({ac, bc});
+            }
+          }
+        }
+      }
+      for (int t = 0; t < nt; ++t) {
+        for (auto& pr : near_t[t]) Bx[pr.first].near.push_back(pr.second);
+        for (auto& pr : m2l_t[t]) Bx[pr.first].m2l.push_back(pr.second);
+        next.insert(next.end(), next_t[t].begin(), next_t[t].end());
+      }
+      pairs.swap(next);
+    }
+    for (auto& t : Bx) {
+      std::sort(t.near.begin(), t.near.end());
+      std::sort(t.m2l.begin(), t.m2l.end());
+    }
+  }
+  pl.t_lists = now() - t1;
+
+  // --- targets: deepest box per node (getBucketAtDeepest) ---
+  const double t2 = now();
+  pl.target_box.resize(n_nodes);
+  std::vector<std::vector<int>> box_targets(nb);
+  for (int i = 0; i < n_nodes; ++i) {
+    const V3 x = {pl.Xs[3 * i], pl.Xs[3 * i + 1], pl.Xs[3 * i + 2]};
+    const int b = Bx[bd.deepest(x)].id;
+    pl.target_box[i] = b;
+    box_targets[b].push_back(i);
+  }
+  // --- near blocks: one per target box (split into <= kMaxRows rows) ---
+  {
+    constexpr int kMaxRows = 256;
+    std::vector<int> tboxes;
+    for (int b = 0; b < nb; ++b)
+      if (!box_targets[b].empty()) tboxes.push_back(b);
+    const int ntb = (int)tboxes.size();
+    std::vector<std::vector<int64_t>> ranges(ntb);
+    std::vector<std::vector<int32_t>> cols(ntb);
+#pragma omp parallel
+    {
+      std::vector<int> mark(n_dof, -1);
+#pragma omp for schedule(dynamic, 8)
+      for (int ti = 0; ti < ntb; ++ti) {
+        const int b = tboxes[ti];
+        const TBox& t = Bx[id2b[b]];
+        std::vector<std::pair<int64_t, int64_t>> rg;
+        auto addbox = [&](int raw) { rg.push_back({pl.box_range[2 * Bx[raw].id], pl.box_range[2 * Bx[raw].id + 1]}); };
+        addbox(id2b[b]);
+        for (int nbx : t.near) addbox(nbx);
+        for (int a = t.parent; a >= 0; a = Bx[a].parent)
+          for (int nbx : Bx[a].near)
+            if (!Bx[nbx].has_children) addbox(nbx);
+        std::sort(rg.begin(), rg.end());
+        std::vector<int64_t> merged;
+        for (auto& r : rg) {
+          if (r.first == r.second) continue;
+          if (!merged.empty() && r.first <= merged.back()) {
+            if (r.first < merged.back()) throw std::runtime_error("overlapping near ranges");
+            merged.back() = std::max(merged.back(), r.second);
+          } else {
+            merged.push_back(r.first);
+            merged.push_back(r.second);
+          }
+        }
+        std::vector<int32_t> cl;
+        for (size_t k = 0; k < merged.size(); k += 2)
+          for (int64_t i = merged[k]; i < merged[k + 1]; ++i) {
+            const int f = pl.face_perm[i];
+            for (int c = 0; c < 3; ++c) {
+              const int d = face_dof[3 * f + c];
+              if (mark[d] != ti) {
+                mark[d] = ti;
+                cl.push_back(d);
+              }
+            }
+          }
+        std::sort(cl.begin(), cl.end());
+        ranges[ti] = std::move(merged);
+        cols[ti] = std::move(cl);
+      }
+    }
+    BlockGemv& G = pl.near;
+    G = BlockGemv();
+    pl.near_ncol_dof.clear();
+    pl.near_range_ptr.assign(1, 0);
+    pl.near_ranges.clear();
+    pl.near_range_prefix.clear();
+    pl.near_nnz = 0;
+    for (int ti = 0; ti < ntb; ++ti) {
+      const int b = tboxes[ti];
+      const auto& tg = box_targets[b];
+      const int ncd = (int)cols[ti].size();
+      const int ncol = ncd + C;
+      for (size_t r0 = 0; r0 < tg.size(); r0 += kMaxRows) {
+        const size_t r1 = std::min(tg.size(), r0 + (size_t)kMaxRows);
+        G.mat_ptr.push_back(G.mat_size);
+        G.mat_size += (int64_t)(r1 - r0) * 2 * ncol;
+        for (size_t r = r0; r < r1; ++r) G.row_out.push_back(tg[r]);
+        G.row_ptr.push_back((int64_t)G.row_out.size());
+        G.col_idx.insert(G.col_idx.end(), cols[ti].begin(), cols[ti].end());
+        for (int c = 0; c < C; ++c) G.col_idx.push_back(n_dof + b * C + c);
+        G.col_ptr.push_back((int64_t)G.col_idx.size());
+        pl.near_ncol_dof.push_back(ncd);
+        int64_t pre = 0;
+        for (size_t k = 0; k < ranges[ti].size(); k += 2) {
+          pl.near_ranges.push_back(ranges[ti][k]);
+          pl.near_ranges.push_back(ranges[ti][k + 1]);
+          pl.near_range_prefix.push_back(pre);
+          pre += ranges[ti][k + 1] - ranges[ti][k];
+        }
+        pl.near_range_ptr.push_back((int64_t)pl.near_range_prefix.size());
+        pl.near_nnz += (int64_t)(r1 - r0) * ncd;
+      }
+    }
+    // L2P coefficients per near row: far = (1/s) Re sum R_n^m(x~) L~_n^m, x~ = (x - c)/s
+    const int64_t nrows = (int64_t)G.row_out.size();
+    pl.l2p_coef.assign((size_t)nrows * C, 0.);
+#pragma omp parallel
+    {
+      Harm H(P);
+#pragma omp for schedule(static)
+      for (int64_t r = 0; r < nrows; ++r) {
+        const int node = G.row_out[r];
+        const int b = pl.target_box[node];
+        const double s = pl.level_side[pl.box_level[b]];
+        const double x = (pl.Xs[3 * node] - pl.box_center[3 * b]) / s;
+        const double y = (pl.Xs[3 * node + 1] - pl.box_center[3 * b + 1]) / s;
+        const double z = (pl.Xs[3 * node + 2] - pl.box_center[3 * b + 2]) / s;
+        H.eval(x, y, z, P);
+        double* out = &pl.l2p_coef[(size_t)r * C];
+        for (int n = 0; n <= P; ++n)
+          for (int m = 0; m <= n; ++m) {
+            const double c = H.cnm[Harm::t(n, m)];
+            const double Rr = c * H.re[Harm::t(n, m)], Ri = c * H.im[Harm::t(n, m)];
+            if (m == 0) {
+              out[pk_re(n, 0)] = Rr / s;
+            } else {
+              out[pk_re(n, m)] = 2.0 * Rr / s;
+              out[pk_im(n, m)] = -2.0 * Ri / s;
+            }
+          }
+      }
+    }
+  }
+  pl.t_near_struct = now() - t2;
+
+  // --- P2M blocks: one per leaf; M~_n^m = sum W b_k (phi_n, phi) (conj R_n^m(x~), conj dR_n^m.n/s) ---
+  const double t3 = now();
+  {
+    std::vector<int> leaves;
+    for (int i = 0; i < nb; ++i)
+      if (pl.box_leaf[i]) leaves.push_back(i);
+    const int nl = (int)leaves.size();
+    std::vector<std::vector<int32_t>> cols(nl);
+    for (int li = 0; li < nl; ++li) {
+      const TBox& t = Bx[id2b[leaves[li]]];
+      std::vector<int32_t> cl;
+      for (int f : t.data)
+        for (int c = 0; c < 3; ++c) cl.push_back(face_dof[3 * f + c]);
+      std::sort(cl.begin(), cl.end());
+      cl.erase(std::unique(cl.begin(), cl.end()), cl.end());
+      cols[li] = std::move(cl);
+    }
+    BlockGemv& G = pl.p2m;
+    G = BlockGemv();
+    for (int li = 0; li < nl; ++li) {
+      const int b = leaves[li];
+      const int ncol = (int)cols[li].size();
+      G.mat_ptr.push_back(G.mat_size);
+      G.mat_size += (int64_t)C * 2 * ncol;
+      for (int r = 0; r < C; ++r) G.row_out.push_back(b * C + r);
+      G.row_ptr.push_back((int64_t)G.row_out.size());
+      G.col_idx.insert(G.col_idx.end(), cols[li].begin(), cols[li].end());
+      G.col_ptr.push_back((int64_t)G.col_idx.size());
+    }
+    pl.p2m_mat.assign(G.mat_size, 0.);
+#pragma omp parallel
+    {
+      Harm H(P + 1);
+#pragma omp for schedule(dynamic, 4)
+      for (int li = 0; li < nl; ++li) {
+        const int b = leaves[li];
+        const TBox& t = Bx[id2b[b]];
+        const double s = pl.level_side[pl.box_level[b]];
+        const int ncol = (int)cols[li].size();
+        double* M = &pl.p2m_mat[G.mat_ptr[li]];
+        for (int f : t.data) {
+          int lc[3];
+          for (int c = 0; c < 3; ++c)
+            lc[c] = (int)(std::lower_bound(cols[li].begin(), cols[li].end(), face_dof[3 * f + c]) - cols[li].begin());
+          const double J = pl.fJ[f];
+          const double nx = pl.fcr[3 * f] / J, ny = pl.fcr[3 * f + 1] / J, nz = pl.fcr[3 * f + 2] / J;
+          for (const auto& rq : rule) {
+            const double W = J * rq[3];
+            double xq[3];
+            for (int a = 0; a < 3; ++a)
+              xq[a] = rq[0] * pl.fX[9 * f + a] + rq[1] * pl.fX[9 * f + 3 + a] + rq[2] * pl.fX[9 * f + 6 + a];
+            H.eval((xq[0] - pl.box_center[3 * b]) / s, (xq[1] - pl.box_center[3 * b + 1]) / s,
+                   (xq[2] - pl.box_center[3 * b + 2]) / s, P);
+            for (int n = 0; n <= P; ++n)
+              for (int m = 0; m <= n; ++m) {
+                const double c = H.cnm[Harm::t(n, m)];
+                const double Rr = c * H.re[Harm::t(n, m)], Ri = c * H.im[Harm::t(n, m)];
+                // grad R_n^m . n = c(n,m) [ -a NR_{n-1}^{m+1} + b NR_{n-1}^{m-1} + nz NR_{n-1}^m ],
+                // a = (nx - i ny)/2, b = (nx + i ny)/2
+                double dr = 0, di = 0;
+                if (n > 0) {
+                  double p1r, p1i, m1r, m1i, z0r, z0i;
+                  H.nr(n - 1, m + 1, p1r, p1i);
+                  H.nr(n - 1, m - 1, m1r, m1i);
+                  H.nr(n - 1, m, z0r, z0i);
+                  const double ar = 0.5 * nx, ai = -0.5 * ny, br = 0.5 * nx, bi = 0.5 * ny;
+                  dr = -(ar * p1r - ai * p1i) + (br * m1r - bi * m1i) + nz * z0r;
+                  di = -(ar * p1i + ai * p1r) + (br * m1i + bi * m1r) + nz * z0i;
+                  dr *= c / s;
+                  di *= c / s;
+                }
+                for (int k = 0; k < 3; ++k) {
+                  const double wk = W * rq[k];
+                  const int col = lc[k];
+                  const int rr = pk_re(n, m);
+                  M[(size_t)rr * 2 * ncol + col] += wk * Rr;
+                  M[(size_t)rr * 2 * ncol + ncol + col] += wk * dr;
+                  if (m > 0) {
+                    const int ri = pk_im(n, m);
+                    M[(size_t)ri * 2 * ncol + col] += -wk * Ri;
+                    M[(size_t)ri * 2 * ncol + ncol + col] += -wk * di;
+                  }
+                }
+              }
+          }
+        }
+      }
+    }
+  }
+  pl.t_p2m = now() - t3;
+
+  // --- translation pairs and operators ---
+  const double t4 = now();
+  {
+    pl.m2m_by_level.assign(pl.n_levels, PairGroups());
+    pl.l2l_by_level.assign(pl.n_levels, PairGroups());
+    for (int l = 1; l < pl.n_levels; ++l) {
+      std::vector<std::vector<std::pair<int, int>>> g(8);
+      for (int c = pl.level_ptr[l]; c < pl.level_ptr[l + 1]; ++c) {
+        const int p = pl.box_parent[c];
+        int oct = 0;
+        for (int a = 0; a < 3; ++a)
+          if (pl.box_center[3 * c + a] > pl.box_center[3 * p + a]) oct |= 1 << a;
+        g[oct].push_back({p, c});
+      }
+      for (int o = 0; o < 8; ++o) {
+        if (g[o].empty()) continue;
+        PairGroups& M = pl.m2m_by_level[l];
+        PairGroups& Lg = pl.l2l_by_level[l];
+        M.op.push_back(o);
+        Lg.op.push_back(8 + o);
+        for (auto& pr : g[o]) {
+          M.dst.push_back(pr.first);
+          M.src.push_back(pr.second);
+          Lg.dst.push_back(pr.second);
+          Lg.src.push_back(pr.first);
+        }
+        M.pair_ptr.push_back(M.n_pairs());
+        Lg.pair_ptr.push_back(Lg.n_pairs());
+      }
+    }
+    // M2L: L(A) += T(offset) M(B) for B in A's M2L list (buckets_for_L2M is the same set by symmetry)
+    std::map<std::array<int, 3>, std::vector<std::pair<int, int>>> byoff;
+    pl.n_m2l_pairs = 0;
+    for (int i = 0; i < nb; ++i) {
+      const TBox& t = Bx[id2b[i]];
+      const double s = pl.level_side[t.level];
+      for (int src : t.m2l) {
+        const TBox& u = Bx[src];
+        std::array<int, 3> o;
+        for (int a = 0; a < 3; ++a) {
+          const double v = (u.X[a] - t.X[a]) / s;
+          o[a] = (int)std::lround(v);
+          if (std::abs(v - o[a]) > 1e-6) throw std::runtime_error("M2L pair not on the level lattice");
+        }
+        byoff[o].push_back({i, u.id});
+        ++pl.n_m2l_pairs;
+      }
+    }
+    pl.m2l = PairGroups();
+    pl.m2l_offsets.clear();
+    int opid = 16;
+    for (auto& kv : byoff) {
+      pl.m2l.op.push_back(opid++);
+      pl.m2l_offsets.push_back(kv.first[0]);
+      pl.m2l_offsets.push_back(kv.first[1]);
+      pl.m2l_offsets.push_back(kv.first[2]);
+      for (auto& pr : kv.second) {
+        pl.m2l.dst.push_back(pr.first);
+        pl.m2l.src.push_back(pr.second);
+      }
+      pl.m2l.pair_ptr.push_back(pl.m2l.n_pairs());
+    }
+    pl.n_ops = opid;
+    pl.ops.assign((size_t)pl.n_ops * C * C, 0.);
+    // operator cache shared by all plans of the process
+    static std::mutex mu;
+    static std::map<std::tuple<int, int, int, int>, std::vector<double>> cache;
+    auto get = [&](int kind, int a, int b, int c, double* out) {
+      const auto key = std::make_tuple(P * 16 + kind, a, b, c);
+      {
+        std::lock_guard<std::mutex> lk(mu);
+        auto it = cache.find(key);
+        if (it != cache.end()) {
+          std::copy(it->second.begin(), it->second.end(), out);
+          return;
+        }
+      }
+      std::vector<double> v((size_t)C * C);
+      if (kind == 0) m2m_operator(P, a, v.data());
+      else if (kind == 1) l2l_operator(P, a, v.data());
+      else m2l_operator(P, a, b, c, v.data());
+      std::copy(v.begin(), v.end(), out);
+      std::lock_guard<std::mutex> lk(mu);
+      cache[key] = std::move(v);
+    };
+    const int nm2l = (int)pl.m2l.op.size();
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int i = 0; i < 16 + nm2l; ++i) {
+      double* out = &pl.ops[(size_t)i * C * C];
+      if (i < 8) get(0, i, 0, 0, out);
+      else if (i < 16) get(1, i - 8, 0, 0, out);
+      else {
+        const int g = i - 16;
+        get(2, pl.m2l_offsets[3 * g], pl.m2l_offsets[3 * g + 1], pl.m2l_offsets[3 * g + 2], out);
+      }
+    }
+  }
+  pl.t_ops = now() - t4;
+}
+
+}  // namespace bemfmm

@@ -1,0 +1,104 @@
+// Host-side FMM plan: the flat, device-ready description of one BVP built from
+// the caller's mesh/DOF arrays. Built by plan.cpp (g++/OpenMP, float128
+// operators in operators.cpp), uploaded and executed by bemfmm.hip.
+#pragma once
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace bemfmm {
+
+struct Options {
+  int p = 10;             // expansion order (Buckets<...,N> template argument, BEM_solveBVP.hpp:563)
+  double theta = 0.25;    // mac_theta (main_time_domain.cpp:63)
+  int leaf_max = 50;      // fmm_bucket_max_points (main_time_domain.cpp:54-55)
+  int max_level = 7;      // fmm_max_level
+  int q = 6;              // p2m_quadrature_points (main_time_domain.cpp:62)
+};
+
+// Box-blocked dense GEMV: for block b, rows [row_ptr[b], row_ptr[b+1]) and
+// gathered columns col_idx[col_ptr[b] .. col_ptr[b+1]).  Row r of block b holds
+// two interleaved dense rows (A0 then A1, each ncol long) at mat_ptr[b] + 2*ncol*(r-row_ptr[b]).
+//   y0[row_out[r]] = A0[r,:] . x0[col_idx], y1[row_out[r]] = A1[r,:] . x1[col_idx]
+struct BlockGemv {
+  std::vector<int64_t> row_ptr{0};
+  std::vector<int64_t> col_ptr{0};
+  std::vector<int64_t> mat_ptr;  // per block
+  std::vector<int32_t> row_out;
+  std::vector<int32_t> col_idx;
+  int64_t mat_size = 0;  // doubles
+  int n_blocks() const { return (int)mat_ptr.size(); }
+};
+
+// Translation pairs grouped by operator: group g uses operator op[g] and pairs
+// [pair_ptr[g], pair_ptr[g+1]) of (dst box, src box).
+struct PairGroups {
+  std::vector<int32_t> op;
+  std::vector<int64_t> pair_ptr{0};
+  std::vector<int32_t> dst, src;
+  int64_t n_pairs() const { return (int64_t)dst.size(); }
+};
+
+struct Plan {
+  Options opt;
+  int P = 10, C = 121;  // C = (P+1)^2 real-field packed coefficients per layer
+  int n_nodes = 0, n_faces = 0, n_dof = 0;
+  double Lscale = 1.0;      // coordinate scale (bounding-box diagonal)
+  double scale = 1.0;       // getScale() of the scaled geometry
+  double near_region = 0;   // scale/25
+  double root_side = 0;     // side of the level-0 cube
+  // scaled geometry
+  std::vector<double> Xs;            // n_nodes*3
+  std::vector<double> fX;            // n_faces*9 vertex coordinates
+  std::vector<double> fcent;         // n_faces*3 (a+b+c)/3
+  std::vector<double> fcr;           // n_faces*3 (2*area*normal)
+  std::vector<double> fJ;            // n_faces
+  std::vector<int32_t> fv;           // n_faces*3 node ids
+  std::vector<int32_t> fdof;         // n_faces*3 DOF ids
+  std::vector<int32_t> dof_kind, dof_point;
+  // tree (boxes numbered level-major, preorder within a level)
+  int n_levels = 0, n_boxes = 0;
+  std::vector<int32_t> level_ptr;    // n_levels+1
+  std::vector<int32_t> box_level, box_parent, box_leaf;
+  std::vector<double> box_center;    // n_boxes*3
+  std::vector<double> level_side;    // ideal side per level (root_side/2^l)
+  std::vector<int32_t> face_perm;    // DFS order of faces
+  std::vector<int64_t> box_range;    // n_boxes*2 range into face_perm
+  // near field + L2P (one block per target box)
+  BlockGemv near;
+  std::vector<int32_t> near_ncol_dof;     // per block: DOF columns (then C L2P columns)
+  std::vector<int64_t> near_range_ptr{0}; // per block ranges into face_perm
+  std::vector<int64_t> near_ranges;       // pairs (start,end)
+  std::vector<int64_t> near_range_prefix; // per range: faces before it within its block
+  std::vector<int32_t> target_box;        // per node
+  std::vector<double> l2p_coef;           // per near row: C values (row order of near.row_out)
+  // P2M (one block per leaf)
+  BlockGemv p2m;
+  std::vector<double> p2m_mat;            // p2m.mat_size
+  // translations
+  std::vector<PairGroups> m2m_by_level;   // index = child level
+  std::vector<PairGroups> l2l_by_level;   // index = child level
+  PairGroups m2l;
+  int n_ops = 0;
+  std::vector<double> ops;                // n_ops * C * C, layout [op][k_in][r_out]
+  std::vector<int32_t> m2l_offsets;       // per M2L op: (dx,dy,dz)
+  // stats
+  int64_t n_m2l_pairs = 0, near_nnz = 0, n_leaves = 0;
+  double t_tree = 0, t_lists = 0, t_near_struct = 0, t_p2m = 0, t_ops = 0;
+};
+
+// Builds everything except the near-field quadrature weights (computed on the GPU).
+// Throws std::runtime_error on invalid input.
+void build_plan(Plan& pl, const Options& opt, int n_nodes, const double* X, int n_faces, const int32_t* tri,
+                const int32_t* face_dof, int n_dof, const int32_t* dof_kind, const int32_t* dof_point);
+
+// Real-field packed translation operators (operators.cpp), computed in __float128
+// and rounded once to double. Layout: out[k_in * C + r_out].
+void m2l_operator(int P, int dx, int dy, int dz, double* out);
+void m2m_operator(int P, int octant, double* out);  // child -> parent; octant bit a set = child on + side of axis a
+void l2l_operator(int P, int octant, double* out);  // parent -> child
+
+inline int pk_re(int n, int m) { return m == 0 ? n * n : n * n + 2 * m - 1; }
+inline int pk_im(int n, int m) { return n * n + 2 * m; }
+
+}  // namespace bemfmm

@@ -1,0 +1,139 @@
+/*
+ * bemfmm.h — C ABI of the MI355X-native FMM-accelerated GMRES BVP engine.
+ *
+ * Drop-in for the matvec/solve path of the reference's BEM_BVP (repository
+ * tomoakihirakawa/BEM_for_Nonlinear_Waves): the caller flattens its Network
+ * (nodes, boundary triangles, per-(point,face) DOFs) into plain arrays, and the
+ * library performs what BEM_BVP::solveGMRES does after that point: coordinate
+ * scaling, sources, octree, interaction lists, near-field quadrature, the
+ * rigid-mode diagonal, and FMM-accelerated GMRES.
+ *
+ * Model of the existing device C ABI in the reference: bem/metal_m2l/metal_m2l.h:23-129
+ * and bem/metal_nearfield/metal_nearfield.h:33-141 (opaque context, int return
+ * codes, host-owned inputs copied at call time).
+ *
+ * Conventions
+ *   - Return codes: 0 = OK, negative = error (see BEMFMM_E_*); bemfmm_last_error()
+ *     returns the message of the last failing call on a context.
+ *   - All pointers without a _device suffix are host memory owned by the caller and
+ *     are only read or written during the call.
+ *   - A context is used by one host thread at a time; several contexts may share a
+ *     device, each with its own HIP stream.
+ *   - DOF kinds: 0 = Dirichlet (unknown phi_n, known phi), 1 = Neumann (unknown phi,
+ *     known phi_n), 2 = CORNER Neumann-face DOF (unknown phi; its row is the
+ *     constraint phi = phi_D, with phi_D passed in phi[]).
+ *     (isDirichletID_BEM / isNeumannID_BEM, bem/BEM_setBoundaryTypes.hpp:29-40)
+ */
+#ifndef BEMFMM_H
+#define BEMFMM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BEMFMM_OK 0
+#define BEMFMM_E_INVALID (-1)      /* invalid argument or geometry */
+#define BEMFMM_E_DEVICE (-2)       /* no usable GPU / HIP runtime error */
+#define BEMFMM_E_OOM (-3)          /* device allocation failed */
+#define BEMFMM_E_NONFINITE (-4)    /* non-finite input or result */
+#define BEMFMM_E_NOT_CONVERGED (-5) /* GMRES stopped above tolerance */
+#define BEMFMM_E_STATE (-6)        /* call out of order (e.g. solve before prepare) */
+
+#define BEMFMM_PRECISION_FP64 0     /* every stage in FP64 */
+#define BEMFMM_PRECISION_FP32_REFINE 1 /* FP32 near field and M2L inside FP64 iterative refinement */
+
+#define BEMFMM_PRECOND_NONE 0      /* reference default (main_time_domain.cpp:38) */
+#define BEMFMM_PRECOND_JACOBI 1    /* diagonal scaling (BEM_solveBVP_GMRES_FMM.hpp:2095-2104) */
+
+typedef struct bemfmm_ctx bemfmm_ctx;
+
+typedef struct bemfmm_opts {
+  int p;              /* expansion order N (reference: Buckets<...,10>, BEM_solveBVP.hpp:563) */
+  double theta;       /* mac_theta, default 0.25 (main_time_domain.cpp:63) */
+  int leaf_max;       /* fmm_bucket_max_points, default 50 (main_time_domain.cpp:54) */
+  int max_level;      /* fmm_max_level, default 7 (main_time_domain.cpp:55) */
+  int q;              /* p2m_quadrature_points, default 6 (main_time_domain.cpp:62) */
+  int precision;      /* BEMFMM_PRECISION_* */
+  int preconditioner; /* BEMFMM_PRECOND_* */
+} bemfmm_opts;
+
+typedef struct bemfmm_result {
+  int iters;              /* GMRES iterations (matvecs inside Arnoldi) */
+  int n_matvec;           /* all matvecs including residual checks */
+  int converged;          /* 1 if |M^-1 (b - A x)|_2 <= tol */
+  double residual;        /* last true residual (absolute, scaled system) */
+  double time_setup;      /* seconds: plan + near-field quadrature + alpha */
+  double time_solve;      /* seconds: RHS + GMRES */
+  int refinement_steps;   /* FP32_REFINE: outer FP64 refinement steps */
+} bemfmm_result;
+
+typedef struct bemfmm_stats {
+  int n_nodes, n_faces, n_dof;
+  int n_levels, n_boxes, n_leaves;
+  int64_t n_m2l_pairs;
+  int64_t near_nnz;       /* sum over targets of near DOF columns */
+  int n_m2l_ops;          /* distinct M2L offsets */
+  double L;               /* coordinate scale (bounding-box diagonal) */
+  double device_bytes;    /* resident device memory of the plan */
+  double t_tree, t_lists, t_near_struct, t_p2m, t_ops, t_upload, t_near_quad, t_alpha;
+} bemfmm_stats;
+
+/* Availability (metal_m2l.h: metal_m2l_is_available / metal_m2l_device_name). */
+int bemfmm_is_available(void);
+const char* bemfmm_device_name(void);
+
+void bemfmm_default_opts(bemfmm_opts* opts);
+
+/* Context lifecycle (replaces the BEM_BVP members B_poles/targets/preconditioner). */
+bemfmm_ctx* bemfmm_ctx_create(int device, const bemfmm_opts* opts);
+void bemfmm_ctx_destroy(bemfmm_ctx* ctx);
+const char* bemfmm_last_error(const bemfmm_ctx* ctx);
+
+/* Geometry and DOF model: replaces WATERS[0] (Network) + setNodeFaceIndices (f2Index).
+ * X: n_nodes*3 coordinates (unscaled); tri: n_faces*3 node ids, outward (out of the fluid);
+ * face_dof: n_faces*3 DOF id of each face corner (pf2Index(p, f)); dof_kind/dof_point: per DOF. */
+int bemfmm_set_geometry(bemfmm_ctx* ctx, int n_nodes, const double* X, int n_faces, const int32_t* tri,
+                        const int32_t* face_dof, int n_dof, const int32_t* dof_kind, const int32_t* dof_point);
+
+/* Known boundary values per DOF (phiOnFace / phinOnFace): phi for Dirichlet DOFs (and phi_D for
+ * CORNER Neumann DOFs), phi_n for Neumann and CORNER Neumann DOFs; other entries are ignored. */
+int bemfmm_set_bc(bemfmm_ctx* ctx, const double* phi, const double* phin);
+
+/* Setup (solveGMRES up to GMRES, BEM_solveBVP_GMRES_FMM.hpp:3076-3450 + initializeFMM,
+ * moments.hpp:1239-1541): scaling, sources, tree, lists, operators, near-field quadrature,
+ * rigid-mode diagonal alpha, preconditioner. */
+int bemfmm_prepare(bemfmm_ctx* ctx);
+
+/* y = A x for the scaled system (return_A_dot_v without preconditioner,
+ * BEM_solveBVP_GMRES_FMM.hpp:2124-2155 -> compute_Ax_minus_b :34-158). n_dof entries. */
+int bemfmm_matvec(bemfmm_ctx* ctx, const double* x, double* y);
+/* Same on device pointers (inputs resident in HBM); ordered on the context's stream. */
+int bemfmm_matvec_device(bemfmm_ctx* ctx, const double* x_device, double* y_device);
+/* Synchronise the context's stream. */
+int bemfmm_sync(bemfmm_ctx* ctx);
+
+/* b = -(A x - b)|_{x=0} with CORNER rows = phi_D (solveSystemGMRES, BEM_solveBVP_GMRES_FMM.hpp:2025-2047). */
+int bemfmm_rhs(bemfmm_ctx* ctx, double* b);
+/* Rigid-mode diagonal coefficients per DOF (updateGeometricProperties, :2674-2716). */
+int bemfmm_alpha(bemfmm_ctx* ctx, double* alpha);
+
+/* Solve (solveSystemGMRES + run_gmres, :2007-2355). x0/x: n_dof, physical units (phi_n unscaled
+ * on Dirichlet DOFs as stored by storePhiPhin, :3462-3489). Returns BEMFMM_E_NOT_CONVERGED if the
+ * true residual stays above tol (x still holds the last iterate). */
+int bemfmm_solve(bemfmm_ctx* ctx, const double* x0, double* x, double tol, int max_iter, int restart,
+                 bemfmm_result* result);
+
+int bemfmm_get_stats(const bemfmm_ctx* ctx, bemfmm_stats* stats);
+
+/* Kernel timing of the last matvec_device call (milliseconds, HIP events on the context's stream):
+ * [0] scatter, [1] P2M, [2] M2M, [3] M2L, [4] L2L, [5] near+L2P, [6] assemble. Enables event
+ * recording on the first call. */
+int bemfmm_matvec_profile(bemfmm_ctx* ctx, double* ms7);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* BEMFMM_H */

@@ -142,9 +142,17 @@ struct Harm {
 // This is the same derivative the reference writes with the polar-angle chain
 // rule, but stays finite on the z axis.
 double lfact(int k) { return std::lgamma(k + 1.0); }
+struct CfacTable {
+  double v[64][64];
+  CfacTable() {
+    for (int n = 0; n < 64; ++n)
+      for (int a = 0; a <= n; ++a) v[n][a] = std::exp(0.5 * (lfact(n - a) + lfact(n + a)));
+  }
+};
 double cfac(int n, int q) {
+  static const CfacTable t;
   const int a = std::abs(q);
-  const double v = std::exp(0.5 * (lfact(n - a) + lfact(n + a)));
+  const double v = t.v[n][a];
   return (q > 0 && (q & 1)) ? -v : v;
 }
 

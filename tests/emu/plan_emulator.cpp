@@ -1,0 +1,222 @@
+// Test-only host emulator of the device matvec pipeline (bemfmm.hip::evaluate), used to
+// check the product's plan (tree, lists, float128 operators, P2M/L2P packing, near
+// quadrature) against the oracle on machines without a GPU. Executes the same plan
+// arrays with plain loops; never linked into libbemfmm.so.
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "../../bem_for_nonlinear_waves_amd/csrc/nearquad.hpp"
+#include "../../bem_for_nonlinear_waves_amd/csrc/plan.hpp"
+
+namespace {
+const double kGW5[25][3] = {
+    {0.046910077030668, 0.046910077030668, 0.0140335872156063}, {0.046910077030668, 0.230765344947158, 0.0283499999999991},
+    {0.046910077030668, 0.5, 0.0336962680968792},               {0.046910077030668, 0.769234655052841, 0.0283499999999991},
+    {0.046910077030668, 0.953089922969332, 0.0140335872156063}, {0.230765344947158, 0.046910077030668, 0.0283499999999991},
+    {0.230765344947158, 0.230765344947158, 0.0572713510559978}, {0.230765344947158, 0.5, 0.0680716331376877},
+    {0.230765344947158, 0.769234655052841, 0.0572713510559978}, {0.230765344947158, 0.953089922969332, 0.0283499999999991},
+    {0.5, 0.046910077030668, 0.0336962680968792},               {0.5, 0.230765344947158, 0.0680716331376877},
+    {0.5, 0.5, 0.0809086419753086},                             {0.5, 0.769234655052841, 0.0680716331376877},
+    {0.5, 0.953089922969332, 0.0336962680968792},               {0.769234655052841, 0.046910077030668, 0.0283499999999991},
+    {0.769234655052841, 0.230765344947158, 0.0572713510559978}, {0.769234655052841, 0.5, 0.0680716331376877},
+    {0.769234655052841, 0.769234655052841, 0.0572713510559978}, {0.769234655052841, 0.953089922969332, 0.0283499999999991},
+    {0.953089922969332, 0.046910077030668, 0.0140335872156063}, {0.953089922969332, 0.230765344947158, 0.0283499999999991},
+    {0.953089922969332, 0.5, 0.0336962680968792},               {0.953089922969332, 0.769234655052841, 0.0283499999999991},
+    {0.953089922969332, 0.953089922969332, 0.0140335872156063}};
+const double kD6[6][4] = {{0.816847572980459, 0.091576213509771, 0.091576213509771, 0.054975871827661},
+                          {0.091576213509771, 0.816847572980459, 0.091576213509771, 0.054975871827661},
+                          {0.091576213509771, 0.091576213509771, 0.816847572980459, 0.054975871827661},
+                          {0.108103018168070, 0.445948490915965, 0.445948490915965, 0.111690794839006},
+                          {0.445948490915965, 0.108103018168070, 0.445948490915965, 0.111690794839006},
+                          {0.445948490915965, 0.445948490915965, 0.108103018168070, 0.111690794839006}};
+
+struct Emu {
+  bemfmm::Plan pl;
+  std::vector<double> near_mat, alpha, phik, phink;
+  std::vector<double> buf0, buf1, M0, M1, y0, y1;
+
+  void gemv(const bemfmm::BlockGemv& G, const double* mat, const double* x0, const double* x1, double* o0, double* o1) {
+#pragma omp parallel for schedule(dynamic, 4)
+    for (int b = 0; b < G.n_blocks(); ++b) {
+      const int64_t c0 = G.col_ptr[b];
+      const int ncol = (int)(G.col_ptr[b + 1] - c0);
+      for (int64_t r = G.row_ptr[b]; r < G.row_ptr[b + 1]; ++r) {
+        const double* A0 = mat + G.mat_ptr[b] + (r - G.row_ptr[b]) * 2 * ncol;
+        const double* A1 = A0 + ncol;
+        double s0 = 0, s1 = 0;
+        for (int c = 0; c < ncol; ++c) {
+          s0 += A0[c] * x0[G.col_idx[c0 + c]];
+          s1 += A1[c] * x1[G.col_idx[c0 + c]];
+        }
+        o0[G.row_out[r]] = s0;
+        o1[G.row_out[r]] = s1;
+      }
+    }
+  }
+  void translate(const bemfmm::PairGroups& g, const double* S0, const double* S1, double* D0, double* D1) {
+    const int C = pl.C;
+    for (size_t k = 0; k < g.op.size(); ++k) {
+      const double* O = &pl.ops[(size_t)g.op[k] * C * C];
+      for (int64_t p = g.pair_ptr[k]; p < g.pair_ptr[k + 1]; ++p) {
+        const double* s0 = S0 + (int64_t)g.src[p] * C;
+        const double* s1 = S1 + (int64_t)g.src[p] * C;
+        double* d0 = D0 + (int64_t)g.dst[p] * C;
+        double* d1 = D1 + (int64_t)g.dst[p] * C;
+        for (int kk = 0; kk < C; ++kk)
+          for (int r = 0; r < C; ++r) {
+            d0[r] += O[(size_t)kk * C + r] * s0[kk];
+            d1[r] += O[(size_t)kk * C + r] * s1[kk];
+          }
+      }
+    }
+  }
+  void evaluate(int mode, const double* x, double* out) {
+    const int n = pl.n_dof, C = pl.C;
+    const int64_t nbC = (int64_t)pl.n_boxes * C;
+    buf0.assign(n + nbC, 0.);
+    buf1.assign(n + nbC, 0.);
+    M0.assign(nbC, 0.);
+    M1.assign(nbC, 0.);
+    y0.assign(pl.n_nodes, 0.);
+    y1.assign(pl.n_nodes, 0.);
+    for (int i = 0; i < n; ++i) {
+      const bool dir = pl.dof_kind[i] == 0;
+      double a = 0, b = 0;
+      if (mode == 0) (dir ? a : b) = x[i];
+      else if (mode == 1) {
+        if (dir) b = phik[i];
+        else a = phink[i];
+      } else b = 1.0;
+      buf0[i] = a;
+      buf1[i] = b;
+    }
+    gemv(pl.p2m, pl.p2m_mat.data(), buf0.data(), buf1.data(), M0.data(), M1.data());
+    for (int l = pl.n_levels - 1; l >= 1; --l) translate(pl.m2m_by_level[l], M0.data(), M1.data(), M0.data(), M1.data());
+    translate(pl.m2l, M0.data(), M1.data(), buf0.data() + n, buf1.data() + n);
+    for (int l = 1; l < pl.n_levels; ++l)
+      translate(pl.l2l_by_level[l], buf0.data() + n, buf1.data() + n, buf0.data() + n, buf1.data() + n);
+    gemv(pl.near, near_mat.data(), buf0.data(), buf1.data(), y0.data(), y1.data());
+    for (int i = 0; i < n; ++i) {
+      const int p = pl.dof_point[i];
+      if (mode == 2) {
+        double a = -y1[p];
+        if (std::fabs(a) < 1e-15) a = 1.0;
+        out[i] = a;
+        continue;
+      }
+      double v = (pl.dof_kind[i] == 2) ? buf1[i] : y0[p] - (y1[p] + alpha[i] * buf1[i]);
+      if (mode == 1) v = (pl.dof_kind[i] == 2) ? phik[i] : -v;
+      out[i] = v;
+    }
+  }
+  void near_quad() {
+    const auto& G = pl.near;
+    const int C = pl.C;
+    near_mat.assign(G.mat_size, 0.);
+    int64_t rows = (int64_t)G.row_out.size();
+    std::vector<int> row_block(rows);
+    for (int b = 0; b < G.n_blocks(); ++b)
+      for (int64_t r = G.row_ptr[b]; r < G.row_ptr[b + 1]; ++r) row_block[r] = b;
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int64_t r = 0; r < rows; ++r) {
+      const int b = row_block[r];
+      const int ncd = pl.near_ncol_dof[b];
+      const int64_t c0 = G.col_ptr[b];
+      const int ncol = (int)(G.col_ptr[b + 1] - c0);
+      double* A0 = &near_mat[G.mat_ptr[b] + (r - G.row_ptr[b]) * 2 * ncol];
+      double* A1 = A0 + ncol;
+      for (int c = 0; c < C; ++c) A0[ncd + c] = A1[ncd + c] = pl.l2p_coef[(size_t)r * C + c];
+      const int t = G.row_out[r];
+      for (int64_t g = pl.near_range_ptr[b]; g < pl.near_range_ptr[b + 1]; ++g)
+        for (int64_t i = pl.near_ranges[2 * g]; i < pl.near_ranges[2 * g + 1]; ++i) {
+          const int f = pl.face_perm[i];
+          int slot = -1;
+          for (int k = 0; k < 3; ++k)
+            if (pl.fv[3 * f + k] == t) slot = k;
+          double WG[3], WGn[3];
+          bemfmm::face_weights(&pl.fX[9 * (size_t)f], pl.fJ[f], &pl.fcr[3 * (size_t)f], &pl.fcent[3 * (size_t)f], slot,
+                               pl.Xs[3 * t], pl.Xs[3 * t + 1], pl.Xs[3 * t + 2], pl.near_region, kGW5, kD6, 6, WG, WGn);
+          for (int k = 0; k < 3; ++k) {
+            const int d = pl.fdof[3 * f + k];
+            const int32_t* cols = &G.col_idx[c0];
+            int lo = 0, hi = ncd - 1;
+            while (lo < hi) {
+              const int mid = (lo + hi) >> 1;
+              if (cols[mid] < d) lo = mid + 1;
+              else hi = mid;
+            }
+            A0[lo] += WG[k];
+            A1[lo] += WGn[k];
+          }
+        }
+    }
+  }
+};
+}  // namespace
+
+extern "C" {
+void* emu_create(int n_nodes, const double* X, int n_faces, const int32_t* tri, const int32_t* face_dof, int n_dof,
+                 const int32_t* kind, const int32_t* point, const double* phi, const double* phin, int p, double theta,
+                 int leaf_max, int max_level) {
+  auto* e = new Emu();
+  bemfmm::Options o;
+  o.p = p;
+  o.theta = theta;
+  o.leaf_max = leaf_max;
+  o.max_level = max_level;
+  o.q = 6;
+  bemfmm::build_plan(e->pl, o, n_nodes, X, n_faces, tri, face_dof, n_dof, kind, point);
+  e->phik.assign(phi, phi + n_dof);
+  e->phink.resize(n_dof);
+  for (int i = 0; i < n_dof; ++i) e->phink[i] = kind[i] != 0 ? phin[i] * e->pl.Lscale : phin[i];
+  e->near_quad();
+  e->alpha.assign(n_dof, 1.0);
+  std::vector<double> a(n_dof);
+  e->evaluate(2, nullptr, a.data());
+  e->alpha = a;
+  return e;
+}
+void emu_destroy(void* h) { delete static_cast<Emu*>(h); }
+void emu_matvec(void* h, const double* x, double* y) { static_cast<Emu*>(h)->evaluate(0, x, y); }
+void emu_rhs(void* h, double* b) { static_cast<Emu*>(h)->evaluate(1, nullptr, b); }
+void emu_alpha(void* h, double* a) {
+  auto* e = static_cast<Emu*>(h);
+  std::memcpy(a, e->alpha.data(), e->alpha.size() * sizeof(double));
+}
+// stats: n_levels, n_boxes, n_leaves, n_m2l_pairs, near_nnz, n_m2l_ops
+void emu_stats(void* h, int64_t* out) {
+  auto* e = static_cast<Emu*>(h);
+  out[0] = e->pl.n_levels;
+  out[1] = e->pl.n_boxes;
+  out[2] = e->pl.n_leaves;
+  out[3] = e->pl.n_m2l_pairs;
+  out[4] = e->pl.near_nnz;
+  out[5] = (int64_t)e->pl.m2l.op.size();
+}
+// near row of node t restricted to DOF columns: returns count, fills idx/wG/wGn
+int emu_near_row(void* h, int t, int32_t* idx, double* wg, double* wgn, int cap) {
+  auto* e = static_cast<Emu*>(h);
+  const auto& G = e->pl.near;
+  for (int b = 0; b < G.n_blocks(); ++b)
+    for (int64_t r = G.row_ptr[b]; r < G.row_ptr[b + 1]; ++r) {
+      if (G.row_out[r] != t) continue;
+      const int ncd = e->pl.near_ncol_dof[b];
+      const int ncol = (int)(G.col_ptr[b + 1] - G.col_ptr[b]);
+      const double* A0 = &e->near_mat[G.mat_ptr[b] + (r - G.row_ptr[b]) * 2 * ncol];
+      for (int c = 0; c < ncd && c < cap; ++c) {
+        idx[c] = G.col_idx[G.col_ptr[b] + c];
+        wg[c] = A0[c];
+        wgn[c] = A0[ncol + c];
+      }
+      return ncd;
+    }
+  return -1;
+}
+// one translation operator (for tests): kind 0 = M2M(octant a), 1 = L2L(octant a), 2 = M2L(a,b,c)
+void emu_operator(int p, int kind, int a, int b, int c, double* out) {
+  if (kind == 0) bemfmm::m2m_operator(p, a, out);
+  else if (kind == 1) bemfmm::l2l_operator(p, a, out);
+  else bemfmm::m2l_operator(p, a, b, c, out);
+}
+}
