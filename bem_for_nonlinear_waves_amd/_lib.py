@@ -63,6 +63,7 @@ SIGNATURES = [
      [ctypes.c_void_p, _dp, _dp, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.POINTER(Result)]),
     ("bemfmm_get_stats", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(Stats)]),
     ("bemfmm_matvec_profile", ctypes.c_int, [ctypes.c_void_p, _dp]),
+    ("bemfmm_residual_history", ctypes.c_int, [ctypes.c_void_p, _dp, ctypes.c_int]),
 ]
 
 _lib = None

@@ -123,6 +123,13 @@ class BEM_BVP:
         self._check(self.lib.bemfmm_matvec_profile(self.ctx, _ptr(out, _lib._dp)))
         return out
 
+    def residual_history(self) -> np.ndarray:
+        n = self.lib.bemfmm_residual_history(self.ctx, None, 0)
+        out = np.empty(max(n, 0))
+        if n > 0:
+            self.lib.bemfmm_residual_history(self.ctx, _ptr(out, _lib._dp), n)
+        return out
+
     # -- solve --------------------------------------------------------------------
     def solveSystemGMRES(self, x0=None, tol: float = 1e-10, max_iter: int = 1000, restart: int = 1000,
                          raise_on_nonconvergence: bool = False):

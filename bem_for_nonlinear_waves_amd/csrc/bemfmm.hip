@@ -268,6 +268,36 @@ __global__ __launch_bounds__(256) void k_near_quad(
   }
 }
 
+// Diagonal of the BIE operator per DOF, read from the near-field row of its point:
+// Dirichlet (unknown phi_n): G_ii; Neumann (unknown phi): -Gn_ii - alpha_i; CORNER rows: 1.
+// dinv = 1/diag (1 where the diagonal vanishes).
+__global__ void k_diag_inv(int n, const int32_t* __restrict__ kind, const int32_t* __restrict__ point,
+                           const int64_t* __restrict__ node_row, const int32_t* __restrict__ row_block,
+                           const int64_t* __restrict__ row_ptr, const int64_t* __restrict__ col_ptr,
+                           const int64_t* __restrict__ mat_ptr, const int32_t* __restrict__ col_idx,
+                           const int32_t* __restrict__ ncol_dof, const double* __restrict__ mat,
+                           const double* __restrict__ alpha, double* __restrict__ dinv) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double dg = 1.0;
+  if (kind[i] != 2) {
+    const int64_t r = node_row[point[i]];
+    const int b = row_block[r];
+    const int64_t c0 = col_ptr[b];
+    const int ncol = (int)(col_ptr[b + 1] - c0);
+    const int ncd = ncol_dof[b];
+    int lo = 0, hi = ncd - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if (col_idx[c0 + mid] < i) lo = mid + 1;
+      else hi = mid;
+    }
+    const double* A0 = mat + mat_ptr[b] + (r - row_ptr[b]) * 2 * ncol;
+    dg = (kind[i] == 0) ? A0[lo] : -A0[ncol + lo] - alpha[i];
+  }
+  dinv[i] = (isfinite(dg) && fabs(dg) > 1e-15) ? 1.0 / dg : 1.0;
+}
+
 // ---- GMRES vector kernels (FP64) ----
 constexpr int kDotBlocks = 256;
 // partial[i * kDotBlocks + blk] = sum over the block's chunk of V_i . w   (grid: kDotBlocks x k)
@@ -434,6 +464,7 @@ struct bemfmm_ctx {
   hipEvent_t ev[8] = {};
   double prof_ms[7] = {};
   int n_matvec = 0;
+  std::vector<double> history;  // GMRES residual estimate per iteration (last solve)
 };
 
 namespace {
@@ -623,8 +654,16 @@ void do_prepare(bemfmm_ctx* c) {
   // ---- rigid-mode alpha (updateGeometricProperties) ----
   evaluate(c, 2, nullptr, c->alpha.p);
   if (c->opts.preconditioner == BEMFMM_PRECOND_JACOBI) {
-    // diagonal of A: alpha-scaled near self weight for Neumann rows, G self weight for Dirichlet rows
-    throw std::runtime_error("Jacobi preconditioner not available in this build");
+    std::vector<int64_t> node_row(c->n_nodes, -1);
+    for (size_t r = 0; r < pl.near.row_out.size(); ++r) node_row[pl.near.row_out[r]] = (int64_t)r;
+    DBuf<int64_t> d_node_row;
+    d_node_row.upload(node_row, s);
+    hipLaunchKernelGGL(k_diag_inv, dim3(grid1(c->n_dof)), dim3(256), 0, s, c->n_dof, c->kind.p, c->point.p, d_node_row.p,
+                       row_block.p, c->near.row_ptr.p, c->near.col_ptr.p, c->near.mat_ptr.p, c->near.col_idx.p,
+                       ncol_dof.p, c->near.mat.p, c->alpha.p, c->jacobi.p);
+    HIPCHECK(hipGetLastError());
+  } else if (c->opts.preconditioner != BEMFMM_PRECOND_NONE) {
+    throw std::runtime_error("unknown preconditioner");
   }
   HIPCHECK(hipStreamSynchronize(s));
   const double t4 = now();
@@ -691,14 +730,20 @@ int do_solve(bemfmm_ctx* c, const double* x0, double* xout, double tol, int max_
   const int64_t n = c->n_dof;
   hipStream_t s = c->stream;
   const int n_mv0 = c->n_matvec;
-  evaluate(c, 1, nullptr, c->bvec.p);  // b
-  HIPCHECK(hipMemcpyAsync(c->xsol.p, x0, n * sizeof(double), hipMemcpyHostToDevice, s));
-  auto Av = [&](const double* v, double* w) { evaluate(c, 0, v, w); };
   const int gN = grid1(n);
+  const bool jac = c->opts.preconditioner == BEMFMM_PRECOND_JACOBI;
+  evaluate(c, 1, nullptr, c->bvec.p);  // b, then M^-1 b (left preconditioning, GMRES_FMM.hpp:2157-2171)
+  if (jac) hipLaunchKernelGGL(k_mul, dim3(gN), dim3(256), 0, s, n, c->jacobi.p, c->bvec.p);
+  HIPCHECK(hipMemcpyAsync(c->xsol.p, x0, n * sizeof(double), hipMemcpyHostToDevice, s));
+  auto Av = [&](const double* v, double* w) {  // return_A_dot_v: M^-1 A v
+    evaluate(c, 0, v, w);
+    if (jac) hipLaunchKernelGGL(k_mul, dim3(gN), dim3(256), 0, s, n, c->jacobi.p, w);
+  };
   int iters = 0;
   double cur_err = 1e300;
   const int restart_max = std::max(1, restart);
   std::vector<double> h1, h2;
+  c->history.clear();
   while (iters < max_iter && cur_err > tol) {
     const int remaining = max_iter - iters;
     const int k_cycle_max = std::min(restart_max, remaining);
@@ -774,11 +819,13 @@ int do_solve(bemfmm_ctx* c, const double* x0, double* xout, double tol, int max_
     };
     add_basis();  // gmres(A, b, x, 1)
     iters += 1;
+    c->history.push_back(std::abs(g.back()));
     int sat = 0;
     while (true) {
       add_basis();
       ++iters;
       const double est = std::abs(g.back());
+      c->history.push_back(est);
       if (!std::isfinite(est)) throw HipError("GMRES breakdown: non-finite residual estimate", BEMFMM_E_NONFINITE);
       if (est <= tol * 10.) {
         update_solution();
@@ -985,6 +1032,14 @@ int bemfmm_solve(bemfmm_ctx* c, const double* x0, double* x, double tol, int max
     if (!x0 || !x || !(tol > 0) || max_iter < 1) throw std::runtime_error("bemfmm_solve: invalid arguments");
     return do_solve(c, x0, x, tol, max_iter, restart, r);
   });
+}
+
+int bemfmm_residual_history(const bemfmm_ctx* c, double* out, int cap) {
+  if (!c) return BEMFMM_E_INVALID;
+  const int n = (int)c->history.size();
+  if (out)
+    for (int i = 0; i < std::min(n, cap); ++i) out[i] = c->history[i];
+  return n;
 }
 
 int bemfmm_get_stats(const bemfmm_ctx* c, bemfmm_stats* s) {

@@ -162,51 +162,44 @@ def cylinder_tank(Lx=20.0, Ly=4.0, Lz=2.0, R=0.5, draft=1.0, h=0.036, a=None, cx
 
     The cylinder (radius R, axis at (cx, cy)) occupies z in [Lz-draft, Lz] and is
     excluded from the fluid. The free surface z = Lz (Dirichlet) is an O-grid
-    around the cylinder inside the square [-a,a]^2 and a structured grid outside;
-    all other faces are Neumann.
+    inside a square of half-side ~a around the cylinder whose edges lie on the
+    lines of the structured free-surface grid outside it; all other faces are Neumann.
     """
     if a is None:
-        a = max(2.0 * R, h * round(2.0 * R / h))
+        a = 2.0 * R
     nx = int(round(Lx / h)); ny = int(round(Ly / h)); nz = int(round(Lz / h))
     hx, hy = Lx / nx, Ly / ny
-    # snap the O-grid square to the free-surface grid
-    ia = int(round(a / hx)); ja = int(round(a / hy))
-    a_x, a_y = ia * hx, ja * hy
-    b = _Builder()
     x0, y0 = -Lx / 2, -Ly / 2
+    gx = lambda i: x0 + (i / nx) * Lx  # same arithmetic as _plane
+    gy = lambda j: y0 + (j / ny) * Ly
+    i_lo, i_hi = int(round((cx - a - x0) / hx)), int(round((cx + a - x0) / hx))
+    j_lo, j_hi = int(round((cy - a - y0) / hy)), int(round((cy + a - y0) / hy))
+    b = _Builder()
     const = lambda n: (lambda p: np.broadcast_to(np.asarray(n, dtype=float), p.shape))
     b.grid(_plane((x0, y0, 0.0), (Lx, 0, 0), (0, Ly, 0), nx, ny), const((0, 0, -1)), False, 0)
     b.grid(_plane((x0, y0, 0.0), (Lx, 0, 0), (0, 0, Lz), nx, nz), const((0, -1, 0)), False, 2)
     b.grid(_plane((x0, y0 + Ly, 0.0), (Lx, 0, 0), (0, 0, Lz), nx, nz), const((0, 1, 0)), False, 3)
     b.grid(_plane((x0, y0, 0.0), (0, Ly, 0), (0, 0, Lz), ny, nz), const((-1, 0, 0)), False, 4)
     b.grid(_plane((x0 + Lx, y0, 0.0), (0, Ly, 0), (0, 0, Lz), ny, nz), const((1, 0, 0)), False, 5)
-    # free surface minus the O-grid square
+    # free surface minus the O-grid square (cells [i_lo, i_hi) x [j_lo, j_hi))
     P = _plane((x0, y0, Lz), (Lx, 0, 0), (0, Ly, 0), nx, ny)
-    xc = x0 + (np.arange(nx) + 0.5) * hx
-    yc = y0 + (np.arange(ny) + 0.5) * hy
-    inside = (np.abs(xc - cx)[:, None] < a_x) & (np.abs(yc - cy)[None, :] < a_y)
-    b.grid(P, const((0, 0, 1)), True, 1, mask=~inside)
-    # O-grid annulus between the square boundary and the circle (free surface)
-    m_side_x, m_side_y = 2 * ia, 2 * ja
-    ring = []
-    s = np.linspace(-a_x, a_x, m_side_x + 1)[:-1]
-    ring.append(np.stack([s, np.full_like(s, -a_y)], 1))
-    s = np.linspace(-a_y, a_y, m_side_y + 1)[:-1]
-    ring.append(np.stack([np.full_like(s, a_x), s], 1))
-    s = np.linspace(a_x, -a_x, m_side_x + 1)[:-1]
-    ring.append(np.stack([s, np.full_like(s, a_y)], 1))
-    s = np.linspace(a_y, -a_y, m_side_y + 1)[:-1]
-    ring.append(np.stack([np.full_like(s, -a_x), s], 1))
-    sq = np.concatenate(ring, 0)
-    sq = np.concatenate([sq, sq[:1]], 0)  # closed
-    ang = np.arctan2(sq[:, 1], sq[:, 0])
+    mask = np.ones((nx, ny), bool)
+    mask[i_lo:i_hi, j_lo:j_hi] = False
+    b.grid(P, const((0, 0, 1)), True, 1, mask=mask)
+    # square boundary through the grid nodes, counter-clockwise from (i_lo, j_lo)
+    ring = [(gx(i), gy(j_lo)) for i in range(i_lo, i_hi)] + [(gx(i_hi), gy(j)) for j in range(j_lo, j_hi)] + \
+           [(gx(i), gy(j_hi)) for i in range(i_hi, i_lo, -1)] + [(gx(i_lo), gy(j)) for j in range(j_hi, j_lo, -1)]
+    sq = np.asarray(ring + ring[:1], dtype=float)
+    rel = sq - np.array([cx, cy])[None, :]
+    ang = np.arctan2(rel[:, 1], rel[:, 0])
     circ = np.stack([R * np.cos(ang), R * np.sin(ang)], 1)
     circ[-1] = circ[0]
-    nr = max(2, int(round((min(a_x, a_y) - R) / h)))
+    half = min(gx(i_hi) - cx, cx - gx(i_lo), gy(j_hi) - cy, cy - gy(j_lo))
+    nr = max(2, int(round((half - R) / h)))
     t = np.linspace(0.0, 1.0, nr + 1)
     G = np.zeros((sq.shape[0], nr + 1, 3))
-    G[:, :, 0] = cx + circ[:, None, 0] * (1 - t[None, :]) + sq[:, None, 0] * t[None, :]
-    G[:, :, 1] = cy + circ[:, None, 1] * (1 - t[None, :]) + sq[:, None, 1] * t[None, :]
+    G[:, :, 0] = (cx + circ[:, None, 0]) * (1 - t[None, :]) + sq[:, None, 0] * t[None, :]
+    G[:, :, 1] = (cy + circ[:, None, 1]) * (1 - t[None, :]) + sq[:, None, 1] * t[None, :]
     G[:, :, 2] = Lz
     b.grid(G, const((0, 0, 1)), True, 1)
     # cylinder side: out of the fluid = towards the axis
@@ -217,17 +210,18 @@ def cylinder_tank(Lx=20.0, Ly=4.0, Lz=2.0, R=0.5, draft=1.0, h=0.036, a=None, cx
     S[:, :, 1] = cy + circ[:, None, 1]
     S[:, :, 2] = zc[None, :]
     b.grid(S, lambda p: -np.stack([p[:, 0] - cx, p[:, 1] - cy, np.zeros(len(p))], 1), False, 6)
-    # cylinder bottom disk (z = Lz - draft): out of the fluid = +z; inner square + annulus
-    bb = 0.5 * R
-    sqb = sq * np.array([bb / a_x, bb / a_y])[None, :]
-    nrb = max(2, int(round((R - bb) / h)))
+    # cylinder bottom disk (z = Lz - draft): out of the fluid = +z; scaled inner square + annulus
+    sc = 0.5 * R / half
+    sqb = np.array([cx, cy])[None, :] + sc * rel
+    nrb = max(2, int(round(0.5 * R / h)))
     t = np.linspace(0.0, 1.0, nrb + 1)
     D = np.zeros((sq.shape[0], nrb + 1, 3))
-    D[:, :, 0] = cx + circ[:, None, 0] * (1 - t[None, :]) + sqb[:, None, 0] * t[None, :]
-    D[:, :, 1] = cy + circ[:, None, 1] * (1 - t[None, :]) + sqb[:, None, 1] * t[None, :]
+    D[:, :, 0] = (cx + circ[:, None, 0]) * (1 - t[None, :]) + sqb[:, None, 0] * t[None, :]
+    D[:, :, 1] = (cy + circ[:, None, 1]) * (1 - t[None, :]) + sqb[:, None, 1] * t[None, :]
     D[:, :, 2] = Lz - draft
     b.grid(D, const((0, 0, 1)), False, 7)
-    Q = _plane((cx - bb, cy - bb, Lz - draft), (2 * bb, 0, 0), (0, 2 * bb, 0), m_side_x, m_side_y)
+    Q = _plane((sqb[0, 0], sqb[0, 1], Lz - draft), (sqb[i_hi - i_lo, 0] - sqb[0, 0], 0, 0),
+               (0, sqb[(i_hi - i_lo) + (j_hi - j_lo), 1] - sqb[0, 1], 0), i_hi - i_lo, j_hi - j_lo)
     b.grid(Q, const((0, 0, 1)), False, 7)
     return b.mesh()
 

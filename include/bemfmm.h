@@ -127,6 +127,10 @@ int bemfmm_solve(bemfmm_ctx* ctx, const double* x0, double* x, double tol, int m
 
 int bemfmm_get_stats(const bemfmm_ctx* ctx, bemfmm_stats* stats);
 
+/* GMRES residual estimate |g_n| per iteration of the last solve (the reference prints it per
+ * inner iteration, BEM_solveBVP_GMRES_FMM.hpp:2297-2300). Copies up to cap values; returns the count. */
+int bemfmm_residual_history(const bemfmm_ctx* ctx, double* out, int cap);
+
 /* Kernel timing of the last matvec_device call (milliseconds, HIP events on the context's stream):
  * [0] scatter, [1] P2M, [2] M2M, [3] M2L, [4] L2L, [5] near+L2P, [6] assemble. Enables event
  * recording on the first call. */

@@ -944,12 +944,30 @@ struct Oracle {
 
   // ---- GMRES: run_gmres control flow (GMRES_FMM.hpp:2275-2355) with the gmres
   // class (basic_linear_systems.hpp:3137-3451): MGS Arnoldi, Givens QR.
+  // Diagonal of the operator (Dirichlet: G_ii, Neumann: -Gn_ii - alpha_i, CORNER: 1) for left Jacobi scaling.
+  std::vector<double> diag_inv() const {
+    std::vector<double> d(n_dof, 1.0);
+    for (int i = 0; i < n_dof; ++i) {
+      if (dof_kind[i] == 2) continue;
+      const int t = dof_point[i];
+      const auto& ix = near_idx[t];
+      const size_t k = std::lower_bound(ix.begin(), ix.end(), i) - ix.begin();
+      double g = 1.0;
+      if (k < ix.size() && ix[k] == i) g = dof_kind[i] == 0 ? near_wG[t][k] : -near_wGn[t][k] - alpha[i];
+      d[i] = (std::isfinite(g) && std::abs(g) > 1e-15) ? 1.0 / g : 1.0;
+    }
+    return d;
+  }
+  int precond = 0;
   int solve(const double* x0, double* xout, double tol, int max_iter, int restart, int* iters_out, double* res_out) {
     std::vector<double> b(n_dof);
     rhs(b.data());
+    const std::vector<double> dinv = precond == 1 ? diag_inv() : std::vector<double>(n_dof, 1.0);
+    for (int i = 0; i < n_dof; ++i) b[i] *= dinv[i];
     auto Av = [&](const std::vector<double>& v) {
       std::vector<double> y(n_dof);
       matvec(v.data(), y.data());
+      for (int i = 0; i < n_dof; ++i) y[i] *= dinv[i];
       return y;
     };
     auto nrm = [](const std::vector<double>& v) {
@@ -1107,7 +1125,9 @@ void oracle_alpha(void* h, double* a) {
   std::copy(o->alpha.begin(), o->alpha.end(), a);
 }
 void oracle_dense(void* h, double* A) { static_cast<Oracle*>(h)->dense(A); }
-int oracle_solve(void* h, const double* x0, double* x, double tol, int max_iter, int restart, int* iters, double* res) {
+int oracle_solve(void* h, const double* x0, double* x, double tol, int max_iter, int restart, int* iters, double* res,
+                 int precond) {
+  static_cast<Oracle*>(h)->precond = precond;
   return static_cast<Oracle*>(h)->solve(x0, x, tol, max_iter, restart, iters, res);
 }
 void oracle_get_stats(void* h, oracle_stats* s) {

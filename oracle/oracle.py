@@ -48,7 +48,8 @@ def lib():
         L.oracle_alpha.argtypes = [ctypes.c_void_p, _dp]
         L.oracle_dense.argtypes = [ctypes.c_void_p, _dp]
         L.oracle_solve.restype = ctypes.c_int
-        L.oracle_solve.argtypes = [ctypes.c_void_p, _dp, _dp, ctypes.c_double, ctypes.c_int, ctypes.c_int, _ip, _dp]
+        L.oracle_solve.argtypes = [ctypes.c_void_p, _dp, _dp, ctypes.c_double, ctypes.c_int, ctypes.c_int, _ip, _dp,
+                                   ctypes.c_int]
         L.oracle_get_stats.argtypes = [ctypes.c_void_p, ctypes.POINTER(OracleStats)]
         L.oracle_near_row.restype = ctypes.c_int
         L.oracle_near_row.argtypes = [ctypes.c_void_p, ctypes.c_int, _ip, _dp, _dp, ctypes.c_int]
@@ -112,13 +113,14 @@ class Oracle:
         lib().oracle_dense(self.h, _p(A, _dp))
         return A
 
-    def solve(self, x0=None, tol=1e-10, max_iter=1000, restart=1000):
+    def solve(self, x0=None, tol=1e-10, max_iter=1000, restart=1000, precond=0):
+        """GMRES as the reference's run_gmres; precond=1: left scaling by the operator diagonal."""
         x0 = np.zeros(self.n_dof) if x0 is None else _d(x0)
         x = np.empty(self.n_dof)
         it = ctypes.c_int(0)
         res = ctypes.c_double(0)
         rc = lib().oracle_solve(self.h, _p(x0, _dp), _p(x, _dp), tol, max_iter, restart, ctypes.byref(it),
-                                ctypes.byref(res))
+                                ctypes.byref(res), precond)
         return x, {"rc": rc, "iters": it.value, "residual": res.value}
 
     def stats(self):

@@ -220,3 +220,33 @@ void emu_operator(int p, int kind, int a, int b, int c, double* out) {
   else bemfmm::m2l_operator(p, a, b, c, out);
 }
 }
+
+extern "C" {
+// Builds only the plan (no near quadrature): out = {n_levels, n_boxes, n_leaves, n_m2l_pairs, near_nnz,
+// n_m2l_ops, near_mat_doubles, p2m_mat_doubles, n_near_blocks}; times = {tree, lists, near_struct, p2m, ops}
+void emu_plan_only(int n_nodes, const double* X, int n_faces, const int32_t* tri, const int32_t* face_dof, int n_dof,
+                   const int32_t* kind, const int32_t* point, int p, double theta, int leaf_max, int max_level,
+                   int64_t* out, double* times) {
+  bemfmm::Plan pl;
+  bemfmm::Options o;
+  o.p = p;
+  o.theta = theta;
+  o.leaf_max = leaf_max;
+  o.max_level = max_level;
+  bemfmm::build_plan(pl, o, n_nodes, X, n_faces, tri, face_dof, n_dof, kind, point);
+  out[0] = pl.n_levels;
+  out[1] = pl.n_boxes;
+  out[2] = pl.n_leaves;
+  out[3] = pl.n_m2l_pairs;
+  out[4] = pl.near_nnz;
+  out[5] = (int64_t)pl.m2l.op.size();
+  out[6] = pl.near.mat_size;
+  out[7] = pl.p2m.mat_size;
+  out[8] = pl.near.n_blocks();
+  times[0] = pl.t_tree;
+  times[1] = pl.t_lists;
+  times[2] = pl.t_near_struct;
+  times[3] = pl.t_p2m;
+  times[4] = pl.t_ops;
+}
+}
