@@ -94,10 +94,14 @@ __global__ void k_scatter(int mode, int n, const int32_t* __restrict__ kind, con
   phi[i] = b;
 }
 
-// Box-blocked dense GEMV (see bemfmm::BlockGemv). One workgroup per block,
-// gathered inputs staged through LDS in chunks, one wavefront per row.
+// Box-blocked dense GEMV (see bemfmm::BlockGemv). One workgroup per block; the gathered
+// inputs are staged through LDS in chunks; each wavefront streams kRPW rows at once and
+// issues all 2*kRPW*2 loads of two column steps before using them (rows past the end of
+// the block are clamped to its last row and their sums discarded), so that enough HBM
+// requests are in flight per lane.
 constexpr int kGemvChunk = 2048;
 constexpr int kGemvMaxRows = 512;
+constexpr int kRPW = 4;
 template <typename T>
 __global__ __launch_bounds__(256) void k_block_gemv(const int64_t* __restrict__ row_ptr, const int64_t* __restrict__ col_ptr,
                                                     const int64_t* __restrict__ mat_ptr, const int32_t* __restrict__ row_out,
@@ -117,27 +121,75 @@ __global__ __launch_bounds__(256) void k_block_gemv(const int64_t* __restrict__ 
   for (int cs = 0; cs < ncol; cs += kGemvChunk) {
     const int cn = min(kGemvChunk, ncol - cs);
     __syncthreads();
-    for (int i = threadIdx.x; i < cn; i += blockDim.x) {
-      const int g = col_idx[c0 + cs + i];
+    const int32_t* ci = col_idx + c0 + cs;
+    int i = threadIdx.x;
+    for (; i + 3 * 256 < cn; i += 4 * 256) {
+      const int g0 = ci[i], g1 = ci[i + 256], g2 = ci[i + 512], g3 = ci[i + 768];
+      const double a0 = x0[g0], a1 = x0[g1], a2 = x0[g2], a3 = x0[g3];
+      const double b0 = x1[g0], b1 = x1[g1], b2 = x1[g2], b3 = x1[g3];
+      xs0[i] = a0;
+      xs0[i + 256] = a1;
+      xs0[i + 512] = a2;
+      xs0[i + 768] = a3;
+      xs1[i] = b0;
+      xs1[i + 256] = b1;
+      xs1[i + 512] = b2;
+      xs1[i + 768] = b3;
+    }
+    for (; i < cn; i += 256) {
+      const int g = ci[i];
       xs0[i] = x0[g];
       xs1[i] = x1[g];
     }
     __syncthreads();
-    for (int r = wave; r < nrow; r += 4) {
-      const T* A0 = M + (int64_t)r * 2 * ncol + cs;
-      const T* A1 = A0 + ncol;
-      double s0 = 0., s1 = 0.;
-      for (int c = lane; c < cn; c += 64) {
-        s0 += (double)A0[c] * xs0[c];
-        s1 += (double)A1[c] * xs1[c];
+    for (int rg = wave * kRPW; rg < nrow; rg += 4 * kRPW) {
+      const T* Ar[kRPW];
+#pragma unroll
+      for (int k = 0; k < kRPW; ++k) Ar[k] = M + (int64_t)min(rg + k, nrow - 1) * 2 * ncol + cs;
+      double s0[kRPW], s1[kRPW];
+#pragma unroll
+      for (int k = 0; k < kRPW; ++k) s0[k] = s1[k] = 0.;
+      int c = lane;
+      for (; c + 64 < cn; c += 128) {
+        double va[kRPW][2], vb[kRPW][2];
+#pragma unroll
+        for (int k = 0; k < kRPW; ++k) {
+          va[k][0] = (double)Ar[k][c];
+          va[k][1] = (double)Ar[k][c + 64];
+          vb[k][0] = (double)Ar[k][ncol + c];
+          vb[k][1] = (double)Ar[k][ncol + c + 64];
+        }
+        const double xa0 = xs0[c], xa1 = xs0[c + 64], xb0 = xs1[c], xb1 = xs1[c + 64];
+#pragma unroll
+        for (int k = 0; k < kRPW; ++k) {
+          s0[k] += va[k][0] * xa0 + va[k][1] * xa1;
+          s1[k] += vb[k][0] * xb0 + vb[k][1] * xb1;
+        }
       }
-      for (int off = 32; off > 0; off >>= 1) {
-        s0 += __shfl_down(s0, off);
-        s1 += __shfl_down(s1, off);
+      if (c < cn) {
+        double va[kRPW], vb[kRPW];
+#pragma unroll
+        for (int k = 0; k < kRPW; ++k) {
+          va[k] = (double)Ar[k][c];
+          vb[k] = (double)Ar[k][ncol + c];
+        }
+#pragma unroll
+        for (int k = 0; k < kRPW; ++k) {
+          s0[k] += va[k] * xs0[c];
+          s1[k] += vb[k] * xs1[c];
+        }
       }
-      if (lane == 0) {
-        acc[2 * r] += s0;
-        acc[2 * r + 1] += s1;
+#pragma unroll
+      for (int k = 0; k < kRPW; ++k) {
+        double u = s0[k], v = s1[k];
+        for (int off = 32; off > 0; off >>= 1) {
+          u += __shfl_down(u, off);
+          v += __shfl_down(v, off);
+        }
+        if (lane == 0 && rg + k < nrow) {
+          acc[2 * (rg + k)] += u;
+          acc[2 * (rg + k) + 1] += v;
+        }
       }
     }
   }
@@ -149,48 +201,162 @@ __global__ __launch_bounds__(256) void k_block_gemv(const int64_t* __restrict__ 
   }
 }
 
-// Operator-grouped translation: for every pair (dst, src) of a tile (<= 16 pairs, one operator),
-// D[dst] += T_op * S[src] for both layers. T_op is C x C stored [k_in][r_out].
-constexpr int kTP = 16;
+// Operator-grouped translation on the matrix cores. For every pair (dst, src) of a
+// tile (<= kTP pairs sharing one operator): D[dst] += T_op * S[src] for both layers.
+// The tile is the GEMM  out[RP x 2kTP] += T[RP x KP] * B[KP x 2kTP]  built from
+// v_mfma_f64_16x16x4 blocks (RP = 64*RTW rows: C padded, KP = C padded to 16, zeros in
+// the padding): wave w owns row tiles [w*RTW, (w+1)*RTW) and all kTCT 16-column tiles.
+// B (the gathered source coefficients) is staged in LDS; T is streamed from the padded
+// operator array [op][KP][RP] through a two-stage register pipeline of kKG k-steps, so
+// that the loads of the next stage are in flight while the current one runs.
+// Workgroups are remapped so that each XCD processes one contiguous run of tiles
+// (tiles are sorted by operator): the tiles of an operator share that XCD's L2.
+constexpr int kTP = 32;
 constexpr int kTCols = 2 * kTP;
 constexpr int kTStride = kTCols + 1;
-__global__ __launch_bounds__(256) void k_translate(int C, const double* __restrict__ ops, const int32_t* __restrict__ tile_op,
-                                                   const int64_t* __restrict__ tile_p0, const int64_t* __restrict__ tile_p1,
-                                                   const int32_t* __restrict__ dst, const int32_t* __restrict__ src,
-                                                   const double* __restrict__ S0, const double* __restrict__ S1,
-                                                   double* __restrict__ D0, double* __restrict__ D1) {
-  extern __shared__ double Bs[];  // C * kTStride
-  const int t = blockIdx.x;
+constexpr int kTCT = kTCols / 16;  // 16-column MFMA tiles per workgroup tile
+constexpr int kXCD = 8;
+constexpr int kKG = 4;  // k-steps (of 4) per pipeline stage
+typedef double double4_t __attribute__((ext_vector_type(4)));
+
+template <int RTW, bool STAGE>
+__global__ __launch_bounds__(256) void k_translate_mfma(int C, int RP, int KP, int ntiles, const double* __restrict__ ops,
+                                                        const int32_t* __restrict__ tile_op, const int64_t* __restrict__ tile_p0,
+                                                        const int64_t* __restrict__ tile_p1, const int32_t* __restrict__ dst,
+                                                        const int32_t* __restrict__ src, const double* __restrict__ S0,
+                                                        const double* __restrict__ S1, double* __restrict__ D0,
+                                                        double* __restrict__ D1) {
+  extern __shared__ double Bs[];  // KP * kTStride
+  const int per_xcd = (ntiles + kXCD - 1) / kXCD;
+  const int t = (blockIdx.x % kXCD) * per_xcd + blockIdx.x / kXCD;
+  if (t >= ntiles) return;
   const int op = tile_op[t];
   const int64_t p0 = tile_p0[t];
   const int np = (int)(tile_p1[t] - p0);
-  for (int i = threadIdx.x; i < kTCols * C; i += blockDim.x) {
-    const int j = i / C, k = i - j * C;  // k fastest: coalesced global reads
-    const int pair = j >> 1, layer = j & 1;
-    double v = 0.;
-    if (pair < np) v = (layer ? S1 : S0)[(int64_t)src[p0 + pair] * C + k];
-    Bs[k * kTStride + j] = v;
+  {
+    // gather the tile's source coefficients: thread -> (layer, k); all kTP loads issued together
+    const int layer = threadIdx.x >> 7;
+    const double* S = layer ? S1 : S0;
+    int sidx[kTP];
+#pragma unroll
+    for (int u = 0; u < kTP; ++u) sidx[u] = (u < np) ? src[p0 + u] : -1;
+    for (int k = threadIdx.x & 127; k < KP; k += 128) {
+      double v[kTP];
+#pragma unroll
+      for (int u = 0; u < kTP; ++u) v[u] = (sidx[u] >= 0 && k < C) ? S[(int64_t)sidx[u] * C + k] : 0.;
+#pragma unroll
+      for (int u = 0; u < kTP; ++u) Bs[k * kTStride + 2 * u + layer] = v[u];
+    }
   }
   __syncthreads();
-  const int r = threadIdx.x & 127;
-  const int cg = threadIdx.x >> 7;
-  if (r >= C) return;
-  const double* O = ops + (int64_t)op * C * C;
-  double acc[kTP];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int li = lane & 15, lk = lane >> 4;
+  const double* O = ops + (int64_t)op * KP * RP + (int64_t)lk * RP + wave * RTW * 16 + li;
+  const double* Bl = Bs + lk * kTStride + li;
+  double4_t acc[RTW][kTCT];
 #pragma unroll
-  for (int j = 0; j < kTP; ++j) acc[j] = 0.;
-  for (int k = 0; k < C; ++k) {
-    const double a = O[(int64_t)k * C + r];
-    const double* bk = Bs + k * kTStride + cg * kTP;
+  for (int a = 0; a < RTW; ++a)
 #pragma unroll
-    for (int j = 0; j < kTP; ++j) acc[j] += a * bk[j];
+    for (int ct = 0; ct < kTCT; ++ct) acc[a][ct] = double4_t{0., 0., 0., 0.};
+  double av[2][kKG][RTW];
+  double bv[2][kKG][kTCT];
+  auto load = [&](int stage, int k0) {
+#pragma unroll
+    for (int s = 0; s < kKG; ++s) {
+      const double* ok = O + (int64_t)(k0 + 4 * s) * RP;
+#pragma unroll
+      for (int a = 0; a < RTW; ++a) av[stage][s][a] = ok[a * 16];
+      const double* bk = Bl + (k0 + 4 * s) * kTStride;
+#pragma unroll
+      for (int ct = 0; ct < kTCT; ++ct) bv[stage][s][ct] = bk[16 * ct];
+    }
+  };
+  auto run = [&](int stage) {
+#pragma unroll
+    for (int s = 0; s < kKG; ++s)
+#pragma unroll
+      for (int a = 0; a < RTW; ++a)
+#pragma unroll
+        for (int ct = 0; ct < kTCT; ++ct)
+          acc[a][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[stage][s][a], bv[stage][s][ct], acc[a][ct], 0, 0, 0);
+  };
+  const int ng = KP / (4 * kKG);
+  load(0, 0);
+  for (int g = 0; g < ng; g += 2) {
+    if (g + 1 < ng) load(1, (g + 1) * 4 * kKG);
+    run(0);
+    if (g + 1 < ng) {
+      if (g + 2 < ng) load(0, (g + 2) * 4 * kKG);
+      run(1);
+    }
+  }
+  // D block layout of v_mfma_f64_16x16x4 (measured on gfx950): register i of a lane holds
+  // row lane/16 + 4*i of column lane%16
+  if (STAGE) {
+    // per-pair results -> staging rows D0[(p0+pair)*2C + layer*C + row] with coalesced stores:
+    // transpose the tile through LDS (column-major, stride RP+1) and let each wave write whole columns
+    __syncthreads();
+    double* T = Bs;  // reuse the staging area (sized for max(KP * kTStride, kTCols * (RP + 1)) doubles)
+#pragma unroll
+    for (int a = 0; a < RTW; ++a) {
+      const int rt = wave * RTW + a;
+#pragma unroll
+      for (int ct = 0; ct < kTCT; ++ct)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) T[(ct * 16 + li) * (RP + 1) + rt * 16 + lk + 4 * i] = acc[a][ct][i];
+    }
+    __syncthreads();
+    for (int col = wave; col < 2 * np; col += 4) {
+      const int pair = col >> 1, layer = col & 1;
+      double* out = D0 + ((p0 + pair) * 2 + layer) * (int64_t)C;
+      for (int r = lane; r < C; r += 64) out[r] = T[col * (RP + 1) + r];
+    }
+    return;
   }
 #pragma unroll
-  for (int j = 0; j < kTP; ++j) {
-    const int col = cg * kTP + j;
-    const int pair = col >> 1, layer = col & 1;
-    if (pair < np) atomicAdd(&(layer ? D1 : D0)[(int64_t)dst[p0 + pair] * C + r], acc[j]);
+  for (int a = 0; a < RTW; ++a) {
+    const int rt = wave * RTW + a;
+#pragma unroll
+    for (int ct = 0; ct < kTCT; ++ct) {
+      const int col = ct * 16 + li;
+      const int pair = col >> 1, layer = col & 1;
+      if (pair >= np) continue;
+      double* D = (layer ? D1 : D0) + (int64_t)dst[p0 + pair] * C;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int row = rt * 16 + lk + 4 * i;
+        if (row < C) atomicAdd(&D[row], acc[a][ct][i]);
+      }
+    }
   }
+}
+
+// M2L reduction: L[t] = sum over the M2L pairs of box t (fixed order) of the staged per-pair
+// results. One workgroup per box; thread j < 2C owns (layer j/C, row j%C). Writes every box,
+// so L needs no clearing before it.
+__global__ __launch_bounds__(256) void k_m2l_reduce(int C, const int64_t* __restrict__ tptr, const int32_t* __restrict__ tpairs,
+                                                    const double* __restrict__ stage, double* __restrict__ L0,
+                                                    double* __restrict__ L1) {
+  const int t = blockIdx.x;
+  const int j = threadIdx.x;
+  if (j >= 2 * C) return;
+  const int64_t a = tptr[t], b = tptr[t + 1];
+  double s0 = 0., s1 = 0., s2 = 0., s3 = 0.;
+  int64_t q = a;
+  for (; q + 3 < b; q += 4) {
+    const double v0 = stage[(int64_t)tpairs[q] * 2 * C + j];
+    const double v1 = stage[(int64_t)tpairs[q + 1] * 2 * C + j];
+    const double v2 = stage[(int64_t)tpairs[q + 2] * 2 * C + j];
+    const double v3 = stage[(int64_t)tpairs[q + 3] * 2 * C + j];
+    s0 += v0;
+    s1 += v1;
+    s2 += v2;
+    s3 += v3;
+  }
+  for (; q < b; ++q) s0 += stage[(int64_t)tpairs[q] * 2 * C + j];
+  const double v = (s0 + s1) + (s2 + s3);
+  if (j < C) L0[(int64_t)t * C + j] = v;
+  else L1[(int64_t)t * C + j - C] = v;
 }
 
 // Av_i = GPhin - (GnPhi + alpha_i phi_i) (CORNER Neumann rows: Av_i = phi_i). mode 1: b = -Av (CORNER: phi_D).
@@ -379,9 +545,13 @@ struct DBuf {
     HIPCHECK(hipMalloc((void**)&p, count * sizeof(T)));
     n = count;
   }
+  // Setup-time upload; completes before returning (the source may be a temporary).
   void upload(const std::vector<T>& v, hipStream_t s) {
     alloc(v.size());
-    if (!v.empty()) HIPCHECK(hipMemcpyAsync(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s));
+    if (!v.empty()) {
+      HIPCHECK(hipMemcpyAsync(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s));
+      HIPCHECK(hipStreamSynchronize(s));
+    }
   }
   size_t bytes() const { return n * sizeof(T); }
 };
@@ -444,10 +614,13 @@ struct bemfmm_ctx {
   bool have_geom = false, have_bc = false, prepared = false;
   // plan and device state
   std::unique_ptr<Plan> plan;
-  int C = 0, n_boxes = 0;
+  int C = 0, n_boxes = 0, RP = 0, KP = 0;
   DGemv near, p2m;
   std::vector<DTiles> m2m, l2l;
   DTiles m2l;
+  DBuf<int64_t> m2l_tptr;    // per box: its M2L pairs in m2l_tpairs
+  DBuf<int32_t> m2l_tpairs;
+  DBuf<double> m2l_stage;    // per-pair M2L results [pair][layer][C]
   DBuf<double> ops;
   DBuf<int32_t> kind, point;
   DBuf<double> alpha, phi_k, phin_k;
@@ -496,11 +669,29 @@ void launch_gemv(bemfmm_ctx* c, DGemv& g, const double* x0, const double* x1, do
                      g.row_out.p, g.col_idx.p, g.mat.p, x0, x1, y0, y1);
 }
 
-void launch_translate(bemfmm_ctx* c, DTiles& t, const double* S0, const double* S1, double* D0, double* D1) {
+void launch_translate(bemfmm_ctx* c, DTiles& t, const double* S0, const double* S1, double* D0, double* D1,
+                      bool stage = false) {
   if (t.ntiles == 0) return;
-  const size_t lds = (size_t)c->C * kTStride * sizeof(double);
-  hipLaunchKernelGGL(k_translate, dim3(t.ntiles), dim3(256), lds, c->stream, c->C, c->ops.p, t.op.p, t.p0.p, t.p1.p,
-                     t.dst.p, t.src.p, S0, S1, D0, D1);
+  const size_t lds = (size_t)std::max(c->KP * kTStride, kTCols * (c->RP + 1)) * sizeof(double);
+  const int grid = ((t.ntiles + kXCD - 1) / kXCD) * kXCD;
+  const int RTW = c->RP / 64;
+#define BEMFMM_TR(N)                                                                                               \
+  if (stage)                                                                                                         \
+    hipLaunchKernelGGL((k_translate_mfma<N, true>), dim3(grid), dim3(256), lds, c->stream, c->C, c->RP, c->KP, t.ntiles, \
+                       c->ops.p, t.op.p, t.p0.p, t.p1.p, t.dst.p, t.src.p, S0, S1, D0, D1);                          \
+  else                                                                                                               \
+    hipLaunchKernelGGL((k_translate_mfma<N, false>), dim3(grid), dim3(256), lds, c->stream, c->C, c->RP, c->KP,        \
+                       t.ntiles, c->ops.p, t.op.p, t.p0.p, t.p1.p, t.dst.p, t.src.p, S0, S1, D0, D1)
+  switch (RTW) {
+  case 1: BEMFMM_TR(1); break;
+  case 2: BEMFMM_TR(2); break;
+  case 3: BEMFMM_TR(3); break;
+  case 4: BEMFMM_TR(4); break;
+  case 5: BEMFMM_TR(5); break;
+  case 6: BEMFMM_TR(6); break;
+  default: BEMFMM_TR(7); break;
+  }
+#undef BEMFMM_TR
 }
 
 void fill(bemfmm_ctx* c, double* p, int64_t n, double v) {
@@ -524,14 +715,14 @@ void evaluate(bemfmm_ctx* c, int mode, const double* in, double* out) {
                      c->kind.p, in, c->phi_k.p, c->phin_k.p, phin, phi);
   fill(c, c->M0.p, nbC, 0.);
   fill(c, c->M1.p, nbC, 0.);
-  fill(c, L0, nbC, 0.);
-  fill(c, L1, nbC, 0.);
   rec(1);
   launch_gemv(c, c->p2m, phin, phi, c->M0.p, c->M1.p);
   rec(2);
   for (int l = pl.n_levels - 1; l >= 1; --l) launch_translate(c, c->m2m[l], c->M0.p, c->M1.p, c->M0.p, c->M1.p);
   rec(3);
-  launch_translate(c, c->m2l, c->M0.p, c->M1.p, L0, L1);
+  launch_translate(c, c->m2l, c->M0.p, c->M1.p, c->m2l_stage.p, nullptr, true);
+  hipLaunchKernelGGL(k_m2l_reduce, dim3(c->n_boxes), dim3(256), 0, c->stream, c->C, c->m2l_tptr.p, c->m2l_tpairs.p,
+                     c->m2l_stage.p, L0, L1);
   rec(4);
   for (int l = 1; l < pl.n_levels; ++l) launch_translate(c, c->l2l[l], L0, L1, L0, L1);
   rec(5);
@@ -561,6 +752,8 @@ void do_prepare(bemfmm_ctx* c) {
   c->C = pl.C;
   c->n_boxes = pl.n_boxes;
   if (pl.C > kGemvMaxRows) throw std::runtime_error("expansion order too large for the P2M block GEMV");
+  if ((pl.C + 63) / 64 > 7) throw std::runtime_error("expansion order too large for the translation kernel");
+  if (2 * pl.C > 256) throw std::runtime_error("expansion order too large for the M2L reduction");
   hipStream_t s = c->stream;
   const double t1 = now();
   // ---- upload ----
@@ -577,7 +770,27 @@ void do_prepare(bemfmm_ctx* c) {
     c->l2l[l].upload(pl.l2l_by_level[l], s);
   }
   c->m2l.upload(pl.m2l, s);
-  c->ops.upload(pl.ops, s);
+  {
+    std::vector<int64_t> tptr(pl.n_boxes + 1, 0);
+    for (int32_t d : pl.m2l.dst) tptr[d + 1]++;
+    for (int b = 0; b < pl.n_boxes; ++b) tptr[b + 1] += tptr[b];
+    std::vector<int32_t> tp(pl.m2l.dst.size());
+    std::vector<int64_t> fillp(tptr.begin(), tptr.end() - 1);
+    for (int64_t q = 0; q < (int64_t)pl.m2l.dst.size(); ++q) tp[fillp[pl.m2l.dst[q]]++] = (int32_t)q;
+    c->m2l_tptr.upload(tptr, s);
+    c->m2l_tpairs.upload(tp, s);
+    c->m2l_stage.alloc(std::max<size_t>(1, pl.m2l.dst.size() * 2 * (size_t)pl.C));
+  }
+  {
+    // operators padded to [op][KP][RP] (KP = C rounded up to 16, RP = C rounded up to 64), zeros in the padding
+    c->RP = (pl.C + 63) / 64 * 64;
+    c->KP = (pl.C + 4 * kKG - 1) / (4 * kKG) * (4 * kKG);
+    std::vector<double> opsP((size_t)pl.n_ops * c->KP * c->RP, 0.);
+    for (int o2 = 0; o2 < pl.n_ops; ++o2)
+      for (int k = 0; k < pl.C; ++k)
+        std::memcpy(&opsP[((size_t)o2 * c->KP + k) * c->RP], &pl.ops[((size_t)o2 * pl.C + k) * pl.C], pl.C * sizeof(double));
+    c->ops.upload(opsP, s);
+  }
   c->kind.upload(pl.dof_kind, s);
   c->point.upload(pl.dof_point, s);
   std::vector<double> phik(c->phi_bc), phink(c->n_dof);
@@ -681,7 +894,7 @@ void do_prepare(bemfmm_ctx* c) {
   st.n_m2l_ops = (int)pl.m2l.op.size();
   st.L = pl.Lscale;
   st.device_bytes = (double)(c->near.bytes() + c->p2m.bytes() + c->m2l.bytes() + c->ops.bytes() + c->buf0.bytes() +
-                             c->buf1.bytes() + c->M0.bytes() + c->M1.bytes());
+                             c->buf1.bytes() + c->M0.bytes() + c->M1.bytes() + c->m2l_stage.bytes());
   st.t_tree = pl.t_tree;
   st.t_lists = pl.t_lists;
   st.t_near_struct = pl.t_near_struct;

@@ -28,7 +28,8 @@ import time
 import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
-sys.path.insert(0, ROOT)
+if ROOT not in sys.path:
+    sys.path.append(ROOT)
 
 METRIC = "BVP solves/s & FMM matvec Mnode/s at N=200k; p50/p99 solve latency"
 HBM_PEAK_GBS = 8000.0
@@ -91,6 +92,8 @@ def main():
     ap.add_argument("--matvecs", type=int, default=20)
     ap.add_argument("--precond", choices=["none", "jacobi"], default="jacobi")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--profile-only", action="store_true",
+                    help="setup + a 256 MiB device copy (counter calibration) + the timed matvecs; no solves")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -144,6 +147,16 @@ def main():
     names = ["scatter", "p2m", "m2m", "m2l", "l2l", "near_l2p", "assemble"]
     log("matvec %.3f ms (%.2f Mnode/s); stages ms: %s" % (t_mv * 1e3, m.n_nodes / t_mv / 1e6,
         " ".join(f"{k}={v:.3f}" for k, v in zip(names, stage))))
+
+    if args.profile_only:
+        cal = torch.empty(32 * 1024 * 1024, dtype=torch.float64, device="cuda")
+        cal2 = torch.empty_like(cal)
+        cal.fill_(1.0)
+        torch.cuda.synchronize()
+        cal2.copy_(cal)  # 256 MiB read + 256 MiB written: FETCH_SIZE/WRITE_SIZE calibration
+        torch.cuda.synchronize()
+        log("profile-only: done")
+        return
 
     # ---- solves ----
     lat = []
