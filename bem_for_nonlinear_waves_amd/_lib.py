@@ -23,7 +23,7 @@ PRECOND_NONE, PRECOND_JACOBI = 0, 1
 class Opts(ctypes.Structure):
     _fields_ = [("p", ctypes.c_int), ("theta", ctypes.c_double), ("leaf_max", ctypes.c_int),
                 ("max_level", ctypes.c_int), ("q", ctypes.c_int), ("precision", ctypes.c_int),
-                ("preconditioner", ctypes.c_int)]
+                ("preconditioner", ctypes.c_int), ("refine_eta", ctypes.c_double)]
 
 
 class Result(ctypes.Structure):
@@ -56,6 +56,8 @@ SIGNATURES = [
     ("bemfmm_prepare", ctypes.c_int, [ctypes.c_void_p]),
     ("bemfmm_matvec", ctypes.c_int, [ctypes.c_void_p, _dp, _dp]),
     ("bemfmm_matvec_device", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+    ("bemfmm_matvec_fp32", ctypes.c_int, [ctypes.c_void_p, _dp, _dp]),
+    ("bemfmm_matvec_fp32_device", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     ("bemfmm_sync", ctypes.c_int, [ctypes.c_void_p]),
     ("bemfmm_rhs", ctypes.c_int, [ctypes.c_void_p, _dp]),
     ("bemfmm_alpha", ctypes.c_int, [ctypes.c_void_p, _dp]),
@@ -64,6 +66,7 @@ SIGNATURES = [
     ("bemfmm_get_stats", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(Stats)]),
     ("bemfmm_matvec_profile", ctypes.c_int, [ctypes.c_void_p, _dp]),
     ("bemfmm_residual_history", ctypes.c_int, [ctypes.c_void_p, _dp, ctypes.c_int]),
+    ("bemfmm_refinement_history", ctypes.c_int, [ctypes.c_void_p, _dp, ctypes.c_int]),
 ]
 
 _lib = None

@@ -49,12 +49,12 @@ class BEM_BVP:
 
     def __init__(self, mesh: Mesh, dofs: DofModel, p: int = 10, theta: float = 0.25, leaf_max: int = 50,
                  max_level: int = 7, q: int = 6, precision: int = _lib.PRECISION_FP64,
-                 preconditioner: int = _lib.PRECOND_NONE, device: int = 0):
+                 preconditioner: int = _lib.PRECOND_NONE, refine_eta: float = 1e-5, device: int = 0):
         self.lib = _lib.load()
         o = _lib.Opts()
         self.lib.bemfmm_default_opts(ctypes.byref(o))
         o.p, o.theta, o.leaf_max, o.max_level, o.q = p, theta, leaf_max, max_level, q
-        o.precision, o.preconditioner = precision, preconditioner
+        o.precision, o.preconditioner, o.refine_eta = precision, preconditioner, refine_eta
         self.ctx = self.lib.bemfmm_ctx_create(device, ctypes.byref(o))
         if not self.ctx:
             raise BEMError(_lib.E_DEVICE, "bemfmm_ctx_create failed (no usable GPU?)")
@@ -97,6 +97,16 @@ class BEM_BVP:
         self._check(self.lib.bemfmm_matvec(self.ctx, _ptr(x, _lib._dp), _ptr(y, _lib._dp)))
         return y
 
+    def compute_Ax_fp32(self, x) -> np.ndarray:
+        """The FP32 operator of the refinement loop (context created with PRECISION_FP32_REFINE)."""
+        x = _d(x)
+        y = np.empty(self.n_dof)
+        self._check(self.lib.bemfmm_matvec_fp32(self.ctx, _ptr(x, _lib._dp), _ptr(y, _lib._dp)))
+        return y
+
+    def matvec_fp32_device(self, x_ptr: int, y_ptr: int):
+        self._check(self.lib.bemfmm_matvec_fp32_device(self.ctx, ctypes.c_void_p(x_ptr), ctypes.c_void_p(y_ptr)))
+
     def matvec_device(self, x_ptr: int, y_ptr: int):
         self._check(self.lib.bemfmm_matvec_device(self.ctx, ctypes.c_void_p(x_ptr), ctypes.c_void_p(y_ptr)))
 
@@ -128,6 +138,13 @@ class BEM_BVP:
         out = np.empty(max(n, 0))
         if n > 0:
             self.lib.bemfmm_residual_history(self.ctx, _ptr(out, _lib._dp), n)
+        return out
+
+    def refinement_history(self) -> np.ndarray:
+        n = self.lib.bemfmm_refinement_history(self.ctx, None, 0)
+        out = np.empty(max(n, 0))
+        if n > 0:
+            self.lib.bemfmm_refinement_history(self.ctx, _ptr(out, _lib._dp), n)
         return out
 
     # -- solve --------------------------------------------------------------------

@@ -204,13 +204,14 @@ __global__ __launch_bounds__(256) void k_block_gemv(const int64_t* __restrict__ 
 // Operator-grouped translation on the matrix cores. For every pair (dst, src) of a
 // tile (<= kTP pairs sharing one operator): D[dst] += T_op * S[src] for both layers.
 // The tile is the GEMM  out[RP x 2kTP] += T[RP x KP] * B[KP x 2kTP]  built from
-// v_mfma_f64_16x16x4 blocks (RP = 64*RTW rows: C padded, KP = C padded to 16, zeros in
-// the padding): wave w owns row tiles [w*RTW, (w+1)*RTW) and all kTCT 16-column tiles.
-// B (the gathered source coefficients) is staged in LDS; T is streamed from the padded
-// operator array [op][KP][RP] through a two-stage register pipeline of kKG k-steps, so
-// that the loads of the next stage are in flight while the current one runs.
+// 16x16x4 MFMA blocks (RP = 64*RTW rows: C padded, KP = C padded to 16, zeros in the
+// padding): wave w owns row tiles [w*RTW, (w+1)*RTW) and all kTCT 16-column tiles.
+// B (the gathered source coefficients, rounded to T) is staged in LDS; T is streamed
+// from the padded operator array [op][KP][RP] through a two-stage register pipeline of
+// kKG k-steps, so that the loads of the next stage are in flight while the current one runs.
 // Workgroups are remapped so that each XCD processes one contiguous run of tiles
 // (tiles are sorted by operator): the tiles of an operator share that XCD's L2.
+// T = double: v_mfma_f64_16x16x4f64; T = float: v_mfma_f32_16x16x4f32 (FP32 accumulation).
 constexpr int kTP = 32;
 constexpr int kTCols = 2 * kTP;
 constexpr int kTStride = kTCols + 1;
@@ -218,15 +219,40 @@ constexpr int kTCT = kTCols / 16;  // 16-column MFMA tiles per workgroup tile
 constexpr int kXCD = 8;
 constexpr int kKG = 4;  // k-steps (of 4) per pipeline stage
 typedef double double4_t __attribute__((ext_vector_type(4)));
+typedef float float4_t __attribute__((ext_vector_type(4)));
 
-template <int RTW, bool STAGE>
-__global__ __launch_bounds__(256) void k_translate_mfma(int C, int RP, int KP, int ntiles, const double* __restrict__ ops,
+template <typename T>
+struct Mfma;
+template <>
+struct Mfma<double> {
+  typedef double4_t acc_t;
+  static __device__ __forceinline__ acc_t run(double a, double b, acc_t c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  // D block layout (measured on gfx950): register i of lane l holds row l/16 + 4*i of column l%16
+  static __device__ __forceinline__ int row(int lk, int i) { return lk + 4 * i; }
+};
+template <>
+struct Mfma<float> {
+  typedef float4_t acc_t;
+  static __device__ __forceinline__ acc_t run(float a, float b, acc_t c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  // D block layout (measured on gfx950): register i of lane l holds row 4*(l/16) + i of column l%16
+  static __device__ __forceinline__ int row(int lk, int i) { return 4 * lk + i; }
+};
+
+// STAGE = false: atomically accumulate into D0/D1 at the pair's dst box (M2M, L2L).
+// STAGE = true: write each pair's two result columns to stage[(pair*2 + layer)*C + row] (M2L).
+template <typename T, int RTW, bool STAGE>
+__global__ __launch_bounds__(256) void k_translate_mfma(int C, int RP, int KP, int ntiles, const T* __restrict__ ops,
                                                         const int32_t* __restrict__ tile_op, const int64_t* __restrict__ tile_p0,
                                                         const int64_t* __restrict__ tile_p1, const int32_t* __restrict__ dst,
                                                         const int32_t* __restrict__ src, const double* __restrict__ S0,
                                                         const double* __restrict__ S1, double* __restrict__ D0,
-                                                        double* __restrict__ D1) {
-  extern __shared__ double Bs[];  // KP * kTStride
+                                                        double* __restrict__ D1, T* __restrict__ stage) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  T* Bs = reinterpret_cast<T*>(smem);  // KP * kTStride
   const int per_xcd = (ntiles + kXCD - 1) / kXCD;
   const int t = (blockIdx.x % kXCD) * per_xcd + blockIdx.x / kXCD;
   if (t >= ntiles) return;
@@ -245,40 +271,39 @@ __global__ __launch_bounds__(256) void k_translate_mfma(int C, int RP, int KP, i
 #pragma unroll
       for (int u = 0; u < kTP; ++u) v[u] = (sidx[u] >= 0 && k < C) ? S[(int64_t)sidx[u] * C + k] : 0.;
 #pragma unroll
-      for (int u = 0; u < kTP; ++u) Bs[k * kTStride + 2 * u + layer] = v[u];
+      for (int u = 0; u < kTP; ++u) Bs[k * kTStride + 2 * u + layer] = (T)v[u];
     }
   }
   __syncthreads();
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int li = lane & 15, lk = lane >> 4;
-  const double* O = ops + (int64_t)op * KP * RP + (int64_t)lk * RP + wave * RTW * 16 + li;
-  const double* Bl = Bs + lk * kTStride + li;
-  double4_t acc[RTW][kTCT];
+  const T* O = ops + (int64_t)op * KP * RP + (int64_t)lk * RP + wave * RTW * 16 + li;
+  const T* Bl = Bs + lk * kTStride + li;
+  typename Mfma<T>::acc_t acc[RTW][kTCT];
 #pragma unroll
   for (int a = 0; a < RTW; ++a)
 #pragma unroll
-    for (int ct = 0; ct < kTCT; ++ct) acc[a][ct] = double4_t{0., 0., 0., 0.};
-  double av[2][kKG][RTW];
-  double bv[2][kKG][kTCT];
-  auto load = [&](int stage, int k0) {
+    for (int ct = 0; ct < kTCT; ++ct) acc[a][ct] = typename Mfma<T>::acc_t{0, 0, 0, 0};
+  T av[2][kKG][RTW];
+  T bv[2][kKG][kTCT];
+  auto load = [&](int st, int k0) {
 #pragma unroll
     for (int s = 0; s < kKG; ++s) {
-      const double* ok = O + (int64_t)(k0 + 4 * s) * RP;
+      const T* ok = O + (int64_t)(k0 + 4 * s) * RP;
 #pragma unroll
-      for (int a = 0; a < RTW; ++a) av[stage][s][a] = ok[a * 16];
-      const double* bk = Bl + (k0 + 4 * s) * kTStride;
+      for (int a = 0; a < RTW; ++a) av[st][s][a] = ok[a * 16];
+      const T* bk = Bl + (k0 + 4 * s) * kTStride;
 #pragma unroll
-      for (int ct = 0; ct < kTCT; ++ct) bv[stage][s][ct] = bk[16 * ct];
+      for (int ct = 0; ct < kTCT; ++ct) bv[st][s][ct] = bk[16 * ct];
     }
   };
-  auto run = [&](int stage) {
+  auto run = [&](int st) {
 #pragma unroll
     for (int s = 0; s < kKG; ++s)
 #pragma unroll
       for (int a = 0; a < RTW; ++a)
 #pragma unroll
-        for (int ct = 0; ct < kTCT; ++ct)
-          acc[a][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[stage][s][a], bv[stage][s][ct], acc[a][ct], 0, 0, 0);
+        for (int ct = 0; ct < kTCT; ++ct) acc[a][ct] = Mfma<T>::run(av[st][s][a], bv[st][s][ct], acc[a][ct]);
   };
   const int ng = KP / (4 * kKG);
   load(0, 0);
@@ -290,26 +315,24 @@ __global__ __launch_bounds__(256) void k_translate_mfma(int C, int RP, int KP, i
       run(1);
     }
   }
-  // D block layout of v_mfma_f64_16x16x4 (measured on gfx950): register i of a lane holds
-  // row lane/16 + 4*i of column lane%16
   if (STAGE) {
-    // per-pair results -> staging rows D0[(p0+pair)*2C + layer*C + row] with coalesced stores:
-    // transpose the tile through LDS (column-major, stride RP+1) and let each wave write whole columns
+    // per-pair results -> staging rows with coalesced stores: transpose the tile through
+    // LDS (column-major, stride RP+1) and let each wave write whole columns
     __syncthreads();
-    double* T = Bs;  // reuse the staging area (sized for max(KP * kTStride, kTCols * (RP + 1)) doubles)
+    T* Tt = Bs;  // reuse the staging area (sized for max(KP * kTStride, kTCols * (RP + 1)) elements)
 #pragma unroll
     for (int a = 0; a < RTW; ++a) {
       const int rt = wave * RTW + a;
 #pragma unroll
       for (int ct = 0; ct < kTCT; ++ct)
 #pragma unroll
-        for (int i = 0; i < 4; ++i) T[(ct * 16 + li) * (RP + 1) + rt * 16 + lk + 4 * i] = acc[a][ct][i];
+        for (int i = 0; i < 4; ++i) Tt[(ct * 16 + li) * (RP + 1) + rt * 16 + Mfma<T>::row(lk, i)] = acc[a][ct][i];
     }
     __syncthreads();
     for (int col = wave; col < 2 * np; col += 4) {
       const int pair = col >> 1, layer = col & 1;
-      double* out = D0 + ((p0 + pair) * 2 + layer) * (int64_t)C;
-      for (int r = lane; r < C; r += 64) out[r] = T[col * (RP + 1) + r];
+      T* out = stage + ((p0 + pair) * 2 + layer) * (int64_t)C;
+      for (int r = lane; r < C; r += 64) out[r] = Tt[col * (RP + 1) + r];
     }
     return;
   }
@@ -324,18 +347,19 @@ __global__ __launch_bounds__(256) void k_translate_mfma(int C, int RP, int KP, i
       double* D = (layer ? D1 : D0) + (int64_t)dst[p0 + pair] * C;
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
-        const int row = rt * 16 + lk + 4 * i;
-        if (row < C) atomicAdd(&D[row], acc[a][ct][i]);
+        const int row = rt * 16 + Mfma<T>::row(lk, i);
+        if (row < C) atomicAdd(&D[row], (double)acc[a][ct][i]);
       }
     }
   }
 }
 
 // M2L reduction: L[t] = sum over the M2L pairs of box t (fixed order) of the staged per-pair
-// results. One workgroup per box; thread j < 2C owns (layer j/C, row j%C). Writes every box,
-// so L needs no clearing before it.
+// results (FP64 accumulation). One workgroup per box; thread j < 2C owns (layer j/C, row j%C).
+// Writes every box, so L needs no clearing before it.
+template <typename T>
 __global__ __launch_bounds__(256) void k_m2l_reduce(int C, const int64_t* __restrict__ tptr, const int32_t* __restrict__ tpairs,
-                                                    const double* __restrict__ stage, double* __restrict__ L0,
+                                                    const T* __restrict__ stage, double* __restrict__ L0,
                                                     double* __restrict__ L1) {
   const int t = blockIdx.x;
   const int j = threadIdx.x;
@@ -357,6 +381,11 @@ __global__ __launch_bounds__(256) void k_m2l_reduce(int C, const int64_t* __rest
   const double v = (s0 + s1) + (s2 + s3);
   if (j < C) L0[(int64_t)t * C + j] = v;
   else L1[(int64_t)t * C + j - C] = v;
+}
+
+// FP64 -> FP32 copy of a resident matrix (FP32 near field / P2M / operators).
+__global__ void k_to_f32(int64_t n, const double* __restrict__ a, float* __restrict__ b) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) b[i] = (float)a[i];
 }
 
 // Av_i = GPhin - (GnPhi + alpha_i phi_i) (CORNER Neumann rows: Av_i = phi_i). mode 1: b = -Av (CORNER: phi_D).
@@ -622,12 +651,14 @@ struct bemfmm_ctx {
   DBuf<int32_t> m2l_tpairs;
   DBuf<double> m2l_stage;    // per-pair M2L results [pair][layer][C]
   DBuf<double> ops;
+  // FP32 copies for BEMFMM_PRECISION_FP32_REFINE (inner matvecs); the M2L staging is shared
+  DBuf<float> near_mat32, p2m_mat32, ops32;
   DBuf<int32_t> kind, point;
   DBuf<double> alpha, phi_k, phin_k;
   DBuf<double> buf0, buf1;   // [phi_n | L0], [phi | L1]
   DBuf<double> M0, M1;
   DBuf<double> y0, y1;
-  DBuf<double> xin, yout, bvec, tmpv, xsol, xbase, jacobi;
+  DBuf<double> xin, yout, bvec, tmpv, xsol, xbase, jacobi, rvec, dvec;
   DBuf<double> V;            // Krylov basis
   int64_t v_cap = 0;
   DBuf<double> partial, hbuf;
@@ -637,7 +668,8 @@ struct bemfmm_ctx {
   hipEvent_t ev[8] = {};
   double prof_ms[7] = {};
   int n_matvec = 0;
-  std::vector<double> history;  // GMRES residual estimate per iteration (last solve)
+  std::vector<double> history;         // GMRES residual estimate per iteration (last solve)
+  std::vector<double> refine_history;  // FP64 residual at each refinement step (last solve, FP32_REFINE)
 };
 
 namespace {
@@ -663,25 +695,27 @@ int guarded(bemfmm_ctx* c, F&& f) {
   }
 }
 
-void launch_gemv(bemfmm_ctx* c, DGemv& g, const double* x0, const double* x1, double* y0, double* y1) {
+template <typename T>
+void launch_gemv(bemfmm_ctx* c, DGemv& g, const T* mat, const double* x0, const double* x1, double* y0, double* y1) {
   if (g.nblk == 0) return;
-  hipLaunchKernelGGL(k_block_gemv<double>, dim3(g.nblk), dim3(256), 0, c->stream, g.row_ptr.p, g.col_ptr.p, g.mat_ptr.p,
-                     g.row_out.p, g.col_idx.p, g.mat.p, x0, x1, y0, y1);
+  hipLaunchKernelGGL(k_block_gemv<T>, dim3(g.nblk), dim3(256), 0, c->stream, g.row_ptr.p, g.col_ptr.p, g.mat_ptr.p,
+                     g.row_out.p, g.col_idx.p, mat, x0, x1, y0, y1);
 }
 
-void launch_translate(bemfmm_ctx* c, DTiles& t, const double* S0, const double* S1, double* D0, double* D1,
-                      bool stage = false) {
+template <typename T>
+void launch_translate(bemfmm_ctx* c, DTiles& t, const T* ops, const double* S0, const double* S1, double* D0, double* D1,
+                      T* stage = nullptr) {
   if (t.ntiles == 0) return;
-  const size_t lds = (size_t)std::max(c->KP * kTStride, kTCols * (c->RP + 1)) * sizeof(double);
+  const size_t lds = (size_t)std::max(c->KP * kTStride, kTCols * (c->RP + 1)) * sizeof(T);
   const int grid = ((t.ntiles + kXCD - 1) / kXCD) * kXCD;
   const int RTW = c->RP / 64;
-#define BEMFMM_TR(N)                                                                                               \
-  if (stage)                                                                                                         \
-    hipLaunchKernelGGL((k_translate_mfma<N, true>), dim3(grid), dim3(256), lds, c->stream, c->C, c->RP, c->KP, t.ntiles, \
-                       c->ops.p, t.op.p, t.p0.p, t.p1.p, t.dst.p, t.src.p, S0, S1, D0, D1);                          \
-  else                                                                                                               \
-    hipLaunchKernelGGL((k_translate_mfma<N, false>), dim3(grid), dim3(256), lds, c->stream, c->C, c->RP, c->KP,        \
-                       t.ntiles, c->ops.p, t.op.p, t.p0.p, t.p1.p, t.dst.p, t.src.p, S0, S1, D0, D1)
+#define BEMFMM_TR(N)                                                                                                    \
+  if (stage)                                                                                                            \
+    hipLaunchKernelGGL((k_translate_mfma<T, N, true>), dim3(grid), dim3(256), lds, c->stream, c->C, c->RP, c->KP,       \
+                       t.ntiles, ops, t.op.p, t.p0.p, t.p1.p, t.dst.p, t.src.p, S0, S1, D0, D1, stage);                  \
+  else                                                                                                                  \
+    hipLaunchKernelGGL((k_translate_mfma<T, N, false>), dim3(grid), dim3(256), lds, c->stream, c->C, c->RP, c->KP,      \
+                       t.ntiles, ops, t.op.p, t.p0.p, t.p1.p, t.dst.p, t.src.p, S0, S1, D0, D1, stage)
   switch (RTW) {
   case 1: BEMFMM_TR(1); break;
   case 2: BEMFMM_TR(2); break;
@@ -699,7 +733,8 @@ void fill(bemfmm_ctx* c, double* p, int64_t n, double v) {
 }
 
 // One evaluation of the BIE operator. mode: 0 = A x (x = in), 1 = rhs, 2 = rigid-mode alpha.
-void evaluate(bemfmm_ctx* c, int mode, const double* in, double* out) {
+// lo: FP32 near field, P2M and translations (FP32 MFMA), FP64 vectors and accumulation across boxes.
+void evaluate(bemfmm_ctx* c, int mode, const double* in, double* out, bool lo = false) {
   Plan& pl = *c->plan;
   const int n = c->n_dof;
   const int64_t nbC = (int64_t)c->n_boxes * c->C;
@@ -716,17 +751,32 @@ void evaluate(bemfmm_ctx* c, int mode, const double* in, double* out) {
   fill(c, c->M0.p, nbC, 0.);
   fill(c, c->M1.p, nbC, 0.);
   rec(1);
-  launch_gemv(c, c->p2m, phin, phi, c->M0.p, c->M1.p);
+  if (lo) launch_gemv(c, c->p2m, c->p2m_mat32.p, phin, phi, c->M0.p, c->M1.p);
+  else launch_gemv(c, c->p2m, c->p2m.mat.p, phin, phi, c->M0.p, c->M1.p);
   rec(2);
-  for (int l = pl.n_levels - 1; l >= 1; --l) launch_translate(c, c->m2m[l], c->M0.p, c->M1.p, c->M0.p, c->M1.p);
+  for (int l = pl.n_levels - 1; l >= 1; --l) {
+    if (lo) launch_translate(c, c->m2m[l], c->ops32.p, c->M0.p, c->M1.p, c->M0.p, c->M1.p);
+    else launch_translate(c, c->m2m[l], c->ops.p, c->M0.p, c->M1.p, c->M0.p, c->M1.p);
+  }
   rec(3);
-  launch_translate(c, c->m2l, c->M0.p, c->M1.p, c->m2l_stage.p, nullptr, true);
-  hipLaunchKernelGGL(k_m2l_reduce, dim3(c->n_boxes), dim3(256), 0, c->stream, c->C, c->m2l_tptr.p, c->m2l_tpairs.p,
-                     c->m2l_stage.p, L0, L1);
+  if (lo) {
+    float* st = reinterpret_cast<float*>(c->m2l_stage.p);
+    launch_translate(c, c->m2l, c->ops32.p, c->M0.p, c->M1.p, nullptr, nullptr, st);
+    hipLaunchKernelGGL(k_m2l_reduce<float>, dim3(c->n_boxes), dim3(256), 0, c->stream, c->C, c->m2l_tptr.p,
+                       c->m2l_tpairs.p, (const float*)st, L0, L1);
+  } else {
+    launch_translate(c, c->m2l, c->ops.p, c->M0.p, c->M1.p, nullptr, nullptr, c->m2l_stage.p);
+    hipLaunchKernelGGL(k_m2l_reduce<double>, dim3(c->n_boxes), dim3(256), 0, c->stream, c->C, c->m2l_tptr.p,
+                       c->m2l_tpairs.p, (const double*)c->m2l_stage.p, L0, L1);
+  }
   rec(4);
-  for (int l = 1; l < pl.n_levels; ++l) launch_translate(c, c->l2l[l], L0, L1, L0, L1);
+  for (int l = 1; l < pl.n_levels; ++l) {
+    if (lo) launch_translate(c, c->l2l[l], c->ops32.p, L0, L1, L0, L1);
+    else launch_translate(c, c->l2l[l], c->ops.p, L0, L1, L0, L1);
+  }
   rec(5);
-  launch_gemv(c, c->near, c->buf0.p, c->buf1.p, c->y0.p, c->y1.p);
+  if (lo) launch_gemv(c, c->near, c->near_mat32.p, c->buf0.p, c->buf1.p, c->y0.p, c->y1.p);
+  else launch_gemv(c, c->near, c->near.mat.p, c->buf0.p, c->buf1.p, c->y0.p, c->y1.p);
   rec(6);
   hipLaunchKernelGGL(k_assemble, dim3(grid1(n)), dim3(256), 0, c->stream, mode, n, c->kind.p, c->point.p, c->alpha.p, phi,
                      c->y0.p, c->y1.p, c->phi_k.p, out);
@@ -738,6 +788,8 @@ void evaluate(bemfmm_ctx* c, int mode, const double* in, double* out) {
 void do_prepare(bemfmm_ctx* c) {
   if (!c->have_geom) throw std::runtime_error("bemfmm_prepare: geometry not set");
   if (!c->have_bc) throw std::runtime_error("bemfmm_prepare: boundary values not set");
+  if (c->opts.precision != BEMFMM_PRECISION_FP64 && c->opts.precision != BEMFMM_PRECISION_FP32_REFINE)
+    throw std::runtime_error("bemfmm_prepare: unknown precision mode");
   const double t0 = now();
   bemfmm::Options o;
   o.p = c->opts.p;
@@ -745,6 +797,12 @@ void do_prepare(bemfmm_ctx* c) {
   o.leaf_max = c->opts.leaf_max;
   o.max_level = c->opts.max_level;
   o.q = c->opts.q;
+  c->prepared = false;
+  // the Krylov basis is sized for n_dof: drop it (re-grown by the next solve)
+  c->V.release();
+  c->partial.release();
+  c->hbuf.release();
+  c->v_cap = 0;
   c->plan.reset(new Plan());
   Plan& pl = *c->plan;
   bemfmm::build_plan(pl, o, c->n_nodes, c->X.data(), c->n_faces, c->tri.data(), c->face_dof.data(), c->n_dof,
@@ -805,7 +863,8 @@ void do_prepare(bemfmm_ctx* c) {
   c->y0.alloc(c->n_nodes);
   c->y1.alloc(c->n_nodes);
   c->alpha.alloc(c->n_dof);
-  for (auto* b : {&c->xin, &c->yout, &c->bvec, &c->tmpv, &c->xsol, &c->xbase, &c->jacobi}) b->alloc(c->n_dof);
+  for (auto* b : {&c->xin, &c->yout, &c->bvec, &c->tmpv, &c->xsol, &c->xbase, &c->jacobi, &c->rvec, &c->dvec})
+    b->alloc(c->n_dof);
   // quadrature inputs
   DBuf<int32_t> row_block, ncol_dof, face_perm, fv, fdof;
   DBuf<int64_t> range_ptr, ranges, range_prefix;
@@ -878,6 +937,22 @@ void do_prepare(bemfmm_ctx* c) {
   } else if (c->opts.preconditioner != BEMFMM_PRECOND_NONE) {
     throw std::runtime_error("unknown preconditioner");
   }
+  if (c->opts.precision == BEMFMM_PRECISION_FP32_REFINE) {
+    // round the resident FP64 matrices once to FP32 (operators were computed in binary128 and rounded to FP64)
+    c->near_mat32.alloc(pl.near.mat_size);
+    hipLaunchKernelGGL(k_to_f32, dim3(grid1(pl.near.mat_size)), dim3(256), 0, s, (int64_t)pl.near.mat_size, c->near.mat.p,
+                       c->near_mat32.p);
+    c->p2m_mat32.alloc(c->p2m.mat.n);
+    hipLaunchKernelGGL(k_to_f32, dim3(grid1(c->p2m.mat.n)), dim3(256), 0, s, (int64_t)c->p2m.mat.n, c->p2m.mat.p,
+                       c->p2m_mat32.p);
+    c->ops32.alloc(c->ops.n);
+    hipLaunchKernelGGL(k_to_f32, dim3(grid1(c->ops.n)), dim3(256), 0, s, (int64_t)c->ops.n, c->ops.p, c->ops32.p);
+    HIPCHECK(hipGetLastError());
+  } else {
+    c->near_mat32.release();
+    c->p2m_mat32.release();
+    c->ops32.release();
+  }
   HIPCHECK(hipStreamSynchronize(s));
   const double t4 = now();
   c->prepared = true;
@@ -894,7 +969,8 @@ void do_prepare(bemfmm_ctx* c) {
   st.n_m2l_ops = (int)pl.m2l.op.size();
   st.L = pl.Lscale;
   st.device_bytes = (double)(c->near.bytes() + c->p2m.bytes() + c->m2l.bytes() + c->ops.bytes() + c->buf0.bytes() +
-                             c->buf1.bytes() + c->M0.bytes() + c->M1.bytes() + c->m2l_stage.bytes());
+                             c->buf1.bytes() + c->M0.bytes() + c->M1.bytes() + c->m2l_stage.bytes() +
+                             c->near_mat32.bytes() + c->p2m_mat32.bytes() + c->ops32.bytes());
   st.t_tree = pl.t_tree;
   st.t_lists = pl.t_lists;
   st.t_near_struct = pl.t_near_struct;
@@ -923,6 +999,10 @@ void ensure_basis(bemfmm_ctx* c, int64_t need) {
 // h[0..k) = V[0..k) . w  (device), copied to host
 void dots(bemfmm_ctx* c, int k, const double* Vp, const double* w, std::vector<double>& h) {
   const int64_t n = c->n_dof;
+  // the reduction buffers are sized by ensure_basis for up to v_cap + 1 products
+  if (k < 1 || k > c->v_cap + 1 || !c->partial.p || !c->hbuf.p || (size_t)k * kDotBlocks > c->partial.n ||
+      (size_t)k > c->hbuf.n)
+    throw HipError("internal: dot-product buffers not sized for " + std::to_string(k) + " products", BEMFMM_E_STATE);
   hipLaunchKernelGGL(k_dots, dim3(kDotBlocks, k), dim3(256), 0, c->stream, Vp, n, w, n, c->partial.p);
   hipLaunchKernelGGL(k_reduce_partials, dim3(k), dim3(256), 0, c->stream, c->partial.p, kDotBlocks, k, c->hbuf.p);
   h.resize(k);
@@ -935,39 +1015,53 @@ double norm2(bemfmm_ctx* c, const double* w) {
   return std::sqrt(h[0]);
 }
 
-// GMRES (run_gmres control flow, BEM_solveBVP_GMRES_FMM.hpp:2275-2355; Arnoldi + Givens,
-// basic_linear_systems.hpp:3137-3451) with classical Gram-Schmidt applied twice on the device.
-int do_solve(bemfmm_ctx* c, const double* x0, double* xout, double tol, int max_iter, int restart, bemfmm_result* res) {
-  if (!c->prepared) throw std::runtime_error("bemfmm_solve: call bemfmm_prepare first");
-  const double ts = now();
+// One GMRES run on  M^-1 A x = rhs  (rhs: device, already preconditioned), x (device) updated
+// in place. confirm = true follows run_gmres (BEM_solveBVP_GMRES_FMM.hpp:2275-2355): restart
+// cycles, an estimate <= 10 tol triggers a true-residual check, up to 5 saturated checks.
+// confirm = false (inner solve of the refinement loop) stops when the estimate reaches tol.
+// Arnoldi + Givens as basic_linear_systems.hpp:3137-3451, with classical Gram-Schmidt applied
+// twice on the device. Returns the last (true or estimated) residual; iters accumulates.
+struct GmresRun {
+  const double* rhs;
+  double* x;
+  double tol;
+  int max_iter;  // iteration budget including what `iters` holds on entry
+  int restart;
+  bool lo;       // FP32 matvecs
+  bool confirm;
+  bool x_zero;   // x is zero on entry: r0 = rhs without a matvec
+};
+
+double gmres_run(bemfmm_ctx* c, const GmresRun& R, int& iters) {
   const int64_t n = c->n_dof;
   hipStream_t s = c->stream;
-  const int n_mv0 = c->n_matvec;
   const int gN = grid1(n);
   const bool jac = c->opts.preconditioner == BEMFMM_PRECOND_JACOBI;
-  evaluate(c, 1, nullptr, c->bvec.p);  // b, then M^-1 b (left preconditioning, GMRES_FMM.hpp:2157-2171)
-  if (jac) hipLaunchKernelGGL(k_mul, dim3(gN), dim3(256), 0, s, n, c->jacobi.p, c->bvec.p);
-  HIPCHECK(hipMemcpyAsync(c->xsol.p, x0, n * sizeof(double), hipMemcpyHostToDevice, s));
   auto Av = [&](const double* v, double* w) {  // return_A_dot_v: M^-1 A v
-    evaluate(c, 0, v, w);
+    evaluate(c, 0, v, w, R.lo);
     if (jac) hipLaunchKernelGGL(k_mul, dim3(gN), dim3(256), 0, s, n, c->jacobi.p, w);
   };
-  int iters = 0;
   double cur_err = 1e300;
-  const int restart_max = std::max(1, restart);
+  const int restart_max = std::max(1, R.restart);
   std::vector<double> h1, h2;
-  c->history.clear();
-  while (iters < max_iter && cur_err > tol) {
-    const int remaining = max_iter - iters;
+  bool first = true;
+  while (iters < R.max_iter && cur_err > R.tol) {
+    const int remaining = R.max_iter - iters;
     const int k_cycle_max = std::min(restart_max, remaining);
     ensure_basis(c, 8);
-    // r0 = b - A x
-    Av(c->xsol.p, c->tmpv.p);
-    hipLaunchKernelGGL(k_sub, dim3(gN), dim3(256), 0, s, n, c->bvec.p, c->tmpv.p, c->V.p);
+    // r0 = rhs - M^-1 A x
+    if (first && R.x_zero) {
+      HIPCHECK(hipMemcpyAsync(c->V.p, R.rhs, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+    } else {
+      Av(R.x, c->tmpv.p);
+      hipLaunchKernelGGL(k_sub, dim3(gN), dim3(256), 0, s, n, R.rhs, c->tmpv.p, c->V.p);
+    }
+    first = false;
     const double beta = norm2(c, c->V.p);
-    HIPCHECK(hipMemcpyAsync(c->xbase.p, c->xsol.p, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+    HIPCHECK(hipMemcpyAsync(c->xbase.p, R.x, n * sizeof(double), hipMemcpyDeviceToDevice, s));
     std::vector<double> cs, sn, g(1, beta);
     std::vector<std::vector<double>> Rm;
+    if (!std::isfinite(beta)) throw HipError("GMRES breakdown: non-finite initial residual", BEMFMM_E_NONFINITE);
     if (beta == 0.) {
       cur_err = 0.;
       break;
@@ -986,7 +1080,7 @@ int do_solve(bemfmm_ctx* c, const double* x0, double* xout, double tol, int max_
       for (int i = 0; i < nb; ++i) col[i] = h1[i] + h2[i];
       const double hn = norm2(c, w);
       col[nb] = hn;
-      hipLaunchKernelGGL(k_axpby, dim3(gN), dim3(256), 0, s, n, 1.0 / hn, w, 0.0, w);
+      hipLaunchKernelGGL(k_axpby, dim3(gN), dim3(256), 0, s, n, hn > 0. ? 1.0 / hn : 0.0, w, 0.0, w);
       ++nb;
       // Givens update of the new column
       const size_t cc = cs.size();
@@ -1020,19 +1114,24 @@ int do_solve(bemfmm_ctx* c, const double* x0, double* xout, double tol, int max_
         for (size_t j = i + 1; j < k; ++j) sum -= Rm[j][i] * y[j];
         y[i] = sum / Rm[i][i];
       }
-      HIPCHECK(hipMemcpyAsync(c->xsol.p, c->xbase.p, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+      HIPCHECK(hipMemcpyAsync(R.x, c->xbase.p, n * sizeof(double), hipMemcpyDeviceToDevice, s));
       HIPCHECK(hipMemcpyAsync(c->hbuf.p, y.data(), k * sizeof(double), hipMemcpyHostToDevice, s));
-      hipLaunchKernelGGL(k_vsum, dim3(gN), dim3(256), 0, s, c->V.p, n, (int)k, c->hbuf.p, 1.0, c->xsol.p, n);
+      hipLaunchKernelGGL(k_vsum, dim3(gN), dim3(256), 0, s, c->V.p, n, (int)k, c->hbuf.p, 1.0, R.x, n);
       HIPCHECK(hipStreamSynchronize(s));
     };
     auto true_residual = [&]() {
-      Av(c->xsol.p, c->tmpv.p);
-      hipLaunchKernelGGL(k_sub, dim3(gN), dim3(256), 0, s, n, c->tmpv.p, c->bvec.p, c->tmpv.p);
+      Av(R.x, c->tmpv.p);
+      hipLaunchKernelGGL(k_sub, dim3(gN), dim3(256), 0, s, n, c->tmpv.p, R.rhs, c->tmpv.p);
       return norm2(c, c->tmpv.p);
     };
     add_basis();  // gmres(A, b, x, 1)
     iters += 1;
     c->history.push_back(std::abs(g.back()));
+    if (!R.confirm && std::abs(g.back()) <= R.tol) {
+      update_solution();
+      cur_err = std::abs(g.back());
+      break;
+    }
     int sat = 0;
     while (true) {
       add_basis();
@@ -1040,18 +1139,91 @@ int do_solve(bemfmm_ctx* c, const double* x0, double* xout, double tol, int max_
       const double est = std::abs(g.back());
       c->history.push_back(est);
       if (!std::isfinite(est)) throw HipError("GMRES breakdown: non-finite residual estimate", BEMFMM_E_NONFINITE);
-      if (est <= tol * 10.) {
+      if (!R.confirm) {
+        if (est <= R.tol || (int)cs.size() >= k_cycle_max) {
+          update_solution();
+          cur_err = est;
+          break;
+        }
+        continue;
+      }
+      if (est <= R.tol * 10.) {
         update_solution();
         cur_err = true_residual();
-        if (cur_err <= tol) break;
+        if (cur_err <= R.tol) break;
         if (sat++ >= 5) break;
       }
-      if (iters >= max_iter && (int)cs.size() >= k_cycle_max) {
+      if (iters >= R.max_iter && (int)cs.size() >= k_cycle_max) {
         update_solution();
-        cur_err = (est <= tol * 10.) ? true_residual() : est;
+        cur_err = (est <= R.tol * 10.) ? true_residual() : est;
         break;
       }
     }
+    if (!R.confirm) break;
+  }
+  return cur_err;
+}
+
+// FP64 preconditioned residual r = M^-1 (b - A x) into out; returns |r|_2.
+double fp64_residual(bemfmm_ctx* c, const double* x, double* out) {
+  const int64_t n = c->n_dof;
+  const int gN = grid1(n);
+  evaluate(c, 0, x, out, false);
+  if (c->opts.preconditioner == BEMFMM_PRECOND_JACOBI)
+    hipLaunchKernelGGL(k_mul, dim3(gN), dim3(256), 0, c->stream, n, c->jacobi.p, out);
+  hipLaunchKernelGGL(k_sub, dim3(gN), dim3(256), 0, c->stream, n, c->bvec.p, out, out);
+  return norm2(c, out);
+}
+
+// Solve (solveSystemGMRES, BEM_solveBVP_GMRES_FMM.hpp:2007-2355).
+// FP64: one run_gmres with FP64 matvecs.
+// FP32_REFINE: iterative refinement. Each step takes the FP64 residual r = M^-1 (b - A x), solves
+// M^-1 A_32 d = r with FP32 matvecs to |r| * refine_eta, and sets x += d; it stops when |r| <= tol
+// (the same FP64 criterion as the FP64 path). If a step fails to halve |r|, the remaining budget
+// goes to an FP64 run_gmres from the current x, so the answer always meets the FP64 criterion.
+int do_solve(bemfmm_ctx* c, const double* x0, double* xout, double tol, int max_iter, int restart, bemfmm_result* res) {
+  if (!c->prepared) throw std::runtime_error("bemfmm_solve: call bemfmm_prepare first");
+  const double ts = now();
+  const int64_t n = c->n_dof;
+  hipStream_t s = c->stream;
+  const int n_mv0 = c->n_matvec;
+  const int gN = grid1(n);
+  const bool jac = c->opts.preconditioner == BEMFMM_PRECOND_JACOBI;
+  ensure_basis(c, 8);  // Krylov basis and the dot-product buffers (norms are taken before the first cycle)
+  evaluate(c, 1, nullptr, c->bvec.p);  // b, then M^-1 b (left preconditioning, GMRES_FMM.hpp:2157-2171)
+  if (jac) hipLaunchKernelGGL(k_mul, dim3(gN), dim3(256), 0, s, n, c->jacobi.p, c->bvec.p);
+  HIPCHECK(hipMemcpyAsync(c->xsol.p, x0, n * sizeof(double), hipMemcpyHostToDevice, s));
+  c->history.clear();
+  c->refine_history.clear();
+  int iters = 0, steps = 0;
+  double cur_err = 0.;
+  if (c->opts.precision == BEMFMM_PRECISION_FP32_REFINE) {
+    const double eta = (c->opts.refine_eta > 0. && c->opts.refine_eta < 1.) ? c->opts.refine_eta : 1e-5;
+    double prev = 1e300;
+    bool fallback = false;
+    while (true) {
+      cur_err = fp64_residual(c, c->xsol.p, c->rvec.p);
+      c->refine_history.push_back(cur_err);
+      if (!std::isfinite(cur_err)) throw HipError("non-finite residual in refinement", BEMFMM_E_NONFINITE);
+      if (cur_err <= tol || iters >= max_iter) break;
+      if (cur_err > 0.5 * prev) {
+        fallback = true;
+        break;
+      }
+      prev = cur_err;
+      fill(c, c->dvec.p, n, 0.);
+      GmresRun in{c->rvec.p, c->dvec.p, std::max(eta * cur_err, 0.5 * tol), max_iter, restart, true, false, true};
+      gmres_run(c, in, iters);
+      hipLaunchKernelGGL(k_axpby, dim3(gN), dim3(256), 0, s, n, 1.0, c->dvec.p, 1.0, c->xsol.p);
+      ++steps;
+    }
+    if (fallback && iters < max_iter) {
+      GmresRun r64{c->bvec.p, c->xsol.p, tol, max_iter, restart, false, true, false};
+      cur_err = gmres_run(c, r64, iters);
+    }
+  } else {
+    GmresRun r64{c->bvec.p, c->xsol.p, tol, max_iter, restart, false, true, false};
+    cur_err = gmres_run(c, r64, iters);
   }
   std::vector<double> x(n);
   HIPCHECK(hipMemcpyAsync(x.data(), c->xsol.p, n * sizeof(double), hipMemcpyDeviceToHost, s));
@@ -1064,7 +1236,7 @@ int do_solve(bemfmm_ctx* c, const double* x0, double* xout, double tol, int max_
     res->converged = cur_err <= tol ? 1 : 0;
     res->residual = cur_err;
     res->time_solve = now() - ts;
-    res->refinement_steps = 0;
+    res->refinement_steps = steps;
   }
   for (int64_t i = 0; i < n; ++i)
     if (!std::isfinite(xout[i])) throw HipError("non-finite solution", BEMFMM_E_NONFINITE);
@@ -1098,6 +1270,7 @@ void bemfmm_default_opts(bemfmm_opts* o) {
   o->q = 6;
   o->precision = BEMFMM_PRECISION_FP64;
   o->preconditioner = BEMFMM_PRECOND_NONE;
+  o->refine_eta = 1e-5;
 }
 
 bemfmm_ctx* bemfmm_ctx_create(int device, const bemfmm_opts* opts) {
@@ -1210,6 +1383,31 @@ int bemfmm_matvec_device(bemfmm_ctx* c, const double* x, double* y) {
   });
 }
 
+int bemfmm_matvec_fp32(bemfmm_ctx* c, const double* x, double* y) {
+  if (!c) return BEMFMM_E_INVALID;
+  return guarded(c, [&]() {
+    if (!c->prepared) throw HipError("bemfmm_matvec_fp32: call bemfmm_prepare first", BEMFMM_E_STATE);
+    if (c->opts.precision != BEMFMM_PRECISION_FP32_REFINE)
+      throw HipError("bemfmm_matvec_fp32: context not created with BEMFMM_PRECISION_FP32_REFINE", BEMFMM_E_STATE);
+    HIPCHECK(hipMemcpyAsync(c->xin.p, x, c->n_dof * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    evaluate(c, 0, c->xin.p, c->yout.p, true);
+    HIPCHECK(hipMemcpyAsync(y, c->yout.p, c->n_dof * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(hipStreamSynchronize(c->stream));
+    return BEMFMM_OK;
+  });
+}
+
+int bemfmm_matvec_fp32_device(bemfmm_ctx* c, const double* x, double* y) {
+  if (!c) return BEMFMM_E_INVALID;
+  return guarded(c, [&]() {
+    if (!c->prepared) throw HipError("bemfmm_matvec_fp32_device: call bemfmm_prepare first", BEMFMM_E_STATE);
+    if (c->opts.precision != BEMFMM_PRECISION_FP32_REFINE)
+      throw HipError("bemfmm_matvec_fp32_device: context not created with BEMFMM_PRECISION_FP32_REFINE", BEMFMM_E_STATE);
+    evaluate(c, 0, x, y, true);
+    return BEMFMM_OK;
+  });
+}
+
 int bemfmm_sync(bemfmm_ctx* c) {
   if (!c) return BEMFMM_E_INVALID;
   return guarded(c, [&]() {
@@ -1252,6 +1450,14 @@ int bemfmm_residual_history(const bemfmm_ctx* c, double* out, int cap) {
   const int n = (int)c->history.size();
   if (out)
     for (int i = 0; i < std::min(n, cap); ++i) out[i] = c->history[i];
+  return n;
+}
+
+int bemfmm_refinement_history(const bemfmm_ctx* c, double* out, int cap) {
+  if (!c) return BEMFMM_E_INVALID;
+  const int n = (int)c->refine_history.size();
+  if (out)
+    for (int i = 0; i < std::min(n, cap); ++i) out[i] = c->refine_history[i];
   return n;
 }
 

@@ -78,6 +78,82 @@ def cpu_baseline(p, leaf_max, max_level, gpu_iters, n_nodes_full, budget_s=25.0)
                        f"{n_nodes_full} nodes at the GPU's {gpu_iters}+3 matvecs per solve")}
 
 
+STAGES = ["scatter", "p2m", "m2m", "m2l", "l2l", "near_l2p", "assemble"]
+
+
+def _log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def setup_bvp(m, d, phi, phin, args, precision):
+    from bem_for_nonlinear_waves_amd import _lib
+    from bem_for_nonlinear_waves_amd.bvp import BEM_BVP
+    pc = _lib.PRECOND_JACOBI if args.precond == "jacobi" else _lib.PRECOND_NONE
+    prec = _lib.PRECISION_FP32_REFINE if precision == "fp32_refine" else _lib.PRECISION_FP64
+    bvp = BEM_BVP(m, d, p=args.p, leaf_max=args.leaf, max_level=args.max_level, preconditioner=pc, precision=prec,
+                  refine_eta=args.refine_eta)
+    bvp.set_bc(phi, phin)
+    t = time.perf_counter()
+    bvp.prepare()
+    t_setup = time.perf_counter() - t
+    st = bvp.stats()
+    _log(f"[{precision}] setup {t_setup:.2f}s: boxes={st['n_boxes']} leaves={st['n_leaves']} "
+         f"m2l_pairs={st['n_m2l_pairs']} near_nnz={st['near_nnz']} device={st['device_bytes']/1e9:.1f}GB "
+         f"(tree {st['t_tree']:.2f} lists {st['t_lists']:.2f} near_struct {st['t_near_struct']:.2f} "
+         f"p2m {st['t_p2m']:.2f} ops {st['t_ops']:.2f} upload {st['t_upload']:.2f} near_quad {st['t_near_quad']:.2f} "
+         f"alpha {st['t_alpha']:.2f})")
+    return bvp, st, t_setup
+
+
+def time_matvec(bvp, n, n_nodes, reps, fp32):
+    """Matvec on device-resident vectors: wall time per matvec and per-stage HIP-event times (ms)."""
+    import torch
+    mv = bvp.matvec_fp32_device if fp32 else bvp.matvec_device
+    xs = torch.randn(n, dtype=torch.float64, device="cuda")
+    ys = torch.empty(n, dtype=torch.float64, device="cuda")
+    torch.cuda.synchronize()
+    bvp.profile()  # enable event recording on the library's stream
+    for _ in range(2):
+        mv(xs.data_ptr(), ys.data_ptr())
+    bvp.sync()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        mv(xs.data_ptr(), ys.data_ptr())
+    bvp.sync()
+    t_mv = (time.perf_counter() - t0) / reps
+    stage = np.zeros(7)
+    for _ in range(5):
+        mv(xs.data_ptr(), ys.data_ptr())
+        stage += bvp.profile()  # synchronises on the last event of this matvec
+    stage /= 5
+    _log("%s matvec %.3f ms (%.2f Mnode/s); stages ms: %s" % ("fp32" if fp32 else "fp64", t_mv * 1e3,
+         n_nodes / t_mv / 1e6, " ".join(f"{k}={v:.3f}" for k, v in zip(STAGES, stage))))
+    return t_mv, stage
+
+
+def run_solves(bvp, n, args, label):
+    import torch
+    x0 = np.zeros(n)
+    for i in range(args.warmup):
+        x, info = bvp.solveSystemGMRES(x0, tol=args.tol, max_iter=args.max_iter, restart=args.max_iter)
+        _log(f"[{label}] warmup solve {i}: iters={info['iters']} steps={info['refinement_steps']} "
+             f"residual={info['residual']:.2e} {info['time_solve']:.2f}s")
+    torch.cuda.synchronize()
+    lat, infos = [], []
+    t_start = time.perf_counter()
+    for i in range(args.steps):
+        ts = time.perf_counter()
+        x, info = bvp.solveSystemGMRES(x0, tol=args.tol, max_iter=args.max_iter, restart=args.max_iter)
+        lat.append(time.perf_counter() - ts)
+        infos.append(info)
+        _log(f"[{label}] solve {i}: iters={info['iters']} matvecs={info['n_matvec']} "
+             f"steps={info['refinement_steps']} residual={info['residual']:.2e} converged={info['converged']} "
+             f"{lat[-1]:.3f}s")
+    torch.cuda.synchronize()
+    t_total = time.perf_counter() - t_start
+    return x, lat, infos, t_total
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -91,6 +167,10 @@ def main():
     ap.add_argument("--max-iter", type=int, default=2000)
     ap.add_argument("--matvecs", type=int, default=20)
     ap.add_argument("--precond", choices=["none", "jacobi"], default="jacobi")
+    ap.add_argument("--precision", choices=["fp32_refine", "fp64"], default="fp32_refine")
+    ap.add_argument("--refine-eta", type=float, default=1e-5)
+    ap.add_argument("--no-fp64-compare", action="store_true",
+                    help="skip the plain FP64 solves timed beside the FP32+refinement ones")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--profile-only", action="store_true",
                     help="setup + a 256 MiB device copy (counter calibration) + the timed matvecs; no solves")
@@ -102,51 +182,17 @@ def main():
 
     import torch
 
-    from bem_for_nonlinear_waves_amd import _lib
-    from bem_for_nonlinear_waves_amd.bvp import BEM_BVP, device_name
+    from bem_for_nonlinear_waves_amd.bvp import device_name
 
-    def log(msg):
-        print(f"[bench] {msg}", file=sys.stderr, flush=True)
-
-    log(f"device {device_name()}")
+    _log(f"device {device_name()}")
     t = time.perf_counter()
     m, d, phi, phin = make_workload(args.h)
-    log(f"mesh: {m.n_nodes} nodes, {m.n_faces} faces, {d.n_dof} DOFs ({time.perf_counter() - t:.1f}s)")
-    pc = _lib.PRECOND_JACOBI if args.precond == "jacobi" else _lib.PRECOND_NONE
-    bvp = BEM_BVP(m, d, p=args.p, leaf_max=args.leaf, max_level=args.max_level, preconditioner=pc)
-    bvp.set_bc(phi, phin)
-    t = time.perf_counter()
-    bvp.prepare()
-    t_setup = time.perf_counter() - t
-    st = bvp.stats()
-    log(f"setup {t_setup:.2f}s: boxes={st['n_boxes']} leaves={st['n_leaves']} m2l_pairs={st['n_m2l_pairs']} "
-        f"near_nnz={st['near_nnz']} device={st['device_bytes']/1e9:.1f}GB "
-        f"(tree {st['t_tree']:.2f} lists {st['t_lists']:.2f} near_struct {st['t_near_struct']:.2f} "
-        f"p2m {st['t_p2m']:.2f} ops {st['t_ops']:.2f} upload {st['t_upload']:.2f} near_quad {st['t_near_quad']:.2f} "
-        f"alpha {st['t_alpha']:.2f})")
-
-    # ---- matvec throughput on device-resident vectors, per-stage HIP event timing ----
     n = d.n_dof
-    xs = torch.randn(n, dtype=torch.float64, device="cuda")
-    ys = torch.empty(n, dtype=torch.float64, device="cuda")
-    torch.cuda.synchronize()
-    bvp.profile()  # enable event recording on the library's stream
-    for _ in range(2):
-        bvp.matvec_device(xs.data_ptr(), ys.data_ptr())
-    bvp.sync()
-    t0 = time.perf_counter()
-    for _ in range(args.matvecs):
-        bvp.matvec_device(xs.data_ptr(), ys.data_ptr())
-    bvp.sync()
-    t_mv = (time.perf_counter() - t0) / args.matvecs
-    stage = np.zeros(7)
-    for _ in range(5):
-        bvp.matvec_device(xs.data_ptr(), ys.data_ptr())
-        stage += bvp.profile()  # synchronises on the last event of this matvec
-    stage /= 5
-    names = ["scatter", "p2m", "m2m", "m2l", "l2l", "near_l2p", "assemble"]
-    log("matvec %.3f ms (%.2f Mnode/s); stages ms: %s" % (t_mv * 1e3, m.n_nodes / t_mv / 1e6,
-        " ".join(f"{k}={v:.3f}" for k, v in zip(names, stage))))
+    _log(f"mesh: {m.n_nodes} nodes, {m.n_faces} faces, {n} DOFs ({time.perf_counter() - t:.1f}s)")
+    fp32 = args.precision == "fp32_refine"
+    bvp, st, t_setup = setup_bvp(m, d, phi, phin, args, args.precision)
+    t_mv64, stage64 = time_matvec(bvp, n, m.n_nodes, args.matvecs, False)
+    t_mv, stage = (time_matvec(bvp, n, m.n_nodes, args.matvecs, True) if fp32 else (t_mv64, stage64))
 
     if args.profile_only:
         cal = torch.empty(32 * 1024 * 1024, dtype=torch.float64, device="cuda")
@@ -155,54 +201,52 @@ def main():
         torch.cuda.synchronize()
         cal2.copy_(cal)  # 256 MiB read + 256 MiB written: FETCH_SIZE/WRITE_SIZE calibration
         torch.cuda.synchronize()
-        log("profile-only: done")
+        _log("profile-only: done")
         return
 
-    # ---- solves ----
-    lat = []
-    iters = []
-    x0 = np.zeros(n)
-    for i in range(args.warmup):
-        x, info = bvp.solveSystemGMRES(x0, tol=args.tol, max_iter=args.max_iter, restart=args.max_iter)
-        log(f"warmup solve {i}: iters={info['iters']} residual={info['residual']:.2e} {info['time_solve']:.2f}s")
-    torch.cuda.synchronize()
-    t_start = time.perf_counter()
-    for i in range(args.steps):
-        ts = time.perf_counter()
-        x, info = bvp.solveSystemGMRES(x0, tol=args.tol, max_iter=args.max_iter, restart=args.max_iter)
-        lat.append(time.perf_counter() - ts)
-        iters.append(info["iters"])
-        log(f"solve {i}: iters={info['iters']} matvecs={info['n_matvec']} residual={info['residual']:.2e} "
-            f"converged={info['converged']} {lat[-1]:.3f}s")
-    torch.cuda.synchronize()
-    t_total = time.perf_counter() - t_start
+    x, lat, infos, t_total = run_solves(bvp, n, args, args.precision)
     solves_per_s = args.steps / t_total
+    iters = [i["iters"] for i in infos]
 
-    # ---- roofline of the dominant kernel (from the live HIP-event stage timings) ----
+    fp64_cmp = None
+    if fp32 and not args.no_fp64_compare:
+        bvp.close()
+        b64, _, t_setup64 = setup_bvp(m, d, phi, phin, args, "fp64")
+        x64, lat64, infos64, t64 = run_solves(b64, n, args, "fp64")
+        b64.close()
+        rel = float(np.linalg.norm(x - x64) / np.linalg.norm(x64))
+        fp64_cmp = {"solves_per_s": args.steps / t64, "p50_solve_ms": float(np.percentile(np.array(lat64) * 1e3, 50)),
+                    "gmres_iters": [i["iters"] for i in infos64], "speedup": solves_per_s / (args.steps / t64),
+                    "rel_diff_solution_vs_fp64": rel, "matvec_ms": t_mv64 * 1e3}
+        _log(f"fp32_refine vs fp64: {fp64_cmp['speedup']:.2f}x solves/s, solution rel diff {rel:.2e}")
+
+    # ---- roofline of the dominant kernel of the matvec the solves run (live HIP-event stage timings) ----
     C = (args.p + 1) ** 2
+    esz = 4.0 if fp32 else 8.0
+    mfma_peak = 157.3 if fp32 else FP64_PEAK_TFLOPS
     m2l_flops = st["n_m2l_pairs"] * 2 * C * C * 2.0
-    near_rows = m.n_nodes
-    near_bytes = (st["near_nnz"] + near_rows * C) * 2 * 8.0  # dense G/Gn weights incl. L2P columns (FP64)
+    near_bytes = (st["near_nnz"] + m.n_nodes * C) * 2 * esz  # dense G/Gn weights incl. L2P columns
     dom = int(np.argmax(stage))
-    if names[dom] == "m2l":
+    if STAGES[dom] == "m2l":
         ach = m2l_flops / (stage[dom] * 1e-3) / 1e12
-        roof = {"kernel": "k_translate (M2L)", "bound": "mfma", "achieved": ach, "peak": FP64_PEAK_TFLOPS,
-                "unit": "TFLOP/s", "frac": ach / FP64_PEAK_TFLOPS, "traffic": None}
+        roof = {"kernel": "k_translate_mfma (M2L)", "bound": "mfma", "achieved": ach, "peak": mfma_peak,
+                "unit": "TFLOP/s", "frac": ach / mfma_peak, "traffic": None}
     else:
         ach = near_bytes / (stage[dom] * 1e-3) / 1e9
-        roof = {"kernel": f"k_block_gemv ({names[dom]})", "bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS,
+        roof = {"kernel": f"k_block_gemv ({STAGES[dom]})", "bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "traffic": None}
-    roof["stage_ms"] = dict(zip(names, [round(float(v), 4) for v in stage]))
+    roof["stage_ms"] = dict(zip(STAGES, [round(float(v), 4) for v in stage]))
     roof["m2l_tflops"] = m2l_flops / (stage[3] * 1e-3) / 1e12
     roof["near_gbs"] = near_bytes / (stage[5] * 1e-3) / 1e9
+    roof["fp64_stage_ms"] = dict(zip(STAGES, [round(float(v), 4) for v in stage64]))
 
     cpu = None
     if not args.no_cpu_baseline:
         try:
             cpu = cpu_baseline(args.p, args.leaf, args.max_level, int(np.median(iters)), m.n_nodes)
-            log(f"cpu baseline: {cpu['sample']}: {cpu['matvec_mnode_per_s']:.3f} Mnode/s")
+            _log(f"cpu baseline: {cpu['sample']}: {cpu['matvec_mnode_per_s']:.3f} Mnode/s")
         except Exception as e:  # the baseline never decides the GPU result
-            log(f"cpu baseline failed: {e}")
+            _log(f"cpu baseline failed: {e}")
 
     lat_ms = np.array(lat) * 1e3
     out = {
@@ -216,18 +260,23 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f32+f64" if fp32 else "f64",
         "data": "synthetic",
-        "config": {"workload": f"cylinder tank 20x4x2 m, R=0.5 draft=1, h={args.h} (configs[2] geometry), "
-                               f"FP64 FMM-GMRES", "n_nodes": m.n_nodes, "n_faces": m.n_faces, "n_dof": n,
+        "config": {"workload": f"cylinder tank 20x4x2 m, R=0.5 draft=1, h={args.h}, "
+                               + ("FP32 FMM-GMRES with FP64 iterative refinement" if fp32 else "FP64 FMM-GMRES"),
+                   "n_nodes": m.n_nodes, "n_faces": m.n_faces, "n_dof": n,
                    "p": args.p, "theta": 0.25, "leaf_max": args.leaf, "max_level": args.max_level, "tol": args.tol,
-                   "precision": "fp64", "preconditioner": args.precond},
+                   "precision": args.precision, "preconditioner": args.precond},
         "matvec_mnode_per_s": m.n_nodes / t_mv / 1e6,
         "matvec_ms": t_mv * 1e3,
+        "matvec_fp64_ms": t_mv64 * 1e3,
         "p50_solve_ms": float(np.percentile(lat_ms, 50)),
         "p99_solve_ms": float(np.percentile(lat_ms, 99)),
         "gmres_iters": iters,
+        "refinement_steps": [i["refinement_steps"] for i in infos],
+        "residual": [i["residual"] for i in infos],
         "setup_s": t_setup,
+        "fp64_path": fp64_cmp,
         "roofline": roof,
         "cpu_baseline": cpu,
     }

@@ -57,6 +57,7 @@ typedef struct bemfmm_opts {
   int q;              /* p2m_quadrature_points, default 6 (main_time_domain.cpp:62) */
   int precision;      /* BEMFMM_PRECISION_* */
   int preconditioner; /* BEMFMM_PRECOND_* */
+  double refine_eta;  /* FP32_REFINE: each inner FP32 solve reduces the FP64 residual by this factor (default 1e-5) */
 } bemfmm_opts;
 
 typedef struct bemfmm_result {
@@ -111,6 +112,13 @@ int bemfmm_prepare(bemfmm_ctx* ctx);
 int bemfmm_matvec(bemfmm_ctx* ctx, const double* x, double* y);
 /* Same on device pointers (inputs resident in HBM); ordered on the context's stream. */
 int bemfmm_matvec_device(bemfmm_ctx* ctx, const double* x_device, double* y_device);
+/* y = A_32 x: the same operator with the near field, P2M and M2M/M2L/L2L products in FP32
+ * (matrices rounded once from FP64; FP32 MFMA accumulation per translation), FP64 vectors and
+ * FP64 sums across boxes. Counterpart of the reference's single-precision Metal M2L
+ * (bem/metal_m2l/metal_m2l.h:23-129, used by BEM_solveBVP_GMRES_FMM.hpp:2246-2320).
+ * Requires a context created with BEMFMM_PRECISION_FP32_REFINE (else BEMFMM_E_STATE). */
+int bemfmm_matvec_fp32(bemfmm_ctx* ctx, const double* x, double* y);
+int bemfmm_matvec_fp32_device(bemfmm_ctx* ctx, const double* x_device, double* y_device);
 /* Synchronise the context's stream. */
 int bemfmm_sync(bemfmm_ctx* ctx);
 
@@ -130,6 +138,12 @@ int bemfmm_get_stats(const bemfmm_ctx* ctx, bemfmm_stats* stats);
 /* GMRES residual estimate |g_n| per iteration of the last solve (the reference prints it per
  * inner iteration, BEM_solveBVP_GMRES_FMM.hpp:2297-2300). Copies up to cap values; returns the count. */
 int bemfmm_residual_history(const bemfmm_ctx* ctx, double* out, int cap);
+
+/* FP64 residual |M^-1 (b - A x)|_2 at the start of each refinement step of the last FP32_REFINE solve
+ * (the last entry is the final residual). The reference's mixed-precision switch
+ * (BEM_solveBVP_GMRES_FMM.hpp:2246-2320) prints the true residual at its one switch point.
+ * Copies up to cap values; returns the count. */
+int bemfmm_refinement_history(const bemfmm_ctx* ctx, double* out, int cap);
 
 /* Kernel timing of the last matvec_device call (milliseconds, HIP events on the context's stream):
  * [0] scatter, [1] P2M, [2] M2M, [3] M2L, [4] L2L, [5] near+L2P, [6] assemble. Enables event
