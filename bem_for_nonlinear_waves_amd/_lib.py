@@ -17,13 +17,14 @@ _ip = ctypes.POINTER(ctypes.c_int32)
 OK = 0
 E_INVALID, E_DEVICE, E_OOM, E_NONFINITE, E_NOT_CONVERGED, E_STATE = -1, -2, -3, -4, -5, -6
 PRECISION_FP64, PRECISION_FP32_REFINE = 0, 1
-PRECOND_NONE, PRECOND_JACOBI = 0, 1
+PRECOND_NONE, PRECOND_JACOBI, PRECOND_SCHWARZ = 0, 1, 2
 
 
 class Opts(ctypes.Structure):
     _fields_ = [("p", ctypes.c_int), ("theta", ctypes.c_double), ("leaf_max", ctypes.c_int),
                 ("max_level", ctypes.c_int), ("q", ctypes.c_int), ("precision", ctypes.c_int),
-                ("preconditioner", ctypes.c_int), ("refine_eta", ctypes.c_double)]
+                ("preconditioner", ctypes.c_int), ("refine_eta", ctypes.c_double), ("schwarz_max_core", ctypes.c_int),
+                ("schwarz_max_block", ctypes.c_int), ("schwarz_overlap", ctypes.c_int)]
 
 
 class Result(ctypes.Structure):
@@ -39,7 +40,8 @@ class Stats(ctypes.Structure):
                 ("L", ctypes.c_double), ("device_bytes", ctypes.c_double), ("t_tree", ctypes.c_double),
                 ("t_lists", ctypes.c_double), ("t_near_struct", ctypes.c_double), ("t_p2m", ctypes.c_double),
                 ("t_ops", ctypes.c_double), ("t_upload", ctypes.c_double), ("t_near_quad", ctypes.c_double),
-                ("t_alpha", ctypes.c_double)]
+                ("t_alpha", ctypes.c_double), ("n_precond_blocks", ctypes.c_int),
+                ("precond_fallback_blocks", ctypes.c_int), ("precond_avg_block", ctypes.c_double)]
 
 
 # (name, restype, argtypes) of every entry point declared in include/bemfmm.h
@@ -67,6 +69,9 @@ SIGNATURES = [
     ("bemfmm_matvec_profile", ctypes.c_int, [ctypes.c_void_p, _dp]),
     ("bemfmm_residual_history", ctypes.c_int, [ctypes.c_void_p, _dp, ctypes.c_int]),
     ("bemfmm_refinement_history", ctypes.c_int, [ctypes.c_void_p, _dp, ctypes.c_int]),
+    ("bemfmm_precond_apply", ctypes.c_int, [ctypes.c_void_p, _dp, _dp]),
+    ("bemfmm_schwarz_blocks", ctypes.c_int,
+     [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64), _ip, _ip]),
 ]
 
 _lib = None

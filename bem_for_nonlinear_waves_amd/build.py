@@ -41,7 +41,7 @@ def build(verbose: bool = True) -> str:
     hipcc = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
     os.makedirs(OUT_DIR, exist_ok=True)
     objs = []
-    for src in ("plan.cpp", "operators.cpp"):
+    for src in ("plan.cpp", "operators.cpp", "schwarz.cpp"):
         obj = os.path.join(OUT_DIR, src.replace(".cpp", ".o"))
         _run(["g++", *CXX_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj])
         objs.append(obj)

@@ -49,12 +49,14 @@ class BEM_BVP:
 
     def __init__(self, mesh: Mesh, dofs: DofModel, p: int = 10, theta: float = 0.25, leaf_max: int = 50,
                  max_level: int = 7, q: int = 6, precision: int = _lib.PRECISION_FP64,
-                 preconditioner: int = _lib.PRECOND_NONE, refine_eta: float = 1e-5, device: int = 0):
+                 preconditioner: int = _lib.PRECOND_NONE, refine_eta: float = 1e-5, schwarz_max_core: int = 64,
+                 schwarz_max_block: int = 128, schwarz_overlap: int = 1, device: int = 0):
         self.lib = _lib.load()
         o = _lib.Opts()
         self.lib.bemfmm_default_opts(ctypes.byref(o))
         o.p, o.theta, o.leaf_max, o.max_level, o.q = p, theta, leaf_max, max_level, q
         o.precision, o.preconditioner, o.refine_eta = precision, preconditioner, refine_eta
+        o.schwarz_max_core, o.schwarz_max_block, o.schwarz_overlap = schwarz_max_core, schwarz_max_block, schwarz_overlap
         self.ctx = self.lib.bemfmm_ctx_create(device, ctypes.byref(o))
         if not self.ctx:
             raise BEMError(_lib.E_DEVICE, "bemfmm_ctx_create failed (no usable GPU?)")
@@ -139,6 +141,26 @@ class BEM_BVP:
         if n > 0:
             self.lib.bemfmm_residual_history(self.ctx, _ptr(out, _lib._dp), n)
         return out
+
+    def precond_apply(self, r) -> np.ndarray:
+        """z = M^-1 r with the context's preconditioner."""
+        r = _d(r)
+        z = np.empty(self.n_dof)
+        self._check(self.lib.bemfmm_precond_apply(self.ctx, _ptr(r, _lib._dp), _ptr(z, _lib._dp)))
+        return z
+
+    def schwarz_blocks(self):
+        """(ptr, idx, ncore) of the Schwarz blocks (empty without the Schwarz preconditioner)."""
+        nb = self.lib.bemfmm_schwarz_blocks(self.ctx, None, None, None)
+        if nb < 0:
+            self._check(nb)
+        ptr = np.zeros(nb + 1, np.int64)
+        ncore = np.zeros(max(nb, 1), np.int32)
+        self.lib.bemfmm_schwarz_blocks(self.ctx, ptr.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), None, None)
+        idx = np.zeros(max(int(ptr[-1]), 1), np.int32)
+        self.lib.bemfmm_schwarz_blocks(self.ctx, ptr.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
+                                       _ptr(idx, _lib._ip), _ptr(ncore, _lib._ip))
+        return ptr, idx[:int(ptr[-1])], ncore[:nb]
 
     def refinement_history(self) -> np.ndarray:
         n = self.lib.bemfmm_refinement_history(self.ctx, None, 0)

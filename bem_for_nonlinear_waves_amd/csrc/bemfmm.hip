@@ -493,6 +493,172 @@ __global__ void k_diag_inv(int n, const int32_t* __restrict__ kind, const int32_
   dinv[i] = (isfinite(dg) && fabs(dg) > 1e-15) ? 1.0 / dg : 1.0;
 }
 
+// ---- restricted additive Schwarz preconditioner ----
+// Operator entry (row DOF i, column DOF j) from the resident near field: the near row of i's point,
+// column j if it is in that row's near set (else 0); G for Dirichlet columns, -Gn otherwise;
+// -alpha_i on Neumann diagonals; CORNER rows are e_i. (buildSparseMatrixForILU, GMRES_FMM.hpp:440-640)
+__device__ double near_entry(int i, int j, const int32_t* __restrict__ kind, const int32_t* __restrict__ point,
+                             const int64_t* __restrict__ node_row, const int32_t* __restrict__ row_block,
+                             const int64_t* __restrict__ row_ptr, const int64_t* __restrict__ col_ptr,
+                             const int64_t* __restrict__ mat_ptr, const int32_t* __restrict__ col_idx,
+                             const int32_t* __restrict__ ncol_dof, const double* __restrict__ mat,
+                             const double* __restrict__ alpha) {
+  const int ki = kind[i];
+  if (ki == 2) return i == j ? 1.0 : 0.0;
+  const int64_t r = node_row[point[i]];
+  const int b = row_block[r];
+  const int64_t c0 = col_ptr[b];
+  const int ncol = (int)(col_ptr[b + 1] - c0);
+  const int ncd = ncol_dof[b];
+  int lo = 0, hi = ncd - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (col_idx[c0 + mid] < j) lo = mid + 1;
+    else hi = mid;
+  }
+  double v = 0.;
+  if (ncd > 0 && col_idx[c0 + lo] == j) {
+    const double* A0 = mat + mat_ptr[b] + (r - row_ptr[b]) * 2 * ncol;
+    v = (kind[j] == 0) ? A0[lo] : -A0[ncol + lo];
+  }
+  if (i == j && ki == 1) v -= alpha[i];
+  return v;
+}
+
+// One workgroup per block (m <= 128 DOFs): extract A_b into LDS, invert it in place by Gauss-Jordan
+// elimination with partial pivoting, and store the core rows of A_b^-1 (ncore x m, row-major).
+// A block with a vanishing or non-finite pivot falls back to the inverse diagonal on its core
+// (as SchwarzAdditive_CRS does when its block LU fails) and is flagged in failed[b].
+// LDS: m*m doubles (matrix) + m doubles (pivot column) + m ints (DOF ids) + m ints (pivot rows).
+constexpr int kSchwarzMaxBlock = 128;
+__global__ __launch_bounds__(256) void k_schwarz_build(
+    const int64_t* __restrict__ bptr, const int32_t* __restrict__ bidx, const int32_t* __restrict__ bncore,
+    const int64_t* __restrict__ inv_ptr, const int32_t* __restrict__ kind, const int32_t* __restrict__ point,
+    const int64_t* __restrict__ node_row, const int32_t* __restrict__ row_block, const int64_t* __restrict__ row_ptr,
+    const int64_t* __restrict__ col_ptr, const int64_t* __restrict__ mat_ptr, const int32_t* __restrict__ col_idx,
+    const int32_t* __restrict__ ncol_dof, const double* __restrict__ mat, const double* __restrict__ alpha,
+    double* __restrict__ inv, int32_t* __restrict__ failed) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int b = blockIdx.x;
+  const int64_t p0 = bptr[b];
+  const int m = (int)(bptr[b + 1] - p0);
+  const int nc = bncore[b];
+  double* A = reinterpret_cast<double*>(smem);   // m*m
+  double* colk = A + m * m;                      // m
+  int* ids = reinterpret_cast<int*>(colk + m);   // m
+  int* piv = ids + m;                            // m
+  __shared__ int bad;
+  const int tid = threadIdx.x;
+  if (tid == 0) bad = 0;
+  for (int t = tid; t < m; t += blockDim.x) ids[t] = bidx[p0 + t];
+  __syncthreads();
+  for (int e = tid; e < m * m; e += blockDim.x)
+    A[e] = near_entry(ids[e / m], ids[e % m], kind, point, node_row, row_block, row_ptr, col_ptr, mat_ptr, col_idx,
+                      ncol_dof, mat, alpha);
+  __syncthreads();
+  for (int k = 0; k < m; ++k) {
+    if (tid < 64) {
+      // pivot: largest |A[i][k]| over i >= k (lowest row on ties)
+      double best = -1.0;
+      int bi = k;
+      for (int i = k + tid; i < m; i += 64) {
+        const double v = fabs(A[i * m + k]);
+        if (v > best) {
+          best = v;
+          bi = i;
+        }
+      }
+      for (int off = 32; off > 0; off >>= 1) {
+        const double ob = __shfl_down(best, off);
+        const int oi = __shfl_down(bi, off);
+        if (ob > best || (ob == best && oi < bi)) {
+          best = ob;
+          bi = oi;
+        }
+      }
+      if (tid == 0) piv[k] = bi;
+    }
+    __syncthreads();
+    const int pr = piv[k];
+    if (pr != k)
+      for (int j = tid; j < m; j += blockDim.x) {
+        const double t = A[k * m + j];
+        A[k * m + j] = A[pr * m + j];
+        A[pr * m + j] = t;
+      }
+    __syncthreads();
+    const double pv = A[k * m + k];
+    if (!(fabs(pv) > 1e-300) || !isfinite(pv)) {
+      if (tid == 0) bad = 1;
+      break;  // uniform: every thread reads the same pivot
+    }
+    const double ipv = 1.0 / pv;
+    for (int i = tid; i < m; i += blockDim.x) colk[i] = A[i * m + k];
+    __syncthreads();
+    for (int j = tid; j < m; j += blockDim.x) A[k * m + j] = (j == k) ? ipv : A[k * m + j] * ipv;
+    __syncthreads();
+    for (int e = tid; e < m * m; e += blockDim.x) {
+      const int i = e / m, j = e % m;
+      if (i == k) continue;
+      A[e] = (j == k) ? -colk[i] * ipv : A[e] - colk[i] * A[k * m + j];
+    }
+    __syncthreads();
+  }
+  __syncthreads();
+  double* out = inv + inv_ptr[b];
+  if (bad) {
+    if (tid == 0) failed[b] = 1;
+    for (int e = tid; e < nc * m; e += blockDim.x) {
+      const int i = e / m, j = e % m;
+      double v = 0.;
+      if (i == j) {
+        const double d = near_entry(ids[i], ids[i], kind, point, node_row, row_block, row_ptr, col_ptr, mat_ptr,
+                                    col_idx, ncol_dof, mat, alpha);
+        v = (isfinite(d) && fabs(d) > 1e-15) ? 1.0 / d : 1.0;
+      }
+      out[e] = v;
+    }
+    return;
+  }
+  if (tid == 0) failed[b] = 0;
+  // undo the row interchanges: swap columns in reverse order
+  for (int k = m - 1; k >= 0; --k) {
+    const int pr = piv[k];
+    if (pr != k)
+      for (int i = tid; i < m; i += blockDim.x) {
+        const double t = A[i * m + k];
+        A[i * m + k] = A[i * m + pr];
+        A[i * m + pr] = t;
+      }
+    __syncthreads();
+  }
+  for (int e = tid; e < nc * m; e += blockDim.x) out[e] = A[e];
+}
+
+// z[core DOFs of b] = (A_b^-1 r|_b)[core]: one workgroup per block; each DOF is written by exactly
+// the one block whose core holds it (restricted additive Schwarz), so z needs no clearing or atomics.
+__global__ __launch_bounds__(256) void k_schwarz_apply(const int64_t* __restrict__ bptr, const int32_t* __restrict__ bidx,
+                                                       const int32_t* __restrict__ bncore, const int64_t* __restrict__ inv_ptr,
+                                                       const double* __restrict__ inv, const double* __restrict__ r,
+                                                       double* __restrict__ z) {
+  __shared__ double rl[kSchwarzMaxBlock];
+  const int b = blockIdx.x;
+  const int64_t p0 = bptr[b];
+  const int m = (int)(bptr[b + 1] - p0);
+  const int nc = bncore[b];
+  for (int t = threadIdx.x; t < m; t += blockDim.x) rl[t] = r[bidx[p0 + t]];
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const double* Ib = inv + inv_ptr[b];
+  for (int i = wave; i < nc; i += 4) {
+    const double* row = Ib + (int64_t)i * m;
+    double s = 0.;
+    for (int j = lane; j < m; j += 64) s += row[j] * rl[j];
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off);
+    if (lane == 0) z[bidx[p0 + i]] = s;
+  }
+}
+
 // ---- GMRES vector kernels (FP64) ----
 constexpr int kDotBlocks = 256;
 // partial[i * kDotBlocks + blk] = sum over the block's chunk of V_i . w   (grid: kDotBlocks x k)
@@ -658,7 +824,13 @@ struct bemfmm_ctx {
   DBuf<double> buf0, buf1;   // [phi_n | L0], [phi | L1]
   DBuf<double> M0, M1;
   DBuf<double> y0, y1;
-  DBuf<double> xin, yout, bvec, tmpv, xsol, xbase, jacobi, rvec, dvec;
+  DBuf<double> xin, yout, bvec, tmpv, xsol, xbase, jacobi, rvec, dvec, ptmp;
+  // restricted additive Schwarz: blocks and the core rows of the block inverses
+  bemfmm::SchwarzBlocks sw_host;
+  int sw_failed = 0;
+  DBuf<int64_t> sw_ptr, sw_invptr;
+  DBuf<int32_t> sw_idx, sw_ncore;
+  DBuf<double> sw_inv;
   DBuf<double> V;            // Krylov basis
   int64_t v_cap = 0;
   DBuf<double> partial, hbuf;
@@ -790,6 +962,10 @@ void do_prepare(bemfmm_ctx* c) {
   if (!c->have_bc) throw std::runtime_error("bemfmm_prepare: boundary values not set");
   if (c->opts.precision != BEMFMM_PRECISION_FP64 && c->opts.precision != BEMFMM_PRECISION_FP32_REFINE)
     throw std::runtime_error("bemfmm_prepare: unknown precision mode");
+  if (c->opts.preconditioner == BEMFMM_PRECOND_SCHWARZ &&
+      (c->opts.schwarz_max_core < 1 || c->opts.schwarz_max_block < c->opts.schwarz_max_core ||
+       c->opts.schwarz_max_block > kSchwarzMaxBlock || c->opts.schwarz_overlap < 0 || c->opts.schwarz_overlap > 8))
+    throw std::runtime_error("bemfmm_prepare: Schwarz block sizes need 1 <= max_core <= max_block <= 128, 0 <= overlap <= 8");
   const double t0 = now();
   bemfmm::Options o;
   o.p = c->opts.p;
@@ -863,7 +1039,7 @@ void do_prepare(bemfmm_ctx* c) {
   c->y0.alloc(c->n_nodes);
   c->y1.alloc(c->n_nodes);
   c->alpha.alloc(c->n_dof);
-  for (auto* b : {&c->xin, &c->yout, &c->bvec, &c->tmpv, &c->xsol, &c->xbase, &c->jacobi, &c->rvec, &c->dvec})
+  for (auto* b : {&c->xin, &c->yout, &c->bvec, &c->tmpv, &c->xsol, &c->xbase, &c->jacobi, &c->rvec, &c->dvec, &c->ptmp})
     b->alloc(c->n_dof);
   // quadrature inputs
   DBuf<int32_t> row_block, ncol_dof, face_perm, fv, fdof;
@@ -925,16 +1101,56 @@ void do_prepare(bemfmm_ctx* c) {
   const double t3 = now();
   // ---- rigid-mode alpha (updateGeometricProperties) ----
   evaluate(c, 2, nullptr, c->alpha.p);
-  if (c->opts.preconditioner == BEMFMM_PRECOND_JACOBI) {
+  const int pc = c->opts.preconditioner;
+  c->sw_host = bemfmm::SchwarzBlocks{};
+  for (auto* b : {&c->sw_inv}) b->release();
+  if (pc == BEMFMM_PRECOND_JACOBI || pc == BEMFMM_PRECOND_SCHWARZ) {
     std::vector<int64_t> node_row(c->n_nodes, -1);
     for (size_t r = 0; r < pl.near.row_out.size(); ++r) node_row[pl.near.row_out[r]] = (int64_t)r;
+    for (int i = 0; i < c->n_dof; ++i)
+      if (c->dof_kind[i] != 2 && node_row[c->dof_point[i]] < 0)
+        throw std::runtime_error("preconditioner: a DOF's point has no near-field row");
     DBuf<int64_t> d_node_row;
     d_node_row.upload(node_row, s);
-    hipLaunchKernelGGL(k_diag_inv, dim3(grid1(c->n_dof)), dim3(256), 0, s, c->n_dof, c->kind.p, c->point.p, d_node_row.p,
-                       row_block.p, c->near.row_ptr.p, c->near.col_ptr.p, c->near.mat_ptr.p, c->near.col_idx.p,
-                       ncol_dof.p, c->near.mat.p, c->alpha.p, c->jacobi.p);
-    HIPCHECK(hipGetLastError());
-  } else if (c->opts.preconditioner != BEMFMM_PRECOND_NONE) {
+    if (pc == BEMFMM_PRECOND_JACOBI) {
+      hipLaunchKernelGGL(k_diag_inv, dim3(grid1(c->n_dof)), dim3(256), 0, s, c->n_dof, c->kind.p, c->point.p, d_node_row.p,
+                         row_block.p, c->near.row_ptr.p, c->near.col_ptr.p, c->near.mat_ptr.p, c->near.col_idx.p,
+                         ncol_dof.p, c->near.mat.p, c->alpha.p, c->jacobi.p);
+      HIPCHECK(hipGetLastError());
+    } else {
+      // seeds in the tree order of the DOFs' points (near rows are grouped by target box)
+      std::vector<int32_t> seed(c->n_dof);
+      for (int i = 0; i < c->n_dof; ++i) seed[i] = i;
+      std::stable_sort(seed.begin(), seed.end(),
+                       [&](int32_t a, int32_t b) { return node_row[c->dof_point[a]] < node_row[c->dof_point[b]]; });
+      bemfmm::build_schwarz_blocks(c->n_dof, c->n_faces, c->face_dof.data(), c->dof_point.data(), seed,
+                                   c->opts.schwarz_max_core, c->opts.schwarz_max_block, c->opts.schwarz_overlap,
+                                   c->sw_host);
+      const bemfmm::SchwarzBlocks& B = c->sw_host;
+      const int nb = B.n_blocks();
+      if (B.max_block > kSchwarzMaxBlock) throw std::runtime_error("schwarz: block larger than the kernel's limit");
+      std::vector<int64_t> invptr(nb + 1, 0);
+      for (int b = 0; b < nb; ++b) invptr[b + 1] = invptr[b] + (int64_t)B.ncore[b] * (B.ptr[b + 1] - B.ptr[b]);
+      c->sw_ptr.upload(B.ptr, s);
+      c->sw_idx.upload(B.idx, s);
+      c->sw_ncore.upload(B.ncore, s);
+      c->sw_invptr.upload(invptr, s);
+      c->sw_inv.alloc((size_t)invptr[nb]);
+      DBuf<int32_t> failed;
+      failed.alloc(nb);
+      const size_t lds = (size_t)B.max_block * B.max_block * sizeof(double) + (size_t)B.max_block * 16;
+      hipLaunchKernelGGL(k_schwarz_build, dim3(nb), dim3(256), lds, s, c->sw_ptr.p, c->sw_idx.p, c->sw_ncore.p,
+                         c->sw_invptr.p, c->kind.p, c->point.p, d_node_row.p, row_block.p, c->near.row_ptr.p,
+                         c->near.col_ptr.p, c->near.mat_ptr.p, c->near.col_idx.p, ncol_dof.p, c->near.mat.p, c->alpha.p,
+                         c->sw_inv.p, failed.p);
+      HIPCHECK(hipGetLastError());
+      std::vector<int32_t> fh(nb);
+      HIPCHECK(hipMemcpyAsync(fh.data(), failed.p, nb * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+      HIPCHECK(hipStreamSynchronize(s));
+      c->sw_failed = 0;
+      for (int32_t f : fh) c->sw_failed += f;
+    }
+  } else if (pc != BEMFMM_PRECOND_NONE) {
     throw std::runtime_error("unknown preconditioner");
   }
   if (c->opts.precision == BEMFMM_PRECISION_FP32_REFINE) {
@@ -970,7 +1186,7 @@ void do_prepare(bemfmm_ctx* c) {
   st.L = pl.Lscale;
   st.device_bytes = (double)(c->near.bytes() + c->p2m.bytes() + c->m2l.bytes() + c->ops.bytes() + c->buf0.bytes() +
                              c->buf1.bytes() + c->M0.bytes() + c->M1.bytes() + c->m2l_stage.bytes() +
-                             c->near_mat32.bytes() + c->p2m_mat32.bytes() + c->ops32.bytes());
+                             c->near_mat32.bytes() + c->p2m_mat32.bytes() + c->ops32.bytes() + c->sw_inv.bytes());
   st.t_tree = pl.t_tree;
   st.t_lists = pl.t_lists;
   st.t_near_struct = pl.t_near_struct;
@@ -979,6 +1195,9 @@ void do_prepare(bemfmm_ctx* c) {
   st.t_upload = t2 - t1;
   st.t_near_quad = t3 - t2;
   st.t_alpha = t4 - t3;
+  st.n_precond_blocks = c->sw_host.n_blocks();
+  st.precond_fallback_blocks = c->sw_failed;
+  st.precond_avg_block = c->sw_host.n_blocks() ? (double)c->sw_host.idx.size() / c->sw_host.n_blocks() : 0.;
   (void)t0;
 }
 
@@ -1015,6 +1234,37 @@ double norm2(bemfmm_ctx* c, const double* w) {
   return std::sqrt(h[0]);
 }
 
+// out = M^-1 in for the context's preconditioner (left preconditioning, return_A_dot_v and the
+// preconditioned RHS, BEM_solveBVP_GMRES_FMM.hpp:2124-2171). in and out may alias except for Schwarz.
+void precond_apply(bemfmm_ctx* c, const double* in, double* out) {
+  const int64_t n = c->n_dof;
+  const int gN = grid1(n);
+  switch (c->opts.preconditioner) {
+  case BEMFMM_PRECOND_JACOBI:
+    if (in != out) HIPCHECK(hipMemcpyAsync(out, in, n * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    hipLaunchKernelGGL(k_mul, dim3(gN), dim3(256), 0, c->stream, n, c->jacobi.p, out);
+    break;
+  case BEMFMM_PRECOND_SCHWARZ:
+    if (in == out) throw std::runtime_error("internal: Schwarz apply needs distinct buffers");
+    hipLaunchKernelGGL(k_schwarz_apply, dim3(c->sw_host.n_blocks()), dim3(256), 0, c->stream, c->sw_ptr.p, c->sw_idx.p,
+                       c->sw_ncore.p, c->sw_invptr.p, c->sw_inv.p, in, out);
+    break;
+  default:
+    if (in != out) HIPCHECK(hipMemcpyAsync(out, in, n * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+  }
+}
+
+// w = M^-1 A v (return_A_dot_v); lo: FP32 operator
+void apply_op(bemfmm_ctx* c, const double* v, double* w, bool lo) {
+  if (c->opts.preconditioner == BEMFMM_PRECOND_SCHWARZ) {
+    evaluate(c, 0, v, c->ptmp.p, lo);
+    precond_apply(c, c->ptmp.p, w);
+  } else {
+    evaluate(c, 0, v, w, lo);
+    precond_apply(c, w, w);
+  }
+}
+
 // One GMRES run on  M^-1 A x = rhs  (rhs: device, already preconditioned), x (device) updated
 // in place. confirm = true follows run_gmres (BEM_solveBVP_GMRES_FMM.hpp:2275-2355): restart
 // cycles, an estimate <= 10 tol triggers a true-residual check, up to 5 saturated checks.
@@ -1036,11 +1286,7 @@ double gmres_run(bemfmm_ctx* c, const GmresRun& R, int& iters) {
   const int64_t n = c->n_dof;
   hipStream_t s = c->stream;
   const int gN = grid1(n);
-  const bool jac = c->opts.preconditioner == BEMFMM_PRECOND_JACOBI;
-  auto Av = [&](const double* v, double* w) {  // return_A_dot_v: M^-1 A v
-    evaluate(c, 0, v, w, R.lo);
-    if (jac) hipLaunchKernelGGL(k_mul, dim3(gN), dim3(256), 0, s, n, c->jacobi.p, w);
-  };
+  auto Av = [&](const double* v, double* w) { apply_op(c, v, w, R.lo); };
   double cur_err = 1e300;
   const int restart_max = std::max(1, R.restart);
   std::vector<double> h1, h2;
@@ -1168,9 +1414,7 @@ double gmres_run(bemfmm_ctx* c, const GmresRun& R, int& iters) {
 double fp64_residual(bemfmm_ctx* c, const double* x, double* out) {
   const int64_t n = c->n_dof;
   const int gN = grid1(n);
-  evaluate(c, 0, x, out, false);
-  if (c->opts.preconditioner == BEMFMM_PRECOND_JACOBI)
-    hipLaunchKernelGGL(k_mul, dim3(gN), dim3(256), 0, c->stream, n, c->jacobi.p, out);
+  apply_op(c, x, out, false);
   hipLaunchKernelGGL(k_sub, dim3(gN), dim3(256), 0, c->stream, n, c->bvec.p, out, out);
   return norm2(c, out);
 }
@@ -1188,10 +1432,9 @@ int do_solve(bemfmm_ctx* c, const double* x0, double* xout, double tol, int max_
   hipStream_t s = c->stream;
   const int n_mv0 = c->n_matvec;
   const int gN = grid1(n);
-  const bool jac = c->opts.preconditioner == BEMFMM_PRECOND_JACOBI;
   ensure_basis(c, 8);  // Krylov basis and the dot-product buffers (norms are taken before the first cycle)
-  evaluate(c, 1, nullptr, c->bvec.p);  // b, then M^-1 b (left preconditioning, GMRES_FMM.hpp:2157-2171)
-  if (jac) hipLaunchKernelGGL(k_mul, dim3(gN), dim3(256), 0, s, n, c->jacobi.p, c->bvec.p);
+  evaluate(c, 1, nullptr, c->ptmp.p);  // b, then M^-1 b (left preconditioning, GMRES_FMM.hpp:2157-2171)
+  precond_apply(c, c->ptmp.p, c->bvec.p);
   HIPCHECK(hipMemcpyAsync(c->xsol.p, x0, n * sizeof(double), hipMemcpyHostToDevice, s));
   c->history.clear();
   c->refine_history.clear();
@@ -1271,6 +1514,9 @@ void bemfmm_default_opts(bemfmm_opts* o) {
   o->precision = BEMFMM_PRECISION_FP64;
   o->preconditioner = BEMFMM_PRECOND_NONE;
   o->refine_eta = 1e-5;
+  o->schwarz_max_core = 64;   // main_time_domain.cpp:47
+  o->schwarz_max_block = 128; // main_time_domain.cpp:48
+  o->schwarz_overlap = 1;     // main_time_domain.cpp:46
 }
 
 bemfmm_ctx* bemfmm_ctx_create(int device, const bemfmm_opts* opts) {
@@ -1451,6 +1697,29 @@ int bemfmm_residual_history(const bemfmm_ctx* c, double* out, int cap) {
   if (out)
     for (int i = 0; i < std::min(n, cap); ++i) out[i] = c->history[i];
   return n;
+}
+
+int bemfmm_precond_apply(bemfmm_ctx* c, const double* r, double* z) {
+  if (!c) return BEMFMM_E_INVALID;
+  return guarded(c, [&]() {
+    if (!c->prepared) throw HipError("bemfmm_precond_apply: call bemfmm_prepare first", BEMFMM_E_STATE);
+    HIPCHECK(hipMemcpyAsync(c->xin.p, r, c->n_dof * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    precond_apply(c, c->xin.p, c->yout.p);
+    HIPCHECK(hipMemcpyAsync(z, c->yout.p, c->n_dof * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(hipStreamSynchronize(c->stream));
+    return BEMFMM_OK;
+  });
+}
+
+int bemfmm_schwarz_blocks(const bemfmm_ctx* c, int64_t* ptr, int32_t* idx, int32_t* ncore) {
+  if (!c) return BEMFMM_E_INVALID;
+  if (!c->prepared) return BEMFMM_E_STATE;
+  const auto& B = c->sw_host;
+  const int nb = B.n_blocks();
+  if (ptr) std::copy(B.ptr.begin(), B.ptr.end(), ptr);
+  if (idx) std::copy(B.idx.begin(), B.idx.end(), idx);
+  if (ncore) std::copy(B.ncore.begin(), B.ncore.end(), ncore);
+  return nb;
 }
 
 int bemfmm_refinement_history(const bemfmm_ctx* c, double* out, int cap) {

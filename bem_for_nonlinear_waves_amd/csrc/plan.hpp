@@ -102,3 +102,26 @@ inline int pk_re(int n, int m) { return m == 0 ? n * n : n * n + 2 * m - 1; }
 inline int pk_im(int n, int m) { return n * n + 2 * m; }
 
 }  // namespace bemfmm
+
+namespace bemfmm {
+
+// Overlapping DOF blocks of the additive Schwarz preconditioner (the reference's
+// SchwarzAdditive_CRS, basic_linear_systems.hpp:2870-3003, blocks built as in
+// BEM_solveBVP_GMRES_FMM.hpp:925-1010). Block b holds DOFs idx[ptr[b] .. ptr[b+1]);
+// its first ncore[b] entries are its core: every DOF lies in exactly one core.
+struct SchwarzBlocks {
+  std::vector<int64_t> ptr{0};
+  std::vector<int32_t> idx;
+  std::vector<int32_t> ncore;
+  int max_block = 0;
+  int n_blocks() const { return (int)ncore.size(); }
+};
+
+// Greedy partition of the DOF graph (DOFs sharing a face or a point are adjacent) into
+// breadth-first cores of at most max_core DOFs, seeded in the order of `seed_order`,
+// each grown by overlap_k rings of neighbours to at most max_block DOFs.
+void build_schwarz_blocks(int n_dof, int n_faces, const int32_t* face_dof, const int32_t* dof_point,
+                          const std::vector<int32_t>& seed_order, int max_core, int max_block, int overlap_k,
+                          SchwarzBlocks& out);
+
+}  // namespace bemfmm

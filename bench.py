@@ -88,7 +88,7 @@ def _log(msg):
 def setup_bvp(m, d, phi, phin, args, precision):
     from bem_for_nonlinear_waves_amd import _lib
     from bem_for_nonlinear_waves_amd.bvp import BEM_BVP
-    pc = _lib.PRECOND_JACOBI if args.precond == "jacobi" else _lib.PRECOND_NONE
+    pc = {"none": _lib.PRECOND_NONE, "jacobi": _lib.PRECOND_JACOBI, "schwarz": _lib.PRECOND_SCHWARZ}[args.precond]
     prec = _lib.PRECISION_FP32_REFINE if precision == "fp32_refine" else _lib.PRECISION_FP64
     bvp = BEM_BVP(m, d, p=args.p, leaf_max=args.leaf, max_level=args.max_level, preconditioner=pc, precision=prec,
                   refine_eta=args.refine_eta)
@@ -97,6 +97,9 @@ def setup_bvp(m, d, phi, phin, args, precision):
     bvp.prepare()
     t_setup = time.perf_counter() - t
     st = bvp.stats()
+    if st["n_precond_blocks"]:
+        _log(f"[{precision}] schwarz blocks={st['n_precond_blocks']} avg={st['precond_avg_block']:.1f} "
+             f"fallback={st['precond_fallback_blocks']}")
     _log(f"[{precision}] setup {t_setup:.2f}s: boxes={st['n_boxes']} leaves={st['n_leaves']} "
          f"m2l_pairs={st['n_m2l_pairs']} near_nnz={st['near_nnz']} device={st['device_bytes']/1e9:.1f}GB "
          f"(tree {st['t_tree']:.2f} lists {st['t_lists']:.2f} near_struct {st['t_near_struct']:.2f} "
@@ -166,7 +169,7 @@ def main():
     ap.add_argument("--tol", type=float, default=1e-10)
     ap.add_argument("--max-iter", type=int, default=2000)
     ap.add_argument("--matvecs", type=int, default=20)
-    ap.add_argument("--precond", choices=["none", "jacobi"], default="jacobi")
+    ap.add_argument("--precond", choices=["none", "jacobi", "schwarz"], default="jacobi")
     ap.add_argument("--precision", choices=["fp32_refine", "fp64"], default="fp32_refine")
     ap.add_argument("--refine-eta", type=float, default=1e-5)
     ap.add_argument("--no-fp64-compare", action="store_true",

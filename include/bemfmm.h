@@ -46,6 +46,8 @@ extern "C" {
 
 #define BEMFMM_PRECOND_NONE 0      /* reference default (main_time_domain.cpp:38) */
 #define BEMFMM_PRECOND_JACOBI 1    /* diagonal scaling (BEM_solveBVP_GMRES_FMM.hpp:2095-2104) */
+#define BEMFMM_PRECOND_SCHWARZ 2   /* overlapping-block Schwarz on the near field (SCHWARZ, GMRES_FMM.hpp:899-1013;
+                                      SchwarzAdditive_CRS, basic_linear_systems.hpp:2870-3003), restricted combination */
 
 typedef struct bemfmm_ctx bemfmm_ctx;
 
@@ -58,6 +60,9 @@ typedef struct bemfmm_opts {
   int precision;      /* BEMFMM_PRECISION_* */
   int preconditioner; /* BEMFMM_PRECOND_* */
   double refine_eta;  /* FP32_REFINE: each inner FP32 solve reduces the FP64 residual by this factor (default 1e-5) */
+  int schwarz_max_core;  /* schwarz_max_core_size, default 64 (main_time_domain.cpp:47) */
+  int schwarz_max_block; /* schwarz_max_block_size, default 128, at most 128 (main_time_domain.cpp:48) */
+  int schwarz_overlap;   /* schwarz_overlap_k: rings of neighbours added to a core, default 1 (main_time_domain.cpp:46) */
 } bemfmm_opts;
 
 typedef struct bemfmm_result {
@@ -79,6 +84,9 @@ typedef struct bemfmm_stats {
   double L;               /* coordinate scale (bounding-box diagonal) */
   double device_bytes;    /* resident device memory of the plan */
   double t_tree, t_lists, t_near_struct, t_p2m, t_ops, t_upload, t_near_quad, t_alpha;
+  int n_precond_blocks;         /* Schwarz: number of blocks */
+  int precond_fallback_blocks;  /* Schwarz: blocks whose elimination failed (diagonal used instead) */
+  double precond_avg_block;     /* Schwarz: mean block size */
 } bemfmm_stats;
 
 /* Availability (metal_m2l.h: metal_m2l_is_available / metal_m2l_device_name). */
@@ -138,6 +146,16 @@ int bemfmm_get_stats(const bemfmm_ctx* ctx, bemfmm_stats* stats);
 /* GMRES residual estimate |g_n| per iteration of the last solve (the reference prints it per
  * inner iteration, BEM_solveBVP_GMRES_FMM.hpp:2297-2300). Copies up to cap values; returns the count. */
 int bemfmm_residual_history(const bemfmm_ctx* ctx, double* out, int cap);
+
+/* z = M^-1 r with the context's preconditioner (none: copy; Jacobi: diagonal scaling; Schwarz: restricted
+ * additive Schwarz over overlapping DOF blocks) — the reference's preconditioner solve
+ * (ilu->solve / schwarz_preconditioner->solve, BEM_solveBVP_GMRES_FMM.hpp:2124-2171). n_dof entries. */
+int bemfmm_precond_apply(bemfmm_ctx* ctx, const double* r, double* z);
+
+/* Schwarz blocks of a prepared context: DOFs idx[ptr[b] .. ptr[b+1]) of block b, the first ncore[b] of them
+ * its core (every DOF lies in exactly one core). Any pointer may be NULL. Returns the number of blocks
+ * (0 without the Schwarz preconditioner); ptr needs n_blocks+1 entries, idx ptr[n_blocks]. */
+int bemfmm_schwarz_blocks(const bemfmm_ctx* ctx, int64_t* ptr, int32_t* idx, int32_t* ncore);
 
 /* FP64 residual |M^-1 (b - A x)|_2 at the start of each refinement step of the last FP32_REFINE solve
  * (the last entry is the final residual). The reference's mixed-precision switch

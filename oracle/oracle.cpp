@@ -958,16 +958,80 @@ struct Oracle {
     }
     return d;
   }
-  int precond = 0;
+  // ---- restricted additive Schwarz over given overlapping blocks (SchwarzAdditive_CRS,
+  // basic_linear_systems.hpp:2870-3003, with the sparse matrix of GMRES_FMM.hpp:440-640).
+  // A_b = operator restricted to block b with the near-field entries of each row (entries of
+  // DOFs outside the row point's near set are 0), -alpha_i on Neumann diagonals, CORNER rows e_i.
+  // Each block is solved by partial-pivot LU; z takes the block's solution on the block's core.
+  std::vector<int64_t> sw_ptr{0};
+  std::vector<int32_t> sw_idx, sw_ncore;
+  double near_entry(int i, int j) const {
+    if (dof_kind[i] == 2) return i == j ? 1.0 : 0.0;
+    const int t = dof_point[i];
+    const auto& ix = near_idx[t];
+    const size_t k = std::lower_bound(ix.begin(), ix.end(), j) - ix.begin();
+    double v = 0.;
+    if (k < ix.size() && ix[k] == j) v = dof_kind[j] == 0 ? near_wG[t][k] : -near_wGn[t][k];
+    if (i == j && dof_kind[i] == 1) v -= alpha[i];
+    return v;
+  }
+  void schwarz_apply(const double* r, double* z) const {
+    for (int i = 0; i < n_dof; ++i) z[i] = 0.;
+    const int nb = (int)sw_ncore.size();
+    for (int b = 0; b < nb; ++b) {
+      const int64_t p0 = sw_ptr[b];
+      const int m = (int)(sw_ptr[b + 1] - p0);
+      const int32_t* id = &sw_idx[p0];
+      std::vector<double> A((size_t)m * m), y(m);
+      for (int i = 0; i < m; ++i) {
+        y[i] = r[id[i]];
+        for (int j = 0; j < m; ++j) A[(size_t)i * m + j] = near_entry(id[i], id[j]);
+      }
+      // LU with partial pivoting, forward/back substitution
+      for (int k = 0; k < m; ++k) {
+        int pr = k;
+        for (int i = k + 1; i < m; ++i)
+          if (std::abs(A[(size_t)i * m + k]) > std::abs(A[(size_t)pr * m + k])) pr = i;
+        if (pr != k) {
+          for (int j = 0; j < m; ++j) std::swap(A[(size_t)k * m + j], A[(size_t)pr * m + j]);
+          std::swap(y[k], y[pr]);
+        }
+        const double pv = A[(size_t)k * m + k];
+        for (int i = k + 1; i < m; ++i) {
+          const double f = A[(size_t)i * m + k] / pv;
+          A[(size_t)i * m + k] = f;
+          for (int j = k + 1; j < m; ++j) A[(size_t)i * m + j] -= f * A[(size_t)k * m + j];
+          y[i] -= f * y[k];
+        }
+      }
+      for (int i = m - 1; i >= 0; --i) {
+        double s = y[i];
+        for (int j = i + 1; j < m; ++j) s -= A[(size_t)i * m + j] * y[j];
+        y[i] = s / A[(size_t)i * m + i];
+      }
+      for (int i = 0; i < sw_ncore[b]; ++i) z[id[i]] = y[i];
+    }
+  }
+
+  int precond = 0;  // 0 none, 1 Jacobi, 2 restricted additive Schwarz (blocks set beforehand)
   int solve(const double* x0, double* xout, double tol, int max_iter, int restart, int* iters_out, double* res_out) {
     std::vector<double> b(n_dof);
     rhs(b.data());
     const std::vector<double> dinv = precond == 1 ? diag_inv() : std::vector<double>(n_dof, 1.0);
-    for (int i = 0; i < n_dof; ++i) b[i] *= dinv[i];
+    std::vector<double> tmp(n_dof);
+    auto apply_m = [&](std::vector<double>& v) {
+      if (precond == 2) {
+        schwarz_apply(v.data(), tmp.data());
+        v.swap(tmp);
+      } else {
+        for (int i = 0; i < n_dof; ++i) v[i] *= dinv[i];
+      }
+    };
+    apply_m(b);
     auto Av = [&](const std::vector<double>& v) {
       std::vector<double> y(n_dof);
       matvec(v.data(), y.data());
-      for (int i = 0; i < n_dof; ++i) y[i] *= dinv[i];
+      apply_m(y);
       return y;
     };
     auto nrm = [](const std::vector<double>& v) {
@@ -1141,6 +1205,17 @@ void oracle_get_stats(void* h, oracle_stats* s) {
   s->near_nnz = o->nnz;
   s->L = o->Lscale;
   s->scale = o->scale;
+}
+void oracle_set_schwarz(void* h, int n_blocks, const int64_t* ptr, const int32_t* idx, const int32_t* ncore) {
+  auto* o = static_cast<Oracle*>(h);
+  o->sw_ptr.assign(ptr, ptr + n_blocks + 1);
+  o->sw_idx.assign(idx, idx + ptr[n_blocks]);
+  o->sw_ncore.assign(ncore, ncore + n_blocks);
+}
+void oracle_schwarz_apply(void* h, const double* r, double* z) { static_cast<Oracle*>(h)->schwarz_apply(r, z); }
+void oracle_diag_inv(void* h, double* d) {
+  const auto v = static_cast<Oracle*>(h)->diag_inv();
+  std::copy(v.begin(), v.end(), d);
 }
 // near-field row of a target node (sorted DOF indices) — for structural parity
 int oracle_near_row(void* h, int t, int* idx_out, double* wg, double* wgn, int cap) {

@@ -53,6 +53,9 @@ def lib():
         L.oracle_get_stats.argtypes = [ctypes.c_void_p, ctypes.POINTER(OracleStats)]
         L.oracle_near_row.restype = ctypes.c_int
         L.oracle_near_row.argtypes = [ctypes.c_void_p, ctypes.c_int, _ip, _dp, _dp, ctypes.c_int]
+        L.oracle_set_schwarz.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_int64), _ip, _ip]
+        L.oracle_schwarz_apply.argtypes = [ctypes.c_void_p, _dp, _dp]
+        L.oracle_diag_inv.argtypes = [ctypes.c_void_p, _dp]
         L.oracle_sph_harmonics.argtypes = [_dp, ctypes.c_int, _dp, _dp]
         L.oracle_dR.argtypes = [_dp, _dp, ctypes.c_int, _dp, _dp]
         L.oracle_translate_chain.argtypes = [ctypes.c_int, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, ctypes.c_int, _dp]
@@ -114,7 +117,8 @@ class Oracle:
         return A
 
     def solve(self, x0=None, tol=1e-10, max_iter=1000, restart=1000, precond=0):
-        """GMRES as the reference's run_gmres; precond=1: left scaling by the operator diagonal."""
+        """GMRES as the reference's run_gmres; precond=1: left scaling by the operator diagonal;
+        precond=2: restricted additive Schwarz over the blocks given to set_schwarz."""
         x0 = np.zeros(self.n_dof) if x0 is None else _d(x0)
         x = np.empty(self.n_dof)
         it = ctypes.c_int(0)
@@ -127,6 +131,22 @@ class Oracle:
         s = OracleStats()
         lib().oracle_get_stats(self.h, ctypes.byref(s))
         return {k: getattr(s, k) for k, _ in OracleStats._fields_}
+
+    def set_schwarz(self, ptr, idx, ncore):
+        self._sw = [np.ascontiguousarray(ptr, np.int64), _i(idx), _i(ncore)]
+        lib().oracle_set_schwarz(self.h, len(self._sw[2]), self._sw[0].ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
+                                 _p(self._sw[1], _ip), _p(self._sw[2], _ip))
+
+    def schwarz_apply(self, r):
+        r = _d(r)
+        z = np.empty(self.n_dof)
+        lib().oracle_schwarz_apply(self.h, _p(r, _dp), _p(z, _dp))
+        return z
+
+    def diag_inv(self):
+        d = np.empty(self.n_dof)
+        lib().oracle_diag_inv(self.h, _p(d, _dp))
+        return d
 
     def near_row(self, t, cap=100000):
         idx = np.empty(cap, np.int32)

@@ -250,3 +250,108 @@ void emu_plan_only(int n_nodes, const double* X, int n_faces, const int32_t* tri
   times[4] = pl.t_ops;
 }
 }
+
+// ---- restricted additive Schwarz: blocking (product code) and a serial mirror of the GPU build/apply ----
+namespace {
+struct EmuSchwarz {
+  bemfmm::SchwarzBlocks B;
+  std::vector<std::vector<double>> inv;  // per block: core rows of A_b^-1 (ncore x m)
+};
+}  // namespace
+
+extern "C" {
+void* emu_schwarz_create(int n_dof, int n_faces, const int32_t* face_dof, const int32_t* dof_point, const int32_t* seed,
+                         int max_core, int max_block, int overlap) {
+  auto* s = new EmuSchwarz();
+  std::vector<int32_t> sd(seed, seed + n_dof);
+  bemfmm::build_schwarz_blocks(n_dof, n_faces, face_dof, dof_point, sd, max_core, max_block, overlap, s->B);
+  return s;
+}
+void emu_schwarz_destroy(void* h) { delete static_cast<EmuSchwarz*>(h); }
+int emu_schwarz_sizes(void* h, int64_t* n_idx) {
+  auto* s = static_cast<EmuSchwarz*>(h);
+  *n_idx = (int64_t)s->B.idx.size();
+  return s->B.n_blocks();
+}
+void emu_schwarz_get(void* h, int64_t* ptr, int32_t* idx, int32_t* ncore) {
+  auto* s = static_cast<EmuSchwarz*>(h);
+  std::copy(s->B.ptr.begin(), s->B.ptr.end(), ptr);
+  std::copy(s->B.idx.begin(), s->B.idx.end(), idx);
+  std::copy(s->B.ncore.begin(), s->B.ncore.end(), ncore);
+}
+// Mirror of k_schwarz_build + k_schwarz_apply: entries looked up in the block-dense near matrix as the
+// kernel does, in-place Gauss-Jordan with partial pivoting and column unscrambling, core rows applied.
+void emu_schwarz_apply(void* he, void* hs, const double* r, double* z) {
+  auto* e = static_cast<Emu*>(he);
+  auto* s = static_cast<EmuSchwarz*>(hs);
+  const auto& G = e->pl.near;
+  const auto& kind = e->pl.dof_kind;
+  std::vector<int64_t> node_row(e->pl.n_nodes, -1);
+  std::vector<int32_t> row_block(G.row_out.size());
+  for (int b = 0; b < G.n_blocks(); ++b)
+    for (int64_t rr = G.row_ptr[b]; rr < G.row_ptr[b + 1]; ++rr) {
+      node_row[G.row_out[rr]] = rr;
+      row_block[rr] = b;
+    }
+  auto entry = [&](int i, int j) {
+    if (kind[i] == 2) return i == j ? 1.0 : 0.0;
+    const int64_t rr = node_row[e->pl.dof_point[i]];
+    const int b = row_block[rr];
+    const int64_t c0 = G.col_ptr[b];
+    const int ncol = (int)(G.col_ptr[b + 1] - c0);
+    const int ncd = e->pl.near_ncol_dof[b];
+    int lo = 0, hi = ncd - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if (G.col_idx[c0 + mid] < j) lo = mid + 1;
+      else hi = mid;
+    }
+    double v = 0.;
+    if (ncd > 0 && G.col_idx[c0 + lo] == j) {
+      const double* A0 = &e->near_mat[G.mat_ptr[b] + (rr - G.row_ptr[b]) * 2 * ncol];
+      v = kind[j] == 0 ? A0[lo] : -A0[ncol + lo];
+    }
+    if (i == j && kind[i] == 1) v -= e->alpha[i];
+    return v;
+  };
+  const auto& B = s->B;
+  for (int i = 0; i < e->pl.n_dof; ++i) z[i] = 0.;
+  for (int b = 0; b < B.n_blocks(); ++b) {
+    const int64_t p0 = B.ptr[b];
+    const int m = (int)(B.ptr[b + 1] - p0);
+    const int nc = B.ncore[b];
+    const int32_t* ids = &B.idx[p0];
+    std::vector<double> A((size_t)m * m), colk(m);
+    std::vector<int> piv(m);
+    for (int q = 0; q < m * m; ++q) A[q] = entry(ids[q / m], ids[q % m]);
+    for (int k = 0; k < m; ++k) {
+      double best = -1.;
+      int bi = k;
+      for (int i = k; i < m; ++i)
+        if (std::fabs(A[i * m + k]) > best) {
+          best = std::fabs(A[i * m + k]);
+          bi = i;
+        }
+      piv[k] = bi;
+      if (bi != k)
+        for (int j = 0; j < m; ++j) std::swap(A[k * m + j], A[bi * m + j]);
+      const double ipv = 1.0 / A[k * m + k];
+      for (int i = 0; i < m; ++i) colk[i] = A[i * m + k];
+      for (int j = 0; j < m; ++j) A[k * m + j] = (j == k) ? ipv : A[k * m + j] * ipv;
+      for (int q = 0; q < m * m; ++q) {
+        const int i = q / m, j = q % m;
+        if (i == k) continue;
+        A[q] = (j == k) ? -colk[i] * ipv : A[q] - colk[i] * A[k * m + j];
+      }
+    }
+    for (int k = m - 1; k >= 0; --k)
+      if (piv[k] != k)
+        for (int i = 0; i < m; ++i) std::swap(A[i * m + k], A[i * m + piv[k]]);
+    for (int i = 0; i < nc; ++i) {
+      double acc = 0.;
+      for (int j = 0; j < m; ++j) acc += A[i * m + j] * r[ids[j]];
+      z[ids[i]] = acc;
+    }
+  }
+}
+}
