@@ -5,15 +5,20 @@ Workload (one GPU): 20 x 4 x 2 m tank with a surface-piercing vertical cylinder
 free surface Dirichlet (seeded 8-mode phi), walls/bottom Neumann phi_n = 0, body
 Neumann phi_n = n.U with U ~ N(0, 0.1) (seed 3); p = 10, theta = 0.25,
 leaf <= 32 faces, depth 7 (reference default), GMRES tol 1e-10 (absolute, on the
-left-preconditioned residual as in the reference), Jacobi scaling by the operator
-diagonal (without it GMRES does not reach 1e-10 within 2000 iterations at this size).
+left-preconditioned residual as in the reference).
 
-A "step" is one BVP solve with the plan (tree, operators, near field, alpha)
-resident in HBM: RHS + GMRES to tolerance, every matvec on the GPU. `value` is
-solves/s over the timed steps; matvec Mnode/s (N / t_matvec) is timed separately
-on device-resident vectors. Setup time (plan + near quadrature + alpha) is
-reported beside it. The CPU baseline is the FP64 oracle restatement of the
-reference algorithm on a bounded sample (see cpu_baseline.sample).
+Default solver: FP32 near field / P2M / translations (FP32 MFMA) inside FP64
+iterative refinement, restricted additive Schwarz preconditioner (blocks of <= 128
+DOFs around cores of <= 64, the reference's SCHWARZ defaults). The FP64 path with
+the same preconditioner is timed beside it (`fp64_path`); `--precond jacobi` gives
+the diagonal scaling (about 4x the iterations at this size).
+
+A "step" is one BVP solve with the plan (tree, operators, near field, alpha,
+preconditioner) resident in HBM: RHS + GMRES to tolerance, every matvec on the GPU.
+`value` is solves/s over the timed steps; matvec Mnode/s (N / t_matvec) is timed
+separately on device-resident vectors. Setup time (plan + near quadrature + alpha +
+preconditioner) is reported beside it. The CPU baseline is the FP64 oracle
+restatement of the reference algorithm on a bounded sample (see cpu_baseline.sample).
 """
 from __future__ import annotations
 
@@ -53,8 +58,10 @@ def cpu_baseline(p, leaf_max, max_level, gpu_iters, n_nodes_full, budget_s=25.0)
     m, d, phi, phin = make_workload(0.12)
     t = time.perf_counter()
     o = O.Oracle(m, d, phi, phin, p=p, leaf_max=leaf_max, max_level=max_level)
-    t_setup = time.perf_counter() - t
+    o.set_m2l_cache(True)
     x = np.random.default_rng(0).standard_normal(d.n_dof)
+    o.matvec(x)  # builds the per-offset M2L coefficients (setup)
+    t_setup = time.perf_counter() - t
     n = 0
     t0 = time.perf_counter()
     while True:
@@ -73,7 +80,8 @@ def cpu_baseline(p, leaf_max, max_level, gpu_iters, n_nodes_full, budget_s=25.0)
         model = "unknown"
     return {"value": 1.0 / t_solve_full, "unit": "solves/s", "cores": threads, "kind": "port",
             "matvec_mnode_per_s": mnode, "cpu": model,
-            "sample": (f"oracle FP64 FMM matvec x{n} on the same tank family at h=0.12 ({m.n_nodes} nodes, "
+            "sample": (f"oracle FP64 FMM matvec (M2L coefficients cached per level and offset) x{n} on the "
+                       f"same tank family at h=0.12 ({m.n_nodes} nodes, "
                        f"setup {t_setup:.1f}s, {tm*1e3:.1f} ms/matvec); solves/s extrapolated linearly in N to "
                        f"{n_nodes_full} nodes at the GPU's {gpu_iters}+3 matvecs per solve")}
 
@@ -169,7 +177,7 @@ def main():
     ap.add_argument("--tol", type=float, default=1e-10)
     ap.add_argument("--max-iter", type=int, default=2000)
     ap.add_argument("--matvecs", type=int, default=20)
-    ap.add_argument("--precond", choices=["none", "jacobi", "schwarz"], default="jacobi")
+    ap.add_argument("--precond", choices=["none", "jacobi", "schwarz"], default="schwarz")
     ap.add_argument("--precision", choices=["fp32_refine", "fp64"], default="fp32_refine")
     ap.add_argument("--refine-eta", type=float, default=1e-5)
     ap.add_argument("--no-fp64-compare", action="store_true",

@@ -261,6 +261,7 @@ struct Bucket {
   std::vector<int> data;  // data1D_vector: all descendant faces (insertion order)
   double off = 0.;        // max_internal_offset
   std::vector<Bucket*> m2l_list, near, l2m;
+  std::vector<int> l2m_op;  // cached-M2L mode: operator index per l2m entry
   std::vector<cd> M0, M1, L0, L1;
   int id = -1;
   bool has_children() const {
@@ -741,12 +742,84 @@ struct Oracle {
       }
     }
   }
+  // Cached-M2L mode (CPU baseline at large N): the per-pair coefficients a = AAA_M2L * I_{j+n}^{m-k}(d)
+  // that SimpleM2L stores per pair (moments.hpp:493-598) are computed once per (level, lattice offset)
+  // from a representative pair and reused by every pair with that offset; the sums are unchanged.
+  bool m2l_cached = false;
+  std::vector<std::vector<cd>> m2l_ops;  // [(j,k>=0) row][(n,m) col]
+  void m2l_coefficients(const V3& d, std::vector<cd>& out) const {
+    const int C = (P + 1) * (P + 1);
+    Harm H;
+    std::vector<cd> Iv((2 * P + 1) * (2 * P + 1));
+    H.eval(d, 2 * P);
+    for (int l = 0; l <= 2 * P; ++l) {
+      const double rl = 1.0 / (std::pow(H.rho, l) * H.rho);
+      for (int qq = -l; qq <= l; ++qq) Iv[idx(l, qq)] = H.Y[idx(l, qq)] * rl;
+    }
+    out.assign((size_t)(P + 1) * (P + 2) / 2 * C, cd(0.));
+    int row = 0;
+    for (int j = 0; j <= P; ++j)
+      for (int k = 0; k <= j; ++k, ++row)
+        for (int n = 0; n <= P; ++n)
+          for (int m = -n; m <= n; ++m)
+            out[(size_t)row * C + idx(n, m)] = T.m2l[(size_t)idx(j, k) * C + idx(n, m)] * Iv[idx(j + n, m - k)];
+  }
+  void build_m2l_cache(const std::vector<Bucket*>& all) {
+    std::map<std::array<long long, 4>, int> key;
+    std::vector<V3> rep;
+    for (Bucket* A : all) {
+      A->l2m_op.clear();
+      const double side = A->bmax[0] - A->bmin[0];
+      for (Bucket* B : A->l2m) {
+        const V3 d = B->X - A->X;
+        const std::array<long long, 4> k{A->level, std::llround(d[0] / side), std::llround(d[1] / side),
+                                         std::llround(d[2] / side)};
+        auto it = key.find(k);
+        if (it == key.end()) {
+          it = key.emplace(k, (int)rep.size()).first;
+          rep.push_back(d);
+        }
+        A->l2m_op.push_back(it->second);
+      }
+    }
+    m2l_ops.assign(rep.size(), {});
+#pragma omp parallel for schedule(dynamic, 1)
+    for (size_t i = 0; i < rep.size(); ++i) m2l_coefficients(rep[i], m2l_ops[i]);
+  }
   void m2l() {
     // SimpleM2L (moments.hpp:493-598, 732-764): L_j^k += sum AAA_M2L * I_{j+n}^{m-k}(s-t) * M_n^m(s)
     const int C = (P + 1) * (P + 1);
     std::vector<Bucket*> all;
     for (auto& lv : level_buckets)
       for (auto* b : lv) all.push_back(b);
+    if (m2l_cached) {
+      if (m2l_ops.empty()) build_m2l_cache(all);
+#pragma omp parallel for schedule(dynamic, 1)
+      for (size_t ai = 0; ai < all.size(); ++ai) {
+        Bucket* A = all[ai];
+        std::fill(A->L0.begin(), A->L0.end(), cd(0.));
+        std::fill(A->L1.begin(), A->L1.end(), cd(0.));
+        for (size_t q = 0; q < A->l2m.size(); ++q) {
+          const Bucket* B = A->l2m[q];
+          const cd* a = m2l_ops[A->l2m_op[q]].data();
+          int row = 0;
+          for (int j = 0; j <= P; ++j)
+            for (int k = 0; k <= j; ++k, ++row) {
+              cd s0 = 0., s1 = 0.;
+              const cd* ar = a + (size_t)row * C;
+              for (int c = 0; c < C; ++c) {
+                s0 += ar[c] * B->M0[c];
+                s1 += ar[c] * B->M1[c];
+              }
+              A->L0[idx(j, k)] += s0;
+              A->L1[idx(j, k)] += s1;
+            }
+        }
+        conj_fill(A->L0);
+        conj_fill(A->L1);
+      }
+      return;
+    }
     // per-(exact offset) operator cache for large problems (m2l_mode 1)
 #pragma omp parallel for schedule(dynamic, 1)
     for (size_t ai = 0; ai < all.size(); ++ai) {
@@ -1205,6 +1278,11 @@ void oracle_get_stats(void* h, oracle_stats* s) {
   s->near_nnz = o->nnz;
   s->L = o->Lscale;
   s->scale = o->scale;
+}
+void oracle_set_m2l_cache(void* h, int on) {
+  auto* o = static_cast<Oracle*>(h);
+  o->m2l_cached = on != 0;
+  o->m2l_ops.clear();
 }
 void oracle_set_schwarz(void* h, int n_blocks, const int64_t* ptr, const int32_t* idx, const int32_t* ncore) {
   auto* o = static_cast<Oracle*>(h);

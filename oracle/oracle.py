@@ -53,6 +53,7 @@ def lib():
         L.oracle_get_stats.argtypes = [ctypes.c_void_p, ctypes.POINTER(OracleStats)]
         L.oracle_near_row.restype = ctypes.c_int
         L.oracle_near_row.argtypes = [ctypes.c_void_p, ctypes.c_int, _ip, _dp, _dp, ctypes.c_int]
+        L.oracle_set_m2l_cache.argtypes = [ctypes.c_void_p, ctypes.c_int]
         L.oracle_set_schwarz.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_int64), _ip, _ip]
         L.oracle_schwarz_apply.argtypes = [ctypes.c_void_p, _dp, _dp]
         L.oracle_diag_inv.argtypes = [ctypes.c_void_p, _dp]
@@ -131,6 +132,10 @@ class Oracle:
         s = OracleStats()
         lib().oracle_get_stats(self.h, ctypes.byref(s))
         return {k: getattr(s, k) for k, _ in OracleStats._fields_}
+
+    def set_m2l_cache(self, on: bool):
+        """Per-(level, lattice offset) M2L coefficients computed once (CPU-baseline mode for large N)."""
+        lib().oracle_set_m2l_cache(self.h, int(bool(on)))
 
     def set_schwarz(self, ptr, idx, ncore):
         self._sw = [np.ascontiguousarray(ptr, np.int64), _i(idx), _i(ncore)]

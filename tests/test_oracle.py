@@ -244,3 +244,15 @@ def test_planted_sign_error_is_caught(oracle_mod):
     x, _ = o.solve(tol=1e-8, max_iter=400)
     ex = np.where(d.dof_kind == 0, phin_ex, phi_ex)
     assert np.linalg.norm(x - ex) / np.linalg.norm(ex) > 1e-2
+
+
+def test_cached_m2l_matches_per_pair(oracle_mod, small_tank):
+    """The CPU-baseline mode (M2L coefficients cached per level and lattice offset) evaluates the
+    same operator as the per-pair SimpleM2L evaluation."""
+    m, d, phi, phin = small_tank
+    o = oracle_mod.Oracle(m, d, phi, phin, p=8, leaf_max=32)
+    x = np.random.default_rng(9).standard_normal(d.n_dof)
+    y0 = o.matvec(x)
+    o.set_m2l_cache(True)
+    y1 = o.matvec(x)
+    assert np.linalg.norm(y1 - y0) <= 1e-13 * np.linalg.norm(y0)
