@@ -131,7 +131,7 @@ class BEM_BVP:
         return {k: getattr(s, k) for k, _ in _lib.Stats._fields_}
 
     def profile(self):
-        out = np.zeros(7)
+        out = np.zeros(8)
         self._check(self.lib.bemfmm_matvec_profile(self.ctx, _ptr(out, _lib._dp)))
         return out
 

@@ -95,31 +95,51 @@ __global__ void k_scatter(int mode, int n, const int32_t* __restrict__ kind, con
 }
 
 // Box-blocked dense GEMV (see bemfmm::BlockGemv). One workgroup per block; the gathered
-// inputs are staged through LDS in chunks; each wavefront streams kRPW rows at once and
-// issues all 2*kRPW*2 loads of two column steps before using them (rows past the end of
-// the block are clamped to its last row and their sums discarded), so that enough HBM
-// requests are in flight per lane.
+// inputs are staged through LDS in chunks; each wavefront streams kRPW rows at once with
+// 16-byte vector loads (double2 / float4: rows are padded to a multiple of 4 columns and start
+// 16-byte aligned) and issues all 2*kRPW*2 loads of two column steps before using them (rows
+// past the end of the block are clamped to its last row and their sums discarded), so that
+// enough HBM requests are in flight per lane. Sums are accumulated in FP64.
 constexpr int kGemvChunk = 2048;
 constexpr int kGemvMaxRows = 512;
 constexpr int kRPW = 4;
+template <typename T>
+struct GemvVec;
+template <>
+struct GemvVec<double> {
+  typedef double2 type;
+  static constexpr int W = 2;
+  static __device__ __forceinline__ double get(const type& v, int i) { return i == 0 ? v.x : v.y; }
+};
+template <>
+struct GemvVec<float> {
+  typedef float4 type;
+  static constexpr int W = 4;
+  static __device__ __forceinline__ double get(const type& v, int i) {
+    return (double)(i == 0 ? v.x : i == 1 ? v.y : i == 2 ? v.z : v.w);
+  }
+};
 template <typename T>
 __global__ __launch_bounds__(256) void k_block_gemv(const int64_t* __restrict__ row_ptr, const int64_t* __restrict__ col_ptr,
                                                     const int64_t* __restrict__ mat_ptr, const int32_t* __restrict__ row_out,
                                                     const int32_t* __restrict__ col_idx, const T* __restrict__ mat,
                                                     const double* __restrict__ x0, const double* __restrict__ x1,
                                                     double* __restrict__ y0, double* __restrict__ y1) {
+  typedef typename GemvVec<T>::type V;
+  constexpr int W = GemvVec<T>::W;
+  constexpr int STEP = 64 * W;  // columns covered by one wave-wide vector load
   __shared__ double xs0[kGemvChunk], xs1[kGemvChunk];
   __shared__ double acc[2 * kGemvMaxRows];
   const int b = blockIdx.x;
   const int64_t r0 = row_ptr[b];
   const int nrow = (int)(row_ptr[b + 1] - r0);
   const int64_t c0 = col_ptr[b];
-  const int ncol = (int)(col_ptr[b + 1] - c0);
+  const int ncol = (int)(col_ptr[b + 1] - c0);  // multiple of 4
   const T* M = mat + mat_ptr[b];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   for (int i = threadIdx.x; i < 2 * nrow; i += blockDim.x) acc[i] = 0.;
   for (int cs = 0; cs < ncol; cs += kGemvChunk) {
-    const int cn = min(kGemvChunk, ncol - cs);
+    const int cn = min(kGemvChunk, ncol - cs);  // multiple of 4
     __syncthreads();
     const int32_t* ci = col_idx + c0 + cs;
     int i = threadIdx.x;
@@ -149,35 +169,46 @@ __global__ __launch_bounds__(256) void k_block_gemv(const int64_t* __restrict__ 
       double s0[kRPW], s1[kRPW];
 #pragma unroll
       for (int k = 0; k < kRPW; ++k) s0[k] = s1[k] = 0.;
-      int c = lane;
-      for (; c + 64 < cn; c += 128) {
-        double va[kRPW][2], vb[kRPW][2];
+      int c = lane * W;
+      for (; c + STEP < cn; c += 2 * STEP) {
+        V va[kRPW][2], vb[kRPW][2];
 #pragma unroll
         for (int k = 0; k < kRPW; ++k) {
-          va[k][0] = (double)Ar[k][c];
-          va[k][1] = (double)Ar[k][c + 64];
-          vb[k][0] = (double)Ar[k][ncol + c];
-          vb[k][1] = (double)Ar[k][ncol + c + 64];
+          va[k][0] = *reinterpret_cast<const V*>(Ar[k] + c);
+          va[k][1] = *reinterpret_cast<const V*>(Ar[k] + c + STEP);
+          vb[k][0] = *reinterpret_cast<const V*>(Ar[k] + ncol + c);
+          vb[k][1] = *reinterpret_cast<const V*>(Ar[k] + ncol + c + STEP);
         }
-        const double xa0 = xs0[c], xa1 = xs0[c + 64], xb0 = xs1[c], xb1 = xs1[c + 64];
+        double xa[2][W], xb[2][W];
 #pragma unroll
-        for (int k = 0; k < kRPW; ++k) {
-          s0[k] += va[k][0] * xa0 + va[k][1] * xa1;
-          s1[k] += vb[k][0] * xb0 + vb[k][1] * xb1;
+        for (int u = 0; u < W; ++u) {
+          xa[0][u] = xs0[c + u];
+          xa[1][u] = xs0[c + STEP + u];
+          xb[0][u] = xs1[c + u];
+          xb[1][u] = xs1[c + STEP + u];
         }
+#pragma unroll
+        for (int k = 0; k < kRPW; ++k)
+#pragma unroll
+          for (int u = 0; u < W; ++u) {
+            s0[k] += GemvVec<T>::get(va[k][0], u) * xa[0][u] + GemvVec<T>::get(va[k][1], u) * xa[1][u];
+            s1[k] += GemvVec<T>::get(vb[k][0], u) * xb[0][u] + GemvVec<T>::get(vb[k][1], u) * xb[1][u];
+          }
       }
       if (c < cn) {
-        double va[kRPW], vb[kRPW];
+        V va[kRPW], vb[kRPW];
 #pragma unroll
         for (int k = 0; k < kRPW; ++k) {
-          va[k] = (double)Ar[k][c];
-          vb[k] = (double)Ar[k][ncol + c];
+          va[k] = *reinterpret_cast<const V*>(Ar[k] + c);
+          vb[k] = *reinterpret_cast<const V*>(Ar[k] + ncol + c);
         }
 #pragma unroll
-        for (int k = 0; k < kRPW; ++k) {
-          s0[k] += va[k] * xs0[c];
-          s1[k] += vb[k] * xs1[c];
-        }
+        for (int k = 0; k < kRPW; ++k)
+#pragma unroll
+          for (int u = 0; u < W; ++u) {
+            s0[k] += GemvVec<T>::get(va[k], u) * xs0[c + u];
+            s1[k] += GemvVec<T>::get(vb[k], u) * xs1[c + u];
+          }
       }
 #pragma unroll
       for (int k = 0; k < kRPW; ++k) {
@@ -837,8 +868,7 @@ struct bemfmm_ctx {
   bemfmm_stats stats{};
   // profiling
   bool prof = false;
-  hipEvent_t ev[8] = {};
-  double prof_ms[7] = {};
+  hipEvent_t ev[9] = {};
   int n_matvec = 0;
   std::vector<double> history;         // GMRES residual estimate per iteration (last solve)
   std::vector<double> refine_history;  // FP64 residual at each refinement step (last solve, FP32_REFINE)
@@ -934,25 +964,27 @@ void evaluate(bemfmm_ctx* c, int mode, const double* in, double* out, bool lo = 
   if (lo) {
     float* st = reinterpret_cast<float*>(c->m2l_stage.p);
     launch_translate(c, c->m2l, c->ops32.p, c->M0.p, c->M1.p, nullptr, nullptr, st);
+    rec(4);
     hipLaunchKernelGGL(k_m2l_reduce<float>, dim3(c->n_boxes), dim3(256), 0, c->stream, c->C, c->m2l_tptr.p,
                        c->m2l_tpairs.p, (const float*)st, L0, L1);
   } else {
     launch_translate(c, c->m2l, c->ops.p, c->M0.p, c->M1.p, nullptr, nullptr, c->m2l_stage.p);
+    rec(4);
     hipLaunchKernelGGL(k_m2l_reduce<double>, dim3(c->n_boxes), dim3(256), 0, c->stream, c->C, c->m2l_tptr.p,
                        c->m2l_tpairs.p, (const double*)c->m2l_stage.p, L0, L1);
   }
-  rec(4);
+  rec(5);
   for (int l = 1; l < pl.n_levels; ++l) {
     if (lo) launch_translate(c, c->l2l[l], c->ops32.p, L0, L1, L0, L1);
     else launch_translate(c, c->l2l[l], c->ops.p, L0, L1, L0, L1);
   }
-  rec(5);
+  rec(6);
   if (lo) launch_gemv(c, c->near, c->near_mat32.p, c->buf0.p, c->buf1.p, c->y0.p, c->y1.p);
   else launch_gemv(c, c->near, c->near.mat.p, c->buf0.p, c->buf1.p, c->y0.p, c->y1.p);
-  rec(6);
+  rec(7);
   hipLaunchKernelGGL(k_assemble, dim3(grid1(n)), dim3(256), 0, c->stream, mode, n, c->kind.p, c->point.p, c->alpha.p, phi,
                      c->y0.p, c->y1.p, c->phi_k.p, out);
-  rec(7);
+  rec(8);
   HIPCHECK(hipGetLastError());
   c->n_matvec++;
 }
@@ -1737,20 +1769,20 @@ int bemfmm_get_stats(const bemfmm_ctx* c, bemfmm_stats* s) {
   return BEMFMM_OK;
 }
 
-int bemfmm_matvec_profile(bemfmm_ctx* c, double* ms7) {
-  if (!c || !ms7) return BEMFMM_E_INVALID;
+int bemfmm_matvec_profile(bemfmm_ctx* c, double* ms8) {
+  if (!c || !ms8) return BEMFMM_E_INVALID;
   return guarded(c, [&]() {
     if (!c->prof) {
       for (auto& e : c->ev) HIPCHECK(hipEventCreate(&e));
       c->prof = true;
-      for (int i = 0; i < 7; ++i) ms7[i] = 0.;
+      for (int i = 0; i < 8; ++i) ms8[i] = 0.;
       return BEMFMM_OK;
     }
-    HIPCHECK(hipEventSynchronize(c->ev[7]));
-    for (int i = 0; i < 7; ++i) {
+    HIPCHECK(hipEventSynchronize(c->ev[8]));
+    for (int i = 0; i < 8; ++i) {
       float ms = 0.f;
       HIPCHECK(hipEventElapsedTime(&ms, c->ev[i], c->ev[i + 1]));
-      ms7[i] = ms;
+      ms8[i] = ms;
     }
     return BEMFMM_OK;
   });

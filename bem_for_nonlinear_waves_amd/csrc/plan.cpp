@@ -22,6 +22,10 @@
 #include "plan.hpp"
 
 namespace bemfmm {
+
+// Block GEMV rows are padded to a multiple of 4 columns, so that every row starts 16-byte aligned
+// and the GEMV streams them with 16-byte vector loads.
+static inline int pad_cols(int n) { return (n + 3) & ~3; }
 namespace {
 
 using V3 = std::array<double, 3>;
@@ -515,7 +519,7 @@ void build_plan(Plan& pl, const Options& opt, int n_nodes, const double* X, int 
       const int b = tboxes[ti];
       const auto& tg = box_targets[b];
       const int ncd = (int)cols[ti].size();
-      const int ncol = ncd + C;
+      const int ncol = pad_cols(ncd + C);
       for (size_t r0 = 0; r0 < tg.size(); r0 += kMaxRows) {
         const size_t r1 = std::min(tg.size(), r0 + (size_t)kMaxRows);
         G.mat_ptr.push_back(G.mat_size);
@@ -524,6 +528,7 @@ void build_plan(Plan& pl, const Options& opt, int n_nodes, const double* X, int 
         G.row_ptr.push_back((int64_t)G.row_out.size());
         G.col_idx.insert(G.col_idx.end(), cols[ti].begin(), cols[ti].end());
         for (int c = 0; c < C; ++c) G.col_idx.push_back(n_dof + b * C + c);
+        for (int c = ncd + C; c < ncol; ++c) G.col_idx.push_back(n_dof + b * C + C - 1);  // zero-weight padding
         G.col_ptr.push_back((int64_t)G.col_idx.size());
         pl.near_ncol_dof.push_back(ncd);
         int64_t pre = 0;
@@ -584,6 +589,8 @@ void build_plan(Plan& pl, const Options& opt, int n_nodes, const double* X, int 
         for (int c = 0; c < 3; ++c) cl.push_back(face_dof[3 * f + c]);
       std::sort(cl.begin(), cl.end());
       cl.erase(std::unique(cl.begin(), cl.end()), cl.end());
+      // zero-weight padding (repeats the last column, so the list stays sorted and lower_bound finds the original)
+      cl.resize(pad_cols((int)cl.size()), cl.back());
       cols[li] = std::move(cl);
     }
     BlockGemv& G = pl.p2m;

@@ -86,7 +86,7 @@ def cpu_baseline(p, leaf_max, max_level, gpu_iters, n_nodes_full, budget_s=25.0)
                        f"{n_nodes_full} nodes at the GPU's {gpu_iters}+3 matvecs per solve")}
 
 
-STAGES = ["scatter", "p2m", "m2m", "m2l", "l2l", "near_l2p", "assemble"]
+STAGES = ["scatter", "p2m", "m2m", "m2l", "m2l_reduce", "l2l", "near_l2p", "assemble"]
 
 
 def _log(msg):
@@ -132,7 +132,7 @@ def time_matvec(bvp, n, n_nodes, reps, fp32):
         mv(xs.data_ptr(), ys.data_ptr())
     bvp.sync()
     t_mv = (time.perf_counter() - t0) / reps
-    stage = np.zeros(7)
+    stage = np.zeros(8)
     for _ in range(5):
         mv(xs.data_ptr(), ys.data_ptr())
         stage += bvp.profile()  # synchronises on the last event of this matvec
@@ -248,7 +248,7 @@ def main():
                 "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "traffic": None}
     roof["stage_ms"] = dict(zip(STAGES, [round(float(v), 4) for v in stage]))
     roof["m2l_tflops"] = m2l_flops / (stage[3] * 1e-3) / 1e12
-    roof["near_gbs"] = near_bytes / (stage[5] * 1e-3) / 1e9
+    roof["near_gbs"] = near_bytes / (stage[6] * 1e-3) / 1e9
     roof["fp64_stage_ms"] = dict(zip(STAGES, [round(float(v), 4) for v in stage64]))
 
     cpu = None

@@ -163,10 +163,10 @@ int bemfmm_schwarz_blocks(const bemfmm_ctx* ctx, int64_t* ptr, int32_t* idx, int
  * Copies up to cap values; returns the count. */
 int bemfmm_refinement_history(const bemfmm_ctx* ctx, double* out, int cap);
 
-/* Kernel timing of the last matvec_device call (milliseconds, HIP events on the context's stream):
- * [0] scatter, [1] P2M, [2] M2M, [3] M2L, [4] L2L, [5] near+L2P, [6] assemble. Enables event
- * recording on the first call. */
-int bemfmm_matvec_profile(bemfmm_ctx* ctx, double* ms7);
+/* Kernel timing of the last matvec (milliseconds, HIP events on the context's stream):
+ * [0] scatter, [1] P2M, [2] M2M, [3] M2L translations, [4] M2L per-box reduction, [5] L2L,
+ * [6] near+L2P, [7] assemble. Enables event recording on the first call (and then returns zeros). */
+int bemfmm_matvec_profile(bemfmm_ctx* ctx, double* ms8);
 
 #ifdef __cplusplus
 }
