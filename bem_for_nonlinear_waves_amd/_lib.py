@@ -70,6 +70,15 @@ SIGNATURES = [
     ("bemfmm_residual_history", ctypes.c_int, [ctypes.c_void_p, _dp, ctypes.c_int]),
     ("bemfmm_refinement_history", ctypes.c_int, [ctypes.c_void_p, _dp, ctypes.c_int]),
     ("bemfmm_precond_apply", ctypes.c_int, [ctypes.c_void_p, _dp, _dp]),
+    # include/bemfmm_server.h (structures in serve.py; passed here as opaque pointers)
+    ("bemfmm_server_default_opts", None, [ctypes.c_void_p]),
+    ("bemfmm_server_create", ctypes.c_void_p, [ctypes.c_void_p]),
+    ("bemfmm_server_destroy", None, [ctypes.c_void_p]),
+    ("bemfmm_server_submit", ctypes.c_int64, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64]),
+    ("bemfmm_server_wait", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, _dp, ctypes.c_void_p, _dp]),
+    ("bemfmm_server_close_session", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64]),
+    ("bemfmm_server_get_stats", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+    ("bemfmm_server_last_error", ctypes.c_char_p, [ctypes.c_void_p]),
     ("bemfmm_schwarz_blocks", ctypes.c_int,
      [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64), _ip, _ip]),
 ]

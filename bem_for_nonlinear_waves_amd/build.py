@@ -1,8 +1,10 @@
 """Builds libbemfmm.so in-tree for gfx950.
 
-Host plan code (plan.cpp, operators.cpp) is compiled with g++ (OpenMP via
-libgomp, __float128 via libquadmath); the HIP device code and C ABI
-(bemfmm.hip) with hipcc for gfx950; the three objects are linked with hipcc.
+Host code (plan.cpp, operators.cpp, schwarz.cpp: the plan, its float128
+operators and the Schwarz blocks; server.cpp: the request loop over the C ABI)
+is compiled with g++ (OpenMP via libgomp, __float128 via libquadmath, threads);
+the HIP device code and C ABI (bemfmm.hip) with hipcc for gfx950; the objects are
+linked with hipcc.
 Usage: python -m bem_for_nonlinear_waves_amd.build
 """
 from __future__ import annotations
@@ -24,7 +26,7 @@ HIP_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-result",
 ]
-CXX_FLAGS = ["-O3", "-fPIC", "-std=c++17", "-fopenmp", "-march=x86-64-v2", "-Wall", "-Wno-unused-function"]
+CXX_FLAGS = ["-O3", "-fPIC", "-std=c++17", "-fopenmp", "-pthread", "-march=x86-64-v2", "-Wall", "-Wno-unused-function"]
 
 
 def _gcc_libdir() -> str:
@@ -41,7 +43,7 @@ def build(verbose: bool = True) -> str:
     hipcc = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
     os.makedirs(OUT_DIR, exist_ok=True)
     objs = []
-    for src in ("plan.cpp", "operators.cpp", "schwarz.cpp"):
+    for src in ("plan.cpp", "operators.cpp", "schwarz.cpp", "server.cpp"):
         obj = os.path.join(OUT_DIR, src.replace(".cpp", ".o"))
         _run(["g++", *CXX_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj])
         objs.append(obj)
@@ -50,7 +52,7 @@ def build(verbose: bool = True) -> str:
     objs.append(obj)
     tmp = LIB + ".tmp"
     _run([hipcc, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs, f"-L{_gcc_libdir()}",
-          f"-Wl,-rpath,{_gcc_libdir()}", "-lgomp", "-lquadmath"])
+          f"-Wl,-rpath,{_gcc_libdir()}", "-lgomp", "-lquadmath", "-lpthread"])
     os.replace(tmp, LIB)
     return LIB
 

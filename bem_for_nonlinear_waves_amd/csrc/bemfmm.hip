@@ -63,8 +63,6 @@ __constant__ double c_gw5[25][3] = {
     {0.953089922969332, 0.046910077030668, 0.0140335872156063}, {0.953089922969332, 0.230765344947158, 0.0283499999999991},
     {0.953089922969332, 0.5, 0.0336962680968792},               {0.953089922969332, 0.769234655052841, 0.0283499999999991},
     {0.953089922969332, 0.953089922969332, 0.0140335872156063}};
-// Dunavant rule used for non-adjacent, non-near faces ({b0,b1,b2,w}); filled per context (q points).
-__constant__ double c_dun[7][4];
 
 // ---------------------------------------------------------------------------
 // kernels
@@ -447,7 +445,12 @@ __global__ __launch_bounds__(256) void k_near_quad(
     const int64_t* __restrict__ ranges, const int64_t* __restrict__ range_prefix, const int32_t* __restrict__ face_perm,
     const double* __restrict__ Xs, const double* __restrict__ fX, const double* __restrict__ fcent,
     const double* __restrict__ fcr, const double* __restrict__ fJ, const int32_t* __restrict__ fv,
-    const int32_t* __restrict__ fdof, const double* __restrict__ l2p, double* __restrict__ mat) {
+    const int32_t* __restrict__ fdof, const double* __restrict__ l2p, const double* __restrict__ dun_rule,
+    double* __restrict__ mat) {
+  // Dunavant rule of the non-adjacent, non-near faces ({b0,b1,b2,w} x q), per context
+  __shared__ double dun[7][4];
+  for (int i = threadIdx.x; i < 28; i += blockDim.x) dun[i / 4][i % 4] = dun_rule[i];
+  __syncthreads();
   const int64_t r = blockIdx.x;
   const int b = row_block[r];
   const int ncd = ncol_dof[b];
@@ -478,7 +481,7 @@ __global__ __launch_bounds__(256) void k_near_quad(
       if (fv[3 * f + k] == t) slot = k;
     double WG[3], WGn[3];
     bemfmm::face_weights(fX + 9 * (int64_t)f, fJ[f], fcr + 3 * (int64_t)f, fcent + 3 * (int64_t)f, slot, x, y, z,
-                         near_region, c_gw5, c_dun, q, WG, WGn);
+                         near_region, c_gw5, dun, q, WG, WGn);
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
       const int d = fdof[3 * f + k];
@@ -1076,7 +1079,7 @@ void do_prepare(bemfmm_ctx* c) {
   // quadrature inputs
   DBuf<int32_t> row_block, ncol_dof, face_perm, fv, fdof;
   DBuf<int64_t> range_ptr, ranges, range_prefix;
-  DBuf<double> Xs, fX, fcent, fcr, fJ, l2p;
+  DBuf<double> Xs, fX, fcent, fcr, fJ, l2p, d_dun;
   {
     std::vector<int32_t> rb(pl.near.row_out.size());
     for (int b = 0; b < pl.near.n_blocks(); ++b)
@@ -1116,7 +1119,7 @@ void do_prepare(bemfmm_ctx* c) {
     const double(*src)[4] = o.q == 1 ? d1 : o.q == 3 ? d3 : o.q == 6 ? d6 : d7;
     for (int g = 0; g < o.q; ++g)
       for (int a = 0; a < 4; ++a) dun[g][a] = src[g][a];
-    HIPCHECK(hipMemcpyToSymbolAsync(HIP_SYMBOL(c_dun), dun, sizeof(dun), 0, hipMemcpyHostToDevice, s));
+    d_dun.upload(std::vector<double>(&dun[0][0], &dun[0][0] + 28), s);
   }
   HIPCHECK(hipStreamSynchronize(s));
   const double t2 = now();
@@ -1127,7 +1130,7 @@ void do_prepare(bemfmm_ctx* c) {
     hipLaunchKernelGGL(k_near_quad, dim3((unsigned)nrows), dim3(256), 0, s, pl.C, o.q, pl.near_region, row_block.p,
                        c->near.row_ptr.p, c->near.col_ptr.p, c->near.mat_ptr.p, c->near.row_out.p, c->near.col_idx.p,
                        ncol_dof.p, range_ptr.p, ranges.p, range_prefix.p, face_perm.p, Xs.p, fX.p, fcent.p, fcr.p,
-                       fJ.p, fv.p, fdof.p, l2p.p, c->near.mat.p);
+                       fJ.p, fv.p, fdof.p, l2p.p, d_dun.p, c->near.mat.p);
   HIPCHECK(hipGetLastError());
   HIPCHECK(hipStreamSynchronize(s));
   const double t3 = now();

@@ -165,6 +165,165 @@ def run_solves(bvp, n, args, label):
     return x, lat, infos, t_total
 
 
+def make_batch_problem(rng, n_target: int):
+    """Tank of about n_target nodes: aspect ratio Lx/Ly ~ U[1, 8], a surface-piercing cylinder with
+    probability 1/4; free surface Dirichlet (seeded 8-mode phi), body phi_n = n.U."""
+    from bem_for_nonlinear_waves_amd import mesh as M
+    a = float(rng.uniform(1.0, 8.0))
+    seed = int(rng.integers(1 << 30))
+    if rng.uniform() < 0.25:
+        Lx, Ly, Lz = 2.0 * a, 2.0, 1.0
+        h = math.sqrt(2.0 * (Lx * Ly + Ly * Lz + Lx * Lz) / n_target)
+        m = M.cylinder_tank(Lx=Lx, Ly=Ly, Lz=Lz, R=0.25, draft=0.5, h=h)
+        sigma = 0.1
+    else:
+        Lx, Ly, Lz = a, 1.0, 1.0
+        h = math.sqrt(2.0 * (Lx * Ly + Ly * Lz + Lx * Lz) / n_target)
+        m = M.box_tank(Lx, Ly, Lz, max(2, round(Lx / h)), max(2, round(Ly / h)), max(2, round(Lz / h)))
+        sigma = 0.0
+    m = M.perturb_free_surface(m, seed, 0.05 * h)
+    d = M.build_dofs(m)
+    phi, phin = M.tank_bc(m, d, seed, Lx, Ly, body_velocity_sigma=sigma)
+    return m, d, phi, phin, h, seed, (Lx, Ly)
+
+
+def _solver_kw(args):
+    from bem_for_nonlinear_waves_amd import _lib
+    return dict(p=args.p, leaf_max=args.leaf, max_level=args.max_level, tol=args.tol, max_iter=args.max_iter,
+                restart=args.max_iter, refine_eta=args.refine_eta,
+                precision=_lib.PRECISION_FP32_REFINE if args.precision == "fp32_refine" else _lib.PRECISION_FP64,
+                preconditioner={"none": _lib.PRECOND_NONE, "jacobi": _lib.PRECOND_JACOBI,
+                                "schwarz": _lib.PRECOND_SCHWARZ}[args.precond])
+
+
+def run_batch(args):
+    """64 independent tank BVPs (N log-uniform in [2k, 40k]) submitted at once to the request loop;
+    every request includes its setup (new geometry). Also timed one at a time (one slot)."""
+    from bem_for_nonlinear_waves_amd.serve import BVPServer
+    rng = np.random.default_rng(4)
+    t = time.perf_counter()
+    sizes = np.exp(rng.uniform(np.log(2000), np.log(40000), args.batch)).astype(int)
+    probs = [make_batch_problem(rng, int(n)) for n in sizes]
+    n_nodes = [p[0].n_nodes for p in probs]
+    _log(f"batch: {len(probs)} tanks, nodes {min(n_nodes)}..{max(n_nodes)} (sum {sum(n_nodes)}), "
+         f"{sum(p[6][1] == 2.0 for p in probs)} with a cylinder ({time.perf_counter() - t:.1f}s)")
+    kw = _solver_kw(args)
+
+    def run(slots):
+        srv = BVPServer(slots=slots)
+        w = [srv.submit(m, d, phi, phin, **kw) for m, d, phi, phin, *_ in probs[:2]]  # warm-up (operator cache)
+        for tk in w:
+            srv.wait(tk)
+        srv.close()
+        srv = BVPServer(slots=slots)
+        t0 = time.perf_counter()
+        tickets = [srv.submit(m, d, phi, phin, **kw) for m, d, phi, phin, *_ in probs]
+        res = [srv.wait(tk) for tk in tickets]
+        wall = time.perf_counter() - t0
+        st = srv.stats()
+        srv.close()
+        conv = sum(int(i["converged"]) for _, i in res)
+        svc = np.array([i["service_s"] for _, i in res]) * 1e3
+        _log(f"batch slots={slots}: {wall:.2f}s ({len(probs) / wall:.2f} solves/s) converged {conv}/{len(probs)} "
+             f"latency p50 {st['p50_latency_s'] * 1e3:.0f} ms p99 {st['p99_latency_s'] * 1e3:.0f} ms, "
+             f"service p50 {np.percentile(svc, 50):.0f} ms")
+        return wall, st, svc, conv, [x for x, _ in res]
+
+    wall, st, svc, conv, xs = run(args.slots)
+    seq = None
+    if not args.no_sequential:
+        wall1, st1, svc1, conv1, xs1 = run(1)
+        diff = max(float(np.linalg.norm(a - b) / np.linalg.norm(b)) for a, b in zip(xs, xs1))
+        seq = {"solves_per_s": len(probs) / wall1, "wall_s": wall1, "speedup": wall1 / wall,
+               "max_rel_diff_vs_mix": diff}
+        _log(f"batch: {wall1 / wall:.2f}x over one slot; results alone vs in the mix differ by <= {diff:.1e}")
+    out = {
+        "metric": METRIC, "value": len(probs) / wall, "unit": "solves/s", "n_gpus": 1, "steps": len(probs),
+        "warmup": 2, "ms_per_step": wall / len(probs) * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32+f64" if args.precision == "fp32_refine" else "f64", "data": "synthetic",
+        "config": {"workload": f"batch of {len(probs)} tank BVPs, N log-uniform in [2k, 40k], aspect 1-8, "
+                               f"1/4 with a cylinder, setup included in every solve",
+                   "slots": args.slots, "p": args.p, "leaf_max": args.leaf, "tol": args.tol,
+                   "precision": args.precision, "preconditioner": args.precond, "total_nodes": int(sum(n_nodes))},
+        "p50_solve_ms": st["p50_latency_s"] * 1e3, "p99_solve_ms": st["p99_latency_s"] * 1e3,
+        "p50_service_ms": float(np.percentile(svc, 50)), "p99_service_ms": float(np.percentile(svc, 99)),
+        "converged": conv, "peak_device_gb": st["peak_bytes"] / 1e9, "one_slot": seq,
+    }
+    print(json.dumps(out))
+
+
+def run_serve(args):
+    """Time-stepping sessions arriving as a Poisson process: each session (a tank of N nodes) runs
+    `stages` BVPs in order, moving its free-surface nodes between stages (seeded N(0, 1e-3 h)
+    displacements); the request loop keeps each session's context resident. Latency = submit -> result."""
+    import threading
+
+    from bem_for_nonlinear_waves_amd import mesh as M
+    from bem_for_nonlinear_waves_amd.serve import BVPServer
+    rng = np.random.default_rng(5)
+    t = time.perf_counter()
+    sizes = rng.uniform(args.serve_min_n, args.serve_max_n, args.sessions).astype(int)
+    arrivals = np.cumsum(rng.exponential(1.0 / args.rate, args.sessions))
+    sessions = []
+    for si, n in enumerate(sizes):
+        m, d, phi, phin, h, seed, (Lx, Ly) = make_batch_problem(rng, int(n))
+        stages = []
+        mk = m
+        for k in range(args.stages):
+            mk = M.perturb_free_surface(mk, seed + 1 + k, 1e-3 * h)
+            ph, phn = M.tank_bc(mk, d, seed + 1 + k, Lx, Ly, body_velocity_sigma=0.1)
+            stages.append((mk, ph, phn))
+        sessions.append((d, stages))
+    _log(f"serve: {args.sessions} sessions x {args.stages} stages, nodes {sizes.min()}..{sizes.max()}, "
+         f"arrivals over {arrivals[-1]:.2f}s ({time.perf_counter() - t:.1f}s to build)")
+    kw = _solver_kw(args)
+    srv = BVPServer(slots=args.slots)
+    w = srv.submit(sessions[0][1][0][0], sessions[0][0], sessions[0][1][0][1], sessions[0][1][0][2], **kw)
+    srv.wait(w)  # warm-up (operator cache)
+    srv.close()
+    srv = BVPServer(slots=args.slots)
+    lat, svc, conv = [], [], []
+    lock = threading.Lock()
+    t0 = time.perf_counter()
+
+    def session(si):
+        d, stages = sessions[si]
+        time.sleep(max(0.0, arrivals[si] - (time.perf_counter() - t0)))
+        for mk, ph, phn in stages:
+            x, info = srv.wait(srv.submit(mk, d, ph, phn, session=si, **kw))
+            with lock:
+                lat.append(info["latency_s"])
+                svc.append(info["service_s"])
+                conv.append(int(info["converged"]))
+        srv.close_session(si)
+
+    th = [threading.Thread(target=session, args=(i,)) for i in range(args.sessions)]
+    for x in th:
+        x.start()
+    for x in th:
+        x.join()
+    wall = time.perf_counter() - t0
+    st = srv.stats()
+    srv.close()
+    lat_ms, svc_ms = np.array(lat) * 1e3, np.array(svc) * 1e3
+    _log(f"serve: {len(lat)} solves in {wall:.2f}s ({len(lat) / wall:.2f} solves/s), latency p50 "
+         f"{np.percentile(lat_ms, 50):.0f} ms p99 {np.percentile(lat_ms, 99):.0f} ms, converged {sum(conv)}/{len(conv)}")
+    out = {
+        "metric": METRIC, "value": len(lat) / wall, "unit": "solves/s", "n_gpus": 1, "steps": len(lat), "warmup": 1,
+        "ms_per_step": wall / len(lat) * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32+f64" if args.precision == "fp32_refine" else "f64", "data": "synthetic",
+        "config": {"workload": f"serving: {args.sessions} time-stepping sessions (Poisson arrivals at {args.rate}/s), "
+                               f"N uniform in [{args.serve_min_n}, {args.serve_max_n}], {args.stages} stages each "
+                               f"(one BVP per stage, free-surface nodes moved between stages, setup included)",
+                   "slots": args.slots, "p": args.p, "leaf_max": args.leaf, "tol": args.tol,
+                   "precision": args.precision, "preconditioner": args.precond},
+        "p50_solve_ms": float(np.percentile(lat_ms, 50)), "p99_solve_ms": float(np.percentile(lat_ms, 99)),
+        "p50_service_ms": float(np.percentile(svc_ms, 50)), "p99_service_ms": float(np.percentile(svc_ms, 99)),
+        "converged": int(sum(conv)), "peak_device_gb": st["peak_bytes"] / 1e9,
+    }
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -183,6 +342,16 @@ def main():
     ap.add_argument("--no-fp64-compare", action="store_true",
                     help="skip the plain FP64 solves timed beside the FP32+refinement ones")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--mode", choices=["solve", "batch", "serve"], default="solve",
+                    help="solve: the 209k-node BVP (default); batch: 64 tanks at once; serve: time-stepping sessions")
+    ap.add_argument("--slots", type=int, default=8, help="batch/serve: concurrent requests")
+    ap.add_argument("--batch", type=int, default=64)
+    ap.add_argument("--no-sequential", action="store_true", help="batch: skip the one-slot comparison")
+    ap.add_argument("--sessions", type=int, default=8)
+    ap.add_argument("--stages", type=int, default=8)
+    ap.add_argument("--rate", type=float, default=50.0, help="serve: session arrivals per second")
+    ap.add_argument("--serve-min-n", type=int, default=5000)
+    ap.add_argument("--serve-max-n", type=int, default=60000)
     ap.add_argument("--profile-only", action="store_true",
                     help="setup + a 256 MiB device copy (counter calibration) + the timed matvecs; no solves")
     args = ap.parse_args()
@@ -196,6 +365,10 @@ def main():
     from bem_for_nonlinear_waves_amd.bvp import device_name
 
     _log(f"device {device_name()}")
+    if args.mode == "batch":
+        return run_batch(args)
+    if args.mode == "serve":
+        return run_serve(args)
     t = time.perf_counter()
     m, d, phi, phin = make_workload(args.h)
     n = d.n_dof

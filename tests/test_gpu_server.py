@@ -1,0 +1,108 @@
+"""The request loop (include/bemfmm_server.h) on the GPU.
+
+* Each result is the same alone and in the mix: problems of uneven sizes solved one at a time
+  (one slot) and all at once (three slots, their kernels overlapping on the device) agree to
+  1e-10 relative, and agree with a direct BEM_BVP solve (the M2M/L2L/near-quadrature atomics
+  make runs differ in the last bits only).
+* Sessions keep their context resident: a second request with the same geometry (the phi_t BVP
+  of an RK stage) skips the setup and equals a fresh solve; moved nodes re-run the setup.
+* Admission by memory: with a budget that holds one problem, requests run one after another and
+  all complete; idle resident sessions are released when memory is needed.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _tank(Lx, nx, seed):
+    from bem_for_nonlinear_waves_amd import mesh as M
+    m = M.box_tank(Lx, 1.0, 1.0, nx, 8, 8)
+    m = M.perturb_free_surface(m, seed, 0.01)
+    d = M.build_dofs(m)
+    phi, phin = M.tank_bc(m, d, seed, Lx, 1.0)
+    return m, d, phi, phin
+
+
+KW = dict(p=8, leaf_max=32, tol=1e-11, max_iter=600)
+
+
+def _direct(m, d, phi, phin):
+    from bem_for_nonlinear_waves_amd.bvp import BEM_BVP
+    b = BEM_BVP(m, d, p=KW["p"], leaf_max=KW["leaf_max"])
+    b.set_bc(phi, phin)
+    b.prepare()
+    x, info = b.solveSystemGMRES(tol=KW["tol"], max_iter=KW["max_iter"])
+    b.close()
+    return x
+
+
+def test_same_alone_and_in_mix():
+    from bem_for_nonlinear_waves_amd.serve import BVPServer
+    probs = [_tank(Lx, nx, s) for s, (Lx, nx) in enumerate([(1.0, 8), (3.0, 30), (2.0, 16), (4.0, 40), (1.5, 12),
+                                                             (2.5, 24)])]
+    print("[gpu] server problems n_dof:", [d.n_dof for _, d, _, _ in probs], flush=True)
+    alone = []
+    srv = BVPServer(slots=1)
+    for m, d, phi, phin in probs:
+        x, info = srv.wait(srv.submit(m, d, phi, phin, **KW))
+        assert info["converged"] == 1
+        alone.append(x)
+    srv.close()
+    srv = BVPServer(slots=3)
+    tickets = [srv.submit(m, d, phi, phin, **KW) for m, d, phi, phin in probs]
+    mixed = [srv.wait(t) for t in tickets]
+    st = srv.stats()
+    print(f"[gpu] mix: completed={st['completed']} p50={st['p50_latency_s']:.3f}s p99={st['p99_latency_s']:.3f}s "
+          f"peak={st['peak_bytes'] / 1e9:.2f}GB", flush=True)
+    srv.close()
+    assert st["completed"] == len(probs) and st["failed"] == 0
+    for (m, d, phi, phin), xa, (xm, info) in zip(probs, alone, mixed):
+        assert info["converged"] == 1
+        assert np.linalg.norm(xm - xa) <= 1e-10 * np.linalg.norm(xa)
+    xd = _direct(*probs[1])
+    assert np.linalg.norm(alone[1] - xd) <= 1e-10 * np.linalg.norm(xd)
+
+
+def test_session_reuse_and_moved_nodes():
+    from bem_for_nonlinear_waves_amd import mesh as M
+    from bem_for_nonlinear_waves_amd.serve import BVPServer
+    m, d, phi, phin = _tank(3.0, 30, 7)
+    srv = BVPServer(slots=2)
+    x1, i1 = srv.wait(srv.submit(m, d, phi, phin, session=5, **KW))
+    # phi_t-like second BVP of the stage: same geometry, other boundary values
+    phi2, phin2 = 0.5 * phi + 0.1, phin.copy()
+    x2, i2 = srv.wait(srv.submit(m, d, phi2, phin2, session=5, **KW))
+    # next stage: free-surface nodes moved
+    m3 = M.perturb_free_surface(m, 11, 0.002)
+    x3, i3 = srv.wait(srv.submit(m3, d, phi, phin, session=5, **KW))
+    st = srv.stats()
+    srv.close_session(5)
+    srv.close()
+    print(f"[gpu] session: service {i1['service_s']:.3f}s (setup) {i2['service_s']:.3f}s (same geometry) "
+          f"{i3['service_s']:.3f}s (moved)", flush=True)
+    assert st["resident_sessions"] == 1 and st["failed"] == 0
+    for x, args in ((x1, (m, d, phi, phin)), (x2, (m, d, phi2, phin2)), (x3, (m3, d, phi, phin))):
+        xd = _direct(*args)
+        assert np.linalg.norm(x - xd) <= 1e-10 * np.linalg.norm(xd)
+    assert i2["service_s"] < i1["service_s"]
+
+
+def test_memory_admission_and_eviction():
+    from bem_for_nonlinear_waves_amd.serve import BVPServer
+    probs = [_tank(2.0, 16, s) for s in range(4)]
+    m0, d0 = probs[0][0], probs[0][1]
+    # the server's estimate of one FP64 context (plan + near field + Krylov basis), plus 5%
+    one = 1.05 * (72e3 * m0.n_nodes + 8.0 * d0.n_dof * min(KW["max_iter"] + 2, 512))
+    srv = BVPServer(slots=4, hbm_budget_bytes=one)
+    tickets = [srv.submit(m, d, phi, phin, session=s, **KW) for s, (m, d, phi, phin) in enumerate(probs)]
+    out = [srv.wait(t) for t in tickets]
+    st = srv.stats()
+    srv.close()
+    print(f"[gpu] admission: evicted={st['evicted']} peak={st['peak_bytes'] / 1e6:.1f}MB budget={one / 1e6:.1f}MB",
+          flush=True)
+    assert st["completed"] == 4 and st["failed"] == 0 and st["evicted"] == 3
+    for (m, d, phi, phin), (x, info) in zip(probs, out):
+        assert info["converged"] == 1
+        xd = _direct(m, d, phi, phin)
+        assert np.linalg.norm(x - xd) <= 1e-10 * np.linalg.norm(xd)
