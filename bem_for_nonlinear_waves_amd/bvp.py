@@ -50,13 +50,14 @@ class BEM_BVP:
     def __init__(self, mesh: Mesh, dofs: DofModel, p: int = 10, theta: float = 0.25, leaf_max: int = 50,
                  max_level: int = 7, q: int = 6, precision: int = _lib.PRECISION_FP64,
                  preconditioner: int = _lib.PRECOND_NONE, refine_eta: float = 1e-5, schwarz_max_core: int = 64,
-                 schwarz_max_block: int = 128, schwarz_overlap: int = 1, device: int = 0):
+                 schwarz_max_block: int = 128, schwarz_overlap: int = 1, m2l_tile_pairs: int = 0, device: int = 0):
         self.lib = _lib.load()
         o = _lib.Opts()
         self.lib.bemfmm_default_opts(ctypes.byref(o))
         o.p, o.theta, o.leaf_max, o.max_level, o.q = p, theta, leaf_max, max_level, q
         o.precision, o.preconditioner, o.refine_eta = precision, preconditioner, refine_eta
         o.schwarz_max_core, o.schwarz_max_block, o.schwarz_overlap = schwarz_max_core, schwarz_max_block, schwarz_overlap
+        o.m2l_tile_pairs = m2l_tile_pairs
         self.ctx = self.lib.bemfmm_ctx_create(device, ctypes.byref(o))
         if not self.ctx:
             raise BEMError(_lib.E_DEVICE, "bemfmm_ctx_create failed (no usable GPU?)")

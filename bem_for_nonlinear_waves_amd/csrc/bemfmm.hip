@@ -231,20 +231,21 @@ __global__ __launch_bounds__(256) void k_block_gemv(const int64_t* __restrict__ 
 }
 
 // Operator-grouped translation on the matrix cores. For every pair (dst, src) of a
-// tile (<= kTP pairs sharing one operator): D[dst] += T_op * S[src] for both layers.
-// The tile is the GEMM  out[RP x 2kTP] += T[RP x KP] * B[KP x 2kTP]  built from
+// tile (<= TP pairs sharing one operator): D[dst] += T_op * S[src] for both layers.
+// The tile is the GEMM  out[RP x 2TP] += T[RP x KP] * B[KP x 2TP]  built from
 // 16x16x4 MFMA blocks (RP = 64*RTW rows: C padded, KP = C padded to 16, zeros in the
-// padding): wave w owns row tiles [w*RTW, (w+1)*RTW) and all kTCT 16-column tiles.
+// padding): wave w owns row tiles [w*RTW, (w+1)*RTW) and all 2TP/16 16-column tiles.
 // B (the gathered source coefficients, rounded to T) is staged in LDS; T is streamed
 // from the padded operator array [op][KP][RP] through a two-stage register pipeline of
 // kKG k-steps, so that the loads of the next stage are in flight while the current one runs.
 // Workgroups are remapped so that each XCD processes one contiguous run of tiles
 // (tiles are sorted by operator): the tiles of an operator share that XCD's L2.
 // T = double: v_mfma_f64_16x16x4f64; T = float: v_mfma_f32_16x16x4f32 (FP32 accumulation).
-constexpr int kTP = 32;
-constexpr int kTCols = 2 * kTP;
-constexpr int kTStride = kTCols + 1;
-constexpr int kTCT = kTCols / 16;  // 16-column MFMA tiles per workgroup tile
+constexpr int kTP = 32;      // pairs per tile of the M2M/L2L translations
+constexpr int kTPM2L = 16;   // default pairs per M2L tile: 3 workgroups per CU (FP64) instead of 2 at 32 pairs;
+                             // measured on the 209k tank: FP64 3.61 -> 3.41 ms, FP32 2.52 -> 2.17 ms; 64 pairs: 4.6 ms
+// LDS elements of a tile of TP pairs: the B stage (KP x (2TP+1)) or the output transpose (2TP x (RP+1))
+inline size_t translate_lds(int TP, int KP, int RP) { return (size_t)std::max(KP * (2 * TP + 1), 2 * TP * (RP + 1)); }
 constexpr int kXCD = 8;
 constexpr int kKG = 4;  // k-steps (of 4) per pipeline stage
 typedef double double4_t __attribute__((ext_vector_type(4)));
@@ -273,7 +274,7 @@ struct Mfma<float> {
 
 // STAGE = false: atomically accumulate into D0/D1 at the pair's dst box (M2M, L2L).
 // STAGE = true: write each pair's two result columns to stage[(pair*2 + layer)*C + row] (M2L).
-template <typename T, int RTW, bool STAGE>
+template <typename T, int RTW, bool STAGE, int TP>
 __global__ __launch_bounds__(256) void k_translate_mfma(int C, int RP, int KP, int ntiles, const T* __restrict__ ops,
                                                         const int32_t* __restrict__ tile_op, const int64_t* __restrict__ tile_p0,
                                                         const int64_t* __restrict__ tile_p1, const int32_t* __restrict__ dst,
@@ -281,6 +282,9 @@ __global__ __launch_bounds__(256) void k_translate_mfma(int C, int RP, int KP, i
                                                         const double* __restrict__ S1, double* __restrict__ D0,
                                                         double* __restrict__ D1, T* __restrict__ stage) {
   extern __shared__ __align__(16) unsigned char smem[];
+  constexpr int kTCols = 2 * TP;
+  constexpr int kTStride = kTCols + 1;
+  constexpr int kTCT = kTCols / 16;  // 16-column MFMA tiles per workgroup tile
   T* Bs = reinterpret_cast<T*>(smem);  // KP * kTStride
   const int per_xcd = (ntiles + kXCD - 1) / kXCD;
   const int t = (blockIdx.x % kXCD) * per_xcd + blockIdx.x / kXCD;
@@ -289,18 +293,22 @@ __global__ __launch_bounds__(256) void k_translate_mfma(int C, int RP, int KP, i
   const int64_t p0 = tile_p0[t];
   const int np = (int)(tile_p1[t] - p0);
   {
-    // gather the tile's source coefficients: thread -> (layer, k); all kTP loads issued together
+    // gather the tile's source coefficients: thread -> (layer, k); 32 loads issued together
     const int layer = threadIdx.x >> 7;
     const double* S = layer ? S1 : S0;
-    int sidx[kTP];
+    constexpr int G = TP < 32 ? TP : 32;
 #pragma unroll
-    for (int u = 0; u < kTP; ++u) sidx[u] = (u < np) ? src[p0 + u] : -1;
-    for (int k = threadIdx.x & 127; k < KP; k += 128) {
-      double v[kTP];
+    for (int u0 = 0; u0 < TP; u0 += G) {
+      int sidx[G];
 #pragma unroll
-      for (int u = 0; u < kTP; ++u) v[u] = (sidx[u] >= 0 && k < C) ? S[(int64_t)sidx[u] * C + k] : 0.;
+      for (int u = 0; u < G; ++u) sidx[u] = (u0 + u < np) ? src[p0 + u0 + u] : -1;
+      for (int k = threadIdx.x & 127; k < KP; k += 128) {
+        double v[G];
 #pragma unroll
-      for (int u = 0; u < kTP; ++u) Bs[k * kTStride + 2 * u + layer] = (T)v[u];
+        for (int u = 0; u < G; ++u) v[u] = (sidx[u] >= 0 && k < C) ? S[(int64_t)sidx[u] * C + k] : 0.;
+#pragma unroll
+        for (int u = 0; u < G; ++u) Bs[k * kTStride + 2 * (u0 + u) + layer] = (T)v[u];
+      }
     }
   }
   __syncthreads();
@@ -348,7 +356,7 @@ __global__ __launch_bounds__(256) void k_translate_mfma(int C, int RP, int KP, i
     // per-pair results -> staging rows with coalesced stores: transpose the tile through
     // LDS (column-major, stride RP+1) and let each wave write whole columns
     __syncthreads();
-    T* Tt = Bs;  // reuse the staging area (sized for max(KP * kTStride, kTCols * (RP + 1)) elements)
+    T* Tt = Bs;  // reuse the staging area (sized by translate_lds)
 #pragma unroll
     for (int a = 0; a < RTW; ++a) {
       const int rt = wave * RTW + a;
@@ -807,14 +815,16 @@ struct DTiles {
   DBuf<int32_t> op, dst, src;
   DBuf<int64_t> p0, p1;
   int ntiles = 0;
-  void upload(const bemfmm::PairGroups& g, hipStream_t s) {
+  int tp = kTP;  // pairs per tile
+  void upload(const bemfmm::PairGroups& g, hipStream_t s, int pairs_per_tile = kTP) {
+    tp = pairs_per_tile;
     std::vector<int32_t> top;
     std::vector<int64_t> tp0, tp1;
     for (size_t k = 0; k < g.op.size(); ++k)
-      for (int64_t a = g.pair_ptr[k]; a < g.pair_ptr[k + 1]; a += kTP) {
+      for (int64_t a = g.pair_ptr[k]; a < g.pair_ptr[k + 1]; a += tp) {
         top.push_back(g.op[k]);
         tp0.push_back(a);
-        tp1.push_back(std::min(g.pair_ptr[k + 1], a + (int64_t)kTP));
+        tp1.push_back(std::min(g.pair_ptr[k + 1], a + (int64_t)tp));
       }
     ntiles = (int)top.size();
     op.upload(top, s);
@@ -911,16 +921,22 @@ template <typename T>
 void launch_translate(bemfmm_ctx* c, DTiles& t, const T* ops, const double* S0, const double* S1, double* D0, double* D1,
                       T* stage = nullptr) {
   if (t.ntiles == 0) return;
-  const size_t lds = (size_t)std::max(c->KP * kTStride, kTCols * (c->RP + 1)) * sizeof(T);
+  const size_t lds = translate_lds(t.tp, c->KP, c->RP) * sizeof(T);
   const int grid = ((t.ntiles + kXCD - 1) / kXCD) * kXCD;
   const int RTW = c->RP / 64;
-#define BEMFMM_TR(N)                                                                                                    \
+#define BEMFMM_TR_TP(N, TPV)                                                                                            \
   if (stage)                                                                                                            \
-    hipLaunchKernelGGL((k_translate_mfma<T, N, true>), dim3(grid), dim3(256), lds, c->stream, c->C, c->RP, c->KP,       \
+    hipLaunchKernelGGL((k_translate_mfma<T, N, true, TPV>), dim3(grid), dim3(256), lds, c->stream, c->C, c->RP, c->KP,  \
                        t.ntiles, ops, t.op.p, t.p0.p, t.p1.p, t.dst.p, t.src.p, S0, S1, D0, D1, stage);                  \
   else                                                                                                                  \
-    hipLaunchKernelGGL((k_translate_mfma<T, N, false>), dim3(grid), dim3(256), lds, c->stream, c->C, c->RP, c->KP,      \
+    hipLaunchKernelGGL((k_translate_mfma<T, N, false, TPV>), dim3(grid), dim3(256), lds, c->stream, c->C, c->RP, c->KP, \
                        t.ntiles, ops, t.op.p, t.p0.p, t.p1.p, t.dst.p, t.src.p, S0, S1, D0, D1, stage)
+#define BEMFMM_TR(N)                                      \
+  do {                                                    \
+    if (stage && t.tp == 16) BEMFMM_TR_TP(N, 16);         \
+    else if (stage && t.tp == 64) BEMFMM_TR_TP(N, 64);    \
+    else BEMFMM_TR_TP(N, 32);                             \
+  } while (0)
   switch (RTW) {
   case 1: BEMFMM_TR(1); break;
   case 2: BEMFMM_TR(2); break;
@@ -931,6 +947,7 @@ void launch_translate(bemfmm_ctx* c, DTiles& t, const T* ops, const double* S0, 
   default: BEMFMM_TR(7); break;
   }
 #undef BEMFMM_TR
+#undef BEMFMM_TR_TP
 }
 
 void fill(bemfmm_ctx* c, double* p, int64_t n, double v) {
@@ -997,6 +1014,9 @@ void do_prepare(bemfmm_ctx* c) {
   if (!c->have_bc) throw std::runtime_error("bemfmm_prepare: boundary values not set");
   if (c->opts.precision != BEMFMM_PRECISION_FP64 && c->opts.precision != BEMFMM_PRECISION_FP32_REFINE)
     throw std::runtime_error("bemfmm_prepare: unknown precision mode");
+  if (c->opts.m2l_tile_pairs != 0 && c->opts.m2l_tile_pairs != 16 && c->opts.m2l_tile_pairs != 32 &&
+      c->opts.m2l_tile_pairs != 64)
+    throw std::runtime_error("bemfmm_prepare: m2l_tile_pairs must be 0 (default), 16, 32 or 64");
   if (c->opts.preconditioner == BEMFMM_PRECOND_SCHWARZ &&
       (c->opts.schwarz_max_core < 1 || c->opts.schwarz_max_block < c->opts.schwarz_max_core ||
        c->opts.schwarz_max_block > kSchwarzMaxBlock || c->opts.schwarz_overlap < 0 || c->opts.schwarz_overlap > 8))
@@ -1038,7 +1058,7 @@ void do_prepare(bemfmm_ctx* c) {
     c->m2m[l].upload(pl.m2m_by_level[l], s);
     c->l2l[l].upload(pl.l2l_by_level[l], s);
   }
-  c->m2l.upload(pl.m2l, s);
+  c->m2l.upload(pl.m2l, s, c->opts.m2l_tile_pairs > 0 ? c->opts.m2l_tile_pairs : kTPM2L);
   {
     std::vector<int64_t> tptr(pl.n_boxes + 1, 0);
     for (int32_t d : pl.m2l.dst) tptr[d + 1]++;
@@ -1552,6 +1572,7 @@ void bemfmm_default_opts(bemfmm_opts* o) {
   o->schwarz_max_core = 64;   // main_time_domain.cpp:47
   o->schwarz_max_block = 128; // main_time_domain.cpp:48
   o->schwarz_overlap = 1;     // main_time_domain.cpp:46
+  o->m2l_tile_pairs = 0;
 }
 
 bemfmm_ctx* bemfmm_ctx_create(int device, const bemfmm_opts* opts) {

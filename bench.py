@@ -99,7 +99,7 @@ def setup_bvp(m, d, phi, phin, args, precision):
     pc = {"none": _lib.PRECOND_NONE, "jacobi": _lib.PRECOND_JACOBI, "schwarz": _lib.PRECOND_SCHWARZ}[args.precond]
     prec = _lib.PRECISION_FP32_REFINE if precision == "fp32_refine" else _lib.PRECISION_FP64
     bvp = BEM_BVP(m, d, p=args.p, leaf_max=args.leaf, max_level=args.max_level, preconditioner=pc, precision=prec,
-                  refine_eta=args.refine_eta)
+                  refine_eta=args.refine_eta, m2l_tile_pairs=args.m2l_tile_pairs)
     bvp.set_bc(phi, phin)
     t = time.perf_counter()
     bvp.prepare()
@@ -339,6 +339,7 @@ def main():
     ap.add_argument("--precond", choices=["none", "jacobi", "schwarz"], default="schwarz")
     ap.add_argument("--precision", choices=["fp32_refine", "fp64"], default="fp32_refine")
     ap.add_argument("--refine-eta", type=float, default=1e-5)
+    ap.add_argument("--m2l-tile-pairs", type=int, default=0, choices=[0, 16, 32, 64])
     ap.add_argument("--no-fp64-compare", action="store_true",
                     help="skip the plain FP64 solves timed beside the FP32+refinement ones")
     ap.add_argument("--no-cpu-baseline", action="store_true")

@@ -63,6 +63,7 @@ typedef struct bemfmm_opts {
   int schwarz_max_core;  /* schwarz_max_core_size, default 64 (main_time_domain.cpp:47) */
   int schwarz_max_block; /* schwarz_max_block_size, default 128, at most 128 (main_time_domain.cpp:48) */
   int schwarz_overlap;   /* schwarz_overlap_k: rings of neighbours added to a core, default 1 (main_time_domain.cpp:46) */
+  int m2l_tile_pairs;    /* M2L box pairs per matrix-core tile: 16, 32 or 64; 0 = default (16) */
 } bemfmm_opts;
 
 typedef struct bemfmm_result {

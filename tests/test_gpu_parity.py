@@ -22,10 +22,10 @@ def _gpu_bvp(m, d, phi, phin, **kw):
     return b
 
 
-@pytest.mark.parametrize("p", [8, 10])
-def test_operator_parity_small_tank(small_tank, oracle_mod, p):
+@pytest.mark.parametrize("p,tile", [(8, 0), (10, 0), (10, 16), (10, 64)])
+def test_operator_parity_small_tank(small_tank, oracle_mod, p, tile):
     m, d, phi, phin = small_tank
-    g = _gpu_bvp(m, d, phi, phin, p=p, leaf_max=32)
+    g = _gpu_bvp(m, d, phi, phin, p=p, leaf_max=32, m2l_tile_pairs=tile)
     o = oracle_mod.Oracle(m, d, phi, phin, p=p, leaf_max=32)
     st, so = g.stats(), o.stats()
     assert st["n_boxes"] == so["n_buckets"] and st["n_leaves"] == so["n_leaves"]
