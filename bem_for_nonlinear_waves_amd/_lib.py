@@ -24,7 +24,8 @@ class Opts(ctypes.Structure):
     _fields_ = [("p", ctypes.c_int), ("theta", ctypes.c_double), ("leaf_max", ctypes.c_int),
                 ("max_level", ctypes.c_int), ("q", ctypes.c_int), ("precision", ctypes.c_int),
                 ("preconditioner", ctypes.c_int), ("refine_eta", ctypes.c_double), ("schwarz_max_core", ctypes.c_int),
-                ("schwarz_max_block", ctypes.c_int), ("schwarz_overlap", ctypes.c_int), ("m2l_tile_pairs", ctypes.c_int)]
+                ("schwarz_max_block", ctypes.c_int), ("schwarz_overlap", ctypes.c_int), ("m2l_tile_pairs", ctypes.c_int),
+                ("dof_reorder", ctypes.c_int)]
 
 
 class Result(ctypes.Structure):
@@ -79,6 +80,7 @@ SIGNATURES = [
     ("bemfmm_server_close_session", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64]),
     ("bemfmm_server_get_stats", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
     ("bemfmm_server_last_error", ctypes.c_char_p, [ctypes.c_void_p]),
+    ("bemfmm_dof_order", ctypes.c_int, [ctypes.c_void_p, _ip]),
     ("bemfmm_schwarz_blocks", ctypes.c_int,
      [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64), _ip, _ip]),
 ]

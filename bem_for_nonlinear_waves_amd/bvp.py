@@ -50,14 +50,15 @@ class BEM_BVP:
     def __init__(self, mesh: Mesh, dofs: DofModel, p: int = 10, theta: float = 0.25, leaf_max: int = 50,
                  max_level: int = 7, q: int = 6, precision: int = _lib.PRECISION_FP64,
                  preconditioner: int = _lib.PRECOND_NONE, refine_eta: float = 1e-5, schwarz_max_core: int = 64,
-                 schwarz_max_block: int = 128, schwarz_overlap: int = 1, m2l_tile_pairs: int = 0, device: int = 0):
+                 schwarz_max_block: int = 128, schwarz_overlap: int = 1, m2l_tile_pairs: int = 0, dof_reorder: int = 1,
+                 device: int = 0):
         self.lib = _lib.load()
         o = _lib.Opts()
         self.lib.bemfmm_default_opts(ctypes.byref(o))
         o.p, o.theta, o.leaf_max, o.max_level, o.q = p, theta, leaf_max, max_level, q
         o.precision, o.preconditioner, o.refine_eta = precision, preconditioner, refine_eta
         o.schwarz_max_core, o.schwarz_max_block, o.schwarz_overlap = schwarz_max_core, schwarz_max_block, schwarz_overlap
-        o.m2l_tile_pairs = m2l_tile_pairs
+        o.m2l_tile_pairs, o.dof_reorder = m2l_tile_pairs, dof_reorder
         self.ctx = self.lib.bemfmm_ctx_create(device, ctypes.byref(o))
         if not self.ctx:
             raise BEMError(_lib.E_DEVICE, "bemfmm_ctx_create failed (no usable GPU?)")
@@ -149,6 +150,12 @@ class BEM_BVP:
         z = np.empty(self.n_dof)
         self._check(self.lib.bemfmm_precond_apply(self.ctx, _ptr(r, _lib._dp), _ptr(z, _lib._dp)))
         return z
+
+    def dof_order(self) -> np.ndarray:
+        """order[i] = caller DOF at internal position i (Morton order unless dof_reorder=0)."""
+        out = np.empty(self.n_dof, np.int32)
+        self._check(self.lib.bemfmm_dof_order(self.ctx, _ptr(out, _lib._ip)))
+        return out
 
     def schwarz_blocks(self):
         """(ptr, idx, ncore) of the Schwarz blocks (empty without the Schwarz preconditioner)."""

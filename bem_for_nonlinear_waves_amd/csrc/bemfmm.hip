@@ -12,6 +12,7 @@
 // Setup: the host builds the plan (plan.cpp); the near-field quadrature
 // (GW5x5 singular/near, Dunavant far; GMRES_FMM.hpp:1824-1891) runs on the GPU.
 #include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
 
 #include <algorithm>
 #include <chrono>
@@ -701,6 +702,47 @@ __global__ __launch_bounds__(256) void k_schwarz_apply(const int64_t* __restrict
   }
 }
 
+// ---- Morton DOF order (GMRES_FMM.hpp:3164-3246) ----
+// 21 bits per axis, interleaved x | y<<1 | z<<2: spread the 21 low bits of v to every third bit
+__device__ __forceinline__ uint64_t spread3(uint32_t v) {
+  uint64_t x = v & 0x1fffffu;
+  x = (x | (x << 32)) & 0x1f00000000ffffull;
+  x = (x | (x << 16)) & 0x1f0000ff0000ffull;
+  x = (x | (x << 8)) & 0x100f00f00f00f00full;
+  x = (x | (x << 4)) & 0x10c30c30c30c30c3ull;
+  x = (x | (x << 2)) & 0x1249249249249249ull;
+  return x;
+}
+// key of each DOF from its point, quantised in the bounding box of all points as the reference does
+// (t = (v - v0)/(v1 - v0) clamped to [0, 1], q = (uint32)(t * (2^21 - 1) + 0.5)); vals = DOF index
+__global__ void k_morton_keys(int n, const int32_t* __restrict__ point, const double* __restrict__ X, double x0, double x1,
+                             double y0, double y1, double z0, double z1, uint64_t* __restrict__ keys,
+                             int32_t* __restrict__ vals) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int p = point[i];
+  auto q = [](double v, double a, double b) -> uint32_t {
+    if (!(b > a)) return 0u;
+    double t = (v - a) / (b - a);
+    t = fmin(fmax(t, 0.0), 1.0);
+    return (uint32_t)(t * (double)((1u << 21) - 1u) + 0.5);
+  };
+  keys[i] = spread3(q(X[3 * p], x0, x1)) | (spread3(q(X[3 * p + 1], y0, y1)) << 1) |
+            (spread3(q(X[3 * p + 2], z0, z1)) << 2);
+  vals[i] = i;
+}
+// caller order <-> internal (Morton) order: internal i holds caller DOF perm[i]
+__global__ void k_perm_gather(int n, const int32_t* __restrict__ perm, const double* __restrict__ in,
+                              double* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = in[perm[i]];
+}
+__global__ void k_perm_scatter(int n, const int32_t* __restrict__ perm, const double* __restrict__ in,
+                               double* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[perm[i]] = in[i];
+}
+
 // ---- GMRES vector kernels (FP64) ----
 constexpr int kDotBlocks = 256;
 // partial[i * kDotBlocks + blk] = sum over the block's chunk of V_i . w   (grid: kDotBlocks x k)
@@ -851,6 +893,10 @@ struct bemfmm_ctx {
   std::vector<int32_t> tri, face_dof, dof_kind, dof_point;
   std::vector<double> phi_bc, phin_bc;
   bool have_geom = false, have_bc = false, prepared = false;
+  // internal DOF numbering (Morton order unless opts.dof_reorder == 0): internal i is caller DOF perm[i]
+  std::vector<int32_t> perm, inv, face_dof_i, kind_i, point_i;
+  bool perm_identity = true;
+  DBuf<int32_t> d_perm;
   // plan and device state
   std::unique_ptr<Plan> plan;
   int C = 0, n_boxes = 0, RP = 0, KP = 0;
@@ -1009,6 +1055,93 @@ void evaluate(bemfmm_ctx* c, int mode, const double* in, double* out, bool lo = 
   c->n_matvec++;
 }
 
+// Internal DOF order (solveGMRES reindexes DOFs by Morton key, GMRES_FMM.hpp:3164-3246): keys of the DOFs'
+// points on the GPU, stable LSD radix sort (hipCUB) of the 63-bit keys, so equal keys (the DOFs of one
+// point) keep the caller's order. opts.dof_reorder == 0 keeps the caller's numbering.
+void dof_order(bemfmm_ctx* c) {
+  const int n = c->n_dof;
+  // indices are used by the key kernel and the renumbering below: reject bad input before either
+  for (int i = 0; i < n; ++i)
+    if (c->dof_point[i] < 0 || c->dof_point[i] >= c->n_nodes || c->dof_kind[i] < 0 || c->dof_kind[i] > 2)
+      throw std::runtime_error("bemfmm_prepare: dof_point or dof_kind out of range");
+  for (int32_t d : c->face_dof)
+    if (d < 0 || d >= n) throw std::runtime_error("bemfmm_prepare: face_dof out of range");
+  for (int32_t v : c->tri)
+    if (v < 0 || v >= c->n_nodes) throw std::runtime_error("bemfmm_prepare: tri out of range");
+  c->perm.resize(n);
+  if (c->opts.dof_reorder) {
+    hipStream_t s = c->stream;
+    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+    for (int p = 0; p < c->n_nodes; ++p)
+      for (int a = 0; a < 3; ++a) {
+        lo[a] = std::min(lo[a], c->X[3 * p + a]);
+        hi[a] = std::max(hi[a], c->X[3 * p + a]);
+      }
+    DBuf<double> dX;
+    DBuf<int32_t> dpt, v0, v1;
+    DBuf<uint64_t> k0, k1;
+    dX.upload(c->X, s);
+    dpt.upload(c->dof_point, s);
+    v0.alloc(n);
+    v1.alloc(n);
+    k0.alloc(n);
+    k1.alloc(n);
+    hipLaunchKernelGGL(k_morton_keys, dim3((n + 255) / 256), dim3(256), 0, s, n, dpt.p, dX.p, lo[0], hi[0], lo[1], hi[1],
+                       lo[2], hi[2], k0.p, v0.p);
+    HIPCHECK(hipGetLastError());
+    size_t tmp_bytes = 0;
+    HIPCHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, k0.p, k1.p, v0.p, v1.p, n, 0, 63, s));
+    DBuf<unsigned char> tmp;
+    tmp.alloc(tmp_bytes);
+    HIPCHECK(hipcub::DeviceRadixSort::SortPairs(tmp.p, tmp_bytes, k0.p, k1.p, v0.p, v1.p, n, 0, 63, s));
+    HIPCHECK(hipMemcpyAsync(c->perm.data(), v1.p, n * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    HIPCHECK(hipStreamSynchronize(s));
+  } else {
+    for (int i = 0; i < n; ++i) c->perm[i] = i;
+  }
+  c->inv.assign(n, -1);
+  c->perm_identity = true;
+  for (int i = 0; i < n; ++i) {
+    const int j = c->perm[i];
+    if (j < 0 || j >= n || c->inv[j] >= 0) throw std::runtime_error("internal: DOF order is not a permutation");
+    c->inv[j] = i;
+    if (j != i) c->perm_identity = false;
+  }
+  c->face_dof_i.resize(c->face_dof.size());
+  for (size_t k = 0; k < c->face_dof.size(); ++k) c->face_dof_i[k] = c->inv[c->face_dof[k]];
+  c->kind_i.resize(n);
+  c->point_i.resize(n);
+  for (int i = 0; i < n; ++i) {
+    c->kind_i[i] = c->dof_kind[c->perm[i]];
+    c->point_i[i] = c->dof_point[c->perm[i]];
+  }
+  c->d_perm.upload(c->perm, c->stream);
+}
+
+// caller-ordered host vector <-> internal order
+void host_to_internal(const bemfmm_ctx* c, const double* in, double* out) {
+  for (int i = 0; i < c->n_dof; ++i) out[i] = in[c->perm[i]];
+}
+void host_to_caller(const bemfmm_ctx* c, const double* in, double* out) {
+  for (int i = 0; i < c->n_dof; ++i) out[c->perm[i]] = in[i];
+}
+
+// knowns in internal order, Neumann phi_n scaled by L (GMRES_FMM.hpp:3076-3091)
+void upload_knowns(bemfmm_ctx* c) {
+  const int n = c->n_dof;
+  std::vector<double> phik(n), phink(n);
+  for (int i = 0; i < n; ++i) {
+    const int j = c->perm[i];
+    phik[i] = c->phi_bc[j];
+    phink[i] = (c->kind_i[i] != 0) ? c->phin_bc[j] * c->plan->Lscale : c->phin_bc[j];
+  }
+  if (c->phi_k.n != (size_t)n) c->phi_k.alloc(n);
+  if (c->phin_k.n != (size_t)n) c->phin_k.alloc(n);
+  HIPCHECK(hipMemcpyAsync(c->phi_k.p, phik.data(), n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  HIPCHECK(hipMemcpyAsync(c->phin_k.p, phink.data(), n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  HIPCHECK(hipStreamSynchronize(c->stream));
+}
+
 void do_prepare(bemfmm_ctx* c) {
   if (!c->have_geom) throw std::runtime_error("bemfmm_prepare: geometry not set");
   if (!c->have_bc) throw std::runtime_error("bemfmm_prepare: boundary values not set");
@@ -1034,10 +1167,11 @@ void do_prepare(bemfmm_ctx* c) {
   c->partial.release();
   c->hbuf.release();
   c->v_cap = 0;
+  dof_order(c);
   c->plan.reset(new Plan());
   Plan& pl = *c->plan;
-  bemfmm::build_plan(pl, o, c->n_nodes, c->X.data(), c->n_faces, c->tri.data(), c->face_dof.data(), c->n_dof,
-                     c->dof_kind.data(), c->dof_point.data());
+  bemfmm::build_plan(pl, o, c->n_nodes, c->X.data(), c->n_faces, c->tri.data(), c->face_dof_i.data(), c->n_dof,
+                     c->kind_i.data(), c->point_i.data());
   c->C = pl.C;
   c->n_boxes = pl.n_boxes;
   if (pl.C > kGemvMaxRows) throw std::runtime_error("expansion order too large for the P2M block GEMV");
@@ -1082,10 +1216,7 @@ void do_prepare(bemfmm_ctx* c) {
   }
   c->kind.upload(pl.dof_kind, s);
   c->point.upload(pl.dof_point, s);
-  std::vector<double> phik(c->phi_bc), phink(c->n_dof);
-  for (int i = 0; i < c->n_dof; ++i) phink[i] = (c->dof_kind[i] != 0) ? c->phin_bc[i] * pl.Lscale : c->phin_bc[i];
-  c->phi_k.upload(phik, s);
-  c->phin_k.upload(phink, s);
+  upload_knowns(c);
   const int64_t nvec = (int64_t)c->n_dof + (int64_t)pl.n_boxes * pl.C;
   c->buf0.alloc(nvec);
   c->buf1.alloc(nvec);
@@ -1163,7 +1294,7 @@ void do_prepare(bemfmm_ctx* c) {
     std::vector<int64_t> node_row(c->n_nodes, -1);
     for (size_t r = 0; r < pl.near.row_out.size(); ++r) node_row[pl.near.row_out[r]] = (int64_t)r;
     for (int i = 0; i < c->n_dof; ++i)
-      if (c->dof_kind[i] != 2 && node_row[c->dof_point[i]] < 0)
+      if (c->kind_i[i] != 2 && node_row[c->point_i[i]] < 0)
         throw std::runtime_error("preconditioner: a DOF's point has no near-field row");
     DBuf<int64_t> d_node_row;
     d_node_row.upload(node_row, s);
@@ -1177,8 +1308,8 @@ void do_prepare(bemfmm_ctx* c) {
       std::vector<int32_t> seed(c->n_dof);
       for (int i = 0; i < c->n_dof; ++i) seed[i] = i;
       std::stable_sort(seed.begin(), seed.end(),
-                       [&](int32_t a, int32_t b) { return node_row[c->dof_point[a]] < node_row[c->dof_point[b]]; });
-      bemfmm::build_schwarz_blocks(c->n_dof, c->n_faces, c->face_dof.data(), c->dof_point.data(), seed,
+                       [&](int32_t a, int32_t b) { return node_row[c->point_i[a]] < node_row[c->point_i[b]]; });
+      bemfmm::build_schwarz_blocks(c->n_dof, c->n_faces, c->face_dof_i.data(), c->point_i.data(), seed,
                                    c->opts.schwarz_max_core, c->opts.schwarz_max_block, c->opts.schwarz_overlap,
                                    c->sw_host);
       const bemfmm::SchwarzBlocks& B = c->sw_host;
@@ -1490,7 +1621,9 @@ int do_solve(bemfmm_ctx* c, const double* x0, double* xout, double tol, int max_
   ensure_basis(c, 8);  // Krylov basis and the dot-product buffers (norms are taken before the first cycle)
   evaluate(c, 1, nullptr, c->ptmp.p);  // b, then M^-1 b (left preconditioning, GMRES_FMM.hpp:2157-2171)
   precond_apply(c, c->ptmp.p, c->bvec.p);
-  HIPCHECK(hipMemcpyAsync(c->xsol.p, x0, n * sizeof(double), hipMemcpyHostToDevice, s));
+  std::vector<double> x0i(n);
+  host_to_internal(c, x0, x0i.data());
+  HIPCHECK(hipMemcpyAsync(c->xsol.p, x0i.data(), n * sizeof(double), hipMemcpyHostToDevice, s));
   c->history.clear();
   c->refine_history.clear();
   int iters = 0, steps = 0;
@@ -1527,7 +1660,7 @@ int do_solve(bemfmm_ctx* c, const double* x0, double* xout, double tol, int max_
   HIPCHECK(hipMemcpyAsync(x.data(), c->xsol.p, n * sizeof(double), hipMemcpyDeviceToHost, s));
   HIPCHECK(hipStreamSynchronize(s));
   const double L = c->plan->Lscale;
-  for (int64_t i = 0; i < n; ++i) xout[i] = (c->dof_kind[i] == 0) ? x[i] / L : x[i];
+  for (int64_t i = 0; i < n; ++i) xout[c->perm[i]] = (c->kind_i[i] == 0) ? x[i] / L : x[i];
   if (res) {
     res->iters = iters;
     res->n_matvec = c->n_matvec - n_mv0;
@@ -1539,6 +1672,39 @@ int do_solve(bemfmm_ctx* c, const double* x0, double* xout, double tol, int max_
   for (int64_t i = 0; i < n; ++i)
     if (!std::isfinite(xout[i])) throw HipError("non-finite solution", BEMFMM_E_NONFINITE);
   return cur_err <= tol ? BEMFMM_OK : BEMFMM_E_NOT_CONVERGED;
+}
+
+// y = A x on host vectors in caller order
+void host_matvec(bemfmm_ctx* c, const double* x, double* y, bool lo) {
+  const int n = c->n_dof;
+  std::vector<double> xi(n), yi(n);
+  host_to_internal(c, x, xi.data());
+  HIPCHECK(hipMemcpyAsync(c->xin.p, xi.data(), n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  evaluate(c, 0, c->xin.p, c->yout.p, lo);
+  HIPCHECK(hipMemcpyAsync(yi.data(), c->yout.p, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHECK(hipStreamSynchronize(c->stream));
+  host_to_caller(c, yi.data(), y);
+}
+
+// y = A x on device vectors in caller order (permuted through the context's buffers unless the order is the caller's)
+void device_matvec(bemfmm_ctx* c, const double* x, double* y, bool lo) {
+  const int n = c->n_dof;
+  if (c->perm_identity) {
+    evaluate(c, 0, x, y, lo);
+    return;
+  }
+  hipLaunchKernelGGL(k_perm_gather, dim3((n + 255) / 256), dim3(256), 0, c->stream, n, c->d_perm.p, x, c->xin.p);
+  evaluate(c, 0, c->xin.p, c->yout.p, lo);
+  hipLaunchKernelGGL(k_perm_scatter, dim3((n + 255) / 256), dim3(256), 0, c->stream, n, c->d_perm.p, c->yout.p, y);
+  HIPCHECK(hipGetLastError());
+}
+
+// device vector (internal order) -> host vector (caller order)
+void download_caller(bemfmm_ctx* c, const double* d, double* out) {
+  std::vector<double> v(c->n_dof);
+  HIPCHECK(hipMemcpyAsync(v.data(), d, c->n_dof * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHECK(hipStreamSynchronize(c->stream));
+  host_to_caller(c, v.data(), out);
 }
 
 }  // namespace
@@ -1573,6 +1739,7 @@ void bemfmm_default_opts(bemfmm_opts* o) {
   o->schwarz_max_block = 128; // main_time_domain.cpp:48
   o->schwarz_overlap = 1;     // main_time_domain.cpp:46
   o->m2l_tile_pairs = 0;
+  o->dof_reorder = 1;
 }
 
 bemfmm_ctx* bemfmm_ctx_create(int device, const bemfmm_opts* opts) {
@@ -1644,12 +1811,7 @@ int bemfmm_set_bc(bemfmm_ctx* c, const double* phi, const double* phin) {
     c->have_bc = true;
     if (c->prepared) {
       // boundary values only change the right-hand side: refresh the knowns in place
-      std::vector<double> phink(c->n_dof);
-      for (int i = 0; i < c->n_dof; ++i)
-        phink[i] = (c->dof_kind[i] != 0) ? c->phin_bc[i] * c->plan->Lscale : c->phin_bc[i];
-      HIPCHECK(hipMemcpyAsync(c->phi_k.p, c->phi_bc.data(), c->n_dof * sizeof(double), hipMemcpyHostToDevice, c->stream));
-      HIPCHECK(hipMemcpyAsync(c->phin_k.p, phink.data(), c->n_dof * sizeof(double), hipMemcpyHostToDevice, c->stream));
-      HIPCHECK(hipStreamSynchronize(c->stream));
+      upload_knowns(c);
     }
     return BEMFMM_OK;
   });
@@ -1668,10 +1830,7 @@ int bemfmm_matvec(bemfmm_ctx* c, const double* x, double* y) {
   if (!c) return BEMFMM_E_INVALID;
   return guarded(c, [&]() {
     if (!c->prepared) throw HipError("bemfmm_matvec: call bemfmm_prepare first", BEMFMM_E_STATE);
-    HIPCHECK(hipMemcpyAsync(c->xin.p, x, c->n_dof * sizeof(double), hipMemcpyHostToDevice, c->stream));
-    evaluate(c, 0, c->xin.p, c->yout.p);
-    HIPCHECK(hipMemcpyAsync(y, c->yout.p, c->n_dof * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-    HIPCHECK(hipStreamSynchronize(c->stream));
+    host_matvec(c, x, y, false);
     return BEMFMM_OK;
   });
 }
@@ -1680,7 +1839,7 @@ int bemfmm_matvec_device(bemfmm_ctx* c, const double* x, double* y) {
   if (!c) return BEMFMM_E_INVALID;
   return guarded(c, [&]() {
     if (!c->prepared) throw HipError("bemfmm_matvec_device: call bemfmm_prepare first", BEMFMM_E_STATE);
-    evaluate(c, 0, x, y);
+    device_matvec(c, x, y, false);
     return BEMFMM_OK;
   });
 }
@@ -1691,10 +1850,7 @@ int bemfmm_matvec_fp32(bemfmm_ctx* c, const double* x, double* y) {
     if (!c->prepared) throw HipError("bemfmm_matvec_fp32: call bemfmm_prepare first", BEMFMM_E_STATE);
     if (c->opts.precision != BEMFMM_PRECISION_FP32_REFINE)
       throw HipError("bemfmm_matvec_fp32: context not created with BEMFMM_PRECISION_FP32_REFINE", BEMFMM_E_STATE);
-    HIPCHECK(hipMemcpyAsync(c->xin.p, x, c->n_dof * sizeof(double), hipMemcpyHostToDevice, c->stream));
-    evaluate(c, 0, c->xin.p, c->yout.p, true);
-    HIPCHECK(hipMemcpyAsync(y, c->yout.p, c->n_dof * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-    HIPCHECK(hipStreamSynchronize(c->stream));
+    host_matvec(c, x, y, true);
     return BEMFMM_OK;
   });
 }
@@ -1705,7 +1861,7 @@ int bemfmm_matvec_fp32_device(bemfmm_ctx* c, const double* x, double* y) {
     if (!c->prepared) throw HipError("bemfmm_matvec_fp32_device: call bemfmm_prepare first", BEMFMM_E_STATE);
     if (c->opts.precision != BEMFMM_PRECISION_FP32_REFINE)
       throw HipError("bemfmm_matvec_fp32_device: context not created with BEMFMM_PRECISION_FP32_REFINE", BEMFMM_E_STATE);
-    evaluate(c, 0, x, y, true);
+    device_matvec(c, x, y, true);
     return BEMFMM_OK;
   });
 }
@@ -1723,8 +1879,7 @@ int bemfmm_rhs(bemfmm_ctx* c, double* b) {
   return guarded(c, [&]() {
     if (!c->prepared) throw HipError("bemfmm_rhs: call bemfmm_prepare first", BEMFMM_E_STATE);
     evaluate(c, 1, nullptr, c->yout.p);
-    HIPCHECK(hipMemcpyAsync(b, c->yout.p, c->n_dof * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-    HIPCHECK(hipStreamSynchronize(c->stream));
+    download_caller(c, c->yout.p, b);
     return BEMFMM_OK;
   });
 }
@@ -1733,8 +1888,7 @@ int bemfmm_alpha(bemfmm_ctx* c, double* a) {
   if (!c) return BEMFMM_E_INVALID;
   return guarded(c, [&]() {
     if (!c->prepared) throw HipError("bemfmm_alpha: call bemfmm_prepare first", BEMFMM_E_STATE);
-    HIPCHECK(hipMemcpyAsync(a, c->alpha.p, c->n_dof * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-    HIPCHECK(hipStreamSynchronize(c->stream));
+    download_caller(c, c->alpha.p, a);
     return BEMFMM_OK;
   });
 }
@@ -1759,12 +1913,20 @@ int bemfmm_precond_apply(bemfmm_ctx* c, const double* r, double* z) {
   if (!c) return BEMFMM_E_INVALID;
   return guarded(c, [&]() {
     if (!c->prepared) throw HipError("bemfmm_precond_apply: call bemfmm_prepare first", BEMFMM_E_STATE);
-    HIPCHECK(hipMemcpyAsync(c->xin.p, r, c->n_dof * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    std::vector<double> ri(c->n_dof);
+    host_to_internal(c, r, ri.data());
+    HIPCHECK(hipMemcpyAsync(c->xin.p, ri.data(), c->n_dof * sizeof(double), hipMemcpyHostToDevice, c->stream));
     precond_apply(c, c->xin.p, c->yout.p);
-    HIPCHECK(hipMemcpyAsync(z, c->yout.p, c->n_dof * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-    HIPCHECK(hipStreamSynchronize(c->stream));
+    download_caller(c, c->yout.p, z);
     return BEMFMM_OK;
   });
+}
+
+int bemfmm_dof_order(const bemfmm_ctx* c, int32_t* order) {
+  if (!c || !order) return BEMFMM_E_INVALID;
+  if (!c->prepared) return BEMFMM_E_STATE;
+  std::copy(c->perm.begin(), c->perm.end(), order);
+  return BEMFMM_OK;
 }
 
 int bemfmm_schwarz_blocks(const bemfmm_ctx* c, int64_t* ptr, int32_t* idx, int32_t* ncore) {
@@ -1773,7 +1935,8 @@ int bemfmm_schwarz_blocks(const bemfmm_ctx* c, int64_t* ptr, int32_t* idx, int32
   const auto& B = c->sw_host;
   const int nb = B.n_blocks();
   if (ptr) std::copy(B.ptr.begin(), B.ptr.end(), ptr);
-  if (idx) std::copy(B.idx.begin(), B.idx.end(), idx);
+  if (idx)
+    for (size_t k = 0; k < B.idx.size(); ++k) idx[k] = c->perm[B.idx[k]];  // caller numbering
   if (ncore) std::copy(B.ncore.begin(), B.ncore.end(), ncore);
   return nb;
 }

@@ -64,6 +64,9 @@ typedef struct bemfmm_opts {
   int schwarz_max_block; /* schwarz_max_block_size, default 128, at most 128 (main_time_domain.cpp:48) */
   int schwarz_overlap;   /* schwarz_overlap_k: rings of neighbours added to a core, default 1 (main_time_domain.cpp:46) */
   int m2l_tile_pairs;    /* M2L box pairs per matrix-core tile: 16, 32 or 64; 0 = default (16) */
+  int dof_reorder;       /* 1 (default): number DOFs internally in Morton order of their points, as solveGMRES
+                            does (BEM_solveBVP_GMRES_FMM.hpp:3164-3246); 0: keep the caller's numbering.
+                            Every array of this ABI stays in the caller's numbering either way. */
 } bemfmm_opts;
 
 typedef struct bemfmm_result {
@@ -152,6 +155,11 @@ int bemfmm_residual_history(const bemfmm_ctx* ctx, double* out, int cap);
  * additive Schwarz over overlapping DOF blocks) — the reference's preconditioner solve
  * (ilu->solve / schwarz_preconditioner->solve, BEM_solveBVP_GMRES_FMM.hpp:2124-2171). n_dof entries. */
 int bemfmm_precond_apply(bemfmm_ctx* ctx, const double* r, double* z);
+
+/* Internal DOF order of a prepared context: order[i] = caller DOF at internal position i (n_dof entries).
+ * With dof_reorder this is the Morton order (21-bit keys in the point bounding box, stable radix sort on
+ * the GPU, equal keys kept in caller order); a caller may renumber its f2Index by it as the reference does. */
+int bemfmm_dof_order(const bemfmm_ctx* ctx, int32_t* order);
 
 /* Schwarz blocks of a prepared context: DOFs idx[ptr[b] .. ptr[b+1]) of block b, the first ncore[b] of them
  * its core (every DOF lies in exactly one core). Any pointer may be NULL. Returns the number of blocks
