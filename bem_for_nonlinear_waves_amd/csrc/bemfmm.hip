@@ -231,6 +231,62 @@ __global__ __launch_bounds__(256) void k_block_gemv(const int64_t* __restrict__ 
   }
 }
 
+// P2M: one workgroup per leaf; the leaf's matrix is column-major per layer (plan.hpp), so thread
+// (layer = tid / 128, row = tid % 128 + 128 k) streams a column's rows with its neighbours: every load is
+// coalesced and every lane has a row, however few DOF columns the leaf has. The gathered phi_n / phi
+// values are staged in LDS in chunks. M[leaf][row] = sum_col A[layer][col][row] x_layer[col] (FP64 sums).
+constexpr int kP2MChunk = 1024;
+template <typename T>
+__global__ __launch_bounds__(256) void k_p2m(int C, int CP, const int64_t* __restrict__ row_ptr,
+                                             const int64_t* __restrict__ col_ptr, const int64_t* __restrict__ mat_ptr,
+                                             const int32_t* __restrict__ row_out, const int32_t* __restrict__ col_idx,
+                                             const T* __restrict__ mat, const double* __restrict__ x0,
+                                             const double* __restrict__ x1, double* __restrict__ M0,
+                                             double* __restrict__ M1) {
+  __shared__ double xs[2][kP2MChunk];
+  constexpr int kRows = 4;  // rows per thread: C <= 512
+  const int b = blockIdx.x;
+  const int64_t c0 = col_ptr[b];
+  const int ncol = (int)(col_ptr[b + 1] - c0);
+  const T* A = mat + mat_ptr[b];
+  const int layer = threadIdx.x >> 7, r0 = threadIdx.x & 127;
+  const T* Al = A + (int64_t)layer * ncol * CP;
+  double acc[kRows] = {0., 0., 0., 0.};
+  for (int cs = 0; cs < ncol; cs += kP2MChunk) {
+    const int cn = min(kP2MChunk, ncol - cs);
+    __syncthreads();
+    for (int i = threadIdx.x; i < cn; i += blockDim.x) {
+      const int g = col_idx[c0 + cs + i];
+      xs[0][i] = x0[g];
+      xs[1][i] = x1[g];
+    }
+    __syncthreads();
+    const double* xl = xs[layer];
+#pragma unroll
+    for (int k = 0; k < kRows; ++k) {
+      const int r = r0 + 128 * k;
+      if (r >= CP) break;
+      const T* a = Al + (int64_t)cs * CP + r;
+      double s0 = 0., s1 = 0.;
+      int c = 0;
+      for (; c + 1 < cn; c += 2) {
+        const double v0 = (double)a[(int64_t)c * CP], v1 = (double)a[(int64_t)(c + 1) * CP];
+        s0 += v0 * xl[c];
+        s1 += v1 * xl[c + 1];
+      }
+      if (c < cn) s0 += (double)a[(int64_t)c * CP] * xl[c];
+      acc[k] += s0 + s1;
+    }
+  }
+  const int64_t base = row_out[row_ptr[b]];  // leaf box * C (rows of a leaf are consecutive)
+  double* out = layer ? M1 : M0;
+#pragma unroll
+  for (int k = 0; k < kRows; ++k) {
+    const int r = r0 + 128 * k;
+    if (r < C) out[base + r] = acc[k];
+  }
+}
+
 // Operator-grouped translation on the matrix cores. For every pair (dst, src) of a
 // tile (<= TP pairs sharing one operator): D[dst] += T_op * S[src] for both layers.
 // The tile is the GEMM  out[RP x 2TP] += T[RP x KP] * B[KP x 2TP]  built from
@@ -899,7 +955,7 @@ struct bemfmm_ctx {
   DBuf<int32_t> d_perm;
   // plan and device state
   std::unique_ptr<Plan> plan;
-  int C = 0, n_boxes = 0, RP = 0, KP = 0;
+  int C = 0, n_boxes = 0, RP = 0, KP = 0, p2m_rows_pad = 0;
   DGemv near, p2m;
   std::vector<DTiles> m2m, l2l;
   DTiles m2l;
@@ -1019,8 +1075,16 @@ void evaluate(bemfmm_ctx* c, int mode, const double* in, double* out, bool lo = 
   fill(c, c->M0.p, nbC, 0.);
   fill(c, c->M1.p, nbC, 0.);
   rec(1);
-  if (lo) launch_gemv(c, c->p2m, c->p2m_mat32.p, phin, phi, c->M0.p, c->M1.p);
-  else launch_gemv(c, c->p2m, c->p2m.mat.p, phin, phi, c->M0.p, c->M1.p);
+  if (c->p2m.nblk > 0) {
+    if (lo)
+      hipLaunchKernelGGL(k_p2m<float>, dim3(c->p2m.nblk), dim3(256), 0, c->stream, c->C, c->p2m_rows_pad,
+                         c->p2m.row_ptr.p, c->p2m.col_ptr.p, c->p2m.mat_ptr.p, c->p2m.row_out.p, c->p2m.col_idx.p,
+                         c->p2m_mat32.p, phin, phi, c->M0.p, c->M1.p);
+    else
+      hipLaunchKernelGGL(k_p2m<double>, dim3(c->p2m.nblk), dim3(256), 0, c->stream, c->C, c->p2m_rows_pad,
+                         c->p2m.row_ptr.p, c->p2m.col_ptr.p, c->p2m.mat_ptr.p, c->p2m.row_out.p, c->p2m.col_idx.p,
+                         c->p2m.mat.p, phin, phi, c->M0.p, c->M1.p);
+  }
   rec(2);
   for (int l = pl.n_levels - 1; l >= 1; --l) {
     if (lo) launch_translate(c, c->m2m[l], c->ops32.p, c->M0.p, c->M1.p, c->M0.p, c->M1.p);
@@ -1174,7 +1238,8 @@ void do_prepare(bemfmm_ctx* c) {
                      c->kind_i.data(), c->point_i.data());
   c->C = pl.C;
   c->n_boxes = pl.n_boxes;
-  if (pl.C > kGemvMaxRows) throw std::runtime_error("expansion order too large for the P2M block GEMV");
+  c->p2m_rows_pad = pl.p2m_rows_pad;
+  if (pl.p2m_rows_pad > 4 * 128) throw std::runtime_error("expansion order too large for the P2M kernel");
   if ((pl.C + 63) / 64 > 7) throw std::runtime_error("expansion order too large for the translation kernel");
   if (2 * pl.C > 256) throw std::runtime_error("expansion order too large for the M2L reduction");
   hipStream_t s = c->stream;

@@ -589,17 +589,17 @@ void build_plan(Plan& pl, const Options& opt, int n_nodes, const double* X, int 
         for (int c = 0; c < 3; ++c) cl.push_back(face_dof[3 * f + c]);
       std::sort(cl.begin(), cl.end());
       cl.erase(std::unique(cl.begin(), cl.end()), cl.end());
-      // zero-weight padding (repeats the last column, so the list stays sorted and lower_bound finds the original)
-      cl.resize(pad_cols((int)cl.size()), cl.back());
       cols[li] = std::move(cl);
     }
     BlockGemv& G = pl.p2m;
     G = BlockGemv();
+    const int CP = (C + 63) / 64 * 64;
+    pl.p2m_rows_pad = CP;
     for (int li = 0; li < nl; ++li) {
       const int b = leaves[li];
       const int ncol = (int)cols[li].size();
       G.mat_ptr.push_back(G.mat_size);
-      G.mat_size += (int64_t)C * 2 * ncol;
+      G.mat_size += (int64_t)CP * 2 * ncol;
       for (int r = 0; r < C; ++r) G.row_out.push_back(b * C + r);
       G.row_ptr.push_back((int64_t)G.row_out.size());
       G.col_idx.insert(G.col_idx.end(), cols[li].begin(), cols[li].end());
@@ -651,12 +651,12 @@ void build_plan(Plan& pl, const Options& opt, int n_nodes, const double* X, int 
                   const double wk = W * rq[k];
                   const int col = lc[k];
                   const int rr = pk_re(n, m);
-                  M[(size_t)rr * 2 * ncol + col] += wk * Rr;
-                  M[(size_t)rr * 2 * ncol + ncol + col] += wk * dr;
+                  M[(size_t)col * CP + rr] += wk * Rr;
+                  M[(size_t)(ncol + col) * CP + rr] += wk * dr;
                   if (m > 0) {
                     const int ri = pk_im(n, m);
-                    M[(size_t)ri * 2 * ncol + col] += -wk * Ri;
-                    M[(size_t)ri * 2 * ncol + ncol + col] += -wk * di;
+                    M[(size_t)col * CP + ri] += -wk * Ri;
+                    M[(size_t)(ncol + col) * CP + ri] += -wk * di;
                   }
                 }
               }

@@ -72,9 +72,12 @@ struct Plan {
   std::vector<int64_t> near_range_prefix; // per range: faces before it within its block
   std::vector<int32_t> target_box;        // per node
   std::vector<double> l2p_coef;           // per near row: C values (row order of near.row_out)
-  // P2M (one block per leaf)
+  // P2M (one block per leaf): rows, columns and row_out as a BlockGemv, but the matrix of leaf b is stored
+  // column-major per layer, p2m_mat[mat_ptr[b] + (layer * ncol + col) * p2m_rows_pad + row], rows padded
+  // to p2m_rows_pad (C rounded up to 64), so that consecutive threads (rows) read consecutive words
   BlockGemv p2m;
   std::vector<double> p2m_mat;            // p2m.mat_size
+  int p2m_rows_pad = 0;
   // translations
   std::vector<PairGroups> m2m_by_level;   // index = child level
   std::vector<PairGroups> l2l_by_level;   // index = child level

@@ -54,6 +54,27 @@ struct Emu {
       }
     }
   }
+  // P2M with the column-major leaf matrices (plan.hpp: p2m_mat[mat_ptr + (layer*ncol + col)*rows_pad + row])
+  void p2m(const double* x0, const double* x1) {
+    const auto& G = pl.p2m;
+    const int CP = pl.p2m_rows_pad;
+#pragma omp parallel for schedule(dynamic, 4)
+    for (int b = 0; b < G.n_blocks(); ++b) {
+      const int64_t c0 = G.col_ptr[b];
+      const int ncol = (int)(G.col_ptr[b + 1] - c0);
+      const double* A = &pl.p2m_mat[G.mat_ptr[b]];
+      for (int64_t r = G.row_ptr[b]; r < G.row_ptr[b + 1]; ++r) {
+        const int row = (int)(r - G.row_ptr[b]);
+        double s0 = 0, s1 = 0;
+        for (int c = 0; c < ncol; ++c) {
+          s0 += A[(size_t)c * CP + row] * x0[G.col_idx[c0 + c]];
+          s1 += A[(size_t)(ncol + c) * CP + row] * x1[G.col_idx[c0 + c]];
+        }
+        M0[G.row_out[r]] = s0;
+        M1[G.row_out[r]] = s1;
+      }
+    }
+  }
   void translate(const bemfmm::PairGroups& g, const double* S0, const double* S1, double* D0, double* D1) {
     const int C = pl.C;
     for (size_t k = 0; k < g.op.size(); ++k) {
@@ -91,7 +112,7 @@ struct Emu {
       buf0[i] = a;
       buf1[i] = b;
     }
-    gemv(pl.p2m, pl.p2m_mat.data(), buf0.data(), buf1.data(), M0.data(), M1.data());
+    p2m(buf0.data(), buf1.data());
     for (int l = pl.n_levels - 1; l >= 1; --l) translate(pl.m2m_by_level[l], M0.data(), M1.data(), M0.data(), M1.data());
     translate(pl.m2l, M0.data(), M1.data(), buf0.data() + n, buf1.data() + n);
     for (int l = 1; l < pl.n_levels; ++l)
