@@ -51,14 +51,14 @@ class BEM_BVP:
                  max_level: int = 7, q: int = 6, precision: int = _lib.PRECISION_FP64,
                  preconditioner: int = _lib.PRECOND_NONE, refine_eta: float = 1e-5, schwarz_max_core: int = 64,
                  schwarz_max_block: int = 128, schwarz_overlap: int = 1, m2l_tile_pairs: int = 0, dof_reorder: int = 1,
-                 device: int = 0):
+                 use_graphs: int = 1, device: int = 0):
         self.lib = _lib.load()
         o = _lib.Opts()
         self.lib.bemfmm_default_opts(ctypes.byref(o))
         o.p, o.theta, o.leaf_max, o.max_level, o.q = p, theta, leaf_max, max_level, q
         o.precision, o.preconditioner, o.refine_eta = precision, preconditioner, refine_eta
         o.schwarz_max_core, o.schwarz_max_block, o.schwarz_overlap = schwarz_max_core, schwarz_max_block, schwarz_overlap
-        o.m2l_tile_pairs, o.dof_reorder = m2l_tile_pairs, dof_reorder
+        o.m2l_tile_pairs, o.dof_reorder, o.use_graphs = m2l_tile_pairs, dof_reorder, use_graphs
         self.ctx = self.lib.bemfmm_ctx_create(device, ctypes.byref(o))
         if not self.ctx:
             raise BEMError(_lib.E_DEVICE, "bemfmm_ctx_create failed (no usable GPU?)")
@@ -131,6 +131,9 @@ class BEM_BVP:
         s = _lib.Stats()
         self._check(self.lib.bemfmm_get_stats(self.ctx, ctypes.byref(s)))
         return {k: getattr(s, k) for k, _ in _lib.Stats._fields_}
+
+    def profile_off(self):
+        self._check(self.lib.bemfmm_matvec_profile(self.ctx, None))
 
     def profile(self):
         out = np.zeros(8)

@@ -971,6 +971,8 @@ struct bemfmm_ctx {
   DBuf<double> M0, M1;
   DBuf<double> y0, y1;
   DBuf<double> xin, yout, bvec, tmpv, xsol, xbase, jacobi, rvec, dvec, ptmp;
+  DBuf<double> gin, gout;            // fixed input/output of the captured matvec graphs
+  hipGraphExec_t gexec[2] = {};      // [FP64, FP32] matvec graphs
   // restricted additive Schwarz: blocks and the core rows of the block inverses
   bemfmm::SchwarzBlocks sw_host;
   int sw_failed = 0;
@@ -1058,7 +1060,8 @@ void fill(bemfmm_ctx* c, double* p, int64_t n, double v) {
 
 // One evaluation of the BIE operator. mode: 0 = A x (x = in), 1 = rhs, 2 = rigid-mode alpha.
 // lo: FP32 near field, P2M and translations (FP32 MFMA), FP64 vectors and accumulation across boxes.
-void evaluate(bemfmm_ctx* c, int mode, const double* in, double* out, bool lo = false) {
+// Launches only (no host synchronisation), so that it can be captured into a HIP graph.
+void evaluate_launch(bemfmm_ctx* c, int mode, const double* in, double* out, bool lo) {
   Plan& pl = *c->plan;
   const int n = c->n_dof;
   const int64_t nbC = (int64_t)c->n_boxes * c->C;
@@ -1116,7 +1119,45 @@ void evaluate(bemfmm_ctx* c, int mode, const double* in, double* out, bool lo = 
                      c->y0.p, c->y1.p, c->phi_k.p, out);
   rec(8);
   HIPCHECK(hipGetLastError());
+}
+
+void drop_graphs(bemfmm_ctx* c) {
+  for (auto& g : c->gexec)
+    if (g) {
+      (void)hipGraphExecDestroy(g);
+      g = nullptr;
+    }
+}
+
+// mode 0 (A x) is replayed from a HIP graph captured once per context and precision on the context's own
+// input/output buffers (about 20 kernels per matvec become one launch; no memset nodes: the zero fills
+// are kernels). Event timing (bemfmm_matvec_profile) uses the eager launches.
+void evaluate(bemfmm_ctx* c, int mode, const double* in, double* out, bool lo = false) {
   c->n_matvec++;
+  if (mode != 0 || !c->opts.use_graphs || c->prof) {
+    evaluate_launch(c, mode, in, out, lo);
+    return;
+  }
+  const int gi = lo ? 1 : 0;
+  const size_t bytes = (size_t)c->n_dof * sizeof(double);
+  if (!c->gexec[gi]) {
+    hipGraph_t g = nullptr;
+    HIPCHECK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    try {
+      evaluate_launch(c, 0, c->gin.p, c->gout.p, lo);
+    } catch (...) {
+      (void)hipStreamEndCapture(c->stream, &g);
+      if (g) (void)hipGraphDestroy(g);
+      throw;
+    }
+    HIPCHECK(hipStreamEndCapture(c->stream, &g));
+    const hipError_t e = hipGraphInstantiate(&c->gexec[gi], g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    HIPCHECK(e);
+  }
+  if (in != c->gin.p) HIPCHECK(hipMemcpyAsync(c->gin.p, in, bytes, hipMemcpyDeviceToDevice, c->stream));
+  HIPCHECK(hipGraphLaunch(c->gexec[gi], c->stream));
+  if (out != c->gout.p) HIPCHECK(hipMemcpyAsync(out, c->gout.p, bytes, hipMemcpyDeviceToDevice, c->stream));
 }
 
 // Internal DOF order (solveGMRES reindexes DOFs by Morton key, GMRES_FMM.hpp:3164-3246): keys of the DOFs'
@@ -1226,6 +1267,7 @@ void do_prepare(bemfmm_ctx* c) {
   o.max_level = c->opts.max_level;
   o.q = c->opts.q;
   c->prepared = false;
+  drop_graphs(c);  // their kernels point into the buffers reallocated below
   // the Krylov basis is sized for n_dof: drop it (re-grown by the next solve)
   c->V.release();
   c->partial.release();
@@ -1290,7 +1332,8 @@ void do_prepare(bemfmm_ctx* c) {
   c->y0.alloc(c->n_nodes);
   c->y1.alloc(c->n_nodes);
   c->alpha.alloc(c->n_dof);
-  for (auto* b : {&c->xin, &c->yout, &c->bvec, &c->tmpv, &c->xsol, &c->xbase, &c->jacobi, &c->rvec, &c->dvec, &c->ptmp})
+  for (auto* b : {&c->xin, &c->yout, &c->bvec, &c->tmpv, &c->xsol, &c->xbase, &c->jacobi, &c->rvec, &c->dvec, &c->ptmp,
+                 &c->gin, &c->gout})
     b->alloc(c->n_dof);
   // quadrature inputs
   DBuf<int32_t> row_block, ncol_dof, face_perm, fv, fdof;
@@ -1805,6 +1848,7 @@ void bemfmm_default_opts(bemfmm_opts* o) {
   o->schwarz_overlap = 1;     // main_time_domain.cpp:46
   o->m2l_tile_pairs = 0;
   o->dof_reorder = 1;
+  o->use_graphs = 1;
 }
 
 bemfmm_ctx* bemfmm_ctx_create(int device, const bemfmm_opts* opts) {
@@ -1834,6 +1878,7 @@ void bemfmm_ctx_destroy(bemfmm_ctx* c) {
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   for (auto& e : c->ev)
     if (e) (void)hipEventDestroy(e);
+  drop_graphs(c);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
 }
@@ -2022,7 +2067,11 @@ int bemfmm_get_stats(const bemfmm_ctx* c, bemfmm_stats* s) {
 }
 
 int bemfmm_matvec_profile(bemfmm_ctx* c, double* ms8) {
-  if (!c || !ms8) return BEMFMM_E_INVALID;
+  if (!c) return BEMFMM_E_INVALID;
+  if (!ms8) {  // turn event recording off (matvecs replay their graph again)
+    c->prof = false;
+    return BEMFMM_OK;
+  }
   return guarded(c, [&]() {
     if (!c->prof) {
       for (auto& e : c->ev) HIPCHECK(hipEventCreate(&e));

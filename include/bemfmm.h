@@ -67,6 +67,8 @@ typedef struct bemfmm_opts {
   int dof_reorder;       /* 1 (default): number DOFs internally in Morton order of their points, as solveGMRES
                             does (BEM_solveBVP_GMRES_FMM.hpp:3164-3246); 0: keep the caller's numbering.
                             Every array of this ABI stays in the caller's numbering either way. */
+  int use_graphs;        /* 1 (default): replay the matvec from a HIP graph captured at the first matvec; 0: launch
+                            its kernels one by one */
 } bemfmm_opts;
 
 typedef struct bemfmm_result {
@@ -174,7 +176,9 @@ int bemfmm_refinement_history(const bemfmm_ctx* ctx, double* out, int cap);
 
 /* Kernel timing of the last matvec (milliseconds, HIP events on the context's stream):
  * [0] scatter, [1] P2M, [2] M2M, [3] M2L translations, [4] M2L per-box reduction, [5] L2L,
- * [6] near+L2P, [7] assemble. Enables event recording on the first call (and then returns zeros). */
+ * [6] near+L2P, [7] assemble. Enables event recording on the first call (and then returns zeros);
+ * while it is on, matvecs launch their kernels one by one instead of replaying the graph.
+ * ms8 = NULL turns it off. */
 int bemfmm_matvec_profile(bemfmm_ctx* ctx, double* ms8);
 
 #ifdef __cplusplus
