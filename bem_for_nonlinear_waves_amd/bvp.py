@@ -62,6 +62,12 @@ class BEM_BVP:
         self.ctx = self.lib.bemfmm_ctx_create(device, ctypes.byref(o))
         if not self.ctx:
             raise BEMError(_lib.E_DEVICE, "bemfmm_ctx_create failed (no usable GPU?)")
+        self.set_geometry(mesh, dofs)
+        self.time_setup = 0.0
+
+    def set_geometry(self, mesh: Mesh, dofs: DofModel):
+        """New nodes / DOF model (an RK stage moves the free surface); call prepare() before solving.
+        The context keeps its device operator table across geometries."""
         self.n_dof = int(dofs.n_dof)
         self._geom = [_d(mesh.X), _i(mesh.tri), _i(dofs.face_dof), _i(dofs.dof_kind), _i(dofs.dof_point)]
         X, tri, fd, kd, pt = self._geom
@@ -69,7 +75,6 @@ class BEM_BVP:
                                                  _ptr(tri, _lib._ip), _ptr(fd, _lib._ip), self.n_dof,
                                                  _ptr(kd, _lib._ip), _ptr(pt, _lib._ip)))
         self.prepared = False
-        self.time_setup = 0.0
 
     def _check(self, rc: int):
         if rc != _lib.OK:

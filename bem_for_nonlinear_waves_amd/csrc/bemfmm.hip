@@ -15,7 +15,9 @@
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
+#include <array>
 #include <chrono>
+#include <map>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -914,13 +916,14 @@ struct DTiles {
   DBuf<int64_t> p0, p1;
   int ntiles = 0;
   int tp = kTP;  // pairs per tile
-  void upload(const bemfmm::PairGroups& g, hipStream_t s, int pairs_per_tile = kTP) {
+  // slot[op] = index of operator op in the context's device operator table
+  void upload(const bemfmm::PairGroups& g, hipStream_t s, const std::vector<int32_t>& slot, int pairs_per_tile = kTP) {
     tp = pairs_per_tile;
     std::vector<int32_t> top;
     std::vector<int64_t> tp0, tp1;
     for (size_t k = 0; k < g.op.size(); ++k)
       for (int64_t a = g.pair_ptr[k]; a < g.pair_ptr[k + 1]; a += tp) {
-        top.push_back(g.op[k]);
+        top.push_back(slot[g.op[k]]);
         tp0.push_back(a);
         tp1.push_back(std::min(g.pair_ptr[k + 1], a + (int64_t)tp));
       }
@@ -965,6 +968,12 @@ struct bemfmm_ctx {
   DBuf<double> ops;
   // FP32 copies for BEMFMM_PRECISION_FP32_REFINE (inner matvecs); the M2L staging is shared
   DBuf<float> near_mat32, p2m_mat32, ops32;
+  // device operator table, [slot][KP][RP], kept across prepares: slot of each (kind, a, b, c) key
+  std::map<std::array<int, 4>, int32_t> op_slot;
+  int ops_cap = 0, ops_used = 0, ops_P = -1, ops32_valid = 0;
+  int ops_uploaded = 0;  // operators uploaded by the last prepare
+  double* op_stage = nullptr;  // pinned host staging for operator uploads
+  size_t op_stage_n = 0;
   DBuf<int32_t> kind, point;
   DBuf<double> alpha, phi_k, phin_k;
   DBuf<double> buf0, buf1;   // [phi_n | L0], [phi | L1]
@@ -1247,6 +1256,70 @@ void upload_knowns(bemfmm_ctx* c) {
   HIPCHECK(hipStreamSynchronize(c->stream));
 }
 
+// Maps the plan's operators to slots of the context's device table [slot][KP][RP] (zero padding), uploading
+// only operators the table does not hold yet: a re-prepare after nodes moved (RK stages of a session) reuses
+// nearly all of them. The table grows by doubling (old slots copied on the device).
+std::vector<int32_t> update_operator_table(bemfmm_ctx* c, const Plan& pl) {
+  hipStream_t s = c->stream;
+  const int C = pl.C, KP = c->KP, RP = c->RP;
+  const int64_t per = (int64_t)KP * RP;
+  if (c->ops_P != pl.P) {
+    c->op_slot.clear();
+    c->ops_used = 0;
+    c->ops_cap = 0;
+    c->ops.release();
+    c->ops32.release();
+    c->ops32_valid = 0;
+    c->ops_P = pl.P;
+  }
+  std::vector<int32_t> slot(pl.n_ops);
+  std::vector<int> fresh;  // plan ops to upload
+  for (int op = 0; op < pl.n_ops; ++op) {
+    const auto key = bemfmm::op_key(pl, op);
+    auto it = c->op_slot.find(key);
+    if (it == c->op_slot.end()) {
+      it = c->op_slot.emplace(key, (int32_t)(c->ops_used + fresh.size())).first;
+      fresh.push_back(op);
+    }
+    slot[op] = it->second;
+  }
+  const int need = c->ops_used + (int)fresh.size();
+  if (need > c->ops_cap) {
+    const int cap = std::max(need, std::max(64, 2 * c->ops_cap));
+    DBuf<double> t;
+    t.alloc((size_t)cap * per);
+    if (c->ops_used > 0)
+      HIPCHECK(hipMemcpyAsync(t.p, c->ops.p, (size_t)c->ops_used * per * sizeof(double), hipMemcpyDeviceToDevice, s));
+    HIPCHECK(hipStreamSynchronize(s));
+    c->ops = std::move(t);
+    c->ops_cap = cap;
+  }
+  // upload the new operators in batches through pinned staging, padded to [KP][RP]
+  constexpr int kBatch = 32;
+  if (!fresh.empty() && c->op_stage_n < (size_t)kBatch * per) {
+    if (c->op_stage) (void)hipHostFree(c->op_stage);
+    c->op_stage = nullptr;
+    HIPCHECK(hipHostMalloc((void**)&c->op_stage, (size_t)kBatch * per * sizeof(double), hipHostMallocDefault));
+    c->op_stage_n = (size_t)kBatch * per;
+  }
+  for (size_t b0 = 0; b0 < fresh.size(); b0 += kBatch) {
+    const int nb = (int)std::min<size_t>(kBatch, fresh.size() - b0);
+    for (int i = 0; i < nb; ++i) {
+      const auto key = bemfmm::op_key(pl, fresh[b0 + i]);
+      const auto v = bemfmm::cached_operator(pl.P, key[0], key[1], key[2], key[3]);
+      double* dst = c->op_stage + (size_t)i * per;
+      std::fill(dst, dst + per, 0.);
+      for (int k = 0; k < C; ++k) std::memcpy(dst + (size_t)k * RP, v->data() + (size_t)k * C, C * sizeof(double));
+    }
+    HIPCHECK(hipMemcpyAsync(c->ops.p + (size_t)(c->ops_used + b0) * per, c->op_stage, (size_t)nb * per * sizeof(double),
+                            hipMemcpyHostToDevice, s));
+    HIPCHECK(hipStreamSynchronize(s));  // the staging is refilled by the next batch
+  }
+  c->ops_used = need;
+  c->ops_uploaded = (int)fresh.size();
+  return slot;
+}
+
 void do_prepare(bemfmm_ctx* c) {
   if (!c->have_geom) throw std::runtime_error("bemfmm_prepare: geometry not set");
   if (!c->have_bc) throw std::runtime_error("bemfmm_prepare: boundary values not set");
@@ -1266,6 +1339,7 @@ void do_prepare(bemfmm_ctx* c) {
   o.leaf_max = c->opts.leaf_max;
   o.max_level = c->opts.max_level;
   o.q = c->opts.q;
+  o.materialize_ops = false;  // operators go from the float128 cache straight into the device table
   c->prepared = false;
   drop_graphs(c);  // their kernels point into the buffers reallocated below
   // the Krylov basis is sized for n_dof: drop it (re-grown by the next solve)
@@ -1291,15 +1365,18 @@ void do_prepare(bemfmm_ctx* c) {
   c->near.mat.alloc(pl.near.mat_size);
   c->p2m.upload(pl.p2m, s);
   c->p2m.mat.upload(pl.p2m_mat, s);
+  c->RP = (pl.C + 63) / 64 * 64;
+  c->KP = (pl.C + 4 * kKG - 1) / (4 * kKG) * (4 * kKG);
+  const std::vector<int32_t> slot = update_operator_table(c, pl);
   c->m2m.clear();
   c->l2l.clear();
   c->m2m.resize(pl.n_levels);
   c->l2l.resize(pl.n_levels);
   for (int l = 1; l < pl.n_levels; ++l) {
-    c->m2m[l].upload(pl.m2m_by_level[l], s);
-    c->l2l[l].upload(pl.l2l_by_level[l], s);
+    c->m2m[l].upload(pl.m2m_by_level[l], s, slot);
+    c->l2l[l].upload(pl.l2l_by_level[l], s, slot);
   }
-  c->m2l.upload(pl.m2l, s, c->opts.m2l_tile_pairs > 0 ? c->opts.m2l_tile_pairs : kTPM2L);
+  c->m2l.upload(pl.m2l, s, slot, c->opts.m2l_tile_pairs > 0 ? c->opts.m2l_tile_pairs : kTPM2L);
   {
     std::vector<int64_t> tptr(pl.n_boxes + 1, 0);
     for (int32_t d : pl.m2l.dst) tptr[d + 1]++;
@@ -1310,16 +1387,6 @@ void do_prepare(bemfmm_ctx* c) {
     c->m2l_tptr.upload(tptr, s);
     c->m2l_tpairs.upload(tp, s);
     c->m2l_stage.alloc(std::max<size_t>(1, pl.m2l.dst.size() * 2 * (size_t)pl.C));
-  }
-  {
-    // operators padded to [op][KP][RP] (KP = C rounded up to 16, RP = C rounded up to 64), zeros in the padding
-    c->RP = (pl.C + 63) / 64 * 64;
-    c->KP = (pl.C + 4 * kKG - 1) / (4 * kKG) * (4 * kKG);
-    std::vector<double> opsP((size_t)pl.n_ops * c->KP * c->RP, 0.);
-    for (int o2 = 0; o2 < pl.n_ops; ++o2)
-      for (int k = 0; k < pl.C; ++k)
-        std::memcpy(&opsP[((size_t)o2 * c->KP + k) * c->RP], &pl.ops[((size_t)o2 * pl.C + k) * pl.C], pl.C * sizeof(double));
-    c->ops.upload(opsP, s);
   }
   c->kind.upload(pl.dof_kind, s);
   c->point.upload(pl.dof_point, s);
@@ -1455,13 +1522,24 @@ void do_prepare(bemfmm_ctx* c) {
     c->p2m_mat32.alloc(c->p2m.mat.n);
     hipLaunchKernelGGL(k_to_f32, dim3(grid1(c->p2m.mat.n)), dim3(256), 0, s, (int64_t)c->p2m.mat.n, c->p2m.mat.p,
                        c->p2m_mat32.p);
-    c->ops32.alloc(c->ops.n);
-    hipLaunchKernelGGL(k_to_f32, dim3(grid1(c->ops.n)), dim3(256), 0, s, (int64_t)c->ops.n, c->ops.p, c->ops32.p);
+    // FP32 operator table: convert the slots that are new (all of them after the table grew)
+    const int64_t per = (int64_t)c->KP * c->RP;
+    if (c->ops32.n != c->ops.n) {
+      c->ops32.alloc(c->ops.n);
+      c->ops32_valid = 0;
+    }
+    if (c->ops32_valid < c->ops_used) {
+      const int64_t n32 = (int64_t)(c->ops_used - c->ops32_valid) * per;
+      hipLaunchKernelGGL(k_to_f32, dim3(grid1(n32)), dim3(256), 0, s, n32, c->ops.p + c->ops32_valid * per,
+                         c->ops32.p + c->ops32_valid * per);
+      c->ops32_valid = c->ops_used;
+    }
     HIPCHECK(hipGetLastError());
   } else {
     c->near_mat32.release();
     c->p2m_mat32.release();
     c->ops32.release();
+    c->ops32_valid = 0;
   }
   HIPCHECK(hipStreamSynchronize(s));
   const double t4 = now();
@@ -1492,6 +1570,7 @@ void do_prepare(bemfmm_ctx* c) {
   st.n_precond_blocks = c->sw_host.n_blocks();
   st.precond_fallback_blocks = c->sw_failed;
   st.precond_avg_block = c->sw_host.n_blocks() ? (double)c->sw_host.idx.size() / c->sw_host.n_blocks() : 0.;
+  st.n_ops_uploaded = c->ops_uploaded;
   (void)t0;
 }
 
@@ -1879,6 +1958,7 @@ void bemfmm_ctx_destroy(bemfmm_ctx* c) {
   for (auto& e : c->ev)
     if (e) (void)hipEventDestroy(e);
   drop_graphs(c);
+  if (c->op_stage) (void)hipHostFree(c->op_stage);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
 }

@@ -14,6 +14,7 @@
 #include <cmath>
 #include <functional>
 #include <map>
+#include <memory>
 #include <mutex>
 #include <stdexcept>
 #include <string>
@@ -730,41 +731,34 @@ void build_plan(Plan& pl, const Options& opt, int n_nodes, const double* X, int 
       pl.m2l.pair_ptr.push_back(pl.m2l.n_pairs());
     }
     pl.n_ops = opid;
-    pl.ops.assign((size_t)pl.n_ops * C * C, 0.);
-    // operator cache shared by all plans of the process
-    static std::mutex mu;
-    static std::map<std::tuple<int, int, int, int>, std::vector<double>> cache;
-    auto get = [&](int kind, int a, int b, int c, double* out) {
-      const auto key = std::make_tuple(P * 16 + kind, a, b, c);
-      {
-        std::lock_guard<std::mutex> lk(mu);
-        auto it = cache.find(key);
-        if (it != cache.end()) {
-          std::copy(it->second.begin(), it->second.end(), out);
-          return;
-        }
-      }
-      std::vector<double> v((size_t)C * C);
-      if (kind == 0) m2m_operator(P, a, v.data());
-      else if (kind == 1) l2l_operator(P, a, v.data());
-      else m2l_operator(P, a, b, c, v.data());
-      std::copy(v.begin(), v.end(), out);
-      std::lock_guard<std::mutex> lk(mu);
-      cache[key] = std::move(v);
-    };
-    const int nm2l = (int)pl.m2l.op.size();
+    if (opt.materialize_ops) pl.ops.assign((size_t)pl.n_ops * C * C, 0.);
+    else pl.ops.clear();
 #pragma omp parallel for schedule(dynamic, 1)
-    for (int i = 0; i < 16 + nm2l; ++i) {
-      double* out = &pl.ops[(size_t)i * C * C];
-      if (i < 8) get(0, i, 0, 0, out);
-      else if (i < 16) get(1, i - 8, 0, 0, out);
-      else {
-        const int g = i - 16;
-        get(2, pl.m2l_offsets[3 * g], pl.m2l_offsets[3 * g + 1], pl.m2l_offsets[3 * g + 2], out);
-      }
+    for (int i = 0; i < pl.n_ops; ++i) {
+      const auto k = op_key(pl, i);
+      const auto v = cached_operator(P, k[0], k[1], k[2], k[3]);  // computes it if missing
+      if (opt.materialize_ops) std::copy(v->begin(), v->end(), &pl.ops[(size_t)i * C * C]);
     }
   }
   pl.t_ops = now() - t4;
+}
+
+std::shared_ptr<const std::vector<double>> cached_operator(int P, int kind, int a, int b, int c) {
+  static std::mutex mu;
+  static std::map<std::array<int, 5>, std::shared_ptr<const std::vector<double>>> cache;
+  const std::array<int, 5> key{P, kind, a, b, c};
+  {
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = cache.find(key);
+    if (it != cache.end()) return it->second;
+  }
+  const int C = (P + 1) * (P + 1);
+  auto v = std::make_shared<std::vector<double>>((size_t)C * C);
+  if (kind == 0) m2m_operator(P, a, v->data());
+  else if (kind == 1) l2l_operator(P, a, v->data());
+  else m2l_operator(P, a, b, c, v->data());
+  std::lock_guard<std::mutex> lk(mu);
+  return cache.emplace(key, std::move(v)).first->second;  // a concurrent insert of the same key wins; same values
 }
 
 }  // namespace bemfmm

@@ -2,7 +2,9 @@
 // the caller's mesh/DOF arrays. Built by plan.cpp (g++/OpenMP, float128
 // operators in operators.cpp), uploaded and executed by bemfmm.hip.
 #pragma once
+#include <array>
 #include <cstdint>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -14,6 +16,7 @@ struct Options {
   int leaf_max = 50;      // fmm_bucket_max_points (main_time_domain.cpp:54-55)
   int max_level = 7;      // fmm_max_level
   int q = 6;              // p2m_quadrature_points (main_time_domain.cpp:62)
+  bool materialize_ops = true;  // fill Plan::ops (else only make sure every operator is in the cache)
 };
 
 // Box-blocked dense GEMV: for block b, rows [row_ptr[b], row_ptr[b+1]) and
@@ -83,7 +86,7 @@ struct Plan {
   std::vector<PairGroups> l2l_by_level;   // index = child level
   PairGroups m2l;
   int n_ops = 0;
-  std::vector<double> ops;                // n_ops * C * C, layout [op][k_in][r_out]
+  std::vector<double> ops;                // n_ops * C * C, layout [op][k_in][r_out] (if materialize_ops)
   std::vector<int32_t> m2l_offsets;       // per M2L op: (dx,dy,dz)
   // stats
   int64_t n_m2l_pairs = 0, near_nnz = 0, n_leaves = 0;
@@ -100,6 +103,17 @@ void build_plan(Plan& pl, const Options& opt, int n_nodes, const double* X, int 
 void m2l_operator(int P, int dx, int dy, int dz, double* out);
 void m2m_operator(int P, int octant, double* out);  // child -> parent; octant bit a set = child on + side of axis a
 void l2l_operator(int P, int octant, double* out);  // parent -> child
+
+// Process-wide cache of translation operators (computed once in float128, immutable afterwards):
+// kind 0 = M2M of octant a, 1 = L2L of octant a, 2 = M2L of lattice offset (a, b, c). Thread-safe.
+std::shared_ptr<const std::vector<double>> cached_operator(int P, int kind, int a, int b, int c);
+// (kind, a, b, c) of operator `op` of a plan: 0-7 M2M, 8-15 L2L, 16+ M2L offsets
+inline std::array<int, 4> op_key(const Plan& pl, int op) {
+  if (op < 8) return {0, op, 0, 0};
+  if (op < 16) return {1, op - 8, 0, 0};
+  const int g = op - 16;
+  return {2, pl.m2l_offsets[3 * g], pl.m2l_offsets[3 * g + 1], pl.m2l_offsets[3 * g + 2]};
+}
 
 inline int pk_re(int n, int m) { return m == 0 ? n * n : n * n + 2 * m - 1; }
 inline int pk_im(int n, int m) { return n * n + 2 * m; }

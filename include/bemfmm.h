@@ -93,6 +93,7 @@ typedef struct bemfmm_stats {
   int n_precond_blocks;         /* Schwarz: number of blocks */
   int precond_fallback_blocks;  /* Schwarz: blocks whose elimination failed (diagonal used instead) */
   double precond_avg_block;     /* Schwarz: mean block size */
+  int n_ops_uploaded;           /* translation operators uploaded by the last prepare (the rest were resident) */
 } bemfmm_stats;
 
 /* Availability (metal_m2l.h: metal_m2l_is_available / metal_m2l_device_name). */
