@@ -43,7 +43,7 @@ class Stats(ctypes.Structure):
                 ("t_ops", ctypes.c_double), ("t_upload", ctypes.c_double), ("t_near_quad", ctypes.c_double),
                 ("t_alpha", ctypes.c_double), ("n_precond_blocks", ctypes.c_int),
                 ("precond_fallback_blocks", ctypes.c_int), ("precond_avg_block", ctypes.c_double),
-                ("n_ops_uploaded", ctypes.c_int)]
+                ("n_ops_uploaded", ctypes.c_int), ("m2l_tile_pairs", ctypes.c_int)]
 
 
 # (name, restype, argtypes) of every entry point declared in include/bemfmm.h

@@ -22,6 +22,7 @@
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -972,6 +973,8 @@ struct bemfmm_ctx {
   std::map<std::array<int, 4>, int32_t> op_slot;
   int ops_cap = 0, ops_used = 0, ops_P = -1, ops32_valid = 0;
   int ops_uploaded = 0;  // operators uploaded by the last prepare
+  std::vector<int32_t> m2l_slot;  // plan op -> table slot (for re-tiling by the autotuner)
+  int m2l_tp = 0;                // pairs per M2L tile in use
   double* op_stage = nullptr;  // pinned host staging for operator uploads
   size_t op_stage_n = 0;
   DBuf<int32_t> kind, point;
@@ -1320,6 +1323,63 @@ std::vector<int32_t> update_operator_table(bemfmm_ctx* c, const Plan& pl) {
   return slot;
 }
 
+// Tile-size autotuner (opts.m2l_tile_pairs == 0): times the M2L translation kernel of this plan with
+// 16 and 32 pairs per tile on the multipoles the rigid-mode evaluation just left in M (one warm-up and
+// two timed launches each, HIP events on the context's stream) and keeps the faster tiling. The choice
+// is kept for the process per (expansion order, size class = log2 of the pair count), so later plans of
+// that class reuse it instead of timing again (and instead of timing behind other contexts' work).
+// Plans with few pairs keep the default.
+void tune_m2l_tiles(bemfmm_ctx* c, const Plan& pl) {
+  c->m2l_tp = c->m2l.tp;
+  if (c->opts.m2l_tile_pairs != 0 || pl.n_m2l_pairs < 20000) return;
+  static std::mutex mu;
+  static std::map<std::pair<int, int>, int> chosen;
+  const std::pair<int, int> cls{pl.P, (int)std::log2((double)pl.n_m2l_pairs)};
+  int known = 0;
+  {
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = chosen.find(cls);
+    if (it != chosen.end()) known = it->second;
+  }
+  if (known == c->m2l.tp) return;
+  if (known) {
+    DTiles alt;
+    alt.upload(pl.m2l, c->stream, c->m2l_slot, known);
+    c->m2l = std::move(alt);
+    c->m2l_tp = known;
+    return;
+  }
+  hipStream_t s = c->stream;
+  hipEvent_t e0, e1;
+  HIPCHECK(hipEventCreate(&e0));
+  HIPCHECK(hipEventCreate(&e1));
+  auto time_tiles = [&](DTiles& t) {
+    float best = 1e30f;
+    for (int r = 0; r < 3; ++r) {
+      HIPCHECK(hipEventRecord(e0, s));
+      launch_translate(c, t, c->ops.p, c->M0.p, c->M1.p, nullptr, nullptr, c->m2l_stage.p);
+      HIPCHECK(hipEventRecord(e1, s));
+      HIPCHECK(hipEventSynchronize(e1));
+      float ms = 0.f;
+      HIPCHECK(hipEventElapsedTime(&ms, e0, e1));
+      if (r > 0) best = std::min(best, ms);
+    }
+    return best;
+  };
+  DTiles alt;
+  alt.upload(pl.m2l, s, c->m2l_slot, c->m2l.tp == 16 ? 32 : 16);
+  const float t_cur = time_tiles(c->m2l), t_alt = time_tiles(alt);
+  if (t_alt < t_cur) c->m2l = std::move(alt);
+  c->m2l_tp = c->m2l.tp;
+  {
+    std::lock_guard<std::mutex> lk(mu);
+    chosen.emplace(cls, c->m2l_tp);
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  HIPCHECK(hipGetLastError());
+}
+
 void do_prepare(bemfmm_ctx* c) {
   if (!c->have_geom) throw std::runtime_error("bemfmm_prepare: geometry not set");
   if (!c->have_bc) throw std::runtime_error("bemfmm_prepare: boundary values not set");
@@ -1377,6 +1437,7 @@ void do_prepare(bemfmm_ctx* c) {
     c->l2l[l].upload(pl.l2l_by_level[l], s, slot);
   }
   c->m2l.upload(pl.m2l, s, slot, c->opts.m2l_tile_pairs > 0 ? c->opts.m2l_tile_pairs : kTPM2L);
+  c->m2l_slot = slot;
   {
     std::vector<int64_t> tptr(pl.n_boxes + 1, 0);
     for (int32_t d : pl.m2l.dst) tptr[d + 1]++;
@@ -1462,6 +1523,7 @@ void do_prepare(bemfmm_ctx* c) {
   const double t3 = now();
   // ---- rigid-mode alpha (updateGeometricProperties) ----
   evaluate(c, 2, nullptr, c->alpha.p);
+  tune_m2l_tiles(c, pl);
   const int pc = c->opts.preconditioner;
   c->sw_host = bemfmm::SchwarzBlocks{};
   for (auto* b : {&c->sw_inv}) b->release();
@@ -1571,6 +1633,7 @@ void do_prepare(bemfmm_ctx* c) {
   st.precond_fallback_blocks = c->sw_failed;
   st.precond_avg_block = c->sw_host.n_blocks() ? (double)c->sw_host.idx.size() / c->sw_host.n_blocks() : 0.;
   st.n_ops_uploaded = c->ops_uploaded;
+  st.m2l_tile_pairs = c->m2l_tp;
   (void)t0;
 }
 

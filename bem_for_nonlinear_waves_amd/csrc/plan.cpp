@@ -459,7 +459,7 @@ void build_plan(Plan& pl, const Options& opt, int n_nodes, const double* X, int 
   }
   // --- near blocks: one per target box (split into <= kMaxRows rows) ---
   {
-    constexpr int kMaxRows = 256;
+    constexpr int kMaxRows = 32;
     std::vector<int> tboxes;
     for (int b = 0; b < nb; ++b)
       if (!box_targets[b].empty()) tboxes.push_back(b);
@@ -516,12 +516,30 @@ void build_plan(Plan& pl, const Options& opt, int n_nodes, const double* X, int 
     pl.near_ranges.clear();
     pl.near_range_prefix.clear();
     pl.near_nnz = 0;
+    // blocks of at most kMaxRows rows, emitted in descending cost (rows x columns) so that the GPU starts
+    // the largest ones first and the end of the launch is made of small blocks (the boxes at the depth
+    // limit hold up to ~10x the median number of targets)
+    struct Chunk {
+      int64_t cost;
+      int ti;
+      size_t r0;
+    };
+    std::vector<Chunk> chunks;
     for (int ti = 0; ti < ntb; ++ti) {
+      const size_t nt = box_targets[tboxes[ti]].size();
+      const int64_t ncol = pad_cols((int)cols[ti].size() + C);
+      for (size_t r0 = 0; r0 < nt; r0 += kMaxRows)
+        chunks.push_back({(int64_t)(std::min(nt, r0 + (size_t)kMaxRows) - r0) * ncol, ti, r0});
+    }
+    std::stable_sort(chunks.begin(), chunks.end(), [](const Chunk& a, const Chunk& b) { return a.cost > b.cost; });
+    for (const Chunk& ch : chunks) {
+      const int ti = ch.ti;
       const int b = tboxes[ti];
       const auto& tg = box_targets[b];
       const int ncd = (int)cols[ti].size();
       const int ncol = pad_cols(ncd + C);
-      for (size_t r0 = 0; r0 < tg.size(); r0 += kMaxRows) {
+      {
+        const size_t r0 = ch.r0;
         const size_t r1 = std::min(tg.size(), r0 + (size_t)kMaxRows);
         G.mat_ptr.push_back(G.mat_size);
         G.mat_size += (int64_t)(r1 - r0) * 2 * ncol;

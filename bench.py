@@ -105,6 +105,7 @@ def setup_bvp(m, d, phi, phin, args, precision):
     bvp.prepare()
     t_setup = time.perf_counter() - t
     st = bvp.stats()
+    _log(f"[{precision}] M2L tile: {st['m2l_tile_pairs']} pairs; operators uploaded {st['n_ops_uploaded']}")
     if st["n_precond_blocks"]:
         _log(f"[{precision}] schwarz blocks={st['n_precond_blocks']} avg={st['precond_avg_block']:.1f} "
              f"fallback={st['precond_fallback_blocks']}")

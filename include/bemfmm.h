@@ -63,7 +63,8 @@ typedef struct bemfmm_opts {
   int schwarz_max_core;  /* schwarz_max_core_size, default 64 (main_time_domain.cpp:47) */
   int schwarz_max_block; /* schwarz_max_block_size, default 128, at most 128 (main_time_domain.cpp:48) */
   int schwarz_overlap;   /* schwarz_overlap_k: rings of neighbours added to a core, default 1 (main_time_domain.cpp:46) */
-  int m2l_tile_pairs;    /* M2L box pairs per matrix-core tile: 16, 32 or 64; 0 = default (16) */
+  int m2l_tile_pairs;    /* M2L box pairs per matrix-core tile: 16, 32 or 64; 0 = chosen by timing 16 and 32 on the
+                            plan during prepare (16 for plans under 20,000 pairs) */
   int dof_reorder;       /* 1 (default): number DOFs internally in Morton order of their points, as solveGMRES
                             does (BEM_solveBVP_GMRES_FMM.hpp:3164-3246); 0: keep the caller's numbering.
                             Every array of this ABI stays in the caller's numbering either way. */
@@ -94,6 +95,7 @@ typedef struct bemfmm_stats {
   int precond_fallback_blocks;  /* Schwarz: blocks whose elimination failed (diagonal used instead) */
   double precond_avg_block;     /* Schwarz: mean block size */
   int n_ops_uploaded;           /* translation operators uploaded by the last prepare (the rest were resident) */
+  int m2l_tile_pairs;           /* M2L pairs per tile in use (autotuned unless set in the options) */
 } bemfmm_stats;
 
 /* Availability (metal_m2l.h: metal_m2l_is_available / metal_m2l_device_name). */
