@@ -1541,11 +1541,14 @@ void do_prepare(bemfmm_ctx* c) {
                          ncol_dof.p, c->near.mat.p, c->alpha.p, c->jacobi.p);
       HIPCHECK(hipGetLastError());
     } else {
-      // seeds in the tree order of the DOFs' points (near rows are grouped by target box)
+      // seeds in a spatially coherent order: the internal (Morton) DOF order, or the order of the DOFs'
+      // target boxes (level-major preorder) when the caller's numbering is kept
       std::vector<int32_t> seed(c->n_dof);
       for (int i = 0; i < c->n_dof; ++i) seed[i] = i;
-      std::stable_sort(seed.begin(), seed.end(),
-                       [&](int32_t a, int32_t b) { return node_row[c->point_i[a]] < node_row[c->point_i[b]]; });
+      if (!c->opts.dof_reorder)
+        std::stable_sort(seed.begin(), seed.end(), [&](int32_t a, int32_t b) {
+          return pl.target_box[c->point_i[a]] < pl.target_box[c->point_i[b]];
+        });
       bemfmm::build_schwarz_blocks(c->n_dof, c->n_faces, c->face_dof_i.data(), c->point_i.data(), seed,
                                    c->opts.schwarz_max_core, c->opts.schwarz_max_block, c->opts.schwarz_overlap,
                                    c->sw_host);

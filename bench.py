@@ -39,6 +39,12 @@ if ROOT not in sys.path:
 METRIC = "BVP solves/s & FMM matvec Mnode/s at N=200k; p50/p99 solve latency"
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
+# HBM bytes per launch of the dominant kernels on the default workload (h = 0.036, p = 10, leaf 32,
+# depth 7), from rocprofv3 FETCH_SIZE (scaled x2) and WRITE_SIZE (x1) passes over
+# `bench.py --profile-only` (summarised in profiles/fmm_matvec_cylinder209k_p10.md); reported only
+# for that workload. Keys: (stage, FP32 operator).
+MEASURED_TRAFFIC = {("m2l", True): 5.943e9, ("m2l", False): 8.618e9,
+                    ("near_l2p", True): 5.177e9, ("near_l2p", False): 9.896e9}
 
 
 def make_workload(h: float, seed: int = 3):
@@ -431,6 +437,9 @@ def main():
         ach = near_bytes / (stage[dom] * 1e-3) / 1e9
         roof = {"kernel": f"k_block_gemv ({STAGES[dom]})", "bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "traffic": None}
+    default_workload = (args.h, args.p, args.leaf, args.max_level) == (0.036, 10, 32, 7)
+    if default_workload:
+        roof["traffic"] = MEASURED_TRAFFIC.get((STAGES[dom], fp32))
     roof["stage_ms"] = dict(zip(STAGES, [round(float(v), 4) for v in stage]))
     roof["m2l_tflops"] = m2l_flops / (stage[3] * 1e-3) / 1e12
     roof["near_gbs"] = near_bytes / (stage[6] * 1e-3) / 1e9
