@@ -25,7 +25,7 @@ class Opts(ctypes.Structure):
                 ("max_level", ctypes.c_int), ("q", ctypes.c_int), ("precision", ctypes.c_int),
                 ("preconditioner", ctypes.c_int), ("refine_eta", ctypes.c_double), ("schwarz_max_core", ctypes.c_int),
                 ("schwarz_max_block", ctypes.c_int), ("schwarz_overlap", ctypes.c_int), ("m2l_tile_pairs", ctypes.c_int),
-                ("dof_reorder", ctypes.c_int), ("use_graphs", ctypes.c_int)]
+                ("dof_reorder", ctypes.c_int), ("use_graphs", ctypes.c_int), ("m2l_kernel", ctypes.c_int)]
 
 
 class Result(ctypes.Structure):
@@ -43,7 +43,8 @@ class Stats(ctypes.Structure):
                 ("t_ops", ctypes.c_double), ("t_upload", ctypes.c_double), ("t_near_quad", ctypes.c_double),
                 ("t_alpha", ctypes.c_double), ("n_precond_blocks", ctypes.c_int),
                 ("precond_fallback_blocks", ctypes.c_int), ("precond_avg_block", ctypes.c_double),
-                ("n_ops_uploaded", ctypes.c_int), ("m2l_tile_pairs", ctypes.c_int)]
+                ("n_ops_uploaded", ctypes.c_int), ("m2l_tile_pairs", ctypes.c_int),
+                ("m2l_rotation", ctypes.c_int)]
 
 
 # (name, restype, argtypes) of every entry point declared in include/bemfmm.h

@@ -15,6 +15,7 @@
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
+#include <utility>
 #include <array>
 #include <chrono>
 #include <map>
@@ -478,6 +479,254 @@ __global__ __launch_bounds__(256) void k_m2l_reduce(int C, const int64_t* __rest
   const double v = (s0 + s1) + (s2 + s3);
   if (j < C) L0[(int64_t)t * C + j] = v;
   else L1[(int64_t)t * C + j - C] = v;
+}
+
+// M2L by rotation, FP32 operator: per pair L = H^T T(|o| z) D M, where D and H^T are the per-degree
+// ((2n+1)^2) real rotation blocks taking the offset o onto the z axis and T(|o| z) is the axial operator
+// (one (P+1-m)^2 block per order m, the same for the real and imaginary parts): 4433 multiply-adds per
+// column at P = 10 instead of 14641. A workgroup takes up to 128 pairs of one offset: the factors are
+// loaded once into LDS and read by every lane at the same address (broadcast); each of the 4 waves owns 32
+// pairs, lane 2u + layer holding column (pair u, layer) in registers. Columns move between the [box][C]
+// layouts and the lanes through a per-wave LDS buffer, 16 rows at a time (coalesced loads and stores).
+// Output: stage[(pair*2 + layer)*C + row], as the matrix-core M2L with STAGE.
+constexpr int kRotWavePairs = 32;
+constexpr int kRotTP = 4 * kRotWavePairs;  // pairs per workgroup tile
+// Device layout of the factors of one offset, in float4s: every row is zero-padded to whole float4s
+// (aligned broadcast reads); blocks D_0..D_P (d = 2n+1 rows), axial T_0..T_P (P+1-m rows), H^T_0..H^T_P.
+__host__ __device__ constexpr int rot_q(int len) { return (len + 3) / 4; }
+__host__ __device__ constexpr int rot_size4(int P) {
+  int s = 0;
+  for (int n = 0; n <= P; ++n) s += 2 * (2 * n + 1) * rot_q(2 * n + 1);
+  for (int m = 0; m <= P; ++m) s += (P + 1 - m) * rot_q(P + 1 - m);
+  return s;
+}
+__host__ __device__ constexpr int rot_size(int P) { return 4 * rot_size4(P); }  // floats per offset
+// compact row-major factors (m2l_rotation_operator) -> the padded device layout
+void rot_pad(int P, const double* in, float* out) {
+  std::fill(out, out + rot_size(P), 0.f);
+  int io = 0, oo = 0;
+  auto block = [&](int rows) {
+    const int q = 4 * rot_q(rows);
+    for (int a = 0; a < rows; ++a)
+      for (int c2 = 0; c2 < rows; ++c2) out[oo + a * q + c2] = (float)in[io + a * rows + c2];
+    io += rows * rows;
+    oo += rows * q;
+  };
+  for (int n = 0; n <= P; ++n) block(2 * n + 1);
+  for (int m = 0; m <= P; ++m) block(P + 1 - m);
+  for (int n = 0; n <= P; ++n) block(2 * n + 1);
+}
+__device__ constexpr int rot_re(int n, int m) { return m == 0 ? n * n : n * n + 2 * m - 1; }
+__device__ constexpr int rot_im(int n, int m) { return n * n + 2 * m; }
+// LDS writes and reads of one wave's lanes, ordered (the buffer is private to the wave)
+__device__ __forceinline__ void wave_lds_sync() {
+  asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+  __builtin_amdgcn_wave_barrier();
+}
+
+// Rows of the three factor stages, in execution order: stage 0 = D (degree n, row a), 1 = axial (order m,
+// row j), 2 = H^T (degree n, row a). Each row is a dot product of one padded factor row (q float4s at
+// float4 offset off) with the lane's coefficients. The rows run software-pipelined: the factors of row r+1
+// are read from LDS before the multiply-adds of row r, with scheduling barriers in between, so every LDS
+// read has a whole row of FMAs to complete (the scheduler alone keeps one read in flight).
+struct RotRow {
+  int stage, blk, a, len, q, off;
+};
+template <int P>
+__host__ __device__ constexpr int rot_rows() { return 3 * (P + 1) * (P + 1); }
+// descriptor of row K (compile-time)
+template <int P>
+__host__ __device__ constexpr RotRow rot_row(int K) {
+  int k = 0, off = 0;
+  for (int st = 0; st < 3; ++st) {
+    if (st == 1) {
+      for (int m = 0; m <= P; ++m) {
+        const int w = P + 1 - m;
+        if (K < k + w) return RotRow{1, m, K - k, w, rot_q(w), off + (K - k) * rot_q(w)};
+        k += w;
+        off += w * rot_q(w);
+      }
+      continue;
+    }
+    for (int n = 0; n <= P; ++n) {
+      const int d = 2 * n + 1;
+      if (K < k + d) return RotRow{st, n, K - k, d, rot_q(d), off + (K - k) * rot_q(d)};
+      k += d;
+      off += d * rot_q(d);
+    }
+  }
+  return RotRow{-1, 0, 0, 0, 0, 0};
+}
+
+// 0, but opaque to the compiler and available only once `dep` is computed: offsetting the reads of row
+// r+1 by it (dep = the result of row r-1) keeps the compiler from clustering and hoisting the factor reads
+// of later rows (with one base address and constant offsets it hoists thousands of them and spills)
+__device__ __forceinline__ int rot_after(float dep) {
+  int z;
+  asm("v_mov_b32 %0, 0" : "=v"(z) : "v"(dep));
+  return z;
+}
+
+template <int P>
+struct RotPipe {
+  static constexpr int QM = rot_q(2 * P + 1);
+  float4 cur[QM], nxt[QM];
+  float tmp[2 * P + 1], ti[P + 1];
+  float last = 0.f;  // result of the previous row
+};
+
+template <int P, int K>
+__device__ __forceinline__ void rot_step(const float4* __restrict__ R4, float* v, RotPipe<P>& s) {
+  constexpr RotRow rw = rot_row<P>(K);
+  if constexpr (K + 1 < rot_rows<P>()) {
+    constexpr RotRow nx = rot_row<P>(K + 1);
+    const float4* Rn = R4 + rot_after(s.last) + nx.off;
+#pragma unroll
+    for (int c4 = 0; c4 < nx.q; ++c4) s.nxt[c4] = Rn[c4];
+  }
+  __builtin_amdgcn_sched_barrier(0);
+  if constexpr (rw.stage != 1) {
+    // degree block: tmp[a] = sum_c B[a][c] v[n^2 + c]
+    constexpr int b = rw.blk * rw.blk;
+    float acc = 0.f;
+#pragma unroll
+    for (int c4 = 0; c4 < rw.q; ++c4) {
+      const float e[4] = {s.cur[c4].x, s.cur[c4].y, s.cur[c4].z, s.cur[c4].w};
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+        if (4 * c4 + u < rw.len) acc = fmaf(e[u], v[b + 4 * c4 + u], acc);
+    }
+    s.tmp[rw.a] = acc;
+    s.last = acc;
+    if constexpr (rw.a == rw.len - 1) {
+#pragma unroll
+      for (int a = 0; a < rw.len; ++a) v[b + a] = s.tmp[a];
+    }
+  } else {
+    // axial block of order m: row j = m + a, inputs n = m..P (real part, and imaginary part for m > 0)
+    constexpr int m = rw.blk;
+    float sr = 0.f, si = 0.f;
+#pragma unroll
+    for (int c4 = 0; c4 < rw.q; ++c4) {
+      const float e[4] = {s.cur[c4].x, s.cur[c4].y, s.cur[c4].z, s.cur[c4].w};
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int n = m + 4 * c4 + u;
+        if (n <= P) {
+          sr = fmaf(e[u], v[rot_re(n, m)], sr);
+          if (m > 0) si = fmaf(e[u], v[rot_im(n, m)], si);
+        }
+      }
+    }
+    s.tmp[rw.a] = sr;
+    s.ti[rw.a] = si;
+    s.last = sr;
+    if constexpr (rw.a == rw.len - 1) {
+#pragma unroll
+      for (int a = 0; a < rw.len; ++a) {
+        v[rot_re(m + a, m)] = s.tmp[a];
+        if (m > 0) v[rot_im(m + a, m)] = s.ti[a];
+      }
+    }
+  }
+  __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+  for (int c4 = 0; c4 < RotPipe<P>::QM; ++c4) s.cur[c4] = s.nxt[c4];
+}
+
+template <int P, int... K>
+__device__ __forceinline__ void rot_run(const float4* __restrict__ R4, float* v, std::integer_sequence<int, K...>) {
+  RotPipe<P> s;
+  constexpr RotRow r0 = rot_row<P>(0);
+#pragma unroll
+  for (int c4 = 0; c4 < r0.q; ++c4) s.cur[c4] = R4[r0.off + c4];
+  (rot_step<P, K>(R4, v, s), ...);
+}
+
+// v <- H^T T(|o| z) D v with the factors of one offset at R4 (LDS)
+template <int P>
+__device__ __forceinline__ void rot_apply(const float4* __restrict__ R4, float* v) {
+  rot_run<P>(R4, v, std::make_integer_sequence<int, rot_rows<P>()>{});
+}
+
+// FP32 multipoles for the rotation M2L: M32[(box*2 + layer)*C + r] = M_layer[box*C + r]
+__global__ void k_pack_m32(int64_t n, int C, const double* __restrict__ M0, const double* __restrict__ M1,
+                           float* __restrict__ M32) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t col = i / C, r = i - col * C;
+    M32[i] = (float)((col & 1) ? M1 : M0)[(col >> 1) * C + r];
+  }
+}
+
+template <int P>
+__global__ __launch_bounds__(256) void k_m2l_rot(int ntiles, const float* __restrict__ rot,
+                                                 const int32_t* __restrict__ tile_op, const int64_t* __restrict__ tile_p0,
+                                                 const int64_t* __restrict__ tile_p1, const int32_t* __restrict__ src,
+                                                 const float* __restrict__ M32, float* __restrict__ stage) {
+  constexpr int C = (P + 1) * (P + 1);
+  constexpr int RS = rot_size(P);
+  constexpr int NCH = (C + 15) / 16;
+  constexpr int kS = 17;  // LDS column stride (odd: conflict-free column reads)
+  __shared__ float4 R4[RS / 4];
+  __shared__ float Tw[4][64 * kS];
+  const int per_xcd = (ntiles + kXCD - 1) / kXCD;
+  const int t = (blockIdx.x % kXCD) * per_xcd + blockIdx.x / kXCD;
+  if (t >= ntiles) return;
+  const int op = tile_op[t];
+  const int64_t p0 = tile_p0[t];
+  const int64_t p1 = tile_p1[t];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int64_t pw = p0 + wave * kRotWavePairs;
+  const int np = (int)max<int64_t>(0, min<int64_t>(kRotWavePairs, p1 - pw));
+  // ---- issue the gather loads first: lane (g, k) loads rows 16 ch + k of columns 4i + g ----
+  const int g = lane >> 4, k = lane & 15;
+  float ld[NCH][16];
+  if (np > 0) {
+    const int my_src = src[pw + min(lane >> 1, np - 1)];  // columns past the tile repeat its last pair
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+      const int col = 4 * i + g;
+      const float* S = M32 + ((int64_t)__shfl(my_src, col) * 2 + (col & 1)) * C;
+#pragma unroll
+      for (int ch = 0; ch < NCH; ++ch) ld[ch][i] = (ch * 16 + k < C) ? S[ch * 16 + k] : 0.f;
+    }
+  }
+  // ---- the offset's factors -> LDS (shared by the 4 waves) ----
+  const float4* g4 = reinterpret_cast<const float4*>(rot + (int64_t)op * RS);
+  for (int i = threadIdx.x; i < RS / 4; i += 256) R4[i] = g4[i];  // RS = 4 * rot_size4(P)
+  __syncthreads();
+  if (np <= 0) return;
+  float* T = Tw[wave];
+  float v[C];
+  // ---- transpose the gathered rows into the lane's column, 16 rows at a time ----
+#pragma unroll
+  for (int ch = 0; ch < NCH; ++ch) {
+#pragma unroll
+    for (int i = 0; i < 16; ++i) T[(4 * i + g) * kS + k] = ld[ch][i];
+    wave_lds_sync();
+#pragma unroll
+    for (int kk = 0; kk < 16; ++kk)
+      if (ch * 16 + kk < C) v[ch * 16 + kk] = T[lane * kS + kk];
+    wave_lds_sync();
+  }
+  // ---- D, axial M2L, H^T ----
+  rot_apply<P>(R4, v);
+  // ---- scatter: lane's registers -> LDS -> rows [16 ch, 16 ch + 16) of column (4i + g) ----
+  float* out = stage + pw * 2 * C;
+#pragma unroll
+  for (int ch = 0; ch < NCH; ++ch) {
+#pragma unroll
+    for (int kk = 0; kk < 16; ++kk)
+      if (ch * 16 + kk < C) T[lane * kS + kk] = v[ch * 16 + kk];
+    wave_lds_sync();
+    const int r = ch * 16 + k;
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+      const int col = 4 * i + g;
+      if (col < 2 * np && r < C) out[(int64_t)col * C + r] = T[col * kS + k];
+    }
+    wave_lds_sync();
+  }
 }
 
 // FP64 -> FP32 copy of a resident matrix (FP32 near field / P2M / operators).
@@ -975,6 +1224,13 @@ struct bemfmm_ctx {
   int ops_uploaded = 0;  // operators uploaded by the last prepare
   std::vector<int32_t> m2l_slot;  // plan op -> table slot (for re-tiling by the autotuner)
   int m2l_tp = 0;                // pairs per M2L tile in use
+  // rotation M2L of the FP32 operator: factors [slot][RS] (slots of the operator table; M2L slots only)
+  bool m2l_rot = false;
+  DBuf<float> rot32;
+  std::vector<char> rot_have;
+  int rot_RS = 0;
+  DTiles m2l_rt;  // M2L tiles of kRotTP pairs (one offset each)
+  DBuf<float> m32;  // FP32 multipoles [box][layer][C] read by the rotation M2L
   double* op_stage = nullptr;  // pinned host staging for operator uploads
   size_t op_stage_n = 0;
   DBuf<int32_t> kind, point;
@@ -1066,6 +1322,26 @@ void launch_translate(bemfmm_ctx* c, DTiles& t, const T* ops, const double* S0, 
 #undef BEMFMM_TR_TP
 }
 
+bool rot_compiled(int P) { return P == 8 || P == 10; }
+
+void launch_m2l_rot(bemfmm_ctx* c, float* stage) {
+  DTiles& t = c->m2l_rt;
+  if (t.ntiles == 0) return;
+  const int grid = ((t.ntiles + kXCD - 1) / kXCD) * kXCD;
+  if (c->rot_RS != rot_size(c->plan->P)) throw std::runtime_error("rotation M2L: factor table stride mismatch");
+  const int64_t nm = (int64_t)c->n_boxes * 2 * c->C;
+  hipLaunchKernelGGL(k_pack_m32, dim3(grid1(nm)), dim3(256), 0, c->stream, nm, c->C, c->M0.p, c->M1.p, c->m32.p);
+#define BEMFMM_ROT(PV)                                                                                          \
+  hipLaunchKernelGGL(k_m2l_rot<PV>, dim3(grid), dim3(256), 0, c->stream, t.ntiles, c->rot32.p, t.op.p, t.p0.p, \
+                     t.p1.p, t.src.p, c->m32.p, stage)
+  switch (c->plan->P) {
+  case 8: BEMFMM_ROT(8); break;
+  case 10: BEMFMM_ROT(10); break;
+  default: throw std::runtime_error("rotation M2L not compiled for this expansion order");
+  }
+#undef BEMFMM_ROT
+}
+
 void fill(bemfmm_ctx* c, double* p, int64_t n, double v) {
   hipLaunchKernelGGL(k_fill, dim3(grid1(n)), dim3(256), 0, c->stream, p, n, v);
 }
@@ -1108,7 +1384,8 @@ void evaluate_launch(bemfmm_ctx* c, int mode, const double* in, double* out, boo
   rec(3);
   if (lo) {
     float* st = reinterpret_cast<float*>(c->m2l_stage.p);
-    launch_translate(c, c->m2l, c->ops32.p, c->M0.p, c->M1.p, nullptr, nullptr, st);
+    if (c->m2l_rot) launch_m2l_rot(c, st);
+    else launch_translate(c, c->m2l, c->ops32.p, c->M0.p, c->M1.p, nullptr, nullptr, st);
     rec(4);
     hipLaunchKernelGGL(k_m2l_reduce<float>, dim3(c->n_boxes), dim3(256), 0, c->stream, c->C, c->m2l_tptr.p,
                        c->m2l_tpairs.p, (const float*)st, L0, L1);
@@ -1323,6 +1600,61 @@ std::vector<int32_t> update_operator_table(bemfmm_ctx* c, const Plan& pl) {
   return slot;
 }
 
+// Rotation factors of the plan's M2L offsets in the FP32 table rot32 [slot][RS], indexed by the operator
+// table's slots; only slots without factors yet are computed (process-wide cache) and uploaded.
+void update_rotation_table(bemfmm_ctx* c, const Plan& pl, const std::vector<int32_t>& slot) {
+  hipStream_t s = c->stream;
+  const int RS = rot_size(pl.P);  // padded device layout
+  if (c->rot_RS != RS) {
+    c->rot32.release();
+    c->rot_have.clear();
+    c->rot_RS = RS;
+  }
+  if ((int64_t)c->rot_have.size() < c->ops_cap) {
+    DBuf<float> t;
+    t.alloc((size_t)c->ops_cap * RS);
+    if (!c->rot_have.empty())
+      HIPCHECK(hipMemcpyAsync(t.p, c->rot32.p, c->rot_have.size() * RS * sizeof(float), hipMemcpyDeviceToDevice, s));
+    HIPCHECK(hipStreamSynchronize(s));
+    c->rot32 = std::move(t);
+    c->rot_have.resize(c->ops_cap, 0);
+  }
+  std::vector<int> fresh;
+  for (int op = 16; op < pl.n_ops; ++op)
+    if (!c->rot_have[slot[op]]) {
+      c->rot_have[slot[op]] = 1;
+      fresh.push_back(op);
+    }
+  if (fresh.empty()) return;
+  const size_t cap = c->op_stage_n * 2 / RS;  // rotations per pinned staging batch (the staging holds doubles)
+  if (cap < 1) {
+    if (c->op_stage) (void)hipHostFree(c->op_stage);
+    c->op_stage = nullptr;
+    c->op_stage_n = (size_t)RS * 64;
+    HIPCHECK(hipHostMalloc((void**)&c->op_stage, c->op_stage_n * sizeof(double), hipHostMallocDefault));
+  }
+  const size_t batch = c->op_stage_n * 2 / RS;
+  float* stg = reinterpret_cast<float*>(c->op_stage);
+  for (size_t b0 = 0; b0 < fresh.size(); b0 += batch) {
+    const size_t nb = std::min(batch, fresh.size() - b0);
+    // consecutive slots are uploaded together: runs of fresh ops with consecutive slots
+    for (size_t i = 0; i < nb; ++i) {
+      const auto key = bemfmm::op_key(pl, fresh[b0 + i]);
+      const auto v = bemfmm::cached_operator(pl.P, 3, key[1], key[2], key[3]);
+      rot_pad(pl.P, v->data(), stg + i * RS);
+    }
+    size_t i = 0;
+    while (i < nb) {
+      size_t j = i + 1;
+      while (j < nb && slot[fresh[b0 + j]] == slot[fresh[b0 + j - 1]] + 1) ++j;
+      HIPCHECK(hipMemcpyAsync(c->rot32.p + (size_t)slot[fresh[b0 + i]] * RS, stg + i * RS, (j - i) * RS * sizeof(float),
+                              hipMemcpyHostToDevice, s));
+      i = j;
+    }
+    HIPCHECK(hipStreamSynchronize(s));  // the staging is refilled by the next batch
+  }
+}
+
 // Tile-size autotuner (opts.m2l_tile_pairs == 0): times the M2L translation kernel of this plan with
 // 16 and 32 pairs per tile on the multipoles the rigid-mode evaluation just left in M (one warm-up and
 // two timed launches each, HIP events on the context's stream) and keeps the faster tiling. The choice
@@ -1388,6 +1720,10 @@ void do_prepare(bemfmm_ctx* c) {
   if (c->opts.m2l_tile_pairs != 0 && c->opts.m2l_tile_pairs != 16 && c->opts.m2l_tile_pairs != 32 &&
       c->opts.m2l_tile_pairs != 64)
     throw std::runtime_error("bemfmm_prepare: m2l_tile_pairs must be 0 (default), 16, 32 or 64");
+  if (c->opts.m2l_kernel < 0 || c->opts.m2l_kernel > 2)
+    throw std::runtime_error("bemfmm_prepare: m2l_kernel must be 0 (default), 1 (matrix cores) or 2 (rotation)");
+  if (c->opts.m2l_kernel == 2 && !rot_compiled(c->opts.p))
+    throw std::runtime_error("bemfmm_prepare: the rotation M2L is compiled for p = 8 and 10 only");
   if (c->opts.preconditioner == BEMFMM_PRECOND_SCHWARZ &&
       (c->opts.schwarz_max_core < 1 || c->opts.schwarz_max_block < c->opts.schwarz_max_core ||
        c->opts.schwarz_max_block > kSchwarzMaxBlock || c->opts.schwarz_overlap < 0 || c->opts.schwarz_overlap > 8))
@@ -1600,11 +1936,18 @@ void do_prepare(bemfmm_ctx* c) {
       c->ops32_valid = c->ops_used;
     }
     HIPCHECK(hipGetLastError());
+    c->m2l_rot = c->opts.m2l_kernel != 1 && rot_compiled(pl.P);
+    if (c->m2l_rot) {
+      update_rotation_table(c, pl, c->m2l_slot);
+      c->m2l_rt.upload(pl.m2l, s, c->m2l_slot, kRotTP);
+      c->m32.alloc((size_t)pl.n_boxes * 2 * pl.C);
+    }
   } else {
     c->near_mat32.release();
     c->p2m_mat32.release();
     c->ops32.release();
     c->ops32_valid = 0;
+    c->m2l_rot = false;
   }
   HIPCHECK(hipStreamSynchronize(s));
   const double t4 = now();
@@ -1637,6 +1980,7 @@ void do_prepare(bemfmm_ctx* c) {
   st.precond_avg_block = c->sw_host.n_blocks() ? (double)c->sw_host.idx.size() / c->sw_host.n_blocks() : 0.;
   st.n_ops_uploaded = c->ops_uploaded;
   st.m2l_tile_pairs = c->m2l_tp;
+  st.m2l_rotation = c->m2l_rot ? 1 : 0;
   (void)t0;
 }
 
@@ -1994,6 +2338,7 @@ void bemfmm_default_opts(bemfmm_opts* o) {
   o->m2l_tile_pairs = 0;
   o->dof_reorder = 1;
   o->use_graphs = 1;
+  o->m2l_kernel = 0;
 }
 
 bemfmm_ctx* bemfmm_ctx_create(int device, const bemfmm_opts* opts) {

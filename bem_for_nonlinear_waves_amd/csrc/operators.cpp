@@ -21,6 +21,8 @@
 #include <quadmath.h>
 
 #include <cmath>
+#include <complex>
+#include <cstdint>
 #include <vector>
 
 #include "plan.hpp"
@@ -207,6 +209,165 @@ void l2l_operator(int P, int octant, double* out) {
       for (int n = j; n <= P; ++n)
         for (int m = 0; m <= n; ++m) pk.add(j, k, n, m, T(j, k, n, m), m > 0 ? T(j, k, n, -m) : qc{0, 0});
   pk.store(out);
+}
+
+
+
+// ---------------------------------------------------------------------------
+// M2L by rotation: T(o) = H^T T(|o| z) D (checked in float128 to 1e-34 relative).
+// D (multipole coefficients) and H (L2P coefficient vectors) are the block-diagonal (one (2n+1)^2
+// block per degree) real representations of the rotation Q with Q o = |o| z: packed-P2M(Q y) =
+// D packed-P2M(y), L2P-row(Q x) = H L2P-row(x); since |Qx - Qy| = |x - y|, the local expansion of
+// the rotated configuration L' = T(|o| z) D M satisfies L = H^T L'. T(|o| z) couples (j,k) only with
+// (n,k) (harmonics on the z axis have order 0), the same block for the real and imaginary parts.
+// The blocks are found by probing: the regular harmonics of degree n at 2n+1 points (the basis is the
+// normalised one, so the probe systems are well conditioned) - double precision is enough for the
+// FP32 operator these factors serve.
+// ---------------------------------------------------------------------------
+namespace {
+void regular_d(double x, double y, double z, int L, std::vector<std::complex<double>>& R) {
+  R.assign((L + 1) * (L + 1), 0.);
+  std::vector<std::complex<double>> NR((L + 1) * (L + 1), 0.);
+  const double r2 = x * x + y * y + z * z;
+  NR[idx(0, 0)] = 1.;
+  for (int m = 1; m <= L; ++m) NR[idx(m, m)] = (1. / (2 * m)) * (std::complex<double>(x, y) * NR[idx(m - 1, m - 1)]);
+  for (int m = 0; m <= L; ++m)
+    for (int n = m + 1; n <= L; ++n) {
+      std::complex<double> a = (double)(2 * n - 1) * z * NR[idx(n - 1, m)];
+      if (n - 2 >= m) a -= r2 * NR[idx(n - 2, m)];
+      NR[idx(n, m)] = (1. / ((n + m) * (n - m))) * a;
+    }
+  for (int n = 0; n <= L; ++n)
+    for (int m = 0; m <= n; ++m) {
+      double c = (double)qt().sqfnm[n][m];
+      if (m & 1) c = -c;
+      R[idx(n, m)] = c * NR[idx(n, m)];
+    }
+}
+// out = B A^-1 for d x d row-major A, B (partial pivoting)
+void right_solve(std::vector<double> A, const std::vector<double>& B, int d, double* out) {
+  std::vector<double> I(d * d, 0.);
+  for (int i = 0; i < d; ++i) I[i * d + i] = 1.;
+  for (int k = 0; k < d; ++k) {
+    int p = k;
+    for (int i = k + 1; i < d; ++i)
+      if (std::fabs(A[i * d + k]) > std::fabs(A[p * d + k])) p = i;
+    for (int j = 0; j < d; ++j) {
+      std::swap(A[k * d + j], A[p * d + j]);
+      std::swap(I[k * d + j], I[p * d + j]);
+    }
+    const double pv = A[k * d + k];
+    for (int j = 0; j < d; ++j) {
+      A[k * d + j] /= pv;
+      I[k * d + j] /= pv;
+    }
+    for (int i = 0; i < d; ++i)
+      if (i != k) {
+        const double f = A[i * d + k];
+        for (int j = 0; j < d; ++j) {
+          A[i * d + j] -= f * A[k * d + j];
+          I[i * d + j] -= f * I[k * d + j];
+        }
+      }
+  }
+  for (int i = 0; i < d; ++i)
+    for (int j = 0; j < d; ++j) {
+      double s = 0.;
+      for (int k = 0; k < d; ++k) s += B[i * d + k] * I[k * d + j];
+      out[i * d + j] = s;
+    }
+}
+}  // namespace
+
+int m2l_rotation_size(int P) {
+  int s = 0;
+  for (int n = 0; n <= P; ++n) s += 2 * (2 * n + 1) * (2 * n + 1);
+  for (int m = 0; m <= P; ++m) s += (P + 1 - m) * (P + 1 - m);
+  return s;
+}
+
+void m2l_rotation_operator(int P, int dx, int dy, int dz, double* out) {
+  const double r = std::sqrt((double)dx * dx + (double)dy * dy + (double)dz * dz);
+  const double ez[3] = {dx / r, dy / r, dz / r};
+  double t[3] = {1., 0., 0.};
+  if (std::fabs(ez[0]) > 0.9) {
+    t[0] = 0.;
+    t[1] = 1.;
+  }
+  const double dt = t[0] * ez[0] + t[1] * ez[1] + t[2] * ez[2];
+  double ex[3] = {t[0] - dt * ez[0], t[1] - dt * ez[1], t[2] - dt * ez[2]};
+  const double nx = std::sqrt(ex[0] * ex[0] + ex[1] * ex[1] + ex[2] * ex[2]);
+  for (double& v : ex) v /= nx;
+  const double ey[3] = {ez[1] * ex[2] - ez[2] * ex[1], ez[2] * ex[0] - ez[0] * ex[2], ez[0] * ex[1] - ez[1] * ex[0]};
+  // probe points: fixed pseudo-random points of the cube [-1, 1]^3 (splitmix64)
+  std::uint64_t seed = 0x9e3779b97f4a7c15ull;
+  auto unit = [&seed]() {
+    std::uint64_t z = (seed += 0x9e3779b97f4a7c15ull);
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    z ^= z >> 31;
+    return 2. * (double)(z >> 11) / 9007199254740992. - 1.;
+  };
+  auto probe = [&unit](int, int, double* p) {
+    for (int a = 0; a < 3; ++a) p[a] = unit();
+  };
+  double* D = out;
+  int off = 0;
+  for (int n = 0; n <= P; ++n) off += (2 * n + 1) * (2 * n + 1);
+  double* Tax = out + off;
+  for (int m = 0; m <= P; ++m) off += (P + 1 - m) * (P + 1 - m);
+  double* Ht = out + off;
+  std::vector<std::complex<double>> Ra, Rb;
+  int boff = 0;
+  for (int n = 0; n <= P; ++n) {
+    const int d = 2 * n + 1;
+    std::vector<double> Am(d * d), Bm(d * d), Al(d * d), Bl(d * d);
+    for (int i = 0; i < d; ++i) {
+      double pnt[3], qp[3];
+      probe(i, d, pnt);
+      for (int a = 0; a < 3; ++a) qp[a] = 0.;
+      for (int a = 0; a < 3; ++a) {
+        qp[0] += ex[a] * pnt[a];
+        qp[1] += ey[a] * pnt[a];
+        qp[2] += ez[a] * pnt[a];
+      }
+      regular_d(pnt[0], pnt[1], pnt[2], n, Ra);
+      regular_d(qp[0], qp[1], qp[2], n, Rb);
+      for (int m = 0; m <= n; ++m) {
+        const std::complex<double> a = Ra[idx(n, m)], b = Rb[idx(n, m)];
+        const int re = m == 0 ? 0 : 2 * m - 1, im = 2 * m;  // offsets of pk_re / pk_im inside the degree block
+        Am[re * d + i] = a.real();
+        Bm[re * d + i] = b.real();
+        Al[re * d + i] = (m ? 2. : 1.) * a.real();
+        Bl[re * d + i] = (m ? 2. : 1.) * b.real();
+        if (m) {
+          Am[im * d + i] = -a.imag();
+          Bm[im * d + i] = -b.imag();
+          Al[im * d + i] = -2. * a.imag();
+          Bl[im * d + i] = -2. * b.imag();
+        }
+      }
+    }
+    std::vector<double> Hn(d * d);
+    right_solve(Am, Bm, d, D + boff);  // D_n (row-major [out][in])
+    right_solve(Al, Bl, d, Hn.data());
+    for (int a = 0; a < d; ++a)
+      for (int c = 0; c < d; ++c) Ht[boff + a * d + c] = Hn[c * d + a];  // H_n^T (row-major [out][in])
+    boff += d * d;
+  }
+  // axial operator T(r z): per order m, block [j - m][n - m] (j, n = m..P), shared by Re and Im parts
+  const std::vector<qc> I = irregular(0, 0, (q)r, 2 * P);
+  int toff = 0;
+  for (int m = 0; m <= P; ++m) {
+    const int w = P + 1 - m;
+    for (int j = m; j <= P; ++j)
+      for (int n = m; n <= P; ++n) {
+        const q a = Anm(n, m) * Anm(j, m) / (((n & 1) ? (q)-1 : (q)1) * Anm(j + n, 0));
+        const qc v = a * (ipow(-2 * m) * I[idx(j + n, 0)]);
+        Tax[toff + (j - m) * w + (n - m)] = (double)v.re;
+      }
+    toff += w * w;
+  }
 }
 
 }  // namespace bemfmm

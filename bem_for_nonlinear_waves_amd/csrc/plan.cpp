@@ -771,10 +771,11 @@ std::shared_ptr<const std::vector<double>> cached_operator(int P, int kind, int 
     if (it != cache.end()) return it->second;
   }
   const int C = (P + 1) * (P + 1);
-  auto v = std::make_shared<std::vector<double>>((size_t)C * C);
+  auto v = std::make_shared<std::vector<double>>(kind == 3 ? (size_t)m2l_rotation_size(P) : (size_t)C * C);
   if (kind == 0) m2m_operator(P, a, v->data());
   else if (kind == 1) l2l_operator(P, a, v->data());
-  else m2l_operator(P, a, b, c, v->data());
+  else if (kind == 2) m2l_operator(P, a, b, c, v->data());
+  else m2l_rotation_operator(P, a, b, c, v->data());
   std::lock_guard<std::mutex> lk(mu);
   return cache.emplace(key, std::move(v)).first->second;  // a concurrent insert of the same key wins; same values
 }

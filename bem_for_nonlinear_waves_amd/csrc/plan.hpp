@@ -101,11 +101,17 @@ void build_plan(Plan& pl, const Options& opt, int n_nodes, const double* X, int 
 // Real-field packed translation operators (operators.cpp), computed in __float128
 // and rounded once to double. Layout: out[k_in * C + r_out].
 void m2l_operator(int P, int dx, int dy, int dz, double* out);
+// M2L factored as H^T T(|o| z) D (rotation of the offset onto the z axis): out holds, row-major
+// [out][in], the degree blocks D_n ((2n+1)^2 each, n = 0..P), the axial blocks ((P+1-m)^2 each,
+// m = 0..P, indexed [j-m][n-m]) and the degree blocks of H^T; m2l_rotation_size(P) doubles.
+int m2l_rotation_size(int P);
+void m2l_rotation_operator(int P, int dx, int dy, int dz, double* out);
 void m2m_operator(int P, int octant, double* out);  // child -> parent; octant bit a set = child on + side of axis a
 void l2l_operator(int P, int octant, double* out);  // parent -> child
 
 // Process-wide cache of translation operators (computed once in float128, immutable afterwards):
-// kind 0 = M2M of octant a, 1 = L2L of octant a, 2 = M2L of lattice offset (a, b, c). Thread-safe.
+// kind 0 = M2M of octant a, 1 = L2L of octant a, 2 = M2L of lattice offset (a, b, c), 3 = the rotation
+// factors of that M2L (m2l_rotation_operator). Thread-safe.
 std::shared_ptr<const std::vector<double>> cached_operator(int P, int kind, int a, int b, int c);
 // (kind, a, b, c) of operator `op` of a plan: 0-7 M2M, 8-15 L2L, 16+ M2L offsets
 inline std::array<int, 4> op_key(const Plan& pl, int op) {

@@ -70,6 +70,9 @@ typedef struct bemfmm_opts {
                             Every array of this ABI stays in the caller's numbering either way. */
   int use_graphs;        /* 1 (default): replay the matvec from a HIP graph captured at the first matvec; 0: launch
                             its kernels one by one */
+  int m2l_kernel;        /* M2L of the FP32 operator (FP32_REFINE): 0 (default) = rotation where compiled for p
+                            (8, 10), else matrix cores; 1 = dense matrix-core tiles; 2 = rotation (error if
+                            not compiled for p). The FP64 operator always uses the dense matrix-core tiles. */
 } bemfmm_opts;
 
 typedef struct bemfmm_result {
@@ -96,6 +99,7 @@ typedef struct bemfmm_stats {
   double precond_avg_block;     /* Schwarz: mean block size */
   int n_ops_uploaded;           /* translation operators uploaded by the last prepare (the rest were resident) */
   int m2l_tile_pairs;           /* M2L pairs per tile in use (autotuned unless set in the options) */
+  int m2l_rotation;             /* 1 if the FP32 operator's M2L runs by rotation */
 } bemfmm_stats;
 
 /* Availability (metal_m2l.h: metal_m2l_is_available / metal_m2l_device_name). */

@@ -22,11 +22,14 @@ def _gpu_bvp(m, d, phi, phin, **kw):
     return b
 
 
-def test_fp32_operator_vs_oracle(small_tank, oracle_mod):
+@pytest.mark.parametrize("p,kernel", [(10, 1), (10, 2), (8, 2)])
+def test_fp32_operator_vs_oracle(small_tank, oracle_mod, p, kernel):
+    """M2L of the FP32 operator on dense matrix-core tiles (kernel 1) or by rotation (kernel 2)."""
     from bem_for_nonlinear_waves_amd import _lib
     m, d, phi, phin = small_tank
-    g = _gpu_bvp(m, d, phi, phin, p=10, leaf_max=32, precision=_lib.PRECISION_FP32_REFINE)
-    o = oracle_mod.Oracle(m, d, phi, phin, p=10, leaf_max=32)
+    g = _gpu_bvp(m, d, phi, phin, p=p, leaf_max=32, precision=_lib.PRECISION_FP32_REFINE, m2l_kernel=kernel)
+    assert g.stats()["m2l_rotation"] == (1 if kernel == 2 else 0)
+    o = oracle_mod.Oracle(m, d, phi, phin, p=p, leaf_max=32)
     rng = np.random.default_rng(11)
     for _ in range(2):
         x = rng.standard_normal(d.n_dof)
@@ -95,3 +98,43 @@ def test_fp32_matvec_requires_refine_mode(tiny_tank):
         g.compute_Ax_fp32(np.zeros(d.n_dof))
     assert e.value.code == _lib.E_STATE
     g.close()
+
+
+def test_rotation_m2l_matches_dense_tiles(small_tank):
+    """The rotation M2L (H^T T(|o| z) D per pair, FP32 factors) and the dense matrix-core M2L give the same
+    FP32 operator to FP32 rounding; the default picks the rotation for p = 10; orders without a compiled
+    rotation kernel are refused when it is requested and fall back to the dense tiles by default."""
+    from bem_for_nonlinear_waves_amd import _lib
+    from bem_for_nonlinear_waves_amd.bvp import BEM_BVP, BEMError
+    m, d, phi, phin = small_tank
+    kw = dict(p=10, leaf_max=32, precision=_lib.PRECISION_FP32_REFINE)
+    a = _gpu_bvp(m, d, phi, phin, m2l_kernel=1, **kw)
+    b = _gpu_bvp(m, d, phi, phin, **kw)
+    assert b.stats()["m2l_rotation"] == 1
+    rng = np.random.default_rng(21)
+    for _ in range(3):
+        x = rng.standard_normal(d.n_dof)
+        ya, yb = a.compute_Ax_fp32(x), b.compute_Ax_fp32(x)
+        rel = np.linalg.norm(ya - yb) / np.linalg.norm(ya)
+        print(f"[gpu] rotation vs dense FP32 M2L rel = {rel:.3e}")
+        assert rel <= 5e-6
+    # a re-prepare with moved nodes reuses the resident factors and stays consistent
+    from bem_for_nonlinear_waves_amd import mesh as M
+    m2 = M.perturb_free_surface(m, 9, 0.001)
+    for g in (a, b):
+        g.set_geometry(m2, d)
+        g.set_bc(phi, phin)
+        g.prepare()
+    x = rng.standard_normal(d.n_dof)
+    assert np.linalg.norm(a.compute_Ax_fp32(x) - b.compute_Ax_fp32(x)) <= 5e-6 * np.linalg.norm(a.compute_Ax_fp32(x))
+    a.close()
+    b.close()
+    c = BEM_BVP(m, d, p=9, leaf_max=32, precision=_lib.PRECISION_FP32_REFINE, m2l_kernel=2)
+    c.set_bc(phi, phin)
+    with pytest.raises(BEMError) as e:
+        c.prepare()
+    assert e.value.code == _lib.E_INVALID
+    c.close()
+    c = _gpu_bvp(m, d, phi, phin, p=9, leaf_max=32, precision=_lib.PRECISION_FP32_REFINE)
+    assert c.stats()["m2l_rotation"] == 0
+    c.close()
