@@ -588,14 +588,18 @@ __device__ __forceinline__ void rot_step(const float4* __restrict__ R4, float* v
   if constexpr (rw.stage != 1) {
     // degree block: tmp[a] = sum_c B[a][c] v[n^2 + c]
     constexpr int b = rw.blk * rw.blk;
-    float acc = 0.f;
+    float acc0 = 0.f, acc1 = 0.f;  // two chains (even / odd columns)
 #pragma unroll
     for (int c4 = 0; c4 < rw.q; ++c4) {
       const float e[4] = {s.cur[c4].x, s.cur[c4].y, s.cur[c4].z, s.cur[c4].w};
 #pragma unroll
       for (int u = 0; u < 4; ++u)
-        if (4 * c4 + u < rw.len) acc = fmaf(e[u], v[b + 4 * c4 + u], acc);
+        if (4 * c4 + u < rw.len) {
+          if (u & 1) acc1 = fmaf(e[u], v[b + 4 * c4 + u], acc1);
+          else acc0 = fmaf(e[u], v[b + 4 * c4 + u], acc0);
+        }
     }
+    const float acc = rw.len > 1 ? acc0 + acc1 : acc0;
     s.tmp[rw.a] = acc;
     s.last = acc;
     if constexpr (rw.a == rw.len - 1) {
@@ -726,6 +730,49 @@ __global__ __launch_bounds__(256) void k_m2l_rot(int ntiles, const float* __rest
       if (col < 2 * np && r < C) out[(int64_t)col * C + r] = T[col * kS + k];
     }
     wave_lds_sync();
+  }
+}
+
+// FP32 staging: the same reduction with 8-byte loads. Thread t < C of each half-block reads floats
+// (2t, 2t+1) of a pair's 2C staged values; the two half-blocks take alternate pairs, 4 pairs in flight
+// each, and are combined in a fixed order (deterministic).
+__global__ __launch_bounds__(256) void k_m2l_reduce_f32(int C, const int64_t* __restrict__ tptr,
+                                                        const int32_t* __restrict__ tpairs,
+                                                        const float* __restrict__ stage, double* __restrict__ L0,
+                                                        double* __restrict__ L1) {
+  __shared__ double2 part[128];
+  const int tb = blockIdx.x;
+  const int h = threadIdx.x >> 7, t = threadIdx.x & 127;
+  const bool act = t < C;
+  const int64_t a = tptr[tb], b = tptr[tb + 1];
+  double sx[4] = {0., 0., 0., 0.}, sy[4] = {0., 0., 0., 0.};
+  if (act) {
+    int64_t q = a + h;
+    for (; q + 6 < b; q += 8) {
+      float2 v[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) v[u] = reinterpret_cast<const float2*>(stage + (int64_t)tpairs[q + 2 * u] * 2 * C)[t];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        sx[u] += v[u].x;
+        sy[u] += v[u].y;
+      }
+    }
+    for (; q < b; q += 2) {
+      const float2 v = reinterpret_cast<const float2*>(stage + (int64_t)tpairs[q] * 2 * C)[t];
+      sx[0] += v.x;
+      sy[0] += v.y;
+    }
+  }
+  double x = (sx[0] + sx[1]) + (sx[2] + sx[3]), y = (sy[0] + sy[1]) + (sy[2] + sy[3]);
+  if (h == 1 && act) part[t] = double2{x, y};
+  __syncthreads();
+  if (h == 0 && act) {
+    x += part[t].x;
+    y += part[t].y;
+    const int j = 2 * t;
+    (j < C ? L0[(int64_t)tb * C + j] : L1[(int64_t)tb * C + j - C]) = x;
+    (j + 1 < C ? L0[(int64_t)tb * C + j + 1] : L1[(int64_t)tb * C + j + 1 - C]) = y;
   }
 }
 
@@ -1387,7 +1434,7 @@ void evaluate_launch(bemfmm_ctx* c, int mode, const double* in, double* out, boo
     if (c->m2l_rot) launch_m2l_rot(c, st);
     else launch_translate(c, c->m2l, c->ops32.p, c->M0.p, c->M1.p, nullptr, nullptr, st);
     rec(4);
-    hipLaunchKernelGGL(k_m2l_reduce<float>, dim3(c->n_boxes), dim3(256), 0, c->stream, c->C, c->m2l_tptr.p,
+    hipLaunchKernelGGL(k_m2l_reduce_f32, dim3(c->n_boxes), dim3(256), 0, c->stream, c->C, c->m2l_tptr.p,
                        c->m2l_tpairs.p, (const float*)st, L0, L1);
   } else {
     launch_translate(c, c->m2l, c->ops.p, c->M0.p, c->M1.p, nullptr, nullptr, c->m2l_stage.p);

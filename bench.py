@@ -42,8 +42,8 @@ FP64_PEAK_TFLOPS = 78.6
 # HBM bytes per launch of the dominant kernels on the default workload (h = 0.036, p = 10, leaf 32,
 # depth 7), from rocprofv3 FETCH_SIZE (scaled x2) and WRITE_SIZE (x1) passes over
 # `bench.py --profile-only` (summarised in profiles/fmm_matvec_cylinder209k_p10.md); reported only
-# for that workload. Keys: (stage, FP32 operator).
-MEASURED_TRAFFIC = {("m2l", True): 5.943e9, ("m2l", False): 8.618e9,
+# for that workload. Keys: (stage, FP32 operator); "m2l_rot" = the rotation M2L of the FP32 operator.
+MEASURED_TRAFFIC = {("m2l", True): 5.943e9, ("m2l", False): 8.618e9, ("m2l_rot", True): 4.249e9,
                     ("near_l2p", True): 5.177e9, ("near_l2p", False): 9.896e9}
 
 
@@ -105,13 +105,15 @@ def setup_bvp(m, d, phi, phin, args, precision):
     pc = {"none": _lib.PRECOND_NONE, "jacobi": _lib.PRECOND_JACOBI, "schwarz": _lib.PRECOND_SCHWARZ}[args.precond]
     prec = _lib.PRECISION_FP32_REFINE if precision == "fp32_refine" else _lib.PRECISION_FP64
     bvp = BEM_BVP(m, d, p=args.p, leaf_max=args.leaf, max_level=args.max_level, preconditioner=pc, precision=prec,
-                  refine_eta=args.refine_eta, m2l_tile_pairs=args.m2l_tile_pairs)
+                  refine_eta=args.refine_eta, m2l_tile_pairs=args.m2l_tile_pairs,
+                  m2l_kernel={"auto": 0, "dense": 1, "rotation": 2}[args.m2l_kernel])
     bvp.set_bc(phi, phin)
     t = time.perf_counter()
     bvp.prepare()
     t_setup = time.perf_counter() - t
     st = bvp.stats()
-    _log(f"[{precision}] M2L tile: {st['m2l_tile_pairs']} pairs; operators uploaded {st['n_ops_uploaded']}")
+    _log(f"[{precision}] M2L tile: {st['m2l_tile_pairs']} pairs; operators uploaded {st['n_ops_uploaded']}; "
+         f"FP32 M2L by rotation: {bool(st['m2l_rotation'])}")
     if st["n_precond_blocks"]:
         _log(f"[{precision}] schwarz blocks={st['n_precond_blocks']} avg={st['precond_avg_block']:.1f} "
              f"fallback={st['precond_fallback_blocks']}")
@@ -357,6 +359,8 @@ def main():
     ap.add_argument("--precision", choices=["fp32_refine", "fp64"], default="fp32_refine")
     ap.add_argument("--refine-eta", type=float, default=1e-5)
     ap.add_argument("--m2l-tile-pairs", type=int, default=0, choices=[0, 16, 32, 64])
+    ap.add_argument("--m2l-kernel", default="auto", choices=["auto", "dense", "rotation"],
+                    help="M2L of the FP32 operator: rotation where compiled (p = 8, 10), or dense matrix-core tiles")
     ap.add_argument("--no-fp64-compare", action="store_true",
                     help="skip the plain FP64 solves timed beside the FP32+refinement ones")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -426,22 +430,34 @@ def main():
     C = (args.p + 1) ** 2
     esz = 4.0 if fp32 else 8.0
     mfma_peak = 157.3 if fp32 else FP64_PEAK_TFLOPS
-    m2l_flops = st["n_m2l_pairs"] * 2 * C * C * 2.0
+    rot = bool(fp32 and st.get("m2l_rotation", 0))
+    # algorithmic multiply-adds per pair and layer: dense C x C operator, or the rotation factors
+    # (D and H^T: (2n+1)^2 per degree; axial: (p+1)^2 for m = 0 and 2 (p+1-m)^2 for m > 0)
+    pp = args.p
+    rot_macs = 2 * sum((2 * k + 1) ** 2 for k in range(pp + 1)) + (pp + 1) ** 2 + 2 * sum((pp + 1 - k) ** 2 for k in range(1, pp + 1))
+    m2l_flops = st["n_m2l_pairs"] * 2 * (rot_macs if rot else C * C) * 2.0
     near_bytes = (st["near_nnz"] + m.n_nodes * C) * 2 * esz  # dense G/Gn weights incl. L2P columns
     dom = int(np.argmax(stage))
     if STAGES[dom] == "m2l":
         ach = m2l_flops / (stage[dom] * 1e-3) / 1e12
-        roof = {"kernel": "k_translate_mfma (M2L)", "bound": "mfma", "achieved": ach, "peak": mfma_peak,
-                "unit": "TFLOP/s", "frac": ach / mfma_peak, "traffic": None}
+        if rot:
+            # FP32 FMAs on the vector ALUs; MI355X's published FP32 peak is 157.3 TFLOP/s for vector and matrix
+            roof = {"kernel": "k_m2l_rot (M2L by rotation, FP32 FMA)", "bound": "mfma", "engine": "valu",
+                    "achieved": ach, "peak": mfma_peak, "unit": "TFLOP/s", "frac": ach / mfma_peak, "traffic": None}
+        else:
+            roof = {"kernel": "k_translate_mfma (M2L)", "bound": "mfma", "achieved": ach, "peak": mfma_peak,
+                    "unit": "TFLOP/s", "frac": ach / mfma_peak, "traffic": None}
     else:
         ach = near_bytes / (stage[dom] * 1e-3) / 1e9
         roof = {"kernel": f"k_block_gemv ({STAGES[dom]})", "bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "traffic": None}
     default_workload = (args.h, args.p, args.leaf, args.max_level) == (0.036, 10, 32, 7)
     if default_workload:
-        roof["traffic"] = MEASURED_TRAFFIC.get((STAGES[dom], fp32))
+        roof["traffic"] = MEASURED_TRAFFIC.get(("m2l_rot" if rot and STAGES[dom] == "m2l" else STAGES[dom], fp32))
     roof["stage_ms"] = dict(zip(STAGES, [round(float(v), 4) for v in stage]))
     roof["m2l_tflops"] = m2l_flops / (stage[3] * 1e-3) / 1e12
+    if rot:  # the same M2L as dense operator multiply-adds per second
+        roof["m2l_dense_equivalent_tflops"] = st["n_m2l_pairs"] * 2 * C * C * 2.0 / (stage[3] * 1e-3) / 1e12
     roof["near_gbs"] = near_bytes / (stage[6] * 1e-3) / 1e9
     roof["fp64_stage_ms"] = dict(zip(STAGES, [round(float(v), 4) for v in stage64]))
 

@@ -234,12 +234,15 @@ int emu_near_row(void* h, int t, int32_t* idx, double* wg, double* wgn, int cap)
     }
   return -1;
 }
-// one translation operator (for tests): kind 0 = M2M(octant a), 1 = L2L(octant a), 2 = M2L(a,b,c)
+// one translation operator (for tests): kind 0 = M2M(octant a), 1 = L2L(octant a), 2 = M2L(a,b,c),
+// 3 = rotation factors of M2L(a,b,c) (emu_rotation_size(p) doubles)
 void emu_operator(int p, int kind, int a, int b, int c, double* out) {
   if (kind == 0) bemfmm::m2m_operator(p, a, out);
   else if (kind == 1) bemfmm::l2l_operator(p, a, out);
-  else bemfmm::m2l_operator(p, a, b, c, out);
+  else if (kind == 2) bemfmm::m2l_operator(p, a, b, c, out);
+  else bemfmm::m2l_rotation_operator(p, a, b, c, out);
 }
+int emu_rotation_size(int p) { return bemfmm::m2l_rotation_size(p); }
 }
 
 extern "C" {

@@ -31,6 +31,8 @@ def emu():
     L.emu_alpha.argtypes = [ctypes.c_void_p, _dp]
     L.emu_stats.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64)]
     L.emu_operator.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp]
+    L.emu_rotation_size.argtypes = [ctypes.c_int]
+    L.emu_rotation_size.restype = ctypes.c_int
     return L
 
 
@@ -98,3 +100,52 @@ def test_m2l_operator_reflection_symmetry(emu):
             s[n * n + 2 * m - 1] = (-1) ** m
             s[n * n + 2 * m] = -((-1) ** m)
     assert np.abs(np.diag(s) @ A @ np.diag(s) - B).max() <= 1e-13 * np.abs(A).max()
+
+
+def _rotation_blocks(emu, P, o):
+    """(D, T, H^T) as dense (P+1)^2 matrices [out, in] from the compact factors of M2L offset o."""
+    C = (P + 1) ** 2
+    f = np.empty(emu.emu_rotation_size(P))
+    emu.emu_operator(P, 3, o[0], o[1], o[2], _p(f, _dp))
+    D, T, Ht = np.zeros((C, C)), np.zeros((C, C)), np.zeros((C, C))
+    k = 0
+    for n in range(P + 1):
+        d = 2 * n + 1
+        D[n * n:n * n + d, n * n:n * n + d] = f[k:k + d * d].reshape(d, d)
+        k += d * d
+
+    def col(n, m, part):  # packed index of (n, m): real part, or imaginary part for m > 0
+        return n * n if m == 0 else n * n + 2 * m - 1 + part
+    for m in range(P + 1):
+        w = P + 1 - m
+        blk = f[k:k + w * w].reshape(w, w)
+        k += w * w
+        for part in ((0,) if m == 0 else (0, 1)):
+            for j in range(m, P + 1):
+                for n in range(m, P + 1):
+                    T[col(j, m, part), col(n, m, part)] = blk[j - m, n - m]
+    for n in range(P + 1):
+        d = 2 * n + 1
+        Ht[n * n:n * n + d, n * n:n * n + d] = f[k:k + d * d].reshape(d, d)
+        k += d * d
+    assert k == f.size
+    return D, T, Ht
+
+
+@pytest.mark.parametrize("P", [8, 10])
+def test_m2l_rotation_factors(emu, P):
+    """The rotation M2L factors (used by the FP32 operator's M2L kernel) reproduce the dense float128-built
+    M2L operator: T(o) = H^T T(|o| z) D for offsets in every octant, on the axes and near the x axis (the
+    other branch of the rotation frame). On the +z axis the rotations are the identity."""
+    C = (P + 1) ** 2
+    offsets = [(2, 3, -1), (4, 0, 0), (-4, 0, 0), (0, 0, 5), (0, 0, -4), (0, 3, 0), (-3, 2, 2), (1, -6, 3),
+               (2, 2, 2), (-7, -7, -7), (6, 1, -1), (-2, -5, 4), (3, -3, 0)]
+    for o in offsets:
+        A = np.empty(C * C)
+        emu.emu_operator(P, 2, o[0], o[1], o[2], _p(A, _dp))
+        A = A.reshape(C, C).T  # [out, in]
+        D, T, Ht = _rotation_blocks(emu, P, o)
+        err = np.abs(Ht @ T @ D - A).max() / np.abs(A).max()
+        assert err <= 1e-12, (o, err)
+    D, T, Ht = _rotation_blocks(emu, P, (0, 0, 3))
+    assert np.abs(D - np.eye(C)).max() <= 1e-12 and np.abs(Ht - np.eye(C)).max() <= 1e-12
