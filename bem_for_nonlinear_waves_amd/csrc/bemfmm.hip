@@ -491,20 +491,25 @@ __global__ __launch_bounds__(256) void k_m2l_reduce(int C, const int64_t* __rest
 // Output: stage[(pair*2 + layer)*C + row], as the matrix-core M2L with STAGE.
 constexpr int kRotWavePairs = 32;
 constexpr int kRotTP = 4 * kRotWavePairs;  // pairs per workgroup tile
-// Device layout of the factors of one offset, in float4s: every row is zero-padded to whole float4s
-// (aligned broadcast reads); blocks D_0..D_P (d = 2n+1 rows), axial T_0..T_P (P+1-m rows), H^T_0..H^T_P.
+// Device layout of the factors of one offset, in float4s, every row zero-padded to whole float4s
+// (aligned broadcast reads): (cos m phi, sin m phi) for m = 1..P; per degree n the y-rotation blocks
+// Y_D (cos-type (n+1) rows, sin-type n rows); per order m the axial block (P+1-m rows); per degree the
+// blocks of Y_H^T. Row lengths equal the block sizes.
 __host__ __device__ constexpr int rot_q(int len) { return (len + 3) / 4; }
 __host__ __device__ constexpr int rot_size4(int P) {
-  int s = 0;
-  for (int n = 0; n <= P; ++n) s += 2 * (2 * n + 1) * rot_q(2 * n + 1);
+  int s = rot_q(2 * P);
+  for (int n = 0; n <= P; ++n) s += 2 * ((n + 1) * rot_q(n + 1) + n * rot_q(n));
   for (int m = 0; m <= P; ++m) s += (P + 1 - m) * rot_q(P + 1 - m);
   return s;
 }
 __host__ __device__ constexpr int rot_size(int P) { return 4 * rot_size4(P); }  // floats per offset
-// compact row-major factors (m2l_rotation_operator) -> the padded device layout
+// compact factors (m2l_rotation_operator) -> the padded device layout
 void rot_pad(int P, const double* in, float* out) {
   std::fill(out, out + rot_size(P), 0.f);
   int io = 0, oo = 0;
+  for (int k = 0; k < 2 * P; ++k) out[oo + k] = (float)in[io + k];
+  io += 2 * P;
+  oo += 4 * rot_q(2 * P);
   auto block = [&](int rows) {
     const int q = 4 * rot_q(rows);
     for (int a = 0; a < rows; ++a)
@@ -512,9 +517,15 @@ void rot_pad(int P, const double* in, float* out) {
     io += rows * rows;
     oo += rows * q;
   };
-  for (int n = 0; n <= P; ++n) block(2 * n + 1);
+  for (int n = 0; n <= P; ++n) {
+    block(n + 1);
+    block(n);
+  }
   for (int m = 0; m <= P; ++m) block(P + 1 - m);
-  for (int n = 0; n <= P; ++n) block(2 * n + 1);
+  for (int n = 0; n <= P; ++n) {
+    block(n + 1);
+    block(n);
+  }
 }
 __device__ constexpr int rot_re(int n, int m) { return m == 0 ? n * n : n * n + 2 * m - 1; }
 __device__ constexpr int rot_im(int n, int m) { return n * n + 2 * m; }
@@ -524,39 +535,43 @@ __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_wave_barrier();
 }
 
-// Rows of the three factor stages, in execution order: stage 0 = D (degree n, row a), 1 = axial (order m,
-// row j), 2 = H^T (degree n, row a). Each row is a dot product of one padded factor row (q float4s at
-// float4 offset off) with the lane's coefficients. The rows run software-pipelined: the factors of row r+1
-// are read from LDS before the multiply-adds of row r, with scheduling barriers in between, so every LDS
-// read has a whole row of FMAs to complete (the scheduler alone keeps one read in flight).
+// Rows of the three factor stages, in execution order: stage 0 = Y_D (degree n: cos-type rows a = 0..n,
+// then sin-type rows a = 0..n-1), 1 = axial (order m, rows j = m + a), 2 = Y_H^T (as stage 0). Each row is
+// a dot product of one padded factor row (q float4s at float4 offset off) with the lane's coefficients.
+// The z-rotations (2x2 per order) run before a degree's Y_D rows and after its Y_H^T rows. The rows run
+// software-pipelined: the factors of row r+1 are read from LDS before the multiply-adds of row r, with
+// scheduling barriers in between, so every LDS read has a whole row of FMAs to complete.
 struct RotRow {
-  int stage, blk, a, len, q, off;
+  int stage, blk, kind, a, len, q, off;  // kind: 0 = cos-type, 1 = sin-type (stages 0, 2)
 };
 template <int P>
-__host__ __device__ constexpr int rot_rows() { return 3 * (P + 1) * (P + 1); }
+__host__ __device__ constexpr int rot_rows() { return 2 * (P + 1) * (P + 1) + (P + 1) * (P + 2) / 2; }
 // descriptor of row K (compile-time)
 template <int P>
 __host__ __device__ constexpr RotRow rot_row(int K) {
-  int k = 0, off = 0;
+  int k = 0, off = rot_q(2 * P);
   for (int st = 0; st < 3; ++st) {
     if (st == 1) {
       for (int m = 0; m <= P; ++m) {
         const int w = P + 1 - m;
-        if (K < k + w) return RotRow{1, m, K - k, w, rot_q(w), off + (K - k) * rot_q(w)};
+        if (K < k + w) return RotRow{1, m, 0, K - k, w, rot_q(w), off + (K - k) * rot_q(w)};
         k += w;
         off += w * rot_q(w);
       }
       continue;
     }
-    for (int n = 0; n <= P; ++n) {
-      const int d = 2 * n + 1;
-      if (K < k + d) return RotRow{st, n, K - k, d, rot_q(d), off + (K - k) * rot_q(d)};
-      k += d;
-      off += d * rot_q(d);
-    }
+    for (int n = 0; n <= P; ++n)
+      for (int kind = 0; kind < 2; ++kind) {
+        const int w = kind == 0 ? n + 1 : n;
+        if (K < k + w) return RotRow{st, n, kind, K - k, w, rot_q(w), off + (K - k) * rot_q(w)};
+        k += w;
+        off += w * rot_q(w);
+      }
   }
-  return RotRow{-1, 0, 0, 0, 0, 0};
+  return RotRow{-1, 0, 0, 0, 0, 0, 0};
 }
+// packed index of entry i of the cos-type (Re of m = 0..n) / sin-type (Im of m = 1..n) list of degree n
+__device__ constexpr int rot_cs(int n, int kind, int i) { return kind == 0 ? rot_re(n, i) : rot_im(n, i + 1); }
 
 // 0, but opaque to the compiler and available only once `dep` is computed: offsetting the reads of row
 // r+1 by it (dep = the result of row r-1) keeps the compiler from clustering and hoisting the factor reads
@@ -569,11 +584,31 @@ __device__ __forceinline__ int rot_after(float dep) {
 
 template <int P>
 struct RotPipe {
-  static constexpr int QM = rot_q(2 * P + 1);
+  static constexpr int QM = rot_q(P + 1);
   float4 cur[QM], nxt[QM];
-  float tmp[2 * P + 1], ti[P + 1];
+  float tmp[P + 1], ti[P + 1];
   float last = 0.f;  // result of the previous row
 };
+
+// (Re, Im) of orders 1..n of degree n rotated about z: forward (c re - s im, s re + c im), or transposed
+template <int P, bool kTransposed>
+__device__ __forceinline__ void rot_z(const float4* __restrict__ R4, float* v, int n) {
+#pragma unroll
+  for (int m = 1; m <= n; ++m) {
+    const float4 cs = R4[(m - 1) / 2];
+    const float c = (m & 1) ? cs.x : cs.z, s = (m & 1) ? cs.y : cs.w;
+    const float re = v[rot_re(n, m)], im = v[rot_im(n, m)];
+    if (kTransposed) {
+      v[rot_re(n, m)] = fmaf(c, re, s * im);
+      v[rot_im(n, m)] = fmaf(c, im, -s * re);
+      // final values: computed here, not sunk to the scatter (which would keep every degree's inputs live)
+      asm volatile("" : "+v"(v[rot_re(n, m)]), "+v"(v[rot_im(n, m)]));
+    } else {
+      v[rot_re(n, m)] = fmaf(c, re, -s * im);
+      v[rot_im(n, m)] = fmaf(c, im, s * re);
+    }
+  }
+}
 
 template <int P, int K>
 __device__ __forceinline__ void rot_step(const float4* __restrict__ R4, float* v, RotPipe<P>& s) {
@@ -586,17 +621,18 @@ __device__ __forceinline__ void rot_step(const float4* __restrict__ R4, float* v
   }
   __builtin_amdgcn_sched_barrier(0);
   if constexpr (rw.stage != 1) {
-    // degree block: tmp[a] = sum_c B[a][c] v[n^2 + c]
-    constexpr int b = rw.blk * rw.blk;
-    float acc0 = 0.f, acc1 = 0.f;  // two chains (even / odd columns)
+    constexpr int n = rw.blk;
+    if constexpr (rw.stage == 0 && rw.kind == 0 && rw.a == 0 && n > 0) rot_z<P, false>(R4 + rot_after(s.last), v, n);
+    // y-rotation row: tmp[a] = sum_i B[a][i] v[list(i)] over the degree's cos- or sin-type list
+    float acc0 = 0.f, acc1 = 0.f;  // two chains (even / odd entries)
 #pragma unroll
     for (int c4 = 0; c4 < rw.q; ++c4) {
       const float e[4] = {s.cur[c4].x, s.cur[c4].y, s.cur[c4].z, s.cur[c4].w};
 #pragma unroll
       for (int u = 0; u < 4; ++u)
         if (4 * c4 + u < rw.len) {
-          if (u & 1) acc1 = fmaf(e[u], v[b + 4 * c4 + u], acc1);
-          else acc0 = fmaf(e[u], v[b + 4 * c4 + u], acc0);
+          if (u & 1) acc1 = fmaf(e[u], v[rot_cs(n, rw.kind, 4 * c4 + u)], acc1);
+          else acc0 = fmaf(e[u], v[rot_cs(n, rw.kind, 4 * c4 + u)], acc0);
         }
     }
     const float acc = rw.len > 1 ? acc0 + acc1 : acc0;
@@ -604,7 +640,8 @@ __device__ __forceinline__ void rot_step(const float4* __restrict__ R4, float* v
     s.last = acc;
     if constexpr (rw.a == rw.len - 1) {
 #pragma unroll
-      for (int a = 0; a < rw.len; ++a) v[b + a] = s.tmp[a];
+      for (int a = 0; a < rw.len; ++a) v[rot_cs(n, rw.kind, a)] = s.tmp[a];
+      if constexpr (rw.stage == 2 && n > 0 && rw.kind == 1) rot_z<P, true>(R4 + rot_after(acc), v, n);
     }
   } else {
     // axial block of order m: row j = m + a, inputs n = m..P (real part, and imaginary part for m > 0)
@@ -647,7 +684,7 @@ __device__ __forceinline__ void rot_run(const float4* __restrict__ R4, float* v,
   (rot_step<P, K>(R4, v, s), ...);
 }
 
-// v <- H^T T(|o| z) D v with the factors of one offset at R4 (LDS)
+// v <- H^T T(|o| z) D v (D = Y_D Z, H^T = Z^T Y_H^T) with the factors of one offset at R4 (LDS)
 template <int P>
 __device__ __forceinline__ void rot_apply(const float4* __restrict__ R4, float* v) {
   rot_run<P>(R4, v, std::make_integer_sequence<int, rot_rows<P>()>{});
@@ -714,7 +751,9 @@ __global__ __launch_bounds__(256) void k_m2l_rot(int ntiles, const float* __rest
     wave_lds_sync();
   }
   // ---- D, axial M2L, H^T ----
+  __builtin_amdgcn_sched_barrier(0);
   rot_apply<P>(R4, v);
+  __builtin_amdgcn_sched_barrier(0);
   // ---- scatter: lane's registers -> LDS -> rows [16 ch, 16 ch + 16) of column (4i + g) ----
   float* out = stage + pw * 2 * C;
 #pragma unroll

@@ -20,6 +20,7 @@
 // (boxes of a level are a regular lattice of pitch s, children have side s/2).
 #include <quadmath.h>
 
+#include <algorithm>
 #include <cmath>
 #include <complex>
 #include <cstdint>
@@ -214,15 +215,25 @@ void l2l_operator(int P, int octant, double* out) {
 
 
 // ---------------------------------------------------------------------------
-// M2L by rotation: T(o) = H^T T(|o| z) D (checked in float128 to 1e-34 relative).
-// D (multipole coefficients) and H (L2P coefficient vectors) are the block-diagonal (one (2n+1)^2
-// block per degree) real representations of the rotation Q with Q o = |o| z: packed-P2M(Q y) =
-// D packed-P2M(y), L2P-row(Q x) = H L2P-row(x); since |Qx - Qy| = |x - y|, the local expansion of
-// the rotated configuration L' = T(|o| z) D M satisfies L = H^T L'. T(|o| z) couples (j,k) only with
-// (n,k) (harmonics on the z axis have order 0), the same block for the real and imaginary parts.
-// The blocks are found by probing: the regular harmonics of degree n at 2n+1 points (the basis is the
-// normalised one, so the probe systems are well conditioned) - double precision is enough for the
-// FP32 operator these factors serve.
+// M2L by rotation: T(o) = H^T T(|o| z) D, with the rotation Q = R_y(-theta) R_z(-phi) taking the offset
+// o = |o| (sin theta cos phi, sin theta sin phi, cos theta) onto +z. D (multipole coefficients) and H
+// (L2P coefficient vectors) are the real representations of Q in the packed basis: packed-P2M(Q y) =
+// D packed-P2M(y), L2P-row(Q x) = H L2P-row(x); since |Qx - Qy| = |x - y|, the local expansion of the
+// rotated configuration L' = T(|o| z) D M satisfies L = H^T L'. Per degree n:
+//   * R_z(-phi) rotates (Re, Im) of each order m by m phi: [[c, -s], [s, c]], c = cos m phi,
+//     s = sin m phi, the same for D and H;
+//   * R_y(-theta) commutes with the reflection y -> -y, so it maps cos-type coefficients
+//     {Re (n, 0..n)} among themselves ((n+1)^2 block) and sin-type {Im (n, 1..n)} among themselves
+//     (n^2 block);
+//   * T(|o| z) couples (j, k) only with (n, k), the same block for the real and imaginary parts.
+// The y-rotation blocks are found by probing the regular harmonics of degree n at 2n+1 pseudo-random
+// points (the packed basis is the normalised one, so the systems are well conditioned) in FP64.
+// Layout of `out` (m2l_rotation_size(P) doubles, blocks row-major [out][in]):
+//   zc: (cos m phi, sin m phi) for m = 1..P;
+//   for n = 0..P: Y_D cos block (n+1)^2, Y_D sin block n^2;
+//   for m = 0..P: axial block (P+1-m)^2, indexed [j-m][n-m];
+//   for n = 0..P: Y_H^T cos block (n+1)^2, Y_H^T sin block n^2.
+// The map is then: D = Y_D Z, H^T = Z^T Y_H^T (Z from zc).
 // ---------------------------------------------------------------------------
 namespace {
 void regular_d(double x, double y, double z, int L, std::vector<std::complex<double>>& R) {
@@ -277,30 +288,11 @@ void right_solve(std::vector<double> A, const std::vector<double>& B, int d, dou
       out[i * d + j] = s;
     }
 }
-}  // namespace
-
-int m2l_rotation_size(int P) {
-  int s = 0;
-  for (int n = 0; n <= P; ++n) s += 2 * (2 * n + 1) * (2 * n + 1);
-  for (int m = 0; m <= P; ++m) s += (P + 1 - m) * (P + 1 - m);
-  return s;
-}
-
-void m2l_rotation_operator(int P, int dx, int dy, int dz, double* out) {
-  const double r = std::sqrt((double)dx * dx + (double)dy * dy + (double)dz * dz);
-  const double ez[3] = {dx / r, dy / r, dz / r};
-  double t[3] = {1., 0., 0.};
-  if (std::fabs(ez[0]) > 0.9) {
-    t[0] = 0.;
-    t[1] = 1.;
-  }
-  const double dt = t[0] * ez[0] + t[1] * ez[1] + t[2] * ez[2];
-  double ex[3] = {t[0] - dt * ez[0], t[1] - dt * ez[1], t[2] - dt * ez[2]};
-  const double nx = std::sqrt(ex[0] * ex[0] + ex[1] * ex[1] + ex[2] * ex[2]);
-  for (double& v : ex) v /= nx;
-  const double ey[3] = {ez[1] * ex[2] - ez[2] * ex[1], ez[2] * ex[0] - ez[0] * ex[2], ez[0] * ex[1] - ez[1] * ex[0]};
-  // probe points: fixed pseudo-random points of the cube [-1, 1]^3 (splitmix64)
-  std::uint64_t seed = 0x9e3779b97f4a7c15ull;
+// the (2n+1)^2 transform of degree n under x -> R_y(-theta) x: kind 0 = packed P2M vectors (Y_D),
+// 1 = L2P rows (Y_H); row-major [out][in] over the packed degree block
+void y_rotation(int n, double theta, int kind, std::vector<double>& X) {
+  const int d = 2 * n + 1;
+  std::uint64_t seed = 0x9e3779b97f4a7c15ull * (std::uint64_t)(2 * n + 1);
   auto unit = [&seed]() {
     std::uint64_t z = (seed += 0x9e3779b97f4a7c15ull);
     z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
@@ -308,65 +300,71 @@ void m2l_rotation_operator(int P, int dx, int dy, int dz, double* out) {
     z ^= z >> 31;
     return 2. * (double)(z >> 11) / 9007199254740992. - 1.;
   };
-  auto probe = [&unit](int, int, double* p) {
-    for (int a = 0; a < 3; ++a) p[a] = unit();
-  };
-  double* D = out;
-  int off = 0;
-  for (int n = 0; n <= P; ++n) off += (2 * n + 1) * (2 * n + 1);
-  double* Tax = out + off;
-  for (int m = 0; m <= P; ++m) off += (P + 1 - m) * (P + 1 - m);
-  double* Ht = out + off;
+  const double c = std::cos(theta), s = std::sin(theta);
+  std::vector<double> A(d * d), B(d * d);
   std::vector<std::complex<double>> Ra, Rb;
-  int boff = 0;
-  for (int n = 0; n <= P; ++n) {
-    const int d = 2 * n + 1;
-    std::vector<double> Am(d * d), Bm(d * d), Al(d * d), Bl(d * d);
-    for (int i = 0; i < d; ++i) {
-      double pnt[3], qp[3];
-      probe(i, d, pnt);
-      for (int a = 0; a < 3; ++a) qp[a] = 0.;
-      for (int a = 0; a < 3; ++a) {
-        qp[0] += ex[a] * pnt[a];
-        qp[1] += ey[a] * pnt[a];
-        qp[2] += ez[a] * pnt[a];
-      }
-      regular_d(pnt[0], pnt[1], pnt[2], n, Ra);
-      regular_d(qp[0], qp[1], qp[2], n, Rb);
-      for (int m = 0; m <= n; ++m) {
-        const std::complex<double> a = Ra[idx(n, m)], b = Rb[idx(n, m)];
-        const int re = m == 0 ? 0 : 2 * m - 1, im = 2 * m;  // offsets of pk_re / pk_im inside the degree block
-        Am[re * d + i] = a.real();
-        Bm[re * d + i] = b.real();
-        Al[re * d + i] = (m ? 2. : 1.) * a.real();
-        Bl[re * d + i] = (m ? 2. : 1.) * b.real();
-        if (m) {
-          Am[im * d + i] = -a.imag();
-          Bm[im * d + i] = -b.imag();
-          Al[im * d + i] = -2. * a.imag();
-          Bl[im * d + i] = -2. * b.imag();
-        }
+  for (int i = 0; i < d; ++i) {
+    const double p[3] = {unit(), unit(), unit()};
+    const double qp[3] = {c * p[0] - s * p[2], p[1], s * p[0] + c * p[2]};  // R_y(-theta) p
+    regular_d(p[0], p[1], p[2], n, Ra);
+    regular_d(qp[0], qp[1], qp[2], n, Rb);
+    for (int m = 0; m <= n; ++m) {
+      const std::complex<double> a = Ra[idx(n, m)], b = Rb[idx(n, m)];
+      const double f = (kind == 1 && m > 0) ? 2. : 1.;  // L2P rows: 2 Re, -2 Im for m > 0
+      const int re = m == 0 ? 0 : 2 * m - 1, im = 2 * m;
+      A[re * d + i] = f * a.real();
+      B[re * d + i] = f * b.real();
+      if (m) {
+        A[im * d + i] = -f * a.imag();
+        B[im * d + i] = -f * b.imag();
       }
     }
-    std::vector<double> Hn(d * d);
-    right_solve(Am, Bm, d, D + boff);  // D_n (row-major [out][in])
-    right_solve(Al, Bl, d, Hn.data());
-    for (int a = 0; a < d; ++a)
-      for (int c = 0; c < d; ++c) Ht[boff + a * d + c] = Hn[c * d + a];  // H_n^T (row-major [out][in])
-    boff += d * d;
+  }
+  X.resize(d * d);
+  right_solve(A, B, d, X.data());
+}
+}  // namespace
+
+int m2l_rotation_size(int P) {
+  int s = 2 * P;
+  for (int n = 0; n <= P; ++n) s += 2 * ((n + 1) * (n + 1) + n * n);
+  for (int m = 0; m <= P; ++m) s += (P + 1 - m) * (P + 1 - m);
+  return s;
+}
+
+void m2l_rotation_operator(int P, int dx, int dy, int dz, double* out) {
+  const double r = std::sqrt((double)dx * dx + (double)dy * dy + (double)dz * dz);
+  const double theta = std::acos(std::max(-1., std::min(1., dz / r))), phi = std::atan2((double)dy, (double)dx);
+  int o = 0;
+  for (int m = 1; m <= P; ++m) {
+    out[o++] = std::cos(m * phi);
+    out[o++] = std::sin(m * phi);
+  }
+  // split a packed degree block into its cos-type (Re of m = 0..n) and sin-type (Im of m = 1..n) blocks
+  auto split = [&](int n, const std::vector<double>& X, bool transpose) {
+    const int d = 2 * n + 1;
+    auto at = [&](int a, int b) { return transpose ? X[b * d + a] : X[a * d + b]; };
+    for (int a = 0; a <= n; ++a)
+      for (int b = 0; b <= n; ++b) out[o++] = at(a == 0 ? 0 : 2 * a - 1, b == 0 ? 0 : 2 * b - 1);
+    for (int a = 1; a <= n; ++a)
+      for (int b = 1; b <= n; ++b) out[o++] = at(2 * a, 2 * b);
+  };
+  std::vector<double> Y;
+  for (int n = 0; n <= P; ++n) {
+    y_rotation(n, theta, 0, Y);
+    split(n, Y, false);
   }
   // axial operator T(r z): per order m, block [j - m][n - m] (j, n = m..P), shared by Re and Im parts
   const std::vector<qc> I = irregular(0, 0, (q)r, 2 * P);
-  int toff = 0;
-  for (int m = 0; m <= P; ++m) {
-    const int w = P + 1 - m;
+  for (int m = 0; m <= P; ++m)
     for (int j = m; j <= P; ++j)
       for (int n = m; n <= P; ++n) {
         const q a = Anm(n, m) * Anm(j, m) / (((n & 1) ? (q)-1 : (q)1) * Anm(j + n, 0));
-        const qc v = a * (ipow(-2 * m) * I[idx(j + n, 0)]);
-        Tax[toff + (j - m) * w + (n - m)] = (double)v.re;
+        out[o++] = (double)(a * (ipow(-2 * m) * I[idx(j + n, 0)])).re;
       }
-    toff += w * w;
+  for (int n = 0; n <= P; ++n) {
+    y_rotation(n, theta, 1, Y);
+    split(n, Y, true);
   }
 }
 

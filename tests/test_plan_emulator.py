@@ -103,19 +103,35 @@ def test_m2l_operator_reflection_symmetry(emu):
 
 
 def _rotation_blocks(emu, P, o):
-    """(D, T, H^T) as dense (P+1)^2 matrices [out, in] from the compact factors of M2L offset o."""
+    """(D, T, H^T) as dense (P+1)^2 matrices [out, in] from the compact factors of M2L offset o:
+    D = Y_D Z, H^T = Z^T Y_H^T with Z the per-order z-rotation, Y the cos/sin-type blocks of each degree."""
     C = (P + 1) ** 2
     f = np.empty(emu.emu_rotation_size(P))
     emu.emu_operator(P, 3, o[0], o[1], o[2], _p(f, _dp))
-    D, T, Ht = np.zeros((C, C)), np.zeros((C, C)), np.zeros((C, C))
-    k = 0
-    for n in range(P + 1):
-        d = 2 * n + 1
-        D[n * n:n * n + d, n * n:n * n + d] = f[k:k + d * d].reshape(d, d)
-        k += d * d
 
     def col(n, m, part):  # packed index of (n, m): real part, or imaginary part for m > 0
         return n * n if m == 0 else n * n + 2 * m - 1 + part
+    k = 0
+    Z = np.eye(C)
+    for m in range(1, P + 1):
+        c, s = f[k], f[k + 1]
+        k += 2
+        for n in range(m, P + 1):
+            re, im = col(n, m, 0), col(n, m, 1)
+            Z[re, re], Z[re, im], Z[im, re], Z[im, im] = c, -s, s, c
+
+    def degree_blocks(k):
+        Y = np.zeros((C, C))
+        for n in range(P + 1):
+            cos_idx = [col(n, m, 0) for m in range(n + 1)]
+            sin_idx = [col(n, m, 1) for m in range(1, n + 1)]
+            for idx in (cos_idx, sin_idx):
+                w = len(idx)
+                Y[np.ix_(idx, idx)] = f[k:k + w * w].reshape(w, w)
+                k += w * w
+        return Y, k
+    YD, k = degree_blocks(k)
+    T = np.zeros((C, C))
     for m in range(P + 1):
         w = P + 1 - m
         blk = f[k:k + w * w].reshape(w, w)
@@ -124,12 +140,9 @@ def _rotation_blocks(emu, P, o):
             for j in range(m, P + 1):
                 for n in range(m, P + 1):
                     T[col(j, m, part), col(n, m, part)] = blk[j - m, n - m]
-    for n in range(P + 1):
-        d = 2 * n + 1
-        Ht[n * n:n * n + d, n * n:n * n + d] = f[k:k + d * d].reshape(d, d)
-        k += d * d
+    YHt, k = degree_blocks(k)
     assert k == f.size
-    return D, T, Ht
+    return YD @ Z, T, Z.T @ YHt
 
 
 @pytest.mark.parametrize("P", [8, 10])
