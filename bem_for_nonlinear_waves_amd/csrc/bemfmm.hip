@@ -2311,12 +2311,21 @@ int do_solve(bemfmm_ctx* c, const double* x0, double* xout, double tol, int max_
   c->refine_history.clear();
   int iters = 0, steps = 0;
   double cur_err = 0.;
+  // a zero initial guess has r0 = M^-1 b: no matvec for the first residual
+  const bool x0_zero = std::all_of(x0i.begin(), x0i.end(), [](double v) { return v == 0.; });
   if (c->opts.precision == BEMFMM_PRECISION_FP32_REFINE) {
     const double eta = (c->opts.refine_eta > 0. && c->opts.refine_eta < 1.) ? c->opts.refine_eta : 1e-5;
     double prev = 1e300;
     bool fallback = false;
+    bool x_zero = x0_zero;
     while (true) {
-      cur_err = fp64_residual(c, c->xsol.p, c->rvec.p);
+      if (x_zero) {
+        HIPCHECK(hipMemcpyAsync(c->rvec.p, c->bvec.p, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+        cur_err = norm2(c, c->rvec.p);
+        x_zero = false;
+      } else {
+        cur_err = fp64_residual(c, c->xsol.p, c->rvec.p);
+      }
       c->refine_history.push_back(cur_err);
       if (!std::isfinite(cur_err)) throw HipError("non-finite residual in refinement", BEMFMM_E_NONFINITE);
       if (cur_err <= tol || iters >= max_iter) break;
@@ -2336,7 +2345,7 @@ int do_solve(bemfmm_ctx* c, const double* x0, double* xout, double tol, int max_
       cur_err = gmres_run(c, r64, iters);
     }
   } else {
-    GmresRun r64{c->bvec.p, c->xsol.p, tol, max_iter, restart, false, true, false};
+    GmresRun r64{c->bvec.p, c->xsol.p, tol, max_iter, restart, false, true, x0_zero};
     cur_err = gmres_run(c, r64, iters);
   }
   std::vector<double> x(n);

@@ -106,7 +106,9 @@ def setup_bvp(m, d, phi, phin, args, precision):
     prec = _lib.PRECISION_FP32_REFINE if precision == "fp32_refine" else _lib.PRECISION_FP64
     bvp = BEM_BVP(m, d, p=args.p, leaf_max=args.leaf, max_level=args.max_level, preconditioner=pc, precision=prec,
                   refine_eta=args.refine_eta, m2l_tile_pairs=args.m2l_tile_pairs,
-                  m2l_kernel={"auto": 0, "dense": 1, "rotation": 2}[args.m2l_kernel])
+                  m2l_kernel={"auto": 0, "dense": 1, "rotation": 2}[args.m2l_kernel],
+                  schwarz_max_core=args.schwarz_core, schwarz_max_block=args.schwarz_block,
+                  schwarz_overlap=args.schwarz_overlap)
     bvp.set_bc(phi, phin)
     t = time.perf_counter()
     bvp.prepare()
@@ -359,6 +361,9 @@ def main():
     ap.add_argument("--precision", choices=["fp32_refine", "fp64"], default="fp32_refine")
     ap.add_argument("--refine-eta", type=float, default=1e-5)
     ap.add_argument("--m2l-tile-pairs", type=int, default=0, choices=[0, 16, 32, 64])
+    ap.add_argument("--schwarz-core", type=int, default=64, help="Schwarz core size (reference default 64)")
+    ap.add_argument("--schwarz-block", type=int, default=128, help="Schwarz block size (reference default 128)")
+    ap.add_argument("--schwarz-overlap", type=int, default=1, help="Schwarz overlap rings (reference default 1)")
     ap.add_argument("--m2l-kernel", default="auto", choices=["auto", "dense", "rotation"],
                     help="M2L of the FP32 operator: rotation where compiled (p = 8, 10), or dense matrix-core tiles")
     ap.add_argument("--no-fp64-compare", action="store_true",
@@ -432,9 +437,11 @@ def main():
     mfma_peak = 157.3 if fp32 else FP64_PEAK_TFLOPS
     rot = bool(fp32 and st.get("m2l_rotation", 0))
     # algorithmic multiply-adds per pair and layer: dense C x C operator, or the rotation factors
-    # (D and H^T: (2n+1)^2 per degree; axial: (p+1)^2 for m = 0 and 2 (p+1-m)^2 for m > 0)
+    # (D = Y Z and H^T = Z^T Y^T: 4 per order m >= 1 of each degree for Z, (n+1)^2 + n^2 per degree for Y;
+    # axial: (p+1)^2 for m = 0 and 2 (p+1-m)^2 for m > 0)
     pp = args.p
-    rot_macs = 2 * sum((2 * k + 1) ** 2 for k in range(pp + 1)) + (pp + 1) ** 2 + 2 * sum((pp + 1 - k) ** 2 for k in range(1, pp + 1))
+    rot_macs = (2 * sum(4 * k + (k + 1) ** 2 + k ** 2 for k in range(pp + 1)) + (pp + 1) ** 2
+                + 2 * sum((pp + 1 - k) ** 2 for k in range(1, pp + 1)))
     m2l_flops = st["n_m2l_pairs"] * 2 * (rot_macs if rot else C * C) * 2.0
     near_bytes = (st["near_nnz"] + m.n_nodes * C) * 2 * esz  # dense G/Gn weights incl. L2P columns
     dom = int(np.argmax(stage))
