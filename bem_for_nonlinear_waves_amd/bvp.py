@@ -201,6 +201,16 @@ class BEM_BVP:
         self._check(rc)
         return x, info
 
+    def solveSystemGMRES_phi_t(self, phit, phint, x0=None, tol: float = 1e-10, max_iter: int = 1000,
+                               restart: int = 1000, raise_on_nonconvergence: bool = False):
+        """Second BVP of an RK stage, solveSystemGMRES(is_time_derivative=true, x0)
+        (BEM_solveBVP.hpp:922-1010, BEM_solveBVP_GMRES_FMM.hpp:2007): the same operator with the time
+        derivatives as knowns (phi_t on Dirichlet DOFs, phin_t on Neumann DOFs). Tree, operators, near
+        field, alpha and preconditioner stay resident: only the right-hand side is rebuilt. The context
+        keeps these knowns until the next set_bc."""
+        self.set_bc(phit, phint)
+        return self.solveSystemGMRES(x0, tol, max_iter, restart, raise_on_nonconvergence)
+
     def solveGMRES(self, phi, phin, x0=None, tol: float = 1e-10, max_iter: int = 1000, restart: int = 1000):
         """Setup (scaling, tree, operators, near field, alpha) + GMRES, as BEM_BVP::solveGMRES."""
         self.set_bc(phi, phin)

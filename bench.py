@@ -47,6 +47,20 @@ MEASURED_TRAFFIC = {("m2l", True): 5.943e9, ("m2l", False): 8.618e9, ("m2l_rot",
                     ("near_l2p", True): 5.177e9, ("near_l2p", False): 9.896e9}
 
 
+def case_workload(path: str, seed: int = 3):
+    """A case in the reference's input format (settings.json + body JSON + OBJ): the fluid surface with
+    boundary types by contact; synthetic knowns: phi from seeded free-surface modes over the bounding
+    box, phi_n = n.U on the faces of the second and later rigid bodies (wave makers, floating bodies)."""
+    from bem_for_nonlinear_waves_amd import inputs
+    from bem_for_nonlinear_waves_amd import mesh as M
+    m, info = inputs.load_case(path)
+    d = M.build_dofs(m)
+    ext = m.X.max(axis=0) - m.X.min(axis=0)
+    groups = tuple(range(2, 2 + max(0, len(info["bodies"]) - 2)))
+    phi, phin = M.tank_bc(m, d, seed, float(ext[0]), float(ext[1]), body_velocity_sigma=0.1, body_group=groups)
+    return m, d, phi, phin
+
+
 def make_workload(h: float, seed: int = 3):
     from bem_for_nonlinear_waves_amd import mesh as M
     m = M.cylinder_tank(Lx=20.0, Ly=4.0, Lz=2.0, R=0.5, draft=1.0, h=h)
@@ -361,9 +375,13 @@ def main():
     ap.add_argument("--precision", choices=["fp32_refine", "fp64"], default="fp32_refine")
     ap.add_argument("--refine-eta", type=float, default=1e-5)
     ap.add_argument("--m2l-tile-pairs", type=int, default=0, choices=[0, 16, 32, 64])
-    ap.add_argument("--schwarz-core", type=int, default=64, help="Schwarz core size (reference default 64)")
+    ap.add_argument("--case", default=None,
+                    help="solve a case in the reference's input format (path to its settings.json) instead of the cylinder tank")
+    # Schwarz blocks: small cores with 3 overlap rings (blocks still <= 128 DOFs) measured best on the
+    # 209k tank (45 iterations, 0.171 s per solve, vs 50 and 0.187 s at the reference default 64/128/1)
+    ap.add_argument("--schwarz-core", type=int, default=32, help="Schwarz core size (reference default 64)")
     ap.add_argument("--schwarz-block", type=int, default=128, help="Schwarz block size (reference default 128)")
-    ap.add_argument("--schwarz-overlap", type=int, default=1, help="Schwarz overlap rings (reference default 1)")
+    ap.add_argument("--schwarz-overlap", type=int, default=3, help="Schwarz overlap rings (reference default 1)")
     ap.add_argument("--m2l-kernel", default="auto", choices=["auto", "dense", "rotation"],
                     help="M2L of the FP32 operator: rotation where compiled (p = 8, 10), or dense matrix-core tiles")
     ap.add_argument("--no-fp64-compare", action="store_true",
@@ -397,7 +415,8 @@ def main():
     if args.mode == "serve":
         return run_serve(args)
     t = time.perf_counter()
-    m, d, phi, phin = make_workload(args.h)
+    _log(f"building the workload ({'case ' + args.case if args.case else f'cylinder tank, h={args.h}'})")
+    m, d, phi, phin = case_workload(args.case) if args.case else make_workload(args.h)
     n = d.n_dof
     _log(f"mesh: {m.n_nodes} nodes, {m.n_faces} faces, {n} DOFs ({time.perf_counter() - t:.1f}s)")
     fp32 = args.precision == "fp32_refine"
@@ -458,7 +477,7 @@ def main():
         ach = near_bytes / (stage[dom] * 1e-3) / 1e9
         roof = {"kernel": f"k_block_gemv ({STAGES[dom]})", "bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "traffic": None}
-    default_workload = (args.h, args.p, args.leaf, args.max_level) == (0.036, 10, 32, 7)
+    default_workload = not args.case and (args.h, args.p, args.leaf, args.max_level) == (0.036, 10, 32, 7)
     if default_workload:
         roof["traffic"] = MEASURED_TRAFFIC.get(("m2l_rot" if rot and STAGES[dom] == "m2l" else STAGES[dom], fp32))
     roof["stage_ms"] = dict(zip(STAGES, [round(float(v), 4) for v in stage]))
@@ -490,11 +509,12 @@ def main():
         "vs_baseline": None,
         "dtype": "f32+f64" if fp32 else "f64",
         "data": "synthetic",
-        "config": {"workload": f"cylinder tank 20x4x2 m, R=0.5 draft=1, h={args.h}, "
+        "config": {"workload": (f"case {args.case}, " if args.case else f"cylinder tank 20x4x2 m, R=0.5 draft=1, h={args.h}, ")
                                + ("FP32 FMM-GMRES with FP64 iterative refinement" if fp32 else "FP64 FMM-GMRES"),
                    "n_nodes": m.n_nodes, "n_faces": m.n_faces, "n_dof": n,
                    "p": args.p, "theta": 0.25, "leaf_max": args.leaf, "max_level": args.max_level, "tol": args.tol,
-                   "precision": args.precision, "preconditioner": args.precond},
+                   "precision": args.precision, "preconditioner": args.precond,
+                   "schwarz": [args.schwarz_core, args.schwarz_block, args.schwarz_overlap]},
         "matvec_mnode_per_s": m.n_nodes / t_mv / 1e6,
         "matvec_ms": t_mv * 1e3,
         "matvec_fp64_ms": t_mv64 * 1e3,

@@ -182,3 +182,31 @@ def test_reprepare_reuses_operator_table(small_tank, oracle_mod):
     assert np.linalg.norm(y1 - o.matvec(x)) <= 1e-10 * np.linalg.norm(y2)
     g.close()
     g2.close()
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_phi_t_second_bvp_on_resident_plan(small_tank, precision):
+    """The phi_t BVP of an RK stage (solveSystemGMRES(is_time_derivative=true, x0)) on the prepared
+    context: no re-prepare (no operator upload, no setup time), and the same solution as a context
+    prepared from scratch with those knowns; a warm start from a nearby solution converges too."""
+    import time
+    from bem_for_nonlinear_waves_amd import mesh as M
+    m, d, phi, phin = small_tank
+    kw = dict(p=10, leaf_max=32, precision=precision, preconditioner=1)
+    g = _gpu_bvp(m, d, phi, phin, **kw)
+    x_phi, info = g.solveSystemGMRES(tol=1e-11, max_iter=600)
+    assert info["converged"] == 1
+    uploaded = g.stats()["n_ops_uploaded"]
+    phit, phint = M.tank_bc(m, d, 17, 4.0, 1.0)
+    t = time.perf_counter()
+    xt, it = g.solveSystemGMRES_phi_t(phit, phint, tol=1e-11, max_iter=600)
+    t_phit = time.perf_counter() - t
+    assert it["converged"] == 1 and g.stats()["n_ops_uploaded"] == uploaded
+    fresh = _gpu_bvp(m, d, phit, phint, **kw)
+    xf, _ = fresh.solveSystemGMRES(tol=1e-11, max_iter=600)
+    print(f"[gpu] phi_t solve on the resident plan {t_phit*1e3:.1f} ms (setup of a fresh context {fresh.time_setup*1e3:.1f} ms)")
+    assert np.linalg.norm(xt - xf) <= 1e-8 * np.linalg.norm(xf)
+    xw, iw = g.solveSystemGMRES_phi_t(phit, phint, x0=xf + 1e-6 * np.abs(xf).max(), tol=1e-11, max_iter=600)
+    assert iw["converged"] == 1 and np.linalg.norm(xw - xf) <= 1e-8 * np.linalg.norm(xf)
+    g.close()
+    fresh.close()
