@@ -103,6 +103,9 @@ __global__ void k_scatter(int mode, int n, const int32_t* __restrict__ kind, con
 // 16-byte aligned) and issues all 2*kRPW*2 loads of two column steps before using them (rows
 // past the end of the block are clamped to its last row and their sums discarded), so that
 // enough HBM requests are in flight per lane. Sums are accumulated in FP64.
+// part 0: all columns, y = A x. The columns of a block are [DOF columns (ncd) | L2P columns | padding];
+// part 1: the DOF columns [0, ncd rounded down to 4), y = A x (needs no FMM result: it runs beside the
+// FMM passes); part 2: the rest (0-3 DOF columns and the L2P columns), y += A x (after L2L).
 constexpr int kGemvChunk = 2048;
 constexpr int kGemvMaxRows = 512;
 constexpr int kRPW = 4;
@@ -127,7 +130,8 @@ __global__ __launch_bounds__(256) void k_block_gemv(const int64_t* __restrict__ 
                                                     const int64_t* __restrict__ mat_ptr, const int32_t* __restrict__ row_out,
                                                     const int32_t* __restrict__ col_idx, const T* __restrict__ mat,
                                                     const double* __restrict__ x0, const double* __restrict__ x1,
-                                                    double* __restrict__ y0, double* __restrict__ y1) {
+                                                    double* __restrict__ y0, double* __restrict__ y1, int part,
+                                                    const int32_t* __restrict__ ncol_dof) {
   typedef typename GemvVec<T>::type V;
   constexpr int W = GemvVec<T>::W;
   constexpr int STEP = 64 * W;  // columns covered by one wave-wide vector load
@@ -141,8 +145,10 @@ __global__ __launch_bounds__(256) void k_block_gemv(const int64_t* __restrict__ 
   const T* M = mat + mat_ptr[b];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   for (int i = threadIdx.x; i < 2 * nrow; i += blockDim.x) acc[i] = 0.;
-  for (int cs = 0; cs < ncol; cs += kGemvChunk) {
-    const int cn = min(kGemvChunk, ncol - cs);  // multiple of 4
+  const int split = part == 0 ? 0 : (ncol_dof[b] & ~3);
+  const int cb = part == 2 ? split : 0, ce = part == 1 ? split : ncol;
+  for (int cs = cb; cs < ce; cs += kGemvChunk) {
+    const int cn = min(kGemvChunk, ce - cs);  // multiple of 4
     __syncthreads();
     const int32_t* ci = col_idx + c0 + cs;
     int i = threadIdx.x;
@@ -230,8 +236,13 @@ __global__ __launch_bounds__(256) void k_block_gemv(const int64_t* __restrict__ 
   __syncthreads();
   for (int r = threadIdx.x; r < nrow; r += blockDim.x) {
     const int o = row_out[r0 + r];
-    y0[o] = acc[2 * r];
-    y1[o] = acc[2 * r + 1];
+    if (part == 2) {  // each row belongs to one block: a plain read-modify-write
+      y0[o] += acc[2 * r];
+      y1[o] += acc[2 * r + 1];
+    } else {
+      y0[o] = acc[2 * r];
+      y1[o] = acc[2 * r + 1];
+    }
   }
 }
 
@@ -1317,6 +1328,10 @@ struct bemfmm_ctx {
   int rot_RS = 0;
   DTiles m2l_rt;  // M2L tiles of kRotTP pairs (one offset each)
   DBuf<float> m32;  // FP32 multipoles [box][layer][C] read by the rotation M2L
+  // near field beside the FMM passes: the near-only GEMV runs on stream2 between a fork and a join event
+  hipStream_t stream2 = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  DBuf<int32_t> near_ncd;  // DOF columns per near block
   double* op_stage = nullptr;  // pinned host staging for operator uploads
   size_t op_stage_n = 0;
   DBuf<int32_t> kind, point;
@@ -1369,10 +1384,11 @@ int guarded(bemfmm_ctx* c, F&& f) {
 }
 
 template <typename T>
-void launch_gemv(bemfmm_ctx* c, DGemv& g, const T* mat, const double* x0, const double* x1, double* y0, double* y1) {
+void launch_gemv(bemfmm_ctx* c, DGemv& g, const T* mat, const double* x0, const double* x1, double* y0, double* y1,
+                 int part = 0, hipStream_t s = nullptr) {
   if (g.nblk == 0) return;
-  hipLaunchKernelGGL(k_block_gemv<T>, dim3(g.nblk), dim3(256), 0, c->stream, g.row_ptr.p, g.col_ptr.p, g.mat_ptr.p,
-                     g.row_out.p, g.col_idx.p, mat, x0, x1, y0, y1);
+  hipLaunchKernelGGL(k_block_gemv<T>, dim3(g.nblk), dim3(256), 0, s ? s : c->stream, g.row_ptr.p, g.col_ptr.p,
+                     g.mat_ptr.p, g.row_out.p, g.col_idx.p, mat, x0, x1, y0, y1, part, c->near_ncd.p);
 }
 
 template <typename T>
@@ -1449,6 +1465,16 @@ void evaluate_launch(bemfmm_ctx* c, int mode, const double* in, double* out, boo
   rec(0);
   hipLaunchKernelGGL(k_scatter, dim3(grid1(n)), dim3(256), 0, c->stream, mode == 0 ? 0 : (mode == 1 ? 1 : 2), n,
                      c->kind.p, in, c->phi_k.p, c->phin_k.p, phin, phi);
+  // the near-field columns need only the scattered DOF values: they run on stream2 beside the FMM passes
+  // (not while profiling, which times the stages one after the other)
+  const bool split = c->opts.overlap_near && !c->prof && c->near.nblk > 0;
+  if (split) {
+    HIPCHECK(hipEventRecord(c->ev_fork, c->stream));
+    HIPCHECK(hipStreamWaitEvent(c->stream2, c->ev_fork, 0));
+    if (lo) launch_gemv(c, c->near, c->near_mat32.p, c->buf0.p, c->buf1.p, c->y0.p, c->y1.p, 1, c->stream2);
+    else launch_gemv(c, c->near, c->near.mat.p, c->buf0.p, c->buf1.p, c->y0.p, c->y1.p, 1, c->stream2);
+    HIPCHECK(hipEventRecord(c->ev_join, c->stream2));
+  }
   fill(c, c->M0.p, nbC, 0.);
   fill(c, c->M1.p, nbC, 0.);
   rec(1);
@@ -1487,8 +1513,10 @@ void evaluate_launch(bemfmm_ctx* c, int mode, const double* in, double* out, boo
     else launch_translate(c, c->l2l[l], c->ops.p, L0, L1, L0, L1);
   }
   rec(6);
-  if (lo) launch_gemv(c, c->near, c->near_mat32.p, c->buf0.p, c->buf1.p, c->y0.p, c->y1.p);
-  else launch_gemv(c, c->near, c->near.mat.p, c->buf0.p, c->buf1.p, c->y0.p, c->y1.p);
+  if (split) HIPCHECK(hipStreamWaitEvent(c->stream, c->ev_join, 0));
+  const int part = split ? 2 : 0;
+  if (lo) launch_gemv(c, c->near, c->near_mat32.p, c->buf0.p, c->buf1.p, c->y0.p, c->y1.p, part);
+  else launch_gemv(c, c->near, c->near.mat.p, c->buf0.p, c->buf1.p, c->y0.p, c->y1.p, part);
   rec(7);
   hipLaunchKernelGGL(k_assemble, dim3(grid1(n)), dim3(256), 0, c->stream, mode, n, c->kind.p, c->point.p, c->alpha.p, phi,
                      c->y0.p, c->y1.p, c->phi_k.p, out);
@@ -1896,6 +1924,7 @@ void do_prepare(bemfmm_ctx* c) {
     row_block.upload(rb, s);
   }
   ncol_dof.upload(pl.near_ncol_dof, s);
+  c->near_ncd.upload(pl.near_ncol_dof, s);
   face_perm.upload(pl.face_perm, s);
   fv.upload(pl.fv, s);
   fdof.upload(pl.fdof, s);
@@ -2434,6 +2463,7 @@ void bemfmm_default_opts(bemfmm_opts* o) {
   o->dof_reorder = 1;
   o->use_graphs = 1;
   o->m2l_kernel = 0;
+  o->overlap_near = 1;
 }
 
 bemfmm_ctx* bemfmm_ctx_create(int device, const bemfmm_opts* opts) {
@@ -2447,6 +2477,9 @@ bemfmm_ctx* bemfmm_ctx_create(int device, const bemfmm_opts* opts) {
     if (device < 0 || device >= n) throw HipError("invalid device index", BEMFMM_E_DEVICE);
     HIPCHECK(hipSetDevice(device));
     HIPCHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    HIPCHECK(hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking));
+    HIPCHECK(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
+    HIPCHECK(hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming));
     return BEMFMM_OK;
   });
   if (rc != BEMFMM_OK) {
@@ -2461,11 +2494,15 @@ void bemfmm_ctx_destroy(bemfmm_ctx* c) {
   if (!c) return;
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
+  if (c->stream2) (void)hipStreamSynchronize(c->stream2);
   for (auto& e : c->ev)
+    if (e) (void)hipEventDestroy(e);
+  for (auto e : {c->ev_fork, c->ev_join})
     if (e) (void)hipEventDestroy(e);
   drop_graphs(c);
   if (c->op_stage) (void)hipHostFree(c->op_stage);
   if (c->stream) (void)hipStreamDestroy(c->stream);
+  if (c->stream2) (void)hipStreamDestroy(c->stream2);
   delete c;
 }
 

@@ -73,6 +73,8 @@ typedef struct bemfmm_opts {
   int m2l_kernel;        /* M2L of the FP32 operator (FP32_REFINE): 0 (default) = rotation where compiled for p
                             (8, 10), else matrix cores; 1 = dense matrix-core tiles; 2 = rotation (error if
                             not compiled for p). The FP64 operator always uses the dense matrix-core tiles. */
+  int overlap_near;      /* 1 (default): the near-field columns of each matvec run on a second stream beside the
+                            FMM passes (they need only the input vector); 0: after L2L, one GEMV */
 } bemfmm_opts;
 
 typedef struct bemfmm_result {
