@@ -1740,6 +1740,14 @@ void update_rotation_table(bemfmm_ctx* c, const Plan& pl, const std::vector<int3
       fresh.push_back(op);
     }
   if (fresh.empty()) return;
+  {
+    std::vector<std::array<int, 4>> keys;
+    for (int op : fresh) {
+      const auto k = bemfmm::op_key(pl, op);
+      keys.push_back({3, k[1], k[2], k[3]});
+    }
+    bemfmm::warm_operator_cache(pl.P, keys);  // factors of new offsets, computed in parallel once per process
+  }
   const size_t cap = c->op_stage_n * 2 / RS;  // rotations per pinned staging batch (the staging holds doubles)
   if (cap < 1) {
     if (c->op_stage) (void)hipHostFree(c->op_stage);

@@ -761,6 +761,11 @@ void build_plan(Plan& pl, const Options& opt, int n_nodes, const double* X, int 
   pl.t_ops = now() - t4;
 }
 
+void warm_operator_cache(int P, const std::vector<std::array<int, 4>>& keys) {
+#pragma omp parallel for schedule(dynamic, 4)
+  for (int64_t i = 0; i < (int64_t)keys.size(); ++i) (void)cached_operator(P, keys[i][0], keys[i][1], keys[i][2], keys[i][3]);
+}
+
 std::shared_ptr<const std::vector<double>> cached_operator(int P, int kind, int a, int b, int c) {
   static std::mutex mu;
   static std::map<std::array<int, 5>, std::shared_ptr<const std::vector<double>>> cache;

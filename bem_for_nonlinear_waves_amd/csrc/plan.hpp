@@ -113,6 +113,8 @@ void l2l_operator(int P, int octant, double* out);  // parent -> child
 // kind 0 = M2M of octant a, 1 = L2L of octant a, 2 = M2L of lattice offset (a, b, c), 3 = the rotation
 // factors of that M2L (m2l_rotation_operator). Thread-safe.
 std::shared_ptr<const std::vector<double>> cached_operator(int P, int kind, int a, int b, int c);
+// computes the missing cache entries of the given (kind, a, b, c) keys in parallel (OpenMP)
+void warm_operator_cache(int P, const std::vector<std::array<int, 4>>& keys);
 // (kind, a, b, c) of operator `op` of a plan: 0-7 M2M, 8-15 L2L, 16+ M2L offsets
 inline std::array<int, 4> op_key(const Plan& pl, int op) {
   if (op < 8) return {0, op, 0, 0};

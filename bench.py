@@ -43,7 +43,7 @@ FP64_PEAK_TFLOPS = 78.6
 # depth 7), from rocprofv3 FETCH_SIZE (scaled x2) and WRITE_SIZE (x1) passes over
 # `bench.py --profile-only` (summarised in profiles/fmm_matvec_cylinder209k_p10.md); reported only
 # for that workload. Keys: (stage, FP32 operator); "m2l_rot" = the rotation M2L of the FP32 operator.
-MEASURED_TRAFFIC = {("m2l", True): 5.943e9, ("m2l", False): 8.618e9, ("m2l_rot", True): 4.249e9,
+MEASURED_TRAFFIC = {("m2l", True): 5.943e9, ("m2l", False): 8.618e9, ("m2l_rot", True): 4.256e9,
                     ("near_l2p", True): 5.177e9, ("near_l2p", False): 9.896e9}
 
 
@@ -295,6 +295,12 @@ def run_serve(args):
 
     from bem_for_nonlinear_waves_amd import mesh as M
     from bem_for_nonlinear_waves_amd.serve import BVPServer
+    if not args.serve_anyway:
+        # this workload (8 concurrent FP32+refinement sessions that re-prepare their contexts between stages)
+        # ended with a segmentation fault of the process in the current build; the cause has not been found,
+        # so the mode does not run until it is
+        sys.exit("bench.py --mode serve is disabled in this build: its last run ended with a segmentation fault "
+                 "whose cause is not yet found (see the serving section of the design notes)")
     rng = np.random.default_rng(5)
     t = time.perf_counter()
     sizes = rng.uniform(args.serve_min_n, args.serve_max_n, args.sessions).astype(int)
@@ -375,6 +381,7 @@ def main():
     ap.add_argument("--precision", choices=["fp32_refine", "fp64"], default="fp32_refine")
     ap.add_argument("--refine-eta", type=float, default=1e-5)
     ap.add_argument("--m2l-tile-pairs", type=int, default=0, choices=[0, 16, 32, 64])
+    ap.add_argument("--serve-anyway", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--case", default=None,
                     help="solve a case in the reference's input format (path to its settings.json) instead of the cylinder tank")
     # Schwarz blocks: small cores with 3 overlap rings (blocks still <= 128 DOFs) measured best on the
