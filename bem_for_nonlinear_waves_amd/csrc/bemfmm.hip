@@ -1327,6 +1327,7 @@ struct bemfmm_ctx {
   std::vector<char> rot_have;
   int rot_RS = 0;
   DTiles m2l_rt;  // M2L tiles of kRotTP pairs (one offset each)
+  int m2l_min_level = 0;  // coarsest level with M2L pairs: M2M into and L2L out of coarser levels are skipped
   DBuf<float> m32;  // FP32 multipoles [box][layer][C] read by the rotation M2L
   // near field beside the FMM passes: the near-only GEMV runs on stream2 between a fork and a join event
   hipStream_t stream2 = nullptr;
@@ -1489,7 +1490,8 @@ void evaluate_launch(bemfmm_ctx* c, int mode, const double* in, double* out, boo
                          c->p2m.mat.p, phin, phi, c->M0.p, c->M1.p);
   }
   rec(2);
-  for (int l = pl.n_levels - 1; l >= 1; --l) {
+  // M2M from level l into l-1 only where level l-1 has M2L pairs (coarser multipoles are not used)
+  for (int l = pl.n_levels - 1; l >= 1 && l - 1 >= c->m2l_min_level; --l) {
     if (lo) launch_translate(c, c->m2m[l], c->ops32.p, c->M0.p, c->M1.p, c->M0.p, c->M1.p);
     else launch_translate(c, c->m2m[l], c->ops.p, c->M0.p, c->M1.p, c->M0.p, c->M1.p);
   }
@@ -1508,7 +1510,8 @@ void evaluate_launch(bemfmm_ctx* c, int mode, const double* in, double* out, boo
                        c->m2l_tpairs.p, (const double*)c->m2l_stage.p, L0, L1);
   }
   rec(5);
-  for (int l = 1; l < pl.n_levels; ++l) {
+  // L2L into level l only from parents at levels with M2L pairs (the locals of coarser levels are zero)
+  for (int l = std::max(1, c->m2l_min_level + 1); l < pl.n_levels; ++l) {
     if (lo) launch_translate(c, c->l2l[l], c->ops32.p, L0, L1, L0, L1);
     else launch_translate(c, c->l2l[l], c->ops.p, L0, L1, L0, L1);
   }
@@ -1896,6 +1899,8 @@ void do_prepare(bemfmm_ctx* c) {
   }
   c->m2l.upload(pl.m2l, s, slot, c->opts.m2l_tile_pairs > 0 ? c->opts.m2l_tile_pairs : kTPM2L);
   c->m2l_slot = slot;
+  c->m2l_min_level = pl.n_levels;  // no M2L pairs: no translations at all
+  for (int32_t d : pl.m2l.dst) c->m2l_min_level = std::min(c->m2l_min_level, (int)pl.box_level[d]);
   {
     std::vector<int64_t> tptr(pl.n_boxes + 1, 0);
     for (int32_t d : pl.m2l.dst) tptr[d + 1]++;
