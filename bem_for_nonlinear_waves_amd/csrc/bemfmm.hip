@@ -701,6 +701,131 @@ __device__ __forceinline__ void rot_apply(const float4* __restrict__ R4, float* 
   rot_run<P>(R4, v, std::make_integer_sequence<int, rot_rows<P>()>{});
 }
 
+// P2M matrices, built on the GPU (plan.cpp builds the same on the host for the CPU emulator):
+// A[layer][col][row] of leaf l = sum over the leaf's faces f and the q Dunavant points (b, w) of f of
+// W b_k (layer 0: conj R_n^m(x~), layer 1: conj grad R_n^m(x~) . n / s) for the local column of the face's
+// k-th DOF, x~ = (x - centre)/s, W = J w, packed by rows as the P2M kernel reads them (moments.hpp:308-368).
+// One workgroup per leaf; per chunk of kP2MPts points, one thread per point builds the unnormalised table
+// NR_n^m (n <= P, m >= 0) in LDS by the recurrences NR_m^m = (x+iy)/(2m) NR_{m-1}^{m-1},
+// NR_n^m = ((2n-1) z NR_{n-1}^m - r^2 NR_{n-2}^m)/((n+m)(n-m)); then thread (layer, row) adds the chunk's
+// contributions to its own row (no two threads write one entry). R_n^m = cnm[n,m] NR_n^m,
+// grad R_n^m . n = cnm (-a NR_{n-1}^{m+1} + b NR_{n-1}^{m-1} + n_z NR_{n-1}^m), a = (n_x - i n_y)/2,
+// b = (n_x + i n_y)/2, NR_n^{-m} = (-1)^m conj NR_n^m.
+constexpr int kP2MPts = 16;
+template <int P>
+__global__ __launch_bounds__(256) void k_p2m_build(int q, int CP, const int64_t* __restrict__ face_ptr,
+                                                   const int32_t* __restrict__ faces, const int32_t* __restrict__ face_lc,
+                                                   const double* __restrict__ leaf_cs, const int64_t* __restrict__ mat_ptr,
+                                                   const int64_t* __restrict__ col_ptr, const double* __restrict__ fX,
+                                                   const double* __restrict__ fcr, const double* __restrict__ fJ,
+                                                   const double* __restrict__ rule, const double* __restrict__ cnm,
+                                                   double* __restrict__ mat) {
+  constexpr int C = (P + 1) * (P + 1);
+  constexpr int NT = (P + 1) * (P + 2) / 2;
+  __shared__ double nre[kP2MPts][NT], nim[kP2MPts][NT];
+  __shared__ double pw[kP2MPts][3], pn[kP2MPts][3];
+  __shared__ int plc[kP2MPts][3];
+  const int l = blockIdx.x;
+  const int ncol = (int)(col_ptr[l + 1] - col_ptr[l]);
+  double* M = mat + mat_ptr[l];
+  const double cx = leaf_cs[4 * l], cy = leaf_cs[4 * l + 1], cz = leaf_cs[4 * l + 2], s = leaf_cs[4 * l + 3];
+  const int tid = threadIdx.x;
+  // thread (layer, row) owns rows `row` of its layer in every column: it zeroes them and adds to them
+  const int layer = tid >= C ? 1 : 0, row = tid - layer * C;
+  const bool owner = tid < 2 * C;
+  int rn = 0, rm = 0, rim = 0;  // row -> (n, m, imaginary part)
+  if (owner) {
+    while ((rn + 1) * (rn + 1) <= row) ++rn;
+    const int k = row - rn * rn;
+    rm = (k + 1) / 2;
+    rim = (k > 0 && (k & 1) == 0) ? 1 : 0;
+    for (int c = 0; c < ncol; ++c) M[(int64_t)(layer * ncol + c) * CP + row] = 0.;
+  }
+  auto tn = [](int n, int m) { return n * (n + 1) / 2 + m; };
+  const int64_t f0 = face_ptr[l];
+  const int npts = (int)(face_ptr[l + 1] - f0) * q;
+  for (int p0 = 0; p0 < npts; p0 += kP2MPts) {
+    const int np = min(kP2MPts, npts - p0);
+    if (tid < np) {
+      const int fi = (p0 + tid) / q, j = (p0 + tid) - fi * q;
+      const int f = faces[f0 + fi];
+      const double b0 = rule[4 * j], b1 = rule[4 * j + 1], b2 = rule[4 * j + 2], w = rule[4 * j + 3];
+      const double J = fJ[f];
+      const double x = (b0 * fX[9 * f] + b1 * fX[9 * f + 3] + b2 * fX[9 * f + 6] - cx) / s;
+      const double y = (b0 * fX[9 * f + 1] + b1 * fX[9 * f + 4] + b2 * fX[9 * f + 7] - cy) / s;
+      const double z = (b0 * fX[9 * f + 2] + b1 * fX[9 * f + 5] + b2 * fX[9 * f + 8] - cz) / s;
+      const double W = J * w;
+      pw[tid][0] = W * b0;
+      pw[tid][1] = W * b1;
+      pw[tid][2] = W * b2;
+      for (int a = 0; a < 3; ++a) {
+        plc[tid][a] = face_lc[3 * (f0 + fi) + a];
+        pn[tid][a] = fcr[3 * f + a] / J;
+      }
+      double* re = nre[tid];
+      double* im = nim[tid];
+      const double r2 = x * x + y * y + z * z;
+      re[0] = 1.;
+      im[0] = 0.;
+      for (int m = 1; m <= P; ++m) {
+        const double pr = re[tn(m - 1, m - 1)], pi = im[tn(m - 1, m - 1)], sc = 1. / (2. * m);
+        re[tn(m, m)] = sc * (x * pr - y * pi);
+        im[tn(m, m)] = sc * (x * pi + y * pr);
+      }
+      for (int m = 0; m <= P; ++m)
+        for (int n = m + 1; n <= P; ++n) {
+          double ar = (2 * n - 1) * z * re[tn(n - 1, m)], ai = (2 * n - 1) * z * im[tn(n - 1, m)];
+          if (n - 2 >= m) {
+            ar -= r2 * re[tn(n - 2, m)];
+            ai -= r2 * im[tn(n - 2, m)];
+          }
+          const double sc = 1. / ((double)(n + m) * (n - m));
+          re[tn(n, m)] = sc * ar;
+          im[tn(n, m)] = sc * ai;
+        }
+    }
+    __syncthreads();
+    if (owner) {
+      const double c = cnm[tn(rn, rm)];
+      for (int pt = 0; pt < np; ++pt) {
+        double v;
+        if (layer == 0) {
+          v = rim ? -c * nim[pt][tn(rn, rm)] : c * nre[pt][tn(rn, rm)];
+        } else if (rn == 0) {
+          v = 0.;
+        } else {
+          // NR_{n-1}^{q}: zero if |q| > n-1; negative q from the conjugate
+          auto nr = [&](int n, int qq, double& r, double& i) {
+            if (n < 0 || qq > n || -qq > n) {
+              r = i = 0.;
+              return;
+            }
+            if (qq >= 0) {
+              r = nre[pt][tn(n, qq)];
+              i = nim[pt][tn(n, qq)];
+            } else {
+              const double sg = ((-qq) & 1) ? -1. : 1.;
+              r = sg * nre[pt][tn(n, -qq)];
+              i = -sg * nim[pt][tn(n, -qq)];
+            }
+          };
+          double p1r, p1i, m1r, m1i, z0r, z0i;
+          nr(rn - 1, rm + 1, p1r, p1i);
+          nr(rn - 1, rm - 1, m1r, m1i);
+          nr(rn - 1, rm, z0r, z0i);
+          const double nx = pn[pt][0], ny = pn[pt][1], nz = pn[pt][2];
+          const double ar = 0.5 * nx, ai = -0.5 * ny, br = 0.5 * nx, bi = 0.5 * ny;
+          const double dr = -(ar * p1r - ai * p1i) + (br * m1r - bi * m1i) + nz * z0r;
+          const double di = -(ar * p1i + ai * p1r) + (br * m1i + bi * m1r) + nz * z0i;
+          v = rim ? -(c / s) * di : (c / s) * dr;
+        }
+        for (int k = 0; k < 3; ++k) M[(int64_t)(layer * ncol + plc[pt][k]) * CP + row] += pw[pt][k] * v;
+      }
+    }
+    __syncthreads();
+  }
+}
+
 // FP32 multipoles for the rotation M2L: M32[(box*2 + layer)*C + r] = M_layer[box*C + r]
 __global__ void k_pack_m32(int64_t n, int C, const double* __restrict__ M0, const double* __restrict__ M1,
                            float* __restrict__ M32) {
@@ -1861,6 +1986,7 @@ void do_prepare(bemfmm_ctx* c) {
   o.max_level = c->opts.max_level;
   o.q = c->opts.q;
   o.materialize_ops = false;  // operators go from the float128 cache straight into the device table
+  o.fill_p2m = false;         // the P2M matrices are built on the GPU (k_p2m_build)
   c->prepared = false;
   drop_graphs(c);  // their kernels point into the buffers reallocated below
   // the Krylov basis is sized for n_dof: drop it (re-grown by the next solve)
@@ -1885,7 +2011,8 @@ void do_prepare(bemfmm_ctx* c) {
   c->near.upload(pl.near, s);
   c->near.mat.alloc(pl.near.mat_size);
   c->p2m.upload(pl.p2m, s);
-  c->p2m.mat.upload(pl.p2m_mat, s);
+  c->p2m.mat.alloc(pl.p2m.mat_size);
+  fill(c, c->p2m.mat.p, pl.p2m.mat_size, 0.);  // rows past C stay zero
   c->RP = (pl.C + 63) / 64 * 64;
   c->KP = (pl.C + 4 * kKG - 1) / (4 * kKG) * (4 * kKG);
   const std::vector<int32_t> slot = update_operator_table(c, pl);
@@ -1971,6 +2098,47 @@ void do_prepare(bemfmm_ctx* c) {
     for (int g = 0; g < o.q; ++g)
       for (int a = 0; a < 4; ++a) dun[g][a] = src[g][a];
     d_dun.upload(std::vector<double>(&dun[0][0], &dun[0][0] + 28), s);
+  }
+  // ---- P2M matrices on the GPU ----
+  if (c->p2m.nblk > 0) {
+    if ((int)pl.p2m_face_ptr.size() != c->p2m.nblk + 1 || pl.p2m_face_lc.size() != 3 * pl.p2m_faces.size())
+      throw std::runtime_error("P2M source lists do not match the P2M blocks");
+    const int NT = (pl.P + 1) * (pl.P + 2) / 2;
+    std::vector<double> cnm(NT);
+    for (int n = 0; n <= pl.P; ++n)
+      for (int m = 0; m <= n; ++m) {
+        double f1 = 1, f2 = 1;
+        for (int k = 2; k <= n - m; ++k) f1 *= k;
+        for (int k = 2; k <= n + m; ++k) f2 *= k;
+        const double v = std::sqrt(f1 * f2);
+        cnm[n * (n + 1) / 2 + m] = (m & 1) ? -v : v;
+      }
+    DBuf<int64_t> fptr;
+    DBuf<int32_t> pfaces, plc;
+    DBuf<double> pcs, dcnm;
+    fptr.upload(pl.p2m_face_ptr, s);
+    pfaces.upload(pl.p2m_faces, s);
+    plc.upload(pl.p2m_face_lc, s);
+    pcs.upload(pl.p2m_leaf_cs, s);
+    dcnm.upload(cnm, s);
+#define BEMFMM_P2MB(PV)                                                                                              \
+  hipLaunchKernelGGL(k_p2m_build<PV>, dim3(c->p2m.nblk), dim3(256), 0, s, o.q, pl.p2m_rows_pad, fptr.p, pfaces.p, \
+                     plc.p, pcs.p, c->p2m.mat_ptr.p, c->p2m.col_ptr.p, fX.p, fcr.p, fJ.p, d_dun.p, dcnm.p, c->p2m.mat.p)
+    switch (pl.P) {
+    case 1: BEMFMM_P2MB(1); break;
+    case 2: BEMFMM_P2MB(2); break;
+    case 3: BEMFMM_P2MB(3); break;
+    case 4: BEMFMM_P2MB(4); break;
+    case 5: BEMFMM_P2MB(5); break;
+    case 6: BEMFMM_P2MB(6); break;
+    case 7: BEMFMM_P2MB(7); break;
+    case 8: BEMFMM_P2MB(8); break;
+    case 9: BEMFMM_P2MB(9); break;
+    case 10: BEMFMM_P2MB(10); break;
+    default: throw std::runtime_error("expansion order outside 1..10 for the P2M build");
+    }
+#undef BEMFMM_P2MB
+    HIPCHECK(hipGetLastError());
   }
   HIPCHECK(hipStreamSynchronize(s));
   const double t2 = now();

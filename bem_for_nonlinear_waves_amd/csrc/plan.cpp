@@ -624,9 +624,26 @@ void build_plan(Plan& pl, const Options& opt, int n_nodes, const double* X, int 
       G.col_idx.insert(G.col_idx.end(), cols[li].begin(), cols[li].end());
       G.col_ptr.push_back((int64_t)G.col_idx.size());
     }
-    pl.p2m_mat.assign(G.mat_size, 0.);
-#pragma omp parallel
-    {
+    pl.p2m_face_ptr.assign(1, 0);
+    pl.p2m_faces.clear();
+    pl.p2m_face_lc.clear();
+    pl.p2m_leaf_cs.clear();
+    for (int li = 0; li < nl; ++li) {
+      const int b = leaves[li];
+      for (int f : Bx[id2b[b]].data) {
+        pl.p2m_faces.push_back(f);
+        for (int c = 0; c < 3; ++c)
+          pl.p2m_face_lc.push_back(
+              (int32_t)(std::lower_bound(cols[li].begin(), cols[li].end(), face_dof[3 * f + c]) - cols[li].begin()));
+      }
+      pl.p2m_face_ptr.push_back((int64_t)pl.p2m_faces.size());
+      for (int a = 0; a < 3; ++a) pl.p2m_leaf_cs.push_back(pl.box_center[3 * b + a]);
+      pl.p2m_leaf_cs.push_back(pl.level_side[pl.box_level[b]]);
+    }
+    if (opt.fill_p2m) pl.p2m_mat.assign(G.mat_size, 0.);
+    else pl.p2m_mat.clear();
+#pragma omp parallel if (opt.fill_p2m)
+    if (opt.fill_p2m) {
       Harm H(P + 1);
 #pragma omp for schedule(dynamic, 4)
       for (int li = 0; li < nl; ++li) {

@@ -17,6 +17,7 @@ struct Options {
   int max_level = 7;      // fmm_max_level
   int q = 6;              // p2m_quadrature_points (main_time_domain.cpp:62)
   bool materialize_ops = true;  // fill Plan::ops (else only make sure every operator is in the cache)
+  bool fill_p2m = true;         // fill Plan::p2m_mat on the host (else only its layout and the per-leaf source lists)
 };
 
 // Box-blocked dense GEMV: for block b, rows [row_ptr[b], row_ptr[b+1]) and
@@ -79,8 +80,14 @@ struct Plan {
   // column-major per layer, p2m_mat[mat_ptr[b] + (layer * ncol + col) * p2m_rows_pad + row], rows padded
   // to p2m_rows_pad (C rounded up to 64), so that consecutive threads (rows) read consecutive words
   BlockGemv p2m;
-  std::vector<double> p2m_mat;            // p2m.mat_size
+  std::vector<double> p2m_mat;            // p2m.mat_size (if fill_p2m)
   int p2m_rows_pad = 0;
+  // P2M sources per leaf block (in block order): faces p2m_faces[p2m_face_ptr[l] .. p2m_face_ptr[l+1]), the
+  // block-local columns of each face's 3 DOFs (p2m_face_lc, 3 per face entry), and the leaf's centre and
+  // side (p2m_leaf_cs, 4 per leaf)
+  std::vector<int64_t> p2m_face_ptr;
+  std::vector<int32_t> p2m_faces, p2m_face_lc;
+  std::vector<double> p2m_leaf_cs;
   // translations
   std::vector<PairGroups> m2m_by_level;   // index = child level
   std::vector<PairGroups> l2l_by_level;   // index = child level
