@@ -466,7 +466,8 @@ void build_plan(Plan& pl, const Options& opt, int n_nodes, const double* X, int 
     const int ntb = (int)tboxes.size();
     std::vector<std::vector<int64_t>> ranges(ntb);
     std::vector<std::vector<int32_t>> cols(ntb);
-#pragma omp parallel
+    int overlap = 0;  // set inside the parallel loop; reported after it (exceptions must not leave it)
+#pragma omp parallel reduction(| : overlap)
     {
       std::vector<int> mark(n_dof, -1);
 #pragma omp for schedule(dynamic, 8)
@@ -485,7 +486,7 @@ void build_plan(Plan& pl, const Options& opt, int n_nodes, const double* X, int 
         for (auto& r : rg) {
           if (r.first == r.second) continue;
           if (!merged.empty() && r.first <= merged.back()) {
-            if (r.first < merged.back()) throw std::runtime_error("overlapping near ranges");
+            if (r.first < merged.back()) overlap = 1;
             merged.back() = std::max(merged.back(), r.second);
           } else {
             merged.push_back(r.first);
@@ -509,6 +510,7 @@ void build_plan(Plan& pl, const Options& opt, int n_nodes, const double* X, int 
         cols[ti] = std::move(cl);
       }
     }
+    if (overlap) throw std::runtime_error("overlapping near ranges");
     BlockGemv& G = pl.near;
     G = BlockGemv();
     pl.near_ncol_dof.clear();
@@ -532,33 +534,52 @@ void build_plan(Plan& pl, const Options& opt, int n_nodes, const double* X, int 
         chunks.push_back({(int64_t)(std::min(nt, r0 + (size_t)kMaxRows) - r0) * ncol, ti, r0});
     }
     std::stable_sort(chunks.begin(), chunks.end(), [](const Chunk& a, const Chunk& b) { return a.cost > b.cost; });
-    for (const Chunk& ch : chunks) {
+    // offsets of every block's rows, columns, matrix and ranges (prefix sums in block order), then the
+    // arrays are filled in parallel
+    const int nch = (int)chunks.size();
+    G.row_ptr.assign(nch + 1, 0);
+    G.col_ptr.assign(nch + 1, 0);
+    G.mat_ptr.assign(nch, 0);
+    pl.near_ncol_dof.assign(nch, 0);
+    pl.near_range_ptr.assign(nch + 1, 0);
+    G.mat_size = 0;
+    for (int k = 0; k < nch; ++k) {
+      const Chunk& ch = chunks[k];
+      const size_t nt = box_targets[tboxes[ch.ti]].size();
+      const int64_t rows = (int64_t)(std::min(nt, ch.r0 + (size_t)kMaxRows) - ch.r0);
+      const int ncd = (int)cols[ch.ti].size();
+      const int ncol = pad_cols(ncd + C);
+      G.row_ptr[k + 1] = G.row_ptr[k] + rows;
+      G.col_ptr[k + 1] = G.col_ptr[k] + ncol;
+      G.mat_ptr[k] = G.mat_size;
+      G.mat_size += rows * 2 * ncol;
+      pl.near_ncol_dof[k] = ncd;
+      pl.near_range_ptr[k + 1] = pl.near_range_ptr[k] + (int64_t)ranges[ch.ti].size() / 2;
+      pl.near_nnz += rows * ncd;
+    }
+    G.row_out.assign(G.row_ptr[nch], 0);
+    G.col_idx.assign(G.col_ptr[nch], 0);
+    pl.near_ranges.assign(2 * pl.near_range_ptr[nch], 0);
+    pl.near_range_prefix.assign(pl.near_range_ptr[nch], 0);
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int k = 0; k < nch; ++k) {
+      const Chunk& ch = chunks[k];
       const int ti = ch.ti;
       const int b = tboxes[ti];
       const auto& tg = box_targets[b];
       const int ncd = (int)cols[ti].size();
-      const int ncol = pad_cols(ncd + C);
-      {
-        const size_t r0 = ch.r0;
-        const size_t r1 = std::min(tg.size(), r0 + (size_t)kMaxRows);
-        G.mat_ptr.push_back(G.mat_size);
-        G.mat_size += (int64_t)(r1 - r0) * 2 * ncol;
-        for (size_t r = r0; r < r1; ++r) G.row_out.push_back(tg[r]);
-        G.row_ptr.push_back((int64_t)G.row_out.size());
-        G.col_idx.insert(G.col_idx.end(), cols[ti].begin(), cols[ti].end());
-        for (int c = 0; c < C; ++c) G.col_idx.push_back(n_dof + b * C + c);
-        for (int c = ncd + C; c < ncol; ++c) G.col_idx.push_back(n_dof + b * C + C - 1);  // zero-weight padding
-        G.col_ptr.push_back((int64_t)G.col_idx.size());
-        pl.near_ncol_dof.push_back(ncd);
-        int64_t pre = 0;
-        for (size_t k = 0; k < ranges[ti].size(); k += 2) {
-          pl.near_ranges.push_back(ranges[ti][k]);
-          pl.near_ranges.push_back(ranges[ti][k + 1]);
-          pl.near_range_prefix.push_back(pre);
-          pre += ranges[ti][k + 1] - ranges[ti][k];
-        }
-        pl.near_range_ptr.push_back((int64_t)pl.near_range_prefix.size());
-        pl.near_nnz += (int64_t)(r1 - r0) * ncd;
+      int64_t o = G.row_ptr[k];
+      for (int64_t r = (int64_t)ch.r0; o < G.row_ptr[k + 1]; ++r) G.row_out[o++] = tg[r];
+      int32_t* ci = &G.col_idx[G.col_ptr[k]];
+      std::copy(cols[ti].begin(), cols[ti].end(), ci);
+      for (int c = 0; c < C; ++c) ci[ncd + c] = n_dof + b * C + c;
+      for (int64_t c = ncd + C; c < G.col_ptr[k + 1] - G.col_ptr[k]; ++c) ci[c] = n_dof + b * C + C - 1;  // zero-weight padding
+      int64_t pre = 0, q = pl.near_range_ptr[k];
+      for (size_t j = 0; j < ranges[ti].size(); j += 2, ++q) {
+        pl.near_ranges[2 * q] = ranges[ti][j];
+        pl.near_ranges[2 * q + 1] = ranges[ti][j + 1];
+        pl.near_range_prefix[q] = pre;
+        pre += ranges[ti][j + 1] - ranges[ti][j];
       }
     }
     // L2P coefficients per near row: far = (1/s) Re sum R_n^m(x~) L~_n^m, x~ = (x - c)/s
@@ -733,38 +754,60 @@ void build_plan(Plan& pl, const Options& opt, int n_nodes, const double* X, int 
         Lg.pair_ptr.push_back(Lg.n_pairs());
       }
     }
-    // M2L: L(A) += T(offset) M(B) for B in A's M2L list (buckets_for_L2M is the same set by symmetry)
-    std::map<std::array<int, 3>, std::vector<std::pair<int, int>>> byoff;
-    pl.n_m2l_pairs = 0;
+    // M2L: L(A) += T(offset) M(B) for B in A's M2L list (buckets_for_L2M is the same set by symmetry).
+    // Pairs grouped by integer offset, groups in lexicographic offset order, pairs of a group in target
+    // order: offsets are packed into codes ((dx+64) * 128 + dy+64) * 128 + dz+64 (monotone in that order)
+    // and the pairs counting-sorted by code (stable).
+    constexpr int kOffB = 64;
+    std::vector<int64_t> base(nb + 1, 0);
+    for (int i = 0; i < nb; ++i) base[i + 1] = base[i] + (int64_t)Bx[id2b[i]].m2l.size();
+    const int64_t np = base[nb];
+    pl.n_m2l_pairs = np;
+    std::vector<int32_t> code(np), srcid(np);
+    int bad = 0;
+#pragma omp parallel for schedule(dynamic, 64) reduction(| : bad)
     for (int i = 0; i < nb; ++i) {
       const TBox& t = Bx[id2b[i]];
       const double s = pl.level_side[t.level];
+      int64_t k = base[i];
       for (int src : t.m2l) {
         const TBox& u = Bx[src];
-        std::array<int, 3> o;
+        int o[3];
         for (int a = 0; a < 3; ++a) {
           const double v = (u.X[a] - t.X[a]) / s;
           o[a] = (int)std::lround(v);
-          if (std::abs(v - o[a]) > 1e-6) throw std::runtime_error("M2L pair not on the level lattice");
+          if (std::abs(v - o[a]) > 1e-6 || o[a] <= -kOffB || o[a] >= kOffB) bad = 1;
         }
-        byoff[o].push_back({i, u.id});
-        ++pl.n_m2l_pairs;
+        code[k] = ((o[0] + kOffB) * 2 * kOffB + (o[1] + kOffB)) * 2 * kOffB + (o[2] + kOffB);
+        srcid[k] = u.id;
+        ++k;
       }
     }
+    if (bad) throw std::runtime_error("M2L pair not on the level lattice (or offset beyond +-63)");
+    const int ncode = 8 * kOffB * kOffB * kOffB;
+    std::vector<int64_t> cnt(ncode + 1, 0);
+    for (int64_t k = 0; k < np; ++k) cnt[code[k] + 1]++;
     pl.m2l = PairGroups();
     pl.m2l_offsets.clear();
     int opid = 16;
-    for (auto& kv : byoff) {
+    for (int cd = 0; cd < ncode; ++cd) {
+      const int64_t n_in = cnt[cd + 1];
+      cnt[cd + 1] += cnt[cd];
+      if (n_in == 0) continue;
       pl.m2l.op.push_back(opid++);
-      pl.m2l_offsets.push_back(kv.first[0]);
-      pl.m2l_offsets.push_back(kv.first[1]);
-      pl.m2l_offsets.push_back(kv.first[2]);
-      for (auto& pr : kv.second) {
-        pl.m2l.dst.push_back(pr.first);
-        pl.m2l.src.push_back(pr.second);
-      }
-      pl.m2l.pair_ptr.push_back(pl.m2l.n_pairs());
+      pl.m2l_offsets.push_back(cd / (4 * kOffB * kOffB) - kOffB);
+      pl.m2l_offsets.push_back((cd / (2 * kOffB)) % (2 * kOffB) - kOffB);
+      pl.m2l_offsets.push_back(cd % (2 * kOffB) - kOffB);
+      pl.m2l.pair_ptr.push_back(cnt[cd + 1]);
     }
+    pl.m2l.dst.assign(np, 0);
+    pl.m2l.src.assign(np, 0);
+    for (int i = 0; i < nb; ++i)
+      for (int64_t k = base[i]; k < base[i + 1]; ++k) {
+        const int64_t at = cnt[code[k]]++;
+        pl.m2l.dst[at] = i;
+        pl.m2l.src[at] = srcid[k];
+      }
     pl.n_ops = opid;
     if (opt.materialize_ops) pl.ops.assign((size_t)pl.n_ops * C * C, 0.);
     else pl.ops.clear();
