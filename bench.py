@@ -235,6 +235,12 @@ def run_batch(args):
     """64 independent tank BVPs (N log-uniform in [2k, 40k]) submitted at once to the request loop;
     every request includes its setup (new geometry). Also timed one at a time (one slot)."""
     from bem_for_nonlinear_waves_amd.serve import BVPServer
+    if not args.serve_anyway:
+        # like --mode serve: FP32+refinement requests on 8 worker threads ended with a segmentation fault of
+        # the process in the current build (intermittently: one earlier run passed); the cause has not been
+        # found, so the mode does not run until it is
+        sys.exit("bench.py --mode batch is disabled in this build: a run ended with a segmentation fault whose "
+                 "cause is not yet found (see the serving section of the design notes)")
     rng = np.random.default_rng(4)
     t = time.perf_counter()
     sizes = np.exp(rng.uniform(np.log(2000), np.log(40000), args.batch)).astype(int)
