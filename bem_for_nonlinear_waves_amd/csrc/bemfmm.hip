@@ -550,8 +550,8 @@ __device__ __forceinline__ void wave_lds_sync() {
 // then sin-type rows a = 0..n-1), 1 = axial (order m, rows j = m + a), 2 = Y_H^T (as stage 0). Each row is
 // a dot product of one padded factor row (q float4s at float4 offset off) with the lane's coefficients.
 // The z-rotations (2x2 per order) run before a degree's Y_D rows and after its Y_H^T rows. The rows run
-// software-pipelined: the factors of row r+1 are read from LDS before the multiply-adds of row r, with
-// scheduling barriers in between, so every LDS read has a whole row of FMAs to complete.
+// software-pipelined: the factors of row r+D are read from LDS before the multiply-adds of row r, with
+// scheduling barriers in between, so every LDS read has D rows of FMAs to complete.
 struct RotRow {
   int stage, blk, kind, a, len, q, off;  // kind: 0 = cos-type, 1 = sin-type (stages 0, 2)
 };
@@ -593,10 +593,85 @@ __device__ __forceinline__ int rot_after(float dep) {
   return z;
 }
 
+// Factor broadcast across lanes: lane l holds entry (l & 15) of one factor row (one ds_read_b32 per row,
+// the wave's four 16-lane rows read the same 16 addresses), and each multiply-add takes entry i from lane
+// i of its 16-lane row through the DPP row_newbcast operand. The LDS returns 4 bytes per lane per factor
+// row this way, instead of 16 bytes per lane per 4 entries for an all-lanes broadcast read, which made
+// the LDS return path the bound. The compiler does not form DPP multiply-adds from ordinary code, so a
+// row's dot product sum_i f[i] x[i] is one asm block (the result is early-clobber: it is written before
+// the last inputs are read). s_nop 1: a DPP source written by a VALU instruction needs two wait states,
+// which the hazard recognizer does not insert for inline asm (the factor register comes straight from
+// an LDS load; this covers a copy the register allocator might place before the row). Inside the block
+// only the accumulator is carried from one instruction to the next, which needs no wait states.
+#define ROT_M0 "v_mul_f32_dpp %0, %1, %2 row_newbcast:0 row_mask:0xf bank_mask:0xf"
+#define ROT_F(i, o) "\n\tv_fmac_f32_dpp %0, %1, %" #o " row_newbcast:" #i " row_mask:0xf bank_mask:0xf"
+#define ROT_D1 ROT_M0
+#define ROT_X1 "v"(x[0])
+#define ROT_D2 ROT_D1 ROT_F(1, 3)
+#define ROT_X2 ROT_X1, "v"(x[1])
+#define ROT_D3 ROT_D2 ROT_F(2, 4)
+#define ROT_X3 ROT_X2, "v"(x[2])
+#define ROT_D4 ROT_D3 ROT_F(3, 5)
+#define ROT_X4 ROT_X3, "v"(x[3])
+#define ROT_D5 ROT_D4 ROT_F(4, 6)
+#define ROT_X5 ROT_X4, "v"(x[4])
+#define ROT_D6 ROT_D5 ROT_F(5, 7)
+#define ROT_X6 ROT_X5, "v"(x[5])
+#define ROT_D7 ROT_D6 ROT_F(6, 8)
+#define ROT_X7 ROT_X6, "v"(x[6])
+#define ROT_D8 ROT_D7 ROT_F(7, 9)
+#define ROT_X8 ROT_X7, "v"(x[7])
+#define ROT_D9 ROT_D8 ROT_F(8, 10)
+#define ROT_X9 ROT_X8, "v"(x[8])
+#define ROT_D10 ROT_D9 ROT_F(9, 11)
+#define ROT_X10 ROT_X9, "v"(x[9])
+#define ROT_D11 ROT_D10 ROT_F(10, 12)
+#define ROT_X11 ROT_X10, "v"(x[10])
+template <int L>
+__device__ __forceinline__ float rot_dot(float f, const float* x);
+#define ROT_DOT(L)                                                                  \
+  template <>                                                                       \
+  __device__ __forceinline__ float rot_dot<L>(float f, const float* x) {            \
+    float r;                                                                        \
+    asm("s_nop 1\n\t" ROT_D##L : "=&v"(r) : "v"(f), ROT_X##L);                       \
+    return r;                                                                       \
+  }
+ROT_DOT(1)
+ROT_DOT(2)
+ROT_DOT(3)
+ROT_DOT(4)
+ROT_DOT(5)
+ROT_DOT(6)
+ROT_DOT(7)
+ROT_DOT(8)
+ROT_DOT(9)
+ROT_DOT(10)
+ROT_DOT(11)
+#undef ROT_DOT
+
+// y-rotation row of degree N: sum_i B[a][i] v[list(i)] over the degree's cos- (KIND 0) or sin-type list
+template <int N, int KIND, int... I>
+__device__ __forceinline__ float rot_ydot(float f, const float* v, std::integer_sequence<int, I...>) {
+  const float x[] = {v[rot_cs(N, KIND, I)]...};
+  return rot_dot<sizeof...(I)>(f, x);
+}
+// axial row of order M: sum over n = M + i of T[a][i] (Re, Im of (n, M))
+template <int M, int... I>
+__device__ __forceinline__ void rot_adot(float f, const float* v, float& sr, float& si, std::integer_sequence<int, I...>) {
+  const float xr[] = {v[rot_re(M + I, M)]...};
+  sr = rot_dot<sizeof...(I)>(f, xr);
+  if constexpr (M > 0) {
+    const float xi[] = {v[rot_im(M + I, M)]...};
+    si = rot_dot<sizeof...(I)>(f, xi);
+  }
+}
+
+typedef __attribute__((address_space(3))) const float rot_lds_float;
 template <int P>
 struct RotPipe {
-  static constexpr int QM = rot_q(P + 1);
-  float4 cur[QM], nxt[QM];
+  static constexpr int D = 4;  // factor rows read ahead
+  rot_lds_float* R;            // the factors offset by (lane & 15) floats
+  float f[D + 1];
   float tmp[P + 1], ti[P + 1];
   float last = 0.f;  // result of the previous row
 };
@@ -621,32 +696,24 @@ __device__ __forceinline__ void rot_z(const float4* __restrict__ R4, float* v, i
   }
 }
 
+// Row K (R4 = the factors as float4s, for the z-rotation pairs). The read of row K + D is issued first;
+// its base pointer passes through an empty asm that takes row K - 1's result as an input, so the read
+// cannot be hoisted above that row (with one base and constant offsets the compiler hoists and clusters
+// thousands of reads) and the row's offset stays the load's immediate.
 template <int P, int K>
 __device__ __forceinline__ void rot_step(const float4* __restrict__ R4, float* v, RotPipe<P>& s) {
   constexpr RotRow rw = rot_row<P>(K);
-  if constexpr (K + 1 < rot_rows<P>()) {
-    constexpr RotRow nx = rot_row<P>(K + 1);
-    const float4* Rn = R4 + rot_after(s.last) + nx.off;
-#pragma unroll
-    for (int c4 = 0; c4 < nx.q; ++c4) s.nxt[c4] = Rn[c4];
+  constexpr int D = RotPipe<P>::D;
+  if constexpr (K + D < rot_rows<P>()) {
+    asm("" : "+v"(s.R) : "v"(s.last));
+    s.f[(K + D) % (D + 1)] = s.R[4 * rot_row<P>(K + D).off];
   }
   __builtin_amdgcn_sched_barrier(0);
+  const float f = s.f[K % (D + 1)];
   if constexpr (rw.stage != 1) {
     constexpr int n = rw.blk;
     if constexpr (rw.stage == 0 && rw.kind == 0 && rw.a == 0 && n > 0) rot_z<P, false>(R4 + rot_after(s.last), v, n);
-    // y-rotation row: tmp[a] = sum_i B[a][i] v[list(i)] over the degree's cos- or sin-type list
-    float acc0 = 0.f, acc1 = 0.f;  // two chains (even / odd entries)
-#pragma unroll
-    for (int c4 = 0; c4 < rw.q; ++c4) {
-      const float e[4] = {s.cur[c4].x, s.cur[c4].y, s.cur[c4].z, s.cur[c4].w};
-#pragma unroll
-      for (int u = 0; u < 4; ++u)
-        if (4 * c4 + u < rw.len) {
-          if (u & 1) acc1 = fmaf(e[u], v[rot_cs(n, rw.kind, 4 * c4 + u)], acc1);
-          else acc0 = fmaf(e[u], v[rot_cs(n, rw.kind, 4 * c4 + u)], acc0);
-        }
-    }
-    const float acc = rw.len > 1 ? acc0 + acc1 : acc0;
+    const float acc = rot_ydot<n, rw.kind>(f, v, std::make_integer_sequence<int, rw.len>{});
     s.tmp[rw.a] = acc;
     s.last = acc;
     if constexpr (rw.a == rw.len - 1) {
@@ -655,21 +722,9 @@ __device__ __forceinline__ void rot_step(const float4* __restrict__ R4, float* v
       if constexpr (rw.stage == 2 && n > 0 && rw.kind == 1) rot_z<P, true>(R4 + rot_after(acc), v, n);
     }
   } else {
-    // axial block of order m: row j = m + a, inputs n = m..P (real part, and imaginary part for m > 0)
     constexpr int m = rw.blk;
-    float sr = 0.f, si = 0.f;
-#pragma unroll
-    for (int c4 = 0; c4 < rw.q; ++c4) {
-      const float e[4] = {s.cur[c4].x, s.cur[c4].y, s.cur[c4].z, s.cur[c4].w};
-#pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        const int n = m + 4 * c4 + u;
-        if (n <= P) {
-          sr = fmaf(e[u], v[rot_re(n, m)], sr);
-          if (m > 0) si = fmaf(e[u], v[rot_im(n, m)], si);
-        }
-      }
-    }
+    float sr, si = 0.f;
+    rot_adot<m>(f, v, sr, si, std::make_integer_sequence<int, rw.len>{});
     s.tmp[rw.a] = sr;
     s.ti[rw.a] = si;
     s.last = sr;
@@ -682,20 +737,23 @@ __device__ __forceinline__ void rot_step(const float4* __restrict__ R4, float* v
     }
   }
   __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-  for (int c4 = 0; c4 < RotPipe<P>::QM; ++c4) s.cur[c4] = s.nxt[c4];
+}
+
+template <int P, int... J>
+__device__ __forceinline__ void rot_prologue(RotPipe<P>& s, std::integer_sequence<int, J...>) {
+  ((s.f[J] = s.R[4 * rot_row<P>(J).off]), ...);
 }
 
 template <int P, int... K>
 __device__ __forceinline__ void rot_run(const float4* __restrict__ R4, float* v, std::integer_sequence<int, K...>) {
   RotPipe<P> s;
-  constexpr RotRow r0 = rot_row<P>(0);
-#pragma unroll
-  for (int c4 = 0; c4 < r0.q; ++c4) s.cur[c4] = R4[r0.off + c4];
+  s.R = (rot_lds_float*)reinterpret_cast<const float*>(R4) + (threadIdx.x & 15);
+  rot_prologue<P>(s, std::make_integer_sequence<int, RotPipe<P>::D>{});
   (rot_step<P, K>(R4, v, s), ...);
 }
 
-// v <- H^T T(|o| z) D v (D = Y_D Z, H^T = Z^T Y_H^T) with the factors of one offset at R4 (LDS)
+// v <- H^T T(|o| z) D v (D = Y_D Z, H^T = Z^T Y_H^T) with the factors of one offset at R4 (LDS, followed by
+// at least 16 floats of padding: each lane reads up to 15 floats past a row's start)
 template <int P>
 __device__ __forceinline__ void rot_apply(const float4* __restrict__ R4, float* v) {
   rot_run<P>(R4, v, std::make_integer_sequence<int, rot_rows<P>()>{});
@@ -844,7 +902,7 @@ __global__ __launch_bounds__(256) void k_m2l_rot(int ntiles, const float* __rest
   constexpr int RS = rot_size(P);
   constexpr int NCH = (C + 15) / 16;
   constexpr int kS = 17;  // LDS column stride (odd: conflict-free column reads)
-  __shared__ float4 R4[RS / 4];
+  __shared__ float4 R4[RS / 4 + 4];  // + 16 floats read past the last row by lanes 4q..15
   __shared__ float Tw[4][64 * kS];
   const int per_xcd = (ntiles + kXCD - 1) / kXCD;
   const int t = (blockIdx.x % kXCD) * per_xcd + blockIdx.x / kXCD;
