@@ -344,8 +344,16 @@ struct Mfma<float> {
   static __device__ __forceinline__ int row(int lk, int i) { return 4 * lk + i; }
 };
 
+// Values per staged M2L column. FP32: C rounded up to 32, so every column starts on a 128-byte line and the
+// staging is written and read in whole lines (at stride C = 121 the rotation M2L's 16-row stores straddled
+// lines, and its writes and the reduction's reads moved 14% and 7% more than the staged bytes); C <= 128
+// (checked in prepare) keeps it <= 128. FP64: C (each wave writes whole 968-byte columns at once: no
+// partial lines to remove, and the padding only added bytes).
+template <typename T>
+__host__ __device__ constexpr int stage_stride(int C) { return sizeof(T) == 4 ? (C + 31) & ~31 : C; }
 // STAGE = false: atomically accumulate into D0/D1 at the pair's dst box (M2M, L2L).
-// STAGE = true: write each pair's two result columns to stage[(pair*2 + layer)*C + row] (M2L).
+// STAGE = true: write each pair's two result columns to stage[(pair*2 + layer)*CS + row] (M2L), CS =
+// stage_stride<T>(C) (rows C..CS-1 written as zeros).
 template <typename T, int RTW, bool STAGE, int TP>
 __global__ __launch_bounds__(256) void k_translate_mfma(int C, int RP, int KP, int ntiles, const T* __restrict__ ops,
                                                         const int32_t* __restrict__ tile_op, const int64_t* __restrict__ tile_p0,
@@ -440,8 +448,9 @@ __global__ __launch_bounds__(256) void k_translate_mfma(int C, int RP, int KP, i
     __syncthreads();
     for (int col = wave; col < 2 * np; col += 4) {
       const int pair = col >> 1, layer = col & 1;
-      T* out = stage + ((p0 + pair) * 2 + layer) * (int64_t)C;
-      for (int r = lane; r < C; r += 64) out[r] = Tt[col * (RP + 1) + r];
+      const int CS = stage_stride<T>(C);  // <= RP
+      T* out = stage + ((p0 + pair) * 2 + layer) * (int64_t)CS;
+      for (int r = lane; r < CS; r += 64) out[r] = r < C ? Tt[col * (RP + 1) + r] : T(0);
     }
     return;
   }
@@ -474,19 +483,21 @@ __global__ __launch_bounds__(256) void k_m2l_reduce(int C, const int64_t* __rest
   const int j = threadIdx.x;
   if (j >= 2 * C) return;
   const int64_t a = tptr[t], b = tptr[t + 1];
+  const int CS = stage_stride<T>(C);
+  const int jj = j < C ? j : CS + j - C;  // (layer, row) -> offset in the pair's two staged columns
   double s0 = 0., s1 = 0., s2 = 0., s3 = 0.;
   int64_t q = a;
   for (; q + 3 < b; q += 4) {
-    const double v0 = stage[(int64_t)tpairs[q] * 2 * C + j];
-    const double v1 = stage[(int64_t)tpairs[q + 1] * 2 * C + j];
-    const double v2 = stage[(int64_t)tpairs[q + 2] * 2 * C + j];
-    const double v3 = stage[(int64_t)tpairs[q + 3] * 2 * C + j];
+    const double v0 = stage[(int64_t)tpairs[q] * 2 * CS + jj];
+    const double v1 = stage[(int64_t)tpairs[q + 1] * 2 * CS + jj];
+    const double v2 = stage[(int64_t)tpairs[q + 2] * 2 * CS + jj];
+    const double v3 = stage[(int64_t)tpairs[q + 3] * 2 * CS + jj];
     s0 += v0;
     s1 += v1;
     s2 += v2;
     s3 += v3;
   }
-  for (; q < b; ++q) s0 += stage[(int64_t)tpairs[q] * 2 * C + j];
+  for (; q < b; ++q) s0 += stage[(int64_t)tpairs[q] * 2 * CS + jj];
   const double v = (s0 + s1) + (s2 + s3);
   if (j < C) L0[(int64_t)t * C + j] = v;
   else L1[(int64_t)t * C + j - C] = v;
@@ -494,12 +505,13 @@ __global__ __launch_bounds__(256) void k_m2l_reduce(int C, const int64_t* __rest
 
 // M2L by rotation, FP32 operator: per pair L = H^T T(|o| z) D M, where D and H^T are the per-degree
 // ((2n+1)^2) real rotation blocks taking the offset o onto the z axis and T(|o| z) is the axial operator
-// (one (P+1-m)^2 block per order m, the same for the real and imaginary parts): 4433 multiply-adds per
-// column at P = 10 instead of 14641. A workgroup takes up to 128 pairs of one offset: the factors are
-// loaded once into LDS and read by every lane at the same address (broadcast); each of the 4 waves owns 32
-// pairs, lane 2u + layer holding column (pair u, layer) in registers. Columns move between the [box][C]
-// layouts and the lanes through a per-wave LDS buffer, 16 rows at a time (coalesced loads and stores).
-// Output: stage[(pair*2 + layer)*C + row], as the matrix-core M2L with STAGE.
+// (one (P+1-m)^2 block per order m, the same for the real and imaginary parts), with the z-rotations as
+// 2x2 blocks per order: 3113 multiply-adds per column at P = 10 instead of 14641. A workgroup takes up to
+// 128 pairs of one offset: the factors are loaded once into LDS, and each factor row is spread over the
+// lanes of every 16-lane row and broadcast to the multiply-adds through DPP (rot_dot); each of the 4 waves
+// owns 32 pairs, lane 2u + layer holding column (pair u, layer) in registers. Columns move between the
+// [box][C] layouts and the lanes through a per-wave LDS buffer, 16 rows at a time (coalesced loads and
+// stores). Output: stage[(pair*2 + layer)*CS + row], as the matrix-core M2L with STAGE.
 constexpr int kRotWavePairs = 32;
 constexpr int kRotTP = 4 * kRotWavePairs;  // pairs per workgroup tile
 // Device layout of the factors of one offset, in float4s, every row zero-padded to whole float4s
@@ -949,34 +961,37 @@ __global__ __launch_bounds__(256) void k_m2l_rot(int ntiles, const float* __rest
   rot_apply<P>(R4, v);
   __builtin_amdgcn_sched_barrier(0);
   // ---- scatter: lane's registers -> LDS -> rows [16 ch, 16 ch + 16) of column (4i + g) ----
-  float* out = stage + pw * 2 * C;
+  constexpr int CS = stage_stride<float>(C);
+  static_assert(16 * NCH == CS, "the staged columns are written 16 rows at a time");
+  float* out = stage + pw * 2 * CS;
 #pragma unroll
   for (int ch = 0; ch < NCH; ++ch) {
 #pragma unroll
-    for (int kk = 0; kk < 16; ++kk)
-      if (ch * 16 + kk < C) T[lane * kS + kk] = v[ch * 16 + kk];
+    for (int kk = 0; kk < 16; ++kk) T[lane * kS + kk] = ch * 16 + kk < C ? v[ch * 16 + kk] : 0.f;
     wave_lds_sync();
     const int r = ch * 16 + k;
 #pragma unroll
     for (int i = 0; i < 16; ++i) {
       const int col = 4 * i + g;
-      if (col < 2 * np && r < C) out[(int64_t)col * C + r] = T[col * kS + k];
+      if (col < 2 * np) out[(int64_t)col * CS + r] = T[col * kS + k];
     }
     wave_lds_sync();
   }
 }
 
-// FP32 staging: the same reduction with 8-byte loads. Thread t < C of each half-block reads floats
-// (2t, 2t+1) of a pair's 2C staged values; the two half-blocks take alternate pairs, 4 pairs in flight
-// each, and are combined in a fixed order (deterministic).
+// FP32 staging: the same reduction with 8-byte loads. Thread t < CS of each half-block reads float2 t of
+// a pair's two staged columns (rows 2t, 2t+1 of one layer, CS = stage_stride<float>(C)); the two
+// half-blocks take alternate pairs, 4 pairs in flight each, and are combined in a fixed order
+// (deterministic).
 __global__ __launch_bounds__(256) void k_m2l_reduce_f32(int C, const int64_t* __restrict__ tptr,
                                                         const int32_t* __restrict__ tpairs,
                                                         const float* __restrict__ stage, double* __restrict__ L0,
                                                         double* __restrict__ L1) {
   __shared__ double2 part[128];
+  const int CS = stage_stride<float>(C);  // <= 128, even
   const int tb = blockIdx.x;
   const int h = threadIdx.x >> 7, t = threadIdx.x & 127;
-  const bool act = t < C;
+  const bool act = t < CS;  // thread t: float2 t of a pair's two staged columns = rows 2t, 2t+1 of one layer
   const int64_t a = tptr[tb], b = tptr[tb + 1];
   double sx[4] = {0., 0., 0., 0.}, sy[4] = {0., 0., 0., 0.};
   if (act) {
@@ -984,7 +999,7 @@ __global__ __launch_bounds__(256) void k_m2l_reduce_f32(int C, const int64_t* __
     for (; q + 6 < b; q += 8) {
       float2 v[4];
 #pragma unroll
-      for (int u = 0; u < 4; ++u) v[u] = reinterpret_cast<const float2*>(stage + (int64_t)tpairs[q + 2 * u] * 2 * C)[t];
+      for (int u = 0; u < 4; ++u) v[u] = reinterpret_cast<const float2*>(stage + (int64_t)tpairs[q + 2 * u] * 2 * CS)[t];
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
         sx[u] += v[u].x;
@@ -992,7 +1007,7 @@ __global__ __launch_bounds__(256) void k_m2l_reduce_f32(int C, const int64_t* __
       }
     }
     for (; q < b; q += 2) {
-      const float2 v = reinterpret_cast<const float2*>(stage + (int64_t)tpairs[q] * 2 * C)[t];
+      const float2 v = reinterpret_cast<const float2*>(stage + (int64_t)tpairs[q] * 2 * CS)[t];
       sx[0] += v.x;
       sy[0] += v.y;
     }
@@ -1003,9 +1018,10 @@ __global__ __launch_bounds__(256) void k_m2l_reduce_f32(int C, const int64_t* __
   if (h == 0 && act) {
     x += part[t].x;
     y += part[t].y;
-    const int j = 2 * t;
-    (j < C ? L0[(int64_t)tb * C + j] : L1[(int64_t)tb * C + j - C]) = x;
-    (j + 1 < C ? L0[(int64_t)tb * C + j + 1] : L1[(int64_t)tb * C + j + 1 - C]) = y;
+    const int layer = 2 * t >= CS ? 1 : 0, r = 2 * t - layer * CS;
+    double* L = (layer ? L1 : L0) + (int64_t)tb * C;
+    if (r < C) L[r] = x;
+    if (r + 1 < C) L[r + 1] = y;
   }
 }
 
@@ -2095,7 +2111,9 @@ void do_prepare(bemfmm_ctx* c) {
     for (int64_t q = 0; q < (int64_t)pl.m2l.dst.size(); ++q) tp[fillp[pl.m2l.dst[q]]++] = (int32_t)q;
     c->m2l_tptr.upload(tptr, s);
     c->m2l_tpairs.upload(tp, s);
-    c->m2l_stage.alloc(std::max<size_t>(1, pl.m2l.dst.size() * 2 * (size_t)pl.C));
+    // FP64 [pair][layer][C], or FP32 [pair][layer][stage_stride<float>(C)] in the same buffer
+    const size_t col = std::max<size_t>(pl.C, (stage_stride<float>(pl.C) + 1) / 2);
+    c->m2l_stage.alloc(std::max<size_t>(1, pl.m2l.dst.size() * 2 * col));
   }
   c->kind.upload(pl.dof_kind, s);
   c->point.upload(pl.dof_point, s);
