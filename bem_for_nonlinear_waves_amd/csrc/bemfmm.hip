@@ -1689,10 +1689,12 @@ void evaluate_launch(bemfmm_ctx* c, int mode, const double* in, double* out, boo
                          c->p2m.mat.p, phin, phi, c->M0.p, c->M1.p);
   }
   rec(2);
-  // M2M from level l into l-1 only where level l-1 has M2L pairs (coarser multipoles are not used)
+  // M2M from level l into l-1 only where level l-1 has M2L pairs (coarser multipoles are not used).
+  // M2M and L2L use the FP64 operators for both operators: their inputs and outputs are FP64, and the
+  // FP64 matrix-core tiles ran faster here than the FP32 ones (0.115 / 0.120 ms against 0.157 / 0.161 ms
+  // per matvec at 209k nodes)
   for (int l = pl.n_levels - 1; l >= 1 && l - 1 >= c->m2l_min_level; --l) {
-    if (lo) launch_translate(c, c->m2m[l], c->ops32.p, c->M0.p, c->M1.p, c->M0.p, c->M1.p);
-    else launch_translate(c, c->m2m[l], c->ops.p, c->M0.p, c->M1.p, c->M0.p, c->M1.p);
+    launch_translate(c, c->m2m[l], c->ops.p, c->M0.p, c->M1.p, c->M0.p, c->M1.p);
   }
   rec(3);
   if (lo) {
@@ -1711,8 +1713,7 @@ void evaluate_launch(bemfmm_ctx* c, int mode, const double* in, double* out, boo
   rec(5);
   // L2L into level l only from parents at levels with M2L pairs (the locals of coarser levels are zero)
   for (int l = std::max(1, c->m2l_min_level + 1); l < pl.n_levels; ++l) {
-    if (lo) launch_translate(c, c->l2l[l], c->ops32.p, L0, L1, L0, L1);
-    else launch_translate(c, c->l2l[l], c->ops.p, L0, L1, L0, L1);
+    launch_translate(c, c->l2l[l], c->ops.p, L0, L1, L0, L1);
   }
   rec(6);
   if (split) HIPCHECK(hipStreamWaitEvent(c->stream, c->ev_join, 0));
@@ -2295,25 +2296,28 @@ void do_prepare(bemfmm_ctx* c) {
     c->p2m_mat32.alloc(c->p2m.mat.n);
     hipLaunchKernelGGL(k_to_f32, dim3(grid1(c->p2m.mat.n)), dim3(256), 0, s, (int64_t)c->p2m.mat.n, c->p2m.mat.p,
                        c->p2m_mat32.p);
-    // FP32 operator table: convert the slots that are new (all of them after the table grew)
-    const int64_t per = (int64_t)c->KP * c->RP;
-    if (c->ops32.n != c->ops.n) {
-      c->ops32.alloc(c->ops.n);
-      c->ops32_valid = 0;
-    }
-    if (c->ops32_valid < c->ops_used) {
-      const int64_t n32 = (int64_t)(c->ops_used - c->ops32_valid) * per;
-      hipLaunchKernelGGL(k_to_f32, dim3(grid1(n32)), dim3(256), 0, s, n32, c->ops.p + c->ops32_valid * per,
-                         c->ops32.p + c->ops32_valid * per);
-      c->ops32_valid = c->ops_used;
-    }
-    HIPCHECK(hipGetLastError());
     c->m2l_rot = c->opts.m2l_kernel != 1 && rot_compiled(pl.P);
     if (c->m2l_rot) {
       update_rotation_table(c, pl, c->m2l_slot);
       c->m2l_rt.upload(pl.m2l, s, c->m2l_slot, kRotTP);
       c->m32.alloc((size_t)pl.n_boxes * 2 * pl.C);
+      c->ops32.release();  // M2M and L2L use the FP64 table
+      c->ops32_valid = 0;
+    } else {
+      // FP32 operator table for the dense M2L: convert the slots that are new (all after the table grew)
+      const int64_t per = (int64_t)c->KP * c->RP;
+      if (c->ops32.n != c->ops.n) {
+        c->ops32.alloc(c->ops.n);
+        c->ops32_valid = 0;
+      }
+      if (c->ops32_valid < c->ops_used) {
+        const int64_t n32 = (int64_t)(c->ops_used - c->ops32_valid) * per;
+        hipLaunchKernelGGL(k_to_f32, dim3(grid1(n32)), dim3(256), 0, s, n32, c->ops.p + c->ops32_valid * per,
+                           c->ops32.p + c->ops32_valid * per);
+        c->ops32_valid = c->ops_used;
+      }
     }
+    HIPCHECK(hipGetLastError());
   } else {
     c->near_mat32.release();
     c->p2m_mat32.release();
