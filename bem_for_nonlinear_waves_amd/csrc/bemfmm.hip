@@ -473,32 +473,31 @@ __global__ __launch_bounds__(256) void k_translate_mfma(int C, int RP, int KP, i
 }
 
 // M2L reduction: L[t] = sum over the M2L pairs of box t (fixed order) of the staged per-pair
-// results (FP64 accumulation). One workgroup per box; thread j < 2C owns (layer j/C, row j%C).
-// Writes every box, so L needs no clearing before it.
+// results (FP64 accumulation). One workgroup per box, the boxes launched in descending pair count
+// (order: the longest reductions start first, no straggler tail); thread j < 2C owns (layer j/C, row
+// j%C). Writes every box, so L needs no clearing before it.
 template <typename T>
-__global__ __launch_bounds__(256) void k_m2l_reduce(int C, const int64_t* __restrict__ tptr, const int32_t* __restrict__ tpairs,
+__global__ __launch_bounds__(256) void k_m2l_reduce(int C, const int32_t* __restrict__ order,
+                                                    const int64_t* __restrict__ tptr, const int32_t* __restrict__ tpairs,
                                                     const T* __restrict__ stage, double* __restrict__ L0,
                                                     double* __restrict__ L1) {
-  const int t = blockIdx.x;
+  const int t = order[blockIdx.x];
   const int j = threadIdx.x;
   if (j >= 2 * C) return;
   const int64_t a = tptr[t], b = tptr[t + 1];
   const int CS = stage_stride<T>(C);
   const int jj = j < C ? j : CS + j - C;  // (layer, row) -> offset in the pair's two staged columns
-  double s0 = 0., s1 = 0., s2 = 0., s3 = 0.;
+  double s[4] = {0., 0., 0., 0.};
   int64_t q = a;
-  for (; q + 3 < b; q += 4) {
-    const double v0 = stage[(int64_t)tpairs[q] * 2 * CS + jj];
-    const double v1 = stage[(int64_t)tpairs[q + 1] * 2 * CS + jj];
-    const double v2 = stage[(int64_t)tpairs[q + 2] * 2 * CS + jj];
-    const double v3 = stage[(int64_t)tpairs[q + 3] * 2 * CS + jj];
-    s0 += v0;
-    s1 += v1;
-    s2 += v2;
-    s3 += v3;
+  for (; q + 7 < b; q += 8) {  // 8 loads in flight
+    double v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) v[u] = stage[(int64_t)tpairs[q + u] * 2 * CS + jj];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) s[u & 3] += v[u];
   }
-  for (; q < b; ++q) s0 += stage[(int64_t)tpairs[q] * 2 * CS + jj];
-  const double v = (s0 + s1) + (s2 + s3);
+  for (; q < b; ++q) s[0] += stage[(int64_t)tpairs[q] * 2 * CS + jj];
+  const double v = (s[0] + s[1]) + (s[2] + s[3]);
   if (j < C) L0[(int64_t)t * C + j] = v;
   else L1[(int64_t)t * C + j - C] = v;
 }
@@ -981,21 +980,32 @@ __global__ __launch_bounds__(256) void k_m2l_rot(int ntiles, const float* __rest
 
 // FP32 staging: the same reduction with 8-byte loads. Thread t < CS of each half-block reads float2 t of
 // a pair's two staged columns (rows 2t, 2t+1 of one layer, CS = stage_stride<float>(C)); the two
-// half-blocks take alternate pairs, 4 pairs in flight each, and are combined in a fixed order
-// (deterministic).
-__global__ __launch_bounds__(256) void k_m2l_reduce_f32(int C, const int64_t* __restrict__ tptr,
+// half-blocks take alternate pairs, 8 pairs in flight each, and are combined in a fixed order
+// (deterministic). Boxes run in descending pair count (order), as in k_m2l_reduce.
+__global__ __launch_bounds__(256) void k_m2l_reduce_f32(int C, const int32_t* __restrict__ order,
+                                                        const int64_t* __restrict__ tptr,
                                                         const int32_t* __restrict__ tpairs,
                                                         const float* __restrict__ stage, double* __restrict__ L0,
                                                         double* __restrict__ L1) {
   __shared__ double2 part[128];
   const int CS = stage_stride<float>(C);  // <= 128, even
-  const int tb = blockIdx.x;
+  const int tb = order[blockIdx.x];
   const int h = threadIdx.x >> 7, t = threadIdx.x & 127;
   const bool act = t < CS;  // thread t: float2 t of a pair's two staged columns = rows 2t, 2t+1 of one layer
   const int64_t a = tptr[tb], b = tptr[tb + 1];
   double sx[4] = {0., 0., 0., 0.}, sy[4] = {0., 0., 0., 0.};
   if (act) {
     int64_t q = a + h;
+    for (; q + 14 < b; q += 16) {  // 8 loads in flight
+      float2 v[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) v[u] = reinterpret_cast<const float2*>(stage + (int64_t)tpairs[q + 2 * u] * 2 * CS)[t];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        sx[u & 3] += v[u].x;
+        sy[u & 3] += v[u].y;
+      }
+    }
     for (; q + 6 < b; q += 8) {
       float2 v[4];
 #pragma unroll
@@ -1509,6 +1519,7 @@ struct bemfmm_ctx {
   std::vector<DTiles> m2m, l2l;
   DTiles m2l;
   DBuf<int64_t> m2l_tptr;    // per box: its M2L pairs in m2l_tpairs
+  DBuf<int32_t> m2l_rorder;  // boxes in descending M2L pair count (launch order of the reduction)
   DBuf<int32_t> m2l_tpairs;
   DBuf<double> m2l_stage;    // per-pair M2L results [pair][layer][C]
   DBuf<double> ops;
@@ -1702,12 +1713,12 @@ void evaluate_launch(bemfmm_ctx* c, int mode, const double* in, double* out, boo
     if (c->m2l_rot) launch_m2l_rot(c, st);
     else launch_translate(c, c->m2l, c->ops32.p, c->M0.p, c->M1.p, nullptr, nullptr, st);
     rec(4);
-    hipLaunchKernelGGL(k_m2l_reduce_f32, dim3(c->n_boxes), dim3(256), 0, c->stream, c->C, c->m2l_tptr.p,
+    hipLaunchKernelGGL(k_m2l_reduce_f32, dim3(c->n_boxes), dim3(256), 0, c->stream, c->C, c->m2l_rorder.p, c->m2l_tptr.p,
                        c->m2l_tpairs.p, (const float*)st, L0, L1);
   } else {
     launch_translate(c, c->m2l, c->ops.p, c->M0.p, c->M1.p, nullptr, nullptr, c->m2l_stage.p);
     rec(4);
-    hipLaunchKernelGGL(k_m2l_reduce<double>, dim3(c->n_boxes), dim3(256), 0, c->stream, c->C, c->m2l_tptr.p,
+    hipLaunchKernelGGL(k_m2l_reduce<double>, dim3(c->n_boxes), dim3(256), 0, c->stream, c->C, c->m2l_rorder.p, c->m2l_tptr.p,
                        c->m2l_tpairs.p, (const double*)c->m2l_stage.p, L0, L1);
   }
   rec(5);
@@ -2112,6 +2123,11 @@ void do_prepare(bemfmm_ctx* c) {
     for (int64_t q = 0; q < (int64_t)pl.m2l.dst.size(); ++q) tp[fillp[pl.m2l.dst[q]]++] = (int32_t)q;
     c->m2l_tptr.upload(tptr, s);
     c->m2l_tpairs.upload(tp, s);
+    std::vector<int32_t> rorder(pl.n_boxes);
+    for (int b = 0; b < pl.n_boxes; ++b) rorder[b] = b;
+    std::stable_sort(rorder.begin(), rorder.end(),
+                     [&](int32_t x, int32_t y) { return tptr[x + 1] - tptr[x] > tptr[y + 1] - tptr[y]; });
+    c->m2l_rorder.upload(rorder, s);
     // FP64 [pair][layer][C], or FP32 [pair][layer][stage_stride<float>(C)] in the same buffer
     const size_t col = std::max<size_t>(pl.C, (stage_stride<float>(pl.C) + 1) / 2);
     c->m2l_stage.alloc(std::max<size_t>(1, pl.m2l.dst.size() * 2 * col));
