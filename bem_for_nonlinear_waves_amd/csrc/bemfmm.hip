@@ -1545,6 +1545,7 @@ struct bemfmm_ctx {
   DBuf<int32_t> near_ncd;  // DOF columns per near block
   double* op_stage = nullptr;  // pinned host staging for operator uploads
   size_t op_stage_n = 0;
+  double* hres = nullptr;      // pinned host copy of hbuf (the Arnoldi column of one iteration)
   DBuf<int32_t> kind, point;
   DBuf<double> alpha, phi_k, phin_k;
   DBuf<double> buf0, buf1;   // [phi_n | L0], [phi | L1]
@@ -2387,18 +2388,26 @@ void ensure_basis(bemfmm_ctx* c, int64_t need) {
   std::swap(c->V.n, nv.n);
   c->v_cap = cap;
   c->partial.alloc((size_t)kDotBlocks * (cap + 1));
-  c->hbuf.alloc((size_t)cap + 1);
+  // [h1 (k) | h2 (k) | w.w] of one Arnoldi step (k <= cap), or the k update coefficients
+  c->hbuf.alloc(2 * (size_t)cap + 2);
+  if (c->hres) (void)hipHostFree(c->hres);
+  c->hres = nullptr;
+  HIPCHECK(hipHostMalloc((void**)&c->hres, (2 * (size_t)cap + 2) * sizeof(double), hipHostMallocDefault));
 }
 
-// h[0..k) = V[0..k) . w  (device), copied to host
-void dots(bemfmm_ctx* c, int k, const double* Vp, const double* w, std::vector<double>& h) {
+// hbuf[off, off + k) = V[0..k) . w on the device (no host copy)
+void dots_dev(bemfmm_ctx* c, int k, const double* Vp, const double* w, size_t off) {
   const int64_t n = c->n_dof;
   // the reduction buffers are sized by ensure_basis for up to v_cap + 1 products
   if (k < 1 || k > c->v_cap + 1 || !c->partial.p || !c->hbuf.p || (size_t)k * kDotBlocks > c->partial.n ||
-      (size_t)k > c->hbuf.n)
+      off + (size_t)k > c->hbuf.n)
     throw HipError("internal: dot-product buffers not sized for " + std::to_string(k) + " products", BEMFMM_E_STATE);
   hipLaunchKernelGGL(k_dots, dim3(kDotBlocks, k), dim3(256), 0, c->stream, Vp, n, w, n, c->partial.p);
-  hipLaunchKernelGGL(k_reduce_partials, dim3(k), dim3(256), 0, c->stream, c->partial.p, kDotBlocks, k, c->hbuf.p);
+  hipLaunchKernelGGL(k_reduce_partials, dim3(k), dim3(256), 0, c->stream, c->partial.p, kDotBlocks, k, c->hbuf.p + off);
+}
+// h[0..k) = V[0..k) . w  (device), copied to host
+void dots(bemfmm_ctx* c, int k, const double* Vp, const double* w, std::vector<double>& h) {
+  dots_dev(c, k, Vp, w, 0);
   h.resize(k);
   HIPCHECK(hipMemcpyAsync(h.data(), c->hbuf.p, k * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HIPCHECK(hipStreamSynchronize(c->stream));
@@ -2464,7 +2473,6 @@ double gmres_run(bemfmm_ctx* c, const GmresRun& R, int& iters) {
   auto Av = [&](const double* v, double* w) { apply_op(c, v, w, R.lo); };
   double cur_err = 1e300;
   const int restart_max = std::max(1, R.restart);
-  std::vector<double> h1, h2;
   bool first = true;
   while (iters < R.max_iter && cur_err > R.tol) {
     const int remaining = R.max_iter - iters;
@@ -2493,13 +2501,18 @@ double gmres_run(bemfmm_ctx* c, const GmresRun& R, int& iters) {
       ensure_basis(c, nb + 1);
       double* w = c->V.p + (int64_t)nb * n;
       Av(c->V.p + (int64_t)(nb - 1) * n, w);
-      dots(c, nb, c->V.p, w, h1);
+      // both Gram-Schmidt passes and the norm run on the device into hbuf = [h1 | h2 | w.w]; the
+      // column comes to the host once per iteration
+      dots_dev(c, nb, c->V.p, w, 0);
       hipLaunchKernelGGL(k_vsum, dim3(gN), dim3(256), 0, s, c->V.p, n, nb, c->hbuf.p, -1.0, w, n);
-      dots(c, nb, c->V.p, w, h2);
-      hipLaunchKernelGGL(k_vsum, dim3(gN), dim3(256), 0, s, c->V.p, n, nb, c->hbuf.p, -1.0, w, n);
+      dots_dev(c, nb, c->V.p, w, nb);
+      hipLaunchKernelGGL(k_vsum, dim3(gN), dim3(256), 0, s, c->V.p, n, nb, c->hbuf.p + nb, -1.0, w, n);
+      dots_dev(c, 1, w, w, 2 * (size_t)nb);
+      HIPCHECK(hipMemcpyAsync(c->hres, c->hbuf.p, (2 * (size_t)nb + 1) * sizeof(double), hipMemcpyDeviceToHost, s));
+      HIPCHECK(hipStreamSynchronize(s));
       std::vector<double> col(nb + 1);
-      for (int i = 0; i < nb; ++i) col[i] = h1[i] + h2[i];
-      const double hn = norm2(c, w);
+      for (int i = 0; i < nb; ++i) col[i] = c->hres[i] + c->hres[nb + i];
+      const double hn = std::sqrt(c->hres[2 * nb]);
       col[nb] = hn;
       hipLaunchKernelGGL(k_axpby, dim3(gN), dim3(256), 0, s, n, hn > 0. ? 1.0 / hn : 0.0, w, 0.0, w);
       ++nb;
@@ -2778,6 +2791,7 @@ void bemfmm_ctx_destroy(bemfmm_ctx* c) {
     if (e) (void)hipEventDestroy(e);
   drop_graphs(c);
   if (c->op_stage) (void)hipHostFree(c->op_stage);
+  if (c->hres) (void)hipHostFree(c->hres);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   if (c->stream2) (void)hipStreamDestroy(c->stream2);
   delete c;
