@@ -2633,7 +2633,7 @@ int do_solve(bemfmm_ctx* c, const double* x0, double* xout, double tol, int max_
   // a zero initial guess has r0 = M^-1 b: no matvec for the first residual
   const bool x0_zero = std::all_of(x0i.begin(), x0i.end(), [](double v) { return v == 0.; });
   if (c->opts.precision == BEMFMM_PRECISION_FP32_REFINE) {
-    const double eta = (c->opts.refine_eta > 0. && c->opts.refine_eta < 1.) ? c->opts.refine_eta : 1e-5;
+    const double eta = (c->opts.refine_eta > 0. && c->opts.refine_eta < 1.) ? c->opts.refine_eta : 3e-7;
     double prev = 1e300;
     bool fallback = false;
     bool x_zero = x0_zero;
@@ -2745,7 +2745,7 @@ void bemfmm_default_opts(bemfmm_opts* o) {
   o->q = 6;
   o->precision = BEMFMM_PRECISION_FP64;
   o->preconditioner = BEMFMM_PRECOND_NONE;
-  o->refine_eta = 1e-5;
+  o->refine_eta = 3e-7;
   o->schwarz_max_core = 64;   // main_time_domain.cpp:47
   o->schwarz_max_block = 128; // main_time_domain.cpp:48
   o->schwarz_overlap = 1;     // main_time_domain.cpp:46

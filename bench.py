@@ -184,6 +184,11 @@ def run_solves(bvp, n, args, label):
         x, info = bvp.solveSystemGMRES(x0, tol=args.tol, max_iter=args.max_iter, restart=args.max_iter)
         _log(f"[{label}] warmup solve {i}: iters={info['iters']} steps={info['refinement_steps']} "
              f"residual={info['residual']:.2e} {info['time_solve']:.2f}s")
+        if info['refinement_steps']:
+            _log(f"[{label}] FP64 residual at each refinement step: "
+                 + " ".join(f"{r:.2e}" for r in bvp.refinement_history()))
+            if os.environ.get("BENCH_GMRES_HISTORY"):
+                _log(f"[{label}] GMRES estimates: " + " ".join(f"{r:.1e}" for r in bvp.residual_history()))
     torch.cuda.synchronize()
     lat, infos = [], []
     t_start = time.perf_counter()
@@ -385,7 +390,7 @@ def main():
     ap.add_argument("--matvecs", type=int, default=20)
     ap.add_argument("--precond", choices=["none", "jacobi", "schwarz"], default="schwarz")
     ap.add_argument("--precision", choices=["fp32_refine", "fp64"], default="fp32_refine")
-    ap.add_argument("--refine-eta", type=float, default=1e-5)
+    ap.add_argument("--refine-eta", type=float, default=3e-7)
     ap.add_argument("--m2l-tile-pairs", type=int, default=0, choices=[0, 16, 32, 64])
     ap.add_argument("--serve-anyway", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--case", default=None,
