@@ -43,7 +43,7 @@ FP64_PEAK_TFLOPS = 78.6
 # depth 7), from rocprofv3 FETCH_SIZE (scaled x2) and WRITE_SIZE (x1) passes over
 # `bench.py --profile-only` (summarised in profiles/fmm_matvec_cylinder209k_p10.md); reported only
 # for that workload. Keys: (stage, FP32 operator); "m2l_rot" = the rotation M2L of the FP32 operator.
-MEASURED_TRAFFIC = {("m2l", True): 5.943e9, ("m2l", False): 8.618e9, ("m2l_rot", True): 4.014e9,
+MEASURED_TRAFFIC = {("m2l", True): 5.943e9, ("m2l", False): 8.692e9, ("m2l_rot", True): 4.018e9,
                     ("near_l2p", True): 5.177e9, ("near_l2p", False): 9.896e9}
 
 
