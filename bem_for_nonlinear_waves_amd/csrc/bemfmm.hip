@@ -302,12 +302,17 @@ __global__ __launch_bounds__(256) void k_p2m(int C, int CP, const int64_t* __res
       const T* a = Al + (int64_t)cs * CP + r;
       double s0 = 0., s1 = 0.;
       int c = 0;
-      for (; c + 1 < cn; c += 2) {
-        const double v0 = (double)a[(int64_t)c * CP], v1 = (double)a[(int64_t)(c + 1) * CP];
-        s0 += v0 * xl[c];
-        s1 += v1 * xl[c + 1];
+      for (; c + 7 < cn; c += 8) {  // 8 loads in flight (streamed: non-temporal)
+        double v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) v[u] = (double)__builtin_nontemporal_load(&a[(int64_t)(c + u) * CP]);
+#pragma unroll
+        for (int u = 0; u < 8; u += 2) {
+          s0 += v[u] * xl[c + u];
+          s1 += v[u + 1] * xl[c + u + 1];
+        }
       }
-      if (c < cn) s0 += (double)a[(int64_t)c * CP] * xl[c];
+      for (; c < cn; ++c) s0 += (double)__builtin_nontemporal_load(&a[(int64_t)c * CP]) * xl[c];
       acc[k] += s0 + s1;
     }
   }
