@@ -2654,6 +2654,11 @@ int do_solve(bemfmm_ctx* c, const double* x0, double* xout, double tol, int max_
   precond_apply(c, c->ptmp.p, c->bvec.p);
   std::vector<double> x0i(n);
   host_to_internal(c, x0, x0i.data());
+  // the unknowns of Dirichlet DOFs (phi_n) are solved for in scaled form, L phi_n, as the result is
+  // returned divided by L below: the initial guess is scaled the same way
+  const double L0 = c->plan->Lscale;
+  for (int64_t i = 0; i < n; ++i)
+    if (c->kind_i[i] == 0) x0i[i] *= L0;
   HIPCHECK(hipMemcpyAsync(c->xsol.p, x0i.data(), n * sizeof(double), hipMemcpyHostToDevice, s));
   c->history.clear();
   c->refine_history.clear();

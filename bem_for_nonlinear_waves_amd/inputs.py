@@ -18,6 +18,7 @@ rely on the angle checks (e.g. a body piercing the surface at a shallow angle) m
 """
 from __future__ import annotations
 
+import dataclasses
 import json
 import os
 
@@ -260,3 +261,307 @@ def write_obj(path: str, X: np.ndarray, tri: np.ndarray, group: str = "surface")
             fh.write(f"v {x[0]:.17g} {x[1]:.17g} {x[2]:.17g}\n")
         for t in tri:
             fh.write(f"f {t[0] + 1} {t[1] + 1} {t[2] + 1}\n")
+
+
+# ---------------------------------------------------------------------------
+# checkpoint files (format version 5, bem/BEM_checkpoint.hpp:18-55; writer :127-322)
+# ---------------------------------------------------------------------------
+CHECKPOINT_MAGIC = b"BEMCKPT\0"
+CHECKPOINT_VERSION = 5
+# per face value of a multiple node: checkpoint face id (-1 = the point's null-face key), phi, phin, phit, phint
+FACE_VALUE = np.dtype([("face", "<i4"), ("phi", "<f8"), ("phin", "<f8"), ("phit", "<f8"), ("phint", "<f8")])
+# per bad-quality event of a face
+FACE_EVENT = np.dtype([("time_step", "<i4"), ("rk_step", "<i4"), ("type", "u1"), ("value", "<f8"), ("threshold", "<f8")])
+_POINT_HEAD = np.dtype([("X", "<f8", 3), ("phiphin", "<f8", 2), ("phiphin_t", "<f8", 2), ("phi_dirichlet", "<f8")])
+_LINE_HEAD = np.dtype([("ends", "<u4", 2), ("X_mid", "<f8", 3), ("phiphin", "<f8", 2), ("phiphin_t", "<f8", 2),
+                       ("corner_offset", "<f8", 3)])
+
+
+@dataclasses.dataclass
+class CheckpointFluid:
+    """One fluid surface of a checkpoint. Face values of multiple nodes are kept in CSR form:
+    the records of point p are ``face_values[fv_ptr[p]:fv_ptr[p + 1]]`` (dtype FACE_VALUE); likewise
+    the bad-quality events of face f are ``events[ev_ptr[f]:ev_ptr[f + 1]]`` (dtype FACE_EVENT).
+    ``lines`` holds the true-quadratic line data when the file has it (endpoint ids, midpoint data,
+    and their face values in the same CSR form), else None."""
+    name: str
+    X: np.ndarray              # (n, 3)
+    phiphin: np.ndarray        # (n, 2) phi, phi_n
+    phiphin_t: np.ndarray      # (n, 2) phi_t, phi_nt
+    phi_dirichlet: np.ndarray  # (n,)
+    fv_ptr: np.ndarray         # (n + 1,) int64
+    face_values: np.ndarray    # (m,) FACE_VALUE
+    tri: np.ndarray            # (F, 3) int64 checkpoint point ids
+    ev_ptr: np.ndarray         # (F + 1,) int64
+    events: np.ndarray         # (k,) FACE_EVENT
+    lines: dict | None = None
+
+
+@dataclasses.dataclass
+class Checkpoint:
+    """Contents of a checkpoint file: header, fluid surfaces, rigid and soft bodies (name, COM,
+    quaternion Q, velocity, acceleration for rigid bodies, point positions) and vortex particles
+    (x, alpha, sigma, volume)."""
+    time_step: int
+    simulation_time: float
+    dt: float
+    fluids: list
+    rigid: list
+    soft: list
+    particles: dict
+
+
+class _Reader:
+    def __init__(self, buf: bytes):
+        self.b, self.o = buf, 0
+
+    def take(self, dtype, count=1):
+        dt = np.dtype(dtype)
+        n = dt.itemsize * count
+        if self.o + n > len(self.b):
+            raise ValueError("checkpoint: unexpected end of file")
+        a = np.frombuffer(self.b, dtype=dt, count=count, offset=self.o)
+        self.o += n
+        return a
+
+    def val(self, dtype):
+        return self.take(dtype)[0]
+
+    def string(self):
+        n = int(self.val("<u4"))
+        s = self.take("u1", n).tobytes()
+        return s.decode("utf-8", errors="replace")
+
+
+def read_checkpoint(path: str) -> Checkpoint:
+    """Read a checkpoint file written by the reference (``writeCheckpointToPath``) or by
+    ``write_checkpoint``. Raises ValueError on a wrong magic, an unknown version or a short file."""
+    with open(path, "rb") as fh:
+        r = _Reader(fh.read())
+    if r.take("u1", 8).tobytes() != CHECKPOINT_MAGIC:
+        raise ValueError("checkpoint: bad magic")
+    version = int(r.val("<u4"))
+    if version != CHECKPOINT_VERSION:
+        raise ValueError(f"checkpoint: version {version}, expected {CHECKPOINT_VERSION}")
+    time_step = int(r.val("<i4"))
+    sim_t = float(r.val("<f8"))
+    dt = float(r.val("<f8"))
+
+    def point_records(count):
+        heads = np.empty(count, dtype=_POINT_HEAD)
+        ptr = np.zeros(count + 1, dtype=np.int64)
+        fvs = []
+        for i in range(count):
+            heads[i] = r.take(_POINT_HEAD)[0]
+            k = int(r.val("<u4"))
+            fvs.append(r.take(FACE_VALUE, k).copy())
+            ptr[i + 1] = ptr[i] + k
+        return heads, ptr, (np.concatenate(fvs) if fvs else np.empty(0, FACE_VALUE))
+
+    fluids = []
+    for _ in range(int(r.val("<u4"))):
+        name = r.string()
+        n = int(r.val("<u4"))
+        heads, fv_ptr, fv = point_records(n)
+        F = int(r.val("<u4"))
+        tri = np.empty((F, 3), dtype=np.int64)
+        ev_ptr = np.zeros(F + 1, dtype=np.int64)
+        evs = []
+        for f in range(F):
+            tri[f] = r.take("<u4", 3)
+            k = int(r.val("<u4"))
+            evs.append(r.take(FACE_EVENT, k).copy())
+            ev_ptr[f + 1] = ev_ptr[f] + k
+        lines = None
+        if int(r.val("u1")):
+            nl = int(r.val("<u4"))
+            lh = np.empty(nl, dtype=_LINE_HEAD)
+            lptr = np.zeros(nl + 1, dtype=np.int64)
+            lfv = []
+            for i in range(nl):
+                lh[i] = r.take(_LINE_HEAD)[0]
+                k = int(r.val("<u4"))
+                lfv.append(r.take(FACE_VALUE, k).copy())
+                lptr[i + 1] = lptr[i] + k
+            lines = {"ends": lh["ends"].astype(np.int64), "X_mid": lh["X_mid"].copy(), "phiphin": lh["phiphin"].copy(),
+                     "phiphin_t": lh["phiphin_t"].copy(), "corner_offset": lh["corner_offset"].copy(), "fv_ptr": lptr,
+                     "face_values": np.concatenate(lfv) if lfv else np.empty(0, FACE_VALUE)}
+        if F and (tri.max() >= n):
+            raise ValueError("checkpoint: face refers to a point id past the point count")
+        fluids.append(CheckpointFluid(name, heads["X"].copy(), heads["phiphin"].copy(), heads["phiphin_t"].copy(),
+                                      heads["phi_dirichlet"].copy(), fv_ptr, fv, tri, ev_ptr,
+                                      np.concatenate(evs) if evs else np.empty(0, FACE_EVENT), lines))
+    rigid = []
+    for _ in range(int(r.val("<u4"))):
+        name = r.string()
+        com, q = r.take("<f8", 3).copy(), r.take("<f8", 4).copy()
+        vel, acc = r.take("<f8", 6).copy(), r.take("<f8", 6).copy()
+        n = int(r.val("<u4"))
+        rigid.append({"name": name, "COM": com, "Q": q, "velocity": vel, "acceleration": acc,
+                      "X": r.take("<f8", 3 * n).reshape(n, 3).copy()})
+    soft = []
+    for _ in range(int(r.val("<u4"))):
+        name = r.string()
+        com, q, vel = r.take("<f8", 3).copy(), r.take("<f8", 4).copy(), r.take("<f8", 6).copy()
+        n = int(r.val("<u4"))
+        soft.append({"name": name, "COM": com, "Q": q, "velocity": vel, "X": r.take("<f8", 3 * n).reshape(n, 3).copy()})
+    npart = int(r.val("<u4"))
+    P = r.take("<f8", 8 * npart).reshape(npart, 8)
+    particles = {"x": P[:, 0:3].copy(), "alpha": P[:, 3:6].copy(), "sigma": P[:, 6].copy(), "volume": P[:, 7].copy()}
+    if r.o != len(r.b):
+        raise ValueError(f"checkpoint: {len(r.b) - r.o} trailing bytes")
+    return Checkpoint(time_step, sim_t, dt, fluids, rigid, soft, particles)
+
+
+def write_checkpoint(path: str, ck: Checkpoint):
+    """Write ``ck`` in the same format (field order of ``writeCheckpointToPath``)."""
+    out = [CHECKPOINT_MAGIC, np.array([CHECKPOINT_VERSION], "<u4").tobytes(), np.array([ck.time_step], "<i4").tobytes(),
+           np.array([ck.simulation_time, ck.dt], "<f8").tobytes()]
+
+    def u32(v):
+        out.append(np.array([v], "<u4").tobytes())
+
+    def string(s):
+        b = s.encode("utf-8")
+        u32(len(b))
+        out.append(b)
+
+    def f8(a, n):
+        a = np.asarray(a, dtype="<f8").reshape(-1)
+        if a.size != n:
+            raise ValueError("checkpoint: wrong record size")
+        out.append(a.tobytes())
+
+    u32(len(ck.fluids))
+    for fl in ck.fluids:
+        string(fl.name)
+        n = len(fl.X)
+        u32(n)
+        head = np.empty(1, dtype=_POINT_HEAD)
+        for i in range(n):
+            head["X"], head["phiphin"], head["phiphin_t"], head["phi_dirichlet"] = (fl.X[i], fl.phiphin[i], fl.phiphin_t[i],
+                                                                                   fl.phi_dirichlet[i])
+            out.append(head.tobytes())
+            rec = fl.face_values[fl.fv_ptr[i]:fl.fv_ptr[i + 1]].astype(FACE_VALUE)
+            u32(len(rec))
+            out.append(rec.tobytes())
+        u32(len(fl.tri))
+        for f in range(len(fl.tri)):
+            out.append(np.asarray(fl.tri[f], dtype="<u4").tobytes())
+            ev = fl.events[fl.ev_ptr[f]:fl.ev_ptr[f + 1]].astype(FACE_EVENT)
+            u32(len(ev))
+            out.append(ev.tobytes())
+        out.append(np.array([1 if fl.lines is not None else 0], "u1").tobytes())
+        if fl.lines is not None:
+            L = fl.lines
+            nl = len(L["ends"])
+            u32(nl)
+            lh = np.empty(1, dtype=_LINE_HEAD)
+            for i in range(nl):
+                lh["ends"], lh["X_mid"], lh["phiphin"] = L["ends"][i], L["X_mid"][i], L["phiphin"][i]
+                lh["phiphin_t"], lh["corner_offset"] = L["phiphin_t"][i], L["corner_offset"][i]
+                out.append(lh.tobytes())
+                rec = L["face_values"][L["fv_ptr"][i]:L["fv_ptr"][i + 1]].astype(FACE_VALUE)
+                u32(len(rec))
+                out.append(rec.tobytes())
+    u32(len(ck.rigid))
+    for b in ck.rigid:
+        string(b["name"])
+        f8(b["COM"], 3), f8(b["Q"], 4), f8(b["velocity"], 6), f8(b["acceleration"], 6)
+        X = np.asarray(b["X"], dtype="<f8").reshape(-1, 3)
+        u32(len(X))
+        out.append(X.tobytes())
+    u32(len(ck.soft))
+    for b in ck.soft:
+        string(b["name"])
+        f8(b["COM"], 3), f8(b["Q"], 4), f8(b["velocity"], 6)
+        X = np.asarray(b["X"], dtype="<f8").reshape(-1, 3)
+        u32(len(X))
+        out.append(X.tobytes())
+    P = ck.particles
+    npart = len(P["x"]) if P else 0
+    u32(npart)
+    if npart:
+        rec = np.concatenate([np.asarray(P["x"]).reshape(-1, 3), np.asarray(P["alpha"]).reshape(-1, 3),
+                              np.asarray(P["sigma"]).reshape(-1, 1), np.asarray(P["volume"]).reshape(-1, 1)], axis=1)
+        out.append(np.ascontiguousarray(rec, dtype="<f8").tobytes())
+    with open(path, "wb") as fh:
+        fh.write(b"".join(out))
+
+
+def checkpoint_mesh(fl: CheckpointFluid, solids=(), face_dirichlet=None, contact_factor: float = 0.5) -> Mesh:
+    """The fluid surface of a checkpoint as a Mesh (point and face ids are the checkpoint's). Boundary
+    types come from ``face_dirichlet`` when given, else from contact with ``solids`` as in
+    ``fluid_mesh`` (the file stores positions and values, not the boundary types, which the reference
+    derives again from contact at every step). The orientation is made outward (face ids unchanged)."""
+    tri = np.asarray(fl.tri, dtype=np.int32)
+    X = np.ascontiguousarray(fl.X, dtype=np.float64)
+    if face_dirichlet is None:
+        return fluid_mesh(X, tri, list(solids), contact_factor)
+    if signed_volume(X, tri) < 0:
+        tri = tri[:, [0, 2, 1]]
+    fd = np.asarray(face_dirichlet, dtype=bool)
+    if fd.shape != (len(tri),):
+        raise ValueError("face_dirichlet needs one entry per face")
+    return Mesh(X, np.ascontiguousarray(tri), fd, np.where(fd, 0, 1).astype(np.int32))
+
+
+def _face_record(fl: CheckpointFluid, p: int, f: int):
+    rec = fl.face_values[fl.fv_ptr[p]:fl.fv_ptr[p + 1]]
+    hit = np.nonzero(rec["face"] == f)[0]
+    return rec[hit[0]] if len(hit) else None
+
+
+def checkpoint_knowns(fl: CheckpointFluid, dofs) -> tuple:
+    """Per-DOF values of a DofModel built on ``checkpoint_mesh(fl, ...)``: (phi, phin, x0).
+
+    phi / phin are the known boundary values in the ABI's convention (phi on Dirichlet DOFs, phi_n on
+    Neumann DOFs, phi_D = phi_Dirichlet and phi_n on CORNER DOFs); x0 holds the stored unknowns (phi_n
+    of Dirichlet DOFs, phi of the others) as the initial guess of solveSystemGMRES. A DOF keyed by a
+    face takes that face's record of its point when there is one (phiOnFace / phinOnFace), else the
+    point's phiphin."""
+    n = dofs.n_dof
+    phi, phin = np.zeros(n), np.zeros(n)
+    for d in range(n):
+        p, f = int(dofs.dof_point[d]), int(dofs.dof_face[d])
+        rec = _face_record(fl, p, f) if (f >= 0 or fl.fv_ptr[p + 1] > fl.fv_ptr[p]) else None
+        if rec is not None:
+            phi[d], phin[d] = rec["phi"], rec["phin"]
+        else:
+            phi[d], phin[d] = fl.phiphin[p]
+    kind = np.asarray(dofs.dof_kind)
+    x0 = np.where(kind == 0, phin, phi)
+    known_phi = np.where(kind == 2, fl.phi_dirichlet[np.asarray(dofs.dof_point)], phi)
+    return known_phi, phin, x0
+
+
+def checkpoint_from_bvp(mesh: Mesh, dofs, phi: np.ndarray, phin: np.ndarray, name: str = "water", time_step: int = 0,
+                        simulation_time: float = 0.0, dt: float = 0.0, phi_dirichlet=None) -> Checkpoint:
+    """A checkpoint of one fluid surface from per-DOF values (the inverse of ``checkpoint_knowns``):
+    a point's phiphin is its null-key DOF's values (or its first DOF's), and every face-keyed DOF of a
+    multiple node becomes a face record of that point."""
+    n = mesh.n_nodes
+    pp = np.zeros((n, 2))
+    seen = np.zeros(n, bool)
+    recs = [[] for _ in range(n)]
+    for d in range(dofs.n_dof):
+        p, f = int(dofs.dof_point[d]), int(dofs.dof_face[d])
+        if f < 0 or not seen[p]:
+            pp[p] = (phi[d], phin[d])
+            seen[p] |= f < 0
+        if bool(dofs.point_multiple[p]):
+            recs[p].append((f, phi[d], phin[d], 0.0, 0.0))
+    fv_ptr = np.zeros(n + 1, dtype=np.int64)
+    fv_ptr[1:] = np.cumsum([len(r) for r in recs])
+    fv = np.array([r for rr in recs for r in rr], dtype=FACE_VALUE) if fv_ptr[-1] else np.empty(0, FACE_VALUE)
+    if phi_dirichlet is None:  # phi_D of CORNER points: the value their CORNER DOFs carry in phi
+        pd = np.zeros(n)
+        corner = np.asarray(dofs.dof_kind) == 2
+        pd[np.asarray(dofs.dof_point)[corner]] = np.asarray(phi)[corner]
+    else:
+        pd = np.asarray(phi_dirichlet, dtype=np.float64)
+    F = mesh.n_faces
+    fl = CheckpointFluid(name, mesh.X.copy(), pp, np.zeros((n, 2)), pd, fv_ptr, fv, mesh.tri.astype(np.int64),
+                         np.zeros(F + 1, dtype=np.int64), np.empty(0, FACE_EVENT))
+    return Checkpoint(time_step, simulation_time, dt, [fl], [], [],
+                      {"x": np.empty((0, 3)), "alpha": np.empty((0, 3)), "sigma": np.empty(0), "volume": np.empty(0)})
