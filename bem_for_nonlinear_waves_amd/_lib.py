@@ -59,6 +59,7 @@ SIGNATURES = [
     ("bemfmm_set_geometry", ctypes.c_int,
      [ctypes.c_void_p, ctypes.c_int, _dp, ctypes.c_int, _ip, _ip, ctypes.c_int, _ip, _ip]),
     ("bemfmm_set_bc", ctypes.c_int, [ctypes.c_void_p, _dp, _dp]),
+    ("bemfmm_set_robin", ctypes.c_int, [ctypes.c_void_p, _dp]),
     ("bemfmm_prepare", ctypes.c_int, [ctypes.c_void_p]),
     ("bemfmm_matvec", ctypes.c_int, [ctypes.c_void_p, _dp, _dp]),
     ("bemfmm_matvec_device", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),

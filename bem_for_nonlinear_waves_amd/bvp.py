@@ -94,6 +94,13 @@ class BEM_BVP:
         self._bc = [_d(phi), _d(phin)]
         self._check(self.lib.bemfmm_set_bc(self.ctx, _ptr(self._bc[0], _lib._dp), _ptr(self._bc[1], _lib._dp)))
 
+    def set_robin(self, kappa):
+        """Robin coefficient per DOF (physical units) for ROBIN DOFs (kind 3: phi_n = kappa phi + g, g passed
+        as their phin in set_bc): the frequency-domain free surface phin = kappa phi, kappa = omega^2 / g
+        (main_freq_domain.cpp:543-555). The operator depends on it: call prepare afterwards."""
+        self._robin = _d(kappa)
+        self._check(self.lib.bemfmm_set_robin(self.ctx, _ptr(self._robin, _lib._dp)))
+
     def prepare(self):
         t = time.perf_counter()
         self._check(self.lib.bemfmm_prepare(self.ctx))

@@ -76,17 +76,23 @@ __global__ void k_fill(double* __restrict__ p, int64_t n, double v) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) p[i] = v;
 }
 
-// mode 0: matvec (unknowns = x, knowns = 0); 1: rhs (unknowns = 0, knowns = BC); 2: rigid mode (phi = 1, phi_n = 0)
-__global__ void k_scatter(int mode, int n, const int32_t* __restrict__ kind, const double* __restrict__ x,
-                          const double* __restrict__ phi_k, const double* __restrict__ phin_k, double* __restrict__ phin,
-                          double* __restrict__ phi) {
+// mode 0: matvec (unknowns = x, knowns = 0); 1: rhs (unknowns = 0, knowns = BC); 2: rigid mode (phi = 1, phi_n = 0).
+// A ROBIN DOF (kind 3) has the unknown phi and phi_n = kappa phi + g: kappa phi belongs to the matvec, the
+// known g (passed as its phi_n) to the right-hand side (main_freq_domain.cpp:543-555 with g = 0).
+__global__ void k_scatter(int mode, int n, const int32_t* __restrict__ kind, const double* __restrict__ kappa,
+                          const double* __restrict__ x, const double* __restrict__ phi_k,
+                          const double* __restrict__ phin_k, double* __restrict__ phin, double* __restrict__ phi) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const bool dir = kind[i] == 0;
   double a = 0., b = 0.;  // a = phi_n, b = phi
   if (mode == 0) {
-    if (dir) a = x[i];
-    else b = x[i];
+    if (dir) {
+      a = x[i];
+    } else {
+      b = x[i];
+      if (kind[i] == 3) a = kappa[i] * x[i];
+    }
   } else if (mode == 1) {
     if (dir) b = phi_k[i];
     else a = phin_k[i];
@@ -1150,9 +1156,10 @@ __global__ __launch_bounds__(256) void k_near_quad(
 }
 
 // Diagonal of the BIE operator per DOF, read from the near-field row of its point:
-// Dirichlet (unknown phi_n): G_ii; Neumann (unknown phi): -Gn_ii - alpha_i; CORNER rows: 1.
-// dinv = 1/diag (1 where the diagonal vanishes).
-__global__ void k_diag_inv(int n, const int32_t* __restrict__ kind, const int32_t* __restrict__ point,
+// Dirichlet (unknown phi_n): G_ii; Neumann (unknown phi): -Gn_ii - alpha_i; ROBIN: kappa_i G_ii - Gn_ii -
+// alpha_i; CORNER rows: 1. dinv = 1/diag (1 where the diagonal vanishes).
+__global__ void k_diag_inv(int n, const int32_t* __restrict__ kind, const double* __restrict__ kappa,
+                           const int32_t* __restrict__ point,
                            const int64_t* __restrict__ node_row, const int32_t* __restrict__ row_block,
                            const int64_t* __restrict__ row_ptr, const int64_t* __restrict__ col_ptr,
                            const int64_t* __restrict__ mat_ptr, const int32_t* __restrict__ col_idx,
@@ -1175,15 +1182,18 @@ __global__ void k_diag_inv(int n, const int32_t* __restrict__ kind, const int32_
     }
     const double* A0 = mat + mat_ptr[b] + (r - row_ptr[b]) * 2 * ncol;
     dg = (kind[i] == 0) ? A0[lo] : -A0[ncol + lo] - alpha[i];
+    if (kind[i] == 3) dg += kappa[i] * A0[lo];
   }
   dinv[i] = (isfinite(dg) && fabs(dg) > 1e-15) ? 1.0 / dg : 1.0;
 }
 
 // ---- restricted additive Schwarz preconditioner ----
 // Operator entry (row DOF i, column DOF j) from the resident near field: the near row of i's point,
-// column j if it is in that row's near set (else 0); G for Dirichlet columns, -Gn otherwise;
-// -alpha_i on Neumann diagonals; CORNER rows are e_i. (buildSparseMatrixForILU, GMRES_FMM.hpp:440-640)
-__device__ double near_entry(int i, int j, const int32_t* __restrict__ kind, const int32_t* __restrict__ point,
+// column j if it is in that row's near set (else 0); G for Dirichlet columns, kappa_j G - Gn for ROBIN
+// columns, -Gn otherwise; -alpha_i on Neumann and ROBIN diagonals; CORNER rows are e_i.
+// (buildSparseMatrixForILU, GMRES_FMM.hpp:440-640)
+__device__ double near_entry(int i, int j, const int32_t* __restrict__ kind, const double* __restrict__ kappa,
+                             const int32_t* __restrict__ point,
                              const int64_t* __restrict__ node_row, const int32_t* __restrict__ row_block,
                              const int64_t* __restrict__ row_ptr, const int64_t* __restrict__ col_ptr,
                              const int64_t* __restrict__ mat_ptr, const int32_t* __restrict__ col_idx,
@@ -1206,8 +1216,9 @@ __device__ double near_entry(int i, int j, const int32_t* __restrict__ kind, con
   if (ncd > 0 && col_idx[c0 + lo] == j) {
     const double* A0 = mat + mat_ptr[b] + (r - row_ptr[b]) * 2 * ncol;
     v = (kind[j] == 0) ? A0[lo] : -A0[ncol + lo];
+    if (kind[j] == 3) v += kappa[j] * A0[lo];
   }
-  if (i == j && ki == 1) v -= alpha[i];
+  if (i == j && (ki == 1 || ki == 3)) v -= alpha[i];
   return v;
 }
 
@@ -1219,8 +1230,9 @@ __device__ double near_entry(int i, int j, const int32_t* __restrict__ kind, con
 constexpr int kSchwarzMaxBlock = 128;
 __global__ __launch_bounds__(256) void k_schwarz_build(
     const int64_t* __restrict__ bptr, const int32_t* __restrict__ bidx, const int32_t* __restrict__ bncore,
-    const int64_t* __restrict__ inv_ptr, const int32_t* __restrict__ kind, const int32_t* __restrict__ point,
-    const int64_t* __restrict__ node_row, const int32_t* __restrict__ row_block, const int64_t* __restrict__ row_ptr,
+    const int64_t* __restrict__ inv_ptr, const int32_t* __restrict__ kind, const double* __restrict__ kappa,
+    const int32_t* __restrict__ point, const int64_t* __restrict__ node_row, const int32_t* __restrict__ row_block,
+    const int64_t* __restrict__ row_ptr,
     const int64_t* __restrict__ col_ptr, const int64_t* __restrict__ mat_ptr, const int32_t* __restrict__ col_idx,
     const int32_t* __restrict__ ncol_dof, const double* __restrict__ mat, const double* __restrict__ alpha,
     double* __restrict__ inv, int32_t* __restrict__ failed) {
@@ -1239,8 +1251,8 @@ __global__ __launch_bounds__(256) void k_schwarz_build(
   for (int t = tid; t < m; t += blockDim.x) ids[t] = bidx[p0 + t];
   __syncthreads();
   for (int e = tid; e < m * m; e += blockDim.x)
-    A[e] = near_entry(ids[e / m], ids[e % m], kind, point, node_row, row_block, row_ptr, col_ptr, mat_ptr, col_idx,
-                      ncol_dof, mat, alpha);
+    A[e] = near_entry(ids[e / m], ids[e % m], kind, kappa, point, node_row, row_block, row_ptr, col_ptr, mat_ptr,
+                      col_idx, ncol_dof, mat, alpha);
   __syncthreads();
   for (int k = 0; k < m; ++k) {
     if (tid < 64) {
@@ -1298,8 +1310,8 @@ __global__ __launch_bounds__(256) void k_schwarz_build(
       const int i = e / m, j = e % m;
       double v = 0.;
       if (i == j) {
-        const double d = near_entry(ids[i], ids[i], kind, point, node_row, row_block, row_ptr, col_ptr, mat_ptr,
-                                    col_idx, ncol_dof, mat, alpha);
+        const double d = near_entry(ids[i], ids[i], kind, kappa, point, node_row, row_block, row_ptr, col_ptr,
+                                    mat_ptr, col_idx, ncol_dof, mat, alpha);
         v = (isfinite(d) && fabs(d) > 1e-15) ? 1.0 / d : 1.0;
       }
       out[e] = v;
@@ -1536,6 +1548,7 @@ struct bemfmm_ctx {
   std::vector<double> X;
   std::vector<int32_t> tri, face_dof, dof_kind, dof_point;
   std::vector<double> phi_bc, phin_bc;
+  std::vector<double> robin_bc;  // kappa per DOF (caller order, physical units); empty = 0
   bool have_geom = false, have_bc = false, prepared = false;
   // internal DOF numbering (Morton order unless opts.dof_reorder == 0): internal i is caller DOF perm[i]
   std::vector<int32_t> perm, inv, face_dof_i, kind_i, point_i;
@@ -1577,6 +1590,7 @@ struct bemfmm_ctx {
   double* hres = nullptr;      // pinned host copy of hbuf (the Arnoldi column of one iteration)
   DBuf<int32_t> kind, point;
   DBuf<double> alpha, phi_k, phin_k;
+  DBuf<double> robin;  // kappa L per DOF (internal order; 0 except on ROBIN DOFs)
   DBuf<double> buf0, buf1;   // [phi_n | L0], [phi | L1]
   DBuf<double> M0, M1;
   DBuf<double> y0, y1;
@@ -1705,7 +1719,7 @@ void evaluate_launch(bemfmm_ctx* c, int mode, const double* in, double* out, boo
   };
   rec(0);
   hipLaunchKernelGGL(k_scatter, dim3(grid1(n)), dim3(256), 0, c->stream, mode == 0 ? 0 : (mode == 1 ? 1 : 2), n,
-                     c->kind.p, in, c->phi_k.p, c->phin_k.p, phin, phi);
+                     c->kind.p, c->robin.p, in, c->phi_k.p, c->phin_k.p, phin, phi);
   // the near-field columns need only the scattered DOF values: they run on stream2 beside the FMM passes
   // (not while profiling, which times the stages one after the other)
   const bool split = c->opts.overlap_near && !c->prof && c->near.nblk > 0;
@@ -1814,7 +1828,7 @@ void dof_order(bemfmm_ctx* c) {
   const int n = c->n_dof;
   // indices are used by the key kernel and the renumbering below: reject bad input before either
   for (int i = 0; i < n; ++i)
-    if (c->dof_point[i] < 0 || c->dof_point[i] >= c->n_nodes || c->dof_kind[i] < 0 || c->dof_kind[i] > 2)
+    if (c->dof_point[i] < 0 || c->dof_point[i] >= c->n_nodes || c->dof_kind[i] < 0 || c->dof_kind[i] > 3)
       throw std::runtime_error("bemfmm_prepare: dof_point or dof_kind out of range");
   for (int32_t d : c->face_dof)
     if (d < 0 || d >= n) throw std::runtime_error("bemfmm_prepare: face_dof out of range");
@@ -1887,10 +1901,17 @@ void upload_knowns(bemfmm_ctx* c) {
     phik[i] = c->phi_bc[j];
     phink[i] = (c->kind_i[i] != 0) ? c->phin_bc[j] * c->plan->Lscale : c->phin_bc[j];
   }
+  // ROBIN: phi_n = kappa phi + g, so in scaled coordinates (phi_n' = L phi_n) kappa' = L kappa
+  std::vector<double> kap(n, 0.);
+  if (!c->robin_bc.empty())
+    for (int i = 0; i < n; ++i)
+      if (c->kind_i[i] == 3) kap[i] = c->robin_bc[c->perm[i]] * c->plan->Lscale;
   if (c->phi_k.n != (size_t)n) c->phi_k.alloc(n);
   if (c->phin_k.n != (size_t)n) c->phin_k.alloc(n);
+  if (c->robin.n != (size_t)n) c->robin.alloc(n);
   HIPCHECK(hipMemcpyAsync(c->phi_k.p, phik.data(), n * sizeof(double), hipMemcpyHostToDevice, c->stream));
   HIPCHECK(hipMemcpyAsync(c->phin_k.p, phink.data(), n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  HIPCHECK(hipMemcpyAsync(c->robin.p, kap.data(), n * sizeof(double), hipMemcpyHostToDevice, c->stream));
   HIPCHECK(hipStreamSynchronize(c->stream));
 }
 
@@ -2291,7 +2312,8 @@ void do_prepare(bemfmm_ctx* c) {
     DBuf<int64_t> d_node_row;
     d_node_row.upload(node_row, s);
     if (pc == BEMFMM_PRECOND_JACOBI) {
-      hipLaunchKernelGGL(k_diag_inv, dim3(grid1(c->n_dof)), dim3(256), 0, s, c->n_dof, c->kind.p, c->point.p, d_node_row.p,
+      hipLaunchKernelGGL(k_diag_inv, dim3(grid1(c->n_dof)), dim3(256), 0, s, c->n_dof, c->kind.p, c->robin.p, c->point.p,
+                         d_node_row.p,
                          row_block.p, c->near.row_ptr.p, c->near.col_ptr.p, c->near.mat_ptr.p, c->near.col_idx.p,
                          ncol_dof.p, c->near.mat.p, c->alpha.p, c->jacobi.p);
       HIPCHECK(hipGetLastError());
@@ -2321,7 +2343,7 @@ void do_prepare(bemfmm_ctx* c) {
       failed.alloc(nb);
       const size_t lds = (size_t)B.max_block * B.max_block * sizeof(double) + (size_t)B.max_block * 16;
       hipLaunchKernelGGL(k_schwarz_build, dim3(nb), dim3(256), lds, s, c->sw_ptr.p, c->sw_idx.p, c->sw_ncore.p,
-                         c->sw_invptr.p, c->kind.p, c->point.p, d_node_row.p, row_block.p, c->near.row_ptr.p,
+                         c->sw_invptr.p, c->kind.p, c->robin.p, c->point.p, d_node_row.p, row_block.p, c->near.row_ptr.p,
                          c->near.col_ptr.p, c->near.mat_ptr.p, c->near.col_idx.p, ncol_dof.p, c->near.mat.p, c->alpha.p,
                          c->sw_inv.p, failed.p);
       HIPCHECK(hipGetLastError());
@@ -2850,6 +2872,7 @@ int bemfmm_set_geometry(bemfmm_ctx* c, int n_nodes, const double* X, int n_faces
     c->have_geom = true;
     c->have_bc = false;
     c->prepared = false;
+    c->robin_bc.clear();
     return BEMFMM_OK;
   });
 }
@@ -2871,6 +2894,19 @@ int bemfmm_set_bc(bemfmm_ctx* c, const double* phi, const double* phin) {
       // boundary values only change the right-hand side: refresh the knowns in place
       upload_knowns(c);
     }
+    return BEMFMM_OK;
+  });
+}
+
+int bemfmm_set_robin(bemfmm_ctx* c, const double* kappa) {
+  if (!c) return BEMFMM_E_INVALID;
+  return guarded(c, [&]() {
+    if (!c->have_geom) throw HipError("bemfmm_set_robin: geometry not set", BEMFMM_E_STATE);
+    if (!kappa) throw std::runtime_error("bemfmm_set_robin: null input");
+    for (int i = 0; i < c->n_dof; ++i)
+      if (c->dof_kind[i] == 3 && !std::isfinite(kappa[i])) throw HipError("non-finite kappa", BEMFMM_E_NONFINITE);
+    c->robin_bc.assign(kappa, kappa + c->n_dof);
+    c->prepared = false;  // the operator and its preconditioner depend on kappa: prepare again
     return BEMFMM_OK;
   });
 }

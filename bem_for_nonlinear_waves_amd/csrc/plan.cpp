@@ -235,7 +235,8 @@ void build_plan(Plan& pl, const Options& opt, int n_nodes, const double* X, int 
     if (face_dof[f] < 0 || face_dof[f] >= n_dof) throw std::runtime_error("face DOF index out of range");
   }
   for (int i = 0; i < n_dof; ++i) {
-    if (dof_kind[i] < 0 || dof_kind[i] > 2) throw std::runtime_error("dof_kind must be 0 (Dirichlet), 1 (Neumann) or 2 (CORNER Neumann)");
+    if (dof_kind[i] < 0 || dof_kind[i] > 3)
+      throw std::runtime_error("dof_kind must be 0 (Dirichlet), 1 (Neumann), 2 (CORNER Neumann) or 3 (ROBIN)");
     if (dof_point[i] < 0 || dof_point[i] >= n_nodes) throw std::runtime_error("dof_point out of range");
   }
   pl.dof_kind.assign(dof_kind, dof_kind + n_dof);

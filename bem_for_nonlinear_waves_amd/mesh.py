@@ -25,7 +25,7 @@ import math
 
 import numpy as np
 
-DIRICHLET, NEUMANN, CORNER_NEUMANN = 0, 1, 2
+DIRICHLET, NEUMANN, CORNER_NEUMANN, ROBIN = 0, 1, 2, 3
 
 
 @dataclasses.dataclass
@@ -395,3 +395,23 @@ def analytic_bc(mesh: Mesh, dofs: DofModel, kind: str = "linear"):
     else:
         raise ValueError(kind)
     return phi_ex.copy(), phin_ex.copy(), phi_ex, phin_ex
+
+
+def robin_dof_kind(mesh: Mesh, dofs: DofModel, face_robin: np.ndarray) -> np.ndarray:
+    """DOF kinds with the Robin DOFs of a frequency-domain problem marked ROBIN, by the rule of
+    main_freq_domain.cpp:413-426: a DOF keyed by a face is Robin if that face is a Robin (free-surface)
+    face; a point's null-key DOF is Robin if the point lies on a Robin face. Robin faces must be
+    non-Dirichlet in ``mesh`` (the reference makes every point Neumann there, :403-408), so the
+    DOFs they carry are Neumann DOFs of ``build_dofs``; CORNER DOFs stay CORNER."""
+    fr = np.asarray(face_robin, dtype=bool)
+    if fr.shape != (mesh.n_faces,):
+        raise ValueError("face_robin needs one entry per face")
+    if (fr & mesh.face_dirichlet).any():
+        raise ValueError("a Robin face must not be Dirichlet")
+    on_robin = np.zeros(mesh.n_nodes, bool)
+    on_robin[mesh.tri[fr].reshape(-1)] = True
+    kind = dofs.dof_kind.copy()
+    f = dofs.dof_face
+    robin = np.where(f >= 0, fr[np.maximum(f, 0)], on_robin[dofs.dof_point])
+    kind[(kind == NEUMANN) & robin] = ROBIN
+    return kind

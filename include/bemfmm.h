@@ -23,6 +23,9 @@
  *     known phi_n), 2 = CORNER Neumann-face DOF (unknown phi; its row is the
  *     constraint phi = phi_D, with phi_D passed in phi[]).
  *     (isDirichletID_BEM / isNeumannID_BEM, bem/BEM_setBoundaryTypes.hpp:29-40)
+ *     3 = ROBIN (unknown phi, phi_n = kappa phi + g with kappa from bemfmm_set_robin and g
+ *     passed as its phi_n; the frequency-domain free surface of bem/main_freq_domain.cpp:411-426,
+ *     543-555, where g = 0 and kappa = omega^2 / g_gravity).
  */
 #ifndef BEMFMM_H
 #define BEMFMM_H
@@ -125,6 +128,13 @@ int bemfmm_set_geometry(bemfmm_ctx* ctx, int n_nodes, const double* X, int n_fac
 /* Known boundary values per DOF (phiOnFace / phinOnFace): phi for Dirichlet DOFs (and phi_D for
  * CORNER Neumann DOFs), phi_n for Neumann and CORNER Neumann DOFs; other entries are ignored. */
 int bemfmm_set_bc(bemfmm_ctx* ctx, const double* phi, const double* phin);
+
+/* Robin coefficient kappa per DOF (n_dof, physical units, used on ROBIN DOFs only): replaces the
+ * postprocess hook that sets phin = kappa * phi on the is_robin DOFs of every matvec
+ * (main_freq_domain.cpp:543-555). kappa is part of the operator and of its preconditioner, so a
+ * call on a prepared context requires bemfmm_prepare again (the operator table stays resident).
+ * Without a call kappa is 0. */
+int bemfmm_set_robin(bemfmm_ctx* ctx, const double* kappa);
 
 /* Setup (solveGMRES up to GMRES, BEM_solveBVP_GMRES_FMM.hpp:3076-3450 + initializeFMM,
  * moments.hpp:1239-1541): scaling, sources, tree, lists, operators, near-field quadrature,

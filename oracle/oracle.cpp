@@ -930,6 +930,21 @@ struct Oracle {
       else phin[i] = phin_known[i];
     }
   }
+  // ROBIN DOFs (kind 3): phi_n = kappa phi + g. The postprocess of the frequency-domain solver sets
+  // phin = kappa * phi on the is_robin DOFs after the unknowns are placed (main_freq_domain.cpp:543-555,
+  // g = 0 there); the known part g enters with the other knowns. kappa_s = kappa L (scaled phi_n).
+  std::vector<double> kappa_s;
+  void add_robin() {
+    if (kappa_s.empty()) return;
+    for (int i = 0; i < n_dof; ++i)
+      if (dof_kind[i] == 3) phin[i] += kappa_s[i] * phi[i];
+  }
+  void set_robin(const double* kappa) {
+    kappa_s.assign(n_dof, 0.);
+    for (int i = 0; i < n_dof; ++i)
+      if (dof_kind[i] == 3) kappa_s[i] = kappa[i] * Lscale;
+  }
+  double kap(int j) const { return kappa_s.empty() ? 0. : kappa_s[j]; }
   void compute_alpha() {
     // rigid mode (GMRES_FMM.hpp:2674-2716): phi=1, phin=0 on all DOFs; alpha = -GnPhi
     alpha.assign(n_dof, 1.0);
@@ -959,6 +974,7 @@ struct Oracle {
   void matvec(const double* x, double* y) {
     set_unknowns(x);
     set_knowns_zero();
+    add_robin();
     std::vector<double> Av;
     ax_minus_b(Av);
     std::copy(Av.begin(), Av.end(), y);
@@ -1007,10 +1023,13 @@ struct Oracle {
           continue;
         }
         for (int j = 0; j < n_dof; ++j) {
-          // unknown of DOF j: phin (Dirichlet) enters through G, phi (Neumann) through -Gn
+          // unknown of DOF j: phin (Dirichlet) enters through G, phi (Neumann) through -Gn, phi (ROBIN)
+          // through kappa_j G - Gn
           row[j] = (dof_kind[j] == 0) ? rowG[j] : -rowGn[j];
+          if (dof_kind[j] == 3) row[j] += kap(j) * rowG[j];
         }
-        if (dof_kind[i] == 1) row[i] -= alpha[i];  // alpha*phi_i is an unknown term only for Neumann rows
+        // alpha*phi_i is an unknown term only for rows whose phi is unknown (Neumann, ROBIN)
+        if (dof_kind[i] == 1 || dof_kind[i] == 3) row[i] -= alpha[i];
       }
     }
   }
@@ -1026,7 +1045,10 @@ struct Oracle {
       const auto& ix = near_idx[t];
       const size_t k = std::lower_bound(ix.begin(), ix.end(), i) - ix.begin();
       double g = 1.0;
-      if (k < ix.size() && ix[k] == i) g = dof_kind[i] == 0 ? near_wG[t][k] : -near_wGn[t][k] - alpha[i];
+      if (k < ix.size() && ix[k] == i) {
+        g = dof_kind[i] == 0 ? near_wG[t][k] : -near_wGn[t][k] - alpha[i];
+        if (dof_kind[i] == 3) g += kap(i) * near_wG[t][k];
+      }
       d[i] = (std::isfinite(g) && std::abs(g) > 1e-15) ? 1.0 / g : 1.0;
     }
     return d;
@@ -1044,8 +1066,11 @@ struct Oracle {
     const auto& ix = near_idx[t];
     const size_t k = std::lower_bound(ix.begin(), ix.end(), j) - ix.begin();
     double v = 0.;
-    if (k < ix.size() && ix[k] == j) v = dof_kind[j] == 0 ? near_wG[t][k] : -near_wGn[t][k];
-    if (i == j && dof_kind[i] == 1) v -= alpha[i];
+    if (k < ix.size() && ix[k] == j) {
+      v = dof_kind[j] == 0 ? near_wG[t][k] : -near_wGn[t][k];
+      if (dof_kind[j] == 3) v += kap(j) * near_wG[t][k];
+    }
+    if (i == j && (dof_kind[i] == 1 || dof_kind[i] == 3)) v -= alpha[i];
     return v;
   }
   void schwarz_apply(const double* r, double* z) const {
@@ -1113,6 +1138,8 @@ struct Oracle {
       return std::sqrt(s);
     };
     std::vector<double> x(x0, x0 + n_dof);
+    for (int i = 0; i < n_dof; ++i)
+      if (dof_kind[i] == 0) x[i] *= Lscale;  // phi_n unknowns are solved scaled, as xout is unscaled below
     int iters = 0;
     double cur_err = 1e300;
     const int restart_max = std::max(1, restart);
@@ -1255,6 +1282,7 @@ void* oracle_create(int n_nodes, const double* X, int n_faces, const int* tri, c
   }
 }
 void oracle_destroy(void* h) { delete static_cast<Oracle*>(h); }
+void oracle_set_robin(void* h, const double* kappa) { static_cast<Oracle*>(h)->set_robin(kappa); }
 void oracle_matvec(void* h, const double* x, double* y) { static_cast<Oracle*>(h)->matvec(x, y); }
 void oracle_rhs(void* h, double* b) { static_cast<Oracle*>(h)->rhs(b); }
 void oracle_alpha(void* h, double* a) {

@@ -43,6 +43,7 @@ def lib():
         L.oracle_create.argtypes = [ctypes.c_int, _dp, ctypes.c_int, _ip, _ip, ctypes.c_int, _ip, _ip, _dp, _dp,
                                     ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_int]
         L.oracle_destroy.argtypes = [ctypes.c_void_p]
+        L.oracle_set_robin.argtypes = [ctypes.c_void_p, _dp]
         L.oracle_matvec.argtypes = [ctypes.c_void_p, _dp, _dp]
         L.oracle_rhs.argtypes = [ctypes.c_void_p, _dp]
         L.oracle_alpha.argtypes = [ctypes.c_void_p, _dp]
@@ -132,6 +133,11 @@ class Oracle:
         s = OracleStats()
         lib().oracle_get_stats(self.h, ctypes.byref(s))
         return {k: getattr(s, k) for k, _ in OracleStats._fields_}
+
+    def set_robin(self, kappa):
+        """Robin coefficient per DOF (physical units) for ROBIN DOFs (kind 3): phi_n = kappa phi + g."""
+        self._kappa = _d(kappa)
+        lib().oracle_set_robin(self.h, _p(self._kappa, _dp))
 
     def set_m2l_cache(self, on: bool):
         """Per-(level, lattice offset) M2L coefficients computed once (CPU-baseline mode for large N)."""
