@@ -100,6 +100,7 @@ class BEM_BVP:
         (main_freq_domain.cpp:543-555). The operator depends on it: call prepare afterwards."""
         self._robin = _d(kappa)
         self._check(self.lib.bemfmm_set_robin(self.ctx, _ptr(self._robin, _lib._dp)))
+        self.prepared = False
 
     def prepare(self):
         t = time.perf_counter()
@@ -217,6 +218,21 @@ class BEM_BVP:
         keeps these knowns until the next set_bc."""
         self.set_bc(phit, phint)
         return self.solveSystemGMRES(x0, tol, max_iter, restart, raise_on_nonconvergence)
+
+    def solveGMRES_complex(self, phi, phin, tol: float = 1e-10, max_iter: int = 1000, restart: int = 1000):
+        """Complex boundary data on a real operator (real kappa on ROBIN DOFs): the frequency-domain
+        diffraction BVP of main_freq_domain.cpp:650-687 (incident-wave phi_n on the body, Robin free
+        surface), which the reference assembles as a dense complex system (solve_linear_bvp,
+        BEM_freqency_domain.hpp:277), solved as two real BVPs on one prepared plan: x = x(Re) + i x(Im).
+        Prepares the context if needed; returns (x, [info_re, info_im])."""
+        phi, phin = np.asarray(phi, dtype=complex), np.asarray(phin, dtype=complex)
+        self.set_bc(phi.real, phin.real)
+        if not getattr(self, "prepared", False):
+            self.prepare()
+        xr, ir = self.solveSystemGMRES(None, tol, max_iter, restart)
+        self.set_bc(phi.imag, phin.imag)
+        xi, ii = self.solveSystemGMRES(None, tol, max_iter, restart)
+        return xr + 1j * xi, [ir, ii]
 
     def solveGMRES(self, phi, phin, x0=None, tol: float = 1e-10, max_iter: int = 1000, restart: int = 1000):
         """Setup (scaling, tree, operators, near field, alpha) + GMRES, as BEM_BVP::solveGMRES."""

@@ -111,3 +111,37 @@ def test_robin_gpu_parity(oracle_mod, precond, precision):
     assert np.linalg.norm(xs - xo) <= 1e-8 * np.linalg.norm(xo)
     assert np.linalg.norm(xs - phi_ex) / np.linalg.norm(phi_ex) < 1e-3
     b.close()
+
+
+@pytest.mark.gpu
+def test_complex_data_on_robin_problem(oracle_mod):
+    """Complex boundary data (the diffraction problem's form): phi = e^{kz} e^{ikx} is harmonic with
+    phi_z = k phi on the flat top, so a homogeneous Robin top (kappa = k) with the exact complex phi_n on
+    the walls reproduces it; the complex solve equals the oracle's real and imaginary solves."""
+    from bem_for_nonlinear_waves_amd.bvp import BEM_BVP
+    m, d, top, kappa, *_ = _robin_problem(24, 12, 12, "exp")
+    k = np.pi / 2.0
+    X = m.X[d.dof_point]
+    nrm = M.dof_normals(m, d)
+    ph = np.exp(k * X[:, 2]) * np.exp(1j * k * X[:, 0])
+    grad = np.stack([1j * k * ph, np.zeros(len(X)), k * ph], axis=1)
+    phn = np.einsum("ij,ij->i", grad, nrm)
+    robin = d.dof_kind == M.ROBIN
+    g = np.where(robin, 0.0, phn)
+    print(f"[gpu] complex robin n_dof={d.n_dof}", flush=True)
+    b = BEM_BVP(m, d, p=10, leaf_max=16, preconditioner=2)
+    b.set_bc(ph.real, g.real)
+    b.set_robin(kappa)
+    x, infos = b.solveGMRES_complex(ph, g, tol=1e-10, max_iter=800)
+    assert all(i["converged"] == 1 for i in infos)
+    err = np.linalg.norm(x - ph) / np.linalg.norm(ph)
+    print(f"[gpu] complex standing wave error {err:.3e}", flush=True)
+    assert err < 3e-2
+    parts = []
+    for part in (np.real, np.imag):
+        o = oracle_mod.Oracle(m, d, part(ph), part(g), p=10, leaf_max=16)
+        o.set_robin(kappa)
+        parts.append(o.solve(tol=1e-10, max_iter=800, precond=1)[0])
+    xo = parts[0] + 1j * parts[1]
+    assert np.linalg.norm(x - xo) <= 1e-8 * np.linalg.norm(xo)
+    b.close()
