@@ -549,7 +549,8 @@ __global__ __launch_bounds__(256) void k_m2l_reduce(int C, const int32_t* __rest
 constexpr int kRotWavePairs = 32;
 constexpr int kRotTP = 4 * kRotWavePairs;  // pairs per workgroup tile
 // Device layout of the factors of one offset, in float4s, every row zero-padded to whole float4s
-// (aligned broadcast reads): (cos m phi, sin m phi) for m = 1..P; per degree n the y-rotation blocks
+// (the z-rotation pairs are read as float4s; a factor row is read one float per lane, lane l taking
+// entry l & 15): (cos m phi, sin m phi) for m = 1..P; per degree n the y-rotation blocks
 // Y_D (cos-type (n+1) rows, sin-type n rows); per order m the axial block (P+1-m rows); per degree the
 // blocks of Y_H^T. Row lengths equal the block sizes.
 __host__ __device__ constexpr int rot_q(int len) { return (len + 3) / 4; }
