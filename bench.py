@@ -486,9 +486,17 @@ def main():
     if STAGES[dom] == "m2l":
         ach = m2l_flops / (stage[dom] * 1e-3) / 1e12
         if rot:
-            # FP32 FMAs on the vector ALUs; MI355X's published FP32 peak is 157.3 TFLOP/s for vector and matrix
-            roof = {"kernel": "k_m2l_rot (M2L by rotation, FP32 FMA)", "bound": "mfma", "engine": "valu",
-                    "achieved": ach, "peak": mfma_peak, "unit": "TFLOP/s", "frac": ach / mfma_peak, "traffic": None}
+            # 2.84e10 FP32 flop against >= 4.5 GB per launch is ~6 flop/B, below the ridge point (157.3 TFLOP/s /
+            # 8 TB/s = 19.7 flop/B): the kernel sits on the bandwidth side of the roofline. Algorithmic bytes per
+            # pair: the source's two gathered FP32 multipole columns (2 C x 4 B) and the two staged result
+            # columns (2 CS x 4 B, CS = C rounded up to 32); the factor loads and the multipole pack in the
+            # stage time are not counted (a conservative rate). Its FP32 FMA rate is reported beside it.
+            CS = (C + 31) // 32 * 32
+            rot_bytes = st["n_m2l_pairs"] * (2 * C + 2 * CS) * 4.0
+            achb = rot_bytes / (stage[dom] * 1e-3) / 1e9
+            roof = {"kernel": "k_m2l_rot (M2L by rotation, FP32 FMA)", "bound": "hbm", "achieved": achb,
+                    "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achb / HBM_PEAK_GBS, "traffic": None,
+                    "algorithmic_bytes": rot_bytes, "valu_tflops": ach, "valu_frac": ach / mfma_peak}
         else:
             roof = {"kernel": "k_translate_mfma (M2L)", "bound": "mfma", "achieved": ach, "peak": mfma_peak,
                     "unit": "TFLOP/s", "frac": ach / mfma_peak, "traffic": None}
