@@ -1229,6 +1229,7 @@ __device__ double near_entry(int i, int j, const int32_t* __restrict__ kind, con
 // (as SchwarzAdditive_CRS does when its block LU fails) and is flagged in failed[b].
 // LDS: m*m doubles (matrix) + m doubles (pivot column) + m ints (DOF ids) + m ints (pivot rows).
 constexpr int kSchwarzMaxBlock = 128;
+static_assert(kSchwarzMaxBlock <= 128, "k_schwarz_build updates one column per thread, two rows per pass of 256 threads");
 __global__ __launch_bounds__(256) void k_schwarz_build(
     const int64_t* __restrict__ bptr, const int32_t* __restrict__ bidx, const int32_t* __restrict__ bncore,
     const int64_t* __restrict__ inv_ptr, const int32_t* __restrict__ kind, const double* __restrict__ kappa,
@@ -1296,10 +1297,16 @@ __global__ __launch_bounds__(256) void k_schwarz_build(
     __syncthreads();
     for (int j = tid; j < m; j += blockDim.x) A[k * m + j] = (j == k) ? ipv : A[k * m + j] * ipv;
     __syncthreads();
-    for (int e = tid; e < m * m; e += blockDim.x) {
-      const int i = e / m, j = e % m;
-      if (i == k) continue;
-      A[e] = (j == k) ? -colk[i] * ipv : A[e] - colk[i] * A[k * m + j];
+    {  // one column per thread, two rows per pass (m <= 128 = blockDim.x / 2): consecutive threads take
+       // consecutive columns of a row (no bank conflicts) and no thread divides by m
+      const int j = tid & 127;
+      if (j < m) {
+        const double akj = A[k * m + j];  // the scaled pivot row (row k is not updated here)
+        for (int i = tid >> 7; i < m; i += 2) {
+          if (i == k) continue;
+          A[i * m + j] = (j == k) ? -colk[i] * ipv : A[i * m + j] - colk[i] * akj;
+        }
+      }
     }
     __syncthreads();
   }
