@@ -145,3 +145,38 @@ def test_complex_data_on_robin_problem(oracle_mod):
     xo = parts[0] + 1j * parts[1]
     assert np.linalg.norm(x - xo) <= 1e-8 * np.linalg.norm(xo)
     b.close()
+
+
+@pytest.mark.gpu
+def test_robin_error_paths():
+    """A non-finite kappa on a ROBIN DOF is rejected; a new kappa on a prepared context requires prepare
+    again (the operator and its preconditioner depend on it); DOF kinds past 3 are rejected."""
+    from bem_for_nonlinear_waves_amd import _lib
+    from bem_for_nonlinear_waves_amd.bvp import BEM_BVP, BEMError
+    m, d, top, kappa, phi, g, *_ = _robin_problem(8, 4, 4, "linear")
+    print(f"[gpu] robin error paths n_dof={d.n_dof}", flush=True)
+    b = BEM_BVP(m, d, p=8, leaf_max=16)
+    bad = kappa.copy()
+    bad[np.argmax(d.dof_kind == M.ROBIN)] = np.nan
+    with pytest.raises(BEMError) as e:
+        b.set_robin(bad)
+    assert e.value.code == _lib.E_NONFINITE
+    b.set_bc(phi, g)
+    b.set_robin(kappa)
+    b.prepare()
+    y1 = b.compute_Ax(np.ones(d.n_dof))
+    b.set_robin(2.0 * kappa)
+    with pytest.raises(BEMError) as e:
+        b.compute_Ax(np.ones(d.n_dof))
+    assert e.value.code == _lib.E_STATE
+    b.prepare()
+    y2 = b.compute_Ax(np.ones(d.n_dof))
+    assert np.linalg.norm(y2 - y1) > 1e-6 * np.linalg.norm(y1)  # the new kappa is in the operator
+    b.close()
+    d4 = dataclasses.replace(d, dof_kind=np.where(d.dof_kind == M.ROBIN, 4, d.dof_kind).astype(np.int32))
+    b4 = BEM_BVP(m, d4, p=8, leaf_max=16)
+    with pytest.raises(BEMError) as e:
+        b4.set_bc(phi, g)
+        b4.prepare()
+    assert e.value.code == _lib.E_INVALID
+    b4.close()
