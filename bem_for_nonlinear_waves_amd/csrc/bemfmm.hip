@@ -1014,60 +1014,66 @@ __global__ __launch_bounds__(256) void k_m2l_rot(int ntiles, const float* __rest
   }
 }
 
-// FP32 staging: the same reduction with 8-byte loads. Thread t < CS of each half-block reads float2 t of
-// a pair's two staged columns (rows 2t, 2t+1 of one layer, CS = stage_stride<float>(C)); the two
-// half-blocks take alternate pairs, 8 pairs in flight each, and are combined in a fixed order
-// (deterministic). Boxes run in descending pair count (order), as in k_m2l_reduce.
+// FP32 staging: the same reduction with 16-byte loads. Thread t < CS/2 of wave w reads float4 t of a
+// pair's two staged columns (rows 4t..4t+3 of one layer, CS = stage_stride<float>(C)); wave w takes the
+// pairs a + w, a + w + 4, ... (4 in flight per thread) and the four waves' sums are combined in a fixed
+// order (deterministic). Boxes run in descending pair count (order), as in k_m2l_reduce.
 __global__ __launch_bounds__(256) void k_m2l_reduce_f32(int C, const int32_t* __restrict__ order,
                                                         const int64_t* __restrict__ tptr,
                                                         const int32_t* __restrict__ tpairs,
                                                         const float* __restrict__ stage, double* __restrict__ L0,
                                                         double* __restrict__ L1) {
-  __shared__ double2 part[128];
-  const int CS = stage_stride<float>(C);  // <= 128, even
+  __shared__ double part[3][4][64];
+  const int CS = stage_stride<float>(C);  // <= 128, a multiple of 32
   const int tb = order[blockIdx.x];
-  const int h = threadIdx.x >> 7, t = threadIdx.x & 127;
-  const bool act = t < CS;  // thread t: float2 t of a pair's two staged columns = rows 2t, 2t+1 of one layer
+  const int w = threadIdx.x >> 6, t = threadIdx.x & 63;
+  const bool act = 4 * t < 2 * CS;
   const int64_t a = tptr[tb], b = tptr[tb + 1];
-  double sx[4] = {0., 0., 0., 0.}, sy[4] = {0., 0., 0., 0.};
+  double s0 = 0., s1 = 0., s2 = 0., s3 = 0.;
   if (act) {
-    int64_t q = a + h;
-    for (; q + 14 < b; q += 16) {  // 8 loads in flight
-      float2 v[8];
+    int64_t q = a + w;
+    for (; q + 12 < b; q += 16) {  // 4 loads in flight
+      float4 v[4];
 #pragma unroll
-      for (int u = 0; u < 8; ++u) v[u] = ld_nt(reinterpret_cast<const float2*>(stage + (int64_t)tpairs[q + 2 * u] * 2 * CS) + t);
-#pragma unroll
-      for (int u = 0; u < 8; ++u) {
-        sx[u & 3] += v[u].x;
-        sy[u & 3] += v[u].y;
-      }
-    }
-    for (; q + 6 < b; q += 8) {
-      float2 v[4];
-#pragma unroll
-      for (int u = 0; u < 4; ++u) v[u] = ld_nt(reinterpret_cast<const float2*>(stage + (int64_t)tpairs[q + 2 * u] * 2 * CS) + t);
+      for (int u = 0; u < 4; ++u) v[u] = ld_nt(reinterpret_cast<const float4*>(stage + (int64_t)tpairs[q + 4 * u] * 2 * CS) + t);
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
-        sx[u] += v[u].x;
-        sy[u] += v[u].y;
+        s0 += v[u].x;
+        s1 += v[u].y;
+        s2 += v[u].z;
+        s3 += v[u].w;
       }
     }
-    for (; q < b; q += 2) {
-      const float2 v = ld_nt(reinterpret_cast<const float2*>(stage + (int64_t)tpairs[q] * 2 * CS) + t);
-      sx[0] += v.x;
-      sy[0] += v.y;
+    for (; q < b; q += 4) {
+      const float4 v = ld_nt(reinterpret_cast<const float4*>(stage + (int64_t)tpairs[q] * 2 * CS) + t);
+      s0 += v.x;
+      s1 += v.y;
+      s2 += v.z;
+      s3 += v.w;
     }
   }
-  double x = (sx[0] + sx[1]) + (sx[2] + sx[3]), y = (sy[0] + sy[1]) + (sy[2] + sy[3]);
-  if (h == 1 && act) part[t] = double2{x, y};
+  if (w > 0 && act) {
+    part[w - 1][0][t] = s0;
+    part[w - 1][1][t] = s1;
+    part[w - 1][2][t] = s2;
+    part[w - 1][3][t] = s3;
+  }
   __syncthreads();
-  if (h == 0 && act) {
-    x += part[t].x;
-    y += part[t].y;
-    const int layer = 2 * t >= CS ? 1 : 0, r = 2 * t - layer * CS;
+  if (w == 0 && act) {
+#pragma unroll
+    for (int v = 0; v < 3; ++v) {
+      s0 += part[v][0][t];
+      s1 += part[v][1][t];
+      s2 += part[v][2][t];
+      s3 += part[v][3][t];
+    }
+    const int f = 4 * t;  // a float4 never straddles the two layers (CS is a multiple of 4)
+    const int layer = f >= CS ? 1 : 0, r = f - layer * CS;
     double* L = (layer ? L1 : L0) + (int64_t)tb * C;
-    if (r < C) L[r] = x;
-    if (r + 1 < C) L[r + 1] = y;
+    if (r < C) L[r] = s0;
+    if (r + 1 < C) L[r + 1] = s1;
+    if (r + 2 < C) L[r + 2] = s2;
+    if (r + 3 < C) L[r + 3] = s3;
   }
 }
 
