@@ -1456,6 +1456,17 @@ __global__ void k_sub(int64_t n, const double* __restrict__ a, const double* __r
 __global__ void k_mul(int64_t n, const double* __restrict__ d, double* __restrict__ y) {
   for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < n; j += (int64_t)gridDim.x * blockDim.x) y[j] *= d[j];
 }
+__global__ void k_mul_to(int64_t n, const double* __restrict__ d, const double* __restrict__ x, double* __restrict__ y) {
+  for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < n; j += (int64_t)gridDim.x * blockDim.x) y[j] = d[j] * x[j];
+}
+// w *= a, the result also written to dup
+__global__ void k_scale_dup(int64_t n, double a, double* __restrict__ w, double* __restrict__ dup) {
+  for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < n; j += (int64_t)gridDim.x * blockDim.x) {
+    const double v = a * w[j];
+    w[j] = v;
+    dup[j] = v;
+  }
+}
 
 // ---------------------------------------------------------------------------
 // device buffers
@@ -1610,6 +1621,7 @@ struct bemfmm_ctx {
   DBuf<double> y0, y1;
   DBuf<double> xin, yout, bvec, tmpv, xsol, xbase, jacobi, rvec, dvec, ptmp;
   DBuf<double> gin, gout;            // fixed input/output of the captured matvec graphs
+  const double* gin_holds = nullptr;  // a vector gin already equals (valid until the next evaluate)
   hipGraphExec_t gexec[2] = {};      // [FP64, FP32] matvec graphs
   // restricted additive Schwarz: blocks and the core rows of the block inverses
   bemfmm::SchwarzBlocks sw_host;
@@ -1806,10 +1818,14 @@ void drop_graphs(bemfmm_ctx* c) {
 
 // mode 0 (A x) is replayed from a HIP graph captured once per context and precision on the context's own
 // input/output buffers (about 20 kernels per matvec become one launch; no memset nodes: the zero fills
-// are kernels). Event timing (bemfmm_matvec_profile) uses the eager launches.
+// are kernels). Event timing (bemfmm_matvec_profile) uses the eager launches. The copy into the graph's
+// input is skipped when gin already equals `in` (scale_basis wrote both); out == gout skips the copy out.
+bool graph_mode(const bemfmm_ctx* c) { return c->opts.use_graphs && !c->prof; }
 void evaluate(bemfmm_ctx* c, int mode, const double* in, double* out, bool lo = false) {
   c->n_matvec++;
-  if (mode != 0 || !c->opts.use_graphs || c->prof) {
+  const double* held = c->gin_holds;
+  c->gin_holds = nullptr;
+  if (mode != 0 || !graph_mode(c)) {
     evaluate_launch(c, mode, in, out, lo);
     return;
   }
@@ -1830,7 +1846,8 @@ void evaluate(bemfmm_ctx* c, int mode, const double* in, double* out, bool lo = 
     (void)hipGraphDestroy(g);
     HIPCHECK(e);
   }
-  if (in != c->gin.p) HIPCHECK(hipMemcpyAsync(c->gin.p, in, bytes, hipMemcpyDeviceToDevice, c->stream));
+  if (in != c->gin.p && in != held)
+    HIPCHECK(hipMemcpyAsync(c->gin.p, in, bytes, hipMemcpyDeviceToDevice, c->stream));
   HIPCHECK(hipGraphLaunch(c->gexec[gi], c->stream));
   if (out != c->gout.p) HIPCHECK(hipMemcpyAsync(out, c->gout.p, bytes, hipMemcpyDeviceToDevice, c->stream));
 }
@@ -2490,8 +2507,8 @@ void precond_apply(bemfmm_ctx* c, const double* in, double* out) {
   const int gN = grid1(n);
   switch (c->opts.preconditioner) {
   case BEMFMM_PRECOND_JACOBI:
-    if (in != out) HIPCHECK(hipMemcpyAsync(out, in, n * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
-    hipLaunchKernelGGL(k_mul, dim3(gN), dim3(256), 0, c->stream, n, c->jacobi.p, out);
+    if (in != out) hipLaunchKernelGGL(k_mul_to, dim3(gN), dim3(256), 0, c->stream, n, c->jacobi.p, in, out);
+    else hipLaunchKernelGGL(k_mul, dim3(gN), dim3(256), 0, c->stream, n, c->jacobi.p, out);
     break;
   case BEMFMM_PRECOND_SCHWARZ:
     if (in == out) throw std::runtime_error("internal: Schwarz apply needs distinct buffers");
@@ -2503,14 +2520,23 @@ void precond_apply(bemfmm_ctx* c, const double* in, double* out) {
   }
 }
 
-// w = M^-1 A v (return_A_dot_v); lo: FP32 operator
+// w = M^-1 A v (return_A_dot_v); lo: FP32 operator. With the graph replay the preconditioner reads the
+// graph's output buffer (no copy out; gout is next written by the next replay on the same stream).
 void apply_op(bemfmm_ctx* c, const double* v, double* w, bool lo) {
-  if (c->opts.preconditioner == BEMFMM_PRECOND_SCHWARZ) {
-    evaluate(c, 0, v, c->ptmp.p, lo);
-    precond_apply(c, c->ptmp.p, w);
+  double* y = graph_mode(c) ? c->gout.p : c->opts.preconditioner == BEMFMM_PRECOND_SCHWARZ ? c->ptmp.p : w;
+  evaluate(c, 0, v, y, lo);
+  precond_apply(c, y, w);
+}
+
+// w *= a for a new basis vector; with the graph replay it is also written to the graph's input buffer,
+// so the next evaluate(w) starts without a copy
+void scale_basis(bemfmm_ctx* c, double a, double* w) {
+  const int64_t n = c->n_dof;
+  if (graph_mode(c)) {
+    hipLaunchKernelGGL(k_scale_dup, dim3(grid1(n)), dim3(256), 0, c->stream, n, a, w, c->gin.p);
+    c->gin_holds = w;
   } else {
-    evaluate(c, 0, v, w, lo);
-    precond_apply(c, w, w);
+    hipLaunchKernelGGL(k_axpby, dim3(grid1(n)), dim3(256), 0, c->stream, n, a, w, 0.0, w);
   }
 }
 
@@ -2560,7 +2586,7 @@ double gmres_run(bemfmm_ctx* c, const GmresRun& R, int& iters) {
       cur_err = 0.;
       break;
     }
-    hipLaunchKernelGGL(k_axpby, dim3(gN), dim3(256), 0, s, n, 1.0 / beta, c->V.p, 0.0, c->V.p);
+    scale_basis(c, 1.0 / beta, c->V.p);
     int nb = 1;  // basis vectors in V
     auto add_basis = [&]() {
       ensure_basis(c, nb + 1);
@@ -2579,7 +2605,7 @@ double gmres_run(bemfmm_ctx* c, const GmresRun& R, int& iters) {
       for (int i = 0; i < nb; ++i) col[i] = c->hres[i] + c->hres[nb + i];
       const double hn = std::sqrt(c->hres[2 * nb]);
       col[nb] = hn;
-      hipLaunchKernelGGL(k_axpby, dim3(gN), dim3(256), 0, s, n, hn > 0. ? 1.0 / hn : 0.0, w, 0.0, w);
+      scale_basis(c, hn > 0. ? 1.0 / hn : 0.0, w);
       ++nb;
       // Givens update of the new column
       const size_t cc = cs.size();
