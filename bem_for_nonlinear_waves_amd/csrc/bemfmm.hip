@@ -342,7 +342,9 @@ __global__ __launch_bounds__(256) void k_p2m(int C, int CP, const int64_t* __res
 // Workgroups are remapped so that each XCD processes one contiguous run of tiles
 // (tiles are sorted by operator): the tiles of an operator share that XCD's L2.
 // T = double: v_mfma_f64_16x16x4f64; T = float: v_mfma_f32_16x16x4f32 (FP32 accumulation).
-constexpr int kTP = 32;      // pairs per tile of the M2M/L2L translations
+constexpr int kTP = 8;       // pairs per tile of the M2M/L2L translations: their small upper levels are
+                             // latency-bound, so more, smaller workgroups pay (209k tank, per matvec:
+                             // M2M and L2L each 0.111 ms at 32 pairs, 0.091 at 16, 0.078 at 8)
 constexpr int kTPM2L = 16;   // default pairs per M2L tile: 3 workgroups per CU (FP64) instead of 2 at 32 pairs;
                              // measured on the 209k tank: FP64 3.61 -> 3.41 ms, FP32 2.52 -> 2.17 ms; 64 pairs: 4.6 ms
 // LDS elements of a tile of TP pairs: the B stage (KP x (2TP+1)) or the output transpose (2TP x (RP+1))
@@ -1688,7 +1690,8 @@ void launch_translate(bemfmm_ctx* c, DTiles& t, const T* ops, const double* S0, 
                        t.ntiles, ops, t.op.p, t.p0.p, t.p1.p, t.dst.p, t.src.p, S0, S1, D0, D1, stage)
 #define BEMFMM_TR(N)                                      \
   do {                                                    \
-    if (stage && t.tp == 16) BEMFMM_TR_TP(N, 16);         \
+    if (!stage && t.tp == 8) BEMFMM_TR_TP(N, 8);          \
+    else if (t.tp == 16) BEMFMM_TR_TP(N, 16);             \
     else if (stage && t.tp == 64) BEMFMM_TR_TP(N, 64);    \
     else BEMFMM_TR_TP(N, 32);                             \
   } while (0)
