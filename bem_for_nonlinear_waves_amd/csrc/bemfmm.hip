@@ -2103,7 +2103,14 @@ void tune_m2l_tiles(bemfmm_ctx* c, const Plan& pl) {
     return;
   }
   hipStream_t s = c->stream;
-  hipEvent_t e0, e1;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  struct Events {  // destroyed on every exit, including a throw from a launch below
+    hipEvent_t* e[2];
+    ~Events() {
+      for (auto* x : e)
+        if (*x) (void)hipEventDestroy(*x);
+    }
+  } guard{{&e0, &e1}};
   HIPCHECK(hipEventCreate(&e0));
   HIPCHECK(hipEventCreate(&e1));
   auto time_tiles = [&](DTiles& t) {
@@ -2128,8 +2135,6 @@ void tune_m2l_tiles(bemfmm_ctx* c, const Plan& pl) {
     std::lock_guard<std::mutex> lk(mu);
     chosen.emplace(cls, c->m2l_tp);
   }
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
   HIPCHECK(hipGetLastError());
 }
 

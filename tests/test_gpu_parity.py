@@ -131,15 +131,18 @@ def test_jacobi_solve_parity(tiny_tank, oracle_mod):
     g.close()
 
 
-@pytest.mark.parametrize("precision", [0, 1])
-def test_graph_replays_match_eager(small_tank, precision):
+@pytest.mark.parametrize("precision,precond", [(0, 1), (1, 1), (0, 2), (1, 2)])
+def test_graph_replays_match_eager(small_tank, precision, precond):
     """The matvec replayed from its captured HIP graph equals the eager launches, on every replay, with
-    other (eager) work and synchronisations between the replays."""
+    other (eager) work and synchronisations between the replays. In the solves the graph's input is
+    written by the basis normalisation and its output read in place by the preconditioner (Jacobi,
+    Schwarz), which the eager path does with ordinary buffers."""
     m, d, phi, phin = small_tank
-    # Jacobi: the unpreconditioned solve takes ~400 iterations here, where last-bit differences (M2M/L2L
-    # atomics) already move the stopping iteration
-    g = _gpu_bvp(m, d, phi, phin, p=10, leaf_max=32, precision=precision, use_graphs=1, preconditioner=1)
-    e = _gpu_bvp(m, d, phi, phin, p=10, leaf_max=32, precision=precision, use_graphs=0, preconditioner=1)
+    # preconditioned: the unpreconditioned solve takes ~400 iterations here, where last-bit differences
+    # (M2M/L2L atomics) already move the stopping iteration
+    print(f"[gpu] graph vs eager precision={precision} precond={precond}", flush=True)
+    g = _gpu_bvp(m, d, phi, phin, p=10, leaf_max=32, precision=precision, use_graphs=1, preconditioner=precond)
+    e = _gpu_bvp(m, d, phi, phin, p=10, leaf_max=32, precision=precision, use_graphs=0, preconditioner=precond)
     mv = (lambda b, x: b.compute_Ax_fp32(x)) if precision == 1 else (lambda b, x: b.compute_Ax(x))
     rng = np.random.default_rng(12)
     for it in range(4):
