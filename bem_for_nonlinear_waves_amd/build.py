@@ -1,10 +1,11 @@
 """Builds libbemfmm.so in-tree for gfx950.
 
 Host code (plan.cpp, operators.cpp, schwarz.cpp: the plan, its float128
-operators and the Schwarz blocks; server.cpp: the request loop over the C ABI)
+operators and the Schwarz blocks; bemfmm.cpp: contexts, launch sequences, GMRES and
+the C ABI over the HIP runtime API; server.cpp: the request loop over the C ABI)
 is compiled with g++ (OpenMP via libgomp, __float128 via libquadmath, threads);
-the HIP device code and C ABI (bemfmm.hip) with hipcc for gfx950; the objects are
-linked with hipcc.
+the device kernels and their launchers (kernels.hip) with hipcc for gfx950; the
+objects are linked with hipcc.
 Usage: python -m bem_for_nonlinear_waves_amd.build
 """
 from __future__ import annotations
@@ -26,6 +27,9 @@ HIP_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-result",
 ]
+ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
+# host files that call the HIP runtime API (bemfmm.cpp) are plain C++ against its header
+HIP_API_FLAGS = ["-D__HIP_PLATFORM_AMD__", f"-I{ROCM}/include"]
 CXX_FLAGS = ["-O3", "-fPIC", "-std=c++17", "-fopenmp", "-pthread", "-march=x86-64-v2", "-Wall", "-Wno-unused-function"]
 
 
@@ -43,12 +47,12 @@ def build(verbose: bool = True) -> str:
     hipcc = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
     os.makedirs(OUT_DIR, exist_ok=True)
     objs = []
-    for src in ("plan.cpp", "operators.cpp", "schwarz.cpp", "server.cpp"):
+    for src in ("plan.cpp", "operators.cpp", "schwarz.cpp", "server.cpp", "bemfmm.cpp"):
         obj = os.path.join(OUT_DIR, src.replace(".cpp", ".o"))
-        _run(["g++", *CXX_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj])
+        _run(["g++", *CXX_FLAGS, *HIP_API_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj])
         objs.append(obj)
-    obj = os.path.join(OUT_DIR, "bemfmm.o")
-    _run([hipcc, *HIP_FLAGS, "-c", os.path.join(CSRC, "bemfmm.hip"), "-o", obj])
+    obj = os.path.join(OUT_DIR, "kernels.o")
+    _run([hipcc, *HIP_FLAGS, "-c", os.path.join(CSRC, "kernels.hip"), "-o", obj])
     objs.append(obj)
     tmp = LIB + ".tmp"
     _run([hipcc, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs, f"-L{_gcc_libdir()}",

@@ -1,0 +1,1729 @@
+// Host side of the MI355X-native FMM-accelerated GMRES BVP engine and its C ABI (include/bemfmm.h):
+// contexts, device buffers, the per-matvec launch sequence (captured into HIP graphs), setup, GMRES and
+// iterative refinement. Device code lives in kernels.hip behind the launchers of kernels.hpp; this file
+// uses only the HIP runtime API.
+//
+// Matvec launch sequence on one HIP stream, all data resident in HBM (the reference's compute_Ax_minus_b,
+// bem/BEM_solveBVP_GMRES_FMM.hpp:34-158, with updateFMM, lib/include/moments.hpp:1547-1574): scatter,
+// P2M, M2M, M2L (+ per-box reduction), L2L, near field + L2P (the near-field DOF columns on a second
+// stream beside the FMM passes), assemble.
+#include <hip/hip_runtime_api.h>
+
+#include <algorithm>
+#include <array>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <shared_mutex>
+#include <stdexcept>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "../../include/bemfmm.h"
+#include "kernels.hpp"
+#include "plan.hpp"
+
+namespace {
+
+using bemfmm::Plan;
+namespace dev = bemfmm::dev;
+using dev::kDotBlocks;
+using dev::kRotTP;
+using dev::kSchwarzMaxBlock;
+using dev::kTP;
+using dev::kTPM2L;
+
+struct HipError : std::runtime_error {
+  int code;
+  HipError(const std::string& s, int c) : std::runtime_error(s), code(c) {}
+};
+
+#define HIPCHECK(expr)                                                                                        \
+  do {                                                                                                        \
+    hipError_t e_ = (expr);                                                                                   \
+    if (e_ != hipSuccess)                                                                                     \
+      throw HipError(std::string(#expr) + ": " + hipGetErrorString(e_),                                      \
+                     e_ == hipErrorOutOfMemory ? BEMFMM_E_OOM : BEMFMM_E_DEVICE);                             \
+  } while (0)
+
+double now() { return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count(); }
+
+// Stream capture against device-wide synchronisation, process-wide. hipFree and hipHostFree perform an
+// implicit hipDeviceSynchronize (HIP runtime API documentation), which synchronises every stream of the
+// device, including a stream that another thread is capturing into a graph at that moment: a
+// synchronisation that stream capture does not allow. With several contexts driven from several host
+// threads (the request loop), one context's frees (re-prepare, Krylov basis growth, teardown) would
+// otherwise run while another context captures its matvec graph. Captures (begin, end, instantiate) hold
+// the gate exclusively; every allocation and free of this library holds it shared.
+std::shared_mutex g_capture_gate;
+
+hipError_t dev_malloc(void** p, size_t bytes) {
+  std::shared_lock<std::shared_mutex> lk(g_capture_gate);
+  return hipMalloc(p, bytes);
+}
+hipError_t dev_free(void* p) {
+  std::shared_lock<std::shared_mutex> lk(g_capture_gate);
+  return hipFree(p);
+}
+hipError_t pinned_malloc(void** p, size_t bytes) {
+  std::shared_lock<std::shared_mutex> lk(g_capture_gate);
+  return hipHostMalloc(p, bytes, hipHostMallocDefault);
+}
+hipError_t pinned_free(void* p) {
+  std::shared_lock<std::shared_mutex> lk(g_capture_gate);
+  return hipHostFree(p);
+}
+
+// ---------------------------------------------------------------------------
+// device buffers
+// ---------------------------------------------------------------------------
+template <typename T>
+struct DBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  DBuf() = default;
+  DBuf(const DBuf&) = delete;
+  DBuf& operator=(const DBuf&) = delete;
+  DBuf(DBuf&& o) noexcept : p(o.p), n(o.n) {
+    o.p = nullptr;
+    o.n = 0;
+  }
+  DBuf& operator=(DBuf&& o) noexcept {
+    if (this != &o) {
+      release();
+      p = o.p;
+      n = o.n;
+      o.p = nullptr;
+      o.n = 0;
+    }
+    return *this;
+  }
+  ~DBuf() { release(); }
+  void release() {
+    if (p) (void)dev_free(p);
+    p = nullptr;
+    n = 0;
+  }
+  void alloc(size_t count) {
+    release();
+    if (count == 0) count = 1;
+    HIPCHECK(dev_malloc((void**)&p, count * sizeof(T)));
+    n = count;
+  }
+  // Setup-time upload; completes before returning (the source may be a temporary).
+  void upload(const std::vector<T>& v, hipStream_t s) {
+    alloc(v.size());
+    if (!v.empty()) {
+      HIPCHECK(hipMemcpyAsync(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s));
+      HIPCHECK(hipStreamSynchronize(s));
+    }
+  }
+  size_t bytes() const { return n * sizeof(T); }
+};
+
+struct DGemv {
+  DBuf<int64_t> row_ptr, col_ptr, mat_ptr;
+  DBuf<int32_t> row_out, col_idx;
+  DBuf<double> mat;
+  int nblk = 0;
+  void upload(const bemfmm::BlockGemv& g, hipStream_t s) {
+    row_ptr.upload(g.row_ptr, s);
+    col_ptr.upload(g.col_ptr, s);
+    mat_ptr.upload(g.mat_ptr, s);
+    row_out.upload(g.row_out, s);
+    col_idx.upload(g.col_idx, s);
+    nblk = g.n_blocks();
+  }
+  size_t bytes() const {
+    return row_ptr.bytes() + col_ptr.bytes() + mat_ptr.bytes() + row_out.bytes() + col_idx.bytes() + mat.bytes();
+  }
+  dev::Gemv view(const int32_t* ncol_dof = nullptr) const {
+    dev::Gemv g;
+    g.row_ptr = row_ptr.p;
+    g.col_ptr = col_ptr.p;
+    g.mat_ptr = mat_ptr.p;
+    g.row_out = row_out.p;
+    g.col_idx = col_idx.p;
+    g.ncol_dof = ncol_dof;
+    g.nblk = nblk;
+    return g;
+  }
+};
+
+struct DTiles {
+  DBuf<int32_t> op, dst, src;
+  DBuf<int64_t> p0, p1;
+  int ntiles = 0;
+  int tp = kTP;  // pairs per tile
+  // slot[op] = index of operator op in the context's device operator table
+  void upload(const bemfmm::PairGroups& g, hipStream_t s, const std::vector<int32_t>& slot, int pairs_per_tile = kTP) {
+    tp = pairs_per_tile;
+    std::vector<int32_t> top;
+    std::vector<int64_t> tp0, tp1;
+    for (size_t k = 0; k < g.op.size(); ++k)
+      for (int64_t a = g.pair_ptr[k]; a < g.pair_ptr[k + 1]; a += tp) {
+        top.push_back(slot[g.op[k]]);
+        tp0.push_back(a);
+        tp1.push_back(std::min(g.pair_ptr[k + 1], a + (int64_t)tp));
+      }
+    ntiles = (int)top.size();
+    op.upload(top, s);
+    p0.upload(tp0, s);
+    p1.upload(tp1, s);
+    dst.upload(g.dst, s);
+    src.upload(g.src, s);
+  }
+  size_t bytes() const { return op.bytes() + dst.bytes() + src.bytes() + p0.bytes() + p1.bytes(); }
+  dev::Tiles view() const {
+    dev::Tiles t;
+    t.op = op.p;
+    t.p0 = p0.p;
+    t.p1 = p1.p;
+    t.dst = dst.p;
+    t.src = src.p;
+    t.ntiles = ntiles;
+    t.tp = tp;
+    return t;
+  }
+};
+
+}  // namespace
+
+struct bemfmm_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  bemfmm_opts opts{};
+  std::string err;
+  // host inputs
+  int n_nodes = 0, n_faces = 0, n_dof = 0;
+  std::vector<double> X;
+  std::vector<int32_t> tri, face_dof, dof_kind, dof_point;
+  std::vector<double> phi_bc, phin_bc;
+  std::vector<double> robin_bc;  // kappa per DOF (caller order, physical units); empty = 0
+  bool have_geom = false, have_bc = false, prepared = false;
+  // internal DOF numbering (Morton order unless opts.dof_reorder == 0): internal i is caller DOF perm[i]
+  std::vector<int32_t> perm, inv, face_dof_i, kind_i, point_i;
+  bool perm_identity = true;
+  DBuf<int32_t> d_perm;
+  // plan and device state
+  std::unique_ptr<Plan> plan;
+  int C = 0, n_boxes = 0, RP = 0, KP = 0, p2m_rows_pad = 0;
+  DGemv near, p2m;
+  std::vector<DTiles> m2m, l2l;
+  DTiles m2l;
+  DBuf<int64_t> m2l_tptr;    // per box: its M2L pairs in m2l_tpairs
+  DBuf<int32_t> m2l_rorder;  // boxes in descending M2L pair count (launch order of the reduction)
+  DBuf<int32_t> m2l_tpairs;
+  DBuf<double> m2l_stage;    // per-pair M2L results [pair][layer][C]
+  DBuf<double> ops;
+  // FP32 copies for BEMFMM_PRECISION_FP32_REFINE (inner matvecs); the M2L staging is shared
+  DBuf<float> near_mat32, p2m_mat32, ops32;
+  // device operator table, [slot][KP][RP], kept across prepares: slot of each (kind, a, b, c) key
+  std::map<std::array<int, 4>, int32_t> op_slot;
+  int ops_cap = 0, ops_used = 0, ops_P = -1, ops32_valid = 0;
+  int ops_uploaded = 0;  // operators uploaded by the last prepare
+  std::vector<int32_t> m2l_slot;  // plan op -> table slot (for re-tiling by the autotuner)
+  int m2l_tp = 0;                // pairs per M2L tile in use
+  // rotation M2L of the FP32 operator: factors [slot][RS] (slots of the operator table; M2L slots only)
+  bool m2l_rot = false;
+  DBuf<float> rot32;
+  std::vector<char> rot_have;
+  int rot_RS = 0;
+  DTiles m2l_rt;  // M2L tiles of kRotTP pairs (one offset each)
+  int m2l_min_level = 0;  // coarsest level with M2L pairs: M2M into and L2L out of coarser levels are skipped
+  DBuf<float> m32;  // FP32 multipoles [box][layer][C] read by the rotation M2L
+  // near field beside the FMM passes: the near-only GEMV runs on stream2 between a fork and a join event
+  hipStream_t stream2 = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  DBuf<int32_t> near_ncd;  // DOF columns per near block
+  double* op_stage = nullptr;  // pinned host staging for operator uploads
+  size_t op_stage_n = 0;
+  double* hres = nullptr;      // pinned host copy of hbuf (the Arnoldi column of one iteration)
+  DBuf<int32_t> kind, point;
+  DBuf<double> alpha, phi_k, phin_k;
+  DBuf<double> robin;  // kappa L per DOF (internal order; 0 except on ROBIN DOFs)
+  DBuf<double> buf0, buf1;   // [phi_n | L0], [phi | L1]
+  DBuf<double> M0, M1;
+  DBuf<double> y0, y1;
+  DBuf<double> xin, yout, bvec, tmpv, xsol, xbase, jacobi, rvec, dvec, ptmp;
+  DBuf<double> gin, gout;            // fixed input/output of the captured matvec graphs
+  const double* gin_holds = nullptr;  // a vector gin already equals (valid until the next evaluate)
+  hipGraphExec_t gexec[2] = {};      // [FP64, FP32] matvec graphs
+  // restricted additive Schwarz: blocks and the core rows of the block inverses
+  bemfmm::SchwarzBlocks sw_host;
+  int sw_failed = 0;
+  DBuf<int64_t> sw_ptr, sw_invptr;
+  DBuf<int32_t> sw_idx, sw_ncore;
+  DBuf<double> sw_inv;
+  DBuf<double> V;            // Krylov basis
+  int64_t v_cap = 0;
+  DBuf<double> partial, hbuf;
+  bemfmm_stats stats{};
+  // profiling
+  bool prof = false;
+  hipEvent_t ev[9] = {};
+  int n_matvec = 0;
+  std::vector<double> history;         // GMRES residual estimate per iteration (last solve)
+  std::vector<double> refine_history;  // FP64 residual at each refinement step (last solve, FP32_REFINE)
+};
+
+namespace {
+
+void set_err(bemfmm_ctx* c, const std::string& s) {
+  if (c) c->err = s;
+}
+
+template <typename F>
+int guarded(bemfmm_ctx* c, F&& f) {
+  try {
+    if (c && c->device >= 0) HIPCHECK(hipSetDevice(c->device));
+    return f();
+  } catch (const HipError& e) {
+    set_err(c, e.what());
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    set_err(c, "host allocation failed");
+    return BEMFMM_E_OOM;
+  } catch (const std::exception& e) {
+    set_err(c, e.what());
+    return BEMFMM_E_INVALID;
+  }
+}
+
+dev::Schwarz schwarz_view(const bemfmm_ctx* c) {
+  dev::Schwarz b;
+  b.bptr = c->sw_ptr.p;
+  b.bidx = c->sw_idx.p;
+  b.bncore = c->sw_ncore.p;
+  b.inv_ptr = c->sw_invptr.p;
+  b.nblk = c->sw_host.n_blocks();
+  return b;
+}
+
+template <typename T>
+void launch_gemv(bemfmm_ctx* c, DGemv& g, const T* mat, const double* x0, const double* x1, double* y0, double* y1,
+                 int part = 0, hipStream_t s = nullptr) {
+  dev::block_gemv(s ? s : c->stream, g.view(c->near_ncd.p), mat, x0, x1, y0, y1, part);
+}
+
+template <typename T>
+void launch_translate(bemfmm_ctx* c, DTiles& t, const T* ops, const double* S0, const double* S1, double* D0, double* D1,
+                      T* stage = nullptr) {
+  dev::translate(c->stream, c->C, c->RP, c->KP, t.view(), ops, S0, S1, D0, D1, stage);
+}
+
+void launch_m2l_rot(bemfmm_ctx* c, float* stage) {
+  DTiles& t = c->m2l_rt;
+  if (t.ntiles == 0) return;
+  if (c->rot_RS != dev::rot_size(c->plan->P)) throw std::runtime_error("rotation M2L: factor table stride mismatch");
+  const int64_t nm = (int64_t)c->n_boxes * 2 * c->C;
+  dev::pack_m32(c->stream, nm, c->C, c->M0.p, c->M1.p, c->m32.p);
+  dev::m2l_rot(c->stream, c->plan->P, t.view(), c->rot32.p, c->m32.p, stage);
+}
+
+void fill(bemfmm_ctx* c, double* p, int64_t n, double v) { dev::fill(c->stream, p, n, v); }
+
+// One evaluation of the BIE operator. mode: 0 = A x (x = in), 1 = rhs, 2 = rigid-mode alpha.
+// lo: FP32 near field, P2M and translations (FP32 MFMA), FP64 vectors and accumulation across boxes.
+// Launches only (no host synchronisation), so that it can be captured into a HIP graph.
+void evaluate_launch(bemfmm_ctx* c, int mode, const double* in, double* out, bool lo) {
+  Plan& pl = *c->plan;
+  const int n = c->n_dof;
+  const int64_t nbC = (int64_t)c->n_boxes * c->C;
+  double* phin = c->buf0.p;
+  double* phi = c->buf1.p;
+  double* L0 = c->buf0.p + n;
+  double* L1 = c->buf1.p + n;
+  auto rec = [&](int i) {
+    if (c->prof) HIPCHECK(hipEventRecord(c->ev[i], c->stream));
+  };
+  rec(0);
+  dev::scatter(c->stream, mode == 0 ? 0 : (mode == 1 ? 1 : 2), n, c->kind.p, c->robin.p, in, c->phi_k.p, c->phin_k.p,
+               phin, phi);
+  // the near-field columns need only the scattered DOF values: they run on stream2 beside the FMM passes
+  // (not while profiling, which times the stages one after the other)
+  const bool split = c->opts.overlap_near && !c->prof && c->near.nblk > 0;
+  if (split) {
+    HIPCHECK(hipEventRecord(c->ev_fork, c->stream));
+    HIPCHECK(hipStreamWaitEvent(c->stream2, c->ev_fork, 0));
+    if (lo) launch_gemv(c, c->near, c->near_mat32.p, c->buf0.p, c->buf1.p, c->y0.p, c->y1.p, 1, c->stream2);
+    else launch_gemv(c, c->near, c->near.mat.p, c->buf0.p, c->buf1.p, c->y0.p, c->y1.p, 1, c->stream2);
+    HIPCHECK(hipEventRecord(c->ev_join, c->stream2));
+  }
+  fill(c, c->M0.p, nbC, 0.);
+  fill(c, c->M1.p, nbC, 0.);
+  rec(1);
+  if (lo) dev::p2m(c->stream, c->C, c->p2m_rows_pad, c->p2m.view(), c->p2m_mat32.p, phin, phi, c->M0.p, c->M1.p);
+  else dev::p2m(c->stream, c->C, c->p2m_rows_pad, c->p2m.view(), c->p2m.mat.p, phin, phi, c->M0.p, c->M1.p);
+  rec(2);
+  // M2M from level l into l-1 only where level l-1 has M2L pairs (coarser multipoles are not used).
+  // M2M and L2L use the FP64 operators for both operators: their inputs and outputs are FP64, and the
+  // FP64 matrix-core tiles ran faster here than the FP32 ones (0.115 / 0.120 ms against 0.157 / 0.161 ms
+  // per matvec at 209k nodes)
+  for (int l = pl.n_levels - 1; l >= 1 && l - 1 >= c->m2l_min_level; --l) {
+    launch_translate(c, c->m2m[l], c->ops.p, c->M0.p, c->M1.p, c->M0.p, c->M1.p);
+  }
+  rec(3);
+  if (lo) {
+    float* st = reinterpret_cast<float*>(c->m2l_stage.p);
+    if (c->m2l_rot) launch_m2l_rot(c, st);
+    else launch_translate(c, c->m2l, c->ops32.p, c->M0.p, c->M1.p, nullptr, nullptr, st);
+    rec(4);
+    dev::m2l_reduce(c->stream, c->C, c->n_boxes, c->m2l_rorder.p, c->m2l_tptr.p, c->m2l_tpairs.p, (const float*)st, L0, L1);
+  } else {
+    launch_translate(c, c->m2l, c->ops.p, c->M0.p, c->M1.p, nullptr, nullptr, c->m2l_stage.p);
+    rec(4);
+    dev::m2l_reduce(c->stream, c->C, c->n_boxes, c->m2l_rorder.p, c->m2l_tptr.p, c->m2l_tpairs.p,
+                    (const double*)c->m2l_stage.p, L0, L1);
+  }
+  rec(5);
+  // L2L into level l only from parents at levels with M2L pairs (the locals of coarser levels are zero)
+  for (int l = std::max(1, c->m2l_min_level + 1); l < pl.n_levels; ++l) {
+    launch_translate(c, c->l2l[l], c->ops.p, L0, L1, L0, L1);
+  }
+  rec(6);
+  if (split) HIPCHECK(hipStreamWaitEvent(c->stream, c->ev_join, 0));
+  const int part = split ? 2 : 0;
+  if (lo) launch_gemv(c, c->near, c->near_mat32.p, c->buf0.p, c->buf1.p, c->y0.p, c->y1.p, part);
+  else launch_gemv(c, c->near, c->near.mat.p, c->buf0.p, c->buf1.p, c->y0.p, c->y1.p, part);
+  rec(7);
+  dev::assemble(c->stream, mode, n, c->kind.p, c->point.p, c->alpha.p, phi, c->y0.p, c->y1.p, c->phi_k.p, out);
+  rec(8);
+  HIPCHECK(hipGetLastError());
+}
+
+// (graph teardown releases the runtime's device resources of the graph: kept off other threads' captures
+// like the library's own frees)
+void drop_graphs(bemfmm_ctx* c) {
+  std::shared_lock<std::shared_mutex> gate(g_capture_gate);
+  for (auto& g : c->gexec)
+    if (g) {
+      (void)hipGraphExecDestroy(g);
+      g = nullptr;
+    }
+}
+
+// mode 0 (A x) is replayed from a HIP graph captured once per context and precision on the context's own
+// input/output buffers (about 20 kernels per matvec become one launch; no memset nodes: the zero fills
+// are kernels). Event timing (bemfmm_matvec_profile) uses the eager launches. The copy into the graph's
+// input is skipped when gin already equals `in` (scale_basis wrote both); out == gout skips the copy out.
+bool graph_mode(const bemfmm_ctx* c) { return c->opts.use_graphs && !c->prof; }
+void evaluate(bemfmm_ctx* c, int mode, const double* in, double* out, bool lo = false) {
+  c->n_matvec++;
+  const double* held = c->gin_holds;
+  c->gin_holds = nullptr;
+  if (mode != 0 || !graph_mode(c)) {
+    evaluate_launch(c, mode, in, out, lo);
+    return;
+  }
+  const int gi = lo ? 1 : 0;
+  const size_t bytes = (size_t)c->n_dof * sizeof(double);
+  if (!c->gexec[gi]) {
+    // no device-wide synchronisation (frees of other contexts) while this stream is being captured
+    std::unique_lock<std::shared_mutex> gate(g_capture_gate);
+    hipGraph_t g = nullptr;
+    HIPCHECK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    try {
+      evaluate_launch(c, 0, c->gin.p, c->gout.p, lo);
+    } catch (...) {
+      (void)hipStreamEndCapture(c->stream, &g);
+      if (g) (void)hipGraphDestroy(g);
+      throw;
+    }
+    HIPCHECK(hipStreamEndCapture(c->stream, &g));
+    const hipError_t e = hipGraphInstantiate(&c->gexec[gi], g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    HIPCHECK(e);
+  }
+  if (in != c->gin.p && in != held)
+    HIPCHECK(hipMemcpyAsync(c->gin.p, in, bytes, hipMemcpyDeviceToDevice, c->stream));
+  HIPCHECK(hipGraphLaunch(c->gexec[gi], c->stream));
+  if (out != c->gout.p) HIPCHECK(hipMemcpyAsync(out, c->gout.p, bytes, hipMemcpyDeviceToDevice, c->stream));
+}
+
+// Internal DOF order (solveGMRES reindexes DOFs by Morton key, GMRES_FMM.hpp:3164-3246): keys of the DOFs'
+// points on the GPU, stable LSD radix sort (hipCUB) of the 63-bit keys, so equal keys (the DOFs of one
+// point) keep the caller's order. opts.dof_reorder == 0 keeps the caller's numbering.
+void dof_order(bemfmm_ctx* c) {
+  const int n = c->n_dof;
+  // indices are used by the key kernel and the renumbering below: reject bad input before either
+  for (int i = 0; i < n; ++i)
+    if (c->dof_point[i] < 0 || c->dof_point[i] >= c->n_nodes || c->dof_kind[i] < 0 || c->dof_kind[i] > 3)
+      throw std::runtime_error("bemfmm_prepare: dof_point or dof_kind out of range");
+  for (int32_t d : c->face_dof)
+    if (d < 0 || d >= n) throw std::runtime_error("bemfmm_prepare: face_dof out of range");
+  for (int32_t v : c->tri)
+    if (v < 0 || v >= c->n_nodes) throw std::runtime_error("bemfmm_prepare: tri out of range");
+  c->perm.resize(n);
+  if (c->opts.dof_reorder) {
+    hipStream_t s = c->stream;
+    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+    for (int p = 0; p < c->n_nodes; ++p)
+      for (int a = 0; a < 3; ++a) {
+        lo[a] = std::min(lo[a], c->X[3 * p + a]);
+        hi[a] = std::max(hi[a], c->X[3 * p + a]);
+      }
+    DBuf<double> dX;
+    DBuf<int32_t> dpt, v0, v1;
+    DBuf<uint64_t> k0, k1;
+    dX.upload(c->X, s);
+    dpt.upload(c->dof_point, s);
+    v0.alloc(n);
+    v1.alloc(n);
+    k0.alloc(n);
+    k1.alloc(n);
+    const double bounds[6] = {lo[0], hi[0], lo[1], hi[1], lo[2], hi[2]};
+    dev::morton_keys(s, n, dpt.p, dX.p, bounds, k0.p, v0.p);
+    HIPCHECK(hipGetLastError());
+    size_t tmp_bytes = 0;
+    HIPCHECK(dev::radix_sort_pairs(s, nullptr, tmp_bytes, k0.p, k1.p, v0.p, v1.p, n, 63));
+    DBuf<unsigned char> tmp;
+    tmp.alloc(tmp_bytes);
+    HIPCHECK(dev::radix_sort_pairs(s, tmp.p, tmp_bytes, k0.p, k1.p, v0.p, v1.p, n, 63));
+    HIPCHECK(hipMemcpyAsync(c->perm.data(), v1.p, n * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    HIPCHECK(hipStreamSynchronize(s));
+  } else {
+    for (int i = 0; i < n; ++i) c->perm[i] = i;
+  }
+  c->inv.assign(n, -1);
+  c->perm_identity = true;
+  for (int i = 0; i < n; ++i) {
+    const int j = c->perm[i];
+    if (j < 0 || j >= n || c->inv[j] >= 0) throw std::runtime_error("internal: DOF order is not a permutation");
+    c->inv[j] = i;
+    if (j != i) c->perm_identity = false;
+  }
+  c->face_dof_i.resize(c->face_dof.size());
+  for (size_t k = 0; k < c->face_dof.size(); ++k) c->face_dof_i[k] = c->inv[c->face_dof[k]];
+  c->kind_i.resize(n);
+  c->point_i.resize(n);
+  for (int i = 0; i < n; ++i) {
+    c->kind_i[i] = c->dof_kind[c->perm[i]];
+    c->point_i[i] = c->dof_point[c->perm[i]];
+  }
+  c->d_perm.upload(c->perm, c->stream);
+}
+
+// caller-ordered host vector <-> internal order
+void host_to_internal(const bemfmm_ctx* c, const double* in, double* out) {
+  for (int i = 0; i < c->n_dof; ++i) out[i] = in[c->perm[i]];
+}
+void host_to_caller(const bemfmm_ctx* c, const double* in, double* out) {
+  for (int i = 0; i < c->n_dof; ++i) out[c->perm[i]] = in[i];
+}
+
+// knowns in internal order, Neumann phi_n scaled by L (GMRES_FMM.hpp:3076-3091)
+void upload_knowns(bemfmm_ctx* c) {
+  const int n = c->n_dof;
+  std::vector<double> phik(n), phink(n);
+  for (int i = 0; i < n; ++i) {
+    const int j = c->perm[i];
+    phik[i] = c->phi_bc[j];
+    phink[i] = (c->kind_i[i] != 0) ? c->phin_bc[j] * c->plan->Lscale : c->phin_bc[j];
+  }
+  // ROBIN: phi_n = kappa phi + g, so in scaled coordinates (phi_n' = L phi_n) kappa' = L kappa
+  std::vector<double> kap(n, 0.);
+  if (!c->robin_bc.empty())
+    for (int i = 0; i < n; ++i)
+      if (c->kind_i[i] == 3) kap[i] = c->robin_bc[c->perm[i]] * c->plan->Lscale;
+  if (c->phi_k.n != (size_t)n) c->phi_k.alloc(n);
+  if (c->phin_k.n != (size_t)n) c->phin_k.alloc(n);
+  if (c->robin.n != (size_t)n) c->robin.alloc(n);
+  HIPCHECK(hipMemcpyAsync(c->phi_k.p, phik.data(), n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  HIPCHECK(hipMemcpyAsync(c->phin_k.p, phink.data(), n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  HIPCHECK(hipMemcpyAsync(c->robin.p, kap.data(), n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  HIPCHECK(hipStreamSynchronize(c->stream));
+}
+
+// Maps the plan's operators to slots of the context's device table [slot][KP][RP] (zero padding), uploading
+// only operators the table does not hold yet: a re-prepare after nodes moved (RK stages of a session) reuses
+// nearly all of them. The table grows by doubling (old slots copied on the device).
+std::vector<int32_t> update_operator_table(bemfmm_ctx* c, const Plan& pl) {
+  hipStream_t s = c->stream;
+  const int C = pl.C, KP = c->KP, RP = c->RP;
+  const int64_t per = (int64_t)KP * RP;
+  if (c->ops_P != pl.P) {
+    c->op_slot.clear();
+    c->ops_used = 0;
+    c->ops_cap = 0;
+    c->ops.release();
+    c->ops32.release();
+    c->ops32_valid = 0;
+    c->ops_P = pl.P;
+  }
+  std::vector<int32_t> slot(pl.n_ops);
+  std::vector<int> fresh;  // plan ops to upload
+  for (int op = 0; op < pl.n_ops; ++op) {
+    const auto key = bemfmm::op_key(pl, op);
+    auto it = c->op_slot.find(key);
+    if (it == c->op_slot.end()) {
+      it = c->op_slot.emplace(key, (int32_t)(c->ops_used + fresh.size())).first;
+      fresh.push_back(op);
+    }
+    slot[op] = it->second;
+  }
+  const int need = c->ops_used + (int)fresh.size();
+  if (need > c->ops_cap) {
+    const int cap = std::max(need, std::max(64, 2 * c->ops_cap));
+    DBuf<double> t;
+    t.alloc((size_t)cap * per);
+    if (c->ops_used > 0)
+      HIPCHECK(hipMemcpyAsync(t.p, c->ops.p, (size_t)c->ops_used * per * sizeof(double), hipMemcpyDeviceToDevice, s));
+    HIPCHECK(hipStreamSynchronize(s));
+    c->ops = std::move(t);
+    c->ops_cap = cap;
+  }
+  // upload the new operators in batches through pinned staging, padded to [KP][RP]
+  constexpr int kBatch = 32;
+  if (!fresh.empty() && c->op_stage_n < (size_t)kBatch * per) {
+    if (c->op_stage) (void)pinned_free(c->op_stage);
+    c->op_stage = nullptr;
+    HIPCHECK(pinned_malloc((void**)&c->op_stage, (size_t)kBatch * per * sizeof(double)));
+    c->op_stage_n = (size_t)kBatch * per;
+  }
+  for (size_t b0 = 0; b0 < fresh.size(); b0 += kBatch) {
+    const int nb = (int)std::min<size_t>(kBatch, fresh.size() - b0);
+    for (int i = 0; i < nb; ++i) {
+      const auto key = bemfmm::op_key(pl, fresh[b0 + i]);
+      const auto v = bemfmm::cached_operator(pl.P, key[0], key[1], key[2], key[3]);
+      double* dst = c->op_stage + (size_t)i * per;
+      std::fill(dst, dst + per, 0.);
+      for (int k = 0; k < C; ++k) std::memcpy(dst + (size_t)k * RP, v->data() + (size_t)k * C, C * sizeof(double));
+    }
+    HIPCHECK(hipMemcpyAsync(c->ops.p + (size_t)(c->ops_used + b0) * per, c->op_stage, (size_t)nb * per * sizeof(double),
+                            hipMemcpyHostToDevice, s));
+    HIPCHECK(hipStreamSynchronize(s));  // the staging is refilled by the next batch
+  }
+  c->ops_used = need;
+  c->ops_uploaded = (int)fresh.size();
+  return slot;
+}
+
+// Rotation factors of the plan's M2L offsets in the FP32 table rot32 [slot][RS], indexed by the operator
+// table's slots; only slots without factors yet are computed (process-wide cache) and uploaded.
+void update_rotation_table(bemfmm_ctx* c, const Plan& pl, const std::vector<int32_t>& slot) {
+  hipStream_t s = c->stream;
+  const int RS = dev::rot_size(pl.P);  // padded device layout
+  if (c->rot_RS != RS) {
+    c->rot32.release();
+    c->rot_have.clear();
+    c->rot_RS = RS;
+  }
+  if ((int64_t)c->rot_have.size() < c->ops_cap) {
+    DBuf<float> t;
+    t.alloc((size_t)c->ops_cap * RS);
+    if (!c->rot_have.empty())
+      HIPCHECK(hipMemcpyAsync(t.p, c->rot32.p, c->rot_have.size() * RS * sizeof(float), hipMemcpyDeviceToDevice, s));
+    HIPCHECK(hipStreamSynchronize(s));
+    c->rot32 = std::move(t);
+    c->rot_have.resize(c->ops_cap, 0);
+  }
+  std::vector<int> fresh;
+  for (int op = 16; op < pl.n_ops; ++op)
+    if (!c->rot_have[slot[op]]) {
+      c->rot_have[slot[op]] = 1;
+      fresh.push_back(op);
+    }
+  if (fresh.empty()) return;
+  {
+    std::vector<std::array<int, 4>> keys;
+    for (int op : fresh) {
+      const auto k = bemfmm::op_key(pl, op);
+      keys.push_back({3, k[1], k[2], k[3]});
+    }
+    bemfmm::warm_operator_cache(pl.P, keys);  // factors of new offsets, computed in parallel once per process
+  }
+  const size_t cap = c->op_stage_n * 2 / RS;  // rotations per pinned staging batch (the staging holds doubles)
+  if (cap < 1) {
+    if (c->op_stage) (void)pinned_free(c->op_stage);
+    c->op_stage = nullptr;
+    c->op_stage_n = (size_t)RS * 64;
+    HIPCHECK(pinned_malloc((void**)&c->op_stage, c->op_stage_n * sizeof(double)));
+  }
+  const size_t batch = c->op_stage_n * 2 / RS;
+  float* stg = reinterpret_cast<float*>(c->op_stage);
+  for (size_t b0 = 0; b0 < fresh.size(); b0 += batch) {
+    const size_t nb = std::min(batch, fresh.size() - b0);
+    // consecutive slots are uploaded together: runs of fresh ops with consecutive slots
+    for (size_t i = 0; i < nb; ++i) {
+      const auto key = bemfmm::op_key(pl, fresh[b0 + i]);
+      const auto v = bemfmm::cached_operator(pl.P, 3, key[1], key[2], key[3]);
+      dev::rot_pad(pl.P, v->data(), stg + i * RS);
+    }
+    size_t i = 0;
+    while (i < nb) {
+      size_t j = i + 1;
+      while (j < nb && slot[fresh[b0 + j]] == slot[fresh[b0 + j - 1]] + 1) ++j;
+      HIPCHECK(hipMemcpyAsync(c->rot32.p + (size_t)slot[fresh[b0 + i]] * RS, stg + i * RS, (j - i) * RS * sizeof(float),
+                              hipMemcpyHostToDevice, s));
+      i = j;
+    }
+    HIPCHECK(hipStreamSynchronize(s));  // the staging is refilled by the next batch
+  }
+}
+
+// Tile-size autotuner (opts.m2l_tile_pairs == 0): times the M2L translation kernel of this plan with
+// 16 and 32 pairs per tile on the multipoles the rigid-mode evaluation just left in M (one warm-up and
+// two timed launches each, HIP events on the context's stream) and keeps the faster tiling. The choice
+// is kept for the process per (expansion order, size class = log2 of the pair count), so later plans of
+// that class reuse it instead of timing again (and instead of timing behind other contexts' work).
+// Plans with few pairs keep the default.
+void tune_m2l_tiles(bemfmm_ctx* c, const Plan& pl) {
+  c->m2l_tp = c->m2l.tp;
+  if (c->opts.m2l_tile_pairs != 0 || pl.n_m2l_pairs < 20000) return;
+  static std::mutex mu;
+  static std::map<std::pair<int, int>, int> chosen;
+  const std::pair<int, int> cls{pl.P, (int)std::log2((double)pl.n_m2l_pairs)};
+  int known = 0;
+  {
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = chosen.find(cls);
+    if (it != chosen.end()) known = it->second;
+  }
+  if (known == c->m2l.tp) return;
+  if (known) {
+    DTiles alt;
+    alt.upload(pl.m2l, c->stream, c->m2l_slot, known);
+    c->m2l = std::move(alt);
+    c->m2l_tp = known;
+    return;
+  }
+  hipStream_t s = c->stream;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  struct Events {  // destroyed on every exit, including a throw from a launch below
+    hipEvent_t* e[2];
+    ~Events() {
+      for (auto* x : e)
+        if (*x) (void)hipEventDestroy(*x);
+    }
+  } guard{{&e0, &e1}};
+  HIPCHECK(hipEventCreate(&e0));
+  HIPCHECK(hipEventCreate(&e1));
+  auto time_tiles = [&](DTiles& t) {
+    float best = 1e30f;
+    for (int r = 0; r < 3; ++r) {
+      HIPCHECK(hipEventRecord(e0, s));
+      launch_translate(c, t, c->ops.p, c->M0.p, c->M1.p, nullptr, nullptr, c->m2l_stage.p);
+      HIPCHECK(hipEventRecord(e1, s));
+      HIPCHECK(hipEventSynchronize(e1));
+      float ms = 0.f;
+      HIPCHECK(hipEventElapsedTime(&ms, e0, e1));
+      if (r > 0) best = std::min(best, ms);
+    }
+    return best;
+  };
+  DTiles alt;
+  alt.upload(pl.m2l, s, c->m2l_slot, c->m2l.tp == 16 ? 32 : 16);
+  const float t_cur = time_tiles(c->m2l), t_alt = time_tiles(alt);
+  if (t_alt < t_cur) c->m2l = std::move(alt);
+  c->m2l_tp = c->m2l.tp;
+  {
+    std::lock_guard<std::mutex> lk(mu);
+    chosen.emplace(cls, c->m2l_tp);
+  }
+  HIPCHECK(hipGetLastError());
+}
+
+void do_prepare(bemfmm_ctx* c) {
+  if (!c->have_geom) throw std::runtime_error("bemfmm_prepare: geometry not set");
+  if (!c->have_bc) throw std::runtime_error("bemfmm_prepare: boundary values not set");
+  if (c->opts.precision != BEMFMM_PRECISION_FP64 && c->opts.precision != BEMFMM_PRECISION_FP32_REFINE)
+    throw std::runtime_error("bemfmm_prepare: unknown precision mode");
+  if (c->opts.m2l_tile_pairs != 0 && c->opts.m2l_tile_pairs != 16 && c->opts.m2l_tile_pairs != 32 &&
+      c->opts.m2l_tile_pairs != 64)
+    throw std::runtime_error("bemfmm_prepare: m2l_tile_pairs must be 0 (default), 16, 32 or 64");
+  if (c->opts.m2l_kernel < 0 || c->opts.m2l_kernel > 2)
+    throw std::runtime_error("bemfmm_prepare: m2l_kernel must be 0 (default), 1 (matrix cores) or 2 (rotation)");
+  if (c->opts.m2l_kernel == 2 && !dev::m2l_rot_compiled(c->opts.p))
+    throw std::runtime_error("bemfmm_prepare: the rotation M2L is compiled for p = 8 and 10 only");
+  if (c->opts.preconditioner == BEMFMM_PRECOND_SCHWARZ &&
+      (c->opts.schwarz_max_core < 1 || c->opts.schwarz_max_block < c->opts.schwarz_max_core ||
+       c->opts.schwarz_max_block > kSchwarzMaxBlock || c->opts.schwarz_overlap < 0 || c->opts.schwarz_overlap > 8))
+    throw std::runtime_error("bemfmm_prepare: Schwarz block sizes need 1 <= max_core <= max_block <= 128, 0 <= overlap <= 8");
+  const double t0 = now();
+  bemfmm::Options o;
+  o.p = c->opts.p;
+  o.theta = c->opts.theta;
+  o.leaf_max = c->opts.leaf_max;
+  o.max_level = c->opts.max_level;
+  o.q = c->opts.q;
+  o.materialize_ops = false;  // operators go from the float128 cache straight into the device table
+  o.fill_p2m = false;         // the P2M matrices are built on the GPU (k_p2m_build)
+  c->prepared = false;
+  drop_graphs(c);  // their kernels point into the buffers reallocated below
+  // the Krylov basis is sized for n_dof: drop it (re-grown by the next solve)
+  c->V.release();
+  c->partial.release();
+  c->hbuf.release();
+  c->v_cap = 0;
+  dof_order(c);
+  c->plan.reset(new Plan());
+  Plan& pl = *c->plan;
+  bemfmm::build_plan(pl, o, c->n_nodes, c->X.data(), c->n_faces, c->tri.data(), c->face_dof_i.data(), c->n_dof,
+                     c->kind_i.data(), c->point_i.data());
+  c->C = pl.C;
+  c->n_boxes = pl.n_boxes;
+  c->p2m_rows_pad = pl.p2m_rows_pad;
+  if (pl.p2m_rows_pad > 4 * 128) throw std::runtime_error("expansion order too large for the P2M kernel");
+  if ((pl.C + 63) / 64 > 7) throw std::runtime_error("expansion order too large for the translation kernel");
+  if (2 * pl.C > 256) throw std::runtime_error("expansion order too large for the M2L reduction");
+  hipStream_t s = c->stream;
+  const double t1 = now();
+  // ---- upload ----
+  c->near.upload(pl.near, s);
+  c->near.mat.alloc(pl.near.mat_size);
+  c->p2m.upload(pl.p2m, s);
+  c->p2m.mat.alloc(pl.p2m.mat_size);
+  fill(c, c->p2m.mat.p, pl.p2m.mat_size, 0.);  // rows past C stay zero
+  c->RP = (pl.C + 63) / 64 * 64;
+  c->KP = (pl.C + 4 * dev::kKG - 1) / (4 * dev::kKG) * (4 * dev::kKG);
+  const std::vector<int32_t> slot = update_operator_table(c, pl);
+  c->m2m.clear();
+  c->l2l.clear();
+  c->m2m.resize(pl.n_levels);
+  c->l2l.resize(pl.n_levels);
+  for (int l = 1; l < pl.n_levels; ++l) {
+    c->m2m[l].upload(pl.m2m_by_level[l], s, slot);
+    c->l2l[l].upload(pl.l2l_by_level[l], s, slot);
+  }
+  c->m2l.upload(pl.m2l, s, slot, c->opts.m2l_tile_pairs > 0 ? c->opts.m2l_tile_pairs : kTPM2L);
+  c->m2l_slot = slot;
+  c->m2l_min_level = pl.n_levels;  // no M2L pairs: no translations at all
+  for (int32_t d : pl.m2l.dst) c->m2l_min_level = std::min(c->m2l_min_level, (int)pl.box_level[d]);
+  {
+    std::vector<int64_t> tptr(pl.n_boxes + 1, 0);
+    for (int32_t d : pl.m2l.dst) tptr[d + 1]++;
+    for (int b = 0; b < pl.n_boxes; ++b) tptr[b + 1] += tptr[b];
+    std::vector<int32_t> tp(pl.m2l.dst.size());
+    std::vector<int64_t> fillp(tptr.begin(), tptr.end() - 1);
+    for (int64_t q = 0; q < (int64_t)pl.m2l.dst.size(); ++q) tp[fillp[pl.m2l.dst[q]]++] = (int32_t)q;
+    c->m2l_tptr.upload(tptr, s);
+    c->m2l_tpairs.upload(tp, s);
+    std::vector<int32_t> rorder(pl.n_boxes);
+    for (int b = 0; b < pl.n_boxes; ++b) rorder[b] = b;
+    std::stable_sort(rorder.begin(), rorder.end(),
+                     [&](int32_t x, int32_t y) { return tptr[x + 1] - tptr[x] > tptr[y + 1] - tptr[y]; });
+    c->m2l_rorder.upload(rorder, s);
+    // FP64 [pair][layer][C], or FP32 [pair][layer][stage_stride<float>(C)] in the same buffer
+    const size_t col = std::max<size_t>(pl.C, (dev::stage_stride<float>(pl.C) + 1) / 2);
+    c->m2l_stage.alloc(std::max<size_t>(1, pl.m2l.dst.size() * 2 * col));
+  }
+  c->kind.upload(pl.dof_kind, s);
+  c->point.upload(pl.dof_point, s);
+  upload_knowns(c);
+  const int64_t nvec = (int64_t)c->n_dof + (int64_t)pl.n_boxes * pl.C;
+  c->buf0.alloc(nvec);
+  c->buf1.alloc(nvec);
+  c->M0.alloc((size_t)pl.n_boxes * pl.C);
+  c->M1.alloc((size_t)pl.n_boxes * pl.C);
+  c->y0.alloc(c->n_nodes);
+  c->y1.alloc(c->n_nodes);
+  c->alpha.alloc(c->n_dof);
+  for (auto* b : {&c->xin, &c->yout, &c->bvec, &c->tmpv, &c->xsol, &c->xbase, &c->jacobi, &c->rvec, &c->dvec, &c->ptmp,
+                 &c->gin, &c->gout})
+    b->alloc(c->n_dof);
+  // quadrature inputs
+  DBuf<int32_t> row_block, ncol_dof, face_perm, fv, fdof;
+  DBuf<int64_t> range_ptr, ranges, range_prefix;
+  DBuf<double> Xs, fX, fcent, fcr, fJ, l2p, d_dun;
+  {
+    std::vector<int32_t> rb(pl.near.row_out.size());
+    for (int b = 0; b < pl.near.n_blocks(); ++b)
+      for (int64_t r = pl.near.row_ptr[b]; r < pl.near.row_ptr[b + 1]; ++r) rb[r] = b;
+    row_block.upload(rb, s);
+  }
+  ncol_dof.upload(pl.near_ncol_dof, s);
+  c->near_ncd.upload(pl.near_ncol_dof, s);
+  face_perm.upload(pl.face_perm, s);
+  fv.upload(pl.fv, s);
+  fdof.upload(pl.fdof, s);
+  range_ptr.upload(pl.near_range_ptr, s);
+  ranges.upload(pl.near_ranges, s);
+  range_prefix.upload(pl.near_range_prefix, s);
+  Xs.upload(pl.Xs, s);
+  fX.upload(pl.fX, s);
+  fcent.upload(pl.fcent, s);
+  fcr.upload(pl.fcr, s);
+  fJ.upload(pl.fJ, s);
+  l2p.upload(pl.l2p_coef, s);
+  {
+    double dun[7][4] = {};
+    static const double d1[1][4] = {{1. / 3., 1. / 3., 1. / 3., 0.5}};
+    static const double d3[3][4] = {{2. / 3., 1. / 6., 1. / 6., 1. / 6.}, {1. / 6., 2. / 3., 1. / 6., 1. / 6.}, {1. / 6., 1. / 6., 2. / 3., 1. / 6.}};
+    static const double d6[6][4] = {{0.816847572980459, 0.091576213509771, 0.091576213509771, 0.054975871827661},
+                                    {0.091576213509771, 0.816847572980459, 0.091576213509771, 0.054975871827661},
+                                    {0.091576213509771, 0.091576213509771, 0.816847572980459, 0.054975871827661},
+                                    {0.108103018168070, 0.445948490915965, 0.445948490915965, 0.111690794839006},
+                                    {0.445948490915965, 0.108103018168070, 0.445948490915965, 0.111690794839006},
+                                    {0.445948490915965, 0.445948490915965, 0.108103018168070, 0.111690794839006}};
+    static const double d7[7][4] = {{1. / 3., 1. / 3., 1. / 3., 0.1125},
+                                    {0.059715871789770, 0.470142064105115, 0.470142064105115, 0.066197076394253},
+                                    {0.470142064105115, 0.059715871789770, 0.470142064105115, 0.066197076394253},
+                                    {0.470142064105115, 0.470142064105115, 0.059715871789770, 0.066197076394253},
+                                    {0.797426985353087, 0.101286507323456, 0.101286507323456, 0.062969590272414},
+                                    {0.101286507323456, 0.797426985353087, 0.101286507323456, 0.062969590272414},
+                                    {0.101286507323456, 0.101286507323456, 0.797426985353087, 0.062969590272414}};
+    const double(*src)[4] = o.q == 1 ? d1 : o.q == 3 ? d3 : o.q == 6 ? d6 : d7;
+    for (int g = 0; g < o.q; ++g)
+      for (int a = 0; a < 4; ++a) dun[g][a] = src[g][a];
+    d_dun.upload(std::vector<double>(&dun[0][0], &dun[0][0] + 28), s);
+  }
+  // ---- P2M matrices on the GPU ----
+  if (c->p2m.nblk > 0) {
+    if ((int)pl.p2m_face_ptr.size() != c->p2m.nblk + 1 || pl.p2m_face_lc.size() != 3 * pl.p2m_faces.size())
+      throw std::runtime_error("P2M source lists do not match the P2M blocks");
+    const int NT = (pl.P + 1) * (pl.P + 2) / 2;
+    std::vector<double> cnm(NT);
+    for (int n = 0; n <= pl.P; ++n)
+      for (int m = 0; m <= n; ++m) {
+        double f1 = 1, f2 = 1;
+        for (int k = 2; k <= n - m; ++k) f1 *= k;
+        for (int k = 2; k <= n + m; ++k) f2 *= k;
+        const double v = std::sqrt(f1 * f2);
+        cnm[n * (n + 1) / 2 + m] = (m & 1) ? -v : v;
+      }
+    DBuf<int64_t> fptr;
+    DBuf<int32_t> pfaces, plc;
+    DBuf<double> pcs, dcnm;
+    fptr.upload(pl.p2m_face_ptr, s);
+    pfaces.upload(pl.p2m_faces, s);
+    plc.upload(pl.p2m_face_lc, s);
+    pcs.upload(pl.p2m_leaf_cs, s);
+    dcnm.upload(cnm, s);
+    dev::P2MBuild a;
+    a.q = o.q;
+    a.CP = pl.p2m_rows_pad;
+    a.nleaves = c->p2m.nblk;
+    a.face_ptr = fptr.p;
+    a.faces = pfaces.p;
+    a.face_lc = plc.p;
+    a.leaf_cs = pcs.p;
+    a.mat_ptr = c->p2m.mat_ptr.p;
+    a.col_ptr = c->p2m.col_ptr.p;
+    a.fX = fX.p;
+    a.fcr = fcr.p;
+    a.fJ = fJ.p;
+    a.rule = d_dun.p;
+    a.cnm = dcnm.p;
+    dev::p2m_build(s, pl.P, a, c->p2m.mat.p);
+    HIPCHECK(hipGetLastError());
+  }
+  HIPCHECK(hipStreamSynchronize(s));
+  const double t2 = now();
+  // ---- near-field quadrature on the GPU ----
+  fill(c, c->near.mat.p, pl.near.mat_size, 0.);
+  const int64_t nrows = (int64_t)pl.near.row_out.size();
+  if (nrows > 0) {
+    dev::NearQuad a;
+    a.C = pl.C;
+    a.q = o.q;
+    a.near_region = pl.near_region;
+    a.row_block = row_block.p;
+    a.ncol_dof = ncol_dof.p;
+    a.range_ptr = range_ptr.p;
+    a.ranges = ranges.p;
+    a.range_prefix = range_prefix.p;
+    a.face_perm = face_perm.p;
+    a.Xs = Xs.p;
+    a.fX = fX.p;
+    a.fcent = fcent.p;
+    a.fcr = fcr.p;
+    a.fJ = fJ.p;
+    a.fv = fv.p;
+    a.fdof = fdof.p;
+    a.l2p = l2p.p;
+    a.dun_rule = d_dun.p;
+    dev::near_quad(s, nrows, a, c->near.view(), c->near.mat.p);
+  }
+  HIPCHECK(hipGetLastError());
+  HIPCHECK(hipStreamSynchronize(s));
+  const double t3 = now();
+  // ---- rigid-mode alpha (updateGeometricProperties) ----
+  evaluate(c, 2, nullptr, c->alpha.p);
+  tune_m2l_tiles(c, pl);
+  const int pc = c->opts.preconditioner;
+  c->sw_host = bemfmm::SchwarzBlocks{};
+  for (auto* b : {&c->sw_inv}) b->release();
+  if (pc == BEMFMM_PRECOND_JACOBI || pc == BEMFMM_PRECOND_SCHWARZ) {
+    std::vector<int64_t> node_row(c->n_nodes, -1);
+    for (size_t r = 0; r < pl.near.row_out.size(); ++r) node_row[pl.near.row_out[r]] = (int64_t)r;
+    for (int i = 0; i < c->n_dof; ++i)
+      if (c->kind_i[i] != 2 && node_row[c->point_i[i]] < 0)
+        throw std::runtime_error("preconditioner: a DOF's point has no near-field row");
+    DBuf<int64_t> d_node_row;
+    d_node_row.upload(node_row, s);
+    dev::NearRows nr;
+    nr.kind = c->kind.p;
+    nr.kappa = c->robin.p;
+    nr.point = c->point.p;
+    nr.node_row = d_node_row.p;
+    nr.row_block = row_block.p;
+    nr.row_ptr = c->near.row_ptr.p;
+    nr.col_ptr = c->near.col_ptr.p;
+    nr.mat_ptr = c->near.mat_ptr.p;
+    nr.col_idx = c->near.col_idx.p;
+    nr.ncol_dof = ncol_dof.p;
+    nr.mat = c->near.mat.p;
+    nr.alpha = c->alpha.p;
+    if (pc == BEMFMM_PRECOND_JACOBI) {
+      dev::diag_inv(s, c->n_dof, nr, c->jacobi.p);
+      HIPCHECK(hipGetLastError());
+    } else {
+      // seeds in a spatially coherent order: the internal (Morton) DOF order, or the order of the DOFs'
+      // target boxes (level-major preorder) when the caller's numbering is kept
+      std::vector<int32_t> seed(c->n_dof);
+      for (int i = 0; i < c->n_dof; ++i) seed[i] = i;
+      if (!c->opts.dof_reorder)
+        std::stable_sort(seed.begin(), seed.end(), [&](int32_t a, int32_t b) {
+          return pl.target_box[c->point_i[a]] < pl.target_box[c->point_i[b]];
+        });
+      bemfmm::build_schwarz_blocks(c->n_dof, c->n_faces, c->face_dof_i.data(), c->point_i.data(), seed,
+                                   c->opts.schwarz_max_core, c->opts.schwarz_max_block, c->opts.schwarz_overlap,
+                                   c->sw_host);
+      const bemfmm::SchwarzBlocks& B = c->sw_host;
+      const int nb = B.n_blocks();
+      if (B.max_block > kSchwarzMaxBlock) throw std::runtime_error("schwarz: block larger than the kernel's limit");
+      std::vector<int64_t> invptr(nb + 1, 0);
+      for (int b = 0; b < nb; ++b) invptr[b + 1] = invptr[b] + (int64_t)B.ncore[b] * (B.ptr[b + 1] - B.ptr[b]);
+      c->sw_ptr.upload(B.ptr, s);
+      c->sw_idx.upload(B.idx, s);
+      c->sw_ncore.upload(B.ncore, s);
+      c->sw_invptr.upload(invptr, s);
+      c->sw_inv.alloc((size_t)invptr[nb]);
+      DBuf<int32_t> failed;
+      failed.alloc(nb);
+      dev::schwarz_build(s, schwarz_view(c), B.max_block, nr, c->sw_inv.p, failed.p);
+      HIPCHECK(hipGetLastError());
+      std::vector<int32_t> fh(nb);
+      HIPCHECK(hipMemcpyAsync(fh.data(), failed.p, nb * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+      HIPCHECK(hipStreamSynchronize(s));
+      c->sw_failed = 0;
+      for (int32_t f : fh) c->sw_failed += f;
+    }
+  } else if (pc != BEMFMM_PRECOND_NONE) {
+    throw std::runtime_error("unknown preconditioner");
+  }
+  if (c->opts.precision == BEMFMM_PRECISION_FP32_REFINE) {
+    // round the resident FP64 matrices once to FP32 (operators were computed in binary128 and rounded to FP64)
+    c->near_mat32.alloc(pl.near.mat_size);
+    dev::to_f32(s, (int64_t)pl.near.mat_size, c->near.mat.p, c->near_mat32.p);
+    c->p2m_mat32.alloc(c->p2m.mat.n);
+    dev::to_f32(s, (int64_t)c->p2m.mat.n, c->p2m.mat.p, c->p2m_mat32.p);
+    c->m2l_rot = c->opts.m2l_kernel != 1 && dev::m2l_rot_compiled(pl.P);
+    if (c->m2l_rot) {
+      update_rotation_table(c, pl, c->m2l_slot);
+      c->m2l_rt.upload(pl.m2l, s, c->m2l_slot, kRotTP);
+      c->m32.alloc((size_t)pl.n_boxes * 2 * pl.C);
+      c->ops32.release();  // M2M and L2L use the FP64 table
+      c->ops32_valid = 0;
+    } else {
+      // FP32 operator table for the dense M2L: convert the slots that are new (all after the table grew)
+      const int64_t per = (int64_t)c->KP * c->RP;
+      if (c->ops32.n != c->ops.n) {
+        c->ops32.alloc(c->ops.n);
+        c->ops32_valid = 0;
+      }
+      if (c->ops32_valid < c->ops_used) {
+        const int64_t n32 = (int64_t)(c->ops_used - c->ops32_valid) * per;
+        dev::to_f32(s, n32, c->ops.p + c->ops32_valid * per, c->ops32.p + c->ops32_valid * per);
+        c->ops32_valid = c->ops_used;
+      }
+    }
+    HIPCHECK(hipGetLastError());
+  } else {
+    c->near_mat32.release();
+    c->p2m_mat32.release();
+    c->ops32.release();
+    c->ops32_valid = 0;
+    c->m2l_rot = false;
+  }
+  HIPCHECK(hipStreamSynchronize(s));
+  const double t4 = now();
+  c->prepared = true;
+  bemfmm_stats& st = c->stats;
+  st = bemfmm_stats{};
+  st.n_nodes = c->n_nodes;
+  st.n_faces = c->n_faces;
+  st.n_dof = c->n_dof;
+  st.n_levels = pl.n_levels;
+  st.n_boxes = pl.n_boxes;
+  st.n_leaves = (int)pl.n_leaves;
+  st.n_m2l_pairs = pl.n_m2l_pairs;
+  st.near_nnz = pl.near_nnz;
+  st.n_m2l_ops = (int)pl.m2l.op.size();
+  st.L = pl.Lscale;
+  st.device_bytes = (double)(c->near.bytes() + c->p2m.bytes() + c->m2l.bytes() + c->ops.bytes() + c->buf0.bytes() +
+                             c->buf1.bytes() + c->M0.bytes() + c->M1.bytes() + c->m2l_stage.bytes() +
+                             c->near_mat32.bytes() + c->p2m_mat32.bytes() + c->ops32.bytes() + c->sw_inv.bytes());
+  st.t_tree = pl.t_tree;
+  st.t_lists = pl.t_lists;
+  st.t_near_struct = pl.t_near_struct;
+  st.t_p2m = pl.t_p2m;
+  st.t_ops = pl.t_ops;
+  st.t_upload = t2 - t1;
+  st.t_near_quad = t3 - t2;
+  st.t_alpha = t4 - t3;
+  st.n_precond_blocks = c->sw_host.n_blocks();
+  st.precond_fallback_blocks = c->sw_failed;
+  st.precond_avg_block = c->sw_host.n_blocks() ? (double)c->sw_host.idx.size() / c->sw_host.n_blocks() : 0.;
+  st.n_ops_uploaded = c->ops_uploaded;
+  st.m2l_tile_pairs = c->m2l_tp;
+  st.m2l_rotation = c->m2l_rot ? 1 : 0;
+  (void)t0;
+}
+
+void ensure_basis(bemfmm_ctx* c, int64_t need) {
+  if (need <= c->v_cap) return;
+  int64_t cap = std::max<int64_t>(need, std::max<int64_t>(64, 2 * c->v_cap));
+  DBuf<double> nv;
+  nv.alloc((size_t)cap * c->n_dof);
+  if (c->v_cap > 0)
+    HIPCHECK(hipMemcpyAsync(nv.p, c->V.p, (size_t)c->v_cap * c->n_dof * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+  std::swap(c->V.p, nv.p);
+  std::swap(c->V.n, nv.n);
+  c->v_cap = cap;
+  c->partial.alloc((size_t)kDotBlocks * (cap + 1));
+  // [h1 (k) | h2 (k) | w.w] of one Arnoldi step (k <= cap), or the k update coefficients
+  c->hbuf.alloc(2 * (size_t)cap + 2);
+  if (c->hres) (void)pinned_free(c->hres);
+  c->hres = nullptr;
+  HIPCHECK(pinned_malloc((void**)&c->hres, (2 * (size_t)cap + 2) * sizeof(double)));
+}
+
+// hbuf[off, off + k) = V[0..k) . w on the device (no host copy)
+void dots_dev(bemfmm_ctx* c, int k, const double* Vp, const double* w, size_t off) {
+  const int64_t n = c->n_dof;
+  // the reduction buffers are sized by ensure_basis for up to v_cap + 1 products
+  if (k < 1 || k > c->v_cap + 1 || !c->partial.p || !c->hbuf.p || (size_t)k * kDotBlocks > c->partial.n ||
+      off + (size_t)k > c->hbuf.n)
+    throw HipError("internal: dot-product buffers not sized for " + std::to_string(k) + " products", BEMFMM_E_STATE);
+  dev::dots(c->stream, k, Vp, n, w, n, c->partial.p);
+  dev::reduce_partials(c->stream, c->partial.p, k, c->hbuf.p + off);
+}
+// h[0..k) = V[0..k) . w  (device), copied to host
+void dots(bemfmm_ctx* c, int k, const double* Vp, const double* w, std::vector<double>& h) {
+  dots_dev(c, k, Vp, w, 0);
+  h.resize(k);
+  HIPCHECK(hipMemcpyAsync(h.data(), c->hbuf.p, k * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHECK(hipStreamSynchronize(c->stream));
+}
+double norm2(bemfmm_ctx* c, const double* w) {
+  std::vector<double> h;
+  dots(c, 1, w, w, h);
+  return std::sqrt(h[0]);
+}
+
+// out = M^-1 in for the context's preconditioner (left preconditioning, return_A_dot_v and the
+// preconditioned RHS, BEM_solveBVP_GMRES_FMM.hpp:2124-2171). in and out may alias except for Schwarz.
+void precond_apply(bemfmm_ctx* c, const double* in, double* out) {
+  const int64_t n = c->n_dof;
+  switch (c->opts.preconditioner) {
+  case BEMFMM_PRECOND_JACOBI:
+    if (in != out) dev::mul_to(c->stream, n, c->jacobi.p, in, out);
+    else dev::mul(c->stream, n, c->jacobi.p, out);
+    break;
+  case BEMFMM_PRECOND_SCHWARZ:
+    if (in == out) throw std::runtime_error("internal: Schwarz apply needs distinct buffers");
+    dev::schwarz_apply(c->stream, schwarz_view(c), c->sw_inv.p, in, out);
+    break;
+  default:
+    if (in != out) HIPCHECK(hipMemcpyAsync(out, in, n * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+  }
+}
+
+// w = M^-1 A v (return_A_dot_v); lo: FP32 operator. With the graph replay the preconditioner reads the
+// graph's output buffer (no copy out; gout is next written by the next replay on the same stream).
+void apply_op(bemfmm_ctx* c, const double* v, double* w, bool lo) {
+  double* y = graph_mode(c) ? c->gout.p : c->opts.preconditioner == BEMFMM_PRECOND_SCHWARZ ? c->ptmp.p : w;
+  evaluate(c, 0, v, y, lo);
+  precond_apply(c, y, w);
+}
+
+// w *= a for a new basis vector; with the graph replay it is also written to the graph's input buffer,
+// so the next evaluate(w) starts without a copy
+void scale_basis(bemfmm_ctx* c, double a, double* w) {
+  const int64_t n = c->n_dof;
+  if (graph_mode(c)) {
+    dev::scale_dup(c->stream, n, a, w, c->gin.p);
+    c->gin_holds = w;
+  } else {
+    dev::axpby(c->stream, n, a, w, 0.0, w);
+  }
+}
+
+// One GMRES run on  M^-1 A x = rhs  (rhs: device, already preconditioned), x (device) updated
+// in place. confirm = true follows run_gmres (BEM_solveBVP_GMRES_FMM.hpp:2275-2355): restart
+// cycles, an estimate <= 10 tol triggers a true-residual check, up to 5 saturated checks.
+// confirm = false (inner solve of the refinement loop) stops when the estimate reaches tol.
+// Arnoldi + Givens as basic_linear_systems.hpp:3137-3451, with classical Gram-Schmidt applied
+// twice on the device. Returns the last (true or estimated) residual; iters accumulates.
+struct GmresRun {
+  const double* rhs;
+  double* x;
+  double tol;
+  int max_iter;  // iteration budget including what `iters` holds on entry
+  int restart;
+  bool lo;       // FP32 matvecs
+  bool confirm;
+  bool x_zero;   // x is zero on entry: r0 = rhs without a matvec
+};
+
+double gmres_run(bemfmm_ctx* c, const GmresRun& R, int& iters) {
+  const int64_t n = c->n_dof;
+  hipStream_t s = c->stream;
+  auto Av = [&](const double* v, double* w) { apply_op(c, v, w, R.lo); };
+  double cur_err = 1e300;
+  const int restart_max = std::max(1, R.restart);
+  bool first = true;
+  while (iters < R.max_iter && cur_err > R.tol) {
+    const int remaining = R.max_iter - iters;
+    const int k_cycle_max = std::min(restart_max, remaining);
+    ensure_basis(c, 8);
+    // r0 = rhs - M^-1 A x
+    if (first && R.x_zero) {
+      HIPCHECK(hipMemcpyAsync(c->V.p, R.rhs, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+    } else {
+      Av(R.x, c->tmpv.p);
+      dev::sub(s, n, R.rhs, c->tmpv.p, c->V.p);
+    }
+    first = false;
+    const double beta = norm2(c, c->V.p);
+    HIPCHECK(hipMemcpyAsync(c->xbase.p, R.x, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+    std::vector<double> cs, sn, g(1, beta);
+    std::vector<std::vector<double>> Rm;
+    if (!std::isfinite(beta)) throw HipError("GMRES breakdown: non-finite initial residual", BEMFMM_E_NONFINITE);
+    if (beta == 0.) {
+      cur_err = 0.;
+      break;
+    }
+    scale_basis(c, 1.0 / beta, c->V.p);
+    int nb = 1;  // basis vectors in V
+    auto add_basis = [&]() {
+      ensure_basis(c, nb + 1);
+      double* w = c->V.p + (int64_t)nb * n;
+      Av(c->V.p + (int64_t)(nb - 1) * n, w);
+      // both Gram-Schmidt passes and the norm run on the device into hbuf = [h1 | h2 | w.w]; the
+      // column comes to the host once per iteration
+      dots_dev(c, nb, c->V.p, w, 0);
+      dev::vsum(s, c->V.p, n, nb, c->hbuf.p, -1.0, w, n);
+      dots_dev(c, nb, c->V.p, w, nb);
+      dev::vsum(s, c->V.p, n, nb, c->hbuf.p + nb, -1.0, w, n);
+      dots_dev(c, 1, w, w, 2 * (size_t)nb);
+      HIPCHECK(hipMemcpyAsync(c->hres, c->hbuf.p, (2 * (size_t)nb + 1) * sizeof(double), hipMemcpyDeviceToHost, s));
+      HIPCHECK(hipStreamSynchronize(s));
+      std::vector<double> col(nb + 1);
+      for (int i = 0; i < nb; ++i) col[i] = c->hres[i] + c->hres[nb + i];
+      const double hn = std::sqrt(c->hres[2 * nb]);
+      col[nb] = hn;
+      scale_basis(c, hn > 0. ? 1.0 / hn : 0.0, w);
+      ++nb;
+      // Givens update of the new column
+      const size_t cc = cs.size();
+      for (size_t i = 0; i < cc; ++i) {
+        const double tmp = std::fma(cs[i], col[i], sn[i] * col[i + 1]);
+        col[i + 1] = std::fma(cs[i], col[i + 1], -sn[i] * col[i]);
+        col[i] = tmp;
+      }
+      double cg = 1.0, sg = 0.0;
+      const double r = std::hypot(col[cc], col[cc + 1]);
+      if (r != 0.0) {
+        cg = col[cc] / r;
+        sg = col[cc + 1] / r;
+      }
+      const double tmp = std::fma(cg, col[cc], sg * col[cc + 1]);
+      col[cc + 1] = std::fma(cg, col[cc + 1], -sg * col[cc]);
+      col[cc] = tmp;
+      cs.push_back(cg);
+      sn.push_back(sg);
+      g.push_back(0.);
+      const double gt = std::fma(cg, g[cc], sg * g[cc + 1]);
+      g[cc + 1] = std::fma(cg, g[cc + 1], -sg * g[cc]);
+      g[cc] = gt;
+      Rm.push_back(col);
+    };
+    auto update_solution = [&]() {
+      const size_t k = cs.size();
+      std::vector<double> y(k, 0.);
+      for (int i = (int)k - 1; i >= 0; --i) {
+        double sum = g[i];
+        for (size_t j = i + 1; j < k; ++j) sum -= Rm[j][i] * y[j];
+        y[i] = sum / Rm[i][i];
+      }
+      HIPCHECK(hipMemcpyAsync(R.x, c->xbase.p, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+      HIPCHECK(hipMemcpyAsync(c->hbuf.p, y.data(), k * sizeof(double), hipMemcpyHostToDevice, s));
+      dev::vsum(s, c->V.p, n, (int)k, c->hbuf.p, 1.0, R.x, n);
+      HIPCHECK(hipStreamSynchronize(s));
+    };
+    auto true_residual = [&]() {
+      Av(R.x, c->tmpv.p);
+      dev::sub(s, n, c->tmpv.p, R.rhs, c->tmpv.p);
+      return norm2(c, c->tmpv.p);
+    };
+    add_basis();  // gmres(A, b, x, 1)
+    iters += 1;
+    c->history.push_back(std::abs(g.back()));
+    if (!R.confirm && std::abs(g.back()) <= R.tol) {
+      update_solution();
+      cur_err = std::abs(g.back());
+      break;
+    }
+    int sat = 0;
+    while (true) {
+      add_basis();
+      ++iters;
+      const double est = std::abs(g.back());
+      c->history.push_back(est);
+      if (!std::isfinite(est)) throw HipError("GMRES breakdown: non-finite residual estimate", BEMFMM_E_NONFINITE);
+      if (!R.confirm) {
+        if (est <= R.tol || (int)cs.size() >= k_cycle_max) {
+          update_solution();
+          cur_err = est;
+          break;
+        }
+        continue;
+      }
+      if (est <= R.tol * 10.) {
+        update_solution();
+        cur_err = true_residual();
+        if (cur_err <= R.tol) break;
+        if (sat++ >= 5) break;
+      }
+      if (iters >= R.max_iter && (int)cs.size() >= k_cycle_max) {
+        update_solution();
+        cur_err = (est <= R.tol * 10.) ? true_residual() : est;
+        break;
+      }
+    }
+    if (!R.confirm) break;
+  }
+  return cur_err;
+}
+
+// FP64 preconditioned residual r = M^-1 (b - A x) into out; returns |r|_2.
+double fp64_residual(bemfmm_ctx* c, const double* x, double* out) {
+  const int64_t n = c->n_dof;
+  apply_op(c, x, out, false);
+  dev::sub(c->stream, n, c->bvec.p, out, out);
+  return norm2(c, out);
+}
+
+// Solve (solveSystemGMRES, BEM_solveBVP_GMRES_FMM.hpp:2007-2355).
+// FP64: one run_gmres with FP64 matvecs.
+// FP32_REFINE: iterative refinement. Each step takes the FP64 residual r = M^-1 (b - A x), solves
+// M^-1 A_32 d = r with FP32 matvecs to |r| * refine_eta, and sets x += d; it stops when |r| <= tol
+// (the same FP64 criterion as the FP64 path). If a step fails to halve |r|, the remaining budget
+// goes to an FP64 run_gmres from the current x, so the answer always meets the FP64 criterion.
+int do_solve(bemfmm_ctx* c, const double* x0, double* xout, double tol, int max_iter, int restart, bemfmm_result* res) {
+  if (!c->prepared) throw std::runtime_error("bemfmm_solve: call bemfmm_prepare first");
+  const double ts = now();
+  const int64_t n = c->n_dof;
+  hipStream_t s = c->stream;
+  const int n_mv0 = c->n_matvec;
+  ensure_basis(c, 8);  // Krylov basis and the dot-product buffers (norms are taken before the first cycle)
+  evaluate(c, 1, nullptr, c->ptmp.p);  // b, then M^-1 b (left preconditioning, GMRES_FMM.hpp:2157-2171)
+  precond_apply(c, c->ptmp.p, c->bvec.p);
+  std::vector<double> x0i(n);
+  host_to_internal(c, x0, x0i.data());
+  // the unknowns of Dirichlet DOFs (phi_n) are solved for in scaled form, L phi_n, as the result is
+  // returned divided by L below: the initial guess is scaled the same way
+  const double L0 = c->plan->Lscale;
+  for (int64_t i = 0; i < n; ++i)
+    if (c->kind_i[i] == 0) x0i[i] *= L0;
+  HIPCHECK(hipMemcpyAsync(c->xsol.p, x0i.data(), n * sizeof(double), hipMemcpyHostToDevice, s));
+  c->history.clear();
+  c->refine_history.clear();
+  int iters = 0, steps = 0;
+  double cur_err = 0.;
+  // a zero initial guess has r0 = M^-1 b: no matvec for the first residual
+  const bool x0_zero = std::all_of(x0i.begin(), x0i.end(), [](double v) { return v == 0.; });
+  if (c->opts.precision == BEMFMM_PRECISION_FP32_REFINE) {
+    const double eta = (c->opts.refine_eta > 0. && c->opts.refine_eta < 1.) ? c->opts.refine_eta : 3e-7;
+    double prev = 1e300;
+    bool fallback = false;
+    bool x_zero = x0_zero;
+    while (true) {
+      if (x_zero) {
+        HIPCHECK(hipMemcpyAsync(c->rvec.p, c->bvec.p, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+        cur_err = norm2(c, c->rvec.p);
+        x_zero = false;
+      } else {
+        cur_err = fp64_residual(c, c->xsol.p, c->rvec.p);
+      }
+      c->refine_history.push_back(cur_err);
+      if (!std::isfinite(cur_err)) throw HipError("non-finite residual in refinement", BEMFMM_E_NONFINITE);
+      if (cur_err <= tol || iters >= max_iter) break;
+      if (cur_err > 0.5 * prev) {
+        fallback = true;
+        break;
+      }
+      prev = cur_err;
+      fill(c, c->dvec.p, n, 0.);
+      GmresRun in{c->rvec.p, c->dvec.p, std::max(eta * cur_err, 0.5 * tol), max_iter, restart, true, false, true};
+      gmres_run(c, in, iters);
+      dev::axpby(s, n, 1.0, c->dvec.p, 1.0, c->xsol.p);
+      ++steps;
+    }
+    if (fallback && iters < max_iter) {
+      GmresRun r64{c->bvec.p, c->xsol.p, tol, max_iter, restart, false, true, false};
+      cur_err = gmres_run(c, r64, iters);
+    }
+  } else {
+    GmresRun r64{c->bvec.p, c->xsol.p, tol, max_iter, restart, false, true, x0_zero};
+    cur_err = gmres_run(c, r64, iters);
+  }
+  std::vector<double> x(n);
+  HIPCHECK(hipMemcpyAsync(x.data(), c->xsol.p, n * sizeof(double), hipMemcpyDeviceToHost, s));
+  HIPCHECK(hipStreamSynchronize(s));
+  const double L = c->plan->Lscale;
+  for (int64_t i = 0; i < n; ++i) xout[c->perm[i]] = (c->kind_i[i] == 0) ? x[i] / L : x[i];
+  if (res) {
+    res->iters = iters;
+    res->n_matvec = c->n_matvec - n_mv0;
+    res->converged = cur_err <= tol ? 1 : 0;
+    res->residual = cur_err;
+    res->time_solve = now() - ts;
+    res->refinement_steps = steps;
+  }
+  for (int64_t i = 0; i < n; ++i)
+    if (!std::isfinite(xout[i])) throw HipError("non-finite solution", BEMFMM_E_NONFINITE);
+  return cur_err <= tol ? BEMFMM_OK : BEMFMM_E_NOT_CONVERGED;
+}
+
+// y = A x on host vectors in caller order
+void host_matvec(bemfmm_ctx* c, const double* x, double* y, bool lo) {
+  const int n = c->n_dof;
+  std::vector<double> xi(n), yi(n);
+  host_to_internal(c, x, xi.data());
+  HIPCHECK(hipMemcpyAsync(c->xin.p, xi.data(), n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  evaluate(c, 0, c->xin.p, c->yout.p, lo);
+  HIPCHECK(hipMemcpyAsync(yi.data(), c->yout.p, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHECK(hipStreamSynchronize(c->stream));
+  host_to_caller(c, yi.data(), y);
+}
+
+// y = A x on device vectors in caller order (permuted through the context's buffers unless the order is the caller's)
+void device_matvec(bemfmm_ctx* c, const double* x, double* y, bool lo) {
+  const int n = c->n_dof;
+  if (c->perm_identity) {
+    evaluate(c, 0, x, y, lo);
+    return;
+  }
+  dev::perm_gather(c->stream, n, c->d_perm.p, x, c->xin.p);
+  evaluate(c, 0, c->xin.p, c->yout.p, lo);
+  dev::perm_scatter(c->stream, n, c->d_perm.p, c->yout.p, y);
+  HIPCHECK(hipGetLastError());
+}
+
+// device vector (internal order) -> host vector (caller order)
+void download_caller(bemfmm_ctx* c, const double* d, double* out) {
+  std::vector<double> v(c->n_dof);
+  HIPCHECK(hipMemcpyAsync(v.data(), d, c->n_dof * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHECK(hipStreamSynchronize(c->stream));
+  host_to_caller(c, v.data(), out);
+}
+
+}  // namespace
+
+extern "C" {
+
+int bemfmm_is_available(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  return n > 0 ? 1 : 0;
+}
+
+const char* bemfmm_device_name(void) {
+  static char name[600] = {0};
+  hipDeviceProp_t p;
+  if (hipGetDeviceProperties(&p, 0) != hipSuccess) return "unavailable";
+  std::snprintf(name, sizeof(name), "%s (%s)", p.name, p.gcnArchName);
+  return name;
+}
+
+void bemfmm_default_opts(bemfmm_opts* o) {
+  if (!o) return;
+  o->p = 10;
+  o->theta = 0.25;
+  o->leaf_max = 50;
+  o->max_level = 7;
+  o->q = 6;
+  o->precision = BEMFMM_PRECISION_FP64;
+  o->preconditioner = BEMFMM_PRECOND_NONE;
+  o->refine_eta = 3e-7;
+  o->schwarz_max_core = 64;   // main_time_domain.cpp:47
+  o->schwarz_max_block = 128; // main_time_domain.cpp:48
+  o->schwarz_overlap = 1;     // main_time_domain.cpp:46
+  o->m2l_tile_pairs = 0;
+  o->dof_reorder = 1;
+  o->use_graphs = 1;
+  o->m2l_kernel = 0;
+  o->overlap_near = 1;
+}
+
+bemfmm_ctx* bemfmm_ctx_create(int device, const bemfmm_opts* opts) {
+  auto* c = new bemfmm_ctx();
+  c->device = device;
+  if (opts) c->opts = *opts;
+  else bemfmm_default_opts(&c->opts);
+  const int rc = guarded(c, [&]() {
+    int n = 0;
+    HIPCHECK(hipGetDeviceCount(&n));
+    if (device < 0 || device >= n) throw HipError("invalid device index", BEMFMM_E_DEVICE);
+    HIPCHECK(hipSetDevice(device));
+    HIPCHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    HIPCHECK(hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking));
+    HIPCHECK(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
+    HIPCHECK(hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming));
+    return BEMFMM_OK;
+  });
+  if (rc != BEMFMM_OK) {
+    std::fprintf(stderr, "bemfmm_ctx_create: %s\n", c->err.c_str());
+    delete c;
+    return nullptr;
+  }
+  return c;
+}
+
+void bemfmm_ctx_destroy(bemfmm_ctx* c) {
+  if (!c) return;
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  if (c->stream2) (void)hipStreamSynchronize(c->stream2);
+  for (auto& e : c->ev)
+    if (e) (void)hipEventDestroy(e);
+  for (auto e : {c->ev_fork, c->ev_join})
+    if (e) (void)hipEventDestroy(e);
+  drop_graphs(c);
+  if (c->op_stage) (void)pinned_free(c->op_stage);
+  if (c->hres) (void)pinned_free(c->hres);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  if (c->stream2) (void)hipStreamDestroy(c->stream2);
+  delete c;
+}
+
+const char* bemfmm_last_error(const bemfmm_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int bemfmm_set_geometry(bemfmm_ctx* c, int n_nodes, const double* X, int n_faces, const int32_t* tri, const int32_t* face_dof,
+                        int n_dof, const int32_t* dof_kind, const int32_t* dof_point) {
+  if (!c) return BEMFMM_E_INVALID;
+  return guarded(c, [&]() {
+    if (n_nodes <= 0 || n_faces <= 0 || n_dof <= 0 || !X || !tri || !face_dof || !dof_kind || !dof_point)
+      throw std::runtime_error("bemfmm_set_geometry: empty or null input");
+    c->n_nodes = n_nodes;
+    c->n_faces = n_faces;
+    c->n_dof = n_dof;
+    c->X.assign(X, X + 3 * (size_t)n_nodes);
+    c->tri.assign(tri, tri + 3 * (size_t)n_faces);
+    c->face_dof.assign(face_dof, face_dof + 3 * (size_t)n_faces);
+    c->dof_kind.assign(dof_kind, dof_kind + n_dof);
+    c->dof_point.assign(dof_point, dof_point + n_dof);
+    c->have_geom = true;
+    c->have_bc = false;
+    c->prepared = false;
+    c->robin_bc.clear();
+    return BEMFMM_OK;
+  });
+}
+
+int bemfmm_set_bc(bemfmm_ctx* c, const double* phi, const double* phin) {
+  if (!c) return BEMFMM_E_INVALID;
+  return guarded(c, [&]() {
+    if (!c->have_geom) throw HipError("bemfmm_set_bc: geometry not set", BEMFMM_E_STATE);
+    if (!phi || !phin) throw std::runtime_error("bemfmm_set_bc: null input");
+    for (int i = 0; i < c->n_dof; ++i) {
+      const bool dir = c->dof_kind[i] == 0;
+      if ((dir || c->dof_kind[i] == 2) && !std::isfinite(phi[i])) throw HipError("non-finite phi", BEMFMM_E_NONFINITE);
+      if (!dir && !std::isfinite(phin[i])) throw HipError("non-finite phin", BEMFMM_E_NONFINITE);
+    }
+    c->phi_bc.assign(phi, phi + c->n_dof);
+    c->phin_bc.assign(phin, phin + c->n_dof);
+    c->have_bc = true;
+    if (c->prepared) {
+      // boundary values only change the right-hand side: refresh the knowns in place
+      upload_knowns(c);
+    }
+    return BEMFMM_OK;
+  });
+}
+
+int bemfmm_set_robin(bemfmm_ctx* c, const double* kappa) {
+  if (!c) return BEMFMM_E_INVALID;
+  return guarded(c, [&]() {
+    if (!c->have_geom) throw HipError("bemfmm_set_robin: geometry not set", BEMFMM_E_STATE);
+    if (!kappa) throw std::runtime_error("bemfmm_set_robin: null input");
+    for (int i = 0; i < c->n_dof; ++i)
+      if (c->dof_kind[i] == 3 && !std::isfinite(kappa[i])) throw HipError("non-finite kappa", BEMFMM_E_NONFINITE);
+    c->robin_bc.assign(kappa, kappa + c->n_dof);
+    c->prepared = false;  // the operator and its preconditioner depend on kappa: prepare again
+    return BEMFMM_OK;
+  });
+}
+
+int bemfmm_prepare(bemfmm_ctx* c) {
+  if (!c) return BEMFMM_E_INVALID;
+  return guarded(c, [&]() {
+    if (!c->have_geom || !c->have_bc) throw HipError("bemfmm_prepare: set geometry and boundary values first", BEMFMM_E_STATE);
+    do_prepare(c);
+    return BEMFMM_OK;
+  });
+}
+
+int bemfmm_matvec(bemfmm_ctx* c, const double* x, double* y) {
+  if (!c) return BEMFMM_E_INVALID;
+  return guarded(c, [&]() {
+    if (!c->prepared) throw HipError("bemfmm_matvec: call bemfmm_prepare first", BEMFMM_E_STATE);
+    host_matvec(c, x, y, false);
+    return BEMFMM_OK;
+  });
+}
+
+int bemfmm_matvec_device(bemfmm_ctx* c, const double* x, double* y) {
+  if (!c) return BEMFMM_E_INVALID;
+  return guarded(c, [&]() {
+    if (!c->prepared) throw HipError("bemfmm_matvec_device: call bemfmm_prepare first", BEMFMM_E_STATE);
+    device_matvec(c, x, y, false);
+    return BEMFMM_OK;
+  });
+}
+
+int bemfmm_matvec_fp32(bemfmm_ctx* c, const double* x, double* y) {
+  if (!c) return BEMFMM_E_INVALID;
+  return guarded(c, [&]() {
+    if (!c->prepared) throw HipError("bemfmm_matvec_fp32: call bemfmm_prepare first", BEMFMM_E_STATE);
+    if (c->opts.precision != BEMFMM_PRECISION_FP32_REFINE)
+      throw HipError("bemfmm_matvec_fp32: context not created with BEMFMM_PRECISION_FP32_REFINE", BEMFMM_E_STATE);
+    host_matvec(c, x, y, true);
+    return BEMFMM_OK;
+  });
+}
+
+int bemfmm_matvec_fp32_device(bemfmm_ctx* c, const double* x, double* y) {
+  if (!c) return BEMFMM_E_INVALID;
+  return guarded(c, [&]() {
+    if (!c->prepared) throw HipError("bemfmm_matvec_fp32_device: call bemfmm_prepare first", BEMFMM_E_STATE);
+    if (c->opts.precision != BEMFMM_PRECISION_FP32_REFINE)
+      throw HipError("bemfmm_matvec_fp32_device: context not created with BEMFMM_PRECISION_FP32_REFINE", BEMFMM_E_STATE);
+    device_matvec(c, x, y, true);
+    return BEMFMM_OK;
+  });
+}
+
+int bemfmm_sync(bemfmm_ctx* c) {
+  if (!c) return BEMFMM_E_INVALID;
+  return guarded(c, [&]() {
+    HIPCHECK(hipStreamSynchronize(c->stream));
+    return BEMFMM_OK;
+  });
+}
+
+int bemfmm_rhs(bemfmm_ctx* c, double* b) {
+  if (!c) return BEMFMM_E_INVALID;
+  return guarded(c, [&]() {
+    if (!c->prepared) throw HipError("bemfmm_rhs: call bemfmm_prepare first", BEMFMM_E_STATE);
+    evaluate(c, 1, nullptr, c->yout.p);
+    download_caller(c, c->yout.p, b);
+    return BEMFMM_OK;
+  });
+}
+
+int bemfmm_alpha(bemfmm_ctx* c, double* a) {
+  if (!c) return BEMFMM_E_INVALID;
+  return guarded(c, [&]() {
+    if (!c->prepared) throw HipError("bemfmm_alpha: call bemfmm_prepare first", BEMFMM_E_STATE);
+    download_caller(c, c->alpha.p, a);
+    return BEMFMM_OK;
+  });
+}
+
+int bemfmm_solve(bemfmm_ctx* c, const double* x0, double* x, double tol, int max_iter, int restart, bemfmm_result* r) {
+  if (!c) return BEMFMM_E_INVALID;
+  return guarded(c, [&]() {
+    if (!x0 || !x || !(tol > 0) || max_iter < 1) throw std::runtime_error("bemfmm_solve: invalid arguments");
+    return do_solve(c, x0, x, tol, max_iter, restart, r);
+  });
+}
+
+int bemfmm_residual_history(const bemfmm_ctx* c, double* out, int cap) {
+  if (!c) return BEMFMM_E_INVALID;
+  const int n = (int)c->history.size();
+  if (out)
+    for (int i = 0; i < std::min(n, cap); ++i) out[i] = c->history[i];
+  return n;
+}
+
+int bemfmm_precond_apply(bemfmm_ctx* c, const double* r, double* z) {
+  if (!c) return BEMFMM_E_INVALID;
+  return guarded(c, [&]() {
+    if (!c->prepared) throw HipError("bemfmm_precond_apply: call bemfmm_prepare first", BEMFMM_E_STATE);
+    std::vector<double> ri(c->n_dof);
+    host_to_internal(c, r, ri.data());
+    HIPCHECK(hipMemcpyAsync(c->xin.p, ri.data(), c->n_dof * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    precond_apply(c, c->xin.p, c->yout.p);
+    download_caller(c, c->yout.p, z);
+    return BEMFMM_OK;
+  });
+}
+
+int bemfmm_dof_order(const bemfmm_ctx* c, int32_t* order) {
+  if (!c || !order) return BEMFMM_E_INVALID;
+  if (!c->prepared) return BEMFMM_E_STATE;
+  std::copy(c->perm.begin(), c->perm.end(), order);
+  return BEMFMM_OK;
+}
+
+int bemfmm_schwarz_blocks(const bemfmm_ctx* c, int64_t* ptr, int32_t* idx, int32_t* ncore) {
+  if (!c) return BEMFMM_E_INVALID;
+  if (!c->prepared) return BEMFMM_E_STATE;
+  const auto& B = c->sw_host;
+  const int nb = B.n_blocks();
+  if (ptr) std::copy(B.ptr.begin(), B.ptr.end(), ptr);
+  if (idx)
+    for (size_t k = 0; k < B.idx.size(); ++k) idx[k] = c->perm[B.idx[k]];  // caller numbering
+  if (ncore) std::copy(B.ncore.begin(), B.ncore.end(), ncore);
+  return nb;
+}
+
+int bemfmm_refinement_history(const bemfmm_ctx* c, double* out, int cap) {
+  if (!c) return BEMFMM_E_INVALID;
+  const int n = (int)c->refine_history.size();
+  if (out)
+    for (int i = 0; i < std::min(n, cap); ++i) out[i] = c->refine_history[i];
+  return n;
+}
+
+int bemfmm_get_stats(const bemfmm_ctx* c, bemfmm_stats* s) {
+  if (!c || !s) return BEMFMM_E_INVALID;
+  if (!c->prepared) return BEMFMM_E_STATE;
+  *s = c->stats;
+  return BEMFMM_OK;
+}
+
+int bemfmm_matvec_profile(bemfmm_ctx* c, double* ms8) {
+  if (!c) return BEMFMM_E_INVALID;
+  if (!ms8) {  // turn event recording off (matvecs replay their graph again)
+    c->prof = false;
+    return BEMFMM_OK;
+  }
+  return guarded(c, [&]() {
+    if (!c->prof) {
+      for (auto& e : c->ev) HIPCHECK(hipEventCreate(&e));
+      c->prof = true;
+      for (int i = 0; i < 8; ++i) ms8[i] = 0.;
+      return BEMFMM_OK;
+    }
+    HIPCHECK(hipEventSynchronize(c->ev[8]));
+    for (int i = 0; i < 8; ++i) {
+      float ms = 0.f;
+      HIPCHECK(hipEventElapsedTime(&ms, c->ev[i], c->ev[i + 1]));
+      ms8[i] = ms;
+    }
+    return BEMFMM_OK;
+  });
+}
+
+}  // extern "C"

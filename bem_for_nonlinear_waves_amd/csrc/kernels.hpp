@@ -1,0 +1,225 @@
+// Launch interface of the device kernels (kernels.hip) for the host side of the library (bemfmm.cpp).
+//
+// Every function enqueues one kernel (or, for the tiled and templated ones, the instance that matches
+// its arguments) on the given HIP stream and returns without synchronising; launch errors surface
+// through hipGetLastError as for any HIP launch. Pointers are device memory. The host side includes
+// only this header and the HIP runtime API, so it holds no device code.
+#pragma once
+
+#include <hip/hip_runtime_api.h>
+
+#include <algorithm>
+#include <cstddef>
+#include <cstdint>
+
+#ifdef __HIPCC__
+#define BEMFMM_HD __host__ __device__
+#else
+#define BEMFMM_HD
+#endif
+
+namespace bemfmm {
+namespace dev {
+
+// ---- layout constants shared by the kernels and the host ----
+constexpr int kTP = 8;       // pairs per tile of the M2M/L2L translations: their small upper levels are
+                             // latency-bound, so more, smaller workgroups pay (209k tank, per matvec:
+                             // M2M and L2L each 0.111 ms at 32 pairs, 0.091 at 16, 0.078 at 8)
+constexpr int kTPM2L = 16;   // default pairs per M2L tile: 3 workgroups per CU (FP64) instead of 2 at 32 pairs;
+                             // measured on the 209k tank: FP64 3.61 -> 3.41 ms, FP32 2.52 -> 2.17 ms; 64 pairs: 4.6 ms
+constexpr int kXCD = 8;      // XCDs of the MI355X: workgroup id % 8 selects the XCD
+constexpr int kKG = 4;       // k-steps (of 4) per pipeline stage of the matrix-core translation
+constexpr int kRotWavePairs = 32;
+constexpr int kRotTP = 4 * kRotWavePairs;  // pairs per workgroup tile of the rotation M2L
+constexpr int kSchwarzMaxBlock = 128;      // DOFs per Schwarz block (LDS of the block build)
+constexpr int kDotBlocks = 256;            // partial sums per dot product
+
+// LDS elements of a translation tile of TP pairs: the B stage (KP x (2TP+1)) or the output transpose
+// (2TP x (RP+1))
+inline size_t translate_lds(int TP, int KP, int RP) { return (size_t)std::max(KP * (2 * TP + 1), 2 * TP * (RP + 1)); }
+
+// Values per staged M2L column. FP32: C rounded up to 32, so every column starts on a 128-byte line and the
+// staging is written and read in whole lines (at stride C = 121 the rotation M2L's 16-row stores straddled
+// lines, and its writes and the reduction's reads moved 14% and 7% more than the staged bytes); C <= 128
+// (checked in prepare) keeps it <= 128. FP64: C (each wave writes whole 968-byte columns at once: no
+// partial lines to remove, and the padding only added bytes).
+template <typename T>
+BEMFMM_HD constexpr int stage_stride(int C) { return sizeof(T) == 4 ? (C + 31) & ~31 : C; }
+
+// Device layout of the rotation-M2L factors of one offset, in float4s, every row zero-padded to whole
+// float4s (the z-rotation pairs are read as float4s; a factor row is read one float per lane, lane l taking
+// entry l & 15): (cos m phi, sin m phi) for m = 1..P; per degree n the y-rotation blocks Y_D (cos-type
+// (n+1) rows, sin-type n rows); per order m the axial block (P+1-m rows); per degree the blocks of Y_H^T.
+// Row lengths equal the block sizes.
+BEMFMM_HD constexpr int rot_q(int len) { return (len + 3) / 4; }
+BEMFMM_HD constexpr int rot_size4(int P) {
+  int s = rot_q(2 * P);
+  for (int n = 0; n <= P; ++n) s += 2 * ((n + 1) * rot_q(n + 1) + n * rot_q(n));
+  for (int m = 0; m <= P; ++m) s += (P + 1 - m) * rot_q(P + 1 - m);
+  return s;
+}
+BEMFMM_HD constexpr int rot_size(int P) { return 4 * rot_size4(P); }  // floats per offset
+// compact factors (m2l_rotation_operator, operators.cpp) -> the padded device layout
+inline void rot_pad(int P, const double* in, float* out) {
+  std::fill(out, out + rot_size(P), 0.f);
+  int io = 0, oo = 0;
+  for (int k = 0; k < 2 * P; ++k) out[oo + k] = (float)in[io + k];
+  io += 2 * P;
+  oo += 4 * rot_q(2 * P);
+  auto block = [&](int rows) {
+    const int q = 4 * rot_q(rows);
+    for (int a = 0; a < rows; ++a)
+      for (int c2 = 0; c2 < rows; ++c2) out[oo + a * q + c2] = (float)in[io + a * rows + c2];
+    io += rows * rows;
+    oo += rows * q;
+  };
+  for (int n = 0; n <= P; ++n) {
+    block(n + 1);
+    block(n);
+  }
+  for (int m = 0; m <= P; ++m) block(P + 1 - m);
+  for (int n = 0; n <= P; ++n) {
+    block(n + 1);
+    block(n);
+  }
+}
+
+// ---- argument bundles ----
+// Box-blocked dense GEMV (near field + L2P, and the P2M leaf blocks): block b has rows
+// [row_ptr[b], row_ptr[b+1]) written to row_out, columns col_idx[col_ptr[b] ..), its matrix at mat_ptr[b].
+struct Gemv {
+  const int64_t* row_ptr = nullptr;
+  const int64_t* col_ptr = nullptr;
+  const int64_t* mat_ptr = nullptr;
+  const int32_t* row_out = nullptr;
+  const int32_t* col_idx = nullptr;
+  const int32_t* ncol_dof = nullptr;  // near blocks: DOF columns before the L2P columns
+  int nblk = 0;
+};
+// Operator-grouped translation tiles: tile t applies operator slot op[t] to the pairs [p0[t], p1[t]).
+struct Tiles {
+  const int32_t* op = nullptr;
+  const int64_t* p0 = nullptr;
+  const int64_t* p1 = nullptr;
+  const int32_t* dst = nullptr;
+  const int32_t* src = nullptr;
+  int ntiles = 0;
+  int tp = kTP;  // pairs per tile
+};
+// Near-field quadrature inputs (k_near_quad)
+struct NearQuad {
+  int C = 0, q = 0;
+  double near_region = 0;
+  const int32_t* row_block = nullptr;
+  const int32_t* ncol_dof = nullptr;
+  const int64_t* range_ptr = nullptr;
+  const int64_t* ranges = nullptr;
+  const int64_t* range_prefix = nullptr;
+  const int32_t* face_perm = nullptr;
+  const double* Xs = nullptr;
+  const double* fX = nullptr;
+  const double* fcent = nullptr;
+  const double* fcr = nullptr;
+  const double* fJ = nullptr;
+  const int32_t* fv = nullptr;
+  const int32_t* fdof = nullptr;
+  const double* l2p = nullptr;
+  const double* dun_rule = nullptr;
+};
+// Resident near-field rows read by the preconditioner builds (operator entries by DOF)
+struct NearRows {
+  const int32_t* kind = nullptr;
+  const double* kappa = nullptr;
+  const int32_t* point = nullptr;
+  const int64_t* node_row = nullptr;
+  const int32_t* row_block = nullptr;
+  const int64_t* row_ptr = nullptr;
+  const int64_t* col_ptr = nullptr;
+  const int64_t* mat_ptr = nullptr;
+  const int32_t* col_idx = nullptr;
+  const int32_t* ncol_dof = nullptr;
+  const double* mat = nullptr;
+  const double* alpha = nullptr;
+};
+// P2M matrix build inputs (k_p2m_build)
+struct P2MBuild {
+  int q = 0, CP = 0, nleaves = 0;
+  const int64_t* face_ptr = nullptr;
+  const int32_t* faces = nullptr;
+  const int32_t* face_lc = nullptr;
+  const double* leaf_cs = nullptr;
+  const int64_t* mat_ptr = nullptr;
+  const int64_t* col_ptr = nullptr;
+  const double* fX = nullptr;
+  const double* fcr = nullptr;
+  const double* fJ = nullptr;
+  const double* rule = nullptr;
+  const double* cnm = nullptr;
+};
+// Restricted additive Schwarz blocks
+struct Schwarz {
+  const int64_t* bptr = nullptr;
+  const int32_t* bidx = nullptr;
+  const int32_t* bncore = nullptr;
+  const int64_t* inv_ptr = nullptr;
+  int nblk = 0;
+};
+
+// ---- matvec pipeline ----
+void fill(hipStream_t s, double* p, int64_t n, double v);
+// mode 0: unknowns = x, knowns = 0; 1: knowns (rhs); 2: rigid mode (phi = 1, phi_n = 0)
+void scatter(hipStream_t s, int mode, int n, const int32_t* kind, const double* kappa, const double* x,
+             const double* phi_k, const double* phin_k, double* phin, double* phi);
+// part 0: y = A x over all columns; 1: y = A x over the DOF columns (rounded down to 4); 2: y += A x over the rest
+void block_gemv(hipStream_t s, const Gemv& g, const double* mat, const double* x0, const double* x1, double* y0,
+                double* y1, int part);
+void block_gemv(hipStream_t s, const Gemv& g, const float* mat, const double* x0, const double* x1, double* y0,
+                double* y1, int part);
+void p2m(hipStream_t s, int C, int CP, const Gemv& g, const double* mat, const double* x0, const double* x1,
+         double* M0, double* M1);
+void p2m(hipStream_t s, int C, int CP, const Gemv& g, const float* mat, const double* x0, const double* x1,
+         double* M0, double* M1);
+// stage == nullptr: D[dst] += T S[src] (atomics; M2M, L2L); else per-pair results to stage (M2L)
+void translate(hipStream_t s, int C, int RP, int KP, const Tiles& t, const double* ops, const double* S0,
+               const double* S1, double* D0, double* D1, double* stage);
+void translate(hipStream_t s, int C, int RP, int KP, const Tiles& t, const float* ops, const double* S0,
+               const double* S1, double* D0, double* D1, float* stage);
+void pack_m32(hipStream_t s, int64_t n, int C, const double* M0, const double* M1, float* M32);
+bool m2l_rot_compiled(int P);
+// rotation M2L of the FP32 operator (throws for an expansion order it is not compiled for)
+void m2l_rot(hipStream_t s, int P, const Tiles& t, const float* rot, const float* M32, float* stage);
+void m2l_reduce(hipStream_t s, int C, int nboxes, const int32_t* order, const int64_t* tptr, const int32_t* tpairs,
+                const double* stage, double* L0, double* L1);
+void m2l_reduce(hipStream_t s, int C, int nboxes, const int32_t* order, const int64_t* tptr, const int32_t* tpairs,
+                const float* stage, double* L0, double* L1);
+void assemble(hipStream_t s, int mode, int n, const int32_t* kind, const int32_t* point, const double* alpha,
+              const double* phi, const double* y0, const double* y1, const double* phi_k, double* out);
+
+// ---- setup ----
+void morton_keys(hipStream_t s, int n, const int32_t* point, const double* X, const double bounds[6], uint64_t* keys,
+                 int32_t* vals);
+// stable LSD radix sort of (key, value) pairs on the low `bits` key bits; tmp == nullptr: only sets tmp_bytes
+hipError_t radix_sort_pairs(hipStream_t s, void* tmp, size_t& tmp_bytes, const uint64_t* keys_in, uint64_t* keys_out,
+                            const int32_t* vals_in, int32_t* vals_out, int n, int bits);
+void p2m_build(hipStream_t s, int P, const P2MBuild& a, double* mat);  // throws for P outside 1..10
+void near_quad(hipStream_t s, int64_t nrows, const NearQuad& a, const Gemv& g, double* mat);
+void to_f32(hipStream_t s, int64_t n, const double* a, float* b);
+void diag_inv(hipStream_t s, int n, const NearRows& a, double* dinv);
+void schwarz_build(hipStream_t s, const Schwarz& b, int max_block, const NearRows& a, double* inv, int32_t* failed);
+
+// ---- GMRES vectors and preconditioner apply ----
+// partial[i * kDotBlocks + blk] = sum over the block's chunk of V_i . w, i < k
+void dots(hipStream_t s, int k, const double* V, int64_t ldv, const double* w, int64_t n, double* partial);
+void reduce_partials(hipStream_t s, const double* partial, int k, double* out);
+void vsum(hipStream_t s, const double* V, int64_t ldv, int k, const double* h, double sign, double* w, int64_t n);
+void axpby(hipStream_t s, int64_t n, double a, const double* x, double b, double* y);
+void sub(hipStream_t s, int64_t n, const double* a, const double* b, double* out);
+void mul(hipStream_t s, int64_t n, const double* d, double* y);
+void mul_to(hipStream_t s, int64_t n, const double* d, const double* x, double* y);
+void scale_dup(hipStream_t s, int64_t n, double a, double* w, double* dup);
+void schwarz_apply(hipStream_t s, const Schwarz& b, const double* inv, const double* r, double* z);
+void perm_gather(hipStream_t s, int n, const int32_t* perm, const double* in, double* out);
+void perm_scatter(hipStream_t s, int n, const int32_t* perm, const double* in, double* out);
+
+}  // namespace dev
+}  // namespace bemfmm

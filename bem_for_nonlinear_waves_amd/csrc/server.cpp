@@ -19,6 +19,7 @@
 #include <map>
 #include <memory>
 #include <mutex>
+#include <set>
 #include <string>
 #include <thread>
 #include <vector>
@@ -61,7 +62,15 @@ struct Session {
   int64_t pending = 0;    // requests queued or running
 };
 
-bool same_opts(const bemfmm_opts& a, const bemfmm_opts& b) { return std::memcmp(&a, &b, sizeof(a)) == 0; }
+// field by field: the structure has padding bytes, which a byte comparison would include
+bool same_opts(const bemfmm_opts& a, const bemfmm_opts& b) {
+  return a.p == b.p && a.theta == b.theta && a.leaf_max == b.leaf_max && a.max_level == b.max_level && a.q == b.q &&
+         a.precision == b.precision && a.preconditioner == b.preconditioner && a.refine_eta == b.refine_eta &&
+         a.schwarz_max_core == b.schwarz_max_core && a.schwarz_max_block == b.schwarz_max_block &&
+         a.schwarz_overlap == b.schwarz_overlap && a.m2l_tile_pairs == b.m2l_tile_pairs &&
+         a.dof_reorder == b.dof_reorder && a.use_graphs == b.use_graphs && a.m2l_kernel == b.m2l_kernel &&
+         a.overlap_near == b.overlap_near;
+}
 
 }  // namespace
 
@@ -71,6 +80,7 @@ struct bemfmm_server {
   std::condition_variable cv_work, cv_done;
   std::deque<std::shared_ptr<Request>> queue;
   std::map<int64_t, std::shared_ptr<Request>> finished;
+  std::set<int64_t> collected;  // tickets already returned by bemfmm_server_wait
   std::map<int64_t, Session> sessions;
   std::vector<std::thread> workers;
   double used_bytes = 0, peak_bytes = 0;
@@ -131,7 +141,17 @@ struct bemfmm_server {
     bemfmm_ctx* c = s ? s->ctx : nullptr;
     bool fresh = false;
     if (!c || !same_opts(s->opts, r.opts)) {
-      if (c) bemfmm_ctx_destroy(c);
+      if (c) {
+        bemfmm_ctx_destroy(c);
+        c = nullptr;
+        if (s) {  // the session holds no context (and no device memory) until the new one exists
+          s->ctx = nullptr;
+          s->X.clear();
+          std::lock_guard<std::mutex> lk(mu);
+          used_bytes -= s->bytes;
+          s->bytes = 0;
+        }
+      }
       c = bemfmm_ctx_create(o.device, &r.opts);
       if (!c) return fail(nullptr, BEMFMM_E_DEVICE);
       fresh = true;
@@ -318,7 +338,9 @@ int bemfmm_server_wait(bemfmm_server* s, int64_t ticket, double* x, bemfmm_resul
   std::shared_ptr<Request> r;
   {
     std::unique_lock<std::mutex> lk(s->mu);
-    if (ticket >= s->next_ticket) return BEMFMM_E_INVALID;
+    // unknown tickets and tickets already collected would never finish
+    if (ticket >= s->next_ticket || s->collected.count(ticket)) return BEMFMM_E_INVALID;
+    s->collected.insert(ticket);
     s->cv_done.wait(lk, [&] { return s->finished.count(ticket) > 0; });
     r = s->finished[ticket];
     s->finished.erase(ticket);

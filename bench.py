@@ -240,12 +240,6 @@ def run_batch(args):
     """64 independent tank BVPs (N log-uniform in [2k, 40k]) submitted at once to the request loop;
     every request includes its setup (new geometry). Also timed one at a time (one slot)."""
     from bem_for_nonlinear_waves_amd.serve import BVPServer
-    if not args.serve_anyway:
-        # like --mode serve: FP32+refinement requests on 8 worker threads ended with a segmentation fault of
-        # the process in the current build (intermittently: one earlier run passed); the cause has not been
-        # found, so the mode does not run until it is
-        sys.exit("bench.py --mode batch is disabled in this build: a run ended with a segmentation fault whose "
-                 "cause is not yet found (see the serving section of the design notes)")
     rng = np.random.default_rng(4)
     t = time.perf_counter()
     sizes = np.exp(rng.uniform(np.log(2000), np.log(40000), args.batch)).astype(int)
@@ -306,12 +300,6 @@ def run_serve(args):
 
     from bem_for_nonlinear_waves_amd import mesh as M
     from bem_for_nonlinear_waves_amd.serve import BVPServer
-    if not args.serve_anyway:
-        # this workload (8 concurrent FP32+refinement sessions that re-prepare their contexts between stages)
-        # ended with a segmentation fault of the process in the current build; the cause has not been found,
-        # so the mode does not run until it is
-        sys.exit("bench.py --mode serve is disabled in this build: its last run ended with a segmentation fault "
-                 "whose cause is not yet found (see the serving section of the design notes)")
     rng = np.random.default_rng(5)
     t = time.perf_counter()
     sizes = rng.uniform(args.serve_min_n, args.serve_max_n, args.sessions).astype(int)
@@ -392,7 +380,6 @@ def main():
     ap.add_argument("--precision", choices=["fp32_refine", "fp64"], default="fp32_refine")
     ap.add_argument("--refine-eta", type=float, default=3e-7)
     ap.add_argument("--m2l-tile-pairs", type=int, default=0, choices=[0, 16, 32, 64])
-    ap.add_argument("--serve-anyway", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--case", default=None,
                     help="solve a case in the reference's input format (path to its settings.json) instead of the cylinder tank")
     # Schwarz blocks: small cores with many overlap rings (blocks still <= 128 DOFs) measured best on the
@@ -412,10 +399,10 @@ def main():
     ap.add_argument("--batch", type=int, default=64)
     ap.add_argument("--no-sequential", action="store_true", help="batch: skip the one-slot comparison")
     ap.add_argument("--sessions", type=int, default=8)
-    ap.add_argument("--stages", type=int, default=8)
+    ap.add_argument("--stages", type=int, default=40, help="serve: RK4 stages (BVPs) per session")
     ap.add_argument("--rate", type=float, default=50.0, help="serve: session arrivals per second")
     ap.add_argument("--serve-min-n", type=int, default=5000)
-    ap.add_argument("--serve-max-n", type=int, default=60000)
+    ap.add_argument("--serve-max-n", type=int, default=200000)
     ap.add_argument("--profile-only", action="store_true",
                     help="setup + a 256 MiB device copy (counter calibration) + the timed matvecs; no solves")
     args = ap.parse_args()

@@ -1,0 +1,673 @@
+// Test-only CPU implementations of the device launchers (bem_for_nonlinear_waves_amd/csrc/kernels.hpp) for
+// the host-stub build: each launcher enqueues, on the emulated stream, a serial loop that computes what the
+// kernel of kernels.hip computes (same indexing, same operand layouts, FP32 where the kernel rounds to
+// FP32). Used only with hipstub.cpp, so that the library's host side runs against real results on a
+// machine without a GPU.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+#include <stdexcept>
+#include <vector>
+
+#include "../../bem_for_nonlinear_waves_amd/csrc/kernels.hpp"
+#include "../../bem_for_nonlinear_waves_amd/csrc/nearquad.hpp"
+#include "hipstub.hpp"
+
+namespace bemfmm {
+namespace dev {
+
+namespace {
+
+// GW5x5 product rule on the unit square (t0, t1, w), as the near-field quadrature kernel uses it
+const double kGW5[25][3] = {
+    {0.046910077030668, 0.046910077030668, 0.0140335872156063}, {0.046910077030668, 0.230765344947158, 0.0283499999999991},
+    {0.046910077030668, 0.5, 0.0336962680968792},               {0.046910077030668, 0.769234655052841, 0.0283499999999991},
+    {0.046910077030668, 0.953089922969332, 0.0140335872156063}, {0.230765344947158, 0.046910077030668, 0.0283499999999991},
+    {0.230765344947158, 0.230765344947158, 0.0572713510559978}, {0.230765344947158, 0.5, 0.0680716331376877},
+    {0.230765344947158, 0.769234655052841, 0.0572713510559978}, {0.230765344947158, 0.953089922969332, 0.0283499999999991},
+    {0.5, 0.046910077030668, 0.0336962680968792},               {0.5, 0.230765344947158, 0.0680716331376877},
+    {0.5, 0.5, 0.0809086419753086},                             {0.5, 0.769234655052841, 0.0680716331376877},
+    {0.5, 0.953089922969332, 0.0336962680968792},               {0.769234655052841, 0.046910077030668, 0.0283499999999991},
+    {0.769234655052841, 0.230765344947158, 0.0572713510559978}, {0.769234655052841, 0.5, 0.0680716331376877},
+    {0.769234655052841, 0.769234655052841, 0.0572713510559978}, {0.769234655052841, 0.953089922969332, 0.0283499999999991},
+    {0.953089922969332, 0.046910077030668, 0.0140335872156063}, {0.953089922969332, 0.230765344947158, 0.0283499999999991},
+    {0.953089922969332, 0.5, 0.0336962680968792},               {0.953089922969332, 0.769234655052841, 0.0283499999999991},
+    {0.953089922969332, 0.953089922969332, 0.0140335872156063}};
+
+template <typename T>
+void gemv_cpu(Gemv g, const T* mat, const double* x0, const double* x1, double* y0, double* y1, int part) {
+  for (int b = 0; b < g.nblk; ++b) {
+    const int64_t r0 = g.row_ptr[b];
+    const int nrow = (int)(g.row_ptr[b + 1] - r0);
+    const int64_t c0 = g.col_ptr[b];
+    const int ncol = (int)(g.col_ptr[b + 1] - c0);
+    const int split = part == 0 ? 0 : (g.ncol_dof[b] & ~3);
+    const int cb = part == 2 ? split : 0, ce = part == 1 ? split : ncol;
+    for (int r = 0; r < nrow; ++r) {
+      const T* A0 = mat + g.mat_ptr[b] + (int64_t)r * 2 * ncol;
+      const T* A1 = A0 + ncol;
+      double s0 = 0., s1 = 0.;
+      for (int c = cb; c < ce; ++c) {
+        const int j = g.col_idx[c0 + c];
+        s0 += (double)A0[c] * x0[j];
+        s1 += (double)A1[c] * x1[j];
+      }
+      const int o = g.row_out[r0 + r];
+      if (part == 2) {
+        y0[o] += s0;
+        y1[o] += s1;
+      } else {
+        y0[o] = s0;
+        y1[o] = s1;
+      }
+    }
+  }
+}
+
+template <typename T>
+void p2m_cpu(int C, int CP, Gemv g, const T* mat, const double* x0, const double* x1, double* M0, double* M1) {
+  for (int b = 0; b < g.nblk; ++b) {
+    const int64_t c0 = g.col_ptr[b];
+    const int ncol = (int)(g.col_ptr[b + 1] - c0);
+    const T* A = mat + g.mat_ptr[b];
+    const int64_t base = g.row_out[g.row_ptr[b]];
+    for (int layer = 0; layer < 2; ++layer) {
+      const double* x = layer ? x1 : x0;
+      double* out = layer ? M1 : M0;
+      for (int r = 0; r < C; ++r) {
+        double s = 0.;
+        for (int c = 0; c < ncol; ++c) s += (double)A[(int64_t)(layer * ncol + c) * CP + r] * x[g.col_idx[c0 + c]];
+        out[base + r] = s;
+      }
+    }
+  }
+}
+
+template <typename T>
+void translate_cpu(int C, int RP, int KP, Tiles t, const T* ops, const double* S0, const double* S1, double* D0,
+                   double* D1, T* stage) {
+  std::vector<T> in(C);
+  for (int k = 0; k < t.ntiles; ++k) {
+    const T* O = ops + (int64_t)t.op[k] * KP * RP;
+    for (int64_t q = t.p0[k]; q < t.p1[k]; ++q)
+      for (int layer = 0; layer < 2; ++layer) {
+        const double* S = (layer ? S1 : S0) + (int64_t)t.src[q] * C;
+        for (int i = 0; i < C; ++i) in[i] = (T)S[i];
+        if (stage) {
+          const int CS = stage_stride<T>(C);
+          T* out = stage + (q * 2 + layer) * (int64_t)CS;
+          for (int r = 0; r < CS; ++r) {
+            T a = 0;
+            if (r < C)
+              for (int i = 0; i < C; ++i) a += O[(int64_t)i * RP + r] * in[i];
+            out[r] = a;
+          }
+        } else {
+          double* D = (layer ? D1 : D0) + (int64_t)t.dst[q] * C;
+          for (int r = 0; r < C; ++r) {
+            T a = 0;
+            for (int i = 0; i < C; ++i) a += O[(int64_t)i * RP + r] * in[i];
+            D[r] += (double)a;
+          }
+        }
+      }
+  }
+}
+
+constexpr int rre(int n, int m) { return m == 0 ? n * n : n * n + 2 * m - 1; }
+constexpr int rim(int n, int m) { return n * n + 2 * m; }
+
+// v <- H^T T(|o| z) D v with the padded factors R of one offset (kernels.hip: rot_apply), FP32
+void rot_apply_cpu(int P, const float* R, float* v) {
+  const float* zc = R;
+  int off = 4 * rot_q(2 * P);
+  auto zrot = [&](int n, bool transposed) {
+    for (int m = 1; m <= n; ++m) {
+      const float c = zc[2 * (m - 1)], s = zc[2 * (m - 1) + 1];
+      const float re = v[rre(n, m)], im = v[rim(n, m)];
+      if (transposed) {
+        v[rre(n, m)] = std::fma(c, re, s * im);
+        v[rim(n, m)] = std::fma(c, im, -s * re);
+      } else {
+        v[rre(n, m)] = std::fma(c, re, -s * im);
+        v[rim(n, m)] = std::fma(c, im, s * re);
+      }
+    }
+  };
+  // cos-type (kind 0: Re m = 0..n) and sin-type (kind 1: Im m = 1..n) lists of degree n
+  auto idx = [](int n, int kind, int i) { return kind == 0 ? rre(n, i) : rim(n, i + 1); };
+  auto yblocks = [&](int n) {
+    for (int kind = 0; kind < 2; ++kind) {
+      const int w = kind == 0 ? n + 1 : n, q = 4 * rot_q(w);
+      std::vector<float> out(w);
+      for (int a = 0; a < w; ++a) {
+        float acc = 0.f;
+        for (int i = 0; i < w; ++i) acc = std::fma(R[off + a * q + i], v[idx(n, kind, i)], acc);
+        out[a] = acc;
+      }
+      for (int a = 0; a < w; ++a) v[idx(n, kind, a)] = out[a];
+      off += w * q;
+    }
+  };
+  for (int n = 0; n <= P; ++n) {
+    if (n > 0) zrot(n, false);
+    yblocks(n);
+  }
+  for (int m = 0; m <= P; ++m) {
+    const int w = P + 1 - m, q = 4 * rot_q(w);
+    std::vector<float> orr(w), oi(w);
+    for (int a = 0; a < w; ++a) {
+      float sr = 0.f, si = 0.f;
+      for (int i = 0; i < w; ++i) {
+        sr = std::fma(R[off + a * q + i], v[rre(m + i, m)], sr);
+        if (m > 0) si = std::fma(R[off + a * q + i], v[rim(m + i, m)], si);
+      }
+      orr[a] = sr;
+      oi[a] = si;
+    }
+    for (int a = 0; a < w; ++a) {
+      v[rre(m + a, m)] = orr[a];
+      if (m > 0) v[rim(m + a, m)] = oi[a];
+    }
+    off += w * q;
+  }
+  for (int n = 0; n <= P; ++n) {
+    yblocks(n);
+    if (n > 0) zrot(n, true);
+  }
+}
+
+uint64_t spread3(uint32_t v) {
+  uint64_t x = v & 0x1fffffu;
+  x = (x | (x << 32)) & 0x1f00000000ffffull;
+  x = (x | (x << 16)) & 0x1f0000ff0000ffull;
+  x = (x | (x << 8)) & 0x100f00f00f00f00full;
+  x = (x | (x << 4)) & 0x10c30c30c30c30c3ull;
+  x = (x | (x << 2)) & 0x1249249249249249ull;
+  return x;
+}
+
+double near_entry(int i, int j, const NearRows& a) {
+  const int ki = a.kind[i];
+  if (ki == 2) return i == j ? 1.0 : 0.0;
+  const int64_t r = a.node_row[a.point[i]];
+  const int b = a.row_block[r];
+  const int64_t c0 = a.col_ptr[b];
+  const int ncol = (int)(a.col_ptr[b + 1] - c0);
+  const int ncd = a.ncol_dof[b];
+  int lo = 0, hi = ncd - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (a.col_idx[c0 + mid] < j) lo = mid + 1;
+    else hi = mid;
+  }
+  double v = 0.;
+  if (ncd > 0 && a.col_idx[c0 + lo] == j) {
+    const double* A0 = a.mat + a.mat_ptr[b] + (r - a.row_ptr[b]) * 2 * ncol;
+    v = (a.kind[j] == 0) ? A0[lo] : -A0[ncol + lo];
+    if (a.kind[j] == 3) v += a.kappa[j] * A0[lo];
+  }
+  if (i == j && (ki == 1 || ki == 3)) v -= a.alpha[i];
+  return v;
+}
+
+}  // namespace
+
+void fill(hipStream_t s, double* p, int64_t n, double v) {
+  hipstub::enqueue(s, [=]() { std::fill(p, p + n, v); });
+}
+
+void scatter(hipStream_t s, int mode, int n, const int32_t* kind, const double* kappa, const double* x,
+             const double* phi_k, const double* phin_k, double* phin, double* phi) {
+  hipstub::enqueue(s, [=]() {
+    for (int i = 0; i < n; ++i) {
+      const bool dir = kind[i] == 0;
+      double a = 0., b = 0.;
+      if (mode == 0) {
+        if (dir) a = x[i];
+        else {
+          b = x[i];
+          if (kind[i] == 3) a = kappa[i] * x[i];
+        }
+      } else if (mode == 1) {
+        if (dir) b = phi_k[i];
+        else a = phin_k[i];
+      } else {
+        b = 1.0;
+      }
+      phin[i] = a;
+      phi[i] = b;
+    }
+  });
+}
+
+void block_gemv(hipStream_t s, const Gemv& g, const double* mat, const double* x0, const double* x1, double* y0,
+                double* y1, int part) {
+  const Gemv gg = g;
+  hipstub::enqueue(s, [=]() { gemv_cpu<double>(gg, mat, x0, x1, y0, y1, part); });
+}
+void block_gemv(hipStream_t s, const Gemv& g, const float* mat, const double* x0, const double* x1, double* y0,
+                double* y1, int part) {
+  const Gemv gg = g;
+  hipstub::enqueue(s, [=]() { gemv_cpu<float>(gg, mat, x0, x1, y0, y1, part); });
+}
+void p2m(hipStream_t s, int C, int CP, const Gemv& g, const double* mat, const double* x0, const double* x1,
+         double* M0, double* M1) {
+  const Gemv gg = g;
+  hipstub::enqueue(s, [=]() { p2m_cpu<double>(C, CP, gg, mat, x0, x1, M0, M1); });
+}
+void p2m(hipStream_t s, int C, int CP, const Gemv& g, const float* mat, const double* x0, const double* x1,
+         double* M0, double* M1) {
+  const Gemv gg = g;
+  hipstub::enqueue(s, [=]() { p2m_cpu<float>(C, CP, gg, mat, x0, x1, M0, M1); });
+}
+void translate(hipStream_t s, int C, int RP, int KP, const Tiles& t, const double* ops, const double* S0,
+               const double* S1, double* D0, double* D1, double* stage) {
+  const Tiles tt = t;
+  hipstub::enqueue(s, [=]() { translate_cpu<double>(C, RP, KP, tt, ops, S0, S1, D0, D1, stage); });
+}
+void translate(hipStream_t s, int C, int RP, int KP, const Tiles& t, const float* ops, const double* S0,
+               const double* S1, double* D0, double* D1, float* stage) {
+  const Tiles tt = t;
+  hipstub::enqueue(s, [=]() { translate_cpu<float>(C, RP, KP, tt, ops, S0, S1, D0, D1, stage); });
+}
+void pack_m32(hipStream_t s, int64_t n, int C, const double* M0, const double* M1, float* M32) {
+  hipstub::enqueue(s, [=]() {
+    for (int64_t i = 0; i < n; ++i) {
+      const int64_t col = i / C, r = i - col * C;
+      M32[i] = (float)((col & 1) ? M1 : M0)[(col >> 1) * C + r];
+    }
+  });
+}
+bool m2l_rot_compiled(int P) { return P == 8 || P == 10; }
+void m2l_rot(hipStream_t s, int P, const Tiles& t, const float* rot, const float* M32, float* stage) {
+  if (!m2l_rot_compiled(P)) throw std::runtime_error("rotation M2L not compiled for this expansion order");
+  const Tiles tt = t;
+  hipstub::enqueue(s, [=]() {
+    const int C = (P + 1) * (P + 1), CS = stage_stride<float>(C), RS = rot_size(P);
+    std::vector<float> v(C);
+    for (int k = 0; k < tt.ntiles; ++k)
+      for (int64_t q = tt.p0[k]; q < tt.p1[k]; ++q)
+        for (int layer = 0; layer < 2; ++layer) {
+          const float* S = M32 + ((int64_t)tt.src[q] * 2 + layer) * C;
+          std::copy(S, S + C, v.begin());
+          rot_apply_cpu(P, rot + (int64_t)tt.op[k] * RS, v.data());
+          float* out = stage + (q * 2 + layer) * (int64_t)CS;
+          for (int r = 0; r < CS; ++r) out[r] = r < C ? v[r] : 0.f;
+        }
+  });
+}
+template <typename T>
+void reduce_cpu(int C, int nboxes, const int32_t* order, const int64_t* tptr, const int32_t* tpairs, const T* stage,
+                double* L0, double* L1) {
+  const int CS = stage_stride<T>(C);
+  for (int k = 0; k < nboxes; ++k) {
+    const int t = order[k];
+    for (int j = 0; j < 2 * C; ++j) {
+      const int jj = j < C ? j : CS + j - C;
+      double s = 0.;
+      for (int64_t q = tptr[t]; q < tptr[t + 1]; ++q) s += (double)stage[(int64_t)tpairs[q] * 2 * CS + jj];
+      if (j < C) L0[(int64_t)t * C + j] = s;
+      else L1[(int64_t)t * C + j - C] = s;
+    }
+  }
+}
+void m2l_reduce(hipStream_t s, int C, int nboxes, const int32_t* order, const int64_t* tptr, const int32_t* tpairs,
+                const double* stage, double* L0, double* L1) {
+  hipstub::enqueue(s, [=]() { reduce_cpu<double>(C, nboxes, order, tptr, tpairs, stage, L0, L1); });
+}
+void m2l_reduce(hipStream_t s, int C, int nboxes, const int32_t* order, const int64_t* tptr, const int32_t* tpairs,
+                const float* stage, double* L0, double* L1) {
+  hipstub::enqueue(s, [=]() { reduce_cpu<float>(C, nboxes, order, tptr, tpairs, stage, L0, L1); });
+}
+void assemble(hipStream_t s, int mode, int n, const int32_t* kind, const int32_t* point, const double* alpha,
+              const double* phi, const double* y0, const double* y1, const double* phi_k, double* out) {
+  hipstub::enqueue(s, [=]() {
+    for (int i = 0; i < n; ++i) {
+      const int p = point[i];
+      if (mode == 2) {
+        double a = -y1[p];
+        if (std::fabs(a) < 1e-15) a = 1.0;
+        out[i] = a;
+        continue;
+      }
+      double v = (kind[i] == 2) ? phi[i] : y0[p] - (y1[p] + alpha[i] * phi[i]);
+      if (mode == 1) v = (kind[i] == 2) ? phi_k[i] : -v;
+      out[i] = v;
+    }
+  });
+}
+
+void morton_keys(hipStream_t s, int n, const int32_t* point, const double* X, const double b[6], uint64_t* keys,
+                 int32_t* vals) {
+  const double x0 = b[0], x1 = b[1], y0 = b[2], y1 = b[3], z0 = b[4], z1 = b[5];
+  hipstub::enqueue(s, [=]() {
+    auto q = [](double v, double a, double c) -> uint32_t {
+      if (!(c > a)) return 0u;
+      double t = (v - a) / (c - a);
+      t = std::fmin(std::fmax(t, 0.0), 1.0);
+      return (uint32_t)(t * (double)((1u << 21) - 1u) + 0.5);
+    };
+    for (int i = 0; i < n; ++i) {
+      const int p = point[i];
+      keys[i] = spread3(q(X[3 * p], x0, x1)) | (spread3(q(X[3 * p + 1], y0, y1)) << 1) |
+                (spread3(q(X[3 * p + 2], z0, z1)) << 2);
+      vals[i] = i;
+    }
+  });
+}
+hipError_t radix_sort_pairs(hipStream_t s, void* tmp, size_t& tmp_bytes, const uint64_t* keys_in, uint64_t* keys_out,
+                            const int32_t* vals_in, int32_t* vals_out, int n, int bits) {
+  if (!tmp) {
+    tmp_bytes = 256;
+    return hipSuccess;
+  }
+  hipstub::enqueue(s, [=]() {
+    const uint64_t mask = bits >= 64 ? ~0ull : ((1ull << bits) - 1);
+    std::vector<int> o(n);
+    std::iota(o.begin(), o.end(), 0);
+    std::stable_sort(o.begin(), o.end(), [&](int a, int b) { return (keys_in[a] & mask) < (keys_in[b] & mask); });
+    for (int i = 0; i < n; ++i) {
+      keys_out[i] = keys_in[o[i]];
+      vals_out[i] = vals_in[o[i]];
+    }
+  });
+  return hipSuccess;
+}
+
+void p2m_build(hipStream_t s, int P, const P2MBuild& a, double* mat) {
+  if (P < 1 || P > 10) throw std::runtime_error("expansion order outside 1..10 for the P2M build");
+  const P2MBuild A = a;
+  hipstub::enqueue(s, [=]() {
+    const int C = (P + 1) * (P + 1), NT = (P + 1) * (P + 2) / 2;
+    auto tn = [](int n, int m) { return n * (n + 1) / 2 + m; };
+    std::vector<double> re(NT), im(NT);
+    for (int l = 0; l < A.nleaves; ++l) {
+      const int ncol = (int)(A.col_ptr[l + 1] - A.col_ptr[l]);
+      double* M = mat + A.mat_ptr[l];
+      const double cx = A.leaf_cs[4 * l], cy = A.leaf_cs[4 * l + 1], cz = A.leaf_cs[4 * l + 2], sc = A.leaf_cs[4 * l + 3];
+      for (int layer = 0; layer < 2; ++layer)
+        for (int c = 0; c < ncol; ++c)
+          for (int r = 0; r < C; ++r) M[(int64_t)(layer * ncol + c) * A.CP + r] = 0.;
+      for (int64_t fi = A.face_ptr[l]; fi < A.face_ptr[l + 1]; ++fi) {
+        const int f = A.faces[fi];
+        for (int j = 0; j < A.q; ++j) {
+          const double b0 = A.rule[4 * j], b1 = A.rule[4 * j + 1], b2 = A.rule[4 * j + 2], w = A.rule[4 * j + 3];
+          const double J = A.fJ[f];
+          const double* X = A.fX + 9 * (int64_t)f;
+          const double x = (b0 * X[0] + b1 * X[3] + b2 * X[6] - cx) / sc;
+          const double y = (b0 * X[1] + b1 * X[4] + b2 * X[7] - cy) / sc;
+          const double z = (b0 * X[2] + b1 * X[5] + b2 * X[8] - cz) / sc;
+          const double W = J * w;
+          const double pw[3] = {W * b0, W * b1, W * b2};
+          const double pn[3] = {A.fcr[3 * f] / J, A.fcr[3 * f + 1] / J, A.fcr[3 * f + 2] / J};
+          const int32_t* plc = A.face_lc + 3 * fi;
+          const double r2 = x * x + y * y + z * z;
+          re[0] = 1.;
+          im[0] = 0.;
+          for (int m = 1; m <= P; ++m) {
+            const double pr = re[tn(m - 1, m - 1)], pi = im[tn(m - 1, m - 1)], s2 = 1. / (2. * m);
+            re[tn(m, m)] = s2 * (x * pr - y * pi);
+            im[tn(m, m)] = s2 * (x * pi + y * pr);
+          }
+          for (int m = 0; m <= P; ++m)
+            for (int n = m + 1; n <= P; ++n) {
+              double ar = (2 * n - 1) * z * re[tn(n - 1, m)], ai = (2 * n - 1) * z * im[tn(n - 1, m)];
+              if (n - 2 >= m) {
+                ar -= r2 * re[tn(n - 2, m)];
+                ai -= r2 * im[tn(n - 2, m)];
+              }
+              const double s2 = 1. / ((double)(n + m) * (n - m));
+              re[tn(n, m)] = s2 * ar;
+              im[tn(n, m)] = s2 * ai;
+            }
+          auto nr = [&](int n, int qq, double& rr, double& ii) {
+            if (n < 0 || qq > n || -qq > n) {
+              rr = ii = 0.;
+              return;
+            }
+            if (qq >= 0) {
+              rr = re[tn(n, qq)];
+              ii = im[tn(n, qq)];
+            } else {
+              const double sg = ((-qq) & 1) ? -1. : 1.;
+              rr = sg * re[tn(n, -qq)];
+              ii = -sg * im[tn(n, -qq)];
+            }
+          };
+          for (int layer = 0; layer < 2; ++layer)
+            for (int row = 0; row < C; ++row) {
+              int rn = 0;
+              while ((rn + 1) * (rn + 1) <= row) ++rn;
+              const int kk = row - rn * rn, rm = (kk + 1) / 2, isim = (kk > 0 && (kk & 1) == 0) ? 1 : 0;
+              const double c = A.cnm[tn(rn, rm)];
+              double v;
+              if (layer == 0) {
+                v = isim ? -c * im[tn(rn, rm)] : c * re[tn(rn, rm)];
+              } else if (rn == 0) {
+                v = 0.;
+              } else {
+                double p1r, p1i, m1r, m1i, z0r, z0i;
+                nr(rn - 1, rm + 1, p1r, p1i);
+                nr(rn - 1, rm - 1, m1r, m1i);
+                nr(rn - 1, rm, z0r, z0i);
+                const double ar = 0.5 * pn[0], ai = -0.5 * pn[1], br = 0.5 * pn[0], bi = 0.5 * pn[1];
+                const double dr = -(ar * p1r - ai * p1i) + (br * m1r - bi * m1i) + pn[2] * z0r;
+                const double di = -(ar * p1i + ai * p1r) + (br * m1i + bi * m1r) + pn[2] * z0i;
+                v = isim ? -(c / sc) * di : (c / sc) * dr;
+              }
+              for (int k = 0; k < 3; ++k) M[(int64_t)(layer * ncol + plc[k]) * A.CP + row] += pw[k] * v;
+            }
+        }
+      }
+    }
+  });
+}
+
+void near_quad(hipStream_t s, int64_t nrows, const NearQuad& a, const Gemv& g, double* mat) {
+  const NearQuad A = a;
+  const Gemv G = g;
+  hipstub::enqueue(s, [=]() {
+    double dun[7][4];
+    for (int i = 0; i < 28; ++i) dun[i / 4][i % 4] = A.dun_rule[i];
+    for (int64_t r = 0; r < nrows; ++r) {
+      const int b = A.row_block[r];
+      const int ncd = A.ncol_dof[b];
+      const int64_t c0 = G.col_ptr[b];
+      const int ncol = (int)(G.col_ptr[b + 1] - c0);
+      double* A0 = mat + G.mat_ptr[b] + (r - G.row_ptr[b]) * 2 * ncol;
+      double* A1 = A0 + ncol;
+      const int t = G.row_out[r];
+      for (int c = 0; c < A.C; ++c) {
+        A0[ncd + c] = A.l2p[r * A.C + c];
+        A1[ncd + c] = A.l2p[r * A.C + c];
+      }
+      const double x = A.Xs[3 * t], y = A.Xs[3 * t + 1], z = A.Xs[3 * t + 2];
+      const int32_t* cols = G.col_idx + c0;
+      for (int64_t gi = A.range_ptr[b]; gi < A.range_ptr[b + 1]; ++gi)
+        for (int64_t fi = A.ranges[2 * gi]; fi < A.ranges[2 * gi + 1]; ++fi) {
+          const int f = A.face_perm[fi];
+          int slot = -1;
+          for (int k = 0; k < 3; ++k)
+            if (A.fv[3 * f + k] == t) slot = k;
+          double WG[3], WGn[3];
+          bemfmm::face_weights(A.fX + 9 * (int64_t)f, A.fJ[f], A.fcr + 3 * (int64_t)f, A.fcent + 3 * (int64_t)f, slot, x, y,
+                               z, A.near_region, kGW5, dun, A.q, WG, WGn);
+          for (int k = 0; k < 3; ++k) {
+            const int d = A.fdof[3 * f + k];
+            int lo2 = 0, hi2 = ncd - 1;
+            while (lo2 < hi2) {
+              const int mid = (lo2 + hi2) >> 1;
+              if (cols[mid] < d) lo2 = mid + 1;
+              else hi2 = mid;
+            }
+            A0[lo2] += WG[k];
+            A1[lo2] += WGn[k];
+          }
+        }
+    }
+  });
+}
+
+void to_f32(hipStream_t s, int64_t n, const double* a, float* b) {
+  hipstub::enqueue(s, [=]() {
+    for (int64_t i = 0; i < n; ++i) b[i] = (float)a[i];
+  });
+}
+
+void diag_inv(hipStream_t s, int n, const NearRows& a, double* dinv) {
+  const NearRows A = a;
+  hipstub::enqueue(s, [=]() {
+    for (int i = 0; i < n; ++i) {
+      const double dg = near_entry(i, i, A);  // CORNER rows: 1
+      dinv[i] = (std::isfinite(dg) && std::fabs(dg) > 1e-15) ? 1.0 / dg : 1.0;
+    }
+  });
+}
+
+void schwarz_build(hipStream_t s, const Schwarz& b, int max_block, const NearRows& a, double* inv, int32_t* failed) {
+  const Schwarz B = b;
+  const NearRows A = a;
+  (void)max_block;
+  hipstub::enqueue(s, [=]() {
+    for (int blk = 0; blk < B.nblk; ++blk) {
+      const int64_t p0 = B.bptr[blk];
+      const int m = (int)(B.bptr[blk + 1] - p0), nc = B.bncore[blk];
+      const int32_t* ids = B.bidx + p0;
+      std::vector<double> M((size_t)m * m), colk(m);
+      std::vector<int> piv(m);
+      for (int e = 0; e < m * m; ++e) M[e] = near_entry(ids[e / m], ids[e % m], A);
+      bool bad = false;
+      for (int k = 0; k < m && !bad; ++k) {
+        double best = -1.0;
+        int bi = k;
+        for (int i = k; i < m; ++i)
+          if (std::fabs(M[i * m + k]) > best) {
+            best = std::fabs(M[i * m + k]);
+            bi = i;
+          }
+        piv[k] = bi;
+        if (bi != k)
+          for (int j = 0; j < m; ++j) std::swap(M[k * m + j], M[bi * m + j]);
+        const double pv = M[k * m + k];
+        if (!(std::fabs(pv) > 1e-300) || !std::isfinite(pv)) {
+          bad = true;
+          break;
+        }
+        const double ipv = 1.0 / pv;
+        for (int i = 0; i < m; ++i) colk[i] = M[i * m + k];
+        for (int j = 0; j < m; ++j) M[k * m + j] = (j == k) ? ipv : M[k * m + j] * ipv;
+        for (int i = 0; i < m; ++i) {
+          if (i == k) continue;
+          for (int j = 0; j < m; ++j) M[i * m + j] = (j == k) ? -colk[i] * ipv : M[i * m + j] - colk[i] * M[k * m + j];
+        }
+      }
+      double* out = inv + B.inv_ptr[blk];
+      if (bad) {
+        failed[blk] = 1;
+        for (int e = 0; e < nc * m; ++e) {
+          const int i = e / m, j = e % m;
+          double v = 0.;
+          if (i == j) {
+            const double d = near_entry(ids[i], ids[i], A);
+            v = (std::isfinite(d) && std::fabs(d) > 1e-15) ? 1.0 / d : 1.0;
+          }
+          out[e] = v;
+        }
+        continue;
+      }
+      failed[blk] = 0;
+      for (int k = m - 1; k >= 0; --k)
+        if (piv[k] != k)
+          for (int i = 0; i < m; ++i) std::swap(M[i * m + k], M[i * m + piv[k]]);
+      for (int e = 0; e < nc * m; ++e) out[e] = M[e];
+    }
+  });
+}
+
+void dots(hipStream_t s, int k, const double* V, int64_t ldv, const double* w, int64_t n, double* partial) {
+  hipstub::enqueue(s, [=]() {
+    for (int i = 0; i < k; ++i) {
+      double acc = 0.;
+      for (int64_t j = 0; j < n; ++j) acc += V[i * ldv + j] * w[j];
+      partial[(int64_t)i * kDotBlocks] = acc;
+      for (int t = 1; t < kDotBlocks; ++t) partial[(int64_t)i * kDotBlocks + t] = 0.;
+    }
+  });
+}
+void reduce_partials(hipStream_t s, const double* partial, int k, double* out) {
+  hipstub::enqueue(s, [=]() {
+    for (int i = 0; i < k; ++i) {
+      double acc = 0.;
+      for (int t = 0; t < kDotBlocks; ++t) acc += partial[(int64_t)i * kDotBlocks + t];
+      out[i] = acc;
+    }
+  });
+}
+void vsum(hipStream_t s, const double* V, int64_t ldv, int k, const double* h, double sign, double* w, int64_t n) {
+  hipstub::enqueue(s, [=]() {
+    for (int64_t j = 0; j < n; ++j) {
+      double acc = 0.;
+      for (int i = 0; i < k; ++i) acc += V[i * ldv + j] * h[i];
+      w[j] += sign * acc;
+    }
+  });
+}
+void axpby(hipStream_t s, int64_t n, double a, const double* x, double b, double* y) {
+  hipstub::enqueue(s, [=]() {
+    for (int64_t j = 0; j < n; ++j) y[j] = a * x[j] + b * y[j];
+  });
+}
+void sub(hipStream_t s, int64_t n, const double* a, const double* b, double* out) {
+  hipstub::enqueue(s, [=]() {
+    for (int64_t j = 0; j < n; ++j) out[j] = a[j] - b[j];
+  });
+}
+void mul(hipStream_t s, int64_t n, const double* d, double* y) {
+  hipstub::enqueue(s, [=]() {
+    for (int64_t j = 0; j < n; ++j) y[j] *= d[j];
+  });
+}
+void mul_to(hipStream_t s, int64_t n, const double* d, const double* x, double* y) {
+  hipstub::enqueue(s, [=]() {
+    for (int64_t j = 0; j < n; ++j) y[j] = d[j] * x[j];
+  });
+}
+void scale_dup(hipStream_t s, int64_t n, double a, double* w, double* dup) {
+  hipstub::enqueue(s, [=]() {
+    for (int64_t j = 0; j < n; ++j) {
+      const double v = a * w[j];
+      w[j] = v;
+      dup[j] = v;
+    }
+  });
+}
+void schwarz_apply(hipStream_t s, const Schwarz& b, const double* inv, const double* r, double* z) {
+  const Schwarz B = b;
+  hipstub::enqueue(s, [=]() {
+    for (int blk = 0; blk < B.nblk; ++blk) {
+      const int64_t p0 = B.bptr[blk];
+      const int m = (int)(B.bptr[blk + 1] - p0), nc = B.bncore[blk];
+      const double* Ib = inv + B.inv_ptr[blk];
+      for (int i = 0; i < nc; ++i) {
+        double acc = 0.;
+        for (int j = 0; j < m; ++j) acc += Ib[(int64_t)i * m + j] * r[B.bidx[p0 + j]];
+        z[B.bidx[p0 + i]] = acc;
+      }
+    }
+  });
+}
+void perm_gather(hipStream_t s, int n, const int32_t* perm, const double* in, double* out) {
+  hipstub::enqueue(s, [=]() {
+    for (int i = 0; i < n; ++i) out[i] = in[perm[i]];
+  });
+}
+void perm_scatter(hipStream_t s, int n, const int32_t* perm, const double* in, double* out) {
+  hipstub::enqueue(s, [=]() {
+    for (int i = 0; i < n; ++i) out[perm[i]] = in[i];
+  });
+}
+
+}  // namespace dev
+}  // namespace bemfmm

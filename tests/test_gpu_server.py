@@ -106,3 +106,81 @@ def test_memory_admission_and_eviction():
         assert info["converged"] == 1
         xd = _direct(m, d, phi, phin)
         assert np.linalg.norm(x - xd) <= 1e-10 * np.linalg.norm(xd)
+
+
+def _fp32_sessions():
+    """Tanks of 2k-40k nodes (one with a cylinder), each a session of two stages (nodes moved between them)."""
+    from bem_for_nonlinear_waves_amd import mesh as M
+    out = []
+    for k, (kind, Lx, n, ny) in enumerate([("box", 4.0, 42, 11), ("box", 6.0, 96, 16), ("cyl", 6.0, 0.045, 0),
+                                            ("box", 8.0, 160, 20), ("box", 3.0, 72, 24), ("box", 8.0, 256, 32)]):
+        if kind == "box":
+            m = M.box_tank(Lx, 1.0, 1.0, n, ny, ny)
+            sigma, Ly = 0.0, 1.0
+        else:
+            m = M.cylinder_tank(Lx=Lx, Ly=2.0, Lz=1.0, R=0.25, draft=0.5, h=n)
+            sigma, Ly = 0.1, 2.0
+        m = M.perturb_free_surface(m, 20 + k, 0.01)
+        d = M.build_dofs(m)
+        stages = []
+        mk = m
+        for s in range(2):
+            if s:
+                mk = M.perturb_free_surface(mk, 200 + k, 1e-3)
+            phi, phin = M.tank_bc(mk, d, 20 + k + s, Lx, Ly, body_velocity_sigma=sigma)
+            stages.append((mk, phi, phin))
+        out.append((d, stages))
+    return out
+
+
+def test_fp32_refine_sessions_same_alone_and_in_mix():
+    """FP32+refinement requests on four slots: sessions of 2k-40k-node tanks whose stages re-prepare their
+    contexts while other contexts capture and replay their matvec graphs. Every result equals the result of
+    the same session run alone on one slot (1e-10 relative)."""
+    from bem_for_nonlinear_waves_amd import _lib
+    from bem_for_nonlinear_waves_amd.serve import BVPServer
+    kw = dict(p=10, leaf_max=32, tol=1e-12, max_iter=600, precision=_lib.PRECISION_FP32_REFINE,
+              preconditioner=_lib.PRECOND_SCHWARZ)
+    sessions = _fp32_sessions()
+    print("[gpu] fp32 sessions n_dof:", [d.n_dof for d, _ in sessions], flush=True)
+    srv = BVPServer(slots=1)
+    alone = []
+    for si, (d, stages) in enumerate(sessions):
+        xs = []
+        for mk, phi, phin in stages:
+            x, info = srv.wait(srv.submit(mk, d, phi, phin, session=si, **kw))
+            assert info["converged"] == 1 and info["refinement_steps"] >= 1
+            xs.append(x)
+        srv.close_session(si)
+        alone.append(xs)
+    srv.close()
+    print("[gpu] fp32 sessions alone done", flush=True)
+    import threading
+    srv = BVPServer(slots=4)
+    mixed = [[None] * len(st) for _, st in sessions]
+    errors = []
+
+    def run(si):
+        d, stages = sessions[si]
+        try:
+            for s, (mk, phi, phin) in enumerate(stages):
+                mixed[si][s] = srv.wait(srv.submit(mk, d, phi, phin, session=si, **kw))
+            srv.close_session(si)
+        except Exception as e:  # reported below
+            errors.append(repr(e))
+
+    th = [threading.Thread(target=run, args=(i,)) for i in range(len(sessions))]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join()
+    st = srv.stats()
+    srv.close()
+    print(f"[gpu] fp32 mix: completed={st['completed']} p50={st['p50_latency_s']:.3f}s "
+          f"p99={st['p99_latency_s']:.3f}s peak={st['peak_bytes'] / 1e9:.2f}GB", flush=True)
+    assert not errors, errors
+    assert st["failed"] == 0
+    for xs, ms in zip(alone, mixed):
+        for xa, (xm, info) in zip(xs, ms):
+            assert info["converged"] == 1
+            assert np.linalg.norm(xm - xa) <= 1e-10 * np.linalg.norm(xa)
