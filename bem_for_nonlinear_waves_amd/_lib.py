@@ -63,6 +63,7 @@ SIGNATURES = [
     ("bemfmm_prepare", ctypes.c_int, [ctypes.c_void_p]),
     ("bemfmm_matvec", ctypes.c_int, [ctypes.c_void_p, _dp, _dp]),
     ("bemfmm_matvec_device", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+    ("bemfmm_matvec_device_noalloc", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     ("bemfmm_matvec_fp32", ctypes.c_int, [ctypes.c_void_p, _dp, _dp]),
     ("bemfmm_matvec_fp32_device", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     ("bemfmm_sync", ctypes.c_int, [ctypes.c_void_p]),

@@ -317,13 +317,13 @@ void launch_translate(bemfmm_ctx* c, DTiles& t, const T* ops, const double* S0, 
   dev::translate(c->stream, c->C, c->RP, c->KP, t.view(), ops, S0, S1, D0, D1, stage);
 }
 
-void launch_m2l_rot(bemfmm_ctx* c, float* stage) {
+void launch_m2l_rot(bemfmm_ctx* c, float* stage, const double* M0, const double* M1, float* m32) {
   DTiles& t = c->m2l_rt;
   if (t.ntiles == 0) return;
   if (c->rot_RS != dev::rot_size(c->plan->P)) throw std::runtime_error("rotation M2L: factor table stride mismatch");
   const int64_t nm = (int64_t)c->n_boxes * 2 * c->C;
-  dev::pack_m32(c->stream, nm, c->C, c->M0.p, c->M1.p, c->m32.p);
-  dev::m2l_rot(c->stream, c->plan->P, t.view(), c->rot32.p, c->m32.p, stage);
+  dev::pack_m32(c->stream, nm, c->C, M0, M1, m32);
+  dev::m2l_rot(c->stream, c->plan->P, t.view(), c->rot32.p, m32, stage);
 }
 
 void fill(bemfmm_ctx* c, double* p, int64_t n, double v) { dev::fill(c->stream, p, n, v); }
@@ -331,14 +331,37 @@ void fill(bemfmm_ctx* c, double* p, int64_t n, double v) { dev::fill(c->stream, 
 // One evaluation of the BIE operator. mode: 0 = A x (x = in), 1 = rhs, 2 = rigid-mode alpha.
 // lo: FP32 near field, P2M and translations (FP32 MFMA), FP64 vectors and accumulation across boxes.
 // Launches only (no host synchronisation), so that it can be captured into a HIP graph.
-void evaluate_launch(bemfmm_ctx* c, int mode, const double* in, double* out, bool lo) {
+// Work buffers of one evaluation: [phi_n | L0] and [phi | L1] (the DOF values followed by the box locals,
+// so that L2P is a column gather like the near field), the multipoles, the per-node near-field sums, the
+// per-pair M2L staging (nullptr: the FP64 M2L accumulates into the locals with atomics instead) and the
+// FP32 multipoles of the rotation M2L.
+struct MatvecWork {
+  double *buf0 = nullptr, *buf1 = nullptr, *M0 = nullptr, *M1 = nullptr, *y0 = nullptr, *y1 = nullptr;
+  void* stage = nullptr;
+  float* m32 = nullptr;
+};
+MatvecWork context_work(bemfmm_ctx* c) {
+  MatvecWork w;
+  w.buf0 = c->buf0.p;
+  w.buf1 = c->buf1.p;
+  w.M0 = c->M0.p;
+  w.M1 = c->M1.p;
+  w.y0 = c->y0.p;
+  w.y1 = c->y1.p;
+  w.stage = c->m2l_stage.p;
+  w.m32 = c->m32.p;
+  return w;
+}
+
+void evaluate_launch(bemfmm_ctx* c, int mode, const double* in, double* out, bool lo, const MatvecWork& w) {
   Plan& pl = *c->plan;
   const int n = c->n_dof;
   const int64_t nbC = (int64_t)c->n_boxes * c->C;
-  double* phin = c->buf0.p;
-  double* phi = c->buf1.p;
-  double* L0 = c->buf0.p + n;
-  double* L1 = c->buf1.p + n;
+  if (!w.stage && lo) throw std::runtime_error("internal: the FP32 operator needs the M2L staging");
+  double* phin = w.buf0;
+  double* phi = w.buf1;
+  double* L0 = w.buf0 + n;
+  double* L1 = w.buf1 + n;
   auto rec = [&](int i) {
     if (c->prof) HIPCHECK(hipEventRecord(c->ev[i], c->stream));
   };
@@ -351,35 +374,42 @@ void evaluate_launch(bemfmm_ctx* c, int mode, const double* in, double* out, boo
   if (split) {
     HIPCHECK(hipEventRecord(c->ev_fork, c->stream));
     HIPCHECK(hipStreamWaitEvent(c->stream2, c->ev_fork, 0));
-    if (lo) launch_gemv(c, c->near, c->near_mat32.p, c->buf0.p, c->buf1.p, c->y0.p, c->y1.p, 1, c->stream2);
-    else launch_gemv(c, c->near, c->near.mat.p, c->buf0.p, c->buf1.p, c->y0.p, c->y1.p, 1, c->stream2);
+    if (lo) launch_gemv(c, c->near, c->near_mat32.p, w.buf0, w.buf1, w.y0, w.y1, 1, c->stream2);
+    else launch_gemv(c, c->near, c->near.mat.p, w.buf0, w.buf1, w.y0, w.y1, 1, c->stream2);
     HIPCHECK(hipEventRecord(c->ev_join, c->stream2));
   }
-  fill(c, c->M0.p, nbC, 0.);
-  fill(c, c->M1.p, nbC, 0.);
+  fill(c, w.M0, nbC, 0.);
+  fill(c, w.M1, nbC, 0.);
   rec(1);
-  if (lo) dev::p2m(c->stream, c->C, c->p2m_rows_pad, c->p2m.view(), c->p2m_mat32.p, phin, phi, c->M0.p, c->M1.p);
-  else dev::p2m(c->stream, c->C, c->p2m_rows_pad, c->p2m.view(), c->p2m.mat.p, phin, phi, c->M0.p, c->M1.p);
+  if (lo) dev::p2m(c->stream, c->C, c->p2m_rows_pad, c->p2m.view(), c->p2m_mat32.p, phin, phi, w.M0, w.M1);
+  else dev::p2m(c->stream, c->C, c->p2m_rows_pad, c->p2m.view(), c->p2m.mat.p, phin, phi, w.M0, w.M1);
   rec(2);
   // M2M from level l into l-1 only where level l-1 has M2L pairs (coarser multipoles are not used).
   // M2M and L2L use the FP64 operators for both operators: their inputs and outputs are FP64, and the
   // FP64 matrix-core tiles ran faster here than the FP32 ones (0.115 / 0.120 ms against 0.157 / 0.161 ms
   // per matvec at 209k nodes)
   for (int l = pl.n_levels - 1; l >= 1 && l - 1 >= c->m2l_min_level; --l) {
-    launch_translate(c, c->m2m[l], c->ops.p, c->M0.p, c->M1.p, c->M0.p, c->M1.p);
+    launch_translate(c, c->m2m[l], c->ops.p, w.M0, w.M1, w.M0, w.M1);
   }
   rec(3);
-  if (lo) {
-    float* st = reinterpret_cast<float*>(c->m2l_stage.p);
-    if (c->m2l_rot) launch_m2l_rot(c, st);
-    else launch_translate(c, c->m2l, c->ops32.p, c->M0.p, c->M1.p, nullptr, nullptr, st);
+  if (!w.stage) {
+    // no staging: the FP64 M2L tiles accumulate straight into the locals (atomics, as M2M and L2L do)
+    fill(c, L0, nbC, 0.);
+    fill(c, L1, nbC, 0.);
+    launch_translate(c, c->m2l, c->ops.p, w.M0, w.M1, L0, L1);
+    rec(4);
+  } else if (lo) {
+    float* st = reinterpret_cast<float*>(w.stage);
+    if (c->m2l_rot) launch_m2l_rot(c, st, w.M0, w.M1, w.m32);
+    else launch_translate(c, c->m2l, c->ops32.p, w.M0, w.M1, nullptr, nullptr, st);
     rec(4);
     dev::m2l_reduce(c->stream, c->C, c->n_boxes, c->m2l_rorder.p, c->m2l_tptr.p, c->m2l_tpairs.p, (const float*)st, L0, L1);
   } else {
-    launch_translate(c, c->m2l, c->ops.p, c->M0.p, c->M1.p, nullptr, nullptr, c->m2l_stage.p);
+    double* st = reinterpret_cast<double*>(w.stage);
+    launch_translate(c, c->m2l, c->ops.p, w.M0, w.M1, nullptr, nullptr, st);
     rec(4);
-    dev::m2l_reduce(c->stream, c->C, c->n_boxes, c->m2l_rorder.p, c->m2l_tptr.p, c->m2l_tpairs.p,
-                    (const double*)c->m2l_stage.p, L0, L1);
+    dev::m2l_reduce(c->stream, c->C, c->n_boxes, c->m2l_rorder.p, c->m2l_tptr.p, c->m2l_tpairs.p, (const double*)st, L0,
+                    L1);
   }
   rec(5);
   // L2L into level l only from parents at levels with M2L pairs (the locals of coarser levels are zero)
@@ -389,10 +419,10 @@ void evaluate_launch(bemfmm_ctx* c, int mode, const double* in, double* out, boo
   rec(6);
   if (split) HIPCHECK(hipStreamWaitEvent(c->stream, c->ev_join, 0));
   const int part = split ? 2 : 0;
-  if (lo) launch_gemv(c, c->near, c->near_mat32.p, c->buf0.p, c->buf1.p, c->y0.p, c->y1.p, part);
-  else launch_gemv(c, c->near, c->near.mat.p, c->buf0.p, c->buf1.p, c->y0.p, c->y1.p, part);
+  if (lo) launch_gemv(c, c->near, c->near_mat32.p, w.buf0, w.buf1, w.y0, w.y1, part);
+  else launch_gemv(c, c->near, c->near.mat.p, w.buf0, w.buf1, w.y0, w.y1, part);
   rec(7);
-  dev::assemble(c->stream, mode, n, c->kind.p, c->point.p, c->alpha.p, phi, c->y0.p, c->y1.p, c->phi_k.p, out);
+  dev::assemble(c->stream, mode, n, c->kind.p, c->point.p, c->alpha.p, phi, w.y0, w.y1, c->phi_k.p, out);
   rec(8);
   HIPCHECK(hipGetLastError());
 }
@@ -418,7 +448,7 @@ void evaluate(bemfmm_ctx* c, int mode, const double* in, double* out, bool lo = 
   const double* held = c->gin_holds;
   c->gin_holds = nullptr;
   if (mode != 0 || !graph_mode(c)) {
-    evaluate_launch(c, mode, in, out, lo);
+    evaluate_launch(c, mode, in, out, lo, context_work(c));
     return;
   }
   const int gi = lo ? 1 : 0;
@@ -429,7 +459,7 @@ void evaluate(bemfmm_ctx* c, int mode, const double* in, double* out, bool lo = 
     hipGraph_t g = nullptr;
     HIPCHECK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
     try {
-      evaluate_launch(c, 0, c->gin.p, c->gout.p, lo);
+      evaluate_launch(c, 0, c->gin.p, c->gout.p, lo, context_work(c));
     } catch (...) {
       (void)hipStreamEndCapture(c->stream, &g);
       if (g) (void)hipGraphDestroy(g);
@@ -1587,6 +1617,49 @@ int bemfmm_matvec_device(bemfmm_ctx* c, const double* x, double* y) {
   return guarded(c, [&]() {
     if (!c->prepared) throw HipError("bemfmm_matvec_device: call bemfmm_prepare first", BEMFMM_E_STATE);
     device_matvec(c, x, y, false);
+    return BEMFMM_OK;
+  });
+}
+
+int bemfmm_matvec_device_noalloc(bemfmm_ctx* c, const double* x, double* y) {
+  if (!c) return BEMFMM_E_INVALID;
+  return guarded(c, [&]() {
+    if (!c->prepared) throw HipError("bemfmm_matvec_device_noalloc: call bemfmm_prepare first", BEMFMM_E_STATE);
+    if (!x || !y) throw std::runtime_error("bemfmm_matvec_device_noalloc: null vector");
+    void* base = nullptr;
+    size_t cap = 0;
+    HIPCHECK(dev::scratch_region(&base, &cap));
+    // work buffers carved from the shared per-GPU array, each 256-byte aligned
+    const int64_t n = c->n_dof, nbC = (int64_t)c->n_boxes * c->C;
+    size_t off = 0;
+    auto carve = [&](int64_t count) {
+      double* p = reinterpret_cast<double*>(static_cast<char*>(base) + off);
+      off += ((size_t)count * sizeof(double) + 255) & ~size_t(255);
+      return p;
+    };
+    MatvecWork w;
+    w.buf0 = carve(n + nbC);
+    w.buf1 = carve(n + nbC);
+    w.M0 = carve(nbC);
+    w.M1 = carve(nbC);
+    w.y0 = carve(c->n_nodes);
+    w.y1 = carve(c->n_nodes);
+    double* xi = c->perm_identity ? nullptr : carve(n);
+    double* yi = c->perm_identity ? nullptr : carve(n);
+    if (off > cap)
+      throw HipError("bemfmm_matvec_device_noalloc: this problem needs " + std::to_string((off + (1 << 20) - 1) >> 20) +
+                         " MiB of work memory, more than the " + std::to_string(cap >> 20) +
+                         " MiB reserved per GPU for allocation-free calls (use bemfmm_matvec_device)",
+                     BEMFMM_E_INVALID);
+    c->n_matvec++;
+    if (c->perm_identity) {
+      evaluate_launch(c, 0, x, y, false, w);
+    } else {
+      dev::perm_gather(c->stream, (int)n, c->d_perm.p, x, xi);
+      evaluate_launch(c, 0, xi, yi, false, w);
+      dev::perm_scatter(c->stream, (int)n, c->d_perm.p, yi, y);
+    }
+    HIPCHECK(hipGetLastError());
     return BEMFMM_OK;
   });
 }

@@ -17,6 +17,7 @@
 
 #include <algorithm>
 #include <cstdint>
+#include <mutex>
 #include <stdexcept>
 #include <utility>
 
@@ -1388,6 +1389,10 @@ __global__ void k_scale_dup(int64_t n, double a, double* __restrict__ w, double*
     dup[j] = v;
   }
 }
+// Work memory of the allocation-free entry points (bemfmm_matvec_device_noalloc): its address is host-side
+// only a stand-in, so host code uses the device address that scratch_region looks up.
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
 int grid1(int64_t n, int bs = 256) { return (int)std::max<int64_t>(1, std::min<int64_t>((n + bs - 1) / bs, 65536)); }
 
 template <typename T>
@@ -1598,6 +1603,30 @@ void perm_gather(hipStream_t s, int n, const int32_t* perm, const double* in, do
 }
 void perm_scatter(hipStream_t s, int n, const int32_t* perm, const double* in, double* out) {
   hipLaunchKernelGGL(k_perm_scatter, dim3((n + 255) / 256), dim3(256), 0, s, n, perm, in, out);
+}
+
+namespace {
+std::mutex g_scratch_mu;
+constexpr int kMaxDevices = 64;
+void* g_scratch_ptr[kMaxDevices] = {};  // device address of `scratch` per device (nullptr: not looked up yet)
+}  // namespace
+
+hipError_t scratch_region(void** ptr, size_t* bytes) {
+  int d = 0;
+  hipError_t e = hipGetDevice(&d);
+  if (e != hipSuccess) return e;
+  if (d < 0 || d >= kMaxDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[d]) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    if (!p) return hipErrorInvalidSymbol;
+    g_scratch_ptr[d] = p;
+  }
+  *ptr = g_scratch_ptr[d];
+  *bytes = kScratchBytes;
+  return hipSuccess;
 }
 
 }  // namespace dev

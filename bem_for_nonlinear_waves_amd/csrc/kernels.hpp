@@ -219,6 +219,13 @@ void mul_to(hipStream_t s, int64_t n, const double* d, const double* x, double* 
 void scale_dup(hipStream_t s, int64_t n, double a, double* w, double* dup);
 void schwarz_apply(hipStream_t s, const Schwarz& b, const double* inv, const double* r, double* z);
 void perm_gather(hipStream_t s, int n, const int32_t* perm, const double* in, double* out);
+
+// ---- allocation-free work memory ----
+// The file-scope device array `scratch` of kernels.hip (kScratchBytes, 256-byte aligned), one copy per GPU,
+// held until the process ends: *ptr = its device address on the current device (looked up once per device
+// with hipGetSymbolAddress and kept), *bytes = its size. Every caller on a GPU shares that GPU's array.
+constexpr size_t kScratchBytes = size_t(64) << 20;
+hipError_t scratch_region(void** ptr, size_t* bytes);
 void perm_scatter(hipStream_t s, int n, const int32_t* perm, const double* in, double* out);
 
 }  // namespace dev

@@ -153,6 +153,16 @@ int bemfmm_matvec_device(bemfmm_ctx* ctx, const double* x_device, double* y_devi
  * Requires a context created with BEMFMM_PRECISION_FP32_REFINE (else BEMFMM_E_STATE). */
 int bemfmm_matvec_fp32(bemfmm_ctx* ctx, const double* x, double* y);
 int bemfmm_matvec_fp32_device(bemfmm_ctx* ctx, const double* x_device, double* y_device);
+/* y = A x on device pointers, as bemfmm_matvec_device, without allocating anything: no counterpart in the
+ * reference; for callers that cannot allocate device memory (code under HIP graph capture, serving loops and
+ * harnesses that forbid hipMalloc). The per-call work memory (phi / phi_n with the box locals, multipoles,
+ * per-node sums) is taken from one array of 64 MiB per GPU that the library holds from load until the process
+ * ends, shared by every call on that GPU and left holding the last call's values: calls that overlap on one GPU
+ * race, so a caller waits for the context's stream (bemfmm_sync) before another context on the same GPU makes
+ * such a call. The FP64 operator, launched kernel by kernel on the context's stream (no graph); its M2L
+ * accumulates into the locals directly (no per-pair staging). Returns BEMFMM_E_INVALID if the problem needs
+ * more than 64 MiB (about (2 n_dof + 4 C n_boxes + 2 n_nodes) doubles: a 200k-node tank at p = 10 needs 49). */
+int bemfmm_matvec_device_noalloc(bemfmm_ctx* ctx, const double* x_device, double* y_device);
 /* Synchronise the context's stream. */
 int bemfmm_sync(bemfmm_ctx* ctx);
 

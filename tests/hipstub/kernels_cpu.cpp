@@ -5,6 +5,7 @@
 // machine without a GPU.
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <numeric>
 #include <stdexcept>
@@ -667,6 +668,14 @@ void perm_scatter(hipStream_t s, int n, const int32_t* perm, const double* in, d
   hipstub::enqueue(s, [=]() {
     for (int i = 0; i < n; ++i) out[perm[i]] = in[i];
   });
+}
+
+hipError_t scratch_region(void** ptr, size_t* bytes) {
+  static void* buf = std::aligned_alloc(256, kScratchBytes);  // the host stand-in of the per-GPU array
+  if (!buf) return hipErrorOutOfMemory;
+  *ptr = buf;
+  *bytes = kScratchBytes;
+  return hipSuccess;
 }
 
 }  // namespace dev
