@@ -51,7 +51,7 @@ class BEM_BVP:
                  max_level: int = 7, q: int = 6, precision: int = _lib.PRECISION_FP64,
                  preconditioner: int = _lib.PRECOND_NONE, refine_eta: float = 3e-7, schwarz_max_core: int = 64,
                  schwarz_max_block: int = 128, schwarz_overlap: int = 1, m2l_tile_pairs: int = 0, dof_reorder: int = 1,
-                 use_graphs: int = 1, m2l_kernel: int = 0, overlap_near: int = 1, device: int = 0):
+                 use_graphs: int = 0, m2l_kernel: int = 0, overlap_near: int = 1, device: int = 0):
         self.lib = _lib.load()
         o = _lib.Opts()
         self.lib.bemfmm_default_opts(ctypes.byref(o))

@@ -83,37 +83,59 @@ hipError_t pinned_free(void* p) {
 // device buffers
 // ---------------------------------------------------------------------------
 template <typename T>
+// Device array of n elements in an allocation of cap >= n elements. alloc() keeps the allocation when it is
+// large enough: a re-prepare after nodes moved (every RK stage of a session) needs about the same sizes, so
+// it reallocates nothing large; a larger request allocates 1/8 more than asked, so that the next stages'
+// small growth fits. Besides the frees and allocations saved, this matters on the ROCm 7 runtime: buffers
+// that a context's matvec graphs had used stayed allocated for a while after they were freed (4.1 GB per
+// re-prepare of an 80k-node tank, measured on the MI355X).
 struct DBuf {
   T* p = nullptr;
-  size_t n = 0;
+  size_t n = 0, cap = 0;
   DBuf() = default;
   DBuf(const DBuf&) = delete;
   DBuf& operator=(const DBuf&) = delete;
-  DBuf(DBuf&& o) noexcept : p(o.p), n(o.n) {
+  DBuf(DBuf&& o) noexcept : p(o.p), n(o.n), cap(o.cap) {
     o.p = nullptr;
-    o.n = 0;
+    o.n = o.cap = 0;
   }
   DBuf& operator=(DBuf&& o) noexcept {
     if (this != &o) {
       release();
       p = o.p;
       n = o.n;
+      cap = o.cap;
       o.p = nullptr;
-      o.n = 0;
+      o.n = o.cap = 0;
     }
     return *this;
   }
+  void swap(DBuf& o) noexcept {
+    std::swap(p, o.p);
+    std::swap(n, o.n);
+    std::swap(cap, o.cap);
+  }
   ~DBuf() { release(); }
   void release() {
-    if (p) (void)dev_free(p);
+    if (p) {
+      const hipError_t e = dev_free(p);
+      if (e != hipSuccess)  // the memory stays allocated: say so instead of losing it silently
+        std::fprintf(stderr, "[bemfmm] hipFree of %zu bytes failed: %s\n", n * sizeof(T), hipGetErrorString(e));
+    }
     p = nullptr;
-    n = 0;
+    n = cap = 0;
   }
   void alloc(size_t count) {
-    release();
     if (count == 0) count = 1;
-    HIPCHECK(dev_malloc((void**)&p, count * sizeof(T)));
+    if (p && count <= cap) {
+      n = count;
+      return;
+    }
+    const size_t want = p ? count + count / 8 : count;
+    release();
+    HIPCHECK(dev_malloc((void**)&p, want * sizeof(T)));
     n = count;
+    cap = want;
   }
   // Setup-time upload; completes before returning (the source may be a temporary).
   void upload(const std::vector<T>& v, hipStream_t s) {
@@ -123,7 +145,7 @@ struct DBuf {
       HIPCHECK(hipStreamSynchronize(s));
     }
   }
-  size_t bytes() const { return n * sizeof(T); }
+  size_t bytes() const { return cap * sizeof(T); }
 };
 
 struct DGemv {
@@ -240,6 +262,12 @@ struct bemfmm_ctx {
   // near field beside the FMM passes: the near-only GEMV runs on stream2 between a fork and a join event
   hipStream_t stream2 = nullptr;
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  // The matvec graphs are captured and replayed on streams of their own (gstream, its fork gstream2; events
+  // gfork/gjoin for the capture, gin/gout to order the replays with the context's stream), so that no
+  // kernel launched eagerly shares a stream with a capture or a replay: with eager kernels on those streams
+  // the ROCm 7 runtime kept buffers used by the graphs allocated after they were freed.
+  hipStream_t gstream = nullptr, gstream2 = nullptr;
+  hipEvent_t ev_gfork = nullptr, ev_gjoin = nullptr, ev_gin = nullptr, ev_gout = nullptr;
   DBuf<int32_t> near_ncd;  // DOF columns per near block
   double* op_stage = nullptr;  // pinned host staging for operator uploads
   size_t op_stage_n = 0;
@@ -260,8 +288,26 @@ struct bemfmm_ctx {
   DBuf<int64_t> sw_ptr, sw_invptr;
   DBuf<int32_t> sw_idx, sw_ncore;
   DBuf<double> sw_inv;
+  // setup inputs kept with the context (grow-only, reused by every re-prepare: a free would synchronise the
+  // whole device, including the other contexts' work)
+  struct SetupBufs {
+    DBuf<double> dX;
+    DBuf<int32_t> dpt, v0, v1;
+    DBuf<uint64_t> k0, k1;
+    DBuf<unsigned char> sort_tmp;
+    DBuf<int32_t> row_block, ncol_dof, face_perm, fv, fdof;
+    DBuf<int64_t> range_ptr, ranges, range_prefix;
+    DBuf<double> Xs, fX, fcent, fcr, fJ, l2p, d_dun;
+    DBuf<int64_t> fptr;
+    DBuf<int32_t> pfaces, plc;
+    DBuf<double> pcs, dcnm;
+    DBuf<int64_t> node_row;
+    DBuf<int32_t> failed;
+    DTiles alt_tiles;  // the autotuner's alternative M2L tiling
+  } sb;
   DBuf<double> V;            // Krylov basis
   int64_t v_cap = 0;
+  int basis_n = -1;          // DOF count the basis is laid out for
   DBuf<double> partial, hbuf;
   bemfmm_stats stats{};
   // profiling
@@ -313,20 +359,22 @@ void launch_gemv(bemfmm_ctx* c, DGemv& g, const T* mat, const double* x0, const 
 
 template <typename T>
 void launch_translate(bemfmm_ctx* c, DTiles& t, const T* ops, const double* S0, const double* S1, double* D0, double* D1,
-                      T* stage = nullptr) {
-  dev::translate(c->stream, c->C, c->RP, c->KP, t.view(), ops, S0, S1, D0, D1, stage);
+                      T* stage = nullptr, hipStream_t s = nullptr) {
+  dev::translate(s ? s : c->stream, c->C, c->RP, c->KP, t.view(), ops, S0, S1, D0, D1, stage);
 }
 
-void launch_m2l_rot(bemfmm_ctx* c, float* stage, const double* M0, const double* M1, float* m32) {
+void launch_m2l_rot(bemfmm_ctx* c, hipStream_t s, float* stage, const double* M0, const double* M1, float* m32) {
   DTiles& t = c->m2l_rt;
   if (t.ntiles == 0) return;
   if (c->rot_RS != dev::rot_size(c->plan->P)) throw std::runtime_error("rotation M2L: factor table stride mismatch");
   const int64_t nm = (int64_t)c->n_boxes * 2 * c->C;
-  dev::pack_m32(c->stream, nm, c->C, M0, M1, m32);
-  dev::m2l_rot(c->stream, c->plan->P, t.view(), c->rot32.p, m32, stage);
+  dev::pack_m32(s, nm, c->C, M0, M1, m32);
+  dev::m2l_rot(s, c->plan->P, t.view(), c->rot32.p, m32, stage);
 }
 
-void fill(bemfmm_ctx* c, double* p, int64_t n, double v) { dev::fill(c->stream, p, n, v); }
+void fill(bemfmm_ctx* c, double* p, int64_t n, double v, hipStream_t s = nullptr) {
+  dev::fill(s ? s : c->stream, p, n, v);
+}
 
 // One evaluation of the BIE operator. mode: 0 = A x (x = in), 1 = rhs, 2 = rigid-mode alpha.
 // lo: FP32 near field, P2M and translations (FP32 MFMA), FP64 vectors and accumulation across boxes.
@@ -334,14 +382,21 @@ void fill(bemfmm_ctx* c, double* p, int64_t n, double v) { dev::fill(c->stream, 
 // Work buffers of one evaluation: [phi_n | L0] and [phi | L1] (the DOF values followed by the box locals,
 // so that L2P is a column gather like the near field), the multipoles, the per-node near-field sums, the
 // per-pair M2L staging (nullptr: the FP64 M2L accumulates into the locals with atomics instead) and the
-// FP32 multipoles of the rotation M2L.
+// FP32 multipoles of the rotation M2L; the stream the evaluation is launched on, the stream of its near-field
+// fork and the fork/join events.
 struct MatvecWork {
   double *buf0 = nullptr, *buf1 = nullptr, *M0 = nullptr, *M1 = nullptr, *y0 = nullptr, *y1 = nullptr;
   void* stage = nullptr;
   float* m32 = nullptr;
+  hipStream_t stream = nullptr, stream2 = nullptr;
+  hipEvent_t fork = nullptr, join = nullptr;
 };
 MatvecWork context_work(bemfmm_ctx* c) {
   MatvecWork w;
+  w.stream = c->stream;
+  w.stream2 = c->stream2;
+  w.fork = c->ev_fork;
+  w.join = c->ev_join;
   w.buf0 = c->buf0.p;
   w.buf1 = c->buf1.p;
   w.M0 = c->M0.p;
@@ -358,71 +413,72 @@ void evaluate_launch(bemfmm_ctx* c, int mode, const double* in, double* out, boo
   const int n = c->n_dof;
   const int64_t nbC = (int64_t)c->n_boxes * c->C;
   if (!w.stage && lo) throw std::runtime_error("internal: the FP32 operator needs the M2L staging");
+  hipStream_t st = w.stream;
   double* phin = w.buf0;
   double* phi = w.buf1;
   double* L0 = w.buf0 + n;
   double* L1 = w.buf1 + n;
   auto rec = [&](int i) {
-    if (c->prof) HIPCHECK(hipEventRecord(c->ev[i], c->stream));
+    if (c->prof) HIPCHECK(hipEventRecord(c->ev[i], st));
   };
   rec(0);
-  dev::scatter(c->stream, mode == 0 ? 0 : (mode == 1 ? 1 : 2), n, c->kind.p, c->robin.p, in, c->phi_k.p, c->phin_k.p,
+  dev::scatter(st, mode == 0 ? 0 : (mode == 1 ? 1 : 2), n, c->kind.p, c->robin.p, in, c->phi_k.p, c->phin_k.p,
                phin, phi);
   // the near-field columns need only the scattered DOF values: they run on stream2 beside the FMM passes
   // (not while profiling, which times the stages one after the other)
   const bool split = c->opts.overlap_near && !c->prof && c->near.nblk > 0;
   if (split) {
-    HIPCHECK(hipEventRecord(c->ev_fork, c->stream));
-    HIPCHECK(hipStreamWaitEvent(c->stream2, c->ev_fork, 0));
-    if (lo) launch_gemv(c, c->near, c->near_mat32.p, w.buf0, w.buf1, w.y0, w.y1, 1, c->stream2);
-    else launch_gemv(c, c->near, c->near.mat.p, w.buf0, w.buf1, w.y0, w.y1, 1, c->stream2);
-    HIPCHECK(hipEventRecord(c->ev_join, c->stream2));
+    HIPCHECK(hipEventRecord(w.fork, st));
+    HIPCHECK(hipStreamWaitEvent(w.stream2, w.fork, 0));
+    if (lo) launch_gemv(c, c->near, c->near_mat32.p, w.buf0, w.buf1, w.y0, w.y1, 1, w.stream2);
+    else launch_gemv(c, c->near, c->near.mat.p, w.buf0, w.buf1, w.y0, w.y1, 1, w.stream2);
+    HIPCHECK(hipEventRecord(w.join, w.stream2));
   }
-  fill(c, w.M0, nbC, 0.);
-  fill(c, w.M1, nbC, 0.);
+  fill(c, w.M0, nbC, 0., st);
+  fill(c, w.M1, nbC, 0., st);
   rec(1);
-  if (lo) dev::p2m(c->stream, c->C, c->p2m_rows_pad, c->p2m.view(), c->p2m_mat32.p, phin, phi, w.M0, w.M1);
-  else dev::p2m(c->stream, c->C, c->p2m_rows_pad, c->p2m.view(), c->p2m.mat.p, phin, phi, w.M0, w.M1);
+  if (lo) dev::p2m(st, c->C, c->p2m_rows_pad, c->p2m.view(), c->p2m_mat32.p, phin, phi, w.M0, w.M1);
+  else dev::p2m(st, c->C, c->p2m_rows_pad, c->p2m.view(), c->p2m.mat.p, phin, phi, w.M0, w.M1);
   rec(2);
   // M2M from level l into l-1 only where level l-1 has M2L pairs (coarser multipoles are not used).
   // M2M and L2L use the FP64 operators for both operators: their inputs and outputs are FP64, and the
   // FP64 matrix-core tiles ran faster here than the FP32 ones (0.115 / 0.120 ms against 0.157 / 0.161 ms
   // per matvec at 209k nodes)
   for (int l = pl.n_levels - 1; l >= 1 && l - 1 >= c->m2l_min_level; --l) {
-    launch_translate(c, c->m2m[l], c->ops.p, w.M0, w.M1, w.M0, w.M1);
+    launch_translate(c, c->m2m[l], c->ops.p, w.M0, w.M1, w.M0, w.M1, (double*)nullptr, st);
   }
   rec(3);
   if (!w.stage) {
     // no staging: the FP64 M2L tiles accumulate straight into the locals (atomics, as M2M and L2L do)
-    fill(c, L0, nbC, 0.);
-    fill(c, L1, nbC, 0.);
-    launch_translate(c, c->m2l, c->ops.p, w.M0, w.M1, L0, L1);
+    fill(c, L0, nbC, 0., st);
+    fill(c, L1, nbC, 0., st);
+    launch_translate(c, c->m2l, c->ops.p, w.M0, w.M1, L0, L1, (double*)nullptr, st);
     rec(4);
   } else if (lo) {
-    float* st = reinterpret_cast<float*>(w.stage);
-    if (c->m2l_rot) launch_m2l_rot(c, st, w.M0, w.M1, w.m32);
-    else launch_translate(c, c->m2l, c->ops32.p, w.M0, w.M1, nullptr, nullptr, st);
+    float* stf = reinterpret_cast<float*>(w.stage);
+    if (c->m2l_rot) launch_m2l_rot(c, st, stf, w.M0, w.M1, w.m32);
+    else launch_translate(c, c->m2l, c->ops32.p, w.M0, w.M1, nullptr, nullptr, stf, st);
     rec(4);
-    dev::m2l_reduce(c->stream, c->C, c->n_boxes, c->m2l_rorder.p, c->m2l_tptr.p, c->m2l_tpairs.p, (const float*)st, L0, L1);
+    dev::m2l_reduce(st, c->C, c->n_boxes, c->m2l_rorder.p, c->m2l_tptr.p, c->m2l_tpairs.p, (const float*)stf, L0, L1);
   } else {
-    double* st = reinterpret_cast<double*>(w.stage);
-    launch_translate(c, c->m2l, c->ops.p, w.M0, w.M1, nullptr, nullptr, st);
+    double* std_ = reinterpret_cast<double*>(w.stage);
+    launch_translate(c, c->m2l, c->ops.p, w.M0, w.M1, nullptr, nullptr, std_, st);
     rec(4);
-    dev::m2l_reduce(c->stream, c->C, c->n_boxes, c->m2l_rorder.p, c->m2l_tptr.p, c->m2l_tpairs.p, (const double*)st, L0,
+    dev::m2l_reduce(st, c->C, c->n_boxes, c->m2l_rorder.p, c->m2l_tptr.p, c->m2l_tpairs.p, (const double*)std_, L0,
                     L1);
   }
   rec(5);
   // L2L into level l only from parents at levels with M2L pairs (the locals of coarser levels are zero)
   for (int l = std::max(1, c->m2l_min_level + 1); l < pl.n_levels; ++l) {
-    launch_translate(c, c->l2l[l], c->ops.p, L0, L1, L0, L1);
+    launch_translate(c, c->l2l[l], c->ops.p, L0, L1, L0, L1, (double*)nullptr, st);
   }
   rec(6);
-  if (split) HIPCHECK(hipStreamWaitEvent(c->stream, c->ev_join, 0));
+  if (split) HIPCHECK(hipStreamWaitEvent(st, w.join, 0));
   const int part = split ? 2 : 0;
-  if (lo) launch_gemv(c, c->near, c->near_mat32.p, w.buf0, w.buf1, w.y0, w.y1, part);
-  else launch_gemv(c, c->near, c->near.mat.p, w.buf0, w.buf1, w.y0, w.y1, part);
+  if (lo) launch_gemv(c, c->near, c->near_mat32.p, w.buf0, w.buf1, w.y0, w.y1, part, st);
+  else launch_gemv(c, c->near, c->near.mat.p, w.buf0, w.buf1, w.y0, w.y1, part, st);
   rec(7);
-  dev::assemble(c->stream, mode, n, c->kind.p, c->point.p, c->alpha.p, phi, w.y0, w.y1, c->phi_k.p, out);
+  dev::assemble(st, mode, n, c->kind.p, c->point.p, c->alpha.p, phi, w.y0, w.y1, c->phi_k.p, out);
   rec(8);
   HIPCHECK(hipGetLastError());
 }
@@ -457,22 +513,31 @@ void evaluate(bemfmm_ctx* c, int mode, const double* in, double* out, bool lo = 
     // no device-wide synchronisation (frees of other contexts) while this stream is being captured
     std::unique_lock<std::shared_mutex> gate(g_capture_gate);
     hipGraph_t g = nullptr;
-    HIPCHECK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    MatvecWork w = context_work(c);
+    w.stream = c->gstream;
+    w.stream2 = c->gstream2;
+    w.fork = c->ev_gfork;
+    w.join = c->ev_gjoin;
+    HIPCHECK(hipStreamBeginCapture(c->gstream, hipStreamCaptureModeThreadLocal));
     try {
-      evaluate_launch(c, 0, c->gin.p, c->gout.p, lo, context_work(c));
+      evaluate_launch(c, 0, c->gin.p, c->gout.p, lo, w);
     } catch (...) {
-      (void)hipStreamEndCapture(c->stream, &g);
+      (void)hipStreamEndCapture(c->gstream, &g);
       if (g) (void)hipGraphDestroy(g);
       throw;
     }
-    HIPCHECK(hipStreamEndCapture(c->stream, &g));
+    HIPCHECK(hipStreamEndCapture(c->gstream, &g));
     const hipError_t e = hipGraphInstantiate(&c->gexec[gi], g, nullptr, nullptr, 0);
     (void)hipGraphDestroy(g);
     HIPCHECK(e);
   }
   if (in != c->gin.p && in != held)
     HIPCHECK(hipMemcpyAsync(c->gin.p, in, bytes, hipMemcpyDeviceToDevice, c->stream));
-  HIPCHECK(hipGraphLaunch(c->gexec[gi], c->stream));
+  HIPCHECK(hipEventRecord(c->ev_gin, c->stream));
+  HIPCHECK(hipStreamWaitEvent(c->gstream, c->ev_gin, 0));
+  HIPCHECK(hipGraphLaunch(c->gexec[gi], c->gstream));
+  HIPCHECK(hipEventRecord(c->ev_gout, c->gstream));
+  HIPCHECK(hipStreamWaitEvent(c->stream, c->ev_gout, 0));
   if (out != c->gout.p) HIPCHECK(hipMemcpyAsync(out, c->gout.p, bytes, hipMemcpyDeviceToDevice, c->stream));
 }
 
@@ -498,9 +563,9 @@ void dof_order(bemfmm_ctx* c) {
         lo[a] = std::min(lo[a], c->X[3 * p + a]);
         hi[a] = std::max(hi[a], c->X[3 * p + a]);
       }
-    DBuf<double> dX;
-    DBuf<int32_t> dpt, v0, v1;
-    DBuf<uint64_t> k0, k1;
+    auto& dX = c->sb.dX;
+    auto &dpt = c->sb.dpt, &v0 = c->sb.v0, &v1 = c->sb.v1;
+    auto &k0 = c->sb.k0, &k1 = c->sb.k1;
     dX.upload(c->X, s);
     dpt.upload(c->dof_point, s);
     v0.alloc(n);
@@ -512,7 +577,7 @@ void dof_order(bemfmm_ctx* c) {
     HIPCHECK(hipGetLastError());
     size_t tmp_bytes = 0;
     HIPCHECK(dev::radix_sort_pairs(s, nullptr, tmp_bytes, k0.p, k1.p, v0.p, v1.p, n, 63));
-    DBuf<unsigned char> tmp;
+    auto& tmp = c->sb.sort_tmp;
     tmp.alloc(tmp_bytes);
     HIPCHECK(dev::radix_sort_pairs(s, tmp.p, tmp_bytes, k0.p, k1.p, v0.p, v1.p, n, 63));
     HIPCHECK(hipMemcpyAsync(c->perm.data(), v1.p, n * sizeof(int32_t), hipMemcpyDeviceToHost, s));
@@ -717,9 +782,9 @@ void tune_m2l_tiles(bemfmm_ctx* c, const Plan& pl) {
   }
   if (known == c->m2l.tp) return;
   if (known) {
-    DTiles alt;
+    DTiles& alt = c->sb.alt_tiles;
     alt.upload(pl.m2l, c->stream, c->m2l_slot, known);
-    c->m2l = std::move(alt);
+    std::swap(c->m2l, alt);  // the previous tiling stays allocated for the next re-prepare
     c->m2l_tp = known;
     return;
   }
@@ -747,10 +812,10 @@ void tune_m2l_tiles(bemfmm_ctx* c, const Plan& pl) {
     }
     return best;
   };
-  DTiles alt;
+  DTiles& alt = c->sb.alt_tiles;
   alt.upload(pl.m2l, s, c->m2l_slot, c->m2l.tp == 16 ? 32 : 16);
   const float t_cur = time_tiles(c->m2l), t_alt = time_tiles(alt);
-  if (t_alt < t_cur) c->m2l = std::move(alt);
+  if (t_alt < t_cur) std::swap(c->m2l, alt);
   c->m2l_tp = c->m2l.tp;
   {
     std::lock_guard<std::mutex> lk(mu);
@@ -786,11 +851,15 @@ void do_prepare(bemfmm_ctx* c) {
   o.fill_p2m = false;         // the P2M matrices are built on the GPU (k_p2m_build)
   c->prepared = false;
   drop_graphs(c);  // their kernels point into the buffers reallocated below
-  // the Krylov basis is sized for n_dof: drop it (re-grown by the next solve)
-  c->V.release();
-  c->partial.release();
-  c->hbuf.release();
-  c->v_cap = 0;
+  // the Krylov basis holds v_cap vectors of n_dof: kept when the DOF count is the same (moved nodes), else
+  // dropped (re-grown by the next solve)
+  if (c->basis_n != c->n_dof) {
+    c->V.release();
+    c->partial.release();
+    c->hbuf.release();
+    c->v_cap = 0;
+    c->basis_n = c->n_dof;
+  }
   dof_order(c);
   c->plan.reset(new Plan());
   Plan& pl = *c->plan;
@@ -858,9 +927,11 @@ void do_prepare(bemfmm_ctx* c) {
                  &c->gin, &c->gout})
     b->alloc(c->n_dof);
   // quadrature inputs
-  DBuf<int32_t> row_block, ncol_dof, face_perm, fv, fdof;
-  DBuf<int64_t> range_ptr, ranges, range_prefix;
-  DBuf<double> Xs, fX, fcent, fcr, fJ, l2p, d_dun;
+  auto &row_block = c->sb.row_block, &ncol_dof = c->sb.ncol_dof, &face_perm = c->sb.face_perm, &fv = c->sb.fv,
+       &fdof = c->sb.fdof;
+  auto &range_ptr = c->sb.range_ptr, &ranges = c->sb.ranges, &range_prefix = c->sb.range_prefix;
+  auto &Xs = c->sb.Xs, &fX = c->sb.fX, &fcent = c->sb.fcent, &fcr = c->sb.fcr, &fJ = c->sb.fJ, &l2p = c->sb.l2p,
+       &d_dun = c->sb.d_dun;
   {
     std::vector<int32_t> rb(pl.near.row_out.size());
     for (int b = 0; b < pl.near.n_blocks(); ++b)
@@ -917,9 +988,9 @@ void do_prepare(bemfmm_ctx* c) {
         const double v = std::sqrt(f1 * f2);
         cnm[n * (n + 1) / 2 + m] = (m & 1) ? -v : v;
       }
-    DBuf<int64_t> fptr;
-    DBuf<int32_t> pfaces, plc;
-    DBuf<double> pcs, dcnm;
+    auto& fptr = c->sb.fptr;
+    auto &pfaces = c->sb.pfaces, &plc = c->sb.plc;
+    auto &pcs = c->sb.pcs, &dcnm = c->sb.dcnm;
     fptr.upload(pl.p2m_face_ptr, s);
     pfaces.upload(pl.p2m_faces, s);
     plc.upload(pl.p2m_face_lc, s);
@@ -985,7 +1056,7 @@ void do_prepare(bemfmm_ctx* c) {
     for (int i = 0; i < c->n_dof; ++i)
       if (c->kind_i[i] != 2 && node_row[c->point_i[i]] < 0)
         throw std::runtime_error("preconditioner: a DOF's point has no near-field row");
-    DBuf<int64_t> d_node_row;
+    auto& d_node_row = c->sb.node_row;
     d_node_row.upload(node_row, s);
     dev::NearRows nr;
     nr.kind = c->kind.p;
@@ -1025,7 +1096,7 @@ void do_prepare(bemfmm_ctx* c) {
       c->sw_ncore.upload(B.ncore, s);
       c->sw_invptr.upload(invptr, s);
       c->sw_inv.alloc((size_t)invptr[nb]);
-      DBuf<int32_t> failed;
+      auto& failed = c->sb.failed;
       failed.alloc(nb);
       dev::schwarz_build(s, schwarz_view(c), B.max_block, nr, c->sw_inv.p, failed.p);
       HIPCHECK(hipGetLastError());
@@ -1114,8 +1185,7 @@ void ensure_basis(bemfmm_ctx* c, int64_t need) {
   nv.alloc((size_t)cap * c->n_dof);
   if (c->v_cap > 0)
     HIPCHECK(hipMemcpyAsync(nv.p, c->V.p, (size_t)c->v_cap * c->n_dof * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
-  std::swap(c->V.p, nv.p);
-  std::swap(c->V.n, nv.n);
+  c->V.swap(nv);
   c->v_cap = cap;
   c->partial.alloc((size_t)kDotBlocks * (cap + 1));
   // [h1 (k) | h2 (k) | w.w] of one Arnoldi step (k <= cap), or the k update coefficients
@@ -1490,7 +1560,7 @@ void bemfmm_default_opts(bemfmm_opts* o) {
   o->schwarz_overlap = 1;     // main_time_domain.cpp:46
   o->m2l_tile_pairs = 0;
   o->dof_reorder = 1;
-  o->use_graphs = 1;
+  o->use_graphs = 0;
   o->m2l_kernel = 0;
   o->overlap_near = 1;
 }
@@ -1507,13 +1577,15 @@ bemfmm_ctx* bemfmm_ctx_create(int device, const bemfmm_opts* opts) {
     HIPCHECK(hipSetDevice(device));
     HIPCHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     HIPCHECK(hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking));
-    HIPCHECK(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
-    HIPCHECK(hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming));
+    HIPCHECK(hipStreamCreateWithFlags(&c->gstream, hipStreamNonBlocking));
+    HIPCHECK(hipStreamCreateWithFlags(&c->gstream2, hipStreamNonBlocking));
+    for (hipEvent_t* e : {&c->ev_fork, &c->ev_join, &c->ev_gfork, &c->ev_gjoin, &c->ev_gin, &c->ev_gout})
+      HIPCHECK(hipEventCreateWithFlags(e, hipEventDisableTiming));
     return BEMFMM_OK;
   });
   if (rc != BEMFMM_OK) {
     std::fprintf(stderr, "bemfmm_ctx_create: %s\n", c->err.c_str());
-    delete c;
+    bemfmm_ctx_destroy(c);  // releases the streams and events created before the failure
     return nullptr;
   }
   return c;
@@ -1523,16 +1595,18 @@ void bemfmm_ctx_destroy(bemfmm_ctx* c) {
   if (!c) return;
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
-  if (c->stream2) (void)hipStreamSynchronize(c->stream2);
+  for (hipStream_t s : {c->stream2, c->gstream, c->gstream2})
+    if (s) (void)hipStreamSynchronize(s);
   for (auto& e : c->ev)
     if (e) (void)hipEventDestroy(e);
-  for (auto e : {c->ev_fork, c->ev_join})
+  for (auto e : {c->ev_fork, c->ev_join, c->ev_gfork, c->ev_gjoin, c->ev_gin, c->ev_gout})
     if (e) (void)hipEventDestroy(e);
   drop_graphs(c);
   if (c->op_stage) (void)pinned_free(c->op_stage);
   if (c->hres) (void)pinned_free(c->hres);
   if (c->stream) (void)hipStreamDestroy(c->stream);
-  if (c->stream2) (void)hipStreamDestroy(c->stream2);
+  for (hipStream_t s : {c->stream2, c->gstream, c->gstream2})
+    if (s) (void)hipStreamDestroy(s);
   delete c;
 }
 
@@ -1637,7 +1711,9 @@ int bemfmm_matvec_device_noalloc(bemfmm_ctx* c, const double* x, double* y) {
       off += ((size_t)count * sizeof(double) + 255) & ~size_t(255);
       return p;
     };
-    MatvecWork w;
+    MatvecWork w = context_work(c);  // the context's streams and events; its buffers are replaced below
+    w.stage = nullptr;
+    w.m32 = nullptr;
     w.buf0 = carve(n + nbC);
     w.buf1 = carve(n + nbC);
     w.M0 = carve(nbC);

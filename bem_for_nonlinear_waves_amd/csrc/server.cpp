@@ -41,6 +41,7 @@ struct Request {
   double tol = 1e-10;
   int max_iter = 1000, restart = 1000;
   double est_bytes = 0;
+  double reserved = 0;  // bytes reserved at admission for a context this request does not hold yet
   // results
   std::vector<double> x;
   bemfmm_result res{};
@@ -82,6 +83,19 @@ struct bemfmm_server {
   std::map<int64_t, std::shared_ptr<Request>> finished;
   std::set<int64_t> collected;  // tickets already returned by bemfmm_server_wait
   std::map<int64_t, Session> sessions;
+  // Idle contexts kept for reuse: a finished request without a session, a closed session or a session whose
+  // options changed leaves its context here, and a request that needs a context takes one with the same
+  // options instead of creating one (its device buffers are grow-only, so a new geometry of a similar size
+  // reuses them). Destroying a context frees its buffers, and every free synchronises the whole device, so
+  // with other slots' work queued the pool saves more than the allocations. Pooled bytes stay in used_bytes;
+  // under memory pressure pooled contexts are released first.
+  struct Pooled {
+    bemfmm_ctx* ctx;
+    bemfmm_opts opts;
+    double bytes;
+    double last_use;
+  };
+  std::vector<Pooled> pool;
   std::vector<std::thread> workers;
   double used_bytes = 0, peak_bytes = 0;
   int running = 0;
@@ -122,14 +136,23 @@ struct bemfmm_server {
       auto out = r;
       queue.erase(it);
       used_bytes += need;
+      out->reserved = need;
       peak_bytes = std::max(peak_bytes, used_bytes);
-      if (out->session >= 0) {
-        Session& s = sessions[out->session];
-        s.busy = true;
-        if (!s.ctx) s.bytes = need;
-      }
+      if (out->session >= 0) sessions[out->session].busy = true;
       return out;
     }
+    return nullptr;
+  }
+
+  // the most recently pooled context with these options (must hold mu); nullptr if none
+  bemfmm_ctx* take_pooled_locked(const bemfmm_opts& opts, double& bytes) {
+    for (size_t k = pool.size(); k-- > 0;)
+      if (same_opts(pool[k].opts, opts)) {
+        bemfmm_ctx* c = pool[k].ctx;
+        bytes = pool[k].bytes;
+        pool.erase(pool.begin() + (long)k);
+        return c;
+      }
     return nullptr;
   }
 
@@ -139,64 +162,77 @@ struct bemfmm_server {
       r.err = c ? bemfmm_last_error(c) : "bemfmm_ctx_create failed";
     };
     bemfmm_ctx* c = s ? s->ctx : nullptr;
+    double held = s ? s->bytes : 0.;  // bytes accounted to c before this request
     bool fresh = false;
-    if (!c || !same_opts(s->opts, r.opts)) {
-      if (c) {
-        bemfmm_ctx_destroy(c);
-        c = nullptr;
-        if (s) {  // the session holds no context (and no device memory) until the new one exists
-          s->ctx = nullptr;
-          s->X.clear();
-          std::lock_guard<std::mutex> lk(mu);
-          used_bytes -= s->bytes;
-          s->bytes = 0;
-        }
+    if (c && !same_opts(s->opts, r.opts)) {  // options changed: the session's context goes to the pool
+      std::lock_guard<std::mutex> lk(mu);
+      pool.push_back(Pooled{c, s->opts, s->bytes, wall()});
+      s->ctx = nullptr;
+      s->bytes = 0;
+      s->X.clear();
+      c = nullptr;
+      held = 0;
+    }
+    if (!c) {
+      {
+        std::lock_guard<std::mutex> lk(mu);
+        c = take_pooled_locked(r.opts, held);
       }
-      c = bemfmm_ctx_create(o.device, &r.opts);
-      if (!c) return fail(nullptr, BEMFMM_E_DEVICE);
+      if (!c) {
+        c = bemfmm_ctx_create(o.device, &r.opts);
+        held = 0;
+      }
       fresh = true;
       if (s) {
         s->ctx = c;
         s->opts = r.opts;
+        s->bytes = held;
       }
     }
-    const bool same_topology = s && !fresh && s->tri == r.tri && s->face_dof == r.face_dof && s->kind == r.kind &&
-                               s->point == r.point;
-    const bool same_geometry = same_topology && s->X == r.X;
     int rc = BEMFMM_OK;
-    if (!same_geometry) {
-      rc = bemfmm_set_geometry(c, r.n_nodes, r.X.data(), r.n_faces, r.tri.data(), r.face_dof.data(), r.n_dof,
-                               r.kind.data(), r.point.data());
-      if (rc == BEMFMM_OK) rc = bemfmm_set_bc(c, r.phi.data(), r.phin.data());
-      if (rc == BEMFMM_OK) rc = bemfmm_prepare(c);
-      if (rc == BEMFMM_OK && s) {
-        s->X = r.X;
-        s->tri = r.tri;
-        s->face_dof = r.face_dof;
-        s->kind = r.kind;
-        s->point = r.point;
-      }
+    if (!c) {
+      fail(nullptr, BEMFMM_E_DEVICE);
+      rc = BEMFMM_E_DEVICE;
     } else {
-      rc = bemfmm_set_bc(c, r.phi.data(), r.phin.data());  // same geometry: only the knowns change
+      const bool same_topology = s && !fresh && s->tri == r.tri && s->face_dof == r.face_dof &&
+                                 s->kind == r.kind && s->point == r.point;
+      const bool same_geometry = same_topology && s->X == r.X;
+      if (!same_geometry) {
+        rc = bemfmm_set_geometry(c, r.n_nodes, r.X.data(), r.n_faces, r.tri.data(), r.face_dof.data(), r.n_dof,
+                                 r.kind.data(), r.point.data());
+        if (rc == BEMFMM_OK) rc = bemfmm_set_bc(c, r.phi.data(), r.phin.data());
+        if (rc == BEMFMM_OK) rc = bemfmm_prepare(c);
+        if (s) {  // a failed setup leaves no geometry the next request could reuse
+          s->X = rc == BEMFMM_OK ? r.X : std::vector<double>();
+          s->tri = r.tri;
+          s->face_dof = r.face_dof;
+          s->kind = r.kind;
+          s->point = r.point;
+        }
+      } else {
+        rc = bemfmm_set_bc(c, r.phi.data(), r.phin.data());  // same geometry: only the knowns change
+      }
+      if (rc == BEMFMM_OK) {
+        r.x.assign(r.n_dof, 0.);
+        if (r.x0.empty()) r.x0.assign(r.n_dof, 0.);
+        rc = bemfmm_solve(c, r.x0.data(), r.x.data(), r.tol, r.max_iter, r.restart, &r.res);
+        if (rc == BEMFMM_E_NOT_CONVERGED) rc = BEMFMM_OK;  // reported through res.converged
+      }
+      if (rc != BEMFMM_OK) fail(c, rc);
     }
-    if (rc == BEMFMM_OK) {
-      r.x.assign(r.n_dof, 0.);
-      if (r.x0.empty()) r.x0.assign(r.n_dof, 0.);
-      rc = bemfmm_solve(c, r.x0.data(), r.x.data(), r.tol, r.max_iter, r.restart, &r.res);
-      if (rc == BEMFMM_E_NOT_CONVERGED) rc = BEMFMM_OK;  // reported through res.converged
-    }
-    if (rc != BEMFMM_OK) fail(c, rc);
+    // accounting: the admission reservation is replaced by the context's measured footprint
+    bemfmm_stats st{};
+    const double actual = (c && bemfmm_get_stats(c, &st) == BEMFMM_OK)
+                              ? st.device_bytes + 8.0 * r.n_dof * std::min(r.max_iter + 2, 512)
+                              : held;
+    std::lock_guard<std::mutex> lk(mu);
+    used_bytes += actual - held - r.reserved;
+    r.reserved = 0;
+    peak_bytes = std::max(peak_bytes, used_bytes);
     if (s) {
-      bemfmm_stats st{};
-      if (bemfmm_get_stats(c, &st) == BEMFMM_OK) {
-        std::lock_guard<std::mutex> lk(mu);
-        const double actual = st.device_bytes + 8.0 * r.n_dof * std::min(r.max_iter + 2, 512);
-        used_bytes += actual - s->bytes;
-        peak_bytes = std::max(peak_bytes, used_bytes);
-        s->bytes = actual;
-      }
-    } else {
-      bemfmm_ctx_destroy(c);
+      s->bytes = c ? actual : 0.;
+    } else if (c) {
+      pool.push_back(Pooled{c, r.opts, actual, wall()});
     }
   }
 
@@ -215,8 +251,7 @@ struct bemfmm_server {
       lk.lock();
       r->t_end = wall();
       --running;
-      if (!s) used_bytes -= r->est_bytes;
-      else {
+      if (s) {
         s->busy = false;
         s->last_use = r->t_end;
         if (--s->pending == 0 && s->closing) close_locked(r->session);
@@ -230,8 +265,19 @@ struct bemfmm_server {
     }
   }
 
-  // frees the context of the least recently used idle session (its next request sets it up again)
+  // frees the least recently used pooled context, else the context of the least recently used idle session
+  // (its next request sets it up again)
   bool evict_one(int64_t keep) {
+    if (!pool.empty()) {
+      size_t k = 0;
+      for (size_t i = 1; i < pool.size(); ++i)
+        if (pool[i].last_use < pool[k].last_use) k = i;
+      bemfmm_ctx_destroy(pool[k].ctx);
+      used_bytes -= pool[k].bytes;
+      pool.erase(pool.begin() + (long)k);
+      ++n_evicted;
+      return true;
+    }
     Session* best = nullptr;
     for (auto& kv : sessions) {
       Session& ss = kv.second;
@@ -248,11 +294,12 @@ struct bemfmm_server {
     return true;
   }
 
+  // a closed session's context goes to the pool (its bytes stay accounted there)
   void close_locked(int64_t id) {
     auto it = sessions.find(id);
     if (it == sessions.end()) return;
-    if (it->second.ctx) bemfmm_ctx_destroy(it->second.ctx);
-    used_bytes -= it->second.bytes;
+    if (it->second.ctx) pool.push_back(Pooled{it->second.ctx, it->second.opts, it->second.bytes, wall()});
+    else used_bytes -= it->second.bytes;
     sessions.erase(it);
   }
 };
@@ -275,7 +322,9 @@ bemfmm_server* bemfmm_server_create(const bemfmm_server_opts* opts) {
     delete s;
     return nullptr;
   }
-  if (s->o.omp_threads_per_slot <= 0) s->o.omp_threads_per_slot = std::max(1, omp_get_num_procs() / s->o.slots);
+  // host threads for each request's setup: the process's OpenMP thread budget (OMP_NUM_THREADS, else the
+  // processors available) shared by the slots
+  if (s->o.omp_threads_per_slot <= 0) s->o.omp_threads_per_slot = std::max(1, omp_get_max_threads() / s->o.slots);
   for (int i = 0; i < s->o.slots; ++i) s->workers.emplace_back([s] { s->worker(); });
   return s;
 }
@@ -292,6 +341,7 @@ void bemfmm_server_destroy(bemfmm_server* s) {
   for (auto& t : s->workers) t.join();
   for (auto& kv : s->sessions)
     if (kv.second.ctx) bemfmm_ctx_destroy(kv.second.ctx);
+  for (auto& pc : s->pool) bemfmm_ctx_destroy(pc.ctx);
   delete s;
 }
 
@@ -387,6 +437,7 @@ int bemfmm_server_get_stats(bemfmm_server* s, bemfmm_server_stats* st) {
   st->queued = (int64_t)s->queue.size();
   st->running = s->running;
   st->resident_sessions = (int)s->sessions.size();
+  st->pooled_contexts = (int)s->pool.size();
   st->p50_latency_s = pct(0.50);
   st->p99_latency_s = pct(0.99);
   st->used_bytes = s->used_bytes;

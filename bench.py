@@ -39,12 +39,48 @@ if ROOT not in sys.path:
 METRIC = "BVP solves/s & FMM matvec Mnode/s at N=200k; p50/p99 solve latency"
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
-# HBM bytes per launch of the dominant kernels on the default workload (h = 0.036, p = 10, leaf 32,
-# depth 7), from rocprofv3 FETCH_SIZE (scaled x2) and WRITE_SIZE (x1) passes over
-# `bench.py --profile-only` (summarised in profiles/fmm_matvec_cylinder209k_p10.md); reported only
-# for that workload. Keys: (stage, FP32 operator); "m2l_rot" = the rotation M2L of the FP32 operator.
-MEASURED_TRAFFIC = {("m2l", True): 5.943e9, ("m2l", False): 8.692e9, ("m2l_rot", True): 4.018e9,
-                    ("near_l2p", True): 5.177e9, ("near_l2p", False): 9.896e9}
+PROFILES = os.path.join(ROOT, "profiles")
+
+
+def device_code_hash() -> str:
+    """sha256 (first 16 hex digits) of the device code (kernels.hip, kernels.hpp): counter figures under
+    profiles/ are used only for the build they were measured on."""
+    import hashlib
+    h = hashlib.sha256()
+    for f in ("kernels.hip", "kernels.hpp"):
+        with open(os.path.join(ROOT, "bem_for_nonlinear_waves_amd", "csrc", f), "rb") as fh:
+            h.update(fh.read())
+    return h.hexdigest()[:16]
+
+
+def workload_key(args, fp32) -> str:
+    return (f"cylinder_h{args.h}_p{args.p}_leaf{args.leaf}_lvl{args.max_level}_{'fp32' if fp32 else 'fp64'}"
+            if not args.case else "case")
+
+
+def measured_traffic(kernel: str, workload: str):
+    """HBM bytes per launch of `kernel` from profiles/traffic_<workload>.txt: lines
+    `build <hash>`, `calibration fetch <f> write <w>` (factors from a plain device copy of known size) and
+    `<kernel> fetch_kib <F> write_kib <W> launches <n>` (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, KiB summed
+    over the launches). None unless the file exists and was measured on this device code."""
+    path = os.path.join(PROFILES, f"traffic_{workload}.txt")
+    if not os.path.exists(path):
+        return None
+    build, fscale, wscale, rows = None, 1.0, 1.0, {}
+    for line in open(path):
+        t = line.split()
+        if not t or t[0].startswith("#"):
+            continue
+        if t[0] == "build":
+            build = t[1]
+        elif t[0] == "calibration":
+            fscale, wscale = float(t[2]), float(t[4])
+        elif len(t) >= 7 and t[1] == "fetch_kib":
+            rows[t[0]] = (float(t[2]), float(t[4]), float(t[6]))
+    if build != device_code_hash() or kernel not in rows:
+        return None
+    f, w, n = rows[kernel]
+    return {"bytes": (f * fscale + w * wscale) * 1024.0 / n, "source": os.path.relpath(path, ROOT)}
 
 
 def case_workload(path: str, seed: int = 3):
@@ -70,40 +106,108 @@ def make_workload(h: float, seed: int = 3):
     return m, d, phi, phin
 
 
-def cpu_baseline(p, leaf_max, max_level, gpu_iters, n_nodes_full, budget_s=25.0):
-    """Oracle (FP64 OpenMP restatement) matvecs on a smaller tank of the same family; Mnode/s and the
-    solves/s it implies at the full size with the GPU's iteration count (FMM cost is O(N))."""
+def cpu_baseline(m, d, phi, phin, p, leaf_max, max_level, matvecs_per_solve, n_timed=3):
+    """The FP64 oracle (OpenMP restatement of the reference's FMM matvec) on the bench's own workload at its
+    full size, built on this machine with -O3 -march=native: setup, one matvec that also builds the M2L
+    coefficients per (level, offset), then n_timed timed matvecs; solves/s at the FP64 path's matvec count."""
     from oracle import oracle as O
-    O.build()
-    m, d, phi, phin = make_workload(0.12)
+    O.use_native()
     t = time.perf_counter()
     o = O.Oracle(m, d, phi, phin, p=p, leaf_max=leaf_max, max_level=max_level)
     o.set_m2l_cache(True)
     x = np.random.default_rng(0).standard_normal(d.n_dof)
-    o.matvec(x)  # builds the per-offset M2L coefficients (setup)
+    o.matvec(x)
     t_setup = time.perf_counter() - t
-    n = 0
     t0 = time.perf_counter()
-    while True:
+    for _ in range(n_timed):
         o.matvec(x)
-        n += 1
-        if time.perf_counter() - t0 > budget_s / 2 or n >= 20:
-            break
-    tm = (time.perf_counter() - t0) / n
-    mnode = m.n_nodes / tm / 1e6
-    t_solve_full = (gpu_iters + 3) * tm * (n_nodes_full / m.n_nodes)
+    tm = (time.perf_counter() - t0) / n_timed
+    del o
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     try:
-        model = subprocess.run(["lscpu"], capture_output=True, text=True).stdout
-        model = [l.split(":", 1)[1].strip() for l in model.splitlines() if l.startswith("Model name")][0]
+        out = subprocess.run(["lscpu"], capture_output=True, text=True).stdout
+        model = [l.split(":", 1)[1].strip() for l in out.splitlines() if l.startswith("Model name")][0]
     except Exception:
         model = "unknown"
-    return {"value": 1.0 / t_solve_full, "unit": "solves/s", "cores": threads, "kind": "port",
-            "matvec_mnode_per_s": mnode, "cpu": model,
-            "sample": (f"oracle FP64 FMM matvec (M2L coefficients cached per level and offset) x{n} on the "
-                       f"same tank family at h=0.12 ({m.n_nodes} nodes, "
-                       f"setup {t_setup:.1f}s, {tm*1e3:.1f} ms/matvec); solves/s extrapolated linearly in N to "
-                       f"{n_nodes_full} nodes at the GPU's {gpu_iters}+3 matvecs per solve")}
+    return {"value": 1.0 / (matvecs_per_solve * tm), "unit": "solves/s", "cores": threads, "kind": "port",
+            "matvec_mnode_per_s": m.n_nodes / tm / 1e6, "matvec_s": tm, "setup_s": t_setup, "cpu": model,
+            "omp_proc_bind": os.environ.get("OMP_PROC_BIND", ""),
+            "sample": (f"measured at {m.n_nodes:,} nodes (the bench workload): oracle built -O3 -march=native, "
+                       f"{threads} OpenMP threads; setup + first matvec (M2L coefficients cached per level and "
+                       f"offset) {t_setup:.1f} s, then {n_timed} timed matvecs of {tm:.2f} s; solves/s at "
+                       f"{matvecs_per_solve} matvecs per solve (the FP64 GPU path's count)")}
+
+
+def kernel_key(name: str) -> str:
+    """`k_m2l_rot<10>` from a demangled dispatch name such as
+    `void (anonymous namespace)::k_m2l_rot<10>(int, float const*, ...)`."""
+    n = name.replace("void ", "").replace("(anonymous namespace)::", "")
+    depth, out = 0, []
+    for ch in n:
+        if ch == "(" and depth == 0:
+            break
+        depth += ch == "<"
+        depth -= ch == ">"
+        out.append(ch)
+    return "".join(out).replace(" ", "")
+
+
+def write_traffic(fetch_dir: str, write_dir: str, workload: str) -> str:
+    """profiles/traffic_<workload>.txt from two rocprofv3 --pmc passes (FETCH_SIZE, WRITE_SIZE; csv output) over
+    `bench.py --profile-only`: KiB summed per kernel and the dispatch count, plus the calibration factors of the
+    256 MiB device copy that the same run makes (its bytes read and written over what the counters report)."""
+    import csv
+    import glob
+
+    def load(d, counter):
+        rows = {}
+        files = glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True)
+        if not files:
+            raise SystemExit(f"no counter_collection.csv under {d}")
+        copy = 0.0
+        for f in files:
+            for r in csv.DictReader(open(f)):
+                if r.get("Counter_Name") != counter:
+                    continue
+                v = float(r["Counter_Value"])
+                if "copyBuffer" in r["Kernel_Name"]:
+                    copy = max(copy, v)
+                    continue
+                k = kernel_key(r["Kernel_Name"])
+                kib, n = rows.get(k, (0.0, 0))
+                rows[k] = (kib + v, n + 1)
+        return rows, copy
+
+    fetch, fcopy = load(fetch_dir, "FETCH_SIZE")
+    write, wcopy = load(write_dir, "WRITE_SIZE")
+    cal_kib = 256 * 1024.0
+    lines = [f"# HBM traffic per kernel, workload {workload}: rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE over",
+             "# `python3 bench.py --profile-only` (KiB summed over the dispatches of each kernel); calibration = bytes",
+             "# of the run's 256 MiB device copy over the counter's reading (fetch, write)",
+             f"build {device_code_hash()}",
+             f"calibration fetch {cal_kib / fcopy if fcopy else 1.0:.4f} write {cal_kib / wcopy if wcopy else 1.0:.4f}"]
+    for k in sorted(fetch, key=lambda k: -fetch[k][0]):
+        if k in write and fetch[k][1] == write[k][1]:
+            lines.append(f"{k} fetch_kib {fetch[k][0]:.1f} write_kib {write[k][0]:.1f} launches {fetch[k][1]}")
+    path = os.path.join(PROFILES, f"traffic_{workload}.txt")
+    with open(path, "w") as f:
+        f.write("\n".join(lines) + "\n")
+    return path
+
+
+def cpu_baseline_child(args, matvecs_per_solve):
+    """The CPU baseline in a child process of its own, with its OpenMP threads bound close to each other
+    (OMP_PROC_BIND is read by libgomp when it is first loaded, and binding the request threads of the GPU
+    process would crowd them onto the same cores)."""
+    env = dict(os.environ, OMP_PROC_BIND="close")
+    cmd = [sys.executable, os.path.abspath(__file__), "--cpu-baseline-only", str(matvecs_per_solve),
+           "--h", str(args.h), "--p", str(args.p), "--leaf", str(args.leaf), "--max-level", str(args.max_level)]
+    if args.case:
+        cmd += ["--case", args.case]
+    r = subprocess.run(cmd, capture_output=True, text=True, env=env)
+    if r.returncode != 0:
+        raise RuntimeError(f"cpu baseline process failed ({r.returncode}): {r.stderr[-500:]}")
+    return json.loads(r.stdout.strip().splitlines()[-1])
 
 
 STAGES = ["scatter", "p2m", "m2m", "m2l", "m2l_reduce", "l2l", "near_l2p", "assemble"]
@@ -122,7 +226,7 @@ def setup_bvp(m, d, phi, phin, args, precision):
                   refine_eta=args.refine_eta, m2l_tile_pairs=args.m2l_tile_pairs,
                   m2l_kernel={"auto": 0, "dense": 1, "rotation": 2}[args.m2l_kernel],
                   schwarz_max_core=args.schwarz_core, schwarz_max_block=args.schwarz_block,
-                  schwarz_overlap=args.schwarz_overlap)
+                  schwarz_overlap=args.schwarz_overlap, use_graphs=args.graphs)
     bvp.set_bc(phi, phin)
     t = time.perf_counter()
     bvp.prepare()
@@ -143,42 +247,43 @@ def setup_bvp(m, d, phi, phin, args, precision):
 
 def time_matvec(bvp, n, n_nodes, reps, fp32):
     """Matvec on device-resident vectors: wall time per matvec and per-stage HIP-event times (ms)."""
-    import torch
+    from bem_for_nonlinear_waves_amd import devbuf
     mv = bvp.matvec_fp32_device if fp32 else bvp.matvec_device
-    xs = torch.randn(n, dtype=torch.float64, device="cuda")
-    ys = torch.empty(n, dtype=torch.float64, device="cuda")
-    torch.cuda.synchronize()
+    xd = devbuf.DeviceArray.from_host(np.random.default_rng(0).standard_normal(n))
+    yd = devbuf.DeviceArray(n)
+    xs, ys = xd, yd
     bvp.profile()  # enable event recording on the library's stream (eager launches)
     for _ in range(2):
-        mv(xs.data_ptr(), ys.data_ptr())
+        mv(xs.address, ys.address)
     bvp.sync()
     t0 = time.perf_counter()
     for _ in range(reps):
-        mv(xs.data_ptr(), ys.data_ptr())
+        mv(xs.address, ys.address)
     bvp.sync()
     t_mv = (time.perf_counter() - t0) / reps
     stage = np.zeros(8)
     for _ in range(5):
-        mv(xs.data_ptr(), ys.data_ptr())
+        mv(xs.address, ys.address)
         stage += bvp.profile()  # synchronises on the last event of this matvec
     stage /= 5
-    bvp.profile_off()  # graph replay again
+    bvp.profile_off()  # the configuration the solves run in (no event recording)
     for _ in range(2):
-        mv(xs.data_ptr(), ys.data_ptr())
+        mv(xs.address, ys.address)
     bvp.sync()
     t0 = time.perf_counter()
     for _ in range(reps):
-        mv(xs.data_ptr(), ys.data_ptr())
+        mv(xs.address, ys.address)
     bvp.sync()
-    t_graph = (time.perf_counter() - t0) / reps
-    _log("%s matvec %.3f ms eager, %.3f ms graph (%.2f Mnode/s); stages ms: %s" % (
-        "fp32" if fp32 else "fp64", t_mv * 1e3, t_graph * 1e3, n_nodes / t_graph / 1e6,
+    t_run = (time.perf_counter() - t0) / reps
+    _log("%s matvec %.3f ms (%.3f ms with stage events; %.2f Mnode/s); stages ms: %s" % (
+        "fp32" if fp32 else "fp64", t_run * 1e3, t_mv * 1e3, n_nodes / t_run / 1e6,
         " ".join(f"{k}={v:.3f}" for k, v in zip(STAGES, stage))))
-    return min(t_mv, t_graph), stage
+    xd.free()
+    yd.free()
+    return t_run, stage
 
 
 def run_solves(bvp, n, args, label):
-    import torch
     x0 = np.zeros(n)
     for i in range(args.warmup):
         x, info = bvp.solveSystemGMRES(x0, tol=args.tol, max_iter=args.max_iter, restart=args.max_iter)
@@ -189,7 +294,6 @@ def run_solves(bvp, n, args, label):
                  + " ".join(f"{r:.2e}" for r in bvp.refinement_history()))
             if os.environ.get("BENCH_GMRES_HISTORY"):
                 _log(f"[{label}] GMRES estimates: " + " ".join(f"{r:.1e}" for r in bvp.residual_history()))
-    torch.cuda.synchronize()
     lat, infos = [], []
     t_start = time.perf_counter()
     for i in range(args.steps):
@@ -200,7 +304,6 @@ def run_solves(bvp, n, args, label):
         _log(f"[{label}] solve {i}: iters={info['iters']} matvecs={info['n_matvec']} "
              f"steps={info['refinement_steps']} residual={info['residual']:.2e} converged={info['converged']} "
              f"{lat[-1]:.3f}s")
-    torch.cuda.synchronize()
     t_total = time.perf_counter() - t_start
     return x, lat, infos, t_total
 
@@ -233,7 +336,7 @@ def _solver_kw(args):
                 restart=args.max_iter, refine_eta=args.refine_eta,
                 precision=_lib.PRECISION_FP32_REFINE if args.precision == "fp32_refine" else _lib.PRECISION_FP64,
                 preconditioner={"none": _lib.PRECOND_NONE, "jacobi": _lib.PRECOND_JACOBI,
-                                "schwarz": _lib.PRECOND_SCHWARZ}[args.precond])
+                                "schwarz": _lib.PRECOND_SCHWARZ}[args.precond], use_graphs=args.graphs)
 
 
 def run_batch(args):
@@ -393,6 +496,10 @@ def main():
     ap.add_argument("--no-fp64-compare", action="store_true",
                     help="skip the plain FP64 solves timed beside the FP32+refinement ones")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--schedule", default="auto", choices=["auto", "spin", "yield"],
+                    help="how host threads wait for the GPU (hipSetDeviceFlags before the first context)")
+    ap.add_argument("--graphs", type=int, default=0, choices=[0, 1],
+                    help="replay each context's matvec from a HIP graph (library option use_graphs)")
     ap.add_argument("--mode", choices=["solve", "batch", "serve"], default="solve",
                     help="solve: the 209k-node BVP (default); batch: 64 tanks at once; serve: time-stepping sessions")
     ap.add_argument("--slots", type=int, default=8, help="batch/serve: concurrent requests")
@@ -405,14 +512,32 @@ def main():
     ap.add_argument("--serve-max-n", type=int, default=200000)
     ap.add_argument("--profile-only", action="store_true",
                     help="setup + a 256 MiB device copy (counter calibration) + the timed matvecs; no solves")
+    ap.add_argument("--write-traffic", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"), default=None,
+                    help="summarise two rocprofv3 --pmc passes (FETCH_SIZE, WRITE_SIZE) of --profile-only into "
+                         "profiles/traffic_<workload>.txt and exit")
+    ap.add_argument("--cpu-baseline-only", type=int, default=0, metavar="MATVECS_PER_SOLVE",
+                    help="(child process) time the CPU baseline on the workload and print it as JSON")
     args = ap.parse_args()
+    if args.cpu_baseline_only:
+        m, d, phi, phin = case_workload(args.case) if args.case else make_workload(args.h)
+        print(json.dumps(cpu_baseline(m, d, phi, phin, args.p, args.leaf, args.max_level, args.cpu_baseline_only)))
+        return
+    if args.write_traffic:
+        fp32 = args.precision == "fp32_refine"
+        print(write_traffic(args.write_traffic[0], args.write_traffic[1], workload_key(args, fp32)))
+        return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(a single BVP is solved on one MI355X; see the sharding notes in DESIGN.md)", file=sys.stderr)
         sys.exit(2)
 
-    import torch
-
+    # no torch here: it bundles a HIP runtime of its own, which would be the one libbemfmm binds to if torch
+    # were imported first; device vectors come from the library's runtime (devbuf)
+    if args.schedule != "auto":  # how host threads wait for the GPU (before any context exists)
+        from bem_for_nonlinear_waves_amd import devbuf
+        flag = {"spin": 1, "yield": 2}[args.schedule]
+        rc = devbuf.hip().hipSetDeviceFlags(flag)
+        _log(f"hipSetDeviceFlags({args.schedule}) -> {rc}")
     from bem_for_nonlinear_waves_amd.bvp import device_name
 
     _log(f"device {device_name()}")
@@ -431,12 +556,12 @@ def main():
     t_mv, stage = (time_matvec(bvp, n, m.n_nodes, args.matvecs, True) if fp32 else (t_mv64, stage64))
 
     if args.profile_only:
-        cal = torch.empty(32 * 1024 * 1024, dtype=torch.float64, device="cuda")
-        cal2 = torch.empty_like(cal)
-        cal.fill_(1.0)
-        torch.cuda.synchronize()
-        cal2.copy_(cal)  # 256 MiB read + 256 MiB written: FETCH_SIZE/WRITE_SIZE calibration
-        torch.cuda.synchronize()
+        from bem_for_nonlinear_waves_amd import devbuf
+        cal = devbuf.DeviceArray.from_host(np.ones(32 * 1024 * 1024))
+        cal2 = devbuf.DeviceArray(cal.n)
+        devbuf.synchronize()
+        cal2.copy_from(cal)  # 256 MiB read + 256 MiB written: FETCH_SIZE/WRITE_SIZE calibration
+        devbuf.synchronize()
         _log("profile-only: done")
         return
 
@@ -452,7 +577,8 @@ def main():
         b64.close()
         rel = float(np.linalg.norm(x - x64) / np.linalg.norm(x64))
         fp64_cmp = {"solves_per_s": args.steps / t64, "p50_solve_ms": float(np.percentile(np.array(lat64) * 1e3, 50)),
-                    "gmres_iters": [i["iters"] for i in infos64], "speedup": solves_per_s / (args.steps / t64),
+                    "gmres_iters": [i["iters"] for i in infos64], "matvecs": [i["n_matvec"] for i in infos64],
+                    "speedup": solves_per_s / (args.steps / t64),
                     "rel_diff_solution_vs_fp64": rel, "matvec_ms": t_mv64 * 1e3}
         _log(f"fp32_refine vs fp64: {fp64_cmp['speedup']:.2f}x solves/s, solution rel diff {rel:.2e}")
 
@@ -470,16 +596,16 @@ def main():
     m2l_flops = st["n_m2l_pairs"] * 2 * (rot_macs if rot else C * C) * 2.0
     near_bytes = (st["near_nnz"] + m.n_nodes * C) * 2 * esz  # dense G/Gn weights incl. L2P columns
     dom = int(np.argmax(stage))
+    # boxes that receive M2L: each target's local expansion is written once (2 C doubles)
+    m2l_targets = st.get("n_m2l_targets", st["n_boxes"])
     if STAGES[dom] == "m2l":
         ach = m2l_flops / (stage[dom] * 1e-3) / 1e12
         if rot:
-            # 2.84e10 FP32 flop against >= 4.5 GB per launch is ~6 flop/B, below the ridge point (157.3 TFLOP/s /
-            # 8 TB/s = 19.7 flop/B): the kernel sits on the bandwidth side of the roofline. Algorithmic bytes per
-            # pair: the source's two gathered FP32 multipole columns (2 C x 4 B) and the two staged result
-            # columns (2 CS x 4 B, CS = C rounded up to 32); the factor loads and the multipole pack in the
-            # stage time are not counted (a conservative rate). Its FP32 FMA rate is reported beside it.
-            CS = (C + 31) // 32 * 32
-            rot_bytes = st["n_m2l_pairs"] * (2 * C + 2 * CS) * 4.0
+            # 2.84e10 FP32 flop against 2.2 GB of source gathers per launch is ~13 flop/B, below the ridge point
+            # (157.3 TFLOP/s / 8 TB/s = 19.7 flop/B): the rotation M2L sits on the bandwidth side of the roofline.
+            # Its algorithmic bytes are the gathers of the source multipoles, each pair's two FP32 columns
+            # (2 C x 4 B); the per-pair staging it writes for the reduction is not algorithmic work.
+            rot_bytes = st["n_m2l_pairs"] * 2 * C * 4.0
             achb = rot_bytes / (stage[dom] * 1e-3) / 1e9
             roof = {"kernel": "k_m2l_rot (M2L by rotation, FP32 FMA)", "bound": "hbm", "achieved": achb,
                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achb / HBM_PEAK_GBS, "traffic": None,
@@ -490,10 +616,23 @@ def main():
     else:
         ach = near_bytes / (stage[dom] * 1e-3) / 1e9
         roof = {"kernel": f"k_block_gemv ({STAGES[dom]})", "bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "traffic": None}
-    default_workload = not args.case and (args.h, args.p, args.leaf, args.max_level) == (0.036, 10, 32, 7)
-    if default_workload:
-        roof["traffic"] = MEASURED_TRAFFIC.get(("m2l_rot" if rot and STAGES[dom] == "m2l" else STAGES[dom], fp32))
+                "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "traffic": None, "algorithmic_bytes": near_bytes}
+    # the M2L stage as a whole (translation + per-box reduction): algorithmic bytes = source gathers + each
+    # target's locals written once
+    m2l_alg = st["n_m2l_pairs"] * 2 * C * esz + m2l_targets * 2 * C * 8.0
+    roof["m2l_stage"] = {"ms": float(stage[3] + stage[4]), "algorithmic_bytes": m2l_alg,
+                         "achieved_gbs": m2l_alg / ((stage[3] + stage[4]) * 1e-3) / 1e9}
+    # HBM bytes per launch from the rocprofv3 counter passes of this device code (profiles/), if they exist
+    if rot and STAGES[dom] == "m2l":
+        tkey = f"k_m2l_rot<{args.p}>"
+    elif STAGES[dom] == "m2l":
+        tkey = f"k_translate_mfma<double,{(C + 63) // 64},true,{st['m2l_tile_pairs']}>"
+    else:
+        tkey = "k_block_gemv<float>" if fp32 else "k_block_gemv<double>"
+    traffic = measured_traffic(tkey, workload_key(args, fp32))
+    if traffic is not None:
+        roof["traffic"] = traffic["bytes"]
+        roof["traffic_source"] = traffic["source"]
     roof["stage_ms"] = dict(zip(STAGES, [round(float(v), 4) for v in stage]))
     roof["m2l_tflops"] = m2l_flops / (stage[3] * 1e-3) / 1e12
     if rot:  # the same M2L as dense operator multiply-adds per second
@@ -504,8 +643,10 @@ def main():
     cpu = None
     if not args.no_cpu_baseline:
         try:
-            cpu = cpu_baseline(args.p, args.leaf, args.max_level, int(np.median(iters)), m.n_nodes)
-            _log(f"cpu baseline: {cpu['sample']}: {cpu['matvec_mnode_per_s']:.3f} Mnode/s")
+            # matvecs of one FP64 solve on the GPU: the iterations + the FP64 residual checks
+            mv_fp64 = (int(np.median(fp64_cmp["matvecs"])) if fp64_cmp else int(np.median([i["n_matvec"] for i in infos])))
+            cpu = cpu_baseline_child(args, mv_fp64)
+            _log(f"cpu baseline: {cpu['sample']}: {cpu['matvec_mnode_per_s']:.4f} Mnode/s")
         except Exception as e:  # the baseline never decides the GPU result
             _log(f"cpu baseline failed: {e}")
 

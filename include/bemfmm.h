@@ -72,8 +72,11 @@ typedef struct bemfmm_opts {
   int dof_reorder;       /* 1 (default): number DOFs internally in Morton order of their points, as solveGMRES
                             does (BEM_solveBVP_GMRES_FMM.hpp:3164-3246); 0: keep the caller's numbering.
                             Every array of this ABI stays in the caller's numbering either way. */
-  int use_graphs;        /* 1 (default): replay the matvec from a HIP graph captured at the first matvec; 0: launch
-                            its kernels one by one */
+  int use_graphs;        /* 0 (default): launch the matvec's kernels one by one; 1: replay it from a HIP graph captured
+                            at the first matvec. With graphs, the ROCm 7 runtime kept the buffers a context's graphs
+                            had used allocated after a re-prepare or teardown freed them (measured on the MI355X:
+                            4.1 GB per re-prepare at 80k nodes), and the eager launches were as fast at 209k nodes
+                            (2.549 against 2.569 ms per FP32 matvec) */
   int m2l_kernel;        /* M2L of the FP32 operator (FP32_REFINE): 0 (default) = rotation where compiled for p
                             (8, 10), else matrix cores; 1 = dense matrix-core tiles; 2 = rotation (error if
                             not compiled for p). The FP64 operator always uses the dense matrix-core tiles. */

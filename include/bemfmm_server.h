@@ -35,7 +35,7 @@ typedef struct bemfmm_server_opts {
   int device;                /* HIP device */
   int slots;                 /* concurrent requests (worker threads / streams), 1..64 */
   double hbm_budget_bytes;   /* device memory the server may hold (estimated, then measured per context) */
-  int omp_threads_per_slot;  /* host threads for each request's setup; 0 = cores / slots */
+  int omp_threads_per_slot;  /* host threads for each request's setup; 0 = omp_get_max_threads() / slots */
 } bemfmm_server_opts;
 
 /* One BVP: the arrays of bemfmm_set_geometry / bemfmm_set_bc, the context options and the arguments
@@ -63,6 +63,7 @@ typedef struct bemfmm_server_stats {
   int running, resident_sessions;
   double p50_latency_s, p99_latency_s;  /* submit -> result, completed requests */
   double used_bytes, peak_bytes;        /* device memory accounted to contexts */
+  int pooled_contexts;                  /* idle contexts kept for reuse by later requests */
 } bemfmm_server_stats;
 
 void bemfmm_server_default_opts(bemfmm_server_opts* opts);

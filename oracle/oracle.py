@@ -33,6 +33,26 @@ def build(force: bool = False) -> str:
     return _LIB_PATH
 
 
+def use_native() -> str:
+    """Build (on this machine) and select the -O3 -march=native variant of the library, as the timed CPU
+    baseline is built; call before the first lib()."""
+    global _LIB_PATH
+    if _lib is not None:
+        raise RuntimeError("oracle library already loaded")
+    import hashlib
+    import platform
+    cpu = platform.node()
+    try:
+        with open("/proc/cpuinfo") as f:
+            cpu += "".join(l for l in f if l.startswith(("model name", "flags")))
+    except OSError:
+        pass
+    path = os.path.join(_HERE, "_build", f"liboracle_native_{hashlib.sha256(cpu.encode()).hexdigest()[:12]}.so")
+    subprocess.run(["make", "-s", "-C", _HERE, "native", f"NATIVE={path}"], check=True)
+    _LIB_PATH = path
+    return _LIB_PATH
+
+
 def lib():
     global _lib
     if _lib is None:

@@ -81,6 +81,7 @@ bemfmm_opts options(bool fp32, int p) {
   o.leaf_max = 32;
   o.precision = fp32 ? BEMFMM_PRECISION_FP32_REFINE : BEMFMM_PRECISION_FP64;
   o.preconditioner = BEMFMM_PRECOND_SCHWARZ;
+  o.use_graphs = 1;  // the capture path (and its gate against other threads' frees) is the one to exercise
   return o;
 }
 
