@@ -26,7 +26,7 @@ class Opts(ctypes.Structure):
                 ("preconditioner", ctypes.c_int), ("refine_eta", ctypes.c_double), ("schwarz_max_core", ctypes.c_int),
                 ("schwarz_max_block", ctypes.c_int), ("schwarz_overlap", ctypes.c_int), ("m2l_tile_pairs", ctypes.c_int),
                 ("dof_reorder", ctypes.c_int), ("use_graphs", ctypes.c_int), ("m2l_kernel", ctypes.c_int),
-                ("overlap_near", ctypes.c_int)]
+                ("overlap_near", ctypes.c_int), ("refine_recycle", ctypes.c_int)]
 
 
 class Result(ctypes.Structure):

@@ -1404,6 +1404,228 @@ double gmres_run(bemfmm_ctx* c, const GmresRun& R, int& iters) {
   return cur_err;
 }
 
+// Least-squares problem min |g - H y| of a Krylov solve whose columns may have several entries below the
+// diagonal, by Givens rotations: each new column gets the rotations so far, then its entries below the diagonal
+// are zeroed bottom-up (rows (i-1, i)), and the same rotations go to g. With one entry below the diagonal (an
+// Arnoldi column) this is the Givens update of basic_linear_systems.hpp:3137-3451.
+struct GivensLsq {
+  struct Rot {
+    int i;
+    double c, s;
+  };
+  std::vector<Rot> rot;
+  std::vector<std::vector<double>> R;  // rotated columns (upper triangular part used)
+  std::vector<double> g;               // rotated right-hand side
+  static void apply(const Rot& r, std::vector<double>& v) {
+    if (r.i + 1 >= (int)v.size()) return;
+    const double a = v[r.i], b = v[r.i + 1];
+    v[r.i] = std::fma(r.c, a, r.s * b);
+    v[r.i + 1] = std::fma(r.c, b, -r.s * a);
+  }
+  void reset(const std::vector<double>& rhs) {
+    rot.clear();
+    R.clear();
+    g = rhs;
+  }
+  void add_column(std::vector<double> col) {
+    if (col.size() > g.size()) g.resize(col.size(), 0.);
+    for (const Rot& r : rot) apply(r, col);
+    const int k = (int)R.size();
+    for (int i = (int)col.size() - 1; i > k; --i) {
+      if (col[i] == 0.) continue;
+      const double h = std::hypot(col[i - 1], col[i]);
+      const Rot r{i - 1, col[i - 1] / h, col[i] / h};
+      apply(r, col);
+      col[i] = 0.;
+      apply(r, g);
+      rot.push_back(r);
+    }
+    R.push_back(std::move(col));
+  }
+  // |g - H y| at the minimiser: the rotated right-hand side below the triangle
+  double residual() const {
+    double s = 0.;
+    for (size_t i = R.size(); i < g.size(); ++i) s += g[i] * g[i];
+    return std::sqrt(s);
+  }
+  std::vector<double> solve() const {
+    const int k = (int)R.size();
+    std::vector<double> y(k, 0.);
+    for (int i = k - 1; i >= 0; --i) {
+      double sum = g[i];
+      for (int j = i + 1; j < k; ++j) sum -= R[j][i] * y[j];
+      y[i] = sum / R[i][i];
+    }
+    return y;
+  }
+};
+
+// Krylov space of the first inner solve of a refinement loop, kept for the second: orthonormal basis
+// W_0..W_{nw-1} (the context's basis buffer) and the Arnoldi columns H[j] with M^-1 A_32 W_j = W H[j].
+// Every inner solve has the same operator M^-1 A_32. After a plain restart, GMRES has to resolve again the
+// slowly converging directions that the first solve already found; GCRO (de Sturler 1996; Parks et al.
+// 2006) keeps C = range(M^-1 A_32 W_{0..k-1}) and runs the second solve's Arnoldi on (I - C C^T) M^-1 A_32.
+struct RecycledKrylov {
+  int nw = 0;
+  std::vector<std::vector<double>> H;
+};
+
+// Thin QR of the nw x k matrix with columns H (modified Gram-Schmidt, twice): Q (k columns of length nw),
+// R (k columns, upper triangular). Returns false if a column is dependent on the earlier ones.
+bool small_qr(const std::vector<std::vector<double>>& H, int nw, std::vector<std::vector<double>>& Q,
+              std::vector<std::vector<double>>& R) {
+  const int k = (int)H.size();
+  Q.assign(k, std::vector<double>(nw, 0.));
+  R.assign(k, std::vector<double>(k, 0.));
+  for (int j = 0; j < k; ++j) {
+    std::vector<double> v(nw, 0.);
+    for (size_t i = 0; i < H[j].size() && (int)i < nw; ++i) v[i] = H[j][i];
+    double n0 = 0.;
+    for (double x : v) n0 += x * x;
+    for (int pass = 0; pass < 2; ++pass)
+      for (int i = 0; i < j; ++i) {
+        double h = 0.;
+        for (int t = 0; t < nw; ++t) h += Q[i][t] * v[t];
+        for (int t = 0; t < nw; ++t) v[t] -= h * Q[i][t];
+        R[j][i] += h;
+      }
+    double nn = 0.;
+    for (double x : v) nn += x * x;
+    nn = std::sqrt(nn);
+    if (!(nn > 1e-12 * std::sqrt(n0))) return false;
+    R[j][j] = nn;
+    for (int t = 0; t < nw; ++t) Q[j][t] = v[t] / nn;
+  }
+  return true;
+}
+
+// One inner solve of the refinement loop: d (device, overwritten) ~ argmin |r - M^-1 A_32 d|, until the
+// estimate is <= tol or `budget` matvecs are spent. Without kept state this is GMRES from r and its Krylov
+// space is kept in K; with kept state (the first solve's), the search space is the kept directions plus the
+// Krylov space of (I - C C^T) M^-1 A_32 from (I - C C^T) r, new vectors orthogonalised against C (held as
+// combinations Q_C of the kept basis) and against each other, both twice (classical Gram-Schmidt); K is
+// emptied afterwards (a third step restarts). Returns the estimate; iters counts the matvecs.
+double refine_inner(bemfmm_ctx* c, RecycledKrylov& K, const double* r, double* d, double tol, int budget,
+                    int max_basis, int& iters) {
+  const int64_t n = c->n_dof;
+  hipStream_t s = c->stream;
+  auto basis = [&](int i) { return c->V.p + (int64_t)i * n; };
+  std::vector<std::vector<double>> QC, RC;
+  const bool recycle = !K.H.empty() && K.nw + 2 < max_basis && small_qr(K.H, K.nw, QC, RC);
+  const int nk = recycle ? K.nw : 0;      // kept physical vectors W_0..W_{nk-1}
+  const int kc = recycle ? (int)K.H.size() : 0;  // kept directions = columns of C
+  // projection of the physical vector at position p onto C (coefficients over W in `pw`, over the new
+  // vectors unchanged): the first nk dot products a become Q_C Q_C^T a; returns Q_C^T a
+  auto project_c = [&](std::vector<double>& coef) {
+    std::vector<double> cc(kc, 0.);
+    for (int j = 0; j < kc; ++j)
+      for (int i = 0; i < nk; ++i) cc[j] += QC[j][i] * coef[i];
+    for (int i = 0; i < nk; ++i) {
+      double v = 0.;
+      for (int j = 0; j < kc; ++j) v += QC[j][i] * cc[j];
+      coef[i] = v;
+    }
+    return cc;
+  };
+  // w <- w minus its components along C and the new vectors [nk, nb), twice; returns the coefficients
+  // (over C, then the new vectors) and |w|^2 in `norm2w`
+  auto orthogonalise = [&](double* w, int nb, double& norm2w) {
+    std::vector<double> total(kc + nb - nk, 0.);
+    for (int pass = 0; pass < 2; ++pass) {
+      dots_dev(c, nb, c->V.p, w, 0);
+      HIPCHECK(hipMemcpyAsync(c->hres, c->hbuf.p, nb * sizeof(double), hipMemcpyDeviceToHost, s));
+      HIPCHECK(hipStreamSynchronize(s));
+      std::vector<double> coef(c->hres, c->hres + nb);
+      const std::vector<double> cc = project_c(coef);
+      for (int j = 0; j < kc; ++j) total[j] += cc[j];
+      for (int i = nk; i < nb; ++i) total[kc + i - nk] += coef[i];
+      HIPCHECK(hipMemcpyAsync(c->hbuf.p, coef.data(), nb * sizeof(double), hipMemcpyHostToDevice, s));
+      dev::vsum(s, c->V.p, n, nb, c->hbuf.p, -1.0, w, n);
+    }
+    std::vector<double> h;
+    dots(c, 1, w, w, h);
+    norm2w = h[0];
+    return total;
+  };
+  if (!recycle) K = RecycledKrylov{};
+  // first new vector: r without its components along C
+  ensure_basis(c, nk + 2);
+  double* w = basis(nk);
+  HIPCHECK(hipMemcpyAsync(w, r, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+  std::vector<double> g;
+  double zn2 = 0.;
+  if (nk > 0) {
+    g = orthogonalise(w, nk, zn2);  // C^T r
+  } else {
+    std::vector<double> h;
+    dots(c, 1, w, w, h);
+    zn2 = h[0];
+  }
+  const double zn = std::sqrt(zn2);
+  if (!std::isfinite(zn)) throw HipError("refinement: non-finite residual", BEMFMM_E_NONFINITE);
+  g.push_back(zn);
+  GivensLsq ls;
+  ls.reset(g);
+  for (int j = 0; j < kc; ++j) ls.add_column(RC[j]);  // images of the kept directions, in C coordinates
+  int nb = nk;
+  if (zn > 0.) {
+    scale_basis(c, 1.0 / zn, w);
+    nb = nk + 1;
+  }
+  double est = ls.residual();
+  int used = 0;
+  bool more = zn > 0.;
+  std::vector<std::vector<double>> Hnew;
+  while (est > tol && used < budget && more && nb < max_basis) {
+    ensure_basis(c, nb + 1);
+    w = basis(nb);
+    apply_op(c, basis(nb - 1), w, true);
+    double hn2 = 0.;
+    std::vector<double> col;
+    if (nk > 0) {
+      col = orthogonalise(w, nb, hn2);
+    } else {
+      // plain Arnoldi: both Gram-Schmidt passes and the norm on the device, one copy to the host
+      dots_dev(c, nb, c->V.p, w, 0);
+      dev::vsum(s, c->V.p, n, nb, c->hbuf.p, -1.0, w, n);
+      dots_dev(c, nb, c->V.p, w, nb);
+      dev::vsum(s, c->V.p, n, nb, c->hbuf.p + nb, -1.0, w, n);
+      dots_dev(c, 1, w, w, 2 * (size_t)nb);
+      HIPCHECK(hipMemcpyAsync(c->hres, c->hbuf.p, (2 * (size_t)nb + 1) * sizeof(double), hipMemcpyDeviceToHost, s));
+      HIPCHECK(hipStreamSynchronize(s));
+      col.resize(nb);
+      for (int i = 0; i < nb; ++i) col[i] = c->hres[i] + c->hres[nb + i];
+      hn2 = c->hres[2 * nb];
+    }
+    const double hn = std::sqrt(hn2);
+    col.push_back(hn);
+    scale_basis(c, hn > 0. ? 1.0 / hn : 0.0, w);
+    more = hn > 0.;  // an exact breakdown leaves nothing new to expand
+    ++nb;
+    if (nk == 0) K.H.push_back(col);
+    ls.add_column(std::move(col));
+    ++used;
+    ++iters;
+    est = ls.residual();
+    c->history.push_back(est);
+    if (!std::isfinite(est)) throw HipError("GMRES breakdown: non-finite residual estimate", BEMFMM_E_NONFINITE);
+  }
+  // d = W_{0..kc-1} y_kept + new vectors y_new (the kept directions are the kept basis vectors themselves)
+  const std::vector<double> y = ls.solve();
+  std::vector<double> coef(nb, 0.);
+  for (int j = 0; j < kc; ++j) coef[j] = y[j];
+  for (size_t j = kc; j < y.size(); ++j) coef[nk + (j - kc)] = y[j];
+  fill(c, d, n, 0.);
+  if (!y.empty()) {
+    HIPCHECK(hipMemcpyAsync(c->hbuf.p, coef.data(), coef.size() * sizeof(double), hipMemcpyHostToDevice, s));
+    dev::vsum(s, c->V.p, n, nb, c->hbuf.p, 1.0, d, n);
+  }
+  HIPCHECK(hipStreamSynchronize(s));
+  if (nk == 0) K.nw = nb;
+  else K = RecycledKrylov{};
+  return est;
+}
+
 // FP64 preconditioned residual r = M^-1 (b - A x) into out; returns |r|_2.
 double fp64_residual(bemfmm_ctx* c, const double* x, double* out) {
   const int64_t n = c->n_dof;
@@ -1446,6 +1668,7 @@ int do_solve(bemfmm_ctx* c, const double* x0, double* xout, double tol, int max_
     double prev = 1e300;
     bool fallback = false;
     bool x_zero = x0_zero;
+    RecycledKrylov K;
     while (true) {
       if (x_zero) {
         HIPCHECK(hipMemcpyAsync(c->rvec.p, c->bvec.p, n * sizeof(double), hipMemcpyDeviceToDevice, s));
@@ -1462,9 +1685,16 @@ int do_solve(bemfmm_ctx* c, const double* x0, double* xout, double tol, int max_
         break;
       }
       prev = cur_err;
-      fill(c, c->dvec.p, n, 0.);
-      GmresRun in{c->rvec.p, c->dvec.p, std::max(eta * cur_err, 0.5 * tol), max_iter, restart, true, false, true};
-      gmres_run(c, in, iters);
+      const double inner_tol = std::max(eta * cur_err, 0.5 * tol);
+      if (c->opts.refine_recycle) {
+        // the second step reuses the first step's Krylov space (GCRO); the basis holds at most restart + 1 vectors
+        const int max_basis = std::max(restart, 1) + 1;
+        refine_inner(c, K, c->rvec.p, c->dvec.p, inner_tol, max_iter - iters, max_basis, iters);
+      } else {
+        fill(c, c->dvec.p, n, 0.);
+        GmresRun in{c->rvec.p, c->dvec.p, inner_tol, max_iter, restart, true, false, true};
+        gmres_run(c, in, iters);
+      }
       dev::axpby(s, n, 1.0, c->dvec.p, 1.0, c->xsol.p);
       ++steps;
     }
@@ -1563,6 +1793,7 @@ void bemfmm_default_opts(bemfmm_opts* o) {
   o->use_graphs = 0;
   o->m2l_kernel = 0;
   o->overlap_near = 1;
+  o->refine_recycle = 1;
 }
 
 bemfmm_ctx* bemfmm_ctx_create(int device, const bemfmm_opts* opts) {

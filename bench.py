@@ -226,7 +226,7 @@ def setup_bvp(m, d, phi, phin, args, precision):
                   refine_eta=args.refine_eta, m2l_tile_pairs=args.m2l_tile_pairs,
                   m2l_kernel={"auto": 0, "dense": 1, "rotation": 2}[args.m2l_kernel],
                   schwarz_max_core=args.schwarz_core, schwarz_max_block=args.schwarz_block,
-                  schwarz_overlap=args.schwarz_overlap, use_graphs=args.graphs)
+                  schwarz_overlap=args.schwarz_overlap, use_graphs=args.graphs, refine_recycle=args.refine_recycle)
     bvp.set_bc(phi, phin)
     t = time.perf_counter()
     bvp.prepare()
@@ -482,6 +482,8 @@ def main():
     ap.add_argument("--precond", choices=["none", "jacobi", "schwarz"], default="schwarz")
     ap.add_argument("--precision", choices=["fp32_refine", "fp64"], default="fp32_refine")
     ap.add_argument("--refine-eta", type=float, default=3e-7)
+    ap.add_argument("--refine-recycle", type=int, default=1, choices=[0, 1],
+                    help="refinement steps after the first keep the earlier steps' Krylov directions (library option)")
     ap.add_argument("--m2l-tile-pairs", type=int, default=0, choices=[0, 16, 32, 64])
     ap.add_argument("--case", default=None,
                     help="solve a case in the reference's input format (path to its settings.json) instead of the cylinder tank")

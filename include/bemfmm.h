@@ -82,6 +82,9 @@ typedef struct bemfmm_opts {
                             not compiled for p). The FP64 operator always uses the dense matrix-core tiles. */
   int overlap_near;      /* 1 (default): the near-field columns of each matvec run on a second stream beside the
                             FMM passes (they need only the input vector); 0: after L2L, one GEMV */
+  int refine_recycle;    /* FP32_REFINE: 1 (default): each inner FP32 solve after the first minimises over the
+                            earlier steps' Krylov directions too (they share the operator M^-1 A_32) and extends
+                            that basis, up to `restart` + 1 vectors; 0: every step restarts GMRES from its residual */
 } bemfmm_opts;
 
 typedef struct bemfmm_result {
