@@ -17,7 +17,7 @@ _ip = ctypes.POINTER(ctypes.c_int32)
 OK = 0
 E_INVALID, E_DEVICE, E_OOM, E_NONFINITE, E_NOT_CONVERGED, E_STATE = -1, -2, -3, -4, -5, -6
 PRECISION_FP64, PRECISION_FP32_REFINE = 0, 1
-PRECOND_NONE, PRECOND_JACOBI, PRECOND_SCHWARZ = 0, 1, 2
+PRECOND_NONE, PRECOND_JACOBI, PRECOND_SCHWARZ, PRECOND_ILU0 = 0, 1, 2, 3
 
 
 class Opts(ctypes.Structure):
@@ -26,7 +26,8 @@ class Opts(ctypes.Structure):
                 ("preconditioner", ctypes.c_int), ("refine_eta", ctypes.c_double), ("schwarz_max_core", ctypes.c_int),
                 ("schwarz_max_block", ctypes.c_int), ("schwarz_overlap", ctypes.c_int), ("m2l_tile_pairs", ctypes.c_int),
                 ("dof_reorder", ctypes.c_int), ("use_graphs", ctypes.c_int), ("m2l_kernel", ctypes.c_int),
-                ("overlap_near", ctypes.c_int), ("refine_recycle", ctypes.c_int)]
+                ("overlap_near", ctypes.c_int), ("refine_recycle", ctypes.c_int),
+                ("ilu_neighborhood", ctypes.c_int), ("ilu_kring", ctypes.c_int)]
 
 
 class Result(ctypes.Structure):
@@ -45,7 +46,7 @@ class Stats(ctypes.Structure):
                 ("t_alpha", ctypes.c_double), ("n_precond_blocks", ctypes.c_int),
                 ("precond_fallback_blocks", ctypes.c_int), ("precond_avg_block", ctypes.c_double),
                 ("n_ops_uploaded", ctypes.c_int), ("m2l_tile_pairs", ctypes.c_int),
-                ("m2l_rotation", ctypes.c_int)]
+                ("m2l_rotation", ctypes.c_int), ("precond_nnz", ctypes.c_int64), ("precond_levels", ctypes.c_int)]
 
 
 # (name, restype, argtypes) of every entry point declared in include/bemfmm.h

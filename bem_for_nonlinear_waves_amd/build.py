@@ -1,7 +1,7 @@
 """Builds libbemfmm.so in-tree for gfx950.
 
-Host code (plan.cpp, operators.cpp, schwarz.cpp: the plan, its float128
-operators and the Schwarz blocks; bemfmm.cpp: contexts, launch sequences, GMRES and
+Host code (plan.cpp, operators.cpp, schwarz.cpp, precond.cpp: the plan, its float128
+operators, the Schwarz blocks and the ILU(0) matrix; bemfmm.cpp: contexts, launch sequences, GMRES and
 the C ABI over the HIP runtime API; server.cpp: the request loop over the C ABI)
 is compiled with g++ (OpenMP via libgomp, __float128 via libquadmath, threads);
 the device kernels and their launchers (kernels.hip) with hipcc for gfx950; the
@@ -47,7 +47,7 @@ def build(verbose: bool = True) -> str:
     hipcc = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
     os.makedirs(OUT_DIR, exist_ok=True)
     objs = []
-    for src in ("plan.cpp", "operators.cpp", "schwarz.cpp", "server.cpp", "bemfmm.cpp"):
+    for src in ("plan.cpp", "operators.cpp", "schwarz.cpp", "precond.cpp", "server.cpp", "bemfmm.cpp"):
         obj = os.path.join(OUT_DIR, src.replace(".cpp", ".o"))
         _run(["g++", *CXX_FLAGS, *HIP_API_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj])
         objs.append(obj)

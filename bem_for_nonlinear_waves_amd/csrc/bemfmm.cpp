@@ -288,6 +288,18 @@ struct bemfmm_ctx {
   DBuf<int64_t> sw_ptr, sw_invptr;
   DBuf<int32_t> sw_idx, sw_ncore;
   DBuf<double> sw_inv;
+  // ILU(0) of the reference's sparse preconditioner matrix: factored rows and the sweeps' level schedules
+  struct IluSegment {
+    int l0, l1;    // levels [l0, l1) of the sweep
+    bool single;   // one large level: one thread per row over the grid
+  };
+  DBuf<int64_t> ilu_ptr, ilu_diag;
+  DBuf<int32_t> ilu_col, ilu_lrows, ilu_lptr, ilu_urows, ilu_uptr;
+  DBuf<double> ilu_val;
+  std::vector<int32_t> ilu_lptr_h, ilu_uptr_h;
+  std::vector<IluSegment> ilu_lseg, ilu_useg;
+  int64_t ilu_nnz = 0;
+  int ilu_levels = 0;
   // setup inputs kept with the context (grow-only, reused by every re-prepare: a free would synchronise the
   // whole device, including the other contexts' work)
   struct SetupBufs {
@@ -836,6 +848,9 @@ void do_prepare(bemfmm_ctx* c) {
     throw std::runtime_error("bemfmm_prepare: m2l_kernel must be 0 (default), 1 (matrix cores) or 2 (rotation)");
   if (c->opts.m2l_kernel == 2 && !dev::m2l_rot_compiled(c->opts.p))
     throw std::runtime_error("bemfmm_prepare: the rotation M2L is compiled for p = 8 and 10 only");
+  if (c->opts.preconditioner == BEMFMM_PRECOND_ILU0 &&
+      (c->opts.ilu_neighborhood < 0 || c->opts.ilu_neighborhood > 1 || c->opts.ilu_kring < 0 || c->opts.ilu_kring > 20))
+    throw std::runtime_error("bemfmm_prepare: ilu_neighborhood must be 0 (BUCKETS) or 1 (K-RING), 0 <= ilu_kring <= 20");
   if (c->opts.preconditioner == BEMFMM_PRECOND_SCHWARZ &&
       (c->opts.schwarz_max_core < 1 || c->opts.schwarz_max_block < c->opts.schwarz_max_core ||
        c->opts.schwarz_max_block > kSchwarzMaxBlock || c->opts.schwarz_overlap < 0 || c->opts.schwarz_overlap > 8))
@@ -1106,6 +1121,49 @@ void do_prepare(bemfmm_ctx* c) {
       c->sw_failed = 0;
       for (int32_t f : fh) c->sw_failed += f;
     }
+  } else if (pc == BEMFMM_PRECOND_ILU0) {
+    // the reference's sparse matrix (buildSparseMatrixForILU) on the host from the scaled geometry and alpha,
+    // ILU(0) factorised on the host as ILU0_CRS does; the sweeps run on the GPU by dependency level
+    std::vector<double> ah(c->n_dof);
+    HIPCHECK(hipMemcpyAsync(ah.data(), c->alpha.p, c->n_dof * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPCHECK(hipStreamSynchronize(s));
+    bemfmm::SparseRows A;
+    bemfmm::build_ilu_rows(c->n_nodes, pl.Xs.data(), c->n_faces, pl.fv.data(), c->face_dof_i.data(), c->n_dof,
+                           c->kind_i.data(), c->point_i.data(), ah.data(), pl.scale, c->opts.ilu_neighborhood,
+                           c->opts.ilu_kring, A);
+    bemfmm::ilu0_factor(A);
+    bemfmm::SweepLevels Lv, Uv;
+    bemfmm::sweep_levels(A, Lv, Uv);
+    c->ilu_nnz = (int64_t)A.col.size();
+    c->ilu_ptr.upload(A.ptr, s);
+    c->ilu_diag.upload(A.diag, s);
+    c->ilu_col.upload(A.col, s);
+    c->ilu_val.upload(A.val, s);
+    c->ilu_lrows.upload(Lv.rows, s);
+    c->ilu_lptr.upload(Lv.ptr, s);
+    c->ilu_urows.upload(Uv.rows, s);
+    c->ilu_uptr.upload(Uv.ptr, s);
+    c->ilu_lptr_h = Lv.ptr;
+    c->ilu_uptr_h = Uv.ptr;
+    // runs of small levels in one workgroup each, large levels alone
+    auto segments = [](const std::vector<int32_t>& ptr, std::vector<bemfmm_ctx::IluSegment>& seg) {
+      seg.clear();
+      const int nl = (int)ptr.size() - 1;
+      for (int l = 0; l < nl;) {
+        if (ptr[l + 1] - ptr[l] >= dev::kIluBigLevel) {
+          seg.push_back({l, l + 1, true});
+          ++l;
+          continue;
+        }
+        int e = l;
+        while (e < nl && ptr[e + 1] - ptr[e] < dev::kIluBigLevel) ++e;
+        seg.push_back({l, e, false});
+        l = e;
+      }
+    };
+    segments(Lv.ptr, c->ilu_lseg);
+    segments(Uv.ptr, c->ilu_useg);
+    c->ilu_levels = Lv.n_levels() + Uv.n_levels();
   } else if (pc != BEMFMM_PRECOND_NONE) {
     throw std::runtime_error("unknown preconditioner");
   }
@@ -1175,6 +1233,8 @@ void do_prepare(bemfmm_ctx* c) {
   st.n_ops_uploaded = c->ops_uploaded;
   st.m2l_tile_pairs = c->m2l_tp;
   st.m2l_rotation = c->m2l_rot ? 1 : 0;
+  st.precond_nnz = pc == BEMFMM_PRECOND_ILU0 ? c->ilu_nnz : 0;
+  st.precond_levels = pc == BEMFMM_PRECOND_ILU0 ? c->ilu_levels : 0;
   (void)t0;
 }
 
@@ -1231,6 +1291,21 @@ void precond_apply(bemfmm_ctx* c, const double* in, double* out) {
     if (in == out) throw std::runtime_error("internal: Schwarz apply needs distinct buffers");
     dev::schwarz_apply(c->stream, schwarz_view(c), c->sw_inv.p, in, out);
     break;
+  case BEMFMM_PRECOND_ILU0: {
+    if (in != out) HIPCHECK(hipMemcpyAsync(out, in, n * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    dev::IluRows A{c->ilu_ptr.p, c->ilu_col.p, c->ilu_val.p, c->ilu_diag.p};
+    for (const auto& g : c->ilu_lseg) {
+      if (g.single) dev::ilu_level(c->stream, A, c->ilu_lrows.p + c->ilu_lptr_h[g.l0],
+                                   c->ilu_lptr_h[g.l0 + 1] - c->ilu_lptr_h[g.l0], false, out);
+      else dev::ilu_run(c->stream, A, c->ilu_lrows.p, c->ilu_lptr.p, g.l0, g.l1, false, out);
+    }
+    for (const auto& g : c->ilu_useg) {
+      if (g.single) dev::ilu_level(c->stream, A, c->ilu_urows.p + c->ilu_uptr_h[g.l0],
+                                   c->ilu_uptr_h[g.l0 + 1] - c->ilu_uptr_h[g.l0], true, out);
+      else dev::ilu_run(c->stream, A, c->ilu_urows.p, c->ilu_uptr.p, g.l0, g.l1, true, out);
+    }
+    break;
+  }
   default:
     if (in != out) HIPCHECK(hipMemcpyAsync(out, in, n * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
   }
@@ -1794,6 +1869,8 @@ void bemfmm_default_opts(bemfmm_opts* o) {
   o->m2l_kernel = 0;
   o->overlap_near = 1;
   o->refine_recycle = 1;
+  o->ilu_neighborhood = 0;  // BUCKETS (main_time_domain.cpp:39)
+  o->ilu_kring = 1;         // main_time_domain.cpp:40
 }
 
 bemfmm_ctx* bemfmm_ctx_create(int device, const bemfmm_opts* opts) {

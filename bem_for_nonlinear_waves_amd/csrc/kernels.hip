@@ -1307,6 +1307,37 @@ __device__ __forceinline__ uint64_t spread3(uint32_t v) {
 }
 // key of each DOF from its point, quantised in the bounding box of all points as the reference does
 // (t = (v - v0)/(v1 - v0) clamped to [0, 1], q = (uint32)(t * (2^21 - 1) + 0.5)); vals = DOF index
+// ILU(0) sweeps: one row per thread, forward z_i -= sum_{q < diag} L_iq z_col, backward
+// z_i = (z_i - sum_{q > diag} U_iq z_col) / U_ii. The columns a row reads belong to earlier levels.
+__device__ __forceinline__ void ilu_row(const IluRows& A, int32_t i, bool backward, double* z) {
+  const int64_t d = A.diag[i];
+  double s = z[i];
+  if (!backward) {
+    for (int64_t q = A.ptr[i]; q < d; ++q) s -= A.val[q] * z[A.col[q]];
+    z[i] = s;
+  } else {
+    for (int64_t q = d + 1; q < A.ptr[i + 1]; ++q) s -= A.val[q] * z[A.col[q]];
+    z[i] = s / A.val[d];
+  }
+}
+__global__ __launch_bounds__(256) void k_ilu_level(IluRows A, const int32_t* __restrict__ rows, int64_t nrows,
+                                                   bool backward, double* z) {
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (t < nrows) ilu_row(A, rows[t], backward, z);
+}
+// a run of small levels in one workgroup: the barrier (with its workgroup-scope fence) makes each level's
+// stores visible to the next level's loads
+__global__ __launch_bounds__(kIluRunThreads) void k_ilu_run(IluRows A, const int32_t* __restrict__ rows,
+                                                            const int32_t* __restrict__ lptr, int l0, int l1,
+                                                            bool backward, double* z) {
+  for (int l = l0; l < l1; ++l) {
+    const int a = lptr[l], b = lptr[l + 1];
+    for (int r = a + (int)threadIdx.x; r < b; r += blockDim.x) ilu_row(A, rows[r], backward, z);
+    __threadfence_block();
+    __syncthreads();
+  }
+}
+
 __global__ void k_morton_keys(int n, const int32_t* __restrict__ point, const double* __restrict__ X, double x0, double x1,
                              double y0, double y1, double z0, double z1, uint64_t* __restrict__ keys,
                              int32_t* __restrict__ vals) {
@@ -1598,6 +1629,14 @@ void schwarz_apply(hipStream_t s, const Schwarz& b, const double* inv, const dou
   if (b.nblk == 0) return;
   hipLaunchKernelGGL(k_schwarz_apply, dim3(b.nblk), dim3(256), 0, s, b.bptr, b.bidx, b.bncore, b.inv_ptr, inv, r, z);
 }
+void ilu_level(hipStream_t s, const IluRows& A, const int32_t* level_rows, int64_t nrows, bool backward, double* z) {
+  if (nrows > 0) k_ilu_level<<<(unsigned)((nrows + 255) / 256), 256, 0, s>>>(A, level_rows, nrows, backward, z);
+}
+void ilu_run(hipStream_t s, const IluRows& A, const int32_t* rows, const int32_t* lptr, int l0, int l1, bool backward,
+             double* z) {
+  if (l1 > l0) k_ilu_run<<<1, kIluRunThreads, 0, s>>>(A, rows, lptr, l0, l1, backward, z);
+}
+
 void perm_gather(hipStream_t s, int n, const int32_t* perm, const double* in, double* out) {
   hipLaunchKernelGGL(k_perm_gather, dim3((n + 255) / 256), dim3(256), 0, s, n, perm, in, out);
 }

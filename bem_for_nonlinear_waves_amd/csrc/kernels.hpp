@@ -218,6 +218,25 @@ void mul(hipStream_t s, int64_t n, const double* d, double* y);
 void mul_to(hipStream_t s, int64_t n, const double* d, const double* x, double* y);
 void scale_dup(hipStream_t s, int64_t n, double a, double* w, double* dup);
 void schwarz_apply(hipStream_t s, const Schwarz& b, const double* inv, const double* r, double* z);
+
+// ILU(0) triangular sweeps (ILU0_CRS::solve, basic_linear_systems.hpp:2562-2603) on the factored rows, level by
+// level. One sweep segment is either one level with many rows (one thread per row over the grid) or a run of
+// consecutive small levels done by one workgroup with a barrier between levels. z holds the right-hand side on
+// entry to the forward sweep and the solution after the backward sweep.
+struct IluRows {
+  const int64_t* ptr = nullptr;   // n + 1
+  const int32_t* col = nullptr;
+  const double* val = nullptr;
+  const int64_t* diag = nullptr;  // position of each row's diagonal
+};
+constexpr int kIluBigLevel = 4096;  // levels with at least this many rows get a launch of their own
+constexpr int kIluRunThreads = 1024;
+// backward: upper part and the division by the diagonal, else the strictly lower part.
+// One level: its nrows rows at `level_rows`, one thread each.
+void ilu_level(hipStream_t s, const IluRows& A, const int32_t* level_rows, int64_t nrows, bool backward, double* z);
+// Levels [l0, l1) in one workgroup (rows[lptr[l] .. lptr[l+1]) per level).
+void ilu_run(hipStream_t s, const IluRows& A, const int32_t* rows, const int32_t* lptr, int l0, int l1, bool backward,
+             double* z);
 void perm_gather(hipStream_t s, int n, const int32_t* perm, const double* in, double* out);
 
 // ---- allocation-free work memory ----

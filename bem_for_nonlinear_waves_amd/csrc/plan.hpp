@@ -157,3 +157,32 @@ void build_schwarz_blocks(int n_dof, int n_faces, const int32_t* face_dof, const
                           SchwarzBlocks& out);
 
 }  // namespace bemfmm
+
+namespace bemfmm {
+
+// Sparse rows of the reference's preconditioner matrix (buildSparseMatrixForILU,
+// BEM_solveBVP_GMRES_FMM.hpp:215-662): row d over columns col[ptr[d] .. ptr[d+1]) (ascending), diag[d] = position
+// of the diagonal entry. After ilu0_factor the strictly lower part holds L (unit diagonal implied) and the
+// diagonal and upper part hold U (ILU0_CRS, basic_linear_systems.hpp:2337-2560).
+struct SparseRows {
+  int n = 0;
+  std::vector<int64_t> ptr{0};
+  std::vector<int32_t> col;
+  std::vector<double> val;
+  std::vector<int64_t> diag;
+};
+// rows of a sweep's dependency levels: rows[ptr[l] .. ptr[l+1]) can be solved once levels < l are done
+struct SweepLevels {
+  std::vector<int32_t> rows;
+  std::vector<int32_t> ptr{0};
+  int n_levels() const { return (int)ptr.size() - 1; }
+};
+// neighborhood 0 = BUCKETS (the reference default), 1 = K-RING with `kring` rings; X are the scaled node
+// coordinates, alpha the rigid-mode diagonal per DOF, scale the getScale() of the scaled geometry
+void build_ilu_rows(int n_nodes, const double* X, int n_faces, const int32_t* fv, const int32_t* face_dof, int n_dof,
+                    const int32_t* dof_kind, const int32_t* dof_point, const double* alpha, double scale,
+                    int neighborhood, int kring, SparseRows& A);
+void ilu0_factor(SparseRows& A, double pivot_min = 1e-20);  // throws std::runtime_error on a bad pivot
+void sweep_levels(const SparseRows& A, SweepLevels& forward, SweepLevels& backward);
+
+}  // namespace bemfmm

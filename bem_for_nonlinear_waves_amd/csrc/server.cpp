@@ -70,7 +70,8 @@ bool same_opts(const bemfmm_opts& a, const bemfmm_opts& b) {
          a.schwarz_max_core == b.schwarz_max_core && a.schwarz_max_block == b.schwarz_max_block &&
          a.schwarz_overlap == b.schwarz_overlap && a.m2l_tile_pairs == b.m2l_tile_pairs &&
          a.dof_reorder == b.dof_reorder && a.use_graphs == b.use_graphs && a.m2l_kernel == b.m2l_kernel &&
-         a.overlap_near == b.overlap_near && a.refine_recycle == b.refine_recycle;
+         a.overlap_near == b.overlap_near && a.refine_recycle == b.refine_recycle &&
+         a.ilu_neighborhood == b.ilu_neighborhood && a.ilu_kring == b.ilu_kring;
 }
 
 }  // namespace

@@ -51,6 +51,9 @@ extern "C" {
 #define BEMFMM_PRECOND_JACOBI 1    /* diagonal scaling (BEM_solveBVP_GMRES_FMM.hpp:2095-2104) */
 #define BEMFMM_PRECOND_SCHWARZ 2   /* overlapping-block Schwarz on the near field (SCHWARZ, GMRES_FMM.hpp:899-1013;
                                       SchwarzAdditive_CRS, basic_linear_systems.hpp:2870-3003), restricted combination */
+#define BEMFMM_PRECOND_ILU0 3      /* ILU(0) (ILU, ILU0_CRS, basic_linear_systems.hpp:2337-2606) of the reference's sparse
+                                      matrix (buildSparseMatrixForILU, GMRES_FMM.hpp:215-662; neighbourhood from
+                                      ilu_neighborhood / ilu_kring); factorised on the host, sweeps on the GPU by level */
 
 typedef struct bemfmm_ctx bemfmm_ctx;
 
@@ -85,6 +88,11 @@ typedef struct bemfmm_opts {
   int refine_recycle;    /* FP32_REFINE: 1 (default): each inner FP32 solve after the first minimises over the
                             earlier steps' Krylov directions too (they share the operator M^-1 A_32) and extends
                             that basis, up to `restart` + 1 vectors; 0: every step restarts GMRES from its residual */
+  int ilu_neighborhood;  /* BEMFMM_PRECOND_ILU0 rows: 0 (default) = BUCKETS (faces sharing the point's cell of a
+                            uniform grid of clamp(cbrt(faces), 8, 64) cells per axis, plus the point's faces),
+                            1 = K-RING (the point's faces and ilu_kring rings of edge neighbours);
+                            ilu_neighborhood_type, main_time_domain.cpp:39 */
+  int ilu_kring;         /* rings for K-RING, default 1 (ilu_kring_num, main_time_domain.cpp:40) */
 } bemfmm_opts;
 
 typedef struct bemfmm_result {
@@ -112,6 +120,8 @@ typedef struct bemfmm_stats {
   int n_ops_uploaded;           /* translation operators uploaded by the last prepare (the rest were resident) */
   int m2l_tile_pairs;           /* M2L pairs per tile in use (autotuned unless set in the options) */
   int m2l_rotation;             /* 1 if the FP32 operator's M2L runs by rotation */
+  int64_t precond_nnz;          /* ILU(0): entries of the sparse matrix */
+  int precond_levels;           /* ILU(0): dependency levels of the forward plus the backward sweep */
 } bemfmm_stats;
 
 /* Availability (metal_m2l.h: metal_m2l_is_available / metal_m2l_device_name). */

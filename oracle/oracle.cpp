@@ -15,6 +15,7 @@
 //   matvec                    bem/BEM_solveBVP_GMRES_FMM.hpp:34-158
 //   rigid-mode alpha          bem/BEM_solveBVP_GMRES_FMM.hpp:2674-2716
 //   GMRES                     bem/BEM_solveBVP_GMRES_FMM.hpp:2007-2355, lib/include/basic_linear_systems.hpp:3137-3451
+//   ILU(0) preconditioner     bem/BEM_solveBVP_GMRES_FMM.hpp:215-662, lib/include/basic_linear_systems.hpp:2337-2603
 //
 // The reference's translation tables AAA_{M2M,M2L,L2L}_FMM.hpp are absent, so
 // the three operators are evaluated from the Greengard & Rokhlin (1997)
@@ -31,6 +32,7 @@
 #include <cstdint>
 #include <cstring>
 #include <map>
+#include <set>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -1111,7 +1113,163 @@ struct Oracle {
     }
   }
 
-  int precond = 0;  // 0 none, 1 Jacobi, 2 restricted additive Schwarz (blocks set beforehand)
+  // ---- the reference's sparse preconditioner matrix and its ILU(0) --------------------------------------
+  // buildSparseMatrixForILU (bem/BEM_solveBVP_GMRES_FMM.hpp:215-662, linear elements): per boundary point, the
+  // faces of its neighbourhood, each integrated with networkFace::setIntegrationInfo's rules
+  // (lib/src/networkFace.cpp:28-85) seen from the face's points in getPoints(origin) order
+  // (lib/include/Network.hpp:2062-2069); ILU0_CRS (lib/include/basic_linear_systems.hpp:2337-2603).
+  std::vector<std::map<int, double>> ilu_S;                 // rows of the sparse matrix
+  std::vector<std::vector<std::pair<int, double>>> ilu_LU;  // factored rows (L strictly below, U from the diagonal)
+  // weights of one face: rule collapsed at P[0] (GW5x5 if singular or near, else GW1x1); singular (the
+  // point is P[0]): single layer only. G_k = sum J w N_k / r, H_k = -sum (R . cr) w N_k / r^3
+  static void ilu_face_rule(const V3 P[3], const V3& x, bool singular, bool near, double G[3], double H[3]) {
+    static const double gw1[1][3] = {{0.5, 0.5, 1.0}};
+    const V3 cr = cross(P[1] - P[0], P[2] - P[0]);
+    const double J = norm(cr);
+    for (int k = 0; k < 3; ++k) G[k] = H[k] = 0.;
+    const bool fine = singular || near;
+    const int nq = fine ? 25 : 1;
+    for (int g = 0; g < nq; ++g) {
+      const double t0 = fine ? kGW5[g][0] : gw1[g][0];
+      const double t1 = fine ? kGW5[g][1] : gw1[g][1];
+      const double ww = fine ? kGW5[g][2] : gw1[g][2];
+      const double N[3] = {t0, t1 * (1. - t0), (1. - t0) * (1. - t1)};
+      const V3 Xq = N[0] * P[0] + N[1] * P[1] + N[2] * P[2];
+      const V3 R = Xq - x;
+      const double r = norm(R), w = ww * (1. - t0);
+      for (int k = 0; k < 3; ++k) {
+        G[k] += J * w / r * N[k];
+        if (!singular) H[k] -= dot(R, cr) * w / (r * r * r) * N[k];
+      }
+    }
+  }
+  void build_ilu(int neighborhood, int kring) {
+    // faces per point
+    std::vector<std::vector<int>> pfaces(n_nodes);
+    for (int f = 0; f < n_faces; ++f)
+      for (int a = 0; a < 3; ++a) pfaces[F[f].v[a]].push_back(f);
+    // BUCKETS grid: cube of half-side 1.05 x the largest extent around the bounding-box centre,
+    // clamp(round(cbrt(faces)), 8, 64) cells across; each face at its centroid, vertices, edge midpoints
+    Bucket grid;
+    std::vector<std::vector<int>> cell_faces;
+    if (neighborhood == 0) {
+      double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+      for (const V3& x : Xn)
+        for (int a = 0; a < 3; ++a) {
+          lo[a] = std::min(lo[a], x[a]);
+          hi[a] = std::max(hi[a], x[a]);
+        }
+      const double ext = std::max({hi[0] - lo[0], hi[1] - lo[1], hi[2] - lo[2]});
+      double ulo[3], uhi[3];
+      for (int a = 0; a < 3; ++a) {
+        ulo[a] = (lo[a] + hi[a]) / 2. - 1.05 * ext;
+        uhi[a] = (lo[a] + hi[a]) / 2. + 1.05 * ext;
+      }
+      const int ncell = std::clamp((int)std::llround(std::cbrt((double)std::max(n_faces, 1))), 8, 64);
+      grid.init(ulo, uhi, (2.1 * ext) / ncell);
+      cell_faces.assign(grid.size[0] * grid.size[1] * grid.size[2], {});
+      for (int f = 0; f < n_faces; ++f) {
+        const V3* X = F[f].X;
+        const V3 pts[7] = {F[f].centroid, X[0], X[1], X[2], 0.5 * (X[0] + X[1]), 0.5 * (X[1] + X[2]),
+                           0.5 * (X[2] + X[0])};
+        for (const V3& q : pts) {
+          auto& cf = cell_faces[grid.cell_of(q)];
+          if (std::find(cf.begin(), cf.end(), f) == cf.end()) cf.push_back(f);
+        }
+      }
+    }
+    // K-RING: faces across each edge
+    std::map<std::pair<int, int>, std::vector<int>> edge_faces;
+    if (neighborhood == 1)
+      for (int f = 0; f < n_faces; ++f)
+        for (int a = 0; a < 3; ++a) {
+          int u = F[f].v[a], v = F[f].v[(a + 1) % 3];
+          edge_faces[{std::min(u, v), std::max(u, v)}].push_back(f);
+        }
+    const double near_switch = scale / 10.;
+    ilu_S.assign(n_dof, {});
+    for (int k = 0; k < n_nodes; ++k) {
+      if (point_dofs[k].empty()) continue;
+      std::set<int> faces(pfaces[k].begin(), pfaces[k].end());
+      if (neighborhood == 0) {
+        for (int f : cell_faces[grid.cell_of(Xn[k])]) faces.insert(f);
+      } else {
+        std::set<int> front = faces;
+        for (int ring = 0; ring < kring; ++ring) {
+          std::set<int> next;
+          for (int f : front)
+            for (int a = 0; a < 3; ++a) {
+              int u = F[f].v[a], v = F[f].v[(a + 1) % 3];
+              for (int g : edge_faces[{std::min(u, v), std::max(u, v)}])
+                if (!faces.count(g)) next.insert(g);
+            }
+          faces.insert(next.begin(), next.end());
+          front = next;
+        }
+      }
+      std::map<int, double> coeffs;
+      for (int f : faces) {
+        const FaceRec& fr = F[f];
+        const int s = fr.v[0] == k ? 0 : fr.v[1] == k ? 1 : 2;  // getPoints(origin)
+        const int ord[3] = {s, (s + 1) % 3, (s + 2) % 3};
+        const V3 P[3] = {fr.X[ord[0]], fr.X[ord[1]], fr.X[ord[2]]};
+        const bool singular = fr.v[ord[0]] == k;
+        const bool near = norm(fr.centroid - Xn[k]) < near_switch;
+        double G[3], H[3];
+        ilu_face_rule(P, Xn[k], singular, near, G, H);
+        for (int a = 0; a < 3; ++a) {
+          const int col = fr.dof[ord[a]];
+          const double val = dof_kind[col] == 0 ? G[a] : -H[a];
+          if (val != 0.) coeffs[col] += val;
+        }
+      }
+      for (int d : point_dofs[k]) {
+        if (dof_kind[d] == 2) {
+          ilu_S[d] = {{d, 1.0}};
+          continue;
+        }
+        ilu_S[d] = coeffs;
+        if (dof_kind[d] != 0) ilu_S[d][d] -= alpha[d];
+        else if (!ilu_S[d].count(d)) ilu_S[d][d] = 1.0;
+      }
+    }
+    // ILU(0): row i in turn, each of its entries k < i (ascending) divided by U_kk and its multiple of row k's
+    // upper part subtracted where row i has an entry
+    ilu_LU.assign(n_dof, {});
+    std::vector<std::map<int, double>> W = ilu_S;
+    for (int i = 0; i < n_dof; ++i) {
+      auto& wi = W[i];
+      for (auto it = wi.begin(); it != wi.end() && it->first < i; ++it) {
+        const int k = it->first;
+        const double piv = W[k].at(k);
+        if (!(std::abs(piv) >= 1e-20)) throw std::runtime_error("oracle ILU(0): bad pivot");
+        it->second /= piv;
+        for (auto jt = W[k].upper_bound(k); jt != W[k].end(); ++jt) {
+          auto e = wi.find(jt->first);
+          if (e != wi.end()) e->second -= it->second * jt->second;
+        }
+      }
+      ilu_LU[i].assign(wi.begin(), wi.end());
+    }
+  }
+  void ilu_apply(const double* r, double* z) const {
+    for (int i = 0; i < n_dof; ++i) {
+      double s = r[i];
+      for (const auto& [j, v] : ilu_LU[i])
+        if (j < i) s -= v * z[j];
+      z[i] = s;
+    }
+    for (int i = n_dof - 1; i >= 0; --i) {
+      double s = z[i], d = 0.;
+      for (const auto& [j, v] : ilu_LU[i]) {
+        if (j > i) s -= v * z[j];
+        if (j == i) d = v;
+      }
+      z[i] = s / d;
+    }
+  }
+
+  int precond = 0;  // 0 none, 1 Jacobi, 2 restricted additive Schwarz (blocks set beforehand), 3 ILU(0) (built)
   int solve(const double* x0, double* xout, double tol, int max_iter, int restart, int* iters_out, double* res_out) {
     std::vector<double> b(n_dof);
     rhs(b.data());
@@ -1120,6 +1278,9 @@ struct Oracle {
     auto apply_m = [&](std::vector<double>& v) {
       if (precond == 2) {
         schwarz_apply(v.data(), tmp.data());
+        v.swap(tmp);
+      } else if (precond == 3) {
+        ilu_apply(v.data(), tmp.data());
         v.swap(tmp);
       } else {
         for (int i = 0; i < n_dof; ++i) v[i] *= dinv[i];
@@ -1319,6 +1480,51 @@ void oracle_set_schwarz(void* h, int n_blocks, const int64_t* ptr, const int32_t
   o->sw_ncore.assign(ncore, ncore + n_blocks);
 }
 void oracle_schwarz_apply(void* h, const double* r, double* z) { static_cast<Oracle*>(h)->schwarz_apply(r, z); }
+// the reference's sparse preconditioner matrix and its ILU(0); returns its entry count (-1 on failure)
+long long oracle_ilu_build(void* h, int neighborhood, int kring) {
+  auto* o = static_cast<Oracle*>(h);
+  try {
+    o->build_ilu(neighborhood, kring);
+  } catch (...) {
+    return -1;
+  }
+  long long nnz = 0;
+  for (const auto& r : o->ilu_S) nnz += (long long)r.size();
+  return nnz;
+}
+// unfactored rows: ptr (n + 1), col, val
+void oracle_ilu_rows(void* h, long long* ptr, int* col, double* val) {
+  auto* o = static_cast<Oracle*>(h);
+  long long q = 0;
+  ptr[0] = 0;
+  for (int i = 0; i < o->n_dof; ++i) {
+    for (const auto& [j, v] : o->ilu_S[i]) {
+      col[q] = j;
+      val[q] = v;
+      ++q;
+    }
+    ptr[i + 1] = q;
+  }
+}
+void oracle_ilu_apply(void* h, const double* r, double* z) { static_cast<Oracle*>(h)->ilu_apply(r, z); }
+// factored rows in the same layout (the pattern of the unfactored rows)
+void oracle_ilu_factored(void* h, long long* ptr, int* col, double* val) {
+  auto* o = static_cast<Oracle*>(h);
+  long long q = 0;
+  ptr[0] = 0;
+  for (int i = 0; i < o->n_dof; ++i) {
+    for (const auto& [j, v] : o->ilu_LU[i]) {
+      col[q] = j;
+      val[q] = v;
+      ++q;
+    }
+    ptr[i + 1] = q;
+  }
+}
+void oracle_ilu_face_rule(const double* P9, const double* x, int singular, int near, double* G, double* H) {
+  const V3 P[3] = {{P9[0], P9[1], P9[2]}, {P9[3], P9[4], P9[5]}, {P9[6], P9[7], P9[8]}};
+  Oracle::ilu_face_rule(P, {x[0], x[1], x[2]}, singular != 0, near != 0, G, H);
+}
 void oracle_diag_inv(void* h, double* d) {
   const auto v = static_cast<Oracle*>(h)->diag_inv();
   std::copy(v.begin(), v.end(), d);

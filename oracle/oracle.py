@@ -78,6 +78,12 @@ def lib():
         L.oracle_set_schwarz.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_int64), _ip, _ip]
         L.oracle_schwarz_apply.argtypes = [ctypes.c_void_p, _dp, _dp]
         L.oracle_diag_inv.argtypes = [ctypes.c_void_p, _dp]
+        L.oracle_ilu_build.restype = ctypes.c_longlong
+        L.oracle_ilu_build.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
+        L.oracle_ilu_rows.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_longlong), _ip, _dp]
+        L.oracle_ilu_factored.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_longlong), _ip, _dp]
+        L.oracle_ilu_apply.argtypes = [ctypes.c_void_p, _dp, _dp]
+        L.oracle_ilu_face_rule.argtypes = [_dp, _dp, ctypes.c_int, ctypes.c_int, _dp, _dp]
         L.oracle_sph_harmonics.argtypes = [_dp, ctypes.c_int, _dp, _dp]
         L.oracle_dR.argtypes = [_dp, _dp, ctypes.c_int, _dp, _dp]
         L.oracle_translate_chain.argtypes = [ctypes.c_int, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, ctypes.c_int, _dp]
@@ -179,12 +185,51 @@ class Oracle:
         lib().oracle_diag_inv(self.h, _p(d, _dp))
         return d
 
+    def ilu_build(self, neighborhood=0, kring=1):
+        """The reference's sparse preconditioner matrix (BUCKETS = 0 / K-RING = 1) and its ILU(0); returns nnz."""
+        nnz = lib().oracle_ilu_build(self.h, neighborhood, kring)
+        if nnz < 0:
+            raise RuntimeError("oracle ILU(0) failed")
+        self._ilu_nnz = int(nnz)
+        return self._ilu_nnz
+
+    def _ilu_get(self, fn):
+        ptr = np.empty(self.n_dof + 1, np.int64)
+        col = np.empty(self._ilu_nnz, np.int32)
+        val = np.empty(self._ilu_nnz)
+        fn(self.h, ptr.ctypes.data_as(ctypes.POINTER(ctypes.c_longlong)), _p(col, _ip), _p(val, _dp))
+        return ptr, col, val
+
+    def ilu_rows(self):
+        """(ptr, col, val) of the unfactored sparse matrix (columns ascending per row)."""
+        return self._ilu_get(lib().oracle_ilu_rows)
+
+    def ilu_factored(self):
+        """(ptr, col, val) of the ILU(0) factors on the same pattern: L strictly below the diagonal, U from it."""
+        return self._ilu_get(lib().oracle_ilu_factored)
+
+    def ilu_apply(self, r):
+        r = _d(r)
+        z = np.empty(self.n_dof)
+        lib().oracle_ilu_apply(self.h, _p(r, _dp), _p(z, _dp))
+        return z
+
     def near_row(self, t, cap=100000):
         idx = np.empty(cap, np.int32)
         wg = np.empty(cap)
         wgn = np.empty(cap)
         n = lib().oracle_near_row(self.h, t, _p(idx, _ip), _p(wg, _dp), _p(wgn, _dp), cap)
         return idx[:n].copy(), wg[:n].copy(), wgn[:n].copy()
+
+
+def ilu_face_rule(P, x, singular, near):
+    """Single- and double-layer weights (G[3], H[3]) of one face (corners P, 3x3, collapsed at P[0]) seen from x,
+    as the reference's sparse preconditioner matrix integrates it."""
+    P = _d(np.asarray(P).reshape(9))
+    x = _d(x)
+    G, H = np.empty(3), np.empty(3)
+    lib().oracle_ilu_face_rule(_p(P, _dp), _p(x, _dp), int(singular), int(near), _p(G, _dp), _p(H, _dp))
+    return G, H
 
 
 def sph_harmonics(x, P):

@@ -379,3 +379,49 @@ void emu_schwarz_apply(void* he, void* hs, const double* r, double* z) {
   }
 }
 }
+
+// ---- the reference's sparse preconditioner matrix and ILU(0) (product host code, precond.cpp) ----
+extern "C" {
+// rows in the caller's numbering (the plan keeps it); alpha per DOF; out: nnz, then ilu_get fills the arrays
+void* emu_ilu_create(int n_nodes, const double* X, int n_faces, const int32_t* tri, const int32_t* face_dof, int n_dof,
+                     const int32_t* kind, const int32_t* point, const double* alpha, int neighborhood, int kring,
+                     int factor, int64_t* nnz, int* levels) {
+  try {
+    bemfmm::Plan pl;
+    bemfmm::Options o;
+    o.p = 4;
+    o.materialize_ops = false;
+    o.fill_p2m = false;
+    bemfmm::build_plan(pl, o, n_nodes, X, n_faces, tri, face_dof, n_dof, kind, point);
+    auto* A = new bemfmm::SparseRows();
+    bemfmm::build_ilu_rows(n_nodes, pl.Xs.data(), n_faces, pl.fv.data(), face_dof, n_dof, kind, point, alpha, pl.scale,
+                           neighborhood, kring, *A);
+    if (factor) bemfmm::ilu0_factor(*A);
+    bemfmm::SweepLevels Lv, Uv;
+    bemfmm::sweep_levels(*A, Lv, Uv);
+    *nnz = (int64_t)A->col.size();
+    levels[0] = Lv.n_levels();
+    levels[1] = Uv.n_levels();
+    return A;
+  } catch (...) {
+    return nullptr;
+  }
+}
+void emu_ilu_get(void* h, int64_t* ptr, int32_t* col, double* val) {
+  auto* A = static_cast<bemfmm::SparseRows*>(h);
+  std::copy(A->ptr.begin(), A->ptr.end(), ptr);
+  std::copy(A->col.begin(), A->col.end(), col);
+  std::copy(A->val.begin(), A->val.end(), val);
+}
+// sweep levels: level of each row in the forward and backward sweeps
+void emu_ilu_levels(void* h, int32_t* lev_fwd, int32_t* lev_bwd) {
+  auto* A = static_cast<bemfmm::SparseRows*>(h);
+  bemfmm::SweepLevels Lv, Uv;
+  bemfmm::sweep_levels(*A, Lv, Uv);
+  for (int l = 0; l < Lv.n_levels(); ++l)
+    for (int r = Lv.ptr[l]; r < Lv.ptr[l + 1]; ++r) lev_fwd[Lv.rows[r]] = l;
+  for (int l = 0; l < Uv.n_levels(); ++l)
+    for (int r = Uv.ptr[l]; r < Uv.ptr[l + 1]; ++r) lev_bwd[Uv.rows[r]] = l;
+}
+void emu_ilu_destroy(void* h) { delete static_cast<bemfmm::SparseRows*>(h); }
+}

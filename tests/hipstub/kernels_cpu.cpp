@@ -659,6 +659,44 @@ void schwarz_apply(hipStream_t s, const Schwarz& b, const double* inv, const dou
     }
   });
 }
+void ilu_level(hipStream_t s, const IluRows& A, const int32_t* level_rows, int64_t nrows, bool backward, double* z) {
+  const IluRows R = A;
+  hipstub::enqueue(s, [=]() {
+    for (int64_t t = 0; t < nrows; ++t) {
+      const int32_t i = level_rows[t];
+      const int64_t d = R.diag[i];
+      double acc = z[i];
+      if (!backward) {
+        for (int64_t q = R.ptr[i]; q < d; ++q) acc -= R.val[q] * z[R.col[q]];
+        z[i] = acc;
+      } else {
+        for (int64_t q = d + 1; q < R.ptr[i + 1]; ++q) acc -= R.val[q] * z[R.col[q]];
+        z[i] = acc / R.val[d];
+      }
+    }
+  });
+}
+void ilu_run(hipStream_t s, const IluRows& A, const int32_t* rows, const int32_t* lptr, int l0, int l1, bool backward,
+             double* z) {
+  for (int l = l0; l < l1; ++l) {
+    // lptr is device memory: read it inside the enqueued work, level by level
+    const IluRows R = A;
+    hipstub::enqueue(s, [=]() {
+      for (int r = lptr[l]; r < lptr[l + 1]; ++r) {
+        const int32_t i = rows[r];
+        const int64_t d = R.diag[i];
+        double acc = z[i];
+        if (!backward) {
+          for (int64_t q = R.ptr[i]; q < d; ++q) acc -= R.val[q] * z[R.col[q]];
+          z[i] = acc;
+        } else {
+          for (int64_t q = d + 1; q < R.ptr[i + 1]; ++q) acc -= R.val[q] * z[R.col[q]];
+          z[i] = acc / R.val[d];
+        }
+      }
+    });
+  }
+}
 void perm_gather(hipStream_t s, int n, const int32_t* perm, const double* in, double* out) {
   hipstub::enqueue(s, [=]() {
     for (int i = 0; i < n; ++i) out[i] = in[perm[i]];
