@@ -27,7 +27,8 @@ class Opts(ctypes.Structure):
                 ("schwarz_max_block", ctypes.c_int), ("schwarz_overlap", ctypes.c_int), ("m2l_tile_pairs", ctypes.c_int),
                 ("dof_reorder", ctypes.c_int), ("use_graphs", ctypes.c_int), ("m2l_kernel", ctypes.c_int),
                 ("overlap_near", ctypes.c_int), ("refine_recycle", ctypes.c_int),
-                ("ilu_neighborhood", ctypes.c_int), ("ilu_kring", ctypes.c_int)]
+                ("ilu_neighborhood", ctypes.c_int), ("ilu_kring", ctypes.c_int), ("schwarz_variant", ctypes.c_int),
+                ("schwarz_core_k", ctypes.c_int)]
 
 
 class Result(ctypes.Structure):

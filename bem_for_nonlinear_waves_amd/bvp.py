@@ -52,7 +52,8 @@ class BEM_BVP:
                  preconditioner: int = _lib.PRECOND_NONE, refine_eta: float = 3e-7, schwarz_max_core: int = 64,
                  schwarz_max_block: int = 128, schwarz_overlap: int = 1, m2l_tile_pairs: int = 0, dof_reorder: int = 1,
                  use_graphs: int = 0, m2l_kernel: int = 0, overlap_near: int = 1, refine_recycle: int = 1,
-                 ilu_neighborhood: int = 0, ilu_kring: int = 1, device: int = 0):
+                 ilu_neighborhood: int = 0, ilu_kring: int = 1, schwarz_variant: int = 0, schwarz_core_k: int = 1,
+                 device: int = 0):
         self.lib = _lib.load()
         o = _lib.Opts()
         self.lib.bemfmm_default_opts(ctypes.byref(o))
@@ -62,6 +63,7 @@ class BEM_BVP:
         o.m2l_tile_pairs, o.dof_reorder, o.use_graphs = m2l_tile_pairs, dof_reorder, use_graphs
         o.m2l_kernel, o.overlap_near, o.refine_recycle = m2l_kernel, overlap_near, refine_recycle
         o.ilu_neighborhood, o.ilu_kring = ilu_neighborhood, ilu_kring
+        o.schwarz_variant, o.schwarz_core_k = schwarz_variant, schwarz_core_k
         self.ctx = self.lib.bemfmm_ctx_create(device, ctypes.byref(o))
         if not self.ctx:
             raise BEMError(_lib.E_DEVICE, "bemfmm_ctx_create failed (no usable GPU?)")

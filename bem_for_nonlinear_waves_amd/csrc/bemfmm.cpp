@@ -288,6 +288,10 @@ struct bemfmm_ctx {
   DBuf<int64_t> sw_ptr, sw_invptr;
   DBuf<int32_t> sw_idx, sw_ncore;
   DBuf<double> sw_inv;
+  // the reference's SCHWARZ (schwarz_variant 1): every block's full inverse, the DOFs' occurrences in the blocks
+  DBuf<int64_t> sw_occ_ptr, sw_occ_pos;
+  DBuf<int32_t> sw_rows_out;
+  DBuf<double> sw_y;
   // ILU(0) of the reference's sparse preconditioner matrix: factored rows and the sweeps' level schedules
   struct IluSegment {
     int l0, l1;    // levels [l0, l1) of the sweep
@@ -855,6 +859,11 @@ void do_prepare(bemfmm_ctx* c) {
       (c->opts.schwarz_max_core < 1 || c->opts.schwarz_max_block < c->opts.schwarz_max_core ||
        c->opts.schwarz_max_block > kSchwarzMaxBlock || c->opts.schwarz_overlap < 0 || c->opts.schwarz_overlap > 8))
     throw std::runtime_error("bemfmm_prepare: Schwarz block sizes need 1 <= max_core <= max_block <= 128, 0 <= overlap <= 8");
+  if (c->opts.preconditioner == BEMFMM_PRECOND_SCHWARZ &&
+      (c->opts.schwarz_variant < 0 || c->opts.schwarz_variant > 1 || c->opts.schwarz_core_k < 0 ||
+       c->opts.schwarz_core_k > 8 || c->opts.ilu_neighborhood < 0 || c->opts.ilu_neighborhood > 1 ||
+       c->opts.ilu_kring < 0 || c->opts.ilu_kring > 20))
+    throw std::runtime_error("bemfmm_prepare: schwarz_variant must be 0 or 1, 0 <= schwarz_core_k <= 8");
   const double t0 = now();
   bemfmm::Options o;
   o.p = c->opts.p;
@@ -1065,7 +1074,64 @@ void do_prepare(bemfmm_ctx* c) {
   const int pc = c->opts.preconditioner;
   c->sw_host = bemfmm::SchwarzBlocks{};
   for (auto* b : {&c->sw_inv}) b->release();
-  if (pc == BEMFMM_PRECOND_JACOBI || pc == BEMFMM_PRECOND_SCHWARZ) {
+  // the reference's sparse matrix (buildSparseMatrixForILU) for ILU(0) and the reference's SCHWARZ: built on the
+  // host from the scaled geometry and alpha
+  auto sparse_rows = [&](bemfmm::SparseRows& A) {
+    std::vector<double> ah(c->n_dof);
+    HIPCHECK(hipMemcpyAsync(ah.data(), c->alpha.p, c->n_dof * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPCHECK(hipStreamSynchronize(s));
+    bemfmm::build_ilu_rows(c->n_nodes, pl.Xs.data(), c->n_faces, pl.fv.data(), c->face_dof_i.data(), c->n_dof,
+                           c->kind_i.data(), c->point_i.data(), ah.data(), pl.scale, c->opts.ilu_neighborhood,
+                           c->opts.ilu_kring, A);
+  };
+  if (pc == BEMFMM_PRECOND_SCHWARZ && c->opts.schwarz_variant == 1) {
+    // SchwarzAdditive_CRS on the reference's blocks (GMRES_FMM.hpp:901-1013): blocks on the matrix graph, every
+    // block's full inverse on the GPU (Gauss-Jordan in LDS), averaged combination
+    bemfmm::SparseRows A;
+    sparse_rows(A);
+    bemfmm::build_schwarz_blocks_rows(A, c->opts.schwarz_core_k, c->opts.schwarz_overlap, c->opts.schwarz_max_core,
+                                      c->opts.schwarz_max_block, c->sw_host);
+    const bemfmm::SchwarzBlocks& B = c->sw_host;
+    const int nb = B.n_blocks();
+    if (B.max_block > kSchwarzMaxBlock) throw std::runtime_error("schwarz: block larger than the kernel's limit");
+    std::vector<int64_t> invptr(nb + 1, 0);
+    std::vector<int32_t> rows_out(nb);
+    for (int b = 0; b < nb; ++b) {
+      const int64_t m = B.ptr[b + 1] - B.ptr[b];
+      rows_out[b] = (int32_t)m;
+      invptr[b + 1] = invptr[b] + m * m;
+    }
+    std::vector<int64_t> occ_ptr(c->n_dof + 1, 0), occ_pos(B.idx.size());
+    for (int32_t g : B.idx) occ_ptr[g + 1]++;
+    for (int g = 0; g < c->n_dof; ++g) occ_ptr[g + 1] += occ_ptr[g];
+    {
+      std::vector<int64_t> fillp(occ_ptr.begin(), occ_ptr.end() - 1);
+      for (size_t q = 0; q < B.idx.size(); ++q) occ_pos[fillp[B.idx[q]]++] = (int64_t)q;
+    }
+    c->ilu_ptr.upload(A.ptr, s);
+    c->ilu_col.upload(A.col, s);
+    c->ilu_val.upload(A.val, s);
+    c->ilu_nnz = (int64_t)A.col.size();
+    c->sw_ptr.upload(B.ptr, s);
+    c->sw_idx.upload(B.idx, s);
+    c->sw_ncore.upload(B.ncore, s);
+    c->sw_invptr.upload(invptr, s);
+    c->sw_rows_out.upload(rows_out, s);
+    c->sw_occ_ptr.upload(occ_ptr, s);
+    c->sw_occ_pos.upload(occ_pos, s);
+    c->sw_y.alloc(B.idx.size());
+    c->sw_inv.alloc((size_t)invptr[nb]);
+    auto& failed = c->sb.failed;
+    failed.alloc(nb);
+    dev::schwarz_build_rows(s, schwarz_view(c), B.max_block, dev::IluRows{c->ilu_ptr.p, c->ilu_col.p, c->ilu_val.p, nullptr},
+                            c->sw_rows_out.p, c->sw_inv.p, failed.p);
+    HIPCHECK(hipGetLastError());
+    std::vector<int32_t> fh(nb);
+    HIPCHECK(hipMemcpyAsync(fh.data(), failed.p, nb * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    HIPCHECK(hipStreamSynchronize(s));
+    c->sw_failed = 0;
+    for (int32_t f : fh) c->sw_failed += f;
+  } else if (pc == BEMFMM_PRECOND_JACOBI || pc == BEMFMM_PRECOND_SCHWARZ) {
     std::vector<int64_t> node_row(c->n_nodes, -1);
     for (size_t r = 0; r < pl.near.row_out.size(); ++r) node_row[pl.near.row_out[r]] = (int64_t)r;
     for (int i = 0; i < c->n_dof; ++i)
@@ -1122,15 +1188,9 @@ void do_prepare(bemfmm_ctx* c) {
       for (int32_t f : fh) c->sw_failed += f;
     }
   } else if (pc == BEMFMM_PRECOND_ILU0) {
-    // the reference's sparse matrix (buildSparseMatrixForILU) on the host from the scaled geometry and alpha,
     // ILU(0) factorised on the host as ILU0_CRS does; the sweeps run on the GPU by dependency level
-    std::vector<double> ah(c->n_dof);
-    HIPCHECK(hipMemcpyAsync(ah.data(), c->alpha.p, c->n_dof * sizeof(double), hipMemcpyDeviceToHost, s));
-    HIPCHECK(hipStreamSynchronize(s));
     bemfmm::SparseRows A;
-    bemfmm::build_ilu_rows(c->n_nodes, pl.Xs.data(), c->n_faces, pl.fv.data(), c->face_dof_i.data(), c->n_dof,
-                           c->kind_i.data(), c->point_i.data(), ah.data(), pl.scale, c->opts.ilu_neighborhood,
-                           c->opts.ilu_kring, A);
+    sparse_rows(A);
     bemfmm::ilu0_factor(A);
     bemfmm::SweepLevels Lv, Uv;
     bemfmm::sweep_levels(A, Lv, Uv);
@@ -1233,7 +1293,8 @@ void do_prepare(bemfmm_ctx* c) {
   st.n_ops_uploaded = c->ops_uploaded;
   st.m2l_tile_pairs = c->m2l_tp;
   st.m2l_rotation = c->m2l_rot ? 1 : 0;
-  st.precond_nnz = pc == BEMFMM_PRECOND_ILU0 ? c->ilu_nnz : 0;
+  st.precond_nnz = (pc == BEMFMM_PRECOND_ILU0 || (pc == BEMFMM_PRECOND_SCHWARZ && c->opts.schwarz_variant == 1))
+                       ? c->ilu_nnz : 0;
   st.precond_levels = pc == BEMFMM_PRECOND_ILU0 ? c->ilu_levels : 0;
   (void)t0;
 }
@@ -1289,7 +1350,11 @@ void precond_apply(bemfmm_ctx* c, const double* in, double* out) {
     break;
   case BEMFMM_PRECOND_SCHWARZ:
     if (in == out) throw std::runtime_error("internal: Schwarz apply needs distinct buffers");
-    dev::schwarz_apply(c->stream, schwarz_view(c), c->sw_inv.p, in, out);
+    if (c->opts.schwarz_variant == 1)
+      dev::schwarz_apply_averaged(c->stream, schwarz_view(c), (int)n, c->sw_occ_ptr.p, c->sw_occ_pos.p, c->sw_inv.p, in,
+                                  c->sw_y.p, out);
+    else
+      dev::schwarz_apply(c->stream, schwarz_view(c), c->sw_inv.p, in, out);
     break;
   case BEMFMM_PRECOND_ILU0: {
     if (in != out) HIPCHECK(hipMemcpyAsync(out, in, n * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
@@ -1871,6 +1936,8 @@ void bemfmm_default_opts(bemfmm_opts* o) {
   o->refine_recycle = 1;
   o->ilu_neighborhood = 0;  // BUCKETS (main_time_domain.cpp:39)
   o->ilu_kring = 1;         // main_time_domain.cpp:40
+  o->schwarz_variant = 0;
+  o->schwarz_core_k = 1;    // main_time_domain.cpp:45
 }
 
 bemfmm_ctx* bemfmm_ctx_create(int device, const bemfmm_opts* opts) {

@@ -1153,25 +1153,43 @@ __device__ double near_entry(int i, int j, const int32_t* __restrict__ kind, con
   return v;
 }
 
+// entry sources of the block build: the resident near field, or the rows of the reference's sparse matrix
+struct NearSrc {
+  NearRows a;
+  __device__ double operator()(int i, int j) const {
+    return near_entry(i, j, a.kind, a.kappa, a.point, a.node_row, a.row_block, a.row_ptr, a.col_ptr, a.mat_ptr,
+                      a.col_idx, a.ncol_dof, a.mat, a.alpha);
+  }
+};
+struct RowsSrc {
+  IluRows A;
+  __device__ double operator()(int i, int j) const {
+    int64_t lo = A.ptr[i], hi = A.ptr[i + 1];  // columns ascending: first column >= j
+    while (lo < hi) {
+      const int64_t mid = (lo + hi) >> 1;
+      if (A.col[mid] < j) lo = mid + 1;
+      else hi = mid;
+    }
+    return (lo < A.ptr[i + 1] && A.col[lo] == j) ? A.val[lo] : 0.0;
+  }
+};
+
 // One workgroup per block (m <= 128 DOFs): extract A_b into LDS, invert it in place by Gauss-Jordan
 // elimination with partial pivoting, and store the core rows of A_b^-1 (ncore x m, row-major).
 // A block with a vanishing or non-finite pivot falls back to the inverse diagonal on its core
 // (as SchwarzAdditive_CRS does when its block LU fails) and is flagged in failed[b].
 // LDS: m*m doubles (matrix) + m doubles (pivot column) + m ints (DOF ids) + m ints (pivot rows).
+// rows_out: the stored rows per block (the core for the restricted combination, all m for the averaged one).
 static_assert(kSchwarzMaxBlock <= 128, "k_schwarz_build updates one column per thread, two rows per pass of 256 threads");
+template <class Src>
 __global__ __launch_bounds__(256) void k_schwarz_build(
-    const int64_t* __restrict__ bptr, const int32_t* __restrict__ bidx, const int32_t* __restrict__ bncore,
-    const int64_t* __restrict__ inv_ptr, const int32_t* __restrict__ kind, const double* __restrict__ kappa,
-    const int32_t* __restrict__ point, const int64_t* __restrict__ node_row, const int32_t* __restrict__ row_block,
-    const int64_t* __restrict__ row_ptr,
-    const int64_t* __restrict__ col_ptr, const int64_t* __restrict__ mat_ptr, const int32_t* __restrict__ col_idx,
-    const int32_t* __restrict__ ncol_dof, const double* __restrict__ mat, const double* __restrict__ alpha,
-    double* __restrict__ inv, int32_t* __restrict__ failed) {
+    const int64_t* __restrict__ bptr, const int32_t* __restrict__ bidx, const int32_t* __restrict__ rows_out,
+    const int64_t* __restrict__ inv_ptr, Src entry, double* __restrict__ inv, int32_t* __restrict__ failed) {
   extern __shared__ __align__(16) unsigned char smem[];
   const int b = blockIdx.x;
   const int64_t p0 = bptr[b];
   const int m = (int)(bptr[b + 1] - p0);
-  const int nc = bncore[b];
+  const int nc = rows_out[b];
   double* A = reinterpret_cast<double*>(smem);   // m*m
   double* colk = A + m * m;                      // m
   int* ids = reinterpret_cast<int*>(colk + m);   // m
@@ -1181,9 +1199,7 @@ __global__ __launch_bounds__(256) void k_schwarz_build(
   if (tid == 0) bad = 0;
   for (int t = tid; t < m; t += blockDim.x) ids[t] = bidx[p0 + t];
   __syncthreads();
-  for (int e = tid; e < m * m; e += blockDim.x)
-    A[e] = near_entry(ids[e / m], ids[e % m], kind, kappa, point, node_row, row_block, row_ptr, col_ptr, mat_ptr,
-                      col_idx, ncol_dof, mat, alpha);
+  for (int e = tid; e < m * m; e += blockDim.x) A[e] = entry(ids[e / m], ids[e % m]);
   __syncthreads();
   for (int k = 0; k < m; ++k) {
     if (tid < 64) {
@@ -1247,8 +1263,7 @@ __global__ __launch_bounds__(256) void k_schwarz_build(
       const int i = e / m, j = e % m;
       double v = 0.;
       if (i == j) {
-        const double d = near_entry(ids[i], ids[i], kind, kappa, point, node_row, row_block, row_ptr, col_ptr,
-                                    mat_ptr, col_idx, ncol_dof, mat, alpha);
+        const double d = entry(ids[i], ids[i]);
         v = (isfinite(d) && fabs(d) > 1e-15) ? 1.0 / d : 1.0;
       }
       out[e] = v;
@@ -1292,6 +1307,40 @@ __global__ __launch_bounds__(256) void k_schwarz_apply(const int64_t* __restrict
     for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off);
     if (lane == 0) z[bidx[p0 + i]] = s;
   }
+}
+
+// Averaged additive Schwarz (SchwarzAdditive_CRS::solve, basic_linear_systems.hpp:2960-3003), two passes
+// with no atomics: every block's local solution A_b^-1 r|_b (all m rows) into y at the block's positions, then
+// each DOF sums the entries of the blocks that hold it (in block order) and divides by their number.
+__global__ __launch_bounds__(256) void k_schwarz_solve_all(const int64_t* __restrict__ bptr,
+                                                           const int32_t* __restrict__ bidx,
+                                                           const int64_t* __restrict__ inv_ptr,
+                                                           const double* __restrict__ inv,
+                                                           const double* __restrict__ r, double* __restrict__ y) {
+  __shared__ double rl[kSchwarzMaxBlock];
+  const int b = blockIdx.x;
+  const int64_t p0 = bptr[b];
+  const int m = (int)(bptr[b + 1] - p0);
+  for (int t = threadIdx.x; t < m; t += blockDim.x) rl[t] = r[bidx[p0 + t]];
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const double* Ib = inv + inv_ptr[b];
+  for (int i = wave; i < m; i += 4) {
+    const double* row = Ib + (int64_t)i * m;
+    double s = 0.;
+    for (int j = lane; j < m; j += 64) s += row[j] * rl[j];
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off);
+    if (lane == 0) y[p0 + i] = s;
+  }
+}
+__global__ void k_schwarz_average(int n, const int64_t* __restrict__ occ_ptr, const int64_t* __restrict__ occ_pos,
+                                  const double* __restrict__ y, double* __restrict__ z) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= n) return;
+  const int64_t a = occ_ptr[g], e = occ_ptr[g + 1];
+  double s = 0.;
+  for (int64_t q = a; q < e; ++q) s += y[occ_pos[q]];
+  z[g] = e > a ? s / (double)(e - a) : 0.;
 }
 
 // ---- Morton DOF order (GMRES_FMM.hpp:3164-3246) ----
@@ -1596,9 +1645,21 @@ void diag_inv(hipStream_t s, int n, const NearRows& a, double* dinv) {
 void schwarz_build(hipStream_t s, const Schwarz& b, int max_block, const NearRows& a, double* inv, int32_t* failed) {
   if (b.nblk == 0) return;
   const size_t lds = (size_t)max_block * max_block * sizeof(double) + (size_t)max_block * 16;
-  hipLaunchKernelGGL(k_schwarz_build, dim3(b.nblk), dim3(256), lds, s, b.bptr, b.bidx, b.bncore, b.inv_ptr, a.kind,
-                     a.kappa, a.point, a.node_row, a.row_block, a.row_ptr, a.col_ptr, a.mat_ptr, a.col_idx, a.ncol_dof,
-                     a.mat, a.alpha, inv, failed);
+  hipLaunchKernelGGL(k_schwarz_build<NearSrc>, dim3(b.nblk), dim3(256), lds, s, b.bptr, b.bidx, b.bncore, b.inv_ptr,
+                     NearSrc{a}, inv, failed);
+}
+void schwarz_build_rows(hipStream_t s, const Schwarz& b, int max_block, const IluRows& A, const int32_t* rows_out,
+                        double* inv, int32_t* failed) {
+  if (b.nblk == 0) return;
+  const size_t lds = (size_t)max_block * max_block * sizeof(double) + (size_t)max_block * 16;
+  hipLaunchKernelGGL(k_schwarz_build<RowsSrc>, dim3(b.nblk), dim3(256), lds, s, b.bptr, b.bidx, rows_out, b.inv_ptr,
+                     RowsSrc{A}, inv, failed);
+}
+void schwarz_apply_averaged(hipStream_t s, const Schwarz& b, int n, const int64_t* occ_ptr, const int64_t* occ_pos,
+                            const double* inv, const double* r, double* y, double* z) {
+  if (b.nblk == 0) return;
+  hipLaunchKernelGGL(k_schwarz_solve_all, dim3(b.nblk), dim3(256), 0, s, b.bptr, b.bidx, b.inv_ptr, inv, r, y);
+  hipLaunchKernelGGL(k_schwarz_average, dim3(grid1(n)), dim3(256), 0, s, n, occ_ptr, occ_pos, y, z);
 }
 
 void dots(hipStream_t s, int k, const double* V, int64_t ldv, const double* w, int64_t n, double* partial) {

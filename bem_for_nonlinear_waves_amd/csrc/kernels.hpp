@@ -230,6 +230,14 @@ struct IluRows {
   const int64_t* diag = nullptr;  // position of each row's diagonal
 };
 constexpr int kIluBigLevel = 4096;  // levels with at least this many rows get a launch of their own
+// Schwarz blocks from the rows of the reference's sparse matrix (its SCHWARZ option): rows_out[b] rows of
+// A_b^-1 stored per block (all m for the averaged combination)
+void schwarz_build_rows(hipStream_t s, const Schwarz& b, int max_block, const IluRows& A, const int32_t* rows_out,
+                        double* inv, int32_t* failed);
+// averaged additive Schwarz: y (sum of block sizes) receives every block's local solution, z[g] the mean of
+// the entries of g (occurrence list occ_pos[occ_ptr[g] .. occ_ptr[g+1]) into y, in block order)
+void schwarz_apply_averaged(hipStream_t s, const Schwarz& b, int n, const int64_t* occ_ptr, const int64_t* occ_pos,
+                            const double* inv, const double* r, double* y, double* z);
 constexpr int kIluRunThreads = 1024;
 // backward: upper part and the division by the diagonal, else the strictly lower part.
 // One level: its nrows rows at `level_rows`, one thread each.

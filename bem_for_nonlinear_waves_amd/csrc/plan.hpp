@@ -184,5 +184,8 @@ void build_ilu_rows(int n_nodes, const double* X, int n_faces, const int32_t* fv
                     int neighborhood, int kring, SparseRows& A);
 void ilu0_factor(SparseRows& A, double pivot_min = 1e-20);  // throws std::runtime_error on a bad pivot
 void sweep_levels(const SparseRows& A, SweepLevels& forward, SweepLevels& backward);
+// the reference's Schwarz blocks on the rows' sparsity graph (cores by breadth-first search to depth core_k)
+void build_schwarz_blocks_rows(const SparseRows& A, int core_k, int overlap_k, int max_core, int max_block,
+                               SchwarzBlocks& out);
 
 }  // namespace bemfmm

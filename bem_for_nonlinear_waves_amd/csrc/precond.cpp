@@ -11,6 +11,8 @@
 // phi_n) and -G_n for the others (unknown phi). Every DOF of the point gets these coefficients; Neumann-type
 // rows add -alpha on the diagonal, CORNER rows are the identity, a missing diagonal is set to 1 (:640-662).
 //
+// build_schwarz_blocks_rows restates the reference's SCHWARZ block partition on the same matrix's graph.
+//
 // ilu0_factor and sweep_levels restate ILU0_CRS (lib/include/basic_linear_systems.hpp:2337-2560): sorted rows,
 // dependency levels of the forward and backward sweeps, IKJ elimination restricted to the pattern, a pivot
 // below 1e-20 is an error.
@@ -44,11 +46,6 @@ const double kGW5R[25][3] = {
     {0.953089922969332, 0.5, 0.0336962680968792},               {0.953089922969332, 0.769234655052841, 0.0283499999999991},
     {0.953089922969332, 0.953089922969332, 0.0140335872156063}};
 const double kGW1R[1][3] = {{0.5, 0.5, 1.0}};
-
-struct FaceGeom {
-  const double* X;     // scaled node coordinates
-  const int32_t* fv;   // node ids per face
-};
 
 // single- and double-layer weights of face f (corners in the rotated order `ord`) seen from point x: rule
 // collapsed at corner ord[0]; singular (x is corner ord[0]): single layer only
@@ -337,6 +334,77 @@ void sweep_levels(const SparseRows& A, SweepLevels& Lv, SweepLevels& Uv) {
     nl = std::max(nl, lev[i] + 1);
   }
   bucket(Uv, nl);
+}
+
+// The reference's Schwarz blocks (BEM_solveBVP_GMRES_FMM.hpp:901-1010) on the sparsity graph of the rows
+// (i -> its columns j != i, ascending): seeds in row order; a core is a breadth-first search from the seed over
+// unassigned DOFs to depth core_k, at most max_core DOFs; the block adds overlap_k rings of neighbours of the
+// core (assigned or not) up to max_block DOFs.
+void build_schwarz_blocks_rows(const SparseRows& A, int core_k, int overlap_k, int max_core, int max_block,
+                               SchwarzBlocks& out) {
+  const int n = A.n;
+  if (max_core < 1 || max_block < max_core) throw std::runtime_error("schwarz: need 1 <= max_core <= max_block");
+  core_k = std::clamp(core_k, 0, 8);
+  overlap_k = std::clamp(overlap_k, 0, 8);
+  std::vector<char> assigned(n, 0);
+  std::vector<int> mark(n, -1);
+  int stamp = 0;
+  std::vector<std::pair<int, int>> q;  // (DOF, depth), consumed from `head`
+  out = SchwarzBlocks{};
+  std::vector<int32_t> core, block;
+  for (int seed = 0; seed < n; ++seed) {
+    if (assigned[seed]) continue;
+    core.clear();
+    ++stamp;
+    q.clear();
+    size_t head = 0;
+    q.push_back({seed, 0});
+    mark[seed] = stamp;
+    while (head < q.size() && (int)core.size() < max_core) {
+      const auto [u, dep] = q[head++];
+      if (!assigned[u]) {
+        assigned[u] = 1;
+        core.push_back(u);
+        if ((int)core.size() >= max_core) break;
+      }
+      if (dep >= core_k) continue;
+      for (int64_t e = A.ptr[u]; e < A.ptr[u + 1]; ++e) {
+        const int v = A.col[e];
+        if (v == u || mark[v] == stamp) continue;
+        mark[v] = stamp;
+        if (assigned[v]) continue;
+        q.push_back({v, dep + 1});
+      }
+    }
+    if (core.empty()) {
+      assigned[seed] = 1;
+      core.push_back(seed);
+    }
+    block = core;
+    ++stamp;
+    q.clear();
+    head = 0;
+    for (int u : core) {
+      mark[u] = stamp;
+      q.push_back({u, 0});
+    }
+    while (head < q.size() && (int)block.size() < max_block) {
+      const auto [u, dep] = q[head++];
+      if (dep >= overlap_k) continue;
+      for (int64_t e = A.ptr[u]; e < A.ptr[u + 1]; ++e) {
+        const int v = A.col[e];
+        if (v == u || mark[v] == stamp) continue;
+        mark[v] = stamp;
+        block.push_back(v);
+        if ((int)block.size() >= max_block) break;
+        q.push_back({v, dep + 1});
+      }
+    }
+    out.idx.insert(out.idx.end(), block.begin(), block.end());
+    out.ptr.push_back((int64_t)out.idx.size());
+    out.ncore.push_back((int32_t)core.size());
+    out.max_block = std::max(out.max_block, (int)block.size());
+  }
 }
 
 }  // namespace bemfmm

@@ -71,7 +71,8 @@ bool same_opts(const bemfmm_opts& a, const bemfmm_opts& b) {
          a.schwarz_overlap == b.schwarz_overlap && a.m2l_tile_pairs == b.m2l_tile_pairs &&
          a.dof_reorder == b.dof_reorder && a.use_graphs == b.use_graphs && a.m2l_kernel == b.m2l_kernel &&
          a.overlap_near == b.overlap_near && a.refine_recycle == b.refine_recycle &&
-         a.ilu_neighborhood == b.ilu_neighborhood && a.ilu_kring == b.ilu_kring;
+         a.ilu_neighborhood == b.ilu_neighborhood && a.ilu_kring == b.ilu_kring &&
+         a.schwarz_variant == b.schwarz_variant && a.schwarz_core_k == b.schwarz_core_k;
 }
 
 }  // namespace

@@ -93,6 +93,13 @@ typedef struct bemfmm_opts {
                             1 = K-RING (the point's faces and ilu_kring rings of edge neighbours);
                             ilu_neighborhood_type, main_time_domain.cpp:39 */
   int ilu_kring;         /* rings for K-RING, default 1 (ilu_kring_num, main_time_domain.cpp:40) */
+  int schwarz_variant;   /* BEMFMM_PRECOND_SCHWARZ: 0 (default) = blocks grown on the mesh DOF graph, entries from
+                            the FMM near field, restricted combination (each DOF from its own core's block);
+                            1 = the reference's SCHWARZ: blocks on the graph of the ILU(0) sparse matrix (cores by
+                            breadth-first search to depth schwarz_core_k, GMRES_FMM.hpp:901-1010), entries from that
+                            matrix, combination averaged over the covering blocks (SchwarzAdditive_CRS::solve,
+                            basic_linear_systems.hpp:2960-3003) */
+  int schwarz_core_k;    /* schwarz_variant 1: core search depth, default 1 (main_time_domain.cpp:45) */
 } bemfmm_opts;
 
 typedef struct bemfmm_result {
@@ -120,7 +127,7 @@ typedef struct bemfmm_stats {
   int n_ops_uploaded;           /* translation operators uploaded by the last prepare (the rest were resident) */
   int m2l_tile_pairs;           /* M2L pairs per tile in use (autotuned unless set in the options) */
   int m2l_rotation;             /* 1 if the FP32 operator's M2L runs by rotation */
-  int64_t precond_nnz;          /* ILU(0): entries of the sparse matrix */
+  int64_t precond_nnz;          /* ILU(0) and the reference's Schwarz: entries of the sparse matrix */
   int precond_levels;           /* ILU(0): dependency levels of the forward plus the backward sweep */
 } bemfmm_stats;
 

@@ -16,6 +16,7 @@
 //   rigid-mode alpha          bem/BEM_solveBVP_GMRES_FMM.hpp:2674-2716
 //   GMRES                     bem/BEM_solveBVP_GMRES_FMM.hpp:2007-2355, lib/include/basic_linear_systems.hpp:3137-3451
 //   ILU(0) preconditioner     bem/BEM_solveBVP_GMRES_FMM.hpp:215-662, lib/include/basic_linear_systems.hpp:2337-2603
+//   SCHWARZ preconditioner    bem/BEM_solveBVP_GMRES_FMM.hpp:901-1013, lib/include/basic_linear_systems.hpp:2800-3003
 //
 // The reference's translation tables AAA_{M2M,M2L,L2L}_FMM.hpp are absent, so
 // the three operators are evaluated from the Greengard & Rokhlin (1997)
@@ -31,6 +32,7 @@
 #include <complex>
 #include <cstdint>
 #include <cstring>
+#include <deque>
 #include <map>
 #include <set>
 #include <memory>
@@ -1269,7 +1271,104 @@ struct Oracle {
     }
   }
 
-  int precond = 0;  // 0 none, 1 Jacobi, 2 restricted additive Schwarz (blocks set beforehand), 3 ILU(0) (built)
+  // the reference's SCHWARZ on the same matrix (bem/BEM_solveBVP_GMRES_FMM.hpp:901-1013,
+  // lib/include/basic_linear_systems.hpp:2800-3003): blocks from breadth-first searches over the matrix graph,
+  // each block solved by LU with partial pivoting, the block solutions averaged over the blocks covering a DOF
+  std::vector<std::vector<int>> swr_blocks;
+  std::vector<int> swr_ncore;
+  void build_schwarz_ref(int core_k, int overlap_k, int max_core, int max_block) {
+    swr_blocks.clear();
+    swr_ncore.clear();
+    std::vector<bool> taken(n_dof, false);
+    for (int seed = 0; seed < n_dof; ++seed) {
+      if (taken[seed]) continue;
+      std::vector<int> core;
+      std::set<int> seen{seed};
+      std::deque<std::pair<int, int>> q{{seed, 0}};
+      while (!q.empty() && (int)core.size() < max_core) {
+        auto [u, dep] = q.front();
+        q.pop_front();
+        if (!taken[u]) {
+          taken[u] = true;
+          core.push_back(u);
+          if ((int)core.size() >= max_core) break;
+        }
+        if (dep >= core_k) continue;
+        for (const auto& [v, val] : ilu_S[u]) {
+          (void)val;
+          if (v == u || seen.count(v)) continue;
+          seen.insert(v);
+          if (!taken[v]) q.push_back({v, dep + 1});
+        }
+      }
+      if (core.empty()) {
+        taken[seed] = true;
+        core.push_back(seed);
+      }
+      std::vector<int> block = core;
+      std::set<int> seen2(core.begin(), core.end());
+      std::deque<std::pair<int, int>> q2;
+      for (int u : core) q2.push_back({u, 0});
+      while (!q2.empty() && (int)block.size() < max_block) {
+        auto [u, dep] = q2.front();
+        q2.pop_front();
+        if (dep >= overlap_k) continue;
+        for (const auto& [v, val] : ilu_S[u]) {
+          (void)val;
+          if (v == u || seen2.count(v)) continue;
+          seen2.insert(v);
+          block.push_back(v);
+          if ((int)block.size() >= max_block) break;
+          q2.push_back({v, dep + 1});
+        }
+      }
+      swr_ncore.push_back((int)core.size());
+      swr_blocks.push_back(block);
+    }
+  }
+  double s_entry(int i, int j) const {
+    auto it = ilu_S[i].find(j);
+    return it == ilu_S[i].end() ? 0.0 : it->second;
+  }
+  void schwarz_ref_apply(const double* r, double* z) const {
+    std::vector<double> sum(n_dof, 0.);
+    std::vector<int> cover(n_dof, 0);
+    for (const auto& blk : swr_blocks) {
+      const int m = (int)blk.size();
+      std::vector<double> A((size_t)m * m), y(m);
+      for (int i = 0; i < m; ++i) {
+        y[i] = r[blk[i]];
+        for (int j = 0; j < m; ++j) A[(size_t)i * m + j] = s_entry(blk[i], blk[j]);
+      }
+      for (int k = 0; k < m; ++k) {  // LU with partial pivoting (lapack_lu's dgetrf/dgetrs)
+        int pr = k;
+        for (int i = k + 1; i < m; ++i)
+          if (std::abs(A[(size_t)i * m + k]) > std::abs(A[(size_t)pr * m + k])) pr = i;
+        if (pr != k) {
+          for (int j = 0; j < m; ++j) std::swap(A[(size_t)k * m + j], A[(size_t)pr * m + j]);
+          std::swap(y[k], y[pr]);
+        }
+        for (int i = k + 1; i < m; ++i) {
+          const double f = A[(size_t)i * m + k] / A[(size_t)k * m + k];
+          for (int j = k + 1; j < m; ++j) A[(size_t)i * m + j] -= f * A[(size_t)k * m + j];
+          y[i] -= f * y[k];
+        }
+      }
+      for (int i = m - 1; i >= 0; --i) {
+        double acc = y[i];
+        for (int j = i + 1; j < m; ++j) acc -= A[(size_t)i * m + j] * y[j];
+        y[i] = acc / A[(size_t)i * m + i];
+      }
+      for (int i = 0; i < m; ++i) {
+        sum[blk[i]] += y[i];
+        cover[blk[i]] += 1;
+      }
+    }
+    for (int g = 0; g < n_dof; ++g) z[g] = cover[g] ? sum[g] / cover[g] : 0.;
+  }
+
+  int precond = 0;  // 0 none, 1 Jacobi, 2 restricted additive Schwarz (blocks set beforehand), 3 ILU(0) (built),
+                    // 4 the reference's SCHWARZ (built)
   int solve(const double* x0, double* xout, double tol, int max_iter, int restart, int* iters_out, double* res_out) {
     std::vector<double> b(n_dof);
     rhs(b.data());
@@ -1281,6 +1380,9 @@ struct Oracle {
         v.swap(tmp);
       } else if (precond == 3) {
         ilu_apply(v.data(), tmp.data());
+        v.swap(tmp);
+      } else if (precond == 4) {
+        schwarz_ref_apply(v.data(), tmp.data());
         v.swap(tmp);
       } else {
         for (int i = 0; i < n_dof; ++i) v[i] *= dinv[i];
@@ -1507,6 +1609,23 @@ void oracle_ilu_rows(void* h, long long* ptr, int* col, double* val) {
   }
 }
 void oracle_ilu_apply(void* h, const double* r, double* z) { static_cast<Oracle*>(h)->ilu_apply(r, z); }
+// the reference's SCHWARZ on the ILU(0) matrix (oracle_ilu_build first): returns the number of blocks
+int oracle_schwarz_ref_build(void* h, int core_k, int overlap_k, int max_core, int max_block) {
+  auto* o = static_cast<Oracle*>(h);
+  o->build_schwarz_ref(core_k, overlap_k, max_core, max_block);
+  return (int)o->swr_blocks.size();
+}
+void oracle_schwarz_ref_blocks(void* h, long long* ptr, int* idx, int* ncore) {
+  auto* o = static_cast<Oracle*>(h);
+  long long q = 0;
+  ptr[0] = 0;
+  for (size_t b = 0; b < o->swr_blocks.size(); ++b) {
+    for (int g : o->swr_blocks[b]) idx[q++] = g;
+    ptr[b + 1] = q;
+    ncore[b] = o->swr_ncore[b];
+  }
+}
+void oracle_schwarz_ref_apply(void* h, const double* r, double* z) { static_cast<Oracle*>(h)->schwarz_ref_apply(r, z); }
 // factored rows in the same layout (the pattern of the unfactored rows)
 void oracle_ilu_factored(void* h, long long* ptr, int* col, double* val) {
   auto* o = static_cast<Oracle*>(h);

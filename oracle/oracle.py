@@ -84,6 +84,10 @@ def lib():
         L.oracle_ilu_factored.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_longlong), _ip, _dp]
         L.oracle_ilu_apply.argtypes = [ctypes.c_void_p, _dp, _dp]
         L.oracle_ilu_face_rule.argtypes = [_dp, _dp, ctypes.c_int, ctypes.c_int, _dp, _dp]
+        L.oracle_schwarz_ref_build.restype = ctypes.c_int
+        L.oracle_schwarz_ref_build.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+        L.oracle_schwarz_ref_blocks.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_longlong), _ip, _ip]
+        L.oracle_schwarz_ref_apply.argtypes = [ctypes.c_void_p, _dp, _dp]
         L.oracle_sph_harmonics.argtypes = [_dp, ctypes.c_int, _dp, _dp]
         L.oracle_dR.argtypes = [_dp, _dp, ctypes.c_int, _dp, _dp]
         L.oracle_translate_chain.argtypes = [ctypes.c_int, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, ctypes.c_int, _dp]
@@ -212,6 +216,22 @@ class Oracle:
         r = _d(r)
         z = np.empty(self.n_dof)
         lib().oracle_ilu_apply(self.h, _p(r, _dp), _p(z, _dp))
+        return z
+
+    def schwarz_ref_build(self, core_k=1, overlap_k=1, max_core=64, max_block=128):
+        """The reference's SCHWARZ blocks on the ILU(0) matrix (ilu_build first); returns (ptr, idx, ncore)."""
+        nb = lib().oracle_schwarz_ref_build(self.h, core_k, overlap_k, max_core, max_block)
+        ptr = np.empty(nb + 1, np.int64)
+        idx = np.empty(self.n_dof * max_block, np.int32)
+        ncore = np.empty(nb, np.int32)
+        lib().oracle_schwarz_ref_blocks(self.h, ptr.ctypes.data_as(ctypes.POINTER(ctypes.c_longlong)), _p(idx, _ip),
+                                        _p(ncore, _ip))
+        return ptr, idx[:ptr[-1]].copy(), ncore
+
+    def schwarz_ref_apply(self, r):
+        r = _d(r)
+        z = np.empty(self.n_dof)
+        lib().oracle_schwarz_ref_apply(self.h, _p(r, _dp), _p(z, _dp))
         return z
 
     def near_row(self, t, cap=100000):

@@ -424,4 +424,15 @@ void emu_ilu_levels(void* h, int32_t* lev_fwd, int32_t* lev_bwd) {
     for (int r = Uv.ptr[l]; r < Uv.ptr[l + 1]; ++r) lev_bwd[Uv.rows[r]] = l;
 }
 void emu_ilu_destroy(void* h) { delete static_cast<bemfmm::SparseRows*>(h); }
+// the reference's Schwarz blocks on the rows of an emu_ilu_create handle (built with factor = 0)
+int emu_schwarz_rows_blocks(void* h, int core_k, int overlap_k, int max_core, int max_block, int64_t* ptr,
+                            int32_t* idx, int32_t* ncore, int64_t cap) {
+  bemfmm::SchwarzBlocks B;
+  bemfmm::build_schwarz_blocks_rows(*static_cast<bemfmm::SparseRows*>(h), core_k, overlap_k, max_core, max_block, B);
+  if ((int64_t)B.idx.size() > cap) return -1;
+  std::copy(B.ptr.begin(), B.ptr.end(), ptr);
+  std::copy(B.idx.begin(), B.idx.end(), idx);
+  std::copy(B.ncore.begin(), B.ncore.end(), ncore);
+  return B.n_blocks();
+}
 }

@@ -527,18 +527,18 @@ void diag_inv(hipStream_t s, int n, const NearRows& a, double* dinv) {
   });
 }
 
-void schwarz_build(hipStream_t s, const Schwarz& b, int max_block, const NearRows& a, double* inv, int32_t* failed) {
+template <class Entry>
+void schwarz_build_with(hipStream_t s, const Schwarz& b, const int32_t* rows_out, Entry entry, double* inv,
+                        int32_t* failed) {
   const Schwarz B = b;
-  const NearRows A = a;
-  (void)max_block;
   hipstub::enqueue(s, [=]() {
     for (int blk = 0; blk < B.nblk; ++blk) {
       const int64_t p0 = B.bptr[blk];
-      const int m = (int)(B.bptr[blk + 1] - p0), nc = B.bncore[blk];
+      const int m = (int)(B.bptr[blk + 1] - p0), nc = rows_out[blk];
       const int32_t* ids = B.bidx + p0;
       std::vector<double> M((size_t)m * m), colk(m);
       std::vector<int> piv(m);
-      for (int e = 0; e < m * m; ++e) M[e] = near_entry(ids[e / m], ids[e % m], A);
+      for (int e = 0; e < m * m; ++e) M[e] = entry(ids[e / m], ids[e % m]);
       bool bad = false;
       for (int k = 0; k < m && !bad; ++k) {
         double best = -1.0;
@@ -571,7 +571,7 @@ void schwarz_build(hipStream_t s, const Schwarz& b, int max_block, const NearRow
           const int i = e / m, j = e % m;
           double v = 0.;
           if (i == j) {
-            const double d = near_entry(ids[i], ids[i], A);
+            const double d = entry(ids[i], ids[i]);
             v = (std::isfinite(d) && std::fabs(d) > 1e-15) ? 1.0 / d : 1.0;
           }
           out[e] = v;
@@ -641,6 +641,42 @@ void scale_dup(hipStream_t s, int64_t n, double a, double* w, double* dup) {
       const double v = a * w[j];
       w[j] = v;
       dup[j] = v;
+    }
+  });
+}
+void schwarz_build(hipStream_t s, const Schwarz& b, int max_block, const NearRows& a, double* inv, int32_t* failed) {
+  (void)max_block;
+  const NearRows A = a;
+  schwarz_build_with(s, b, b.bncore, [A](int i, int j) { return near_entry(i, j, A); }, inv, failed);
+}
+void schwarz_build_rows(hipStream_t s, const Schwarz& b, int max_block, const IluRows& A, const int32_t* rows_out,
+                        double* inv, int32_t* failed) {
+  (void)max_block;
+  const IluRows R = A;
+  schwarz_build_with(s, b, rows_out, [R](int i, int j) {
+    for (int64_t q = R.ptr[i]; q < R.ptr[i + 1]; ++q)
+      if (R.col[q] == j) return R.val[q];
+    return 0.0;
+  }, inv, failed);
+}
+void schwarz_apply_averaged(hipStream_t s, const Schwarz& b, int n, const int64_t* occ_ptr, const int64_t* occ_pos,
+                            const double* inv, const double* r, double* y, double* z) {
+  const Schwarz B = b;
+  hipstub::enqueue(s, [=]() {
+    for (int blk = 0; blk < B.nblk; ++blk) {
+      const int64_t p0 = B.bptr[blk];
+      const int m = (int)(B.bptr[blk + 1] - p0);
+      const double* Ib = inv + B.inv_ptr[blk];
+      for (int i = 0; i < m; ++i) {
+        double acc = 0.;
+        for (int j = 0; j < m; ++j) acc += Ib[(int64_t)i * m + j] * r[B.bidx[p0 + j]];
+        y[p0 + i] = acc;
+      }
+    }
+    for (int g = 0; g < n; ++g) {
+      double acc = 0.;
+      for (int64_t q = occ_ptr[g]; q < occ_ptr[g + 1]; ++q) acc += y[occ_pos[q]];
+      z[g] = occ_ptr[g + 1] > occ_ptr[g] ? acc / (double)(occ_ptr[g + 1] - occ_ptr[g]) : 0.;
     }
   });
 }

@@ -1,6 +1,6 @@
-"""The reference's sparse preconditioner matrix and its ILU(0) (bem/BEM_solveBVP_GMRES_FMM.hpp:215-662,
-lib/include/basic_linear_systems.hpp:2337-2603): the oracle restatement pinned by what the mathematics fixes,
-then the product's host build (precond.cpp, through the CPU emulator) against the oracle.
+"""The reference's sparse preconditioner matrix, its ILU(0) and its SCHWARZ (bem/BEM_solveBVP_GMRES_FMM.hpp:
+215-1013, lib/include/basic_linear_systems.hpp:2337-3003): the oracle restatement pinned by what the mathematics
+fixes, then the product's host build (precond.cpp, through the CPU emulator) against the oracle.
 
 Pins of the oracle:
 * the per-face rules against adaptive integration of the same integrals (scipy): the single-layer weights of a
@@ -13,7 +13,9 @@ Pins of the oracle:
   stored (i, j), which defines ILU(0) for a given row order (a full pattern cannot be built to check exactness:
   the faces at a point carry no double-layer weight, so its own Neumann columns are always absent);
 * the matrix: the CORNER rows are the identity, the rows of the DOFs of one point share their off-diagonal
-  entries, and the Neumann-type diagonals carry -alpha.
+  entries, and the Neumann-type diagonals carry -alpha;
+* SCHWARZ: its cores partition the DOFs and each block starts with its core; one block over every DOF inverts the
+  matrix (M^-1 S = I); blocks of one DOF give the inverse diagonal.
 """
 import ctypes
 import os
@@ -170,3 +172,60 @@ def test_product_ilu_rows_and_factors_equal_the_oracle(oracle_mod, neighborhood,
         ptr, col, val = out[factor]
         assert np.array_equal(ptr, ref[0]) and np.array_equal(col, ref[1])
         assert np.abs(val - ref[2]).max() <= 1e-12 * np.abs(ref[2]).max(), factor
+
+
+def test_oracle_schwarz_reference_pins(oracle_mod):
+    m, d, phi, phin = _tank(4, 2, 2)
+    o = oracle_mod.Oracle(m, d, phi, phin, p=8, leaf_max=16)
+    o.ilu_build(1, 20)
+    ptr, col, val = o.ilu_rows()
+    n = d.n_dof
+    S = _dense(n, ptr, col, val)
+    x = np.random.default_rng(4).standard_normal(n)
+    # one block over every DOF: the exact inverse
+    bp, bi, nc = o.schwarz_ref_build(core_k=8, overlap_k=8, max_core=n, max_block=n)
+    assert len(nc) == 1 and nc[0] == n
+    assert np.linalg.norm(o.schwarz_ref_apply(S @ x) - x) <= 1e-10 * np.linalg.norm(x)
+    # blocks of one DOF: the inverse diagonal
+    bp, bi, nc = o.schwarz_ref_build(core_k=1, overlap_k=0, max_core=1, max_block=1)
+    assert len(nc) == n
+    assert np.allclose(o.schwarz_ref_apply(x), x / np.diag(S), rtol=1e-14)
+    # default blocking on the BUCKETS matrix: cores partition the DOFs, blocks start with their core
+    o.ilu_build(0, 1)
+    bp, bi, nc = o.schwarz_ref_build()
+    cores = np.concatenate([bi[bp[b]:bp[b] + nc[b]] for b in range(len(nc))])
+    assert np.array_equal(np.sort(cores), np.arange(n))
+    assert np.diff(bp).max() <= 128 and nc.max() <= 64
+
+
+@pytest.mark.parametrize("neighborhood,core_k,overlap_k,max_core,max_block",
+                         [(0, 1, 1, 64, 128), (1, 1, 1, 64, 128), (0, 2, 2, 16, 40)])
+def test_product_schwarz_blocks_equal_the_oracle(oracle_mod, neighborhood, core_k, overlap_k, max_core, max_block):
+    import subprocess
+    subprocess.run(["make", "-s", "-C", os.path.join(HERE, "emu")], check=True)
+    m, d, phi, phin = _tank(10, 5, 5)
+    o = oracle_mod.Oracle(m, d, phi, phin, p=8, leaf_max=16)
+    o.ilu_build(neighborhood, 1)
+    ref = o.schwarz_ref_build(core_k, overlap_k, max_core, max_block)
+    alpha = np.ascontiguousarray(o.alpha())
+    lib = _emu()
+    lib.emu_schwarz_rows_blocks.restype = ctypes.c_int
+    lib.emu_schwarz_rows_blocks.argtypes = [ctypes.c_void_p] + [ctypes.c_int] * 4 + [ctypes.c_void_p] * 3 + [
+        ctypes.c_int64]
+    X = np.ascontiguousarray(m.X, np.float64)
+    arrs = [np.ascontiguousarray(a, np.int32) for a in (m.tri, d.face_dof, d.dof_kind, d.dof_point)]
+    nnz = np.zeros(1, np.int64)
+    lev = np.zeros(2, np.int32)
+    h = lib.emu_ilu_create(m.n_nodes, X.ctypes.data, m.n_faces, arrs[0].ctypes.data, arrs[1].ctypes.data, d.n_dof,
+                           arrs[2].ctypes.data, arrs[3].ctypes.data, alpha.ctypes.data, neighborhood, 1, 0,
+                           nnz.ctypes.data, lev.ctypes.data)
+    cap = d.n_dof * max_block
+    ptr = np.empty(d.n_dof + 1, np.int64)
+    idx = np.empty(cap, np.int32)
+    ncore = np.empty(d.n_dof, np.int32)
+    nb = lib.emu_schwarz_rows_blocks(h, core_k, overlap_k, max_core, max_block, ptr.ctypes.data, idx.ctypes.data,
+                                     ncore.ctypes.data, cap)
+    lib.emu_ilu_destroy(h)
+    assert nb == len(ref[2])
+    assert np.array_equal(ptr[:nb + 1], ref[0]) and np.array_equal(idx[:ptr[nb]], ref[1])
+    assert np.array_equal(ncore[:nb], ref[2])
