@@ -17,7 +17,7 @@ _ip = ctypes.POINTER(ctypes.c_int32)
 OK = 0
 E_INVALID, E_DEVICE, E_OOM, E_NONFINITE, E_NOT_CONVERGED, E_STATE = -1, -2, -3, -4, -5, -6
 PRECISION_FP64, PRECISION_FP32_REFINE = 0, 1
-PRECOND_NONE, PRECOND_JACOBI, PRECOND_SCHWARZ, PRECOND_ILU0 = 0, 1, 2, 3
+PRECOND_NONE, PRECOND_JACOBI, PRECOND_SCHWARZ, PRECOND_ILU0, PRECOND_MILU0, PRECOND_ILUT = 0, 1, 2, 3, 4, 5
 
 
 class Opts(ctypes.Structure):
@@ -28,7 +28,8 @@ class Opts(ctypes.Structure):
                 ("dof_reorder", ctypes.c_int), ("use_graphs", ctypes.c_int), ("m2l_kernel", ctypes.c_int),
                 ("overlap_near", ctypes.c_int), ("refine_recycle", ctypes.c_int),
                 ("ilu_neighborhood", ctypes.c_int), ("ilu_kring", ctypes.c_int), ("schwarz_variant", ctypes.c_int),
-                ("schwarz_core_k", ctypes.c_int)]
+                ("schwarz_core_k", ctypes.c_int), ("milu_omega", ctypes.c_double), ("ilut_drop_tol", ctypes.c_double),
+                ("ilut_max_entries", ctypes.c_int), ("ilut_pivot_min", ctypes.c_double)]
 
 
 class Result(ctypes.Structure):

@@ -53,7 +53,8 @@ class BEM_BVP:
                  schwarz_max_block: int = 128, schwarz_overlap: int = 1, m2l_tile_pairs: int = 0, dof_reorder: int = 1,
                  use_graphs: int = 0, m2l_kernel: int = 0, overlap_near: int = 1, refine_recycle: int = 1,
                  ilu_neighborhood: int = 0, ilu_kring: int = 1, schwarz_variant: int = 0, schwarz_core_k: int = 1,
-                 device: int = 0):
+                 milu_omega: float = 1.0, ilut_drop_tol: float = 1e-3, ilut_max_entries: int = 50,
+                 ilut_pivot_min: float = 1e-12, device: int = 0):
         self.lib = _lib.load()
         o = _lib.Opts()
         self.lib.bemfmm_default_opts(ctypes.byref(o))
@@ -64,6 +65,8 @@ class BEM_BVP:
         o.m2l_kernel, o.overlap_near, o.refine_recycle = m2l_kernel, overlap_near, refine_recycle
         o.ilu_neighborhood, o.ilu_kring = ilu_neighborhood, ilu_kring
         o.schwarz_variant, o.schwarz_core_k = schwarz_variant, schwarz_core_k
+        o.milu_omega, o.ilut_drop_tol, o.ilut_max_entries, o.ilut_pivot_min = (milu_omega, ilut_drop_tol,
+                                                                              ilut_max_entries, ilut_pivot_min)
         self.ctx = self.lib.bemfmm_ctx_create(device, ctypes.byref(o))
         if not self.ctx:
             raise BEMError(_lib.E_DEVICE, "bemfmm_ctx_create failed (no usable GPU?)")

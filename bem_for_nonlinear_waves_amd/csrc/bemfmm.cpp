@@ -852,7 +852,13 @@ void do_prepare(bemfmm_ctx* c) {
     throw std::runtime_error("bemfmm_prepare: m2l_kernel must be 0 (default), 1 (matrix cores) or 2 (rotation)");
   if (c->opts.m2l_kernel == 2 && !dev::m2l_rot_compiled(c->opts.p))
     throw std::runtime_error("bemfmm_prepare: the rotation M2L is compiled for p = 8 and 10 only");
-  if (c->opts.preconditioner == BEMFMM_PRECOND_ILU0 &&
+  if (c->opts.preconditioner == BEMFMM_PRECOND_ILUT &&
+      (!(c->opts.ilut_drop_tol >= 0.) || c->opts.ilut_max_entries < 0 || !(c->opts.ilut_pivot_min > 0.)))
+    throw std::runtime_error("bemfmm_prepare: ILUT needs ilut_drop_tol >= 0, ilut_max_entries >= 0, ilut_pivot_min > 0");
+  if (c->opts.preconditioner == BEMFMM_PRECOND_MILU0 && !(c->opts.milu_omega >= 0.))
+    throw std::runtime_error("bemfmm_prepare: MILU(0) needs milu_omega >= 0");
+  if ((c->opts.preconditioner == BEMFMM_PRECOND_ILU0 || c->opts.preconditioner == BEMFMM_PRECOND_MILU0 ||
+       c->opts.preconditioner == BEMFMM_PRECOND_ILUT) &&
       (c->opts.ilu_neighborhood < 0 || c->opts.ilu_neighborhood > 1 || c->opts.ilu_kring < 0 || c->opts.ilu_kring > 20))
     throw std::runtime_error("bemfmm_prepare: ilu_neighborhood must be 0 (BUCKETS) or 1 (K-RING), 0 <= ilu_kring <= 20");
   if (c->opts.preconditioner == BEMFMM_PRECOND_SCHWARZ &&
@@ -1187,11 +1193,18 @@ void do_prepare(bemfmm_ctx* c) {
       c->sw_failed = 0;
       for (int32_t f : fh) c->sw_failed += f;
     }
-  } else if (pc == BEMFMM_PRECOND_ILU0) {
-    // ILU(0) factorised on the host as ILU0_CRS does; the sweeps run on the GPU by dependency level
+  } else if (pc == BEMFMM_PRECOND_ILU0 || pc == BEMFMM_PRECOND_MILU0 || pc == BEMFMM_PRECOND_ILUT) {
+    // ILU(0) / MILU(0) / ILUT factorised on the host as ILU0_CRS and ILUT_CRS do; the sweeps run on the GPU by
+    // dependency level
     bemfmm::SparseRows A;
     sparse_rows(A);
-    bemfmm::ilu0_factor(A);
+    if (pc == BEMFMM_PRECOND_ILUT) {
+      bemfmm::SparseRows F;
+      bemfmm::ilut_factor(A, c->opts.ilut_drop_tol, c->opts.ilut_max_entries, c->opts.ilut_pivot_min, F);
+      A = std::move(F);
+    } else {
+      bemfmm::ilu0_factor(A, 1e-20, pc == BEMFMM_PRECOND_MILU0, c->opts.milu_omega);
+    }
     bemfmm::SweepLevels Lv, Uv;
     bemfmm::sweep_levels(A, Lv, Uv);
     c->ilu_nnz = (int64_t)A.col.size();
@@ -1293,9 +1306,9 @@ void do_prepare(bemfmm_ctx* c) {
   st.n_ops_uploaded = c->ops_uploaded;
   st.m2l_tile_pairs = c->m2l_tp;
   st.m2l_rotation = c->m2l_rot ? 1 : 0;
-  st.precond_nnz = (pc == BEMFMM_PRECOND_ILU0 || (pc == BEMFMM_PRECOND_SCHWARZ && c->opts.schwarz_variant == 1))
-                       ? c->ilu_nnz : 0;
-  st.precond_levels = pc == BEMFMM_PRECOND_ILU0 ? c->ilu_levels : 0;
+  const bool ilu_kind = pc == BEMFMM_PRECOND_ILU0 || pc == BEMFMM_PRECOND_MILU0 || pc == BEMFMM_PRECOND_ILUT;
+  st.precond_nnz = (ilu_kind || (pc == BEMFMM_PRECOND_SCHWARZ && c->opts.schwarz_variant == 1)) ? c->ilu_nnz : 0;
+  st.precond_levels = ilu_kind ? c->ilu_levels : 0;
   (void)t0;
 }
 
@@ -1356,7 +1369,9 @@ void precond_apply(bemfmm_ctx* c, const double* in, double* out) {
     else
       dev::schwarz_apply(c->stream, schwarz_view(c), c->sw_inv.p, in, out);
     break;
-  case BEMFMM_PRECOND_ILU0: {
+  case BEMFMM_PRECOND_ILU0:
+  case BEMFMM_PRECOND_MILU0:
+  case BEMFMM_PRECOND_ILUT: {
     if (in != out) HIPCHECK(hipMemcpyAsync(out, in, n * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
     dev::IluRows A{c->ilu_ptr.p, c->ilu_col.p, c->ilu_val.p, c->ilu_diag.p};
     for (const auto& g : c->ilu_lseg) {
@@ -1938,6 +1953,10 @@ void bemfmm_default_opts(bemfmm_opts* o) {
   o->ilu_kring = 1;         // main_time_domain.cpp:40
   o->schwarz_variant = 0;
   o->schwarz_core_k = 1;    // main_time_domain.cpp:45
+  o->milu_omega = 1.0;        // main_time_domain.cpp:41
+  o->ilut_drop_tol = 1e-3;    // main_time_domain.cpp:42
+  o->ilut_max_entries = 50;   // main_time_domain.cpp:43
+  o->ilut_pivot_min = 1e-12;  // main_time_domain.cpp:44
 }
 
 bemfmm_ctx* bemfmm_ctx_create(int device, const bemfmm_opts* opts) {

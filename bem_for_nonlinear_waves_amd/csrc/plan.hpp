@@ -182,7 +182,11 @@ struct SweepLevels {
 void build_ilu_rows(int n_nodes, const double* X, int n_faces, const int32_t* fv, const int32_t* face_dof, int n_dof,
                     const int32_t* dof_kind, const int32_t* dof_point, const double* alpha, double scale,
                     int neighborhood, int kring, SparseRows& A);
-void ilu0_factor(SparseRows& A, double pivot_min = 1e-20);  // throws std::runtime_error on a bad pivot
+// ILU(0) in place (modified: MILU(0), the dropped fill-in times omega moved to the diagonal); throws
+// std::runtime_error on a bad pivot
+void ilu0_factor(SparseRows& A, double pivot_min = 1e-20, bool modified = false, double omega = 1.0);
+// ILUT (threshold dropping, at most max_entries per L and U row): factors of A into F (same row layout)
+void ilut_factor(const SparseRows& A, double drop_tol, int max_entries, double pivot_min, SparseRows& F);
 void sweep_levels(const SparseRows& A, SweepLevels& forward, SweepLevels& backward);
 // the reference's Schwarz blocks on the rows' sparsity graph (cores by breadth-first search to depth core_k)
 void build_schwarz_blocks_rows(const SparseRows& A, int core_k, int overlap_k, int max_core, int max_block,

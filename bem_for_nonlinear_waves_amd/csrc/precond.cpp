@@ -14,11 +14,13 @@
 // build_schwarz_blocks_rows restates the reference's SCHWARZ block partition on the same matrix's graph.
 //
 // ilu0_factor and sweep_levels restate ILU0_CRS (lib/include/basic_linear_systems.hpp:2337-2560): sorted rows,
-// dependency levels of the forward and backward sweeps, IKJ elimination restricted to the pattern, a pivot
-// below 1e-20 is an error.
+// dependency levels of the forward and backward sweeps, IKJ elimination restricted to the pattern (MILU(0): the
+// dropped fill-in times omega subtracted from the diagonal), a pivot below 1e-20 is an error; ilut_factor
+// restates ILUT_CRS.
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <map>
 #include <stdexcept>
 #include <string>
 #include <utility>
@@ -272,8 +274,9 @@ void build_ilu_rows(int n_nodes, const double* X, int n_faces, const int32_t* fv
   }
 }
 
-void ilu0_factor(SparseRows& A, double pivot_min) {
+void ilu0_factor(SparseRows& A, double pivot_min, bool modified, double omega) {
   const int n = A.n;
+  if (modified && !(omega >= 0. && std::isfinite(omega))) throw std::runtime_error("MILU(0): omega must be >= 0");
   for (int i = 0; i < n; ++i) {
     const int64_t di = A.diag[i], end = A.ptr[i + 1];
     for (int64_t ik = A.ptr[i]; ik < di; ++ik) {
@@ -295,15 +298,88 @@ void ilu0_factor(SparseRows& A, double pivot_min) {
         } else if (ci < ck) {
           ++pi;
         } else {
+          // fill-in outside the pattern: dropped (MILU(0) moves it to the diagonal)
+          if (modified) A.val[di] -= omega * (lik * A.val[pk]);
           ++pk;
         }
       }
+      if (modified)
+        for (; pk < kend; ++pk) A.val[di] -= omega * (lik * A.val[pk]);
     }
   }
   for (int i = 0; i < n; ++i) {
     const double d = A.val[A.diag[i]];
     if (!std::isfinite(d) || std::abs(d) < pivot_min)
       throw std::runtime_error("ILU(0): zero or non-finite pivot at row " + std::to_string(i));
+  }
+}
+
+// ILUT_CRS (basic_linear_systems.hpp:2610-2760): row by row, the working row eliminated by the finished U rows
+// with fill-in allowed; entries below drop_tol * |row of A|_1 dropped, then the max_entries largest of the L and
+// of the U part kept; a diagonal below pivot_min is set to +-pivot_min. F gets L (strictly lower), the diagonal
+// and U in one row each.
+void ilut_factor(const SparseRows& A, double drop_tol, int max_entries, double pivot_min, SparseRows& F) {
+  const int n = A.n;
+  if (max_entries < 0 || !(drop_tol >= 0.) || !(pivot_min > 0.)) throw std::runtime_error("ILUT: invalid parameters");
+  std::vector<std::vector<std::pair<int32_t, double>>> U(n);  // diagonal first, then the upper part ascending
+  std::vector<std::vector<std::pair<int32_t, double>>> L(n);
+  std::vector<double> udiag(n, 0.);
+  auto keep_largest = [&](std::vector<std::pair<int32_t, double>>& v) {
+    if ((int)v.size() <= max_entries) return;
+    std::nth_element(v.begin(), v.begin() + max_entries, v.end(),
+                     [](const auto& a, const auto& b) { return std::abs(a.second) > std::abs(b.second); });
+    v.resize(max_entries);
+  };
+  std::map<int32_t, double> w;
+  for (int i = 0; i < n; ++i) {
+    w.clear();
+    double norm1 = 0.;
+    for (int64_t q = A.ptr[i]; q < A.ptr[i + 1]; ++q) {
+      w[A.col[q]] += A.val[q];
+      norm1 += std::abs(A.val[q]);
+    }
+    const double thr = drop_tol * norm1;
+    for (auto it = w.begin(); it != w.end() && it->first < i; ++it) {
+      const int k = it->first;
+      if (std::abs(udiag[k]) < pivot_min) throw std::runtime_error("ILUT: zero pivot at row " + std::to_string(k));
+      const double lik = it->second / udiag[k];
+      it->second = lik;
+      for (const auto& [j, ukj] : U[k])
+        if (j > k) w[j] -= lik * ukj;
+    }
+    double diag = 0.;
+    if (auto it = w.find(i); it != w.end()) diag = it->second;
+    if (std::abs(diag) < pivot_min) diag = diag >= 0. ? pivot_min : -pivot_min;
+    std::vector<std::pair<int32_t, double>> lc, uc;
+    for (const auto& [c, v] : w) {
+      if (c == i || v == 0.) continue;
+      if (drop_tol > 0. && std::abs(v) < thr) continue;
+      (c < i ? lc : uc).push_back({c, v});
+    }
+    keep_largest(lc);
+    keep_largest(uc);
+    std::sort(lc.begin(), lc.end());
+    std::sort(uc.begin(), uc.end());
+    uc.insert(uc.begin(), {i, diag});
+    L[i] = std::move(lc);
+    U[i] = std::move(uc);
+    udiag[i] = diag;
+  }
+  F = SparseRows{};
+  F.n = n;
+  F.ptr.assign(n + 1, 0);
+  F.diag.assign(n, -1);
+  for (int i = 0; i < n; ++i) {
+    for (const auto& e : L[i]) {
+      F.col.push_back(e.first);
+      F.val.push_back(e.second);
+    }
+    F.diag[i] = (int64_t)F.col.size();
+    for (const auto& e : U[i]) {
+      F.col.push_back(e.first);
+      F.val.push_back(e.second);
+    }
+    F.ptr[i + 1] = (int64_t)F.col.size();
   }
 }
 

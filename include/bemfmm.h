@@ -54,6 +54,9 @@ extern "C" {
 #define BEMFMM_PRECOND_ILU0 3      /* ILU(0) (ILU, ILU0_CRS, basic_linear_systems.hpp:2337-2606) of the reference's sparse
                                       matrix (buildSparseMatrixForILU, GMRES_FMM.hpp:215-662; neighbourhood from
                                       ilu_neighborhood / ilu_kring); factorised on the host, sweeps on the GPU by level */
+#define BEMFMM_PRECOND_MILU0 4     /* MILU(0): ILU(0) with the dropped fill-in times milu_omega moved to the diagonal
+                                      (MILU, ILU0_CRS(modified = true), basic_linear_systems.hpp:2379-2520) */
+#define BEMFMM_PRECOND_ILUT 5      /* ILUT (ILUT_CRS, basic_linear_systems.hpp:2610-2807) on the same sparse matrix */
 
 typedef struct bemfmm_ctx bemfmm_ctx;
 
@@ -100,6 +103,10 @@ typedef struct bemfmm_opts {
                             matrix, combination averaged over the covering blocks (SchwarzAdditive_CRS::solve,
                             basic_linear_systems.hpp:2960-3003) */
   int schwarz_core_k;    /* schwarz_variant 1: core search depth, default 1 (main_time_domain.cpp:45) */
+  double milu_omega;     /* MILU(0) diagonal compensation, default 1 (main_time_domain.cpp:41) */
+  double ilut_drop_tol;  /* ILUT: entries below drop_tol * |row of A|_1 are dropped, default 1e-3 (:42) */
+  int ilut_max_entries;  /* ILUT: largest entries kept per L row and per U row, default 50 (:43) */
+  double ilut_pivot_min; /* ILUT: smaller diagonals are set to +-ilut_pivot_min, default 1e-12 (:44) */
 } bemfmm_opts;
 
 typedef struct bemfmm_result {
@@ -127,8 +134,8 @@ typedef struct bemfmm_stats {
   int n_ops_uploaded;           /* translation operators uploaded by the last prepare (the rest were resident) */
   int m2l_tile_pairs;           /* M2L pairs per tile in use (autotuned unless set in the options) */
   int m2l_rotation;             /* 1 if the FP32 operator's M2L runs by rotation */
-  int64_t precond_nnz;          /* ILU(0) and the reference's Schwarz: entries of the sparse matrix */
-  int precond_levels;           /* ILU(0): dependency levels of the forward plus the backward sweep */
+  int64_t precond_nnz;          /* ILU(0)/MILU(0)/ILUT (the factors) and the reference's Schwarz (the matrix): entries */
+  int precond_levels;           /* ILU(0)/MILU(0)/ILUT: dependency levels of the forward plus the backward sweep */
 } bemfmm_stats;
 
 /* Availability (metal_m2l.h: metal_m2l_is_available / metal_m2l_device_name). */

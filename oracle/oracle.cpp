@@ -1235,10 +1235,49 @@ struct Oracle {
         else if (!ilu_S[d].count(d)) ilu_S[d][d] = 1.0;
       }
     }
-    // ILU(0): row i in turn, each of its entries k < i (ascending) divided by U_kk and its multiple of row k's
-    // upper part subtracted where row i has an entry
     ilu_LU.assign(n_dof, {});
     std::vector<std::map<int, double>> W = ilu_S;
+    if (ilu_kind == 2) {
+      // ILUT (ILUT_CRS, basic_linear_systems.hpp:2610-2760): fill-in allowed, then entries below
+      // drop_tol * |row|_1 dropped and the largest max_entries of each of the L and U parts kept
+      for (int i = 0; i < n_dof; ++i) {
+        std::map<int, double> wi = ilu_S[i];
+        double norm1 = 0.;
+        for (const auto& [c, v] : ilu_S[i]) norm1 += std::abs(v);
+        for (auto it = wi.begin(); it != wi.end() && it->first < i; ++it) {
+          const int k = it->first;
+          double dk = 0.;
+          for (const auto& [c, v] : W[k])
+            if (c == k) dk = v;
+          if (!(std::abs(dk) >= ilut_pivot_min)) throw std::runtime_error("oracle ILUT: bad pivot");
+          it->second /= dk;
+          for (const auto& [c, v] : W[k])
+            if (c > k) wi[c] -= it->second * v;
+        }
+        double dg = wi.count(i) ? wi[i] : 0.;
+        if (std::abs(dg) < ilut_pivot_min) dg = dg >= 0. ? ilut_pivot_min : -ilut_pivot_min;
+        std::vector<std::pair<int, double>> lo, up;
+        for (const auto& [c, v] : wi) {
+          if (c == i || v == 0. || (ilut_drop_tol > 0. && std::abs(v) < ilut_drop_tol * norm1)) continue;
+          (c < i ? lo : up).push_back({c, v});
+        }
+        for (auto* part : {&lo, &up}) {
+          if ((int)part->size() > ilut_max_entries) {
+            std::sort(part->begin(), part->end(),
+                      [](const auto& a, const auto& b) { return std::abs(a.second) > std::abs(b.second); });
+            part->resize(ilut_max_entries);
+          }
+        }
+        W[i].clear();
+        for (const auto& e : lo) W[i][e.first] = e.second;
+        W[i][i] = dg;
+        for (const auto& e : up) W[i][e.first] = e.second;
+        ilu_LU[i].assign(W[i].begin(), W[i].end());
+      }
+      return;
+    }
+    // ILU(0): row i in turn, each of its entries k < i (ascending) divided by U_kk and its multiple of row k's
+    // upper part subtracted where row i has an entry (MILU(0): elsewhere subtracted from the diagonal, times omega)
     for (int i = 0; i < n_dof; ++i) {
       auto& wi = W[i];
       for (auto it = wi.begin(); it != wi.end() && it->first < i; ++it) {
@@ -1249,11 +1288,15 @@ struct Oracle {
         for (auto jt = W[k].upper_bound(k); jt != W[k].end(); ++jt) {
           auto e = wi.find(jt->first);
           if (e != wi.end()) e->second -= it->second * jt->second;
+          else if (ilu_kind == 1) wi.at(i) -= milu_omega * (it->second * jt->second);
         }
       }
       ilu_LU[i].assign(wi.begin(), wi.end());
     }
   }
+  int ilu_kind = 0;  // 0 ILU(0), 1 MILU(0), 2 ILUT
+  double milu_omega = 1.0, ilut_drop_tol = 1e-3, ilut_pivot_min = 1e-12;
+  int ilut_max_entries = 50;
   void ilu_apply(const double* r, double* z) const {
     for (int i = 0; i < n_dof; ++i) {
       double s = r[i];
@@ -1626,7 +1669,21 @@ void oracle_schwarz_ref_blocks(void* h, long long* ptr, int* idx, int* ncore) {
   }
 }
 void oracle_schwarz_ref_apply(void* h, const double* r, double* z) { static_cast<Oracle*>(h)->schwarz_ref_apply(r, z); }
-// factored rows in the same layout (the pattern of the unfactored rows)
+// factorisation used by oracle_ilu_build: 0 ILU(0), 1 MILU(0) (omega), 2 ILUT (drop_tol, max_entries, pivot_min)
+void oracle_ilu_kind(void* h, int kind, double omega, double drop_tol, int max_entries, double pivot_min) {
+  auto* o = static_cast<Oracle*>(h);
+  o->ilu_kind = kind;
+  o->milu_omega = omega;
+  o->ilut_drop_tol = drop_tol;
+  o->ilut_max_entries = max_entries;
+  o->ilut_pivot_min = pivot_min;
+}
+long long oracle_ilu_factored_nnz(void* h) {
+  long long q = 0;
+  for (const auto& r : static_cast<Oracle*>(h)->ilu_LU) q += (long long)r.size();
+  return q;
+}
+// factored rows: ptr (n + 1), col, val (ILU(0)/MILU(0): the pattern of the unfactored rows)
 void oracle_ilu_factored(void* h, long long* ptr, int* col, double* val) {
   auto* o = static_cast<Oracle*>(h);
   long long q = 0;

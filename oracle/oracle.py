@@ -84,6 +84,10 @@ def lib():
         L.oracle_ilu_factored.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_longlong), _ip, _dp]
         L.oracle_ilu_apply.argtypes = [ctypes.c_void_p, _dp, _dp]
         L.oracle_ilu_face_rule.argtypes = [_dp, _dp, ctypes.c_int, ctypes.c_int, _dp, _dp]
+        L.oracle_ilu_kind.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int,
+                                      ctypes.c_double]
+        L.oracle_ilu_factored_nnz.restype = ctypes.c_longlong
+        L.oracle_ilu_factored_nnz.argtypes = [ctypes.c_void_p]
         L.oracle_schwarz_ref_build.restype = ctypes.c_int
         L.oracle_schwarz_ref_build.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]
         L.oracle_schwarz_ref_blocks.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_longlong), _ip, _ip]
@@ -189,28 +193,30 @@ class Oracle:
         lib().oracle_diag_inv(self.h, _p(d, _dp))
         return d
 
-    def ilu_build(self, neighborhood=0, kring=1):
-        """The reference's sparse preconditioner matrix (BUCKETS = 0 / K-RING = 1) and its ILU(0); returns nnz."""
+    def ilu_build(self, neighborhood=0, kring=1, kind=0, omega=1.0, drop_tol=1e-3, max_entries=50, pivot_min=1e-12):
+        """The reference's sparse preconditioner matrix (BUCKETS = 0 / K-RING = 1) and its factorisation (kind 0
+        ILU(0), 1 MILU(0), 2 ILUT); returns the matrix's nnz."""
+        lib().oracle_ilu_kind(self.h, kind, omega, drop_tol, max_entries, pivot_min)
         nnz = lib().oracle_ilu_build(self.h, neighborhood, kring)
         if nnz < 0:
             raise RuntimeError("oracle ILU(0) failed")
         self._ilu_nnz = int(nnz)
         return self._ilu_nnz
 
-    def _ilu_get(self, fn):
+    def _ilu_get(self, fn, nnz):
         ptr = np.empty(self.n_dof + 1, np.int64)
-        col = np.empty(self._ilu_nnz, np.int32)
-        val = np.empty(self._ilu_nnz)
+        col = np.empty(nnz, np.int32)
+        val = np.empty(nnz)
         fn(self.h, ptr.ctypes.data_as(ctypes.POINTER(ctypes.c_longlong)), _p(col, _ip), _p(val, _dp))
         return ptr, col, val
 
     def ilu_rows(self):
         """(ptr, col, val) of the unfactored sparse matrix (columns ascending per row)."""
-        return self._ilu_get(lib().oracle_ilu_rows)
+        return self._ilu_get(lib().oracle_ilu_rows, self._ilu_nnz)
 
     def ilu_factored(self):
-        """(ptr, col, val) of the ILU(0) factors on the same pattern: L strictly below the diagonal, U from it."""
-        return self._ilu_get(lib().oracle_ilu_factored)
+        """(ptr, col, val) of the factors: L strictly below the diagonal, U from it."""
+        return self._ilu_get(lib().oracle_ilu_factored, int(lib().oracle_ilu_factored_nnz(self.h)))
 
     def ilu_apply(self, r):
         r = _d(r)

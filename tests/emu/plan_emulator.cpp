@@ -396,7 +396,13 @@ void* emu_ilu_create(int n_nodes, const double* X, int n_faces, const int32_t* t
     auto* A = new bemfmm::SparseRows();
     bemfmm::build_ilu_rows(n_nodes, pl.Xs.data(), n_faces, pl.fv.data(), face_dof, n_dof, kind, point, alpha, pl.scale,
                            neighborhood, kring, *A);
-    if (factor) bemfmm::ilu0_factor(*A);
+    // factor: 0 none, 1 ILU(0), 2 MILU(0) (omega 1), 3 ILUT (drop_tol 1e-3, 50 entries, pivot_min 1e-12)
+    if (factor == 1 || factor == 2) bemfmm::ilu0_factor(*A, 1e-20, factor == 2, 1.0);
+    if (factor == 3) {
+      bemfmm::SparseRows F;
+      bemfmm::ilut_factor(*A, 1e-3, 50, 1e-12, F);
+      *A = std::move(F);
+    }
     bemfmm::SweepLevels Lv, Uv;
     bemfmm::sweep_levels(*A, Lv, Uv);
     *nnz = (int64_t)A->col.size();

@@ -24,29 +24,39 @@ def _bvp(m, d, phi, phin, **kw):
     return b
 
 
-@pytest.mark.parametrize("neighborhood,kring", [(0, 1), (1, 1), (1, 2)])
-def test_ilu_apply_parity(small_tank, oracle_mod, neighborhood, kring):
+@pytest.mark.parametrize("kind,neighborhood,kring", [(0, 0, 1), (0, 1, 1), (0, 1, 2), (1, 1, 1), (2, 1, 1), (2, 1, 2)])
+def test_ilu_apply_parity(small_tank, oracle_mod, kind, neighborhood, kring):
+    """kind 0 ILU(0), 1 MILU(0), 2 ILUT (the reference's default parameters). MILU(0) runs on the K-RING matrix:
+    on this tank's BUCKETS matrix its diagonal compensation leaves a pivot of 8e-5 (the largest is 6.8), which
+    turns the 1e-16 rounding differences of the two sides' matrices into 1e-4 differences of the factors; on the
+    K-RING matrix the factors agree to 1e-12 and the smallest pivot (4e-5) amplifies that in the sweeps, so MILU(0)
+    is held to 1e-8. ILUT too runs on the K-RING matrix: on the BUCKETS one, entries within rounding of the drop
+    threshold are kept on one side and dropped on the other (same count, other pattern); on K-RING the factors
+    agree to 1e-16."""
     from bem_for_nonlinear_waves_amd import _lib
     m, d, phi, phin = small_tank
-    g = _bvp(m, d, phi, phin, p=8, leaf_max=32, preconditioner=_lib.PRECOND_ILU0, dof_reorder=0,
-             ilu_neighborhood=neighborhood, ilu_kring=kring)
+    pc = [_lib.PRECOND_ILU0, _lib.PRECOND_MILU0, _lib.PRECOND_ILUT][kind]
+    g = _bvp(m, d, phi, phin, p=8, leaf_max=32, preconditioner=pc, dof_reorder=0, ilu_neighborhood=neighborhood,
+             ilu_kring=kring)
     st = g.stats()
     o = oracle_mod.Oracle(m, d, phi, phin, p=8, leaf_max=32)
-    nnz = o.ilu_build(neighborhood, kring)
-    print(f"[gpu] ILU(0) nnz={st['precond_nnz']} (oracle {nnz}) levels={st['precond_levels']}", flush=True)
+    o.ilu_build(neighborhood, kring, kind=kind)
+    nnz = len(o.ilu_factored()[1])
+    print(f"[gpu] ILU kind {kind}: factor nnz={st['precond_nnz']} (oracle {nnz}) levels={st['precond_levels']}",
+          flush=True)
     assert st["precond_nnz"] == nnz
     rng = np.random.default_rng(31)
     for _ in range(2):
         r = rng.standard_normal(d.n_dof)
         z_g, z_o = g.precond_apply(r), o.ilu_apply(r)
         rel = np.linalg.norm(z_g - z_o) / np.linalg.norm(z_o)
-        print(f"[gpu] ILU(0) apply vs oracle: {rel:.3e}", flush=True)
-        assert rel <= 1e-10
+        print(f"[gpu] ILU kind {kind} apply vs oracle: {rel:.3e}", flush=True)
+        assert rel <= (1e-8 if kind == 1 else 1e-10)
     g.close()
 
 
-@pytest.mark.parametrize("precision", [0, 1])
-def test_ilu_solve_vs_dense_lu_and_oracle_iterations(small_tank, oracle_mod, precision):
+@pytest.mark.parametrize("precision,pc_name", [(0, "ILU0"), (1, "ILU0"), (0, "MILU0"), (0, "ILUT")])
+def test_ilu_solve_vs_dense_lu_and_oracle_iterations(small_tank, oracle_mod, precision, pc_name):
     from bem_for_nonlinear_waves_amd import _lib
     m, d, phi, phin = small_tank
     o = oracle_mod.Oracle(m, d, phi, phin, p=8, leaf_max=32)
@@ -54,14 +64,15 @@ def test_ilu_solve_vs_dense_lu_and_oracle_iterations(small_tank, oracle_mod, pre
     L = o.stats()["L"]
     x_lu = np.linalg.solve(A, b)
     x_lu = np.where(d.dof_kind == 0, x_lu / L, x_lu)
-    g = _bvp(m, d, phi, phin, p=8, leaf_max=32, preconditioner=_lib.PRECOND_ILU0, precision=precision)
+    g = _bvp(m, d, phi, phin, p=8, leaf_max=32, preconditioner=getattr(_lib, "PRECOND_" + pc_name),
+             precision=precision, ilu_neighborhood=1 if pc_name == "MILU0" else 0)
     x, info = g.solveSystemGMRES(tol=1e-10, max_iter=1000)
     rel = np.linalg.norm(x - x_lu) / np.linalg.norm(x_lu)
-    print(f"[gpu] ILU(0) solve precision={precision}: iters={info['iters']} steps={info['refinement_steps']} "
+    print(f"[gpu] {pc_name} solve precision={precision}: iters={info['iters']} steps={info['refinement_steps']} "
           f"residual={info['residual']:.3e} vs dense LU {rel:.3e}", flush=True)
     assert info["converged"] == 1 and rel <= 1e-8
     g.close()
-    if precision == 0:
+    if precision == 0 and pc_name == "ILU0":
         g = _bvp(m, d, phi, phin, p=8, leaf_max=32, preconditioner=_lib.PRECOND_ILU0, dof_reorder=0)
         _, info0 = g.solveSystemGMRES(tol=1e-10, max_iter=1000)
         g.close()
