@@ -29,7 +29,8 @@ class Opts(ctypes.Structure):
                 ("overlap_near", ctypes.c_int), ("refine_recycle", ctypes.c_int),
                 ("ilu_neighborhood", ctypes.c_int), ("ilu_kring", ctypes.c_int), ("schwarz_variant", ctypes.c_int),
                 ("schwarz_core_k", ctypes.c_int), ("milu_omega", ctypes.c_double), ("ilut_drop_tol", ctypes.c_double),
-                ("ilut_max_entries", ctypes.c_int), ("ilut_pivot_min", ctypes.c_double)]
+                ("ilut_max_entries", ctypes.c_int), ("ilut_pivot_min", ctypes.c_double),
+                ("plane_wave_digits", ctypes.c_int)]
 
 
 class Result(ctypes.Structure):
@@ -48,7 +49,8 @@ class Stats(ctypes.Structure):
                 ("t_alpha", ctypes.c_double), ("n_precond_blocks", ctypes.c_int),
                 ("precond_fallback_blocks", ctypes.c_int), ("precond_avg_block", ctypes.c_double),
                 ("n_ops_uploaded", ctypes.c_int), ("m2l_tile_pairs", ctypes.c_int),
-                ("m2l_rotation", ctypes.c_int), ("precond_nnz", ctypes.c_int64), ("precond_levels", ctypes.c_int)]
+                ("m2l_rotation", ctypes.c_int), ("precond_nnz", ctypes.c_int64), ("precond_levels", ctypes.c_int),
+                ("m2l_plane_wave", ctypes.c_int)]
 
 
 # (name, restype, argtypes) of every entry point declared in include/bemfmm.h

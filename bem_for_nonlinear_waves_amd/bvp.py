@@ -54,7 +54,7 @@ class BEM_BVP:
                  use_graphs: int = 0, m2l_kernel: int = 0, overlap_near: int = 1, refine_recycle: int = 1,
                  ilu_neighborhood: int = 0, ilu_kring: int = 1, schwarz_variant: int = 0, schwarz_core_k: int = 1,
                  milu_omega: float = 1.0, ilut_drop_tol: float = 1e-3, ilut_max_entries: int = 50,
-                 ilut_pivot_min: float = 1e-12, device: int = 0):
+                 ilut_pivot_min: float = 1e-12, plane_wave_digits: int = 10, device: int = 0):
         self.lib = _lib.load()
         o = _lib.Opts()
         self.lib.bemfmm_default_opts(ctypes.byref(o))
@@ -67,6 +67,7 @@ class BEM_BVP:
         o.schwarz_variant, o.schwarz_core_k = schwarz_variant, schwarz_core_k
         o.milu_omega, o.ilut_drop_tol, o.ilut_max_entries, o.ilut_pivot_min = (milu_omega, ilut_drop_tol,
                                                                               ilut_max_entries, ilut_pivot_min)
+        o.plane_wave_digits = plane_wave_digits
         self.ctx = self.lib.bemfmm_ctx_create(device, ctypes.byref(o))
         if not self.ctx:
             raise BEMError(_lib.E_DEVICE, "bemfmm_ctx_create failed (no usable GPU?)")

@@ -257,6 +257,8 @@ struct bemfmm_ctx {
   std::vector<char> rot_have;
   int rot_RS = 0;
   DTiles m2l_rt;  // M2L tiles of kRotTP pairs (one offset each)
+  bool m2l_pw_on = false;  // FP32 operator's M2L through the plane-wave composed operators (m2l_kernel 3)
+  DTiles m2l_pw;           // its tiles (operator slots of the plane-wave M2L offsets)
   int m2l_min_level = 0;  // coarsest level with M2L pairs: M2M into and L2L out of coarser levels are skipped
   DBuf<float> m32;  // FP32 multipoles [box][layer][C] read by the rotation M2L
   // near field beside the FMM passes: the near-only GEMV runs on stream2 between a fork and a join event
@@ -473,7 +475,7 @@ void evaluate_launch(bemfmm_ctx* c, int mode, const double* in, double* out, boo
   } else if (lo) {
     float* stf = reinterpret_cast<float*>(w.stage);
     if (c->m2l_rot) launch_m2l_rot(c, st, stf, w.M0, w.M1, w.m32);
-    else launch_translate(c, c->m2l, c->ops32.p, w.M0, w.M1, nullptr, nullptr, stf, st);
+    else launch_translate(c, c->m2l_pw_on ? c->m2l_pw : c->m2l, c->ops32.p, w.M0, w.M1, nullptr, nullptr, stf, st);
     rec(4);
     dev::m2l_reduce(st, c->C, c->n_boxes, c->m2l_rorder.p, c->m2l_tptr.p, c->m2l_tpairs.p, (const float*)stf, L0, L1);
   } else {
@@ -654,7 +656,13 @@ void upload_knowns(bemfmm_ctx* c) {
 // Maps the plan's operators to slots of the context's device table [slot][KP][RP] (zero padding), uploading
 // only operators the table does not hold yet: a re-prepare after nodes moved (RK stages of a session) reuses
 // nearly all of them. The table grows by doubling (old slots copied on the device).
-std::vector<int32_t> update_operator_table(bemfmm_ctx* c, const Plan& pl) {
+// pw_kind != 0: the M2L offsets take that cache kind (plane-wave composed operators) instead of SimpleM2L's.
+std::vector<int32_t> update_operator_table(bemfmm_ctx* c, const Plan& pl, int pw_kind = 0) {
+  auto key_of = [&](int op) {
+    auto k = bemfmm::op_key(pl, op);
+    if (pw_kind && k[0] == 2) k[0] = pw_kind;
+    return k;
+  };
   hipStream_t s = c->stream;
   const int C = pl.C, KP = c->KP, RP = c->RP;
   const int64_t per = (int64_t)KP * RP;
@@ -670,7 +678,7 @@ std::vector<int32_t> update_operator_table(bemfmm_ctx* c, const Plan& pl) {
   std::vector<int32_t> slot(pl.n_ops);
   std::vector<int> fresh;  // plan ops to upload
   for (int op = 0; op < pl.n_ops; ++op) {
-    const auto key = bemfmm::op_key(pl, op);
+    const auto key = key_of(op);
     auto it = c->op_slot.find(key);
     if (it == c->op_slot.end()) {
       it = c->op_slot.emplace(key, (int32_t)(c->ops_used + fresh.size())).first;
@@ -689,6 +697,13 @@ std::vector<int32_t> update_operator_table(bemfmm_ctx* c, const Plan& pl) {
     c->ops = std::move(t);
     c->ops_cap = cap;
   }
+  // operators not in the process-wide cache yet are computed in parallel first
+  {
+    std::vector<std::array<int, 4>> keys;
+    keys.reserve(fresh.size());
+    for (int op : fresh) keys.push_back(key_of(op));
+    bemfmm::warm_operator_cache(pl.P, keys);
+  }
   // upload the new operators in batches through pinned staging, padded to [KP][RP]
   constexpr int kBatch = 32;
   if (!fresh.empty() && c->op_stage_n < (size_t)kBatch * per) {
@@ -700,7 +715,7 @@ std::vector<int32_t> update_operator_table(bemfmm_ctx* c, const Plan& pl) {
   for (size_t b0 = 0; b0 < fresh.size(); b0 += kBatch) {
     const int nb = (int)std::min<size_t>(kBatch, fresh.size() - b0);
     for (int i = 0; i < nb; ++i) {
-      const auto key = bemfmm::op_key(pl, fresh[b0 + i]);
+      const auto key = key_of(fresh[b0 + i]);
       const auto v = bemfmm::cached_operator(pl.P, key[0], key[1], key[2], key[3]);
       double* dst = c->op_stage + (size_t)i * per;
       std::fill(dst, dst + per, 0.);
@@ -848,8 +863,11 @@ void do_prepare(bemfmm_ctx* c) {
   if (c->opts.m2l_tile_pairs != 0 && c->opts.m2l_tile_pairs != 16 && c->opts.m2l_tile_pairs != 32 &&
       c->opts.m2l_tile_pairs != 64)
     throw std::runtime_error("bemfmm_prepare: m2l_tile_pairs must be 0 (default), 16, 32 or 64");
-  if (c->opts.m2l_kernel < 0 || c->opts.m2l_kernel > 2)
-    throw std::runtime_error("bemfmm_prepare: m2l_kernel must be 0 (default), 1 (matrix cores) or 2 (rotation)");
+  if (c->opts.m2l_kernel < 0 || c->opts.m2l_kernel > 4)
+    throw std::runtime_error("bemfmm_prepare: m2l_kernel must be 0 (default), 1 (matrix cores), 2 (rotation), "
+                             "3 (plane waves, FP32 operator) or 4 (plane waves, both operators)");
+  if (c->opts.m2l_kernel >= 3 && c->opts.plane_wave_digits != 6 && c->opts.plane_wave_digits != 10)
+    throw std::runtime_error("bemfmm_prepare: plane_wave_digits must be 6 or 10");
   if (c->opts.m2l_kernel == 2 && !dev::m2l_rot_compiled(c->opts.p))
     throw std::runtime_error("bemfmm_prepare: the rotation M2L is compiled for p = 8 and 10 only");
   if (c->opts.preconditioner == BEMFMM_PRECOND_ILUT &&
@@ -911,7 +929,10 @@ void do_prepare(bemfmm_ctx* c) {
   fill(c, c->p2m.mat.p, pl.p2m.mat_size, 0.);  // rows past C stay zero
   c->RP = (pl.C + 63) / 64 * 64;
   c->KP = (pl.C + 4 * dev::kKG - 1) / (4 * dev::kKG) * (4 * dev::kKG);
-  const std::vector<int32_t> slot = update_operator_table(c, pl);
+  // plane-wave composed M2L operators: kind 4 (10-digit rule) or 5 (6-digit); m2l_kernel 4 uses them for the FP64
+  // operator too, 3 only for the FP32 one (its own tiles, below)
+  const int pw_kind = c->opts.m2l_kernel >= 3 ? (c->opts.plane_wave_digits == 6 ? 5 : 4) : 0;
+  const std::vector<int32_t> slot = update_operator_table(c, pl, c->opts.m2l_kernel == 4 ? pw_kind : 0);
   c->m2m.clear();
   c->l2l.clear();
   c->m2m.resize(pl.n_levels);
@@ -1246,7 +1267,13 @@ void do_prepare(bemfmm_ctx* c) {
     dev::to_f32(s, (int64_t)pl.near.mat_size, c->near.mat.p, c->near_mat32.p);
     c->p2m_mat32.alloc(c->p2m.mat.n);
     dev::to_f32(s, (int64_t)c->p2m.mat.n, c->p2m.mat.p, c->p2m_mat32.p);
-    c->m2l_rot = c->opts.m2l_kernel != 1 && dev::m2l_rot_compiled(pl.P);
+    c->m2l_rot = (c->opts.m2l_kernel == 0 || c->opts.m2l_kernel == 2) && dev::m2l_rot_compiled(pl.P);
+    c->m2l_pw_on = c->opts.m2l_kernel == 3;
+    if (c->m2l_pw_on) {
+      // the FP32 operator's M2L through plane waves: extra table slots for the composed operators, own tiles
+      const std::vector<int32_t> slot_pw = update_operator_table(c, pl, pw_kind);
+      c->m2l_pw.upload(pl.m2l, s, slot_pw, c->m2l_tp);
+    }
     if (c->m2l_rot) {
       update_rotation_table(c, pl, c->m2l_slot);
       c->m2l_rt.upload(pl.m2l, s, c->m2l_slot, kRotTP);
@@ -1273,6 +1300,7 @@ void do_prepare(bemfmm_ctx* c) {
     c->ops32.release();
     c->ops32_valid = 0;
     c->m2l_rot = false;
+    c->m2l_pw_on = false;
   }
   HIPCHECK(hipStreamSynchronize(s));
   const double t4 = now();
@@ -1291,7 +1319,8 @@ void do_prepare(bemfmm_ctx* c) {
   st.L = pl.Lscale;
   st.device_bytes = (double)(c->near.bytes() + c->p2m.bytes() + c->m2l.bytes() + c->ops.bytes() + c->buf0.bytes() +
                              c->buf1.bytes() + c->M0.bytes() + c->M1.bytes() + c->m2l_stage.bytes() +
-                             c->near_mat32.bytes() + c->p2m_mat32.bytes() + c->ops32.bytes() + c->sw_inv.bytes());
+                             c->near_mat32.bytes() + c->p2m_mat32.bytes() + c->ops32.bytes() + c->sw_inv.bytes() +
+                             c->m2l_pw.bytes());
   st.t_tree = pl.t_tree;
   st.t_lists = pl.t_lists;
   st.t_near_struct = pl.t_near_struct;
@@ -1306,6 +1335,7 @@ void do_prepare(bemfmm_ctx* c) {
   st.n_ops_uploaded = c->ops_uploaded;
   st.m2l_tile_pairs = c->m2l_tp;
   st.m2l_rotation = c->m2l_rot ? 1 : 0;
+  st.m2l_plane_wave = c->opts.m2l_kernel == 4 ? 2 : c->m2l_pw_on ? 1 : 0;
   const bool ilu_kind = pc == BEMFMM_PRECOND_ILU0 || pc == BEMFMM_PRECOND_MILU0 || pc == BEMFMM_PRECOND_ILUT;
   st.precond_nnz = (ilu_kind || (pc == BEMFMM_PRECOND_SCHWARZ && c->opts.schwarz_variant == 1)) ? c->ilu_nnz : 0;
   st.precond_levels = ilu_kind ? c->ilu_levels : 0;
@@ -1957,6 +1987,7 @@ void bemfmm_default_opts(bemfmm_opts* o) {
   o->ilut_drop_tol = 1e-3;    // main_time_domain.cpp:42
   o->ilut_max_entries = 50;   // main_time_domain.cpp:43
   o->ilut_pivot_min = 1e-12;  // main_time_domain.cpp:44
+  o->plane_wave_digits = 10;
 }
 
 bemfmm_ctx* bemfmm_ctx_create(int device, const bemfmm_opts* opts) {

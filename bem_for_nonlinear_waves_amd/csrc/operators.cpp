@@ -25,6 +25,7 @@
 #include <complex>
 #include <cstdint>
 #include <vector>
+#include <stdexcept>
 
 #include "plan.hpp"
 
@@ -324,6 +325,208 @@ void y_rotation(int n, double theta, int kind, std::vector<double>& X) {
   right_solve(A, B, d, X.data());
 }
 }  // namespace
+
+// ---------------------------------------------------------------------------
+// M2L through plane waves (exponential expansions; Greengard & Rokhlin 1997, section 7; the reference's
+// lib_plane_wave_m2l.hpp:293-490, whose direction rotation is switched off at :306-313). For a box pair the
+// translation t = target centre - source centre is turned by the rotation Q that takes its dominant axis
+// (Up/Down/North/South/East/West) onto +z; in that frame, for z > 0 (our harmonics: Y_n^{-m} = conj(Y_n^m),
+// I_n^m = Y_n^m / r^(n+1)),
+//   I_n^m(x) = i^|m| / sqrt((n-|m|)!(n+|m|)!) int_0^inf lambda^n e^{-lambda z}
+//              (1/2pi) int_0^2pi e^{i lambda (x cos a + y sin a)} e^{i m a} da dlambda,
+//   e^{-lambda z + i lambda (x cos a + y sin a)} = sum_{n,m} (-lambda)^n i^|m| e^{-i m a} R_n^m(x) / sqrt((n-|m|)!(n+|m|)!),
+// and the integrals are replaced by the generalized Gaussian rule in lambda (nodes, weights, M(k) angles of
+// the published tables) and the trapezoidal rule in a. The multipole is converted to exponentials (M2X),
+// translated diagonally (X2X: e^{-lambda t_z} e^{i lambda (t_x cos a + t_y sin a)}) and converted to a local
+// expansion (X2L); the rules are designed for point separations 1 <= z <= 4, so lengths are divided by
+// sigma = t'_z / 2.5 (multipole coefficients of degree n times sigma^-n, local ones sigma^-(n+1)). Here the
+// whole chain of one lattice offset is composed into one packed C x C operator (rotation of the multipole,
+// M2X, X2X, X2L, rotation back), evaluated in double: the GPU applies it as a dense contraction like the
+// other M2L operators.
+namespace {
+struct PwNode {
+  double lambda, weight;
+  int M;
+};
+// Greengard & Rokhlin (1997), Tables 6 and 7: 6-digit (18 nodes, 558 exponentials) and 10-digit rules (30 nodes,
+// 1751 exponentials); the same published values as lib_plane_wave_quadrature.hpp:41-100
+const PwNode kPw6[18] = {
+    {0.05278852766117, 0.13438265914335, 5},  {0.26949859838931, 0.29457752727395, 8},
+    {0.63220353174689, 0.42607819361148, 12}, {1.11307564277608, 0.53189220776549, 16},
+    {1.68939496140213, 0.61787306245538, 20}, {2.34376200469530, 0.68863156078905, 25},
+    {3.06269982907806, 0.74749099381426, 29}, {3.83562941265296, 0.79699192718599, 34},
+    {4.65424734321562, 0.83917454386997, 38}, {5.51209386593581, 0.87570092283745, 43},
+    {6.40421268377278, 0.90792943590067, 47}, {7.32688001906175, 0.93698393742461, 51},
+    {8.27740099258238, 0.96382546688788, 56}, {9.25397180602489, 0.98932985769673, 59},
+    {10.25560272374640, 1.01438284597917, 59}, {11.28208829787774, 1.04003654374165, 51},
+    {12.33406790967692, 1.06815489269567, 4}, {13.41492024017240, 1.10907580975537, 1}};
+const PwNode kPw10[30] = {
+    {0.03239542384523, 0.08289159611006, 7},   {0.16861844033714, 0.18838810673274, 10},
+    {0.40611377169029, 0.28485143005306, 14},  {0.73466473057596, 0.37041553715895, 18},
+    {1.14340561998398, 0.44539043894975, 22},  {1.62232408412252, 0.51100452150290, 26},
+    {2.16276138867422, 0.56865283856139, 30},  {2.75739199003682, 0.61958013174010, 35},
+    {3.40002470112078, 0.66481004321965, 39},  {4.08539104793552, 0.70517204769960, 43},
+    {4.80897515497095, 0.74134967169016, 48},  {5.56688915983444, 0.77392103530415, 53},
+    {6.35578243654166, 0.80338600122756, 57},  {7.17277232990713, 0.83018277269650, 62},
+    {8.01538803542112, 0.85469824839953, 66},  {8.88152313049502, 0.87727539085565, 71},
+    {9.76939480982937, 0.89821948245755, 76},  {10.67750922034750, 0.91780416582368, 80},
+    {11.60463289992789, 0.93627766216629, 85}, {12.54977061299652, 0.95386940504388, 89},
+    {13.51215012257297, 0.97079739700556, 94}, {14.49121482655196, 0.98727684670885, 97},
+    {15.48662587630224, 1.00353112433459, 103}, {16.49827659770404, 1.01980697905712, 107},
+    {17.52632405530625, 1.03639774457222, 110}, {18.57124579700721, 1.05368191266322, 112},
+    {19.63393428118300, 1.07219343903929, 108}, {20.71585163675095, 1.09278318162014, 84},
+    {21.81939113866225, 1.11737373706779, 4},  {22.95080495008893, 1.15786184931141, 1}};
+
+// dense packed C x C matrices [in][out] of the rotation factors of m2l_rotation_operator(P, a): D (multipole
+// into the frame where a is +z) and H^T (local back)
+void rotation_dense(int P, int ax, int ay, int az, std::vector<double>& Dm, std::vector<double>& HTm) {
+  const int C = (P + 1) * (P + 1);
+  std::vector<double> f(m2l_rotation_size(P));
+  m2l_rotation_operator(P, ax, ay, az, f.data());
+  // Z (z-rotation per order, [[c, -s], [s, c]] on (Re, Im)) then Y (cos/sin blocks per degree)
+  auto zrot = [&](std::vector<double>& v, bool transposed) {
+    for (int n = 1; n <= P; ++n)
+      for (int m = 1; m <= n; ++m) {
+        const double c = f[2 * (m - 1)], sn = f[2 * (m - 1) + 1];
+        const double re = v[pk_re(n, m)], im = v[pk_im(n, m)];
+        if (!transposed) {
+          v[pk_re(n, m)] = c * re - sn * im;
+          v[pk_im(n, m)] = sn * re + c * im;
+        } else {
+          v[pk_re(n, m)] = c * re + sn * im;
+          v[pk_im(n, m)] = -sn * re + c * im;
+        }
+      }
+  };
+  auto cs = [](int n, int kind, int a) { return kind == 0 ? pk_re(n, a) : pk_im(n, a + 1); };
+  // offsets of the Y_D and Y_H^T blocks in f
+  std::vector<int> yd(P + 1), yh(P + 1);
+  int o = 2 * P;
+  for (int n = 0; n <= P; ++n) {
+    yd[n] = o;
+    o += (n + 1) * (n + 1) + n * n;
+  }
+  for (int m = 0; m <= P; ++m) o += (P + 1 - m) * (P + 1 - m);
+  for (int n = 0; n <= P; ++n) {
+    yh[n] = o;
+    o += (n + 1) * (n + 1) + n * n;
+  }
+  auto yblocks = [&](std::vector<double>& v, const std::vector<int>& base) {
+    for (int n = 0; n <= P; ++n) {
+      int b = base[n];
+      for (int kind = 0; kind < 2; ++kind) {
+        const int w = kind == 0 ? n + 1 : n;
+        std::vector<double> out(w, 0.);
+        for (int a = 0; a < w; ++a)
+          for (int i = 0; i < w; ++i) out[a] += f[b + a * w + i] * v[cs(n, kind, i)];
+        for (int a = 0; a < w; ++a) v[cs(n, kind, a)] = out[a];
+        b += w * w;
+      }
+    }
+  };
+  Dm.assign((size_t)C * C, 0.);
+  HTm.assign((size_t)C * C, 0.);
+  for (int c = 0; c < C; ++c) {
+    std::vector<double> v(C, 0.);
+    v[c] = 1.;
+    zrot(v, false);
+    yblocks(v, yd);
+    for (int r = 0; r < C; ++r) Dm[(size_t)c * C + r] = v[r];
+    std::fill(v.begin(), v.end(), 0.);
+    v[c] = 1.;
+    yblocks(v, yh);
+    zrot(v, true);
+    for (int r = 0; r < C; ++r) HTm[(size_t)c * C + r] = v[r];
+  }
+}
+}  // namespace
+
+void m2l_plane_wave_operator(int P, int digits, int dx, int dy, int dz, double* out) {
+  if (digits != 6 && digits != 10) throw std::runtime_error("plane-wave M2L: digits must be 6 or 10");
+  const PwNode* rule = digits == 6 ? kPw6 : kPw10;
+  const int nq = digits == 6 ? 18 : 30;
+  const int C = (P + 1) * (P + 1);
+  // t = target - source = -o; the axis of its largest component, signed, is turned onto +z
+  const double t[3] = {-(double)dx, -(double)dy, -(double)dz};
+  int ax = 2;
+  if (std::fabs(t[0]) > std::fabs(t[2]) && std::fabs(t[0]) >= std::fabs(t[1])) ax = 0;
+  else if (std::fabs(t[1]) > std::fabs(t[2])) ax = 1;
+  int a[3] = {0, 0, 0};
+  a[ax] = t[ax] > 0 ? 1 : -1;
+  // m2l_rotation_operator(P, a) rotates the offset a onto +z: Q = R_y(-theta) R_z(-phi)
+  const double theta = std::acos((double)a[2]), phi = std::atan2((double)a[1], (double)a[0]);
+  const double cp = std::cos(phi), sp = std::sin(phi), ct = std::cos(theta), st = std::sin(theta);
+  const double u0 = cp * t[0] + sp * t[1], u1 = -sp * t[0] + cp * t[1], u2 = t[2];  // R_z(-phi) t
+  const double tr[3] = {ct * u0 - st * u2, u1, st * u0 + ct * u2};                   // R_y(-theta)
+  const double sigma = tr[2] / 2.5;
+  const double ts[3] = {tr[0] / sigma, tr[1] / sigma, tr[2] / sigma};
+  // F_q(d) = sum over the angles of node q of e^{i d a} e^{i lambda (ts_x cos a + ts_y sin a)}, d = m - k
+  const int D = 2 * P;
+  std::vector<std::complex<double>> F((size_t)nq * (2 * D + 1));
+  for (int q = 0; q < nq; ++q) {
+    std::complex<double>* Fq = F.data() + (size_t)q * (2 * D + 1);
+    for (int j = 0; j < rule[q].M; ++j) {
+      const double al = 2. * M_PI * j / rule[q].M;
+      const std::complex<double> base =
+          std::exp(std::complex<double>(0., rule[q].lambda * (ts[0] * std::cos(al) + ts[1] * std::sin(al))));
+      const std::complex<double> rot(std::cos(al), std::sin(al));
+      std::complex<double> up = base, dn = base;  // base e^{i d al} for d = 0, 1, .. and 0, -1, ..
+      Fq[D] += base;
+      for (int d = 1; d <= D; ++d) {
+        up *= rot;
+        dn *= std::conj(rot);
+        Fq[D + d] += up;
+        Fq[D - d] += dn;
+      }
+    }
+    const double f = (rule[q].weight / rule[q].M) * std::exp(-rule[q].lambda * ts[2]);
+    for (int d = 0; d <= 2 * D; ++d) Fq[d] *= f;
+  }
+  auto ipw = [](int e) {  // i^e
+    const int r = ((e % 4) + 4) % 4;
+    return r == 0 ? std::complex<double>(1, 0) : r == 1 ? std::complex<double>(0, 1) : r == 2 ? std::complex<double>(-1, 0) : std::complex<double>(0, -1);
+  };
+  auto nrm = [](int n, int m) { return 1. / std::sqrt(std::tgamma(n - std::abs(m) + 1.) * std::tgamma(n + std::abs(m) + 1.)); };
+  // lambda_q^n (-lambda_q)^j sigma^-(n+j+1) per node and degree pair
+  std::vector<double> lp((size_t)nq * (P + 1) * (P + 1));
+  for (int q = 0; q < nq; ++q)
+    for (int n = 0; n <= P; ++n)
+      for (int j = 0; j <= P; ++j)
+        lp[((size_t)q * (P + 1) + n) * (P + 1) + j] =
+            std::pow(rule[q].lambda, n) * std::pow(-rule[q].lambda, j) * std::pow(sigma, -(n + j + 1));
+  // T[(j,k),(n,m)] in the rotated, scaled frame
+  Packer pk(P);
+  auto T = [&](int j, int k, int n, int m) -> qc {
+    std::complex<double> acc = 0.;
+    for (int q = 0; q < nq; ++q)
+      acc += lp[((size_t)q * (P + 1) + n) * (P + 1) + j] * F[(size_t)q * (2 * D + 1) + (m - k) + D];
+    const std::complex<double> v = acc * ipw(std::abs(m) + std::abs(k)) * (nrm(n, m) * nrm(j, k));
+    return qc{(q)v.real(), (q)v.imag()};
+  };
+  for (int j = 0; j <= P; ++j)
+    for (int k = 0; k <= j; ++k)
+      for (int n = 0; n <= P; ++n)
+        for (int m = 0; m <= n; ++m) pk.add(j, k, n, m, T(j, k, n, m), m > 0 ? T(j, k, n, -m) : qc{0, 0});
+  std::vector<double> X((size_t)C * C);
+  pk.store(X.data());
+  // out = H^T X D (all [in][out]): out[c][r] = sum_a sum_b D[c][a] X[a][b] H^T[b][r]
+  std::vector<double> Dm, HTm, XD((size_t)C * C, 0.);
+  rotation_dense(P, a[0], a[1], a[2], Dm, HTm);
+  for (int c = 0; c < C; ++c)
+    for (int aa = 0; aa < C; ++aa) {
+      const double d = Dm[(size_t)c * C + aa];
+      if (d == 0.) continue;
+      for (int b = 0; b < C; ++b) XD[(size_t)c * C + b] += d * X[(size_t)aa * C + b];
+    }
+  // H^T maps within a degree: the packed entries of degree n are n^2 .. (n+1)^2 - 1
+  for (int c = 0; c < C; ++c)
+    for (int r = 0; r < C; ++r) {
+      const int n = (int)std::sqrt((double)r + 0.5);
+      double acc = 0.;
+      for (int b = n * n; b < (n + 1) * (n + 1); ++b) acc += XD[(size_t)c * C + b] * HTm[(size_t)b * C + r];
+      out[(size_t)c * C + r] = acc;
+    }
+}
 
 int m2l_rotation_size(int P) {
   int s = 2 * P;

@@ -841,7 +841,8 @@ std::shared_ptr<const std::vector<double>> cached_operator(int P, int kind, int 
   if (kind == 0) m2m_operator(P, a, v->data());
   else if (kind == 1) l2l_operator(P, a, v->data());
   else if (kind == 2) m2l_operator(P, a, b, c, v->data());
-  else m2l_rotation_operator(P, a, b, c, v->data());
+  else if (kind == 3) m2l_rotation_operator(P, a, b, c, v->data());
+  else m2l_plane_wave_operator(P, kind == 4 ? 10 : 6, a, b, c, v->data());
   std::lock_guard<std::mutex> lk(mu);
   return cache.emplace(key, std::move(v)).first->second;  // a concurrent insert of the same key wins; same values
 }

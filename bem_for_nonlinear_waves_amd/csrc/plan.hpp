@@ -112,13 +112,18 @@ void m2l_operator(int P, int dx, int dy, int dz, double* out);
 // [out][in], the degree blocks D_n ((2n+1)^2 each, n = 0..P), the axial blocks ((P+1-m)^2 each,
 // m = 0..P, indexed [j-m][n-m]) and the degree blocks of H^T; m2l_rotation_size(P) doubles.
 int m2l_rotation_size(int P);
+// M2L composed through plane waves (rotation of the dominant axis of the translation onto +z, multipole to
+// exponentials, diagonal translation, exponentials to local, rotation back) with the 6- or 10-digit rule,
+// in the layout of m2l_operator
+void m2l_plane_wave_operator(int P, int digits, int dx, int dy, int dz, double* out);
 void m2l_rotation_operator(int P, int dx, int dy, int dz, double* out);
 void m2m_operator(int P, int octant, double* out);  // child -> parent; octant bit a set = child on + side of axis a
 void l2l_operator(int P, int octant, double* out);  // parent -> child
 
-// Process-wide cache of translation operators (computed once in float128, immutable afterwards):
+// Process-wide cache of translation operators (computed once, immutable afterwards):
 // kind 0 = M2M of octant a, 1 = L2L of octant a, 2 = M2L of lattice offset (a, b, c), 3 = the rotation
-// factors of that M2L (m2l_rotation_operator). Thread-safe.
+// factors of that M2L (m2l_rotation_operator), 4 / 5 = that M2L through plane waves with the 10- / 6-digit
+// rule (m2l_plane_wave_operator). Thread-safe.
 std::shared_ptr<const std::vector<double>> cached_operator(int P, int kind, int a, int b, int c);
 // computes the missing cache entries of the given (kind, a, b, c) keys in parallel (OpenMP)
 void warm_operator_cache(int P, const std::vector<std::array<int, 4>>& keys);

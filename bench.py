@@ -224,7 +224,8 @@ def setup_bvp(m, d, phi, phin, args, precision):
     prec = _lib.PRECISION_FP32_REFINE if precision == "fp32_refine" else _lib.PRECISION_FP64
     bvp = BEM_BVP(m, d, p=args.p, leaf_max=args.leaf, max_level=args.max_level, preconditioner=pc, precision=prec,
                   refine_eta=args.refine_eta, m2l_tile_pairs=args.m2l_tile_pairs,
-                  m2l_kernel={"auto": 0, "dense": 1, "rotation": 2}[args.m2l_kernel],
+                  m2l_kernel={"auto": 0, "dense": 1, "rotation": 2, "plane-wave": 3, "plane-wave-fp64": 4}[args.m2l_kernel],
+                  plane_wave_digits=args.plane_wave_digits,
                   schwarz_max_core=args.schwarz_core, schwarz_max_block=args.schwarz_block,
                   schwarz_overlap=args.schwarz_overlap, use_graphs=args.graphs, refine_recycle=args.refine_recycle)
     bvp.set_bc(phi, phin)
@@ -493,8 +494,11 @@ def main():
     ap.add_argument("--schwarz-core", type=int, default=8, help="Schwarz core size (reference default 64)")
     ap.add_argument("--schwarz-block", type=int, default=128, help="Schwarz block size (reference default 128)")
     ap.add_argument("--schwarz-overlap", type=int, default=6, help="Schwarz overlap rings (reference default 1)")
-    ap.add_argument("--m2l-kernel", default="auto", choices=["auto", "dense", "rotation"],
-                    help="M2L of the FP32 operator: rotation where compiled (p = 8, 10), or dense matrix-core tiles")
+    ap.add_argument("--m2l-kernel", default="auto", choices=["auto", "dense", "rotation", "plane-wave", "plane-wave-fp64"],
+                    help="M2L of the FP32 operator: rotation where compiled (p = 8, 10), dense matrix-core tiles, or "
+                         "the plane-wave composed operators on the matrix-core tiles (plane-wave-fp64: for the FP64 "
+                         "operator too)")
+    ap.add_argument("--plane-wave-digits", type=int, default=10, choices=[6, 10])
     ap.add_argument("--no-fp64-compare", action="store_true",
                     help="skip the plain FP64 solves timed beside the FP32+refinement ones")
     ap.add_argument("--no-cpu-baseline", action="store_true")

@@ -85,7 +85,11 @@ typedef struct bemfmm_opts {
                             (2.549 against 2.569 ms per FP32 matvec) */
   int m2l_kernel;        /* M2L of the FP32 operator (FP32_REFINE): 0 (default) = rotation where compiled for p
                             (8, 10), else matrix cores; 1 = dense matrix-core tiles; 2 = rotation (error if
-                            not compiled for p). The FP64 operator always uses the dense matrix-core tiles. */
+                            not compiled for p); 3 = plane waves (lib_plane_wave_m2l.hpp:293-490, rotations to all 6
+                            directions): per offset the chain rotate / multipole-to-exponential / diagonal shift /
+                            exponential-to-local / rotate back composed into one operator, applied by the dense
+                            matrix-core tiles. The FP64 operator uses the dense SimpleM2L tiles, except with 4 =
+                            plane waves for both operators. */
   int overlap_near;      /* 1 (default): the near-field columns of each matvec run on a second stream beside the
                             FMM passes (they need only the input vector); 0: after L2L, one GEMV */
   int refine_recycle;    /* FP32_REFINE: 1 (default): each inner FP32 solve after the first minimises over the
@@ -107,6 +111,9 @@ typedef struct bemfmm_opts {
   double ilut_drop_tol;  /* ILUT: entries below drop_tol * |row of A|_1 are dropped, default 1e-3 (:42) */
   int ilut_max_entries;  /* ILUT: largest entries kept per L row and per U row, default 50 (:43) */
   double ilut_pivot_min; /* ILUT: smaller diagonals are set to +-ilut_pivot_min, default 1e-12 (:44) */
+  int plane_wave_digits; /* m2l_kernel 3 / 4: quadrature of the exponential expansions, 10 (default; 1751
+                            exponentials, operators within ~1e-11 of SimpleM2L) or 6 (558, ~1e-7); the published
+                            rules of Greengard & Rokhlin 1997 (lib_plane_wave_quadrature.hpp) */
 } bemfmm_opts;
 
 typedef struct bemfmm_result {
@@ -136,6 +143,7 @@ typedef struct bemfmm_stats {
   int m2l_rotation;             /* 1 if the FP32 operator's M2L runs by rotation */
   int64_t precond_nnz;          /* ILU(0)/MILU(0)/ILUT (the factors) and the reference's Schwarz (the matrix): entries */
   int precond_levels;           /* ILU(0)/MILU(0)/ILUT: dependency levels of the forward plus the backward sweep */
+  int m2l_plane_wave;           /* 1: the FP32 operator's M2L through plane waves; 2: both operators' */
 } bemfmm_stats;
 
 /* Availability (metal_m2l.h: metal_m2l_is_available / metal_m2l_device_name). */

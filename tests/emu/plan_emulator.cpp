@@ -240,7 +240,8 @@ void emu_operator(int p, int kind, int a, int b, int c, double* out) {
   if (kind == 0) bemfmm::m2m_operator(p, a, out);
   else if (kind == 1) bemfmm::l2l_operator(p, a, out);
   else if (kind == 2) bemfmm::m2l_operator(p, a, b, c, out);
-  else bemfmm::m2l_rotation_operator(p, a, b, c, out);
+  else if (kind == 3) bemfmm::m2l_rotation_operator(p, a, b, c, out);
+  else bemfmm::m2l_plane_wave_operator(p, kind == 4 ? 10 : 6, a, b, c, out);
 }
 int emu_rotation_size(int p) { return bemfmm::m2l_rotation_size(p); }
 }
