@@ -30,7 +30,7 @@ class Opts(ctypes.Structure):
                 ("ilu_neighborhood", ctypes.c_int), ("ilu_kring", ctypes.c_int), ("schwarz_variant", ctypes.c_int),
                 ("schwarz_core_k", ctypes.c_int), ("milu_omega", ctypes.c_double), ("ilut_drop_tol", ctypes.c_double),
                 ("ilut_max_entries", ctypes.c_int), ("ilut_pivot_min", ctypes.c_double),
-                ("plane_wave_digits", ctypes.c_int)]
+                ("plane_wave_digits", ctypes.c_int), ("tree_on_gpu", ctypes.c_int)]
 
 
 class Result(ctypes.Structure):
@@ -50,7 +50,8 @@ class Stats(ctypes.Structure):
                 ("precond_fallback_blocks", ctypes.c_int), ("precond_avg_block", ctypes.c_double),
                 ("n_ops_uploaded", ctypes.c_int), ("m2l_tile_pairs", ctypes.c_int),
                 ("m2l_rotation", ctypes.c_int), ("precond_nnz", ctypes.c_int64), ("precond_levels", ctypes.c_int),
-                ("m2l_plane_wave", ctypes.c_int)]
+                ("m2l_plane_wave", ctypes.c_int), ("tree_on_gpu", ctypes.c_int), ("t_tune", ctypes.c_double),
+                ("t_precond", ctypes.c_double), ("t_fp32", ctypes.c_double)]
 
 
 # (name, restype, argtypes) of every entry point declared in include/bemfmm.h
@@ -93,6 +94,7 @@ SIGNATURES = [
     ("bemfmm_dof_order", ctypes.c_int, [ctypes.c_void_p, _ip]),
     ("bemfmm_schwarz_blocks", ctypes.c_int,
      [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64), _ip, _ip]),
+    ("bemfmm_tree_digest", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint64)]),
 ]
 
 _lib = None

@@ -4,7 +4,7 @@ Host code (plan.cpp, operators.cpp, schwarz.cpp, precond.cpp: the plan, its floa
 operators, the Schwarz blocks and the ILU(0) matrix; bemfmm.cpp: contexts, launch sequences, GMRES and
 the C ABI over the HIP runtime API; server.cpp: the request loop over the C ABI)
 is compiled with g++ (OpenMP via libgomp, __float128 via libquadmath, threads);
-the device kernels and their launchers (kernels.hip) with hipcc for gfx950; the
+the device kernels and their launchers (kernels.hip, tree.hip) with hipcc for gfx950; the
 objects are linked with hipcc.
 Usage: python -m bem_for_nonlinear_waves_amd.build
 """
@@ -53,6 +53,10 @@ def build(verbose: bool = True) -> str:
         objs.append(obj)
     obj = os.path.join(OUT_DIR, "kernels.o")
     _run([hipcc, *HIP_FLAGS, "-c", os.path.join(CSRC, "kernels.hip"), "-o", obj])
+    objs.append(obj)
+    # the tree kernels round their box arithmetic as the host plan does: no contraction into FMAs
+    obj = os.path.join(OUT_DIR, "tree.o")
+    _run([hipcc, *HIP_FLAGS, "-ffp-contract=off", "-c", os.path.join(CSRC, "tree.hip"), "-o", obj])
     objs.append(obj)
     tmp = LIB + ".tmp"
     _run([hipcc, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs, f"-L{_gcc_libdir()}",

@@ -54,7 +54,8 @@ class BEM_BVP:
                  use_graphs: int = 0, m2l_kernel: int = 0, overlap_near: int = 1, refine_recycle: int = 1,
                  ilu_neighborhood: int = 0, ilu_kring: int = 1, schwarz_variant: int = 0, schwarz_core_k: int = 1,
                  milu_omega: float = 1.0, ilut_drop_tol: float = 1e-3, ilut_max_entries: int = 50,
-                 ilut_pivot_min: float = 1e-12, plane_wave_digits: int = 10, device: int = 0):
+                 ilut_pivot_min: float = 1e-12, plane_wave_digits: int = 10, tree_on_gpu: int = 1,
+                 device: int = 0):
         self.lib = _lib.load()
         o = _lib.Opts()
         self.lib.bemfmm_default_opts(ctypes.byref(o))
@@ -68,6 +69,7 @@ class BEM_BVP:
         o.milu_omega, o.ilut_drop_tol, o.ilut_max_entries, o.ilut_pivot_min = (milu_omega, ilut_drop_tol,
                                                                               ilut_max_entries, ilut_pivot_min)
         o.plane_wave_digits = plane_wave_digits
+        o.tree_on_gpu = tree_on_gpu
         self.ctx = self.lib.bemfmm_ctx_create(device, ctypes.byref(o))
         if not self.ctx:
             raise BEMError(_lib.E_DEVICE, "bemfmm_ctx_create failed (no usable GPU?)")
@@ -194,6 +196,12 @@ class BEM_BVP:
         self.lib.bemfmm_schwarz_blocks(self.ctx, ptr.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
                                        _ptr(idx, _lib._ip), _ptr(ncore, _lib._ip))
         return ptr, idx[:int(ptr[-1])], ncore[:nb]
+
+    def tree_digest(self) -> int:
+        """64-bit digest of the prepared tree, face order, target boxes and interaction lists."""
+        out = ctypes.c_uint64(0)
+        self._check(self.lib.bemfmm_tree_digest(self.ctx, ctypes.byref(out)))
+        return int(out.value)
 
     def refinement_history(self) -> np.ndarray:
         n = self.lib.bemfmm_refinement_history(self.ctx, None, 0)

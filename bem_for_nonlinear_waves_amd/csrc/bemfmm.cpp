@@ -30,6 +30,7 @@
 
 namespace {
 
+using bemfmm::BoxGeom;
 using bemfmm::Plan;
 namespace dev = bemfmm::dev;
 using dev::kDotBlocks;
@@ -136,6 +137,33 @@ struct DBuf {
     HIPCHECK(dev_malloc((void**)&p, want * sizeof(T)));
     n = count;
     cap = want;
+  }
+  // Grows to at least `count` elements keeping the first `keep` (device-to-device copy on s, completed before
+  // the old allocation is freed); a larger allocation takes half as much again for the next growth.
+  void reserve_keep(size_t count, size_t keep, hipStream_t s) {
+    if (p && count <= cap) {
+      n = std::max(n, count);
+      return;
+    }
+    const size_t want = std::max<size_t>(count + count / 2, 1);
+    T* q = nullptr;
+    HIPCHECK(dev_malloc((void**)&q, want * sizeof(T)));
+    if (p && keep) {
+      HIPCHECK(hipMemcpyAsync(q, p, std::min(keep, n) * sizeof(T), hipMemcpyDeviceToDevice, s));
+      HIPCHECK(hipStreamSynchronize(s));
+    }
+    release();
+    p = q;
+    n = count;
+    cap = want;
+  }
+  // Setup-time download of the first `count` elements; completes before returning.
+  void download(std::vector<T>& v, size_t count, hipStream_t s) const {
+    v.resize(count);
+    if (count) {
+      HIPCHECK(hipMemcpyAsync(v.data(), p, count * sizeof(T), hipMemcpyDeviceToHost, s));
+      HIPCHECK(hipStreamSynchronize(s));
+    }
   }
   // Setup-time upload; completes before returning (the source may be a temporary).
   void upload(const std::vector<T>& v, hipStream_t s) {
@@ -323,6 +351,16 @@ struct bemfmm_ctx {
     DBuf<int32_t> failed;
     DTiles alt_tiles;  // the autotuner's alternative M2L tiling
   } sb;
+  // the plan's tree stage on the device (plan_tree_lists_device)
+  struct TreeBufs {
+    DBuf<double> pole, maxoff, X, off;
+    DBuf<BoxGeom> geom;
+    DBuf<int32_t> fbox, flags, ci, vals, perm, parent, level, count, nchild, first_child, child_tab, target;
+    DBuf<int32_t> nearf, m2lc, nextc, near_at, m2l_at, next_at, near_idx, m2l_idx, err;
+    DBuf<uint64_t> fkey, keys0, keys1, pk, pairs, next, near_all, m2l_all;
+    DBuf<int64_t> range, near_ptr, m2l_ptr, tot;
+    DBuf<uint32_t> ctr;
+  } tb;
   DBuf<double> V;            // Krylov basis
   int64_t v_cap = 0;
   int basis_n = -1;          // DOF count the basis is laid out for
@@ -855,6 +893,215 @@ void tune_m2l_tiles(bemfmm_ctx* c, const Plan& pl) {
   HIPCHECK(hipGetLastError());
 }
 
+int bits_for(uint64_t v) {
+  int b = 1;
+  while (b < 64 && (v >> b)) ++b;
+  return b;
+}
+
+// The plan's tree stage on the device (tree.hip): level by level, the faces of every box that splits are keyed
+// by (box, cell, face) and radix-sorted, each distinct (box, cell) becomes a child box (in depth-first order),
+// and each face moves into its child if the child's inside test holds; then the faces sorted by path key give
+// the depth-first face order and the box ranges, the internal offsets go up the levels, every node descends
+// to its deepest box, and the near / M2L lists are built level-synchronously from the root's children pairs.
+// The result equals plan_tree_lists (same boxes, face order, lists and target boxes): the box arithmetic is
+// shared (treemath.hpp) and compiled without FMA contraction on both sides.
+void plan_tree_lists_device(bemfmm_ctx* c, Plan& pl) {
+  const bemfmm::Options& opt = pl.opt;
+  if (opt.max_level > 11) {  // 5-bit digits per level in 64-bit path keys
+    bemfmm::plan_tree_lists(pl);
+    return;
+  }
+  hipStream_t s = c->stream;
+  auto& T = c->tb;
+  const int F = pl.n_faces, N = pl.n_nodes;
+  const double t0 = now();
+  auto sort_keys = [&](const uint64_t* in, uint64_t* out, int64_t n, int end_bit) {
+    size_t bytes = 0;
+    HIPCHECK(dev::sort_keys_u64(s, nullptr, bytes, in, out, n, 0, end_bit));
+    c->sb.sort_tmp.alloc(bytes);
+    HIPCHECK(dev::sort_keys_u64(s, c->sb.sort_tmp.p, bytes, in, out, n, 0, end_bit));
+  };
+  auto scan = [&](bool inclusive, const int32_t* in, int32_t* out, int64_t n) {
+    size_t bytes = 0;
+    if (inclusive) HIPCHECK(dev::scan_inclusive_i32(s, nullptr, bytes, in, out, n));
+    else HIPCHECK(dev::scan_exclusive_i32(s, nullptr, bytes, in, out, n));
+    c->sb.sort_tmp.alloc(bytes);
+    if (inclusive) HIPCHECK(dev::scan_inclusive_i32(s, c->sb.sort_tmp.p, bytes, in, out, n));
+    else HIPCHECK(dev::scan_exclusive_i32(s, c->sb.sort_tmp.p, bytes, in, out, n));
+  };
+  auto read_err = [&]() {
+    int32_t e = 0;
+    HIPCHECK(hipMemcpyAsync(&e, T.err.p, sizeof(e), hipMemcpyDeviceToHost, s));
+    HIPCHECK(hipStreamSynchronize(s));
+    if (e & 1) throw std::runtime_error("octree box with more than 27 cells");
+    if (e & 2) throw std::runtime_error("empty octree box (face on a box boundary)");
+  };
+  // box arrays, grown keeping the first `keep` boxes
+  auto boxes = [&](size_t need, size_t keep) {
+    T.geom.reserve_keep(need, keep, s);
+    T.parent.reserve_keep(need, keep, s);
+    T.level.reserve_keep(need, keep, s);
+    T.pk.reserve_keep(need, keep, s);
+    T.count.reserve_keep(need, keep, s);
+    T.nchild.reserve_keep(need, keep, s);
+    T.first_child.reserve_keep(need, keep, s);
+    T.child_tab.reserve_keep(need * dev::kTreeCells, keep * dev::kTreeCells, s);
+  };
+  T.pole.upload(pl.pole, s);
+  T.maxoff.upload(pl.maxoff, s);
+  T.X.upload(pl.Xs, s);
+  T.fbox.alloc(F);
+  T.fkey.alloc(F);
+  T.keys0.alloc(F);
+  T.keys1.alloc(F);
+  T.flags.alloc(F);
+  T.ci.alloc(F);
+  T.err.alloc(1);
+  T.ctr.alloc(1);
+  T.tot.alloc(3);
+  dev::fill_i32(s, F, T.fbox.p, 0);
+  dev::fill_u64(s, F, T.fkey.p, 0);
+  dev::fill_i32(s, 1, T.err.p, 0);
+  boxes(std::max<size_t>(64, (size_t)F / 16), 0);
+  {
+    const int32_t root_i[5] = {-1, 0, F, 0, -1};  // parent, level, count, nchild, first_child
+    const uint64_t zero = 0;
+    std::vector<int32_t> tab(dev::kTreeCells, -1);
+    HIPCHECK(hipMemcpyAsync(T.geom.p, &pl.root, sizeof(BoxGeom), hipMemcpyHostToDevice, s));
+    HIPCHECK(hipMemcpyAsync(T.parent.p, &root_i[0], 4, hipMemcpyHostToDevice, s));
+    HIPCHECK(hipMemcpyAsync(T.level.p, &root_i[1], 4, hipMemcpyHostToDevice, s));
+    HIPCHECK(hipMemcpyAsync(T.count.p, &root_i[2], 4, hipMemcpyHostToDevice, s));
+    HIPCHECK(hipMemcpyAsync(T.nchild.p, &root_i[3], 4, hipMemcpyHostToDevice, s));
+    HIPCHECK(hipMemcpyAsync(T.first_child.p, &root_i[4], 4, hipMemcpyHostToDevice, s));
+    HIPCHECK(hipMemcpyAsync(T.pk.p, &zero, 8, hipMemcpyHostToDevice, s));
+    HIPCHECK(hipMemcpyAsync(T.child_tab.p, tab.data(), tab.size() * 4, hipMemcpyHostToDevice, s));
+    HIPCHECK(hipStreamSynchronize(s));  // the sources are on this stack frame
+  }
+  std::vector<int32_t> level_ptr{0, 1};
+  int digits = 0;
+  for (int l = 0;; ++l) {
+    const int lb = level_ptr[l], le = level_ptr[l + 1];
+    dev::fill_i32(s, 1, (int32_t*)T.ctr.p, 0);
+    dev::tree_split(s, F, lb, opt.leaf_max, l < opt.max_level ? 1 : 0, T.pole.p, T.fbox.p, T.fkey.p, T.geom.p,
+                    T.count.p, T.keys0.p, T.ctr.p, T.err.p);
+    HIPCHECK(hipGetLastError());
+    ++digits;
+    uint32_t ne = 0;
+    HIPCHECK(hipMemcpyAsync(&ne, T.ctr.p, sizeof(ne), hipMemcpyDeviceToHost, s));
+    read_err();
+    if (ne == 0) break;
+    sort_keys(T.keys0.p, T.keys1.p, ne, 32 + bits_for((uint64_t)(le - lb) * 32));
+    dev::tree_heads(s, (int)ne, T.keys1.p, T.flags.p);
+    scan(true, T.flags.p, T.ci.p, ne);
+    int32_t nc = 0;
+    HIPCHECK(hipMemcpyAsync(&nc, T.ci.p + (ne - 1), sizeof(nc), hipMemcpyDeviceToHost, s));
+    HIPCHECK(hipStreamSynchronize(s));
+    boxes((size_t)le + nc, (size_t)le);
+    dev::tree_children(s, (int)ne, T.keys1.p, T.ci.p, lb, le, T.geom.p, T.parent.p, T.level.p, T.pk.p, T.count.p,
+                       T.nchild.p, T.first_child.p, T.child_tab.p);
+    dev::tree_assign(s, (int)ne, T.keys1.p, T.ci.p, le, T.pole.p, T.geom.p, T.fbox.p, T.fkey.p, T.count.p);
+    dev::tree_nonempty(s, nc, le, T.count.p, T.err.p);
+    HIPCHECK(hipGetLastError());
+    read_err();
+    level_ptr.push_back(le + nc);
+  }
+  const int nl = (int)level_ptr.size() - 1, nb = level_ptr.back();
+  // depth-first face order and box ranges
+  T.vals.alloc(F);
+  T.perm.alloc(F);
+  T.range.alloc(2 * (size_t)nb);
+  T.off.alloc(nb);
+  T.target.alloc(N);
+  dev::iota_i32(s, F, T.vals.p);
+  {
+    size_t bytes = 0;
+    HIPCHECK(dev::sort_pairs_u64_i32(s, nullptr, bytes, T.fkey.p, T.keys1.p, T.vals.p, T.perm.p, F, 0, 5 * digits));
+    c->sb.sort_tmp.alloc(bytes);
+    HIPCHECK(dev::sort_pairs_u64_i32(s, c->sb.sort_tmp.p, bytes, T.fkey.p, T.keys1.p, T.vals.p, T.perm.p, F, 0,
+                                     5 * digits));
+  }
+  dev::tree_ranges(s, nb, digits, T.level.p, T.pk.p, T.keys1.p, F, T.range.p);
+  dev::tree_leaf_off(s, nb, T.nchild.p, T.range.p, T.perm.p, T.maxoff.p, T.off.p);
+  for (int l = nl - 2; l >= 0; --l)
+    dev::tree_parent_off(s, level_ptr[l], level_ptr[l + 1], T.nchild.p, T.first_child.p, T.off.p);
+  dev::tree_targets(s, N, T.X.p, T.geom.p, T.nchild.p, T.child_tab.p, T.target.p);
+  HIPCHECK(hipGetLastError());
+  std::vector<BoxGeom> geom;
+  std::vector<int32_t> nchild, first_child;
+  T.geom.download(geom, nb, s);
+  T.nchild.download(nchild, nb, s);
+  T.first_child.download(first_child, nb, s);
+  T.parent.download(pl.box_parent, nb, s);
+  T.perm.download(pl.face_perm, F, s);
+  T.range.download(pl.box_range, 2 * (size_t)nb, s);
+  T.target.download(pl.target_box, N, s);
+  pl.n_levels = nl;
+  pl.n_boxes = nb;
+  pl.level_ptr = level_ptr;
+  pl.box_level.resize(nb);
+  pl.box_leaf.resize(nb);
+  pl.box_center.resize(3 * (size_t)nb);
+  pl.n_leaves = 0;
+  for (int l = 0; l < nl; ++l)
+    for (int b = level_ptr[l]; b < level_ptr[l + 1]; ++b) pl.box_level[b] = l;
+  for (int b = 0; b < nb; ++b) {
+    pl.box_leaf[b] = nchild[b] == 0 ? 1 : 0;
+    pl.n_leaves += pl.box_leaf[b];
+    for (int a = 0; a < 3; ++a) pl.box_center[3 * (size_t)b + a] = geom[b].X[a];
+  }
+  const double t1 = now();
+  pl.t_tree += t1 - t0;
+
+  // near / M2L lists, level-synchronous from the pairs of the root's children
+  {
+    std::vector<uint64_t> p0;
+    for (int a = first_child[0]; a < first_child[0] + nchild[0]; ++a)
+      for (int b = first_child[0]; b < first_child[0] + nchild[0]; ++b)
+        p0.push_back(((uint64_t)(uint32_t)a << 32) | (uint32_t)b);
+    T.pairs.upload(p0, s);
+  }
+  int64_t np = nchild[0] * (int64_t)nchild[0], n_near = 0, n_m2l = 0;
+  while (np > 0) {
+    if (np > INT32_MAX) throw std::runtime_error("too many near candidate pairs in one level");
+    for (auto* b : {&T.nearf, &T.m2lc, &T.nextc, &T.near_at, &T.m2l_at, &T.next_at}) b->alloc(np);
+    dev::list_count(s, (int)np, T.pairs.p, T.geom.p, T.off.p, T.nchild.p, T.first_child.p, opt.theta, T.nearf.p,
+                    T.m2lc.p, T.nextc.p);
+    scan(false, T.nearf.p, T.near_at.p, np);
+    scan(false, T.m2lc.p, T.m2l_at.p, np);
+    scan(false, T.nextc.p, T.next_at.p, np);
+    dev::list_totals(s, (int)np, T.nearf.p, T.m2lc.p, T.nextc.p, T.near_at.p, T.m2l_at.p, T.next_at.p, T.tot.p);
+    int64_t tot[3];
+    HIPCHECK(hipMemcpyAsync(tot, T.tot.p, sizeof(tot), hipMemcpyDeviceToHost, s));
+    HIPCHECK(hipStreamSynchronize(s));
+    if (tot[1] > INT32_MAX || tot[2] > INT32_MAX) throw std::runtime_error("too many box pairs in one level");
+    T.near_all.reserve_keep((size_t)(n_near + tot[0]), (size_t)n_near, s);
+    T.m2l_all.reserve_keep((size_t)(n_m2l + tot[1]), (size_t)n_m2l, s);
+    T.next.alloc(std::max<int64_t>(tot[2], 1));
+    dev::list_write(s, (int)np, T.pairs.p, T.geom.p, T.off.p, T.nchild.p, T.first_child.p, opt.theta, T.nearf.p,
+                    T.near_at.p, T.m2l_at.p, T.next_at.p, T.near_all.p + n_near, T.m2l_all.p + n_m2l, T.next.p);
+    HIPCHECK(hipGetLastError());
+    n_near += tot[0];
+    n_m2l += tot[1];
+    T.pairs.swap(T.next);
+    np = tot[2];
+  }
+  auto to_csr = [&](DBuf<uint64_t>& all, int64_t n, DBuf<int64_t>& ptr, DBuf<int32_t>& idx, std::vector<int64_t>& hp,
+                    std::vector<int32_t>& hi) {
+    T.keys0.alloc(std::max<int64_t>(n, 1));
+    if (n) sort_keys(all.p, T.keys0.p, n, 32 + bits_for((uint64_t)nb));
+    ptr.alloc(nb + 1);
+    idx.alloc(std::max<int64_t>(n, 1));
+    dev::csr_from_keys(s, nb, T.keys0.p, n, ptr.p, idx.p);
+    HIPCHECK(hipGetLastError());
+    ptr.download(hp, nb + 1, s);
+    idx.download(hi, n, s);
+  };
+  to_csr(T.near_all, n_near, T.near_ptr, T.near_idx, pl.near_ptr, pl.near_idx);
+  to_csr(T.m2l_all, n_m2l, T.m2l_ptr, T.m2l_idx, pl.m2l_ptr, pl.m2l_idx);
+  pl.t_lists = now() - t1;
+}
+
 void do_prepare(bemfmm_ctx* c) {
   if (!c->have_geom) throw std::runtime_error("bemfmm_prepare: geometry not set");
   if (!c->have_bc) throw std::runtime_error("bemfmm_prepare: boundary values not set");
@@ -911,8 +1158,11 @@ void do_prepare(bemfmm_ctx* c) {
   dof_order(c);
   c->plan.reset(new Plan());
   Plan& pl = *c->plan;
-  bemfmm::build_plan(pl, o, c->n_nodes, c->X.data(), c->n_faces, c->tri.data(), c->face_dof_i.data(), c->n_dof,
-                     c->kind_i.data(), c->point_i.data());
+  bemfmm::plan_geometry(pl, o, c->n_nodes, c->X.data(), c->n_faces, c->tri.data(), c->face_dof_i.data(), c->n_dof,
+                        c->kind_i.data(), c->point_i.data());
+  if (c->opts.tree_on_gpu) plan_tree_lists_device(c, pl);
+  else bemfmm::plan_tree_lists(pl);
+  bemfmm::plan_finish(pl);
   c->C = pl.C;
   c->n_boxes = pl.n_boxes;
   c->p2m_rows_pad = pl.p2m_rows_pad;
@@ -1097,7 +1347,11 @@ void do_prepare(bemfmm_ctx* c) {
   const double t3 = now();
   // ---- rigid-mode alpha (updateGeometricProperties) ----
   evaluate(c, 2, nullptr, c->alpha.p);
+  HIPCHECK(hipStreamSynchronize(s));
+  const double t3a = now();
   tune_m2l_tiles(c, pl);
+  HIPCHECK(hipStreamSynchronize(s));
+  const double t3b = now();
   const int pc = c->opts.preconditioner;
   c->sw_host = bemfmm::SchwarzBlocks{};
   for (auto* b : {&c->sw_inv}) b->release();
@@ -1261,6 +1515,8 @@ void do_prepare(bemfmm_ctx* c) {
   } else if (pc != BEMFMM_PRECOND_NONE) {
     throw std::runtime_error("unknown preconditioner");
   }
+  HIPCHECK(hipStreamSynchronize(s));
+  const double t3c = now();
   if (c->opts.precision == BEMFMM_PRECISION_FP32_REFINE) {
     // round the resident FP64 matrices once to FP32 (operators were computed in binary128 and rounded to FP64)
     c->near_mat32.alloc(pl.near.mat_size);
@@ -1328,7 +1584,10 @@ void do_prepare(bemfmm_ctx* c) {
   st.t_ops = pl.t_ops;
   st.t_upload = t2 - t1;
   st.t_near_quad = t3 - t2;
-  st.t_alpha = t4 - t3;
+  st.t_alpha = t3a - t3;
+  st.t_tune = t3b - t3a;
+  st.t_precond = t3c - t3b;
+  st.t_fp32 = t4 - t3c;
   st.n_precond_blocks = c->sw_host.n_blocks();
   st.precond_fallback_blocks = c->sw_failed;
   st.precond_avg_block = c->sw_host.n_blocks() ? (double)c->sw_host.idx.size() / c->sw_host.n_blocks() : 0.;
@@ -1339,6 +1598,7 @@ void do_prepare(bemfmm_ctx* c) {
   const bool ilu_kind = pc == BEMFMM_PRECOND_ILU0 || pc == BEMFMM_PRECOND_MILU0 || pc == BEMFMM_PRECOND_ILUT;
   st.precond_nnz = (ilu_kind || (pc == BEMFMM_PRECOND_SCHWARZ && c->opts.schwarz_variant == 1)) ? c->ilu_nnz : 0;
   st.precond_levels = ilu_kind ? c->ilu_levels : 0;
+  st.tree_on_gpu = (c->opts.tree_on_gpu && o.max_level <= 11) ? 1 : 0;
   (void)t0;
 }
 
@@ -1988,6 +2248,7 @@ void bemfmm_default_opts(bemfmm_opts* o) {
   o->ilut_max_entries = 50;   // main_time_domain.cpp:43
   o->ilut_pivot_min = 1e-12;  // main_time_domain.cpp:44
   o->plane_wave_digits = 10;
+  o->tree_on_gpu = 1;
 }
 
 bemfmm_ctx* bemfmm_ctx_create(int device, const bemfmm_opts* opts) {
@@ -2260,6 +2521,46 @@ int bemfmm_schwarz_blocks(const bemfmm_ctx* c, int64_t* ptr, int32_t* idx, int32
     for (size_t k = 0; k < B.idx.size(); ++k) idx[k] = c->perm[B.idx[k]];  // caller numbering
   if (ncore) std::copy(B.ncore.begin(), B.ncore.end(), ncore);
   return nb;
+}
+
+int bemfmm_tree_digest(const bemfmm_ctx* c, uint64_t* digest) {
+  if (!c || !digest) return BEMFMM_E_INVALID;
+  if (!c->prepared || !c->plan) return BEMFMM_E_STATE;
+  const Plan& pl = *c->plan;
+  uint64_t h = 0x9e3779b97f4a7c15ull;
+  auto mix = [&](uint64_t v) {  // splitmix64 finaliser over a running state
+    h += v + 0x9e3779b97f4a7c15ull;
+    uint64_t z = h;
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    h = z ^ (z >> 31);
+  };
+  auto add_i32 = [&](const std::vector<int32_t>& v) {
+    mix(v.size());
+    for (int32_t x : v) mix((uint32_t)x);
+  };
+  auto add_i64 = [&](const std::vector<int64_t>& v) {
+    mix(v.size());
+    for (int64_t x : v) mix((uint64_t)x);
+  };
+  add_i32(pl.level_ptr);
+  add_i32(pl.box_parent);
+  add_i32(pl.box_leaf);
+  mix(pl.box_center.size());
+  for (double x : pl.box_center) {
+    uint64_t u;
+    std::memcpy(&u, &x, 8);
+    mix(u);
+  }
+  add_i32(pl.face_perm);
+  add_i64(pl.box_range);
+  add_i32(pl.target_box);
+  add_i64(pl.near_ptr);
+  add_i32(pl.near_idx);
+  add_i64(pl.m2l_ptr);
+  add_i32(pl.m2l_idx);
+  *digest = h;
+  return BEMFMM_OK;
 }
 
 int bemfmm_refinement_history(const bemfmm_ctx* c, double* out, int cap) {

@@ -13,7 +13,9 @@
 // Setup: the near-field quadrature (GW5x5 singular/near, Dunavant far; GMRES_FMM.hpp:1824-1891), the P2M
 // matrices, the Morton keys and the Schwarz block inverses run here too.
 #include <hip/hip_runtime.h>
-#include <hipcub/hipcub.hpp>
+
+#include <cstring>  // rocPRIM's texture iterator uses memset without including it
+#include <rocprim/rocprim.hpp>
 
 #include <algorithm>
 #include <cstdint>
@@ -1602,7 +1604,8 @@ void morton_keys(hipStream_t s, int n, const int32_t* point, const double* X, co
 
 hipError_t radix_sort_pairs(hipStream_t s, void* tmp, size_t& tmp_bytes, const uint64_t* keys_in, uint64_t* keys_out,
                             const int32_t* vals_in, int32_t* vals_out, int n, int bits) {
-  return hipcub::DeviceRadixSort::SortPairs(tmp, tmp_bytes, keys_in, keys_out, vals_in, vals_out, n, 0, bits, s);
+  return rocprim::radix_sort_pairs(tmp, tmp_bytes, keys_in, keys_out, vals_in, vals_out, (size_t)n, 0u, (unsigned)bits,
+                                   s);
 }
 
 void p2m_build(hipStream_t s, int P, const P2MBuild& a, double* mat) {

@@ -12,6 +12,8 @@
 #include <cstddef>
 #include <cstdint>
 
+#include "treemath.hpp"
+
 #ifdef __HIPCC__
 #define BEMFMM_HD __host__ __device__
 #else
@@ -202,6 +204,61 @@ void morton_keys(hipStream_t s, int n, const int32_t* point, const double* X, co
 hipError_t radix_sort_pairs(hipStream_t s, void* tmp, size_t& tmp_bytes, const uint64_t* keys_in, uint64_t* keys_out,
                             const int32_t* vals_in, int32_t* vals_out, int n, int bits);
 void p2m_build(hipStream_t s, int P, const P2MBuild& a, double* mat);  // throws for P outside 1..10
+
+// ---- plan tree stage (tree.hip): octree, interaction lists, target boxes ----
+// Per level l (boxes [lb, le), faces in per-face arrays of F): tree_split appends one key
+// ((box - lb) * 32 + cell) << 32 | face per face whose box splits (more than leaf_max faces and may_split) and
+// shifts every face's path key by one digit; after the keys are sorted, tree_heads + an inclusive scan number
+// the distinct (box, cell) runs, tree_children creates one child box per run at le + run, and tree_assign
+// moves each face into its run's child if the child's inside test holds (else the face stays with its box).
+// err |= 1: a box with more than kTreeCells cells, 2 (tree_nonempty): a child that kept no face.
+constexpr int kTreeCells = 27;  // cells per box at most (2 per axis; 3 at the root when its side rounds up)
+void tree_split(hipStream_t s, int F, int lb, int leaf_max, int may_split, const double* pole, const int32_t* fbox,
+                uint64_t* fkey, const BoxGeom* geom, const int32_t* count, uint64_t* out, uint32_t* counter,
+                int32_t* err);
+void tree_heads(hipStream_t s, int n, const uint64_t* keys, int32_t* flags);
+void tree_children(hipStream_t s, int n, const uint64_t* keys, const int32_t* ci, int lb, int le, BoxGeom* geom,
+                   int32_t* parent, int32_t* level, uint64_t* pk, int32_t* count, int32_t* nchild,
+                   int32_t* first_child, int32_t* child_tab);
+void tree_assign(hipStream_t s, int n, const uint64_t* keys, const int32_t* ci, int le, const double* pole,
+                 const BoxGeom* geom, int32_t* fbox, uint64_t* fkey, int32_t* count);
+void tree_nonempty(hipStream_t s, int nc, int le, const int32_t* count, int32_t* err);
+void iota_i32(hipStream_t s, int n, int32_t* out);
+void fill_i32(hipStream_t s, int64_t n, int32_t* p, int32_t v);
+void fill_u64(hipStream_t s, int64_t n, uint64_t* p, uint64_t v);
+// box b's faces are those whose `digits`-digit path key starts with its own path key pk[b] (level[b] digits)
+void tree_ranges(hipStream_t s, int nb, int digits, const int32_t* level, const uint64_t* pk, const uint64_t* keys,
+                 int F, int64_t* range);
+// internal offsets: leaves from their faces, then each level's parents (deepest first) from their children
+void tree_leaf_off(hipStream_t s, int nb, const int32_t* nchild, const int64_t* range, const int32_t* perm,
+                   const double* maxoff, double* off);
+void tree_parent_off(hipStream_t s, int lb, int le, const int32_t* nchild, const int32_t* first_child, double* off);
+void tree_targets(hipStream_t s, int N, const double* X, const BoxGeom* geom, const int32_t* nchild,
+                  const int32_t* child_tab, int32_t* target);
+// near candidates (A, B) of one level, as keys A << 32 | B: list_count flags the near pairs and counts the child
+// pairs that are M2L pairs or next level's candidates; list_write writes them at the exclusive scans of the
+// counts; list_totals writes the three totals to out[0..2]
+void list_count(hipStream_t s, int np, const uint64_t* pairs, const BoxGeom* geom, const double* off,
+                const int32_t* nchild, const int32_t* first_child, double theta, int32_t* nearf, int32_t* m2lc,
+                int32_t* nextc);
+void list_write(hipStream_t s, int np, const uint64_t* pairs, const BoxGeom* geom, const double* off,
+                const int32_t* nchild, const int32_t* first_child, double theta, const int32_t* nearf,
+                const int32_t* near_at, const int32_t* m2l_at, const int32_t* next_at, uint64_t* near_out,
+                uint64_t* m2l_out, uint64_t* next_out);
+void list_totals(hipStream_t s, int np, const int32_t* nearf, const int32_t* m2lc, const int32_t* nextc,
+                 const int32_t* near_at, const int32_t* m2l_at, const int32_t* next_at, int64_t* out);
+// sorted keys A << 32 | B -> CSR over A in [0, nb]: ptr (nb + 1), idx = B
+void csr_from_keys(hipStream_t s, int nb, const uint64_t* keys, int64_t n, int64_t* ptr, int32_t* idx);
+// rocPRIM radix sorts and scans; tmp == nullptr: only sets tmp_bytes
+hipError_t sort_keys_u64(hipStream_t s, void* tmp, size_t& tmp_bytes, const uint64_t* in, uint64_t* out, int64_t n,
+                         int begin_bit, int end_bit);
+hipError_t sort_pairs_u64_i32(hipStream_t s, void* tmp, size_t& tmp_bytes, const uint64_t* keys_in,
+                              uint64_t* keys_out, const int32_t* vals_in, int32_t* vals_out, int64_t n, int begin_bit,
+                              int end_bit);
+hipError_t scan_inclusive_i32(hipStream_t s, void* tmp, size_t& tmp_bytes, const int32_t* in, int32_t* out,
+                              int64_t n);
+hipError_t scan_exclusive_i32(hipStream_t s, void* tmp, size_t& tmp_bytes, const int32_t* in, int32_t* out,
+                              int64_t n);
 void near_quad(hipStream_t s, int64_t nrows, const NearQuad& a, const Gemv& g, double* mat);
 void to_f32(hipStream_t s, int64_t n, const double* a, float* b);
 void diag_inv(hipStream_t s, int n, const NearRows& a, double* dinv);

@@ -30,7 +30,7 @@ static inline int pad_cols(int n) { return (n + 3) & ~3; }
 namespace {
 
 using V3 = std::array<double, 3>;
-inline double hyp3(double a, double b, double c) { return std::hypot(a, b, c); }
+inline double hyp3(double a, double b, double c) { return hypot3(a, b, c); }
 
 double now() { return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count(); }
 
@@ -65,10 +65,7 @@ const std::vector<std::array<double, 4>>& dunavant(int q) {
 
 struct TBox {
   int level = 0, parent = -1;
-  double bmin[3], bmax[3];
-  V3 X;
-  double dL = 0, inv_dL = 0;
-  int size[3] = {1, 1, 1};
+  BoxGeom g;
   std::vector<int> cells;
   std::vector<std::vector<int>> cell_data;
   std::vector<int> data;
@@ -76,83 +73,52 @@ struct TBox {
   bool has_children = false;
   int id = -1;  // final level-major id
   std::vector<int> near, m2l;
-  void init(const double lo[3], const double hi[3], double dl) {
-    for (int a = 0; a < 3; ++a) {
-      bmin[a] = lo[a];
-      bmax[a] = hi[a];
-      X[a] = (lo[a] + hi[a]) / 2.;
-      double s = std::ceil((hi[a] - lo[a]) / dl);
-      if (s <= 0.) s = 1.;
-      size[a] = (int)s;
-    }
-    dL = dl;
-    inv_dL = 1.0 / dl;
-    cells.assign(size[0] * size[1] * size[2], -1);
-    cell_data.assign(size[0] * size[1] * size[2], {});
-  }
-  bool inside(const V3& x) const {
-    for (int a = 0; a < 3; ++a) {
-      const int i = (int)std::floor((x[a] - bmin[a]) * inv_dL);
-      if (i < 0 || i >= size[a]) return false;
-    }
-    return true;
-  }
-  int cell_of(const V3& x) const {
-    int ijk[3];
-    for (int a = 0; a < 3; ++a) ijk[a] = std::clamp((int)std::floor((x[a] - bmin[a]) * inv_dL), 0, size[a] - 1);
-    return (ijk[0] * size[1] + ijk[1]) * size[2] + ijk[2];
+  void init(const BoxGeom& geom) {
+    g = geom;
+    cells.assign(box_cells(g), -1);
+    cell_data.assign(box_cells(g), {});
   }
 };
 
 struct Builder {
   Plan& pl;
-  const Options& opt;
   std::vector<TBox> B;
-  std::vector<V3> pole;
-  std::vector<double> maxoff;
-  explicit Builder(Plan& p, const Options& o) : pl(p), opt(o) {}
+  explicit Builder(Plan& p) : pl(p) {}
+  const double* pole(int f) const { return &pl.pole[3 * (size_t)f]; }
 
   void add(int b, int f) {
     B[b].data.push_back(f);
-    B[b].cell_data[B[b].cell_of(pole[f])].push_back(f);
+    B[b].cell_data[box_cell_of(B[b].g, pole(f))].push_back(f);
   }
   void grow(int b) {
     // generateTree with the grow condition of BEM_solveBVP_GMRES_FMM.hpp:1199
-    const bool cond = !B[b].data.empty() && (int)B[b].data.size() > opt.leaf_max && B[b].level < opt.max_level;
+    const bool cond = !B[b].data.empty() && (int)B[b].data.size() > pl.opt.leaf_max && B[b].level < pl.opt.max_level;
     if (!cond) return;
     const int ncell = (int)B[b].cells.size();
     for (int c = 0; c < ncell; ++c) {
       if (B[b].cell_data[c].empty()) continue;
-      const int k = c % B[b].size[2], j = (c / B[b].size[2]) % B[b].size[1], i = c / (B[b].size[2] * B[b].size[1]);
-      const int ijk[3] = {i, j, k};
-      double lo[3], hi[3];
-      for (int a = 0; a < 3; ++a) {
-        lo[a] = B[b].dL * ijk[a] + B[b].bmin[a];
-        hi[a] = B[b].dL * (ijk[a] + 1) + B[b].bmin[a];
-      }
+      BoxGeom cg;
+      box_child(B[b].g, c, cg);
       TBox ch;
-      ch.init(lo, hi, B[b].dL * (0.5 + 1e-13));
+      ch.init(cg);
       ch.parent = b;
       ch.level = B[b].level + 1;
       const int cid = (int)B.size();
       B.push_back(std::move(ch));
       for (int f : B[b].cell_data[c])
-        if (B[cid].inside(pole[f])) add(cid, f);
+        if (box_inside(B[cid].g, pole(f))) add(cid, f);
       B[b].cells[c] = cid;
       B[b].has_children = true;
       grow(cid);
     }
   }
   bool is_far(int a, int b) const {
-    // isFar (moments.hpp:948-954)
-    const double eff = std::max(B[a].dL + 2.0 * B[a].off, B[b].dL + 2.0 * B[b].off);
-    const double d = hyp3(B[b].X[0] - B[a].X[0], B[b].X[1] - B[a].X[1], B[b].X[2] - B[a].X[2]);
-    return opt.theta > std::sqrt(3.0) * eff / d;
+    return boxes_far(B[a].g.X, B[a].g.dL, B[a].off, B[b].g.X, B[b].g.dL, B[b].off, pl.opt.theta);
   }
-  int deepest(const V3& x) const {
+  int deepest(const double* x) const {
     int cur = 0;
     while (B[cur].has_children) {
-      const int nx = B[cur].cells[B[cur].cell_of(x)];
+      const int nx = B[cur].cells[box_cell_of(B[cur].g, x)];
       if (nx < 0) return cur;
       cur = nx;
     }
@@ -219,14 +185,13 @@ struct Harm {
 
 }  // namespace
 
-void build_plan(Plan& pl, const Options& opt, int n_nodes, const double* X, int n_faces, const int32_t* tri,
-                const int32_t* face_dof, int n_dof, const int32_t* dof_kind, const int32_t* dof_point) {
+void plan_geometry(Plan& pl, const Options& opt, int n_nodes, const double* X, int n_faces, const int32_t* tri,
+                   const int32_t* face_dof, int n_dof, const int32_t* dof_kind, const int32_t* dof_point) {
   if (opt.p < 1 || opt.p > 20) throw std::runtime_error("expansion order p must be in [1, 20]");
   if (n_nodes <= 0 || n_faces <= 0 || n_dof <= 0) throw std::runtime_error("empty geometry");
   pl.opt = opt;
   pl.P = opt.p;
   pl.C = (opt.p + 1) * (opt.p + 1);
-  const int P = pl.P, C = pl.C;
   pl.n_nodes = n_nodes;
   pl.n_faces = n_faces;
   pl.n_dof = n_dof;
@@ -267,7 +232,6 @@ void build_plan(Plan& pl, const Options& opt, int n_nodes, const double* X, int 
   pl.near_region = pl.scale / 25.;
 
   // --- faces and P2M sources (GMRES_FMM.hpp:1799-1937) ---
-  Builder bd(pl, opt);
   const auto& rule = dunavant(opt.q);
   pl.fX.resize(9 * (size_t)n_faces);
   pl.fcent.resize(3 * (size_t)n_faces);
@@ -275,8 +239,10 @@ void build_plan(Plan& pl, const Options& opt, int n_nodes, const double* X, int 
   pl.fJ.resize(n_faces);
   pl.fv.assign(tri, tri + 3 * (size_t)n_faces);
   pl.fdof.assign(face_dof, face_dof + 3 * (size_t)n_faces);
-  bd.pole.resize(n_faces);
-  bd.maxoff.resize(n_faces);
+  pl.pole.resize(3 * (size_t)n_faces);
+  pl.maxoff.resize(n_faces);
+  int degenerate = 0;
+#pragma omp parallel for schedule(static) reduction(| : degenerate)
   for (int f = 0; f < n_faces; ++f) {
     V3 x[3];
     for (int k = 0; k < 3; ++k)
@@ -288,23 +254,25 @@ void build_plan(Plan& pl, const Options& opt, int n_nodes, const double* X, int 
     const V3 e2 = {x[2][0] - x[0][0], x[2][1] - x[0][1], x[2][2] - x[0][2]};
     const V3 cr = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2], e1[0] * e2[1] - e1[1] * e2[0]};
     const double J = hyp3(cr[0], cr[1], cr[2]);
-    if (!(J > 0)) throw std::runtime_error("degenerate triangle (zero area)");
+    if (!(J > 0)) degenerate = 1;
     pl.fJ[f] = J;
+    double* pf = &pl.pole[3 * (size_t)f];
     for (int a = 0; a < 3; ++a) {
       pl.fcr[3 * f + a] = J * (cr[a] / J);
       pl.fcent[3 * f + a] = (x[0][a] + x[1][a] + x[2][a]) / 3.;
-      bd.pole[f][a] = (1. / 3.) * x[0][a] + (1. / 3.) * x[1][a] + (1. / 3.) * x[2][a];
+      pf[a] = (1. / 3.) * x[0][a] + (1. / 3.) * x[1][a] + (1. / 3.) * x[2][a];
     }
     double mo = 0.;
     for (const auto& r : rule) {
       V3 xq;
       for (int a = 0; a < 3; ++a) xq[a] = r[0] * x[0][a] + r[1] * x[1][a] + r[2] * x[2][a];
-      mo = std::max(mo, hyp3(xq[0] - bd.pole[f][0], xq[1] - bd.pole[f][1], xq[2] - bd.pole[f][2]));
+      mo = std::max(mo, hyp3(xq[0] - pf[0], xq[1] - pf[1], xq[2] - pf[2]));
     }
-    bd.maxoff[f] = mo;
+    pl.maxoff[f] = mo;
   }
+  if (degenerate) throw std::runtime_error("degenerate triangle (zero area)");
 
-  // --- octree (initializeBucket GMRES_FMM.hpp:1178-1201; generateTree) ---
+  // --- root box (initializeBucket GMRES_FMM.hpp:1178-1201) ---
   {
     double blo[3], bhi[3];
     const double dL = 1.05 * pl.scale / 2.;
@@ -313,14 +281,26 @@ void build_plan(Plan& pl, const Options& opt, int n_nodes, const double* X, int 
       blo[a] = c - dL;
       bhi[a] = c + dL;
     }
-    TBox root;
-    root.init(blo, bhi, dL);
-    bd.B.push_back(std::move(root));
+    box_init(blo, bhi, dL, pl.root);
     pl.root_side = bhi[0] - blo[0];
   }
   for (int f = 0; f < n_faces; ++f)
-    if (bd.B[0].inside(bd.pole[f])) bd.add(0, f);
-  if ((int)bd.B[0].data.size() != n_faces) throw std::runtime_error("face outside the root box");
+    if (!box_inside(pl.root, &pl.pole[3 * (size_t)f])) throw std::runtime_error("face outside the root box");
+  pl.t_tree = now() - t0;
+}
+
+void plan_tree_lists(Plan& pl) {
+  const Options& opt = pl.opt;
+  const int n_nodes = pl.n_nodes, n_faces = pl.n_faces;
+  const double t0 = now();
+  // --- octree (generateTree) ---
+  Builder bd(pl);
+  {
+    TBox root;
+    root.init(pl.root);
+    bd.B.push_back(std::move(root));
+  }
+  for (int f = 0; f < n_faces; ++f) bd.add(0, f);
   bd.grow(0);
   auto& Bx = bd.B;
   // level lists in preorder (rebuildHierarchyLists)
@@ -351,14 +331,12 @@ void build_plan(Plan& pl, const Options& opt, int n_nodes, const double* X, int 
   pl.box_parent.resize(nb);
   pl.box_leaf.resize(nb);
   pl.box_center.resize(3 * (size_t)nb);
-  pl.level_side.resize(pl.n_levels);
-  for (int l = 0; l < pl.n_levels; ++l) pl.level_side[l] = pl.root_side / std::ldexp(1.0, l);
   for (int i = 0; i < nb; ++i) {
     const TBox& t = Bx[id2b[i]];
     pl.box_level[i] = t.level;
     pl.box_parent[i] = t.parent >= 0 ? Bx[t.parent].id : -1;
     pl.box_leaf[i] = t.has_children ? 0 : 1;
-    for (int a = 0; a < 3; ++a) pl.box_center[3 * i + a] = t.X[a];
+    for (int a = 0; a < 3; ++a) pl.box_center[3 * i + a] = t.g.X[a];
   }
   // DFS face order; every box's faces are a contiguous range
   pl.face_perm.clear();
@@ -387,7 +365,11 @@ void build_plan(Plan& pl, const Options& opt, int n_nodes, const double* X, int 
   }
   pl.n_leaves = 0;
   for (int i = 0; i < nb; ++i) pl.n_leaves += pl.box_leaf[i];
-  pl.t_tree = now() - t0;
+  // --- targets: deepest box per node (getBucketAtDeepest) ---
+  pl.target_box.resize(n_nodes);
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < n_nodes; ++i) pl.target_box[i] = Bx[bd.deepest(&pl.Xs[3 * (size_t)i])].id;
+  pl.t_tree += now() - t0;
 
   // --- max_internal_offset (moments.hpp:1323-1337) ---
   const double t1 = now();
@@ -395,7 +377,7 @@ void build_plan(Plan& pl, const Options& opt, int n_nodes, const double* X, int 
     TBox& t = Bx[id2b[i]];
     t.off = 0.;
     if (!t.has_children) {
-      for (int f : t.data) t.off = std::max(t.off, bd.maxoff[f]);
+      for (int f : t.data) t.off = std::max(t.off, pl.maxoff[f]);
     } else {
       for (int c : t.cells)
         if (c >= 0) t.off = std::max(t.off, Bx[c].off);
@@ -441,23 +423,42 @@ void build_plan(Plan& pl, const Options& opt, int n_nodes, const double* X, int 
       }
       pairs.swap(next);
     }
-    for (auto& t : Bx) {
-      std::sort(t.near.begin(), t.near.end());
-      std::sort(t.m2l.begin(), t.m2l.end());
+  }
+  // as CSR by box id, ascending
+  auto to_csr = [&](std::vector<int> TBox::*list, std::vector<int64_t>& ptr, std::vector<int32_t>& idx) {
+    ptr.assign(nb + 1, 0);
+    for (int i = 0; i < nb; ++i) ptr[i + 1] = ptr[i] + (int64_t)(Bx[id2b[i]].*list).size();
+    idx.assign(ptr[nb], 0);
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int i = 0; i < nb; ++i) {
+      int32_t* o = &idx[ptr[i]];
+      int k = 0;
+      for (int r : Bx[id2b[i]].*list) o[k++] = Bx[r].id;
+      std::sort(o, o + k);
     }
-  }
+  };
+  to_csr(&TBox::near, pl.near_ptr, pl.near_idx);
+  to_csr(&TBox::m2l, pl.m2l_ptr, pl.m2l_idx);
   pl.t_lists = now() - t1;
+}
 
-  // --- targets: deepest box per node (getBucketAtDeepest) ---
+void build_plan(Plan& pl, const Options& opt, int n_nodes, const double* X, int n_faces, const int32_t* tri,
+                const int32_t* face_dof, int n_dof, const int32_t* dof_kind, const int32_t* dof_point) {
+  plan_geometry(pl, opt, n_nodes, X, n_faces, tri, face_dof, n_dof, dof_kind, dof_point);
+  plan_tree_lists(pl);
+  plan_finish(pl);
+}
+
+void plan_finish(Plan& pl) {
+  const Options& opt = pl.opt;
+  const int P = pl.P, C = pl.C, n_nodes = pl.n_nodes, n_dof = pl.n_dof, nb = pl.n_boxes;
+  const int32_t* face_dof = pl.fdof.data();
+  const auto& rule = dunavant(opt.q);
+  pl.level_side.resize(pl.n_levels);
+  for (int l = 0; l < pl.n_levels; ++l) pl.level_side[l] = pl.root_side / std::ldexp(1.0, l);
   const double t2 = now();
-  pl.target_box.resize(n_nodes);
   std::vector<std::vector<int>> box_targets(nb);
-  for (int i = 0; i < n_nodes; ++i) {
-    const V3 x = {pl.Xs[3 * i], pl.Xs[3 * i + 1], pl.Xs[3 * i + 2]};
-    const int b = Bx[bd.deepest(x)].id;
-    pl.target_box[i] = b;
-    box_targets[b].push_back(i);
-  }
+  for (int i = 0; i < n_nodes; ++i) box_targets[pl.target_box[i]].push_back(i);
   // --- near blocks: one per target box (split into <= kMaxRows rows) ---
   {
     constexpr int kMaxRows = 32;
@@ -474,14 +475,13 @@ void build_plan(Plan& pl, const Options& opt, int n_nodes, const double* X, int 
 #pragma omp for schedule(dynamic, 8)
       for (int ti = 0; ti < ntb; ++ti) {
         const int b = tboxes[ti];
-        const TBox& t = Bx[id2b[b]];
         std::vector<std::pair<int64_t, int64_t>> rg;
-        auto addbox = [&](int raw) { rg.push_back({pl.box_range[2 * Bx[raw].id], pl.box_range[2 * Bx[raw].id + 1]}); };
-        addbox(id2b[b]);
-        for (int nbx : t.near) addbox(nbx);
-        for (int a = t.parent; a >= 0; a = Bx[a].parent)
-          for (int nbx : Bx[a].near)
-            if (!Bx[nbx].has_children) addbox(nbx);
+        auto addbox = [&](int id) { rg.push_back({pl.box_range[2 * id], pl.box_range[2 * id + 1]}); };
+        addbox(b);
+        for (int64_t k = pl.near_ptr[b]; k < pl.near_ptr[b + 1]; ++k) addbox(pl.near_idx[k]);
+        for (int a = pl.box_parent[b]; a >= 0; a = pl.box_parent[a])
+          for (int64_t k = pl.near_ptr[a]; k < pl.near_ptr[a + 1]; ++k)
+            if (pl.box_leaf[pl.near_idx[k]]) addbox(pl.near_idx[k]);
         std::sort(rg.begin(), rg.end());
         std::vector<int64_t> merged;
         for (auto& r : rg) {
@@ -624,10 +624,12 @@ void build_plan(Plan& pl, const Options& opt, int n_nodes, const double* X, int 
     const int nl = (int)leaves.size();
     std::vector<std::vector<int32_t>> cols(nl);
     for (int li = 0; li < nl; ++li) {
-      const TBox& t = Bx[id2b[leaves[li]]];
+      const int b = leaves[li];
       std::vector<int32_t> cl;
-      for (int f : t.data)
+      for (int64_t i = pl.box_range[2 * b]; i < pl.box_range[2 * b + 1]; ++i) {
+        const int f = pl.face_perm[i];
         for (int c = 0; c < 3; ++c) cl.push_back(face_dof[3 * f + c]);
+      }
       std::sort(cl.begin(), cl.end());
       cl.erase(std::unique(cl.begin(), cl.end()), cl.end());
       cols[li] = std::move(cl);
@@ -652,7 +654,8 @@ void build_plan(Plan& pl, const Options& opt, int n_nodes, const double* X, int 
     pl.p2m_leaf_cs.clear();
     for (int li = 0; li < nl; ++li) {
       const int b = leaves[li];
-      for (int f : Bx[id2b[b]].data) {
+      for (int64_t i = pl.box_range[2 * b]; i < pl.box_range[2 * b + 1]; ++i) {
+        const int f = pl.face_perm[i];
         pl.p2m_faces.push_back(f);
         for (int c = 0; c < 3; ++c)
           pl.p2m_face_lc.push_back(
@@ -670,11 +673,11 @@ void build_plan(Plan& pl, const Options& opt, int n_nodes, const double* X, int 
 #pragma omp for schedule(dynamic, 4)
       for (int li = 0; li < nl; ++li) {
         const int b = leaves[li];
-        const TBox& t = Bx[id2b[b]];
         const double s = pl.level_side[pl.box_level[b]];
         const int ncol = (int)cols[li].size();
         double* M = &pl.p2m_mat[G.mat_ptr[li]];
-        for (int f : t.data) {
+        for (int64_t fi = pl.box_range[2 * b]; fi < pl.box_range[2 * b + 1]; ++fi) {
+          const int f = pl.face_perm[fi];
           int lc[3];
           for (int c = 0; c < 3; ++c)
             lc[c] = (int)(std::lower_bound(cols[li].begin(), cols[li].end(), face_dof[3 * f + c]) - cols[li].begin());
@@ -761,26 +764,27 @@ void build_plan(Plan& pl, const Options& opt, int n_nodes, const double* X, int 
     // and the pairs counting-sorted by code (stable).
     constexpr int kOffB = 64;
     std::vector<int64_t> base(nb + 1, 0);
-    for (int i = 0; i < nb; ++i) base[i + 1] = base[i] + (int64_t)Bx[id2b[i]].m2l.size();
+    for (int i = 0; i < nb; ++i) base[i + 1] = base[i] + (pl.m2l_ptr[i + 1] - pl.m2l_ptr[i]);
     const int64_t np = base[nb];
     pl.n_m2l_pairs = np;
     std::vector<int32_t> code(np), srcid(np);
     int bad = 0;
 #pragma omp parallel for schedule(dynamic, 64) reduction(| : bad)
     for (int i = 0; i < nb; ++i) {
-      const TBox& t = Bx[id2b[i]];
-      const double s = pl.level_side[t.level];
+      const double s = pl.level_side[pl.box_level[i]];
+      const double* tX = &pl.box_center[3 * (size_t)i];
       int64_t k = base[i];
-      for (int src : t.m2l) {
-        const TBox& u = Bx[src];
+      for (int64_t q = pl.m2l_ptr[i]; q < pl.m2l_ptr[i + 1]; ++q) {
+        const int src = pl.m2l_idx[q];
+        const double* uX = &pl.box_center[3 * (size_t)src];
         int o[3];
         for (int a = 0; a < 3; ++a) {
-          const double v = (u.X[a] - t.X[a]) / s;
+          const double v = (uX[a] - tX[a]) / s;
           o[a] = (int)std::lround(v);
           if (std::abs(v - o[a]) > 1e-6 || o[a] <= -kOffB || o[a] >= kOffB) bad = 1;
         }
         code[k] = ((o[0] + kOffB) * 2 * kOffB + (o[1] + kOffB)) * 2 * kOffB + (o[2] + kOffB);
-        srcid[k] = u.id;
+        srcid[k] = src;
         ++k;
       }
     }

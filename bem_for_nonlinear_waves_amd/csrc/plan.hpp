@@ -8,6 +8,8 @@
 #include <string>
 #include <vector>
 
+#include "treemath.hpp"
+
 namespace bemfmm {
 
 struct Options {
@@ -60,6 +62,11 @@ struct Plan {
   std::vector<int32_t> fv;           // n_faces*3 node ids
   std::vector<int32_t> fdof;         // n_faces*3 DOF ids
   std::vector<int32_t> dof_kind, dof_point;
+  // inputs of the tree: per face the P2M pole (vertex mean with weights 1/3) and the largest distance from it
+  // to the face's quadrature points; the root box (cube of half-side 1.05 scale/2 about the bounds' centre)
+  std::vector<double> pole;          // n_faces*3
+  std::vector<double> maxoff;        // n_faces
+  BoxGeom root{};
   // tree (boxes numbered level-major, preorder within a level)
   int n_levels = 0, n_boxes = 0;
   std::vector<int32_t> level_ptr;    // n_levels+1
@@ -68,6 +75,10 @@ struct Plan {
   std::vector<double> level_side;    // ideal side per level (root_side/2^l)
   std::vector<int32_t> face_perm;    // DFS order of faces
   std::vector<int64_t> box_range;    // n_boxes*2 range into face_perm
+  // interaction lists by box id, ascending: near[near_ptr[b] .. near_ptr[b+1]) are the boxes of b's level
+  // that are not far from b (b excluded), m2l[...] the far boxes among the children of b's parent's near boxes
+  std::vector<int64_t> near_ptr, m2l_ptr;  // n_boxes+1
+  std::vector<int32_t> near_idx, m2l_idx;
   // near field + L2P (one block per target box)
   BlockGemv near;
   std::vector<int32_t> near_ncol_dof;     // per block: DOF columns (then C L2P columns)
@@ -100,10 +111,18 @@ struct Plan {
   double t_tree = 0, t_lists = 0, t_near_struct = 0, t_p2m = 0, t_ops = 0;
 };
 
-// Builds everything except the near-field quadrature weights (computed on the GPU).
-// Throws std::runtime_error on invalid input.
+// Builds everything except the near-field quadrature weights (computed on the GPU): plan_geometry, then
+// plan_tree_lists, then plan_finish. Throws std::runtime_error on invalid input.
 void build_plan(Plan& pl, const Options& opt, int n_nodes, const double* X, int n_faces, const int32_t* tri,
                 const int32_t* face_dof, int n_dof, const int32_t* dof_kind, const int32_t* dof_point);
+// input checks, coordinate scaling, faces and their P2M sources, the root box
+void plan_geometry(Plan& pl, const Options& opt, int n_nodes, const double* X, int n_faces, const int32_t* tri,
+                   const int32_t* face_dof, int n_dof, const int32_t* dof_kind, const int32_t* dof_point);
+// the octree, its interaction lists and the target box of every node, on the host (sets n_levels .. m2l_idx,
+// target_box, n_leaves)
+void plan_tree_lists(Plan& pl);
+// near blocks, L2P coefficients, P2M blocks and the translation groups from the tree arrays
+void plan_finish(Plan& pl);
 
 // Real-field packed translation operators (operators.cpp), computed in __float128
 // and rounded once to double. Layout: out[k_in * C + r_out].

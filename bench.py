@@ -242,8 +242,34 @@ def setup_bvp(m, d, phi, phin, args, precision):
          f"m2l_pairs={st['n_m2l_pairs']} near_nnz={st['near_nnz']} device={st['device_bytes']/1e9:.1f}GB "
          f"(tree {st['t_tree']:.2f} lists {st['t_lists']:.2f} near_struct {st['t_near_struct']:.2f} "
          f"p2m {st['t_p2m']:.2f} ops {st['t_ops']:.2f} upload {st['t_upload']:.2f} near_quad {st['t_near_quad']:.2f} "
-         f"alpha {st['t_alpha']:.2f})")
+         f"alpha {st['t_alpha']:.2f} tune {st['t_tune']:.2f} precond {st['t_precond']:.2f} fp32 {st['t_fp32']:.2f})")
     return bvp, st, t_setup
+
+
+SETUP_STAGES = ("t_tree", "t_lists", "t_near_struct", "t_p2m", "t_ops", "t_upload", "t_near_quad", "t_alpha", "t_tune",
+                "t_precond", "t_fp32")
+
+
+def time_reprepare(bvp, m, d, args, stages=3, seed=11):
+    """Re-prepare after the nodes moved, as every RK stage of a time-stepping session does: free-surface nodes
+    displaced by N(0, 1e-3 h), then bemfmm_set_geometry + bemfmm_prepare on the same context (tree, lists, near
+    structure and quadrature, P2M, alpha and preconditioner rebuilt; operators and buffers reused)."""
+    from bem_for_nonlinear_waves_amd import mesh as M
+    times, mk = [], m
+    for k in range(stages):
+        mk = M.perturb_free_surface(mk, seed + k, 1e-3 * args.h)
+        phi, phin = M.tank_bc(mk, d, seed + k, 20.0, 4.0, body_velocity_sigma=0.1)
+        bvp.set_geometry(mk, d)
+        bvp.set_bc(phi, phin)
+        t = time.perf_counter()
+        bvp.prepare()
+        times.append(time.perf_counter() - t)
+    st = bvp.stats()
+    out = {"s": times, "tree_on_gpu": int(st["tree_on_gpu"]),
+           "stages_s": {k[2:]: round(float(st[k]), 4) for k in SETUP_STAGES}}
+    _log(f"re-prepare after moved nodes: {' '.join(f'{t:.3f}' for t in times)} s; last: " +
+         " ".join(f"{k} {v:.3f}" for k, v in out["stages_s"].items()))
+    return out
 
 
 def time_matvec(bvp, n, n_nodes, reps, fp32):
@@ -574,6 +600,7 @@ def main():
     x, lat, infos, t_total = run_solves(bvp, n, args, args.precision)
     solves_per_s = args.steps / t_total
     iters = [i["iters"] for i in infos]
+    reprep = time_reprepare(bvp, m, d, args) if not args.case else None
 
     fp64_cmp = None
     if fp32 and not args.no_fp64_compare:
@@ -685,6 +712,7 @@ def main():
         "refinement_steps": [i["refinement_steps"] for i in infos],
         "residual": [i["residual"] for i in infos],
         "setup_s": t_setup,
+        "reprepare": reprep,
         "fp64_path": fp64_cmp,
         "roofline": roof,
         "cpu_baseline": cpu,

@@ -114,6 +114,9 @@ typedef struct bemfmm_opts {
   int plane_wave_digits; /* m2l_kernel 3 / 4: quadrature of the exponential expansions, 10 (default; 1751
                             exponentials, operators within ~1e-11 of SimpleM2L) or 6 (558, ~1e-7); the published
                             rules of Greengard & Rokhlin 1997 (lib_plane_wave_quadrature.hpp) */
+  int tree_on_gpu;       /* 1 (default): the octree, the near / M2L lists and the target boxes are built on the GPU
+                            (level-synchronous radix sorts on path keys); 0: on the host. Both give the same plan
+                            (generateTree, lib_spatial_partitioning.hpp:1079-1115; moments.hpp:968-997) */
 } bemfmm_opts;
 
 typedef struct bemfmm_result {
@@ -134,6 +137,8 @@ typedef struct bemfmm_stats {
   int n_m2l_ops;          /* distinct M2L offsets */
   double L;               /* coordinate scale (bounding-box diagonal) */
   double device_bytes;    /* resident device memory of the plan */
+  /* seconds per setup stage: geometry + octree + target boxes, interaction lists, near-field structure, P2M
+     blocks, translation groups and operators, uploads + P2M matrices, near-field quadrature, rigid-mode matvec */
   double t_tree, t_lists, t_near_struct, t_p2m, t_ops, t_upload, t_near_quad, t_alpha;
   int n_precond_blocks;         /* Schwarz: number of blocks */
   int precond_fallback_blocks;  /* Schwarz: blocks whose elimination failed (diagonal used instead) */
@@ -144,6 +149,10 @@ typedef struct bemfmm_stats {
   int64_t precond_nnz;          /* ILU(0)/MILU(0)/ILUT (the factors) and the reference's Schwarz (the matrix): entries */
   int precond_levels;           /* ILU(0)/MILU(0)/ILUT: dependency levels of the forward plus the backward sweep */
   int m2l_plane_wave;           /* 1: the FP32 operator's M2L through plane waves; 2: both operators' */
+  int tree_on_gpu;              /* 1 if the last prepare built the tree and lists on the GPU */
+  double t_tune;                /* seconds: M2L tile autotuning (after t_alpha, the rigid-mode matvec) */
+  double t_precond;             /* seconds: preconditioner build */
+  double t_fp32;                /* seconds: FP32 copies of the operator (FP32_REFINE) */
 } bemfmm_stats;
 
 /* Availability (metal_m2l.h: metal_m2l_is_available / metal_m2l_device_name). */
@@ -235,6 +244,11 @@ int bemfmm_dof_order(const bemfmm_ctx* ctx, int32_t* order);
  * its core (every DOF lies in exactly one core). Any pointer may be NULL. Returns the number of blocks
  * (0 without the Schwarz preconditioner); ptr needs n_blocks+1 entries, idx ptr[n_blocks]. */
 int bemfmm_schwarz_blocks(const bemfmm_ctx* ctx, int64_t* ptr, int32_t* idx, int32_t* ncore);
+
+/* 64-bit digest of a prepared context's tree stage: level offsets, parents, leaf flags, box centres, the
+ * depth-first face order, box face ranges, target boxes and the near / M2L lists. No counterpart in the
+ * reference: equal digests mean equal trees and lists (host and GPU builds, two prepares of one mesh). */
+int bemfmm_tree_digest(const bemfmm_ctx* ctx, uint64_t* digest);
 
 /* FP64 residual |M^-1 (b - A x)|_2 at the start of each refinement step of the last FP32_REFINE solve
  * (the last entry is the final residual). The reference's mixed-precision switch

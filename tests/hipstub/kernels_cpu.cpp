@@ -377,6 +377,253 @@ hipError_t radix_sort_pairs(hipStream_t s, void* tmp, size_t& tmp_bytes, const u
   return hipSuccess;
 }
 
+// ---- plan tree stage (tree.hip) ----
+void tree_split(hipStream_t s, int F, int lb, int leaf_max, int may_split, const double* pole, const int32_t* fbox,
+                uint64_t* fkey, const BoxGeom* geom, const int32_t* count, uint64_t* out, uint32_t* counter,
+                int32_t* err) {
+  hipstub::enqueue(s, [=]() {
+    for (int f = 0; f < F; ++f) {
+      fkey[f] <<= 5;
+      const int b = fbox[f];
+      if (b < lb || !may_split || count[b] <= leaf_max) continue;
+      const int cell = box_cell_of(geom[b], &pole[3 * (size_t)f]);
+      if (cell >= kTreeCells) {
+        *err |= 1;
+        continue;
+      }
+      out[(*counter)++] = ((uint64_t)((uint32_t)(b - lb) * 32u + (uint32_t)cell) << 32) | (uint32_t)f;
+    }
+  });
+}
+void tree_heads(hipStream_t s, int n, const uint64_t* keys, int32_t* flags) {
+  hipstub::enqueue(s, [=]() {
+    for (int i = 0; i < n; ++i) flags[i] = (i == 0 || (keys[i] >> 32) != (keys[i - 1] >> 32)) ? 1 : 0;
+  });
+}
+void tree_children(hipStream_t s, int n, const uint64_t* keys, const int32_t* ci, int lb, int le, BoxGeom* geom,
+                   int32_t* parent, int32_t* level, uint64_t* pk, int32_t* count, int32_t* nchild,
+                   int32_t* first_child, int32_t* child_tab) {
+  hipstub::enqueue(s, [=]() {
+    for (int i = 0; i < n; ++i) {
+      const uint32_t hi = (uint32_t)(keys[i] >> 32);
+      if (i > 0 && (uint32_t)(keys[i - 1] >> 32) == hi) continue;
+      const int pb = (int)(hi / 32u), cell = (int)(hi % 32u);
+      const int P = lb + pb, c = le + ci[i] - 1;
+      box_child(geom[P], cell, geom[c]);
+      parent[c] = P;
+      level[c] = level[P] + 1;
+      pk[c] = pk[P] * 32u + (uint64_t)(cell + 1);
+      count[c] = 0;
+      nchild[c] = 0;
+      first_child[c] = -1;
+      for (int k = 0; k < kTreeCells; ++k) child_tab[(int64_t)c * kTreeCells + k] = -1;
+      child_tab[(int64_t)P * kTreeCells + cell] = c;
+      ++nchild[P];
+      if (i == 0 || (uint32_t)(keys[i - 1] >> 32) / 32u != (uint32_t)pb) first_child[P] = c;
+    }
+  });
+}
+void tree_assign(hipStream_t s, int n, const uint64_t* keys, const int32_t* ci, int le, const double* pole,
+                 const BoxGeom* geom, int32_t* fbox, uint64_t* fkey, int32_t* count) {
+  hipstub::enqueue(s, [=]() {
+    for (int i = 0; i < n; ++i) {
+      const int f = (int)(uint32_t)keys[i];
+      const int cell = (int)((uint32_t)(keys[i] >> 32) % 32u);
+      const int c = le + ci[i] - 1;
+      if (box_inside(geom[c], &pole[3 * (size_t)f])) {
+        fbox[f] = c;
+        fkey[f] |= (uint64_t)(cell + 1);
+        ++count[c];
+      }
+    }
+  });
+}
+void tree_nonempty(hipStream_t s, int nc, int le, const int32_t* count, int32_t* err) {
+  hipstub::enqueue(s, [=]() {
+    for (int i = 0; i < nc; ++i)
+      if (count[le + i] == 0) *err |= 2;
+  });
+}
+void iota_i32(hipStream_t s, int n, int32_t* out) {
+  hipstub::enqueue(s, [=]() { std::iota(out, out + n, 0); });
+}
+void fill_i32(hipStream_t s, int64_t n, int32_t* p, int32_t v) {
+  hipstub::enqueue(s, [=]() { std::fill(p, p + n, v); });
+}
+void fill_u64(hipStream_t s, int64_t n, uint64_t* p, uint64_t v) {
+  hipstub::enqueue(s, [=]() { std::fill(p, p + n, v); });
+}
+void tree_ranges(hipStream_t s, int nb, int digits, const int32_t* level, const uint64_t* pk, const uint64_t* keys,
+                 int F, int64_t* range) {
+  hipstub::enqueue(s, [=]() {
+    for (int b = 0; b < nb; ++b) {
+      const int sh = 5 * (digits - level[b]);
+      range[2 * b] = std::lower_bound(keys, keys + F, pk[b] << sh) - keys;
+      range[2 * b + 1] = std::lower_bound(keys, keys + F, (pk[b] + 1) << sh) - keys;
+    }
+  });
+}
+void tree_leaf_off(hipStream_t s, int nb, const int32_t* nchild, const int64_t* range, const int32_t* perm,
+                   const double* maxoff, double* off) {
+  hipstub::enqueue(s, [=]() {
+    for (int b = 0; b < nb; ++b) {
+      double o = 0.;
+      if (nchild[b] == 0)
+        for (int64_t i = range[2 * b]; i < range[2 * b + 1]; ++i) o = o < maxoff[perm[i]] ? maxoff[perm[i]] : o;
+      off[b] = o;
+    }
+  });
+}
+void tree_parent_off(hipStream_t s, int lb, int le, const int32_t* nchild, const int32_t* first_child, double* off) {
+  hipstub::enqueue(s, [=]() {
+    for (int b = lb; b < le; ++b) {
+      if (nchild[b] == 0) continue;
+      double o = 0.;
+      for (int c = first_child[b]; c < first_child[b] + nchild[b]; ++c) o = o < off[c] ? off[c] : o;
+      off[b] = o;
+    }
+  });
+}
+void tree_targets(hipStream_t s, int N, const double* X, const BoxGeom* geom, const int32_t* nchild,
+                  const int32_t* child_tab, int32_t* target) {
+  hipstub::enqueue(s, [=]() {
+    for (int i = 0; i < N; ++i) {
+      int cur = 0;
+      while (nchild[cur] > 0) {
+        const int cell = box_cell_of(geom[cur], &X[3 * (size_t)i]);
+        const int nx = cell < kTreeCells ? child_tab[(int64_t)cur * kTreeCells + cell] : -1;
+        if (nx < 0) break;
+        cur = nx;
+      }
+      target[i] = cur;
+    }
+  });
+}
+namespace {
+bool far_cpu(const BoxGeom* g, const double* off, int a, int b, double theta) {
+  return boxes_far(g[a].X, g[a].dL, off[a], g[b].X, g[b].dL, off[b], theta);
+}
+}  // namespace
+void list_count(hipStream_t s, int np, const uint64_t* pairs, const BoxGeom* geom, const double* off,
+                const int32_t* nchild, const int32_t* first_child, double theta, int32_t* nearf, int32_t* m2lc,
+                int32_t* nextc) {
+  hipstub::enqueue(s, [=]() {
+    for (int i = 0; i < np; ++i) {
+      const int A = (int)(pairs[i] >> 32), B = (int)(uint32_t)pairs[i];
+      int nf = 0, nm = 0, nn = 0;
+      if (!far_cpu(geom, off, A, B, theta)) {
+        nf = A != B;
+        for (int ac = first_child[A]; ac < first_child[A] + nchild[A]; ++ac)
+          for (int bc = first_child[B]; bc < first_child[B] + nchild[B]; ++bc) {
+            if (ac != bc && far_cpu(geom, off, ac, bc, theta)) ++nm;
+            else ++nn;
+          }
+      }
+      nearf[i] = nf;
+      m2lc[i] = nm;
+      nextc[i] = nn;
+    }
+  });
+}
+void list_write(hipStream_t s, int np, const uint64_t* pairs, const BoxGeom* geom, const double* off,
+                const int32_t* nchild, const int32_t* first_child, double theta, const int32_t* nearf,
+                const int32_t* near_at, const int32_t* m2l_at, const int32_t* next_at, uint64_t* near_out,
+                uint64_t* m2l_out, uint64_t* next_out) {
+  hipstub::enqueue(s, [=]() {
+    for (int i = 0; i < np; ++i) {
+      const int A = (int)(pairs[i] >> 32), B = (int)(uint32_t)pairs[i];
+      if (nearf[i]) near_out[near_at[i]] = pairs[i];
+      if (far_cpu(geom, off, A, B, theta)) continue;
+      int64_t km = m2l_at[i], kn = next_at[i];
+      for (int ac = first_child[A]; ac < first_child[A] + nchild[A]; ++ac)
+        for (int bc = first_child[B]; bc < first_child[B] + nchild[B]; ++bc) {
+          const uint64_t key = ((uint64_t)(uint32_t)ac << 32) | (uint32_t)bc;
+          if (ac != bc && far_cpu(geom, off, ac, bc, theta)) m2l_out[km++] = key;
+          else next_out[kn++] = key;
+        }
+    }
+  });
+}
+void list_totals(hipStream_t s, int np, const int32_t* nearf, const int32_t* m2lc, const int32_t* nextc,
+                 const int32_t* near_at, const int32_t* m2l_at, const int32_t* next_at, int64_t* out) {
+  hipstub::enqueue(s, [=]() {
+    out[0] = (int64_t)near_at[np - 1] + nearf[np - 1];
+    out[1] = (int64_t)m2l_at[np - 1] + m2lc[np - 1];
+    out[2] = (int64_t)next_at[np - 1] + nextc[np - 1];
+  });
+}
+void csr_from_keys(hipStream_t s, int nb, const uint64_t* keys, int64_t n, int64_t* ptr, int32_t* idx) {
+  hipstub::enqueue(s, [=]() {
+    for (int64_t b = 0; b <= nb; ++b) ptr[b] = std::lower_bound(keys, keys + n, (uint64_t)b << 32) - keys;
+    for (int64_t i = 0; i < n; ++i) idx[i] = (int32_t)(uint32_t)keys[i];
+  });
+}
+hipError_t sort_keys_u64(hipStream_t s, void* tmp, size_t& tmp_bytes, const uint64_t* in, uint64_t* out, int64_t n,
+                         int begin_bit, int end_bit) {
+  if (!tmp) {
+    tmp_bytes = 256;
+    return hipSuccess;
+  }
+  hipstub::enqueue(s, [=]() {
+    const uint64_t mask = (end_bit >= 64 ? ~0ull : ((1ull << end_bit) - 1)) & ~((1ull << begin_bit) - 1);
+    std::vector<uint64_t> v(in, in + n);
+    std::stable_sort(v.begin(), v.end(), [&](uint64_t a, uint64_t b) { return (a & mask) < (b & mask); });
+    std::copy(v.begin(), v.end(), out);
+  });
+  return hipSuccess;
+}
+hipError_t sort_pairs_u64_i32(hipStream_t s, void* tmp, size_t& tmp_bytes, const uint64_t* keys_in,
+                              uint64_t* keys_out, const int32_t* vals_in, int32_t* vals_out, int64_t n, int begin_bit,
+                              int end_bit) {
+  if (!tmp) {
+    tmp_bytes = 256;
+    return hipSuccess;
+  }
+  hipstub::enqueue(s, [=]() {
+    const uint64_t mask = (end_bit >= 64 ? ~0ull : ((1ull << end_bit) - 1)) & ~((1ull << begin_bit) - 1);
+    std::vector<int64_t> o(n);
+    std::iota(o.begin(), o.end(), 0);
+    std::stable_sort(o.begin(), o.end(), [&](int64_t a, int64_t b) { return (keys_in[a] & mask) < (keys_in[b] & mask); });
+    std::vector<uint64_t> k(n);
+    std::vector<int32_t> v(n);
+    for (int64_t i = 0; i < n; ++i) {
+      k[i] = keys_in[o[i]];
+      v[i] = vals_in[o[i]];
+    }
+    std::copy(k.begin(), k.end(), keys_out);
+    std::copy(v.begin(), v.end(), vals_out);
+  });
+  return hipSuccess;
+}
+hipError_t scan_inclusive_i32(hipStream_t s, void* tmp, size_t& tmp_bytes, const int32_t* in, int32_t* out,
+                              int64_t n) {
+  if (!tmp) {
+    tmp_bytes = 256;
+    return hipSuccess;
+  }
+  hipstub::enqueue(s, [=]() {
+    int32_t a = 0;
+    for (int64_t i = 0; i < n; ++i) out[i] = a += in[i];
+  });
+  return hipSuccess;
+}
+hipError_t scan_exclusive_i32(hipStream_t s, void* tmp, size_t& tmp_bytes, const int32_t* in, int32_t* out,
+                              int64_t n) {
+  if (!tmp) {
+    tmp_bytes = 256;
+    return hipSuccess;
+  }
+  hipstub::enqueue(s, [=]() {
+    int32_t a = 0;
+    for (int64_t i = 0; i < n; ++i) {
+      const int32_t v = in[i];
+      out[i] = a;
+      a += v;
+    }
+  });
+  return hipSuccess;
+}
+
 void p2m_build(hipStream_t s, int P, const P2MBuild& a, double* mat) {
   if (P < 1 || P > 10) throw std::runtime_error("expansion order outside 1..10 for the P2M build");
   const P2MBuild A = a;
