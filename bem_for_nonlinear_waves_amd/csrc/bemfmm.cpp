@@ -314,6 +314,10 @@ struct bemfmm_ctx {
   hipGraphExec_t gexec[2] = {};      // [FP64, FP32] matvec graphs
   // restricted additive Schwarz: blocks and the core rows of the block inverses
   bemfmm::SchwarzBlocks sw_host;
+  // the mesh-graph Schwarz blocks (schwarz_variant 0) in the caller's DOF numbering, with the key of the topology
+  // and options they were built for: a re-prepare after nodes moved keeps them (they depend on the DOF graph only)
+  bemfmm::SchwarzBlocks sw_caller;
+  uint64_t sw_topo_key = 0;
   int sw_failed = 0;
   DBuf<int64_t> sw_ptr, sw_invptr;
   DBuf<int32_t> sw_idx, sw_ncore;
@@ -361,6 +365,14 @@ struct bemfmm_ctx {
     DBuf<int64_t> range, near_ptr, m2l_ptr, tot;
     DBuf<uint32_t> ctr;
   } tb;
+  // the near-field structure built on the device (near_struct_device)
+  struct NearBufs {
+    DBuf<uint64_t> k0, k1, rkeys, rkeys2, dk0, dk1, ck0, ck1;
+    DBuf<int64_t> tptr, mstart, mend, nfaces, fstart, dptr, v_rows, v_cols, v_mat, v_rng, v_nnz, nnzp;
+    DBuf<int32_t> tnodes, flags, at, tbox, rcnt, rat, rends, rends2, mcnt, uflag, uat, dofs, ccnt, cat, cti;
+    DBuf<double> lside, cnm;
+  } nbuf;
+  int64_t near_rows = 0;  // rows of the near-field blocks
   DBuf<double> V;            // Krylov basis
   int64_t v_cap = 0;
   int basis_n = -1;          // DOF count the basis is laid out for
@@ -899,6 +911,38 @@ int bits_for(uint64_t v) {
   return b;
 }
 
+// rocPRIM calls of the setup on the context's stream, with their temporary storage in sb.sort_tmp
+void dsort_keys(bemfmm_ctx* c, const uint64_t* in, uint64_t* out, int64_t n, int end_bit) {
+  if (n <= 0) return;
+  size_t bytes = 0;
+  HIPCHECK(dev::sort_keys_u64(c->stream, nullptr, bytes, in, out, n, 0, end_bit));
+  c->sb.sort_tmp.alloc(bytes);
+  HIPCHECK(dev::sort_keys_u64(c->stream, c->sb.sort_tmp.p, bytes, in, out, n, 0, end_bit));
+}
+void dscan_i32(bemfmm_ctx* c, bool inclusive, const int32_t* in, int32_t* out, int64_t n) {
+  if (n <= 0) return;
+  size_t bytes = 0;
+  if (inclusive) HIPCHECK(dev::scan_inclusive_i32(c->stream, nullptr, bytes, in, out, n));
+  else HIPCHECK(dev::scan_exclusive_i32(c->stream, nullptr, bytes, in, out, n));
+  c->sb.sort_tmp.alloc(bytes);
+  if (inclusive) HIPCHECK(dev::scan_inclusive_i32(c->stream, c->sb.sort_tmp.p, bytes, in, out, n));
+  else HIPCHECK(dev::scan_exclusive_i32(c->stream, c->sb.sort_tmp.p, bytes, in, out, n));
+}
+void dscan_i64(bemfmm_ctx* c, const int64_t* in, int64_t* out, int64_t n) {
+  if (n <= 0) return;
+  size_t bytes = 0;
+  HIPCHECK(dev::scan_exclusive_i64(c->stream, nullptr, bytes, in, out, n));
+  c->sb.sort_tmp.alloc(bytes);
+  HIPCHECK(dev::scan_exclusive_i64(c->stream, c->sb.sort_tmp.p, bytes, in, out, n));
+}
+template <typename T>
+T dread(bemfmm_ctx* c, const T* p) {
+  T v{};
+  HIPCHECK(hipMemcpyAsync(&v, p, sizeof(T), hipMemcpyDeviceToHost, c->stream));
+  HIPCHECK(hipStreamSynchronize(c->stream));
+  return v;
+}
+
 // The plan's tree stage on the device (tree.hip): level by level, the faces of every box that splits are keyed
 // by (box, cell, face) and radix-sorted, each distinct (box, cell) becomes a child box (in depth-first order),
 // and each face moves into its child if the child's inside test holds; then the faces sorted by path key give
@@ -1102,6 +1146,150 @@ void plan_tree_lists_device(bemfmm_ctx* c, Plan& pl) {
   pl.t_lists = now() - t1;
 }
 
+// The near-field blocks on the device, from the tree stage's device arrays: the layout plan_finish builds on the
+// host (one block per target box, split into chunks of <= 32 rows in descending cost; columns: the box's sorted
+// DOFs, its C L2P columns, zero-weight padding; the box's merged face ranges; the L2P coefficients per row),
+// written straight into the context's near-field buffers. Face DOFs are read in face_perm order (sb.fdof).
+void near_struct_device(bemfmm_ctx* c, Plan& pl) {
+  hipStream_t s = c->stream;
+  auto& T = c->tb;
+  auto& W = c->nbuf;
+  const int N = pl.n_nodes, nb = pl.n_boxes, C = pl.C, P = pl.P;
+  const double t0 = now();
+  // target nodes by box
+  W.k0.alloc(N);
+  W.k1.alloc(N);
+  dev::node_keys(s, N, T.target.p, W.k0.p);
+  dsort_keys(c, W.k0.p, W.k1.p, N, 32 + bits_for((uint64_t)nb));
+  W.tptr.alloc(nb + 1);
+  W.tnodes.alloc(N);
+  dev::csr_from_keys(s, nb, W.k1.p, N, W.tptr.p, W.tnodes.p);
+  // boxes with targets
+  W.flags.alloc(nb);
+  W.at.alloc(nb);
+  dev::ns_tbox_flags(s, nb, W.tptr.p, W.flags.p);
+  dscan_i32(c, true, W.flags.p, W.at.p, nb);
+  const int ntb = dread(c, W.at.p + (nb - 1));
+  W.tbox.alloc(ntb);
+  dev::ns_tbox_compact(s, nb, W.flags.p, W.at.p, W.tbox.p);
+  // merged face ranges per target box
+  W.rcnt.alloc(ntb + 1);
+  W.rat.alloc(ntb + 1);
+  dev::ns_range_count(s, ntb, W.tbox.p, T.parent.p, T.near_ptr.p, T.near_idx.p, T.nchild.p, W.rcnt.p);
+  dscan_i32(c, false, W.rcnt.p, W.rat.p, ntb + 1);
+  const int R = dread(c, W.rat.p + ntb);
+  for (auto* b : {&W.rkeys, &W.rkeys2}) b->alloc(R);
+  for (auto* b : {&W.rends, &W.rends2}) b->alloc(R);
+  dev::ns_range_write(s, ntb, W.tbox.p, T.parent.p, T.near_ptr.p, T.near_idx.p, T.nchild.p, T.range.p, W.rat.p,
+                      W.rkeys.p, W.rends.p);
+  {
+    size_t bytes = 0;
+    const int eb = 32 + bits_for((uint64_t)ntb);
+    HIPCHECK(dev::sort_pairs_u64_i32(s, nullptr, bytes, W.rkeys.p, W.rkeys2.p, W.rends.p, W.rends2.p, R, 0, eb));
+    c->sb.sort_tmp.alloc(bytes);
+    HIPCHECK(dev::sort_pairs_u64_i32(s, c->sb.sort_tmp.p, bytes, W.rkeys.p, W.rkeys2.p, W.rends.p, W.rends2.p, R, 0,
+                                     eb));
+  }
+  W.mstart.alloc(R);
+  W.mend.alloc(R);
+  W.mcnt.alloc(ntb + 1);
+  W.nfaces.alloc(ntb + 1);
+  dev::fill_i32(s, 1, T.err.p, 0);
+  dev::ns_range_merge(s, ntb, W.rat.p, W.rcnt.p, W.rkeys2.p, W.rends2.p, W.mstart.p, W.mend.p, W.mcnt.p, W.nfaces.p,
+                      T.err.p);
+  if (dread(c, T.err.p) & 4) throw std::runtime_error("overlapping near ranges");
+  W.fstart.alloc(ntb + 1);
+  dscan_i64(c, W.nfaces.p, W.fstart.p, ntb + 1);
+  const int64_t nfc = dread(c, W.fstart.p + ntb);
+  // DOF columns per target box: sorted, unique
+  W.dk0.alloc(3 * nfc);
+  W.dk1.alloc(3 * nfc);
+  W.uflag.alloc(3 * nfc);
+  W.uat.alloc(3 * nfc);
+  if (3 * nfc > INT32_MAX) throw std::runtime_error("near field too large for the device structure build");
+  dev::ns_dof_keys(s, nfc, ntb, W.fstart.p, W.rat.p, W.mcnt.p, W.mstart.p, W.mend.p, c->sb.fdof.p, W.dk0.p);
+  dsort_keys(c, W.dk0.p, W.dk1.p, 3 * nfc, 32 + bits_for((uint64_t)ntb));
+  dev::unique_flags(s, 3 * nfc, W.dk1.p, W.uflag.p);
+  dscan_i32(c, true, W.uflag.p, W.uat.p, 3 * nfc);
+  const int64_t nu = dread(c, W.uat.p + (3 * nfc - 1));
+  dev::compact_u64(s, 3 * nfc, W.dk1.p, W.uflag.p, W.uat.p, W.dk0.p);
+  W.dptr.alloc(ntb + 1);
+  W.dofs.alloc(nu);
+  dev::csr_from_keys(s, ntb, W.dk0.p, nu, W.dptr.p, W.dofs.p);
+  // chunks in descending cost
+  W.ccnt.alloc(ntb + 1);
+  W.cat.alloc(ntb + 1);
+  dev::ns_chunk_count(s, ntb, W.tptr.p, W.tbox.p, W.ccnt.p);
+  dscan_i32(c, false, W.ccnt.p, W.cat.p, ntb + 1);
+  const int nch = dread(c, W.cat.p + ntb);
+  if (nch >= (1 << 24)) throw std::runtime_error("too many near blocks for the device structure build");
+  W.cti.alloc(nch);
+  W.ck0.alloc(nch);
+  W.ck1.alloc(nch);
+  dev::ns_chunk_ti(s, ntb, W.cat.p, W.cti.p);
+  dev::fill_i32(s, 1, T.err.p, 0);
+  dev::ns_chunk_keys(s, ntb, W.tptr.p, W.tbox.p, W.cat.p, W.dptr.p, C, W.ck0.p, T.err.p);
+  dsort_keys(c, W.ck0.p, W.ck1.p, nch, 46);
+  if (dread(c, T.err.p) & 8) throw std::runtime_error("near block too large for the device structure build");
+  // block offsets
+  for (auto* b : {&W.v_rows, &W.v_cols, &W.v_mat, &W.v_rng, &W.v_nnz, &W.nnzp}) b->alloc(nch + 1);
+  c->near.row_ptr.alloc(nch + 1);
+  c->near.col_ptr.alloc(nch + 1);
+  c->near.mat_ptr.alloc(nch + 1);
+  c->sb.range_ptr.alloc(nch + 1);
+  dev::ns_chunk_sizes(s, nch, W.ck1.p, W.cti.p, W.cat.p, W.tptr.p, W.tbox.p, W.dptr.p, W.mcnt.p, C, W.v_rows.p,
+                      W.v_cols.p, W.v_mat.p, W.v_rng.p, W.v_nnz.p);
+  dscan_i64(c, W.v_rows.p, c->near.row_ptr.p, nch + 1);
+  dscan_i64(c, W.v_cols.p, c->near.col_ptr.p, nch + 1);
+  dscan_i64(c, W.v_mat.p, c->near.mat_ptr.p, nch + 1);
+  dscan_i64(c, W.v_rng.p, c->sb.range_ptr.p, nch + 1);
+  dscan_i64(c, W.v_nnz.p, W.nnzp.p, nch + 1);
+  const int64_t nrows = dread(c, c->near.row_ptr.p + nch), ncols = dread(c, c->near.col_ptr.p + nch);
+  const int64_t msize = dread(c, c->near.mat_ptr.p + nch), nrng = dread(c, c->sb.range_ptr.p + nch);
+  const int64_t nnz = dread(c, W.nnzp.p + nch);
+  if (nrows != N) throw std::runtime_error("internal: near rows do not cover the nodes");
+  c->near.row_out.alloc(nrows);
+  c->sb.row_block.alloc(nrows);
+  c->sb.node_row.alloc(N);
+  c->near.col_idx.alloc(ncols);
+  c->sb.ncol_dof.alloc(nch);
+  c->sb.ranges.alloc(2 * nrng);
+  c->sb.range_prefix.alloc(nrng);
+  dev::ns_fill(s, nch, W.ck1.p, W.cti.p, W.cat.p, W.tptr.p, W.tnodes.p, W.tbox.p, W.dptr.p, W.dofs.p, W.rat.p,
+               W.mcnt.p, W.mstart.p, W.mend.p, C, pl.n_dof, c->near.row_ptr.p, c->near.col_ptr.p, c->sb.range_ptr.p,
+               c->near.row_out.p, c->sb.row_block.p, c->sb.node_row.p, c->near.col_idx.p, c->sb.ncol_dof.p,
+               c->sb.ranges.p, c->sb.range_prefix.p);
+  HIPCHECK(hipGetLastError());
+  c->near_ncd.alloc(nch);
+  HIPCHECK(hipMemcpyAsync(c->near_ncd.p, c->sb.ncol_dof.p, nch * sizeof(int32_t), hipMemcpyDeviceToDevice, s));
+  c->near.nblk = nch;
+  // L2P coefficients per row
+  {
+    const int NT = (P + 1) * (P + 2) / 2;
+    std::vector<double> cnm(NT);
+    for (int n = 0; n <= P; ++n)
+      for (int m = 0; m <= n; ++m) {
+        double f1 = 1, f2 = 1;
+        for (int k = 2; k <= n - m; ++k) f1 *= k;
+        for (int k = 2; k <= n + m; ++k) f2 *= k;
+        const double v = std::sqrt(f1 * f2);
+        cnm[n * (n + 1) / 2 + m] = (m & 1) ? -v : v;
+      }
+    W.cnm.upload(cnm, s);
+    W.lside.upload(pl.level_side, s);
+  }
+  c->sb.l2p.alloc((size_t)nrows * C);
+  dev::l2p_coef(s, nrows, P, c->near.row_out.p, T.target.p, T.X.p, T.geom.p, T.level.p, W.lside.p, W.cnm.p,
+                c->sb.l2p.p);
+  HIPCHECK(hipGetLastError());
+  c->sb.ncol_dof.download(pl.near_ncol_dof, nch, s);
+  pl.near = bemfmm::BlockGemv();
+  pl.near.mat_size = msize;
+  pl.near_nnz = nnz;
+  c->near_rows = nrows;
+  pl.t_near_struct = now() - t0;
+}
+
 void do_prepare(bemfmm_ctx* c) {
   if (!c->have_geom) throw std::runtime_error("bemfmm_prepare: geometry not set");
   if (!c->have_bc) throw std::runtime_error("bemfmm_prepare: boundary values not set");
@@ -1160,9 +1348,11 @@ void do_prepare(bemfmm_ctx* c) {
   Plan& pl = *c->plan;
   bemfmm::plan_geometry(pl, o, c->n_nodes, c->X.data(), c->n_faces, c->tri.data(), c->face_dof_i.data(), c->n_dof,
                         c->kind_i.data(), c->point_i.data());
-  if (c->opts.tree_on_gpu) plan_tree_lists_device(c, pl);
+  // the tree stage and the near-field structure on the GPU (path keys use 5 bits per level of a 64-bit key)
+  const bool dev_plan = c->opts.tree_on_gpu && o.max_level <= 11;
+  if (dev_plan) plan_tree_lists_device(c, pl);
   else bemfmm::plan_tree_lists(pl);
-  bemfmm::plan_finish(pl);
+  bemfmm::plan_finish(pl, !dev_plan);
   c->C = pl.C;
   c->n_boxes = pl.n_boxes;
   c->p2m_rows_pad = pl.p2m_rows_pad;
@@ -1172,8 +1362,7 @@ void do_prepare(bemfmm_ctx* c) {
   hipStream_t s = c->stream;
   const double t1 = now();
   // ---- upload ----
-  c->near.upload(pl.near, s);
-  c->near.mat.alloc(pl.near.mat_size);
+  if (!dev_plan) c->near.upload(pl.near, s);
   c->p2m.upload(pl.p2m, s);
   c->p2m.mat.alloc(pl.p2m.mat_size);
   fill(c, c->p2m.mat.p, pl.p2m.mat_size, 0.);  // rows past C stay zero
@@ -1233,26 +1422,52 @@ void do_prepare(bemfmm_ctx* c) {
   auto &range_ptr = c->sb.range_ptr, &ranges = c->sb.ranges, &range_prefix = c->sb.range_prefix;
   auto &Xs = c->sb.Xs, &fX = c->sb.fX, &fcent = c->sb.fcent, &fcr = c->sb.fcr, &fJ = c->sb.fJ, &l2p = c->sb.l2p,
        &d_dun = c->sb.d_dun;
-  {
+  face_perm.upload(pl.face_perm, s);
+  // per-face arrays on the device in the depth-first face order (position in face_perm): the faces of a box
+  // and of a near range are consecutive, so the quadrature and P2M kernels read them coalesced
+  std::vector<int32_t> fv_p(3 * (size_t)c->n_faces), fdof_p(3 * (size_t)c->n_faces);
+  std::vector<double> fX_p(9 * (size_t)c->n_faces), fcent_p(3 * (size_t)c->n_faces), fcr_p(3 * (size_t)c->n_faces),
+      fJ_p(c->n_faces);
+  std::vector<int32_t> face_pos(c->n_faces);
+#pragma omp parallel for schedule(static)
+  for (int k = 0; k < c->n_faces; ++k) {
+    const size_t f = (size_t)pl.face_perm[k];
+    face_pos[f] = k;
+    for (int a = 0; a < 3; ++a) {
+      fv_p[3 * (size_t)k + a] = pl.fv[3 * f + a];
+      fdof_p[3 * (size_t)k + a] = pl.fdof[3 * f + a];
+      fcent_p[3 * (size_t)k + a] = pl.fcent[3 * f + a];
+      fcr_p[3 * (size_t)k + a] = pl.fcr[3 * f + a];
+    }
+    for (int a = 0; a < 9; ++a) fX_p[9 * (size_t)k + a] = pl.fX[9 * f + a];
+    fJ_p[k] = pl.fJ[f];
+  }
+  fv.upload(fv_p, s);
+  fdof.upload(fdof_p, s);
+  if (dev_plan) {
+    near_struct_device(c, pl);
+  } else {
     std::vector<int32_t> rb(pl.near.row_out.size());
     for (int b = 0; b < pl.near.n_blocks(); ++b)
       for (int64_t r = pl.near.row_ptr[b]; r < pl.near.row_ptr[b + 1]; ++r) rb[r] = b;
     row_block.upload(rb, s);
+    ncol_dof.upload(pl.near_ncol_dof, s);
+    c->near_ncd.upload(pl.near_ncol_dof, s);
+    range_ptr.upload(pl.near_range_ptr, s);
+    ranges.upload(pl.near_ranges, s);
+    range_prefix.upload(pl.near_range_prefix, s);
+    l2p.upload(pl.l2p_coef, s);
+    std::vector<int64_t> node_row(c->n_nodes, -1);
+    for (size_t r = 0; r < pl.near.row_out.size(); ++r) node_row[pl.near.row_out[r]] = (int64_t)r;
+    c->sb.node_row.upload(node_row, s);
+    c->near_rows = (int64_t)pl.near.row_out.size();
   }
-  ncol_dof.upload(pl.near_ncol_dof, s);
-  c->near_ncd.upload(pl.near_ncol_dof, s);
-  face_perm.upload(pl.face_perm, s);
-  fv.upload(pl.fv, s);
-  fdof.upload(pl.fdof, s);
-  range_ptr.upload(pl.near_range_ptr, s);
-  ranges.upload(pl.near_ranges, s);
-  range_prefix.upload(pl.near_range_prefix, s);
+  c->near.mat.alloc(pl.near.mat_size);
   Xs.upload(pl.Xs, s);
-  fX.upload(pl.fX, s);
-  fcent.upload(pl.fcent, s);
-  fcr.upload(pl.fcr, s);
-  fJ.upload(pl.fJ, s);
-  l2p.upload(pl.l2p_coef, s);
+  fX.upload(fX_p, s);
+  fcent.upload(fcent_p, s);
+  fcr.upload(fcr_p, s);
+  fJ.upload(fJ_p, s);
   {
     double dun[7][4] = {};
     static const double d1[1][4] = {{1. / 3., 1. / 3., 1. / 3., 0.5}};
@@ -1293,7 +1508,11 @@ void do_prepare(bemfmm_ctx* c) {
     auto &pfaces = c->sb.pfaces, &plc = c->sb.plc;
     auto &pcs = c->sb.pcs, &dcnm = c->sb.dcnm;
     fptr.upload(pl.p2m_face_ptr, s);
-    pfaces.upload(pl.p2m_faces, s);
+    {
+      std::vector<int32_t> pf(pl.p2m_faces.size());
+      for (size_t k = 0; k < pf.size(); ++k) pf[k] = face_pos[pl.p2m_faces[k]];
+      pfaces.upload(pf, s);
+    }
     plc.upload(pl.p2m_face_lc, s);
     pcs.upload(pl.p2m_leaf_cs, s);
     dcnm.upload(cnm, s);
@@ -1319,11 +1538,12 @@ void do_prepare(bemfmm_ctx* c) {
   const double t2 = now();
   // ---- near-field quadrature on the GPU ----
   fill(c, c->near.mat.p, pl.near.mat_size, 0.);
-  const int64_t nrows = (int64_t)pl.near.row_out.size();
+  const int64_t nrows = c->near_rows;
   if (nrows > 0) {
     dev::NearQuad a;
     a.C = pl.C;
     a.q = o.q;
+    for (int32_t v : pl.near_ncol_dof) a.max_ncd = std::max(a.max_ncd, (int)v);
     a.near_region = pl.near_region;
     a.row_block = row_block.p;
     a.ncol_dof = ncol_dof.p;
@@ -1413,13 +1633,7 @@ void do_prepare(bemfmm_ctx* c) {
     c->sw_failed = 0;
     for (int32_t f : fh) c->sw_failed += f;
   } else if (pc == BEMFMM_PRECOND_JACOBI || pc == BEMFMM_PRECOND_SCHWARZ) {
-    std::vector<int64_t> node_row(c->n_nodes, -1);
-    for (size_t r = 0; r < pl.near.row_out.size(); ++r) node_row[pl.near.row_out[r]] = (int64_t)r;
-    for (int i = 0; i < c->n_dof; ++i)
-      if (c->kind_i[i] != 2 && node_row[c->point_i[i]] < 0)
-        throw std::runtime_error("preconditioner: a DOF's point has no near-field row");
-    auto& d_node_row = c->sb.node_row;
-    d_node_row.upload(node_row, s);
+    auto& d_node_row = c->sb.node_row;  // every node is the target of one near row (node -> row)
     dev::NearRows nr;
     nr.kind = c->kind.p;
     nr.kappa = c->robin.p;
@@ -1445,9 +1659,28 @@ void do_prepare(bemfmm_ctx* c) {
         std::stable_sort(seed.begin(), seed.end(), [&](int32_t a, int32_t b) {
           return pl.target_box[c->point_i[a]] < pl.target_box[c->point_i[b]];
         });
-      bemfmm::build_schwarz_blocks(c->n_dof, c->n_faces, c->face_dof_i.data(), c->point_i.data(), seed,
-                                   c->opts.schwarz_max_core, c->opts.schwarz_max_block, c->opts.schwarz_overlap,
-                                   c->sw_host);
+      uint64_t key = 0xcbf29ce484222325ull;
+      auto mix = [&](uint64_t v) { key = (key ^ v) * 0x100000001b3ull; };
+      mix((uint64_t)c->n_dof);
+      mix((uint64_t)c->n_faces);
+      for (int32_t v : c->face_dof) mix((uint32_t)v);
+      for (int32_t v : c->dof_point) mix((uint32_t)v);
+      mix((uint64_t)c->opts.schwarz_max_core);
+      mix((uint64_t)c->opts.schwarz_max_block);
+      mix((uint64_t)c->opts.schwarz_overlap);
+      mix((uint64_t)c->opts.dof_reorder);
+      if (key == c->sw_topo_key && c->sw_caller.n_blocks() > 0) {
+        // same DOF graph: the blocks of the first prepare, in this prepare's internal numbering
+        c->sw_host = c->sw_caller;
+        for (int32_t& g : c->sw_host.idx) g = c->inv[g];
+      } else {
+        bemfmm::build_schwarz_blocks(c->n_dof, c->n_faces, c->face_dof_i.data(), c->point_i.data(), seed,
+                                     c->opts.schwarz_max_core, c->opts.schwarz_max_block, c->opts.schwarz_overlap,
+                                     c->sw_host);
+        c->sw_caller = c->sw_host;
+        for (int32_t& g : c->sw_caller.idx) g = c->perm[g];
+        c->sw_topo_key = key;
+      }
       const bemfmm::SchwarzBlocks& B = c->sw_host;
       const int nb = B.n_blocks();
       if (B.max_block > kSchwarzMaxBlock) throw std::runtime_error("schwarz: block larger than the kernel's limit");

@@ -1037,7 +1037,10 @@ __global__ __launch_bounds__(256) void k_near_quad(
     const double* __restrict__ Xs, const double* __restrict__ fX, const double* __restrict__ fcent,
     const double* __restrict__ fcr, const double* __restrict__ fJ, const int32_t* __restrict__ fv,
     const int32_t* __restrict__ fdof, const double* __restrict__ l2p, const double* __restrict__ dun_rule,
-    double* __restrict__ mat) {
+    int lds_cols, double* __restrict__ mat) {
+  // a row of at most lds_cols DOF columns accumulates in LDS (ds_add_f64) and is written once; a longer one
+  // adds into the zero-filled matrix with global atomics
+  extern __shared__ __align__(16) double acc[];  // 2 * lds_cols
   // Dunavant rule of the non-adjacent, non-near faces ({b0,b1,b2,w} x q), per context
   __shared__ double dun[7][4];
   for (int i = threadIdx.x; i < 28; i += blockDim.x) dun[i / 4][i % 4] = dun_rule[i];
@@ -1054,6 +1057,13 @@ __global__ __launch_bounds__(256) void k_near_quad(
     A0[ncd + c] = l2p[r * C + c];
     A1[ncd + c] = l2p[r * C + c];
   }
+  const bool in_lds = ncd <= lds_cols;
+  double* S0 = in_lds ? acc : A0;
+  double* S1 = in_lds ? acc + ncd : A1;
+  if (in_lds) {
+    for (int c = threadIdx.x; c < 2 * ncd; c += blockDim.x) acc[c] = 0.;
+    __syncthreads();
+  }
   const double x = Xs[3 * t], y = Xs[3 * t + 1], z = Xs[3 * t + 2];
   const int64_t g0 = range_ptr[b], g1 = range_ptr[b + 1];
   const int64_t nf = range_prefix[g1 - 1] + (ranges[2 * (g1 - 1) + 1] - ranges[2 * (g1 - 1)]);
@@ -1066,12 +1076,12 @@ __global__ __launch_bounds__(256) void k_near_quad(
       if (range_prefix[mid] <= i) lo = mid;
       else hi = mid - 1;
     }
-    const int f = face_perm[ranges[2 * lo] + (i - range_prefix[lo])];
+    const int64_t f = ranges[2 * lo] + (i - range_prefix[lo]);  // face arrays are in face_perm order
     int slot = -1;
     for (int k = 0; k < 3; ++k)
       if (fv[3 * f + k] == t) slot = k;
     double WG[3], WGn[3];
-    bemfmm::face_weights(fX + 9 * (int64_t)f, fJ[f], fcr + 3 * (int64_t)f, fcent + 3 * (int64_t)f, slot, x, y, z,
+    bemfmm::face_weights(fX + 9 * f, fJ[f], fcr + 3 * f, fcent + 3 * f, slot, x, y, z,
                          near_region, c_gw5, dun, q, WG, WGn);
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
@@ -1082,8 +1092,15 @@ __global__ __launch_bounds__(256) void k_near_quad(
         if (cols[mid] < d) lo2 = mid + 1;
         else hi2 = mid;
       }
-      atomicAdd(&A0[lo2], WG[k]);
-      atomicAdd(&A1[lo2], WGn[k]);
+      atomicAdd(&S0[lo2], WG[k]);
+      atomicAdd(&S1[lo2], WGn[k]);
+    }
+  }
+  if (in_lds) {
+    __syncthreads();
+    for (int c = threadIdx.x; c < ncd; c += blockDim.x) {
+      A0[c] = acc[c];
+      A1[c] = acc[ncd + c];
     }
   }
 }
@@ -1155,9 +1172,45 @@ __device__ double near_entry(int i, int j, const int32_t* __restrict__ kind, con
   return v;
 }
 
-// entry sources of the block build: the resident near field, or the rows of the reference's sparse matrix
+// entry sources of the block build: the resident near field, or the rows of the reference's sparse matrix.
+// row(i) resolves once what every entry of row i needs (its sorted column list and its values), col(r, k) is
+// the k-th column of that list and val(r, k, i, j) the operator entry when column j sits at position k; the
+// block build searches the columns of several rows at once, so that their loads overlap.
+struct SrcRow {
+  int64_t base = 0;  // start of the sorted column list
+  int64_t aux = 0;   // start of the row's values
+  int n = 0;         // columns in the list; -1: identity row (CORNER)
+  int ncol = 0;      // near field: stride between the G and Gn rows
+};
 struct NearSrc {
   NearRows a;
+  __device__ SrcRow row(int i) const {
+    SrcRow r;
+    if (a.kind[i] == 2) {
+      r.n = -1;
+      return r;
+    }
+    const int64_t q = a.node_row[a.point[i]];
+    const int b = a.row_block[q];
+    r.base = a.col_ptr[b];
+    r.ncol = (int)(a.col_ptr[b + 1] - r.base);
+    r.n = a.ncol_dof[b];
+    r.aux = a.mat_ptr[b] + (q - a.row_ptr[b]) * 2 * r.ncol;
+    return r;
+  }
+  __device__ int col(const SrcRow& r, int k) const { return a.col_idx[r.base + k]; }
+  __device__ double val(const SrcRow& r, int k, int i, int j) const {
+    const double g = a.mat[r.aux + k];
+    double v = (a.kind[j] == 0) ? g : -a.mat[r.aux + r.ncol + k];
+    if (a.kind[j] == 3) v += a.kappa[j] * g;
+    (void)i;
+    return v;
+  }
+  // Neumann-type diagonals carry -alpha (also where the column search found nothing)
+  __device__ double diag_shift(int i) const {
+    const int ki = a.kind[i];
+    return (ki == 1 || ki == 3) ? -a.alpha[i] : 0.0;
+  }
   __device__ double operator()(int i, int j) const {
     return near_entry(i, j, a.kind, a.kappa, a.point, a.node_row, a.row_block, a.row_ptr, a.col_ptr, a.mat_ptr,
                       a.col_idx, a.ncol_dof, a.mat, a.alpha);
@@ -1165,6 +1218,16 @@ struct NearSrc {
 };
 struct RowsSrc {
   IluRows A;
+  __device__ SrcRow row(int i) const {
+    SrcRow r;
+    r.base = A.ptr[i];
+    r.aux = r.base;
+    r.n = (int)(A.ptr[i + 1] - r.base);
+    return r;
+  }
+  __device__ int col(const SrcRow& r, int k) const { return A.col[r.base + k]; }
+  __device__ double val(const SrcRow& r, int k, int, int) const { return A.val[r.aux + k]; }
+  __device__ double diag_shift(int) const { return 0.0; }
   __device__ double operator()(int i, int j) const {
     int64_t lo = A.ptr[i], hi = A.ptr[i + 1];  // columns ascending: first column >= j
     while (lo < hi) {
@@ -1180,11 +1243,17 @@ struct RowsSrc {
 // elimination with partial pivoting, and store the core rows of A_b^-1 (ncore x m, row-major).
 // A block with a vanishing or non-finite pivot falls back to the inverse diagonal on its core
 // (as SchwarzAdditive_CRS does when its block LU fails) and is flagged in failed[b].
-// LDS: m*m doubles (matrix) + m doubles (pivot column) + m ints (DOF ids) + m ints (pivot rows).
+// LDS: m*m doubles (matrix) + m doubles (pivot column) + m ints (DOF ids) + m ints (pivot rows) + m row
+// references. The gather takes one column per thread and searches it in kGatherRows rows at once (their
+// dependent loads overlap instead of following each other).
 // rows_out: the stored rows per block (the core for the restricted combination, all m for the averaged one).
-static_assert(kSchwarzMaxBlock <= 128, "k_schwarz_build updates one column per thread, two rows per pass of 256 threads");
+// 1024 threads (16 waves) per workgroup: the block's LDS allows one workgroup per CU, and the gather's loads and
+// the elimination's LDS reads need that many waves in flight (256 threads: 140 ms for the 28,632 blocks of the
+// 209k tank, latency-bound).
+constexpr int kSchwarzThreads = 1024;
+static_assert(kSchwarzMaxBlock <= 128, "k_schwarz_build updates one column per thread, 8 rows per pass");
 template <class Src>
-__global__ __launch_bounds__(256) void k_schwarz_build(
+__global__ __launch_bounds__(kSchwarzThreads) void k_schwarz_build(
     const int64_t* __restrict__ bptr, const int32_t* __restrict__ bidx, const int32_t* __restrict__ rows_out,
     const int64_t* __restrict__ inv_ptr, Src entry, double* __restrict__ inv, int32_t* __restrict__ failed) {
   extern __shared__ __align__(16) unsigned char smem[];
@@ -1196,12 +1265,57 @@ __global__ __launch_bounds__(256) void k_schwarz_build(
   double* colk = A + m * m;                      // m
   int* ids = reinterpret_cast<int*>(colk + m);   // m
   int* piv = ids + m;                            // m
+  SrcRow* rows = reinterpret_cast<SrcRow*>(piv + m + (m & 1));  // m, 8-byte aligned
   __shared__ int bad;
   const int tid = threadIdx.x;
   if (tid == 0) bad = 0;
   for (int t = tid; t < m; t += blockDim.x) ids[t] = bidx[p0 + t];
   __syncthreads();
-  for (int e = tid; e < m * m; e += blockDim.x) A[e] = entry(ids[e / m], ids[e % m]);
+  for (int t = tid; t < m; t += blockDim.x) rows[t] = entry.row(ids[t]);
+  __syncthreads();
+  {
+    constexpr int kGatherRows = 8;
+    const int j = tid & 127;
+    if (j < m) {
+      const int dj = ids[j];
+      constexpr int kRowGroups = kSchwarzThreads / 128;
+      for (int i0 = tid >> 7; i0 < m; i0 += kRowGroups * kGatherRows) {
+        int lo[kGatherRows], hi[kGatherRows];
+#pragma unroll
+        for (int u = 0; u < kGatherRows; ++u) {
+          const int i = i0 + kRowGroups * u;
+          lo[u] = 0;
+          hi[u] = (i < m && rows[i].n > 0) ? rows[i].n - 1 : 0;
+        }
+        bool active = true;
+        while (active) {
+          active = false;
+#pragma unroll
+          for (int u = 0; u < kGatherRows; ++u)
+            if (lo[u] < hi[u]) {
+              const int mid = (lo[u] + hi[u]) >> 1;
+              if (entry.col(rows[i0 + kRowGroups * u], mid) < dj) lo[u] = mid + 1;
+              else hi[u] = mid;
+              active = true;
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kGatherRows; ++u) {
+          const int i = i0 + kRowGroups * u;
+          if (i >= m) continue;
+          const SrcRow& r = rows[i];
+          double v;
+          if (r.n < 0) {
+            v = (i == j) ? 1.0 : 0.0;
+          } else {
+            v = (r.n > 0 && entry.col(r, lo[u]) == dj) ? entry.val(r, lo[u], ids[i], dj) : 0.0;
+            if (i == j) v += entry.diag_shift(ids[i]);
+          }
+          A[i * m + j] = v;
+        }
+      }
+    }
+  }
   __syncthreads();
   for (int k = 0; k < m; ++k) {
     if (tid < 64) {
@@ -1244,12 +1358,12 @@ __global__ __launch_bounds__(256) void k_schwarz_build(
     __syncthreads();
     for (int j = tid; j < m; j += blockDim.x) A[k * m + j] = (j == k) ? ipv : A[k * m + j] * ipv;
     __syncthreads();
-    {  // one column per thread, two rows per pass (m <= 128 = blockDim.x / 2): consecutive threads take
+    {  // one column per thread, 8 rows per pass (m <= 128 = blockDim.x / 8): consecutive threads take
        // consecutive columns of a row (no bank conflicts) and no thread divides by m
       const int j = tid & 127;
       if (j < m) {
         const double akj = A[k * m + j];  // the scaled pivot row (row k is not updated here)
-        for (int i = tid >> 7; i < m; i += 2) {
+        for (int i = tid >> 7; i < m; i += kSchwarzThreads / 128) {
           if (i == k) continue;
           A[i * m + j] = (j == k) ? -colk[i] * ipv : A[i * m + j] - colk[i] * akj;
         }
@@ -1631,9 +1745,12 @@ void p2m_build(hipStream_t s, int P, const P2MBuild& a, double* mat) {
 
 void near_quad(hipStream_t s, int64_t nrows, const NearQuad& a, const Gemv& g, double* mat) {
   if (nrows == 0) return;
-  hipLaunchKernelGGL(k_near_quad, dim3((unsigned)nrows), dim3(256), 0, s, a.C, a.q, a.near_region, a.row_block,
-                     g.row_ptr, g.col_ptr, g.mat_ptr, g.row_out, g.col_idx, a.ncol_dof, a.range_ptr, a.ranges,
-                     a.range_prefix, a.face_perm, a.Xs, a.fX, a.fcent, a.fcr, a.fJ, a.fv, a.fdof, a.l2p, a.dun_rule, mat);
+  // rows of up to kNearQuadLdsCols DOF columns (most of them) accumulate in 64 KB of LDS: two workgroups per CU
+  const int lds_cols = std::min(a.max_ncd, kNearQuadLdsCols);
+  hipLaunchKernelGGL(k_near_quad, dim3((unsigned)nrows), dim3(256), (size_t)2 * lds_cols * sizeof(double), s, a.C,
+                     a.q, a.near_region, a.row_block, g.row_ptr, g.col_ptr, g.mat_ptr, g.row_out, g.col_idx,
+                     a.ncol_dof, a.range_ptr, a.ranges, a.range_prefix, a.face_perm, a.Xs, a.fX, a.fcent, a.fcr, a.fJ,
+                     a.fv, a.fdof, a.l2p, a.dun_rule, lds_cols, mat);
 }
 
 void to_f32(hipStream_t s, int64_t n, const double* a, float* b) {
@@ -1645,17 +1762,22 @@ void diag_inv(hipStream_t s, int n, const NearRows& a, double* dinv) {
                      a.row_ptr, a.col_ptr, a.mat_ptr, a.col_idx, a.ncol_dof, a.mat, a.alpha, dinv);
 }
 
+static size_t schwarz_build_lds(int m) {
+  return (size_t)m * m * sizeof(double) + (size_t)m * sizeof(double) + (size_t)(2 * m + (m & 1)) * sizeof(int) +
+         (size_t)m * sizeof(SrcRow);
+}
+
 void schwarz_build(hipStream_t s, const Schwarz& b, int max_block, const NearRows& a, double* inv, int32_t* failed) {
   if (b.nblk == 0) return;
-  const size_t lds = (size_t)max_block * max_block * sizeof(double) + (size_t)max_block * 16;
-  hipLaunchKernelGGL(k_schwarz_build<NearSrc>, dim3(b.nblk), dim3(256), lds, s, b.bptr, b.bidx, b.bncore, b.inv_ptr,
+  const size_t lds = schwarz_build_lds(max_block);
+  hipLaunchKernelGGL(k_schwarz_build<NearSrc>, dim3(b.nblk), dim3(kSchwarzThreads), lds, s, b.bptr, b.bidx, b.bncore, b.inv_ptr,
                      NearSrc{a}, inv, failed);
 }
 void schwarz_build_rows(hipStream_t s, const Schwarz& b, int max_block, const IluRows& A, const int32_t* rows_out,
                         double* inv, int32_t* failed) {
   if (b.nblk == 0) return;
-  const size_t lds = (size_t)max_block * max_block * sizeof(double) + (size_t)max_block * 16;
-  hipLaunchKernelGGL(k_schwarz_build<RowsSrc>, dim3(b.nblk), dim3(256), lds, s, b.bptr, b.bidx, rows_out, b.inv_ptr,
+  const size_t lds = schwarz_build_lds(max_block);
+  hipLaunchKernelGGL(k_schwarz_build<RowsSrc>, dim3(b.nblk), dim3(kSchwarzThreads), lds, s, b.bptr, b.bidx, rows_out, b.inv_ptr,
                      RowsSrc{A}, inv, failed);
 }
 void schwarz_apply_averaged(hipStream_t s, const Schwarz& b, int n, const int64_t* occ_ptr, const int64_t* occ_pos,

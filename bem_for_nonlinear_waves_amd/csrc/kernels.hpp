@@ -109,8 +109,10 @@ struct Tiles {
   int tp = kTP;  // pairs per tile
 };
 // Near-field quadrature inputs (k_near_quad)
+constexpr int kNearQuadLdsCols = 4096;  // DOF columns of a near row accumulated in LDS (2 x 32 KB)
 struct NearQuad {
   int C = 0, q = 0;
+  int max_ncd = 0;  // largest DOF column count of a near block
   double near_region = 0;
   const int32_t* row_block = nullptr;
   const int32_t* ncol_dof = nullptr;
@@ -249,7 +251,47 @@ void list_totals(hipStream_t s, int np, const int32_t* nearf, const int32_t* m2l
                  const int32_t* near_at, const int32_t* m2l_at, const int32_t* next_at, int64_t* out);
 // sorted keys A << 32 | B -> CSR over A in [0, nb]: ptr (nb + 1), idx = B
 void csr_from_keys(hipStream_t s, int nb, const uint64_t* keys, int64_t n, int64_t* ptr, int32_t* idx);
+// ---- near-field blocks on the device (tree.hip), the plan's near structure from the tree stage ----
+// target nodes grouped by box (node_keys, then a sort and csr_from_keys); boxes with targets (flags, scan,
+// compact); per target box its ranges into the face order (own box, near boxes, near leaves of ancestors:
+// count, scan, write, sort, merge in place; err |= 4 on overlapping ranges), then its DOF columns (keys per face
+// corner, sorted, unique, CSR); chunks of <= kNearRows rows in descending cost (err |= 8 if a chunk's cost
+// reaches 2^22); their sizes scanned into the block offsets; fill; the L2P coefficients per row.
+constexpr int kNearRows = 32;
+void node_keys(hipStream_t s, int N, const int32_t* target, uint64_t* keys);
+void ns_tbox_flags(hipStream_t s, int nb, const int64_t* tptr, int32_t* flags);
+void ns_tbox_compact(hipStream_t s, int nb, const int32_t* flags, const int32_t* at, int32_t* tbox);
+void ns_chunk_count(hipStream_t s, int ntb, const int64_t* tptr, const int32_t* tbox, int32_t* cnt);
+// (ns_range_count, ns_range_merge and ns_chunk_count also write entry ntb: count 0, for the scans' totals)
+void ns_range_count(hipStream_t s, int ntb, const int32_t* tbox, const int32_t* parent, const int64_t* near_ptr,
+                    const int32_t* near_idx, const int32_t* nchild, int32_t* cnt);
+void ns_range_write(hipStream_t s, int ntb, const int32_t* tbox, const int32_t* parent, const int64_t* near_ptr,
+                    const int32_t* near_idx, const int32_t* nchild, const int64_t* box_range, const int32_t* at,
+                    uint64_t* keys, int32_t* ends);
+void ns_range_merge(hipStream_t s, int ntb, const int32_t* at, const int32_t* cnt, const uint64_t* keys,
+                    const int32_t* ends, int64_t* mstart, int64_t* mend, int32_t* mcnt, int64_t* nfaces, int32_t* err);
+void ns_dof_keys(hipStream_t s, int64_t total, int ntb, const int64_t* fstart, const int32_t* at, const int32_t* mcnt,
+                 const int64_t* mstart, const int64_t* mend, const int32_t* fdof, uint64_t* keys);
+void unique_flags(hipStream_t s, int64_t n, const uint64_t* keys, int32_t* flags);
+void compact_u64(hipStream_t s, int64_t n, const uint64_t* keys, const int32_t* flags, const int32_t* at,
+                 uint64_t* out);
+void ns_chunk_keys(hipStream_t s, int ntb, const int64_t* tptr, const int32_t* tbox, const int32_t* chunk_at,
+                   const int64_t* dptr, int C, uint64_t* keys, int32_t* err);
+void ns_chunk_ti(hipStream_t s, int ntb, const int32_t* chunk_at, int32_t* chunk_ti);
+void ns_chunk_sizes(hipStream_t s, int nch, const uint64_t* order, const int32_t* chunk_ti, const int32_t* chunk_at,
+                    const int64_t* tptr, const int32_t* tbox, const int64_t* dptr, const int32_t* mcnt, int C,
+                    int64_t* rows_v, int64_t* cols_v, int64_t* mat_v, int64_t* rng_v, int64_t* nnz_v);
+void ns_fill(hipStream_t s, int nch, const uint64_t* order, const int32_t* chunk_ti, const int32_t* chunk_at,
+             const int64_t* tptr, const int32_t* tnodes, const int32_t* tbox, const int64_t* dptr, const int32_t* dofs,
+             const int32_t* rat, const int32_t* mcnt, const int64_t* mstart, const int64_t* mend, int C, int n_dof,
+             const int64_t* row_ptr, const int64_t* col_ptr, const int64_t* range_ptr, int32_t* row_out,
+             int32_t* row_block, int64_t* node_row, int32_t* col_idx, int32_t* ncol_dof, int64_t* ranges,
+             int64_t* range_prefix);
+void l2p_coef(hipStream_t s, int64_t nrows, int P, const int32_t* row_out, const int32_t* target, const double* X,
+              const BoxGeom* geom, const int32_t* level, const double* level_side, const double* cnm, double* out);
 // rocPRIM radix sorts and scans; tmp == nullptr: only sets tmp_bytes
+hipError_t scan_exclusive_i64(hipStream_t s, void* tmp, size_t& tmp_bytes, const int64_t* in, int64_t* out,
+                              int64_t n);
 hipError_t sort_keys_u64(hipStream_t s, void* tmp, size_t& tmp_bytes, const uint64_t* in, uint64_t* out, int64_t n,
                          int begin_bit, int end_bit);
 hipError_t sort_pairs_u64_i32(hipStream_t s, void* tmp, size_t& tmp_bytes, const uint64_t* keys_in,

@@ -449,7 +449,7 @@ void build_plan(Plan& pl, const Options& opt, int n_nodes, const double* X, int 
   plan_finish(pl);
 }
 
-void plan_finish(Plan& pl) {
+void plan_finish(Plan& pl, bool near_on_host) {
   const Options& opt = pl.opt;
   const int P = pl.P, C = pl.C, n_nodes = pl.n_nodes, n_dof = pl.n_dof, nb = pl.n_boxes;
   const int32_t* face_dof = pl.fdof.data();
@@ -457,10 +457,10 @@ void plan_finish(Plan& pl) {
   pl.level_side.resize(pl.n_levels);
   for (int l = 0; l < pl.n_levels; ++l) pl.level_side[l] = pl.root_side / std::ldexp(1.0, l);
   const double t2 = now();
-  std::vector<std::vector<int>> box_targets(nb);
-  for (int i = 0; i < n_nodes; ++i) box_targets[pl.target_box[i]].push_back(i);
   // --- near blocks: one per target box (split into <= kMaxRows rows) ---
-  {
+  if (near_on_host) {
+    std::vector<std::vector<int>> box_targets(nb);
+    for (int i = 0; i < n_nodes; ++i) box_targets[pl.target_box[i]].push_back(i);
     constexpr int kMaxRows = 32;
     std::vector<int> tboxes;
     for (int b = 0; b < nb; ++b)
@@ -613,7 +613,7 @@ void plan_finish(Plan& pl) {
       }
     }
   }
-  pl.t_near_struct = now() - t2;
+  if (near_on_host) pl.t_near_struct = now() - t2;
 
   // --- P2M blocks: one per leaf; M~_n^m = sum W b_k (phi_n, phi) (conj R_n^m(x~), conj dR_n^m.n/s) ---
   const double t3 = now();

@@ -121,8 +121,9 @@ void plan_geometry(Plan& pl, const Options& opt, int n_nodes, const double* X, i
 // the octree, its interaction lists and the target box of every node, on the host (sets n_levels .. m2l_idx,
 // target_box, n_leaves)
 void plan_tree_lists(Plan& pl);
-// near blocks, L2P coefficients, P2M blocks and the translation groups from the tree arrays
-void plan_finish(Plan& pl);
+// near blocks and L2P coefficients (unless near_on_host is false: the device builds them), P2M blocks and the
+// translation groups from the tree arrays
+void plan_finish(Plan& pl, bool near_on_host = true);
 
 // Real-field packed translation operators (operators.cpp), computed in __float128
 // and rounded once to double. Layout: out[k_in * C + r_out].

@@ -110,15 +110,32 @@ __global__ void k_tree_assign(int n, const uint64_t* __restrict__ keys, const in
                               const double* __restrict__ pole, const BoxGeom* __restrict__ geom,
                               int32_t* __restrict__ fbox, uint64_t* __restrict__ fkey, int32_t* count) {
   const int i = blockIdx.x * kTB + threadIdx.x;
-  if (i >= n) return;
-  const int f = (int)(uint32_t)keys[i];
-  const int cell = (int)((uint32_t)(keys[i] >> 32) % 32u);
-  const int c = le + ci[i] - 1;
-  const double x[3] = {pole[3 * f], pole[3 * f + 1], pole[3 * f + 2]};
-  if (box_inside(geom[c], x)) {
-    fbox[f] = c;
-    fkey[f] |= (uint64_t)(cell + 1);
-    atomicAdd(&count[c], 1);
+  int c = -1;
+  bool ins = false;
+  if (i < n) {
+    const int f = (int)(uint32_t)keys[i];
+    const int cell = (int)((uint32_t)(keys[i] >> 32) % 32u);
+    c = le + ci[i] - 1;
+    const double x[3] = {pole[3 * f], pole[3 * f + 1], pole[3 * f + 2]};
+    ins = box_inside(geom[c], x);
+    if (ins) {
+      fbox[f] = c;
+      fkey[f] |= (uint64_t)(cell + 1);
+    }
+  }
+  // the faces of one child are consecutive (sorted keys): the last lane of each run in the wave adds the run's
+  // count, one atomic per run instead of one per face
+  const int lane = __lane_id();
+  const uint64_t insm = __ballot(ins);
+  const int cprev = __shfl_up(c, 1), cnext = __shfl_down(c, 1);
+  const bool head = lane == 0 || cprev != c;
+  const bool tail = lane == 63 || cnext != c;
+  const uint64_t heads = __ballot(head);
+  if (tail && c >= 0) {
+    const uint64_t upto = (2ull << lane) - 1ull;  // lanes 0 .. lane (all ones for lane 63)
+    const int start = 63 - __clzll(heads & upto);
+    const int cnt = __popcll(insm & upto & ~((1ull << start) - 1ull));
+    if (cnt) atomicAdd(&count[c], cnt);
   }
 }
 
@@ -250,6 +267,287 @@ __global__ void k_csr_from_keys(int nb, const uint64_t* __restrict__ keys, int64
   if (i < n) idx[i] = (int32_t)(uint32_t)keys[i];
 }
 
+__global__ void k_node_keys(int N, const int32_t* __restrict__ target, uint64_t* __restrict__ keys) {
+  const int i = blockIdx.x * kTB + threadIdx.x;
+  if (i < N) keys[i] = ((uint64_t)(uint32_t)target[i] << 32) | (uint32_t)i;
+}
+
+// boxes holding target nodes, in box order: flags for the scan, then the compacted list
+__global__ void k_ns_tbox_flags(int nb, const int64_t* __restrict__ tptr, int32_t* __restrict__ flags) {
+  const int b = blockIdx.x * kTB + threadIdx.x;
+  if (b < nb) flags[b] = tptr[b + 1] > tptr[b] ? 1 : 0;
+}
+__global__ void k_ns_tbox_compact(int nb, const int32_t* __restrict__ flags, const int32_t* __restrict__ at,
+                                  int32_t* __restrict__ tbox) {
+  const int b = blockIdx.x * kTB + threadIdx.x;
+  if (b < nb && flags[b]) tbox[at[b] - 1] = b;
+}
+// chunks of <= kNearRows rows per target box (entry ntb: 0, so that the exclusive scan ends with the total)
+__global__ void k_ns_chunk_count(int ntb, const int64_t* __restrict__ tptr, const int32_t* __restrict__ tbox,
+                                 int32_t* __restrict__ cnt) {
+  const int ti = blockIdx.x * kTB + threadIdx.x;
+  if (ti > ntb) return;
+  cnt[ti] = ti == ntb ? 0 : (int32_t)((tptr[tbox[ti] + 1] - tptr[tbox[ti]] + kNearRows - 1) / kNearRows);
+}
+
+// ---- near-field blocks (plan.cpp plan_finish, "near blocks") ----
+// ranges per target box ti (box b): b itself, b's near boxes, and the near leaves of b's ancestors
+__global__ void k_ns_range_count(int ntb, const int32_t* __restrict__ tbox, const int32_t* __restrict__ parent,
+                                 const int64_t* __restrict__ near_ptr, const int32_t* __restrict__ near_idx,
+                                 const int32_t* __restrict__ nchild, int32_t* __restrict__ cnt) {
+  const int ti = blockIdx.x * kTB + threadIdx.x;
+  if (ti > ntb) return;
+  if (ti == ntb) {  // 0, so that the exclusive scan ends with the total
+    cnt[ti] = 0;
+    return;
+  }
+  const int b = tbox[ti];
+  int n = 1 + (int)(near_ptr[b + 1] - near_ptr[b]);
+  for (int a = parent[b]; a >= 0; a = parent[a])
+    for (int64_t k = near_ptr[a]; k < near_ptr[a + 1]; ++k) n += nchild[near_idx[k]] == 0;
+  cnt[ti] = n;
+}
+
+__global__ void k_ns_range_write(int ntb, const int32_t* __restrict__ tbox, const int32_t* __restrict__ parent,
+                                 const int64_t* __restrict__ near_ptr, const int32_t* __restrict__ near_idx,
+                                 const int32_t* __restrict__ nchild, const int64_t* __restrict__ box_range,
+                                 const int32_t* __restrict__ at, uint64_t* __restrict__ keys,
+                                 int32_t* __restrict__ ends) {
+  const int ti = blockIdx.x * kTB + threadIdx.x;
+  if (ti >= ntb) return;
+  const int b = tbox[ti];
+  int k = at[ti];
+  auto put = [&](int box) {
+    keys[k] = ((uint64_t)(uint32_t)ti << 32) | (uint32_t)box_range[2 * box];
+    ends[k] = (int32_t)box_range[2 * box + 1];
+    ++k;
+  };
+  put(b);
+  for (int64_t q = near_ptr[b]; q < near_ptr[b + 1]; ++q) put(near_idx[q]);
+  for (int a = parent[b]; a >= 0; a = parent[a])
+    for (int64_t q = near_ptr[a]; q < near_ptr[a + 1]; ++q)
+      if (nchild[near_idx[q]] == 0) put(near_idx[q]);
+}
+
+// merges each target box's sorted ranges in place (empty ones dropped, touching ones joined); err |= 4 on an
+// overlap. Writes the merged count and the face count of the target box.
+__global__ void k_ns_range_merge(int ntb, const int32_t* __restrict__ at, const int32_t* __restrict__ cnt,
+                                 const uint64_t* __restrict__ keys, const int32_t* __restrict__ ends,
+                                 int64_t* __restrict__ mstart, int64_t* __restrict__ mend, int32_t* __restrict__ mcnt,
+                                 int64_t* __restrict__ nfaces, int32_t* err) {
+  const int ti = blockIdx.x * kTB + threadIdx.x;
+  if (ti > ntb) return;
+  if (ti == ntb) {
+    mcnt[ti] = 0;
+    nfaces[ti] = 0;
+    return;
+  }
+  const int k0 = at[ti], k1 = k0 + cnt[ti];
+  int m = 0;
+  int64_t nf = 0;
+  bool overlap = false;
+  for (int k = k0; k < k1; ++k) {
+    const int64_t a = (int64_t)(uint32_t)keys[k], e = ends[k];
+    if (a == e) continue;
+    if (m > 0 && a <= mend[k0 + m - 1]) {
+      if (a < mend[k0 + m - 1]) overlap = true;
+      if (e > mend[k0 + m - 1]) mend[k0 + m - 1] = e;
+    } else {
+      mstart[k0 + m] = a;
+      mend[k0 + m] = e;
+      ++m;
+    }
+  }
+  for (int q = 0; q < m; ++q) nf += mend[k0 + q] - mstart[k0 + q];
+  if (overlap) atomicOr(err, 4);
+  mcnt[ti] = m;
+  nfaces[ti] = nf;
+}
+
+// one key (target box, DOF) per face corner of each target box's merged ranges; fdof in face_perm order
+__global__ void k_ns_dof_keys(int64_t total, int ntb, const int64_t* __restrict__ fstart,
+                              const int32_t* __restrict__ at, const int32_t* __restrict__ mcnt,
+                              const int64_t* __restrict__ mstart, const int64_t* __restrict__ mend,
+                              const int32_t* __restrict__ fdof, uint64_t* __restrict__ keys) {
+  const int64_t e = (int64_t)blockIdx.x * kTB + threadIdx.x;
+  if (e >= total) return;
+  int lo = 0, hi = ntb - 1;  // last target box with fstart <= e
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (fstart[mid] <= e) lo = mid;
+    else hi = mid - 1;
+  }
+  const int ti = lo;
+  int64_t i = e - fstart[ti];
+  int64_t pos = -1;
+  for (int q = at[ti]; q < at[ti] + mcnt[ti]; ++q) {
+    const int64_t len = mend[q] - mstart[q];
+    if (i < len) {
+      pos = mstart[q] + i;
+      break;
+    }
+    i -= len;
+  }
+  for (int c = 0; c < 3; ++c) keys[3 * e + c] = ((uint64_t)(uint32_t)ti << 32) | (uint32_t)fdof[3 * pos + c];
+}
+
+__global__ void k_unique_flags(int64_t n, const uint64_t* __restrict__ keys, int32_t* __restrict__ flags) {
+  const int64_t i = (int64_t)blockIdx.x * kTB + threadIdx.x;
+  if (i < n) flags[i] = (i == 0 || keys[i] != keys[i - 1]) ? 1 : 0;
+}
+
+__global__ void k_compact_u64(int64_t n, const uint64_t* __restrict__ keys, const int32_t* __restrict__ flags,
+                              const int32_t* __restrict__ at, uint64_t* __restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * kTB + threadIdx.x;
+  if (i < n && flags[i]) out[at[i] - 1] = keys[i];
+}
+
+// chunks of <= kNearRows rows per target box, keyed for a stable sort by descending cost (rows x padded columns)
+__global__ void k_ns_chunk_keys(int ntb, const int64_t* __restrict__ tptr, const int32_t* __restrict__ tbox,
+                                const int32_t* __restrict__ chunk_at, const int64_t* __restrict__ dptr, int C,
+                                uint64_t* __restrict__ keys, int32_t* err) {
+  const int ti = blockIdx.x * kTB + threadIdx.x;
+  if (ti >= ntb) return;
+  const int64_t nt = tptr[tbox[ti] + 1] - tptr[tbox[ti]];
+  const int64_t ncol = ((dptr[ti + 1] - dptr[ti]) + C + 3) & ~(int64_t)3;
+  int k = chunk_at[ti];
+  for (int64_t r0 = 0; r0 < nt; r0 += kNearRows, ++k) {
+    const int64_t rows = (nt - r0 < kNearRows) ? nt - r0 : kNearRows;
+    const int64_t cost = rows * ncol;
+    if (cost >= (1 << 22)) atomicOr(err, 8);
+    keys[k] = ((uint64_t)((1 << 22) - 1 - cost) << 24) | (uint32_t)k;
+  }
+}
+
+// per chunk in cost order: its target box, first row, rows, columns, DOF columns, ranges
+__global__ void k_ns_chunk_sizes(int nch, const uint64_t* __restrict__ order, const int32_t* __restrict__ chunk_ti,
+                                 const int32_t* __restrict__ chunk_at, const int64_t* __restrict__ tptr,
+                                 const int32_t* __restrict__ tbox, const int64_t* __restrict__ dptr,
+                                 const int32_t* __restrict__ mcnt, int C, int64_t* __restrict__ rows_v,
+                                 int64_t* __restrict__ cols_v, int64_t* __restrict__ mat_v, int64_t* __restrict__ rng_v,
+                                 int64_t* __restrict__ nnz_v) {
+  const int k = blockIdx.x * kTB + threadIdx.x;
+  if (k > nch) return;
+  if (k == nch) {  // the exclusive scans' totals land at index nch
+    rows_v[k] = cols_v[k] = mat_v[k] = rng_v[k] = nnz_v[k] = 0;
+    return;
+  }
+  const int c = (int)(order[k] & 0xFFFFFFu);  // the chunk index: the key's low 24 bits
+  const int ti = chunk_ti[c];
+  const int64_t r0 = (int64_t)(c - chunk_at[ti]) * kNearRows;
+  const int64_t nt = tptr[tbox[ti] + 1] - tptr[tbox[ti]];
+  const int64_t rows = (nt - r0 < kNearRows) ? nt - r0 : kNearRows;
+  const int64_t ncd = dptr[ti + 1] - dptr[ti];
+  const int64_t ncol = (ncd + C + 3) & ~(int64_t)3;
+  rows_v[k] = rows;
+  cols_v[k] = ncol;
+  mat_v[k] = rows * 2 * ncol;
+  rng_v[k] = mcnt[ti];
+  nnz_v[k] = rows * ncd;
+}
+
+__global__ void k_ns_chunk_ti(int ntb, const int32_t* __restrict__ chunk_at, int32_t* __restrict__ chunk_ti) {
+  const int ti = blockIdx.x * kTB + threadIdx.x;
+  if (ti >= ntb) return;
+  for (int k = chunk_at[ti]; k < chunk_at[ti + 1]; ++k) chunk_ti[k] = ti;
+}
+
+// one workgroup per chunk: its rows (target nodes in node order), columns (the target box's DOFs, its C L2P
+// columns, zero-weight padding), ranges with their face prefixes, and the row -> chunk / node -> row maps
+__global__ void k_ns_fill(int nch, const uint64_t* __restrict__ order, const int32_t* __restrict__ chunk_ti,
+                          const int32_t* __restrict__ chunk_at, const int64_t* __restrict__ tptr,
+                          const int32_t* __restrict__ tnodes, const int32_t* __restrict__ tbox,
+                          const int64_t* __restrict__ dptr, const int32_t* __restrict__ dofs,
+                          const int32_t* __restrict__ rat, const int32_t* __restrict__ mcnt,
+                          const int64_t* __restrict__ mstart, const int64_t* __restrict__ mend, int C, int n_dof,
+                          const int64_t* __restrict__ row_ptr, const int64_t* __restrict__ col_ptr,
+                          const int64_t* __restrict__ range_ptr, int32_t* __restrict__ row_out,
+                          int32_t* __restrict__ row_block, int64_t* __restrict__ node_row, int32_t* __restrict__ col_idx,
+                          int32_t* __restrict__ ncol_dof, int64_t* __restrict__ ranges,
+                          int64_t* __restrict__ range_prefix) {
+  const int k = blockIdx.x;
+  const int c = (int)(order[k] & 0xFFFFFFu);  // the chunk index: the key's low 24 bits
+  const int ti = chunk_ti[c];
+  const int b = tbox[ti];
+  const int64_t r0 = (int64_t)(c - chunk_at[ti]) * kNearRows;
+  const int64_t rows = row_ptr[k + 1] - row_ptr[k];
+  const int64_t ncd = dptr[ti + 1] - dptr[ti];
+  const int64_t ncol = col_ptr[k + 1] - col_ptr[k];
+  for (int64_t r = threadIdx.x; r < rows; r += blockDim.x) {
+    const int node = tnodes[tptr[b] + r0 + r];
+    row_out[row_ptr[k] + r] = node;
+    row_block[row_ptr[k] + r] = k;
+    node_row[node] = row_ptr[k] + r;
+  }
+  int32_t* ci = col_idx + col_ptr[k];
+  for (int64_t j = threadIdx.x; j < ncol; j += blockDim.x)
+    ci[j] = j < ncd ? dofs[dptr[ti] + j] : (int32_t)(n_dof + (int64_t)b * C + (j < ncd + C ? j - ncd : C - 1));
+  if (threadIdx.x == 0) {
+    ncol_dof[k] = (int32_t)ncd;
+    int64_t pre = 0, q = range_ptr[k];
+    for (int g = rat[ti]; g < rat[ti] + mcnt[ti]; ++g, ++q) {
+      ranges[2 * q] = mstart[g];
+      ranges[2 * q + 1] = mend[g];
+      range_prefix[q] = pre;
+      pre += mend[g] - mstart[g];
+    }
+  }
+}
+
+// L2P coefficients per near row: (1/s) Re sum R_n^m(x~) L~_n^m, x~ = (x - c)/s, as plan.cpp computes them (same
+// recurrences, same order of operations): thread (row, m) runs the diagonal recurrence to NR_m^m, then n > m
+__global__ void k_l2p_coef(int64_t nrows, int P, const int32_t* __restrict__ row_out, const int32_t* __restrict__ target,
+                           const double* __restrict__ X, const BoxGeom* __restrict__ geom,
+                           const int32_t* __restrict__ level, const double* __restrict__ level_side,
+                           const double* __restrict__ cnm, double* __restrict__ out) {
+  const int64_t e = (int64_t)blockIdx.x * kTB + threadIdx.x;
+  if (e >= nrows * (P + 1)) return;
+  const int64_t r = e / (P + 1);
+  const int m = (int)(e - r * (P + 1));
+  const int C = (P + 1) * (P + 1);
+  const int node = row_out[r];
+  const int b = target[node];
+  const double s = level_side[level[b]];
+  const double x = (X[3 * node] - geom[b].X[0]) / s;
+  const double y = (X[3 * node + 1] - geom[b].X[1]) / s;
+  const double z = (X[3 * node + 2] - geom[b].X[2]) / s;
+  const double r2 = x * x + y * y + z * z;
+  double pr = 1.0, pi = 0.0;
+  for (int mm = 1; mm <= m; ++mm) {
+    const double sc = 1.0 / (2.0 * mm);
+    const double nr = sc * (x * pr - y * pi), ni = sc * (x * pi + y * pr);
+    pr = nr;
+    pi = ni;
+  }
+  double* o = out + r * C;
+  auto put = [&](int n, double re, double im) {
+    const double c = cnm[n * (n + 1) / 2 + m];
+    const double Rr = c * re, Ri = c * im;
+    if (m == 0) {
+      o[n * n] = Rr / s;
+    } else {
+      o[n * n + 2 * m - 1] = 2.0 * Rr / s;
+      o[n * n + 2 * m] = -2.0 * Ri / s;
+    }
+  };
+  put(m, pr, pi);
+  double r1 = pr, i1 = pi, r0 = 0., i0 = 0.;  // NR_{n-1}^m, NR_{n-2}^m
+  for (int n = m + 1; n <= P; ++n) {
+    double ar = (2 * n - 1) * z * r1, ai = (2 * n - 1) * z * i1;
+    if (n - 2 >= m) {
+      ar -= r2 * r0;
+      ai -= r2 * i0;
+    }
+    const double sc = 1.0 / ((double)(n + m) * (n - m));
+    const double nr = sc * ar, ni = sc * ai;
+    r0 = r1;
+    i0 = i1;
+    r1 = nr;
+    i1 = ni;
+    put(n, nr, ni);
+  }
+}
+
 }  // namespace
 
 void tree_split(hipStream_t s, int F, int lb, int leaf_max, int may_split, const double* pole, const int32_t* fbox,
@@ -334,6 +632,86 @@ void list_totals(hipStream_t s, int np, const int32_t* nearf, const int32_t* m2l
 void csr_from_keys(hipStream_t s, int nb, const uint64_t* keys, int64_t n, int64_t* ptr, int32_t* idx) {
   const int64_t m = std::max<int64_t>(n, (int64_t)nb + 1);
   hipLaunchKernelGGL(k_csr_from_keys, dim3(grid_of(m)), dim3(kTB), 0, s, nb, keys, n, ptr, idx);
+}
+
+void node_keys(hipStream_t s, int N, const int32_t* target, uint64_t* keys) {
+  if (N > 0) hipLaunchKernelGGL(k_node_keys, dim3(grid_of(N)), dim3(kTB), 0, s, N, target, keys);
+}
+void ns_tbox_flags(hipStream_t s, int nb, const int64_t* tptr, int32_t* flags) {
+  hipLaunchKernelGGL(k_ns_tbox_flags, dim3(grid_of(nb)), dim3(kTB), 0, s, nb, tptr, flags);
+}
+void ns_tbox_compact(hipStream_t s, int nb, const int32_t* flags, const int32_t* at, int32_t* tbox) {
+  hipLaunchKernelGGL(k_ns_tbox_compact, dim3(grid_of(nb)), dim3(kTB), 0, s, nb, flags, at, tbox);
+}
+void ns_chunk_count(hipStream_t s, int ntb, const int64_t* tptr, const int32_t* tbox, int32_t* cnt) {
+  hipLaunchKernelGGL(k_ns_chunk_count, dim3(grid_of(ntb + 1)), dim3(kTB), 0, s, ntb, tptr, tbox, cnt);
+}
+void ns_range_count(hipStream_t s, int ntb, const int32_t* tbox, const int32_t* parent, const int64_t* near_ptr,
+                    const int32_t* near_idx, const int32_t* nchild, int32_t* cnt) {
+  hipLaunchKernelGGL(k_ns_range_count, dim3(grid_of(ntb + 1)), dim3(kTB), 0, s, ntb, tbox, parent, near_ptr, near_idx,
+                     nchild, cnt);
+}
+void ns_range_write(hipStream_t s, int ntb, const int32_t* tbox, const int32_t* parent, const int64_t* near_ptr,
+                    const int32_t* near_idx, const int32_t* nchild, const int64_t* box_range, const int32_t* at,
+                    uint64_t* keys, int32_t* ends) {
+  if (ntb > 0)
+    hipLaunchKernelGGL(k_ns_range_write, dim3(grid_of(ntb)), dim3(kTB), 0, s, ntb, tbox, parent, near_ptr, near_idx,
+                       nchild, box_range, at, keys, ends);
+}
+void ns_range_merge(hipStream_t s, int ntb, const int32_t* at, const int32_t* cnt, const uint64_t* keys,
+                    const int32_t* ends, int64_t* mstart, int64_t* mend, int32_t* mcnt, int64_t* nfaces, int32_t* err) {
+  hipLaunchKernelGGL(k_ns_range_merge, dim3(grid_of(ntb + 1)), dim3(kTB), 0, s, ntb, at, cnt, keys, ends, mstart, mend,
+                     mcnt, nfaces, err);
+}
+void ns_dof_keys(hipStream_t s, int64_t total, int ntb, const int64_t* fstart, const int32_t* at, const int32_t* mcnt,
+                 const int64_t* mstart, const int64_t* mend, const int32_t* fdof, uint64_t* keys) {
+  if (total > 0)
+    hipLaunchKernelGGL(k_ns_dof_keys, dim3(grid_of(total)), dim3(kTB), 0, s, total, ntb, fstart, at, mcnt, mstart, mend,
+                       fdof, keys);
+}
+void unique_flags(hipStream_t s, int64_t n, const uint64_t* keys, int32_t* flags) {
+  if (n > 0) hipLaunchKernelGGL(k_unique_flags, dim3(grid_of(n)), dim3(kTB), 0, s, n, keys, flags);
+}
+void compact_u64(hipStream_t s, int64_t n, const uint64_t* keys, const int32_t* flags, const int32_t* at,
+                 uint64_t* out) {
+  if (n > 0) hipLaunchKernelGGL(k_compact_u64, dim3(grid_of(n)), dim3(kTB), 0, s, n, keys, flags, at, out);
+}
+void ns_chunk_keys(hipStream_t s, int ntb, const int64_t* tptr, const int32_t* tbox, const int32_t* chunk_at,
+                   const int64_t* dptr, int C, uint64_t* keys, int32_t* err) {
+  if (ntb > 0)
+    hipLaunchKernelGGL(k_ns_chunk_keys, dim3(grid_of(ntb)), dim3(kTB), 0, s, ntb, tptr, tbox, chunk_at, dptr, C, keys,
+                       err);
+}
+void ns_chunk_ti(hipStream_t s, int ntb, const int32_t* chunk_at, int32_t* chunk_ti) {
+  if (ntb > 0) hipLaunchKernelGGL(k_ns_chunk_ti, dim3(grid_of(ntb)), dim3(kTB), 0, s, ntb, chunk_at, chunk_ti);
+}
+void ns_chunk_sizes(hipStream_t s, int nch, const uint64_t* order, const int32_t* chunk_ti, const int32_t* chunk_at,
+                    const int64_t* tptr, const int32_t* tbox, const int64_t* dptr, const int32_t* mcnt, int C,
+                    int64_t* rows_v, int64_t* cols_v, int64_t* mat_v, int64_t* rng_v, int64_t* nnz_v) {
+  hipLaunchKernelGGL(k_ns_chunk_sizes, dim3(grid_of(nch + 1)), dim3(kTB), 0, s, nch, order, chunk_ti, chunk_at, tptr,
+                     tbox, dptr, mcnt, C, rows_v, cols_v, mat_v, rng_v, nnz_v);
+}
+void ns_fill(hipStream_t s, int nch, const uint64_t* order, const int32_t* chunk_ti, const int32_t* chunk_at,
+             const int64_t* tptr, const int32_t* tnodes, const int32_t* tbox, const int64_t* dptr, const int32_t* dofs,
+             const int32_t* rat, const int32_t* mcnt, const int64_t* mstart, const int64_t* mend, int C, int n_dof,
+             const int64_t* row_ptr, const int64_t* col_ptr, const int64_t* range_ptr, int32_t* row_out,
+             int32_t* row_block, int64_t* node_row, int32_t* col_idx, int32_t* ncol_dof, int64_t* ranges,
+             int64_t* range_prefix) {
+  if (nch > 0)
+    hipLaunchKernelGGL(k_ns_fill, dim3(nch), dim3(kTB), 0, s, nch, order, chunk_ti, chunk_at, tptr, tnodes, tbox, dptr,
+                       dofs, rat, mcnt, mstart, mend, C, n_dof, row_ptr, col_ptr, range_ptr, row_out, row_block,
+                       node_row, col_idx, ncol_dof, ranges, range_prefix);
+}
+void l2p_coef(hipStream_t s, int64_t nrows, int P, const int32_t* row_out, const int32_t* target, const double* X,
+              const BoxGeom* geom, const int32_t* level, const double* level_side, const double* cnm, double* out) {
+  const int64_t n = nrows * (P + 1);
+  if (n > 0)
+    hipLaunchKernelGGL(k_l2p_coef, dim3(grid_of(n)), dim3(kTB), 0, s, nrows, P, row_out, target, X, geom, level,
+                       level_side, cnm, out);
+}
+hipError_t scan_exclusive_i64(hipStream_t s, void* tmp, size_t& tmp_bytes, const int64_t* in, int64_t* out,
+                              int64_t n) {
+  return rocprim::exclusive_scan(tmp, tmp_bytes, in, out, (int64_t)0, (size_t)n, rocprim::plus<int64_t>(), s);
 }
 
 hipError_t sort_keys_u64(hipStream_t s, void* tmp, size_t& tmp_bytes, const uint64_t* in, uint64_t* out, int64_t n,

@@ -624,6 +624,257 @@ hipError_t scan_exclusive_i32(hipStream_t s, void* tmp, size_t& tmp_bytes, const
   return hipSuccess;
 }
 
+// ---- near-field blocks on the device (tree.hip) ----
+void node_keys(hipStream_t s, int N, const int32_t* target, uint64_t* keys) {
+  hipstub::enqueue(s, [=]() {
+    for (int i = 0; i < N; ++i) keys[i] = ((uint64_t)(uint32_t)target[i] << 32) | (uint32_t)i;
+  });
+}
+void ns_tbox_flags(hipStream_t s, int nb, const int64_t* tptr, int32_t* flags) {
+  hipstub::enqueue(s, [=]() {
+    for (int b = 0; b < nb; ++b) flags[b] = tptr[b + 1] > tptr[b] ? 1 : 0;
+  });
+}
+void ns_tbox_compact(hipStream_t s, int nb, const int32_t* flags, const int32_t* at, int32_t* tbox) {
+  hipstub::enqueue(s, [=]() {
+    for (int b = 0; b < nb; ++b)
+      if (flags[b]) tbox[at[b] - 1] = b;
+  });
+}
+void ns_chunk_count(hipStream_t s, int ntb, const int64_t* tptr, const int32_t* tbox, int32_t* cnt) {
+  hipstub::enqueue(s, [=]() {
+    for (int ti = 0; ti <= ntb; ++ti)
+      cnt[ti] = ti == ntb ? 0 : (int32_t)((tptr[tbox[ti] + 1] - tptr[tbox[ti]] + kNearRows - 1) / kNearRows);
+  });
+}
+void ns_range_count(hipStream_t s, int ntb, const int32_t* tbox, const int32_t* parent, const int64_t* near_ptr,
+                    const int32_t* near_idx, const int32_t* nchild, int32_t* cnt) {
+  hipstub::enqueue(s, [=]() {
+    for (int ti = 0; ti < ntb; ++ti) {
+      const int b = tbox[ti];
+      int n = 1 + (int)(near_ptr[b + 1] - near_ptr[b]);
+      for (int a = parent[b]; a >= 0; a = parent[a])
+        for (int64_t k = near_ptr[a]; k < near_ptr[a + 1]; ++k) n += nchild[near_idx[k]] == 0;
+      cnt[ti] = n;
+    }
+    cnt[ntb] = 0;
+  });
+}
+void ns_range_write(hipStream_t s, int ntb, const int32_t* tbox, const int32_t* parent, const int64_t* near_ptr,
+                    const int32_t* near_idx, const int32_t* nchild, const int64_t* box_range, const int32_t* at,
+                    uint64_t* keys, int32_t* ends) {
+  hipstub::enqueue(s, [=]() {
+    for (int ti = 0; ti < ntb; ++ti) {
+      const int b = tbox[ti];
+      int k = at[ti];
+      auto put = [&](int box) {
+        keys[k] = ((uint64_t)(uint32_t)ti << 32) | (uint32_t)box_range[2 * box];
+        ends[k] = (int32_t)box_range[2 * box + 1];
+        ++k;
+      };
+      put(b);
+      for (int64_t q = near_ptr[b]; q < near_ptr[b + 1]; ++q) put(near_idx[q]);
+      for (int a = parent[b]; a >= 0; a = parent[a])
+        for (int64_t q = near_ptr[a]; q < near_ptr[a + 1]; ++q)
+          if (nchild[near_idx[q]] == 0) put(near_idx[q]);
+    }
+  });
+}
+void ns_range_merge(hipStream_t s, int ntb, const int32_t* at, const int32_t* cnt, const uint64_t* keys,
+                    const int32_t* ends, int64_t* mstart, int64_t* mend, int32_t* mcnt, int64_t* nfaces, int32_t* err) {
+  hipstub::enqueue(s, [=]() {
+    for (int ti = 0; ti < ntb; ++ti) {
+      const int k0 = at[ti], k1 = k0 + cnt[ti];
+      int m = 0;
+      int64_t nf = 0;
+      for (int k = k0; k < k1; ++k) {
+        const int64_t a = (int64_t)(uint32_t)keys[k], e = ends[k];
+        if (a == e) continue;
+        if (m > 0 && a <= mend[k0 + m - 1]) {
+          if (a < mend[k0 + m - 1]) *err |= 4;
+          if (e > mend[k0 + m - 1]) mend[k0 + m - 1] = e;
+        } else {
+          mstart[k0 + m] = a;
+          mend[k0 + m] = e;
+          ++m;
+        }
+      }
+      for (int q = 0; q < m; ++q) nf += mend[k0 + q] - mstart[k0 + q];
+      mcnt[ti] = m;
+      nfaces[ti] = nf;
+    }
+    mcnt[ntb] = 0;
+    nfaces[ntb] = 0;
+  });
+}
+void ns_dof_keys(hipStream_t s, int64_t total, int ntb, const int64_t* fstart, const int32_t* at, const int32_t* mcnt,
+                 const int64_t* mstart, const int64_t* mend, const int32_t* fdof, uint64_t* keys) {
+  hipstub::enqueue(s, [=]() {
+    for (int ti = 0; ti < ntb; ++ti) {
+      int64_t e = fstart[ti];
+      for (int q = at[ti]; q < at[ti] + mcnt[ti]; ++q)
+        for (int64_t pos = mstart[q]; pos < mend[q]; ++pos, ++e)
+          for (int c = 0; c < 3; ++c) keys[3 * e + c] = ((uint64_t)(uint32_t)ti << 32) | (uint32_t)fdof[3 * pos + c];
+    }
+    (void)total;
+  });
+}
+void unique_flags(hipStream_t s, int64_t n, const uint64_t* keys, int32_t* flags) {
+  hipstub::enqueue(s, [=]() {
+    for (int64_t i = 0; i < n; ++i) flags[i] = (i == 0 || keys[i] != keys[i - 1]) ? 1 : 0;
+  });
+}
+void compact_u64(hipStream_t s, int64_t n, const uint64_t* keys, const int32_t* flags, const int32_t* at,
+                 uint64_t* out) {
+  hipstub::enqueue(s, [=]() {
+    for (int64_t i = 0; i < n; ++i)
+      if (flags[i]) out[at[i] - 1] = keys[i];
+  });
+}
+void ns_chunk_keys(hipStream_t s, int ntb, const int64_t* tptr, const int32_t* tbox, const int32_t* chunk_at,
+                   const int64_t* dptr, int C, uint64_t* keys, int32_t* err) {
+  hipstub::enqueue(s, [=]() {
+    for (int ti = 0; ti < ntb; ++ti) {
+      const int64_t nt = tptr[tbox[ti] + 1] - tptr[tbox[ti]];
+      const int64_t ncol = ((dptr[ti + 1] - dptr[ti]) + C + 3) & ~(int64_t)3;
+      int k = chunk_at[ti];
+      for (int64_t r0 = 0; r0 < nt; r0 += kNearRows, ++k) {
+        const int64_t cost = std::min<int64_t>(nt - r0, kNearRows) * ncol;
+        if (cost >= (1 << 22)) *err |= 8;
+        keys[k] = ((uint64_t)((1 << 22) - 1 - cost) << 24) | (uint32_t)k;
+      }
+    }
+  });
+}
+void ns_chunk_ti(hipStream_t s, int ntb, const int32_t* chunk_at, int32_t* chunk_ti) {
+  hipstub::enqueue(s, [=]() {
+    for (int ti = 0; ti < ntb; ++ti)
+      for (int k = chunk_at[ti]; k < chunk_at[ti + 1]; ++k) chunk_ti[k] = ti;
+  });
+}
+void ns_chunk_sizes(hipStream_t s, int nch, const uint64_t* order, const int32_t* chunk_ti, const int32_t* chunk_at,
+                    const int64_t* tptr, const int32_t* tbox, const int64_t* dptr, const int32_t* mcnt, int C,
+                    int64_t* rows_v, int64_t* cols_v, int64_t* mat_v, int64_t* rng_v, int64_t* nnz_v) {
+  hipstub::enqueue(s, [=]() {
+    for (int k = 0; k < nch; ++k) {
+      const int c = (int)(order[k] & 0xFFFFFFu);  // the chunk index: the key's low 24 bits
+      const int ti = chunk_ti[c];
+      const int64_t r0 = (int64_t)(c - chunk_at[ti]) * kNearRows;
+      const int64_t nt = tptr[tbox[ti] + 1] - tptr[tbox[ti]];
+      const int64_t rows = std::min<int64_t>(nt - r0, kNearRows);
+      const int64_t ncd = dptr[ti + 1] - dptr[ti];
+      const int64_t ncol = (ncd + C + 3) & ~(int64_t)3;
+      rows_v[k] = rows;
+      cols_v[k] = ncol;
+      mat_v[k] = rows * 2 * ncol;
+      rng_v[k] = mcnt[ti];
+      nnz_v[k] = rows * ncd;
+    }
+    rows_v[nch] = cols_v[nch] = mat_v[nch] = rng_v[nch] = nnz_v[nch] = 0;
+  });
+}
+void ns_fill(hipStream_t s, int nch, const uint64_t* order, const int32_t* chunk_ti, const int32_t* chunk_at,
+             const int64_t* tptr, const int32_t* tnodes, const int32_t* tbox, const int64_t* dptr, const int32_t* dofs,
+             const int32_t* rat, const int32_t* mcnt, const int64_t* mstart, const int64_t* mend, int C, int n_dof,
+             const int64_t* row_ptr, const int64_t* col_ptr, const int64_t* range_ptr, int32_t* row_out,
+             int32_t* row_block, int64_t* node_row, int32_t* col_idx, int32_t* ncol_dof, int64_t* ranges,
+             int64_t* range_prefix) {
+  hipstub::enqueue(s, [=]() {
+    for (int k = 0; k < nch; ++k) {
+      const int c = (int)(order[k] & 0xFFFFFFu);  // the chunk index: the key's low 24 bits
+      const int ti = chunk_ti[c];
+      const int b = tbox[ti];
+      const int64_t r0 = (int64_t)(c - chunk_at[ti]) * kNearRows;
+      const int64_t rows = row_ptr[k + 1] - row_ptr[k];
+      const int64_t ncd = dptr[ti + 1] - dptr[ti];
+      const int64_t ncol = col_ptr[k + 1] - col_ptr[k];
+      for (int64_t r = 0; r < rows; ++r) {
+        const int node = tnodes[tptr[b] + r0 + r];
+        row_out[row_ptr[k] + r] = node;
+        row_block[row_ptr[k] + r] = k;
+        node_row[node] = row_ptr[k] + r;
+      }
+      int32_t* ci = col_idx + col_ptr[k];
+      for (int64_t j = 0; j < ncol; ++j)
+        ci[j] = j < ncd ? dofs[dptr[ti] + j] : (int32_t)(n_dof + (int64_t)b * C + (j < ncd + C ? j - ncd : C - 1));
+      ncol_dof[k] = (int32_t)ncd;
+      int64_t pre = 0, q = range_ptr[k];
+      for (int g = rat[ti]; g < rat[ti] + mcnt[ti]; ++g, ++q) {
+        ranges[2 * q] = mstart[g];
+        ranges[2 * q + 1] = mend[g];
+        range_prefix[q] = pre;
+        pre += mend[g] - mstart[g];
+      }
+    }
+  });
+}
+void l2p_coef(hipStream_t s, int64_t nrows, int P, const int32_t* row_out, const int32_t* target, const double* X,
+              const BoxGeom* geom, const int32_t* level, const double* level_side, const double* cnm, double* out) {
+  hipstub::enqueue(s, [=]() {
+    const int C = (P + 1) * (P + 1);
+    for (int64_t r = 0; r < nrows; ++r)
+      for (int m = 0; m <= P; ++m) {
+        const int node = row_out[r];
+        const int b = target[node];
+        const double sd = level_side[level[b]];
+        const double x = (X[3 * node] - geom[b].X[0]) / sd;
+        const double y = (X[3 * node + 1] - geom[b].X[1]) / sd;
+        const double z = (X[3 * node + 2] - geom[b].X[2]) / sd;
+        const double r2 = x * x + y * y + z * z;
+        double pr = 1.0, pi = 0.0;
+        for (int mm = 1; mm <= m; ++mm) {
+          const double sc = 1.0 / (2.0 * mm);
+          const double nr = sc * (x * pr - y * pi), ni = sc * (x * pi + y * pr);
+          pr = nr;
+          pi = ni;
+        }
+        double* o = out + r * C;
+        auto put = [&](int n, double re, double im) {
+          const double c = cnm[n * (n + 1) / 2 + m];
+          const double Rr = c * re, Ri = c * im;
+          if (m == 0) {
+            o[n * n] = Rr / sd;
+          } else {
+            o[n * n + 2 * m - 1] = 2.0 * Rr / sd;
+            o[n * n + 2 * m] = -2.0 * Ri / sd;
+          }
+        };
+        put(m, pr, pi);
+        double r1 = pr, i1 = pi, r0 = 0., i0 = 0.;
+        for (int n = m + 1; n <= P; ++n) {
+          double ar = (2 * n - 1) * z * r1, ai = (2 * n - 1) * z * i1;
+          if (n - 2 >= m) {
+            ar -= r2 * r0;
+            ai -= r2 * i0;
+          }
+          const double sc = 1.0 / ((double)(n + m) * (n - m));
+          const double nr = sc * ar, ni = sc * ai;
+          r0 = r1;
+          i0 = i1;
+          r1 = nr;
+          i1 = ni;
+          put(n, nr, ni);
+        }
+      }
+  });
+}
+hipError_t scan_exclusive_i64(hipStream_t s, void* tmp, size_t& tmp_bytes, const int64_t* in, int64_t* out,
+                              int64_t n) {
+  if (!tmp) {
+    tmp_bytes = 256;
+    return hipSuccess;
+  }
+  hipstub::enqueue(s, [=]() {
+    int64_t a = 0;
+    for (int64_t i = 0; i < n; ++i) {
+      const int64_t v = in[i];
+      out[i] = a;
+      a += v;
+    }
+  });
+  return hipSuccess;
+}
+
 void p2m_build(hipStream_t s, int P, const P2MBuild& a, double* mat) {
   if (P < 1 || P > 10) throw std::runtime_error("expansion order outside 1..10 for the P2M build");
   const P2MBuild A = a;
@@ -735,7 +986,7 @@ void near_quad(hipStream_t s, int64_t nrows, const NearQuad& a, const Gemv& g, d
       const int32_t* cols = G.col_idx + c0;
       for (int64_t gi = A.range_ptr[b]; gi < A.range_ptr[b + 1]; ++gi)
         for (int64_t fi = A.ranges[2 * gi]; fi < A.ranges[2 * gi + 1]; ++fi) {
-          const int f = A.face_perm[fi];
+          const int64_t f = fi;  // face arrays in face_perm order
           int slot = -1;
           for (int k = 0; k < 3; ++k)
             if (A.fv[3 * f + k] == t) slot = k;
