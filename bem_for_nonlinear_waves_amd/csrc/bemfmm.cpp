@@ -370,6 +370,8 @@ struct bemfmm_ctx {
     DBuf<uint64_t> k0, k1, rkeys, rkeys2, dk0, dk1, ck0, ck1;
     DBuf<int64_t> tptr, mstart, mend, nfaces, fstart, dptr, v_rows, v_cols, v_mat, v_rng, v_nnz, nnzp;
     DBuf<int32_t> tnodes, flags, at, tbox, rcnt, rat, rends, rends2, mcnt, uflag, uat, dofs, ccnt, cat, cti;
+    DBuf<int32_t> face_cols;       // local column of each face corner per target box (near quadrature)
+    DBuf<int64_t> block_fstart;    // per near block: its target box's first entry in face_cols / 3
     DBuf<double> lside, cnm;
   } nbuf;
   int64_t near_rows = 0;  // rows of the near-field blocks
@@ -1255,10 +1257,14 @@ void near_struct_device(bemfmm_ctx* c, Plan& pl) {
   c->sb.ncol_dof.alloc(nch);
   c->sb.ranges.alloc(2 * nrng);
   c->sb.range_prefix.alloc(nrng);
+  W.block_fstart.alloc(nch);
+  W.face_cols.alloc(3 * nfc);
+  dev::ns_face_cols(s, nfc, ntb, W.fstart.p, W.rat.p, W.mcnt.p, W.mstart.p, W.mend.p, c->sb.fdof.p, W.dptr.p,
+                    W.dofs.p, W.face_cols.p);
   dev::ns_fill(s, nch, W.ck1.p, W.cti.p, W.cat.p, W.tptr.p, W.tnodes.p, W.tbox.p, W.dptr.p, W.dofs.p, W.rat.p,
                W.mcnt.p, W.mstart.p, W.mend.p, C, pl.n_dof, c->near.row_ptr.p, c->near.col_ptr.p, c->sb.range_ptr.p,
                c->near.row_out.p, c->sb.row_block.p, c->sb.node_row.p, c->near.col_idx.p, c->sb.ncol_dof.p,
-               c->sb.ranges.p, c->sb.range_prefix.p);
+               c->sb.ranges.p, c->sb.range_prefix.p, W.fstart.p, W.block_fstart.p);
   HIPCHECK(hipGetLastError());
   c->near_ncd.alloc(nch);
   HIPCHECK(hipMemcpyAsync(c->near_ncd.p, c->sb.ncol_dof.p, nch * sizeof(int32_t), hipMemcpyDeviceToDevice, s));
@@ -1544,6 +1550,10 @@ void do_prepare(bemfmm_ctx* c) {
     a.C = pl.C;
     a.q = o.q;
     for (int32_t v : pl.near_ncol_dof) a.max_ncd = std::max(a.max_ncd, (int)v);
+    if (dev_plan) {
+      a.face_cols = c->nbuf.face_cols.p;
+      a.block_fstart = c->nbuf.block_fstart.p;
+    }
     a.near_region = pl.near_region;
     a.row_block = row_block.p;
     a.ncol_dof = ncol_dof.p;

@@ -1037,7 +1037,8 @@ __global__ __launch_bounds__(256) void k_near_quad(
     const double* __restrict__ Xs, const double* __restrict__ fX, const double* __restrict__ fcent,
     const double* __restrict__ fcr, const double* __restrict__ fJ, const int32_t* __restrict__ fv,
     const int32_t* __restrict__ fdof, const double* __restrict__ l2p, const double* __restrict__ dun_rule,
-    int lds_cols, double* __restrict__ mat) {
+    const int32_t* __restrict__ face_cols, const int64_t* __restrict__ block_fstart, int lds_cols,
+    double* __restrict__ mat) {
   // a row of at most lds_cols DOF columns accumulates in LDS (ds_add_f64) and is written once; a longer one
   // adds into the zero-filled matrix with global atomics
   extern __shared__ __align__(16) double acc[];  // 2 * lds_cols
@@ -1068,6 +1069,7 @@ __global__ __launch_bounds__(256) void k_near_quad(
   const int64_t g0 = range_ptr[b], g1 = range_ptr[b + 1];
   const int64_t nf = range_prefix[g1 - 1] + (ranges[2 * (g1 - 1) + 1] - ranges[2 * (g1 - 1)]);
   const int32_t* cols = col_idx + c0;
+  const int32_t* fcols = face_cols ? face_cols + 3 * block_fstart[b] : nullptr;
   for (int64_t i = threadIdx.x; i < nf; i += blockDim.x) {
     // locate the range holding local face i
     int64_t lo = g0, hi = g1 - 1;
@@ -1085,12 +1087,17 @@ __global__ __launch_bounds__(256) void k_near_quad(
                          near_region, c_gw5, dun, q, WG, WGn);
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-      const int d = fdof[3 * f + k];
-      int lo2 = 0, hi2 = ncd - 1;
-      while (lo2 < hi2) {
-        const int mid = (lo2 + hi2) >> 1;
-        if (cols[mid] < d) lo2 = mid + 1;
-        else hi2 = mid;
+      int lo2 = 0;
+      if (fcols) {
+        lo2 = fcols[3 * i + k];
+      } else {
+        const int d = fdof[3 * f + k];
+        int hi2 = ncd - 1;
+        while (lo2 < hi2) {
+          const int mid = (lo2 + hi2) >> 1;
+          if (cols[mid] < d) lo2 = mid + 1;
+          else hi2 = mid;
+        }
       }
       atomicAdd(&S0[lo2], WG[k]);
       atomicAdd(&S1[lo2], WGn[k]);
@@ -1750,7 +1757,7 @@ void near_quad(hipStream_t s, int64_t nrows, const NearQuad& a, const Gemv& g, d
   hipLaunchKernelGGL(k_near_quad, dim3((unsigned)nrows), dim3(256), (size_t)2 * lds_cols * sizeof(double), s, a.C,
                      a.q, a.near_region, a.row_block, g.row_ptr, g.col_ptr, g.mat_ptr, g.row_out, g.col_idx,
                      a.ncol_dof, a.range_ptr, a.ranges, a.range_prefix, a.face_perm, a.Xs, a.fX, a.fcent, a.fcr, a.fJ,
-                     a.fv, a.fdof, a.l2p, a.dun_rule, lds_cols, mat);
+                     a.fv, a.fdof, a.l2p, a.dun_rule, a.face_cols, a.block_fstart, lds_cols, mat);
 }
 
 void to_f32(hipStream_t s, int64_t n, const double* a, float* b) {

@@ -113,6 +113,9 @@ constexpr int kNearQuadLdsCols = 4096;  // DOF columns of a near row accumulated
 struct NearQuad {
   int C = 0, q = 0;
   int max_ncd = 0;  // largest DOF column count of a near block
+  // optional: the local column of each face corner, block b's faces from 3 * block_fstart[b] (else searched)
+  const int32_t* face_cols = nullptr;
+  const int64_t* block_fstart = nullptr;
   double near_region = 0;
   const int32_t* row_block = nullptr;
   const int32_t* ncol_dof = nullptr;
@@ -286,7 +289,11 @@ void ns_fill(hipStream_t s, int nch, const uint64_t* order, const int32_t* chunk
              const int32_t* rat, const int32_t* mcnt, const int64_t* mstart, const int64_t* mend, int C, int n_dof,
              const int64_t* row_ptr, const int64_t* col_ptr, const int64_t* range_ptr, int32_t* row_out,
              int32_t* row_block, int64_t* node_row, int32_t* col_idx, int32_t* ncol_dof, int64_t* ranges,
-             int64_t* range_prefix);
+             int64_t* range_prefix, const int64_t* fstart, int64_t* block_fstart);
+// local DOF column of each face corner of the target boxes' merged ranges (3 per face, at 3 fstart[ti] + ...)
+void ns_face_cols(hipStream_t s, int64_t total, int ntb, const int64_t* fstart, const int32_t* at, const int32_t* mcnt,
+                  const int64_t* mstart, const int64_t* mend, const int32_t* fdof, const int64_t* dptr,
+                  const int32_t* dofs, int32_t* cols);
 void l2p_coef(hipStream_t s, int64_t nrows, int P, const int32_t* row_out, const int32_t* target, const double* X,
               const BoxGeom* geom, const int32_t* level, const double* level_side, const double* cnm, double* out);
 // rocPRIM radix sorts and scans; tmp == nullptr: only sets tmp_bytes

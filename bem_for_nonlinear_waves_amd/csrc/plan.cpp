@@ -623,6 +623,7 @@ void plan_finish(Plan& pl, bool near_on_host) {
       if (pl.box_leaf[i]) leaves.push_back(i);
     const int nl = (int)leaves.size();
     std::vector<std::vector<int32_t>> cols(nl);
+#pragma omp parallel for schedule(dynamic, 32)
     for (int li = 0; li < nl; ++li) {
       const int b = leaves[li];
       std::vector<int32_t> cl;
@@ -638,32 +639,39 @@ void plan_finish(Plan& pl, bool near_on_host) {
     G = BlockGemv();
     const int CP = (C + 63) / 64 * 64;
     pl.p2m_rows_pad = CP;
+    // offsets per leaf (prefix sums), then every leaf's entries filled in parallel
+    G.mat_ptr.assign(nl, 0);
+    G.row_ptr.assign(nl + 1, 0);
+    G.col_ptr.assign(nl + 1, 0);
+    pl.p2m_face_ptr.assign(nl + 1, 0);
     for (int li = 0; li < nl; ++li) {
       const int b = leaves[li];
-      const int ncol = (int)cols[li].size();
-      G.mat_ptr.push_back(G.mat_size);
-      G.mat_size += (int64_t)CP * 2 * ncol;
-      for (int r = 0; r < C; ++r) G.row_out.push_back(b * C + r);
-      G.row_ptr.push_back((int64_t)G.row_out.size());
-      G.col_idx.insert(G.col_idx.end(), cols[li].begin(), cols[li].end());
-      G.col_ptr.push_back((int64_t)G.col_idx.size());
+      G.mat_ptr[li] = G.mat_size;
+      G.mat_size += (int64_t)CP * 2 * (int64_t)cols[li].size();
+      G.row_ptr[li + 1] = G.row_ptr[li] + C;
+      G.col_ptr[li + 1] = G.col_ptr[li] + (int64_t)cols[li].size();
+      pl.p2m_face_ptr[li + 1] = pl.p2m_face_ptr[li] + (pl.box_range[2 * b + 1] - pl.box_range[2 * b]);
     }
-    pl.p2m_face_ptr.assign(1, 0);
-    pl.p2m_faces.clear();
-    pl.p2m_face_lc.clear();
-    pl.p2m_leaf_cs.clear();
+    G.row_out.assign(G.row_ptr[nl], 0);
+    G.col_idx.assign(G.col_ptr[nl], 0);
+    pl.p2m_faces.assign(pl.p2m_face_ptr[nl], 0);
+    pl.p2m_face_lc.assign(3 * pl.p2m_face_ptr[nl], 0);
+    pl.p2m_leaf_cs.assign(4 * (size_t)nl, 0.);
+#pragma omp parallel for schedule(dynamic, 32)
     for (int li = 0; li < nl; ++li) {
       const int b = leaves[li];
-      for (int64_t i = pl.box_range[2 * b]; i < pl.box_range[2 * b + 1]; ++i) {
+      for (int r = 0; r < C; ++r) G.row_out[G.row_ptr[li] + r] = b * C + r;
+      std::copy(cols[li].begin(), cols[li].end(), &G.col_idx[G.col_ptr[li]]);
+      int64_t k = pl.p2m_face_ptr[li];
+      for (int64_t i = pl.box_range[2 * b]; i < pl.box_range[2 * b + 1]; ++i, ++k) {
         const int f = pl.face_perm[i];
-        pl.p2m_faces.push_back(f);
+        pl.p2m_faces[k] = f;
         for (int c = 0; c < 3; ++c)
-          pl.p2m_face_lc.push_back(
-              (int32_t)(std::lower_bound(cols[li].begin(), cols[li].end(), face_dof[3 * f + c]) - cols[li].begin()));
+          pl.p2m_face_lc[3 * k + c] =
+              (int32_t)(std::lower_bound(cols[li].begin(), cols[li].end(), face_dof[3 * f + c]) - cols[li].begin());
       }
-      pl.p2m_face_ptr.push_back((int64_t)pl.p2m_faces.size());
-      for (int a = 0; a < 3; ++a) pl.p2m_leaf_cs.push_back(pl.box_center[3 * b + a]);
-      pl.p2m_leaf_cs.push_back(pl.level_side[pl.box_level[b]]);
+      for (int a = 0; a < 3; ++a) pl.p2m_leaf_cs[4 * (size_t)li + a] = pl.box_center[3 * b + a];
+      pl.p2m_leaf_cs[4 * (size_t)li + 3] = pl.level_side[pl.box_level[b]];
     }
     if (opt.fill_p2m) pl.p2m_mat.assign(G.mat_size, 0.);
     else pl.p2m_mat.clear();

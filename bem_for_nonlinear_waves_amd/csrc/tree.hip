@@ -391,6 +391,46 @@ __global__ void k_ns_dof_keys(int64_t total, int ntb, const int64_t* __restrict_
   for (int c = 0; c < 3; ++c) keys[3 * e + c] = ((uint64_t)(uint32_t)ti << 32) | (uint32_t)fdof[3 * pos + c];
 }
 
+// the local DOF column of every face corner of every target box's merged ranges (in the near quadrature's local
+// face order), so that the quadrature of each row reads three columns per face instead of searching them
+__global__ void k_ns_face_cols(int64_t total, int ntb, const int64_t* __restrict__ fstart,
+                               const int32_t* __restrict__ at, const int32_t* __restrict__ mcnt,
+                               const int64_t* __restrict__ mstart, const int64_t* __restrict__ mend,
+                               const int32_t* __restrict__ fdof, const int64_t* __restrict__ dptr,
+                               const int32_t* __restrict__ dofs, int32_t* __restrict__ cols) {
+  const int64_t e = (int64_t)blockIdx.x * kTB + threadIdx.x;
+  if (e >= total) return;
+  int lo = 0, hi = ntb - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (fstart[mid] <= e) lo = mid;
+    else hi = mid - 1;
+  }
+  const int ti = lo;
+  int64_t i = e - fstart[ti];
+  int64_t pos = -1;
+  for (int q = at[ti]; q < at[ti] + mcnt[ti]; ++q) {
+    const int64_t len = mend[q] - mstart[q];
+    if (i < len) {
+      pos = mstart[q] + i;
+      break;
+    }
+    i -= len;
+  }
+  const int32_t* d = dofs + dptr[ti];
+  const int n = (int)(dptr[ti + 1] - dptr[ti]);
+  for (int c = 0; c < 3; ++c) {
+    const int v = fdof[3 * pos + c];
+    int a = 0, b = n - 1;
+    while (a < b) {
+      const int mid = (a + b) >> 1;
+      if (d[mid] < v) a = mid + 1;
+      else b = mid;
+    }
+    cols[3 * e + c] = a;
+  }
+}
+
 __global__ void k_unique_flags(int64_t n, const uint64_t* __restrict__ keys, int32_t* __restrict__ flags) {
   const int64_t i = (int64_t)blockIdx.x * kTB + threadIdx.x;
   if (i < n) flags[i] = (i == 0 || keys[i] != keys[i - 1]) ? 1 : 0;
@@ -464,7 +504,8 @@ __global__ void k_ns_fill(int nch, const uint64_t* __restrict__ order, const int
                           const int64_t* __restrict__ range_ptr, int32_t* __restrict__ row_out,
                           int32_t* __restrict__ row_block, int64_t* __restrict__ node_row, int32_t* __restrict__ col_idx,
                           int32_t* __restrict__ ncol_dof, int64_t* __restrict__ ranges,
-                          int64_t* __restrict__ range_prefix) {
+                          int64_t* __restrict__ range_prefix, const int64_t* __restrict__ fstart,
+                          int64_t* __restrict__ block_fstart) {
   const int k = blockIdx.x;
   const int c = (int)(order[k] & 0xFFFFFFu);  // the chunk index: the key's low 24 bits
   const int ti = chunk_ti[c];
@@ -484,6 +525,7 @@ __global__ void k_ns_fill(int nch, const uint64_t* __restrict__ order, const int
     ci[j] = j < ncd ? dofs[dptr[ti] + j] : (int32_t)(n_dof + (int64_t)b * C + (j < ncd + C ? j - ncd : C - 1));
   if (threadIdx.x == 0) {
     ncol_dof[k] = (int32_t)ncd;
+    block_fstart[k] = fstart[ti];
     int64_t pre = 0, q = range_ptr[k];
     for (int g = rat[ti]; g < rat[ti] + mcnt[ti]; ++g, ++q) {
       ranges[2 * q] = mstart[g];
@@ -669,6 +711,13 @@ void ns_dof_keys(hipStream_t s, int64_t total, int ntb, const int64_t* fstart, c
     hipLaunchKernelGGL(k_ns_dof_keys, dim3(grid_of(total)), dim3(kTB), 0, s, total, ntb, fstart, at, mcnt, mstart, mend,
                        fdof, keys);
 }
+void ns_face_cols(hipStream_t s, int64_t total, int ntb, const int64_t* fstart, const int32_t* at, const int32_t* mcnt,
+                  const int64_t* mstart, const int64_t* mend, const int32_t* fdof, const int64_t* dptr,
+                  const int32_t* dofs, int32_t* cols) {
+  if (total > 0)
+    hipLaunchKernelGGL(k_ns_face_cols, dim3(grid_of(total)), dim3(kTB), 0, s, total, ntb, fstart, at, mcnt, mstart,
+                       mend, fdof, dptr, dofs, cols);
+}
 void unique_flags(hipStream_t s, int64_t n, const uint64_t* keys, int32_t* flags) {
   if (n > 0) hipLaunchKernelGGL(k_unique_flags, dim3(grid_of(n)), dim3(kTB), 0, s, n, keys, flags);
 }
@@ -696,11 +745,11 @@ void ns_fill(hipStream_t s, int nch, const uint64_t* order, const int32_t* chunk
              const int32_t* rat, const int32_t* mcnt, const int64_t* mstart, const int64_t* mend, int C, int n_dof,
              const int64_t* row_ptr, const int64_t* col_ptr, const int64_t* range_ptr, int32_t* row_out,
              int32_t* row_block, int64_t* node_row, int32_t* col_idx, int32_t* ncol_dof, int64_t* ranges,
-             int64_t* range_prefix) {
+             int64_t* range_prefix, const int64_t* fstart, int64_t* block_fstart) {
   if (nch > 0)
     hipLaunchKernelGGL(k_ns_fill, dim3(nch), dim3(kTB), 0, s, nch, order, chunk_ti, chunk_at, tptr, tnodes, tbox, dptr,
                        dofs, rat, mcnt, mstart, mend, C, n_dof, row_ptr, col_ptr, range_ptr, row_out, row_block,
-                       node_row, col_idx, ncol_dof, ranges, range_prefix);
+                       node_row, col_idx, ncol_dof, ranges, range_prefix, fstart, block_fstart);
 }
 void l2p_coef(hipStream_t s, int64_t nrows, int P, const int32_t* row_out, const int32_t* target, const double* X,
               const BoxGeom* geom, const int32_t* level, const double* level_side, const double* cnm, double* out) {

@@ -719,6 +719,21 @@ void ns_dof_keys(hipStream_t s, int64_t total, int ntb, const int64_t* fstart, c
     (void)total;
   });
 }
+void ns_face_cols(hipStream_t s, int64_t total, int ntb, const int64_t* fstart, const int32_t* at, const int32_t* mcnt,
+                  const int64_t* mstart, const int64_t* mend, const int32_t* fdof, const int64_t* dptr,
+                  const int32_t* dofs, int32_t* cols) {
+  hipstub::enqueue(s, [=]() {
+    for (int ti = 0; ti < ntb; ++ti) {
+      int64_t e = fstart[ti];
+      const int32_t* d = dofs + dptr[ti];
+      const int64_t n = dptr[ti + 1] - dptr[ti];
+      for (int q = at[ti]; q < at[ti] + mcnt[ti]; ++q)
+        for (int64_t pos = mstart[q]; pos < mend[q]; ++pos, ++e)
+          for (int c = 0; c < 3; ++c) cols[3 * e + c] = (int32_t)(std::lower_bound(d, d + n, fdof[3 * pos + c]) - d);
+    }
+    (void)total;
+  });
+}
 void unique_flags(hipStream_t s, int64_t n, const uint64_t* keys, int32_t* flags) {
   hipstub::enqueue(s, [=]() {
     for (int64_t i = 0; i < n; ++i) flags[i] = (i == 0 || keys[i] != keys[i - 1]) ? 1 : 0;
@@ -778,7 +793,7 @@ void ns_fill(hipStream_t s, int nch, const uint64_t* order, const int32_t* chunk
              const int32_t* rat, const int32_t* mcnt, const int64_t* mstart, const int64_t* mend, int C, int n_dof,
              const int64_t* row_ptr, const int64_t* col_ptr, const int64_t* range_ptr, int32_t* row_out,
              int32_t* row_block, int64_t* node_row, int32_t* col_idx, int32_t* ncol_dof, int64_t* ranges,
-             int64_t* range_prefix) {
+             int64_t* range_prefix, const int64_t* fstart, int64_t* block_fstart) {
   hipstub::enqueue(s, [=]() {
     for (int k = 0; k < nch; ++k) {
       const int c = (int)(order[k] & 0xFFFFFFu);  // the chunk index: the key's low 24 bits
@@ -798,6 +813,7 @@ void ns_fill(hipStream_t s, int nch, const uint64_t* order, const int32_t* chunk
       for (int64_t j = 0; j < ncol; ++j)
         ci[j] = j < ncd ? dofs[dptr[ti] + j] : (int32_t)(n_dof + (int64_t)b * C + (j < ncd + C ? j - ncd : C - 1));
       ncol_dof[k] = (int32_t)ncd;
+      block_fstart[k] = fstart[ti];
       int64_t pre = 0, q = range_ptr[k];
       for (int g = rat[ti]; g < rat[ti] + mcnt[ti]; ++g, ++q) {
         ranges[2 * q] = mstart[g];
@@ -984,8 +1000,10 @@ void near_quad(hipStream_t s, int64_t nrows, const NearQuad& a, const Gemv& g, d
       }
       const double x = A.Xs[3 * t], y = A.Xs[3 * t + 1], z = A.Xs[3 * t + 2];
       const int32_t* cols = G.col_idx + c0;
+      const int32_t* fcols = A.face_cols ? A.face_cols + 3 * A.block_fstart[b] : nullptr;
+      int64_t li = 0;  // local face index in the block's ranges
       for (int64_t gi = A.range_ptr[b]; gi < A.range_ptr[b + 1]; ++gi)
-        for (int64_t fi = A.ranges[2 * gi]; fi < A.ranges[2 * gi + 1]; ++fi) {
+        for (int64_t fi = A.ranges[2 * gi]; fi < A.ranges[2 * gi + 1]; ++fi, ++li) {
           const int64_t f = fi;  // face arrays in face_perm order
           int slot = -1;
           for (int k = 0; k < 3; ++k)
@@ -1001,6 +1019,7 @@ void near_quad(hipStream_t s, int64_t nrows, const NearQuad& a, const Gemv& g, d
               if (cols[mid] < d) lo2 = mid + 1;
               else hi2 = mid;
             }
+            if (fcols && fcols[3 * li + k] != lo2) throw std::runtime_error("near quadrature: face column table");
             A0[lo2] += WG[k];
             A1[lo2] += WGn[k];
           }
