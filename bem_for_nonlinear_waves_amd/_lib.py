@@ -30,7 +30,7 @@ class Opts(ctypes.Structure):
                 ("ilu_neighborhood", ctypes.c_int), ("ilu_kring", ctypes.c_int), ("schwarz_variant", ctypes.c_int),
                 ("schwarz_core_k", ctypes.c_int), ("milu_omega", ctypes.c_double), ("ilut_drop_tol", ctypes.c_double),
                 ("ilut_max_entries", ctypes.c_int), ("ilut_pivot_min", ctypes.c_double),
-                ("plane_wave_digits", ctypes.c_int), ("tree_on_gpu", ctypes.c_int)]
+                ("plane_wave_digits", ctypes.c_int), ("tree_on_gpu", ctypes.c_int), ("element_order", ctypes.c_int)]
 
 
 class Result(ctypes.Structure):

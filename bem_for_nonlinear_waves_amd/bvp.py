@@ -55,7 +55,7 @@ class BEM_BVP:
                  ilu_neighborhood: int = 0, ilu_kring: int = 1, schwarz_variant: int = 0, schwarz_core_k: int = 1,
                  milu_omega: float = 1.0, ilut_drop_tol: float = 1e-3, ilut_max_entries: int = 50,
                  ilut_pivot_min: float = 1e-12, plane_wave_digits: int = 10, tree_on_gpu: int = 1,
-                 device: int = 0):
+                 element_order: int = 1, device: int = 0):
         self.lib = _lib.load()
         o = _lib.Opts()
         self.lib.bemfmm_default_opts(ctypes.byref(o))
@@ -70,6 +70,8 @@ class BEM_BVP:
                                                                               ilut_max_entries, ilut_pivot_min)
         o.plane_wave_digits = plane_wave_digits
         o.tree_on_gpu = tree_on_gpu
+        o.element_order = element_order
+        self.element_order = element_order
         self.ctx = self.lib.bemfmm_ctx_create(device, ctypes.byref(o))
         if not self.ctx:
             raise BEMError(_lib.E_DEVICE, "bemfmm_ctx_create failed (no usable GPU?)")
@@ -80,6 +82,9 @@ class BEM_BVP:
         """New nodes / DOF model (an RK stage moves the free surface); call prepare() before solving.
         The context keeps its device operator table across geometries."""
         self.n_dof = int(dofs.n_dof)
+        npf = 6 if self.element_order == 2 else 3
+        if np.shape(mesh.tri)[1] != npf or np.shape(dofs.face_dof)[1] != npf:
+            raise ValueError(f"element_order {self.element_order} needs {npf} nodes and DOF entries per face")
         self._geom = [_d(mesh.X), _i(mesh.tri), _i(dofs.face_dof), _i(dofs.dof_kind), _i(dofs.dof_point)]
         X, tri, fd, kd, pt = self._geom
         self._check(self.lib.bemfmm_set_geometry(self.ctx, mesh.n_nodes, _ptr(X, _lib._dp), mesh.n_faces,

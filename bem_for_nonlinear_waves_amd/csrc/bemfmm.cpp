@@ -347,7 +347,7 @@ struct bemfmm_ctx {
     DBuf<unsigned char> sort_tmp;
     DBuf<int32_t> row_block, ncol_dof, face_perm, fv, fdof;
     DBuf<int64_t> range_ptr, ranges, range_prefix;
-    DBuf<double> Xs, fX, fcent, fcr, fJ, l2p, d_dun;
+    DBuf<double> Xs, fX, fcent, fcr, fJ, l2p, d_dun, d_dun6;
     DBuf<int64_t> fptr;
     DBuf<int32_t> pfaces, plc;
     DBuf<double> pcs, dcnm;
@@ -620,8 +620,13 @@ void dof_order(bemfmm_ctx* c) {
   for (int i = 0; i < n; ++i)
     if (c->dof_point[i] < 0 || c->dof_point[i] >= c->n_nodes || c->dof_kind[i] < 0 || c->dof_kind[i] > 3)
       throw std::runtime_error("bemfmm_prepare: dof_point or dof_kind out of range");
-  for (int32_t d : c->face_dof)
+  // quadratic faces: an edge midpoint may carry no DOF (-1), its weight then goes to the edge's two vertices
+  const size_t npf = c->opts.element_order == 2 ? 6 : 3;
+  for (size_t k = 0; k < c->face_dof.size(); ++k) {
+    const int32_t d = c->face_dof[k];
+    if (d == -1 && k % npf >= 3) continue;
     if (d < 0 || d >= n) throw std::runtime_error("bemfmm_prepare: face_dof out of range");
+  }
   for (int32_t v : c->tri)
     if (v < 0 || v >= c->n_nodes) throw std::runtime_error("bemfmm_prepare: tri out of range");
   c->perm.resize(n);
@@ -664,7 +669,8 @@ void dof_order(bemfmm_ctx* c) {
     if (j != i) c->perm_identity = false;
   }
   c->face_dof_i.resize(c->face_dof.size());
-  for (size_t k = 0; k < c->face_dof.size(); ++k) c->face_dof_i[k] = c->inv[c->face_dof[k]];
+  for (size_t k = 0; k < c->face_dof.size(); ++k)
+    c->face_dof_i[k] = c->face_dof[k] < 0 ? -1 : c->inv[c->face_dof[k]];  // -1: edge midpoint without a DOF
   c->kind_i.resize(n);
   c->point_i.resize(n);
   for (int i = 0; i < n; ++i) {
@@ -1204,17 +1210,19 @@ void near_struct_device(bemfmm_ctx* c, Plan& pl) {
   dscan_i64(c, W.nfaces.p, W.fstart.p, ntb + 1);
   const int64_t nfc = dread(c, W.fstart.p + ntb);
   // DOF columns per target box: sorted, unique
-  W.dk0.alloc(3 * nfc);
-  W.dk1.alloc(3 * nfc);
-  W.uflag.alloc(3 * nfc);
-  W.uat.alloc(3 * nfc);
-  if (3 * nfc > INT32_MAX) throw std::runtime_error("near field too large for the device structure build");
-  dev::ns_dof_keys(s, nfc, ntb, W.fstart.p, W.rat.p, W.mcnt.p, W.mstart.p, W.mend.p, c->sb.fdof.p, W.dk0.p);
-  dsort_keys(c, W.dk0.p, W.dk1.p, 3 * nfc, 32 + bits_for((uint64_t)ntb));
-  dev::unique_flags(s, 3 * nfc, W.dk1.p, W.uflag.p);
-  dscan_i32(c, true, W.uflag.p, W.uat.p, 3 * nfc);
-  const int64_t nu = dread(c, W.uat.p + (3 * nfc - 1));
-  dev::compact_u64(s, 3 * nfc, W.dk1.p, W.uflag.p, W.uat.p, W.dk0.p);
+  const int npf = pl.npf;
+  const int64_t nkeys = npf * nfc;
+  W.dk0.alloc(nkeys);
+  W.dk1.alloc(nkeys);
+  W.uflag.alloc(nkeys);
+  W.uat.alloc(nkeys);
+  if (nkeys > INT32_MAX) throw std::runtime_error("near field too large for the device structure build");
+  dev::ns_dof_keys(s, npf, nfc, ntb, W.fstart.p, W.rat.p, W.mcnt.p, W.mstart.p, W.mend.p, c->sb.fdof.p, W.dk0.p);
+  dsort_keys(c, W.dk0.p, W.dk1.p, nkeys, 32 + bits_for((uint64_t)ntb));
+  dev::unique_flags(s, nkeys, W.dk1.p, W.uflag.p);
+  dscan_i32(c, true, W.uflag.p, W.uat.p, nkeys);
+  const int64_t nu = dread(c, W.uat.p + (nkeys - 1));
+  dev::compact_u64(s, nkeys, W.dk1.p, W.uflag.p, W.uat.p, W.dk0.p);
   W.dptr.alloc(ntb + 1);
   W.dofs.alloc(nu);
   dev::csr_from_keys(s, ntb, W.dk0.p, nu, W.dptr.p, W.dofs.p);
@@ -1258,8 +1266,8 @@ void near_struct_device(bemfmm_ctx* c, Plan& pl) {
   c->sb.ranges.alloc(2 * nrng);
   c->sb.range_prefix.alloc(nrng);
   W.block_fstart.alloc(nch);
-  W.face_cols.alloc(3 * nfc);
-  dev::ns_face_cols(s, nfc, ntb, W.fstart.p, W.rat.p, W.mcnt.p, W.mstart.p, W.mend.p, c->sb.fdof.p, W.dptr.p,
+  W.face_cols.alloc(nkeys);
+  dev::ns_face_cols(s, npf, nfc, ntb, W.fstart.p, W.rat.p, W.mcnt.p, W.mstart.p, W.mend.p, c->sb.fdof.p, W.dptr.p,
                     W.dofs.p, W.face_cols.p);
   dev::ns_fill(s, nch, W.ck1.p, W.cti.p, W.cat.p, W.tptr.p, W.tnodes.p, W.tbox.p, W.dptr.p, W.dofs.p, W.rat.p,
                W.mcnt.p, W.mstart.p, W.mend.p, C, pl.n_dof, c->near.row_ptr.p, c->near.col_ptr.p, c->sb.range_ptr.p,
@@ -1304,6 +1312,14 @@ void do_prepare(bemfmm_ctx* c) {
   if (c->opts.m2l_tile_pairs != 0 && c->opts.m2l_tile_pairs != 16 && c->opts.m2l_tile_pairs != 32 &&
       c->opts.m2l_tile_pairs != 64)
     throw std::runtime_error("bemfmm_prepare: m2l_tile_pairs must be 0 (default), 16, 32 or 64");
+  if (c->opts.element_order != 1 && c->opts.element_order != 2)
+    throw std::runtime_error("bemfmm_prepare: element_order must be 1 (linear) or 2 (true quadratic)");
+  if (c->opts.element_order == 2 &&
+      (c->opts.preconditioner == BEMFMM_PRECOND_ILU0 || c->opts.preconditioner == BEMFMM_PRECOND_MILU0 ||
+       c->opts.preconditioner == BEMFMM_PRECOND_ILUT ||
+       (c->opts.preconditioner == BEMFMM_PRECOND_SCHWARZ && c->opts.schwarz_variant == 1)))
+    throw std::runtime_error("bemfmm_prepare: quadratic elements take the NONE, JACOBI or mesh-graph SCHWARZ "
+                             "preconditioner (the sparse preconditioner matrix is built for linear faces)");
   if (c->opts.m2l_kernel < 0 || c->opts.m2l_kernel > 4)
     throw std::runtime_error("bemfmm_prepare: m2l_kernel must be 0 (default), 1 (matrix cores), 2 (rotation), "
                              "3 (plane waves, FP32 operator) or 4 (plane waves, both operators)");
@@ -1336,6 +1352,7 @@ void do_prepare(bemfmm_ctx* c) {
   o.leaf_max = c->opts.leaf_max;
   o.max_level = c->opts.max_level;
   o.q = c->opts.q;
+  o.npf = c->opts.element_order == 2 ? 6 : 3;
   o.materialize_ops = false;  // operators go from the float128 cache straight into the device table
   o.fill_p2m = false;         // the P2M matrices are built on the GPU (k_p2m_build)
   c->prepared = false;
@@ -1431,21 +1448,24 @@ void do_prepare(bemfmm_ctx* c) {
   face_perm.upload(pl.face_perm, s);
   // per-face arrays on the device in the depth-first face order (position in face_perm): the faces of a box
   // and of a near range are consecutive, so the quadrature and P2M kernels read them coalesced
-  std::vector<int32_t> fv_p(3 * (size_t)c->n_faces), fdof_p(3 * (size_t)c->n_faces);
-  std::vector<double> fX_p(9 * (size_t)c->n_faces), fcent_p(3 * (size_t)c->n_faces), fcr_p(3 * (size_t)c->n_faces),
+  const int npf = pl.npf;
+  std::vector<int32_t> fv_p((size_t)npf * c->n_faces), fdof_p((size_t)npf * c->n_faces);
+  std::vector<double> fX_p((size_t)3 * npf * c->n_faces), fcent_p(3 * (size_t)c->n_faces), fcr_p(3 * (size_t)c->n_faces),
       fJ_p(c->n_faces);
   std::vector<int32_t> face_pos(c->n_faces);
 #pragma omp parallel for schedule(static)
   for (int k = 0; k < c->n_faces; ++k) {
     const size_t f = (size_t)pl.face_perm[k];
     face_pos[f] = k;
+    for (int a = 0; a < npf; ++a) {
+      fv_p[(size_t)npf * k + a] = pl.fv[npf * f + a];
+      fdof_p[(size_t)npf * k + a] = pl.fdof[npf * f + a];
+    }
     for (int a = 0; a < 3; ++a) {
-      fv_p[3 * (size_t)k + a] = pl.fv[3 * f + a];
-      fdof_p[3 * (size_t)k + a] = pl.fdof[3 * f + a];
       fcent_p[3 * (size_t)k + a] = pl.fcent[3 * f + a];
       fcr_p[3 * (size_t)k + a] = pl.fcr[3 * f + a];
     }
-    for (int a = 0; a < 9; ++a) fX_p[9 * (size_t)k + a] = pl.fX[9 * f + a];
+    for (int a = 0; a < 3 * npf; ++a) fX_p[(size_t)3 * npf * k + a] = pl.fX[3 * npf * f + a];
     fJ_p[k] = pl.fJ[f];
   }
   fv.upload(fv_p, s);
@@ -1495,10 +1515,11 @@ void do_prepare(bemfmm_ctx* c) {
     for (int g = 0; g < o.q; ++g)
       for (int a = 0; a < 4; ++a) dun[g][a] = src[g][a];
     d_dun.upload(std::vector<double>(&dun[0][0], &dun[0][0] + 28), s);
+    c->sb.d_dun6.upload(std::vector<double>(&d6[0][0], &d6[0][0] + 24), s);  // far rule of quadratic faces
   }
   // ---- P2M matrices on the GPU ----
   if (c->p2m.nblk > 0) {
-    if ((int)pl.p2m_face_ptr.size() != c->p2m.nblk + 1 || pl.p2m_face_lc.size() != 3 * pl.p2m_faces.size())
+    if ((int)pl.p2m_face_ptr.size() != c->p2m.nblk + 1 || pl.p2m_face_lc.size() != (size_t)npf * pl.p2m_faces.size())
       throw std::runtime_error("P2M source lists do not match the P2M blocks");
     const int NT = (pl.P + 1) * (pl.P + 2) / 2;
     std::vector<double> cnm(NT);
@@ -1524,6 +1545,7 @@ void do_prepare(bemfmm_ctx* c) {
     dcnm.upload(cnm, s);
     dev::P2MBuild a;
     a.q = o.q;
+    a.npf = npf;
     a.CP = pl.p2m_rows_pad;
     a.nleaves = c->p2m.nblk;
     a.face_ptr = fptr.p;
@@ -1549,6 +1571,8 @@ void do_prepare(bemfmm_ctx* c) {
     dev::NearQuad a;
     a.C = pl.C;
     a.q = o.q;
+    a.npf = pl.npf;
+    a.dun6_rule = c->sb.d_dun6.p;
     for (int32_t v : pl.near_ncol_dof) a.max_ncd = std::max(a.max_ncd, (int)v);
     if (dev_plan) {
       a.face_cols = c->nbuf.face_cols.p;
@@ -1686,7 +1710,7 @@ void do_prepare(bemfmm_ctx* c) {
       } else {
         bemfmm::build_schwarz_blocks(c->n_dof, c->n_faces, c->face_dof_i.data(), c->point_i.data(), seed,
                                      c->opts.schwarz_max_core, c->opts.schwarz_max_block, c->opts.schwarz_overlap,
-                                     c->sw_host);
+                                     c->sw_host, pl.npf);
         c->sw_caller = c->sw_host;
         for (int32_t& g : c->sw_caller.idx) g = c->perm[g];
         c->sw_topo_key = key;
@@ -2492,6 +2516,7 @@ void bemfmm_default_opts(bemfmm_opts* o) {
   o->ilut_pivot_min = 1e-12;  // main_time_domain.cpp:44
   o->plane_wave_digits = 10;
   o->tree_on_gpu = 1;
+  o->element_order = 1;
 }
 
 bemfmm_ctx* bemfmm_ctx_create(int device, const bemfmm_opts* opts) {
@@ -2551,8 +2576,10 @@ int bemfmm_set_geometry(bemfmm_ctx* c, int n_nodes, const double* X, int n_faces
     c->n_faces = n_faces;
     c->n_dof = n_dof;
     c->X.assign(X, X + 3 * (size_t)n_nodes);
-    c->tri.assign(tri, tri + 3 * (size_t)n_faces);
-    c->face_dof.assign(face_dof, face_dof + 3 * (size_t)n_faces);
+    // element_order 2: six nodes and six DOF entries per face (vertices, then edge midpoints)
+    const size_t npf = c->opts.element_order == 2 ? 6 : 3;
+    c->tri.assign(tri, tri + npf * (size_t)n_faces);
+    c->face_dof.assign(face_dof, face_dof + npf * (size_t)n_faces);
     c->dof_kind.assign(dof_kind, dof_kind + n_dof);
     c->dof_point.assign(dof_point, dof_point + n_dof);
     c->have_geom = true;

@@ -743,7 +743,10 @@ __device__ __forceinline__ void rot_apply(const float4* __restrict__ R4, float* 
 // grad R_n^m . n = cnm (-a NR_{n-1}^{m+1} + b NR_{n-1}^{m-1} + n_z NR_{n-1}^m), a = (n_x - i n_y)/2,
 // b = (n_x + i n_y)/2, NR_n^{-m} = (-1)^m conj NR_n^m.
 constexpr int kP2MPts = 16;
-template <int P>
+// NPF = 3: linear triangles (fX 9 values per face, one normal); NPF = 6: true quadratic faces (fX 18 values, the
+// point and normal from the 6-node geometry at each Dunavant point, densities weighted by the redistributed
+// TriShape<6>; a face-local column of -1 marks an edge midpoint without a DOF).
+template <int P, int NPF>
 __global__ __launch_bounds__(256) void k_p2m_build(int q, int CP, const int64_t* __restrict__ face_ptr,
                                                    const int32_t* __restrict__ faces, const int32_t* __restrict__ face_lc,
                                                    const double* __restrict__ leaf_cs, const int64_t* __restrict__ mat_ptr,
@@ -754,8 +757,8 @@ __global__ __launch_bounds__(256) void k_p2m_build(int q, int CP, const int64_t*
   constexpr int C = (P + 1) * (P + 1);
   constexpr int NT = (P + 1) * (P + 2) / 2;
   __shared__ double nre[kP2MPts][NT], nim[kP2MPts][NT];
-  __shared__ double pw[kP2MPts][3], pn[kP2MPts][3];
-  __shared__ int plc[kP2MPts][3];
+  __shared__ double pw[kP2MPts][NPF], pn[kP2MPts][3];
+  __shared__ int plc[kP2MPts][NPF];
   const int l = blockIdx.x;
   const int ncol = (int)(col_ptr[l + 1] - col_ptr[l]);
   double* M = mat + mat_ptr[l];
@@ -781,17 +784,36 @@ __global__ __launch_bounds__(256) void k_p2m_build(int q, int CP, const int64_t*
       const int fi = (p0 + tid) / q, j = (p0 + tid) - fi * q;
       const int f = faces[f0 + fi];
       const double b0 = rule[4 * j], b1 = rule[4 * j + 1], b2 = rule[4 * j + 2], w = rule[4 * j + 3];
-      const double J = fJ[f];
-      const double x = (b0 * fX[9 * f] + b1 * fX[9 * f + 3] + b2 * fX[9 * f + 6] - cx) / s;
-      const double y = (b0 * fX[9 * f + 1] + b1 * fX[9 * f + 4] + b2 * fX[9 * f + 7] - cy) / s;
-      const double z = (b0 * fX[9 * f + 2] + b1 * fX[9 * f + 5] + b2 * fX[9 * f + 8] - cz) / s;
-      const double W = J * w;
-      pw[tid][0] = W * b0;
-      pw[tid][1] = W * b1;
-      pw[tid][2] = W * b2;
-      for (int a = 0; a < 3; ++a) {
-        plc[tid][a] = face_lc[3 * (f0 + fi) + a];
-        pn[tid][a] = fcr[3 * f + a] / J;
+      double x, y, z;
+      if constexpr (NPF == 3) {
+        const double J = fJ[f];
+        x = (b0 * fX[9 * f] + b1 * fX[9 * f + 3] + b2 * fX[9 * f + 6] - cx) / s;
+        y = (b0 * fX[9 * f + 1] + b1 * fX[9 * f + 4] + b2 * fX[9 * f + 7] - cy) / s;
+        z = (b0 * fX[9 * f + 2] + b1 * fX[9 * f + 5] + b2 * fX[9 * f + 8] - cz) / s;
+        const double W = J * w;
+        pw[tid][0] = W * b0;
+        pw[tid][1] = W * b1;
+        pw[tid][2] = W * b2;
+        for (int a = 0; a < 3; ++a) {
+          plc[tid][a] = face_lc[3 * (f0 + fi) + a];
+          pn[tid][a] = fcr[3 * f + a] / J;
+        }
+      } else {
+        double xq[3], cr[3], N[6];
+        bemfmm::tri6_geom(fX + 18 * (int64_t)f, b0, b1, xq, cr);
+        unsigned active = 0;
+        for (int e = 0; e < 3; ++e) active |= (face_lc[6 * (f0 + fi) + 3 + e] >= 0 ? 1u : 0u) << e;
+        bemfmm::tri6_density(b0, b1, active, N);
+        const double Jq = sqrt(cr[0] * cr[0] + cr[1] * cr[1] + cr[2] * cr[2]);
+        x = (xq[0] - cx) / s;
+        y = (xq[1] - cy) / s;
+        z = (xq[2] - cz) / s;
+        const double W = Jq * w;
+        for (int k = 0; k < 6; ++k) {
+          pw[tid][k] = W * N[k];
+          plc[tid][k] = face_lc[6 * (f0 + fi) + k];
+        }
+        for (int a = 0; a < 3; ++a) pn[tid][a] = Jq > 1e-30 ? cr[a] / Jq : fcr[3 * f + a] / fJ[f];
       }
       double* re = nre[tid];
       double* im = nim[tid];
@@ -850,7 +872,8 @@ __global__ __launch_bounds__(256) void k_p2m_build(int q, int CP, const int64_t*
           const double di = -(ar * p1i + ai * p1r) + (br * m1i + bi * m1r) + nz * z0i;
           v = rim ? -(c / s) * di : (c / s) * dr;
         }
-        for (int k = 0; k < 3; ++k) M[(int64_t)(layer * ncol + plc[pt][k]) * CP + row] += pw[pt][k] * v;
+        for (int k = 0; k < NPF; ++k)
+          if (NPF == 3 || plc[pt][k] >= 0) M[(int64_t)(layer * ncol + plc[pt][k]) * CP + row] += pw[pt][k] * v;
       }
     }
     __syncthreads();
@@ -1029,6 +1052,7 @@ __global__ void k_assemble(int mode, int n, const int32_t* __restrict__ kind, co
 // Near-field quadrature: one workgroup per near row (target node); threads stride over the
 // faces of the target box's near ranges and scatter (WG, WGn) of the three corners into the row.
 // fill_direct_linear / fill_nonadj_linear (BEM_solveBVP_GMRES_FMM.hpp:1824-1891).
+template <int NPF>
 __global__ __launch_bounds__(256) void k_near_quad(
     int C, int q, double near_region, const int32_t* __restrict__ row_block, const int64_t* __restrict__ row_ptr,
     const int64_t* __restrict__ col_ptr, const int64_t* __restrict__ mat_ptr, const int32_t* __restrict__ row_out,
@@ -1037,14 +1061,16 @@ __global__ __launch_bounds__(256) void k_near_quad(
     const double* __restrict__ Xs, const double* __restrict__ fX, const double* __restrict__ fcent,
     const double* __restrict__ fcr, const double* __restrict__ fJ, const int32_t* __restrict__ fv,
     const int32_t* __restrict__ fdof, const double* __restrict__ l2p, const double* __restrict__ dun_rule,
-    const int32_t* __restrict__ face_cols, const int64_t* __restrict__ block_fstart, int lds_cols,
+    const double* __restrict__ dun6_rule, const int32_t* __restrict__ face_cols, const int64_t* __restrict__ block_fstart, int lds_cols,
     double* __restrict__ mat) {
   // a row of at most lds_cols DOF columns accumulates in LDS (ds_add_f64) and is written once; a longer one
   // adds into the zero-filled matrix with global atomics
   extern __shared__ __align__(16) double acc[];  // 2 * lds_cols
   // Dunavant rule of the non-adjacent, non-near faces ({b0,b1,b2,w} x q), per context
-  __shared__ double dun[7][4];
+  __shared__ double dun[7][4], dun6[6][4];
   for (int i = threadIdx.x; i < 28; i += blockDim.x) dun[i / 4][i % 4] = dun_rule[i];
+  if (NPF == 6)
+    for (int i = threadIdx.x; i < 24; i += blockDim.x) dun6[i / 4][i % 4] = dun6_rule[i];
   __syncthreads();
   const int64_t r = blockIdx.x;
   const int b = row_block[r];
@@ -1069,7 +1095,7 @@ __global__ __launch_bounds__(256) void k_near_quad(
   const int64_t g0 = range_ptr[b], g1 = range_ptr[b + 1];
   const int64_t nf = range_prefix[g1 - 1] + (ranges[2 * (g1 - 1) + 1] - ranges[2 * (g1 - 1)]);
   const int32_t* cols = col_idx + c0;
-  const int32_t* fcols = face_cols ? face_cols + 3 * block_fstart[b] : nullptr;
+  const int32_t* fcols = face_cols ? face_cols + NPF * block_fstart[b] : nullptr;
   for (int64_t i = threadIdx.x; i < nf; i += blockDim.x) {
     // locate the range holding local face i
     int64_t lo = g0, hi = g1 - 1;
@@ -1080,18 +1106,26 @@ __global__ __launch_bounds__(256) void k_near_quad(
     }
     const int64_t f = ranges[2 * lo] + (i - range_prefix[lo]);  // face arrays are in face_perm order
     int slot = -1;
-    for (int k = 0; k < 3; ++k)
-      if (fv[3 * f + k] == t) slot = k;
-    double WG[3], WGn[3];
-    bemfmm::face_weights(fX + 9 * f, fJ[f], fcr + 3 * f, fcent + 3 * f, slot, x, y, z,
-                         near_region, c_gw5, dun, q, WG, WGn);
+    for (int k = 0; k < NPF; ++k)
+      if (fv[NPF * f + k] == t) slot = k;
+    double WG[NPF], WGn[NPF];
+    if constexpr (NPF == 3) {
+      bemfmm::face_weights(fX + 9 * f, fJ[f], fcr + 3 * f, fcent + 3 * f, slot, x, y, z, near_region, c_gw5, dun, q, WG,
+                           WGn);
+    } else {
+      unsigned active = 0;
+      for (int e = 0; e < 3; ++e) active |= (fdof[6 * f + 3 + e] >= 0 ? 1u : 0u) << e;
+      bemfmm::face_weights_quad(fX + 18 * f, active, fcent + 3 * f, slot, x, y, z, near_region, c_gw5, dun, q, dun6,
+                                WG, WGn);
+    }
 #pragma unroll
-    for (int k = 0; k < 3; ++k) {
+    for (int k = 0; k < NPF; ++k) {
+      const int d = fdof[NPF * f + k];
+      if (NPF == 6 && d < 0) continue;  // edge midpoint without a DOF: its weights were redistributed
       int lo2 = 0;
       if (fcols) {
-        lo2 = fcols[3 * i + k];
+        lo2 = fcols[NPF * i + k];
       } else {
-        const int d = fdof[3 * f + k];
         int hi2 = ncd - 1;
         while (lo2 < hi2) {
           const int mid = (lo2 + hi2) >> 1;
@@ -1732,8 +1766,12 @@ hipError_t radix_sort_pairs(hipStream_t s, void* tmp, size_t& tmp_bytes, const u
 void p2m_build(hipStream_t s, int P, const P2MBuild& a, double* mat) {
   if (a.nleaves == 0) return;
 #define BEMFMM_P2MB(PV)                                                                                              \
-  hipLaunchKernelGGL(k_p2m_build<PV>, dim3(a.nleaves), dim3(256), 0, s, a.q, a.CP, a.face_ptr, a.faces, a.face_lc, \
-                     a.leaf_cs, a.mat_ptr, a.col_ptr, a.fX, a.fcr, a.fJ, a.rule, a.cnm, mat)
+  if (a.npf == 6)                                                                                                    \
+    hipLaunchKernelGGL((k_p2m_build<PV, 6>), dim3(a.nleaves), dim3(256), 0, s, a.q, a.CP, a.face_ptr, a.faces,      \
+                       a.face_lc, a.leaf_cs, a.mat_ptr, a.col_ptr, a.fX, a.fcr, a.fJ, a.rule, a.cnm, mat);           \
+  else                                                                                                               \
+    hipLaunchKernelGGL((k_p2m_build<PV, 3>), dim3(a.nleaves), dim3(256), 0, s, a.q, a.CP, a.face_ptr, a.faces,      \
+                       a.face_lc, a.leaf_cs, a.mat_ptr, a.col_ptr, a.fX, a.fcr, a.fJ, a.rule, a.cnm, mat)
   switch (P) {
   case 1: BEMFMM_P2MB(1); break;
   case 2: BEMFMM_P2MB(2); break;
@@ -1754,10 +1792,11 @@ void near_quad(hipStream_t s, int64_t nrows, const NearQuad& a, const Gemv& g, d
   if (nrows == 0) return;
   // rows of up to kNearQuadLdsCols DOF columns (most of them) accumulate in 64 KB of LDS: two workgroups per CU
   const int lds_cols = std::min(a.max_ncd, kNearQuadLdsCols);
-  hipLaunchKernelGGL(k_near_quad, dim3((unsigned)nrows), dim3(256), (size_t)2 * lds_cols * sizeof(double), s, a.C,
-                     a.q, a.near_region, a.row_block, g.row_ptr, g.col_ptr, g.mat_ptr, g.row_out, g.col_idx,
-                     a.ncol_dof, a.range_ptr, a.ranges, a.range_prefix, a.face_perm, a.Xs, a.fX, a.fcent, a.fcr, a.fJ,
-                     a.fv, a.fdof, a.l2p, a.dun_rule, a.face_cols, a.block_fstart, lds_cols, mat);
+  auto kern = a.npf == 6 ? k_near_quad<6> : k_near_quad<3>;
+  hipLaunchKernelGGL(kern, dim3((unsigned)nrows), dim3(256), (size_t)2 * lds_cols * sizeof(double), s, a.C, a.q,
+                     a.near_region, a.row_block, g.row_ptr, g.col_ptr, g.mat_ptr, g.row_out, g.col_idx, a.ncol_dof,
+                     a.range_ptr, a.ranges, a.range_prefix, a.face_perm, a.Xs, a.fX, a.fcent, a.fcr, a.fJ, a.fv, a.fdof,
+                     a.l2p, a.dun_rule, a.dun6_rule, a.face_cols, a.block_fstart, lds_cols, mat);
 }
 
 void to_f32(hipStream_t s, int64_t n, const double* a, float* b) {

@@ -112,6 +112,8 @@ struct Tiles {
 constexpr int kNearQuadLdsCols = 4096;  // DOF columns of a near row accumulated in LDS (2 x 32 KB)
 struct NearQuad {
   int C = 0, q = 0;
+  int npf = 3;                        // nodes per face: 3 linear, 6 true quadratic (fX 18, fv / fdof 6 per face)
+  const double* dun6_rule = nullptr;  // quadratic faces: the 6-point Dunavant rule of far non-adjacent faces
   int max_ncd = 0;  // largest DOF column count of a near block
   // optional: the local column of each face corner, block b's faces from 3 * block_fstart[b] (else searched)
   const int32_t* face_cols = nullptr;
@@ -151,6 +153,7 @@ struct NearRows {
 // P2M matrix build inputs (k_p2m_build)
 struct P2MBuild {
   int q = 0, CP = 0, nleaves = 0;
+  int npf = 3;  // nodes (and DOF columns) per face: 3 linear, 6 true quadratic
   const int64_t* face_ptr = nullptr;
   const int32_t* faces = nullptr;
   const int32_t* face_lc = nullptr;
@@ -273,8 +276,8 @@ void ns_range_write(hipStream_t s, int ntb, const int32_t* tbox, const int32_t* 
                     uint64_t* keys, int32_t* ends);
 void ns_range_merge(hipStream_t s, int ntb, const int32_t* at, const int32_t* cnt, const uint64_t* keys,
                     const int32_t* ends, int64_t* mstart, int64_t* mend, int32_t* mcnt, int64_t* nfaces, int32_t* err);
-void ns_dof_keys(hipStream_t s, int64_t total, int ntb, const int64_t* fstart, const int32_t* at, const int32_t* mcnt,
-                 const int64_t* mstart, const int64_t* mend, const int32_t* fdof, uint64_t* keys);
+void ns_dof_keys(hipStream_t s, int npf, int64_t total, int ntb, const int64_t* fstart, const int32_t* at,
+                 const int32_t* mcnt, const int64_t* mstart, const int64_t* mend, const int32_t* fdof, uint64_t* keys);
 void unique_flags(hipStream_t s, int64_t n, const uint64_t* keys, int32_t* flags);
 void compact_u64(hipStream_t s, int64_t n, const uint64_t* keys, const int32_t* flags, const int32_t* at,
                  uint64_t* out);
@@ -290,10 +293,11 @@ void ns_fill(hipStream_t s, int nch, const uint64_t* order, const int32_t* chunk
              const int64_t* row_ptr, const int64_t* col_ptr, const int64_t* range_ptr, int32_t* row_out,
              int32_t* row_block, int64_t* node_row, int32_t* col_idx, int32_t* ncol_dof, int64_t* ranges,
              int64_t* range_prefix, const int64_t* fstart, int64_t* block_fstart);
-// local DOF column of each face corner of the target boxes' merged ranges (3 per face, at 3 fstart[ti] + ...)
-void ns_face_cols(hipStream_t s, int64_t total, int ntb, const int64_t* fstart, const int32_t* at, const int32_t* mcnt,
-                  const int64_t* mstart, const int64_t* mend, const int32_t* fdof, const int64_t* dptr,
-                  const int32_t* dofs, int32_t* cols);
+// local DOF column of each face node of the target boxes' merged ranges (npf per face, from npf fstart[ti]; -1 for
+// an edge midpoint without a DOF)
+void ns_face_cols(hipStream_t s, int npf, int64_t total, int ntb, const int64_t* fstart, const int32_t* at,
+                  const int32_t* mcnt, const int64_t* mstart, const int64_t* mend, const int32_t* fdof,
+                  const int64_t* dptr, const int32_t* dofs, int32_t* cols);
 void l2p_coef(hipStream_t s, int64_t nrows, int P, const int32_t* row_out, const int32_t* target, const double* X,
               const BoxGeom* geom, const int32_t* level, const double* level_side, const double* cnm, double* out);
 // rocPRIM radix sorts and scans; tmp == nullptr: only sets tmp_bytes

@@ -20,6 +20,7 @@
 #include <string>
 #include <tuple>
 
+#include "nearquad.hpp"
 #include "plan.hpp"
 
 namespace bemfmm {
@@ -195,9 +196,13 @@ void plan_geometry(Plan& pl, const Options& opt, int n_nodes, const double* X, i
   pl.n_nodes = n_nodes;
   pl.n_faces = n_faces;
   pl.n_dof = n_dof;
-  for (int f = 0; f < 3 * n_faces; ++f) {
-    if (tri[f] < 0 || tri[f] >= n_nodes) throw std::runtime_error("triangle node index out of range");
-    if (face_dof[f] < 0 || face_dof[f] >= n_dof) throw std::runtime_error("face DOF index out of range");
+  if (opt.npf != 3 && opt.npf != 6) throw std::runtime_error("nodes per face must be 3 (linear) or 6 (quadratic)");
+  const int npf = pl.npf = opt.npf;
+  for (int64_t e = 0; e < (int64_t)npf * n_faces; ++e) {
+    if (tri[e] < 0 || tri[e] >= n_nodes) throw std::runtime_error("triangle node index out of range");
+    const bool midpoint = npf == 6 && e % 6 >= 3;  // an edge midpoint may carry no DOF (-1)
+    if ((face_dof[e] < 0 && !(midpoint && face_dof[e] == -1)) || face_dof[e] >= n_dof)
+      throw std::runtime_error("face DOF index out of range");
   }
   for (int i = 0; i < n_dof; ++i) {
     if (dof_kind[i] < 0 || dof_kind[i] > 3)
@@ -233,22 +238,23 @@ void plan_geometry(Plan& pl, const Options& opt, int n_nodes, const double* X, i
 
   // --- faces and P2M sources (GMRES_FMM.hpp:1799-1937) ---
   const auto& rule = dunavant(opt.q);
-  pl.fX.resize(9 * (size_t)n_faces);
+  pl.fX.resize(3 * (size_t)npf * n_faces);
   pl.fcent.resize(3 * (size_t)n_faces);
   pl.fcr.resize(3 * (size_t)n_faces);
   pl.fJ.resize(n_faces);
-  pl.fv.assign(tri, tri + 3 * (size_t)n_faces);
-  pl.fdof.assign(face_dof, face_dof + 3 * (size_t)n_faces);
+  pl.fv.assign(tri, tri + (size_t)npf * n_faces);
+  pl.fdof.assign(face_dof, face_dof + (size_t)npf * n_faces);
   pl.pole.resize(3 * (size_t)n_faces);
   pl.maxoff.resize(n_faces);
   int degenerate = 0;
 #pragma omp parallel for schedule(static) reduction(| : degenerate)
   for (int f = 0; f < n_faces; ++f) {
     V3 x[3];
-    for (int k = 0; k < 3; ++k)
+    for (int k = 0; k < npf; ++k)
       for (int a = 0; a < 3; ++a) {
-        x[k][a] = pl.Xs[3 * tri[3 * f + k] + a];
-        pl.fX[9 * f + 3 * k + a] = x[k][a];
+        const double v = pl.Xs[3 * tri[(int64_t)npf * f + k] + a];
+        if (k < 3) x[k][a] = v;
+        pl.fX[(size_t)3 * npf * f + 3 * k + a] = v;
       }
     const V3 e1 = {x[1][0] - x[0][0], x[1][1] - x[0][1], x[1][2] - x[0][2]};
     const V3 e2 = {x[2][0] - x[0][0], x[2][1] - x[0][1], x[2][2] - x[0][2]};
@@ -260,12 +266,18 @@ void plan_geometry(Plan& pl, const Options& opt, int n_nodes, const double* X, i
     for (int a = 0; a < 3; ++a) {
       pl.fcr[3 * f + a] = J * (cr[a] / J);
       pl.fcent[3 * f + a] = (x[0][a] + x[1][a] + x[2][a]) / 3.;
-      pf[a] = (1. / 3.) * x[0][a] + (1. / 3.) * x[1][a] + (1. / 3.) * x[2][a];
+      // the tree position: Dot({1/3,1/3,1/3}, X012) for linear faces, the centroid (a+b+c)/3 for quadratic ones
+      pf[a] = npf == 3 ? (1. / 3.) * x[0][a] + (1. / 3.) * x[1][a] + (1. / 3.) * x[2][a] : pl.fcent[3 * f + a];
     }
     double mo = 0.;
     for (const auto& r : rule) {
       V3 xq;
-      for (int a = 0; a < 3; ++a) xq[a] = r[0] * x[0][a] + r[1] * x[1][a] + r[2] * x[2][a];
+      if (npf == 3) {
+        for (int a = 0; a < 3; ++a) xq[a] = r[0] * x[0][a] + r[1] * x[1][a] + r[2] * x[2][a];
+      } else {
+        double cr6[3];
+        tri6_geom(&pl.fX[(size_t)18 * f], r[0], r[1], xq.data(), cr6);
+      }
       mo = std::max(mo, hyp3(xq[0] - pf[0], xq[1] - pf[1], xq[2] - pf[2]));
     }
     pl.maxoff[f] = mo;
@@ -451,6 +463,7 @@ void build_plan(Plan& pl, const Options& opt, int n_nodes, const double* X, int 
 
 void plan_finish(Plan& pl, bool near_on_host) {
   const Options& opt = pl.opt;
+  const int npf = pl.npf;
   const int P = pl.P, C = pl.C, n_nodes = pl.n_nodes, n_dof = pl.n_dof, nb = pl.n_boxes;
   const int32_t* face_dof = pl.fdof.data();
   const auto& rule = dunavant(opt.q);
@@ -498,8 +511,9 @@ void plan_finish(Plan& pl, bool near_on_host) {
         for (size_t k = 0; k < merged.size(); k += 2)
           for (int64_t i = merged[k]; i < merged[k + 1]; ++i) {
             const int f = pl.face_perm[i];
-            for (int c = 0; c < 3; ++c) {
-              const int d = face_dof[3 * f + c];
+            for (int c = 0; c < npf; ++c) {
+              const int d = face_dof[(int64_t)npf * f + c];
+              if (d < 0) continue;
               if (mark[d] != ti) {
                 mark[d] = ti;
                 cl.push_back(d);
@@ -629,7 +643,8 @@ void plan_finish(Plan& pl, bool near_on_host) {
       std::vector<int32_t> cl;
       for (int64_t i = pl.box_range[2 * b]; i < pl.box_range[2 * b + 1]; ++i) {
         const int f = pl.face_perm[i];
-        for (int c = 0; c < 3; ++c) cl.push_back(face_dof[3 * f + c]);
+        for (int c = 0; c < npf; ++c)
+          if (face_dof[(int64_t)npf * f + c] >= 0) cl.push_back(face_dof[(int64_t)npf * f + c]);
       }
       std::sort(cl.begin(), cl.end());
       cl.erase(std::unique(cl.begin(), cl.end()), cl.end());
@@ -655,7 +670,7 @@ void plan_finish(Plan& pl, bool near_on_host) {
     G.row_out.assign(G.row_ptr[nl], 0);
     G.col_idx.assign(G.col_ptr[nl], 0);
     pl.p2m_faces.assign(pl.p2m_face_ptr[nl], 0);
-    pl.p2m_face_lc.assign(3 * pl.p2m_face_ptr[nl], 0);
+    pl.p2m_face_lc.assign((size_t)npf * pl.p2m_face_ptr[nl], 0);
     pl.p2m_leaf_cs.assign(4 * (size_t)nl, 0.);
 #pragma omp parallel for schedule(dynamic, 32)
     for (int li = 0; li < nl; ++li) {
@@ -666,13 +681,16 @@ void plan_finish(Plan& pl, bool near_on_host) {
       for (int64_t i = pl.box_range[2 * b]; i < pl.box_range[2 * b + 1]; ++i, ++k) {
         const int f = pl.face_perm[i];
         pl.p2m_faces[k] = f;
-        for (int c = 0; c < 3; ++c)
-          pl.p2m_face_lc[3 * k + c] =
-              (int32_t)(std::lower_bound(cols[li].begin(), cols[li].end(), face_dof[3 * f + c]) - cols[li].begin());
+        for (int c = 0; c < npf; ++c) {
+          const int d = face_dof[(int64_t)npf * f + c];
+          pl.p2m_face_lc[(size_t)npf * k + c] =
+              d < 0 ? -1 : (int32_t)(std::lower_bound(cols[li].begin(), cols[li].end(), d) - cols[li].begin());
+        }
       }
       for (int a = 0; a < 3; ++a) pl.p2m_leaf_cs[4 * (size_t)li + a] = pl.box_center[3 * b + a];
       pl.p2m_leaf_cs[4 * (size_t)li + 3] = pl.level_side[pl.box_level[b]];
     }
+    if (opt.fill_p2m && npf != 3) throw std::runtime_error("host P2M matrices are built for linear faces only");
     if (opt.fill_p2m) pl.p2m_mat.assign(G.mat_size, 0.);
     else pl.p2m_mat.clear();
 #pragma omp parallel if (opt.fill_p2m)

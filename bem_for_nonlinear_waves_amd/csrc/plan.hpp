@@ -18,6 +18,8 @@ struct Options {
   int leaf_max = 50;      // fmm_bucket_max_points (main_time_domain.cpp:54-55)
   int max_level = 7;      // fmm_max_level
   int q = 6;              // p2m_quadrature_points (main_time_domain.cpp:62)
+  int npf = 3;            // nodes per face: 3 linear triangles, 6 true quadratic (vertices, then the midpoints of
+                          // edges (0,1), (1,2), (2,0); a midpoint's DOF may be -1: no DOF, weight redistributed)
   bool materialize_ops = true;  // fill Plan::ops (else only make sure every operator is in the cache)
   bool fill_p2m = true;         // fill Plan::p2m_mat on the host (else only its layout and the per-leaf source lists)
 };
@@ -55,12 +57,13 @@ struct Plan {
   double root_side = 0;     // side of the level-0 cube
   // scaled geometry
   std::vector<double> Xs;            // n_nodes*3
-  std::vector<double> fX;            // n_faces*9 vertex coordinates
+  int npf = 3;                       // nodes per face (3 or 6)
+  std::vector<double> fX;            // n_faces*3*npf node coordinates
   std::vector<double> fcent;         // n_faces*3 (a+b+c)/3
   std::vector<double> fcr;           // n_faces*3 (2*area*normal)
   std::vector<double> fJ;            // n_faces
-  std::vector<int32_t> fv;           // n_faces*3 node ids
-  std::vector<int32_t> fdof;         // n_faces*3 DOF ids
+  std::vector<int32_t> fv;           // n_faces*npf node ids
+  std::vector<int32_t> fdof;         // n_faces*npf DOF ids (-1: quadratic edge midpoint without a DOF)
   std::vector<int32_t> dof_kind, dof_point;
   // inputs of the tree: per face the P2M pole (vertex mean with weights 1/3) and the largest distance from it
   // to the face's quadrature points; the root box (cube of half-side 1.05 scale/2 about the bounds' centre)
@@ -179,7 +182,7 @@ struct SchwarzBlocks {
 // each grown by overlap_k rings of neighbours to at most max_block DOFs.
 void build_schwarz_blocks(int n_dof, int n_faces, const int32_t* face_dof, const int32_t* dof_point,
                           const std::vector<int32_t>& seed_order, int max_core, int max_block, int overlap_k,
-                          SchwarzBlocks& out);
+                          SchwarzBlocks& out, int npf = 3);
 
 }  // namespace bemfmm
 

@@ -20,15 +20,17 @@ namespace bemfmm {
 
 void build_schwarz_blocks(int n_dof, int n_faces, const int32_t* face_dof, const int32_t* dof_point,
                           const std::vector<int32_t>& seed_order, int max_core, int max_block, int overlap_k,
-                          SchwarzBlocks& out) {
+                          SchwarzBlocks& out, int npf) {
   if (max_core < 1 || max_block < max_core) throw std::runtime_error("schwarz: need 1 <= max_core <= max_block");
   if ((int)seed_order.size() != n_dof) throw std::runtime_error("schwarz: seed order must list every DOF once");
   // adjacency (CSR) of the DOF graph
   std::vector<std::vector<int32_t>> adj(n_dof);
   for (int f = 0; f < n_faces; ++f)
-    for (int a = 0; a < 3; ++a)
-      for (int b = 0; b < 3; ++b)
-        if (a != b) adj[face_dof[3 * f + a]].push_back(face_dof[3 * f + b]);
+    for (int a = 0; a < npf; ++a)
+      for (int b = 0; b < npf; ++b) {
+        const int32_t da = face_dof[(int64_t)npf * f + a], db = face_dof[(int64_t)npf * f + b];
+        if (a != b && da >= 0 && db >= 0) adj[da].push_back(db);
+      }
   {
     int n_points = 0;
     for (int d = 0; d < n_dof; ++d) n_points = std::max(n_points, dof_point[d] + 1);

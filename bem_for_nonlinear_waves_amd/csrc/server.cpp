@@ -75,7 +75,7 @@ bool same_opts(const bemfmm_opts& a, const bemfmm_opts& b) {
          a.schwarz_variant == b.schwarz_variant && a.schwarz_core_k == b.schwarz_core_k &&
          a.milu_omega == b.milu_omega && a.ilut_drop_tol == b.ilut_drop_tol && a.ilut_max_entries == b.ilut_max_entries &&
          a.ilut_pivot_min == b.ilut_pivot_min && a.plane_wave_digits == b.plane_wave_digits &&
-         a.tree_on_gpu == b.tree_on_gpu;
+         a.tree_on_gpu == b.tree_on_gpu && a.element_order == b.element_order;
 }
 
 }  // namespace

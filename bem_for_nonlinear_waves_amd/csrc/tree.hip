@@ -364,8 +364,9 @@ __global__ void k_ns_range_merge(int ntb, const int32_t* __restrict__ at, const 
   nfaces[ti] = nf;
 }
 
-// one key (target box, DOF) per face corner of each target box's merged ranges; fdof in face_perm order
-__global__ void k_ns_dof_keys(int64_t total, int ntb, const int64_t* __restrict__ fstart,
+// one key (target box, DOF) per face node of each target box's merged ranges; fdof in face_perm order, npf per
+// face (an edge midpoint without a DOF contributes its first end vertex's DOF, which the face has anyway)
+__global__ void k_ns_dof_keys(int npf, int64_t total, int ntb, const int64_t* __restrict__ fstart,
                               const int32_t* __restrict__ at, const int32_t* __restrict__ mcnt,
                               const int64_t* __restrict__ mstart, const int64_t* __restrict__ mend,
                               const int32_t* __restrict__ fdof, uint64_t* __restrict__ keys) {
@@ -388,12 +389,16 @@ __global__ void k_ns_dof_keys(int64_t total, int ntb, const int64_t* __restrict_
     }
     i -= len;
   }
-  for (int c = 0; c < 3; ++c) keys[3 * e + c] = ((uint64_t)(uint32_t)ti << 32) | (uint32_t)fdof[3 * pos + c];
+  for (int c = 0; c < npf; ++c) {
+    int d = fdof[npf * pos + c];
+    if (d < 0) d = fdof[npf * pos + (c - 3)];
+    keys[npf * e + c] = ((uint64_t)(uint32_t)ti << 32) | (uint32_t)d;
+  }
 }
 
 // the local DOF column of every face corner of every target box's merged ranges (in the near quadrature's local
 // face order), so that the quadrature of each row reads three columns per face instead of searching them
-__global__ void k_ns_face_cols(int64_t total, int ntb, const int64_t* __restrict__ fstart,
+__global__ void k_ns_face_cols(int npf, int64_t total, int ntb, const int64_t* __restrict__ fstart,
                                const int32_t* __restrict__ at, const int32_t* __restrict__ mcnt,
                                const int64_t* __restrict__ mstart, const int64_t* __restrict__ mend,
                                const int32_t* __restrict__ fdof, const int64_t* __restrict__ dptr,
@@ -419,15 +424,19 @@ __global__ void k_ns_face_cols(int64_t total, int ntb, const int64_t* __restrict
   }
   const int32_t* d = dofs + dptr[ti];
   const int n = (int)(dptr[ti + 1] - dptr[ti]);
-  for (int c = 0; c < 3; ++c) {
-    const int v = fdof[3 * pos + c];
+  for (int c = 0; c < npf; ++c) {
+    const int v = fdof[npf * pos + c];
+    if (v < 0) {  // edge midpoint without a DOF
+      cols[npf * e + c] = -1;
+      continue;
+    }
     int a = 0, b = n - 1;
     while (a < b) {
       const int mid = (a + b) >> 1;
       if (d[mid] < v) a = mid + 1;
       else b = mid;
     }
-    cols[3 * e + c] = a;
+    cols[npf * e + c] = a;
   }
 }
 
@@ -705,18 +714,18 @@ void ns_range_merge(hipStream_t s, int ntb, const int32_t* at, const int32_t* cn
   hipLaunchKernelGGL(k_ns_range_merge, dim3(grid_of(ntb + 1)), dim3(kTB), 0, s, ntb, at, cnt, keys, ends, mstart, mend,
                      mcnt, nfaces, err);
 }
-void ns_dof_keys(hipStream_t s, int64_t total, int ntb, const int64_t* fstart, const int32_t* at, const int32_t* mcnt,
-                 const int64_t* mstart, const int64_t* mend, const int32_t* fdof, uint64_t* keys) {
+void ns_dof_keys(hipStream_t s, int npf, int64_t total, int ntb, const int64_t* fstart, const int32_t* at,
+                 const int32_t* mcnt, const int64_t* mstart, const int64_t* mend, const int32_t* fdof, uint64_t* keys) {
   if (total > 0)
-    hipLaunchKernelGGL(k_ns_dof_keys, dim3(grid_of(total)), dim3(kTB), 0, s, total, ntb, fstart, at, mcnt, mstart, mend,
-                       fdof, keys);
+    hipLaunchKernelGGL(k_ns_dof_keys, dim3(grid_of(total)), dim3(kTB), 0, s, npf, total, ntb, fstart, at, mcnt, mstart,
+                       mend, fdof, keys);
 }
-void ns_face_cols(hipStream_t s, int64_t total, int ntb, const int64_t* fstart, const int32_t* at, const int32_t* mcnt,
-                  const int64_t* mstart, const int64_t* mend, const int32_t* fdof, const int64_t* dptr,
-                  const int32_t* dofs, int32_t* cols) {
+void ns_face_cols(hipStream_t s, int npf, int64_t total, int ntb, const int64_t* fstart, const int32_t* at,
+                  const int32_t* mcnt, const int64_t* mstart, const int64_t* mend, const int32_t* fdof,
+                  const int64_t* dptr, const int32_t* dofs, int32_t* cols) {
   if (total > 0)
-    hipLaunchKernelGGL(k_ns_face_cols, dim3(grid_of(total)), dim3(kTB), 0, s, total, ntb, fstart, at, mcnt, mstart,
-                       mend, fdof, dptr, dofs, cols);
+    hipLaunchKernelGGL(k_ns_face_cols, dim3(grid_of(total)), dim3(kTB), 0, s, npf, total, ntb, fstart, at, mcnt,
+                       mstart, mend, fdof, dptr, dofs, cols);
 }
 void unique_flags(hipStream_t s, int64_t n, const uint64_t* keys, int32_t* flags) {
   if (n > 0) hipLaunchKernelGGL(k_unique_flags, dim3(grid_of(n)), dim3(kTB), 0, s, n, keys, flags);

@@ -304,11 +304,109 @@ def build_dofs(mesh: Mesh, multiple_angle_deg: float = 20.0) -> DofModel:
                     ent[:, 1].astype(np.int32), pclass, multiple)
 
 
+# ---------------------------------------------------------------------------
+# true quadratic elements: edge-midpoint nodes and their DOFs
+# ---------------------------------------------------------------------------
+def to_quadratic(mesh: Mesh, project=None) -> Mesh:
+    """The same surface with 6-node faces: (p0, p1, p2, m01, m12, m20), the midpoint nodes appended after the
+    vertices (one per edge). ``project`` maps the (k, 3) straight-edge midpoints and the (k, 2) end vertex ids to
+    their positions on the curved surface (default: the straight-edge midpoints). The midpoint X is the
+    reference's networkLine::X_mid (Network.hpp:280), the order that of getQuadDOFIndices
+    (bem/BEM_setBoundaryTypes.hpp:120-133)."""
+    tri = np.asarray(mesh.tri)[:, :3]
+    e = np.stack([tri[:, [0, 1]], tri[:, [1, 2]], tri[:, [2, 0]]], 1).reshape(-1, 2)
+    key = np.sort(e, axis=1)
+    uniq, inv = np.unique(key, axis=0, return_inverse=True)
+    Xm = 0.5 * (mesh.X[uniq[:, 0]] + mesh.X[uniq[:, 1]])
+    if project is not None:
+        Xm = np.asarray(project(Xm, uniq), dtype=np.float64)
+    mid = mesh.n_nodes + inv.reshape(-1, 3)
+    tri6 = np.concatenate([tri, mid], 1).astype(np.int32)
+    return Mesh(np.concatenate([mesh.X, Xm], 0), tri6, mesh.face_dirichlet.copy(), mesh.face_group.copy())
+
+
+def vertex_mesh(mesh6: Mesh) -> Mesh:
+    """The linear view of a quadratic mesh (vertices only, midpoint nodes kept unreferenced)."""
+    return Mesh(mesh6.X, np.ascontiguousarray(mesh6.tri[:, :3]), mesh6.face_dirichlet, mesh6.face_group)
+
+
+def build_dofs_quadratic(mesh6: Mesh, multiple_angle_deg: float = 20.0) -> DofModel:
+    """DOF model of true quadratic faces (setNodeFaceIndices with use_true_quadratic_element,
+    bem/BEM_setBoundaryTypes.hpp:190-233; lf2Index :113-117): the vertex DOFs of build_dofs, then per edge
+    midpoint a primary DOF (Dirichlet if both faces of the edge are Dirichlet, else Neumann; Dirichlet also for
+    a CORNER edge between a Dirichlet and a Neumann face) and, on a CORNER edge, a CORNER Neumann DOF for its
+    Neumann face. A face's midpoint entry is the primary DOF, or the CORNER DOF for the Neumann face of a CORNER
+    edge (edges are never multiple nodes otherwise, however sharp)."""
+    nv = int(mesh6.tri[:, :3].max()) + 1  # vertices first, then the midpoint nodes (to_quadratic)
+    if mesh6.tri[:, 3:].min() < nv:
+        raise ValueError("midpoint nodes must follow the vertices")
+    lin = Mesh(mesh6.X[:nv], np.ascontiguousarray(mesh6.tri[:, :3]), mesh6.face_dirichlet, mesh6.face_group)
+    d = build_dofs(lin, multiple_angle_deg)
+    F = mesh6.n_faces
+    mids = mesh6.tri[:, 3:6]
+    nd = mesh6.face_dirichlet
+    nm = mesh6.n_nodes
+    n_dir = np.zeros(nm, np.int64)
+    n_neu = np.zeros(nm, np.int64)
+    np.add.at(n_dir, mids[nd].reshape(-1), 1)
+    np.add.at(n_neu, mids[~nd].reshape(-1), 1)
+    used = np.flatnonzero((n_dir + n_neu) > 0)
+    kinds, points, faces = [], [], []
+    primary = np.full(nm, -1, np.int64)
+    corner_dof = np.full(nm, -1, np.int64)
+    k = d.n_dof
+    for m in used:
+        corner = n_dir[m] > 0 and n_neu[m] > 0
+        primary[m] = k
+        kinds.append(DIRICHLET if n_dir[m] > 0 else NEUMANN)
+        points.append(m)
+        faces.append(-1)
+        k += 1
+        if corner:
+            corner_dof[m] = k
+            kinds.append(CORNER_NEUMANN)
+            points.append(m)
+            faces.append(-2)  # set to the Neumann face below
+            k += 1
+    face_dof = np.empty((F, 6), np.int32)
+    face_dof[:, :3] = d.face_dof
+    dof_face = np.concatenate([d.dof_face, np.asarray(faces, np.int32)])
+    for f in range(F):
+        for e in range(3):
+            m = int(mids[f, e])
+            if corner_dof[m] >= 0 and not nd[f]:
+                face_dof[f, 3 + e] = corner_dof[m]
+                dof_face[corner_dof[m]] = f
+            else:
+                face_dof[f, 3 + e] = primary[m]
+    pclass = np.concatenate([d.point_class, np.zeros(nm - lin.n_nodes, np.int8)])
+    pclass[used] = np.where((n_dir[used] > 0) & (n_neu[used] > 0), 2, np.where(n_dir[used] > 0, 0, 1))
+    multiple = np.concatenate([d.point_multiple, np.zeros(nm - lin.n_nodes, bool)])
+    return DofModel(int(k), face_dof, np.concatenate([d.dof_kind, np.asarray(kinds, np.int32)]).astype(np.int32),
+                    np.concatenate([d.dof_point, np.asarray(points, np.int32)]).astype(np.int32),
+                    dof_face.astype(np.int32), pclass, multiple)
+
+
+def standing_wave(mesh: Mesh, dofs: DofModel, Lx: float, n: int = 1):
+    """phi = cosh(k z) cos(k x), k = n pi / Lx: harmonic, with zero normal derivative on the walls x = 0, Lx,
+    y = const and the bottom z = 0 of a box tank; returns (phi, phin) boundary data (phi on Dirichlet DOFs, 0
+    elsewhere as the Neumann data) and the exact (phi, phin) per DOF (phin on the free surface z = Lz only)."""
+    kk = n * math.pi / Lx
+    X = mesh.X[dofs.dof_point]
+    phi_ex = np.cosh(kk * X[:, 2]) * np.cos(kk * X[:, 0])
+    phin_ex = np.zeros(dofs.n_dof)
+    top = dofs.dof_kind == DIRICHLET
+    phin_ex[top] = kk * np.sinh(kk * X[top, 2]) * np.cos(kk * X[top, 0])
+    return phi_ex.copy(), np.zeros(dofs.n_dof), phi_ex, phin_ex
+
+
 def point_normals(mesh: Mesh) -> np.ndarray:
     fn, fa = mesh.face_normals(), mesh.face_areas()
     pn = np.zeros((mesh.n_nodes, 3))
-    np.add.at(pn, mesh.tri.reshape(-1), np.repeat(fn * fa[:, None], 3, axis=0))
-    return pn / np.linalg.norm(pn, axis=1, keepdims=True)
+    k = mesh.tri.shape[1]  # 3, or 6 with the edge midpoints of quadratic faces
+    np.add.at(pn, mesh.tri.reshape(-1), np.repeat(fn * fa[:, None], k, axis=0))
+    nrm = np.linalg.norm(pn, axis=1, keepdims=True)
+    return pn / np.where(nrm > 0, nrm, 1.0)
 
 
 def dof_normals(mesh: Mesh, dofs: DofModel) -> np.ndarray:
@@ -318,7 +416,7 @@ def dof_normals(mesh: Mesh, dofs: DofModel) -> np.ndarray:
     fn, fa = mesh.face_normals(), mesh.face_areas()
     dn = np.zeros((mesh.n_nodes, 3))
     w = (fn * fa[:, None]) * mesh.face_dirichlet[:, None]
-    np.add.at(dn, mesh.tri.reshape(-1), np.repeat(w, 3, axis=0))
+    np.add.at(dn, mesh.tri.reshape(-1), np.repeat(w, mesh.tri.shape[1], axis=0))
     nd = np.linalg.norm(dn, axis=1)
     corner = dofs.point_class == 2
     pn[corner] = dn[corner] / nd[corner, None]

@@ -117,6 +117,12 @@ typedef struct bemfmm_opts {
   int tree_on_gpu;       /* 1 (default): the octree, the near / M2L lists and the target boxes are built on the GPU
                             (level-synchronous radix sorts on path keys); 0: on the host. Both give the same plan
                             (generateTree, lib_spatial_partitioning.hpp:1079-1115; moments.hpp:968-997) */
+  int element_order;     /* 1 (default): linear triangles; 2: true quadratic faces (isTrueQuadraticElement,
+                            BEM_solveBVP_GMRES_FMM.hpp:1311-1657): bemfmm_set_geometry then takes six nodes and six
+                            DOF entries per face (vertices p0 p1 p2, then the midpoint nodes of edges p0p1, p1p2, p2p0;
+                            a midpoint DOF of -1 has its weight redistributed to the edge's end vertices), the
+                            midpoint nodes are targets with their own BIE rows (:111-144); preconditioners NONE,
+                            JACOBI or SCHWARZ (schwarz_variant 0) */
 } bemfmm_opts;
 
 typedef struct bemfmm_result {

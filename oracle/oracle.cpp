@@ -317,9 +317,10 @@ struct Bucket {
 };
 
 struct FaceRec {
-  int v[3];
-  int dof[3];
-  V3 X[3];
+  int npf = 3;  // 3: linear triangle; 6: true quadratic face (vertices, then the midpoints of edges 01, 12, 20)
+  int v[6];
+  int dof[6];   // -1: edge midpoint without a DOF (its weight goes half to each end vertex)
+  V3 X[6];
   V3 centroid;  // (a+b+c)/3
   V3 pole;      // Dot({1/3,1/3,1/3}, X012): tree position
   V3 nhat;
@@ -327,9 +328,52 @@ struct FaceRec {
   double J;     // 2*area
   double maxoff;
   std::vector<V3> qx;
-  std::vector<double> qw;                  // W = J * w_q
+  std::vector<double> qw;                  // W = J * w_q (J at the source point for quadratic faces)
   std::vector<std::array<double, 3>> qb;   // barycentric
+  std::vector<std::array<double, 6>> qN;   // density weights per face node (the barycentrics, or TriShape<6>)
+  std::vector<V3> qn;                      // unit normal at the source point
 };
+
+// ---- true quadratic faces (BEM_solveBVP_GMRES_FMM.hpp:1311-1657) ----
+// TriShape<6> and its derivatives, t2 = 1 - t0 - t1 (basic_arithmetic_array_operations.hpp:1467-1566)
+void q6_shape(double t0, double t1, double N[6], double D0[6], double D1[6]) {
+  const double t2 = 1. - t0 - t1;
+  const double n[6] = {t0 * (2. * t0 - 1.), t1 * (2. * t1 - 1.), t2 * (2. * t2 - 1.), 4. * t0 * t1, 4. * t1 * t2,
+                       4. * t2 * t0};
+  const double a[6] = {-1. + 4. * t0, 0., 1. - 4. * t2, 4. * t1, -4. * t1, -4. * t0 + 4. * t2};
+  const double b[6] = {0., -1. + 4. * t1, 1. - 4. * t2, 4. * t0, 4. * t2 - 4. * t1, -4. * t0};
+  for (int k = 0; k < 6; ++k) {
+    N[k] = n[k];
+    D0[k] = a[k];
+    D1[k] = b[k];
+  }
+}
+// X(t0, t1) and dX/dt0 x dX/dt1 of the 6-node geometry
+V3 q6_point(const FaceRec& R, double t0, double t1, V3& crs) {
+  double N[6], D0[6], D1[6];
+  q6_shape(t0, t1, N, D0, D1);
+  V3 x{0., 0., 0.}, a{0., 0., 0.}, b{0., 0., 0.};
+  for (int k = 0; k < 6; ++k) {
+    x = x + N[k] * R.X[k];
+    a = a + D0[k] * R.X[k];
+    b = b + D1[k] * R.X[k];
+  }
+  crs = cross(a, b);
+  return x;
+}
+// trueQuadN6 (Network.hpp:1607-1632)
+std::array<double, 6> q6_density(const FaceRec& R, double t0, double t1) {
+  double N[6], D0[6], D1[6];
+  q6_shape(t0, t1, N, D0, D1);
+  std::array<double, 6> out{N[0], N[1], N[2], N[3], N[4], N[5]};
+  for (int e = 0; e < 3; ++e)
+    if (R.dof[3 + e] < 0) {
+      out[e] += 0.5 * out[3 + e];
+      out[(e + 1) % 3] += 0.5 * out[3 + e];
+      out[3 + e] = 0.;
+    }
+  return out;
+}
 
 struct Oracle {
   // inputs
@@ -342,6 +386,7 @@ struct Oracle {
   double Lscale = 1.0, scale = 1.0, near_region = 0.;
   // settings
   int P = 10, leaf_max = 50, max_level = 7, q = 6, m2l_mode = 0;
+  int npf = 3;  // nodes per face (6: true quadratic)
   double theta = 0.25;
   // tree
   Bucket root;
@@ -468,9 +513,10 @@ struct Oracle {
     F.resize(n_faces);
     for (int f = 0; f < n_faces; ++f) {
       FaceRec& R = F[f];
-      for (int k = 0; k < 3; ++k) {
-        R.v[k] = tri[3 * f + k];
-        R.dof[k] = fdof[3 * f + k];
+      R.npf = npf;
+      for (int k = 0; k < npf; ++k) {
+        R.v[k] = tri[npf * f + k];
+        R.dof[k] = fdof[npf * f + k];
         R.X[k] = Xn[R.v[k]];
       }
       const V3 c = cross(R.X[1] - R.X[0], R.X[2] - R.X[0]);
@@ -481,15 +527,29 @@ struct Oracle {
         R.centroid[a] = (R.X[0][a] + R.X[1][a] + R.X[2][a]) / 3.;
         R.pole[a] = (1. / 3.) * R.X[0][a] + (1. / 3.) * R.X[1][a] + (1. / 3.) * R.X[2][a];
       }
+      if (npf == 6) R.pole = R.centroid;  // pole.X = F->centroid for quadratic faces
       R.qx.clear();
       R.qw.clear();
       R.qb.clear();
+      R.qN.clear();
+      R.qn.clear();
       R.maxoff = 0.;
       for (const auto& r : rule) {
         V3 xq;
-        for (int a = 0; a < 3; ++a) xq[a] = r[0] * R.X[0][a] + r[1] * R.X[1][a] + r[2] * R.X[2][a];
+        if (npf == 3) {
+          for (int a = 0; a < 3; ++a) xq[a] = r[0] * R.X[0][a] + r[1] * R.X[1][a] + r[2] * R.X[2][a];
+          R.qw.push_back(R.J * r[3]);
+          R.qN.push_back({r[0], r[1], r[2], 0., 0., 0.});
+          R.qn.push_back(R.nhat);
+        } else {
+          V3 crs;
+          xq = q6_point(R, r[0], r[1], crs);
+          const double Jq = norm(crs);
+          R.qw.push_back(Jq * r[3]);
+          R.qN.push_back(q6_density(R, r[0], r[1]));
+          R.qn.push_back(Jq > 1e-30 ? (1.0 / Jq) * crs : R.nhat);
+        }
         R.qx.push_back(xq);
-        R.qw.push_back(R.J * r[3]);
         R.qb.push_back({r[0], r[1], r[2]});
         R.maxoff = std::max(R.maxoff, norm(xq - R.pole));
       }
@@ -579,6 +639,10 @@ struct Oracle {
         touched.push_back(d);
       }
     };
+    if (R.npf == 6) {
+      face_direct_quad(R, t, addw);
+      return;
+    }
     int slot = -1;
     for (int k = 0; k < 3; ++k)
       if (R.v[k] == t) slot = k;
@@ -643,6 +707,60 @@ struct Oracle {
     for (int r = 0; r < 3; ++r) addw(R.dof[r], WG[r], WGn[r]);
   }
 
+  // fill_direct_true_quad (vertex and edge-midpoint Duffy, GMRES_FMM.hpp:1424-1558) and fill_nonadj_true_quad
+  // (:1562-1592): weights per face node, the target node's own double-layer weight dropped
+  template <class Add>
+  void face_direct_quad(const FaceRec& R, int t, Add&& addw) const {
+    const V3& x = Xn[t];
+    int slot = -1;
+    for (int k = 0; k < 6; ++k)
+      if (R.v[k] == t) slot = k;
+    double WG[6] = {0, 0, 0, 0, 0, 0}, WGn[6] = {0, 0, 0, 0, 0, 0};
+    auto at = [&](double b0, double b1, double w, int drop) {
+      V3 crs;
+      const V3 xq = q6_point(R, b0, b1, crs);
+      const auto N = q6_density(R, b0, b1);
+      const V3 Rv = xq - x;
+      const double r = norm(Rv);
+      const double g = norm(crs) * w / r;
+      const double gn = -dot((1.0 / r) * Rv, crs) * w / (r * r);
+      for (int k = 0; k < 6; ++k) {
+        WG[k] += g * N[k];
+        if (k != drop) WGn[k] += gn * N[k];
+      }
+    };
+    if (slot >= 0 && slot < 3) {
+      for (const auto& w : kGW5) {
+        const double t0 = w[0], t1 = w[1], ww = w[2];
+        double b[3];
+        b[slot] = t0;
+        b[(slot + 1) % 3] = t1 * (1. - t0);
+        b[(slot + 2) % 3] = (1. - t0) * (1. - t1);
+        at(b[0], b[1], ww * (1. - t0), slot);
+      }
+    } else if (slot >= 3) {
+      const int e = slot - 3, va = e, vb = (e + 1) % 3, vc = (e + 2) % 3;
+      for (const auto& w : kGW5) {
+        const double t0 = w[0], t1 = w[1], ww = w[2];
+        const double o = 1. - t0, s1 = t1 * o, s2 = o * (1. - t1);
+        double b[3];
+        b[va] = 0.5 * t0;
+        b[vb] = 0.5 * t0 + s1;
+        b[vc] = s2;
+        at(b[0], b[1], ww * o * 0.5, slot);
+        b[va] = 0.5 * t0 + s2;
+        b[vb] = 0.5 * t0;
+        b[vc] = s1;
+        at(b[0], b[1], ww * o * 0.5, slot);
+      }
+    } else {
+      const auto& rule = norm(R.centroid - x) < near_region ? dunavant(q) : dunavant(6);
+      for (const auto& r : rule) at(r[0], r[1], r[3], -1);
+    }
+    for (int k = 0; k < 6; ++k)
+      if (R.dof[k] >= 0) addw(R.dof[k], WG[k], WGn[k]);
+  }
+
   std::vector<int> near_faces_of(const Bucket* bd) const {
     std::vector<int> out;
     auto setNear = [&](const Bucket* B) { out.insert(out.end(), B->data.begin(), B->data.end()); };
@@ -693,14 +811,18 @@ struct Oracle {
       for (int f : b->data) {
         const FaceRec& R = F[f];
         for (size_t s = 0; s < R.qx.size(); ++s) {
-          const auto& bb = R.qb[s];
-          const double w_phi = R.qw[s] * (bb[0] * phi[R.dof[0]] + bb[1] * phi[R.dof[1]] + bb[2] * phi[R.dof[2]]);
-          const double w_phin = R.qw[s] * (bb[0] * phin[R.dof[0]] + bb[1] * phin[R.dof[1]] + bb[2] * phin[R.dof[2]]);
+          double sphi = 0., sphin = 0.;
+          for (int k = 0; k < R.npf; ++k)
+            if (R.dof[k] >= 0) {
+              sphi += R.qN[s][k] * phi[R.dof[k]];
+              sphin += R.qN[s][k] * phin[R.dof[k]];
+            }
+          const double w_phi = R.qw[s] * sphi, w_phin = R.qw[s] * sphin;
           S.eval(R.qx[s] - b->X, P);
           for (int n = 0; n <= P; ++n)
             for (int m = 0; m <= n; ++m) {
               b->M0[idx(n, m)] += w_phin * S.R[idx(n, -m)];
-              b->M1[idx(n, m)] += w_phi * S.dR(n, -m, R.nhat);
+              b->M1[idx(n, m)] += w_phi * S.dR(n, -m, R.qn[s]);
             }
         }
       }
@@ -1013,10 +1135,11 @@ struct Oracle {
           const V3 Rv = R.qx[s] - Xn[t];
           const double r = norm(Rv);
           const double g = R.qw[s] / r;
-          const double gn = -R.qw[s] * dot(Rv, R.nhat) / (r * r * r);
-          for (int k = 0; k < 3; ++k) {
-            rowG[R.dof[k]] += g * R.qb[s][k];
-            rowGn[R.dof[k]] += gn * R.qb[s][k];
+          const double gn = -R.qw[s] * dot(Rv, R.qn[s]) / (r * r * r);
+          for (int k = 0; k < R.npf; ++k) {
+            if (R.dof[k] < 0) continue;
+            rowG[R.dof[k]] += g * R.qN[s][k];
+            rowGn[R.dof[k]] += gn * R.qN[s][k];
           }
         }
       }
@@ -1568,11 +1691,17 @@ struct oracle_stats {
   double scale;
 };
 
-void* oracle_create(int n_nodes, const double* X, int n_faces, const int* tri, const int* face_dof, int n_dof,
-                    const int* dof_kind, const int* dof_point, const double* phi, const double* phin, int p,
-                    double theta, int leaf_max, int max_level, int q) {
+// TriShape<6> and its t0 / t1 derivatives at (t0, t1), 6 values each
+void oracle_tri6_shape(double t0, double t1, double* N, double* D0, double* D1) { q6_shape(t0, t1, N, D0, D1); }
+
+// npf = 6: true quadratic faces (tri and face_dof with six entries per face)
+void* oracle_create_npf(int n_nodes, const double* X, int n_faces, const int* tri, const int* face_dof, int n_dof,
+                        const int* dof_kind, const int* dof_point, const double* phi, const double* phin, int p,
+                        double theta, int leaf_max, int max_level, int q, int npf) {
   try {
+    if (npf != 3 && npf != 6) return nullptr;
     auto* o = new Oracle();
+    o->npf = npf;
     o->n_nodes = n_nodes;
     o->n_faces = n_faces;
     o->n_dof = n_dof;
@@ -1586,6 +1715,12 @@ void* oracle_create(int n_nodes, const double* X, int n_faces, const int* tri, c
   } catch (...) {
     return nullptr;
   }
+}
+void* oracle_create(int n_nodes, const double* X, int n_faces, const int* tri, const int* face_dof, int n_dof,
+                    const int* dof_kind, const int* dof_point, const double* phi, const double* phin, int p,
+                    double theta, int leaf_max, int max_level, int q) {
+  return oracle_create_npf(n_nodes, X, n_faces, tri, face_dof, n_dof, dof_kind, dof_point, phi, phin, p, theta,
+                           leaf_max, max_level, q, 3);
 }
 void oracle_destroy(void* h) { delete static_cast<Oracle*>(h); }
 void oracle_set_robin(void* h, const double* kappa) { static_cast<Oracle*>(h)->set_robin(kappa); }

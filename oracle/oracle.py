@@ -62,6 +62,9 @@ def lib():
         L.oracle_create.restype = ctypes.c_void_p
         L.oracle_create.argtypes = [ctypes.c_int, _dp, ctypes.c_int, _ip, _ip, ctypes.c_int, _ip, _ip, _dp, _dp,
                                     ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+        L.oracle_tri6_shape.argtypes = [ctypes.c_double, ctypes.c_double, _dp, _dp, _dp]
+        L.oracle_create_npf.restype = ctypes.c_void_p
+        L.oracle_create_npf.argtypes = L.oracle_create.argtypes + [ctypes.c_int]
         L.oracle_destroy.argtypes = [ctypes.c_void_p]
         L.oracle_set_robin.argtypes = [ctypes.c_void_p, _dp]
         L.oracle_matvec.argtypes = [ctypes.c_void_p, _dp, _dp]
@@ -111,6 +114,13 @@ def _p(a, t):
     return a.ctypes.data_as(t)
 
 
+def tri6_shape(t0: float, t1: float):
+    """TriShape<6> and its derivatives in t0 and t1 at (t0, t1), as the oracle evaluates them."""
+    N, D0, D1 = np.zeros(6), np.zeros(6), np.zeros(6)
+    lib().oracle_tri6_shape(float(t0), float(t1), _p(N, _dp), _p(D0, _dp), _p(D1, _dp))
+    return N, D0, D1
+
+
 class Oracle:
     """CPU FP64 FMM-GMRES problem: same inputs as the product's C ABI."""
 
@@ -120,9 +130,12 @@ class Oracle:
                       _d(phin)]
         X, tri, fd, kind, pt, ph, phn = self._keep
         self.n_dof = int(dofs.n_dof)
-        self.h = L.oracle_create(mesh.n_nodes, _p(X, _dp), mesh.n_faces, _p(tri, _ip), _p(fd, _ip), self.n_dof,
-                                 _p(kind, _ip), _p(pt, _ip), _p(ph, _dp), _p(phn, _dp), p, theta, leaf_max,
-                                 max_level, q)
+        npf = int(np.shape(mesh.tri)[1])  # 3 linear, 6 true quadratic faces
+        if np.shape(dofs.face_dof)[1] != npf:
+            raise ValueError("face_dof needs one entry per face node")
+        self.h = L.oracle_create_npf(mesh.n_nodes, _p(X, _dp), mesh.n_faces, _p(tri, _ip), _p(fd, _ip), self.n_dof,
+                                     _p(kind, _ip), _p(pt, _ip), _p(ph, _dp), _p(phn, _dp), p, theta, leaf_max,
+                                     max_level, q, npf)
         if not self.h:
             raise RuntimeError("oracle_create failed")
 

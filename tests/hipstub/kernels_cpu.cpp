@@ -707,21 +707,25 @@ void ns_range_merge(hipStream_t s, int ntb, const int32_t* at, const int32_t* cn
     nfaces[ntb] = 0;
   });
 }
-void ns_dof_keys(hipStream_t s, int64_t total, int ntb, const int64_t* fstart, const int32_t* at, const int32_t* mcnt,
-                 const int64_t* mstart, const int64_t* mend, const int32_t* fdof, uint64_t* keys) {
+void ns_dof_keys(hipStream_t s, int npf, int64_t total, int ntb, const int64_t* fstart, const int32_t* at,
+                 const int32_t* mcnt, const int64_t* mstart, const int64_t* mend, const int32_t* fdof, uint64_t* keys) {
   hipstub::enqueue(s, [=]() {
     for (int ti = 0; ti < ntb; ++ti) {
       int64_t e = fstart[ti];
       for (int q = at[ti]; q < at[ti] + mcnt[ti]; ++q)
         for (int64_t pos = mstart[q]; pos < mend[q]; ++pos, ++e)
-          for (int c = 0; c < 3; ++c) keys[3 * e + c] = ((uint64_t)(uint32_t)ti << 32) | (uint32_t)fdof[3 * pos + c];
+          for (int c = 0; c < npf; ++c) {
+            int d = fdof[npf * pos + c];
+            if (d < 0) d = fdof[npf * pos + (c - 3)];
+            keys[npf * e + c] = ((uint64_t)(uint32_t)ti << 32) | (uint32_t)d;
+          }
     }
     (void)total;
   });
 }
-void ns_face_cols(hipStream_t s, int64_t total, int ntb, const int64_t* fstart, const int32_t* at, const int32_t* mcnt,
-                  const int64_t* mstart, const int64_t* mend, const int32_t* fdof, const int64_t* dptr,
-                  const int32_t* dofs, int32_t* cols) {
+void ns_face_cols(hipStream_t s, int npf, int64_t total, int ntb, const int64_t* fstart, const int32_t* at,
+                  const int32_t* mcnt, const int64_t* mstart, const int64_t* mend, const int32_t* fdof,
+                  const int64_t* dptr, const int32_t* dofs, int32_t* cols) {
   hipstub::enqueue(s, [=]() {
     for (int ti = 0; ti < ntb; ++ti) {
       int64_t e = fstart[ti];
@@ -729,7 +733,10 @@ void ns_face_cols(hipStream_t s, int64_t total, int ntb, const int64_t* fstart, 
       const int64_t n = dptr[ti + 1] - dptr[ti];
       for (int q = at[ti]; q < at[ti] + mcnt[ti]; ++q)
         for (int64_t pos = mstart[q]; pos < mend[q]; ++pos, ++e)
-          for (int c = 0; c < 3; ++c) cols[3 * e + c] = (int32_t)(std::lower_bound(d, d + n, fdof[3 * pos + c]) - d);
+          for (int c = 0; c < npf; ++c) {
+            const int v = fdof[npf * pos + c];
+            cols[npf * e + c] = v < 0 ? -1 : (int32_t)(std::lower_bound(d, d + n, v) - d);
+          }
     }
     (void)total;
   });
@@ -909,15 +916,33 @@ void p2m_build(hipStream_t s, int P, const P2MBuild& a, double* mat) {
         const int f = A.faces[fi];
         for (int j = 0; j < A.q; ++j) {
           const double b0 = A.rule[4 * j], b1 = A.rule[4 * j + 1], b2 = A.rule[4 * j + 2], w = A.rule[4 * j + 3];
-          const double J = A.fJ[f];
-          const double* X = A.fX + 9 * (int64_t)f;
-          const double x = (b0 * X[0] + b1 * X[3] + b2 * X[6] - cx) / sc;
-          const double y = (b0 * X[1] + b1 * X[4] + b2 * X[7] - cy) / sc;
-          const double z = (b0 * X[2] + b1 * X[5] + b2 * X[8] - cz) / sc;
-          const double W = J * w;
-          const double pw[3] = {W * b0, W * b1, W * b2};
-          const double pn[3] = {A.fcr[3 * f] / J, A.fcr[3 * f + 1] / J, A.fcr[3 * f + 2] / J};
-          const int32_t* plc = A.face_lc + 3 * fi;
+          const int npf = A.npf;
+          const int32_t* plc = A.face_lc + (int64_t)npf * fi;
+          double x, y, z, pw[6], pn[3];
+          if (npf == 3) {
+            const double J = A.fJ[f];
+            const double* X = A.fX + 9 * (int64_t)f;
+            x = (b0 * X[0] + b1 * X[3] + b2 * X[6] - cx) / sc;
+            y = (b0 * X[1] + b1 * X[4] + b2 * X[7] - cy) / sc;
+            z = (b0 * X[2] + b1 * X[5] + b2 * X[8] - cz) / sc;
+            const double W = J * w;
+            pw[0] = W * b0;
+            pw[1] = W * b1;
+            pw[2] = W * b2;
+            for (int a = 0; a < 3; ++a) pn[a] = A.fcr[3 * f + a] / J;
+          } else {
+            double xq[3], cr[3], N[6];
+            bemfmm::tri6_geom(A.fX + 18 * (int64_t)f, b0, b1, xq, cr);
+            unsigned active = 0;
+            for (int e = 0; e < 3; ++e) active |= (plc[3 + e] >= 0 ? 1u : 0u) << e;
+            bemfmm::tri6_density(b0, b1, active, N);
+            const double Jq = std::sqrt(cr[0] * cr[0] + cr[1] * cr[1] + cr[2] * cr[2]);
+            x = (xq[0] - cx) / sc;
+            y = (xq[1] - cy) / sc;
+            z = (xq[2] - cz) / sc;
+            for (int k = 0; k < 6; ++k) pw[k] = Jq * w * N[k];
+            for (int a = 0; a < 3; ++a) pn[a] = Jq > 1e-30 ? cr[a] / Jq : A.fcr[3 * f + a] / A.fJ[f];
+          }
           const double r2 = x * x + y * y + z * z;
           re[0] = 1.;
           im[0] = 0.;
@@ -972,7 +997,8 @@ void p2m_build(hipStream_t s, int P, const P2MBuild& a, double* mat) {
                 const double di = -(ar * p1i + ai * p1r) + (br * m1i + bi * m1r) + pn[2] * z0i;
                 v = isim ? -(c / sc) * di : (c / sc) * dr;
               }
-              for (int k = 0; k < 3; ++k) M[(int64_t)(layer * ncol + plc[k]) * A.CP + row] += pw[k] * v;
+              for (int k = 0; k < npf; ++k)
+                if (plc[k] >= 0) M[(int64_t)(layer * ncol + plc[k]) * A.CP + row] += pw[k] * v;
             }
         }
       }
@@ -984,8 +1010,11 @@ void near_quad(hipStream_t s, int64_t nrows, const NearQuad& a, const Gemv& g, d
   const NearQuad A = a;
   const Gemv G = g;
   hipstub::enqueue(s, [=]() {
-    double dun[7][4];
+    double dun[7][4], dun6[6][4] = {};
     for (int i = 0; i < 28; ++i) dun[i / 4][i % 4] = A.dun_rule[i];
+    if (A.npf == 6)
+      for (int i = 0; i < 24; ++i) dun6[i / 4][i % 4] = A.dun6_rule[i];
+    const int npf = A.npf;
     for (int64_t r = 0; r < nrows; ++r) {
       const int b = A.row_block[r];
       const int ncd = A.ncol_dof[b];
@@ -1000,26 +1029,34 @@ void near_quad(hipStream_t s, int64_t nrows, const NearQuad& a, const Gemv& g, d
       }
       const double x = A.Xs[3 * t], y = A.Xs[3 * t + 1], z = A.Xs[3 * t + 2];
       const int32_t* cols = G.col_idx + c0;
-      const int32_t* fcols = A.face_cols ? A.face_cols + 3 * A.block_fstart[b] : nullptr;
+      const int32_t* fcols = A.face_cols ? A.face_cols + npf * A.block_fstart[b] : nullptr;
       int64_t li = 0;  // local face index in the block's ranges
       for (int64_t gi = A.range_ptr[b]; gi < A.range_ptr[b + 1]; ++gi)
         for (int64_t fi = A.ranges[2 * gi]; fi < A.ranges[2 * gi + 1]; ++fi, ++li) {
           const int64_t f = fi;  // face arrays in face_perm order
           int slot = -1;
-          for (int k = 0; k < 3; ++k)
-            if (A.fv[3 * f + k] == t) slot = k;
-          double WG[3], WGn[3];
-          bemfmm::face_weights(A.fX + 9 * (int64_t)f, A.fJ[f], A.fcr + 3 * (int64_t)f, A.fcent + 3 * (int64_t)f, slot, x, y,
-                               z, A.near_region, kGW5, dun, A.q, WG, WGn);
-          for (int k = 0; k < 3; ++k) {
-            const int d = A.fdof[3 * f + k];
+          for (int k = 0; k < npf; ++k)
+            if (A.fv[npf * f + k] == t) slot = k;
+          double WG[6], WGn[6];
+          if (npf == 3) {
+            bemfmm::face_weights(A.fX + 9 * (int64_t)f, A.fJ[f], A.fcr + 3 * (int64_t)f, A.fcent + 3 * (int64_t)f, slot,
+                                 x, y, z, A.near_region, kGW5, dun, A.q, WG, WGn);
+          } else {
+            unsigned active = 0;
+            for (int e = 0; e < 3; ++e) active |= (A.fdof[6 * f + 3 + e] >= 0 ? 1u : 0u) << e;
+            bemfmm::face_weights_quad(A.fX + 18 * (int64_t)f, active, A.fcent + 3 * (int64_t)f, slot, x, y, z,
+                                      A.near_region, kGW5, dun, A.q, dun6, WG, WGn);
+          }
+          for (int k = 0; k < npf; ++k) {
+            const int d = A.fdof[npf * f + k];
+            if (d < 0) continue;
             int lo2 = 0, hi2 = ncd - 1;
             while (lo2 < hi2) {
               const int mid = (lo2 + hi2) >> 1;
               if (cols[mid] < d) lo2 = mid + 1;
               else hi2 = mid;
             }
-            if (fcols && fcols[3 * li + k] != lo2) throw std::runtime_error("near quadrature: face column table");
+            if (fcols && fcols[npf * li + k] != lo2) throw std::runtime_error("near quadrature: face column table");
             A0[lo2] += WG[k];
             A1[lo2] += WGn[k];
           }
