@@ -142,6 +142,12 @@ def kernel_key(name: str) -> str:
     """`k_m2l_rot<10>` from a demangled dispatch name such as
     `void (anonymous namespace)::k_m2l_rot<10>(int, float const*, ...)`."""
     n = name.replace("void ", "").replace("(anonymous namespace)::", "")
+    if "rocprim::" in n:  # library sorts and scans of the setup: one line per algorithm stage
+        for stage in ("radix_sort_onesweep_iteration", "radix_sort_onesweep_global_offsets", "radix_sort_block_sort",
+                      "merge_sort_block_merge", "merge_mergepath_partition", "scan_impl", "transform", "reduce"):
+            if stage in n:
+                return f"rocprim::{stage}"
+        return "rocprim::other"
     depth, out = 0, []
     for ch in n:
         if ch == "(" and depth == 0:
