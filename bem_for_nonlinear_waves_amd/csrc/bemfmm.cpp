@@ -15,6 +15,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <future>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -298,6 +299,7 @@ struct bemfmm_ctx {
   // the ROCm 7 runtime kept buffers used by the graphs allocated after they were freed.
   hipStream_t gstream = nullptr, gstream2 = nullptr;
   hipEvent_t ev_gfork = nullptr, ev_gjoin = nullptr, ev_gin = nullptr, ev_gout = nullptr;
+  hipEvent_t prep_ev[2] = {};  // around the near-field quadrature of a prepare (timing)
   DBuf<int32_t> near_ncd;  // DOF columns per near block
   double* op_stage = nullptr;  // pinned host staging for operator uploads
   size_t op_stage_n = 0;
@@ -1373,58 +1375,26 @@ void do_prepare(bemfmm_ctx* c) {
                         c->kind_i.data(), c->point_i.data());
   // the tree stage and the near-field structure on the GPU (path keys use 5 bits per level of a 64-bit key)
   const bool dev_plan = c->opts.tree_on_gpu && o.max_level <= 11;
-  if (dev_plan) plan_tree_lists_device(c, pl);
-  else bemfmm::plan_tree_lists(pl);
-  bemfmm::plan_finish(pl, !dev_plan);
+  // With the device plan, the host's share of the plan (P2M source lists, translation pair groups) runs on a
+  // host thread beside the device near structure and the near-field quadrature, which do not read it; it is
+  // joined before its first use below. (The future's destructor waits for it if an exception leaves first.)
+  std::future<void> host_finish;
+  if (dev_plan) {
+    plan_tree_lists_device(c, pl);
+    bemfmm::plan_level_sides(pl);
+    host_finish = std::async(std::launch::async, [&pl] { bemfmm::plan_finish(pl, false); });
+  } else {
+    bemfmm::plan_tree_lists(pl);
+    bemfmm::plan_finish(pl, true);
+  }
   c->C = pl.C;
   c->n_boxes = pl.n_boxes;
-  c->p2m_rows_pad = pl.p2m_rows_pad;
-  if (pl.p2m_rows_pad > 4 * 128) throw std::runtime_error("expansion order too large for the P2M kernel");
   if ((pl.C + 63) / 64 > 7) throw std::runtime_error("expansion order too large for the translation kernel");
   if (2 * pl.C > 256) throw std::runtime_error("expansion order too large for the M2L reduction");
   hipStream_t s = c->stream;
   const double t1 = now();
-  // ---- upload ----
+  // ---- upload: DOF data, work vectors, face arrays, near structure ----
   if (!dev_plan) c->near.upload(pl.near, s);
-  c->p2m.upload(pl.p2m, s);
-  c->p2m.mat.alloc(pl.p2m.mat_size);
-  fill(c, c->p2m.mat.p, pl.p2m.mat_size, 0.);  // rows past C stay zero
-  c->RP = (pl.C + 63) / 64 * 64;
-  c->KP = (pl.C + 4 * dev::kKG - 1) / (4 * dev::kKG) * (4 * dev::kKG);
-  // plane-wave composed M2L operators: kind 4 (10-digit rule) or 5 (6-digit); m2l_kernel 4 uses them for the FP64
-  // operator too, 3 only for the FP32 one (its own tiles, below)
-  const int pw_kind = c->opts.m2l_kernel >= 3 ? (c->opts.plane_wave_digits == 6 ? 5 : 4) : 0;
-  const std::vector<int32_t> slot = update_operator_table(c, pl, c->opts.m2l_kernel == 4 ? pw_kind : 0);
-  c->m2m.clear();
-  c->l2l.clear();
-  c->m2m.resize(pl.n_levels);
-  c->l2l.resize(pl.n_levels);
-  for (int l = 1; l < pl.n_levels; ++l) {
-    c->m2m[l].upload(pl.m2m_by_level[l], s, slot);
-    c->l2l[l].upload(pl.l2l_by_level[l], s, slot);
-  }
-  c->m2l.upload(pl.m2l, s, slot, c->opts.m2l_tile_pairs > 0 ? c->opts.m2l_tile_pairs : kTPM2L);
-  c->m2l_slot = slot;
-  c->m2l_min_level = pl.n_levels;  // no M2L pairs: no translations at all
-  for (int32_t d : pl.m2l.dst) c->m2l_min_level = std::min(c->m2l_min_level, (int)pl.box_level[d]);
-  {
-    std::vector<int64_t> tptr(pl.n_boxes + 1, 0);
-    for (int32_t d : pl.m2l.dst) tptr[d + 1]++;
-    for (int b = 0; b < pl.n_boxes; ++b) tptr[b + 1] += tptr[b];
-    std::vector<int32_t> tp(pl.m2l.dst.size());
-    std::vector<int64_t> fillp(tptr.begin(), tptr.end() - 1);
-    for (int64_t q = 0; q < (int64_t)pl.m2l.dst.size(); ++q) tp[fillp[pl.m2l.dst[q]]++] = (int32_t)q;
-    c->m2l_tptr.upload(tptr, s);
-    c->m2l_tpairs.upload(tp, s);
-    std::vector<int32_t> rorder(pl.n_boxes);
-    for (int b = 0; b < pl.n_boxes; ++b) rorder[b] = b;
-    std::stable_sort(rorder.begin(), rorder.end(),
-                     [&](int32_t x, int32_t y) { return tptr[x + 1] - tptr[x] > tptr[y + 1] - tptr[y]; });
-    c->m2l_rorder.upload(rorder, s);
-    // FP64 [pair][layer][C], or FP32 [pair][layer][stage_stride<float>(C)] in the same buffer
-    const size_t col = std::max<size_t>(pl.C, (dev::stage_stride<float>(pl.C) + 1) / 2);
-    c->m2l_stage.alloc(std::max<size_t>(1, pl.m2l.dst.size() * 2 * col));
-  }
   c->kind.upload(pl.dof_kind, s);
   c->point.upload(pl.dof_point, s);
   upload_knowns(c);
@@ -1517,6 +1487,84 @@ void do_prepare(bemfmm_ctx* c) {
     d_dun.upload(std::vector<double>(&dun[0][0], &dun[0][0] + 28), s);
     c->sb.d_dun6.upload(std::vector<double>(&d6[0][0], &d6[0][0] + 24), s);  // far rule of quadratic faces
   }
+  // ---- near-field quadrature on the GPU (timed by events: the host goes on while it runs) ----
+  HIPCHECK(hipEventRecord(c->prep_ev[0], s));
+  fill(c, c->near.mat.p, pl.near.mat_size, 0.);
+  const int64_t nrows = c->near_rows;
+  if (nrows > 0) {
+    dev::NearQuad a;
+    a.C = pl.C;
+    a.q = o.q;
+    a.npf = pl.npf;
+    a.dun6_rule = c->sb.d_dun6.p;
+    for (int32_t v : pl.near_ncol_dof) a.max_ncd = std::max(a.max_ncd, (int)v);
+    if (dev_plan) {
+      a.face_cols = c->nbuf.face_cols.p;
+      a.block_fstart = c->nbuf.block_fstart.p;
+    }
+    a.near_region = pl.near_region;
+    a.row_block = row_block.p;
+    a.ncol_dof = ncol_dof.p;
+    a.range_ptr = range_ptr.p;
+    a.ranges = ranges.p;
+    a.range_prefix = range_prefix.p;
+    a.face_perm = face_perm.p;
+    a.Xs = Xs.p;
+    a.fX = fX.p;
+    a.fcent = fcent.p;
+    a.fcr = fcr.p;
+    a.fJ = fJ.p;
+    a.fv = fv.p;
+    a.fdof = fdof.p;
+    a.l2p = l2p.p;
+    a.dun_rule = d_dun.p;
+    dev::near_quad(s, nrows, a, c->near.view(), c->near.mat.p);
+  }
+  HIPCHECK(hipGetLastError());
+  HIPCHECK(hipEventRecord(c->prep_ev[1], s));
+  // ---- the host's share of the plan, then the P2M blocks and the translation operators ----
+  if (host_finish.valid()) host_finish.get();
+  c->p2m_rows_pad = pl.p2m_rows_pad;
+  if (pl.p2m_rows_pad > 4 * 128) throw std::runtime_error("expansion order too large for the P2M kernel");
+  c->p2m.upload(pl.p2m, s);
+  c->p2m.mat.alloc(pl.p2m.mat_size);
+  fill(c, c->p2m.mat.p, pl.p2m.mat_size, 0.);  // rows past C stay zero
+  c->RP = (pl.C + 63) / 64 * 64;
+  c->KP = (pl.C + 4 * dev::kKG - 1) / (4 * dev::kKG) * (4 * dev::kKG);
+  // plane-wave composed M2L operators: kind 4 (10-digit rule) or 5 (6-digit); m2l_kernel 4 uses them for the FP64
+  // operator too, 3 only for the FP32 one (its own tiles, below)
+  const int pw_kind = c->opts.m2l_kernel >= 3 ? (c->opts.plane_wave_digits == 6 ? 5 : 4) : 0;
+  const std::vector<int32_t> slot = update_operator_table(c, pl, c->opts.m2l_kernel == 4 ? pw_kind : 0);
+  c->m2m.clear();
+  c->l2l.clear();
+  c->m2m.resize(pl.n_levels);
+  c->l2l.resize(pl.n_levels);
+  for (int l = 1; l < pl.n_levels; ++l) {
+    c->m2m[l].upload(pl.m2m_by_level[l], s, slot);
+    c->l2l[l].upload(pl.l2l_by_level[l], s, slot);
+  }
+  c->m2l.upload(pl.m2l, s, slot, c->opts.m2l_tile_pairs > 0 ? c->opts.m2l_tile_pairs : kTPM2L);
+  c->m2l_slot = slot;
+  c->m2l_min_level = pl.n_levels;  // no M2L pairs: no translations at all
+  for (int32_t d : pl.m2l.dst) c->m2l_min_level = std::min(c->m2l_min_level, (int)pl.box_level[d]);
+  {
+    std::vector<int64_t> tptr(pl.n_boxes + 1, 0);
+    for (int32_t d : pl.m2l.dst) tptr[d + 1]++;
+    for (int b = 0; b < pl.n_boxes; ++b) tptr[b + 1] += tptr[b];
+    std::vector<int32_t> tp(pl.m2l.dst.size());
+    std::vector<int64_t> fillp(tptr.begin(), tptr.end() - 1);
+    for (int64_t q = 0; q < (int64_t)pl.m2l.dst.size(); ++q) tp[fillp[pl.m2l.dst[q]]++] = (int32_t)q;
+    c->m2l_tptr.upload(tptr, s);
+    c->m2l_tpairs.upload(tp, s);
+    std::vector<int32_t> rorder(pl.n_boxes);
+    for (int b = 0; b < pl.n_boxes; ++b) rorder[b] = b;
+    std::stable_sort(rorder.begin(), rorder.end(),
+                     [&](int32_t x, int32_t y) { return tptr[x + 1] - tptr[x] > tptr[y + 1] - tptr[y]; });
+    c->m2l_rorder.upload(rorder, s);
+    // FP64 [pair][layer][C], or FP32 [pair][layer][stage_stride<float>(C)] in the same buffer
+    const size_t col = std::max<size_t>(pl.C, (dev::stage_stride<float>(pl.C) + 1) / 2);
+    c->m2l_stage.alloc(std::max<size_t>(1, pl.m2l.dst.size() * 2 * col));
+  }
   // ---- P2M matrices on the GPU ----
   if (c->p2m.nblk > 0) {
     if ((int)pl.p2m_face_ptr.size() != c->p2m.nblk + 1 || pl.p2m_face_lc.size() != (size_t)npf * pl.p2m_faces.size())
@@ -1563,42 +1611,12 @@ void do_prepare(bemfmm_ctx* c) {
     HIPCHECK(hipGetLastError());
   }
   HIPCHECK(hipStreamSynchronize(s));
-  const double t2 = now();
-  // ---- near-field quadrature on the GPU ----
-  fill(c, c->near.mat.p, pl.near.mat_size, 0.);
-  const int64_t nrows = c->near_rows;
-  if (nrows > 0) {
-    dev::NearQuad a;
-    a.C = pl.C;
-    a.q = o.q;
-    a.npf = pl.npf;
-    a.dun6_rule = c->sb.d_dun6.p;
-    for (int32_t v : pl.near_ncol_dof) a.max_ncd = std::max(a.max_ncd, (int)v);
-    if (dev_plan) {
-      a.face_cols = c->nbuf.face_cols.p;
-      a.block_fstart = c->nbuf.block_fstart.p;
-    }
-    a.near_region = pl.near_region;
-    a.row_block = row_block.p;
-    a.ncol_dof = ncol_dof.p;
-    a.range_ptr = range_ptr.p;
-    a.ranges = ranges.p;
-    a.range_prefix = range_prefix.p;
-    a.face_perm = face_perm.p;
-    a.Xs = Xs.p;
-    a.fX = fX.p;
-    a.fcent = fcent.p;
-    a.fcr = fcr.p;
-    a.fJ = fJ.p;
-    a.fv = fv.p;
-    a.fdof = fdof.p;
-    a.l2p = l2p.p;
-    a.dun_rule = d_dun.p;
-    dev::near_quad(s, nrows, a, c->near.view(), c->near.mat.p);
-  }
-  HIPCHECK(hipGetLastError());
-  HIPCHECK(hipStreamSynchronize(s));
+  float nq_ms = 0.f;
+  HIPCHECK(hipEventElapsedTime(&nq_ms, c->prep_ev[0], c->prep_ev[1]));
   const double t3 = now();
+  // the near quadrature ran beside the host work of this stage: its own time from the events, the rest of the
+  // stage as the upload time
+  const double t_near_quad = 1e-3 * nq_ms, t_upload = std::max(0., (t3 - t1) - t_near_quad);
   // ---- rigid-mode alpha (updateGeometricProperties) ----
   evaluate(c, 2, nullptr, c->alpha.p);
   HIPCHECK(hipStreamSynchronize(s));
@@ -1849,8 +1867,8 @@ void do_prepare(bemfmm_ctx* c) {
   st.t_near_struct = pl.t_near_struct;
   st.t_p2m = pl.t_p2m;
   st.t_ops = pl.t_ops;
-  st.t_upload = t2 - t1;
-  st.t_near_quad = t3 - t2;
+  st.t_upload = t_upload;
+  st.t_near_quad = t_near_quad;
   st.t_alpha = t3a - t3;
   st.t_tune = t3b - t3a;
   st.t_precond = t3c - t3b;
@@ -2535,6 +2553,7 @@ bemfmm_ctx* bemfmm_ctx_create(int device, const bemfmm_opts* opts) {
     HIPCHECK(hipStreamCreateWithFlags(&c->gstream2, hipStreamNonBlocking));
     for (hipEvent_t* e : {&c->ev_fork, &c->ev_join, &c->ev_gfork, &c->ev_gjoin, &c->ev_gin, &c->ev_gout})
       HIPCHECK(hipEventCreateWithFlags(e, hipEventDisableTiming));
+    for (hipEvent_t& e : c->prep_ev) HIPCHECK(hipEventCreate(&e));
     return BEMFMM_OK;
   });
   if (rc != BEMFMM_OK) {
@@ -2553,7 +2572,7 @@ void bemfmm_ctx_destroy(bemfmm_ctx* c) {
     if (s) (void)hipStreamSynchronize(s);
   for (auto& e : c->ev)
     if (e) (void)hipEventDestroy(e);
-  for (auto e : {c->ev_fork, c->ev_join, c->ev_gfork, c->ev_gjoin, c->ev_gin, c->ev_gout})
+  for (auto e : {c->ev_fork, c->ev_join, c->ev_gfork, c->ev_gjoin, c->ev_gin, c->ev_gout, c->prep_ev[0], c->prep_ev[1]})
     if (e) (void)hipEventDestroy(e);
   drop_graphs(c);
   if (c->op_stage) (void)pinned_free(c->op_stage);

@@ -19,6 +19,21 @@
 
 namespace bemfmm {
 
+// 1 / sqrt(a) for a > 0. On the GPU: the hardware estimate (v_rsq_f64, about 23 correct bits) refined by two
+// Newton steps y += y (1 - a y^2) / 2, each of which doubles the correct bits, to within an ulp or two of the
+// correctly rounded value; it replaces a correctly rounded square root followed by a division, the two longest
+// instruction sequences of the quadrature loop. On the host: the correctly rounded expression.
+BEMFMM_HD inline double inv_sqrt(double a) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  double y = __builtin_amdgcn_rsq(a);
+  y = fma(0.5 * y, fma(-(a * y), y, 1.0), y);
+  y = fma(0.5 * y, fma(-(a * y), y, 1.0), y);
+  return y;
+#else
+  return 1. / sqrt(a);
+#endif
+}
+
 BEMFMM_HD inline void face_weights(const double* Xf, double J, const double* cr, const double* cent, int slot, double x,
                                    double y, double z, double near_region, const double (*gw5)[3],
                                    const double (*dun)[4], int q, double WG[3], double WGn[3]) {
@@ -34,7 +49,7 @@ BEMFMM_HD inline void face_weights(const double* Xf, double J, const double* cr,
       const double qx = N0 * Xf[3 * p0] + N1 * Xf[3 * p1] + N2 * Xf[3 * p2] - x;
       const double qy = N0 * Xf[3 * p0 + 1] + N1 * Xf[3 * p1 + 1] + N2 * Xf[3 * p2 + 1] - y;
       const double qz = N0 * Xf[3 * p0 + 2] + N1 * Xf[3 * p1 + 2] + N2 * Xf[3 * p2 + 2] - z;
-      const double nr_inv = 1. / sqrt(qx * qx + qy * qy + qz * qz);
+      const double nr_inv = inv_sqrt(qx * qx + qy * qy + qz * qz);
       const double tmp = ww * (1. - t0) * nr_inv;
       const double rc = -((qx * nr_inv) * crx + (qy * nr_inv) * cry + (qz * nr_inv) * crz);
       G0 += J * (tmp * N0);
@@ -61,7 +76,7 @@ BEMFMM_HD inline void face_weights(const double* Xf, double J, const double* cr,
       const double qx = N[0] * Xf[0] + N[1] * Xf[3] + N[2] * Xf[6] - x;
       const double qy = N[0] * Xf[1] + N[1] * Xf[4] + N[2] * Xf[7] - y;
       const double qz = N[0] * Xf[2] + N[1] * Xf[5] + N[2] * Xf[8] - z;
-      const double nr_inv = 1. / sqrt(qx * qx + qy * qy + qz * qz);
+      const double nr_inv = inv_sqrt(qx * qx + qy * qy + qz * qz);
       const double rc = (qx * nr_inv) * crx + (qy * nr_inv) * cry + (qz * nr_inv) * crz;
       for (int k = 0; k < 3; ++k) {
         const double tmp = (ww * (1. - t0) * N[k]) * nr_inv;
@@ -76,7 +91,7 @@ BEMFMM_HD inline void face_weights(const double* Xf, double J, const double* cr,
       const double qx = N[0] * Xf[0] + N[1] * Xf[3] + N[2] * Xf[6] - x;
       const double qy = N[0] * Xf[1] + N[1] * Xf[4] + N[2] * Xf[7] - y;
       const double qz = N[0] * Xf[2] + N[1] * Xf[5] + N[2] * Xf[8] - z;
-      const double nr_inv = 1. / sqrt(qx * qx + qy * qy + qz * qz);
+      const double nr_inv = inv_sqrt(qx * qx + qy * qy + qz * qz);
       const double rc = (qx * nr_inv) * crx + (qy * nr_inv) * cry + (qz * nr_inv) * crz;
       for (int k = 0; k < 3; ++k) {
         const double tmp = ww * N[k] * nr_inv;
@@ -148,10 +163,10 @@ BEMFMM_HD inline void face_weights_quad(const double* X6, unsigned active, const
     tri6_geom(X6, b0, b1, xq, cr);
     tri6_density(b0, b1, active, N);
     const double Rx = xq[0] - x, Ry = xq[1] - y, Rz = xq[2] - z;
-    const double nr = sqrt(Rx * Rx + Ry * Ry + Rz * Rz);
+    const double ir = inv_sqrt(Rx * Rx + Ry * Ry + Rz * Rz);
     const double J = sqrt(cr[0] * cr[0] + cr[1] * cr[1] + cr[2] * cr[2]);
-    const double g = J * w / nr;
-    const double gn = -((Rx / nr) * cr[0] + (Ry / nr) * cr[1] + (Rz / nr) * cr[2]) * w / nr / nr;
+    const double g = J * w * ir;
+    const double gn = -((Rx * ir) * cr[0] + (Ry * ir) * cr[1] + (Rz * ir) * cr[2]) * w * ir * ir;
     for (int k = 0; k < 6; ++k) {
       WG[k] += g * N[k];
       if (k != drop) WGn[k] += gn * N[k];

@@ -461,14 +461,18 @@ void build_plan(Plan& pl, const Options& opt, int n_nodes, const double* X, int 
   plan_finish(pl);
 }
 
+void plan_level_sides(Plan& pl) {
+  pl.level_side.resize(pl.n_levels);
+  for (int l = 0; l < pl.n_levels; ++l) pl.level_side[l] = pl.root_side / std::ldexp(1.0, l);
+}
+
 void plan_finish(Plan& pl, bool near_on_host) {
   const Options& opt = pl.opt;
   const int npf = pl.npf;
   const int P = pl.P, C = pl.C, n_nodes = pl.n_nodes, n_dof = pl.n_dof, nb = pl.n_boxes;
   const int32_t* face_dof = pl.fdof.data();
   const auto& rule = dunavant(opt.q);
-  pl.level_side.resize(pl.n_levels);
-  for (int l = 0; l < pl.n_levels; ++l) pl.level_side[l] = pl.root_side / std::ldexp(1.0, l);
+  if ((int)pl.level_side.size() != pl.n_levels) plan_level_sides(pl);
   const double t2 = now();
   // --- near blocks: one per target box (split into <= kMaxRows rows) ---
   if (near_on_host) {

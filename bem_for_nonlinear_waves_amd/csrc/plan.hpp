@@ -124,8 +124,11 @@ void plan_geometry(Plan& pl, const Options& opt, int n_nodes, const double* X, i
 // the octree, its interaction lists and the target box of every node, on the host (sets n_levels .. m2l_idx,
 // target_box, n_leaves)
 void plan_tree_lists(Plan& pl);
+// the box side of every level (from the tree's level count and the root side)
+void plan_level_sides(Plan& pl);
 // near blocks and L2P coefficients (unless near_on_host is false: the device builds them), P2M blocks and the
-// translation groups from the tree arrays
+// translation groups from the tree arrays; writes no field that the device near-structure stage reads or writes
+// (level_side is only set here when plan_level_sides has not run), so the two may run at the same time
 void plan_finish(Plan& pl, bool near_on_host = true);
 
 // Real-field packed translation operators (operators.cpp), computed in __float128
