@@ -22,6 +22,7 @@
 #include <shared_mutex>
 #include <stdexcept>
 #include <string>
+#include <type_traits>
 #include <utility>
 #include <vector>
 
@@ -280,6 +281,8 @@ struct bemfmm_ctx {
   int ops_uploaded = 0;  // operators uploaded by the last prepare
   std::vector<int32_t> m2l_slot;  // plan op -> table slot (for re-tiling by the autotuner)
   int m2l_tp = 0;                // pairs per M2L tile in use
+  int gemv_var[2] = {0, 0};       // near GEMV launch shape (dev::Gemv::variant) for the FP64 / FP32 operator
+  int p2m_var[2] = {0, 0};        // P2M launch shape for the FP64 / FP32 operator
   // rotation M2L of the FP32 operator: factors [slot][RS] (slots of the operator table; M2L slots only)
   bool m2l_rot = false;
   DBuf<float> rot32;
@@ -426,7 +429,17 @@ dev::Schwarz schwarz_view(const bemfmm_ctx* c) {
 template <typename T>
 void launch_gemv(bemfmm_ctx* c, DGemv& g, const T* mat, const double* x0, const double* x1, double* y0, double* y1,
                  int part = 0, hipStream_t s = nullptr) {
-  dev::block_gemv(s ? s : c->stream, g.view(c->near_ncd.p), mat, x0, x1, y0, y1, part);
+  dev::Gemv v = g.view(c->near_ncd.p);
+  v.variant = c->gemv_var[std::is_same<T, float>::value ? 1 : 0];
+  dev::block_gemv(s ? s : c->stream, v, mat, x0, x1, y0, y1, part);
+}
+
+template <typename T>
+void launch_p2m(bemfmm_ctx* c, const T* mat, const double* x0, const double* x1, double* M0, double* M1,
+                hipStream_t s) {
+  dev::Gemv v = c->p2m.view();
+  v.variant = c->p2m_var[std::is_same<T, float>::value ? 1 : 0];
+  dev::p2m(s, c->C, c->p2m_rows_pad, v, mat, x0, x1, M0, M1);
 }
 
 template <typename T>
@@ -509,8 +522,8 @@ void evaluate_launch(bemfmm_ctx* c, int mode, const double* in, double* out, boo
   fill(c, w.M0, nbC, 0., st);
   fill(c, w.M1, nbC, 0., st);
   rec(1);
-  if (lo) dev::p2m(st, c->C, c->p2m_rows_pad, c->p2m.view(), c->p2m_mat32.p, phin, phi, w.M0, w.M1);
-  else dev::p2m(st, c->C, c->p2m_rows_pad, c->p2m.view(), c->p2m.mat.p, phin, phi, w.M0, w.M1);
+  if (lo) launch_p2m(c, c->p2m_mat32.p, phin, phi, w.M0, w.M1, st);
+  else launch_p2m(c, c->p2m.mat.p, phin, phi, w.M0, w.M1, st);
   rec(2);
   // M2M from level l into l-1 only where level l-1 has M2L pairs (coarser multipoles are not used).
   // M2M and L2L use the FP64 operators for both operators: their inputs and outputs are FP64, and the
@@ -915,6 +928,100 @@ void tune_m2l_tiles(bemfmm_ctx* c, const Plan& pl) {
   HIPCHECK(hipGetLastError());
 }
 
+// Launch shapes of the near-field GEMV (rows per wave: 4, 2, 8 for variants 0, 1, 2) and of P2M (loads in flight:
+// 8, 4, 16), per operator precision: as given in the options, or timed with HIP events (best of two runs after a
+// warm-up) at the first prepare of a problem class (precision, expansion order, near-field entries and P2M matrix
+// size to within a factor of two) and reused by later contexts of that class. The timed runs read the context's
+// work vectors, cleared first, and write work buffers that every matvec rewrites.
+constexpr int kRowsPerWave[3] = {4, 2, 8}, kP2MLoads[3] = {8, 4, 16};
+
+void tune_launch_shapes(bemfmm_ctx* c, const Plan& pl) {
+  auto variant_of = [](const int* tab, int v) {
+    for (int k = 0; k < 3; ++k)
+      if (tab[k] == v) return k;
+    return 0;
+  };
+  const bool fp32 = c->opts.precision == BEMFMM_PRECISION_FP32_REFINE;
+  for (int k = 0; k < 2; ++k) {
+    c->gemv_var[k] = variant_of(kRowsPerWave, c->opts.near_rows_per_wave);
+    c->p2m_var[k] = variant_of(kP2MLoads, c->opts.p2m_loads);
+  }
+  const bool tune_gemv = c->opts.near_rows_per_wave == 0 && pl.near_nnz >= (int64_t(1) << 21) && c->near.nblk > 0;
+  const bool tune_p2m = c->opts.p2m_loads == 0 && c->p2m.nblk >= 256;
+  if (!tune_gemv && !tune_p2m) return;
+  static std::mutex mu;
+  static std::map<std::array<int, 4>, std::array<int, 4>> chosen;  // class -> {gemv FP64, FP32, p2m FP64, FP32}
+  const std::array<int, 4> cls{fp32 ? 1 : 0, pl.P, (int)std::log2((double)std::max<int64_t>(pl.near_nnz, 1)),
+                               (int)std::log2((double)std::max<size_t>(c->p2m.mat.n, 1))};
+  {
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = chosen.find(cls);
+    if (it != chosen.end()) {
+      if (tune_gemv) c->gemv_var[0] = it->second[0], c->gemv_var[1] = it->second[1];
+      if (tune_p2m) c->p2m_var[0] = it->second[2], c->p2m_var[1] = it->second[3];
+      return;
+    }
+  }
+  hipStream_t s = c->stream;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  struct Events {  // destroyed on every exit, including a throw from a launch below
+    hipEvent_t* e[2];
+    ~Events() {
+      for (auto* x : e)
+        if (*x) (void)hipEventDestroy(*x);
+    }
+  } guard{{&e0, &e1}};
+  HIPCHECK(hipEventCreate(&e0));
+  HIPCHECK(hipEventCreate(&e1));
+  const int64_t nvec = (int64_t)c->n_dof + (int64_t)pl.n_boxes * pl.C;
+  fill(c, c->buf0.p, nvec, 0.);
+  fill(c, c->buf1.p, nvec, 0.);
+  auto best_of = [&](auto&& launch) {
+    float best = 1e30f;
+    for (int r = 0; r < 3; ++r) {
+      HIPCHECK(hipEventRecord(e0, s));
+      launch();
+      HIPCHECK(hipEventRecord(e1, s));
+      HIPCHECK(hipEventSynchronize(e1));
+      float ms = 0.f;
+      HIPCHECK(hipEventElapsedTime(&ms, e0, e1));
+      if (r > 0) best = std::min(best, ms);
+    }
+    return best;
+  };
+  for (int prec = 0; prec < (fp32 ? 2 : 1); ++prec) {
+    if (tune_gemv) {
+      float bt = 1e30f;
+      int bv = 0;
+      for (int v = 0; v < 3; ++v) {
+        c->gemv_var[prec] = v;
+        const float t = best_of([&] {
+          if (prec) launch_gemv(c, c->near, c->near_mat32.p, c->buf0.p, c->buf1.p, c->y0.p, c->y1.p);
+          else launch_gemv(c, c->near, c->near.mat.p, c->buf0.p, c->buf1.p, c->y0.p, c->y1.p);
+        });
+        if (t < bt) bt = t, bv = v;
+      }
+      c->gemv_var[prec] = bv;
+    }
+    if (tune_p2m) {
+      float bt = 1e30f;
+      int bv = 0;
+      for (int v = 0; v < 3; ++v) {
+        c->p2m_var[prec] = v;
+        const float t = best_of([&] {
+          if (prec) launch_p2m(c, c->p2m_mat32.p, c->buf0.p, c->buf1.p, c->M0.p, c->M1.p, s);
+          else launch_p2m(c, c->p2m.mat.p, c->buf0.p, c->buf1.p, c->M0.p, c->M1.p, s);
+        });
+        if (t < bt) bt = t, bv = v;
+      }
+      c->p2m_var[prec] = bv;
+    }
+  }
+  HIPCHECK(hipGetLastError());
+  std::lock_guard<std::mutex> lk(mu);
+  chosen.emplace(cls, std::array<int, 4>{c->gemv_var[0], c->gemv_var[1], c->p2m_var[0], c->p2m_var[1]});
+}
+
 int bits_for(uint64_t v) {
   int b = 1;
   while (b < 64 && (v >> b)) ++b;
@@ -1314,6 +1421,11 @@ void do_prepare(bemfmm_ctx* c) {
   if (c->opts.m2l_tile_pairs != 0 && c->opts.m2l_tile_pairs != 16 && c->opts.m2l_tile_pairs != 32 &&
       c->opts.m2l_tile_pairs != 64)
     throw std::runtime_error("bemfmm_prepare: m2l_tile_pairs must be 0 (default), 16, 32 or 64");
+  if (c->opts.near_rows_per_wave != 0 && c->opts.near_rows_per_wave != 2 && c->opts.near_rows_per_wave != 4 &&
+      c->opts.near_rows_per_wave != 8)
+    throw std::runtime_error("bemfmm_prepare: near_rows_per_wave must be 0 (tuned), 2, 4 or 8");
+  if (c->opts.p2m_loads != 0 && c->opts.p2m_loads != 4 && c->opts.p2m_loads != 8 && c->opts.p2m_loads != 16)
+    throw std::runtime_error("bemfmm_prepare: p2m_loads must be 0 (tuned), 4, 8 or 16");
   if (c->opts.element_order != 1 && c->opts.element_order != 2)
     throw std::runtime_error("bemfmm_prepare: element_order must be 1 (linear) or 2 (true quadratic)");
   if (c->opts.element_order == 2 &&
@@ -1843,6 +1955,7 @@ void do_prepare(bemfmm_ctx* c) {
     c->m2l_rot = false;
     c->m2l_pw_on = false;
   }
+  tune_launch_shapes(c, pl);
   HIPCHECK(hipStreamSynchronize(s));
   const double t4 = now();
   c->prepared = true;
@@ -1873,6 +1986,11 @@ void do_prepare(bemfmm_ctx* c) {
   st.t_tune = t3b - t3a;
   st.t_precond = t3c - t3b;
   st.t_fp32 = t4 - t3c;
+  for (int k = 0; k < 2; ++k) {
+    const bool used = k == 0 || c->opts.precision == BEMFMM_PRECISION_FP32_REFINE;
+    st.near_rows_per_wave[k] = used ? kRowsPerWave[c->gemv_var[k]] : 0;
+    st.p2m_loads[k] = used ? kP2MLoads[c->p2m_var[k]] : 0;
+  }
   st.n_precond_blocks = c->sw_host.n_blocks();
   st.precond_fallback_blocks = c->sw_failed;
   st.precond_avg_block = c->sw_host.n_blocks() ? (double)c->sw_host.idx.size() / c->sw_host.n_blocks() : 0.;

@@ -83,9 +83,9 @@ __global__ void k_scatter(int mode, int n, const int32_t* __restrict__ kind, con
 }
 
 // Box-blocked dense GEMV (see bemfmm::BlockGemv). One workgroup per block; the gathered
-// inputs are staged through LDS in chunks; each wavefront streams kRPW rows at once with
+// inputs are staged through LDS in chunks; each wavefront streams RPW rows at once with
 // 16-byte vector loads (double2 / float4: rows are padded to a multiple of 4 columns and start
-// 16-byte aligned) and issues all 2*kRPW*2 loads of two column steps before using them (rows
+// 16-byte aligned) and issues all 2*RPW*2 loads of two column steps before using them (rows
 // past the end of the block are clamped to its last row and their sums discarded), so that
 // enough HBM requests are in flight per lane. Sums are accumulated in FP64.
 // part 0: all columns, y = A x. The columns of a block are [DOF columns (ncd) | L2P columns | padding];
@@ -111,7 +111,6 @@ __device__ __forceinline__ float2 ld_nt(const float2* p) {
 }
 constexpr int kGemvChunk = 2048;
 constexpr int kGemvMaxRows = 512;
-constexpr int kRPW = 4;
 template <typename T>
 struct GemvVec;
 template <>
@@ -128,7 +127,9 @@ struct GemvVec<float> {
     return (double)(i == 0 ? v.x : i == 1 ? v.y : i == 2 ? v.z : v.w);
   }
 };
-template <typename T>
+// RPW: rows per wave and pass (each staged x value is read from LDS once per RPW rows); chosen per problem class
+// by the autotuner of the prepare (2, 4 or 8)
+template <typename T, int RPW>
 __global__ __launch_bounds__(256) void k_block_gemv(const int64_t* __restrict__ row_ptr, const int64_t* __restrict__ col_ptr,
                                                     const int64_t* __restrict__ mat_ptr, const int32_t* __restrict__ row_out,
                                                     const int32_t* __restrict__ col_idx, const T* __restrict__ mat,
@@ -174,18 +175,18 @@ __global__ __launch_bounds__(256) void k_block_gemv(const int64_t* __restrict__ 
       xs1[i] = x1[g];
     }
     __syncthreads();
-    for (int rg = wave * kRPW; rg < nrow; rg += 4 * kRPW) {
-      const T* Ar[kRPW];
+    for (int rg = wave * RPW; rg < nrow; rg += 4 * RPW) {
+      const T* Ar[RPW];
 #pragma unroll
-      for (int k = 0; k < kRPW; ++k) Ar[k] = M + (int64_t)min(rg + k, nrow - 1) * 2 * ncol + cs;
-      double s0[kRPW], s1[kRPW];
+      for (int k = 0; k < RPW; ++k) Ar[k] = M + (int64_t)min(rg + k, nrow - 1) * 2 * ncol + cs;
+      double s0[RPW], s1[RPW];
 #pragma unroll
-      for (int k = 0; k < kRPW; ++k) s0[k] = s1[k] = 0.;
+      for (int k = 0; k < RPW; ++k) s0[k] = s1[k] = 0.;
       int c = lane * W;
       for (; c + STEP < cn; c += 2 * STEP) {
-        V va[kRPW][2], vb[kRPW][2];
+        V va[RPW][2], vb[RPW][2];
 #pragma unroll
-        for (int k = 0; k < kRPW; ++k) {
+        for (int k = 0; k < RPW; ++k) {
           va[k][0] = ld_nt(reinterpret_cast<const V*>(Ar[k] + c));
           va[k][1] = ld_nt(reinterpret_cast<const V*>(Ar[k] + c + STEP));
           vb[k][0] = ld_nt(reinterpret_cast<const V*>(Ar[k] + ncol + c));
@@ -200,7 +201,7 @@ __global__ __launch_bounds__(256) void k_block_gemv(const int64_t* __restrict__ 
           xb[1][u] = xs1[c + STEP + u];
         }
 #pragma unroll
-        for (int k = 0; k < kRPW; ++k)
+        for (int k = 0; k < RPW; ++k)
 #pragma unroll
           for (int u = 0; u < W; ++u) {
             s0[k] += GemvVec<T>::get(va[k][0], u) * xa[0][u] + GemvVec<T>::get(va[k][1], u) * xa[1][u];
@@ -208,14 +209,14 @@ __global__ __launch_bounds__(256) void k_block_gemv(const int64_t* __restrict__ 
           }
       }
       if (c < cn) {
-        V va[kRPW], vb[kRPW];
+        V va[RPW], vb[RPW];
 #pragma unroll
-        for (int k = 0; k < kRPW; ++k) {
+        for (int k = 0; k < RPW; ++k) {
           va[k] = ld_nt(reinterpret_cast<const V*>(Ar[k] + c));
           vb[k] = ld_nt(reinterpret_cast<const V*>(Ar[k] + ncol + c));
         }
 #pragma unroll
-        for (int k = 0; k < kRPW; ++k)
+        for (int k = 0; k < RPW; ++k)
 #pragma unroll
           for (int u = 0; u < W; ++u) {
             s0[k] += GemvVec<T>::get(va[k], u) * xs0[c + u];
@@ -223,7 +224,7 @@ __global__ __launch_bounds__(256) void k_block_gemv(const int64_t* __restrict__ 
           }
       }
 #pragma unroll
-      for (int k = 0; k < kRPW; ++k) {
+      for (int k = 0; k < RPW; ++k) {
         double u = s0[k], v = s1[k];
         for (int off = 32; off > 0; off >>= 1) {
           u += __shfl_down(u, off);
@@ -253,8 +254,9 @@ __global__ __launch_bounds__(256) void k_block_gemv(const int64_t* __restrict__ 
 // (layer = tid / 128, row = tid % 128 + 128 k) streams a column's rows with its neighbours: every load is
 // coalesced and every lane has a row, however few DOF columns the leaf has. The gathered phi_n / phi
 // values are staged in LDS in chunks. M[leaf][row] = sum_col A[layer][col][row] x_layer[col] (FP64 sums).
+// U: column loads in flight per thread (4, 8 or 16), chosen per problem class by the autotuner of the prepare
 constexpr int kP2MChunk = 1024;
-template <typename T>
+template <typename T, int U>
 __global__ __launch_bounds__(256) void k_p2m(int C, int CP, const int64_t* __restrict__ row_ptr,
                                              const int64_t* __restrict__ col_ptr, const int64_t* __restrict__ mat_ptr,
                                              const int32_t* __restrict__ row_out, const int32_t* __restrict__ col_idx,
@@ -287,12 +289,12 @@ __global__ __launch_bounds__(256) void k_p2m(int C, int CP, const int64_t* __res
       const T* a = Al + (int64_t)cs * CP + r;
       double s0 = 0., s1 = 0.;
       int c = 0;
-      for (; c + 7 < cn; c += 8) {  // 8 loads in flight (streamed: non-temporal)
-        double v[8];
+      for (; c + U - 1 < cn; c += U) {  // U loads in flight (streamed: non-temporal)
+        double v[U];
 #pragma unroll
-        for (int u = 0; u < 8; ++u) v[u] = (double)__builtin_nontemporal_load(&a[(int64_t)(c + u) * CP]);
+        for (int u = 0; u < U; ++u) v[u] = (double)__builtin_nontemporal_load(&a[(int64_t)(c + u) * CP]);
 #pragma unroll
-        for (int u = 0; u < 8; u += 2) {
+        for (int u = 0; u < U; u += 2) {
           s0 += v[u] * xl[c + u];
           s1 += v[u + 1] * xl[c + u + 1];
         }
@@ -1680,30 +1682,38 @@ void scatter(hipStream_t s, int mode, int n, const int32_t* kind, const double* 
   hipLaunchKernelGGL(k_scatter, dim3(grid1(n)), dim3(256), 0, s, mode, n, kind, kappa, x, phi_k, phin_k, phin, phi);
 }
 
+template <typename T>
+void block_gemv_t(hipStream_t s, const Gemv& g, const T* mat, const double* x0, const double* x1, double* y0,
+                  double* y1, int part) {
+  if (g.nblk == 0) return;
+  auto k = g.variant == 1 ? k_block_gemv<T, 2> : g.variant == 2 ? k_block_gemv<T, 8> : k_block_gemv<T, 4>;
+  hipLaunchKernelGGL(k, dim3(g.nblk), dim3(256), 0, s, g.row_ptr, g.col_ptr, g.mat_ptr, g.row_out, g.col_idx, mat, x0,
+                     x1, y0, y1, part, g.ncol_dof);
+}
 void block_gemv(hipStream_t s, const Gemv& g, const double* mat, const double* x0, const double* x1, double* y0,
                 double* y1, int part) {
-  if (g.nblk == 0) return;
-  hipLaunchKernelGGL(k_block_gemv<double>, dim3(g.nblk), dim3(256), 0, s, g.row_ptr, g.col_ptr, g.mat_ptr, g.row_out,
-                     g.col_idx, mat, x0, x1, y0, y1, part, g.ncol_dof);
+  block_gemv_t(s, g, mat, x0, x1, y0, y1, part);
 }
 void block_gemv(hipStream_t s, const Gemv& g, const float* mat, const double* x0, const double* x1, double* y0,
                 double* y1, int part) {
-  if (g.nblk == 0) return;
-  hipLaunchKernelGGL(k_block_gemv<float>, dim3(g.nblk), dim3(256), 0, s, g.row_ptr, g.col_ptr, g.mat_ptr, g.row_out,
-                     g.col_idx, mat, x0, x1, y0, y1, part, g.ncol_dof);
+  block_gemv_t(s, g, mat, x0, x1, y0, y1, part);
 }
 
+template <typename T>
+void p2m_t(hipStream_t s, int C, int CP, const Gemv& g, const T* mat, const double* x0, const double* x1, double* M0,
+           double* M1) {
+  if (g.nblk == 0) return;
+  auto k = g.variant == 1 ? k_p2m<T, 4> : g.variant == 2 ? k_p2m<T, 16> : k_p2m<T, 8>;
+  hipLaunchKernelGGL(k, dim3(g.nblk), dim3(256), 0, s, C, CP, g.row_ptr, g.col_ptr, g.mat_ptr, g.row_out, g.col_idx,
+                     mat, x0, x1, M0, M1);
+}
 void p2m(hipStream_t s, int C, int CP, const Gemv& g, const double* mat, const double* x0, const double* x1,
          double* M0, double* M1) {
-  if (g.nblk == 0) return;
-  hipLaunchKernelGGL(k_p2m<double>, dim3(g.nblk), dim3(256), 0, s, C, CP, g.row_ptr, g.col_ptr, g.mat_ptr, g.row_out,
-                     g.col_idx, mat, x0, x1, M0, M1);
+  p2m_t(s, C, CP, g, mat, x0, x1, M0, M1);
 }
 void p2m(hipStream_t s, int C, int CP, const Gemv& g, const float* mat, const double* x0, const double* x1,
          double* M0, double* M1) {
-  if (g.nblk == 0) return;
-  hipLaunchKernelGGL(k_p2m<float>, dim3(g.nblk), dim3(256), 0, s, C, CP, g.row_ptr, g.col_ptr, g.mat_ptr, g.row_out,
-                     g.col_idx, mat, x0, x1, M0, M1);
+  p2m_t(s, C, CP, g, mat, x0, x1, M0, M1);
 }
 
 void translate(hipStream_t s, int C, int RP, int KP, const Tiles& t, const double* ops, const double* S0,

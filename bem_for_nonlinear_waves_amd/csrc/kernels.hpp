@@ -97,6 +97,8 @@ struct Gemv {
   const int32_t* col_idx = nullptr;
   const int32_t* ncol_dof = nullptr;  // near blocks: DOF columns before the L2P columns
   int nblk = 0;
+  // launch shape: near GEMV 0 = 4 rows per wave, 1 = 2, 2 = 8; P2M 0 = 8 loads in flight per thread, 1 = 4, 2 = 16
+  int variant = 0;
 };
 // Operator-grouped translation tiles: tile t applies operator slot op[t] to the pairs [p0[t], p1[t]).
 struct Tiles {

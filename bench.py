@@ -240,7 +240,8 @@ def setup_bvp(m, d, phi, phin, args, precision):
     t_setup = time.perf_counter() - t
     st = bvp.stats()
     _log(f"[{precision}] M2L tile: {st['m2l_tile_pairs']} pairs; operators uploaded {st['n_ops_uploaded']}; "
-         f"FP32 M2L by rotation: {bool(st['m2l_rotation'])}")
+         f"FP32 M2L by rotation: {bool(st['m2l_rotation'])}; near GEMV rows/wave {st['near_rows_per_wave']}, "
+         f"P2M loads in flight {st['p2m_loads']} (FP64, FP32 operator)")
     if st["n_precond_blocks"]:
         _log(f"[{precision}] schwarz blocks={st['n_precond_blocks']} avg={st['precond_avg_block']:.1f} "
              f"fallback={st['precond_fallback_blocks']}")
@@ -718,6 +719,8 @@ def main():
         "refinement_steps": [i["refinement_steps"] for i in infos],
         "residual": [i["residual"] for i in infos],
         "setup_s": t_setup,
+        "launch_shapes": {"m2l_tile_pairs": st["m2l_tile_pairs"], "near_rows_per_wave": st["near_rows_per_wave"],
+                          "p2m_loads": st["p2m_loads"]},
         "reprepare": reprep,
         "fp64_path": fp64_cmp,
         "roofline": roof,

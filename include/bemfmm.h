@@ -123,6 +123,11 @@ typedef struct bemfmm_opts {
                             a midpoint DOF of -1 has its weight redistributed to the edge's end vertices), the
                             midpoint nodes are targets with their own BIE rows (:111-144); preconditioners NONE,
                             JACOBI or SCHWARZ (schwarz_variant 0) */
+  int near_rows_per_wave; /* near-field GEMV launch shape: rows each wavefront streams at once, 2, 4 or 8; 0
+                             (default): chosen by timing the three at the first prepare of a problem class (FP64 and
+                             FP32 operators separately; no counterpart in the reference) */
+  int p2m_loads;          /* P2M launch shape: column loads in flight per thread, 4, 8 or 16; 0 (default): timed
+                             like near_rows_per_wave */
 } bemfmm_opts;
 
 typedef struct bemfmm_result {
@@ -158,7 +163,9 @@ typedef struct bemfmm_stats {
   int tree_on_gpu;              /* 1 if the last prepare built the tree and lists on the GPU */
   double t_tune;                /* seconds: M2L tile autotuning (after t_alpha, the rigid-mode matvec) */
   double t_precond;             /* seconds: preconditioner build */
-  double t_fp32;                /* seconds: FP32 copies of the operator (FP32_REFINE) */
+  double t_fp32;                /* seconds: FP32 copies of the operator (FP32_REFINE) and launch-shape tuning */
+  int near_rows_per_wave[2];    /* near GEMV launch shape in use: FP64 operator, FP32 operator */
+  int p2m_loads[2];             /* P2M launch shape in use: FP64 operator, FP32 operator */
 } bemfmm_stats;
 
 /* Availability (metal_m2l.h: metal_m2l_is_available / metal_m2l_device_name). */
