@@ -967,8 +967,10 @@ __global__ __launch_bounds__(256) void k_m2l_rot(int ntiles, const float* __rest
 
 // FP32 staging: the same reduction with 16-byte loads. Thread t < CS/2 of wave w reads float4 t of a
 // pair's two staged columns (rows 4t..4t+3 of one layer, CS = stage_stride<float>(C)); wave w takes the
-// pairs a + w, a + w + 4, ... (4 in flight per thread) and the four waves' sums are combined in a fixed
-// order (deterministic). Boxes run in descending pair count (order), as in k_m2l_reduce.
+// pairs a + w, a + w + 4, ... (kReduceLoads in flight per thread, summed in pair order) and the four waves'
+// sums are combined in a fixed order (deterministic). Boxes run in descending pair count (order), as in
+// k_m2l_reduce.
+constexpr int kReduceLoads = 8;
 __global__ __launch_bounds__(256) void k_m2l_reduce_f32(int C, const int32_t* __restrict__ order,
                                                         const int64_t* __restrict__ tptr,
                                                         const int32_t* __restrict__ tpairs,
@@ -983,12 +985,13 @@ __global__ __launch_bounds__(256) void k_m2l_reduce_f32(int C, const int32_t* __
   double s0 = 0., s1 = 0., s2 = 0., s3 = 0.;
   if (act) {
     int64_t q = a + w;
-    for (; q + 12 < b; q += 16) {  // 4 loads in flight
-      float4 v[4];
+    for (; q + 4 * (kReduceLoads - 1) < b; q += 4 * kReduceLoads) {
+      float4 v[kReduceLoads];
 #pragma unroll
-      for (int u = 0; u < 4; ++u) v[u] = ld_nt(reinterpret_cast<const float4*>(stage + (int64_t)tpairs[q + 4 * u] * 2 * CS) + t);
+      for (int u = 0; u < kReduceLoads; ++u)
+        v[u] = ld_nt(reinterpret_cast<const float4*>(stage + (int64_t)tpairs[q + 4 * u] * 2 * CS) + t);
 #pragma unroll
-      for (int u = 0; u < 4; ++u) {
+      for (int u = 0; u < kReduceLoads; ++u) {
         s0 += v[u].x;
         s1 += v[u].y;
         s2 += v[u].z;
