@@ -31,7 +31,7 @@ class Opts(ctypes.Structure):
                 ("schwarz_core_k", ctypes.c_int), ("milu_omega", ctypes.c_double), ("ilut_drop_tol", ctypes.c_double),
                 ("ilut_max_entries", ctypes.c_int), ("ilut_pivot_min", ctypes.c_double),
                 ("plane_wave_digits", ctypes.c_int), ("tree_on_gpu", ctypes.c_int), ("element_order", ctypes.c_int),
-                ("near_rows_per_wave", ctypes.c_int), ("p2m_loads", ctypes.c_int)]
+                ("near_rows_per_wave", ctypes.c_int), ("p2m_loads", ctypes.c_int), ("inner_p", ctypes.c_int)]
 
 
 class Result(ctypes.Structure):
@@ -53,7 +53,7 @@ class Stats(ctypes.Structure):
                 ("m2l_rotation", ctypes.c_int), ("precond_nnz", ctypes.c_int64), ("precond_levels", ctypes.c_int),
                 ("m2l_plane_wave", ctypes.c_int), ("tree_on_gpu", ctypes.c_int), ("t_tune", ctypes.c_double),
                 ("t_precond", ctypes.c_double), ("t_fp32", ctypes.c_double),
-                ("near_rows_per_wave", ctypes.c_int * 2), ("p2m_loads", ctypes.c_int * 2)]
+                ("near_rows_per_wave", ctypes.c_int * 2), ("p2m_loads", ctypes.c_int * 2), ("inner_p", ctypes.c_int)]
 
 
 # (name, restype, argtypes) of every entry point declared in include/bemfmm.h

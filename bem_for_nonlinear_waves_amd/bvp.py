@@ -55,7 +55,8 @@ class BEM_BVP:
                  ilu_neighborhood: int = 0, ilu_kring: int = 1, schwarz_variant: int = 0, schwarz_core_k: int = 1,
                  milu_omega: float = 1.0, ilut_drop_tol: float = 1e-3, ilut_max_entries: int = 50,
                  ilut_pivot_min: float = 1e-12, plane_wave_digits: int = 10, tree_on_gpu: int = 1,
-                 element_order: int = 1, near_rows_per_wave: int = 0, p2m_loads: int = 0, device: int = 0):
+                 element_order: int = 1, near_rows_per_wave: int = 0, p2m_loads: int = 0, inner_p: int = 0,
+                 device: int = 0):
         self.lib = _lib.load()
         o = _lib.Opts()
         self.lib.bemfmm_default_opts(ctypes.byref(o))
@@ -71,7 +72,7 @@ class BEM_BVP:
         o.plane_wave_digits = plane_wave_digits
         o.tree_on_gpu = tree_on_gpu
         o.element_order = element_order
-        o.near_rows_per_wave, o.p2m_loads = near_rows_per_wave, p2m_loads
+        o.near_rows_per_wave, o.p2m_loads, o.inner_p = near_rows_per_wave, p2m_loads, inner_p
         self.element_order = element_order
         self.ctx = self.lib.bemfmm_ctx_create(device, ctypes.byref(o))
         if not self.ctx:

@@ -285,9 +285,14 @@ struct bemfmm_ctx {
   int p2m_var[2] = {0, 0};        // P2M launch shape for the FP64 / FP32 operator
   // rotation M2L of the FP32 operator: factors [slot][RS] (slots of the operator table; M2L slots only)
   bool m2l_rot = false;
-  DBuf<float> rot32;
-  std::vector<char> rot_have;
-  int rot_RS = 0;
+  // FP32 rotation factors [operator slot][RS] of the rotation M2L at the plan's order (rot) and at the inner
+  // operator's lower order P_in (rot_in; P_in = 0: the inner operator uses the plan's order)
+  struct RotTable {
+    DBuf<float> f;
+    std::vector<char> have;
+    int RS = 0;
+  } rot, rot_in;
+  int P_in = 0;
   DTiles m2l_rt;  // M2L tiles of kRotTP pairs (one offset each)
   bool m2l_pw_on = false;  // FP32 operator's M2L through the plane-wave composed operators (m2l_kernel 3)
   DTiles m2l_pw;           // its tiles (operator slots of the plane-wave M2L offsets)
@@ -448,13 +453,17 @@ void launch_translate(bemfmm_ctx* c, DTiles& t, const T* ops, const double* S0, 
   dev::translate(s ? s : c->stream, c->C, c->RP, c->KP, t.view(), ops, S0, S1, D0, D1, stage);
 }
 
+// the FP32 operator's M2L by rotation, at the inner order P_in when one is set: the first C_in coefficients of
+// every multipole (degrees <= P_in; the packing is degree-major) are rotated, translated and rotated back
 void launch_m2l_rot(bemfmm_ctx* c, hipStream_t s, float* stage, const double* M0, const double* M1, float* m32) {
   DTiles& t = c->m2l_rt;
   if (t.ntiles == 0) return;
-  if (c->rot_RS != dev::rot_size(c->plan->P)) throw std::runtime_error("rotation M2L: factor table stride mismatch");
-  const int64_t nm = (int64_t)c->n_boxes * 2 * c->C;
-  dev::pack_m32(s, nm, c->C, M0, M1, m32);
-  dev::m2l_rot(s, c->plan->P, t.view(), c->rot32.p, m32, stage);
+  const int P = c->P_in ? c->P_in : c->plan->P;
+  const bemfmm_ctx::RotTable& R = c->P_in ? c->rot_in : c->rot;
+  if (R.RS != dev::rot_size(P)) throw std::runtime_error("rotation M2L: factor table stride mismatch");
+  const int Cm = (P + 1) * (P + 1);
+  dev::pack_m32(s, (int64_t)c->n_boxes * 2 * Cm, Cm, c->C, M0, M1, m32);
+  dev::m2l_rot(s, P, t.view(), R.f.p, m32, stage);
 }
 
 void fill(bemfmm_ctx* c, double* p, int64_t n, double v, hipStream_t s = nullptr) {
@@ -544,7 +553,10 @@ void evaluate_launch(bemfmm_ctx* c, int mode, const double* in, double* out, boo
     if (c->m2l_rot) launch_m2l_rot(c, st, stf, w.M0, w.M1, w.m32);
     else launch_translate(c, c->m2l_pw_on ? c->m2l_pw : c->m2l, c->ops32.p, w.M0, w.M1, nullptr, nullptr, stf, st);
     rec(4);
-    dev::m2l_reduce(st, c->C, c->n_boxes, c->m2l_rorder.p, c->m2l_tptr.p, c->m2l_tpairs.p, (const float*)stf, L0, L1);
+    // rows of the locals above the inner operator's order are zero (L2L and L2P then use its order throughout)
+    const int Cm = (c->m2l_rot && c->P_in) ? (c->P_in + 1) * (c->P_in + 1) : c->C;
+    dev::m2l_reduce(st, Cm, c->C, c->n_boxes, c->m2l_rorder.p, c->m2l_tptr.p, c->m2l_tpairs.p, (const float*)stf, L0,
+                    L1);
   } else {
     double* std_ = reinterpret_cast<double*>(w.stage);
     launch_translate(c, c->m2l, c->ops.p, w.M0, w.M1, nullptr, nullptr, std_, st);
@@ -803,29 +815,30 @@ std::vector<int32_t> update_operator_table(bemfmm_ctx* c, const Plan& pl, int pw
   return slot;
 }
 
-// Rotation factors of the plan's M2L offsets in the FP32 table rot32 [slot][RS], indexed by the operator
-// table's slots; only slots without factors yet are computed (process-wide cache) and uploaded.
-void update_rotation_table(bemfmm_ctx* c, const Plan& pl, const std::vector<int32_t>& slot) {
+// Rotation factors of order P of the plan's M2L offsets in the FP32 table R [slot][RS], indexed by the
+// operator table's slots; only slots without factors yet are computed (process-wide cache) and uploaded.
+void update_rotation_table(bemfmm_ctx* c, const Plan& pl, const std::vector<int32_t>& slot, bemfmm_ctx::RotTable& R,
+                           int P) {
   hipStream_t s = c->stream;
-  const int RS = dev::rot_size(pl.P);  // padded device layout
-  if (c->rot_RS != RS) {
-    c->rot32.release();
-    c->rot_have.clear();
-    c->rot_RS = RS;
+  const int RS = dev::rot_size(P);  // padded device layout
+  if (R.RS != RS) {
+    R.f.release();
+    R.have.clear();
+    R.RS = RS;
   }
-  if ((int64_t)c->rot_have.size() < c->ops_cap) {
+  if ((int64_t)R.have.size() < c->ops_cap) {
     DBuf<float> t;
     t.alloc((size_t)c->ops_cap * RS);
-    if (!c->rot_have.empty())
-      HIPCHECK(hipMemcpyAsync(t.p, c->rot32.p, c->rot_have.size() * RS * sizeof(float), hipMemcpyDeviceToDevice, s));
+    if (!R.have.empty())
+      HIPCHECK(hipMemcpyAsync(t.p, R.f.p, R.have.size() * RS * sizeof(float), hipMemcpyDeviceToDevice, s));
     HIPCHECK(hipStreamSynchronize(s));
-    c->rot32 = std::move(t);
-    c->rot_have.resize(c->ops_cap, 0);
+    R.f = std::move(t);
+    R.have.resize(c->ops_cap, 0);
   }
   std::vector<int> fresh;
   for (int op = 16; op < pl.n_ops; ++op)
-    if (!c->rot_have[slot[op]]) {
-      c->rot_have[slot[op]] = 1;
+    if (!R.have[slot[op]]) {
+      R.have[slot[op]] = 1;
       fresh.push_back(op);
     }
   if (fresh.empty()) return;
@@ -835,7 +848,7 @@ void update_rotation_table(bemfmm_ctx* c, const Plan& pl, const std::vector<int3
       const auto k = bemfmm::op_key(pl, op);
       keys.push_back({3, k[1], k[2], k[3]});
     }
-    bemfmm::warm_operator_cache(pl.P, keys);  // factors of new offsets, computed in parallel once per process
+    bemfmm::warm_operator_cache(P, keys);  // factors of new offsets, computed in parallel once per process
   }
   const size_t cap = c->op_stage_n * 2 / RS;  // rotations per pinned staging batch (the staging holds doubles)
   if (cap < 1) {
@@ -851,14 +864,14 @@ void update_rotation_table(bemfmm_ctx* c, const Plan& pl, const std::vector<int3
     // consecutive slots are uploaded together: runs of fresh ops with consecutive slots
     for (size_t i = 0; i < nb; ++i) {
       const auto key = bemfmm::op_key(pl, fresh[b0 + i]);
-      const auto v = bemfmm::cached_operator(pl.P, 3, key[1], key[2], key[3]);
-      dev::rot_pad(pl.P, v->data(), stg + i * RS);
+      const auto v = bemfmm::cached_operator(P, 3, key[1], key[2], key[3]);
+      dev::rot_pad(P, v->data(), stg + i * RS);
     }
     size_t i = 0;
     while (i < nb) {
       size_t j = i + 1;
       while (j < nb && slot[fresh[b0 + j]] == slot[fresh[b0 + j - 1]] + 1) ++j;
-      HIPCHECK(hipMemcpyAsync(c->rot32.p + (size_t)slot[fresh[b0 + i]] * RS, stg + i * RS, (j - i) * RS * sizeof(float),
+      HIPCHECK(hipMemcpyAsync(R.f.p + (size_t)slot[fresh[b0 + i]] * RS, stg + i * RS, (j - i) * RS * sizeof(float),
                               hipMemcpyHostToDevice, s));
       i = j;
     }
@@ -1421,6 +1434,11 @@ void do_prepare(bemfmm_ctx* c) {
   if (c->opts.m2l_tile_pairs != 0 && c->opts.m2l_tile_pairs != 16 && c->opts.m2l_tile_pairs != 32 &&
       c->opts.m2l_tile_pairs != 64)
     throw std::runtime_error("bemfmm_prepare: m2l_tile_pairs must be 0 (default), 16, 32 or 64");
+  if (c->opts.inner_p != 0 &&
+      (c->opts.precision != BEMFMM_PRECISION_FP32_REFINE || c->opts.inner_p >= c->opts.p ||
+       !dev::m2l_rot_inner_compiled(c->opts.inner_p) || (c->opts.m2l_kernel != 0 && c->opts.m2l_kernel != 2)))
+    throw std::runtime_error("bemfmm_prepare: inner_p must be 0 (the plan's order) or an order below p that the "
+                             "rotation M2L runs (6 or 8), with FP32_REFINE and m2l_kernel 0 or 2");
   if (c->opts.near_rows_per_wave != 0 && c->opts.near_rows_per_wave != 2 && c->opts.near_rows_per_wave != 4 &&
       c->opts.near_rows_per_wave != 8)
     throw std::runtime_error("bemfmm_prepare: near_rows_per_wave must be 0 (tuned), 2, 4 or 8");
@@ -1920,7 +1938,8 @@ void do_prepare(bemfmm_ctx* c) {
     dev::to_f32(s, (int64_t)pl.near.mat_size, c->near.mat.p, c->near_mat32.p);
     c->p2m_mat32.alloc(c->p2m.mat.n);
     dev::to_f32(s, (int64_t)c->p2m.mat.n, c->p2m.mat.p, c->p2m_mat32.p);
-    c->m2l_rot = (c->opts.m2l_kernel == 0 || c->opts.m2l_kernel == 2) && dev::m2l_rot_compiled(pl.P);
+    c->P_in = c->opts.inner_p;  // validated above: below the plan's order, with the rotation M2L
+    c->m2l_rot = (c->opts.m2l_kernel == 0 || c->opts.m2l_kernel == 2) && (c->P_in || dev::m2l_rot_compiled(pl.P));
     c->m2l_pw_on = c->opts.m2l_kernel == 3;
     if (c->m2l_pw_on) {
       // the FP32 operator's M2L through plane waves: extra table slots for the composed operators, own tiles
@@ -1928,7 +1947,8 @@ void do_prepare(bemfmm_ctx* c) {
       c->m2l_pw.upload(pl.m2l, s, slot_pw, c->m2l_tp);
     }
     if (c->m2l_rot) {
-      update_rotation_table(c, pl, c->m2l_slot);
+      if (c->P_in) update_rotation_table(c, pl, c->m2l_slot, c->rot_in, c->P_in);
+      else update_rotation_table(c, pl, c->m2l_slot, c->rot, pl.P);
       c->m2l_rt.upload(pl.m2l, s, c->m2l_slot, kRotTP);
       c->m32.alloc((size_t)pl.n_boxes * 2 * pl.C);
       c->ops32.release();  // M2M and L2L use the FP64 table
@@ -1954,6 +1974,7 @@ void do_prepare(bemfmm_ctx* c) {
     c->ops32_valid = 0;
     c->m2l_rot = false;
     c->m2l_pw_on = false;
+    c->P_in = 0;
   }
   tune_launch_shapes(c, pl);
   HIPCHECK(hipStreamSynchronize(s));
@@ -1997,6 +2018,7 @@ void do_prepare(bemfmm_ctx* c) {
   st.n_ops_uploaded = c->ops_uploaded;
   st.m2l_tile_pairs = c->m2l_tp;
   st.m2l_rotation = c->m2l_rot ? 1 : 0;
+  st.inner_p = c->P_in;
   st.m2l_plane_wave = c->opts.m2l_kernel == 4 ? 2 : c->m2l_pw_on ? 1 : 0;
   const bool ilu_kind = pc == BEMFMM_PRECOND_ILU0 || pc == BEMFMM_PRECOND_MILU0 || pc == BEMFMM_PRECOND_ILUT;
   st.precond_nnz = (ilu_kind || (pc == BEMFMM_PRECOND_SCHWARZ && c->opts.schwarz_variant == 1)) ? c->ilu_nnz : 0;

@@ -882,12 +882,13 @@ __global__ __launch_bounds__(256) void k_p2m_build(int q, int CP, const int64_t*
   }
 }
 
-// FP32 multipoles for the rotation M2L: M32[(box*2 + layer)*C + r] = M_layer[box*C + r]
-__global__ void k_pack_m32(int64_t n, int C, const double* __restrict__ M0, const double* __restrict__ M1,
+// FP32 multipoles for the rotation M2L: M32[(box*2 + layer)*C + r] = M_layer[box*Cs + r], r < C: the first C
+// coefficients (degrees <= P of a P' >= P expansion stored with stride Cs: the packing is degree-major)
+__global__ void k_pack_m32(int64_t n, int C, int Cs, const double* __restrict__ M0, const double* __restrict__ M1,
                            float* __restrict__ M32) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
     const int64_t col = i / C, r = i - col * C;
-    M32[i] = (float)((col & 1) ? M1 : M0)[(col >> 1) * C + r];
+    M32[i] = (float)((col & 1) ? M1 : M0)[(col >> 1) * Cs + r];
   }
 }
 
@@ -971,7 +972,8 @@ __global__ __launch_bounds__(256) void k_m2l_rot(int ntiles, const float* __rest
 // sums are combined in a fixed order (deterministic). Boxes run in descending pair count (order), as in
 // k_m2l_reduce.
 constexpr int kReduceLoads = 8;
-__global__ __launch_bounds__(256) void k_m2l_reduce_f32(int C, const int32_t* __restrict__ order,
+// Cout >= C: the locals are stored with stride Cout; rows [C, Cout) (degrees above the M2L's order) are zeroed.
+__global__ __launch_bounds__(256) void k_m2l_reduce_f32(int C, int Cout, const int32_t* __restrict__ order,
                                                         const int64_t* __restrict__ tptr,
                                                         const int32_t* __restrict__ tpairs,
                                                         const float* __restrict__ stage, double* __restrict__ L0,
@@ -1012,6 +1014,10 @@ __global__ __launch_bounds__(256) void k_m2l_reduce_f32(int C, const int32_t* __
     part[w - 1][2][t] = s2;
     part[w - 1][3][t] = s3;
   }
+  for (int r = C + threadIdx.x; r < Cout; r += blockDim.x) {
+    L0[(int64_t)tb * Cout + r] = 0.;
+    L1[(int64_t)tb * Cout + r] = 0.;
+  }
   __syncthreads();
   if (w == 0 && act) {
 #pragma unroll
@@ -1023,7 +1029,7 @@ __global__ __launch_bounds__(256) void k_m2l_reduce_f32(int C, const int32_t* __
     }
     const int f = 4 * t;  // a float4 never straddles the two layers (CS is a multiple of 4)
     const int layer = f >= CS ? 1 : 0, r = f - layer * CS;
-    double* L = (layer ? L1 : L0) + (int64_t)tb * C;
+    double* L = (layer ? L1 : L0) + (int64_t)tb * Cout;
     if (r < C) L[r] = s0;
     if (r + 1 < C) L[r + 1] = s1;
     if (r + 2 < C) L[r + 2] = s2;
@@ -1728,16 +1734,20 @@ void translate(hipStream_t s, int C, int RP, int KP, const Tiles& t, const float
   launch_translate<float>(s, C, RP, KP, t, ops, S0, S1, D0, D1, stage);
 }
 
-void pack_m32(hipStream_t s, int64_t n, int C, const double* M0, const double* M1, float* M32) {
-  hipLaunchKernelGGL(k_pack_m32, dim3(grid1(n)), dim3(256), 0, s, n, C, M0, M1, M32);
+void pack_m32(hipStream_t s, int64_t n, int C, int Cs, const double* M0, const double* M1, float* M32) {
+  hipLaunchKernelGGL(k_pack_m32, dim3(grid1(n)), dim3(256), 0, s, n, C, Cs, M0, M1, M32);
 }
 
 bool m2l_rot_compiled(int P) { return P == 8 || P == 10; }
+bool m2l_rot_inner_compiled(int P) { return P == 6 || P == 8 || P == 10; }
 
 void m2l_rot(hipStream_t s, int P, const Tiles& t, const float* rot, const float* M32, float* stage) {
   if (t.ntiles == 0) return;
   const int grid = ((t.ntiles + kXCD - 1) / kXCD) * kXCD;
   switch (P) {
+  case 6:
+    hipLaunchKernelGGL(k_m2l_rot<6>, dim3(grid), dim3(256), 0, s, t.ntiles, rot, t.op, t.p0, t.p1, t.src, M32, stage);
+    break;
   case 8:
     hipLaunchKernelGGL(k_m2l_rot<8>, dim3(grid), dim3(256), 0, s, t.ntiles, rot, t.op, t.p0, t.p1, t.src, M32, stage);
     break;
@@ -1753,10 +1763,11 @@ void m2l_reduce(hipStream_t s, int C, int nboxes, const int32_t* order, const in
   if (nboxes == 0) return;
   hipLaunchKernelGGL(k_m2l_reduce<double>, dim3(nboxes), dim3(256), 0, s, C, order, tptr, tpairs, stage, L0, L1);
 }
-void m2l_reduce(hipStream_t s, int C, int nboxes, const int32_t* order, const int64_t* tptr, const int32_t* tpairs,
-                const float* stage, double* L0, double* L1) {
+void m2l_reduce(hipStream_t s, int C, int Cout, int nboxes, const int32_t* order, const int64_t* tptr,
+                const int32_t* tpairs, const float* stage, double* L0, double* L1) {
   if (nboxes == 0) return;
-  hipLaunchKernelGGL(k_m2l_reduce_f32, dim3(nboxes), dim3(256), 0, s, C, order, tptr, tpairs, stage, L0, L1);
+  if (Cout < C || Cout > 256) throw std::runtime_error("m2l_reduce: local stride out of range");
+  hipLaunchKernelGGL(k_m2l_reduce_f32, dim3(nboxes), dim3(256), 0, s, C, Cout, order, tptr, tpairs, stage, L0, L1);
 }
 
 void assemble(hipStream_t s, int mode, int n, const int32_t* kind, const int32_t* point, const double* alpha,

@@ -196,14 +196,19 @@ void translate(hipStream_t s, int C, int RP, int KP, const Tiles& t, const doubl
                const double* S1, double* D0, double* D1, double* stage);
 void translate(hipStream_t s, int C, int RP, int KP, const Tiles& t, const float* ops, const double* S0,
                const double* S1, double* D0, double* D1, float* stage);
-void pack_m32(hipStream_t s, int64_t n, int C, const double* M0, const double* M1, float* M32);
+// the first C coefficients of every box's two multipoles (stored with stride Cs >= C) as FP32 columns of C
+void pack_m32(hipStream_t s, int64_t n, int C, int Cs, const double* M0, const double* M1, float* M32);
+// expansion orders the rotation M2L runs the FP32 operator with (m2l_rot_compiled), and also its inner
+// operator at a lower order than the plan's (m2l_rot_inner_compiled)
 bool m2l_rot_compiled(int P);
+bool m2l_rot_inner_compiled(int P);
 // rotation M2L of the FP32 operator (throws for an expansion order it is not compiled for)
 void m2l_rot(hipStream_t s, int P, const Tiles& t, const float* rot, const float* M32, float* stage);
 void m2l_reduce(hipStream_t s, int C, int nboxes, const int32_t* order, const int64_t* tptr, const int32_t* tpairs,
                 const double* stage, double* L0, double* L1);
-void m2l_reduce(hipStream_t s, int C, int nboxes, const int32_t* order, const int64_t* tptr, const int32_t* tpairs,
-                const float* stage, double* L0, double* L1);
+// FP32 staging of C rows per column; the locals are stored with stride Cout >= C, rows [C, Cout) set to zero
+void m2l_reduce(hipStream_t s, int C, int Cout, int nboxes, const int32_t* order, const int64_t* tptr,
+                const int32_t* tpairs, const float* stage, double* L0, double* L1);
 void assemble(hipStream_t s, int mode, int n, const int32_t* kind, const int32_t* point, const double* alpha,
               const double* phi, const double* y0, const double* y1, const double* phi_k, double* out);
 

@@ -55,12 +55,13 @@ class BVPServer:
     def submit(self, mesh: Mesh, dofs: DofModel, phi, phin, session: int = -1, x0=None, tol: float = 1e-10,
                max_iter: int = 1000, restart: int = 1000, p: int = 10, theta: float = 0.25, leaf_max: int = 50,
                max_level: int = 7, q: int = 6, precision: int = _lib.PRECISION_FP64,
-               preconditioner: int = _lib.PRECOND_NONE, refine_eta: float = 3e-7, use_graphs: int = 0) -> int:
+               preconditioner: int = _lib.PRECOND_NONE, refine_eta: float = 3e-7, use_graphs: int = 0,
+               inner_p: int = 0) -> int:
         o = _lib.Opts()
         self.lib.bemfmm_default_opts(ctypes.byref(o))
         o.p, o.theta, o.leaf_max, o.max_level, o.q = p, theta, leaf_max, max_level, q
         o.precision, o.preconditioner, o.refine_eta = precision, preconditioner, refine_eta
-        o.use_graphs = use_graphs
+        o.use_graphs, o.inner_p = use_graphs, inner_p
         keep = [np.ascontiguousarray(mesh.X, np.float64), np.ascontiguousarray(mesh.tri, np.int32),
                 np.ascontiguousarray(dofs.face_dof, np.int32), np.ascontiguousarray(dofs.dof_kind, np.int32),
                 np.ascontiguousarray(dofs.dof_point, np.int32), np.ascontiguousarray(phi, np.float64),

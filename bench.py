@@ -233,7 +233,8 @@ def setup_bvp(m, d, phi, phin, args, precision):
                   m2l_kernel={"auto": 0, "dense": 1, "rotation": 2, "plane-wave": 3, "plane-wave-fp64": 4}[args.m2l_kernel],
                   plane_wave_digits=args.plane_wave_digits,
                   schwarz_max_core=args.schwarz_core, schwarz_max_block=args.schwarz_block,
-                  schwarz_overlap=args.schwarz_overlap, use_graphs=args.graphs, refine_recycle=args.refine_recycle)
+                  schwarz_overlap=args.schwarz_overlap, use_graphs=args.graphs, refine_recycle=args.refine_recycle,
+                  inner_p=_inner_p(args) if precision == "fp32_refine" else 0)
     bvp.set_bc(phi, phin)
     t = time.perf_counter()
     bvp.prepare()
@@ -364,13 +365,21 @@ def make_batch_problem(rng, n_target: int):
     return m, d, phi, phin, h, seed, (Lx, Ly)
 
 
+def _inner_p(args) -> int:
+    """Expansion order of the FP32 operator's M2L: --inner-p when it applies (FP32 + refinement, the rotation
+    M2L, below --p), else 0 (the plan's order)."""
+    ok = args.precision == "fp32_refine" and args.m2l_kernel in ("auto", "rotation") and 0 < args.inner_p < args.p
+    return args.inner_p if ok else 0
+
+
 def _solver_kw(args):
     from bem_for_nonlinear_waves_amd import _lib
     return dict(p=args.p, leaf_max=args.leaf, max_level=args.max_level, tol=args.tol, max_iter=args.max_iter,
                 restart=args.max_iter, refine_eta=args.refine_eta,
                 precision=_lib.PRECISION_FP32_REFINE if args.precision == "fp32_refine" else _lib.PRECISION_FP64,
                 preconditioner={"none": _lib.PRECOND_NONE, "jacobi": _lib.PRECOND_JACOBI,
-                                "schwarz": _lib.PRECOND_SCHWARZ}[args.precond], use_graphs=args.graphs)
+                                "schwarz": _lib.PRECOND_SCHWARZ}[args.precond], use_graphs=args.graphs,
+                inner_p=_inner_p(args))
 
 
 def run_batch(args):
@@ -421,7 +430,8 @@ def run_batch(args):
         "config": {"workload": f"batch of {len(probs)} tank BVPs, N log-uniform in [2k, 40k], aspect 1-8, "
                                f"1/4 with a cylinder, setup included in every solve",
                    "slots": args.slots, "p": args.p, "leaf_max": args.leaf, "tol": args.tol,
-                   "precision": args.precision, "preconditioner": args.precond, "total_nodes": int(sum(n_nodes))},
+                   "precision": args.precision, "preconditioner": args.precond, "total_nodes": int(sum(n_nodes)),
+                   "inner_p": _inner_p(args)},
         "p50_solve_ms": st["p50_latency_s"] * 1e3, "p99_solve_ms": st["p99_latency_s"] * 1e3,
         "p50_service_ms": float(np.percentile(svc, 50)), "p99_service_ms": float(np.percentile(svc, 99)),
         "converged": conv, "peak_device_gb": st["peak_bytes"] / 1e9, "one_slot": seq,
@@ -493,7 +503,7 @@ def run_serve(args):
                                f"N uniform in [{args.serve_min_n}, {args.serve_max_n}], {args.stages} stages each "
                                f"(one BVP per stage, free-surface nodes moved between stages, setup included)",
                    "slots": args.slots, "p": args.p, "leaf_max": args.leaf, "tol": args.tol,
-                   "precision": args.precision, "preconditioner": args.precond},
+                   "precision": args.precision, "preconditioner": args.precond, "inner_p": _inner_p(args)},
         "p50_solve_ms": float(np.percentile(lat_ms, 50)), "p99_solve_ms": float(np.percentile(lat_ms, 99)),
         "p50_service_ms": float(np.percentile(svc_ms, 50)), "p99_service_ms": float(np.percentile(svc_ms, 99)),
         "converged": int(sum(conv)), "peak_device_gb": st["peak_bytes"] / 1e9,
@@ -516,6 +526,8 @@ def main():
     ap.add_argument("--precond", choices=["none", "jacobi", "schwarz"], default="schwarz")
     ap.add_argument("--precision", choices=["fp32_refine", "fp64"], default="fp32_refine")
     ap.add_argument("--refine-eta", type=float, default=3e-7)
+    ap.add_argument("--inner-p", type=int, default=6, choices=[0, 6, 8],
+                    help="fp32_refine: expansion order of the FP32 operator's M2L (0: --p); the FP64 residuals keep --p")
     ap.add_argument("--refine-recycle", type=int, default=1, choices=[0, 1],
                     help="refinement steps after the first keep the earlier steps' Krylov directions (library option)")
     ap.add_argument("--m2l-tile-pairs", type=int, default=0, choices=[0, 16, 32, 64])
@@ -630,7 +642,9 @@ def main():
     # algorithmic multiply-adds per pair and layer: dense C x C operator, or the rotation factors
     # (D = Y Z and H^T = Z^T Y^T: 4 per order m >= 1 of each degree for Z, (n+1)^2 + n^2 per degree for Y;
     # axial: (p+1)^2 for m = 0 and 2 (p+1-m)^2 for m > 0)
-    pp = args.p
+    # the FP32 operator's M2L runs at the inner order when one is set (its source gathers are C_m coefficients)
+    pp = (st.get("inner_p", 0) or args.p) if rot else args.p
+    C_m = (pp + 1) ** 2
     rot_macs = (2 * sum(4 * k + (k + 1) ** 2 + k ** 2 for k in range(pp + 1)) + (pp + 1) ** 2
                 + 2 * sum((pp + 1 - k) ** 2 for k in range(1, pp + 1)))
     m2l_flops = st["n_m2l_pairs"] * 2 * (rot_macs if rot else C * C) * 2.0
@@ -645,7 +659,7 @@ def main():
             # (157.3 TFLOP/s / 8 TB/s = 19.7 flop/B): the rotation M2L sits on the bandwidth side of the roofline.
             # Its algorithmic bytes are the gathers of the source multipoles, each pair's two FP32 columns
             # (2 C x 4 B); the per-pair staging it writes for the reduction is not algorithmic work.
-            rot_bytes = st["n_m2l_pairs"] * 2 * C * 4.0
+            rot_bytes = st["n_m2l_pairs"] * 2 * C_m * 4.0
             achb = rot_bytes / (stage[dom] * 1e-3) / 1e9
             roof = {"kernel": "k_m2l_rot (M2L by rotation, FP32 FMA)", "bound": "hbm", "achieved": achb,
                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achb / HBM_PEAK_GBS, "traffic": None,
@@ -659,16 +673,17 @@ def main():
                 "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "traffic": None, "algorithmic_bytes": near_bytes}
     # the M2L stage as a whole (translation + per-box reduction): algorithmic bytes = source gathers + each
     # target's locals written once
-    m2l_alg = st["n_m2l_pairs"] * 2 * C * esz + m2l_targets * 2 * C * 8.0
+    m2l_alg = st["n_m2l_pairs"] * 2 * C_m * esz + m2l_targets * 2 * C * 8.0
     roof["m2l_stage"] = {"ms": float(stage[3] + stage[4]), "algorithmic_bytes": m2l_alg,
                          "achieved_gbs": m2l_alg / ((stage[3] + stage[4]) * 1e-3) / 1e9}
     # HBM bytes per launch from the rocprofv3 counter passes of this device code (profiles/), if they exist
     if rot and STAGES[dom] == "m2l":
-        tkey = f"k_m2l_rot<{args.p}>"
+        tkey = f"k_m2l_rot<{pp}>"
     elif STAGES[dom] == "m2l":
         tkey = f"k_translate_mfma<double,{(C + 63) // 64},true,{st['m2l_tile_pairs']}>"
     else:
-        tkey = "k_block_gemv<float>" if fp32 else "k_block_gemv<double>"
+        tkey = (f"k_block_gemv<float,{st['near_rows_per_wave'][1]}>" if fp32
+                else f"k_block_gemv<double,{st['near_rows_per_wave'][0]}>")
     traffic = measured_traffic(tkey, workload_key(args, fp32))
     if traffic is not None:
         roof["traffic"] = traffic["bytes"]
@@ -676,7 +691,7 @@ def main():
     roof["stage_ms"] = dict(zip(STAGES, [round(float(v), 4) for v in stage]))
     roof["m2l_tflops"] = m2l_flops / (stage[3] * 1e-3) / 1e12
     if rot:  # the same M2L as dense operator multiply-adds per second
-        roof["m2l_dense_equivalent_tflops"] = st["n_m2l_pairs"] * 2 * C * C * 2.0 / (stage[3] * 1e-3) / 1e12
+        roof["m2l_dense_equivalent_tflops"] = st["n_m2l_pairs"] * 2 * C_m * C_m * 2.0 / (stage[3] * 1e-3) / 1e12
     roof["near_gbs"] = near_bytes / (stage[6] * 1e-3) / 1e9
     roof["fp64_stage_ms"] = dict(zip(STAGES, [round(float(v), 4) for v in stage64]))
 
@@ -709,7 +724,8 @@ def main():
                    "n_nodes": m.n_nodes, "n_faces": m.n_faces, "n_dof": n,
                    "p": args.p, "theta": 0.25, "leaf_max": args.leaf, "max_level": args.max_level, "tol": args.tol,
                    "precision": args.precision, "preconditioner": args.precond,
-                   "schwarz": [args.schwarz_core, args.schwarz_block, args.schwarz_overlap]},
+                   "schwarz": [args.schwarz_core, args.schwarz_block, args.schwarz_overlap],
+                   "inner_p": _inner_p(args)},
         "matvec_mnode_per_s": m.n_nodes / t_mv / 1e6,
         "matvec_ms": t_mv * 1e3,
         "matvec_fp64_ms": t_mv64 * 1e3,

@@ -128,6 +128,12 @@ typedef struct bemfmm_opts {
                              FP32 operators separately; no counterpart in the reference) */
   int p2m_loads;          /* P2M launch shape: column loads in flight per thread, 4, 8 or 16; 0 (default): timed
                              like near_rows_per_wave */
+  int inner_p;            /* FP32_REFINE with the rotation M2L: expansion order of the FP32 operator's M2L, below p
+                             (6 or 8); 0 (default): p. The inner solves then run on the expansions truncated at
+                             inner_p (P2M, M2M, L2L and L2P keep p, whose leading terms are the inner_p ones); the
+                             FP64 residual of every refinement step keeps p, so the result is that of the p operator
+                             (the reference switches its single-precision M2L to double near convergence instead,
+                             BEM_solveBVP_GMRES_FMM.hpp:2306-2319) */
 } bemfmm_opts;
 
 typedef struct bemfmm_result {
@@ -166,6 +172,7 @@ typedef struct bemfmm_stats {
   double t_fp32;                /* seconds: FP32 copies of the operator (FP32_REFINE) and launch-shape tuning */
   int near_rows_per_wave[2];    /* near GEMV launch shape in use: FP64 operator, FP32 operator */
   int p2m_loads[2];             /* P2M launch shape in use: FP64 operator, FP32 operator */
+  int inner_p;                  /* expansion order of the FP32 operator's M2L when below p, else 0 */
 } bemfmm_stats;
 
 /* Availability (metal_m2l.h: metal_m2l_is_available / metal_m2l_device_name). */

@@ -273,17 +273,18 @@ void translate(hipStream_t s, int C, int RP, int KP, const Tiles& t, const float
   const Tiles tt = t;
   hipstub::enqueue(s, [=]() { translate_cpu<float>(C, RP, KP, tt, ops, S0, S1, D0, D1, stage); });
 }
-void pack_m32(hipStream_t s, int64_t n, int C, const double* M0, const double* M1, float* M32) {
+void pack_m32(hipStream_t s, int64_t n, int C, int Cs, const double* M0, const double* M1, float* M32) {
   hipstub::enqueue(s, [=]() {
     for (int64_t i = 0; i < n; ++i) {
       const int64_t col = i / C, r = i - col * C;
-      M32[i] = (float)((col & 1) ? M1 : M0)[(col >> 1) * C + r];
+      M32[i] = (float)((col & 1) ? M1 : M0)[(col >> 1) * Cs + r];
     }
   });
 }
 bool m2l_rot_compiled(int P) { return P == 8 || P == 10; }
+bool m2l_rot_inner_compiled(int P) { return P == 6 || P == 8 || P == 10; }
 void m2l_rot(hipStream_t s, int P, const Tiles& t, const float* rot, const float* M32, float* stage) {
-  if (!m2l_rot_compiled(P)) throw std::runtime_error("rotation M2L not compiled for this expansion order");
+  if (!m2l_rot_inner_compiled(P)) throw std::runtime_error("rotation M2L not compiled for this expansion order");
   const Tiles tt = t;
   hipstub::enqueue(s, [=]() {
     const int C = (P + 1) * (P + 1), CS = stage_stride<float>(C), RS = rot_size(P);
@@ -300,8 +301,8 @@ void m2l_rot(hipStream_t s, int P, const Tiles& t, const float* rot, const float
   });
 }
 template <typename T>
-void reduce_cpu(int C, int nboxes, const int32_t* order, const int64_t* tptr, const int32_t* tpairs, const T* stage,
-                double* L0, double* L1) {
+void reduce_cpu(int C, int Cout, int nboxes, const int32_t* order, const int64_t* tptr, const int32_t* tpairs,
+                const T* stage, double* L0, double* L1) {
   const int CS = stage_stride<T>(C);
   for (int k = 0; k < nboxes; ++k) {
     const int t = order[k];
@@ -309,18 +310,20 @@ void reduce_cpu(int C, int nboxes, const int32_t* order, const int64_t* tptr, co
       const int jj = j < C ? j : CS + j - C;
       double s = 0.;
       for (int64_t q = tptr[t]; q < tptr[t + 1]; ++q) s += (double)stage[(int64_t)tpairs[q] * 2 * CS + jj];
-      if (j < C) L0[(int64_t)t * C + j] = s;
-      else L1[(int64_t)t * C + j - C] = s;
+      if (j < C) L0[(int64_t)t * Cout + j] = s;
+      else L1[(int64_t)t * Cout + j - C] = s;
     }
+    for (int r = C; r < Cout; ++r) L0[(int64_t)t * Cout + r] = L1[(int64_t)t * Cout + r] = 0.;
   }
 }
 void m2l_reduce(hipStream_t s, int C, int nboxes, const int32_t* order, const int64_t* tptr, const int32_t* tpairs,
                 const double* stage, double* L0, double* L1) {
-  hipstub::enqueue(s, [=]() { reduce_cpu<double>(C, nboxes, order, tptr, tpairs, stage, L0, L1); });
+  hipstub::enqueue(s, [=]() { reduce_cpu<double>(C, C, nboxes, order, tptr, tpairs, stage, L0, L1); });
 }
-void m2l_reduce(hipStream_t s, int C, int nboxes, const int32_t* order, const int64_t* tptr, const int32_t* tpairs,
-                const float* stage, double* L0, double* L1) {
-  hipstub::enqueue(s, [=]() { reduce_cpu<float>(C, nboxes, order, tptr, tpairs, stage, L0, L1); });
+void m2l_reduce(hipStream_t s, int C, int Cout, int nboxes, const int32_t* order, const int64_t* tptr,
+                const int32_t* tpairs, const float* stage, double* L0, double* L1) {
+  if (Cout < C) throw std::runtime_error("m2l_reduce: local stride out of range");
+  hipstub::enqueue(s, [=]() { reduce_cpu<float>(C, Cout, nboxes, order, tptr, tpairs, stage, L0, L1); });
 }
 void assemble(hipStream_t s, int mode, int n, const int32_t* kind, const int32_t* point, const double* alpha,
               const double* phi, const double* y0, const double* y1, const double* phi_k, double* out) {
