@@ -280,8 +280,10 @@ def time_reprepare(bvp, m, d, args, stages=3, seed=11):
     return out
 
 
-def time_matvec(bvp, n, n_nodes, reps, fp32):
-    """Matvec on device-resident vectors: wall time per matvec and per-stage HIP-event times (ms)."""
+def time_matvec(bvp, n, n_nodes, reps, fp32, whole_only=False):
+    """Matvec on device-resident vectors: wall time per matvec and per-stage HIP-event times (ms). whole_only: only
+    the stage-timed configuration (every kernel alone on the stream, the near-field GEMV as one launch), for the
+    profiler's per-launch figures."""
     from bem_for_nonlinear_waves_amd import devbuf
     mv = bvp.matvec_fp32_device if fp32 else bvp.matvec_device
     xd = devbuf.DeviceArray.from_host(np.random.default_rng(0).standard_normal(n))
@@ -302,6 +304,10 @@ def time_matvec(bvp, n, n_nodes, reps, fp32):
         stage += bvp.profile()  # synchronises on the last event of this matvec
     stage /= 5
     bvp.profile_off()  # the configuration the solves run in (no event recording)
+    if whole_only:
+        xd.free()
+        yd.free()
+        return t_mv, stage
     for _ in range(2):
         mv(xs.address, ys.address)
     bvp.sync()
@@ -562,7 +568,8 @@ def main():
     ap.add_argument("--serve-min-n", type=int, default=5000)
     ap.add_argument("--serve-max-n", type=int, default=200000)
     ap.add_argument("--profile-only", action="store_true",
-                    help="setup + a 256 MiB device copy (counter calibration) + the timed matvecs; no solves")
+                    help="setup + the stage-timed matvecs (every kernel alone, the near field as one launch) + a "
+                         "256 MiB device copy (counter calibration); no solves")
     ap.add_argument("--write-traffic", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"), default=None,
                     help="summarise two rocprofv3 --pmc passes (FETCH_SIZE, WRITE_SIZE) of --profile-only into "
                          "profiles/traffic_<workload>.txt and exit")
@@ -603,8 +610,8 @@ def main():
     _log(f"mesh: {m.n_nodes} nodes, {m.n_faces} faces, {n} DOFs ({time.perf_counter() - t:.1f}s)")
     fp32 = args.precision == "fp32_refine"
     bvp, st, t_setup = setup_bvp(m, d, phi, phin, args, args.precision)
-    t_mv64, stage64 = time_matvec(bvp, n, m.n_nodes, args.matvecs, False)
-    t_mv, stage = (time_matvec(bvp, n, m.n_nodes, args.matvecs, True) if fp32 else (t_mv64, stage64))
+    t_mv64, stage64 = time_matvec(bvp, n, m.n_nodes, args.matvecs, False, args.profile_only)
+    t_mv, stage = (time_matvec(bvp, n, m.n_nodes, args.matvecs, True, args.profile_only) if fp32 else (t_mv64, stage64))
 
     if args.profile_only:
         from bem_for_nonlinear_waves_amd import devbuf
