@@ -274,7 +274,9 @@ struct bemfmm_ctx {
   DBuf<double> m2l_stage;    // per-pair M2L results [pair][layer][C]
   DBuf<double> ops;
   // FP32 copies for BEMFMM_PRECISION_FP32_REFINE (inner matvecs); the M2L staging is shared
-  DBuf<float> near_mat32, p2m_mat32, ops32;
+  DBuf<float> near_mat32, p2m_mat32, ops32;  // near_mat32: the FP32 near field folded for A x (k_fold_near)
+  DBuf<int64_t> nu_col_ptr, nu_mat_ptr;     // its blocks' column and weight offsets
+  DBuf<int32_t> nu_col_idx;                 // its columns, into [x | L0 | L1]
   // device operator table, [slot][KP][RP], kept across prepares: slot of each (kind, a, b, c) key
   std::map<std::array<int, 4>, int32_t> op_slot;
   int ops_cap = 0, ops_used = 0, ops_P = -1, ops32_valid = 0;
@@ -439,6 +441,17 @@ void launch_gemv(bemfmm_ctx* c, DGemv& g, const T* mat, const double* x0, const 
   dev::block_gemv(s ? s : c->stream, v, mat, x0, x1, y0, y1, part);
 }
 
+// y = U [x | L0 | L1] over the folded FP32 near field (A x of the FP32 operator: y holds y0 - y1)
+void launch_gemv_u(bemfmm_ctx* c, const double* x, const double* L0, const double* L1, double* y, int part,
+                   hipStream_t s) {
+  dev::Gemv v = c->near.view(c->near_ncd.p);
+  v.col_ptr = c->nu_col_ptr.p;
+  v.mat_ptr = c->nu_mat_ptr.p;
+  v.col_idx = c->nu_col_idx.p;
+  v.variant = c->gemv_var[1];
+  dev::block_gemv_u(s, v, c->near_mat32.p, c->n_dof, (int64_t)c->n_boxes * c->C, x, L0, L1, y, part);
+}
+
 template <typename T>
 void launch_p2m(bemfmm_ctx* c, const T* mat, const double* x0, const double* x1, double* M0, double* M1,
                 hipStream_t s) {
@@ -507,6 +520,7 @@ void evaluate_launch(bemfmm_ctx* c, int mode, const double* in, double* out, boo
   const int n = c->n_dof;
   const int64_t nbC = (int64_t)c->n_boxes * c->C;
   if (!w.stage && lo) throw std::runtime_error("internal: the FP32 operator needs the M2L staging");
+  if (lo && mode != 0) throw std::runtime_error("internal: the FP32 operator evaluates A x only");
   hipStream_t st = w.stream;
   double* phin = w.buf0;
   double* phi = w.buf1;
@@ -524,7 +538,7 @@ void evaluate_launch(bemfmm_ctx* c, int mode, const double* in, double* out, boo
   if (split) {
     HIPCHECK(hipEventRecord(w.fork, st));
     HIPCHECK(hipStreamWaitEvent(w.stream2, w.fork, 0));
-    if (lo) launch_gemv(c, c->near, c->near_mat32.p, w.buf0, w.buf1, w.y0, w.y1, 1, w.stream2);
+    if (lo) launch_gemv_u(c, in, L0, L1, w.y0, 1, w.stream2);
     else launch_gemv(c, c->near, c->near.mat.p, w.buf0, w.buf1, w.y0, w.y1, 1, w.stream2);
     HIPCHECK(hipEventRecord(w.join, w.stream2));
   }
@@ -572,10 +586,10 @@ void evaluate_launch(bemfmm_ctx* c, int mode, const double* in, double* out, boo
   rec(6);
   if (split) HIPCHECK(hipStreamWaitEvent(st, w.join, 0));
   const int part = split ? 2 : 0;
-  if (lo) launch_gemv(c, c->near, c->near_mat32.p, w.buf0, w.buf1, w.y0, w.y1, part, st);
+  if (lo) launch_gemv_u(c, in, L0, L1, w.y0, part, st);
   else launch_gemv(c, c->near, c->near.mat.p, w.buf0, w.buf1, w.y0, w.y1, part, st);
   rec(7);
-  dev::assemble(st, mode, n, c->kind.p, c->point.p, c->alpha.p, phi, w.y0, w.y1, c->phi_k.p, out);
+  dev::assemble(st, mode, n, c->kind.p, c->point.p, c->alpha.p, phi, w.y0, lo ? nullptr : w.y1, c->phi_k.p, out);
   rec(8);
   HIPCHECK(hipGetLastError());
 }
@@ -1009,7 +1023,7 @@ void tune_launch_shapes(bemfmm_ctx* c, const Plan& pl) {
       for (int v = 0; v < 3; ++v) {
         c->gemv_var[prec] = v;
         const float t = best_of([&] {
-          if (prec) launch_gemv(c, c->near, c->near_mat32.p, c->buf0.p, c->buf1.p, c->y0.p, c->y1.p);
+          if (prec) launch_gemv_u(c, c->buf0.p, c->buf0.p + c->n_dof, c->buf1.p + c->n_dof, c->y0.p, 0, s);
           else launch_gemv(c, c->near, c->near.mat.p, c->buf0.p, c->buf1.p, c->y0.p, c->y1.p);
         });
         if (t < bt) bt = t, bv = v;
@@ -1934,8 +1948,28 @@ void do_prepare(bemfmm_ctx* c) {
   const double t3c = now();
   if (c->opts.precision == BEMFMM_PRECISION_FP32_REFINE) {
     // round the resident FP64 matrices once to FP32 (operators were computed in binary128 and rounded to FP64)
-    c->near_mat32.alloc(pl.near.mat_size);
-    dev::to_f32(s, (int64_t)pl.near.mat_size, c->near.mat.p, c->near_mat32.p);
+    {
+      // the near field folded for A x: per block rows x (DOF columns + 2 C L2P columns, padded to 4)
+      const int nb = c->near.nblk;
+      std::vector<int64_t> rp;
+      c->near.row_ptr.download(rp, nb + 1, s);
+      std::vector<int64_t> ucp(nb + 1, 0), ump(nb, 0);
+      int64_t msz = 0;
+      for (int b = 0; b < nb; ++b) {
+        const int64_t ncu = (pl.near_ncol_dof[b] + 2 * (int64_t)pl.C + 3) / 4 * 4;
+        ucp[b + 1] = ucp[b] + ncu;
+        ump[b] = msz;
+        msz += (rp[b + 1] - rp[b]) * ncu;
+      }
+      if ((int64_t)c->n_dof + 2 * (int64_t)pl.n_boxes * pl.C >= (int64_t(1) << 31))
+        throw std::runtime_error("FP32 near field: column index beyond 32 bits");
+      c->nu_col_ptr.upload(ucp, s);
+      c->nu_mat_ptr.upload(ump, s);
+      c->nu_col_idx.alloc(std::max<int64_t>(1, ucp[nb]));
+      c->near_mat32.alloc(std::max<int64_t>(1, msz));
+      dev::fold_near(s, c->near.view(c->near_ncd.p), pl.C, c->n_dof, (int64_t)pl.n_boxes * pl.C, c->near.mat.p,
+                     c->kind.p, c->robin.p, c->nu_col_ptr.p, c->nu_mat_ptr.p, c->nu_col_idx.p, c->near_mat32.p);
+    }
     c->p2m_mat32.alloc(c->p2m.mat.n);
     dev::to_f32(s, (int64_t)c->p2m.mat.n, c->p2m.mat.p, c->p2m_mat32.p);
     c->P_in = c->opts.inner_p;  // validated above: below the plan's order, with the rotation M2L
@@ -1969,6 +2003,7 @@ void do_prepare(bemfmm_ctx* c) {
     HIPCHECK(hipGetLastError());
   } else {
     c->near_mat32.release();
+    c->nu_col_idx.release();
     c->p2m_mat32.release();
     c->ops32.release();
     c->ops32_valid = 0;
@@ -1994,7 +2029,7 @@ void do_prepare(bemfmm_ctx* c) {
   st.L = pl.Lscale;
   st.device_bytes = (double)(c->near.bytes() + c->p2m.bytes() + c->m2l.bytes() + c->ops.bytes() + c->buf0.bytes() +
                              c->buf1.bytes() + c->M0.bytes() + c->M1.bytes() + c->m2l_stage.bytes() +
-                             c->near_mat32.bytes() + c->p2m_mat32.bytes() + c->ops32.bytes() + c->sw_inv.bytes() +
+                             c->near_mat32.bytes() + c->nu_col_idx.bytes() + c->p2m_mat32.bytes() + c->ops32.bytes() + c->sw_inv.bytes() +
                              c->m2l_pw.bytes());
   st.t_tree = pl.t_tree;
   st.t_lists = pl.t_lists;

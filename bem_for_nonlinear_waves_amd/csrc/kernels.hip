@@ -250,6 +250,142 @@ __global__ __launch_bounds__(256) void k_block_gemv(const int64_t* __restrict__ 
   }
 }
 
+// The FP32 operator's near field folded for A x (mode 0, the only one the FP32 operator runs): in A x a DOF
+// column meets either phi_n = x (Dirichlet: weight G) or phi = x (Neumann and CORNER: -Gn; ROBIN: phi = x and
+// phi_n = kappa x, kappa G - Gn), so one weight per entry carries the row's y0 - y1. Block b of the FP64 near field
+// (rows x [G | Gn], columns [DOF (ncd) | L2P (C) | padding]) becomes rows x U, columns [DOF (ncd) | L2P on the
+// single-layer locals (C) | -L2P on the double-layer locals (C) | padding to 4]; the column indices address
+// [x (n) | L0 (nbC) | L1 (nbC)]. One workgroup per block.
+__global__ __launch_bounds__(256) void k_fold_near(int C, int n, int64_t nbC, const int64_t* __restrict__ row_ptr,
+                                                   const int64_t* __restrict__ col_ptr, const int64_t* __restrict__ mat_ptr,
+                                                   const int32_t* __restrict__ ncol_dof,
+                                                   const int32_t* __restrict__ col_idx, const double* __restrict__ mat,
+                                                   const int32_t* __restrict__ kind, const double* __restrict__ kappa,
+                                                   const int64_t* __restrict__ ucol_ptr,
+                                                   const int64_t* __restrict__ umat_ptr, int32_t* __restrict__ ucol_idx,
+                                                   float* __restrict__ umat) {
+  const int b = blockIdx.x;
+  const int nrow = (int)(row_ptr[b + 1] - row_ptr[b]);
+  const int ncol = (int)(col_ptr[b + 1] - col_ptr[b]);
+  const int ncu = (int)(ucol_ptr[b + 1] - ucol_ptr[b]);
+  const int ncd = ncol_dof[b];
+  const int32_t* ci = col_idx + col_ptr[b];
+  int32_t* cu = ucol_idx + ucol_ptr[b];
+  for (int c = threadIdx.x; c < ncu; c += blockDim.x)
+    cu[c] = c < ncd + C ? ci[c] : (c < ncd + 2 * C ? ci[c - C] + (int32_t)nbC : ci[ncd]);
+  const double* A = mat + mat_ptr[b];
+  float* U = umat + umat_ptr[b];
+  for (int64_t e = threadIdx.x; e < (int64_t)nrow * ncu; e += blockDim.x) {
+    const int r = (int)(e / ncu), c = (int)(e - (int64_t)r * ncu);
+    const double* G = A + (int64_t)r * 2 * ncol;
+    const double* Gn = G + ncol;
+    double u = 0.;
+    if (c < ncd) {
+      const int d = ci[c];
+      const int k = kind[d];
+      u = k == 0 ? G[c] : (k == 3 ? kappa[d] * G[c] - Gn[c] : -Gn[c]);
+    } else if (c < ncd + C) {
+      u = G[c];
+    } else if (c < ncd + 2 * C) {
+      u = -Gn[c - C];
+    }
+    U[(int64_t)r * ncu + c] = (float)u;
+  }
+}
+
+// y = U x over the folded FP32 near field (k_fold_near): the block GEMV of k_block_gemv with one weight per entry
+// and one sum per row; the staged inputs are gathered from x (column < n) or the locals L0 / L1. part as in
+// k_block_gemv (1: the DOF columns, written; 2: the rest, added).
+template <int RPW>
+__global__ __launch_bounds__(256) void k_block_gemv_u(const int64_t* __restrict__ row_ptr, const int64_t* __restrict__ col_ptr,
+                                                      const int64_t* __restrict__ mat_ptr, const int32_t* __restrict__ row_out,
+                                                      const int32_t* __restrict__ col_idx, const float* __restrict__ mat,
+                                                      int n, int64_t nbC, const double* __restrict__ x,
+                                                      const double* __restrict__ L0, const double* __restrict__ L1,
+                                                      double* __restrict__ y, int part, const int32_t* __restrict__ ncol_dof) {
+  typedef float4 V;
+  constexpr int W = 4;
+  constexpr int STEP = 64 * W;
+  __shared__ double xs[kGemvChunk];
+  __shared__ double acc[kGemvMaxRows];
+  const int b = blockIdx.x;
+  const int64_t r0 = row_ptr[b];
+  const int nrow = (int)(row_ptr[b + 1] - r0);
+  const int64_t c0 = col_ptr[b];
+  const int ncol = (int)(col_ptr[b + 1] - c0);  // multiple of 4
+  const float* M = mat + mat_ptr[b];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (int i = threadIdx.x; i < nrow; i += blockDim.x) acc[i] = 0.;
+  const int split = part == 0 ? 0 : (ncol_dof[b] & ~3);
+  const int cb = part == 2 ? split : 0, ce = part == 1 ? split : ncol;
+  auto gather = [&](int g) -> double { return g < n ? x[g] : (g < n + nbC ? L0[g - n] : L1[g - n - nbC]); };
+  for (int cs = cb; cs < ce; cs += kGemvChunk) {
+    const int cn = min(kGemvChunk, ce - cs);  // multiple of 4
+    __syncthreads();
+    const int32_t* ci = col_idx + c0 + cs;
+    int i = threadIdx.x;
+    for (; i + 3 * 256 < cn; i += 4 * 256) {
+      const int g0 = ci[i], g1 = ci[i + 256], g2 = ci[i + 512], g3 = ci[i + 768];
+      const double a0 = gather(g0), a1 = gather(g1), a2 = gather(g2), a3 = gather(g3);
+      xs[i] = a0;
+      xs[i + 256] = a1;
+      xs[i + 512] = a2;
+      xs[i + 768] = a3;
+    }
+    for (; i < cn; i += 256) xs[i] = gather(ci[i]);
+    __syncthreads();
+    for (int rg = wave * RPW; rg < nrow; rg += 4 * RPW) {
+      const float* Ar[RPW];
+#pragma unroll
+      for (int k = 0; k < RPW; ++k) Ar[k] = M + (int64_t)min(rg + k, nrow - 1) * ncol + cs;
+      double s0[RPW];
+#pragma unroll
+      for (int k = 0; k < RPW; ++k) s0[k] = 0.;
+      int c = lane * W;
+      for (; c + STEP < cn; c += 2 * STEP) {
+        V va[RPW][2];
+#pragma unroll
+        for (int k = 0; k < RPW; ++k) {
+          va[k][0] = ld_nt(reinterpret_cast<const V*>(Ar[k] + c));
+          va[k][1] = ld_nt(reinterpret_cast<const V*>(Ar[k] + c + STEP));
+        }
+        double xa[2][W];
+#pragma unroll
+        for (int u = 0; u < W; ++u) {
+          xa[0][u] = xs[c + u];
+          xa[1][u] = xs[c + STEP + u];
+        }
+#pragma unroll
+        for (int k = 0; k < RPW; ++k)
+#pragma unroll
+          for (int u = 0; u < W; ++u)
+            s0[k] += GemvVec<float>::get(va[k][0], u) * xa[0][u] + GemvVec<float>::get(va[k][1], u) * xa[1][u];
+      }
+      if (c < cn) {
+        V va[RPW];
+#pragma unroll
+        for (int k = 0; k < RPW; ++k) va[k] = ld_nt(reinterpret_cast<const V*>(Ar[k] + c));
+#pragma unroll
+        for (int k = 0; k < RPW; ++k)
+#pragma unroll
+          for (int u = 0; u < W; ++u) s0[k] += GemvVec<float>::get(va[k], u) * xs[c + u];
+      }
+#pragma unroll
+      for (int k = 0; k < RPW; ++k) {
+        double u = s0[k];
+        for (int off = 32; off > 0; off >>= 1) u += __shfl_down(u, off);
+        if (lane == 0 && rg + k < nrow) acc[rg + k] += u;
+      }
+    }
+  }
+  __syncthreads();
+  for (int r = threadIdx.x; r < nrow; r += blockDim.x) {
+    const int o = row_out[r0 + r];
+    if (part == 2) y[o] += acc[r];  // each row belongs to one block: a plain read-modify-write
+    else y[o] = acc[r];
+  }
+}
+
 // P2M: one workgroup per leaf; the leaf's matrix is column-major per layer (plan.hpp), so thread
 // (layer = tid / 128, row = tid % 128 + 128 k) streams a column's rows with its neighbours: every load is
 // coalesced and every lane has a row, however few DOF columns the leaf has. The gathered phi_n / phi
@@ -1055,7 +1191,8 @@ __global__ void k_assemble(int mode, int n, const int32_t* __restrict__ kind, co
     out[i] = a;
     return;
   }
-  double v = (kind[i] == 2) ? phi[i] : y0[p] - (y1[p] + alpha[i] * phi[i]);
+  // y1 == nullptr: y0 already holds y0 - y1 (the folded FP32 near field of A x)
+  double v = (kind[i] == 2) ? phi[i] : y0[p] - ((y1 ? y1[p] : 0.) + alpha[i] * phi[i]);
   if (mode == 1) v = (kind[i] == 2) ? phi_k[i] : -v;
   out[i] = v;
 }
@@ -1723,6 +1860,20 @@ void p2m(hipStream_t s, int C, int CP, const Gemv& g, const double* mat, const d
 void p2m(hipStream_t s, int C, int CP, const Gemv& g, const float* mat, const double* x0, const double* x1,
          double* M0, double* M1) {
   p2m_t(s, C, CP, g, mat, x0, x1, M0, M1);
+}
+
+void fold_near(hipStream_t s, const Gemv& g, int C, int n, int64_t nbC, const double* mat, const int32_t* kind,
+               const double* kappa, const int64_t* ucol_ptr, const int64_t* umat_ptr, int32_t* ucol_idx, float* umat) {
+  if (g.nblk == 0) return;
+  hipLaunchKernelGGL(k_fold_near, dim3(g.nblk), dim3(256), 0, s, C, n, nbC, g.row_ptr, g.col_ptr, g.mat_ptr, g.ncol_dof,
+                     g.col_idx, mat, kind, kappa, ucol_ptr, umat_ptr, ucol_idx, umat);
+}
+void block_gemv_u(hipStream_t s, const Gemv& g, const float* mat, int n, int64_t nbC, const double* x,
+                  const double* L0, const double* L1, double* y, int part) {
+  if (g.nblk == 0) return;
+  auto k = g.variant == 1 ? k_block_gemv_u<2> : g.variant == 2 ? k_block_gemv_u<8> : k_block_gemv_u<4>;
+  hipLaunchKernelGGL(k, dim3(g.nblk), dim3(256), 0, s, g.row_ptr, g.col_ptr, g.mat_ptr, g.row_out, g.col_idx, mat, n,
+                     nbC, x, L0, L1, y, part, g.ncol_dof);
 }
 
 void translate(hipStream_t s, int C, int RP, int KP, const Tiles& t, const double* ops, const double* S0,

@@ -655,7 +655,9 @@ def main():
     rot_macs = (2 * sum(4 * k + (k + 1) ** 2 + k ** 2 for k in range(pp + 1)) + (pp + 1) ** 2
                 + 2 * sum((pp + 1 - k) ** 2 for k in range(1, pp + 1)))
     m2l_flops = st["n_m2l_pairs"] * 2 * (rot_macs if rot else C * C) * 2.0
-    near_bytes = (st["near_nnz"] + m.n_nodes * C) * 2 * esz  # dense G/Gn weights incl. L2P columns
+    # near field + L2P: the FP64 operator reads G and Gn per entry (and the L2P columns of both layers' rows); the
+    # FP32 operator's is folded for A x, one weight per entry and 2 C L2P columns per row (single and double layer)
+    near_bytes = ((st["near_nnz"] + m.n_nodes * 2 * C) * 4.0 if fp32 else (st["near_nnz"] + m.n_nodes * C) * 2 * 8.0)
     dom = int(np.argmax(stage))
     # boxes that receive M2L: each target's local expansion is written once (2 C doubles)
     m2l_targets = st.get("n_m2l_targets", st["n_boxes"])
@@ -676,7 +678,9 @@ def main():
                     "unit": "TFLOP/s", "frac": ach / mfma_peak, "traffic": None}
     else:
         ach = near_bytes / (stage[dom] * 1e-3) / 1e9
-        roof = {"kernel": f"k_block_gemv ({STAGES[dom]})", "bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS,
+        roof = {"kernel": ("k_block_gemv_u (near field + L2P folded for A x, FP32)" if fp32
+                           else "k_block_gemv (near field + L2P, FP64)"), "bound": "hbm", "achieved": ach,
+                "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "traffic": None, "algorithmic_bytes": near_bytes}
     # the M2L stage as a whole (translation + per-box reduction): algorithmic bytes = source gathers + each
     # target's locals written once
@@ -689,7 +693,7 @@ def main():
     elif STAGES[dom] == "m2l":
         tkey = f"k_translate_mfma<double,{(C + 63) // 64},true,{st['m2l_tile_pairs']}>"
     else:
-        tkey = (f"k_block_gemv<float,{st['near_rows_per_wave'][1]}>" if fp32
+        tkey = (f"k_block_gemv_u<{st['near_rows_per_wave'][1]}>" if fp32
                 else f"k_block_gemv<double,{st['near_rows_per_wave'][0]}>")
     traffic = measured_traffic(tkey, workload_key(args, fp32))
     if traffic is not None:

@@ -253,6 +253,63 @@ void block_gemv(hipStream_t s, const Gemv& g, const float* mat, const double* x0
   const Gemv gg = g;
   hipstub::enqueue(s, [=]() { gemv_cpu<float>(gg, mat, x0, x1, y0, y1, part); });
 }
+void fold_near(hipStream_t s, const Gemv& g, int C, int n, int64_t nbC, const double* mat, const int32_t* kind,
+               const double* kappa, const int64_t* ucol_ptr, const int64_t* umat_ptr, int32_t* ucol_idx, float* umat) {
+  const Gemv gg = g;
+  hipstub::enqueue(s, [=]() {
+    for (int b = 0; b < gg.nblk; ++b) {
+      const int nrow = (int)(gg.row_ptr[b + 1] - gg.row_ptr[b]);
+      const int ncol = (int)(gg.col_ptr[b + 1] - gg.col_ptr[b]);
+      const int ncu = (int)(ucol_ptr[b + 1] - ucol_ptr[b]);
+      const int ncd = gg.ncol_dof[b];
+      const int32_t* ci = gg.col_idx + gg.col_ptr[b];
+      int32_t* cu = ucol_idx + ucol_ptr[b];
+      for (int c = 0; c < ncu; ++c) cu[c] = c < ncd + C ? ci[c] : (c < ncd + 2 * C ? ci[c - C] + (int32_t)nbC : ci[ncd]);
+      for (int r = 0; r < nrow; ++r) {
+        const double* G = mat + gg.mat_ptr[b] + (int64_t)r * 2 * ncol;
+        const double* Gn = G + ncol;
+        float* U = umat + umat_ptr[b] + (int64_t)r * ncu;
+        for (int c = 0; c < ncu; ++c) {
+          double u = 0.;
+          if (c < ncd) {
+            const int d = ci[c];
+            u = kind[d] == 0 ? G[c] : (kind[d] == 3 ? kappa[d] * G[c] - Gn[c] : -Gn[c]);
+          } else if (c < ncd + C) {
+            u = G[c];
+          } else if (c < ncd + 2 * C) {
+            u = -Gn[c - C];
+          }
+          U[c] = (float)u;
+        }
+      }
+    }
+  });
+}
+void block_gemv_u(hipStream_t s, const Gemv& g, const float* mat, int n, int64_t nbC, const double* x,
+                  const double* L0, const double* L1, double* y, int part) {
+  const Gemv gg = g;
+  hipstub::enqueue(s, [=]() {
+    for (int b = 0; b < gg.nblk; ++b) {
+      const int64_t r0 = gg.row_ptr[b];
+      const int nrow = (int)(gg.row_ptr[b + 1] - r0);
+      const int ncol = (int)(gg.col_ptr[b + 1] - gg.col_ptr[b]);
+      const int split = part == 0 ? 0 : (gg.ncol_dof[b] & ~3);
+      const int cb = part == 2 ? split : 0, ce = part == 1 ? split : ncol;
+      const int32_t* ci = gg.col_idx + gg.col_ptr[b];
+      for (int r = 0; r < nrow; ++r) {
+        const float* U = mat + gg.mat_ptr[b] + (int64_t)r * ncol;
+        double acc = 0.;
+        for (int c = cb; c < ce; ++c) {
+          const int g = ci[c];
+          acc += (double)U[c] * (g < n ? x[g] : (g < n + nbC ? L0[g - n] : L1[g - n - nbC]));
+        }
+        const int o = gg.row_out[r0 + r];
+        if (part == 2) y[o] += acc;
+        else y[o] = acc;
+      }
+    }
+  });
+}
 void p2m(hipStream_t s, int C, int CP, const Gemv& g, const double* mat, const double* x0, const double* x1,
          double* M0, double* M1) {
   const Gemv gg = g;
@@ -336,7 +393,7 @@ void assemble(hipStream_t s, int mode, int n, const int32_t* kind, const int32_t
         out[i] = a;
         continue;
       }
-      double v = (kind[i] == 2) ? phi[i] : y0[p] - (y1[p] + alpha[i] * phi[i]);
+      double v = (kind[i] == 2) ? phi[i] : y0[p] - ((y1 ? y1[p] : 0.) + alpha[i] * phi[i]);
       if (mode == 1) v = (kind[i] == 2) ? phi_k[i] : -v;
       out[i] = v;
     }
