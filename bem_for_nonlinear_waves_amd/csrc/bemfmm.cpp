@@ -456,8 +456,11 @@ template <typename T>
 void launch_p2m(bemfmm_ctx* c, const T* mat, const double* x0, const double* x1, double* M0, double* M1,
                 hipStream_t s) {
   dev::Gemv v = c->p2m.view();
-  v.variant = c->p2m_var[std::is_same<T, float>::value ? 1 : 0];
-  dev::p2m(s, c->C, c->p2m_rows_pad, v, mat, x0, x1, M0, M1);
+  const bool f32 = std::is_same<T, float>::value;
+  v.variant = c->p2m_var[f32 ? 1 : 0];
+  // the FP32 operator at an inner order needs the multipoles' first (P_in+1)^2 coefficients only
+  const int rows = (f32 && c->m2l_rot && c->P_in) ? (c->P_in + 1) * (c->P_in + 1) : 0;
+  dev::p2m(s, c->C, c->p2m_rows_pad, v, mat, x0, x1, M0, M1, rows);
 }
 
 template <typename T>
@@ -1947,16 +1950,21 @@ void do_prepare(bemfmm_ctx* c) {
   HIPCHECK(hipStreamSynchronize(s));
   const double t3c = now();
   if (c->opts.precision == BEMFMM_PRECISION_FP32_REFINE) {
+    c->P_in = c->opts.inner_p;  // validated above: below the plan's order, with the rotation M2L
+    c->m2l_rot = (c->opts.m2l_kernel == 0 || c->opts.m2l_kernel == 2) && (c->P_in || dev::m2l_rot_compiled(pl.P));
     // round the resident FP64 matrices once to FP32 (operators were computed in binary128 and rounded to FP64)
     {
-      // the near field folded for A x: per block rows x (DOF columns + 2 C L2P columns, padded to 4)
+      // the near field folded for A x: per block rows x (DOF columns + 2 Cl L2P columns, padded to 4)
       const int nb = c->near.nblk;
       std::vector<int64_t> rp;
       c->near.row_ptr.download(rp, nb + 1, s);
       std::vector<int64_t> ucp(nb + 1, 0), ump(nb, 0);
       int64_t msz = 0;
+      // L2P columns per layer: the inner order's coefficients when the FP32 M2L runs below p (the locals are
+      // zero above it), else all C
+      const int Cl = (c->m2l_rot && c->P_in) ? (c->P_in + 1) * (c->P_in + 1) : pl.C;
       for (int b = 0; b < nb; ++b) {
-        const int64_t ncu = (pl.near_ncol_dof[b] + 2 * (int64_t)pl.C + 3) / 4 * 4;
+        const int64_t ncu = (pl.near_ncol_dof[b] + 2 * (int64_t)Cl + 3) / 4 * 4;
         ucp[b + 1] = ucp[b] + ncu;
         ump[b] = msz;
         msz += (rp[b + 1] - rp[b]) * ncu;
@@ -1967,13 +1975,11 @@ void do_prepare(bemfmm_ctx* c) {
       c->nu_mat_ptr.upload(ump, s);
       c->nu_col_idx.alloc(std::max<int64_t>(1, ucp[nb]));
       c->near_mat32.alloc(std::max<int64_t>(1, msz));
-      dev::fold_near(s, c->near.view(c->near_ncd.p), pl.C, c->n_dof, (int64_t)pl.n_boxes * pl.C, c->near.mat.p,
+      dev::fold_near(s, c->near.view(c->near_ncd.p), pl.C, Cl, c->n_dof, (int64_t)pl.n_boxes * pl.C, c->near.mat.p,
                      c->kind.p, c->robin.p, c->nu_col_ptr.p, c->nu_mat_ptr.p, c->nu_col_idx.p, c->near_mat32.p);
     }
     c->p2m_mat32.alloc(c->p2m.mat.n);
     dev::to_f32(s, (int64_t)c->p2m.mat.n, c->p2m.mat.p, c->p2m_mat32.p);
-    c->P_in = c->opts.inner_p;  // validated above: below the plan's order, with the rotation M2L
-    c->m2l_rot = (c->opts.m2l_kernel == 0 || c->opts.m2l_kernel == 2) && (c->P_in || dev::m2l_rot_compiled(pl.P));
     c->m2l_pw_on = c->opts.m2l_kernel == 3;
     if (c->m2l_pw_on) {
       // the FP32 operator's M2L through plane waves: extra table slots for the composed operators, own tiles

@@ -256,7 +256,9 @@ __global__ __launch_bounds__(256) void k_block_gemv(const int64_t* __restrict__ 
 // (rows x [G | Gn], columns [DOF (ncd) | L2P (C) | padding]) becomes rows x U, columns [DOF (ncd) | L2P on the
 // single-layer locals (C) | -L2P on the double-layer locals (C) | padding to 4]; the column indices address
 // [x (n) | L0 (nbC) | L1 (nbC)]. One workgroup per block.
-__global__ __launch_bounds__(256) void k_fold_near(int C, int n, int64_t nbC, const int64_t* __restrict__ row_ptr,
+// Cl <= C: the L2P columns kept per layer (the inner order's (p_in+1)^2 when the FP32 operator's M2L runs at a
+// lower order: its locals are zero above it).
+__global__ __launch_bounds__(256) void k_fold_near(int C, int Cl, int n, int64_t nbC, const int64_t* __restrict__ row_ptr,
                                                    const int64_t* __restrict__ col_ptr, const int64_t* __restrict__ mat_ptr,
                                                    const int32_t* __restrict__ ncol_dof,
                                                    const int32_t* __restrict__ col_idx, const double* __restrict__ mat,
@@ -272,7 +274,7 @@ __global__ __launch_bounds__(256) void k_fold_near(int C, int n, int64_t nbC, co
   const int32_t* ci = col_idx + col_ptr[b];
   int32_t* cu = ucol_idx + ucol_ptr[b];
   for (int c = threadIdx.x; c < ncu; c += blockDim.x)
-    cu[c] = c < ncd + C ? ci[c] : (c < ncd + 2 * C ? ci[c - C] + (int32_t)nbC : ci[ncd]);
+    cu[c] = c < ncd + Cl ? ci[c] : (c < ncd + 2 * Cl ? ci[c - Cl] + (int32_t)nbC : ci[ncd]);
   const double* A = mat + mat_ptr[b];
   float* U = umat + umat_ptr[b];
   for (int64_t e = threadIdx.x; e < (int64_t)nrow * ncu; e += blockDim.x) {
@@ -284,10 +286,10 @@ __global__ __launch_bounds__(256) void k_fold_near(int C, int n, int64_t nbC, co
       const int d = ci[c];
       const int k = kind[d];
       u = k == 0 ? G[c] : (k == 3 ? kappa[d] * G[c] - Gn[c] : -Gn[c]);
-    } else if (c < ncd + C) {
+    } else if (c < ncd + Cl) {
       u = G[c];
-    } else if (c < ncd + 2 * C) {
-      u = -Gn[c - C];
+    } else if (c < ncd + 2 * Cl) {
+      u = -Gn[c - Cl];
     }
     U[(int64_t)r * ncu + c] = (float)u;
   }
@@ -393,7 +395,7 @@ __global__ __launch_bounds__(256) void k_block_gemv_u(const int64_t* __restrict_
 // U: column loads in flight per thread (4, 8 or 16), chosen per problem class by the autotuner of the prepare
 constexpr int kP2MChunk = 1024;
 template <typename T, int U>
-__global__ __launch_bounds__(256) void k_p2m(int C, int CP, const int64_t* __restrict__ row_ptr,
+__global__ __launch_bounds__(256) void k_p2m(int C, int CP, int Crows, const int64_t* __restrict__ row_ptr,
                                              const int64_t* __restrict__ col_ptr, const int64_t* __restrict__ mat_ptr,
                                              const int32_t* __restrict__ row_out, const int32_t* __restrict__ col_idx,
                                              const T* __restrict__ mat, const double* __restrict__ x0,
@@ -421,7 +423,7 @@ __global__ __launch_bounds__(256) void k_p2m(int C, int CP, const int64_t* __res
 #pragma unroll
     for (int k = 0; k < kRows; ++k) {
       const int r = r0 + 128 * k;
-      if (r >= CP) break;
+      if (r >= Crows) break;
       const T* a = Al + (int64_t)cs * CP + r;
       double s0 = 0., s1 = 0.;
       int c = 0;
@@ -444,7 +446,7 @@ __global__ __launch_bounds__(256) void k_p2m(int C, int CP, const int64_t* __res
 #pragma unroll
   for (int k = 0; k < kRows; ++k) {
     const int r = r0 + 128 * k;
-    if (r < C) out[base + r] = acc[k];
+    if (r < Crows) out[base + r] = acc[k];
   }
 }
 
@@ -1846,27 +1848,29 @@ void block_gemv(hipStream_t s, const Gemv& g, const float* mat, const double* x0
 }
 
 template <typename T>
-void p2m_t(hipStream_t s, int C, int CP, const Gemv& g, const T* mat, const double* x0, const double* x1, double* M0,
-           double* M1) {
+void p2m_t(hipStream_t s, int C, int CP, int Crows, const Gemv& g, const T* mat, const double* x0, const double* x1,
+           double* M0, double* M1) {
   if (g.nblk == 0) return;
+  if (Crows < 1 || Crows > C || C > CP) throw std::runtime_error("p2m: rows out of range");
   auto k = g.variant == 1 ? k_p2m<T, 4> : g.variant == 2 ? k_p2m<T, 16> : k_p2m<T, 8>;
-  hipLaunchKernelGGL(k, dim3(g.nblk), dim3(256), 0, s, C, CP, g.row_ptr, g.col_ptr, g.mat_ptr, g.row_out, g.col_idx,
-                     mat, x0, x1, M0, M1);
+  hipLaunchKernelGGL(k, dim3(g.nblk), dim3(256), 0, s, C, CP, Crows, g.row_ptr, g.col_ptr, g.mat_ptr, g.row_out,
+                     g.col_idx, mat, x0, x1, M0, M1);
 }
 void p2m(hipStream_t s, int C, int CP, const Gemv& g, const double* mat, const double* x0, const double* x1,
-         double* M0, double* M1) {
-  p2m_t(s, C, CP, g, mat, x0, x1, M0, M1);
+         double* M0, double* M1, int Crows) {
+  p2m_t(s, C, CP, Crows > 0 ? Crows : C, g, mat, x0, x1, M0, M1);
 }
 void p2m(hipStream_t s, int C, int CP, const Gemv& g, const float* mat, const double* x0, const double* x1,
-         double* M0, double* M1) {
-  p2m_t(s, C, CP, g, mat, x0, x1, M0, M1);
+         double* M0, double* M1, int Crows) {
+  p2m_t(s, C, CP, Crows > 0 ? Crows : C, g, mat, x0, x1, M0, M1);
 }
 
-void fold_near(hipStream_t s, const Gemv& g, int C, int n, int64_t nbC, const double* mat, const int32_t* kind,
+void fold_near(hipStream_t s, const Gemv& g, int C, int Cl, int n, int64_t nbC, const double* mat, const int32_t* kind,
                const double* kappa, const int64_t* ucol_ptr, const int64_t* umat_ptr, int32_t* ucol_idx, float* umat) {
   if (g.nblk == 0) return;
-  hipLaunchKernelGGL(k_fold_near, dim3(g.nblk), dim3(256), 0, s, C, n, nbC, g.row_ptr, g.col_ptr, g.mat_ptr, g.ncol_dof,
-                     g.col_idx, mat, kind, kappa, ucol_ptr, umat_ptr, ucol_idx, umat);
+  if (Cl < 1 || Cl > C) throw std::runtime_error("fold_near: L2P columns out of range");
+  hipLaunchKernelGGL(k_fold_near, dim3(g.nblk), dim3(256), 0, s, C, Cl, n, nbC, g.row_ptr, g.col_ptr, g.mat_ptr,
+                     g.ncol_dof, g.col_idx, mat, kind, kappa, ucol_ptr, umat_ptr, ucol_idx, umat);
 }
 void block_gemv_u(hipStream_t s, const Gemv& g, const float* mat, int n, int64_t nbC, const double* x,
                   const double* L0, const double* L1, double* y, int part) {

@@ -67,7 +67,8 @@ void gemv_cpu(Gemv g, const T* mat, const double* x0, const double* x1, double* 
 }
 
 template <typename T>
-void p2m_cpu(int C, int CP, Gemv g, const T* mat, const double* x0, const double* x1, double* M0, double* M1) {
+void p2m_cpu(int C, int CP, Gemv g, const T* mat, const double* x0, const double* x1, double* M0, double* M1,
+             int Crows) {
   for (int b = 0; b < g.nblk; ++b) {
     const int64_t c0 = g.col_ptr[b];
     const int ncol = (int)(g.col_ptr[b + 1] - c0);
@@ -76,7 +77,7 @@ void p2m_cpu(int C, int CP, Gemv g, const T* mat, const double* x0, const double
     for (int layer = 0; layer < 2; ++layer) {
       const double* x = layer ? x1 : x0;
       double* out = layer ? M1 : M0;
-      for (int r = 0; r < C; ++r) {
+      for (int r = 0; r < Crows; ++r) {
         double s = 0.;
         for (int c = 0; c < ncol; ++c) s += (double)A[(int64_t)(layer * ncol + c) * CP + r] * x[g.col_idx[c0 + c]];
         out[base + r] = s;
@@ -253,8 +254,9 @@ void block_gemv(hipStream_t s, const Gemv& g, const float* mat, const double* x0
   const Gemv gg = g;
   hipstub::enqueue(s, [=]() { gemv_cpu<float>(gg, mat, x0, x1, y0, y1, part); });
 }
-void fold_near(hipStream_t s, const Gemv& g, int C, int n, int64_t nbC, const double* mat, const int32_t* kind,
+void fold_near(hipStream_t s, const Gemv& g, int C, int Cl, int n, int64_t nbC, const double* mat, const int32_t* kind,
                const double* kappa, const int64_t* ucol_ptr, const int64_t* umat_ptr, int32_t* ucol_idx, float* umat) {
+  if (Cl < 1 || Cl > C) throw std::runtime_error("fold_near: L2P columns out of range");
   const Gemv gg = g;
   hipstub::enqueue(s, [=]() {
     for (int b = 0; b < gg.nblk; ++b) {
@@ -264,7 +266,7 @@ void fold_near(hipStream_t s, const Gemv& g, int C, int n, int64_t nbC, const do
       const int ncd = gg.ncol_dof[b];
       const int32_t* ci = gg.col_idx + gg.col_ptr[b];
       int32_t* cu = ucol_idx + ucol_ptr[b];
-      for (int c = 0; c < ncu; ++c) cu[c] = c < ncd + C ? ci[c] : (c < ncd + 2 * C ? ci[c - C] + (int32_t)nbC : ci[ncd]);
+      for (int c = 0; c < ncu; ++c) cu[c] = c < ncd + Cl ? ci[c] : (c < ncd + 2 * Cl ? ci[c - Cl] + (int32_t)nbC : ci[ncd]);
       for (int r = 0; r < nrow; ++r) {
         const double* G = mat + gg.mat_ptr[b] + (int64_t)r * 2 * ncol;
         const double* Gn = G + ncol;
@@ -274,10 +276,10 @@ void fold_near(hipStream_t s, const Gemv& g, int C, int n, int64_t nbC, const do
           if (c < ncd) {
             const int d = ci[c];
             u = kind[d] == 0 ? G[c] : (kind[d] == 3 ? kappa[d] * G[c] - Gn[c] : -Gn[c]);
-          } else if (c < ncd + C) {
+          } else if (c < ncd + Cl) {
             u = G[c];
-          } else if (c < ncd + 2 * C) {
-            u = -Gn[c - C];
+          } else if (c < ncd + 2 * Cl) {
+            u = -Gn[c - Cl];
           }
           U[c] = (float)u;
         }
@@ -311,14 +313,16 @@ void block_gemv_u(hipStream_t s, const Gemv& g, const float* mat, int n, int64_t
   });
 }
 void p2m(hipStream_t s, int C, int CP, const Gemv& g, const double* mat, const double* x0, const double* x1,
-         double* M0, double* M1) {
+         double* M0, double* M1, int Crows) {
   const Gemv gg = g;
-  hipstub::enqueue(s, [=]() { p2m_cpu<double>(C, CP, gg, mat, x0, x1, M0, M1); });
+  const int rows = Crows > 0 ? Crows : C;
+  hipstub::enqueue(s, [=]() { p2m_cpu<double>(C, CP, gg, mat, x0, x1, M0, M1, rows); });
 }
 void p2m(hipStream_t s, int C, int CP, const Gemv& g, const float* mat, const double* x0, const double* x1,
-         double* M0, double* M1) {
+         double* M0, double* M1, int Crows) {
   const Gemv gg = g;
-  hipstub::enqueue(s, [=]() { p2m_cpu<float>(C, CP, gg, mat, x0, x1, M0, M1); });
+  const int rows = Crows > 0 ? Crows : C;
+  hipstub::enqueue(s, [=]() { p2m_cpu<float>(C, CP, gg, mat, x0, x1, M0, M1, rows); });
 }
 void translate(hipStream_t s, int C, int RP, int KP, const Tiles& t, const double* ops, const double* S0,
                const double* S1, double* D0, double* D1, double* stage) {
