@@ -1455,7 +1455,7 @@ void do_prepare(bemfmm_ctx* c) {
       (c->opts.precision != BEMFMM_PRECISION_FP32_REFINE || c->opts.inner_p >= c->opts.p ||
        !dev::m2l_rot_inner_compiled(c->opts.inner_p) || (c->opts.m2l_kernel != 0 && c->opts.m2l_kernel != 2)))
     throw std::runtime_error("bemfmm_prepare: inner_p must be 0 (the plan's order) or an order below p that the "
-                             "rotation M2L runs (6 or 8), with FP32_REFINE and m2l_kernel 0 or 2");
+                             "rotation M2L runs (5, 6 or 8), with FP32_REFINE and m2l_kernel 0 or 2");
   if (c->opts.near_rows_per_wave != 0 && c->opts.near_rows_per_wave != 2 && c->opts.near_rows_per_wave != 4 &&
       c->opts.near_rows_per_wave != 8)
     throw std::runtime_error("bemfmm_prepare: near_rows_per_wave must be 0 (tuned), 2, 4 or 8");

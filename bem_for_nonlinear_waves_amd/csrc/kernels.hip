@@ -1037,7 +1037,7 @@ __global__ __launch_bounds__(256) void k_m2l_rot(int ntiles, const float* __rest
                                                  const float* __restrict__ M32, float* __restrict__ stage) {
   constexpr int C = (P + 1) * (P + 1);
   constexpr int RS = rot_size(P);
-  constexpr int NCH = (C + 15) / 16;
+  constexpr int NCH = stage_stride<float>(C) / 16;  // 16-row chunks of the staged column (rows >= C are zero)
   constexpr int kS = 17;  // LDS column stride (odd: conflict-free column reads)
   __shared__ float4 R4[RS / 4 + 4];  // + 16 floats read past the last row by lanes 4q..15
   __shared__ float Tw[4][64 * kS];
@@ -1894,12 +1894,15 @@ void pack_m32(hipStream_t s, int64_t n, int C, int Cs, const double* M0, const d
 }
 
 bool m2l_rot_compiled(int P) { return P == 8 || P == 10; }
-bool m2l_rot_inner_compiled(int P) { return P == 6 || P == 8 || P == 10; }
+bool m2l_rot_inner_compiled(int P) { return P == 5 || P == 6 || P == 8 || P == 10; }
 
 void m2l_rot(hipStream_t s, int P, const Tiles& t, const float* rot, const float* M32, float* stage) {
   if (t.ntiles == 0) return;
   const int grid = ((t.ntiles + kXCD - 1) / kXCD) * kXCD;
   switch (P) {
+  case 5:
+    hipLaunchKernelGGL(k_m2l_rot<5>, dim3(grid), dim3(256), 0, s, t.ntiles, rot, t.op, t.p0, t.p1, t.src, M32, stage);
+    break;
   case 6:
     hipLaunchKernelGGL(k_m2l_rot<6>, dim3(grid), dim3(256), 0, s, t.ntiles, rot, t.op, t.p0, t.p1, t.src, M32, stage);
     break;

@@ -532,7 +532,7 @@ def main():
     ap.add_argument("--precond", choices=["none", "jacobi", "schwarz"], default="schwarz")
     ap.add_argument("--precision", choices=["fp32_refine", "fp64"], default="fp32_refine")
     ap.add_argument("--refine-eta", type=float, default=3e-7)
-    ap.add_argument("--inner-p", type=int, default=6, choices=[0, 6, 8],
+    ap.add_argument("--inner-p", type=int, default=6, choices=[0, 5, 6, 8],
                     help="fp32_refine: expansion order of the FP32 operator's M2L (0: --p); the FP64 residuals keep --p")
     ap.add_argument("--refine-recycle", type=int, default=1, choices=[0, 1],
                     help="refinement steps after the first keep the earlier steps' Krylov directions (library option)")

@@ -129,7 +129,7 @@ typedef struct bemfmm_opts {
   int p2m_loads;          /* P2M launch shape: column loads in flight per thread, 4, 8 or 16; 0 (default): timed
                              like near_rows_per_wave */
   int inner_p;            /* FP32_REFINE with the rotation M2L: expansion order of the FP32 operator's M2L, below p
-                             (6 or 8); 0 (default): p. The inner solves then run on the expansions truncated at
+                             (5, 6 or 8); 0 (default): p. The inner solves then run on the expansions truncated at
                              inner_p (P2M, M2M, L2L and L2P keep p, whose leading terms are the inner_p ones); the
                              FP64 residual of every refinement step keeps p, so the result is that of the p operator
                              (the reference switches its single-precision M2L to double near convergence instead,

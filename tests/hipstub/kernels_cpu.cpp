@@ -343,7 +343,7 @@ void pack_m32(hipStream_t s, int64_t n, int C, int Cs, const double* M0, const d
   });
 }
 bool m2l_rot_compiled(int P) { return P == 8 || P == 10; }
-bool m2l_rot_inner_compiled(int P) { return P == 6 || P == 8 || P == 10; }
+bool m2l_rot_inner_compiled(int P) { return P == 5 || P == 6 || P == 8 || P == 10; }
 void m2l_rot(hipStream_t s, int P, const Tiles& t, const float* rot, const float* M32, float* stage) {
   if (!m2l_rot_inner_compiled(P)) throw std::runtime_error("rotation M2L not compiled for this expansion order");
   const Tiles tt = t;

@@ -29,7 +29,7 @@ def _rel(a, b):
     return float(np.linalg.norm(a - b) / np.linalg.norm(b))
 
 
-@pytest.mark.parametrize("p,inner", [(10, 8), (10, 6), (8, 6)])
+@pytest.mark.parametrize("p,inner", [(10, 8), (10, 6), (10, 5), (8, 6)])
 def test_inner_operator_is_the_lower_order_fmm(small_tank, oracle_mod, p, inner):
     m, d, phi, phin = small_tank
     g = _bvp(m, d, phi, phin, p=p, leaf_max=32, precision=_lib.PRECISION_FP32_REFINE, inner_p=inner,
@@ -48,7 +48,7 @@ def test_inner_operator_is_the_lower_order_fmm(small_tank, oracle_mod, p, inner)
     g.close()
 
 
-@pytest.mark.parametrize("inner", [8, 6])
+@pytest.mark.parametrize("inner", [8, 6, 5])
 def test_inner_order_solve_reaches_dense_lu(tiny_tank, oracle_mod, inner):
     m, d, phi, phin = tiny_tank
     g = _bvp(m, d, phi, phin, p=10, leaf_max=16, precision=_lib.PRECISION_FP32_REFINE, inner_p=inner,
@@ -70,6 +70,7 @@ def test_inner_order_outside_its_range_is_refused(tiny_tank):
     m, d, phi, phin = tiny_tank
     bad = [dict(p=10, inner_p=10, precision=_lib.PRECISION_FP32_REFINE),  # not below p
            dict(p=10, inner_p=7, precision=_lib.PRECISION_FP32_REFINE),   # no rotation kernel at 7
+           dict(p=10, inner_p=4, precision=_lib.PRECISION_FP32_REFINE),   # nor at 4
            dict(p=10, inner_p=8),                                         # FP64 mode
            dict(p=10, inner_p=8, precision=_lib.PRECISION_FP32_REFINE, m2l_kernel=1)]  # dense FP32 M2L
     for kw in bad:

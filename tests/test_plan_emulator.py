@@ -145,7 +145,7 @@ def _rotation_blocks(emu, P, o):
     return YD @ Z, T, Z.T @ YHt
 
 
-@pytest.mark.parametrize("P", [6, 8, 10])
+@pytest.mark.parametrize("P", [5, 6, 8, 10])
 def test_m2l_rotation_factors(emu, P):
     """The rotation M2L factors (used by the FP32 operator's M2L kernel) reproduce the dense float128-built
     M2L operator: T(o) = H^T T(|o| z) D for offsets in every octant, on the axes and near the x axis (the
