@@ -657,7 +657,9 @@ def main():
     m2l_flops = st["n_m2l_pairs"] * 2 * (rot_macs if rot else C * C) * 2.0
     # near field + L2P: the FP64 operator reads G and Gn per entry (and the L2P columns of both layers' rows); the
     # FP32 operator's is folded for A x, one weight per entry and 2 C L2P columns per row (single and double layer)
-    near_bytes = ((st["near_nnz"] + m.n_nodes * 2 * C) * 4.0 if fp32 else (st["near_nnz"] + m.n_nodes * C) * 2 * 8.0)
+    # (the folded L2P columns stop at the inner order when one is set: its locals are zero above it)
+    C_l = (st.get("inner_p", 0) + 1) ** 2 if (fp32 and st.get("inner_p", 0)) else C
+    near_bytes = ((st["near_nnz"] + m.n_nodes * 2 * C_l) * 4.0 if fp32 else (st["near_nnz"] + m.n_nodes * C) * 2 * 8.0)
     dom = int(np.argmax(stage))
     # boxes that receive M2L: each target's local expansion is written once (2 C doubles)
     m2l_targets = st.get("n_m2l_targets", st["n_boxes"])
