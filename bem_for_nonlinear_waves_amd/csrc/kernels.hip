@@ -1104,11 +1104,12 @@ __global__ __launch_bounds__(256) void k_m2l_rot(int ntiles, const float* __rest
   }
 }
 
-// FP32 staging: the same reduction with 16-byte loads. Thread t < CS/2 of wave w reads float4 t of a
-// pair's two staged columns (rows 4t..4t+3 of one layer, CS = stage_stride<float>(C)); wave w takes the
-// pairs a + w, a + w + 4, ... (kReduceLoads in flight per thread, summed in pair order) and the four waves'
-// sums are combined in a fixed order (deterministic). Boxes run in descending pair count (order), as in
-// k_m2l_reduce.
+// FP32 staging: the same reduction with 16-byte loads. A pair's two staged columns (CS = stage_stride<float>(C)
+// floats each) are LP = CS / 2 float4s; a wave reads 64 / LP pairs at once (1 at C = 121, 2 at the inner orders
+// 5 and 6, whose columns are 64 floats), lane (sub, t) float4 t of its pair. The workgroup's G = 4 x 64 / LP
+// lane groups take the pairs a + g, a + g + G, ... (kReduceLoads in flight per lane, summed in pair order) and
+// the groups' sums are combined in group order (deterministic). Boxes run in descending pair count (order),
+// as in k_m2l_reduce.
 constexpr int kReduceLoads = 8;
 // Cout >= C: the locals are stored with stride Cout; rows [C, Cout) (degrees above the M2L's order) are zeroed.
 __global__ __launch_bounds__(256) void k_m2l_reduce_f32(int C, int Cout, const int32_t* __restrict__ order,
@@ -1116,20 +1117,23 @@ __global__ __launch_bounds__(256) void k_m2l_reduce_f32(int C, int Cout, const i
                                                         const int32_t* __restrict__ tpairs,
                                                         const float* __restrict__ stage, double* __restrict__ L0,
                                                         double* __restrict__ L1) {
-  __shared__ double part[3][4][64];
+  __shared__ double part[4][256];  // [component][group g * LP + t]
   const int CS = stage_stride<float>(C);  // <= 128, a multiple of 32
+  const int LP = CS / 2, ppw = 64 / LP, G = 4 * ppw;
   const int tb = order[blockIdx.x];
-  const int w = threadIdx.x >> 6, t = threadIdx.x & 63;
-  const bool act = 4 * t < 2 * CS;
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int sub = lane / LP, t = lane - sub * LP;
+  const bool act = sub < ppw;  // lanes past ppw * LP idle (LP = 48 at C = 81)
+  const int g = w * ppw + sub;
   const int64_t a = tptr[tb], b = tptr[tb + 1];
   double s0 = 0., s1 = 0., s2 = 0., s3 = 0.;
   if (act) {
-    int64_t q = a + w;
-    for (; q + 4 * (kReduceLoads - 1) < b; q += 4 * kReduceLoads) {
+    int64_t q = a + g;
+    for (; q + (int64_t)G * (kReduceLoads - 1) < b; q += (int64_t)G * kReduceLoads) {
       float4 v[kReduceLoads];
 #pragma unroll
       for (int u = 0; u < kReduceLoads; ++u)
-        v[u] = ld_nt(reinterpret_cast<const float4*>(stage + (int64_t)tpairs[q + 4 * u] * 2 * CS) + t);
+        v[u] = ld_nt(reinterpret_cast<const float4*>(stage + (int64_t)tpairs[q + (int64_t)G * u] * 2 * CS) + t);
 #pragma unroll
       for (int u = 0; u < kReduceLoads; ++u) {
         s0 += v[u].x;
@@ -1138,40 +1142,39 @@ __global__ __launch_bounds__(256) void k_m2l_reduce_f32(int C, int Cout, const i
         s3 += v[u].w;
       }
     }
-    for (; q < b; q += 4) {
+    for (; q < b; q += G) {
       const float4 v = ld_nt(reinterpret_cast<const float4*>(stage + (int64_t)tpairs[q] * 2 * CS) + t);
       s0 += v.x;
       s1 += v.y;
       s2 += v.z;
       s3 += v.w;
     }
-  }
-  if (w > 0 && act) {
-    part[w - 1][0][t] = s0;
-    part[w - 1][1][t] = s1;
-    part[w - 1][2][t] = s2;
-    part[w - 1][3][t] = s3;
+    part[0][g * LP + t] = s0;
+    part[1][g * LP + t] = s1;
+    part[2][g * LP + t] = s2;
+    part[3][g * LP + t] = s3;
   }
   for (int r = C + threadIdx.x; r < Cout; r += blockDim.x) {
     L0[(int64_t)tb * Cout + r] = 0.;
     L1[(int64_t)tb * Cout + r] = 0.;
   }
   __syncthreads();
-  if (w == 0 && act) {
-#pragma unroll
-    for (int v = 0; v < 3; ++v) {
-      s0 += part[v][0][t];
-      s1 += part[v][1][t];
-      s2 += part[v][2][t];
-      s3 += part[v][3][t];
+  if (threadIdx.x < LP) {
+    const int tt = threadIdx.x;
+    double S0 = 0., S1 = 0., S2 = 0., S3 = 0.;
+    for (int gg = 0; gg < G; ++gg) {
+      S0 += part[0][gg * LP + tt];
+      S1 += part[1][gg * LP + tt];
+      S2 += part[2][gg * LP + tt];
+      S3 += part[3][gg * LP + tt];
     }
-    const int f = 4 * t;  // a float4 never straddles the two layers (CS is a multiple of 4)
+    const int f = 4 * tt;  // a float4 never straddles the two layers (CS is a multiple of 4)
     const int layer = f >= CS ? 1 : 0, r = f - layer * CS;
     double* L = (layer ? L1 : L0) + (int64_t)tb * Cout;
-    if (r < C) L[r] = s0;
-    if (r + 1 < C) L[r + 1] = s1;
-    if (r + 2 < C) L[r + 2] = s2;
-    if (r + 3 < C) L[r + 3] = s3;
+    if (r < C) L[r] = S0;
+    if (r + 1 < C) L[r + 1] = S1;
+    if (r + 2 < C) L[r + 2] = S2;
+    if (r + 3 < C) L[r + 3] = S3;
   }
 }
 
