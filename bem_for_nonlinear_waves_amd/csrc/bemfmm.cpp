@@ -295,6 +295,12 @@ struct bemfmm_ctx {
     int RS = 0;
   } rot, rot_in;
   int P_in = 0;
+  // the FP32 operator at the inner order runs its expansions compact: (P_in+1)^2 coefficients per box
+  // (C_in, its padded RP_in / KP_in), M2M and L2L on the leading blocks of the 16 octant operators (ops_in,
+  // tiles m2m_in / l2l_in index it directly)
+  int C_in = 0, RP_in = 0, KP_in = 0;
+  DBuf<double> ops_in;
+  std::vector<DTiles> m2m_in, l2l_in;
   DTiles m2l_rt;  // M2L tiles of kRotTP pairs (one offset each)
   bool m2l_pw_on = false;  // FP32 operator's M2L through the plane-wave composed operators (m2l_kernel 3)
   DTiles m2l_pw;           // its tiles (operator slots of the plane-wave M2L offsets)
@@ -441,6 +447,9 @@ void launch_gemv(bemfmm_ctx* c, DGemv& g, const T* mat, const double* x0, const 
   dev::block_gemv(s ? s : c->stream, v, mat, x0, x1, y0, y1, part);
 }
 
+// coefficients per box of the FP32 operator's expansions (the inner order's when one is set)
+int fp32_coeffs(const bemfmm_ctx* c) { return (c->m2l_rot && c->P_in) ? c->C_in : c->C; }
+
 // y = U [x | L0 | L1] over the folded FP32 near field (A x of the FP32 operator: y holds y0 - y1)
 void launch_gemv_u(bemfmm_ctx* c, const double* x, const double* L0, const double* L1, double* y, int part,
                    hipStream_t s) {
@@ -449,7 +458,7 @@ void launch_gemv_u(bemfmm_ctx* c, const double* x, const double* L0, const doubl
   v.mat_ptr = c->nu_mat_ptr.p;
   v.col_idx = c->nu_col_idx.p;
   v.variant = c->gemv_var[1];
-  dev::block_gemv_u(s, v, c->near_mat32.p, c->n_dof, (int64_t)c->n_boxes * c->C, x, L0, L1, y, part);
+  dev::block_gemv_u(s, v, c->near_mat32.p, c->n_dof, (int64_t)c->n_boxes * fp32_coeffs(c), x, L0, L1, y, part);
 }
 
 template <typename T>
@@ -458,8 +467,8 @@ void launch_p2m(bemfmm_ctx* c, const T* mat, const double* x0, const double* x1,
   dev::Gemv v = c->p2m.view();
   const bool f32 = std::is_same<T, float>::value;
   v.variant = c->p2m_var[f32 ? 1 : 0];
-  // the FP32 operator at an inner order needs the multipoles' first (P_in+1)^2 coefficients only
-  const int rows = (f32 && c->m2l_rot && c->P_in) ? (c->P_in + 1) * (c->P_in + 1) : 0;
+  // the FP32 operator at an inner order computes the multipoles' first C_in coefficients, stored at stride C_in
+  const int rows = (f32 && c->m2l_rot && c->P_in) ? c->C_in : 0;
   dev::p2m(s, c->C, c->p2m_rows_pad, v, mat, x0, x1, M0, M1, rows);
 }
 
@@ -478,7 +487,7 @@ void launch_m2l_rot(bemfmm_ctx* c, hipStream_t s, float* stage, const double* M0
   const bemfmm_ctx::RotTable& R = c->P_in ? c->rot_in : c->rot;
   if (R.RS != dev::rot_size(P)) throw std::runtime_error("rotation M2L: factor table stride mismatch");
   const int Cm = (P + 1) * (P + 1);
-  dev::pack_m32(s, (int64_t)c->n_boxes * 2 * Cm, Cm, c->C, M0, M1, m32);
+  dev::pack_m32(s, (int64_t)c->n_boxes * 2 * Cm, Cm, fp32_coeffs(c), M0, M1, m32);
   dev::m2l_rot(s, P, t.view(), R.f.p, m32, stage);
 }
 
@@ -545,19 +554,39 @@ void evaluate_launch(bemfmm_ctx* c, int mode, const double* in, double* out, boo
     else launch_gemv(c, c->near, c->near.mat.p, w.buf0, w.buf1, w.y0, w.y1, 1, w.stream2);
     HIPCHECK(hipEventRecord(w.join, w.stream2));
   }
-  fill(c, w.M0, nbC, 0., st);
-  fill(c, w.M1, nbC, 0., st);
+  // the FP32 operator at the inner order: expansions of C_in coefficients per box throughout (compact)
+  const bool compact = lo && c->m2l_rot && c->P_in;
+  const int64_t nbCw = compact ? (int64_t)c->n_boxes * c->C_in : nbC;
+  fill(c, w.M0, nbCw, 0., st);
+  fill(c, w.M1, nbCw, 0., st);
   rec(1);
   if (lo) launch_p2m(c, c->p2m_mat32.p, phin, phi, w.M0, w.M1, st);
   else launch_p2m(c, c->p2m.mat.p, phin, phi, w.M0, w.M1, st);
   rec(2);
+  auto translate_levels = [&](bool up, double* A0, double* A1) {
+    if (up) {
+      for (int l = pl.n_levels - 1; l >= 1 && l - 1 >= c->m2l_min_level; --l) {
+        if (compact)
+          dev::translate(st, c->C_in, c->RP_in, c->KP_in, c->m2m_in[l].view(), c->ops_in.p, A0, A1, A0, A1,
+                         (double*)nullptr);
+        else
+          launch_translate(c, c->m2m[l], c->ops.p, A0, A1, A0, A1, (double*)nullptr, st);
+      }
+    } else {
+      for (int l = std::max(1, c->m2l_min_level + 1); l < pl.n_levels; ++l) {
+        if (compact)
+          dev::translate(st, c->C_in, c->RP_in, c->KP_in, c->l2l_in[l].view(), c->ops_in.p, A0, A1, A0, A1,
+                         (double*)nullptr);
+        else
+          launch_translate(c, c->l2l[l], c->ops.p, A0, A1, A0, A1, (double*)nullptr, st);
+      }
+    }
+  };
   // M2M from level l into l-1 only where level l-1 has M2L pairs (coarser multipoles are not used).
   // M2M and L2L use the FP64 operators for both operators: their inputs and outputs are FP64, and the
   // FP64 matrix-core tiles ran faster here than the FP32 ones (0.115 / 0.120 ms against 0.157 / 0.161 ms
   // per matvec at 209k nodes)
-  for (int l = pl.n_levels - 1; l >= 1 && l - 1 >= c->m2l_min_level; --l) {
-    launch_translate(c, c->m2m[l], c->ops.p, w.M0, w.M1, w.M0, w.M1, (double*)nullptr, st);
-  }
+  translate_levels(true, w.M0, w.M1);
   rec(3);
   if (!w.stage) {
     // no staging: the FP64 M2L tiles accumulate straight into the locals (atomics, as M2M and L2L do)
@@ -570,9 +599,8 @@ void evaluate_launch(bemfmm_ctx* c, int mode, const double* in, double* out, boo
     if (c->m2l_rot) launch_m2l_rot(c, st, stf, w.M0, w.M1, w.m32);
     else launch_translate(c, c->m2l_pw_on ? c->m2l_pw : c->m2l, c->ops32.p, w.M0, w.M1, nullptr, nullptr, stf, st);
     rec(4);
-    // rows of the locals above the inner operator's order are zero (L2L and L2P then use its order throughout)
-    const int Cm = (c->m2l_rot && c->P_in) ? (c->P_in + 1) * (c->P_in + 1) : c->C;
-    dev::m2l_reduce(st, Cm, c->C, c->n_boxes, c->m2l_rorder.p, c->m2l_tptr.p, c->m2l_tpairs.p, (const float*)stf, L0,
+    const int Cw = fp32_coeffs(c);
+    dev::m2l_reduce(st, Cw, Cw, c->n_boxes, c->m2l_rorder.p, c->m2l_tptr.p, c->m2l_tpairs.p, (const float*)stf, L0,
                     L1);
   } else {
     double* std_ = reinterpret_cast<double*>(w.stage);
@@ -583,9 +611,7 @@ void evaluate_launch(bemfmm_ctx* c, int mode, const double* in, double* out, boo
   }
   rec(5);
   // L2L into level l only from parents at levels with M2L pairs (the locals of coarser levels are zero)
-  for (int l = std::max(1, c->m2l_min_level + 1); l < pl.n_levels; ++l) {
-    launch_translate(c, c->l2l[l], c->ops.p, L0, L1, L0, L1, (double*)nullptr, st);
-  }
+  translate_levels(false, L0, L1);
   rec(6);
   if (split) HIPCHECK(hipStreamWaitEvent(st, w.join, 0));
   const int part = split ? 2 : 0;
@@ -1952,6 +1978,35 @@ void do_prepare(bemfmm_ctx* c) {
   if (c->opts.precision == BEMFMM_PRECISION_FP32_REFINE) {
     c->P_in = c->opts.inner_p;  // validated above: below the plan's order, with the rotation M2L
     c->m2l_rot = (c->opts.m2l_kernel == 0 || c->opts.m2l_kernel == 2) && (c->P_in || dev::m2l_rot_compiled(pl.P));
+    c->C_in = c->RP_in = c->KP_in = 0;
+    if (c->m2l_rot && c->P_in) {
+      // compact expansions of the inner order: M2M and L2L on the leading C_in x C_in blocks of the octant
+      // operators (the packing is degree-major and M2M / L2L are triangular in the degree, so these blocks are
+      // the inner order's operators), laid out [op][KP_in][RP_in] as the matrix-core tiles read them
+      c->C_in = (c->P_in + 1) * (c->P_in + 1);
+      c->RP_in = (c->C_in + 63) / 64 * 64;
+      c->KP_in = (c->C_in + 4 * dev::kKG - 1) / (4 * dev::kKG) * (4 * dev::kKG);
+      const int64_t per = (int64_t)c->KP_in * c->RP_in;
+      std::vector<double> t((size_t)16 * per, 0.);
+      for (int o = 0; o < 16; ++o) {
+        const auto key = bemfmm::op_key(pl, o);
+        if (key[0] != (o < 8 ? 0 : 1)) throw std::runtime_error("inner order: operators 0..15 are not M2M / L2L");
+        const auto v = bemfmm::cached_operator(pl.P, key[0], key[1], key[2], key[3]);
+        for (int k = 0; k < c->C_in; ++k)
+          for (int r = 0; r < c->C_in; ++r) t[(size_t)o * per + (size_t)k * c->RP_in + r] = (*v)[(size_t)k * pl.C + r];
+      }
+      c->ops_in.upload(t, s);
+      std::vector<int32_t> ident(16);
+      for (int o = 0; o < 16; ++o) ident[o] = o;
+      c->m2m_in.clear();
+      c->l2l_in.clear();
+      c->m2m_in.resize(pl.n_levels);
+      c->l2l_in.resize(pl.n_levels);
+      for (int l = 1; l < pl.n_levels; ++l) {
+        c->m2m_in[l].upload(pl.m2m_by_level[l], s, ident);
+        c->l2l_in[l].upload(pl.l2l_by_level[l], s, ident);
+      }
+    }
     // round the resident FP64 matrices once to FP32 (operators were computed in binary128 and rounded to FP64)
     {
       // the near field folded for A x: per block rows x (DOF columns + 2 Cl L2P columns, padded to 4)
@@ -1962,7 +2017,7 @@ void do_prepare(bemfmm_ctx* c) {
       int64_t msz = 0;
       // L2P columns per layer: the inner order's coefficients when the FP32 M2L runs below p (the locals are
       // zero above it), else all C
-      const int Cl = (c->m2l_rot && c->P_in) ? (c->P_in + 1) * (c->P_in + 1) : pl.C;
+      const int Cl = (c->m2l_rot && c->P_in) ? c->C_in : pl.C;
       for (int b = 0; b < nb; ++b) {
         const int64_t ncu = (pl.near_ncol_dof[b] + 2 * (int64_t)Cl + 3) / 4 * 4;
         ucp[b + 1] = ucp[b] + ncu;
@@ -1975,7 +2030,7 @@ void do_prepare(bemfmm_ctx* c) {
       c->nu_mat_ptr.upload(ump, s);
       c->nu_col_idx.alloc(std::max<int64_t>(1, ucp[nb]));
       c->near_mat32.alloc(std::max<int64_t>(1, msz));
-      dev::fold_near(s, c->near.view(c->near_ncd.p), pl.C, Cl, c->n_dof, (int64_t)pl.n_boxes * pl.C, c->near.mat.p,
+      dev::fold_near(s, c->near.view(c->near_ncd.p), pl.C, Cl, c->n_dof, (int64_t)pl.n_boxes * Cl, c->near.mat.p,
                      c->kind.p, c->robin.p, c->nu_col_ptr.p, c->nu_mat_ptr.p, c->nu_col_idx.p, c->near_mat32.p);
     }
     c->p2m_mat32.alloc(c->p2m.mat.n);

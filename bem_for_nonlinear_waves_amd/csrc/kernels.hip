@@ -254,10 +254,10 @@ __global__ __launch_bounds__(256) void k_block_gemv(const int64_t* __restrict__ 
 // column meets either phi_n = x (Dirichlet: weight G) or phi = x (Neumann and CORNER: -Gn; ROBIN: phi = x and
 // phi_n = kappa x, kappa G - Gn), so one weight per entry carries the row's y0 - y1. Block b of the FP64 near field
 // (rows x [G | Gn], columns [DOF (ncd) | L2P (C) | padding]) becomes rows x U, columns [DOF (ncd) | L2P on the
-// single-layer locals (C) | -L2P on the double-layer locals (C) | padding to 4]; the column indices address
-// [x (n) | L0 (nbC) | L1 (nbC)]. One workgroup per block.
-// Cl <= C: the L2P columns kept per layer (the inner order's (p_in+1)^2 when the FP32 operator's M2L runs at a
-// lower order: its locals are zero above it).
+// single-layer locals (Cl) | -L2P on the double-layer locals (Cl) | padding to 4]; the column indices address
+// [x (n) | L0 (nbC = boxes x Cl) | L1 (nbC)]. One workgroup per block.
+// Cl <= C: the locals' coefficients per box, and the L2P columns kept per layer (the inner order's (p_in+1)^2 when
+// the FP32 operator runs its expansions at a lower order; else C).
 __global__ __launch_bounds__(256) void k_fold_near(int C, int Cl, int n, int64_t nbC, const int64_t* __restrict__ row_ptr,
                                                    const int64_t* __restrict__ col_ptr, const int64_t* __restrict__ mat_ptr,
                                                    const int32_t* __restrict__ ncol_dof,
@@ -273,8 +273,10 @@ __global__ __launch_bounds__(256) void k_fold_near(int C, int Cl, int n, int64_t
   const int ncd = ncol_dof[b];
   const int32_t* ci = col_idx + col_ptr[b];
   int32_t* cu = ucol_idx + ucol_ptr[b];
+  // L2P columns: local j of the target box, at stride Cl in [L0 | L1] (the FP64 field's index is n + box C + j)
+  const int32_t l0 = n + (ci[ncd] - n) / C * Cl;
   for (int c = threadIdx.x; c < ncu; c += blockDim.x)
-    cu[c] = c < ncd + Cl ? ci[c] : (c < ncd + 2 * Cl ? ci[c - Cl] + (int32_t)nbC : ci[ncd]);
+    cu[c] = c < ncd ? ci[c] : (c < ncd + Cl ? l0 + (c - ncd) : (c < ncd + 2 * Cl ? l0 + (int32_t)nbC + (c - ncd - Cl) : l0));
   const double* A = mat + mat_ptr[b];
   float* U = umat + umat_ptr[b];
   for (int64_t e = threadIdx.x; e < (int64_t)nrow * ncu; e += blockDim.x) {
@@ -395,7 +397,7 @@ __global__ __launch_bounds__(256) void k_block_gemv_u(const int64_t* __restrict_
 // U: column loads in flight per thread (4, 8 or 16), chosen per problem class by the autotuner of the prepare
 constexpr int kP2MChunk = 1024;
 template <typename T, int U>
-__global__ __launch_bounds__(256) void k_p2m(int C, int CP, int Crows, const int64_t* __restrict__ row_ptr,
+__global__ __launch_bounds__(256) void k_p2m(int C, int CP, int Crows, int Cout, const int64_t* __restrict__ row_ptr,
                                              const int64_t* __restrict__ col_ptr, const int64_t* __restrict__ mat_ptr,
                                              const int32_t* __restrict__ row_out, const int32_t* __restrict__ col_idx,
                                              const T* __restrict__ mat, const double* __restrict__ x0,
@@ -441,7 +443,7 @@ __global__ __launch_bounds__(256) void k_p2m(int C, int CP, int Crows, const int
       acc[k] += s0 + s1;
     }
   }
-  const int64_t base = row_out[row_ptr[b]];  // leaf box * C (rows of a leaf are consecutive)
+  const int64_t base = (int64_t)(row_out[row_ptr[b]] / C) * Cout;  // leaf box * Cout (row_out holds leaf * C)
   double* out = layer ? M1 : M0;
 #pragma unroll
   for (int k = 0; k < kRows; ++k) {
@@ -1872,8 +1874,8 @@ void p2m_t(hipStream_t s, int C, int CP, int Crows, const Gemv& g, const T* mat,
   if (g.nblk == 0) return;
   if (Crows < 1 || Crows > C || C > CP) throw std::runtime_error("p2m: rows out of range");
   auto k = g.variant == 1 ? k_p2m<T, 4> : g.variant == 2 ? k_p2m<T, 16> : k_p2m<T, 8>;
-  hipLaunchKernelGGL(k, dim3(g.nblk), dim3(256), 0, s, C, CP, Crows, g.row_ptr, g.col_ptr, g.mat_ptr, g.row_out,
-                     g.col_idx, mat, x0, x1, M0, M1);
+  hipLaunchKernelGGL(k, dim3(g.nblk), dim3(256), 0, s, C, CP, Crows, Crows == C ? C : Crows, g.row_ptr, g.col_ptr,
+                     g.mat_ptr, g.row_out, g.col_idx, mat, x0, x1, M0, M1);
 }
 void p2m(hipStream_t s, int C, int CP, const Gemv& g, const double* mat, const double* x0, const double* x1,
          double* M0, double* M1, int Crows) {

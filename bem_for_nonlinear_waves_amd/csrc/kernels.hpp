@@ -188,13 +188,15 @@ void block_gemv(hipStream_t s, const Gemv& g, const double* mat, const double* x
 void block_gemv(hipStream_t s, const Gemv& g, const float* mat, const double* x0, const double* x1, double* y0,
                 double* y1, int part);
 // the FP32 near field folded for A x: g is the FP64 near field's structure (with ncol_dof); writes the folded
-// block columns ucol_idx (into [x | L0 | L1]) and weights umat at the offsets ucol_ptr / umat_ptr
+// block columns ucol_idx (into [x | L0 | L1], the locals at stride Cl, nbC = boxes x Cl) and weights umat at the
+// offsets ucol_ptr / umat_ptr
 void fold_near(hipStream_t s, const Gemv& g, int C, int Cl, int n, int64_t nbC, const double* mat, const int32_t* kind,
                const double* kappa, const int64_t* ucol_ptr, const int64_t* umat_ptr, int32_t* ucol_idx, float* umat);
 // y = U [x | L0 | L1] over the folded blocks (g: col_ptr / mat_ptr / col_idx of the folded layout); part as above
 void block_gemv_u(hipStream_t s, const Gemv& g, const float* mat, int n, int64_t nbC, const double* x,
                   const double* L0, const double* L1, double* y, int part);
-// Crows: multipole rows computed (the first Crows coefficients; 0 = all C)
+// Crows: multipole rows computed (the first Crows coefficients; 0 = all C); fewer than C: the multipoles are
+// stored with stride Crows
 void p2m(hipStream_t s, int C, int CP, const Gemv& g, const double* mat, const double* x0, const double* x1,
          double* M0, double* M1, int Crows = 0);
 void p2m(hipStream_t s, int C, int CP, const Gemv& g, const float* mat, const double* x0, const double* x1,

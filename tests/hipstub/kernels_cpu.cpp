@@ -73,7 +73,7 @@ void p2m_cpu(int C, int CP, Gemv g, const T* mat, const double* x0, const double
     const int64_t c0 = g.col_ptr[b];
     const int ncol = (int)(g.col_ptr[b + 1] - c0);
     const T* A = mat + g.mat_ptr[b];
-    const int64_t base = g.row_out[g.row_ptr[b]];
+    const int64_t base = (int64_t)(g.row_out[g.row_ptr[b]] / C) * Crows;  // stride Crows (C unless fewer rows)
     for (int layer = 0; layer < 2; ++layer) {
       const double* x = layer ? x1 : x0;
       double* out = layer ? M1 : M0;
@@ -266,7 +266,9 @@ void fold_near(hipStream_t s, const Gemv& g, int C, int Cl, int n, int64_t nbC, 
       const int ncd = gg.ncol_dof[b];
       const int32_t* ci = gg.col_idx + gg.col_ptr[b];
       int32_t* cu = ucol_idx + ucol_ptr[b];
-      for (int c = 0; c < ncu; ++c) cu[c] = c < ncd + Cl ? ci[c] : (c < ncd + 2 * Cl ? ci[c - Cl] + (int32_t)nbC : ci[ncd]);
+      const int32_t l0 = n + (ci[ncd] - n) / C * Cl;
+      for (int c = 0; c < ncu; ++c)
+        cu[c] = c < ncd ? ci[c] : (c < ncd + Cl ? l0 + (c - ncd) : (c < ncd + 2 * Cl ? l0 + (int32_t)nbC + (c - ncd - Cl) : l0));
       for (int r = 0; r < nrow; ++r) {
         const double* G = mat + gg.mat_ptr[b] + (int64_t)r * 2 * ncol;
         const double* Gn = G + ncol;
