@@ -397,6 +397,7 @@ struct bemfmm_ctx {
   int64_t v_cap = 0;
   int basis_n = -1;          // DOF count the basis is laid out for
   DBuf<double> partial, hbuf;
+  DBuf<double> gcro_proj;  // Q_C Q_C^T over the kept basis vectors of a recycled inner solve (nk x nk)
   bemfmm_stats stats{};
   // profiling
   bool prof = false;
@@ -2133,8 +2134,9 @@ void ensure_basis(bemfmm_ctx* c, int64_t need) {
   c->V.swap(nv);
   c->v_cap = cap;
   c->partial.alloc((size_t)kDotBlocks * (cap + 1));
-  // [h1 (k) | h2 (k) | w.w] of one Arnoldi step (k <= cap), or the k update coefficients
-  c->hbuf.alloc(2 * (size_t)cap + 2);
+  // [h1 (k) | h2 (k) | w.w | projected coefficients (k)] of one Arnoldi step (k <= cap), or the k update
+  // coefficients
+  c->hbuf.alloc(3 * (size_t)cap + 3);
   if (c->hres) (void)pinned_free(c->hres);
   c->hres = nullptr;
   HIPCHECK(pinned_malloc((void**)&c->hres, (2 * (size_t)cap + 2) * sizeof(double)));
@@ -2494,25 +2496,41 @@ double refine_inner(bemfmm_ctx* c, RecycledKrylov& K, const double* r, double* d
     return cc;
   };
   // w <- w minus its components along C and the new vectors [nk, nb), twice; returns the coefficients
-  // (over C, then the new vectors) and |w|^2 in `norm2w`
+  // (over C, then the new vectors) and |w|^2 in `norm2w`. Both passes and the norm run on the device
+  // (hbuf = [dots of pass 1 | dots of pass 2 | w.w | projected coefficients]; the first nk dots are projected
+  // by Q_C Q_C^T there); the raw dots come to the host once, for the coefficients over C.
   auto orthogonalise = [&](double* w, int nb, double& norm2w) {
     std::vector<double> total(kc + nb - nk, 0.);
+    const size_t pb = 2 * (size_t)nb + 1;
     for (int pass = 0; pass < 2; ++pass) {
-      dots_dev(c, nb, c->V.p, w, 0);
-      HIPCHECK(hipMemcpyAsync(c->hres, c->hbuf.p, nb * sizeof(double), hipMemcpyDeviceToHost, s));
-      HIPCHECK(hipStreamSynchronize(s));
-      std::vector<double> coef(c->hres, c->hres + nb);
+      dots_dev(c, nb, c->V.p, w, (size_t)pass * nb);
+      dev::proj_coef(s, nk, nb, c->gcro_proj.p, c->hbuf.p + (size_t)pass * nb, c->hbuf.p + pb);
+      dev::vsum(s, c->V.p, n, nb, c->hbuf.p + pb, -1.0, w, n);
+    }
+    dots_dev(c, 1, w, w, 2 * (size_t)nb);
+    HIPCHECK(hipMemcpyAsync(c->hres, c->hbuf.p, pb * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPCHECK(hipStreamSynchronize(s));
+    for (int pass = 0; pass < 2; ++pass) {
+      std::vector<double> coef(c->hres + (size_t)pass * nb, c->hres + (size_t)(pass + 1) * nb);
       const std::vector<double> cc = project_c(coef);
       for (int j = 0; j < kc; ++j) total[j] += cc[j];
       for (int i = nk; i < nb; ++i) total[kc + i - nk] += coef[i];
-      HIPCHECK(hipMemcpyAsync(c->hbuf.p, coef.data(), nb * sizeof(double), hipMemcpyHostToDevice, s));
-      dev::vsum(s, c->V.p, n, nb, c->hbuf.p, -1.0, w, n);
     }
-    std::vector<double> h;
-    dots(c, 1, w, w, h);
-    norm2w = h[0];
+    norm2w = c->hres[2 * (size_t)nb];
     return total;
   };
+  std::vector<double> proj_h;  // kept alive until the end of the solve (its upload is asynchronous)
+  if (recycle) {
+    proj_h.assign((size_t)nk * nk, 0.);
+    for (int i = 0; i < nk; ++i)
+      for (int i2 = 0; i2 < nk; ++i2) {
+        double v = 0.;
+        for (int j = 0; j < kc; ++j) v += QC[j][i] * QC[j][i2];
+        proj_h[(size_t)i * nk + i2] = v;
+      }
+    if (c->gcro_proj.n < proj_h.size()) c->gcro_proj.alloc(proj_h.size());
+    HIPCHECK(hipMemcpyAsync(c->gcro_proj.p, proj_h.data(), proj_h.size() * sizeof(double), hipMemcpyHostToDevice, s));
+  }
   if (!recycle) K = RecycledKrylov{};
   // first new vector: r without its components along C
   ensure_basis(c, nk + 2);

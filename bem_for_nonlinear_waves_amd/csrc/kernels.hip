@@ -1777,6 +1777,19 @@ __global__ void k_vsum(const double* __restrict__ V, int64_t ldv, int k, const d
     w[j] += sign * s;
   }
 }
+// out[i] = sum_j P[i * nk + j] in[j] for i < nk, out[i] = in[i] for nk <= i < nb (one workgroup; each sum in
+// column order): the Gram-Schmidt coefficients over kept basis vectors projected onto a kept subspace
+__global__ void k_proj_coef(int nk, int nb, const double* __restrict__ P, const double* __restrict__ in,
+                            double* __restrict__ out) {
+  for (int i = threadIdx.x; i < nb; i += blockDim.x) {
+    double v = in[i];
+    if (i < nk) {
+      v = 0.;
+      for (int j = 0; j < nk; ++j) v += P[(int64_t)i * nk + j] * in[j];
+    }
+    out[i] = v;
+  }
+}
 __global__ void k_axpby(int64_t n, double a, const double* __restrict__ x, double b, double* __restrict__ y) {
   for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < n; j += (int64_t)gridDim.x * blockDim.x) y[j] = a * x[j] + b * y[j];
 }
@@ -2044,6 +2057,11 @@ void reduce_partials(hipStream_t s, const double* partial, int k, double* out) {
 }
 void vsum(hipStream_t s, const double* V, int64_t ldv, int k, const double* h, double sign, double* w, int64_t n) {
   hipLaunchKernelGGL(k_vsum, dim3(grid1(n)), dim3(256), 0, s, V, ldv, k, h, sign, w, n);
+}
+void proj_coef(hipStream_t s, int nk, int nb, const double* P, const double* in, double* out) {
+  if (nb <= 0) return;
+  if (nk < 0 || nk > nb || in == out) throw std::runtime_error("proj_coef: invalid sizes or aliased buffers");
+  hipLaunchKernelGGL(k_proj_coef, dim3(1), dim3(256), 0, s, nk, nb, P, in, out);
 }
 void axpby(hipStream_t s, int64_t n, double a, const double* x, double b, double* y) {
   hipLaunchKernelGGL(k_axpby, dim3(grid1(n)), dim3(256), 0, s, n, a, x, b, y);

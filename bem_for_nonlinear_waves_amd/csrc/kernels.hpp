@@ -339,6 +339,8 @@ void schwarz_build(hipStream_t s, const Schwarz& b, int max_block, const NearRow
 void dots(hipStream_t s, int k, const double* V, int64_t ldv, const double* w, int64_t n, double* partial);
 void reduce_partials(hipStream_t s, const double* partial, int k, double* out);
 void vsum(hipStream_t s, const double* V, int64_t ldv, int k, const double* h, double sign, double* w, int64_t n);
+// out[0..nk) = P in[0..nk) (P: nk x nk, row-major), out[nk..nb) = in[nk..nb); in and out must not overlap
+void proj_coef(hipStream_t s, int nk, int nb, const double* P, const double* in, double* out);
 void axpby(hipStream_t s, int64_t n, double a, const double* x, double b, double* y);
 void sub(hipStream_t s, int64_t n, const double* a, const double* b, double* out);
 void mul(hipStream_t s, int64_t n, const double* d, double* y);

@@ -1235,6 +1235,20 @@ void vsum(hipStream_t s, const double* V, int64_t ldv, int k, const double* h, d
     }
   });
 }
+void proj_coef(hipStream_t s, int nk, int nb, const double* P, const double* in, double* out) {
+  if (nb <= 0) return;
+  if (nk < 0 || nk > nb || in == out) throw std::runtime_error("proj_coef: invalid sizes or aliased buffers");
+  hipstub::enqueue(s, [=]() {
+    for (int i = 0; i < nb; ++i) {
+      double v = in[i];
+      if (i < nk) {
+        v = 0.;
+        for (int j = 0; j < nk; ++j) v += P[(int64_t)i * nk + j] * in[j];
+      }
+      out[i] = v;
+    }
+  });
+}
 void axpby(hipStream_t s, int64_t n, double a, const double* x, double b, double* y) {
   hipstub::enqueue(s, [=]() {
     for (int64_t j = 0; j < n; ++j) y[j] = a * x[j] + b * y[j];
