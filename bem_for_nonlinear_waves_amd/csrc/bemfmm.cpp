@@ -296,11 +296,11 @@ struct bemfmm_ctx {
   } rot, rot_in;
   int P_in = 0;
   // the FP32 operator at the inner order runs its expansions compact: (P_in+1)^2 coefficients per box
-  // (C_in, its padded RP_in / KP_in), M2M and L2L on the leading blocks of the 16 octant operators (ops_in,
-  // tiles m2m_in / l2l_in index it directly)
+  // (C_in, its padded RP_in / KP_in), M2M and L2L on the leading blocks of the 16 octant operators (ops_in):
+  // the tiles of m2m / l2l with their own operator index per tile (m2m_in_op / l2l_in_op index ops_in)
   int C_in = 0, RP_in = 0, KP_in = 0;
   DBuf<double> ops_in;
-  std::vector<DTiles> m2m_in, l2l_in;
+  std::vector<DBuf<int32_t>> m2m_in_op, l2l_in_op;
   DTiles m2l_rt;  // M2L tiles of kRotTP pairs (one offset each)
   bool m2l_pw_on = false;  // FP32 operator's M2L through the plane-wave composed operators (m2l_kernel 3)
   DTiles m2l_pw;           // its tiles (operator slots of the plane-wave M2L offsets)
@@ -564,20 +564,26 @@ void evaluate_launch(bemfmm_ctx* c, int mode, const double* in, double* out, boo
   if (lo) launch_p2m(c, c->p2m_mat32.p, phin, phi, w.M0, w.M1, st);
   else launch_p2m(c, c->p2m.mat.p, phin, phi, w.M0, w.M1, st);
   rec(2);
+  // the compact translations use the tiles of m2m / l2l with their ops_in operator indices
+  auto compact_view = [&](const DTiles& t, const DBuf<int32_t>& op) {
+    dev::Tiles v = t.view();
+    v.op = op.p;
+    return v;
+  };
   auto translate_levels = [&](bool up, double* A0, double* A1) {
     if (up) {
       for (int l = pl.n_levels - 1; l >= 1 && l - 1 >= c->m2l_min_level; --l) {
         if (compact)
-          dev::translate(st, c->C_in, c->RP_in, c->KP_in, c->m2m_in[l].view(), c->ops_in.p, A0, A1, A0, A1,
-                         (double*)nullptr);
+          dev::translate(st, c->C_in, c->RP_in, c->KP_in, compact_view(c->m2m[l], c->m2m_in_op[l]), c->ops_in.p, A0,
+                         A1, A0, A1, (double*)nullptr);
         else
           launch_translate(c, c->m2m[l], c->ops.p, A0, A1, A0, A1, (double*)nullptr, st);
       }
     } else {
       for (int l = std::max(1, c->m2l_min_level + 1); l < pl.n_levels; ++l) {
         if (compact)
-          dev::translate(st, c->C_in, c->RP_in, c->KP_in, c->l2l_in[l].view(), c->ops_in.p, A0, A1, A0, A1,
-                         (double*)nullptr);
+          dev::translate(st, c->C_in, c->RP_in, c->KP_in, compact_view(c->l2l[l], c->l2l_in_op[l]), c->ops_in.p, A0,
+                         A1, A0, A1, (double*)nullptr);
         else
           launch_translate(c, c->l2l[l], c->ops.p, A0, A1, A0, A1, (double*)nullptr, st);
       }
@@ -1997,15 +2003,19 @@ void do_prepare(bemfmm_ctx* c) {
           for (int r = 0; r < c->C_in; ++r) t[(size_t)o * per + (size_t)k * c->RP_in + r] = (*v)[(size_t)k * pl.C + r];
       }
       c->ops_in.upload(t, s);
-      std::vector<int32_t> ident(16);
-      for (int o = 0; o < 16; ++o) ident[o] = o;
-      c->m2m_in.clear();
-      c->l2l_in.clear();
-      c->m2m_in.resize(pl.n_levels);
-      c->l2l_in.resize(pl.n_levels);
+      // per tile of m2m[l] / l2l[l] (same pairs, same tiling), the plan's operator index 0..15 = its ops_in slot
+      auto tile_ops = [&](const bemfmm::PairGroups& g, const DTiles& base, DBuf<int32_t>& out) {
+        std::vector<int32_t> top;
+        for (size_t k = 0; k < g.op.size(); ++k)
+          for (int64_t a = g.pair_ptr[k]; a < g.pair_ptr[k + 1]; a += base.tp) top.push_back(g.op[k]);
+        if ((int)top.size() != base.ntiles) throw std::runtime_error("inner order: M2M / L2L tiles do not match");
+        out.upload(top, s);
+      };
+      c->m2m_in_op.resize(pl.n_levels);
+      c->l2l_in_op.resize(pl.n_levels);
       for (int l = 1; l < pl.n_levels; ++l) {
-        c->m2m_in[l].upload(pl.m2m_by_level[l], s, ident);
-        c->l2l_in[l].upload(pl.l2l_by_level[l], s, ident);
+        tile_ops(pl.m2m_by_level[l], c->m2m[l], c->m2m_in_op[l]);
+        tile_ops(pl.l2l_by_level[l], c->l2l[l], c->l2l_in_op[l]);
       }
     }
     // round the resident FP64 matrices once to FP32 (operators were computed in binary128 and rounded to FP64)
